@@ -1,0 +1,202 @@
+"""Benchmark of the per-step particle-mesh loop on one MI355X.
+
+Default workload (N = 1): the FP64 streamer step on a 128x128x256 mesh with
+h = 5 um, a 1e6-pair Gaussian seed, CIC deposition, V(2,2) multigrid to 1e-8
+relative residual, per-cell merge/split (8..64) and dt = 2e-13 s.  One step =
+deposit + Poisson solve + face fields + fused push/collisions (mesh gather, ion
+tracker) + diagnostics + canonical cell sort + per-cell resample.
+
+Prints one JSON line.  `value` is particle-steps/s of whole steps with the
+state resident in HBM.  `roofline` is for the dominant kernel (the fused
+advance): algorithmic bytes per launch (128 B per particle-step in FP64) over its
+average duration measured with HIP events on the library's stream.
+`cpu_baseline` times the oracle (single-threaded C restatement) on one full step
+of the same state.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "particle-steps/s & %HBM peak; swarm solves/s, p99 vs plain at <=1% stat err"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_PSTEP_F64 = 128.0
+
+
+def cpu_baseline_streamer(sim, ctx_arrays, budget_s=20.0):
+    """One full oracle step on the device state (bounded: a single step)."""
+    from oracle import refcpu as R
+
+    popl, fields = sim.popl, sim.fields
+    ps = popl.download()
+    n = len(ps)
+    P = R.Pop(ps.x, ps.v, ps.w, ps.s, ps.active, ps.id, cap=int(n * 1.5) + 1000, next_id=ps.next_id)
+    grid = fields.grid
+    M = R.Mesh(grid.n, grid.h)
+    M.qfixed[:] = fields.get("qfixed")
+    M.u[:] = fields.get("u")
+    tab = ctx_arrays["table"]
+    T = R.Table(tab.energy, tab.rate, tab.kind, tab.param, tab.maxrate, egrid=tab.egrid)
+    conf = sim.ws_conf
+    t0 = time.perf_counter()
+    R.sort_by_cell(P, M)
+    M.qpart[:] = 0.0
+    R.deposit(P, M, 1, 1, -1.0)
+    R.assemble_charge(M)
+    rc, iters, _ = R.mg_solve(grid.n, conf.bc, conf.s, conf.levels, conf.smooth1, conf.smooth2,
+                              conf.tolerance, conf.maxiter, M.u, M.q)
+    R.efield_faces(M, sim.eb)
+    f = R.mesh_field(M)
+    R.advance(P, T, f, sim.dt, M, seed=popl.seed, step=10_000)
+    R.sort_by_cell(P, M)
+    R.resample_cells(P, M, sim.ntarget, sim.nmax, seed=popl.seed, step=10_000)
+    el = time.perf_counter() - t0
+    return {"value": n / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"one full streamer step of the device state after warmup ({n} particles, "
+                      f"{grid.n[0]}x{grid.n[1]}x{grid.n[2]} mesh, {iters} fmg iterations), "
+                      f"single-threaded oracle, {el:.1f} s"}
+
+
+def run_streamer(args):
+    from mcstreamer_amd import streamer
+    from mcstreamer_amd.jumc import context
+
+    ctx = context()
+    n = tuple(args.mesh)
+    cfg = streamer.make_config1(n=n, h=args.h, npairs=args.npairs, seed=args.seed)
+    sim = streamer.Streamer(cfg["popl"], cfg["fields"], cfg["ws"], cfg["eb"], cfg["dt"],
+                            cfg["ntarget"], cfg["nmax"], scheme="cic")
+    sim.ws_conf = cfg["conf"]
+    for i in range(args.warmup):
+        out = sim.step()
+        print(f"warmup {i}: n={out['advance'].n_in} mg_iters={out['mg_iters']}", file=sys.stderr, flush=True)
+    ctx.synchronize()
+    ctx.profile_reset()
+    ctx.profile(True)
+    psteps = 0
+    vcycles = 0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        out = sim.step()
+        psteps += out["advance"].n_in
+        vcycles += max(out["mg_iters"] - 1, 0)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    ctx.profile(False)
+    kt = {}
+    for k in ("advance", "sort_count", "sort_scatter", "sort_gather", "deposit", "faces", "resample",
+              "mg_rbgs", "mg_residual", "mg_restrict", "mg_prolong"):
+        ms, cnt = ctx.profile_read(k)
+        kt[k] = {"ms_total": ms, "launches": cnt}
+    adv_ms, adv_n = kt["advance"]["ms_total"], kt["advance"]["launches"]
+    adv_avg_s = adv_ms / 1e3 / max(adv_n, 1)
+    bytes_per_launch = BYTES_PER_PSTEP_F64 * psteps / max(args.steps, 1)
+    achieved = bytes_per_launch / adv_avg_s / 1e9
+    result = {
+        "metric": METRIC,
+        "value": psteps / el,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"FP64 streamer step, {n[0]}x{n[1]}x{n[2]} mesh h={args.h:g} m, "
+                               f"{args.npairs:g}-pair Gaussian seed, CIC, V(2,2) to 1e-8 rel, "
+                               "merge/split 8-64, dt=2e-13 s",
+                   "mesh": list(n), "particles_per_step": psteps / max(args.steps, 1),
+                   "parallelism": "single GPU"},
+        "vcycles_per_s": vcycles / el,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_advance<1,1> (fused push + collisions, mesh gather, ion tracker)",
+                     "avg_launch_ms": adv_avg_s * 1e3,
+                     "bytes_per_launch": bytes_per_launch},
+        "kernels_ms": kt,
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline_streamer(sim, {"table": cfg["table"]})
+    else:
+        result["cpu_baseline"] = None
+    return result
+
+
+def run_swarm(args):
+    """Config-0 style swarm: 1e4 electrons at 100 Td in the N2-like gas."""
+    from mcstreamer_amd import constants as co
+    from mcstreamer_amd import gases, jumc, lxcat
+
+    ctx = jumc.context()
+    e_ev = np.geomspace(1e-3, 1e3, 4000)
+    path = os.path.join(ROOT, "_work", "bench_gas_c0.json")
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    gases.write_gas(path, e_ev, with_attacher=False)
+    proc, rate, maxrate, _ = lxcat.load_lxcat([path], gases.densities(0.0), e_ev * co.eV)
+    table = jumc.CollisionTable(proc, e_ev * co.eV, rate, maxrate, egrid="log")
+    n0 = args.swarm_n
+    rng = np.random.default_rng(args.seed)
+    v = rng.normal(size=(n0, 3)) * np.sqrt(co.eV / co.electron_mass)
+    ps = jumc.ParticleStates(np.zeros((n0, 3)), v, np.ones(n0), -np.log1p(-rng.uniform(size=n0)),
+                             np.ones(n0, dtype=np.uint8))
+    pop = jumc.Population(10 * n0, ps, table, seed=args.seed)
+    field = jumc.FixedField(-100 * co.Td * gases.N_GAS)
+    from mcstreamer_amd import swarm
+
+    for _ in range(args.warmup):
+        swarm.swarm_step(pop, field, 1e-12, n0)
+    ctx.synchronize()
+    ctx.profile_reset()
+    ctx.profile(True)
+    psteps = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = swarm.swarm_step(pop, field, 1e-12, n0)
+        psteps += st.n_in
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    ms, cnt = ctx.profile_read("advance")
+    avg = ms / 1e3 / max(cnt, 1)
+    ach = BYTES_PER_PSTEP_F64 * psteps / max(args.steps, 1) / avg / 1e9
+    return {"metric": METRIC, "value": psteps / el, "unit": "particle-steps/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": f"FP64 swarm, {n0} electrons, 100 Td, N2-like gas, dt=1e-12 s"},
+            "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": ach / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": avg * 1e3},
+            "cpu_baseline": None}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=["streamer", "swarm"], default="streamer")
+    ap.add_argument("--mesh", type=int, nargs=3, default=[128, 128, 256])
+    ap.add_argument("--h", type=float, default=5e-6)
+    ap.add_argument("--npairs", type=int, default=1_000_000)
+    ap.add_argument("--swarm-n", type=int, default=10_000)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: the particle-mesh step is measured at N=1 "
+              "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
+        sys.exit(2)
+    res = run_streamer(args) if args.workload == "streamer" else run_swarm(args)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,242 @@
+/*
+ * mcs.h -- C ABI of libmcs, the MI355X (gfx950) implementation of the
+ * per-step particle-mesh loop of aluque/mcstreamer.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and returns
+ * an int32_t status (MCS_OK = 0).  The library owns every device buffer; host
+ * pointers are borrowed for the duration of a call only.  One context owns one
+ * HIP stream; calls on a context are not re-entrant.
+ *
+ * Each declaration names the reference function (file:line, paths relative to
+ * the reference tree) whose behaviour it replaces.  A Julia binding calls these
+ * with `ccall`; the Python mirror in mcstreamer_amd/ calls them with ctypes.
+ */
+#ifndef MCS_H
+#define MCS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------
+ * MCS_E_CAPACITY     : JuMC/src/population.jl:90 "Maximum number of particles reached"
+ * MCS_E_FIELD        : MCStreamer/src/gridfields.jl:178-187 "Too high electric field" (|E| > 1e8 V/m)
+ * MCS_E_TABLE_RANGE  : JuMC/src/util.jl:27 (energy beyond the last table node; BoundsError)
+ * MCS_E_NONCONVERGED : Multigrid/src/Multigrid.jl:458-464 (logged there, returned here)
+ */
+#define MCS_OK 0
+#define MCS_E_ARG 1
+#define MCS_E_HIP 2
+#define MCS_E_CAPACITY 3
+#define MCS_E_FIELD 4
+#define MCS_E_TABLE_RANGE 5
+#define MCS_E_NONCONVERGED 6
+#define MCS_E_NAN 7
+
+/* Collision process kinds (JuMC/src/electron.jl:70-84, collisions.jl:80). */
+#define MCS_PROC_NULL 0
+#define MCS_PROC_ELASTIC 1
+#define MCS_PROC_EXCITATION 2
+#define MCS_PROC_IONIZATION 3
+#define MCS_PROC_ATTACHMENT 4
+#define MCS_MAX_PROC 32
+
+/* Energy grid of a collision table. */
+#define MCS_EGRID_UNIFORM 0 /* the reference's LinRange (util.jl:23-30) */
+#define MCS_EGRID_LOG 1     /* log-spaced nodes, rate linear in energy between nodes */
+
+/* Field descriptor kinds: Swarm.FixedField (Swarm/src/Swarm.jl:28-32) and
+ * MCStreamer.FieldInterp (MCStreamer/src/gridfields.jl:189-290). */
+#define MCS_FIELD_UNIFORM 0
+#define MCS_FIELD_MESH 1
+
+/* Mesh arrays.  Cell-centred arrays carry one ghost layer: (nx+2)(ny+2)(nz+2),
+ * x fastest.  Face arrays: EX (nx+1)(ny+2)(nz+2), EY (nx+2)(ny+1)(nz+2),
+ * EZ (nx+2)(ny+2)(nz+1). */
+#define MCS_ARR_QFIXED 0
+#define MCS_ARR_QPART 1
+#define MCS_ARR_Q 2
+#define MCS_ARR_U 3
+#define MCS_ARR_EX 4
+#define MCS_ARR_EY 5
+#define MCS_ARR_EZ 6
+
+#define MCS_DEPOSIT_NGP 0 /* MCStreamer/src/poisson.jl:65-84 */
+#define MCS_DEPOSIT_CIC 1 /* trilinear cloud-in-cell */
+
+/* Multigrid boundary conditions (Multigrid/src/boundaries.jl:161-169). */
+#define MCS_BC_DIRICHLET 0
+#define MCS_BC_NEUMANN 1
+
+#define MCS_PREC_F64 0
+#define MCS_PREC_F32 1
+
+typedef struct mcs_ctx mcs_ctx;
+typedef struct mcs_table mcs_table;
+typedef struct mcs_pop mcs_pop;
+typedef struct mcs_mesh mcs_mesh;
+typedef struct mcs_mg mcs_mg;
+
+typedef struct mcs_field {
+    int32_t kind;      /* MCS_FIELD_UNIFORM or MCS_FIELD_MESH */
+    double e[3];       /* uniform field (V/m) */
+    mcs_mesh *mesh;    /* face fields for MCS_FIELD_MESH */
+} mcs_field;
+
+typedef struct mcs_step_stats {
+    int64_t n_in;                   /* population size when the step started */
+    int64_t n_out;                  /* after births were appended */
+    int64_t births;
+    int64_t deaths;
+    int64_t flights;                /* free flights (collision or end of step) */
+    int64_t events[MCS_MAX_PROC];   /* collisions selected per table row (null included) */
+    int32_t flags;                  /* bit 0 table range, bit 1 field limit, bit 2 capacity */
+} mcs_step_stats;
+
+typedef struct mcs_moments {
+    int64_t nparticles;             /* population.jl:63 */
+    int64_t nactive;                /* population.jl:130 */
+    double weight;                  /* population.jl:114-124 */
+    double wenergy;                 /* sum w*eps over actives (meanenergy numerator, :136-150) */
+    double maxenergy;               /* over all slots, active or not (:156-158) */
+    double sum_x[3];                /* unweighted sums over actives (Swarm.jl:126-127) */
+    double sum_x2[3];
+    double sum_v[3];                /* Swarm.jl:152 */
+    double sum_energy;
+    double collrate[MCS_MAX_PROC];  /* sum of per-process rates (Swarm.collrate, Swarm.jl:242-248) */
+} mcs_moments;
+
+typedef struct mcs_resample_stats {
+    int64_t n_in;
+    int64_t n_out;
+    int64_t cells_merged;           /* cells with p > nmax */
+    int64_t clones;                 /* particles created by roulette */
+    int64_t splits;                 /* particles created by splitting */
+    int64_t removed;
+} mcs_resample_stats;
+
+typedef struct mcs_mg_config {
+    int32_t n[3];       /* interior cells per dimension at the finest level */
+    int32_t bc[6];      /* x-, x+, y-, y+, z-, z+ ; applied in this order */
+    double s;           /* right-hand-side factor (MGConfig.s, Multigrid.jl:48-49) */
+    int32_t levels;     /* coarsenings (MGConfig.levels) */
+    int32_t smooth1;
+    int32_t smooth2;
+    double tolerance;   /* converged when ||r||/sqrt(N) < s*tolerance (Multigrid.jl:391-398);
+                           a negative value -t means t times the rms of b (relative mode) */
+    int32_t maxiter;
+    int32_t precision;  /* MCS_PREC_F64, or MCS_PREC_F32 smoother inside FP64 refinement */
+} mcs_mg_config;
+
+/* ---- context ------------------------------------------------------------- */
+int32_t mcs_ctx_create(int32_t device, mcs_ctx **out);
+int32_t mcs_ctx_destroy(mcs_ctx *ctx);
+const char *mcs_last_error(const mcs_ctx *ctx);
+int32_t mcs_ctx_synchronize(mcs_ctx *ctx);
+/* The hipStream_t the context launches on (for callers that time or order work). */
+int32_t mcs_ctx_stream(mcs_ctx *ctx, void **stream);
+/* Per-kernel HIP-event timing of the hot kernels; names: "advance", "sort_count",
+ * "sort_scatter", "sort_gather", "deposit", "faces", "resample", "mg_rbgs", "mg_residual",
+ * "mg_restrict", "mg_prolong". */
+int32_t mcs_ctx_profile(mcs_ctx *ctx, int32_t enable);
+int32_t mcs_ctx_profile_read(mcs_ctx *ctx, const char *kernel, double *total_ms, int64_t *launches);
+int32_t mcs_ctx_profile_reset(mcs_ctx *ctx);
+
+/* ---- collision table: JuMC.CollisionTable (JuMC/src/collisions.jl:86-99) ---
+ * rate is energy-major: rate[k*nproc + j] is process j at node k (the reference's
+ * column-major (nproc x nE) matrix, collisions.jl:94-95, lxcat.jl:99).  energy holds
+ * the ne node energies in J.  param[j] is the threshold (J) of excitation,
+ * ionization and attachment rows and the mass ratio of elastic rows. */
+int32_t mcs_table_create(mcs_ctx *ctx, int32_t nproc, int32_t ne, int32_t egrid_kind,
+                         const double *energy, const double *rate, const int32_t *kind,
+                         const double *param, double maxrate, mcs_table **out);
+int32_t mcs_table_destroy(mcs_table *t);
+
+/* ---- particle population: JuMC.Population (JuMC/src/population.jl:4-36) -----
+ * Host layout of x and v is n x 3 row-major (Julia Vector{SVector{3,Float64}}). */
+int32_t mcs_pop_create(mcs_ctx *ctx, int64_t capacity, mcs_pop **out);
+int32_t mcs_pop_destroy(mcs_pop *p);
+int32_t mcs_pop_upload(mcs_pop *p, int64_t n, const double *x, const double *v,
+                       const double *w, const double *s, const uint8_t *active,
+                       const uint64_t *id, uint64_t next_id);
+int32_t mcs_pop_download(mcs_pop *p, int64_t nmax, int64_t *n, double *x, double *v,
+                         double *w, double *s, uint8_t *active, uint64_t *id,
+                         uint64_t *next_id);
+/* nparticles (population.jl:63): slots in use, active or not. */
+int32_t mcs_pop_size(mcs_pop *p, int64_t *n);
+/* weight/actives/meanenergy/maxenergy/spread (population.jl:74-183) and the swarm
+ * tallies of Swarm.swarm (Swarm.jl:126-156) in one device reduction. */
+int32_t mcs_pop_moments(mcs_pop *p, const mcs_table *t, mcs_moments *out);
+
+/* ---- the fused step: JuMC.advance! (JuMC/src/mixed_population.jl:24-54) ---
+ * Yoshida push (electron.jl:51-62), null-collision sampling (collisions.jl:236-269),
+ * collision kinematics (electron.jl:108-139) and births/deaths (collisions.jl:173-192,
+ * population.jl:88-106) in one kernel.  tracker = NULL is VoidCollisionTracker; a mesh
+ * adds +-w/dV ion charge to its QFIXED array (gridfields.jl:298-324). */
+int32_t mcs_advance(mcs_pop *p, const mcs_table *t, const mcs_field *field, double dt,
+                    mcs_mesh *tracker, uint64_t seed, uint32_t instance, uint32_t step,
+                    mcs_step_stats *stats);
+
+/* repack! (population.jl:191-215): last active particle into the first hole. */
+int32_t mcs_repack(mcs_pop *p);
+
+/* Swarm.resample! (Swarm/src/Swarm.jl:193-223): global roulette / Poisson cloning
+ * towards n0 when the active count is off by at least tol. did = 1 if it acted. */
+int32_t mcs_swarm_resample(mcs_pop *p, int64_t n0, double tol, uint64_t seed,
+                           uint32_t instance, uint32_t step, int32_t *did);
+
+/* ---- mesh: 3D Cartesian generalisation of MCStreamer.Grid/GridFields
+ * (MCStreamer/src/grid.jl:4-100, gridfields.jl:107-184); origin at 0, spacing h. */
+int32_t mcs_mesh_create(mcs_ctx *ctx, const int32_t n[3], double h, mcs_mesh **out);
+int32_t mcs_mesh_destroy(mcs_mesh *m);
+int32_t mcs_mesh_set(mcs_mesh *m, int32_t which, const double *host);
+int32_t mcs_mesh_get(mcs_mesh *m, int32_t which, double *host);
+int32_t mcs_mesh_zero(mcs_mesh *m, int32_t which);
+
+/* Canonical cell order: drop inactive particles and sort the rest by (cell, id);
+ * particles outside the mesh go last.  Builds the per-cell start offsets used by
+ * deposit and resample.  Replaces repack! where the order is free (see DESIGN). */
+int32_t mcs_sort_by_cell(mcs_pop *p, mcs_mesh *m);
+
+/* density! (MCStreamer/src/poisson.jl:65-84): add w*val/dV of every active particle
+ * to array `which` (QFIXED or QPART) with NGP or CIC weights.  Needs the population in
+ * canonical cell order (mcs_sort_by_cell); accumulation is a per-cell gather, no atomics. */
+int32_t mcs_deposit(mcs_pop *p, mcs_mesh *m, int32_t which, int32_t scheme, double val);
+
+/* q = e*(qfixed + qpart) (poisson.jl:19-26 with ne = 0 and no denoiser). */
+int32_t mcs_assemble_charge(mcs_mesh *m);
+
+/* Face fields from the potential (poisson.jl:47-57): E = -grad u on faces plus eb
+ * on z, with the tangential ghost copies. */
+int32_t mcs_efield_faces(mcs_mesh *m, double eb);
+
+/* MCStreamer.resample! (MCStreamer/src/resample.jl:7-84) per sorted cell: systematic
+ * roulette to ntarget equal weights where a cell holds more than nmax particles,
+ * pairwise splitting where it holds fewer than ntarget and w > 2.  Output stays in
+ * canonical cell order and is compacted. */
+int32_t mcs_resample_cells(mcs_pop *p, mcs_mesh *m, int32_t ntarget, int32_t nmax,
+                           uint64_t seed, uint32_t instance, uint32_t step,
+                           mcs_resample_stats *stats);
+
+/* ---- multigrid: Multigrid.allocate / solve (Multigrid/src/Multigrid.jl:302-324,
+ * 449-467), the fmg!/mgv! cycle (:334-434) with red-black Gauss-Seidel, residual,
+ * restriction, trilinear prolongation and the coarse direct solve. */
+int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg **out);
+int32_t mcs_mg_destroy(mcs_mg *mg);
+/* Solve L u + s b = 0 on host arrays with ghosts ((nx+2)(ny+2)(nz+2), x fastest);
+ * u is the initial guess on entry and the solution on return. */
+int32_t mcs_mg_solve_host(mcs_mg *mg, double *u, const double *b, int32_t *iters,
+                          double *resid);
+/* Same, on a mesh: u = mesh U, b = mesh Q (device resident). */
+int32_t mcs_mg_solve_mesh(mcs_mg *mg, mcs_mesh *m, int32_t *iters, double *resid);
+/* setinhom! (boundaries.jl:202-214): b += (2/s) val on the first valid layer of face
+ * `face` (0..5); val has the face's interior size, first index fastest. */
+int32_t mcs_mg_setinhom_mesh(mcs_mg *mg, mcs_mesh *m, int32_t face, const double *val);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MCS_H */

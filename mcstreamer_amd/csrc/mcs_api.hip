@@ -1,0 +1,721 @@
+// mcs_api.hip -- context, collision table, particle population and the
+// population reductions of libmcs.
+#include <cstdarg>
+#include <cstring>
+
+#include "mcs_device.h"
+
+// ---------------------------------------------------------------- errors
+
+int32_t mcs_fail(mcs_ctx *ctx, int32_t code, const char *fmt, ...)
+{
+    if (ctx) {
+        char buf[1024];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+extern "C" const char *mcs_last_error(const mcs_ctx *ctx)
+{
+    return ctx ? ctx->err.c_str() : "null context";
+}
+
+// ---------------------------------------------------------------- context
+
+extern "C" int32_t mcs_ctx_create(int32_t device, mcs_ctx **out)
+{
+    if (!out) return MCS_E_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return MCS_E_HIP;
+    if (device < 0 || device >= ndev) return MCS_E_ARG;
+    if (hipSetDevice(device) != hipSuccess) return MCS_E_HIP;
+    mcs_ctx *c = new mcs_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return MCS_E_HIP;
+    }
+    if (hipHostMalloc((void **)&c->h_small, 512 * sizeof(int64_t), hipHostMallocDefault) !=
+            hipSuccess ||
+        hipMalloc((void **)&c->d_small, 512 * sizeof(int64_t)) != hipSuccess) {
+        delete c;
+        return MCS_E_HIP;
+    }
+    *out = c;
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_ctx_destroy(mcs_ctx *ctx)
+{
+    if (!ctx) return MCS_OK;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (auto &kv : ctx->profile)
+        for (auto &pr : kv.second.pending) {
+            hipEventDestroy(pr.first);
+            hipEventDestroy(pr.second);
+        }
+    if (ctx->scratch) hipFree(ctx->scratch);
+    if (ctx->d_small) hipFree(ctx->d_small);
+    if (ctx->h_small) hipHostFree(ctx->h_small);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_ctx_synchronize(mcs_ctx *ctx)
+{
+    if (!ctx) return MCS_E_ARG;
+    MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_ctx_stream(mcs_ctx *ctx, void **stream)
+{
+    if (!ctx || !stream) return MCS_E_ARG;
+    *stream = (void *)ctx->stream;
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_ctx_profile(mcs_ctx *ctx, int32_t enable)
+{
+    if (!ctx) return MCS_E_ARG;
+    ctx->prof = enable != 0;
+    return MCS_OK;
+}
+
+static int32_t settle_profile(mcs_ctx *ctx)
+{
+    MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto &kv : ctx->profile) {
+        for (auto &pr : kv.second.pending) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, pr.first, pr.second);
+            kv.second.total_ms += ms;
+            kv.second.launches += 1;
+            hipEventDestroy(pr.first);
+            hipEventDestroy(pr.second);
+        }
+        kv.second.pending.clear();
+    }
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_ctx_profile_read(mcs_ctx *ctx, const char *kernel, double *total_ms,
+                                        int64_t *launches)
+{
+    if (!ctx || !kernel) return MCS_E_ARG;
+    int32_t rc = settle_profile(ctx);
+    if (rc) return rc;
+    auto it = ctx->profile.find(kernel);
+    if (it == ctx->profile.end()) {
+        if (total_ms) *total_ms = 0.0;
+        if (launches) *launches = 0;
+        return MCS_OK;
+    }
+    if (total_ms) *total_ms = it->second.total_ms;
+    if (launches) *launches = it->second.launches;
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_ctx_profile_reset(mcs_ctx *ctx)
+{
+    if (!ctx) return MCS_E_ARG;
+    int32_t rc = settle_profile(ctx);
+    if (rc) return rc;
+    ctx->profile.clear();
+    return MCS_OK;
+}
+
+int32_t mcs_scratch(mcs_ctx *ctx, size_t bytes, void **out)
+{
+    if (bytes > ctx->scratch_bytes) {
+        if (ctx->scratch) {
+            MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            MCS_HIP(ctx, hipFree(ctx->scratch));
+            ctx->scratch = nullptr;
+        }
+        size_t nb = bytes < (1u << 20) ? (1u << 20) : bytes + bytes / 4;
+        MCS_HIP(ctx, hipMalloc(&ctx->scratch, nb));
+        ctx->scratch_bytes = nb;
+    }
+    *out = ctx->scratch;
+    return MCS_OK;
+}
+
+int32_t mcs_readback(mcs_ctx *ctx, const int64_t *dev, int64_t *host, int n)
+{
+    MCS_HIP(ctx, hipMemcpyAsync(ctx->h_small, dev, sizeof(int64_t) * n, hipMemcpyDeviceToHost,
+                                ctx->stream));
+    MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    memcpy(host, ctx->h_small, sizeof(int64_t) * n);
+    return MCS_OK;
+}
+
+// ---------------------------------------------------------------- scans
+
+#define SCAN_ITEMS 8
+#define SCAN_TILE (MCS_BLOCK * SCAN_ITEMS)
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(MCS_BLOCK) void k_scan_reduce(const TI *in, int64_t len, TO *bsum)
+{
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    TO acc = 0;
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        int64_t i = base + (int64_t)it * MCS_BLOCK + threadIdx.x;
+        if (i < len) acc += (TO)in[i];
+    }
+    TO tot;
+    block_excl_scan<TO>(acc, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+template <typename TO>
+__global__ __launch_bounds__(MCS_BLOCK) void k_scan_top(TO *bsum, int64_t nb, TO *total_out)
+{
+    TO carry = 0;
+    for (int64_t base = 0; base < nb; base += MCS_BLOCK) {
+        int64_t i = base + threadIdx.x;
+        TO v = i < nb ? bsum[i] : 0;
+        TO tot;
+        TO ex = block_excl_scan<TO>(v, &tot);
+        if (i < nb) bsum[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total_out = carry;
+}
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(MCS_BLOCK) void k_scan_down(const TI *in, int64_t len,
+                                                         const TO *bsum, TO *out)
+{
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    TO vals[SCAN_ITEMS];
+    TO acc = 0;
+#pragma unroll
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        int64_t i = base + it;
+        vals[it] = i < len ? (TO)in[i] : 0;
+        acc += vals[it];
+    }
+    TO tot;
+    TO ex = block_excl_scan<TO>(acc, &tot) + bsum[blockIdx.x];
+#pragma unroll
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        int64_t i = base + it;
+        if (i < len) out[i] = ex;
+        ex += vals[it];
+    }
+}
+
+template <typename TI, typename TO>
+static int32_t scan_impl(mcs_ctx *ctx, const TI *in, TO *out, int64_t len)
+{
+    int64_t nb = cdiv(len, SCAN_TILE);
+    if (nb == 0) nb = 1;
+    void *scr;
+    int32_t rc = mcs_scratch(ctx, sizeof(TO) * (size_t)(nb + 1), &scr);
+    if (rc) return rc;
+    TO *bsum = (TO *)scr;
+    hipLaunchKernelGGL((k_scan_reduce<TI, TO>), dim3((unsigned)nb), dim3(MCS_BLOCK), 0,
+                       ctx->stream, in, len, bsum);
+    hipLaunchKernelGGL((k_scan_top<TO>), dim3(1), dim3(MCS_BLOCK), 0, ctx->stream, bsum, nb,
+                       out + len);
+    hipLaunchKernelGGL((k_scan_down<TI, TO>), dim3((unsigned)nb), dim3(MCS_BLOCK), 0,
+                       ctx->stream, in, len, bsum, out);
+    MCS_LAUNCH_CHECK(ctx);
+    return MCS_OK;
+}
+
+int32_t mcs_scan_i32(mcs_ctx *ctx, const int32_t *in, int32_t *out, int64_t len)
+{
+    return scan_impl<int32_t, int32_t>(ctx, in, out, len);
+}
+
+int32_t mcs_scan_i64(mcs_ctx *ctx, const int64_t *in, int64_t *out, int64_t len)
+{
+    return scan_impl<int64_t, int64_t>(ctx, in, out, len);
+}
+
+// ---------------------------------------------------------------- table
+
+extern "C" int32_t mcs_table_create(mcs_ctx *ctx, int32_t nproc, int32_t ne, int32_t egrid_kind,
+                                    const double *energy, const double *rate,
+                                    const int32_t *kind, const double *param, double maxrate,
+                                    mcs_table **out)
+{
+    if (!ctx || !out || !energy || !rate || !kind || !param) return MCS_E_ARG;
+    if (nproc < 1 || nproc > MCS_MAX_PROC || ne < 2)
+        return mcs_fail(ctx, MCS_E_ARG, "table: nproc must be 1..%d and ne >= 2", MCS_MAX_PROC);
+    if (egrid_kind != MCS_EGRID_UNIFORM && egrid_kind != MCS_EGRID_LOG)
+        return mcs_fail(ctx, MCS_E_ARG, "table: unknown energy grid kind %d", egrid_kind);
+    if (!(maxrate > 0.0)) return mcs_fail(ctx, MCS_E_ARG, "table: maxrate must be positive");
+    for (int k = 1; k < ne; ++k)
+        if (!(energy[k] > energy[k - 1]))
+            return mcs_fail(ctx, MCS_E_ARG, "table: energy nodes must increase");
+    if (egrid_kind == MCS_EGRID_LOG && !(energy[0] > 0.0))
+        return mcs_fail(ctx, MCS_E_ARG, "table: log grid needs a positive first node");
+    for (int j = 0; j < nproc; ++j)
+        if (kind[j] < MCS_PROC_NULL || kind[j] > MCS_PROC_ATTACHMENT)
+            return mcs_fail(ctx, MCS_E_ARG, "table: process %d has unknown kind %d", j, kind[j]);
+    hipSetDevice(ctx->device);
+    mcs_table *t = new mcs_table();
+    t->ctx = ctx;
+    t->nproc = nproc;
+    t->ne = ne;
+    t->egrid = egrid_kind;
+    t->maxrate = maxrate;
+    t->step = (energy[ne - 1] - energy[0]) / (double)(ne - 1);
+    t->inv_dlog = egrid_kind == MCS_EGRID_LOG
+                      ? (double)(ne - 1) / mcs_log(energy[ne - 1] / energy[0])
+                      : 0.0;
+    t->h_kind.assign(kind, kind + nproc);
+    if (hipMalloc(&t->energy, sizeof(double) * ne) != hipSuccess ||
+        hipMalloc(&t->rate, sizeof(double) * (size_t)ne * nproc) != hipSuccess ||
+        hipMalloc(&t->kind, sizeof(int32_t) * nproc) != hipSuccess ||
+        hipMalloc(&t->param, sizeof(double) * nproc) != hipSuccess) {
+        delete t;
+        return mcs_fail(ctx, MCS_E_HIP, "table: device allocation failed");
+    }
+    hipMemcpy(t->energy, energy, sizeof(double) * ne, hipMemcpyHostToDevice);
+    hipMemcpy(t->rate, rate, sizeof(double) * (size_t)ne * nproc, hipMemcpyHostToDevice);
+    hipMemcpy(t->kind, kind, sizeof(int32_t) * nproc, hipMemcpyHostToDevice);
+    hipMemcpy(t->param, param, sizeof(double) * nproc, hipMemcpyHostToDevice);
+    *out = t;
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_table_destroy(mcs_table *t)
+{
+    if (!t) return MCS_OK;
+    hipSetDevice(t->ctx->device);
+    hipFree(t->energy);
+    hipFree(t->rate);
+    hipFree(t->kind);
+    hipFree(t->param);
+    delete t;
+    return MCS_OK;
+}
+
+// ---------------------------------------------------------------- population
+
+static int32_t alloc_buf(mcs_ctx *ctx, PopBuf &b, int64_t cap)
+{
+    for (int d = 0; d < 3; ++d) {
+        MCS_HIP(ctx, hipMalloc(&b.x[d], sizeof(double) * cap));
+        MCS_HIP(ctx, hipMalloc(&b.v[d], sizeof(double) * cap));
+    }
+    MCS_HIP(ctx, hipMalloc(&b.w, sizeof(double) * cap));
+    MCS_HIP(ctx, hipMalloc(&b.s, sizeof(double) * cap));
+    MCS_HIP(ctx, hipMalloc(&b.active, cap));
+    MCS_HIP(ctx, hipMalloc(&b.id, sizeof(uint64_t) * cap));
+    return MCS_OK;
+}
+
+static void free_buf(PopBuf &b)
+{
+    for (int d = 0; d < 3; ++d) {
+        hipFree(b.x[d]);
+        hipFree(b.v[d]);
+    }
+    hipFree(b.w);
+    hipFree(b.s);
+    hipFree(b.active);
+    hipFree(b.id);
+    b = PopBuf();
+}
+
+extern "C" int32_t mcs_pop_create(mcs_ctx *ctx, int64_t capacity, mcs_pop **out)
+{
+    if (!ctx || !out) return MCS_E_ARG;
+    if (capacity < 1 || capacity >= (int64_t)1 << 31)
+        return mcs_fail(ctx, MCS_E_ARG, "population capacity must be in [1, 2^31)");
+    hipSetDevice(ctx->device);
+    mcs_pop *p = new mcs_pop();
+    p->ctx = ctx;
+    p->cap = capacity;
+    p->n = 0;
+    p->next_id = 0;
+    int32_t rc = alloc_buf(ctx, p->buf[0], capacity);
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    p->logcap = capacity;
+    int64_t nblk = cdiv(capacity, MCS_BLOCK) + 2;
+    if (hipMalloc(&p->log_parent, sizeof(int32_t) * p->logcap) != hipSuccess ||
+        hipMalloc(&p->log_seq, sizeof(int32_t) * p->logcap) != hipSuccess ||
+        hipMalloc(&p->log_w, sizeof(double) * p->logcap) != hipSuccess ||
+        hipMalloc(&p->log_s, sizeof(double) * p->logcap) != hipSuccess ||
+        hipMalloc(&p->log_act, p->logcap) != hipSuccess ||
+        hipMalloc(&p->i32a, sizeof(int32_t) * (capacity + 1)) != hipSuccess ||
+        hipMalloc(&p->i32b, sizeof(int32_t) * (capacity + 1)) != hipSuccess ||
+        hipMalloc(&p->i32c, sizeof(int32_t) * (capacity + 1)) != hipSuccess ||
+        hipMalloc(&p->u64a, sizeof(uint64_t) * (capacity + 1)) != hipSuccess ||
+        hipMalloc(&p->blk, sizeof(int64_t) * (2 * nblk + 4)) != hipSuccess)
+        return mcs_fail(ctx, MCS_E_HIP, "population: device allocation failed");
+    for (int d = 0; d < 3; ++d) {
+        if (hipMalloc(&p->log_x[d], sizeof(double) * p->logcap) != hipSuccess ||
+            hipMalloc(&p->log_v[d], sizeof(double) * p->logcap) != hipSuccess)
+            return mcs_fail(ctx, MCS_E_HIP, "population: device allocation failed");
+    }
+    *out = p;
+    return MCS_OK;
+}
+
+int32_t mcs_pop_ensure_alt(mcs_pop *p)
+{
+    if (p->has_alt) return MCS_OK;
+    int32_t rc = alloc_buf(p->ctx, p->buf[1], p->cap);
+    if (rc) return rc;
+    p->has_alt = true;
+    return MCS_OK;
+}
+
+void mcs_pop_swap(mcs_pop *p)
+{
+    PopBuf t = p->buf[0];
+    p->buf[0] = p->buf[1];
+    p->buf[1] = t;
+}
+
+extern "C" int32_t mcs_pop_destroy(mcs_pop *p)
+{
+    if (!p) return MCS_OK;
+    hipSetDevice(p->ctx->device);
+    hipStreamSynchronize(p->ctx->stream);
+    free_buf(p->buf[0]);
+    if (p->has_alt) free_buf(p->buf[1]);
+    hipFree(p->log_parent);
+    hipFree(p->log_seq);
+    for (int d = 0; d < 3; ++d) {
+        hipFree(p->log_x[d]);
+        hipFree(p->log_v[d]);
+    }
+    hipFree(p->log_w);
+    hipFree(p->log_s);
+    hipFree(p->log_act);
+    hipFree(p->i32a);
+    hipFree(p->i32b);
+    hipFree(p->i32c);
+    hipFree(p->u64a);
+    hipFree(p->blk);
+    delete p;
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_pop_upload(mcs_pop *p, int64_t n, const double *x, const double *v,
+                                  const double *w, const double *s, const uint8_t *active,
+                                  const uint64_t *id, uint64_t next_id)
+{
+    if (!p) return MCS_E_ARG;
+    mcs_ctx *ctx = p->ctx;
+    if (n < 0 || n > p->cap)
+        return mcs_fail(ctx, MCS_E_CAPACITY, "Maximum number of particles reached (%lld > %lld)",
+                        (long long)n, (long long)p->cap);
+    if (n > 0 && (!x || !v || !w || !s || !active))
+        return mcs_fail(ctx, MCS_E_ARG, "pop_upload: null array");
+    hipSetDevice(ctx->device);
+    MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<double> tmp((size_t)n);
+    PopBuf &b = p->buf[0];
+    for (int d = 0; d < 3; ++d) {
+        for (int64_t i = 0; i < n; ++i) tmp[i] = x[3 * i + d];
+        MCS_HIP(ctx, hipMemcpy(b.x[d], tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        for (int64_t i = 0; i < n; ++i) tmp[i] = v[3 * i + d];
+        MCS_HIP(ctx, hipMemcpy(b.v[d], tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    }
+    MCS_HIP(ctx, hipMemcpy(b.w, w, sizeof(double) * n, hipMemcpyHostToDevice));
+    MCS_HIP(ctx, hipMemcpy(b.s, s, sizeof(double) * n, hipMemcpyHostToDevice));
+    MCS_HIP(ctx, hipMemcpy(b.active, active, n, hipMemcpyHostToDevice));
+    uint64_t nid = next_id;
+    if (id) {
+        MCS_HIP(ctx, hipMemcpy(b.id, id, sizeof(uint64_t) * n, hipMemcpyHostToDevice));
+        for (int64_t i = 0; i < n; ++i)
+            if (id[i] + 1 > nid) nid = id[i] + 1;
+    } else {
+        std::vector<uint64_t> ids((size_t)n);
+        for (int64_t i = 0; i < n; ++i) ids[i] = next_id + (uint64_t)i;
+        MCS_HIP(ctx, hipMemcpy(b.id, ids.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice));
+        nid = next_id + (uint64_t)n;
+    }
+    p->n = n;
+    p->next_id = nid;
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_pop_download(mcs_pop *p, int64_t nmax, int64_t *nout, double *x,
+                                    double *v, double *w, double *s, uint8_t *active,
+                                    uint64_t *id, uint64_t *next_id)
+{
+    if (!p) return MCS_E_ARG;
+    mcs_ctx *ctx = p->ctx;
+    hipSetDevice(ctx->device);
+    MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (nout) *nout = p->n;
+    if (next_id) *next_id = p->next_id;
+    int64_t n = p->n < nmax ? p->n : nmax;
+    if (n <= 0) return MCS_OK;
+    PopBuf &b = p->buf[0];
+    std::vector<double> tmp((size_t)n);
+    for (int d = 0; d < 3; ++d) {
+        if (x) {
+            MCS_HIP(ctx, hipMemcpy(tmp.data(), b.x[d], sizeof(double) * n, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; ++i) x[3 * i + d] = tmp[i];
+        }
+        if (v) {
+            MCS_HIP(ctx, hipMemcpy(tmp.data(), b.v[d], sizeof(double) * n, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; ++i) v[3 * i + d] = tmp[i];
+        }
+    }
+    if (w) MCS_HIP(ctx, hipMemcpy(w, b.w, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (s) MCS_HIP(ctx, hipMemcpy(s, b.s, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (active) MCS_HIP(ctx, hipMemcpy(active, b.active, n, hipMemcpyDeviceToHost));
+    if (id) MCS_HIP(ctx, hipMemcpy(id, b.id, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_pop_size(mcs_pop *p, int64_t *n)
+{
+    if (!p || !n) return MCS_E_ARG;
+    *n = p->n;
+    return MCS_OK;
+}
+
+// ---------------------------------------------------------------- moments
+
+struct Neu {
+    double s, c;
+};
+
+__device__ __forceinline__ void neu_add(Neu &a, double x)
+{
+    double t = a.s + x;
+    if (fabs(a.s) >= fabs(x))
+        a.c += (a.s - t) + x;
+    else
+        a.c += (x - t) + a.s;
+    a.s = t;
+}
+
+__device__ __forceinline__ void neu_merge(Neu &a, const Neu &b)
+{
+    double t = a.s + b.s;
+    double c;
+    if (fabs(a.s) >= fabs(b.s))
+        c = (a.s - t) + b.s;
+    else
+        c = (b.s - t) + a.s;
+    a.s = t;
+    a.c = a.c + b.c + c;
+}
+
+#define MOM_NV 13   // weight, wenergy, sum_energy, x[3], x2[3], v[3]
+#define MOM_REC (2 * MOM_NV + 2)  // + maxenergy + nactive
+#define MOM_BLOCKS 512
+
+__global__ __launch_bounds__(MCS_BLOCK) void k_moments(PopBuf b, int64_t n, double *partial)
+{
+    Neu acc[MOM_NV];
+#pragma unroll
+    for (int q = 0; q < MOM_NV; ++q) acc[q] = Neu{0.0, 0.0};
+    double emax = -HUGE_VAL;
+    double nact = 0.0;
+    for (int64_t i = gidx(); i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double v0 = b.v[0][i], v1 = b.v[1][i], v2 = b.v[2][i];
+        double eng = energy_dev(v0, v1, v2);
+        emax = fmax(emax, eng);
+        if (!b.active[i]) continue;
+        double x0 = b.x[0][i], x1 = b.x[1][i], x2 = b.x[2][i];
+        double w = b.w[i];
+        nact += 1.0;
+        neu_add(acc[0], w);
+        neu_add(acc[1], w * eng);
+        neu_add(acc[2], eng);
+        neu_add(acc[3], x0);
+        neu_add(acc[4], x1);
+        neu_add(acc[5], x2);
+        neu_add(acc[6], x0 * x0);
+        neu_add(acc[7], x1 * x1);
+        neu_add(acc[8], x2 * x2);
+        neu_add(acc[9], v0);
+        neu_add(acc[10], v1);
+        neu_add(acc[11], v2);
+    }
+    __shared__ double sh[MCS_BLOCK];
+    double *rec = partial + (int64_t)blockIdx.x * MOM_REC;
+    for (int q = 0; q < MOM_NV; ++q) {
+        // tree over the block for (s, c)
+        __shared__ double shc[MCS_BLOCK];
+        sh[threadIdx.x] = acc[q].s;
+        shc[threadIdx.x] = acc[q].c;
+        __syncthreads();
+        for (int o = MCS_BLOCK / 2; o > 0; o >>= 1) {
+            if (threadIdx.x < o) {
+                Neu a{sh[threadIdx.x], shc[threadIdx.x]};
+                Neu bb{sh[threadIdx.x + o], shc[threadIdx.x + o]};
+                neu_merge(a, bb);
+                sh[threadIdx.x] = a.s;
+                shc[threadIdx.x] = a.c;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            rec[2 * q] = sh[0];
+            rec[2 * q + 1] = shc[0];
+        }
+        __syncthreads();
+    }
+    sh[threadIdx.x] = emax;
+    __syncthreads();
+    for (int o = MCS_BLOCK / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) rec[2 * MOM_NV] = sh[0];
+    __syncthreads();
+    sh[threadIdx.x] = nact;
+    __syncthreads();
+    for (int o = MCS_BLOCK / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) rec[2 * MOM_NV + 1] = sh[0];
+}
+
+// Combine the block records in block order: thread q owns quantity q.
+__global__ void k_moments_final(const double *partial, int nb, double *out)
+{
+    int q = threadIdx.x;
+    if (q < MOM_NV) {
+        Neu a{0.0, 0.0};
+        for (int b = 0; b < nb; ++b) {
+            Neu r{partial[(int64_t)b * MOM_REC + 2 * q], partial[(int64_t)b * MOM_REC + 2 * q + 1]};
+            neu_merge(a, r);
+        }
+        out[q] = a.s + a.c;
+    } else if (q == MOM_NV) {
+        double m = -HUGE_VAL;
+        for (int b = 0; b < nb; ++b) m = fmax(m, partial[(int64_t)b * MOM_REC + 2 * MOM_NV]);
+        out[q] = m;
+    } else if (q == MOM_NV + 1) {
+        double c = 0.0;
+        for (int b = 0; b < nb; ++b) c += partial[(int64_t)b * MOM_REC + 2 * MOM_NV + 1];
+        out[q] = c;
+    }
+}
+
+#define CR_CHUNK 8
+
+__global__ __launch_bounds__(MCS_BLOCK) void k_collrate(PopBuf b, int64_t n, const double *energy,
+                                                        const double *rate, int32_t nproc,
+                                                        int32_t ne, int32_t egrid, double step,
+                                                        double inv_dlog, int j0, double *partial)
+{
+    Neu acc[CR_CHUNK];
+#pragma unroll
+    for (int q = 0; q < CR_CHUNK; ++q) acc[q] = Neu{0.0, 0.0};
+    int32_t flags = 0;
+    for (int64_t i = gidx(); i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (!b.active[i]) continue;
+        double eng = energy_dev(b.v[0][i], b.v[1][i], b.v[2][i]);
+        int64_t k;
+        double w;
+        indweight_dev(energy, ne, egrid, step, inv_dlog, eng, k, w, flags);
+#pragma unroll
+        for (int q = 0; q < CR_CHUNK; ++q) {
+            int j = j0 + q;
+            if (j < nproc)
+                neu_add(acc[q], w * rate[k * nproc + j] + (1.0 - w) * rate[(k + 1) * nproc + j]);
+        }
+    }
+    __shared__ double sh[MCS_BLOCK], shc[MCS_BLOCK];
+    for (int q = 0; q < CR_CHUNK; ++q) {
+        sh[threadIdx.x] = acc[q].s;
+        shc[threadIdx.x] = acc[q].c;
+        __syncthreads();
+        for (int o = MCS_BLOCK / 2; o > 0; o >>= 1) {
+            if (threadIdx.x < o) {
+                Neu a{sh[threadIdx.x], shc[threadIdx.x]};
+                Neu bb{sh[threadIdx.x + o], shc[threadIdx.x + o]};
+                neu_merge(a, bb);
+                sh[threadIdx.x] = a.s;
+                shc[threadIdx.x] = a.c;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            partial[((int64_t)blockIdx.x * CR_CHUNK + q) * 2] = sh[0];
+            partial[((int64_t)blockIdx.x * CR_CHUNK + q) * 2 + 1] = shc[0];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_collrate_final(const double *partial, int nb, double *out)
+{
+    int q = threadIdx.x;
+    if (q >= CR_CHUNK) return;
+    Neu a{0.0, 0.0};
+    for (int b = 0; b < nb; ++b) {
+        Neu r{partial[((int64_t)b * CR_CHUNK + q) * 2], partial[((int64_t)b * CR_CHUNK + q) * 2 + 1]};
+        neu_merge(a, r);
+    }
+    out[q] = a.s + a.c;
+}
+
+extern "C" int32_t mcs_pop_moments(mcs_pop *p, const mcs_table *t, mcs_moments *out)
+{
+    if (!p || !out) return MCS_E_ARG;
+    mcs_ctx *ctx = p->ctx;
+    hipSetDevice(ctx->device);
+    memset(out, 0, sizeof(*out));
+    out->nparticles = p->n;
+    if (p->n == 0) return MCS_OK;
+    int nb = (int)cdiv(p->n, MCS_BLOCK);
+    if (nb > MOM_BLOCKS) nb = MOM_BLOCKS;
+    void *scr;
+    int32_t rc = mcs_scratch(ctx, sizeof(double) * ((size_t)nb * (MOM_REC + 2 * CR_CHUNK) + 64), &scr);
+    if (rc) return rc;
+    double *partial = (double *)scr;
+    double *res = partial + (size_t)nb * (MOM_REC + 2 * CR_CHUNK);
+    hipLaunchKernelGGL(k_moments, dim3(nb), dim3(MCS_BLOCK), 0, ctx->stream, p->buf[0], p->n,
+                       partial);
+    hipLaunchKernelGGL(k_moments_final, dim3(1), dim3(64), 0, ctx->stream, partial, nb, res);
+    MCS_LAUNCH_CHECK(ctx);
+    double h[MOM_NV + 2];
+    MCS_HIP(ctx, hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    out->weight = h[0];
+    out->wenergy = h[1];
+    out->sum_energy = h[2];
+    for (int d = 0; d < 3; ++d) {
+        out->sum_x[d] = h[3 + d];
+        out->sum_x2[d] = h[6 + d];
+        out->sum_v[d] = h[9 + d];
+    }
+    out->maxenergy = h[MOM_NV];
+    out->nactive = (int64_t)h[MOM_NV + 1];
+    if (t) {
+        for (int j0 = 0; j0 < t->nproc; j0 += CR_CHUNK) {
+            hipLaunchKernelGGL(k_collrate, dim3(nb), dim3(MCS_BLOCK), 0, ctx->stream, p->buf[0],
+                               p->n, t->energy, t->rate, t->nproc, t->ne, t->egrid, t->step,
+                               t->inv_dlog, j0, partial);
+            hipLaunchKernelGGL(k_collrate_final, dim3(1), dim3(64), 0, ctx->stream, partial, nb,
+                               res);
+            MCS_LAUNCH_CHECK(ctx);
+            double cr[CR_CHUNK];
+            MCS_HIP(ctx, hipMemcpyAsync(cr, res, sizeof(cr), hipMemcpyDeviceToHost, ctx->stream));
+            MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            for (int q = 0; q < CR_CHUNK && j0 + q < t->nproc; ++q) out->collrate[j0 + q] = cr[q];
+        }
+    }
+    return MCS_OK;
+}

@@ -1,0 +1,208 @@
+// mcs_device.h -- device-side building blocks shared by the libmcs kernels.
+#pragma once
+
+#include "mcs_internal.h"
+
+struct TableView {
+    const double *energy;
+    const double *rate;
+    const int32_t *kind;
+    const double *param;
+    int32_t nproc, ne, egrid;
+    double step, inv_dlog, maxrate;
+};
+
+struct MeshView {
+    int32_t n[3];
+    double h;
+    const double *ex, *ey, *ez;
+    double *qfixed;
+};
+
+static inline TableView table_view(const mcs_table *t)
+{
+    TableView v;
+    v.energy = t->energy;
+    v.rate = t->rate;
+    v.kind = t->kind;
+    v.param = t->param;
+    v.nproc = t->nproc;
+    v.ne = t->ne;
+    v.egrid = t->egrid;
+    v.step = t->step;
+    v.inv_dlog = t->inv_dlog;
+    v.maxrate = t->maxrate;
+    return v;
+}
+
+static inline MeshView mesh_view(const mcs_mesh *m)
+{
+    MeshView v;
+    for (int d = 0; d < 3; ++d) v.n[d] = m ? m->n[d] : 0;
+    v.h = m ? m->h : 0.0;
+    v.ex = m ? m->arr[MCS_ARR_EX] : nullptr;
+    v.ey = m ? m->arr[MCS_ARR_EY] : nullptr;
+    v.ez = m ? m->arr[MCS_ARR_EZ] : nullptr;
+    v.qfixed = m ? m->arr[MCS_ARR_QFIXED] : nullptr;
+    return v;
+}
+
+// energy(p) = 0.5*m*(v1^2 + v2^2 + v3^2)  (JuMC/src/electron.jl:27)
+__device__ __forceinline__ double energy_dev(double v0, double v1, double v2)
+{
+    return MCS_HALF_ME * ((v0 * v0 + v1 * v1) + v2 * v2);
+}
+
+// indweight (JuMC/src/util.jl:23-30) on a uniform or log grid; see the header of
+// mcs_table_create for the grid definitions.
+__device__ __forceinline__ void indweight_dev(const double *energy, int32_t ne, int32_t egrid,
+                                              double step, double inv_dlog, double eng,
+                                              int64_t &k, double &w, int32_t &flags)
+{
+    if (eng >= energy[ne - 1]) {
+        k = ne - 2;
+        w = 0.0;
+        flags |= MCS_FLAG_TABLE_RANGE;
+        return;
+    }
+    if (egrid == MCS_EGRID_UNIFORM) {
+        double kf = floor((eng - energy[0]) / step);
+        if (kf < 0.0) kf = 0.0;
+        int64_t kk = (int64_t)kf;
+        if (kk > ne - 2) kk = ne - 2;
+        double ww = (energy[kk + 1] - eng) / step;
+        k = kk;
+        w = fmin(fmax(ww, 0.0), 1.0);
+    } else {
+        if (eng <= energy[0]) {
+            k = 0;
+            w = 1.0;
+            return;
+        }
+        double kf = floor(mcs_log(eng / energy[0]) * inv_dlog);
+        if (kf < 0.0) kf = 0.0;
+        if (kf > (double)(ne - 2)) kf = (double)(ne - 2);
+        int64_t kk = (int64_t)kf;
+        while (kk > 0 && energy[kk] > eng) kk -= 1;
+        while (kk < ne - 2 && energy[kk + 1] <= eng) kk += 1;
+        double ww = (energy[kk + 1] - eng) / (energy[kk + 1] - energy[kk]);
+        k = kk;
+        w = fmin(fmax(ww, 0.0), 1.0);
+    }
+}
+
+// normdivrem + staggered (MCStreamer/src/gridfields.jl:194-209) for one axis.
+__device__ __forceinline__ void axis_index_dev(double xc, double h, int32_t n, int64_t &m,
+                                               double &d, int64_t &m1, double &d1)
+{
+    double r = xc / h;
+    int64_t mi = (int64_t)trunc(r);
+    if (mi > n - 1) mi = n - 1;
+    double di = r - (double)mi;
+    m = mi;
+    d = di;
+    if (di < 0.5) {
+        m1 = mi;
+        d1 = di + 0.5;
+    } else {
+        m1 = mi + 1;
+        d1 = di - 0.5;
+    }
+}
+
+__device__ __forceinline__ double trilerp_dev(const double *__restrict__ a, int64_t sx,
+                                              int64_t sxy, int64_t base, double dx, double dy,
+                                              double dz)
+{
+    double c00 = a[base] * (1.0 - dx) + a[base + 1] * dx;
+    double c10 = a[base + sx] * (1.0 - dx) + a[base + sx + 1] * dx;
+    double c01 = a[base + sxy] * (1.0 - dx) + a[base + sxy + 1] * dx;
+    double c11 = a[base + sxy + sx] * (1.0 - dx) + a[base + sxy + sx + 1] * dx;
+    double c0 = c00 * (1.0 - dy) + c10 * dy;
+    double c1 = c01 * (1.0 - dy) + c11 * dy;
+    return c0 * (1.0 - dz) + c1 * dz;
+}
+
+// FieldInterp (MCStreamer/src/gridfields.jl:215-290) in 3D Cartesian form: zero
+// outside the open box, staggered trilinear per component, |E| > 1e8 flagged.
+__device__ __forceinline__ void efield_mesh_dev(const MeshView &M, const double x[3],
+                                                double e[3], int32_t &flags)
+{
+    double h = M.h;
+    if (!(0.0 < x[0] && x[0] < (double)M.n[0] * h && 0.0 < x[1] && x[1] < (double)M.n[1] * h &&
+          0.0 < x[2] && x[2] < (double)M.n[2] * h)) {
+        e[0] = 0.0;
+        e[1] = 0.0;
+        e[2] = 0.0;
+        return;
+    }
+    int64_t mx, my, mz, m1x, m1y, m1z;
+    double dx, dy, dz, d1x, d1y, d1z;
+    axis_index_dev(x[0], h, M.n[0], mx, dx, m1x, d1x);
+    axis_index_dev(x[1], h, M.n[1], my, dy, m1y, d1y);
+    axis_index_dev(x[2], h, M.n[2], mz, dz, m1z, d1z);
+    int64_t nx = M.n[0], ny = M.n[1];
+    {
+        int64_t sx = nx + 1, sxy = (nx + 1) * (ny + 2);
+        e[0] = trilerp_dev(M.ex, sx, sxy, mx + sx * m1y + sxy * m1z, dx, d1y, d1z);
+    }
+    {
+        int64_t sx = nx + 2, sxy = (nx + 2) * (ny + 1);
+        e[1] = trilerp_dev(M.ey, sx, sxy, m1x + sx * my + sxy * m1z, d1x, dy, d1z);
+    }
+    {
+        int64_t sx = nx + 2, sxy = (nx + 2) * (ny + 2);
+        e[2] = trilerp_dev(M.ez, sx, sxy, m1x + sx * m1y + sxy * mz, d1x, d1y, dz);
+    }
+    double e2 = (e[0] * e[0] + e[1] * e[1]) + e[2] * e[2];
+    if (e2 > MCS_EFIELD_LIMIT2) flags |= MCS_FLAG_FIELD;
+}
+
+template <typename T>
+__device__ __forceinline__ T block_excl_scan(T v, T *total)
+{
+    __shared__ T wsum[MCS_BLOCK / MCS_WAVE];
+    int lane = threadIdx.x & (MCS_WAVE - 1), wid = threadIdx.x / MCS_WAVE;
+    T x = v;
+#pragma unroll
+    for (int o = 1; o < MCS_WAVE; o <<= 1) {
+        T y = __shfl_up(x, o, MCS_WAVE);
+        if (lane >= o) x += y;
+    }
+    if (lane == MCS_WAVE - 1) wsum[wid] = x;
+    __syncthreads();
+    T pre = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < MCS_BLOCK / MCS_WAVE; ++k) {
+        if (k < wid) pre += wsum[k];
+        tot += wsum[k];
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + x - v;
+}
+
+__device__ __forceinline__ void copy_particle(const PopBuf &src, int64_t si, const PopBuf &dst,
+                                              int64_t di)
+{
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        dst.x[d][di] = src.x[d][si];
+        dst.v[d][di] = src.v[d][si];
+    }
+    dst.w[di] = src.w[si];
+    dst.s[di] = src.s[si];
+    dst.active[di] = src.active[si];
+    dst.id[di] = src.id[si];
+}
+
+// cell index of a position (grid.jl:84-90 generalised): -1 outside [0, n h)^3
+__device__ __forceinline__ int64_t cell_of(const int32_t n[3], double h, double x0, double x1,
+                                           double x2)
+{
+    double f0 = floor(x0 / h), f1 = floor(x1 / h), f2 = floor(x2 / h);
+    if (!(f0 >= 0.0 && f0 < (double)n[0] && f1 >= 0.0 && f1 < (double)n[1] && f2 >= 0.0 &&
+          f2 < (double)n[2]))
+        return -1;
+    return (int64_t)f0 + (int64_t)n[0] * ((int64_t)f1 + (int64_t)n[1] * (int64_t)f2);
+}

@@ -1,0 +1,181 @@
+// mcs_internal.h -- shared definitions of libmcs (HIP, gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/mcs.h"
+#include "mcs_phys.h"
+#include "mcs_rng.h"
+
+#pragma clang fp contract(off)
+
+#define MCS_BLOCK 256
+#define MCS_WAVE 64
+
+#define MCS_FLAG_TABLE_RANGE 1
+#define MCS_FLAG_FIELD 2
+#define MCS_FLAG_CAPACITY 4
+
+struct ProfEntry {
+    double total_ms = 0.0;
+    int64_t launches = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+
+struct mcs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool prof = false;
+    std::map<std::string, ProfEntry> profile;
+    // small pinned host buffer for read-backs
+    int64_t *h_small = nullptr;
+    int64_t *d_small = nullptr;   // device counters block (512 int64)
+    // growable device scratch used by scans and reductions
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+};
+
+// set error text and return the code
+int32_t mcs_fail(mcs_ctx *ctx, int32_t code, const char *fmt, ...);
+
+#define MCS_HIP(ctx, call)                                                                  \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return mcs_fail((ctx), MCS_E_HIP, "%s failed: %s (%s:%d)", #call,               \
+                            hipGetErrorString(e_), __FILE__, __LINE__);                     \
+    } while (0)
+
+#define MCS_LAUNCH_CHECK(ctx)                                                               \
+    do {                                                                                    \
+        hipError_t e_ = hipGetLastError();                                                  \
+        if (e_ != hipSuccess)                                                               \
+            return mcs_fail((ctx), MCS_E_HIP, "kernel launch failed: %s (%s:%d)",          \
+                            hipGetErrorString(e_), __FILE__, __LINE__);                     \
+    } while (0)
+
+// Records a HIP event pair around the enclosed launches when profiling is on.
+struct ProfScope {
+    mcs_ctx *ctx;
+    const char *name;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(mcs_ctx *c, const char *n) : ctx(c), name(n)
+    {
+        if (ctx->prof) {
+            hipEventCreate(&a);
+            hipEventCreate(&b);
+            hipEventRecord(a, ctx->stream);
+        }
+    }
+    ~ProfScope()
+    {
+        if (ctx->prof) {
+            hipEventRecord(b, ctx->stream);
+            ctx->profile[name].pending.emplace_back(a, b);
+        }
+    }
+};
+
+// scratch of at least `bytes`, reused across calls on the context's stream
+int32_t mcs_scratch(mcs_ctx *ctx, size_t bytes, void **out);
+// pinned read-back of n int64 values from device
+int32_t mcs_readback(mcs_ctx *ctx, const int64_t *dev, int64_t *host, int n);
+
+struct mcs_table {
+    mcs_ctx *ctx;
+    int32_t nproc, ne, egrid;
+    double maxrate;
+    double inv_dlog;
+    double step;
+    double *energy;   // ne
+    double *rate;     // ne * nproc
+    int32_t *kind;    // nproc
+    double *param;    // nproc
+    std::vector<int32_t> h_kind;
+};
+
+// Particle SoA in HBM.  Buffer set 0 is live; set 1 receives sort/resample output.
+struct PopBuf {
+    double *x[3] = {nullptr, nullptr, nullptr};
+    double *v[3] = {nullptr, nullptr, nullptr};
+    double *w = nullptr, *s = nullptr;
+    uint8_t *active = nullptr;
+    uint64_t *id = nullptr;
+};
+
+struct mcs_pop {
+    mcs_ctx *ctx;
+    int64_t cap;
+    int64_t n;
+    uint64_t next_id;
+    PopBuf buf[2];
+    bool has_alt = false;
+    // birth log (structure of arrays)
+    int64_t logcap = 0;
+    int32_t *log_parent = nullptr;
+    int32_t *log_seq = nullptr;
+    double *log_x[3] = {nullptr, nullptr, nullptr};
+    double *log_v[3] = {nullptr, nullptr, nullptr};
+    double *log_w = nullptr, *log_s = nullptr;
+    uint8_t *log_act = nullptr;
+    // per-particle scratch (int32 x cap): local offsets, keys, ranks, permutation
+    int32_t *i32a = nullptr, *i32b = nullptr, *i32c = nullptr;
+    uint64_t *u64a = nullptr;
+    int64_t *blk = nullptr;   // block totals, then their prefix (2 * (cap/256 + 2) + 4)
+};
+
+struct mcs_mesh {
+    mcs_ctx *ctx;
+    int32_t n[3];
+    double h;
+    int64_t ncells;
+    int64_t ncg;      // cells with ghosts
+    double *arr[7];   // QFIXED..EZ
+    int64_t arr_len[7];
+    int32_t *cell_start = nullptr;  // ncells + 2
+    int32_t *cell_cnt = nullptr;    // ncells + 2 (counts, then the long-segment list)
+    int64_t *cnt64 = nullptr;       // ncells + 2 (per-cell resample output counts)
+    int64_t *pre64 = nullptr;       // ncells + 2 (their prefix)
+    bool sorted_valid = false;
+};
+
+struct MgLevel {
+    int32_t n[3];
+    int64_t len;
+    double *res, *res1, *sol;
+    float *fres, *fsol;   // FP32 smoother copies (precision mode)
+};
+
+struct mcs_mg {
+    mcs_ctx *ctx;
+    mcs_mg_config cfg;
+    std::vector<MgLevel> lv;
+    double *ainv = nullptr;   // coarse inverse, ncoarse^2
+    int64_t ncoarse = 0;
+    double *u_tmp = nullptr, *b_tmp = nullptr;  // host-solve staging (finest size)
+    double *partial = nullptr;                   // norm partial sums
+    int64_t npartial = 0;
+    int32_t *bc_dev = nullptr;                   // boundary kinds on the device
+    double tol_abs = 0.0;                        // tolerance in force for the current solve
+};
+
+// ---- device helpers --------------------------------------------------------
+
+__device__ __forceinline__ int64_t gidx() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+
+static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// exclusive scan of int32 counts (len entries) into out (len+1 entries, out[len] = total).
+int32_t mcs_scan_i32(mcs_ctx *ctx, const int32_t *in, int32_t *out, int64_t len);
+// exclusive scan of int64 values (len entries) into out (len+1 entries).
+int32_t mcs_scan_i64(mcs_ctx *ctx, const int64_t *in, int64_t *out, int64_t len);
+
+int32_t mcs_pop_ensure_alt(mcs_pop *p);
+void mcs_pop_swap(mcs_pop *p);

@@ -1,0 +1,677 @@
+// mcs_mesh.hip -- the mesh side of the step: canonical cell sort, charge
+// deposition, charge assembly, face fields and per-cell resampling.
+#include <cstring>
+
+#include "mcs_device.h"
+
+#define M_SORTED 0
+#define M_BIG 1
+
+// ---------------------------------------------------------------- mesh object
+
+static int64_t arr_len(const int32_t n[3], int which)
+{
+    int64_t nx = n[0], ny = n[1], nz = n[2];
+    switch (which) {
+    case MCS_ARR_EX: return (nx + 1) * (ny + 2) * (nz + 2);
+    case MCS_ARR_EY: return (nx + 2) * (ny + 1) * (nz + 2);
+    case MCS_ARR_EZ: return (nx + 2) * (ny + 2) * (nz + 1);
+    default: return (nx + 2) * (ny + 2) * (nz + 2);
+    }
+}
+
+extern "C" int32_t mcs_mesh_create(mcs_ctx *ctx, const int32_t n[3], double h, mcs_mesh **out)
+{
+    if (!ctx || !n || !out) return MCS_E_ARG;
+    if (n[0] < 1 || n[1] < 1 || n[2] < 1 || !(h > 0.0))
+        return mcs_fail(ctx, MCS_E_ARG, "mesh: sizes must be >= 1 and h > 0");
+    int64_t ncells = (int64_t)n[0] * n[1] * n[2];
+    if (ncells >= ((int64_t)1 << 31) - 2) return mcs_fail(ctx, MCS_E_ARG, "mesh too large");
+    hipSetDevice(ctx->device);
+    mcs_mesh *m = new mcs_mesh();
+    m->ctx = ctx;
+    for (int d = 0; d < 3; ++d) m->n[d] = n[d];
+    m->h = h;
+    m->ncells = ncells;
+    m->ncg = arr_len(n, MCS_ARR_Q);
+    for (int a = 0; a < 7; ++a) {
+        m->arr_len[a] = arr_len(n, a);
+        if (hipMalloc(&m->arr[a], sizeof(double) * m->arr_len[a]) != hipSuccess)
+            return mcs_fail(ctx, MCS_E_HIP, "mesh: device allocation failed");
+        hipMemset(m->arr[a], 0, sizeof(double) * m->arr_len[a]);
+    }
+    if (hipMalloc(&m->cell_start, sizeof(int32_t) * (ncells + 2)) != hipSuccess ||
+        hipMalloc(&m->cell_cnt, sizeof(int32_t) * (ncells + 2)) != hipSuccess ||
+        hipMalloc(&m->cnt64, sizeof(int64_t) * (ncells + 2)) != hipSuccess ||
+        hipMalloc(&m->pre64, sizeof(int64_t) * (ncells + 2)) != hipSuccess)
+        return mcs_fail(ctx, MCS_E_HIP, "mesh: device allocation failed");
+    hipMemset(m->cell_start, 0, sizeof(int32_t) * (ncells + 2));
+    *out = m;
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_mesh_destroy(mcs_mesh *m)
+{
+    if (!m) return MCS_OK;
+    hipSetDevice(m->ctx->device);
+    hipStreamSynchronize(m->ctx->stream);
+    for (int a = 0; a < 7; ++a) hipFree(m->arr[a]);
+    hipFree(m->cell_start);
+    hipFree(m->cell_cnt);
+    hipFree(m->cnt64);
+    hipFree(m->pre64);
+    delete m;
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_mesh_set(mcs_mesh *m, int32_t which, const double *host)
+{
+    if (!m || !host || which < 0 || which > 6) return MCS_E_ARG;
+    mcs_ctx *ctx = m->ctx;
+    hipSetDevice(ctx->device);
+    MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    MCS_HIP(ctx, hipMemcpy(m->arr[which], host, sizeof(double) * m->arr_len[which],
+                           hipMemcpyHostToDevice));
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_mesh_get(mcs_mesh *m, int32_t which, double *host)
+{
+    if (!m || !host || which < 0 || which > 6) return MCS_E_ARG;
+    mcs_ctx *ctx = m->ctx;
+    hipSetDevice(ctx->device);
+    MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    MCS_HIP(ctx, hipMemcpy(host, m->arr[which], sizeof(double) * m->arr_len[which],
+                           hipMemcpyDeviceToHost));
+    return MCS_OK;
+}
+
+__global__ void k_zero_f64(double *a, int64_t n)
+{
+    int64_t i = gidx();
+    if (i < n) a[i] = 0.0;
+}
+
+__global__ void k_zero_i32(int32_t *a, int64_t n)
+{
+    int64_t i = gidx();
+    if (i < n) a[i] = 0;
+}
+
+extern "C" int32_t mcs_mesh_zero(mcs_mesh *m, int32_t which)
+{
+    if (!m || which < 0 || which > 6) return MCS_E_ARG;
+    mcs_ctx *ctx = m->ctx;
+    hipSetDevice(ctx->device);
+    hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)cdiv(m->arr_len[which], MCS_BLOCK)), dim3(MCS_BLOCK),
+                       0, ctx->stream, m->arr[which], m->arr_len[which]);
+    MCS_LAUNCH_CHECK(ctx);
+    return MCS_OK;
+}
+
+// ---------------------------------------------------------------- sort by cell
+
+struct MeshGeom {
+    int32_t n[3];
+    double h;
+    int64_t ncells;
+};
+
+static inline MeshGeom geom(const mcs_mesh *m)
+{
+    MeshGeom g;
+    for (int d = 0; d < 3; ++d) g.n[d] = m->n[d];
+    g.h = m->h;
+    g.ncells = m->ncells;
+    return g;
+}
+
+// Counting pass.  Lanes of a wave that share a cell take consecutive ranks from a
+// single atomic (ballot over equal keys), so a mostly sorted population costs one
+// atomic per cell run rather than one per particle.
+__global__ __launch_bounds__(MCS_BLOCK) void k_sort_count(PopBuf b, int64_t n, MeshGeom g,
+                                                          int32_t *cnt, int32_t *key_out,
+                                                          int32_t *rank_out)
+{
+    int64_t i = gidx();
+    int lane = threadIdx.x & (MCS_WAVE - 1);
+    int key = -1;
+    if (i < n && b.active[i]) {
+        int64_t c = cell_of(g.n, g.h, b.x[0][i], b.x[1][i], b.x[2][i]);
+        key = c < 0 ? (int)g.ncells : (int)c;
+    }
+    unsigned long long todo = __ballot(key >= 0);
+    int rank = 0;
+    while (todo) {
+        int leader = __ffsll((unsigned long long)todo) - 1;
+        int lk = __shfl(key, leader, MCS_WAVE);
+        unsigned long long peers = __ballot(key == lk) & todo;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&cnt[lk], __popcll(peers));
+        base = __shfl(base, leader, MCS_WAVE);
+        if ((peers >> lane) & 1ULL) rank = base + __popcll(peers & ((1ULL << lane) - 1ULL));
+        todo &= ~peers;
+    }
+    if (i < n) {
+        key_out[i] = key;
+        rank_out[i] = rank;
+    }
+}
+
+__global__ __launch_bounds__(MCS_BLOCK) void k_sort_scatter(PopBuf b, int64_t n,
+                                                            const int32_t *key,
+                                                            const int32_t *rank,
+                                                            const int32_t *cs, int32_t *perm,
+                                                            uint64_t *sid)
+{
+    int64_t i = gidx();
+    if (i >= n) return;
+    int k = key[i];
+    if (k < 0) return;
+    int64_t pos = (int64_t)cs[k] + rank[i];
+    perm[pos] = (int32_t)i;
+    sid[pos] = b.id[i];
+}
+
+#define SEG_SMALL 64
+
+// Within each cell, order by particle id (insertion sort for short segments;
+// long ones are listed for the block-wide rank sort below).
+__global__ __launch_bounds__(MCS_BLOCK) void k_segsort_small(const int32_t *cs, int64_t nseg,
+                                                             int32_t *perm, uint64_t *sid,
+                                                             int32_t *biglist, int64_t *nbig)
+{
+    int64_t c = gidx();
+    if (c >= nseg) return;
+    int64_t a = cs[c], e = cs[c + 1];
+    int64_t len = e - a;
+    if (len <= 1) return;
+    if (len > SEG_SMALL) {
+        unsigned long long k = atomicAdd((unsigned long long *)nbig, 1ULL);
+        biglist[k] = (int32_t)c;
+        return;
+    }
+    for (int64_t q = a + 1; q < e; ++q) {
+        uint64_t key = sid[q];
+        int32_t pv = perm[q];
+        int64_t r = q - 1;
+        while (r >= a && sid[r] > key) {
+            sid[r + 1] = sid[r];
+            perm[r + 1] = perm[r];
+            r -= 1;
+        }
+        sid[r + 1] = key;
+        perm[r + 1] = pv;
+    }
+}
+
+__global__ __launch_bounds__(MCS_BLOCK) void k_segsort_big(const int32_t *cs, int32_t *perm,
+                                                           uint64_t *sid, const int32_t *biglist,
+                                                           const int64_t *nbig, int32_t *tmp_perm,
+                                                           uint64_t *tmp_sid)
+{
+    int64_t nb = *nbig;
+    for (int64_t q = blockIdx.x; q < nb; q += gridDim.x) {
+        int32_t c = biglist[q];
+        int64_t a = cs[c], e = cs[c + 1];
+        for (int64_t t = a + threadIdx.x; t < e; t += blockDim.x) {
+            uint64_t key = sid[t];
+            int64_t rank = 0;
+            for (int64_t u = a; u < e; ++u) rank += sid[u] < key ? 1 : 0;
+            tmp_perm[a + rank] = perm[t];
+            tmp_sid[a + rank] = key;
+        }
+        __syncthreads();
+        for (int64_t t = a + threadIdx.x; t < e; t += blockDim.x) {
+            perm[t] = tmp_perm[t];
+            sid[t] = tmp_sid[t];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(MCS_BLOCK) void k_sort_gather(PopBuf src, PopBuf dst,
+                                                           const int32_t *perm, int64_t na)
+{
+    int64_t pos = gidx();
+    if (pos >= na) return;
+    copy_particle(src, perm[pos], dst, pos);
+}
+
+extern "C" int32_t mcs_sort_by_cell(mcs_pop *p, mcs_mesh *m)
+{
+    if (!p || !m) return MCS_E_ARG;
+    mcs_ctx *ctx = p->ctx;
+    hipSetDevice(ctx->device);
+    int32_t rc = mcs_pop_ensure_alt(p);
+    if (rc) return rc;
+    int64_t n = p->n;
+    int64_t nseg = m->ncells + 1;
+    hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)cdiv(nseg + 1, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                       ctx->stream, m->cell_cnt, nseg + 1);
+    int64_t *ctr = ctx->d_small;
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, ctx->stream, (int32_t *)ctr, (int64_t)8);
+    if (n > 0) {
+        ProfScope ps(ctx, "sort_count");
+        hipLaunchKernelGGL(k_sort_count, dim3((unsigned)cdiv(n, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                           ctx->stream, p->buf[0], n, geom(m), m->cell_cnt, p->i32a, p->i32b);
+    }
+    MCS_LAUNCH_CHECK(ctx);
+    rc = mcs_scan_i32(ctx, m->cell_cnt, m->cell_start, nseg);
+    if (rc) return rc;
+    if (n > 0) {
+        ProfScope ps(ctx, "sort_scatter");
+        hipLaunchKernelGGL(k_sort_scatter, dim3((unsigned)cdiv(n, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                           ctx->stream, p->buf[0], n, (const int32_t *)p->i32a,
+                           (const int32_t *)p->i32b, (const int32_t *)m->cell_start, p->i32c,
+                           p->u64a);
+        // the count buffer is free again: it holds the list of long segments
+        hipLaunchKernelGGL(k_segsort_small, dim3((unsigned)cdiv(nseg, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                           ctx->stream, (const int32_t *)m->cell_start, nseg, p->i32c, p->u64a,
+                           m->cell_cnt, ctr + M_BIG);
+        hipLaunchKernelGGL(k_segsort_big, dim3(512), dim3(MCS_BLOCK), 0, ctx->stream,
+                           (const int32_t *)m->cell_start, p->i32c, p->u64a,
+                           (const int32_t *)m->cell_cnt, (const int64_t *)(ctr + M_BIG), p->i32b,
+                           (uint64_t *)p->buf[1].id);
+    }
+    MCS_LAUNCH_CHECK(ctx);
+    int64_t na;
+    {
+        int32_t tot;
+        MCS_HIP(ctx, hipMemcpyAsync(&tot, m->cell_start + nseg, sizeof(int32_t),
+                                    hipMemcpyDeviceToHost, ctx->stream));
+        MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        na = tot;
+    }
+    if (na > 0) {
+        ProfScope ps(ctx, "sort_gather");
+        hipLaunchKernelGGL(k_sort_gather, dim3((unsigned)cdiv(na, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                           ctx->stream, p->buf[0], p->buf[1], (const int32_t *)p->i32c, na);
+    }
+    MCS_LAUNCH_CHECK(ctx);
+    mcs_pop_swap(p);
+    p->n = na;
+    m->sorted_valid = true;
+    return MCS_OK;
+}
+
+// ---------------------------------------------------------------- deposit
+
+struct DepArgs {
+    PopBuf b;
+    const int32_t *cs;
+    MeshGeom g;
+    double *arr;
+    double val;
+};
+
+// density! as a per-cell gather over the canonical order: cell c sums the shares of
+// the particles of its 27 neighbours in (cell, id) order, which is the order in
+// which a sequential scatter over the sorted population adds them.  No atomics.
+template <int CIC>
+__global__ __launch_bounds__(MCS_BLOCK) void k_deposit(DepArgs a)
+{
+    int64_t c = gidx();
+    if (c >= a.g.ncells) return;
+    const int64_t nx = a.g.n[0], ny = a.g.n[1], nz = a.g.n[2];
+    int64_t i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+    int64_t idx = (i + 1) + (nx + 2) * ((j + 1) + (ny + 2) * (k + 1));
+    const double h = a.g.h;
+    const double dV = (h * h) * h;
+    double acc = a.arr[idx];
+    if (!CIC) {
+        int64_t s0 = a.cs[c], s1 = a.cs[c + 1];
+        for (int64_t q = s0; q < s1; ++q) acc += (a.b.w[q] * a.val) / dV;
+        a.arr[idx] = acc;
+        return;
+    }
+    for (int dk = -1; dk <= 1; ++dk) {
+        int64_t kk = k + dk;
+        if (kk < 0 || kk >= nz) continue;
+        for (int dj = -1; dj <= 1; ++dj) {
+            int64_t jj = j + dj;
+            if (jj < 0 || jj >= ny) continue;
+            for (int di = -1; di <= 1; ++di) {
+                int64_t ii = i + di;
+                if (ii < 0 || ii >= nx) continue;
+                int64_t nc = ii + nx * (jj + ny * kk);
+                int64_t s0 = a.cs[nc], s1 = a.cs[nc + 1];
+                for (int64_t q = s0; q < s1; ++q) {
+                    double gx = a.b.x[0][q] / h - 0.5;
+                    double gy = a.b.x[1][q] / h - 0.5;
+                    double gz = a.b.x[2][q] / h - 0.5;
+                    double fx = floor(gx), fy = floor(gy), fz = floor(gz);
+                    int64_t ix = (int64_t)fx, iy = (int64_t)fy, iz = (int64_t)fz;
+                    if ((i != ix && i != ix + 1) || (j != iy && j != iy + 1) ||
+                        (k != iz && k != iz + 1))
+                        continue;
+                    double wx = i == ix ? 1.0 - (gx - fx) : gx - fx;
+                    double wy = j == iy ? 1.0 - (gy - fy) : gy - fy;
+                    double wz = k == iz ? 1.0 - (gz - fz) : gz - fz;
+                    double base = (a.b.w[q] * a.val) / dV;
+                    acc += base * ((wx * wy) * wz);
+                }
+            }
+        }
+    }
+    a.arr[idx] = acc;
+}
+
+extern "C" int32_t mcs_deposit(mcs_pop *p, mcs_mesh *m, int32_t which, int32_t scheme, double val)
+{
+    if (!p || !m) return MCS_E_ARG;
+    mcs_ctx *ctx = p->ctx;
+    if (which != MCS_ARR_QFIXED && which != MCS_ARR_QPART)
+        return mcs_fail(ctx, MCS_E_ARG, "deposit: target must be QFIXED or QPART");
+    if (scheme != MCS_DEPOSIT_NGP && scheme != MCS_DEPOSIT_CIC)
+        return mcs_fail(ctx, MCS_E_ARG, "deposit: unknown scheme %d", scheme);
+    if (!m->sorted_valid)
+        return mcs_fail(ctx, MCS_E_ARG, "deposit: population is not in cell order (call mcs_sort_by_cell)");
+    hipSetDevice(ctx->device);
+    DepArgs a;
+    a.b = p->buf[0];
+    a.cs = m->cell_start;
+    a.g = geom(m);
+    a.arr = m->arr[which];
+    a.val = val;
+    {
+        ProfScope ps(ctx, "deposit");
+        dim3 g((unsigned)cdiv(m->ncells, MCS_BLOCK)), bl(MCS_BLOCK);
+        if (scheme == MCS_DEPOSIT_CIC)
+            hipLaunchKernelGGL(k_deposit<1>, g, bl, 0, ctx->stream, a);
+        else
+            hipLaunchKernelGGL(k_deposit<0>, g, bl, 0, ctx->stream, a);
+    }
+    MCS_LAUNCH_CHECK(ctx);
+    return MCS_OK;
+}
+
+// ---------------------------------------------------------------- charge, faces
+
+__global__ __launch_bounds__(MCS_BLOCK) void k_assemble(MeshGeom g, const double *qf,
+                                                        const double *qp, double *q)
+{
+    int64_t c = gidx();
+    int64_t sx = g.n[0] + 2, sy = g.n[1] + 2;
+    int64_t tot = sx * sy * (g.n[2] + 2);
+    if (c >= tot) return;
+    int64_t i = c % sx, j = (c / sx) % sy, k = c / (sx * sy);
+    bool interior = i >= 1 && i <= g.n[0] && j >= 1 && j <= g.n[1] && k >= 1 && k <= g.n[2];
+    q[c] = interior ? (qf[c] + qp[c]) * MCS_ELEMENTARY_CHARGE : 0.0;
+}
+
+extern "C" int32_t mcs_assemble_charge(mcs_mesh *m)
+{
+    if (!m) return MCS_E_ARG;
+    mcs_ctx *ctx = m->ctx;
+    hipSetDevice(ctx->device);
+    hipLaunchKernelGGL(k_assemble, dim3((unsigned)cdiv(m->ncg, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                       ctx->stream, geom(m), (const double *)m->arr[MCS_ARR_QFIXED],
+                       (const double *)m->arr[MCS_ARR_QPART], m->arr[MCS_ARR_Q]);
+    MCS_LAUNCH_CHECK(ctx);
+    return MCS_OK;
+}
+
+__device__ __forceinline__ int64_t clampi(int64_t v, int64_t lo, int64_t hi)
+{
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
+// Face fields (MCStreamer/src/poisson.jl:47-57).  Ghost faces take the value of
+// the nearest interior face, which is what the reference's ghost copies produce.
+__global__ __launch_bounds__(MCS_BLOCK) void k_faces(MeshGeom g, const double *u, double *ex,
+                                                     double *ey, double *ez, double eb)
+{
+    int64_t t = gidx();
+    const int64_t nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const int64_t ux = nx + 2, uxy = (nx + 2) * (ny + 2);
+    const double h = g.h;
+    int64_t lx = (nx + 1) * (ny + 2) * (nz + 2);
+    int64_t ly = (nx + 2) * (ny + 1) * (nz + 2);
+    int64_t lz = (nx + 2) * (ny + 2) * (nz + 1);
+    if (t < lx) {
+        int64_t i = t % (nx + 1), j = (t / (nx + 1)) % (ny + 2), k = t / ((nx + 1) * (ny + 2));
+        int64_t jj = clampi(j, 1, ny), kk = clampi(k, 1, nz);
+        int64_t c = i + ux * jj + uxy * kk;
+        ex[t] = (u[c] - u[c + 1]) / h;
+        return;
+    }
+    t -= lx;
+    if (t < ly) {
+        int64_t i = t % (nx + 2), j = (t / (nx + 2)) % (ny + 1), k = t / ((nx + 2) * (ny + 1));
+        int64_t ii = clampi(i, 1, nx), kk = clampi(k, 1, nz);
+        int64_t c = ii + ux * j + uxy * kk;
+        ey[t] = (u[c] - u[c + ux]) / h;
+        return;
+    }
+    t -= ly;
+    if (t < lz) {
+        int64_t i = t % (nx + 2), j = (t / (nx + 2)) % (ny + 2), k = t / ((nx + 2) * (ny + 2));
+        int64_t ii = clampi(i, 1, nx), jj = clampi(j, 1, ny);
+        int64_t c = ii + ux * jj + uxy * k;
+        ez[t] = eb + (u[c] - u[c + uxy]) / h;
+    }
+}
+
+extern "C" int32_t mcs_efield_faces(mcs_mesh *m, double eb)
+{
+    if (!m) return MCS_E_ARG;
+    mcs_ctx *ctx = m->ctx;
+    hipSetDevice(ctx->device);
+    int64_t tot = m->arr_len[MCS_ARR_EX] + m->arr_len[MCS_ARR_EY] + m->arr_len[MCS_ARR_EZ];
+    {
+        ProfScope ps(ctx, "faces");
+        hipLaunchKernelGGL(k_faces, dim3((unsigned)cdiv(tot, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                           ctx->stream, geom(m), (const double *)m->arr[MCS_ARR_U],
+                           m->arr[MCS_ARR_EX], m->arr[MCS_ARR_EY], m->arr[MCS_ARR_EZ], eb);
+    }
+    MCS_LAUNCH_CHECK(ctx);
+    return MCS_OK;
+}
+
+// ---------------------------------------------------------------- resample!
+
+#define R_MERGED 2
+#define R_CLONES 3
+#define R_SPLITS 4
+#define R_REMOVED 5
+
+// One cell of MCStreamer/src/resample.jl:16-76 on the canonical order (the
+// oracle restatement documents the rules).  WRITE = false counts the outputs;
+// WRITE = true replays the same draws and writes them: surviving originals first,
+// then the created particles, ids next_id + (created before this one).
+template <bool WRITE>
+__device__ void resample_cell(int64_t c, int64_t a, int64_t e, int32_t ntarget, int32_t nmax,
+                              uint64_t seed, uint32_t instance, uint32_t step, const PopBuf &src,
+                              const PopBuf &dst, int64_t base, uint64_t idbase, int64_t &nout,
+                              int64_t &ncreated, int64_t *ctr)
+{
+    int64_t pc = e - a;
+    nout = 0;
+    ncreated = 0;
+    if (pc == 0) return;
+    mcs_stream rs;
+    mcs_stream_init(&rs, seed, instance, MCS_PHASE_CELL_RESAMPLE, (uint64_t)c, step);
+    double wtotal = 0.0, xx = 0.0, wcum = 0.0;
+    int64_t cnt = 0, pk = 0;
+    for (int64_t i = a; i < e; ++i) {
+        cnt += 1;
+        wtotal += src.w[i];
+        if (cnt == nmax) xx = mcs_samplemin(&rs, (double)ntarget);
+    }
+    const bool merged = pc > nmax;
+    const int64_t created_at = merged ? 0 : pc;
+    int64_t splits = 0, removed = 0;
+    for (int64_t i = a; i < e; ++i) {
+        double wi = src.w[i];
+        if (pc > nmax) {
+            while ((wtotal * xx < wcum + wi) && pk < ntarget) {
+                pk += 1;
+                if (WRITE) {
+                    int64_t d = base + created_at + ncreated;
+                    copy_particle(src, i, dst, d);
+                    dst.w[d] = wtotal / (double)ntarget;
+                    dst.id[d] = idbase + (uint64_t)ncreated;
+                }
+                ncreated += 1;
+                if (pk < ntarget) xx = xx + mcs_samplemin(&rs, (double)(ntarget - pk)) * (1.0 - xx);
+            }
+            wcum = wcum + wi;
+            removed += 1;
+        } else {
+            double wkeep = wi;
+            if (pc < ntarget && wi > 2.0) {
+                double wnew = rint(wi / 2.0);
+                if (WRITE) {
+                    int64_t d = base + created_at + ncreated;
+                    copy_particle(src, i, dst, d);
+                    dst.w[d] = wnew;
+                    dst.id[d] = idbase + (uint64_t)ncreated;
+                }
+                ncreated += 1;
+                splits += 1;
+                wkeep = wi - wnew;
+                pc += 1;
+            }
+            if (WRITE) {
+                int64_t d = base + (i - a);
+                copy_particle(src, i, dst, d);
+                dst.w[d] = wkeep;
+            }
+        }
+    }
+    nout = (merged ? 0 : (e - a)) + ncreated;
+    if (!WRITE && ctr) {
+        if (merged) {
+            atomicAdd((unsigned long long *)&ctr[R_MERGED], 1ULL);
+            atomicAdd((unsigned long long *)&ctr[R_CLONES], (unsigned long long)ncreated);
+            atomicAdd((unsigned long long *)&ctr[R_REMOVED], (unsigned long long)removed);
+        } else if (splits) {
+            atomicAdd((unsigned long long *)&ctr[R_SPLITS], (unsigned long long)splits);
+        }
+    }
+}
+
+struct ResArgs {
+    PopBuf src, dst;
+    const int32_t *cs;
+    int64_t ncells;
+    int32_t ntarget, nmax;
+    uint64_t seed;
+    uint32_t instance, step;
+    int64_t *packed;   // ncells + 1 counts: out | created << 32
+    const int64_t *prefix;
+    uint64_t next_id;
+    int32_t *new_cs;
+    int64_t *ctr;
+};
+
+__global__ __launch_bounds__(MCS_BLOCK) void k_resample_count(ResArgs r)
+{
+    int64_t c = gidx();
+    if (c > r.ncells) return;
+    int64_t a = r.cs[c], e = r.cs[c + 1];
+    if (c == r.ncells) {  // particles outside the mesh are kept as they are
+        r.packed[c] = e - a;
+        return;
+    }
+    int64_t nout, ncr;
+    resample_cell<false>(c, a, e, r.ntarget, r.nmax, r.seed, r.instance, r.step, r.src, r.dst, 0,
+                         0, nout, ncr, r.ctr);
+    r.packed[c] = nout | (ncr << 32);
+}
+
+__global__ __launch_bounds__(MCS_BLOCK) void k_resample_write(ResArgs r)
+{
+    int64_t c = gidx();
+    if (c > r.ncells) return;
+    int64_t a = r.cs[c], e = r.cs[c + 1];
+    int64_t pre = r.prefix[c];
+    int64_t base = pre & 0xffffffffLL;
+    int64_t crbase = pre >> 32;
+    r.new_cs[c] = (int32_t)base;
+    if (c == r.ncells) {
+        for (int64_t i = a; i < e; ++i) copy_particle(r.src, i, r.dst, base + (i - a));
+        int64_t tot = r.prefix[c + 1];
+        r.new_cs[c + 1] = (int32_t)(tot & 0xffffffffLL);
+        return;
+    }
+    int64_t nout, ncr;
+    resample_cell<true>(c, a, e, r.ntarget, r.nmax, r.seed, r.instance, r.step, r.src, r.dst, base,
+                        r.next_id + (uint64_t)crbase, nout, ncr, nullptr);
+}
+
+extern "C" int32_t mcs_resample_cells(mcs_pop *p, mcs_mesh *m, int32_t ntarget, int32_t nmax,
+                                      uint64_t seed, uint32_t instance, uint32_t step,
+                                      mcs_resample_stats *stats)
+{
+    if (!p || !m) return MCS_E_ARG;
+    mcs_ctx *ctx = p->ctx;
+    if (ntarget < 1 || nmax < ntarget)
+        return mcs_fail(ctx, MCS_E_ARG, "resample: need 1 <= ntarget <= nmax");
+    if (!m->sorted_valid)
+        return mcs_fail(ctx, MCS_E_ARG, "resample: population is not in cell order (call mcs_sort_by_cell)");
+    hipSetDevice(ctx->device);
+    int32_t rc = mcs_pop_ensure_alt(p);
+    if (rc) return rc;
+    if (stats) memset(stats, 0, sizeof(*stats));
+    int64_t nseg = m->ncells + 1;
+    int64_t *ctr = ctx->d_small;
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, ctx->stream, (int32_t *)ctr, (int64_t)16);
+    ResArgs r;
+    r.src = p->buf[0];
+    r.dst = p->buf[1];
+    r.cs = m->cell_start;
+    r.ncells = m->ncells;
+    r.ntarget = ntarget;
+    r.nmax = nmax;
+    r.seed = seed;
+    r.instance = instance;
+    r.step = step;
+    r.packed = m->cnt64;
+    r.prefix = m->pre64;
+    r.next_id = p->next_id;
+    r.new_cs = m->cell_cnt;
+    r.ctr = ctr;
+    {
+        ProfScope ps(ctx, "resample");
+        hipLaunchKernelGGL(k_resample_count, dim3((unsigned)cdiv(nseg, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                           ctx->stream, r);
+    }
+    MCS_LAUNCH_CHECK(ctx);
+    rc = mcs_scan_i64(ctx, m->cnt64, m->pre64, nseg);
+    if (rc) return rc;
+    MCS_HIP(ctx, hipMemcpyAsync(ctx->h_small, m->pre64 + nseg, sizeof(int64_t), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    MCS_HIP(ctx, hipMemcpyAsync(ctx->h_small + 1, ctr, sizeof(int64_t) * 8, hipMemcpyDeviceToHost,
+                                ctx->stream));
+    MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int64_t tot = ctx->h_small[0];
+    int64_t h[8];
+    memcpy(h, ctx->h_small + 1, sizeof(h));
+    int64_t nout = tot & 0xffffffffLL;
+    int64_t ncreated = tot >> 32;
+    if (nout > p->cap)
+        return mcs_fail(ctx, MCS_E_CAPACITY, "Maximum number of particles reached (%lld > %lld)",
+                        (long long)nout, (long long)p->cap);
+    {
+        ProfScope ps(ctx, "resample");
+        hipLaunchKernelGGL(k_resample_write, dim3((unsigned)cdiv(nseg, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                           ctx->stream, r);
+    }
+    MCS_LAUNCH_CHECK(ctx);
+    MCS_HIP(ctx, hipMemcpyAsync(m->cell_start, m->cell_cnt, sizeof(int32_t) * (nseg + 1),
+                                hipMemcpyDeviceToDevice, ctx->stream));
+    mcs_pop_swap(p);
+    if (stats) {
+        stats->n_in = p->n;
+        stats->n_out = nout;
+        stats->cells_merged = h[R_MERGED];
+        stats->clones = h[R_CLONES];
+        stats->splits = h[R_SPLITS];
+        stats->removed = h[R_REMOVED];
+    }
+    p->n = nout;
+    p->next_id += (uint64_t)ncreated;
+    return MCS_OK;
+}

@@ -1,0 +1,562 @@
+// mcs_step.hip -- the fused particle step (JuMC.advance!), birth placement,
+// repack! and the swarm's global resample.
+#include <cstring>
+
+#include "mcs_device.h"
+
+// counters block layout in ctx->d_small
+#define C_LOG 0
+#define C_FLIGHTS 1
+#define C_DEATHS 2
+#define C_FLAGS 3
+#define C_AUX 4
+#define C_EVENTS 8
+
+struct BirthLog {
+    int32_t *parent, *seq;
+    double *x[3], *v[3];
+    double *w, *s;
+    uint8_t *act;
+    int64_t cap;
+};
+
+static inline BirthLog birth_log(mcs_pop *p)
+{
+    BirthLog L;
+    L.parent = p->log_parent;
+    L.seq = p->log_seq;
+    for (int d = 0; d < 3; ++d) {
+        L.x[d] = p->log_x[d];
+        L.v[d] = p->log_v[d];
+    }
+    L.w = p->log_w;
+    L.s = p->log_s;
+    L.act = p->log_act;
+    L.cap = p->logcap;
+    return L;
+}
+
+struct AdvArgs {
+    PopBuf b;
+    int64_t n0;
+    TableView T;
+    MeshView M;
+    double e[3];
+    double dt;
+    uint64_t seed;
+    uint32_t instance, step;
+    BirthLog L;
+    int32_t *loff;
+    int64_t *btot;
+    int64_t *ctr;
+};
+
+__device__ __forceinline__ void efield_at(const AdvArgs &a, const double x[3], double e[3],
+                                          int32_t &flags, int field_mesh)
+{
+    if (field_mesh) {
+        efield_mesh_dev(a.M, x, e, flags);
+    } else {
+        e[0] = a.e[0];
+        e[1] = a.e[1];
+        e[2] = a.e[2];
+    }
+}
+
+// track1 (MCStreamer/src/gridfields.jl:303-313): val/dV into the NGP cell.
+__device__ __forceinline__ void track_dev(const MeshView &M, const double x[3], double val)
+{
+    int64_t c[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        double f = floor(x[d] / M.h);
+        if (!(f >= 0.0 && f < (double)M.n[d])) return;
+        c[d] = (int64_t)f;
+    }
+    double dV = (M.h * M.h) * M.h;
+    int64_t idx = (c[0] + 1) + (int64_t)(M.n[0] + 2) * ((c[1] + 1) + (int64_t)(M.n[1] + 2) * (c[2] + 1));
+    atomicAdd(&M.qfixed[idx], val / dV);
+}
+
+// The fused step: one thread per particle slot that exists when the step starts
+// (JuMC/src/mixed_population.jl:27 fixes the range, so children are not advanced).
+// Per particle, free flights to the next (null) collision until dt is used up
+// (:31-51), the Yoshida push of electron.jl:51-62 for each flight, the sampler of
+// collisions.jl:236-269 and the outcomes of electron.jl:108-139.  Children go to a
+// birth log tagged (parent, ordinal); k_place writes them after the parents in
+// parent order, which is the append order of the single-threaded reference.
+template <int FIELD_MESH, int TRACK>
+__global__ __launch_bounds__(MCS_BLOCK) void k_advance(AdvArgs a)
+{
+    __shared__ int s_ev[MCS_MAX_PROC];
+    __shared__ int s_flights, s_deaths, s_flags;
+    if (threadIdx.x < MCS_MAX_PROC) s_ev[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        s_flights = 0;
+        s_deaths = 0;
+        s_flags = 0;
+    }
+    __syncthreads();
+
+    const int64_t i = gidx();
+    int nb = 0;
+    int32_t flags = 0;
+    int flights = 0, deaths = 0;
+    if (i < a.n0 && a.b.active[i]) {
+        const double maxrate = a.T.maxrate;
+        double x[3] = {a.b.x[0][i], a.b.x[1][i], a.b.x[2][i]};
+        double v[3] = {a.b.v[0][i], a.b.v[1][i], a.b.v[2][i]};
+        const double w = a.b.w[i];
+        double s = a.b.s[i];
+        uint8_t act = 1;
+        mcs_stream rs;
+        mcs_stream_init(&rs, a.seed, a.instance, MCS_PHASE_ADVANCE, a.b.id[i], a.step);
+        double trem = a.dt;
+        const double c1 = MCS_YOSHIDA_C1, c2 = MCS_YOSHIDA_C2;
+        while (trem > 0.0) {
+            double tnext = s / maxrate;
+            double tt;
+            bool collides;
+            if (trem > tnext) {
+                tt = tnext;
+                collides = true;
+            } else {
+                tt = trem;
+                collides = false;
+                s = s - tt * maxrate;
+            }
+            // advance_free_yoshida
+            {
+                double e[3];
+                double k1 = c1 * tt, k2 = c2 * tt;
+                double f1 = (MCS_YOSHIDA_D1 * tt) * MCS_QM;
+                double f2 = (MCS_YOSHIDA_D2 * tt) * MCS_QM;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) x[d] = x[d] + k1 * v[d];
+                efield_at(a, x, e, flags, FIELD_MESH);
+#pragma unroll
+                for (int d = 0; d < 3; ++d) v[d] = v[d] + f1 * e[d];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) x[d] = x[d] + k2 * v[d];
+                efield_at(a, x, e, flags, FIELD_MESH);
+#pragma unroll
+                for (int d = 0; d < 3; ++d) v[d] = v[d] + f2 * e[d];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) x[d] = x[d] + k2 * v[d];
+                efield_at(a, x, e, flags, FIELD_MESH);
+#pragma unroll
+                for (int d = 0; d < 3; ++d) v[d] = v[d] + f1 * e[d];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) x[d] = x[d] + k1 * v[d];
+            }
+            flights += 1;
+            if (collides) {
+                double eng = energy_dev(v[0], v[1], v[2]);
+                int64_t k;
+                double wg;
+                indweight_dev(a.T.energy, a.T.ne, a.T.egrid, a.T.step, a.T.inv_dlog, eng, k, wg,
+                              flags);
+                double xi = mcs_u01(&rs) * maxrate;
+                const double *r0 = a.T.rate + k * a.T.nproc;
+                const double *r1 = r0 + a.T.nproc;
+                for (int j = 0; j < a.T.nproc; ++j) {
+                    double nu = wg * r0[j] + (1.0 - wg) * r1[j];
+                    if (nu > xi) {
+                        atomicAdd(&s_ev[j], 1);
+                        int kind = a.T.kind[j];
+                        if (kind == MCS_PROC_ELASTIC) {
+                            double mr = a.T.param[j];
+                            double c = mr / (1.0 + mr);
+                            double vcm0 = c * v[0], vcm1 = c * v[1], vcm2 = c * v[2];
+                            double d0 = v[0] - vcm0, d1 = v[1] - vcm1, d2 = v[2] - vcm2;
+                            double nrm = sqrt((d0 * d0 + d1 * d1) + d2 * d2);
+                            double om[3];
+                            mcs_randsphere(&rs, om);
+                            v[0] = om[0] * nrm + vcm0;
+                            v[1] = om[1] * nrm + vcm1;
+                            v[2] = om[2] * nrm + vcm2;
+                        } else if (kind == MCS_PROC_EXCITATION) {
+                            double e1 = eng - a.T.param[j];
+                            if (e1 < 0.0) e1 = 0.0;
+                            double vabs = sqrt((2.0 * e1) / MCS_ELECTRON_MASS);
+                            double om[3];
+                            mcs_randsphere(&rs, om);
+                            v[0] = om[0] * vabs;
+                            v[1] = om[1] * vabs;
+                            v[2] = om[2] * vabs;
+                        } else if (kind == MCS_PROC_IONIZATION) {
+                            double e1 = eng - a.T.param[j];
+                            if (e1 < 0.0) e1 = 0.0;
+                            e1 = e1 / 2.0;
+                            double vabs = sqrt((2.0 * e1) / MCS_ELECTRON_MASS);
+                            double om1[3], om2[3];
+                            mcs_randsphere(&rs, om1);
+                            mcs_randsphere(&rs, om2);
+                            v[0] = om1[0] * vabs;
+                            v[1] = om1[1] * vabs;
+                            v[2] = om1[2] * vabs;
+                            double sc = mcs_nextcoll(&rs);
+                            if (TRACK) track_dev(a.M, x, w);
+                            unsigned long long slot =
+                                atomicAdd((unsigned long long *)&a.ctr[C_LOG], 1ULL);
+                            if ((int64_t)slot < a.L.cap) {
+                                a.L.parent[slot] = (int32_t)i;
+                                a.L.seq[slot] = nb;
+#pragma unroll
+                                for (int d = 0; d < 3; ++d) {
+                                    a.L.x[d][slot] = x[d];
+                                    a.L.v[d][slot] = om2[d] * vabs;
+                                }
+                                a.L.w[slot] = w;
+                                a.L.s[slot] = sc;
+                                a.L.act[slot] = act;
+                            } else {
+                                flags |= MCS_FLAG_CAPACITY;
+                            }
+                            nb += 1;
+                        } else if (kind == MCS_PROC_ATTACHMENT) {
+                            if (TRACK) track_dev(a.M, x, -w);
+                            if (act) deaths += 1;
+                            act = 0;
+                        }
+                        break;
+                    }
+                    xi = xi - nu;
+                }
+                s = mcs_nextcoll(&rs);
+            }
+            trem = trem - tt;
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            a.b.x[d][i] = x[d];
+            a.b.v[d][i] = v[d];
+        }
+        a.b.s[i] = s;
+        if (!act) a.b.active[i] = 0;
+    }
+    int tot;
+    int ex = block_excl_scan<int>(nb, &tot);
+    if (nb > 0) a.loff[i] = ex;
+    if (threadIdx.x == 0) a.btot[blockIdx.x] = tot;
+    if (flights) atomicAdd(&s_flights, flights);
+    if (deaths) atomicAdd(&s_deaths, deaths);
+    if (flags) atomicOr(&s_flags, flags);
+    __syncthreads();
+    if (threadIdx.x < a.T.nproc && s_ev[threadIdx.x])
+        atomicAdd((unsigned long long *)&a.ctr[C_EVENTS + threadIdx.x],
+                  (unsigned long long)s_ev[threadIdx.x]);
+    if (threadIdx.x == 0) {
+        if (s_flights) atomicAdd((unsigned long long *)&a.ctr[C_FLIGHTS], (unsigned long long)s_flights);
+        if (s_deaths) atomicAdd((unsigned long long *)&a.ctr[C_DEATHS], (unsigned long long)s_deaths);
+        if (s_flags) atomicOr((unsigned long long *)&a.ctr[C_FLAGS], (unsigned long long)s_flags);
+    }
+}
+
+// Children land at n0 + (births of earlier blocks) + (births of earlier threads in
+// the block) + ordinal, with ids continuing next_id in that order.
+__global__ __launch_bounds__(MCS_BLOCK) void k_place_births(BirthLog L, int64_t count,
+                                                            const int32_t *loff,
+                                                            const int64_t *bpre, int64_t n0,
+                                                            uint64_t next_id, PopBuf b)
+{
+    int64_t r = gidx();
+    if (r >= count) return;
+    int32_t par = L.parent[r];
+    int64_t dst = n0 + bpre[par / MCS_BLOCK] + loff[par] + L.seq[r];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        b.x[d][dst] = L.x[d][r];
+        b.v[d][dst] = L.v[d][r];
+    }
+    b.w[dst] = L.w[r];
+    b.s[dst] = L.s[r];
+    b.active[dst] = L.act[r];
+    b.id[dst] = next_id + (uint64_t)(dst - n0);
+}
+
+__global__ void k_fill_i64(int64_t *a, int64_t n, int64_t v)
+{
+    int64_t i = gidx();
+    if (i < n) a[i] = v;
+}
+
+extern "C" int32_t mcs_advance(mcs_pop *p, const mcs_table *t, const mcs_field *field, double dt,
+                               mcs_mesh *tracker, uint64_t seed, uint32_t instance, uint32_t step,
+                               mcs_step_stats *stats)
+{
+    if (!p || !t || !field) return MCS_E_ARG;
+    mcs_ctx *ctx = p->ctx;
+    hipSetDevice(ctx->device);
+    if (!(dt >= 0.0)) return mcs_fail(ctx, MCS_E_ARG, "advance: dt must be >= 0");
+    if (field->kind == MCS_FIELD_MESH && !field->mesh)
+        return mcs_fail(ctx, MCS_E_ARG, "advance: mesh field without a mesh");
+    if (stats) memset(stats, 0, sizeof(*stats));
+    const int64_t n0 = p->n;
+    if (stats) {
+        stats->n_in = n0;
+        stats->n_out = n0;
+    }
+    if (n0 == 0) return MCS_OK;
+    int64_t nblk = cdiv(n0, MCS_BLOCK);
+    int64_t *btot = p->blk;
+    int64_t *bpre = p->blk + (nblk + 1);
+    int64_t *ctr = ctx->d_small;
+    hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, ctx->stream, ctr, (int64_t)64, (int64_t)0);
+
+    AdvArgs a;
+    a.b = p->buf[0];
+    a.n0 = n0;
+    a.T = table_view(t);
+    a.M = mesh_view(field->kind == MCS_FIELD_MESH ? field->mesh : tracker);
+    if (field->kind == MCS_FIELD_MESH && tracker && tracker != field->mesh)
+        return mcs_fail(ctx, MCS_E_ARG, "advance: tracker mesh must be the field mesh");
+    for (int d = 0; d < 3; ++d) a.e[d] = field->e[d];
+    a.dt = dt;
+    a.seed = seed;
+    a.instance = instance;
+    a.step = step;
+    a.L = birth_log(p);
+    a.loff = p->i32a;
+    a.btot = btot;
+    a.ctr = ctr;
+    {
+        ProfScope ps(ctx, "advance");
+        int fm = field->kind == MCS_FIELD_MESH;
+        int tr = tracker != nullptr;
+        dim3 g((unsigned)nblk), bl(MCS_BLOCK);
+        if (fm && tr)
+            hipLaunchKernelGGL((k_advance<1, 1>), g, bl, 0, ctx->stream, a);
+        else if (fm)
+            hipLaunchKernelGGL((k_advance<1, 0>), g, bl, 0, ctx->stream, a);
+        else if (tr)
+            hipLaunchKernelGGL((k_advance<0, 1>), g, bl, 0, ctx->stream, a);
+        else
+            hipLaunchKernelGGL((k_advance<0, 0>), g, bl, 0, ctx->stream, a);
+    }
+    MCS_LAUNCH_CHECK(ctx);
+    int32_t rc = mcs_scan_i64(ctx, btot, bpre, nblk);
+    if (rc) return rc;
+    MCS_HIP(ctx, hipMemcpyAsync(ctr + C_AUX, bpre + nblk, sizeof(int64_t), hipMemcpyDeviceToDevice,
+                                ctx->stream));
+    int64_t h[C_EVENTS + MCS_MAX_PROC];
+    rc = mcs_readback(ctx, ctr, h, C_EVENTS + t->nproc);
+    if (rc) return rc;
+    int64_t logged = h[C_LOG];
+    int64_t births = h[C_AUX];
+    int32_t flags = (int32_t)h[C_FLAGS];
+    if (stats) {
+        stats->births = births;
+        stats->deaths = h[C_DEATHS];
+        stats->flights = h[C_FLIGHTS];
+        for (int j = 0; j < t->nproc; ++j) stats->events[j] = h[C_EVENTS + j];
+        stats->flags = flags;
+    }
+    if (logged > p->logcap || n0 + births > p->cap) {
+        if (stats) stats->flags |= MCS_FLAG_CAPACITY;
+        return mcs_fail(ctx, MCS_E_CAPACITY, "Maximum number of particles reached (%lld + %lld > %lld)",
+                        (long long)n0, (long long)births, (long long)p->cap);
+    }
+    if (births > 0) {
+        hipLaunchKernelGGL(k_place_births, dim3((unsigned)cdiv(births, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                           ctx->stream, birth_log(p), births, (const int32_t *)p->i32a,
+                           (const int64_t *)bpre, n0, p->next_id, p->buf[0]);
+        MCS_LAUNCH_CHECK(ctx);
+        p->n = n0 + births;
+        p->next_id += (uint64_t)births;
+    }
+    if (stats) stats->n_out = p->n;
+    if (flags & MCS_FLAG_FIELD) return mcs_fail(ctx, MCS_E_FIELD, "Too high electric field");
+    if (flags & MCS_FLAG_TABLE_RANGE)
+        return mcs_fail(ctx, MCS_E_TABLE_RANGE, "particle energy beyond the collision table");
+    return MCS_OK;
+}
+
+// ---------------------------------------------------------------- repack!
+
+__global__ __launch_bounds__(MCS_BLOCK) void k_count_active(const uint8_t *act, int64_t n,
+                                                            int64_t *btot)
+{
+    int64_t i = gidx();
+    int a = (i < n && act[i]) ? 1 : 0;
+    int tot;
+    block_excl_scan<int>(a, &tot);
+    if (threadIdx.x == 0) btot[blockIdx.x] = tot;
+}
+
+// Rank matching equivalent of the two-pointer loop of population.jl:191-215: the
+// k-th hole below A (the number of actives) receives the k-th active from the top.
+__global__ __launch_bounds__(MCS_BLOCK) void k_repack_ranks(const uint8_t *act, int64_t n,
+                                                            const int64_t *bpre, int64_t A,
+                                                            int32_t *holes, int32_t *srcs)
+{
+    int64_t i = gidx();
+    int a = (i < n && act[i]) ? 1 : 0;
+    int tot;
+    int ex = block_excl_scan<int>(a, &tot);
+    if (i >= n) return;
+    int64_t before = bpre[blockIdx.x] + ex;
+    if (i < A && !a) holes[i - before] = (int32_t)i;
+    if (i >= A && a) srcs[A - before - 1] = (int32_t)i;
+}
+
+__global__ __launch_bounds__(MCS_BLOCK) void k_repack_move(PopBuf b, const int32_t *holes,
+                                                           const int32_t *srcs, int64_t h)
+{
+    int64_t r = gidx();
+    if (r >= h) return;
+    copy_particle(b, srcs[r], b, holes[r]);
+}
+
+static int32_t count_actives(mcs_pop *p, int64_t *A, int64_t **bpre_out)
+{
+    mcs_ctx *ctx = p->ctx;
+    int64_t n = p->n;
+    int64_t nblk = cdiv(n, MCS_BLOCK);
+    int64_t *btot = p->blk;
+    int64_t *bpre = p->blk + (nblk + 1);
+    hipLaunchKernelGGL(k_count_active, dim3((unsigned)nblk), dim3(MCS_BLOCK), 0, ctx->stream,
+                       p->buf[0].active, n, btot);
+    MCS_LAUNCH_CHECK(ctx);
+    int32_t rc = mcs_scan_i64(ctx, btot, bpre, nblk);
+    if (rc) return rc;
+    rc = mcs_readback(ctx, bpre + nblk, A, 1);
+    *bpre_out = bpre;
+    return rc;
+}
+
+extern "C" int32_t mcs_repack(mcs_pop *p)
+{
+    if (!p) return MCS_E_ARG;
+    mcs_ctx *ctx = p->ctx;
+    hipSetDevice(ctx->device);
+    int64_t n = p->n;
+    if (n == 0) return MCS_OK;
+    int64_t A;
+    int64_t *bpre;
+    int32_t rc = count_actives(p, &A, &bpre);
+    if (rc) return rc;
+    // holes below A = actives at or above A
+    int64_t h = 0;
+    if (A < n) {
+        int64_t nblk = cdiv(n, MCS_BLOCK);
+        hipLaunchKernelGGL(k_repack_ranks, dim3((unsigned)nblk), dim3(MCS_BLOCK), 0, ctx->stream,
+                           p->buf[0].active, n, (const int64_t *)bpre, A, p->i32a, p->i32b);
+        MCS_LAUNCH_CHECK(ctx);
+        // h = A - (actives in [0, A)); read the prefix at A's block and recount inside it
+        int64_t blockA = A / MCS_BLOCK;
+        int64_t pre;
+        rc = mcs_readback(ctx, bpre + blockA, &pre, 1);
+        if (rc) return rc;
+        std::vector<uint8_t> tail((size_t)(A - blockA * MCS_BLOCK));
+        if (!tail.empty()) {
+            MCS_HIP(ctx, hipMemcpy(tail.data(), p->buf[0].active + blockA * MCS_BLOCK, tail.size(),
+                                   hipMemcpyDeviceToHost));
+            for (uint8_t c : tail) pre += c ? 1 : 0;
+        }
+        h = A - pre;
+    }
+    if (h > 0) {
+        hipLaunchKernelGGL(k_repack_move, dim3((unsigned)cdiv(h, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                           ctx->stream, p->buf[0], (const int32_t *)p->i32a,
+                           (const int32_t *)p->i32b, h);
+        MCS_LAUNCH_CHECK(ctx);
+    }
+    p->n = A;
+    return MCS_OK;
+}
+
+// ---------------------------------------------------------------- Swarm.resample!
+
+__global__ __launch_bounds__(MCS_BLOCK) void k_swarm_drop(PopBuf b, int64_t n, double pdrop,
+                                                          uint64_t seed, uint32_t instance,
+                                                          uint32_t step)
+{
+    int64_t i = gidx();
+    if (i >= n || !b.active[i]) return;
+    mcs_stream rs;
+    mcs_stream_init(&rs, seed, instance, MCS_PHASE_SWARM_RESAMPLE, b.id[i], step);
+    if (mcs_u01(&rs) < pdrop) b.active[i] = 0;
+}
+
+__global__ __launch_bounds__(MCS_BLOCK) void k_swarm_count(PopBuf b, int64_t n, double lam,
+                                                           uint64_t seed, uint32_t instance,
+                                                           uint32_t step, int32_t *cnt,
+                                                           int64_t *btot)
+{
+    int64_t i = gidx();
+    int c = 0;
+    if (i < n && b.active[i]) {
+        mcs_stream rs;
+        mcs_stream_init(&rs, seed, instance, MCS_PHASE_SWARM_RESAMPLE, b.id[i], step);
+        c = (int)mcs_poisson(&rs, lam);
+    }
+    if (i < n) cnt[i] = c;
+    int tot;
+    block_excl_scan<int>(c, &tot);
+    if (threadIdx.x == 0) btot[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(MCS_BLOCK) void k_swarm_place(PopBuf b, int64_t n, const int32_t *cnt,
+                                                           const int64_t *bpre, uint64_t next_id)
+{
+    int64_t i = gidx();
+    int c = i < n ? cnt[i] : 0;
+    int tot;
+    int ex = block_excl_scan<int>(c, &tot);
+    if (c == 0) return;
+    int64_t base = n + bpre[blockIdx.x] + ex;
+    for (int j = 0; j < c; ++j) {
+        copy_particle(b, i, b, base + j);
+        b.id[base + j] = next_id + (uint64_t)(base + j - n);
+    }
+}
+
+extern "C" int32_t mcs_swarm_resample(mcs_pop *p, int64_t n0, double tol, uint64_t seed,
+                                      uint32_t instance, uint32_t step, int32_t *did)
+{
+    if (!p || n0 <= 0) return MCS_E_ARG;
+    mcs_ctx *ctx = p->ctx;
+    hipSetDevice(ctx->device);
+    if (did) *did = 0;
+    int64_t n = p->n;
+    if (n == 0) return MCS_OK;
+    int64_t ncur;
+    int64_t *bpre;
+    int32_t rc = count_actives(p, &ncur, &bpre);
+    if (rc) return rc;
+    if (fabs((double)(ncur - n0) / (double)n0) < tol) return MCS_OK;
+    if (ncur == 0) return MCS_OK;
+    if (did) *did = 1;
+    int64_t nblk = cdiv(n, MCS_BLOCK);
+    if (ncur > n0) {
+        double pdrop = (double)(ncur - n0) / (double)ncur;
+        hipLaunchKernelGGL(k_swarm_drop, dim3((unsigned)nblk), dim3(MCS_BLOCK), 0, ctx->stream,
+                           p->buf[0], n, pdrop, seed, instance, step);
+        MCS_LAUNCH_CHECK(ctx);
+        return MCS_OK;
+    }
+    double lam = (double)(n0 - ncur) / (double)ncur;
+    int64_t *btot = p->blk;
+    bpre = p->blk + (nblk + 1);
+    hipLaunchKernelGGL(k_swarm_count, dim3((unsigned)nblk), dim3(MCS_BLOCK), 0, ctx->stream,
+                       p->buf[0], n, lam, seed, instance, step, p->i32a, btot);
+    MCS_LAUNCH_CHECK(ctx);
+    rc = mcs_scan_i64(ctx, btot, bpre, nblk);
+    if (rc) return rc;
+    int64_t added;
+    rc = mcs_readback(ctx, bpre + nblk, &added, 1);
+    if (rc) return rc;
+    if (n + added > p->cap)
+        return mcs_fail(ctx, MCS_E_CAPACITY, "Maximum number of particles reached (%lld + %lld > %lld)",
+                        (long long)n, (long long)added, (long long)p->cap);
+    if (added > 0) {
+        hipLaunchKernelGGL(k_swarm_place, dim3((unsigned)nblk), dim3(MCS_BLOCK), 0, ctx->stream,
+                           p->buf[0], n, (const int32_t *)p->i32a, (const int64_t *)bpre,
+                           p->next_id);
+        MCS_LAUNCH_CHECK(ctx);
+        p->n = n + added;
+        p->next_id += (uint64_t)added;
+    }
+    return MCS_OK;
+}

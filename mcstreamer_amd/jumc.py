@@ -1,0 +1,240 @@
+"""Host mirror of the JuMC API on top of libmcs.
+
+Reference names map as follows (Julia's `!` dropped):
+  CollisionTable (JuMC/src/collisions.jl:86-99)   -> CollisionTable
+  Population (population.jl:4-36)                 -> Population
+  advance! (mixed_population.jl:24-54)            -> advance
+  repack! (population.jl:191-215)                 -> repack
+  nparticles/actives/weight/meanenergy/maxenergy (population.jl:63-158)
+  FixedField (Swarm/src/Swarm.jl:28-32)           -> FixedField
+All compute runs in libmcs on the GPU; this module only marshals arguments.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr, i32ptr, u8ptr, u64ptr
+
+
+class Context:
+    """One HIP device and stream (mcs_ctx)."""
+
+    def __init__(self, device=0):
+        L = _lib.lib()
+        h = ctypes.c_void_p()
+        rc = L.mcs_ctx_create(int(device), ctypes.byref(h))
+        if rc != _lib.MCS_OK:
+            raise _lib.MCSError(rc, f"cannot create a context on device {device}")
+        self.handle = h
+        self.device = device
+
+    def synchronize(self):
+        check(self.handle, _lib.lib().mcs_ctx_synchronize(self.handle))
+
+    def stream_ptr(self):
+        s = ctypes.c_void_p()
+        check(self.handle, _lib.lib().mcs_ctx_stream(self.handle, ctypes.byref(s)))
+        return s.value
+
+    def profile(self, enable=True):
+        check(self.handle, _lib.lib().mcs_ctx_profile(self.handle, 1 if enable else 0))
+
+    def profile_read(self, kernel):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        check(self.handle, _lib.lib().mcs_ctx_profile_read(self.handle, kernel.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def profile_reset(self):
+        check(self.handle, _lib.lib().mcs_ctx_profile_reset(self.handle))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.lib().mcs_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default = None
+
+
+def context():
+    """The process-wide default context on device 0."""
+    global _default
+    if _default is None:
+        _default = Context(0)
+    return _default
+
+
+class CollisionTable:
+    """Rate table on an energy grid: proc, energy (J), rate (nproc x ne), maxrate."""
+
+    def __init__(self, proc, energy, rate, maxrate, egrid="uniform", ctx=None):
+        self.ctx = ctx or context()
+        self.proc = list(proc)
+        self.energy = np.ascontiguousarray(energy, dtype=np.float64)
+        self.rate = np.ascontiguousarray(rate, dtype=np.float64)
+        self.maxrate = float(maxrate)
+        self.egrid = {"uniform": _lib.EGRID_UNIFORM, "log": _lib.EGRID_LOG}[egrid]
+        nproc, ne = self.rate.shape
+        if nproc != len(self.proc) or ne != self.energy.size:
+            raise ValueError("rate must have shape (len(proc), len(energy))")
+        self.kind = np.array([p.code for p in self.proc], dtype=np.int32)
+        self.param = np.array([p.param for p in self.proc], dtype=np.float64)
+        emajor = np.ascontiguousarray(self.rate.T)
+        h = ctypes.c_void_p()
+        check(self.ctx.handle, _lib.lib().mcs_table_create(
+            self.ctx.handle, nproc, ne, self.egrid, dptr(self.energy), dptr(emajor),
+            i32ptr(self.kind), dptr(self.param), self.maxrate, ctypes.byref(h)))
+        self.handle = h
+
+    def __len__(self):
+        return len(self.proc)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.lib().mcs_table_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class FixedField:
+    """Uniform field (0, 0, ez) (Swarm/src/Swarm.jl:28-32)."""
+
+    def __init__(self, ez, ex=0.0, ey=0.0):
+        self.e = (float(ex), float(ey), float(ez))
+
+    def cstruct(self):
+        f = _lib.Field()
+        f.kind = _lib.FIELD_UNIFORM
+        for d in range(3):
+            f.e[d] = self.e[d]
+        f.mesh = None
+        return f
+
+
+class ParticleStates:
+    """Host copy of a population's columns (x, v: n x 3; w, s; active; id)."""
+
+    def __init__(self, x, v, w, s, active, id=None, next_id=None):
+        self.x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        n = self.x.shape[0]
+        self.v = np.ascontiguousarray(v, dtype=np.float64).reshape(n, 3)
+        self.w = np.ascontiguousarray(w, dtype=np.float64).reshape(n)
+        self.s = np.ascontiguousarray(s, dtype=np.float64).reshape(n)
+        self.active = np.ascontiguousarray(active, dtype=np.uint8).reshape(n)
+        self.id = None if id is None else np.ascontiguousarray(id, dtype=np.uint64).reshape(n)
+        self.next_id = next_id
+
+    def __len__(self):
+        return self.x.shape[0]
+
+
+class Population:
+    """A device-resident particle population with capacity `max_particles`
+    (JuMC.Population, population.jl:26-36).  `seed` and `instance` key the
+    per-particle Philox streams; `step` counts the advance! calls."""
+
+    def __init__(self, max_particles, particles=None, collisions=None, seed=0, instance=0, ctx=None):
+        self.ctx = ctx or context()
+        self.collisions = collisions
+        self.seed = int(seed)
+        self.instance = int(instance)
+        self.step = 0
+        h = ctypes.c_void_p()
+        check(self.ctx.handle, _lib.lib().mcs_pop_create(self.ctx.handle, int(max_particles), ctypes.byref(h)))
+        self.handle = h
+        self.capacity = int(max_particles)
+        if particles is not None:
+            self.upload(particles)
+
+    def upload(self, ps, next_id=0):
+        n = len(ps)
+        nid = ps.next_id if ps.next_id is not None else next_id
+        check(self.ctx.handle, _lib.lib().mcs_pop_upload(
+            self.handle, n, dptr(ps.x), dptr(ps.v), dptr(ps.w), dptr(ps.s), u8ptr(ps.active),
+            u64ptr(ps.id) if ps.id is not None else None, int(nid)))
+
+    def download(self):
+        n = self.nparticles()
+        x = np.zeros((n, 3))
+        v = np.zeros((n, 3))
+        w = np.zeros(n)
+        s = np.zeros(n)
+        act = np.zeros(n, dtype=np.uint8)
+        ids = np.zeros(n, dtype=np.uint64)
+        nn = ctypes.c_int64()
+        nid = ctypes.c_uint64()
+        check(self.ctx.handle, _lib.lib().mcs_pop_download(
+            self.handle, n, ctypes.byref(nn), dptr(x), dptr(v), dptr(w), dptr(s), u8ptr(act),
+            u64ptr(ids), ctypes.byref(nid)))
+        return ParticleStates(x, v, w, s, act, ids, nid.value)
+
+    def nparticles(self):
+        n = ctypes.c_int64()
+        check(self.ctx.handle, _lib.lib().mcs_pop_size(self.handle, ctypes.byref(n)))
+        return n.value
+
+    def moments(self, table=None):
+        m = _lib.Moments()
+        check(self.ctx.handle, _lib.lib().mcs_pop_moments(
+            self.handle, table.handle if table is not None else None, ctypes.byref(m)))
+        return m
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.lib().mcs_pop_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def advance(popl, efield, dt, tracker=None, *, step=None, stats=True):
+    """advance! (mixed_population.jl:24-54) on the GPU.  Returns StepStats.
+
+    `efield` is a FixedField or a streamer FieldInterp; `tracker` is None
+    (VoidCollisionTracker) or a CollisionTracker."""
+    st = _lib.StepStats()
+    fld = efield.cstruct()
+    trk = tracker.mesh_handle() if tracker is not None else None
+    s = popl.step if step is None else int(step)
+    check(popl.ctx.handle, _lib.lib().mcs_advance(
+        popl.handle, popl.collisions.handle, ctypes.byref(fld), float(dt), trk,
+        popl.seed, popl.instance, s, ctypes.byref(st) if stats else None))
+    if step is None:
+        popl.step += 1
+    return st
+
+
+def repack(popl):
+    check(popl.ctx.handle, _lib.lib().mcs_repack(popl.handle))
+
+
+def nparticles(popl):
+    return popl.nparticles()
+
+
+def actives(popl):
+    return popl.moments().nactive
+
+
+def weight(popl):
+    return popl.moments().weight
+
+
+def meanenergy(popl):
+    m = popl.moments()
+    return m.wenergy / m.weight
+
+
+def maxenergy(popl):
+    return popl.moments().maxenergy

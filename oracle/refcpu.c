@@ -1,0 +1,1083 @@
+/*
+ * refcpu.c -- ORACLE (test infrastructure only; see refcpu.h).
+ *
+ * Single-threaded FP64 restatement of the reference's per-step loop.  Built
+ * with -O2 -ffp-contract=off so that every a*b+c rounds twice, as Julia does
+ * without @fastmath/muladd.  Every function cites the reference lines it
+ * restates (paths relative to the reference tree).
+ */
+#include "refcpu.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../mcstreamer_amd/csrc/mcs_rng.h"
+#include "../mcstreamer_amd/csrc/mcs_phys.h"
+
+#define FLAG_TABLE_RANGE 1
+#define FLAG_FIELD 2
+#define FLAG_CAPACITY 4
+
+/* ---------------------------------------------------------------- helpers */
+
+void ref_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    mcs_philox4x32_10(ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1], out);
+}
+
+void ref_stream_u01(uint64_t seed, uint32_t instance, uint32_t phase, uint64_t id,
+                    uint32_t step, int64_t count, double *out)
+{
+    mcs_stream st;
+    mcs_stream_init(&st, seed, instance, phase, id, step);
+    for (int64_t i = 0; i < count; ++i) out[i] = mcs_u01(&st);
+}
+
+void ref_math(int32_t which, int64_t count, const double *in, double *out, double *out2)
+{
+    for (int64_t i = 0; i < count; ++i) {
+        switch (which) {
+        case 0: out[i] = mcs_log(in[i]); break;
+        case 1: out[i] = mcs_exp(in[i]); break;
+        case 2: mcs_sincos2pi(in[i], &out[i], &out2[i]); break;
+        default: out[i] = sqrt(in[i]); break;
+        }
+    }
+}
+
+/* energy(p) = 0.5 * mass(p) * (v1^2 + v2^2 + v3^2)   (JuMC/src/electron.jl:27) */
+static inline double energy3(const double *v)
+{
+    return MCS_HALF_ME * ((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+}
+
+/* indweight (JuMC/src/util.jl:23-30): index of the node before eng and the
+ * weight of that node.  Uniform grid: k = floor((eng - e0)/step),
+ * w = (e[k+1] - eng)/step.  Log grid: k located exactly by node comparisons,
+ * w = (e[k+1] - eng)/(e[k+1] - e[k]).  Energies past the last node flag a range
+ * error (the reference throws a BoundsError) and use the last interval. */
+int32_t ref_indweight(const ref_table *t, double eng, int64_t *kout, double *wout)
+{
+    const double *e = t->energy;
+    int64_t ne = t->ne;
+    int32_t flag = 0;
+    int64_t k;
+    double w;
+    if (eng >= e[ne - 1]) {
+        *kout = ne - 2;
+        *wout = 0.0;
+        return FLAG_TABLE_RANGE;
+    }
+    if (t->egrid == 0) {
+        double step = (e[ne - 1] - e[0]) / (double)(ne - 1);
+        double kf = floor((eng - e[0]) / step);
+        if (kf < 0.0) kf = 0.0;
+        k = (int64_t)kf;
+        if (k > ne - 2) k = ne - 2;
+        w = (e[k + 1] - eng) / step;
+    } else {
+        if (eng <= e[0]) {
+            *kout = 0;
+            *wout = 1.0;
+            return 0;
+        }
+        double inv_dlog = (double)(ne - 1) / mcs_log(e[ne - 1] / e[0]);
+        double kf = floor(mcs_log(eng / e[0]) * inv_dlog);
+        if (kf < 0.0) kf = 0.0;
+        if (kf > (double)(ne - 2)) kf = (double)(ne - 2);
+        k = (int64_t)kf;
+        while (k > 0 && e[k] > eng) k -= 1;
+        while (k < ne - 2 && e[k + 1] <= eng) k += 1;
+        w = (e[k + 1] - eng) / (e[k + 1] - e[k]);
+    }
+    if (w < 0.0) w = 0.0;
+    if (w > 1.0) w = 1.0;
+    *kout = k;
+    *wout = w;
+    return flag;
+}
+
+/* rate(c, j, pre) = w*R[j,k] + (1-w)*R[j,k+1]   (JuMC/src/collisions.jl:112-116) */
+static inline double rate_j(const ref_table *t, int j, int64_t k, double w)
+{
+    return w * t->rate[k * t->nproc + j] + (1.0 - w) * t->rate[(k + 1) * t->nproc + j];
+}
+
+/* ------------------------------------------------------------- fields */
+
+static inline int64_t cidx(const int32_t n[3], int64_t i, int64_t j, int64_t k)
+{
+    return i + (int64_t)(n[0] + 2) * (j + (int64_t)(n[1] + 2) * k);
+}
+
+/* normdivrem + staggered (MCStreamer/src/gridfields.jl:194-209) for one axis:
+ * face index m with offset d, and ghost-inclusive centre index m1 with offset d1. */
+static inline void axis_index(double xc, double h, int32_t n, int64_t *m, double *d,
+                              int64_t *m1, double *d1)
+{
+    double r = xc / h;
+    int64_t mi = (int64_t)trunc(r);
+    if (mi > n - 1) mi = n - 1;
+    double di = r - (double)mi;
+    *m = mi;
+    *d = di;
+    if (di < 0.5) {
+        *m1 = mi;
+        *d1 = di + 0.5;
+    } else {
+        *m1 = mi + 1;
+        *d1 = di - 0.5;
+    }
+}
+
+static inline double trilerp(const double *a, int64_t sx, int64_t sxy, int64_t base,
+                             double dx, double dy, double dz)
+{
+    double c00 = a[base] * (1.0 - dx) + a[base + 1] * dx;
+    double c10 = a[base + sx] * (1.0 - dx) + a[base + sx + 1] * dx;
+    double c01 = a[base + sxy] * (1.0 - dx) + a[base + sxy + 1] * dx;
+    double c11 = a[base + sxy + sx] * (1.0 - dx) + a[base + sxy + sx + 1] * dx;
+    double c0 = c00 * (1.0 - dy) + c10 * dy;
+    double c1 = c01 * (1.0 - dy) + c11 * dy;
+    return c0 * (1.0 - dz) + c1 * dz;
+}
+
+/* FieldInterp (MCStreamer/src/gridfields.jl:215-290), 3D Cartesian: zero outside
+ * the open box (inside(), grid.jl:45-49); each component interpolated trilinearly
+ * from its staggered face array; |E| > 1e8 V/m raises the field flag. */
+void ref_efield(const ref_field *f, const double x[3], double e[3], int32_t *flags)
+{
+    if (f->kind == 0) {
+        e[0] = f->e[0];
+        e[1] = f->e[1];
+        e[2] = f->e[2];
+        return;
+    }
+    const ref_mesh *m = f->mesh;
+    const int32_t *n = m->n;
+    double h = m->h;
+    if (!(0.0 < x[0] && x[0] < (double)n[0] * h && 0.0 < x[1] && x[1] < (double)n[1] * h &&
+          0.0 < x[2] && x[2] < (double)n[2] * h)) {
+        e[0] = 0.0;
+        e[1] = 0.0;
+        e[2] = 0.0;
+        return;
+    }
+    int64_t mx, my, mz, m1x, m1y, m1z;
+    double dx, dy, dz, d1x, d1y, d1z;
+    axis_index(x[0], h, n[0], &mx, &dx, &m1x, &d1x);
+    axis_index(x[1], h, n[1], &my, &dy, &m1y, &d1y);
+    axis_index(x[2], h, n[2], &mz, &dz, &m1z, &d1z);
+    int64_t nx = n[0], ny = n[1];
+    /* EX: (nx+1)(ny+2)(nz+2) */
+    {
+        int64_t sx = nx + 1, sxy = (nx + 1) * (ny + 2);
+        e[0] = trilerp(m->ex, sx, sxy, mx + sx * m1y + sxy * m1z, dx, d1y, d1z);
+    }
+    /* EY: (nx+2)(ny+1)(nz+2) */
+    {
+        int64_t sx = nx + 2, sxy = (nx + 2) * (ny + 1);
+        e[1] = trilerp(m->ey, sx, sxy, m1x + sx * my + sxy * m1z, d1x, dy, d1z);
+    }
+    /* EZ: (nx+2)(ny+2)(nz+1) */
+    {
+        int64_t sx = nx + 2, sxy = (nx + 2) * (ny + 2);
+        e[2] = trilerp(m->ez, sx, sxy, m1x + sx * m1y + sxy * mz, d1x, d1y, dz);
+    }
+    double e2 = (e[0] * e[0] + e[1] * e[1]) + e[2] * e[2];
+    if (e2 > MCS_EFIELD_LIMIT2) *flags |= FLAG_FIELD;
+}
+
+/* advance_free_yoshida (JuMC/src/electron.jl:51-62):
+ *   x1 = x + (c1 dt) v;  v1 = v + ((d1 dt) q/m) E(x1);  ...;  x4 = x3 + (c4 dt) v3. */
+void ref_yoshida(const ref_field *f, double x[3], double v[3], double dt, int32_t *flags)
+{
+    double e[3];
+    double c1 = MCS_YOSHIDA_C1 * dt, c2 = MCS_YOSHIDA_C2 * dt;
+    double c3 = MCS_YOSHIDA_C3 * dt, c4 = MCS_YOSHIDA_C4 * dt;
+    double f1 = (MCS_YOSHIDA_D1 * dt) * MCS_QM;
+    double f2 = (MCS_YOSHIDA_D2 * dt) * MCS_QM;
+    double f3 = (MCS_YOSHIDA_D3 * dt) * MCS_QM;
+    for (int d = 0; d < 3; ++d) x[d] = x[d] + c1 * v[d];
+    ref_efield(f, x, e, flags);
+    for (int d = 0; d < 3; ++d) v[d] = v[d] + f1 * e[d];
+    for (int d = 0; d < 3; ++d) x[d] = x[d] + c2 * v[d];
+    ref_efield(f, x, e, flags);
+    for (int d = 0; d < 3; ++d) v[d] = v[d] + f2 * e[d];
+    for (int d = 0; d < 3; ++d) x[d] = x[d] + c3 * v[d];
+    ref_efield(f, x, e, flags);
+    for (int d = 0; d < 3; ++d) v[d] = v[d] + f3 * e[d];
+    for (int d = 0; d < 3; ++d) x[d] = x[d] + c4 * v[d];
+}
+
+/* track1 (MCStreamer/src/gridfields.jl:303-313): val/dV into the NGP cell of x. */
+static void track1(ref_mesh *m, const double x[3], double val)
+{
+    if (!m) return;
+    int64_t c[3];
+    for (int d = 0; d < 3; ++d) {
+        double f = floor(x[d] / m->h);
+        if (!(f >= 0.0 && f < (double)m->n[d])) return;
+        c[d] = (int64_t)f;
+    }
+    double dV = (m->h * m->h) * m->h;
+    m->qfixed[cidx(m->n, c[0] + 1, c[1] + 1, c[2] + 1)] += val / dV;
+}
+
+/* add_particle! (JuMC/src/population.jl:88-97) */
+static int append(ref_pop *p, const double x[3], const double v[3], double w, double s,
+                  uint8_t active)
+{
+    if (p->n >= p->cap) return 0;
+    int64_t i = p->n;
+    for (int d = 0; d < 3; ++d) {
+        p->x[3 * i + d] = x[d];
+        p->v[3 * i + d] = v[d];
+    }
+    p->w[i] = w;
+    p->s[i] = s;
+    p->active[i] = active;
+    p->id[i] = p->next_id;
+    p->next_id += 1;
+    p->n += 1;
+    return 1;
+}
+
+/* advance! (JuMC/src/mixed_population.jl:24-54) with do_one_collision!
+ * (collisions.jl:236-269), collide (electron.jl:108-139), apply!
+ * (collisions.jl:173-192) and the tracker (gridfields.jl:315-318).
+ * The particle range is fixed when the loop starts, so children are not
+ * advanced in the step they are born; a particle removed by attachment keeps
+ * flying and colliding until its time step is used up. */
+int32_t ref_advance(ref_pop *p, const ref_table *t, const ref_field *f, double dt,
+                    ref_mesh *tracker, uint64_t seed, uint32_t instance, uint32_t step,
+                    ref_stats *st)
+{
+    memset(st, 0, sizeof(*st));
+    int64_t n0 = p->n;
+    st->n_in = n0;
+    double maxrate = t->maxrate;
+    for (int64_t i = 0; i < n0; ++i) {
+        if (!p->active[i]) continue;
+        mcs_stream rs;
+        mcs_stream_init(&rs, seed, instance, MCS_PHASE_ADVANCE, p->id[i], step);
+        double x[3] = {p->x[3 * i], p->x[3 * i + 1], p->x[3 * i + 2]};
+        double v[3] = {p->v[3 * i], p->v[3 * i + 1], p->v[3 * i + 2]};
+        double w = p->w[i];
+        double s = p->s[i];
+        uint8_t act = p->active[i];
+        double trem = dt;
+        while (trem > 0.0) {
+            double tnext = s / maxrate;
+            double tt;
+            int collides;
+            if (trem > tnext) {
+                tt = tnext;
+                collides = 1;
+            } else {
+                tt = trem;
+                collides = 0;
+                s = s - tt * maxrate;
+            }
+            ref_yoshida(f, x, v, tt, &st->flags);
+            st->flights += 1;
+            if (collides) {
+                double eng = energy3(v);
+                int64_t k;
+                double wgt;
+                st->flags |= ref_indweight(t, eng, &k, &wgt);
+                double xi = mcs_u01(&rs) * maxrate;
+                for (int j = 0; j < t->nproc; ++j) {
+                    double nu = rate_j(t, j, k, wgt);
+                    if (nu > xi) {
+                        st->events[j] += 1;
+                        int kind = t->kind[j];
+                        if (kind == 1) { /* Elastic, electron.jl:133-139 */
+                            double mr = t->param[j];
+                            double c = mr / (1.0 + mr);
+                            double vcm[3] = {c * v[0], c * v[1], c * v[2]};
+                            double dv[3] = {v[0] - vcm[0], v[1] - vcm[1], v[2] - vcm[2]};
+                            double nrm = sqrt((dv[0] * dv[0] + dv[1] * dv[1]) + dv[2] * dv[2]);
+                            double om[3];
+                            mcs_randsphere(&rs, om);
+                            for (int d = 0; d < 3; ++d) v[d] = om[d] * nrm + vcm[d];
+                        } else if (kind == 2) { /* Excitation, electron.jl:108-114 */
+                            double e1 = eng - t->param[j];
+                            if (e1 < 0.0) e1 = 0.0;
+                            double vabs = sqrt((2.0 * e1) / MCS_ELECTRON_MASS);
+                            double om[3];
+                            mcs_randsphere(&rs, om);
+                            for (int d = 0; d < 3; ++d) v[d] = om[d] * vabs;
+                        } else if (kind == 3) { /* Ionization, electron.jl:116-127 */
+                            double e1 = eng - t->param[j];
+                            if (e1 < 0.0) e1 = 0.0;
+                            e1 = e1 / 2.0;
+                            double vabs = sqrt((2.0 * e1) / MCS_ELECTRON_MASS);
+                            double om1[3], om2[3], vc[3];
+                            mcs_randsphere(&rs, om1);
+                            mcs_randsphere(&rs, om2);
+                            for (int d = 0; d < 3; ++d) {
+                                v[d] = om1[d] * vabs;
+                                vc[d] = om2[d] * vabs;
+                            }
+                            double sc = mcs_nextcoll(&rs);
+                            track1(tracker, x, w);
+                            /* apply!: parent state written, child appended */
+                            if (append(p, x, vc, w, sc, act))
+                                st->births += 1;
+                            else
+                                st->flags |= FLAG_CAPACITY;
+                        } else if (kind == 4) { /* Attachment, electron.jl:129-131 */
+                            track1(tracker, x, -w);
+                            if (act) st->deaths += 1;
+                            act = 0;
+                        }
+                        break;
+                    } else {
+                        xi = xi - nu;
+                    }
+                }
+                s = mcs_nextcoll(&rs);
+            }
+            trem = trem - tt;
+        }
+        for (int d = 0; d < 3; ++d) {
+            p->x[3 * i + d] = x[d];
+            p->v[3 * i + d] = v[d];
+        }
+        p->s[i] = s;
+        p->active[i] = act;
+    }
+    st->n_out = p->n;
+    return st->flags;
+}
+
+static void copy_slot(ref_pop *dst, int64_t di, const ref_pop *src, int64_t si)
+{
+    for (int d = 0; d < 3; ++d) {
+        dst->x[3 * di + d] = src->x[3 * si + d];
+        dst->v[3 * di + d] = src->v[3 * si + d];
+    }
+    dst->w[di] = src->w[si];
+    dst->s[di] = src->s[si];
+    dst->active[di] = src->active[si];
+    dst->id[di] = src->id[si];
+}
+
+/* repack! (JuMC/src/population.jl:191-215): two pointers, the last active
+ * particle moves into the first hole; n becomes the number of actives. */
+void ref_repack(ref_pop *p)
+{
+    int64_t l = p->n - 1;
+    if (p->n == 0) return;
+    while (l >= 0 && !p->active[l]) l -= 1;
+    int64_t i = 0;
+    while (i <= l) {
+        if (!p->active[i]) {
+            copy_slot(p, i, p, l);
+            l -= 1;
+            while (l >= 0 && !p->active[l]) l -= 1;
+        }
+        i += 1;
+    }
+    p->n = l + 1;
+}
+
+/* weight, actives, meanenergy, maxenergy (population.jl:114-158) and the swarm
+ * tallies (Swarm.jl:126-127,152,156 with collrate :242-248), summed in order. */
+void ref_moments_compute(const ref_pop *p, const ref_table *t, ref_moments *m)
+{
+    memset(m, 0, sizeof(*m));
+    m->nparticles = p->n;
+    double emax = -HUGE_VAL;
+    for (int64_t i = 0; i < p->n; ++i) {
+        const double *v = &p->v[3 * i];
+        const double *x = &p->x[3 * i];
+        double eng = energy3(v);
+        if (eng > emax) emax = eng;
+        if (!p->active[i]) continue;
+        m->nactive += 1;
+        m->weight += p->w[i];
+        m->wenergy += p->w[i] * eng;
+        m->sum_energy += eng;
+        for (int d = 0; d < 3; ++d) {
+            m->sum_x[d] += x[d];
+            m->sum_x2[d] += x[d] * x[d];
+            m->sum_v[d] += v[d];
+        }
+        if (t) {
+            int64_t k;
+            double w;
+            ref_indweight(t, eng, &k, &w);
+            for (int j = 0; j < t->nproc; ++j) m->collrate[j] += rate_j(t, j, k, w);
+        }
+    }
+    m->maxenergy = p->n > 0 ? emax : 0.0;
+}
+
+/* Swarm.resample! (Swarm/src/Swarm.jl:193-223): one draw per active particle;
+ * drop with probability (ncur-n)/ncur, or append Poisson((n-ncur)/ncur) copies.
+ * The loop range is fixed at its start, so copies are not cloned again. */
+int32_t ref_swarm_resample(ref_pop *p, int64_t n0, double tol, uint64_t seed,
+                           uint32_t instance, uint32_t step, int32_t *did)
+{
+    int64_t ncur = 0;
+    *did = 0;
+    for (int64_t i = 0; i < p->n; ++i) ncur += p->active[i] ? 1 : 0;
+    if (fabs((double)(ncur - n0) / (double)n0) < tol) return 0;
+    if (ncur == 0) return 0;
+    *did = 1;
+    int64_t nloop = p->n;
+    if (ncur > n0) {
+        double pdrop = (double)(ncur - n0) / (double)ncur;
+        for (int64_t i = 0; i < nloop; ++i) {
+            if (!p->active[i]) continue;
+            mcs_stream rs;
+            mcs_stream_init(&rs, seed, instance, MCS_PHASE_SWARM_RESAMPLE, p->id[i], step);
+            if (mcs_u01(&rs) < pdrop) p->active[i] = 0;
+        }
+    } else {
+        double lam = (double)(n0 - ncur) / (double)ncur;
+        for (int64_t i = 0; i < nloop; ++i) {
+            if (!p->active[i]) continue;
+            mcs_stream rs;
+            mcs_stream_init(&rs, seed, instance, MCS_PHASE_SWARM_RESAMPLE, p->id[i], step);
+            int64_t nnew = mcs_poisson(&rs, lam);
+            for (int64_t j = 0; j < nnew; ++j) {
+                double x[3] = {p->x[3 * i], p->x[3 * i + 1], p->x[3 * i + 2]};
+                double v[3] = {p->v[3 * i], p->v[3 * i + 1], p->v[3 * i + 2]};
+                if (!append(p, x, v, p->w[i], p->s[i], p->active[i])) return FLAG_CAPACITY;
+            }
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ mesh */
+
+/* cellindex (MCStreamer/src/grid.jl:84-90), 3D: floor(x/h) per axis; particles
+ * outside [0, n h) get key ncells. */
+int64_t ref_cell_key(const ref_mesh *m, const double x[3])
+{
+    int64_t c[3];
+    int64_t ncells = (int64_t)m->n[0] * m->n[1] * m->n[2];
+    for (int d = 0; d < 3; ++d) {
+        double f = floor(x[d] / m->h);
+        if (!(f >= 0.0 && f < (double)m->n[d])) return ncells;
+        c[d] = (int64_t)f;
+    }
+    return c[0] + (int64_t)m->n[0] * (c[1] + (int64_t)m->n[1] * c[2]);
+}
+
+typedef struct { int64_t key; uint64_t id; int64_t src; } sort_item;
+
+static int cmp_item(const void *a, const void *b)
+{
+    const sort_item *x = (const sort_item *)a, *y = (const sort_item *)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    if (x->id != y->id) return x->id < y->id ? -1 : 1;
+    return 0;
+}
+
+/* Canonical cell order: actives sorted by (cell, id), outside particles last. */
+int32_t ref_sort_by_cell(ref_pop *p, ref_mesh *m, ref_pop *scr)
+{
+    int64_t ncells = (int64_t)m->n[0] * m->n[1] * m->n[2];
+    sort_item *items = (sort_item *)malloc(sizeof(sort_item) * (size_t)(p->n + 1));
+    int64_t na = 0;
+    for (int64_t i = 0; i < p->n; ++i) {
+        if (!p->active[i]) continue;
+        items[na].key = ref_cell_key(m, &p->x[3 * i]);
+        items[na].id = p->id[i];
+        items[na].src = i;
+        na++;
+    }
+    qsort(items, (size_t)na, sizeof(sort_item), cmp_item);
+    for (int64_t c = 0; c <= ncells + 1; ++c) m->cell_start[c] = 0;
+    for (int64_t a = 0; a < na; ++a) {
+        copy_slot(scr, a, p, items[a].src);
+        m->cell_start[items[a].key + 1] += 1;
+    }
+    for (int64_t c = 1; c <= ncells + 1; ++c) m->cell_start[c] += m->cell_start[c - 1];
+    for (int64_t a = 0; a < na; ++a) copy_slot(p, a, scr, a);
+    p->n = na;
+    free(items);
+    return 0;
+}
+
+/* density! (MCStreamer/src/poisson.jl:65-84): arr[cell] += w*val/dV per active
+ * particle in population order (NGP), or the same amount spread with trilinear
+ * weights over the 8 nearest cell centres (CIC); shares outside the mesh drop. */
+void ref_deposit(const ref_pop *p, ref_mesh *m, int32_t which, int32_t scheme, double val)
+{
+    double *arr = which == 0 ? m->qfixed : m->qpart;
+    double h = m->h;
+    double dV = (h * h) * h;
+    int64_t ncells = (int64_t)m->n[0] * m->n[1] * m->n[2];
+    for (int64_t i = 0; i < p->n; ++i) {
+        if (!p->active[i]) continue;
+        const double *x = &p->x[3 * i];
+        int64_t key = ref_cell_key(m, x);
+        if (key >= ncells) continue;
+        double base = (p->w[i] * val) / dV;
+        if (scheme == 0) {
+            int64_t c0 = key % m->n[0];
+            int64_t c1 = (key / m->n[0]) % m->n[1];
+            int64_t c2 = key / ((int64_t)m->n[0] * m->n[1]);
+            arr[cidx(m->n, c0 + 1, c1 + 1, c2 + 1)] += base;
+            continue;
+        }
+        int64_t i0[3];
+        double fw[3];
+        for (int d = 0; d < 3; ++d) {
+            double g = x[d] / h - 0.5;
+            double fl = floor(g);
+            i0[d] = (int64_t)fl;
+            fw[d] = g - fl;
+        }
+        for (int dz = 0; dz < 2; ++dz) {
+            int64_t kz = i0[2] + dz;
+            if (kz < 0 || kz >= m->n[2]) continue;
+            double wz = dz ? fw[2] : 1.0 - fw[2];
+            for (int dy = 0; dy < 2; ++dy) {
+                int64_t ky = i0[1] + dy;
+                if (ky < 0 || ky >= m->n[1]) continue;
+                double wy = dy ? fw[1] : 1.0 - fw[1];
+                for (int dx = 0; dx < 2; ++dx) {
+                    int64_t kx = i0[0] + dx;
+                    if (kx < 0 || kx >= m->n[0]) continue;
+                    double wx = dx ? fw[0] : 1.0 - fw[0];
+                    arr[cidx(m->n, kx + 1, ky + 1, kz + 1)] += base * ((wx * wy) * wz);
+                }
+            }
+        }
+    }
+}
+
+/* poisson! charge assembly (poisson.jl:19-26): q = (qfixed + qpart - ne) e, ne = 0. */
+void ref_assemble_charge(ref_mesh *m)
+{
+    int64_t tot = (int64_t)(m->n[0] + 2) * (m->n[1] + 2) * (m->n[2] + 2);
+    for (int64_t c = 0; c < tot; ++c) m->q[c] = 0.0;
+    for (int64_t k = 1; k <= m->n[2]; ++k)
+        for (int64_t j = 1; j <= m->n[1]; ++j)
+            for (int64_t i = 1; i <= m->n[0]; ++i) {
+                int64_t c = cidx(m->n, i, j, k);
+                m->q[c] = (m->qfixed[c] + m->qpart[c]) * MCS_ELEMENTARY_CHARGE;
+            }
+}
+
+/* Face fields (poisson.jl:47-57): E_d = (u[c-1] - u[c])/h on faces, plus eb on
+ * z; tangential ghost layers copy their neighbours, y (or x) first, then z. */
+void ref_efield_faces(ref_mesh *m, double eb)
+{
+    int64_t nx = m->n[0], ny = m->n[1], nz = m->n[2];
+    double h = m->h;
+    const double *u = m->u;
+#define U(i, j, k) u[cidx(m->n, (i), (j), (k))]
+    /* EX (nx+1)(ny+2)(nz+2) */
+#define EX(i, j, k) m->ex[(i) + (nx + 1) * ((j) + (ny + 2) * (k))]
+    for (int64_t k = 1; k <= nz; ++k)
+        for (int64_t j = 1; j <= ny; ++j)
+            for (int64_t i = 0; i <= nx; ++i) EX(i, j, k) = (U(i, j, k) - U(i + 1, j, k)) / h;
+    for (int64_t k = 1; k <= nz; ++k)
+        for (int64_t i = 0; i <= nx; ++i) {
+            EX(i, 0, k) = EX(i, 1, k);
+            EX(i, ny + 1, k) = EX(i, ny, k);
+        }
+    for (int64_t j = 0; j <= ny + 1; ++j)
+        for (int64_t i = 0; i <= nx; ++i) {
+            EX(i, j, 0) = EX(i, j, 1);
+            EX(i, j, nz + 1) = EX(i, j, nz);
+        }
+#undef EX
+#define EY(i, j, k) m->ey[(i) + (nx + 2) * ((j) + (ny + 1) * (k))]
+    for (int64_t k = 1; k <= nz; ++k)
+        for (int64_t j = 0; j <= ny; ++j)
+            for (int64_t i = 1; i <= nx; ++i) EY(i, j, k) = (U(i, j, k) - U(i, j + 1, k)) / h;
+    for (int64_t k = 1; k <= nz; ++k)
+        for (int64_t j = 0; j <= ny; ++j) {
+            EY(0, j, k) = EY(1, j, k);
+            EY(nx + 1, j, k) = EY(nx, j, k);
+        }
+    for (int64_t j = 0; j <= ny; ++j)
+        for (int64_t i = 0; i <= nx + 1; ++i) {
+            EY(i, j, 0) = EY(i, j, 1);
+            EY(i, j, nz + 1) = EY(i, j, nz);
+        }
+#undef EY
+#define EZ(i, j, k) m->ez[(i) + (nx + 2) * ((j) + (ny + 2) * (k))]
+    for (int64_t k = 0; k <= nz; ++k)
+        for (int64_t j = 1; j <= ny; ++j)
+            for (int64_t i = 1; i <= nx; ++i)
+                EZ(i, j, k) = eb + (U(i, j, k) - U(i, j, k + 1)) / h;
+    for (int64_t k = 0; k <= nz; ++k)
+        for (int64_t j = 1; j <= ny; ++j) {
+            EZ(0, j, k) = EZ(1, j, k);
+            EZ(nx + 1, j, k) = EZ(nx, j, k);
+        }
+    for (int64_t k = 0; k <= nz; ++k)
+        for (int64_t i = 0; i <= nx + 1; ++i) {
+            EZ(i, 0, k) = EZ(i, 1, k);
+            EZ(i, ny + 1, k) = EZ(i, ny, k);
+        }
+#undef EZ
+#undef U
+}
+
+/* resample! (MCStreamer/src/resample.jl:7-84) on the canonical cell order.
+ * Pass 1 per cell: count, total weight, and the first systematic offset
+ * x = samplemin(ntarget) drawn when the count reaches nmax (:16-29).
+ * Pass 2 in order (:31-76): a cell with p > nmax emits copies of weight
+ * wtotal/ntarget while wtotal*x < wcum + w and advances x by
+ * samplemin(ntarget-pk)*(1-x), then drops the original; a cell with
+ * p < ntarget splits particles of w > 2 into round(w/2) and w - round(w/2),
+ * raising p each time.  Output per cell: surviving originals, then the new
+ * particles in creation order (new ids in output order); outside particles last. */
+int32_t ref_resample_cells(ref_pop *p, ref_mesh *m, int32_t ntarget, int32_t nmax,
+                           uint64_t seed, uint32_t instance, uint32_t step, ref_pop *scr,
+                           ref_resample_stats *st)
+{
+    memset(st, 0, sizeof(*st));
+    st->n_in = p->n;
+    int64_t ncells = (int64_t)m->n[0] * m->n[1] * m->n[2];
+    int64_t *cs = m->cell_start;
+    int64_t out = 0;
+    int64_t *newstart = (int64_t *)malloc(sizeof(int64_t) * (size_t)(ncells + 2));
+    /* created particles of one cell, staged before the survivors are known */
+    int64_t stage_cap = 1024;
+    int64_t *stage_src = (int64_t *)malloc(sizeof(int64_t) * (size_t)stage_cap);
+    double *stage_w = (double *)malloc(sizeof(double) * (size_t)stage_cap);
+    for (int64_t c = 0; c < ncells; ++c) {
+        newstart[c] = out;
+        int64_t a = cs[c], b = cs[c + 1];
+        int64_t pc = b - a;
+        if (pc == 0) continue;
+        mcs_stream rs;
+        mcs_stream_init(&rs, seed, instance, MCS_PHASE_CELL_RESAMPLE, (uint64_t)c, step);
+        double wtotal = 0.0, xx = 0.0, wcum = 0.0;
+        int64_t cnt = 0, pk = 0, nstage = 0;
+        for (int64_t i = a; i < b; ++i) {
+            cnt += 1;
+            wtotal += p->w[i];
+            if (cnt == nmax) xx = mcs_samplemin(&rs, (double)ntarget);
+        }
+        int merged = pc > nmax;
+        if (merged) st->cells_merged += 1;
+        for (int64_t i = a; i < b; ++i) {
+            double wi = p->w[i];
+            if (pc > nmax) {
+                while ((wtotal * xx < wcum + wi) && pk < ntarget) {
+                    pk += 1;
+                    if (nstage == stage_cap) {
+                        stage_cap *= 2;
+                        stage_src = (int64_t *)realloc(stage_src, sizeof(int64_t) * (size_t)stage_cap);
+                        stage_w = (double *)realloc(stage_w, sizeof(double) * (size_t)stage_cap);
+                    }
+                    stage_src[nstage] = i;
+                    stage_w[nstage] = wtotal / (double)ntarget;
+                    nstage++;
+                    st->clones += 1;
+                    if (pk < ntarget)
+                        xx = xx + mcs_samplemin(&rs, (double)(ntarget - pk)) * (1.0 - xx);
+                }
+                wcum = wcum + wi;
+                st->removed += 1;
+            } else if (pc < ntarget && wi > 2.0) {
+                double wnew = rint(wi / 2.0);
+                if (nstage == stage_cap) {
+                    stage_cap *= 2;
+                    stage_src = (int64_t *)realloc(stage_src, sizeof(int64_t) * (size_t)stage_cap);
+                    stage_w = (double *)realloc(stage_w, sizeof(double) * (size_t)stage_cap);
+                }
+                stage_src[nstage] = i;
+                stage_w[nstage] = wnew;
+                nstage++;
+                st->splits += 1;
+                p->w[i] = wi - wnew;
+                pc += 1;
+            }
+        }
+        if (out + (merged ? 0 : (b - a)) + nstage > scr->cap) {
+            free(newstart);
+            free(stage_src);
+            free(stage_w);
+            return FLAG_CAPACITY;
+        }
+        if (!merged) {
+            for (int64_t i = a; i < b; ++i) copy_slot(scr, out++, p, i);
+        }
+        for (int64_t q = 0; q < nstage; ++q) {
+            copy_slot(scr, out, p, stage_src[q]);
+            scr->w[out] = stage_w[q];
+            scr->id[out] = p->next_id;
+            p->next_id += 1;
+            out++;
+        }
+    }
+    newstart[ncells] = out;
+    for (int64_t i = cs[ncells]; i < cs[ncells + 1]; ++i) {
+        if (out >= scr->cap) {
+            free(newstart);
+            free(stage_src);
+            free(stage_w);
+            return FLAG_CAPACITY;
+        }
+        copy_slot(scr, out++, p, i);
+    }
+    newstart[ncells + 1] = out;
+    if (out > p->cap) {
+        free(newstart);
+        free(stage_src);
+        free(stage_w);
+        return FLAG_CAPACITY;
+    }
+    for (int64_t i = 0; i < out; ++i) copy_slot(p, i, scr, i);
+    p->n = out;
+    memcpy(cs, newstart, sizeof(int64_t) * (size_t)(ncells + 2));
+    st->n_out = out;
+    free(newstart);
+    free(stage_src);
+    free(stage_w);
+    return 0;
+}
+
+/* ------------------------------------------------------------- multigrid */
+
+static inline int64_t gsz(const int32_t n[3])
+{
+    return (int64_t)(n[0] + 2) * (n[1] + 2) * (n[2] + 2);
+}
+
+/* apply! with setbc! (Multigrid/src/boundaries.jl:171-194): faces in the order
+ * x-, x+, y-, y+, z-, z+; each ghost plane (full extent, ghosts included) is
+ * coef * (adjacent valid plane), coef = -1 Dirichlet, +1 Neumann. */
+void ref_mg_apply_bc(const int32_t n[3], const int32_t bc[6], double *a)
+{
+    int64_t sx = 1, sy = n[0] + 2, sz = (int64_t)(n[0] + 2) * (n[1] + 2);
+    int64_t ext[3] = {n[0] + 2, n[1] + 2, n[2] + 2};
+    int64_t str[3] = {sx, sy, sz};
+    for (int face = 0; face < 6; ++face) {
+        int d = face / 2;
+        int hi = face & 1;
+        double coef = bc[face] == 0 ? -1.0 : 1.0;
+        int64_t g = hi ? n[d] + 1 : 0;
+        int64_t vld = hi ? n[d] : 1;
+        int d1 = (d + 1) % 3, d2 = (d + 2) % 3;
+        for (int64_t b2 = 0; b2 < ext[d2]; ++b2)
+            for (int64_t b1 = 0; b1 < ext[d1]; ++b1) {
+                int64_t base = b1 * str[d1] + b2 * str[d2];
+                a[base + g * str[d]] = coef * a[base + vld * str[d]];
+            }
+    }
+}
+
+/* gauss_seidel! with redblack3 (Multigrid.jl:168-178, redblack.jl:49-61):
+ * colour 0 = cells with even (i+j+k) in interior coordinates, then colour 1;
+ * u += 1.0*(L u + b)/6 with L u = -6u + u[+x] + u[+y] + u[+z] + u[-x] + u[-y] + u[-z]. */
+void ref_mg_gauss_seidel(const int32_t n[3], double *u, const double *b)
+{
+    int64_t sy = n[0] + 2, sz = (int64_t)(n[0] + 2) * (n[1] + 2);
+    for (int par = 0; par < 2; ++par)
+        for (int64_t k = 1; k <= n[2]; ++k)
+            for (int64_t j = 1; j <= n[1]; ++j)
+                for (int64_t i = 1; i <= n[0]; ++i) {
+                    if ((((i - 1) + (j - 1) + (k - 1)) & 1) != par) continue;
+                    int64_t c = i + sy * j + sz * k;
+                    double l = -6.0 * u[c];
+                    l += u[c + 1];
+                    l += u[c + sy];
+                    l += u[c + sz];
+                    l += u[c - 1];
+                    l += u[c - sy];
+                    l += u[c - sz];
+                    u[c] += (1.0 * (l + b[c])) / 6.0;
+                }
+}
+
+/* residual! (Multigrid.jl:186-193): r = s b + L u on the interior. */
+void ref_mg_residual(const int32_t n[3], double *r, const double *u, const double *b, double s)
+{
+    int64_t sy = n[0] + 2, sz = (int64_t)(n[0] + 2) * (n[1] + 2);
+    for (int64_t k = 1; k <= n[2]; ++k)
+        for (int64_t j = 1; j <= n[1]; ++j)
+            for (int64_t i = 1; i <= n[0]; ++i) {
+                int64_t c = i + sy * j + sz * k;
+                double l = -6.0 * u[c];
+                l += u[c + 1];
+                l += u[c + sy];
+                l += u[c + sz];
+                l += u[c - 1];
+                l += u[c - sy];
+                l += u[c - sz];
+                r[c] = s * b[c] + l;
+            }
+}
+
+/* restrict! (Multigrid.jl:213-226): rh = 4 * (sum of the 8 children)/8, children
+ * visited with the x offset slowest (cubestencil order, Multigrid.jl:133-144). */
+void ref_mg_restrict(const int32_t nf[3], double *rh, const double *r)
+{
+    int32_t nc[3] = {nf[0] / 2, nf[1] / 2, nf[2] / 2};
+    int64_t fy = nf[0] + 2, fz = (int64_t)(nf[0] + 2) * (nf[1] + 2);
+    int64_t cy = nc[0] + 2, cz = (int64_t)(nc[0] + 2) * (nc[1] + 2);
+    for (int64_t k = 1; k <= nc[2]; ++k)
+        for (int64_t j = 1; j <= nc[1]; ++j)
+            for (int64_t i = 1; i <= nc[0]; ++i) {
+                int64_t f = (2 * i - 1) + fy * (2 * j - 1) + fz * (2 * k - 1);
+                double s = 0.0;
+                for (int dx = 0; dx < 2; ++dx)
+                    for (int dy = 0; dy < 2; ++dy)
+                        for (int dz = 0; dz < 2; ++dz) s += r[f + dx + fy * dy + fz * dz];
+                rh[i + cy * j + cz * k] = (4.0 * s) / 8.0;
+            }
+}
+
+/* interpolate! (Multigrid.jl:232-258): each fine child of a coarse cell gets
+ * sum_sh w(sh) * rh[coarse + delta.*sh], delta = 2s-1, w = prod(3-2 sh_i)/64,
+ * sh in cubestencil order; added to (add=1) or written into r. */
+void ref_mg_interpolate(const int32_t nf[3], double *r, const double *rh, int32_t add)
+{
+    int32_t nc[3] = {nf[0] / 2, nf[1] / 2, nf[2] / 2};
+    int64_t fy = nf[0] + 2, fz = (int64_t)(nf[0] + 2) * (nf[1] + 2);
+    int64_t cy = nc[0] + 2, cz = (int64_t)(nc[0] + 2) * (nc[1] + 2);
+    for (int64_t k = 1; k <= nc[2]; ++k)
+        for (int64_t j = 1; j <= nc[1]; ++j)
+            for (int64_t i = 1; i <= nc[0]; ++i) {
+                int64_t cc = i + cy * j + cz * k;
+                for (int sx = 0; sx < 2; ++sx)
+                    for (int sy = 0; sy < 2; ++sy)
+                        for (int sz = 0; sz < 2; ++sz) {
+                            int64_t fidx = (2 * i - 1 + sx) + fy * (2 * j - 1 + sy) + fz * (2 * k - 1 + sz);
+                            int64_t ddx = 2 * sx - 1, ddy = 2 * sy - 1, ddz = 2 * sz - 1;
+                            double v = 0.0;
+                            for (int hx = 0; hx < 2; ++hx)
+                                for (int hy = 0; hy < 2; ++hy)
+                                    for (int hz = 0; hz < 2; ++hz) {
+                                        double wgt = (double)((3 - 2 * hx) * (3 - 2 * hy) * (3 - 2 * hz)) / 64.0;
+                                        v += wgt * rh[cc + ddx * hx + cy * ddy * hy + cz * ddz * hz];
+                                    }
+                            if (add)
+                                r[fidx] += v;
+                            else
+                                r[fidx] = v;
+                        }
+            }
+}
+
+/* buildmat (Multigrid.jl:261-295): the 7-point operator on the coarsest interior
+ * with the ghost rule folded in (a missing neighbour adds coef to the diagonal),
+ * inverted by Gauss-Jordan elimination with partial pivoting (the reference
+ * factorises with UMFPACK; any exact solve restates it up to rounding). */
+int32_t ref_mg_coarse_inverse(const int32_t n[3], const int32_t bc[6], double *ainv)
+{
+    int64_t N = (int64_t)n[0] * n[1] * n[2];
+    double *a = (double *)calloc((size_t)(N * N), sizeof(double));
+    for (int64_t k = 0; k < n[2]; ++k)
+        for (int64_t j = 0; j < n[1]; ++j)
+            for (int64_t i = 0; i < n[0]; ++i) {
+                int64_t row = i + n[0] * (j + n[1] * k);
+                int64_t c[3] = {i, j, k};
+                a[row * N + row] -= 6.0;
+                /* lplstencil order: +x, +y, +z, -x, -y, -z */
+                for (int q = 0; q < 6; ++q) {
+                    int d = q % 3;
+                    int sgn = q < 3 ? 1 : -1;
+                    int64_t cc[3] = {c[0], c[1], c[2]};
+                    cc[d] += sgn;
+                    if (cc[d] < 0 || cc[d] >= n[d]) {
+                        int face = 2 * d + (sgn > 0 ? 1 : 0);
+                        a[row * N + row] += bc[face] == 0 ? -1.0 : 1.0;
+                    } else {
+                        int64_t col = cc[0] + n[0] * (cc[1] + n[1] * cc[2]);
+                        a[row * N + col] += 1.0;
+                    }
+                }
+            }
+    for (int64_t r = 0; r < N; ++r)
+        for (int64_t c = 0; c < N; ++c) ainv[r * N + c] = r == c ? 1.0 : 0.0;
+    for (int64_t col = 0; col < N; ++col) {
+        int64_t piv = col;
+        double best = fabs(a[col * N + col]);
+        for (int64_t r = col + 1; r < N; ++r)
+            if (fabs(a[r * N + col]) > best) {
+                best = fabs(a[r * N + col]);
+                piv = r;
+            }
+        if (best == 0.0) {
+            free(a);
+            return 1;
+        }
+        if (piv != col) {
+            for (int64_t c = 0; c < N; ++c) {
+                double t = a[col * N + c];
+                a[col * N + c] = a[piv * N + c];
+                a[piv * N + c] = t;
+                t = ainv[col * N + c];
+                ainv[col * N + c] = ainv[piv * N + c];
+                ainv[piv * N + c] = t;
+            }
+        }
+        double d = a[col * N + col];
+        for (int64_t c = 0; c < N; ++c) {
+            a[col * N + c] = a[col * N + c] / d;
+            ainv[col * N + c] = ainv[col * N + c] / d;
+        }
+        for (int64_t r = 0; r < N; ++r) {
+            if (r == col) continue;
+            double fct = a[r * N + col];
+            if (fct == 0.0) continue;
+            for (int64_t c = 0; c < N; ++c) {
+                a[r * N + c] = a[r * N + c] - fct * a[col * N + c];
+                ainv[r * N + c] = ainv[r * N + c] - fct * ainv[col * N + c];
+            }
+        }
+    }
+    free(a);
+    return 0;
+}
+
+typedef struct {
+    int32_t n[8][3];
+    double *res[8], *res1[8], *sol[8];
+    double *ainv;
+    int64_t ncoarse;
+    double tol_abs;
+} ref_ws;
+
+/* mgv! (Multigrid.jl:334-377) */
+static void ref_mgv(const ref_mg *cfg, ref_ws *ws, double *x, const double *b, int level)
+{
+    const int32_t *n = ws->n[level];
+    if (level == cfg->levels) {
+        for (int it = 0; it < 50; ++it) {
+            ref_mg_apply_bc(n, cfg->bc, x);
+            ref_mg_gauss_seidel(n, x, b);
+        }
+        return;
+    }
+    for (int it = 0; it < cfg->smooth1; ++it) {
+        ref_mg_apply_bc(n, cfg->bc, x);
+        ref_mg_gauss_seidel(n, x, b);
+    }
+    double *r = ws->res1[level];
+    ref_mg_apply_bc(n, cfg->bc, x);
+    ref_mg_residual(n, r, x, b, 1.0);
+    double *rh = ws->res[level + 1];
+    ref_mg_apply_bc(n, cfg->bc, r);
+    ref_mg_restrict(n, rh, r);
+    double *xh = ws->sol[level + 1];
+    memset(xh, 0, sizeof(double) * (size_t)gsz(ws->n[level + 1]));
+    ref_mgv(cfg, ws, xh, rh, level + 1);
+    ref_mg_interpolate(n, x, xh, 1);
+    for (int it = 0; it < cfg->smooth2; ++it) {
+        ref_mg_apply_bc(n, cfg->bc, x);
+        ref_mg_gauss_seidel(n, x, b);
+    }
+}
+
+/* fmg! (Multigrid.jl:380-434).  The coarsest correction is ws.mat \ btop, i.e.
+ * the operator applied inverse to the restricted residual, as written there. */
+static int ref_fmg(const ref_mg *cfg, ref_ws *ws, double *x, const double *b, double *eps_out)
+{
+    int L = cfg->levels;
+    const int32_t *n = ws->n[0];
+    ref_mg_apply_bc(n, cfg->bc, x);
+    memset(ws->res[0], 0, sizeof(double) * (size_t)gsz(n));
+    ref_mg_residual(n, ws->res[0], x, b, cfg->s);
+    double ss = 0.0;
+    int64_t tot = gsz(n);
+    for (int64_t c = 0; c < tot; ++c) ss += ws->res[0][c] * ws->res[0][c];
+    double eps = sqrt(ss) / sqrt((double)n[0] * (double)n[1] * (double)n[2]);
+    *eps_out = eps;
+    if (cfg->s * ws->tol_abs > eps) return 0;
+    for (int k = 0; k < L; ++k) ref_mg_restrict(ws->n[k], ws->res[k + 1], ws->res[k]);
+    /* coarse direct solve */
+    {
+        const int32_t *nc = ws->n[L];
+        double *z = ws->sol[L];
+        int64_t N = ws->ncoarse;
+        double *bt = (double *)malloc(sizeof(double) * (size_t)N);
+        for (int64_t k = 0; k < nc[2]; ++k)
+            for (int64_t j = 0; j < nc[1]; ++j)
+                for (int64_t i = 0; i < nc[0]; ++i)
+                    bt[i + nc[0] * (j + nc[1] * k)] = ws->res[L][cidx(nc, i + 1, j + 1, k + 1)];
+        for (int64_t k = 0; k < nc[2]; ++k)
+            for (int64_t j = 0; j < nc[1]; ++j)
+                for (int64_t i = 0; i < nc[0]; ++i) {
+                    int64_t row = i + nc[0] * (j + nc[1] * k);
+                    double acc = 0.0;
+                    for (int64_t c = 0; c < N; ++c) acc += ws->ainv[row * N + c] * bt[c];
+                    z[cidx(nc, i + 1, j + 1, k + 1)] = acc;
+                }
+        free(bt);
+    }
+    double *z = ws->sol[L];
+    for (int k = 1; k <= L; ++k) {
+        int lev = L - k;
+        double *z1 = ws->sol[lev];
+        memset(z1, 0, sizeof(double) * (size_t)gsz(ws->n[lev]));
+        ref_mg_apply_bc(ws->n[lev + 1], cfg->bc, z);
+        ref_mg_interpolate(ws->n[lev], z1, z, 0);
+        ref_mgv(cfg, ws, z1, ws->res[lev], lev);
+        z = z1;
+    }
+    for (int64_t c = 0; c < tot; ++c) x[c] += z[c];
+    ref_mg_apply_bc(n, cfg->bc, x);
+    return 1;
+}
+
+/* solve (Multigrid.jl:449-467): fmg! until converged or iter > maxiter. */
+int32_t ref_mg_solve(const ref_mg *cfg, double *u, const double *b, int32_t *iters,
+                     double *resid)
+{
+    ref_ws ws;
+    memset(&ws, 0, sizeof(ws));
+    if (cfg->levels < 0 || cfg->levels > 7) return 1;
+    for (int l = 0; l <= cfg->levels; ++l) {
+        for (int d = 0; d < 3; ++d) {
+            if (cfg->n[d] % (1 << l)) return 1;
+            ws.n[l][d] = cfg->n[d] >> l;
+        }
+        int64_t sz = gsz(ws.n[l]);
+        ws.res[l] = (double *)calloc((size_t)sz, sizeof(double));
+        ws.res1[l] = (double *)calloc((size_t)sz, sizeof(double));
+        ws.sol[l] = (double *)calloc((size_t)sz, sizeof(double));
+    }
+    const int32_t *nc = ws.n[cfg->levels];
+    ws.ncoarse = (int64_t)nc[0] * nc[1] * nc[2];
+    ws.ainv = (double *)malloc(sizeof(double) * (size_t)(ws.ncoarse * ws.ncoarse));
+    int32_t rc = ref_mg_coarse_inverse(nc, cfg->bc, ws.ainv);
+    ws.tol_abs = cfg->tolerance;
+    if (cfg->tolerance < 0.0) {
+        const int32_t *n0 = ws.n[0];
+        double ss = 0.0;
+        for (int64_t k = 1; k <= n0[2]; ++k)
+            for (int64_t j = 1; j <= n0[1]; ++j)
+                for (int64_t i = 1; i <= n0[0]; ++i) {
+                    double v = b[cidx(n0, i, j, k)];
+                    ss += v * v;
+                }
+        double brms = sqrt(ss) / sqrt((double)n0[0] * (double)n0[1] * (double)n0[2]);
+        ws.tol_abs = -cfg->tolerance * brms;
+        if (!(ws.tol_abs > 0.0)) ws.tol_abs = 2.2250738585072014e-308;
+    }
+    int iter = 0;
+    int cont = 1;
+    double eps = 0.0;
+    while (rc == 0 && cont && iter <= cfg->maxiter) {
+        cont = ref_fmg(cfg, &ws, u, b, &eps);
+        iter += 1;
+    }
+    *iters = iter;
+    *resid = eps;
+    for (int l = 0; l <= cfg->levels; ++l) {
+        free(ws.res[l]);
+        free(ws.res1[l]);
+        free(ws.sol[l]);
+    }
+    free(ws.ainv);
+    if (rc) return 1;
+    return iter > cfg->maxiter ? 6 : 0;
+}

@@ -1,0 +1,393 @@
+"""ctypes wrapper of the oracle (oracle/refcpu.c) -- test infrastructure only.
+
+Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the
+checker; the product (mcstreamer_amd/) never imports it.  `build()` compiles
+refcpu.c with gcc (contraction off) into oracle/librefcpu.so.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "refcpu.c")
+LIB = os.path.join(HERE, "librefcpu.so")
+MAXP = 32
+
+CFLAGS = ["-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared", "-std=c11", "-lm"]
+
+
+def build(omp=False):
+    """Compile the oracle (single-threaded; it is the checker, not a product path)."""
+    tmp = LIB + ".tmp"
+    cmd = ["gcc", *CFLAGS, SRC, "-o", tmp]
+    subprocess.run(cmd, check=True, capture_output=True, text=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+_PD = ctypes.POINTER(ctypes.c_double)
+_PU8 = ctypes.POINTER(ctypes.c_uint8)
+_PU64 = ctypes.POINTER(ctypes.c_uint64)
+_PI64 = ctypes.POINTER(ctypes.c_int64)
+_PI32 = ctypes.POINTER(ctypes.c_int32)
+_PU32 = ctypes.POINTER(ctypes.c_uint32)
+
+
+class RefPop(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("cap", ctypes.c_int64), ("next_id", ctypes.c_uint64),
+                ("x", _PD), ("v", _PD), ("w", _PD), ("s", _PD), ("active", _PU8), ("id", _PU64)]
+
+
+class RefTable(ctypes.Structure):
+    _fields_ = [("nproc", ctypes.c_int32), ("ne", ctypes.c_int32), ("egrid", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("energy", _PD), ("rate", _PD), ("kind", _PI32),
+                ("param", _PD), ("maxrate", ctypes.c_double)]
+
+
+class RefMesh(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32 * 3), ("pad", ctypes.c_int32), ("h", ctypes.c_double),
+                ("qfixed", _PD), ("qpart", _PD), ("q", _PD), ("u", _PD), ("ex", _PD),
+                ("ey", _PD), ("ez", _PD), ("cell_start", _PI64)]
+
+
+class RefField(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("pad", ctypes.c_int32), ("e", ctypes.c_double * 3),
+                ("mesh", ctypes.POINTER(RefMesh))]
+
+
+class RefStats(ctypes.Structure):
+    _fields_ = [("n_in", ctypes.c_int64), ("n_out", ctypes.c_int64), ("births", ctypes.c_int64),
+                ("deaths", ctypes.c_int64), ("flights", ctypes.c_int64),
+                ("events", ctypes.c_int64 * MAXP), ("flags", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
+
+
+class RefMoments(ctypes.Structure):
+    _fields_ = [("nparticles", ctypes.c_int64), ("nactive", ctypes.c_int64),
+                ("weight", ctypes.c_double), ("wenergy", ctypes.c_double),
+                ("maxenergy", ctypes.c_double), ("sum_x", ctypes.c_double * 3),
+                ("sum_x2", ctypes.c_double * 3), ("sum_v", ctypes.c_double * 3),
+                ("sum_energy", ctypes.c_double), ("collrate", ctypes.c_double * MAXP)]
+
+
+class RefResampleStats(ctypes.Structure):
+    _fields_ = [("n_in", ctypes.c_int64), ("n_out", ctypes.c_int64),
+                ("cells_merged", ctypes.c_int64), ("clones", ctypes.c_int64),
+                ("splits", ctypes.c_int64), ("removed", ctypes.c_int64)]
+
+
+class RefMG(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32 * 3), ("bc", ctypes.c_int32 * 6), ("s", ctypes.c_double),
+                ("levels", ctypes.c_int32), ("smooth1", ctypes.c_int32),
+                ("smooth2", ctypes.c_int32), ("maxiter", ctypes.c_int32),
+                ("tolerance", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(SRC):
+            build()
+        L = ctypes.CDLL(LIB)
+        P = ctypes.POINTER
+        L.ref_philox.argtypes = [_PU32, _PU32, _PU32]
+        L.ref_stream_u01.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                     ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int64, _PD]
+        L.ref_math.argtypes = [ctypes.c_int32, ctypes.c_int64, _PD, _PD, _PD]
+        L.ref_indweight.argtypes = [P(RefTable), ctypes.c_double, _PI64, _PD]
+        L.ref_indweight.restype = ctypes.c_int32
+        L.ref_efield.argtypes = [P(RefField), _PD, _PD, _PI32]
+        L.ref_yoshida.argtypes = [P(RefField), _PD, _PD, ctypes.c_double, _PI32]
+        L.ref_advance.argtypes = [P(RefPop), P(RefTable), P(RefField), ctypes.c_double,
+                                  P(RefMesh), ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                  P(RefStats)]
+        L.ref_advance.restype = ctypes.c_int32
+        L.ref_repack.argtypes = [P(RefPop)]
+        L.ref_moments_compute.argtypes = [P(RefPop), P(RefTable), P(RefMoments)]
+        L.ref_swarm_resample.argtypes = [P(RefPop), ctypes.c_int64, ctypes.c_double,
+                                         ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _PI32]
+        L.ref_swarm_resample.restype = ctypes.c_int32
+        L.ref_cell_key.argtypes = [P(RefMesh), _PD]
+        L.ref_cell_key.restype = ctypes.c_int64
+        L.ref_sort_by_cell.argtypes = [P(RefPop), P(RefMesh), P(RefPop)]
+        L.ref_deposit.argtypes = [P(RefPop), P(RefMesh), ctypes.c_int32, ctypes.c_int32, ctypes.c_double]
+        L.ref_assemble_charge.argtypes = [P(RefMesh)]
+        L.ref_efield_faces.argtypes = [P(RefMesh), ctypes.c_double]
+        L.ref_resample_cells.argtypes = [P(RefPop), P(RefMesh), ctypes.c_int32, ctypes.c_int32,
+                                         ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                         P(RefPop), P(RefResampleStats)]
+        L.ref_resample_cells.restype = ctypes.c_int32
+        L.ref_mg_apply_bc.argtypes = [_PI32, _PI32, _PD]
+        L.ref_mg_gauss_seidel.argtypes = [_PI32, _PD, _PD]
+        L.ref_mg_residual.argtypes = [_PI32, _PD, _PD, _PD, ctypes.c_double]
+        L.ref_mg_restrict.argtypes = [_PI32, _PD, _PD]
+        L.ref_mg_interpolate.argtypes = [_PI32, _PD, _PD, ctypes.c_int32]
+        L.ref_mg_coarse_inverse.argtypes = [_PI32, _PI32, _PD]
+        L.ref_mg_solve.argtypes = [P(RefMG), _PD, _PD, _PI32, _PD]
+        L.ref_mg_solve.restype = ctypes.c_int32
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_PD)
+
+
+# ---------------------------------------------------------------- high level
+
+
+class Pop:
+    """Numpy-backed population for the oracle (x, v as n x 3)."""
+
+    def __init__(self, x, v, w, s, active, ids=None, cap=None, next_id=None):
+        x = np.asarray(x, dtype=np.float64).reshape(-1, 3)
+        n = x.shape[0]
+        cap = max(int(cap if cap is not None else n), n, 1)
+        self.cap = cap
+        self.x = np.zeros((cap, 3))
+        self.v = np.zeros((cap, 3))
+        self.w = np.zeros(cap)
+        self.s = np.zeros(cap)
+        self.active = np.zeros(cap, dtype=np.uint8)
+        self.id = np.zeros(cap, dtype=np.uint64)
+        self.x[:n] = x
+        self.v[:n] = np.asarray(v, dtype=np.float64).reshape(n, 3)
+        self.w[:n] = w
+        self.s[:n] = s
+        self.active[:n] = active
+        if ids is None:
+            ids = np.arange(n, dtype=np.uint64)
+        self.id[:n] = ids
+        self.n = n
+        self.next_id = int(next_id) if next_id is not None else (int(np.max(ids)) + 1 if n else 0)
+
+    def cstruct(self):
+        p = RefPop()
+        p.n = self.n
+        p.cap = self.cap
+        p.next_id = self.next_id
+        p.x = _d(self.x)
+        p.v = _d(self.v)
+        p.w = _d(self.w)
+        p.s = _d(self.s)
+        p.active = self.active.ctypes.data_as(_PU8)
+        p.id = self.id.ctypes.data_as(_PU64)
+        return p
+
+    def sync_from(self, p):
+        self.n = p.n
+        self.next_id = p.next_id
+
+    def view(self):
+        n = self.n
+        return dict(x=self.x[:n].copy(), v=self.v[:n].copy(), w=self.w[:n].copy(),
+                    s=self.s[:n].copy(), active=self.active[:n].copy(), id=self.id[:n].copy())
+
+    def scratch(self):
+        return Pop(np.zeros((0, 3)), np.zeros((0, 3)), [], [], [], cap=self.cap)
+
+
+class Table:
+    def __init__(self, energy, rate, kind, param, maxrate, egrid=0):
+        self.energy = np.ascontiguousarray(energy, dtype=np.float64)
+        self.rate_pm = np.ascontiguousarray(rate, dtype=np.float64)  # nproc x ne
+        self.rate = np.ascontiguousarray(self.rate_pm.T)            # energy-major
+        self.kind = np.ascontiguousarray(kind, dtype=np.int32)
+        self.param = np.ascontiguousarray(param, dtype=np.float64)
+        self.maxrate = float(maxrate)
+        self.egrid = int(egrid)
+
+    def cstruct(self):
+        t = RefTable()
+        t.nproc = self.kind.size
+        t.ne = self.energy.size
+        t.egrid = self.egrid
+        t.energy = _d(self.energy)
+        t.rate = _d(self.rate)
+        t.kind = self.kind.ctypes.data_as(_PI32)
+        t.param = _d(self.param)
+        t.maxrate = self.maxrate
+        return t
+
+
+class Mesh:
+    def __init__(self, n, h):
+        self.n = tuple(int(v) for v in n)
+        nx, ny, nz = self.n
+        self.h = float(h)
+        g = (nx + 2) * (ny + 2) * (nz + 2)
+        self.qfixed = np.zeros(g)
+        self.qpart = np.zeros(g)
+        self.q = np.zeros(g)
+        self.u = np.zeros(g)
+        self.ex = np.zeros((nx + 1) * (ny + 2) * (nz + 2))
+        self.ey = np.zeros((nx + 2) * (ny + 1) * (nz + 2))
+        self.ez = np.zeros((nx + 2) * (ny + 2) * (nz + 1))
+        self.cell_start = np.zeros(nx * ny * nz + 2, dtype=np.int64)
+        self._c = None
+
+    def cstruct(self):
+        m = RefMesh()
+        for d in range(3):
+            m.n[d] = self.n[d]
+        m.h = self.h
+        m.qfixed = _d(self.qfixed)
+        m.qpart = _d(self.qpart)
+        m.q = _d(self.q)
+        m.u = _d(self.u)
+        m.ex = _d(self.ex)
+        m.ey = _d(self.ey)
+        m.ez = _d(self.ez)
+        m.cell_start = self.cell_start.ctypes.data_as(_PI64)
+        self._c = m
+        return m
+
+
+def uniform_field(e):
+    f = RefField()
+    f.kind = 0
+    for d in range(3):
+        f.e[d] = e[d]
+    return f
+
+
+def mesh_field(mesh):
+    f = RefField()
+    f.kind = 1
+    mc = mesh.cstruct()
+    f.mesh = ctypes.pointer(mc)
+    f._keep = mc
+    return f
+
+
+def advance(pop, table, field, dt, tracker_mesh=None, seed=0, instance=0, step=0):
+    L = lib()
+    p = pop.cstruct()
+    t = table.cstruct()
+    st = RefStats()
+    trk = ctypes.pointer(tracker_mesh.cstruct()) if tracker_mesh is not None else None
+    L.ref_advance(ctypes.byref(p), ctypes.byref(t), ctypes.byref(field), float(dt), trk,
+                  int(seed), int(instance), int(step), ctypes.byref(st))
+    pop.sync_from(p)
+    return st
+
+
+def repack(pop):
+    p = pop.cstruct()
+    lib().ref_repack(ctypes.byref(p))
+    pop.sync_from(p)
+
+
+def moments(pop, table=None):
+    p = pop.cstruct()
+    m = RefMoments()
+    t = table.cstruct() if table is not None else None
+    lib().ref_moments_compute(ctypes.byref(p), ctypes.byref(t) if t is not None else None,
+                              ctypes.byref(m))
+    return m
+
+
+def swarm_resample(pop, n0, tol, seed=0, instance=0, step=0):
+    p = pop.cstruct()
+    did = ctypes.c_int32()
+    rc = lib().ref_swarm_resample(ctypes.byref(p), int(n0), float(tol), int(seed), int(instance),
+                                  int(step), ctypes.byref(did))
+    pop.sync_from(p)
+    return rc, did.value
+
+
+def sort_by_cell(pop, mesh):
+    p = pop.cstruct()
+    scr = pop.scratch()
+    sp = scr.cstruct()
+    lib().ref_sort_by_cell(ctypes.byref(p), ctypes.byref(mesh.cstruct()), ctypes.byref(sp))
+    pop.sync_from(p)
+
+
+def deposit(pop, mesh, which, scheme, val):
+    p = pop.cstruct()
+    lib().ref_deposit(ctypes.byref(p), ctypes.byref(mesh.cstruct()), int(which), int(scheme), float(val))
+
+
+def assemble_charge(mesh):
+    lib().ref_assemble_charge(ctypes.byref(mesh.cstruct()))
+
+
+def efield_faces(mesh, eb):
+    lib().ref_efield_faces(ctypes.byref(mesh.cstruct()), float(eb))
+
+
+def resample_cells(pop, mesh, ntarget, nmax, seed=0, instance=0, step=0):
+    p = pop.cstruct()
+    scr = pop.scratch()
+    sp = scr.cstruct()
+    st = RefResampleStats()
+    rc = lib().ref_resample_cells(ctypes.byref(p), ctypes.byref(mesh.cstruct()), int(ntarget),
+                                  int(nmax), int(seed), int(instance), int(step), ctypes.byref(sp),
+                                  ctypes.byref(st))
+    pop.sync_from(p)
+    return rc, st
+
+
+def mg_solve(n, bc, s, levels, smooth1, smooth2, tolerance, maxiter, u, b):
+    cfg = RefMG()
+    for d in range(3):
+        cfg.n[d] = n[d]
+    for f in range(6):
+        cfg.bc[f] = bc[f]
+    cfg.s = s
+    cfg.levels = levels
+    cfg.smooth1 = smooth1
+    cfg.smooth2 = smooth2
+    cfg.maxiter = maxiter
+    cfg.tolerance = tolerance
+    it = ctypes.c_int32()
+    res = ctypes.c_double()
+    rc = lib().ref_mg_solve(ctypes.byref(cfg), _d(u), _d(b), ctypes.byref(it), ctypes.byref(res))
+    return rc, it.value, res.value
+
+
+def n32(n):
+    return np.ascontiguousarray(n, dtype=np.int32)
+
+
+def mg_op(name, *args):
+    """Direct access to the multigrid building blocks for tests."""
+    L = lib()
+    fn = getattr(L, "ref_mg_" + name)
+    conv = []
+    for a in args:
+        if isinstance(a, np.ndarray) and a.dtype == np.float64:
+            conv.append(_d(a))
+        elif isinstance(a, np.ndarray) and a.dtype == np.int32:
+            conv.append(a.ctypes.data_as(_PI32))
+        else:
+            conv.append(a)
+    return fn(*conv)
+
+
+def philox(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    lib().ref_philox(c, k, o)
+    return [o[i] for i in range(4)]
+
+
+def stream_u01(seed, instance, phase, pid, step, count):
+    out = np.zeros(count)
+    lib().ref_stream_u01(int(seed), int(instance), int(phase), int(pid), int(step), int(count), _d(out))
+    return out
+
+
+def math(which, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    o1 = np.zeros_like(x)
+    o2 = np.zeros_like(x)
+    lib().ref_math(int(which), x.size, _d(x), _d(o1), _d(o2))
+    return o1, o2
