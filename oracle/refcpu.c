@@ -34,6 +34,25 @@ void ref_stream_u01(uint64_t seed, uint32_t instance, uint32_t phase, uint64_t i
     for (int64_t i = 0; i < count; ++i) out[i] = mcs_u01(&st);
 }
 
+/* Draws of the reference's random primitives, one stream per sample (id0 + i):
+ * kind 0 rand(), 1 nextcoll() (util.jl:17), 2 Poisson(param) (Swarm.jl:216),
+ * 3 samplemin(param) (resample.jl:84), 4 randsphere() (util.jl:4-12, 3 values). */
+void ref_draws(uint64_t seed, uint32_t instance, uint32_t phase, uint64_t id0, uint32_t step,
+               int32_t kind, double param, int64_t count, double *out)
+{
+    for (int64_t i = 0; i < count; ++i) {
+        mcs_stream st;
+        mcs_stream_init(&st, seed, instance, phase, id0 + (uint64_t)i, step);
+        switch (kind) {
+        case 0: out[i] = mcs_u01(&st); break;
+        case 1: out[i] = mcs_nextcoll(&st); break;
+        case 2: out[i] = (double)mcs_poisson(&st, param); break;
+        case 3: out[i] = mcs_samplemin(&st, param); break;
+        default: mcs_randsphere(&st, &out[3 * i]); break;
+        }
+    }
+}
+
 void ref_math(int32_t which, int64_t count, const double *in, double *out, double *out2)
 {
     for (int64_t i = 0; i < count; ++i) {

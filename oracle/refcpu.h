@@ -117,6 +117,8 @@ typedef struct ref_mg {
 void ref_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void ref_stream_u01(uint64_t seed, uint32_t instance, uint32_t phase, uint64_t id,
                     uint32_t step, int64_t count, double *out);
+void ref_draws(uint64_t seed, uint32_t instance, uint32_t phase, uint64_t id0, uint32_t step,
+               int32_t kind, double param, int64_t count, double *out);
 void ref_math(int32_t which, int64_t count, const double *in, double *out, double *out2);
 int32_t ref_indweight(const ref_table *t, double eng, int64_t *k, double *w);
 void ref_efield(const ref_field *f, const double x[3], double e[3], int32_t *flags);

@@ -98,6 +98,8 @@ def lib():
         L.ref_philox.argtypes = [_PU32, _PU32, _PU32]
         L.ref_stream_u01.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                      ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int64, _PD]
+        L.ref_draws.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+                                ctypes.c_uint32, ctypes.c_int32, ctypes.c_double, ctypes.c_int64, _PD]
         L.ref_math.argtypes = [ctypes.c_int32, ctypes.c_int64, _PD, _PD, _PD]
         L.ref_indweight.argtypes = [P(RefTable), ctypes.c_double, _PI64, _PD]
         L.ref_indweight.restype = ctypes.c_int32
@@ -383,6 +385,13 @@ def stream_u01(seed, instance, phase, pid, step, count):
     out = np.zeros(count)
     lib().ref_stream_u01(int(seed), int(instance), int(phase), int(pid), int(step), int(count), _d(out))
     return out
+
+
+def draws(kind, count, param=0.0, seed=1, instance=0, phase=0, id0=0, step=0):
+    out = np.zeros(count * (3 if kind == 4 else 1))
+    lib().ref_draws(int(seed), int(instance), int(phase), int(id0), int(step), int(kind),
+                    float(param), int(count), _d(out))
+    return out.reshape(count, 3) if kind == 4 else out
 
 
 def math(which, x):
