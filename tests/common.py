@@ -36,3 +36,31 @@ def maxwell_states(n, kT_ev=1.0, seed=0, x0=(0.0, 0.0, 0.0), spread=0.0):
     x = np.tile(np.asarray(x0, dtype=np.float64), (n, 1)) + rng.normal(size=(n, 3)) * spread
     s = -np.log1p(-rng.uniform(size=n))
     return x, v, np.ones(n), s, np.ones(n, dtype=np.uint8)
+
+
+def mesh_geometry(n, h):
+    nx, ny, nz = n
+    return np.array(n, dtype=np.float64) * h
+
+
+def random_particles(n, L, seed, p_inactive=0.0, p_outside=0.0, kT_ev=2.0):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0, 1, size=(n, 3)) * L
+    out = rng.uniform(size=n) < p_outside
+    x[out] = x[out] + rng.choice([-1.0, 1.0], size=(out.sum(), 1)) * L * 0.6
+    v = rng.normal(size=(n, 3)) * np.sqrt(kT_ev * co.eV / co.electron_mass)
+    w = rng.integers(1, 6, size=n).astype(np.float64)
+    s = -np.log1p(-rng.uniform(size=n))
+    act = (rng.uniform(size=n) >= p_inactive).astype(np.uint8)
+    return x, v, w, s, act
+
+
+def smooth_potential(n, h, amp=0.5, seed=0):
+    """A smooth cell-centred potential with ghosts (x fastest), volts."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = n
+    k, j, i = np.meshgrid(np.arange(nz + 2), np.arange(ny + 2), np.arange(nx + 2), indexing="ij")
+    ph = rng.uniform(0, 2 * np.pi, 3)
+    u = amp * (np.sin(2 * np.pi * i / nx + ph[0]) * np.cos(2 * np.pi * j / ny + ph[1])
+               + np.sin(np.pi * k / nz + ph[2]))
+    return u.reshape(-1)
