@@ -407,6 +407,8 @@ extern "C" int32_t mcs_pop_destroy(mcs_pop *p)
     hipFree(p->i32c);
     hipFree(p->u64a);
     hipFree(p->blk);
+    if (p->dep_w) hipFree(p->dep_w);
+    if (p->dep_k) hipFree(p->dep_k);
     delete p;
     return MCS_OK;
 }
@@ -590,26 +592,45 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_moments(PopBuf b, int64_t n, doub
     if (threadIdx.x == 0) rec[2 * MOM_NV + 1] = sh[0];
 }
 
-// Combine the block records in block order: thread q owns quantity q.
-__global__ void k_moments_final(const double *partial, int nb, double *out)
+// Combine the block records with a fixed pairwise tree: block q of the grid owns
+// quantity q (deterministic for a given grid size).
+__global__ __launch_bounds__(MCS_BLOCK) void k_moments_final(const double *partial, int nb, double *out)
 {
-    int q = threadIdx.x;
-    if (q < MOM_NV) {
-        Neu a{0.0, 0.0};
-        for (int b = 0; b < nb; ++b) {
-            Neu r{partial[(int64_t)b * MOM_REC + 2 * q], partial[(int64_t)b * MOM_REC + 2 * q + 1]};
+    int q = blockIdx.x;
+    __shared__ double sh[MCS_BLOCK], shc[MCS_BLOCK];
+    Neu a{0.0, 0.0};
+    double m = -HUGE_VAL;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        const double *rec = partial + (int64_t)b * MOM_REC;
+        if (q < MOM_NV) {
+            Neu r{rec[2 * q], rec[2 * q + 1]};
             neu_merge(a, r);
+        } else if (q == MOM_NV) {
+            m = fmax(m, rec[2 * MOM_NV]);
+        } else {
+            a.s += rec[2 * MOM_NV + 1];
         }
-        out[q] = a.s + a.c;
-    } else if (q == MOM_NV) {
-        double m = -HUGE_VAL;
-        for (int b = 0; b < nb; ++b) m = fmax(m, partial[(int64_t)b * MOM_REC + 2 * MOM_NV]);
-        out[q] = m;
-    } else if (q == MOM_NV + 1) {
-        double c = 0.0;
-        for (int b = 0; b < nb; ++b) c += partial[(int64_t)b * MOM_REC + 2 * MOM_NV + 1];
-        out[q] = c;
     }
+    sh[threadIdx.x] = q == MOM_NV ? m : a.s;
+    shc[threadIdx.x] = a.c;
+    __syncthreads();
+    for (int o = MCS_BLOCK / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            if (q < MOM_NV) {
+                Neu x{sh[threadIdx.x], shc[threadIdx.x]};
+                Neu y{sh[threadIdx.x + o], shc[threadIdx.x + o]};
+                neu_merge(x, y);
+                sh[threadIdx.x] = x.s;
+                shc[threadIdx.x] = x.c;
+            } else if (q == MOM_NV) {
+                sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+            } else {
+                sh[threadIdx.x] += sh[threadIdx.x + o];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = q < MOM_NV ? sh[0] + shc[0] : sh[0];
 }
 
 #define CR_CHUNK 8
@@ -688,7 +709,7 @@ extern "C" int32_t mcs_pop_moments(mcs_pop *p, const mcs_table *t, mcs_moments *
     double *res = partial + (size_t)nb * (MOM_REC + 2 * CR_CHUNK);
     hipLaunchKernelGGL(k_moments, dim3(nb), dim3(MCS_BLOCK), 0, ctx->stream, p->buf[0], p->n,
                        partial);
-    hipLaunchKernelGGL(k_moments_final, dim3(1), dim3(64), 0, ctx->stream, partial, nb, res);
+    hipLaunchKernelGGL(k_moments_final, dim3(MOM_NV + 2), dim3(MCS_BLOCK), 0, ctx->stream, partial, nb, res);
     MCS_LAUNCH_CHECK(ctx);
     double h[MOM_NV + 2];
     MCS_HIP(ctx, hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));

@@ -129,6 +129,8 @@ struct mcs_pop {
     int32_t *i32a = nullptr, *i32b = nullptr, *i32c = nullptr;
     uint64_t *u64a = nullptr;
     int64_t *blk = nullptr;   // block totals, then their prefix (2 * (cap/256 + 2) + 4)
+    void *dep_w = nullptr;     // CIC deposit records (double4 x cap), allocated on first use
+    uint64_t *dep_k = nullptr;
 };
 
 struct mcs_mesh {
@@ -164,6 +166,11 @@ struct mcs_mg {
     int64_t npartial = 0;
     int32_t *bc_dev = nullptr;                   // boundary kinds on the device
     double tol_abs = 0.0;                        // tolerance in force for the current solve
+    size_t tail_lds = 0;                         // LDS budget of the single-workgroup tail
+    bool use_graph = true;                       // replay the cycle body as a HIP graph
+    hipGraphExec_t graph_exec = nullptr;
+    double *graph_x = nullptr;
+    const double *graph_b = nullptr;
 };
 
 // ---- device helpers --------------------------------------------------------

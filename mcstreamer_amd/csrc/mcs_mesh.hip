@@ -4,7 +4,7 @@
 
 #include "mcs_device.h"
 
-#define M_SORTED 0
+#define M_SHORT 0
 #define M_BIG 1
 
 // ---------------------------------------------------------------- mesh object
@@ -175,44 +175,63 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_sort_scatter(PopBuf b, int64_t n,
 
 #define SEG_SMALL 64
 
-// Within each cell, order by particle id (insertion sort for short segments;
-// long ones are listed for the block-wide rank sort below).
-__global__ __launch_bounds__(MCS_BLOCK) void k_segsort_small(const int32_t *cs, int64_t nseg,
-                                                             int32_t *perm, uint64_t *sid,
-                                                             int32_t *biglist, int64_t *nbig)
+// Within each cell, order by particle id.  Thread per cell: segments of 2..64
+// particles go to a list sorted one wave per segment (k_segsort_wave), longer
+// ones to a list sorted one block per segment (k_segsort_big).  The lists share
+// one buffer: short ones fill it from the front, long ones from the back.
+__global__ __launch_bounds__(MCS_BLOCK) void k_segsort_list(const int32_t *cs, int64_t nseg,
+                                                            int32_t *lists, int64_t *nshort,
+                                                            int64_t *nbig)
 {
     int64_t c = gidx();
     if (c >= nseg) return;
-    int64_t a = cs[c], e = cs[c + 1];
-    int64_t len = e - a;
+    int64_t len = (int64_t)cs[c + 1] - cs[c];
     if (len <= 1) return;
     if (len > SEG_SMALL) {
         unsigned long long k = atomicAdd((unsigned long long *)nbig, 1ULL);
-        biglist[k] = (int32_t)c;
-        return;
+        lists[nseg - 1 - (int64_t)k] = (int32_t)c;
+    } else {
+        unsigned long long k = atomicAdd((unsigned long long *)nshort, 1ULL);
+        lists[k] = (int32_t)c;
     }
-    for (int64_t q = a + 1; q < e; ++q) {
-        uint64_t key = sid[q];
-        int32_t pv = perm[q];
-        int64_t r = q - 1;
-        while (r >= a && sid[r] > key) {
-            sid[r + 1] = sid[r];
-            perm[r + 1] = perm[r];
-            r -= 1;
+}
+
+// One wave per short segment: lane l holds element l; its rank is the number of
+// smaller ids in the segment (ids are unique).
+__global__ __launch_bounds__(MCS_BLOCK) void k_segsort_wave(const int32_t *cs, int32_t *perm,
+                                                            uint64_t *sid, const int32_t *lists,
+                                                            const int64_t *nshort)
+{
+    const int lane = threadIdx.x & (MCS_WAVE - 1);
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / MCS_WAVE;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / MCS_WAVE;
+    const int64_t ns = *nshort;
+    for (int64_t q = wave; q < ns; q += nwaves) {
+        int32_t c = lists[q];
+        int64_t a = cs[c];
+        int len = (int)((int64_t)cs[c + 1] - a);
+        uint64_t key = lane < len ? sid[a + lane] : ~0ULL;
+        int32_t pv = lane < len ? perm[a + lane] : 0;
+        int rank = 0;
+        for (int m = 0; m < len; ++m) {
+            uint64_t other = __shfl(key, m, MCS_WAVE);
+            rank += other < key ? 1 : 0;
         }
-        sid[r + 1] = key;
-        perm[r + 1] = pv;
+        if (lane < len) {
+            sid[a + rank] = key;
+            perm[a + rank] = pv;
+        }
     }
 }
 
 __global__ __launch_bounds__(MCS_BLOCK) void k_segsort_big(const int32_t *cs, int32_t *perm,
-                                                           uint64_t *sid, const int32_t *biglist,
-                                                           const int64_t *nbig, int32_t *tmp_perm,
-                                                           uint64_t *tmp_sid)
+                                                           uint64_t *sid, const int32_t *lists,
+                                                           int64_t nseg, const int64_t *nbig,
+                                                           int32_t *tmp_perm, uint64_t *tmp_sid)
 {
     int64_t nb = *nbig;
     for (int64_t q = blockIdx.x; q < nb; q += gridDim.x) {
-        int32_t c = biglist[q];
+        int32_t c = lists[nseg - 1 - q];
         int64_t a = cs[c], e = cs[c + 1];
         for (int64_t t = a + threadIdx.x; t < e; t += blockDim.x) {
             uint64_t key = sid[t];
@@ -265,13 +284,16 @@ extern "C" int32_t mcs_sort_by_cell(mcs_pop *p, mcs_mesh *m)
                            ctx->stream, p->buf[0], n, (const int32_t *)p->i32a,
                            (const int32_t *)p->i32b, (const int32_t *)m->cell_start, p->i32c,
                            p->u64a);
-        // the count buffer is free again: it holds the list of long segments
-        hipLaunchKernelGGL(k_segsort_small, dim3((unsigned)cdiv(nseg, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
-                           ctx->stream, (const int32_t *)m->cell_start, nseg, p->i32c, p->u64a,
-                           m->cell_cnt, ctr + M_BIG);
+        // the count buffer is free again: it holds the lists of segments to sort
+        hipLaunchKernelGGL(k_segsort_list, dim3((unsigned)cdiv(nseg, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                           ctx->stream, (const int32_t *)m->cell_start, nseg, m->cell_cnt,
+                           ctr + M_SHORT, ctr + M_BIG);
+        hipLaunchKernelGGL(k_segsort_wave, dim3(2048), dim3(MCS_BLOCK), 0, ctx->stream,
+                           (const int32_t *)m->cell_start, p->i32c, p->u64a,
+                           (const int32_t *)m->cell_cnt, (const int64_t *)(ctr + M_SHORT));
         hipLaunchKernelGGL(k_segsort_big, dim3(512), dim3(MCS_BLOCK), 0, ctx->stream,
                            (const int32_t *)m->cell_start, p->i32c, p->u64a,
-                           (const int32_t *)m->cell_cnt, (const int64_t *)(ctr + M_BIG), p->i32b,
+                           (const int32_t *)m->cell_cnt, nseg, (const int64_t *)(ctr + M_BIG), p->i32b,
                            (uint64_t *)p->buf[1].id);
     }
     MCS_LAUNCH_CHECK(ctx);
@@ -303,18 +325,63 @@ struct DepArgs {
     MeshGeom g;
     double *arr;
     double val;
+    // per-particle CIC record, computed once: the lower cell index per axis packed
+    // in 21-bit fields (offset by one), and {fr_x, fr_y, fr_z, w*val/dV}
+    uint64_t *pk;
+    double4 *pw;
+    int64_t n;
 };
+
+#define DEP_MASK 0x1FFFFFULL
+
+// Per particle: g = x/h - 1/2, i0 = floor(g), fr = g - i0, base = (w val)/dV,
+// exactly the quantities a sequential CIC scatter computes per particle.
+__global__ __launch_bounds__(MCS_BLOCK) void k_deposit_prep(DepArgs a)
+{
+    int64_t q = gidx();
+    if (q >= a.n) return;
+    const double h = a.g.h;
+    const double dV = (h * h) * h;
+    double fr[3];
+    uint64_t key = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        double gq = a.b.x[d][q] / h - 0.5;
+        double fl = floor(gq);
+        fr[d] = gq - fl;
+        // particles outside [-1, n] along an axis are never gathered: clamp the field
+        double fc = fmin(fmax(fl + 1.0, 0.0), (double)DEP_MASK);
+        key |= ((uint64_t)fc & DEP_MASK) << (21 * d);
+    }
+    a.pk[q] = key;
+    a.pw[q] = make_double4(fr[0], fr[1], fr[2], (a.b.w[q] * a.val) / dV);
+}
+
+__device__ __forceinline__ void dep_visit(double &acc, uint64_t key, const double4 &wv, int64_t i,
+                                          int64_t j, int64_t k)
+{
+    int64_t ix = (int64_t)(key & DEP_MASK) - 1;
+    int64_t iy = (int64_t)((key >> 21) & DEP_MASK) - 1;
+    int64_t iz = (int64_t)((key >> 42) & DEP_MASK) - 1;
+    if ((i != ix && i != ix + 1) || (j != iy && j != iy + 1) || (k != iz && k != iz + 1)) return;
+    double wx = i == ix ? 1.0 - wv.x : wv.x;
+    double wy = j == iy ? 1.0 - wv.y : wv.y;
+    double wz = k == iz ? 1.0 - wv.z : wv.z;
+    acc += wv.w * ((wx * wy) * wz);
+}
 
 // density! as a per-cell gather over the canonical order: cell c sums the shares of
 // the particles of its 27 neighbours in (cell, id) order, which is the order in
 // which a sequential scatter over the sorted population adds them.  No atomics.
+// Loads run four particles ahead of the (sequential) accumulation.
 template <int CIC>
 __global__ __launch_bounds__(MCS_BLOCK) void k_deposit(DepArgs a)
 {
     int64_t c = gidx();
     if (c >= a.g.ncells) return;
     const int64_t nx = a.g.n[0], ny = a.g.n[1], nz = a.g.n[2];
-    int64_t i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+    uint32_t cu = (uint32_t)c, qy = cu / (uint32_t)nx;
+    int64_t i = cu - qy * (uint32_t)nx, j = qy % (uint32_t)ny, k = qy / (uint32_t)ny;
     int64_t idx = (i + 1) + (nx + 2) * ((j + 1) + (ny + 2) * (k + 1));
     const double h = a.g.h;
     const double dV = (h * h) * h;
@@ -325,33 +392,25 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_deposit(DepArgs a)
         a.arr[idx] = acc;
         return;
     }
+    // the three neighbours along x of one (y, z) row hold one contiguous range
+    const int64_t ilo = i > 0 ? i - 1 : 0, ihi = i + 1 < nx ? i + 1 : nx - 1;
     for (int dk = -1; dk <= 1; ++dk) {
         int64_t kk = k + dk;
         if (kk < 0 || kk >= nz) continue;
         for (int dj = -1; dj <= 1; ++dj) {
             int64_t jj = j + dj;
             if (jj < 0 || jj >= ny) continue;
-            for (int di = -1; di <= 1; ++di) {
-                int64_t ii = i + di;
-                if (ii < 0 || ii >= nx) continue;
-                int64_t nc = ii + nx * (jj + ny * kk);
-                int64_t s0 = a.cs[nc], s1 = a.cs[nc + 1];
-                for (int64_t q = s0; q < s1; ++q) {
-                    double gx = a.b.x[0][q] / h - 0.5;
-                    double gy = a.b.x[1][q] / h - 0.5;
-                    double gz = a.b.x[2][q] / h - 0.5;
-                    double fx = floor(gx), fy = floor(gy), fz = floor(gz);
-                    int64_t ix = (int64_t)fx, iy = (int64_t)fy, iz = (int64_t)fz;
-                    if ((i != ix && i != ix + 1) || (j != iy && j != iy + 1) ||
-                        (k != iz && k != iz + 1))
-                        continue;
-                    double wx = i == ix ? 1.0 - (gx - fx) : gx - fx;
-                    double wy = j == iy ? 1.0 - (gy - fy) : gy - fy;
-                    double wz = k == iz ? 1.0 - (gz - fz) : gz - fz;
-                    double base = (a.b.w[q] * a.val) / dV;
-                    acc += base * ((wx * wy) * wz);
-                }
+            int64_t row = nx * (jj + ny * kk);
+            int64_t q = a.cs[row + ilo], s1 = a.cs[row + ihi + 1];
+            for (; q + 4 <= s1; q += 4) {
+                uint64_t k0 = a.pk[q], k1 = a.pk[q + 1], k2 = a.pk[q + 2], k3 = a.pk[q + 3];
+                double4 w0 = a.pw[q], w1 = a.pw[q + 1], w2 = a.pw[q + 2], w3 = a.pw[q + 3];
+                dep_visit(acc, k0, w0, i, j, k);
+                dep_visit(acc, k1, w1, i, j, k);
+                dep_visit(acc, k2, w2, i, j, k);
+                dep_visit(acc, k3, w3, i, j, k);
             }
+            for (; q < s1; ++q) dep_visit(acc, a.pk[q], a.pw[q], i, j, k);
         }
     }
     a.arr[idx] = acc;
@@ -374,13 +433,24 @@ extern "C" int32_t mcs_deposit(mcs_pop *p, mcs_mesh *m, int32_t which, int32_t s
     a.g = geom(m);
     a.arr = m->arr[which];
     a.val = val;
+    if (scheme == MCS_DEPOSIT_CIC && !p->dep_w) {
+        MCS_HIP(ctx, hipMalloc(&p->dep_w, sizeof(double4) * p->cap));
+        MCS_HIP(ctx, hipMalloc(&p->dep_k, sizeof(uint64_t) * p->cap));
+    }
+    a.pk = p->dep_k;
+    a.pw = (double4 *)p->dep_w;
+    a.n = p->n;
     {
         ProfScope ps(ctx, "deposit");
         dim3 g((unsigned)cdiv(m->ncells, MCS_BLOCK)), bl(MCS_BLOCK);
-        if (scheme == MCS_DEPOSIT_CIC)
+        if (scheme == MCS_DEPOSIT_CIC) {
+            if (p->n > 0)
+                hipLaunchKernelGGL(k_deposit_prep, dim3((unsigned)cdiv(p->n, MCS_BLOCK)), bl, 0,
+                                   ctx->stream, a);
             hipLaunchKernelGGL(k_deposit<1>, g, bl, 0, ctx->stream, a);
-        else
+        } else {
             hipLaunchKernelGGL(k_deposit<0>, g, bl, 0, ctx->stream, a);
+        }
     }
     MCS_LAUNCH_CHECK(ctx);
     return MCS_OK;
@@ -533,11 +603,7 @@ __device__ void resample_cell(int64_t c, int64_t a, int64_t e, int32_t ntarget, 
                 wkeep = wi - wnew;
                 pc += 1;
             }
-            if (WRITE) {
-                int64_t d = base + (i - a);
-                copy_particle(src, i, dst, d);
-                dst.w[d] = wkeep;
-            }
+            if (WRITE && wkeep != wi) dst.w[base + (i - a)] = wkeep;  // survivors copied by k_resample_copy
         }
     }
     nout = (merged ? 0 : (e - a)) + ncreated;
@@ -591,7 +657,6 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_resample_write(ResArgs r)
     int64_t crbase = pre >> 32;
     r.new_cs[c] = (int32_t)base;
     if (c == r.ncells) {
-        for (int64_t i = a; i < e; ++i) copy_particle(r.src, i, r.dst, base + (i - a));
         int64_t tot = r.prefix[c + 1];
         r.new_cs[c + 1] = (int32_t)(tot & 0xffffffffLL);
         return;
@@ -599,6 +664,21 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_resample_write(ResArgs r)
     int64_t nout, ncr;
     resample_cell<true>(c, a, e, r.ntarget, r.nmax, r.seed, r.instance, r.step, r.src, r.dst, base,
                         r.next_id + (uint64_t)crbase, nout, ncr, nullptr);
+}
+
+// Survivors (particles of cells that are not merged, and those outside the mesh)
+// move as whole records, one thread per particle; their slot is the cell's output
+// start plus their rank in the cell.
+__global__ __launch_bounds__(MCS_BLOCK) void k_resample_copy(ResArgs r, MeshGeom g, int64_t n)
+{
+    int64_t i = gidx();
+    if (i >= n) return;
+    int64_t c = cell_of(g.n, g.h, r.src.x[0][i], r.src.x[1][i], r.src.x[2][i]);
+    if (c < 0) c = r.ncells;
+    int64_t a = r.cs[c];
+    if (c < r.ncells && r.cs[c + 1] - a > r.nmax) return;
+    int64_t base = r.prefix[c] & 0xffffffffLL;
+    copy_particle(r.src, i, r.dst, base + (i - a));
 }
 
 extern "C" int32_t mcs_resample_cells(mcs_pop *p, mcs_mesh *m, int32_t ntarget, int32_t nmax,
@@ -656,6 +736,9 @@ extern "C" int32_t mcs_resample_cells(mcs_pop *p, mcs_mesh *m, int32_t ntarget, 
                         (long long)nout, (long long)p->cap);
     {
         ProfScope ps(ctx, "resample");
+        if (p->n > 0)
+            hipLaunchKernelGGL(k_resample_copy, dim3((unsigned)cdiv(p->n, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                               ctx->stream, r, geom(m), p->n);
         hipLaunchKernelGGL(k_resample_write, dim3((unsigned)cdiv(nseg, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
                            ctx->stream, r);
     }

@@ -1,14 +1,35 @@
 // mcs_mg.hip -- geometric multigrid for L u + s b = 0 on a cell-centred 3D
 // mesh with one ghost layer (Multigrid/src/Multigrid.jl, redblack.jl,
-// boundaries.jl).  The host drives the same sequence of operations as
-// fmg!/mgv!; each operation is a coalesced stencil kernel over one level.
+// boundaries.jl).  The host drives the operation sequence of fmg!/mgv!; each
+// operation is a coalesced stencil kernel over one level, and the levels below
+// a few thousand cells run as one single-workgroup kernel.
+//
+// Ghost handling.  apply! (boundaries.jl:180-194) sets every ghost plane to
+// +-1 times the adjacent valid plane, x first, then y, then z; a ghost cell
+// therefore equals the product of the +-1 factors of its ghost directions
+// times the nearest interior cell.  For the 7-point stencil the ghost neighbour
+// of a boundary cell is that cell itself times the factor, so smoothing and
+// residual kernels read it from the cell directly ("folded") -- the same value
+// the materialised ghost would hold at that moment.  Ghosts are written to
+// memory only where a later operation reads them: before the last smoothing
+// sweep of a level (the prolongation into the finer level reads them), before
+// the coarse-to-fine prolongation of fmg!, and at the start and end of fmg!.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "mcs_device.h"
 
+#define MG_MAXL 13
+#define MG_TAIL_LDS (128 * 1024)
+#define MG_TAIL_THREADS 64
+
 struct Lvl {
     int32_t n[3];
+};
+
+struct BcCoef {
+    double lo[3], hi[3];
 };
 
 __device__ __forceinline__ int64_t g_index(const Lvl &L, int64_t i, int64_t j, int64_t k)
@@ -16,66 +37,214 @@ __device__ __forceinline__ int64_t g_index(const Lvl &L, int64_t i, int64_t j, i
     return i + (int64_t)(L.n[0] + 2) * (j + (int64_t)(L.n[1] + 2) * k);
 }
 
-// setbc! for both faces of one dimension (boundaries.jl:171-178): the ghost
-// plane, over the full extent of the other two dimensions, is coef times the
-// adjacent valid plane.
-__global__ __launch_bounds__(MCS_BLOCK) void k_mg_bc(double *a, Lvl L, int d, double clo, double chi)
+// t -> (i, j, k) with i fastest, in 32-bit arithmetic (levels hold < 2^31 cells).
+__device__ __forceinline__ void split3(int64_t t64, uint32_t nx, uint32_t ny, int64_t &i, int64_t &j,
+                                       int64_t &k)
 {
-    int64_t ext[3] = {L.n[0] + 2, L.n[1] + 2, L.n[2] + 2};
-    int64_t str[3] = {1, ext[0], ext[0] * ext[1]};
-    int d1 = (d + 1) % 3, d2 = (d + 2) % 3;
-    int64_t t = gidx();
-    if (t >= ext[d1] * ext[d2]) return;
-    int64_t b1 = t % ext[d1], b2 = t / ext[d1];
-    int64_t base = b1 * str[d1] + b2 * str[d2];
-    a[base] = clo * a[base + str[d]];
-    a[base + (L.n[d] + 1) * str[d]] = chi * a[base + L.n[d] * str[d]];
+    uint32_t t = (uint32_t)t64;
+    uint32_t q = t / nx;
+    i = t - q * nx;
+    uint32_t r = q / ny;
+    j = q - r * ny;
+    k = r;
 }
 
-// One colour of gauss_seidel! (Multigrid.jl:168-178 via redblack3): cells with
-// (i+j+k) % 2 == par in interior coordinates; each row holds nx/2 of them.
-__global__ __launch_bounds__(MCS_BLOCK) void k_mg_rbgs(double *u, const double *b, Lvl L, int par)
+// L u at interior cell (i, j, k) with folded ghosts, in lplstencil order
+// (+x, +y, +z, -x, -y, -z; Multigrid.jl:124-130,152-158).
+__device__ __forceinline__ double lap_fold(const double *__restrict__ u, const Lvl &L,
+                                          const BcCoef &bc, int64_t i, int64_t j, int64_t k,
+                                          int64_t c, double uc)
 {
-    int64_t hx = L.n[0] / 2;
-    int64_t t = gidx();
-    if (t >= hx * L.n[1] * L.n[2]) return;
-    int64_t ti = t % hx, j = (t / hx) % L.n[1], k = t / (hx * L.n[1]);
-    int64_t i = 2 * ti + ((j + k + par) & 1);
     const int64_t sy = L.n[0] + 2, sz = (int64_t)(L.n[0] + 2) * (L.n[1] + 2);
-    int64_t c = (i + 1) + sy * (j + 1) + sz * (k + 1);
-    double l = -6.0 * u[c];
-    l += u[c + 1];
-    l += u[c + sy];
-    l += u[c + sz];
-    l += u[c - 1];
-    l += u[c - sy];
-    l += u[c - sz];
-    u[c] += (1.0 * (l + b[c])) / 6.0;
+    double l = -6.0 * uc;
+    l += (i + 1 < L.n[0]) ? u[c + 1] : bc.hi[0] * uc;
+    l += (j + 1 < L.n[1]) ? u[c + sy] : bc.hi[1] * uc;
+    l += (k + 1 < L.n[2]) ? u[c + sz] : bc.hi[2] * uc;
+    l += (i > 0) ? u[c - 1] : bc.lo[0] * uc;
+    l += (j > 0) ? u[c - sy] : bc.lo[1] * uc;
+    l += (k > 0) ? u[c - sz] : bc.lo[2] * uc;
+    return l;
 }
 
-// residual! (Multigrid.jl:186-193): r = s b + L u; optional per-block sum of r^2.
+// ---- per-item bodies shared by the grid kernels and the tail kernel ----------
+
+// One cell of one colour of gauss_seidel! (Multigrid.jl:168-178, redblack.jl:49-61):
+// colour par holds cells with (i+j+k) % 2 == par; each row has nx/2 of them.
+__device__ __forceinline__ void gs_item(double *u, const double *b, const Lvl &L, const BcCoef &bc,
+                                        int par, int64_t t)
+{
+    int64_t ti, j, k;
+    split3(t, (uint32_t)(L.n[0] / 2), (uint32_t)L.n[1], ti, j, k);
+    int64_t i = 2 * ti + ((j + k + par) & 1);
+    int64_t c = g_index(L, i + 1, j + 1, k + 1);
+    double uc = u[c];
+    double l = lap_fold(u, L, bc, i, j, k, c, uc);
+    u[c] = uc + (1.0 * (l + b[c])) / 6.0;
+}
+
+// Ghost shell item: x faces (full y-z extent), then y faces (interior x, full z),
+// then z faces (interior x, y).
+__device__ __forceinline__ void bc_item(double *a, const Lvl &L, const BcCoef &bc, int64_t t)
+{
+    const int64_t nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    int64_t i, j, k;
+    int64_t nA = 2 * (ny + 2) * (nz + 2), nB = 2 * nx * (nz + 2);
+    int64_t ua, ub, side;
+    if (t < nA) {
+        split3(t, (uint32_t)(ny + 2), (uint32_t)(nz + 2), ua, ub, side);
+        i = side ? nx + 1 : 0;
+        j = ua;
+        k = ub;
+    } else if (t < nA + nB) {
+        split3(t - nA, (uint32_t)nx, (uint32_t)(nz + 2), ua, ub, side);
+        j = side ? ny + 1 : 0;
+        i = 1 + ua;
+        k = ub;
+    } else {
+        split3(t - nA - nB, (uint32_t)nx, (uint32_t)ny, ua, ub, side);
+        k = side ? nz + 1 : 0;
+        i = 1 + ua;
+        j = 1 + ub;
+    }
+    double f = 1.0;
+    int64_t ci = i, cj = j, ck = k;
+    if (i == 0) { f = f * bc.lo[0]; ci = 1; } else if (i == nx + 1) { f = f * bc.hi[0]; ci = nx; }
+    if (j == 0) { f = f * bc.lo[1]; cj = 1; } else if (j == ny + 1) { f = f * bc.hi[1]; cj = ny; }
+    if (k == 0) { f = f * bc.lo[2]; ck = 1; } else if (k == nz + 1) { f = f * bc.hi[2]; ck = nz; }
+    a[g_index(L, i, j, k)] = f * a[g_index(L, ci, cj, ck)];
+}
+
+__device__ __forceinline__ int64_t bc_count(const Lvl &L)
+{
+    const int64_t nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    return 2 * (ny + 2) * (nz + 2) + 2 * nx * (nz + 2) + 2 * nx * ny;
+}
+
+// residual! (Multigrid.jl:186-193) at the 8 children of coarse cell t, summed in
+// cubestencil order (x offset slowest) and restricted as 4*sum/8 (restrict!,
+// Multigrid.jl:213-226).  The fine residual itself is not stored.
+__device__ __forceinline__ void resid_restrict_item(double *rh, const double *u, const double *b,
+                                                    double s, const Lvl &F, const BcCoef &bc,
+                                                    int64_t t)
+{
+    int64_t ic, jc, kc;
+    split3(t, (uint32_t)(F.n[0] / 2), (uint32_t)(F.n[1] / 2), ic, jc, kc);
+    double sum = 0.0;
+#pragma unroll
+    for (int dx = 0; dx < 2; ++dx)
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+            for (int dz = 0; dz < 2; ++dz) {
+                int64_t i = 2 * ic + dx, j = 2 * jc + dy, k = 2 * kc + dz;
+                int64_t c = g_index(F, i + 1, j + 1, k + 1);
+                double uc = u[c];
+                double l = lap_fold(u, F, bc, i, j, k, c, uc);
+                sum += s * b[c] + l;
+            }
+    Lvl C;
+    C.n[0] = F.n[0] / 2;
+    C.n[1] = F.n[1] / 2;
+    C.n[2] = F.n[2] / 2;
+    rh[g_index(C, ic + 1, jc + 1, kc + 1)] = (4.0 * sum) / 8.0;
+}
+
+// restrict! of a stored fine array.
+__device__ __forceinline__ void restrict_item(double *rh, const double *r, const Lvl &F, int64_t t)
+{
+    const int64_t nc0 = F.n[0] / 2, nc1 = F.n[1] / 2;
+    int64_t i, j, k;
+    split3(t, (uint32_t)nc0, (uint32_t)nc1, i, j, k);
+    const int64_t fy = F.n[0] + 2, fz = (int64_t)(F.n[0] + 2) * (F.n[1] + 2);
+    const int64_t cy = nc0 + 2, cz = (int64_t)(nc0 + 2) * (nc1 + 2);
+    int64_t f = (2 * i + 1) + fy * (2 * j + 1) + fz * (2 * k + 1);
+    double s = 0.0;
+    s += r[f];
+    s += r[f + fz];
+    s += r[f + fy];
+    s += r[f + fy + fz];
+    s += r[f + 1];
+    s += r[f + 1 + fz];
+    s += r[f + 1 + fy];
+    s += r[f + 1 + fy + fz];
+    rh[(i + 1) + cy * (j + 1) + cz * (k + 1)] = (4.0 * s) / 8.0;
+}
+
+// interpolate! (Multigrid.jl:232-258): weights prod(3 - 2 sh_i)/64 over the coarse
+// neighbours towards each fine child, sh in cubestencil order; add or overwrite.
+__device__ __forceinline__ void interp_item(double *r, const double *rh, const Lvl &F, int add,
+                                            int64_t t)
+{
+    const int64_t nc0 = F.n[0] / 2, nc1 = F.n[1] / 2;
+    int64_t i, j, k;
+    split3(t, (uint32_t)nc0, (uint32_t)nc1, i, j, k);
+    const int64_t fy = F.n[0] + 2, fz = (int64_t)(F.n[0] + 2) * (F.n[1] + 2);
+    const int64_t cy = nc0 + 2, cz = (int64_t)(nc0 + 2) * (nc1 + 2);
+    int64_t cc = (i + 1) + cy * (j + 1) + cz * (k + 1);
+    const double w0 = 27.0 / 64.0, w1 = 9.0 / 64.0, w2 = 3.0 / 64.0, w3 = 1.0 / 64.0;
+#pragma unroll
+    for (int sx = 0; sx < 2; ++sx)
+#pragma unroll
+        for (int sy = 0; sy < 2; ++sy)
+#pragma unroll
+            for (int sz = 0; sz < 2; ++sz) {
+                int64_t fidx = (2 * i + 1 + sx) + fy * (2 * j + 1 + sy) + fz * (2 * k + 1 + sz);
+                int64_t ox = 2 * sx - 1, oy = (2 * sy - 1) * cy, oz = (2 * sz - 1) * cz;
+                double v = 0.0;
+                v += w0 * rh[cc];
+                v += w1 * rh[cc + oz];
+                v += w1 * rh[cc + oy];
+                v += w2 * rh[cc + oy + oz];
+                v += w1 * rh[cc + ox];
+                v += w2 * rh[cc + ox + oz];
+                v += w2 * rh[cc + ox + oy];
+                v += w3 * rh[cc + ox + oy + oz];
+                if (add)
+                    r[fidx] += v;
+                else
+                    r[fidx] = v;
+            }
+}
+
+// ---- grid kernels ------------------------------------------------------------
+
+__global__ __launch_bounds__(MCS_BLOCK) void k_mg_bc(double *a, Lvl L, BcCoef bc)
+{
+    int64_t t = gidx();
+    if (t < bc_count(L)) bc_item(a, L, bc, t);
+}
+
+__global__ __launch_bounds__(MCS_BLOCK) void k_mg_rbgs(double *u, const double *b, Lvl L, BcCoef bc,
+                                                       int par)
+{
+    int64_t t = gidx();
+    if (t < (int64_t)(L.n[0] / 2) * L.n[1] * L.n[2]) gs_item(u, b, L, bc, par, t);
+}
+
+__global__ __launch_bounds__(MCS_BLOCK) void k_mg_resid_restrict(double *rh, const double *u,
+                                                                 const double *b, Lvl F, BcCoef bc)
+{
+    int64_t t = gidx();
+    if (t < (int64_t)(F.n[0] / 2) * (F.n[1] / 2) * (F.n[2] / 2))
+        resid_restrict_item(rh, u, b, 1.0, F, bc, t);
+}
+
+// residual! stored on the interior, with per-block sums of r^2 (fmg!'s norm).
 __global__ __launch_bounds__(MCS_BLOCK) void k_mg_residual(double *r, const double *u, const double *b,
-                                                           double s, Lvl L, double *partial)
+                                                           double s, Lvl L, BcCoef bc, double *partial)
 {
     int64_t t = gidx();
     int64_t N = (int64_t)L.n[0] * L.n[1] * L.n[2];
     double sq = 0.0;
     if (t < N) {
-        int64_t i = t % L.n[0], j = (t / L.n[0]) % L.n[1], k = t / ((int64_t)L.n[0] * L.n[1]);
-        const int64_t sy = L.n[0] + 2, sz = (int64_t)(L.n[0] + 2) * (L.n[1] + 2);
-        int64_t c = (i + 1) + sy * (j + 1) + sz * (k + 1);
-        double l = -6.0 * u[c];
-        l += u[c + 1];
-        l += u[c + sy];
-        l += u[c + sz];
-        l += u[c - 1];
-        l += u[c - sy];
-        l += u[c - sz];
-        double rv = s * b[c] + l;
+        int64_t i, j, k;
+        split3(t, (uint32_t)L.n[0], (uint32_t)L.n[1], i, j, k);
+        int64_t c = g_index(L, i + 1, j + 1, k + 1);
+        double uc = u[c];
+        double rv = s * b[c] + lap_fold(u, L, bc, i, j, k, c, uc);
         r[c] = rv;
         sq = rv * rv;
     }
-    if (!partial) return;
     __shared__ double sh[MCS_BLOCK];
     sh[threadIdx.x] = sq;
     __syncthreads();
@@ -92,7 +261,8 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_mg_sumsq(const double *b, Lvl L, 
     int64_t N = (int64_t)L.n[0] * L.n[1] * L.n[2];
     double sq = 0.0;
     if (t < N) {
-        int64_t i = t % L.n[0], j = (t / L.n[0]) % L.n[1], k = t / ((int64_t)L.n[0] * L.n[1]);
+        int64_t i, j, k;
+        split3(t, (uint32_t)L.n[0], (uint32_t)L.n[1], i, j, k);
         double v = b[g_index(L, i + 1, j + 1, k + 1)];
         sq = v * v;
     }
@@ -120,119 +290,37 @@ __global__ void k_sum_partials(const double *partial, int64_t nb, double *out)
     if (threadIdx.x == 0) *out = sh[0];
 }
 
-// restrict! (Multigrid.jl:213-226): 4*(sum of 8 children)/8, x offset slowest.
 __global__ __launch_bounds__(MCS_BLOCK) void k_mg_restrict(double *rh, const double *r, Lvl F)
 {
-    int32_t nc0 = F.n[0] / 2, nc1 = F.n[1] / 2, nc2 = F.n[2] / 2;
     int64_t t = gidx();
-    if (t >= (int64_t)nc0 * nc1 * nc2) return;
-    int64_t i = t % nc0, j = (t / nc0) % nc1, k = t / ((int64_t)nc0 * nc1);
-    const int64_t fy = F.n[0] + 2, fz = (int64_t)(F.n[0] + 2) * (F.n[1] + 2);
-    const int64_t cy = nc0 + 2, cz = (int64_t)(nc0 + 2) * (nc1 + 2);
-    int64_t f = (2 * i + 1) + fy * (2 * j + 1) + fz * (2 * k + 1);
-    double s = 0.0;
-    s += r[f];
-    s += r[f + fz];
-    s += r[f + fy];
-    s += r[f + fy + fz];
-    s += r[f + 1];
-    s += r[f + 1 + fz];
-    s += r[f + 1 + fy];
-    s += r[f + 1 + fy + fz];
-    rh[(i + 1) + cy * (j + 1) + cz * (k + 1)] = (4.0 * s) / 8.0;
+    if (t < (int64_t)(F.n[0] / 2) * (F.n[1] / 2) * (F.n[2] / 2)) restrict_item(rh, r, F, t);
 }
 
-// interpolate! (Multigrid.jl:232-258): trilinear weights 27/64 ... 1/64 over the
-// coarse neighbours in the direction of each fine child; add or overwrite.
 __global__ __launch_bounds__(MCS_BLOCK) void k_mg_interp(double *r, const double *rh, Lvl F, int add)
 {
-    int32_t nc0 = F.n[0] / 2, nc1 = F.n[1] / 2, nc2 = F.n[2] / 2;
     int64_t t = gidx();
-    if (t >= (int64_t)nc0 * nc1 * nc2) return;
-    int64_t i = t % nc0, j = (t / nc0) % nc1, k = t / ((int64_t)nc0 * nc1);
-    const int64_t fy = F.n[0] + 2, fz = (int64_t)(F.n[0] + 2) * (F.n[1] + 2);
-    const int64_t cy = nc0 + 2, cz = (int64_t)(nc0 + 2) * (nc1 + 2);
-    int64_t cc = (i + 1) + cy * (j + 1) + cz * (k + 1);
-    const double wt[8] = {27.0 / 64.0, 9.0 / 64.0, 9.0 / 64.0, 3.0 / 64.0,
-                          9.0 / 64.0,  3.0 / 64.0, 3.0 / 64.0, 1.0 / 64.0};
-#pragma unroll
-    for (int sx = 0; sx < 2; ++sx)
-#pragma unroll
-        for (int sy = 0; sy < 2; ++sy)
-#pragma unroll
-            for (int sz = 0; sz < 2; ++sz) {
-                int64_t fidx = (2 * i + 1 + sx) + fy * (2 * j + 1 + sy) + fz * (2 * k + 1 + sz);
-                int64_t ox = 2 * sx - 1, oy = (2 * sy - 1) * cy, oz = (2 * sz - 1) * cz;
-                double v = 0.0;
-                v += wt[0] * rh[cc];
-                v += wt[1] * rh[cc + oz];
-                v += wt[2] * rh[cc + oy];
-                v += wt[3] * rh[cc + oy + oz];
-                v += wt[4] * rh[cc + ox];
-                v += wt[5] * rh[cc + ox + oz];
-                v += wt[6] * rh[cc + ox + oy];
-                v += wt[7] * rh[cc + ox + oy + oz];
-                if (add)
-                    r[fidx] += v;
-                else
-                    r[fidx] = v;
-            }
+    if (t < (int64_t)(F.n[0] / 2) * (F.n[1] / 2) * (F.n[2] / 2)) interp_item(r, rh, F, add, t);
 }
 
 // Coarse direct solve of fmg! (Multigrid.jl:404-410): z = A^-1 btop, one row per
-// thread, columns summed in order.
+// thread, columns summed in order.  btop is staged in LDS (at most 4096 cells).
 __global__ __launch_bounds__(MCS_BLOCK) void k_mg_coarse(const double *ainv, const double *res,
                                                          double *z, Lvl C)
 {
-    int64_t N = (int64_t)C.n[0] * C.n[1] * C.n[2];
+    __shared__ double bt[4096];
+    const int64_t N = (int64_t)C.n[0] * C.n[1] * C.n[2];
+    for (int64_t col = threadIdx.x; col < N; col += blockDim.x) {
+        int64_t i, j, k;
+        split3(col, (uint32_t)C.n[0], (uint32_t)C.n[1], i, j, k);
+        bt[col] = res[g_index(C, i + 1, j + 1, k + 1)];
+    }
+    __syncthreads();
     for (int64_t row = threadIdx.x; row < N; row += blockDim.x) {
         double acc = 0.0;
-        for (int64_t col = 0; col < N; ++col) {
-            int64_t i = col % C.n[0], j = (col / C.n[0]) % C.n[1], k = col / ((int64_t)C.n[0] * C.n[1]);
-            acc += ainv[row * N + col] * res[g_index(C, i + 1, j + 1, k + 1)];
-        }
-        int64_t i = row % C.n[0], j = (row / C.n[0]) % C.n[1], k = row / ((int64_t)C.n[0] * C.n[1]);
+        for (int64_t col = 0; col < N; ++col) acc += ainv[row * N + col] * bt[col];
+        int64_t i, j, k;
+        split3(row, (uint32_t)C.n[0], (uint32_t)C.n[1], i, j, k);
         z[g_index(C, i + 1, j + 1, k + 1)] = acc;
-    }
-}
-
-// The 50 (apply!, gauss_seidel!) rounds at the coarsest level of mgv!
-// (Multigrid.jl:340-346) in one workgroup, phases separated by barriers.
-__global__ __launch_bounds__(1024) void k_mg_coarsest(double *x, const double *b, Lvl L,
-                                                      const int32_t *bc_coef, int rounds)
-{
-    int64_t ext[3] = {L.n[0] + 2, L.n[1] + 2, L.n[2] + 2};
-    int64_t str[3] = {1, ext[0], ext[0] * ext[1]};
-    const int64_t N = (int64_t)L.n[0] * L.n[1] * L.n[2];
-    for (int it = 0; it < rounds; ++it) {
-        for (int d = 0; d < 3; ++d) {
-            int d1 = (d + 1) % 3, d2 = (d + 2) % 3;
-            double clo = bc_coef[2 * d] == MCS_BC_DIRICHLET ? -1.0 : 1.0;
-            double chi = bc_coef[2 * d + 1] == MCS_BC_DIRICHLET ? -1.0 : 1.0;
-            for (int64_t t = threadIdx.x; t < ext[d1] * ext[d2]; t += blockDim.x) {
-                int64_t b1 = t % ext[d1], b2 = t / ext[d1];
-                int64_t base = b1 * str[d1] + b2 * str[d2];
-                x[base] = clo * x[base + str[d]];
-                x[base + (L.n[d] + 1) * str[d]] = chi * x[base + L.n[d] * str[d]];
-            }
-            __syncthreads();
-        }
-        for (int par = 0; par < 2; ++par) {
-            for (int64_t t = threadIdx.x; t < N; t += blockDim.x) {
-                int64_t i = t % L.n[0], j = (t / L.n[0]) % L.n[1], k = t / ((int64_t)L.n[0] * L.n[1]);
-                if (((i + j + k) & 1) != par) continue;
-                int64_t c = (i + 1) + str[1] * (j + 1) + str[2] * (k + 1);
-                double l = -6.0 * x[c];
-                l += x[c + 1];
-                l += x[c + str[1]];
-                l += x[c + str[2]];
-                l += x[c - 1];
-                l += x[c - str[1]];
-                l += x[c - str[2]];
-                x[c] += (1.0 * (l + b[c])) / 6.0;
-            }
-            __syncthreads();
-        }
     }
 }
 
@@ -242,10 +330,15 @@ __global__ void k_mg_fill0(double *a, int64_t n)
     if (i < n) a[i] = 0.0;
 }
 
-__global__ void k_mg_axpy(double *x, const double *z, int64_t n)
+// x += z on the interior (the ghosts are rewritten by the apply! that follows).
+__global__ __launch_bounds__(MCS_BLOCK) void k_mg_axpy(double *x, const double *z, Lvl L)
 {
-    int64_t i = gidx();
-    if (i < n) x[i] += z[i];
+    int64_t t = gidx();
+    if (t >= (int64_t)L.n[0] * L.n[1] * L.n[2]) return;
+    int64_t i, j, k;
+        split3(t, (uint32_t)L.n[0], (uint32_t)L.n[1], i, j, k);
+    int64_t c = g_index(L, i + 1, j + 1, k + 1);
+    x[c] += z[c];
 }
 
 // setinhom! (boundaries.jl:202-214): b += (2/s) val on the first valid layer.
@@ -263,6 +356,147 @@ __global__ void k_mg_inhom(double *b, Lvl L, int face, double c, const double *v
     idx[d2] = a2 + 1;
     int64_t cidx = g_index(L, idx[0], idx[1], idx[2]);
     b[cidx] = b[cidx] + c * val[t];
+}
+
+// ---- single-workgroup tail: mgv! from a small level down to the coarsest -----
+
+struct Tail {
+    Lvl n[MG_MAXL];
+    int64_t len[MG_MAXL];
+    int64_t off_x[MG_MAXL];   // LDS offsets (doubles) of the level solutions
+    int64_t off_b[MG_MAXL];   // and of their right-hand sides
+    int L;
+    int smooth1, smooth2;
+    BcCoef bc;
+};
+
+#define BLOOP(t, N) for (int64_t t = threadIdx.x; t < (N); t += blockDim.x)
+
+// The 50 (apply!, gauss_seidel!) rounds of the coarsest level (Multigrid.jl:340-346)
+// with one cell per lane of a single wave, neighbours exchanged by shuffles; the
+// arithmetic is that of gs_item.  The ghosts are materialised from the values at
+// the start of the last round, as the last apply! leaves them.
+__device__ void coarsest_in_registers(double *x, const double *b, const Lvl &Lv, const BcCoef &bc)
+{
+    const int lane = threadIdx.x;
+    const int nx = Lv.n[0], ny = Lv.n[1], nz = Lv.n[2];
+    const int N = nx * ny * nz;
+    const bool own = lane < N;
+    int i = 0, j = 0, k = 0;
+    if (own) {
+        i = lane % nx;
+        j = (lane / nx) % ny;
+        k = lane / (nx * ny);
+    }
+    const int64_t c = g_index(Lv, i + 1, j + 1, k + 1);
+    double u = own ? x[c] : 0.0;
+    const double bb = own ? b[c] : 0.0;
+    const int par_me = (i + j + k) & 1;
+    const int sxp = (i + 1 < nx) ? lane + 1 : lane, sxm = (i > 0) ? lane - 1 : lane;
+    const int syp = (j + 1 < ny) ? lane + nx : lane, sym = (j > 0) ? lane - nx : lane;
+    const int szp = (k + 1 < nz) ? lane + nx * ny : lane, szm = (k > 0) ? lane - nx * ny : lane;
+    for (int it = 0; it < 50; ++it) {
+        if (it == 49) {
+            if (own) x[c] = u;
+            __syncthreads();
+            BLOOP(t, bc_count(Lv)) bc_item(x, Lv, bc, t);
+            __syncthreads();
+        }
+        for (int par = 0; par < 2; ++par) {
+            double vxp = __shfl(u, sxp, MCS_WAVE), vyp = __shfl(u, syp, MCS_WAVE);
+            double vzp = __shfl(u, szp, MCS_WAVE), vxm = __shfl(u, sxm, MCS_WAVE);
+            double vym = __shfl(u, sym, MCS_WAVE), vzm = __shfl(u, szm, MCS_WAVE);
+            if (own && par_me == par) {
+                double l = -6.0 * u;
+                l += (i + 1 < nx) ? vxp : bc.hi[0] * u;
+                l += (j + 1 < ny) ? vyp : bc.hi[1] * u;
+                l += (k + 1 < nz) ? vzp : bc.hi[2] * u;
+                l += (i > 0) ? vxm : bc.lo[0] * u;
+                l += (j > 0) ? vym : bc.lo[1] * u;
+                l += (k > 0) ? vzm : bc.lo[2] * u;
+                u = u + (1.0 * (l + bb)) / 6.0;
+            }
+        }
+    }
+    if (own) x[c] = u;
+    __syncthreads();
+}
+
+// mgv! (Multigrid.jl:334-377) for levels start..L inside one workgroup with every
+// level held in LDS: x0/b0 (the start level, ghosts included) are loaded, the
+// levels below live only in LDS, and x0 is written back.  Phases are separated by
+// barriers and follow the grid path's order exactly.
+__global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, double *x0,
+                                                             const double *b0)
+{
+    extern __shared__ double lds[];
+    const BcCoef bc = T.bc;
+    BLOOP(t, T.len[start]) {
+        lds[T.off_x[start] + t] = x0[t];
+        lds[T.off_b[start] + t] = b0[t];
+    }
+    __syncthreads();
+    for (int lev = start; lev < T.L; ++lev) {
+        double *x = lds + T.off_x[lev];
+        const double *b = lds + T.off_b[lev];
+        const Lvl Lv = T.n[lev];
+        const int64_t ncol = (int64_t)(Lv.n[0] / 2) * Lv.n[1] * Lv.n[2];
+        for (int it = 0; it < T.smooth1; ++it)
+            for (int par = 0; par < 2; ++par) {
+                BLOOP(t, ncol) gs_item(x, b, Lv, bc, par, t);
+                __syncthreads();
+            }
+        if (T.smooth2 == 0) {
+            BLOOP(t, bc_count(Lv)) bc_item(x, Lv, bc, t);
+            __syncthreads();
+        }
+        const int64_t nc = (int64_t)(Lv.n[0] / 2) * (Lv.n[1] / 2) * (Lv.n[2] / 2);
+        BLOOP(t, nc) resid_restrict_item(lds + T.off_b[lev + 1], x, b, 1.0, Lv, bc, t);
+        BLOOP(t, T.len[lev + 1]) lds[T.off_x[lev + 1] + t] = 0.0;
+        __syncthreads();
+    }
+    {
+        const int lev = T.L;
+        double *x = lds + T.off_x[lev];
+        const double *b = lds + T.off_b[lev];
+        const Lvl Lv = T.n[lev];
+        const int64_t N = (int64_t)Lv.n[0] * Lv.n[1] * Lv.n[2];
+        if (N <= MG_TAIL_THREADS) {
+            coarsest_in_registers(x, b, Lv, bc);
+        } else {
+            const int64_t ncol = (int64_t)(Lv.n[0] / 2) * Lv.n[1] * Lv.n[2];
+            for (int it = 0; it < 50; ++it) {
+                if (it == 49) {
+                    BLOOP(t, bc_count(Lv)) bc_item(x, Lv, bc, t);
+                    __syncthreads();
+                }
+                for (int par = 0; par < 2; ++par) {
+                    BLOOP(t, ncol) gs_item(x, b, Lv, bc, par, t);
+                    __syncthreads();
+                }
+            }
+        }
+    }
+    for (int lev = T.L - 1; lev >= start; --lev) {
+        double *x = lds + T.off_x[lev];
+        const double *b = lds + T.off_b[lev];
+        const Lvl Lv = T.n[lev];
+        const int64_t nc = (int64_t)(Lv.n[0] / 2) * (Lv.n[1] / 2) * (Lv.n[2] / 2);
+        BLOOP(t, nc) interp_item(x, lds + T.off_x[lev + 1], Lv, 1, t);
+        __syncthreads();
+        const int64_t ncol = (int64_t)(Lv.n[0] / 2) * Lv.n[1] * Lv.n[2];
+        for (int it = 0; it < T.smooth2; ++it) {
+            if (it == T.smooth2 - 1) {
+                BLOOP(t, bc_count(Lv)) bc_item(x, Lv, bc, t);
+                __syncthreads();
+            }
+            for (int par = 0; par < 2; ++par) {
+                BLOOP(t, ncol) gs_item(x, b, Lv, bc, par, t);
+                __syncthreads();
+            }
+        }
+    }
+    BLOOP(t, T.len[start]) x0[t] = lds[T.off_x[start] + t];
 }
 
 // ---------------------------------------------------------------- host side
@@ -335,9 +569,18 @@ static int coarse_inverse(const int32_t n[3], const int32_t bc[6], std::vector<d
 
 extern "C" int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg **out)
 {
+    int lds_max = 0;
+    if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) !=
+        hipSuccess)
+        lds_max = 64 * 1024;
+    size_t tail_lds = (size_t)lds_max < (size_t)MG_TAIL_LDS ? (size_t)lds_max : (size_t)MG_TAIL_LDS;
+    if (tail_lds > 64 * 1024 &&
+        hipFuncSetAttribute((const void *)k_mg_tail, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)tail_lds) != hipSuccess)
+        tail_lds = 64 * 1024;
     if (!ctx || !cfg || !out) return MCS_E_ARG;
     const mcs_mg_config &c = *cfg;
-    if (c.levels < 0 || c.levels > 12 || c.smooth1 < 0 || c.smooth2 < 0 || c.maxiter < 0)
+    if (c.levels < 0 || c.levels >= MG_MAXL || c.smooth1 < 0 || c.smooth2 < 0 || c.maxiter < 0)
         return mcs_fail(ctx, MCS_E_ARG, "mg: bad levels/smoothing/maxiter");
     for (int d = 0; d < 3; ++d) {
         if (c.n[d] < 1 || c.n[d] % (1 << c.levels))
@@ -354,18 +597,20 @@ extern "C" int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg 
     mcs_mg *mg = new mcs_mg();
     mg->ctx = ctx;
     mg->cfg = c;
+    mg->tail_lds = tail_lds;
+    const char *ge = getenv("MCS_MG_GRAPH");
+    mg->use_graph = !(ge && ge[0] == '0');
     for (int l = 0; l <= c.levels; ++l) {
         MgLevel L;
         for (int d = 0; d < 3; ++d) L.n[d] = c.n[d] >> l;
         L.len = glen(L.n);
         L.fres = nullptr;
         L.fsol = nullptr;
+        L.res1 = nullptr;
         if (hipMalloc(&L.res, sizeof(double) * L.len) != hipSuccess ||
-            hipMalloc(&L.res1, sizeof(double) * L.len) != hipSuccess ||
             hipMalloc(&L.sol, sizeof(double) * L.len) != hipSuccess)
             return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation failed");
         hipMemset(L.res, 0, sizeof(double) * L.len);
-        hipMemset(L.res1, 0, sizeof(double) * L.len);
         hipMemset(L.sol, 0, sizeof(double) * L.len);
         mg->lv.push_back(L);
     }
@@ -395,7 +640,6 @@ extern "C" int32_t mcs_mg_destroy(mcs_mg *mg)
     hipStreamSynchronize(mg->ctx->stream);
     for (auto &L : mg->lv) {
         hipFree(L.res);
-        hipFree(L.res1);
         hipFree(L.sol);
     }
     hipFree(mg->ainv);
@@ -403,6 +647,7 @@ extern "C" int32_t mcs_mg_destroy(mcs_mg *mg)
     if (mg->u_tmp) hipFree(mg->u_tmp);
     if (mg->b_tmp) hipFree(mg->b_tmp);
     if (mg->bc_dev) hipFree(mg->bc_dev);
+    if (mg->graph_exec) hipGraphExecDestroy(mg->graph_exec);
     delete mg;
     return MCS_OK;
 }
@@ -414,18 +659,34 @@ static inline Lvl lvl(const mcs_mg *mg, int l)
     return L;
 }
 
+static inline BcCoef coefs(const mcs_mg *mg)
+{
+    BcCoef c;
+    for (int d = 0; d < 3; ++d) {
+        c.lo[d] = mg->cfg.bc[2 * d] == MCS_BC_DIRICHLET ? -1.0 : 1.0;
+        c.hi[d] = mg->cfg.bc[2 * d + 1] == MCS_BC_DIRICHLET ? -1.0 : 1.0;
+    }
+    return c;
+}
+
+static inline int64_t ncells(const mcs_mg *mg, int l)
+{
+    return (int64_t)mg->lv[l].n[0] * mg->lv[l].n[1] * mg->lv[l].n[2];
+}
+
+static inline int64_t ncoarse_cells(const mcs_mg *mg, int lf)
+{
+    return (int64_t)(mg->lv[lf].n[0] / 2) * (mg->lv[lf].n[1] / 2) * (mg->lv[lf].n[2] / 2);
+}
+
+static inline dim3 grid_for(int64_t n) { return dim3((unsigned)cdiv(n > 0 ? n : 1, MCS_BLOCK)); }
+
 static void mg_bc(mcs_mg *mg, int l, double *a)
 {
     Lvl L = lvl(mg, l);
-    hipStream_t st = mg->ctx->stream;
-    for (int d = 0; d < 3; ++d) {
-        int d1 = (d + 1) % 3, d2 = (d + 2) % 3;
-        int64_t cnt = (int64_t)(L.n[d1] + 2) * (L.n[d2] + 2);
-        double clo = mg->cfg.bc[2 * d] == MCS_BC_DIRICHLET ? -1.0 : 1.0;
-        double chi = mg->cfg.bc[2 * d + 1] == MCS_BC_DIRICHLET ? -1.0 : 1.0;
-        hipLaunchKernelGGL(k_mg_bc, dim3((unsigned)cdiv(cnt, MCS_BLOCK)), dim3(MCS_BLOCK), 0, st, a, L,
-                           d, clo, chi);
-    }
+    int64_t nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    int64_t cnt = 2 * (ny + 2) * (nz + 2) + 2 * nx * (nz + 2) + 2 * nx * ny;
+    hipLaunchKernelGGL(k_mg_bc, grid_for(cnt), dim3(MCS_BLOCK), 0, mg->ctx->stream, a, L, coefs(mg));
 }
 
 static void mg_gs(mcs_mg *mg, int l, double *u, const double *b)
@@ -434,42 +695,20 @@ static void mg_gs(mcs_mg *mg, int l, double *u, const double *b)
     int64_t cnt = (int64_t)(L.n[0] / 2) * L.n[1] * L.n[2];
     ProfScope ps(mg->ctx, "mg_rbgs");
     for (int par = 0; par < 2; ++par)
-        hipLaunchKernelGGL(k_mg_rbgs, dim3((unsigned)cdiv(cnt, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
-                           mg->ctx->stream, u, b, L, par);
-}
-
-static void mg_residual(mcs_mg *mg, int l, double *r, const double *u, const double *b, double s,
-                        double *partial)
-{
-    Lvl L = lvl(mg, l);
-    int64_t N = (int64_t)L.n[0] * L.n[1] * L.n[2];
-    ProfScope ps(mg->ctx, "mg_residual");
-    hipLaunchKernelGGL(k_mg_residual, dim3((unsigned)cdiv(N, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
-                       mg->ctx->stream, r, u, b, s, L, partial);
-}
-
-static void mg_restrict(mcs_mg *mg, int lf, double *rh, const double *r)
-{
-    Lvl F = lvl(mg, lf);
-    int64_t Nc = (int64_t)(F.n[0] / 2) * (F.n[1] / 2) * (F.n[2] / 2);
-    ProfScope ps(mg->ctx, "mg_restrict");
-    hipLaunchKernelGGL(k_mg_restrict, dim3((unsigned)cdiv(Nc, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
-                       mg->ctx->stream, rh, r, F);
-}
-
-static void mg_interp(mcs_mg *mg, int lf, double *r, const double *rh, int add)
-{
-    Lvl F = lvl(mg, lf);
-    int64_t Nc = (int64_t)(F.n[0] / 2) * (F.n[1] / 2) * (F.n[2] / 2);
-    ProfScope ps(mg->ctx, "mg_prolong");
-    hipLaunchKernelGGL(k_mg_interp, dim3((unsigned)cdiv(Nc, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
-                       mg->ctx->stream, r, rh, F, add);
+        hipLaunchKernelGGL(k_mg_rbgs, grid_for(cnt), dim3(MCS_BLOCK), 0, mg->ctx->stream, u, b, L,
+                           coefs(mg), par);
 }
 
 static void mg_fill0(mcs_mg *mg, double *a, int64_t n)
 {
-    hipLaunchKernelGGL(k_mg_fill0, dim3((unsigned)cdiv(n, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
-                       mg->ctx->stream, a, n);
+    hipLaunchKernelGGL(k_mg_fill0, grid_for(n), dim3(MCS_BLOCK), 0, mg->ctx->stream, a, n);
+}
+
+static void mg_interp(mcs_mg *mg, int lf, double *r, const double *rh, int add)
+{
+    ProfScope ps(mg->ctx, "mg_prolong");
+    hipLaunchKernelGGL(k_mg_interp, grid_for(ncoarse_cells(mg, lf)), dim3(MCS_BLOCK), 0,
+                       mg->ctx->stream, r, rh, lvl(mg, lf), add);
 }
 
 static int32_t ensure_bc_dev(mcs_mg *mg)
@@ -480,42 +719,68 @@ static int32_t ensure_bc_dev(mcs_mg *mg)
     return MCS_OK;
 }
 
-// mgv! (Multigrid.jl:334-377)
+// LDS bytes for the levels level..L (solution and right-hand side of each).
+static size_t tail_bytes(const mcs_mg *mg, int level)
+{
+    size_t tot = 0;
+    for (int l = level; l <= mg->cfg.levels; ++l) tot += 2 * (size_t)mg->lv[l].len * sizeof(double);
+    return tot;
+}
+
+static bool in_tail(const mcs_mg *mg, int level) { return tail_bytes(mg, level) <= mg->tail_lds; }
+
+static void mg_tail(mcs_mg *mg, int level, double *x, const double *b)
+{
+    Tail T;
+    memset(&T, 0, sizeof(T));
+    T.L = mg->cfg.levels;
+    T.smooth1 = mg->cfg.smooth1;
+    T.smooth2 = mg->cfg.smooth2;
+    T.bc = coefs(mg);
+    int64_t off = 0;
+    for (int l = 0; l <= T.L; ++l) {
+        T.n[l] = lvl(mg, l);
+        T.len[l] = mg->lv[l].len;
+        if (l >= level) {
+            T.off_x[l] = off;
+            off += T.len[l];
+            T.off_b[l] = off;
+            off += T.len[l];
+        }
+    }
+    ProfScope ps(mg->ctx, "mg_tail");
+    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(MG_TAIL_THREADS), (size_t)off * sizeof(double),
+                       mg->ctx->stream, T, level, x, b);
+}
+
+// mgv! (Multigrid.jl:334-377).  apply! before each smoothing sweep and before the
+// residual is folded into the kernels; the ghosts are materialised before the
+// last post-smoothing sweep (they are what the caller's prolongation reads).
 static void mgv(mcs_mg *mg, double *x, const double *b, int level)
 {
-    const int L = mg->cfg.levels;
-    if (level == L) {
-        Lvl C = lvl(mg, level);
-        if (mg->lv[level].len <= 65536) {
-            hipLaunchKernelGGL(k_mg_coarsest, dim3(1), dim3(1024), 0, mg->ctx->stream, x, b, C,
-                               (const int32_t *)mg->bc_dev, 50);
-        } else {
-            for (int it = 0; it < 50; ++it) {
-                mg_bc(mg, level, x);
-                mg_gs(mg, level, x, b);
-            }
-        }
+    if (in_tail(mg, level)) {
+        mg_tail(mg, level, x, b);
         return;
     }
-    for (int it = 0; it < mg->cfg.smooth1; ++it) {
-        mg_bc(mg, level, x);
-        mg_gs(mg, level, x, b);
+    for (int it = 0; it < mg->cfg.smooth1; ++it) mg_gs(mg, level, x, b);
+    if (mg->cfg.smooth2 == 0) mg_bc(mg, level, x);
+    {
+        ProfScope ps(mg->ctx, "mg_residual");
+        hipLaunchKernelGGL(k_mg_resid_restrict, grid_for(ncoarse_cells(mg, level)), dim3(MCS_BLOCK), 0,
+                           mg->ctx->stream, mg->lv[level + 1].res, (const double *)x, b, lvl(mg, level),
+                           coefs(mg));
     }
-    double *r = mg->lv[level].res1;
-    mg_bc(mg, level, x);
-    mg_residual(mg, level, r, x, b, 1.0, nullptr);
-    double *rh = mg->lv[level + 1].res;
-    mg_bc(mg, level, r);
-    mg_restrict(mg, level, rh, r);
     double *xh = mg->lv[level + 1].sol;
     mg_fill0(mg, xh, mg->lv[level + 1].len);
-    mgv(mg, xh, rh, level + 1);
+    mgv(mg, xh, mg->lv[level + 1].res, level + 1);
     mg_interp(mg, level, x, xh, 1);
     for (int it = 0; it < mg->cfg.smooth2; ++it) {
-        mg_bc(mg, level, x);
+        if (it == mg->cfg.smooth2 - 1) mg_bc(mg, level, x);
         mg_gs(mg, level, x, b);
     }
 }
+
+static void fmg_body(mcs_mg *mg, double *x, const double *b);
 
 // fmg! (Multigrid.jl:380-434).  Returns 1 when a correction was applied, 0 when
 // the residual already meets the tolerance, negative on error.
@@ -525,8 +790,11 @@ static int fmg(mcs_mg *mg, double *x, const double *b, double *eps_out)
     const int L = mg->cfg.levels;
     const MgLevel &F = mg->lv[0];
     mg_bc(mg, 0, x);
-    mg_fill0(mg, F.res, F.len);
-    mg_residual(mg, 0, F.res, x, b, mg->cfg.s, mg->partial);
+    {
+        ProfScope ps(ctx, "mg_residual");
+        hipLaunchKernelGGL(k_mg_residual, grid_for(ncells(mg, 0)), dim3(MCS_BLOCK), 0, ctx->stream,
+                           F.res, (const double *)x, b, mg->cfg.s, lvl(mg, 0), coefs(mg), mg->partial);
+    }
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(MCS_BLOCK), 0, ctx->stream,
                        (const double *)mg->partial, mg->npartial, mg->partial + mg->npartial);
     double ss = 0.0;
@@ -535,31 +803,64 @@ static int fmg(mcs_mg *mg, double *x, const double *b, double *eps_out)
         hipStreamSynchronize(ctx->stream) != hipSuccess)
         return -1;
     memcpy(&ss, ctx->h_small, sizeof(double));
-    int64_t N = (int64_t)F.n[0] * F.n[1] * F.n[2];
     double eps = sqrt(ss) / sqrt((double)F.n[0] * (double)F.n[1] * (double)F.n[2]);
-    (void)N;
     *eps_out = eps;
     if (mg->cfg.s * mg->tol_abs > eps) return 0;
-    for (int k = 0; k < L; ++k) mg_restrict(mg, k, mg->lv[k + 1].res, mg->lv[k].res);
-    {
-        Lvl C = lvl(mg, L);
-        hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(MCS_BLOCK), 0, ctx->stream,
-                           (const double *)mg->ainv, (const double *)mg->lv[L].res, mg->lv[L].sol, C);
+    if (mg->use_graph) {
+        if (!mg->graph_exec || mg->graph_x != x || mg->graph_b != b) {
+            if (mg->graph_exec) {
+                hipGraphExecDestroy(mg->graph_exec);
+                mg->graph_exec = nullptr;
+            }
+            bool prof = ctx->prof;
+            ctx->prof = false;
+            hipGraph_t g = nullptr;
+            if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed) != hipSuccess) return -1;
+            fmg_body(mg, x, b);
+            hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+            ctx->prof = prof;
+            if (e != hipSuccess || hipGraphInstantiate(&mg->graph_exec, g, nullptr, nullptr, 0) != hipSuccess)
+                return -1;
+            hipGraphDestroy(g);
+            mg->graph_x = x;
+            mg->graph_b = b;
+        }
+        ProfScope ps(ctx, "mg_cycle");
+        if (hipGraphLaunch(mg->graph_exec, ctx->stream) != hipSuccess) return -1;
+        return 1;
     }
+    ProfScope ps(ctx, "mg_cycle");
+    fmg_body(mg, x, b);
+    return 1;
+}
+
+// The part of fmg! after the convergence test: restrictions, coarse solve, the
+// coarse-to-fine sweep of V-cycles and the update x += z.  No host interaction,
+// so it is captured once into a HIP graph per (x, b) pair and replayed.
+static void fmg_body(mcs_mg *mg, double *x, const double *b)
+{
+    mcs_ctx *ctx = mg->ctx;
+    const int L = mg->cfg.levels;
+    (void)b;
+    for (int k = 0; k < L; ++k) {
+        ProfScope ps(ctx, "mg_restrict");
+        hipLaunchKernelGGL(k_mg_restrict, grid_for(ncoarse_cells(mg, k)), dim3(MCS_BLOCK), 0, ctx->stream,
+                           mg->lv[k + 1].res, (const double *)mg->lv[k].res, lvl(mg, k));
+    }
+    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(MCS_BLOCK), 0, ctx->stream, (const double *)mg->ainv,
+                       (const double *)mg->lv[L].res, mg->lv[L].sol, lvl(mg, L));
     double *z = mg->lv[L].sol;
     for (int k = 1; k <= L; ++k) {
         int lev = L - k;
         double *z1 = mg->lv[lev].sol;
-        mg_fill0(mg, z1, mg->lv[lev].len);
         mg_bc(mg, lev + 1, z);
         mg_interp(mg, lev, z1, z, 0);
         mgv(mg, z1, mg->lv[lev].res, lev);
         z = z1;
     }
-    hipLaunchKernelGGL(k_mg_axpy, dim3((unsigned)cdiv(F.len, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
-                       ctx->stream, x, (const double *)z, F.len);
+    hipLaunchKernelGGL(k_mg_axpy, grid_for(ncells(mg, 0)), dim3(MCS_BLOCK), 0, ctx->stream, x,
+                       (const double *)z, lvl(mg, 0));
     mg_bc(mg, 0, x);
-    return 1;
 }
 
 // solve (Multigrid.jl:449-467)

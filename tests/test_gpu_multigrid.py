@@ -58,3 +58,30 @@ def test_solve_nonconverged_reports():
     bb = b.reshape(n[2] + 2, n[1] + 2, n[0] + 2).transpose(2, 1, 0)
     it, res = mg.solve(conf, u, bb, ws)
     assert it == 2  # maxiter + 1 attempts, as the reference's while loop (Multigrid.jl:452-455)
+
+
+def test_graph_replay_matches_eager(monkeypatch):
+    """The cycle body is replayed from a HIP graph; every replay (several solves
+    with host work in between) must equal the eagerly launched path bitwise."""
+    from mcstreamer_amd import _lib
+    import ctypes
+    n = (32, 32, 64)
+    g = (n[0] + 2) * (n[1] + 2) * (n[2] + 2)
+    conf = _cfg((N, N, N, N, D, D), 0.9, 4, -1e-9)
+    outs = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("MCS_MG_GRAPH", mode)
+        ws = mg.Workspace(conf, n)
+        res = []
+        for rep in range(3):
+            rng = np.random.default_rng(rep)
+            b = rng.normal(size=g)
+            u = np.zeros(g)
+            it = ctypes.c_int32()
+            r = ctypes.c_double()
+            assert _lib.lib().mcs_mg_solve_host(ws.handle, _lib.dptr(u), _lib.dptr(b), ctypes.byref(it), ctypes.byref(r)) == 0
+            res.append((u.copy(), it.value))
+        outs[mode] = res
+    for (ue, ie), (ug, ig) in zip(outs["0"], outs["1"]):
+        assert ie == ig
+        np.testing.assert_array_equal(ue, ug)
