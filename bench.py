@@ -177,6 +177,19 @@ def run_swarm(args):
             "cpu_baseline": None}
 
 
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """A plain device copy of known size: its FETCH_SIZE/WRITE_SIZE counters give
+    the factor that turns the counters of the step's kernels into bytes."""
+    import torch
+
+    x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(f"copy calibration: {reps} x {nbytes} bytes read and written", file=sys.stderr)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,7 +202,11 @@ def main():
     ap.add_argument("--swarm-n", type=int, default=10_000)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 1 GiB device to device (reference kernel for HBM counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the particle-mesh step is measured at N=1 "
               "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)

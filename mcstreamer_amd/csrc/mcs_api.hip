@@ -553,6 +553,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_moments(PopBuf b, int64_t n, doub
     }
     __shared__ double sh[MCS_BLOCK];
     double *rec = partial + (int64_t)blockIdx.x * MOM_REC;
+#pragma unroll
     for (int q = 0; q < MOM_NV; ++q) {
         // tree over the block for (s, c)
         __shared__ double shc[MCS_BLOCK];
@@ -658,6 +659,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_collrate(PopBuf b, int64_t n, con
         }
     }
     __shared__ double sh[MCS_BLOCK], shc[MCS_BLOCK];
+#pragma unroll
     for (int q = 0; q < CR_CHUNK; ++q) {
         sh[threadIdx.x] = acc[q].s;
         shc[threadIdx.x] = acc[q].c;

@@ -14,7 +14,7 @@ struct TableView {
 
 struct MeshView {
     int32_t n[3];
-    double h;
+    double h, inv_h;
     const double *ex, *ey, *ez;
     double *qfixed;
 };
@@ -40,6 +40,7 @@ static inline MeshView mesh_view(const mcs_mesh *m)
     MeshView v;
     for (int d = 0; d < 3; ++d) v.n[d] = m ? m->n[d] : 0;
     v.h = m ? m->h : 0.0;
+    v.inv_h = m ? 1.0 / m->h : 0.0;
     v.ex = m ? m->arr[MCS_ARR_EX] : nullptr;
     v.ey = m ? m->arr[MCS_ARR_EY] : nullptr;
     v.ez = m ? m->arr[MCS_ARR_EZ] : nullptr;
@@ -91,11 +92,12 @@ __device__ __forceinline__ void indweight_dev(const double *energy, int32_t ne, 
     }
 }
 
-// normdivrem + staggered (MCStreamer/src/gridfields.jl:194-209) for one axis.
-__device__ __forceinline__ void axis_index_dev(double xc, double h, int32_t n, int64_t &m,
+// normdivrem + staggered (MCStreamer/src/gridfields.jl:194-209) for one axis;
+// normdivrem is @fastmath there, x * (1/h) stands for x / h.
+__device__ __forceinline__ void axis_index_dev(double xc, double inv_h, int32_t n, int64_t &m,
                                                double &d, int64_t &m1, double &d1)
 {
-    double r = xc / h;
+    double r = xc * inv_h;
     int64_t mi = (int64_t)trunc(r);
     if (mi > n - 1) mi = n - 1;
     double di = r - (double)mi;
@@ -138,9 +140,9 @@ __device__ __forceinline__ void efield_mesh_dev(const MeshView &M, const double 
     }
     int64_t mx, my, mz, m1x, m1y, m1z;
     double dx, dy, dz, d1x, d1y, d1z;
-    axis_index_dev(x[0], h, M.n[0], mx, dx, m1x, d1x);
-    axis_index_dev(x[1], h, M.n[1], my, dy, m1y, d1y);
-    axis_index_dev(x[2], h, M.n[2], mz, dz, m1z, d1z);
+    axis_index_dev(x[0], M.inv_h, M.n[0], mx, dx, m1x, d1x);
+    axis_index_dev(x[1], M.inv_h, M.n[1], my, dy, m1y, d1y);
+    axis_index_dev(x[2], M.inv_h, M.n[2], mz, dz, m1z, d1z);
     int64_t nx = M.n[0], ny = M.n[1];
     {
         int64_t sx = nx + 1, sxy = (nx + 1) * (ny + 2);

@@ -21,8 +21,8 @@
 #include "mcs_device.h"
 
 #define MG_MAXL 13
-#define MG_TAIL_LDS (128 * 1024)
-#define MG_TAIL_THREADS 64
+#define MG_TAIL_LDS (160 * 1024)
+#define MG_TAIL_THREADS 512
 
 struct Lvl {
     int32_t n[3];
@@ -42,10 +42,21 @@ __device__ __forceinline__ void split3(int64_t t64, uint32_t nx, uint32_t ny, in
                                        int64_t &k)
 {
     uint32_t t = (uint32_t)t64;
-    uint32_t q = t / nx;
-    i = t - q * nx;
-    uint32_t r = q / ny;
-    j = q - r * ny;
+    uint32_t q, r;
+    if ((nx & (nx - 1)) == 0) {
+        q = t >> __builtin_ctz(nx);
+        i = t & (nx - 1);
+    } else {
+        q = t / nx;
+        i = t - q * nx;
+    }
+    if ((ny & (ny - 1)) == 0) {
+        r = q >> __builtin_ctz(ny);
+        j = q & (ny - 1);
+    } else {
+        r = q / ny;
+        j = q - r * ny;
+    }
     k = r;
 }
 
@@ -80,6 +91,23 @@ __device__ __forceinline__ void gs_item(double *u, const double *b, const Lvl &L
     double uc = u[c];
     double l = lap_fold(u, L, bc, i, j, k, c, uc);
     u[c] = uc + (1.0 * (l + b[c])) / 6.0;
+}
+
+// Two cells of one colour at once: both cells' operands are loaded before either
+// is stored (within a colour every neighbour read belongs to the other colour).
+__device__ __forceinline__ void gs_item2(double *u, const double *b, const Lvl &L, const BcCoef &bc,
+                                         int par, int64_t t0, int64_t t1, bool has1)
+{
+    int64_t ti0, j0, k0, ti1, j1, k1;
+    split3(t0, (uint32_t)(L.n[0] / 2), (uint32_t)L.n[1], ti0, j0, k0);
+    split3(has1 ? t1 : t0, (uint32_t)(L.n[0] / 2), (uint32_t)L.n[1], ti1, j1, k1);
+    int64_t i0 = 2 * ti0 + ((j0 + k0 + par) & 1), i1 = 2 * ti1 + ((j1 + k1 + par) & 1);
+    int64_t c0 = g_index(L, i0 + 1, j0 + 1, k0 + 1), c1 = g_index(L, i1 + 1, j1 + 1, k1 + 1);
+    double u0 = u[c0], u1 = u[c1], b0 = b[c0], b1 = b[c1];
+    double l0 = lap_fold(u, L, bc, i0, j0, k0, c0, u0);
+    double l1 = lap_fold(u, L, bc, i1, j1, k1, c1, u1);
+    u[c0] = u0 + (1.0 * (l0 + b0)) / 6.0;
+    if (has1) u[c1] = u1 + (1.0 * (l1 + b1)) / 6.0;
 }
 
 // Ghost shell item: x faces (full y-z extent), then y faces (interior x, full z),
@@ -373,9 +401,11 @@ struct Tail {
 #define BLOOP(t, N) for (int64_t t = threadIdx.x; t < (N); t += blockDim.x)
 
 // The 50 (apply!, gauss_seidel!) rounds of the coarsest level (Multigrid.jl:340-346)
-// with one cell per lane of a single wave, neighbours exchanged by shuffles; the
+// with one cell per lane of the first wave, neighbours exchanged by shuffles; the
 // arithmetic is that of gs_item.  The ghosts are materialised from the values at
-// the start of the last round, as the last apply! leaves them.
+// the start of the last round, as the last apply! leaves them.  Called by wave 0
+// only; LDS accesses of one wave complete in order, the fences keep the compiler
+// from reordering them.
 __device__ void coarsest_in_registers(double *x, const double *b, const Lvl &Lv, const BcCoef &bc)
 {
     const int lane = threadIdx.x;
@@ -398,9 +428,11 @@ __device__ void coarsest_in_registers(double *x, const double *b, const Lvl &Lv,
     for (int it = 0; it < 50; ++it) {
         if (it == 49) {
             if (own) x[c] = u;
-            __syncthreads();
-            BLOOP(t, bc_count(Lv)) bc_item(x, Lv, bc, t);
-            __syncthreads();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            for (int64_t t = lane; t < bc_count(Lv); t += MCS_WAVE) bc_item(x, Lv, bc, t);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
         }
         for (int par = 0; par < 2; ++par) {
             double vxp = __shfl(u, sxp, MCS_WAVE), vyp = __shfl(u, syp, MCS_WAVE);
@@ -419,31 +451,28 @@ __device__ void coarsest_in_registers(double *x, const double *b, const Lvl &Lv,
         }
     }
     if (own) x[c] = u;
-    __syncthreads();
 }
 
-// mgv! (Multigrid.jl:334-377) for levels start..L inside one workgroup with every
-// level held in LDS: x0/b0 (the start level, ghosts included) are loaded, the
-// levels below live only in LDS, and x0 is written back.  Phases are separated by
-// barriers and follow the grid path's order exactly.
+// mgv! (Multigrid.jl:334-377) for levels start..L inside one workgroup: the level
+// solutions and the deeper right-hand sides live in LDS, the start level's
+// right-hand side is read from global memory; x0 is loaded and written back.
+// Phases are separated by barriers and follow the grid path's order exactly.
 __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, double *x0,
                                                              const double *b0)
 {
     extern __shared__ double lds[];
     const BcCoef bc = T.bc;
-    BLOOP(t, T.len[start]) {
-        lds[T.off_x[start] + t] = x0[t];
-        lds[T.off_b[start] + t] = b0[t];
-    }
+    BLOOP(t, T.len[start]) lds[T.off_x[start] + t] = x0[t];
     __syncthreads();
     for (int lev = start; lev < T.L; ++lev) {
         double *x = lds + T.off_x[lev];
-        const double *b = lds + T.off_b[lev];
+        const double *b = lev == start ? b0 : lds + T.off_b[lev];
         const Lvl Lv = T.n[lev];
         const int64_t ncol = (int64_t)(Lv.n[0] / 2) * Lv.n[1] * Lv.n[2];
         for (int it = 0; it < T.smooth1; ++it)
             for (int par = 0; par < 2; ++par) {
-                BLOOP(t, ncol) gs_item(x, b, Lv, bc, par, t);
+                for (int64_t t = threadIdx.x; t < ncol; t += 2 * (int64_t)blockDim.x)
+                    gs_item2(x, b, Lv, bc, par, t, t + blockDim.x, t + blockDim.x < ncol);
                 __syncthreads();
             }
         if (T.smooth2 == 0) {
@@ -458,11 +487,12 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
     {
         const int lev = T.L;
         double *x = lds + T.off_x[lev];
-        const double *b = lds + T.off_b[lev];
+        const double *b = lev == start ? b0 : lds + T.off_b[lev];
         const Lvl Lv = T.n[lev];
         const int64_t N = (int64_t)Lv.n[0] * Lv.n[1] * Lv.n[2];
-        if (N <= MG_TAIL_THREADS) {
-            coarsest_in_registers(x, b, Lv, bc);
+        if (N <= MCS_WAVE) {
+            if (threadIdx.x < MCS_WAVE) coarsest_in_registers(x, b, Lv, bc);
+            __syncthreads();
         } else {
             const int64_t ncol = (int64_t)(Lv.n[0] / 2) * Lv.n[1] * Lv.n[2];
             for (int it = 0; it < 50; ++it) {
@@ -479,7 +509,7 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
     }
     for (int lev = T.L - 1; lev >= start; --lev) {
         double *x = lds + T.off_x[lev];
-        const double *b = lds + T.off_b[lev];
+        const double *b = lev == start ? b0 : lds + T.off_b[lev];
         const Lvl Lv = T.n[lev];
         const int64_t nc = (int64_t)(Lv.n[0] / 2) * (Lv.n[1] / 2) * (Lv.n[2] / 2);
         BLOOP(t, nc) interp_item(x, lds + T.off_x[lev + 1], Lv, 1, t);
@@ -491,7 +521,8 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
                 __syncthreads();
             }
             for (int par = 0; par < 2; ++par) {
-                BLOOP(t, ncol) gs_item(x, b, Lv, bc, par, t);
+                for (int64_t t = threadIdx.x; t < ncol; t += 2 * (int64_t)blockDim.x)
+                    gs_item2(x, b, Lv, bc, par, t, t + blockDim.x, t + blockDim.x < ncol);
                 __syncthreads();
             }
         }
@@ -719,11 +750,12 @@ static int32_t ensure_bc_dev(mcs_mg *mg)
     return MCS_OK;
 }
 
-// LDS bytes for the levels level..L (solution and right-hand side of each).
+// LDS bytes of a tail from `level`: the solutions of levels level..L and the
+// right-hand sides of levels level+1..L.
 static size_t tail_bytes(const mcs_mg *mg, int level)
 {
-    size_t tot = 0;
-    for (int l = level; l <= mg->cfg.levels; ++l) tot += 2 * (size_t)mg->lv[l].len * sizeof(double);
+    size_t tot = (size_t)mg->lv[level].len * sizeof(double);
+    for (int l = level + 1; l <= mg->cfg.levels; ++l) tot += 2 * (size_t)mg->lv[l].len * sizeof(double);
     return tot;
 }
 
@@ -745,7 +777,7 @@ static void mg_tail(mcs_mg *mg, int level, double *x, const double *b)
             T.off_x[l] = off;
             off += T.len[l];
             T.off_b[l] = off;
-            off += T.len[l];
+            if (l > level) off += T.len[l];
         }
     }
     ProfScope ps(mg->ctx, "mg_tail");

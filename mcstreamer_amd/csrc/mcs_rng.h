@@ -79,8 +79,8 @@ typedef struct mcs_stream {
     uint32_t k0, k1;       /* key */
     uint32_t c1, c2, c3;   /* fixed counter words */
     uint32_t blk;          /* next block index (counter word 0) */
-    uint32_t buf[4];
-    int pos;               /* next unused word in buf; 4 = empty */
+    uint32_t w0, w1, w2, w3; /* current block; named words keep them in registers */
+    int pos;               /* next unused word of the block; 4 = empty */
 } mcs_stream;
 
 MCS_FN void mcs_stream_init(mcs_stream *st, uint64_t seed, uint32_t instance,
@@ -93,22 +93,27 @@ MCS_FN void mcs_stream_init(mcs_stream *st, uint64_t seed, uint32_t instance,
     st->c3 = ((uint32_t)(id >> 32) & 0x00FFFFFFu) | (phase << 24);
     st->blk = 0;
     st->pos = 4;
-    st->buf[0] = st->buf[1] = st->buf[2] = st->buf[3] = 0;
+    st->w0 = st->w1 = st->w2 = st->w3 = 0;
 }
 
 MCS_FN void mcs_stream_refill(mcs_stream *st)
 {
-    mcs_philox4x32_10(st->blk, st->c1, st->c2, st->c3, st->k0, st->k1, st->buf);
+    uint32_t o[4];
+    mcs_philox4x32_10(st->blk, st->c1, st->c2, st->c3, st->k0, st->k1, o);
+    st->w0 = o[0];
+    st->w1 = o[1];
+    st->w2 = o[2];
+    st->w3 = o[3];
     st->blk += 1;
     st->pos = 0;
 }
 
-/* Uniform double on [0, 1) with 53 random bits (two words). */
+/* Uniform double on [0, 1) with 53 random bits (two words: (w0, w1) then (w2, w3)). */
 MCS_FN double mcs_u01(mcs_stream *st)
 {
     if (st->pos > 2) mcs_stream_refill(st);
-    uint32_t a = st->buf[st->pos];
-    uint32_t b = st->buf[st->pos + 1];
+    uint32_t a = st->pos == 0 ? st->w0 : st->w2;
+    uint32_t b = st->pos == 0 ? st->w1 : st->w3;
     st->pos += 2;
     return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
 }
@@ -117,7 +122,7 @@ MCS_FN double mcs_u01(mcs_stream *st)
 MCS_FN float mcs_u01f(mcs_stream *st)
 {
     if (st->pos > 3) mcs_stream_refill(st);
-    uint32_t a = st->buf[st->pos];
+    uint32_t a = st->pos == 0 ? st->w0 : (st->pos == 1 ? st->w1 : (st->pos == 2 ? st->w2 : st->w3));
     st->pos += 1;
     return (float)(a >> 8) * (1.0f / 16777216.0f);
 }

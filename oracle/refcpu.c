@@ -131,11 +131,13 @@ static inline int64_t cidx(const int32_t n[3], int64_t i, int64_t j, int64_t k)
 }
 
 /* normdivrem + staggered (MCStreamer/src/gridfields.jl:194-209) for one axis:
- * face index m with offset d, and ghost-inclusive centre index m1 with offset d1. */
-static inline void axis_index(double xc, double h, int32_t n, int64_t *m, double *d,
+ * face index m with offset d, and ghost-inclusive centre index m1 with offset d1.
+ * normdivrem is @fastmath in the reference, which licenses x * (1/h) for x / h;
+ * this restatement fixes that choice. */
+static inline void axis_index(double xc, double inv_h, int32_t n, int64_t *m, double *d,
                               int64_t *m1, double *d1)
 {
-    double r = xc / h;
+    double r = xc * inv_h;
     int64_t mi = (int64_t)trunc(r);
     if (mi > n - 1) mi = n - 1;
     double di = r - (double)mi;
@@ -185,9 +187,10 @@ void ref_efield(const ref_field *f, const double x[3], double e[3], int32_t *fla
     }
     int64_t mx, my, mz, m1x, m1y, m1z;
     double dx, dy, dz, d1x, d1y, d1z;
-    axis_index(x[0], h, n[0], &mx, &dx, &m1x, &d1x);
-    axis_index(x[1], h, n[1], &my, &dy, &m1y, &d1y);
-    axis_index(x[2], h, n[2], &mz, &dz, &m1z, &d1z);
+    double inv_h = 1.0 / h;
+    axis_index(x[0], inv_h, n[0], &mx, &dx, &m1x, &d1x);
+    axis_index(x[1], inv_h, n[1], &my, &dy, &m1y, &d1y);
+    axis_index(x[2], inv_h, n[2], &mz, &dz, &m1z, &d1z);
     int64_t nx = n[0], ny = n[1];
     /* EX: (nx+1)(ny+2)(nz+2) */
     {
