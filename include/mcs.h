@@ -235,6 +235,63 @@ int32_t mcs_mg_solve_mesh(mcs_mg *mg, mcs_mesh *m, int32_t *iters, double *resid
  * `face` (0..5); val has the face's interior size, first index fastest. */
 int32_t mcs_mg_setinhom_mesh(mcs_mg *mg, mcs_mesh *m, int32_t face, const double *val);
 
+/* ---- batched swarms: many Swarm.swarm instances (Swarm/src/Swarm.jl:102-142) in
+ * one set of launches.  Each instance ("slot") owns a slab of a shared particle
+ * store, its own collision table, uniform field, time step and resample target,
+ * and a Philox key (seed, instance) so that its result does not depend on the
+ * other slots.  One step of a slot = advance! + the tallies of Swarm.jl:126-130
+ * (fused into the step kernel) + Swarm.resample! (Swarm.jl:193-223), and every
+ * `compact_every` steps a stable compaction of the slab (the reference never
+ * repacks a swarm; compaction only reorders, it changes no particle). */
+#define MCS_TALLY_NQ 4   /* v_z, energy, ionization rate, attachment rate */
+
+typedef struct mcs_swarm_batch mcs_swarm_batch;
+
+typedef struct mcs_swarm_params {
+    double e[3];          /* uniform field (V/m) */
+    double dt;
+    int64_t n0;           /* resample target */
+    double tol;           /* resample tolerance (0.1 in Swarm.jl:193) */
+    int64_t capacity;     /* slab size */
+    uint32_t instance;    /* second Philox key word: the request id */
+    int32_t relax_steps;  /* steps before the tallies start */
+    int32_t block_steps;  /* steps per block mean (statistical error) */
+    int32_t pad;
+} mcs_swarm_params;
+
+typedef struct mcs_swarm_tally {
+    int64_t steps;        /* steps done */
+    int64_t n;            /* slab slots in use (active or not) */
+    int64_t nactive;      /* actives after the last step's advance */
+    int64_t nrecorded;    /* steps entering the time averages */
+    int64_t nblocks;      /* complete blocks */
+    int32_t flags;        /* bit 0 table range, bit 1 field, bit 2 capacity */
+    int32_t pad;
+    double sum[MCS_TALLY_NQ];    /* sum over recorded steps of the per-step means */
+    double bm_s1[MCS_TALLY_NQ];  /* sum of block means */
+    double bm_s2[MCS_TALLY_NQ];  /* sum of squared block means */
+    double last[MCS_TALLY_NQ];   /* per-step means of the last step */
+} mcs_swarm_tally;
+
+int32_t mcs_swarm_batch_create(mcs_ctx *ctx, int64_t total_capacity, int32_t max_slots,
+                               uint64_t seed, mcs_swarm_batch **out);
+int32_t mcs_swarm_batch_destroy(mcs_swarm_batch *b);
+/* Admit an instance: copies its n initial particles (ids 0..n-1) into a free slab
+ * (first fit).  MCS_E_CAPACITY when no slab of params->capacity is free. */
+int32_t mcs_swarm_batch_add(mcs_swarm_batch *b, const mcs_table *t, const mcs_swarm_params *params,
+                            int64_t n, const double *x, const double *v, const double *w,
+                            const double *s, const uint8_t *active, int32_t *slot);
+int32_t mcs_swarm_batch_remove(mcs_swarm_batch *b, int32_t slot);
+/* nsteps steps of every slot; the step sequence is replayed from a HIP graph. */
+int32_t mcs_swarm_batch_step(mcs_swarm_batch *b, int32_t nsteps);
+int32_t mcs_swarm_batch_tally(mcs_swarm_batch *b, int32_t slot, mcs_swarm_tally *out);
+int32_t mcs_swarm_batch_tally_all(mcs_swarm_batch *b, int32_t nslots, mcs_swarm_tally *out);
+int32_t mcs_swarm_batch_download(mcs_swarm_batch *b, int32_t slot, int64_t nmax, int64_t *n,
+                                 double *x, double *v, double *w, double *s, uint8_t *active,
+                                 uint64_t *id);
+/* Launch shape and compaction cadence (workgroup size 128 or 256). */
+int32_t mcs_swarm_batch_config(mcs_swarm_batch *b, int32_t block, int32_t compact_every);
+
 #ifdef __cplusplus
 }
 #endif

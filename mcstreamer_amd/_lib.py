@@ -103,6 +103,39 @@ class MGConfigC(ctypes.Structure):
     ]
 
 
+TALLY_NQ = 4   # v_z, energy, ionization rate, attachment rate
+
+
+class SwarmParams(ctypes.Structure):
+    _fields_ = [
+        ("e", ctypes.c_double * 3),
+        ("dt", ctypes.c_double),
+        ("n0", ctypes.c_int64),
+        ("tol", ctypes.c_double),
+        ("capacity", ctypes.c_int64),
+        ("instance", ctypes.c_uint32),
+        ("relax_steps", ctypes.c_int32),
+        ("block_steps", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+    ]
+
+
+class SwarmTally(ctypes.Structure):
+    _fields_ = [
+        ("steps", ctypes.c_int64),
+        ("n", ctypes.c_int64),
+        ("nactive", ctypes.c_int64),
+        ("nrecorded", ctypes.c_int64),
+        ("nblocks", ctypes.c_int64),
+        ("flags", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("sum", ctypes.c_double * TALLY_NQ),
+        ("bm_s1", ctypes.c_double * TALLY_NQ),
+        ("bm_s2", ctypes.c_double * TALLY_NQ),
+        ("last", ctypes.c_double * TALLY_NQ),
+    ]
+
+
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32
 _I64 = ctypes.c_int64
@@ -151,6 +184,15 @@ SIGNATURES = {
     "mcs_mg_solve_host": [_P, _PD, _PD, _PI32, _PD],
     "mcs_mg_solve_mesh": [_P, _P, _PI32, _PD],
     "mcs_mg_setinhom_mesh": [_P, _P, _I32, _PD],
+    "mcs_swarm_batch_create": [_P, _I64, _I32, _U64, _PP],
+    "mcs_swarm_batch_destroy": [_P],
+    "mcs_swarm_batch_add": [_P, _P, ctypes.POINTER(SwarmParams), _I64, _PD, _PD, _PD, _PD, _PU8, _PI32],
+    "mcs_swarm_batch_remove": [_P, _I32],
+    "mcs_swarm_batch_step": [_P, _I32],
+    "mcs_swarm_batch_tally": [_P, _I32, ctypes.POINTER(SwarmTally)],
+    "mcs_swarm_batch_tally_all": [_P, _I32, ctypes.POINTER(SwarmTally)],
+    "mcs_swarm_batch_download": [_P, _I32, _I64, _PI64, _PD, _PD, _PD, _PD, _PU8, _PU64],
+    "mcs_swarm_batch_config": [_P, _I32, _I32],
 }
 
 _lib = None

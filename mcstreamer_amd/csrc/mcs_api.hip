@@ -234,6 +234,22 @@ static int32_t scan_impl(mcs_ctx *ctx, const TI *in, TO *out, int64_t len)
     return MCS_OK;
 }
 
+// Same scan with a caller-owned block-sum buffer (cdiv(len, 2048) + 1 entries),
+// for launch sequences that are captured into graphs.
+int32_t mcs_scan_i64_buf(mcs_ctx *ctx, const int64_t *in, int64_t *out, int64_t len, int64_t *bsum)
+{
+    int64_t nb = cdiv(len, SCAN_TILE);
+    if (nb == 0) nb = 1;
+    hipLaunchKernelGGL((k_scan_reduce<int64_t, int64_t>), dim3((unsigned)nb), dim3(MCS_BLOCK), 0,
+                       ctx->stream, in, len, bsum);
+    hipLaunchKernelGGL((k_scan_top<int64_t>), dim3(1), dim3(MCS_BLOCK), 0, ctx->stream, bsum, nb,
+                       out + len);
+    hipLaunchKernelGGL((k_scan_down<int64_t, int64_t>), dim3((unsigned)nb), dim3(MCS_BLOCK), 0,
+                       ctx->stream, in, len, (const int64_t *)bsum, out);
+    MCS_LAUNCH_CHECK(ctx);
+    return MCS_OK;
+}
+
 int32_t mcs_scan_i32(mcs_ctx *ctx, const int32_t *in, int32_t *out, int64_t len)
 {
     return scan_impl<int32_t, int32_t>(ctx, in, out, len);

@@ -183,6 +183,8 @@ static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 int32_t mcs_scan_i32(mcs_ctx *ctx, const int32_t *in, int32_t *out, int64_t len);
 // exclusive scan of int64 values (len entries) into out (len+1 entries).
 int32_t mcs_scan_i64(mcs_ctx *ctx, const int64_t *in, int64_t *out, int64_t len);
+int32_t mcs_scan_i64_buf(mcs_ctx *ctx, const int64_t *in, int64_t *out, int64_t len, int64_t *bsum);
+#define MCS_SCAN_TILE 2048
 
 int32_t mcs_pop_ensure_alt(mcs_pop *p);
 void mcs_pop_swap(mcs_pop *p);

@@ -2,7 +2,7 @@
 // repack! and the swarm's global resample.
 #include <cstring>
 
-#include "mcs_device.h"
+#include "mcs_advance.h"
 
 // counters block layout in ctx->d_small
 #define C_LOG 0
@@ -51,33 +51,6 @@ struct AdvArgs {
     int64_t *ctr;
 };
 
-__device__ __forceinline__ void efield_at(const AdvArgs &a, const double x[3], double e[3],
-                                          int32_t &flags, int field_mesh)
-{
-    if (field_mesh) {
-        efield_mesh_dev(a.M, x, e, flags);
-    } else {
-        e[0] = a.e[0];
-        e[1] = a.e[1];
-        e[2] = a.e[2];
-    }
-}
-
-// track1 (MCStreamer/src/gridfields.jl:303-313): val/dV into the NGP cell.
-__device__ __forceinline__ void track_dev(const MeshView &M, const double x[3], double val)
-{
-    int64_t c[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        double f = floor(x[d] / M.h);
-        if (!(f >= 0.0 && f < (double)M.n[d])) return;
-        c[d] = (int64_t)f;
-    }
-    double dV = (M.h * M.h) * M.h;
-    int64_t idx = (c[0] + 1) + (int64_t)(M.n[0] + 2) * ((c[1] + 1) + (int64_t)(M.n[1] + 2) * (c[2] + 1));
-    atomicAdd(&M.qfixed[idx], val / dV);
-}
-
 // The fused step: one thread per particle slot that exists when the step starts
 // (JuMC/src/mixed_population.jl:27 fixes the range, so children are not advanced).
 // Per particle, free flights to the next (null) collision until dt is used up
@@ -103,7 +76,6 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_advance(AdvArgs a)
     int32_t flags = 0;
     int flights = 0, deaths = 0;
     if (i < a.n0 && a.b.active[i]) {
-        const double maxrate = a.T.maxrate;
         double x[3] = {a.b.x[0][i], a.b.x[1][i], a.b.x[2][i]};
         double v[3] = {a.b.v[0][i], a.b.v[1][i], a.b.v[2][i]};
         const double w = a.b.w[i];
@@ -111,122 +83,27 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_advance(AdvArgs a)
         uint8_t act = 1;
         mcs_stream rs;
         mcs_stream_init(&rs, a.seed, a.instance, MCS_PHASE_ADVANCE, a.b.id[i], a.step);
-        double trem = a.dt;
-        const double c1 = MCS_YOSHIDA_C1, c2 = MCS_YOSHIDA_C2;
-        while (trem > 0.0) {
-            double tnext = s / maxrate;
-            double tt;
-            bool collides;
-            if (trem > tnext) {
-                tt = tnext;
-                collides = true;
-            } else {
-                tt = trem;
-                collides = false;
-                s = s - tt * maxrate;
-            }
-            // advance_free_yoshida
-            {
-                double e[3];
-                double k1 = c1 * tt, k2 = c2 * tt;
-                double f1 = (MCS_YOSHIDA_D1 * tt) * MCS_QM;
-                double f2 = (MCS_YOSHIDA_D2 * tt) * MCS_QM;
+        advance_one<FIELD_MESH, TRACK>(
+            x, v, w, s, act, rs, a.dt, a.T, a.M, a.e, flags, flights, deaths,
+            [&](const double *bx, const double *bv, double bs, uint8_t bact) {
+                unsigned long long slot = atomicAdd((unsigned long long *)&a.ctr[C_LOG], 1ULL);
+                if ((int64_t)slot < a.L.cap) {
+                    a.L.parent[slot] = (int32_t)i;
+                    a.L.seq[slot] = nb;
 #pragma unroll
-                for (int d = 0; d < 3; ++d) x[d] = x[d] + k1 * v[d];
-                efield_at(a, x, e, flags, FIELD_MESH);
-#pragma unroll
-                for (int d = 0; d < 3; ++d) v[d] = v[d] + f1 * e[d];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) x[d] = x[d] + k2 * v[d];
-                efield_at(a, x, e, flags, FIELD_MESH);
-#pragma unroll
-                for (int d = 0; d < 3; ++d) v[d] = v[d] + f2 * e[d];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) x[d] = x[d] + k2 * v[d];
-                efield_at(a, x, e, flags, FIELD_MESH);
-#pragma unroll
-                for (int d = 0; d < 3; ++d) v[d] = v[d] + f1 * e[d];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) x[d] = x[d] + k1 * v[d];
-            }
-            flights += 1;
-            if (collides) {
-                double eng = energy_dev(v[0], v[1], v[2]);
-                int64_t k;
-                double wg;
-                indweight_dev(a.T.energy, a.T.ne, a.T.egrid, a.T.step, a.T.inv_dlog, eng, k, wg,
-                              flags);
-                double xi = mcs_u01(&rs) * maxrate;
-                const double *r0 = a.T.rate + k * a.T.nproc;
-                const double *r1 = r0 + a.T.nproc;
-                for (int j = 0; j < a.T.nproc; ++j) {
-                    double nu = wg * r0[j] + (1.0 - wg) * r1[j];
-                    if (nu > xi) {
-                        atomicAdd(&s_ev[j], 1);
-                        int kind = a.T.kind[j];
-                        if (kind == MCS_PROC_ELASTIC) {
-                            double mr = a.T.param[j];
-                            double c = mr / (1.0 + mr);
-                            double vcm0 = c * v[0], vcm1 = c * v[1], vcm2 = c * v[2];
-                            double d0 = v[0] - vcm0, d1 = v[1] - vcm1, d2 = v[2] - vcm2;
-                            double nrm = sqrt((d0 * d0 + d1 * d1) + d2 * d2);
-                            double om[3];
-                            mcs_randsphere(&rs, om);
-                            v[0] = om[0] * nrm + vcm0;
-                            v[1] = om[1] * nrm + vcm1;
-                            v[2] = om[2] * nrm + vcm2;
-                        } else if (kind == MCS_PROC_EXCITATION) {
-                            double e1 = eng - a.T.param[j];
-                            if (e1 < 0.0) e1 = 0.0;
-                            double vabs = sqrt((2.0 * e1) / MCS_ELECTRON_MASS);
-                            double om[3];
-                            mcs_randsphere(&rs, om);
-                            v[0] = om[0] * vabs;
-                            v[1] = om[1] * vabs;
-                            v[2] = om[2] * vabs;
-                        } else if (kind == MCS_PROC_IONIZATION) {
-                            double e1 = eng - a.T.param[j];
-                            if (e1 < 0.0) e1 = 0.0;
-                            e1 = e1 / 2.0;
-                            double vabs = sqrt((2.0 * e1) / MCS_ELECTRON_MASS);
-                            double om1[3], om2[3];
-                            mcs_randsphere(&rs, om1);
-                            mcs_randsphere(&rs, om2);
-                            v[0] = om1[0] * vabs;
-                            v[1] = om1[1] * vabs;
-                            v[2] = om1[2] * vabs;
-                            double sc = mcs_nextcoll(&rs);
-                            if (TRACK) track_dev(a.M, x, w);
-                            unsigned long long slot =
-                                atomicAdd((unsigned long long *)&a.ctr[C_LOG], 1ULL);
-                            if ((int64_t)slot < a.L.cap) {
-                                a.L.parent[slot] = (int32_t)i;
-                                a.L.seq[slot] = nb;
-#pragma unroll
-                                for (int d = 0; d < 3; ++d) {
-                                    a.L.x[d][slot] = x[d];
-                                    a.L.v[d][slot] = om2[d] * vabs;
-                                }
-                                a.L.w[slot] = w;
-                                a.L.s[slot] = sc;
-                                a.L.act[slot] = act;
-                            } else {
-                                flags |= MCS_FLAG_CAPACITY;
-                            }
-                            nb += 1;
-                        } else if (kind == MCS_PROC_ATTACHMENT) {
-                            if (TRACK) track_dev(a.M, x, -w);
-                            if (act) deaths += 1;
-                            act = 0;
-                        }
-                        break;
+                    for (int d = 0; d < 3; ++d) {
+                        a.L.x[d][slot] = bx[d];
+                        a.L.v[d][slot] = bv[d];
                     }
-                    xi = xi - nu;
+                    a.L.w[slot] = w;
+                    a.L.s[slot] = bs;
+                    a.L.act[slot] = bact;
+                } else {
+                    flags |= MCS_FLAG_CAPACITY;
                 }
-                s = mcs_nextcoll(&rs);
-            }
-            trem = trem - tt;
-        }
+                nb += 1;
+            },
+            [&](int j) { atomicAdd(&s_ev[j], 1); });
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             a.b.x[d][i] = x[d];

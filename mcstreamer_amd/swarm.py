@@ -13,8 +13,8 @@ import numpy as np
 
 from . import _lib
 from . import constants as co
-from ._lib import check
-from .jumc import FixedField, advance
+from ._lib import check, dptr, u8ptr, u64ptr
+from .jumc import FixedField, ParticleStates, advance, context
 
 
 def resample(popl, n, tol=0.1, step=None):
@@ -83,3 +83,114 @@ def swarm(popl, en_td, dt, nt, n0, n_gas, l=100, tol=0.1):
         "mean_energy": m.sum_energy / m.nactive,
         "steps": i,
     }
+
+
+TALLY_NAMES = ("drift_velocity", "mean_energy", "ionization_rate", "attachment_rate")
+
+
+class SwarmBatch:
+    """Many independent swarm instances (Swarm.swarm, Swarm.jl:102-142) advanced
+    together on one device.
+
+    Each instance ("slot") owns a slab of one particle store, its own collision
+    table, uniform field, time step and resample target n0, and draws from the
+    Philox key (seed, instance), so its trajectory is the one a lone
+    Population(seed=seed, instance=instance) would follow under swarm_step.  A
+    step of every slot is advance! + the per-step tallies (mean v_z, mean energy,
+    mean ionization and attachment rate over the actives) + resample!; every
+    `compact_every` steps each slab is compacted (stable, actives only).
+    """
+
+    def __init__(self, total_capacity, max_slots, seed=0, ctx=None):
+        self.ctx = ctx or context()
+        self.seed = int(seed)
+        h = ctypes.c_void_p()
+        check(self.ctx.handle, _lib.lib().mcs_swarm_batch_create(
+            self.ctx.handle, int(total_capacity), int(max_slots), self.seed, ctypes.byref(h)))
+        self.handle = h
+        self.max_slots = int(max_slots)
+        self._tables = {}
+
+    def configure(self, block=256, compact_every=8):
+        check(self.ctx.handle, _lib.lib().mcs_swarm_batch_config(self.handle, int(block), int(compact_every)))
+
+    def add(self, table, particles, e, dt, n0, capacity, instance, tol=0.1, relax_steps=0,
+            block_steps=100):
+        """Admit one instance; returns its slot.  Raises MCSError (capacity) when no
+        slab of `capacity` particles is free."""
+        prm = _lib.SwarmParams()
+        for d in range(3):
+            prm.e[d] = float(e[d])
+        prm.dt = float(dt)
+        prm.n0 = int(n0)
+        prm.tol = float(tol)
+        prm.capacity = int(capacity)
+        prm.instance = int(instance)
+        prm.relax_steps = int(relax_steps)
+        prm.block_steps = int(block_steps)
+        ps = particles
+        slot = ctypes.c_int32()
+        check(self.ctx.handle, _lib.lib().mcs_swarm_batch_add(
+            self.handle, table.handle, ctypes.byref(prm), len(ps), dptr(ps.x), dptr(ps.v), dptr(ps.w),
+            dptr(ps.s), u8ptr(ps.active), ctypes.byref(slot)))
+        self._tables[slot.value] = table
+        return slot.value
+
+    def remove(self, slot):
+        check(self.ctx.handle, _lib.lib().mcs_swarm_batch_remove(self.handle, int(slot)))
+        self._tables.pop(int(slot), None)
+
+    def step(self, nsteps=1):
+        check(self.ctx.handle, _lib.lib().mcs_swarm_batch_step(self.handle, int(nsteps)))
+
+    def tally(self, slot):
+        t = _lib.SwarmTally()
+        check(self.ctx.handle, _lib.lib().mcs_swarm_batch_tally(self.handle, int(slot), ctypes.byref(t)))
+        return t
+
+    def tallies(self, nslots=None):
+        n = self.max_slots if nslots is None else int(nslots)
+        arr = (_lib.SwarmTally * max(n, 1))()
+        check(self.ctx.handle, _lib.lib().mcs_swarm_batch_tally_all(self.handle, n, arr))
+        return list(arr)[:n]
+
+    def download(self, slot):
+        t = self.tally(slot)
+        n = int(t.n)
+        x = np.zeros((n, 3))
+        v = np.zeros((n, 3))
+        w = np.zeros(n)
+        s = np.zeros(n)
+        act = np.zeros(n, dtype=np.uint8)
+        ids = np.zeros(n, dtype=np.uint64)
+        nn = ctypes.c_int64()
+        check(self.ctx.handle, _lib.lib().mcs_swarm_batch_download(
+            self.handle, int(slot), n, ctypes.byref(nn), dptr(x), dptr(v), dptr(w), dptr(s),
+            u8ptr(act), u64ptr(ids)))
+        return ParticleStates(x, v, w, s, act, ids)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.lib().mcs_swarm_batch_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def tally_estimates(t):
+    """Time-averaged tallies of one slot and their relative statistical errors
+    (standard error of the block means over |mean|; inf with fewer than 2 blocks)."""
+    out = {}
+    nrec = max(int(t.nrecorded), 1)
+    nb = int(t.nblocks)
+    for q, name in enumerate(TALLY_NAMES):
+        mean = t.sum[q] / nrec
+        if nb >= 2:
+            m1 = t.bm_s1[q] / nb
+            var = max(t.bm_s2[q] / nb - m1 * m1, 0.0) * nb / (nb - 1)
+            err = np.sqrt(var / nb) / abs(m1) if m1 != 0 else np.inf
+        else:
+            err = np.inf
+        out[name] = (mean, err)
+    return out
