@@ -177,6 +177,147 @@ def run_swarm(args):
             "cpu_baseline": None}
 
 
+def c3_instances(n_inst, seed):
+    """Batched-swarm instances: E/N log-uniform on 10-1000 Td, O2 fraction uniform
+    on 0-0.3, n0 log-uniform on 1e3-1e6, dt = 1e-12 s * 100 Td / (E/N)."""
+    from mcstreamer_amd.serve import Request
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_inst):
+        en = 10 ** rng.uniform(1.0, 3.0)
+        o2 = rng.uniform(0.0, 0.3)
+        n0 = int(round(10 ** rng.uniform(3.0, 6.0)))
+        out.append(Request(i, 0.0, en, o2, n0, 0.01))
+    return out
+
+
+def run_batch(args):
+    """1024 independent swarm instances in one batch (graph-replayed steps,
+    autotuned launch shape), against one plain FP64 launch sequence per instance."""
+    from mcstreamer_amd import jumc, swarm
+    from mcstreamer_amd.serve import GasTables, Policy, _field, request_dt
+
+    ctx = jumc.context()
+    insts = c3_instances(args.instances, args.seed)
+    pol = Policy()
+    tables = GasTables(ctx=ctx)
+    total = sum(cdiv512(pol.capacity(r.n0)) for r in insts)
+    b = swarm.SwarmBatch(total, len(insts), seed=args.seed)
+    t0 = time.perf_counter()
+    for r in insts:
+        b.add(tables.get(r.o2), r.n0, _field(r.en_td), request_dt(r.en_td), r.n0, pol.capacity(r.n0),
+              instance=r.rid, relax_steps=pol.relax_steps, block_steps=pol.block_steps)
+    ctx.synchronize()
+    t_admit = time.perf_counter() - t0
+    print(f"admitted {len(insts)} instances, capacity {total}, {t_admit:.1f} s", file=sys.stderr, flush=True)
+    tune = b.autotune() if not args.no_autotune else {}
+    print(f"autotune: {tune}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        b.step(1)
+    ctx.synchronize()
+    n_before = sum(t.nactive for t in b.tallies(len(insts)))
+    t0 = time.perf_counter()
+    b.step(args.steps)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    print(f"batched: {args.steps} steps in {el:.2f} s", file=sys.stderr, flush=True)
+    tl = b.tallies(len(insts))
+    n_after = sum(t.nactive for t in tl)
+    flags = sum(1 for t in tl if t.flags)
+    psteps = 0.5 * (n_before + n_after) * args.steps
+    events = sum(r.n0 * tables.get(r.o2).maxrate * request_dt(r.en_td) for r in insts)
+    t_step = el / args.steps
+    solves = len(insts) / (args.solve_steps * t_step)
+    # one plain FP64 launch sequence per instance (advance!, tallies, resample!)
+    plain = None
+    if not args.no_plain:
+        tp = 0.0
+        for k, r in enumerate(insts):
+            if k % 128 == 0:
+                print(f"plain path: instance {k}", file=sys.stderr, flush=True)
+            tab = tables.get(r.o2)
+            pop = jumc.Population(2 * pol.capacity(r.n0), None, tab, seed=args.seed, instance=r.rid)
+            pop.init_thermal(r.n0, pol.init_kT_ev)
+            f = jumc.FixedField(_field(r.en_td)[2])
+            swarm.swarm_step(pop, f, request_dt(r.en_td), r.n0)
+            pop.moments(tab)
+            ctx.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(2):
+                swarm.swarm_step(pop, f, request_dt(r.en_td), r.n0)
+                pop.moments(tab)
+            ctx.synchronize()
+            tp += (time.perf_counter() - t1) / 2
+            del pop
+        plain = {"s_per_step_all_instances": tp, "solves_per_s": len(insts) / (args.solve_steps * tp)}
+    ach = BYTES_PER_PSTEP_F64 * psteps / el / 1e9
+    return {"metric": METRIC, "value": psteps / el, "unit": "particle-steps/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_step * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": f"batched swarms: {len(insts)} instances, E/N log-uniform 10-1000 Td, "
+                                   "O2 0-0.3, n0 log-uniform 1e3-1e6, dt=1e-12 s*100/(E/N)",
+                       "sum_n0": int(sum(r.n0 for r in insts)), "events_per_step": events,
+                       "solve_steps": args.solve_steps, "parallelism": "single GPU"},
+            "swarm_solves_per_s": solves,
+            "plain_fp64_per_instance": plain,
+            "speedup_vs_plain": (solves / plain["solves_per_s"]) if plain else None,
+            "autotune_s_per_step": {f"{k[0]}x{k[1]}": v for k, v in tune.items()},
+            "tuned": list(getattr(b, "tuned", ())), "admit_s": t_admit, "slots_flagged": flags,
+            "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                         "kernel": "whole batched step (128 B per particle-step over the step time)"},
+            "cpu_baseline": None}
+
+
+def cdiv512(n):
+    return (n + 511) // 512 * 512
+
+
+def run_serve(args):
+    """Served swarm requests: open-loop Poisson arrivals, continuous batching with
+    footprint and work-budget admission, retirement at the target error on W."""
+    from mcstreamer_amd import jumc
+    from mcstreamer_amd.serve import (GasTables, PlainServer, Policy, SwarmServer, draw_requests,
+                                      fifo_latencies, percentiles)
+
+    ctx = jumc.context()
+    reqs = draw_requests(args.rate, args.serve_seconds, seed=args.seed)
+    pol = Policy()
+    tables = GasTables(ctx=ctx)
+    tables.warm()
+    srv = SwarmServer(args.serve_capacity, max_slots=4096, seed=args.seed, policy=pol, tables=tables)
+    t0 = time.perf_counter()
+    done, unfinished = srv.run(reqs, wall_limit=args.serve_seconds * args.serve_drain,
+                               log=lambda r: print(f"retired {r.rid} after {r.steps} steps, "
+                                                   f"latency {r.latency:.2f} s", file=sys.stderr,
+                                                   flush=True))
+    el = time.perf_counter() - t0
+    lat = [r.latency for r in done]
+    out = {"metric": METRIC, "value": len(done) / el, "unit": "swarm solves/s", "n_gpus": 1,
+           "steps": srv.steps_run, "warmup": 0, "ms_per_step": el / max(srv.steps_run, 1) * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": f"served swarm requests, Poisson {args.rate:g} req/s for "
+                                  f"{args.serve_seconds:g} s, target error on W 0.5-2%",
+                      "arrived": len(reqs), "completed": len(done), "unfinished": len(unfinished),
+                      "converged": sum(1 for r in done if r.converged),
+                      "policy": pol.__dict__, "parallelism": "single GPU"},
+           "latency_s": percentiles(lat), "particle_steps_per_s": srv.particle_steps / el,
+           "cpu_baseline": None}
+    if args.serve_plain > 0:
+        ps = PlainServer(seed=args.seed, policy=pol, tables=tables)
+        sample = [r for r in reqs[: args.serve_plain]]
+        svc = ps.service_times(sample)
+        st = [s for s, _, _ in svc]
+        out["plain_fp64_per_request"] = {
+            "requests": len(sample), "solves_per_s": len(sample) / sum(st),
+            "latency_s_fifo": percentiles(fifo_latencies(sample, st)),
+            "batched_latency_s_same_requests": percentiles([r.latency for r in done if r.rid < len(sample)])}
+    return out
+
+
 def copy_calibration(nbytes=1 << 30, reps=3):
     """A plain device copy of known size: its FETCH_SIZE/WRITE_SIZE counters give
     the factor that turns the counters of the step's kernels into bytes."""
@@ -195,7 +336,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["streamer", "swarm"], default="streamer")
+    ap.add_argument("--workload", choices=["streamer", "swarm", "batch", "serve"], default="streamer")
+    ap.add_argument("--instances", type=int, default=1024)
+    ap.add_argument("--solve-steps", type=int, default=5000)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--serve-drain", type=float, default=1.5)
+    ap.add_argument("--serve-capacity", type=int, default=200_000_000)
+    ap.add_argument("--serve-plain", type=int, default=0)
     ap.add_argument("--mesh", type=int, nargs=3, default=[128, 128, 256])
     ap.add_argument("--h", type=float, default=5e-6)
     ap.add_argument("--npairs", type=int, default=1_000_000)
@@ -211,7 +361,8 @@ def main():
         print("bench.py runs on one GPU only: the particle-mesh step is measured at N=1 "
               "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
         sys.exit(2)
-    res = run_streamer(args) if args.workload == "streamer" else run_swarm(args)
+    res = {"streamer": run_streamer, "swarm": run_swarm, "batch": run_batch,
+           "serve": run_serve}[args.workload](args)
     print(json.dumps(res))
 
 

@@ -257,6 +257,9 @@ typedef struct mcs_swarm_params {
     int32_t relax_steps;  /* steps before the tallies start */
     int32_t block_steps;  /* steps per block mean (statistical error) */
     int32_t pad;
+    double init_vth;      /* x == NULL in mcs_swarm_batch_add: n electrons at the origin,
+                             Maxwellian velocity components init_vth * N(0,1), w = 1,
+                             drawn on the device from phase INIT of the slot's key */
 } mcs_swarm_params;
 
 typedef struct mcs_swarm_tally {
@@ -277,7 +280,8 @@ int32_t mcs_swarm_batch_create(mcs_ctx *ctx, int64_t total_capacity, int32_t max
                                uint64_t seed, mcs_swarm_batch **out);
 int32_t mcs_swarm_batch_destroy(mcs_swarm_batch *b);
 /* Admit an instance: copies its n initial particles (ids 0..n-1) into a free slab
- * (first fit).  MCS_E_CAPACITY when no slab of params->capacity is free. */
+ * (first fit), or draws them on the device when x is NULL (see init_vth).
+ * MCS_E_CAPACITY when no slab of params->capacity is free. */
 int32_t mcs_swarm_batch_add(mcs_swarm_batch *b, const mcs_table *t, const mcs_swarm_params *params,
                             int64_t n, const double *x, const double *v, const double *w,
                             const double *s, const uint8_t *active, int32_t *slot);
@@ -289,6 +293,10 @@ int32_t mcs_swarm_batch_tally_all(mcs_swarm_batch *b, int32_t nslots, mcs_swarm_
 int32_t mcs_swarm_batch_download(mcs_swarm_batch *b, int32_t slot, int64_t nmax, int64_t *n,
                                  double *x, double *v, double *w, double *s, uint8_t *active,
                                  uint64_t *id);
+/* A single population of n thermal electrons at the origin, drawn exactly as the
+ * x == NULL form of mcs_swarm_batch_add (ids 0..n-1), so the plain per-instance
+ * path and a batch slot with the same (seed, instance) start identically. */
+int32_t mcs_pop_init_thermal(mcs_pop *p, int64_t n, uint64_t seed, uint32_t instance, double vth);
 /* Launch shape and compaction cadence (workgroup size 128 or 256). */
 int32_t mcs_swarm_batch_config(mcs_swarm_batch *b, int32_t block, int32_t compact_every);
 

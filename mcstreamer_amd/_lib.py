@@ -117,6 +117,7 @@ class SwarmParams(ctypes.Structure):
         ("relax_steps", ctypes.c_int32),
         ("block_steps", ctypes.c_int32),
         ("pad", ctypes.c_int32),
+        ("init_vth", ctypes.c_double),
     ]
 
 
@@ -193,6 +194,7 @@ SIGNATURES = {
     "mcs_swarm_batch_tally_all": [_P, _I32, ctypes.POINTER(SwarmTally)],
     "mcs_swarm_batch_download": [_P, _I32, _I64, _PI64, _PD, _PD, _PD, _PD, _PU8, _PU64],
     "mcs_swarm_batch_config": [_P, _I32, _I32],
+    "mcs_pop_init_thermal": [_P, _I64, _U64, _U32, _D],
 }
 
 _lib = None

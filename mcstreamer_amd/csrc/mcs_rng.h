@@ -295,4 +295,26 @@ MCS_FN double mcs_samplemin(mcs_stream *st, double m)
     return 1.0 - mcs_exp(mcs_log(1.0 - u) / m);
 }
 
+/* Initial swarm electron (ElectronState(x0, v0) of Swarm/src/Swarm.jl:104-106 with
+ * a Maxwellian instead of v0 = 0): velocity components vth * N(0,1) by two
+ * Box-Muller pairs (u1,u2) -> (n0,n1), (u3,u4) -> n2, then s = nextcoll()
+ * (electron.jl:17).  Drawn from phase INIT of particle `id`, step 0. */
+MCS_FN void mcs_maxwell_init(uint64_t seed, uint32_t instance, uint64_t id, double vth,
+                             double v[3], double *s)
+{
+    mcs_stream st;
+    mcs_stream_init(&st, seed, instance, MCS_PHASE_INIT, id, 0u);
+    double u1 = mcs_u01(&st), u2 = mcs_u01(&st);
+    double r = sqrt(-2.0 * mcs_log(1.0 - u1));
+    double sn, cs;
+    mcs_sincos2pi(u2, &sn, &cs);
+    v[0] = vth * (r * cs);
+    v[1] = vth * (r * sn);
+    double u3 = mcs_u01(&st), u4 = mcs_u01(&st);
+    r = sqrt(-2.0 * mcs_log(1.0 - u3));
+    mcs_sincos2pi(u4, &sn, &cs);
+    v[2] = vth * (r * cs);
+    *s = mcs_nextcoll(&st);
+}
+
 #endif /* MCS_RNG_H */

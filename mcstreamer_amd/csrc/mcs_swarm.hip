@@ -417,6 +417,26 @@ __global__ __launch_bounds__(BS) void k_sw_copy_back(SwArgs a)
     if (i < a.S.base[slot] + a.S.n[slot]) copy_particle(a.alt, i, a.b, i);
 }
 
+// n electrons at the origin with Maxwellian velocities (mcs_maxwell_init).
+__global__ __launch_bounds__(MCS_BLOCK) void k_sw_init(PopBuf b, int64_t base, int64_t n, uint64_t seed,
+                                                       uint32_t key, double vth)
+{
+    int64_t i = gidx();
+    if (i >= n) return;
+    double v[3], s;
+    mcs_maxwell_init(seed, key, (uint64_t)i, vth, v, &s);
+    const int64_t d = base + i;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        b.x[q][d] = 0.0;
+        b.v[q][d] = v[q];
+    }
+    b.w[d] = 1.0;
+    b.s[d] = s;
+    b.active[d] = 1;
+    b.id[d] = (uint64_t)i;
+}
+
 __global__ void k_sw_zero(int64_t *p, int n)
 {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
@@ -621,7 +641,10 @@ extern "C" int32_t mcs_swarm_batch_add(mcs_swarm_batch *b, const mcs_table *t,
     if (prm->capacity < 1 || n < 0 || n > prm->capacity || prm->n0 < 1 || !(prm->dt >= 0.0) ||
         prm->block_steps < 1 || prm->relax_steps < 0)
         return mcs_fail(ctx, MCS_E_ARG, "swarm batch: bad instance parameters");
-    if (n > 0 && (!x || !v || !w || !s || !active)) return mcs_fail(ctx, MCS_E_ARG, "swarm batch: null array");
+    const bool dev_init = (x == nullptr);
+    if (n > 0 && !dev_init && (!v || !w || !s || !active))
+        return mcs_fail(ctx, MCS_E_ARG, "swarm batch: null array");
+    if (dev_init && !(prm->init_vth >= 0.0)) return mcs_fail(ctx, MCS_E_ARG, "swarm batch: bad init_vth");
     int slot = -1;
     for (int k = 0; k < b->max_slots; ++k)
         if (!b->slots[k].used) {
@@ -647,19 +670,27 @@ extern "C" int32_t mcs_swarm_batch_add(mcs_swarm_batch *b, const mcs_table *t,
     if (base < 0) return mcs_fail(ctx, MCS_E_CAPACITY, "swarm batch: no free slab of %lld particles", (long long)cap);
     MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     // particles
-    std::vector<double> tmp((size_t)n);
-    for (int d = 0; d < 3; ++d) {
+    if (dev_init && n > 0) {
+        hipLaunchKernelGGL(k_sw_init, dim3((unsigned)cdiv(n, MCS_BLOCK)), dim3(MCS_BLOCK), 0, ctx->stream,
+                           b->buf, base, n, b->seed, prm->instance, prm->init_vth);
+        MCS_LAUNCH_CHECK(ctx);
+        MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    std::vector<double> tmp((size_t)(dev_init ? 0 : n));
+    for (int d = 0; d < 3 && !dev_init; ++d) {
         for (int64_t i = 0; i < n; ++i) tmp[i] = x[3 * i + d];
         MCS_HIP(ctx, hipMemcpy(b->buf.x[d] + base, tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
         for (int64_t i = 0; i < n; ++i) tmp[i] = v[3 * i + d];
         MCS_HIP(ctx, hipMemcpy(b->buf.v[d] + base, tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     }
-    MCS_HIP(ctx, hipMemcpy(b->buf.w + base, w, sizeof(double) * n, hipMemcpyHostToDevice));
-    MCS_HIP(ctx, hipMemcpy(b->buf.s + base, s, sizeof(double) * n, hipMemcpyHostToDevice));
-    MCS_HIP(ctx, hipMemcpy(b->buf.active + base, active, n, hipMemcpyHostToDevice));
-    std::vector<uint64_t> ids((size_t)n);
-    for (int64_t i = 0; i < n; ++i) ids[i] = (uint64_t)i;
-    MCS_HIP(ctx, hipMemcpy(b->buf.id + base, ids.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice));
+    if (!dev_init) {
+        MCS_HIP(ctx, hipMemcpy(b->buf.w + base, w, sizeof(double) * n, hipMemcpyHostToDevice));
+        MCS_HIP(ctx, hipMemcpy(b->buf.s + base, s, sizeof(double) * n, hipMemcpyHostToDevice));
+        MCS_HIP(ctx, hipMemcpy(b->buf.active + base, active, n, hipMemcpyHostToDevice));
+        std::vector<uint64_t> ids((size_t)n);
+        for (int64_t i = 0; i < n; ++i) ids[i] = (uint64_t)i;
+        MCS_HIP(ctx, hipMemcpy(b->buf.id + base, ids.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice));
+    }
     // slot state
     SlotDev &S = b->S;
     int32_t rc = 0;
@@ -977,5 +1008,22 @@ extern "C" int32_t mcs_swarm_batch_download(mcs_swarm_batch *b, int32_t slot, in
     if (s) MCS_HIP(ctx, hipMemcpy(s, b->buf.s + base, sizeof(double) * m, hipMemcpyDeviceToHost));
     if (active) MCS_HIP(ctx, hipMemcpy(active, b->buf.active + base, m, hipMemcpyDeviceToHost));
     if (id) MCS_HIP(ctx, hipMemcpy(id, b->buf.id + base, sizeof(uint64_t) * m, hipMemcpyDeviceToHost));
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_pop_init_thermal(mcs_pop *p, int64_t n, uint64_t seed, uint32_t instance, double vth)
+{
+    if (!p) return MCS_E_ARG;
+    mcs_ctx *ctx = p->ctx;
+    hipSetDevice(ctx->device);
+    if (n < 0 || n > p->cap || !(vth >= 0.0))
+        return mcs_fail(ctx, MCS_E_ARG, "pop_init_thermal: n must be in [0, capacity], vth >= 0");
+    if (n > 0) {
+        hipLaunchKernelGGL(k_sw_init, dim3((unsigned)cdiv(n, MCS_BLOCK)), dim3(MCS_BLOCK), 0, ctx->stream,
+                           p->buf[0], (int64_t)0, n, seed, instance, vth);
+        MCS_LAUNCH_CHECK(ctx);
+    }
+    p->n = n;
+    p->next_id = (uint64_t)n;
     return MCS_OK;
 }

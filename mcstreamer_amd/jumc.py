@@ -14,6 +14,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
+from . import constants as co
 from ._lib import check, dptr, i32ptr, u8ptr, u64ptr
 
 
@@ -162,6 +163,13 @@ class Population:
         check(self.ctx.handle, _lib.lib().mcs_pop_upload(
             self.handle, n, dptr(ps.x), dptr(ps.v), dptr(ps.w), dptr(ps.s), u8ptr(ps.active),
             u64ptr(ps.id) if ps.id is not None else None, int(nid)))
+
+    def init_thermal(self, n, kT_ev=1.0):
+        """n electrons at the origin with a Maxwellian of temperature kT_ev, drawn on
+        the device from this population's (seed, instance) key; ids 0..n-1."""
+        vth = float(np.sqrt(kT_ev * co.eV / co.electron_mass))
+        check(self.ctx.handle, _lib.lib().mcs_pop_init_thermal(self.handle, int(n), self.seed,
+                                                                self.instance, vth))
 
     def download(self):
         n = self.nparticles()

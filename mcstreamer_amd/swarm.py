@@ -114,10 +114,35 @@ class SwarmBatch:
     def configure(self, block=256, compact_every=8):
         check(self.ctx.handle, _lib.lib().mcs_swarm_batch_config(self.handle, int(block), int(compact_every)))
 
+    def autotune(self, candidates=((256, 4), (128, 4), (256, 8), (128, 8), (256, 16), (256, 2)),
+                 min_steps=2):
+        """Project-written autotuner of the launch shape: runs real steps under each
+        (workgroup size, compaction cadence) candidate, one compaction per trial,
+        and keeps the fastest per step.  The choice never changes results (a slot's
+        trajectory is the same under every shape and cadence); the trial steps count
+        as ordinary steps.  Returns {candidate: seconds per step}."""
+        import time as _t
+        timings = {}
+        for blk, ce in candidates:
+            self.configure(blk, ce)
+            self.step(1)                      # capture the graphs for this shape
+            self.ctx.synchronize()
+            k = max(min_steps, ce)
+            t0 = _t.perf_counter()
+            self.step(k)
+            self.ctx.synchronize()
+            timings[(blk, ce)] = (_t.perf_counter() - t0) / k
+        best = min(timings, key=timings.get)
+        self.configure(*best)
+        self.tuned = best
+        return timings
+
     def add(self, table, particles, e, dt, n0, capacity, instance, tol=0.1, relax_steps=0,
-            block_steps=100):
-        """Admit one instance; returns its slot.  Raises MCSError (capacity) when no
-        slab of `capacity` particles is free."""
+            block_steps=100, init_kT_ev=1.0):
+        """Admit one instance; returns its slot.  `particles` is a ParticleStates, or
+        an int n for n electrons at the origin with a Maxwellian of temperature
+        init_kT_ev drawn on the device.  Raises MCSError (capacity) when no slab of
+        `capacity` particles is free."""
         prm = _lib.SwarmParams()
         for d in range(3):
             prm.e[d] = float(e[d])
@@ -128,11 +153,17 @@ class SwarmBatch:
         prm.instance = int(instance)
         prm.relax_steps = int(relax_steps)
         prm.block_steps = int(block_steps)
-        ps = particles
+        prm.init_vth = float(np.sqrt(init_kT_ev * co.eV / co.electron_mass))
         slot = ctypes.c_int32()
-        check(self.ctx.handle, _lib.lib().mcs_swarm_batch_add(
-            self.handle, table.handle, ctypes.byref(prm), len(ps), dptr(ps.x), dptr(ps.v), dptr(ps.w),
-            dptr(ps.s), u8ptr(ps.active), ctypes.byref(slot)))
+        if isinstance(particles, (int, np.integer)):
+            check(self.ctx.handle, _lib.lib().mcs_swarm_batch_add(
+                self.handle, table.handle, ctypes.byref(prm), int(particles), None, None, None, None,
+                None, ctypes.byref(slot)))
+        else:
+            ps = particles
+            check(self.ctx.handle, _lib.lib().mcs_swarm_batch_add(
+                self.handle, table.handle, ctypes.byref(prm), len(ps), dptr(ps.x), dptr(ps.v),
+                dptr(ps.w), dptr(ps.s), u8ptr(ps.active), ctypes.byref(slot)))
         self._tables[slot.value] = table
         return slot.value
 

@@ -476,6 +476,27 @@ int32_t ref_swarm_resample(ref_pop *p, int64_t n0, double tol, uint64_t seed,
     return 0;
 }
 
+/* Initial swarm population: n electrons at the origin, Maxwellian velocities,
+ * w = 1, ids 0..n-1 (Swarm/src/Swarm.jl:104-106 with a thermal start), from the
+ * shared mcs_maxwell_init draw sequence. */
+void ref_swarm_init(ref_pop *p, int64_t n, uint64_t seed, uint32_t instance, double vth)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        double v[3], s;
+        mcs_maxwell_init(seed, instance, (uint64_t)i, vth, v, &s);
+        for (int d = 0; d < 3; ++d) {
+            p->x[3 * i + d] = 0.0;
+            p->v[3 * i + d] = v[d];
+        }
+        p->w[i] = 1.0;
+        p->s[i] = s;
+        p->active[i] = 1;
+        p->id[i] = (uint64_t)i;
+    }
+    p->n = n;
+    p->next_id = (uint64_t)n;
+}
+
 /* ------------------------------------------------------------------ mesh */
 
 /* cellindex (MCStreamer/src/grid.jl:84-90), 3D: floor(x/h) per axis; particles

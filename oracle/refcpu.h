@@ -133,6 +133,7 @@ void ref_repack(ref_pop *p);
 /* population reductions and Swarm tallies */
 void ref_moments_compute(const ref_pop *p, const ref_table *t, ref_moments *m);
 /* Swarm.resample! */
+void ref_swarm_init(ref_pop *p, int64_t n, uint64_t seed, uint32_t instance, double vth);
 int32_t ref_swarm_resample(ref_pop *p, int64_t n0, double tol, uint64_t seed,
                            uint32_t instance, uint32_t step, int32_t *did);
 

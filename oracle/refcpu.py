@@ -111,6 +111,9 @@ def lib():
         L.ref_advance.restype = ctypes.c_int32
         L.ref_repack.argtypes = [P(RefPop)]
         L.ref_moments_compute.argtypes = [P(RefPop), P(RefTable), P(RefMoments)]
+        L.ref_swarm_init.argtypes = [P(RefPop), ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint32,
+                                     ctypes.c_double]
+        L.ref_swarm_init.restype = None
         L.ref_swarm_resample.argtypes = [P(RefPop), ctypes.c_int64, ctypes.c_double,
                                          ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _PI32]
         L.ref_swarm_resample.restype = ctypes.c_int32
@@ -301,6 +304,15 @@ def swarm_resample(pop, n0, tol, seed=0, instance=0, step=0):
                                   int(step), ctypes.byref(did))
     pop.sync_from(p)
     return rc, did.value
+
+
+def swarm_init(n, seed, instance, vth, cap=None):
+    """Oracle population of n thermal electrons at the origin (ids 0..n-1)."""
+    P = Pop(np.zeros((0, 3)), np.zeros((0, 3)), [], [], [], cap=max(int(cap or n), int(n), 1))
+    p = P.cstruct()
+    lib().ref_swarm_init(ctypes.byref(p), int(n), int(seed), int(instance), float(vth))
+    P.sync_from(p)
+    return P
 
 
 def sort_by_cell(pop, mesh):

@@ -163,3 +163,24 @@ def test_batch_add_remove_between_steps():
         batch.add(gt, pd, (0, 0, -E), dt, 100, capacity=40_000, instance=14)
     est = tally_estimates(batch.tally(sb))
     assert set(est) == {"drift_velocity", "mean_energy", "ionization_rate", "attachment_rate"}
+
+
+def test_device_thermal_init_matches_oracle():
+    # the x == NULL form of add and mcs_pop_init_thermal draw exactly the oracle's
+    # initial electrons (phase INIT of the instance key)
+    tabs = _tables()
+    gt = jumc.CollisionTable(*tabs[0.0], egrid="log")
+    kT = 1.0
+    vth = np.sqrt(kT * co.eV / co.electron_mass)
+    batch = SwarmBatch(total_capacity=50_000, max_slots=2, seed=SEED)
+    slot = batch.add(gt, 3000, (0, 0, -1e6), 1e-12, 3000, capacity=10_000, instance=41, init_kT_ev=kT)
+    got = batch.download(slot)
+    ref = R.swarm_init(3000, SEED, 41, vth).view()
+    for key in ("x", "v", "w", "s", "active", "id"):
+        np.testing.assert_array_equal(getattr(got, key), ref[key], err_msg=key)
+    pop = jumc.Population(5000, None, gt, seed=SEED, instance=41)
+    pop.init_thermal(3000, kT)
+    p = pop.download()
+    np.testing.assert_array_equal(p.v, ref["v"])
+    np.testing.assert_array_equal(p.s, ref["s"])
+    assert p.next_id == 3000

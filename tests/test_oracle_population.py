@@ -102,3 +102,31 @@ def test_swarm_resample_drop_and_clone():
     P = _pop(10500, 1.0, 4)
     rc, did = R.swarm_resample(P, 10000, 0.1)
     assert did == 0 and P.n == 10500
+
+
+def test_swarm_init_draw_order_and_distribution():
+    # the first particles against Box-Muller written with numpy's own log/sincos on
+    # the same uniforms (phase INIT = 3, step 0): pins the draw order and formula
+    from scipy import stats
+    vth = np.sqrt(2.0 * co.eV / co.electron_mass)
+    seed, inst = 77, 5
+    P = R.swarm_init(50_000, seed, inst, vth)
+    v = P.view()["v"]
+    s = P.view()["s"]
+    for pid in (0, 1, 4999):
+        u = R.stream_u01(seed, inst, 3, pid, 0, 5)
+        r1 = np.sqrt(-2.0 * np.log1p(-u[0]))
+        r2 = np.sqrt(-2.0 * np.log1p(-u[2]))
+        want = vth * np.array([r1 * np.cos(2 * np.pi * u[1]), r1 * np.sin(2 * np.pi * u[1]),
+                               r2 * np.cos(2 * np.pi * u[3])])
+        np.testing.assert_allclose(v[pid], want, rtol=1e-12, atol=1e-12 * vth)
+        np.testing.assert_allclose(s[pid], -np.log1p(-u[4]), rtol=1e-13)
+    assert P.n == 50_000 and P.next_id == 50_000
+    np.testing.assert_array_equal(P.view()["id"], np.arange(50_000, dtype=np.uint64))
+    assert (P.view()["x"] == 0).all() and (P.view()["w"] == 1).all()
+    # Maxwellian: each component N(0, vth^2), s ~ Exp(1); mean energy 3/2 kT
+    for d in range(3):
+        assert stats.kstest(v[:, d] / vth, "norm").pvalue > 1e-3
+    assert stats.kstest(s, "expon").pvalue > 1e-3
+    e = 0.5 * co.electron_mass * (v ** 2).sum(axis=1) / co.eV
+    assert abs(e.mean() - 3.0) < 5 * np.sqrt(6.0 / 50_000)   # var of (kT/2) chi2_3 = 1.5 (kT)^2
