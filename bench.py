@@ -203,7 +203,7 @@ def run_batch(args):
     pol = Policy()
     tables = GasTables(ctx=ctx)
     total = sum(cdiv512(pol.capacity(r.n0)) for r in insts)
-    b = swarm.SwarmBatch(total, len(insts), seed=args.seed)
+    b = swarm.SwarmBatch(total, len(insts), seed=args.seed, precision=args.precision)
     t0 = time.perf_counter()
     for r in insts:
         b.add(tables.get(r.o2), r.n0, _field(r.en_td), request_dt(r.en_td), r.n0, pol.capacity(r.n0),
@@ -254,7 +254,8 @@ def run_batch(args):
     ach = BYTES_PER_PSTEP_F64 * psteps / el / 1e9
     return {"metric": METRIC, "value": psteps / el, "unit": "particle-steps/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_step * 1e3,
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32 split (f64 tallies)" if args.precision == "f32" else "f64",
             "data": "synthetic",
             "config": {"workload": f"batched swarms: {len(insts)} instances, E/N log-uniform 10-1000 Td, "
                                    "O2 0-0.3, n0 log-uniform 1e3-1e6, dt=1e-12 s*100/(E/N)",
@@ -287,7 +288,8 @@ def run_serve(args):
     pol = Policy()
     tables = GasTables(ctx=ctx)
     tables.warm()
-    srv = SwarmServer(args.serve_capacity, max_slots=4096, seed=args.seed, policy=pol, tables=tables)
+    srv = SwarmServer(args.serve_capacity, max_slots=4096, seed=args.seed, policy=pol, tables=tables,
+                      precision=args.precision)
     t0 = time.perf_counter()
     done, unfinished = srv.run(reqs, wall_limit=args.serve_seconds * args.serve_drain,
                                log=lambda r: print(f"retired {r.rid} after {r.steps} steps, "
@@ -297,7 +299,8 @@ def run_serve(args):
     lat = [r.latency for r in done]
     out = {"metric": METRIC, "value": len(done) / el, "unit": "swarm solves/s", "n_gpus": 1,
            "steps": srv.steps_run, "warmup": 0, "ms_per_step": el / max(srv.steps_run, 1) * 1e3,
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 split (f64 tallies)" if args.precision == "f32" else "f64",
            "data": "synthetic",
            "config": {"workload": f"served swarm requests, Poisson {args.rate:g} req/s for "
                                   f"{args.serve_seconds:g} s, target error on W 0.5-2%",
@@ -346,6 +349,8 @@ def main():
     ap.add_argument("--serve-drain", type=float, default=1.5)
     ap.add_argument("--serve-capacity", type=int, default=200_000_000)
     ap.add_argument("--serve-plain", type=int, default=0)
+    ap.add_argument("--precision", choices=["f64", "f32"], default="f32",
+                    help="store and step precision of the batched/served swarm workloads")
     ap.add_argument("--mesh", type=int, nargs=3, default=[128, 128, 256])
     ap.add_argument("--h", type=float, default=5e-6)
     ap.add_argument("--npairs", type=int, default=1_000_000)

@@ -276,8 +276,12 @@ typedef struct mcs_swarm_tally {
     double last[MCS_TALLY_NQ];   /* per-step means of the last step */
 } mcs_swarm_tally;
 
+/* precision MCS_PREC_F64: the FP64 store and step (bit-exact with a lone FP64
+ * population); MCS_PREC_F32: split store (packed cell of a 1e-5 m virtual grid +
+ * FP32 offset, FP32 velocity/weight/clock) with the FP32 step; tallies are FP64
+ * compensated sums in both. */
 int32_t mcs_swarm_batch_create(mcs_ctx *ctx, int64_t total_capacity, int32_t max_slots,
-                               uint64_t seed, mcs_swarm_batch **out);
+                               uint64_t seed, int32_t precision, mcs_swarm_batch **out);
 int32_t mcs_swarm_batch_destroy(mcs_swarm_batch *b);
 /* Admit an instance: copies its n initial particles (ids 0..n-1) into a free slab
  * (first fit), or draws them on the device when x is NULL (see init_vth).

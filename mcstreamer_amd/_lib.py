@@ -185,7 +185,7 @@ SIGNATURES = {
     "mcs_mg_solve_host": [_P, _PD, _PD, _PI32, _PD],
     "mcs_mg_solve_mesh": [_P, _P, _PI32, _PD],
     "mcs_mg_setinhom_mesh": [_P, _P, _I32, _PD],
-    "mcs_swarm_batch_create": [_P, _I64, _I32, _U64, _PP],
+    "mcs_swarm_batch_create": [_P, _I64, _I32, _U64, _I32, _PP],
     "mcs_swarm_batch_destroy": [_P],
     "mcs_swarm_batch_add": [_P, _P, ctypes.POINTER(SwarmParams), _I64, _PD, _PD, _PD, _PD, _PU8, _PI32],
     "mcs_swarm_batch_remove": [_P, _I32],

@@ -288,6 +288,7 @@ extern "C" int32_t mcs_table_create(mcs_ctx *ctx, int32_t nproc, int32_t ne, int
     t->ne = ne;
     t->egrid = egrid_kind;
     t->maxrate = maxrate;
+    t->e0 = energy[0];
     t->step = (energy[ne - 1] - energy[0]) / (double)(ne - 1);
     t->inv_dlog = egrid_kind == MCS_EGRID_LOG
                       ? (double)(ne - 1) / mcs_log(energy[ne - 1] / energy[0])
@@ -304,6 +305,19 @@ extern "C" int32_t mcs_table_create(mcs_ctx *ctx, int32_t nproc, int32_t ne, int
     hipMemcpy(t->rate, rate, sizeof(double) * (size_t)ne * nproc, hipMemcpyHostToDevice);
     hipMemcpy(t->kind, kind, sizeof(int32_t) * nproc, hipMemcpyHostToDevice);
     hipMemcpy(t->param, param, sizeof(double) * nproc, hipMemcpyHostToDevice);
+    {
+        std::vector<float> e32(energy, energy + ne), r32(rate, rate + (size_t)ne * nproc),
+            p32(param, param + nproc);
+        if (hipMalloc(&t->energy32, sizeof(float) * ne) != hipSuccess ||
+            hipMalloc(&t->rate32, sizeof(float) * (size_t)ne * nproc) != hipSuccess ||
+            hipMalloc(&t->param32, sizeof(float) * nproc) != hipSuccess) {
+            mcs_table_destroy(t);
+            return mcs_fail(ctx, MCS_E_HIP, "table: device allocation failed");
+        }
+        hipMemcpy(t->energy32, e32.data(), sizeof(float) * ne, hipMemcpyHostToDevice);
+        hipMemcpy(t->rate32, r32.data(), sizeof(float) * r32.size(), hipMemcpyHostToDevice);
+        hipMemcpy(t->param32, p32.data(), sizeof(float) * nproc, hipMemcpyHostToDevice);
+    }
     *out = t;
     return MCS_OK;
 }
@@ -316,6 +330,9 @@ extern "C" int32_t mcs_table_destroy(mcs_table *t)
     hipFree(t->rate);
     hipFree(t->kind);
     hipFree(t->param);
+    hipFree(t->energy32);
+    hipFree(t->rate32);
+    hipFree(t->param32);
     delete t;
     return MCS_OK;
 }

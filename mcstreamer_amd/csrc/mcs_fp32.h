@@ -1,0 +1,346 @@
+// mcs_fp32.h -- the split-precision particle step: positions stored as a packed
+// integer cell plus an FP32 offset inside the cell (in cell units), velocities,
+// weights and collision clocks in FP32, collision sampling in FP32 from 24-bit
+// uniforms.  The physics is JuMC.advance! (mixed_population.jl:24-54) exactly as
+// in the FP64 body (mcs_advance.h); only the arithmetic precision differs, so the
+// results agree with the FP64 path statistically, not bitwise.
+#pragma once
+
+#include "mcs_device.h"
+
+// Packed cell coordinates: i in bits 0-9, j in 10-19, k in 20-30.
+#define MCS_CX_BITS 10
+#define MCS_CY_BITS 10
+#define MCS_CZ_BITS 11
+#define MCS_CX_MAX (1 << MCS_CX_BITS)
+#define MCS_CY_MAX (1 << MCS_CY_BITS)
+#define MCS_CZ_MAX (1 << MCS_CZ_BITS)
+
+struct SplitBuf {
+    int32_t *cell = nullptr;
+    float *off[3] = {nullptr, nullptr, nullptr};
+    float *v[3] = {nullptr, nullptr, nullptr};
+    float *w = nullptr, *s = nullptr;
+    uint8_t *active = nullptr;
+    uint64_t *id = nullptr;
+};
+
+__host__ __device__ __forceinline__ int32_t pack_cell(int32_t i, int32_t j, int32_t k)
+{
+    return i | (j << MCS_CX_BITS) | (k << (MCS_CX_BITS + MCS_CY_BITS));
+}
+
+__host__ __device__ __forceinline__ void unpack_cell(int32_t c, int32_t &i, int32_t &j, int32_t &k)
+{
+    i = c & (MCS_CX_MAX - 1);
+    j = (c >> MCS_CX_BITS) & (MCS_CY_MAX - 1);
+    k = (c >> (MCS_CX_BITS + MCS_CY_BITS)) & (MCS_CZ_MAX - 1);
+}
+
+struct TableView32 {
+    const float *energy;
+    const float *rate;
+    const int32_t *kind;
+    const float *param;
+    int32_t nproc, ne, egrid;
+    float e0, inv_step, inv_dlog, maxrate, inv_maxrate;
+};
+
+static inline TableView32 table_view32(const mcs_table *t)
+{
+    TableView32 v;
+    v.energy = t->energy32;
+    v.rate = t->rate32;
+    v.kind = t->kind;
+    v.param = t->param32;
+    v.nproc = t->nproc;
+    v.ne = t->ne;
+    v.egrid = t->egrid;
+    v.e0 = (float)t->e0;
+    v.inv_step = (float)(1.0 / t->step);
+    v.inv_dlog = (float)t->inv_dlog;
+    v.maxrate = (float)t->maxrate;
+    v.inv_maxrate = (float)(1.0 / t->maxrate);
+    return v;
+}
+
+// Face fields in FP32 on the mesh, cell size h (m).
+struct MeshView32 {
+    int32_t n[3];
+    float h, inv_h;
+    const float *ex, *ey, *ez;
+    double *qfixed;
+    double inv_dv;
+};
+
+#define MCS_HALF_ME_F 4.55469178e-31f
+#define MCS_QM_F (-175882002360.2124f)
+#define MCS_ELECTRON_MASS_F 9.10938356e-31f
+
+__device__ __forceinline__ float energy32(const float v[3])
+{
+    return MCS_HALF_ME_F * ((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+}
+
+// indweight (JuMC/src/util.jl:23-30) in FP32: node k with energy[k] <= eng <
+// energy[k+1] (the float nodes), weight of node k.
+__device__ __forceinline__ void indweight32(const TableView32 &T, float eng, int32_t &k, float &w,
+                                            int32_t &flags)
+{
+    const float *E = T.energy;
+    if (eng >= E[T.ne - 1]) {
+        k = T.ne - 2;
+        w = 0.0f;
+        flags |= MCS_FLAG_TABLE_RANGE;
+        return;
+    }
+    int32_t kk;
+    if (T.egrid == MCS_EGRID_UNIFORM) {
+        kk = (int32_t)floorf((eng - T.e0) * T.inv_step);
+    } else {
+        if (eng <= T.e0) {
+            k = 0;
+            w = 1.0f;
+            return;
+        }
+        kk = (int32_t)floorf(__logf(eng / T.e0) * T.inv_dlog);
+    }
+    kk = kk < 0 ? 0 : (kk > T.ne - 2 ? T.ne - 2 : kk);
+    while (kk > 0 && E[kk] > eng) kk -= 1;
+    while (kk < T.ne - 2 && E[kk + 1] <= eng) kk += 1;
+    const float e1 = E[kk + 1], e0 = E[kk];
+    float ww = (e1 - eng) / (e1 - e0);
+    k = kk;
+    w = fminf(fmaxf(ww, 0.0f), 1.0f);
+}
+
+// nextcoll() = -log(rand()) from a 24-bit uniform; 1 - u >= 2^-24.
+__device__ __forceinline__ float nextcoll32(mcs_stream *st)
+{
+    return -__logf(1.0f - mcs_u01f(st));
+}
+
+// randsphere() (util.jl:4-12) with phi drawn first; the hardware sine and cosine
+// take the angle in turns.
+__device__ __forceinline__ void randsphere32(mcs_stream *st, float out[3])
+{
+    const float uphi = mcs_u01f(st);
+    const float sp = __builtin_amdgcn_sinf(uphi), cp = __builtin_amdgcn_cosf(uphi);
+    const float u = 2.0f * mcs_u01f(st) - 1.0f;
+    const float r = sqrtf(fmaxf(1.0f - u * u, 0.0f));
+    out[0] = r * cp;
+    out[1] = r * sp;
+    out[2] = u;
+}
+
+// Staggered trilinear gather of one face array in split coordinates (the FP32
+// form of efield_mesh_dev): per axis the integer cell m and fraction d of the
+// position, then the half-cell shifted index for the staggered axes.
+__device__ __forceinline__ float trilerp32(const float *__restrict__ a, int32_t sx, int32_t sxy,
+                                           int32_t base, float dx, float dy, float dz)
+{
+    float c00 = a[base] * (1.0f - dx) + a[base + 1] * dx;
+    float c10 = a[base + sx] * (1.0f - dx) + a[base + sx + 1] * dx;
+    float c01 = a[base + sxy] * (1.0f - dx) + a[base + sxy + 1] * dx;
+    float c11 = a[base + sxy + sx] * (1.0f - dx) + a[base + sxy + sx + 1] * dx;
+    float c0 = c00 * (1.0f - dy) + c10 * dy;
+    float c1 = c01 * (1.0f - dy) + c11 * dy;
+    return c0 * (1.0f - dz) + c1 * dz;
+}
+
+// Returns false (field zero) outside the open box.
+__device__ __forceinline__ bool axis32(int32_t c, float off, int32_t n, int32_t &m, float &d,
+                                       int32_t &m1, float &d1)
+{
+    const float fl = floorf(off);
+    int32_t mi = c + (int32_t)fl;
+    float di = off - fl;
+    if (mi < 0 || mi > n - 1 || (mi == 0 && di == 0.0f)) return false;
+    m = mi;
+    d = di;
+    if (di < 0.5f) {
+        m1 = mi;
+        d1 = di + 0.5f;
+    } else {
+        m1 = mi + 1;
+        d1 = di - 0.5f;
+    }
+    return true;
+}
+
+__device__ __forceinline__ void efield32(const MeshView32 &M, const int32_t c[3], const float off[3],
+                                         float e[3], int32_t &flags)
+{
+    int32_t mx, my, mz, m1x, m1y, m1z;
+    float dx, dy, dz, d1x, d1y, d1z;
+    if (!axis32(c[0], off[0], M.n[0], mx, dx, m1x, d1x) || !axis32(c[1], off[1], M.n[1], my, dy, m1y, d1y) ||
+        !axis32(c[2], off[2], M.n[2], mz, dz, m1z, d1z)) {
+        e[0] = 0.0f;
+        e[1] = 0.0f;
+        e[2] = 0.0f;
+        return;
+    }
+    const int32_t nx = M.n[0], ny = M.n[1];
+    {
+        int32_t sx = nx + 1, sxy = (nx + 1) * (ny + 2);
+        e[0] = trilerp32(M.ex, sx, sxy, mx + sx * m1y + sxy * m1z, dx, d1y, d1z);
+    }
+    {
+        int32_t sx = nx + 2, sxy = (nx + 2) * (ny + 1);
+        e[1] = trilerp32(M.ey, sx, sxy, m1x + sx * my + sxy * m1z, d1x, dy, d1z);
+    }
+    {
+        int32_t sx = nx + 2, sxy = (nx + 2) * (ny + 2);
+        e[2] = trilerp32(M.ez, sx, sxy, m1x + sx * m1y + sxy * mz, d1x, d1y, dz);
+    }
+    const float e2 = (e[0] * e[0] + e[1] * e[1]) + e[2] * e[2];
+    if (e2 > (float)MCS_EFIELD_LIMIT2) flags |= MCS_FLAG_FIELD;
+}
+
+// NGP ion-charge tracker (gridfields.jl:303-313) from split coordinates.
+__device__ __forceinline__ void track32(const MeshView32 &M, const int32_t c[3], const float off[3],
+                                        double val)
+{
+    int32_t q[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        q[d] = c[d] + (int32_t)floorf(off[d]);
+        if (q[d] < 0 || q[d] >= M.n[d]) return;
+    }
+    const int64_t idx = (int64_t)(q[0] + 1) +
+                        (int64_t)(M.n[0] + 2) * ((int64_t)(q[1] + 1) + (int64_t)(M.n[1] + 2) * (q[2] + 1));
+    atomicAdd(&M.qfixed[idx], val * M.inv_dv);
+}
+
+// Moves whole cells from the offset into the packed coordinates, clamped to
+// [0, nmax) per axis (beyond the clamp the offset carries the distance).
+__device__ __forceinline__ void rebase32(int32_t c[3], float off[3], const int32_t nmax[3])
+{
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        int32_t nc = c[d] + (int32_t)floorf(off[d]);
+        nc = nc < 0 ? 0 : (nc > nmax[d] - 1 ? nmax[d] - 1 : nc);
+        off[d] = off[d] - (float)(nc - c[d]);
+        c[d] = nc;
+    }
+}
+
+// One particle through one time step in FP32 (JuMC/src/mixed_population.jl:31-51;
+// flights, Yoshida push of electron.jl:51-62 in a mesh field or its exact
+// parabola in a uniform one, collisions of collisions.jl:236-269 and
+// electron.jl:108-139).  Draws: 24-bit uniforms, one word each, in the order of
+// the FP64 body.  `birth(c, off, v, s, active)` receives each ionization child
+// at the parent's position, `event(j)` each selected table row.
+template <int FIELD_MESH, int TRACK, typename Birth, typename Event>
+__device__ __forceinline__ void advance_one32(int32_t c[3], float off[3], float v[3], const float w,
+                                              float &s, uint8_t &act, mcs_stream &rs, const float dt,
+                                              const TableView32 &T, const MeshView32 &M,
+                                              const float *acc_uni, const float inv_h, int32_t &flags,
+                                              int &flights, int &deaths, Birth &&birth, Event &&event)
+{
+    const float maxrate = T.maxrate;
+    float trem = dt;
+    const float c1 = (float)MCS_YOSHIDA_C1, c2 = (float)MCS_YOSHIDA_C2;
+    const float d1 = (float)MCS_YOSHIDA_D1, d2 = (float)MCS_YOSHIDA_D2;
+    while (trem > 0.0f) {
+        const float tnext = s * T.inv_maxrate;
+        float tt;
+        bool collides;
+        if (trem > tnext) {
+            tt = tnext;
+            collides = true;
+        } else {
+            tt = trem;
+            collides = false;
+            s = s - tt * maxrate;
+        }
+        if (FIELD_MESH) {
+            float e[3];
+            const float k1 = c1 * tt * inv_h, k2 = c2 * tt * inv_h;
+            const float f1 = (d1 * tt) * MCS_QM_F, f2 = (d2 * tt) * MCS_QM_F;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) off[d] = off[d] + k1 * v[d];
+            efield32(M, c, off, e, flags);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) v[d] = v[d] + f1 * e[d];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) off[d] = off[d] + k2 * v[d];
+            efield32(M, c, off, e, flags);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) v[d] = v[d] + f2 * e[d];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) off[d] = off[d] + k2 * v[d];
+            efield32(M, c, off, e, flags);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) v[d] = v[d] + f1 * e[d];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) off[d] = off[d] + k1 * v[d];
+        } else {
+            // the Yoshida map in a uniform field is the exact parabola
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const float dv = acc_uni[d] * tt;
+                off[d] = off[d] + (v[d] + 0.5f * dv) * (tt * inv_h);
+                v[d] = v[d] + dv;
+            }
+        }
+        flights += 1;
+        if (collides) {
+            const float eng = energy32(v);
+            int32_t k;
+            float wg;
+            indweight32(T, eng, k, wg, flags);
+            float xi = mcs_u01f(&rs) * maxrate;
+            const float *r0 = T.rate + k * T.nproc;
+            const float *r1 = r0 + T.nproc;
+            for (int j = 0; j < T.nproc; ++j) {
+                const float nu = wg * r0[j] + (1.0f - wg) * r1[j];
+                if (nu > xi) {
+                    event(j);
+                    const int kind = T.kind[j];
+                    if (kind == MCS_PROC_ELASTIC) {
+                        const float mr = T.param[j];
+                        const float cm = mr / (1.0f + mr);
+                        const float vc0 = cm * v[0], vc1 = cm * v[1], vc2 = cm * v[2];
+                        const float e0 = v[0] - vc0, e1 = v[1] - vc1, e2 = v[2] - vc2;
+                        const float nrm = sqrtf((e0 * e0 + e1 * e1) + e2 * e2);
+                        float om[3];
+                        randsphere32(&rs, om);
+                        v[0] = om[0] * nrm + vc0;
+                        v[1] = om[1] * nrm + vc1;
+                        v[2] = om[2] * nrm + vc2;
+                    } else if (kind == MCS_PROC_EXCITATION) {
+                        const float e1 = fmaxf(eng - T.param[j], 0.0f);
+                        const float vabs = sqrtf((2.0f * e1) / MCS_ELECTRON_MASS_F);
+                        float om[3];
+                        randsphere32(&rs, om);
+                        v[0] = om[0] * vabs;
+                        v[1] = om[1] * vabs;
+                        v[2] = om[2] * vabs;
+                    } else if (kind == MCS_PROC_IONIZATION) {
+                        const float e1 = fmaxf(eng - T.param[j], 0.0f) * 0.5f;
+                        const float vabs = sqrtf((2.0f * e1) / MCS_ELECTRON_MASS_F);
+                        float om1[3], om2[3];
+                        randsphere32(&rs, om1);
+                        randsphere32(&rs, om2);
+                        v[0] = om1[0] * vabs;
+                        v[1] = om1[1] * vabs;
+                        v[2] = om1[2] * vabs;
+                        const float sc = nextcoll32(&rs);
+                        if (TRACK) track32(M, c, off, (double)w);
+                        const float vc[3] = {om2[0] * vabs, om2[1] * vabs, om2[2] * vabs};
+                        birth(c, off, vc, sc, act);
+                    } else if (kind == MCS_PROC_ATTACHMENT) {
+                        if (TRACK) track32(M, c, off, -(double)w);
+                        if (act) deaths += 1;
+                        act = 0;
+                    }
+                    break;
+                }
+                xi = xi - nu;
+            }
+            s = nextcoll32(&rs);
+        }
+        trem = trem - tt;
+    }
+}

@@ -92,12 +92,15 @@ struct mcs_table {
     mcs_ctx *ctx;
     int32_t nproc, ne, egrid;
     double maxrate;
+    double e0;        // first energy node
     double inv_dlog;
     double step;
     double *energy;   // ne
     double *rate;     // ne * nproc
     int32_t *kind;    // nproc
     double *param;    // nproc
+    // FP32 copies for the split-precision kernels
+    float *energy32 = nullptr, *rate32 = nullptr, *param32 = nullptr;
     std::vector<int32_t> h_kind;
 };
 

@@ -7,17 +7,30 @@
 // reduction over the slot's consecutive tiles.  Every kernel runs on a grid that
 // depends only on the set of slots, never on particle counts (those live on the
 // device), so a step is captured once into a HIP graph and replayed.
+//
+// The store is either the FP64 SoA of the single population (bit-exact with the
+// oracle) or the split FP32 layout of mcs_fp32.h (packed cell of a virtual grid
+// plus an FP32 offset; FP32 push and collisions); tallies are FP64 compensated
+// sums in both.
 #include <algorithm>
 #include <cstring>
 #include <vector>
 
 #include "mcs_advance.h"
+#include "mcs_fp32.h"
 
 #define SW_NQ MCS_TALLY_NQ
 #define SW_NT (SW_NQ + 1)     // count + the tallied quantities
 #define SW_MODE_NONE 0
 #define SW_MODE_DROP 1
 #define SW_MODE_CLONE 2
+
+// Virtual grid of the FP32 store: cells of SW_H32 m, x = 0 at the corner of cell
+// (SW_OX, SW_OY, SW_OZ); the packed coordinates span the whole 10/10/11-bit range.
+#define SW_H32 1e-5
+#define SW_OX (MCS_CX_MAX / 2)
+#define SW_OY (MCS_CY_MAX / 2)
+#define SW_OZ (MCS_CZ_MAX / 2)
 
 struct SlotDev {
     int64_t *base, *cap, *n, *n0;
@@ -26,12 +39,15 @@ struct SlotDev {
     uint32_t *key, *step;
     uint64_t *next_id;
     TableView *tv;
+    TableView32 *tv32;
     int32_t *first_tile, *ntiles;
     // tallies: per slot SW_NQ each
     double *sum, *sumc, *cur, *bm1, *bm2, *last;
     int64_t *steps, *nrec, *nblk, *curcnt, *nact;
 };
 
+// Birth logs: one record per ionization child, written in any order and placed
+// by (parent's tile offset, offset within the tile, sequence within the parent).
 struct SwLog {
     int64_t *parent;
     int32_t *seq;
@@ -41,18 +57,31 @@ struct SwLog {
     int64_t cap;
 };
 
-struct SwArgs {
-    PopBuf b, alt;
+struct SwLog32 {
+    int64_t *parent;
+    int32_t *seq;
+    int32_t *cell;
+    float *off[3], *v[3];
+    float *s;
+    uint8_t *act;
+    int64_t cap;
+};
+
+template <typename Buf, typename Log>
+struct SwArgsT {
+    Buf b, alt;
     SlotDev S;
     const int32_t *tile_slot;
     const int64_t *tile_start;
     uint64_t seed;
-    SwLog L;
+    Log L;
     int64_t *log_count;
     int32_t *loff, *cnt;
     int64_t *tile_births, *tile_bpre, *tile_clones, *tile_cpre, *tile_act, *tile_apre;
     double *tile_part;   // ntiles x SW_NT x 2 (Neumaier sum, compensation)
 };
+using SwArgs64 = SwArgsT<PopBuf, SwLog>;
+using SwArgs32 = SwArgsT<SplitBuf, SwLog32>;
 
 struct Neu2 {
     double s, c;
@@ -100,6 +129,25 @@ __device__ __forceinline__ T bscan(T v, T *total)
     return pre + x - v;
 }
 
+__device__ __forceinline__ void copy_rec(const PopBuf &src, int64_t si, const PopBuf &dst, int64_t di)
+{
+    copy_particle(src, si, dst, di);
+}
+
+__device__ __forceinline__ void copy_rec(const SplitBuf &src, int64_t si, const SplitBuf &dst, int64_t di)
+{
+    dst.cell[di] = src.cell[si];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        dst.off[d][di] = src.off[d][si];
+        dst.v[d][di] = src.v[d][si];
+    }
+    dst.w[di] = src.w[si];
+    dst.s[di] = src.s[si];
+    dst.active[di] = src.active[si];
+    dst.id[di] = src.id[si];
+}
+
 // Tallies of one particle state (Swarm.jl:126-130,152,156 with collrate :242-248):
 // count, v_z, energy, summed rate of the ionization rows, of the attachment rows.
 __device__ __forceinline__ void tally_add(Neu2 acc[SW_NT], const double v[3], const TableView &T)
@@ -122,6 +170,31 @@ __device__ __forceinline__ void tally_add(Neu2 acc[SW_NT], const double v[3], co
     nadd(acc[2], eng);
     nadd(acc[3], ion);
     nadd(acc[4], att);
+}
+
+// The same tallies of an FP32 state: the state and the interpolated rates are
+// FP32, the accumulation FP64 (compensated).
+__device__ __forceinline__ void tally_add32(Neu2 acc[SW_NT], const float v[3], const TableView32 &T)
+{
+    const float eng = energy32(v);
+    int32_t k;
+    float w;
+    int32_t fl = 0;
+    indweight32(T, eng, k, w, fl);
+    const float *r0 = T.rate + k * T.nproc;
+    const float *r1 = r0 + T.nproc;
+    float ion = 0.0f, att = 0.0f;
+    for (int j = 0; j < T.nproc; ++j) {
+        int kd = T.kind[j];
+        if (kd == MCS_PROC_IONIZATION) ion += w * r0[j] + (1.0f - w) * r1[j];
+        else if (kd == MCS_PROC_ATTACHMENT) att += w * r0[j] + (1.0f - w) * r1[j];
+    }
+    const double vd[3] = {(double)v[0], (double)v[1], (double)v[2]};
+    nadd(acc[0], 1.0);
+    nadd(acc[1], vd[2]);
+    nadd(acc[2], energy_dev(vd[0], vd[1], vd[2]));
+    nadd(acc[3], (double)ion);
+    nadd(acc[4], (double)att);
 }
 
 // Block-reduce SW_NT Neumaier pairs into tile_part[tile].
@@ -153,9 +226,9 @@ __device__ __forceinline__ void tile_reduce(Neu2 acc[SW_NT], double *out)
 }
 
 // advance! of every slot's particles (JuMC/src/mixed_population.jl:24-54), births
-// to the log, and the post-step tallies of parents and children.
+// to the log, and the post-step tallies of parents and children.  FP64 store.
 template <int BS>
-__global__ __launch_bounds__(BS) void k_sw_advance(SwArgs a)
+__global__ __launch_bounds__(BS) void k_sw_advance(SwArgs64 a)
 {
     const int64_t tile = blockIdx.x;
     const int slot = a.tile_slot[tile];
@@ -216,9 +289,139 @@ __global__ __launch_bounds__(BS) void k_sw_advance(SwArgs a)
     if (flags) atomicOr(&a.S.flags[slot], flags);
 }
 
+// The same step on the FP32 split store.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_sw_advance32(SwArgs32 a)
+{
+    const int64_t tile = blockIdx.x;
+    const int slot = a.tile_slot[tile];
+    const int64_t i = a.tile_start[tile] + threadIdx.x;
+    const int64_t lim = a.S.base[slot] + a.S.n[slot];
+    const TableView32 T = a.S.tv32[slot];
+    Neu2 acc[SW_NT];
+#pragma unroll
+    for (int q = 0; q < SW_NT; ++q) acc[q] = Neu2{0.0, 0.0};
+    int nb = 0;
+    int32_t flags = 0;
+    if (i < lim && a.b.active[i]) {
+        int32_t c[3];
+        unpack_cell(a.b.cell[i], c[0], c[1], c[2]);
+        float off[3] = {a.b.off[0][i], a.b.off[1][i], a.b.off[2][i]};
+        float v[3] = {a.b.v[0][i], a.b.v[1][i], a.b.v[2][i]};
+        const float w = a.b.w[i];
+        float s = a.b.s[i];
+        uint8_t act = 1;
+        mcs_stream rs;
+        mcs_stream_init(&rs, a.seed, a.S.key[slot], MCS_PHASE_ADVANCE, a.b.id[i], a.S.step[slot]);
+        const float acc_u[3] = {(float)(MCS_QM * a.S.e[3 * slot]), (float)(MCS_QM * a.S.e[3 * slot + 1]),
+                                (float)(MCS_QM * a.S.e[3 * slot + 2])};
+        MeshView32 M;
+        int flights = 0, deaths = 0;
+        advance_one32<0, 0>(
+            c, off, v, w, s, act, rs, (float)a.S.dt[slot], T, M, acc_u, (float)(1.0 / SW_H32), flags,
+            flights, deaths,
+            [&](const int32_t *bc, const float *boff, const float *bv, float bs, uint8_t bact) {
+                unsigned long long r = atomicAdd((unsigned long long *)a.log_count, 1ULL);
+                if ((int64_t)r < a.L.cap) {
+                    a.L.parent[r] = i;
+                    a.L.seq[r] = nb;
+                    a.L.cell[r] = pack_cell(bc[0], bc[1], bc[2]);
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        a.L.off[d][r] = boff[d];
+                        a.L.v[d][r] = bv[d];
+                    }
+                    a.L.s[r] = bs;
+                    a.L.act[r] = bact;
+                } else {
+                    flags |= MCS_FLAG_CAPACITY;
+                }
+                if (bact) tally_add32(acc, bv, T);
+                nb += 1;
+            },
+            [&](int) {});
+        const int32_t nmax[3] = {MCS_CX_MAX, MCS_CY_MAX, MCS_CZ_MAX};
+        rebase32(c, off, nmax);
+        a.b.cell[i] = pack_cell(c[0], c[1], c[2]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            a.b.off[d][i] = off[d];
+            a.b.v[d][i] = v[d];
+        }
+        a.b.s[i] = s;
+        if (!act) a.b.active[i] = 0;
+        if (act) tally_add32(acc, v, T);
+    }
+    int tot;
+    int ex = bscan<BS, int>(nb, &tot);
+    if (nb > 0) a.loff[i] = ex;
+    if (threadIdx.x == 0) a.tile_births[tile] = tot;
+    tile_reduce<BS>(acc, a.tile_part + tile * (2 * SW_NT));
+    if (flags) atomicOr(&a.S.flags[slot], flags);
+}
+
+// The tile of a particle index: bisection over the (sorted) tile starts.
+__device__ __forceinline__ int64_t tile_of(const int64_t *tile_start, int64_t ntiles, int64_t i)
+{
+    int64_t lo = 0, hi = ntiles - 1;
+    while (lo < hi) {
+        int64_t mid = (lo + hi + 1) >> 1;
+        if (tile_start[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void put_child(const SwArgs64 &a, int64_t r, int64_t i, int64_t d)
+{
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        a.b.x[q][d] = a.L.x[q][r];
+        a.b.v[q][d] = a.L.v[q][r];
+    }
+    a.b.w[d] = a.b.w[i];
+    a.b.s[d] = a.L.s[r];
+    a.b.active[d] = a.L.act[r];
+}
+
+__device__ __forceinline__ void put_child(const SwArgs32 &a, int64_t r, int64_t i, int64_t d)
+{
+    a.b.cell[d] = a.L.cell[r];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        a.b.off[q][d] = a.L.off[q][r];
+        a.b.v[q][d] = a.L.v[q][r];
+    }
+    a.b.w[d] = a.b.w[i];
+    a.b.s[d] = a.L.s[r];
+    a.b.active[d] = a.L.act[r];
+}
+
+// Children from the birth log to their slabs: rank within the slot in parent
+// order (tile offset + offset within the tile + sequence within the parent),
+// ids next_id + rank.  Grid-stride over the log with a fixed grid.
+template <typename A>
+__global__ __launch_bounds__(MCS_BLOCK) void k_sw_births(A a, int64_t logcap, int64_t ntiles)
+{
+    const int64_t cnt = *a.log_count < logcap ? *a.log_count : logcap;
+    for (int64_t r = gidx(); r < cnt; r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = a.L.parent[r];
+        const int64_t tile = tile_of(a.tile_start, ntiles, i);
+        const int slot = a.tile_slot[tile];
+        const int64_t base = a.S.base[slot], cap = a.S.cap[slot], n = a.S.n[slot];
+        const int64_t f = a.S.first_tile[slot];
+        const int64_t rank = (a.tile_bpre[tile] - a.tile_bpre[f]) + a.loff[i] + a.L.seq[r];
+        if (n + rank >= cap) continue;
+        const int64_t d = base + n + rank;
+        put_child(a, r, i, d);
+        a.b.id[d] = a.S.next_id[slot] + (uint64_t)rank;
+    }
+}
+
 // Per slot: births -> n, next_id; tallies of this step; resample decision of
 // Swarm.resample! (Swarm.jl:194-199).  One workgroup per slot.
-__global__ __launch_bounds__(MCS_BLOCK) void k_sw_slot_step(SwArgs a)
+template <typename A>
+__global__ __launch_bounds__(MCS_BLOCK) void k_sw_slot_step(A a)
 {
     const int slot = blockIdx.x;
     if (!a.S.used[slot]) return;
@@ -309,8 +512,8 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_sw_slot_step(SwArgs a)
 
 // Per-slot clone counts / drops of Swarm.resample! (Swarm.jl:200-221): one draw
 // stream per particle (phase swarm-resample), as in mcs_swarm_resample.
-template <int BS>
-__global__ __launch_bounds__(BS) void k_sw_resample(SwArgs a)
+template <int BS, typename A>
+__global__ __launch_bounds__(BS) void k_sw_resample(A a)
 {
     const int64_t tile = blockIdx.x;
     const int slot = a.tile_slot[tile];
@@ -334,8 +537,8 @@ __global__ __launch_bounds__(BS) void k_sw_resample(SwArgs a)
     if (threadIdx.x == 0) a.tile_clones[tile] = tot;
 }
 
-template <int BS>
-__global__ __launch_bounds__(BS) void k_sw_clone_place(SwArgs a)
+template <int BS, typename A>
+__global__ __launch_bounds__(BS) void k_sw_clone_place(A a)
 {
     const int64_t tile = blockIdx.x;
     const int slot = a.tile_slot[tile];
@@ -350,12 +553,13 @@ __global__ __launch_bounds__(BS) void k_sw_clone_place(SwArgs a)
         int64_t rank = off + j;
         if (n + rank >= cap) break;
         int64_t d = base + n + rank;
-        copy_particle(a.b, i, a.b, d);
+        copy_rec(a.b, i, a.b, d);
         a.b.id[d] = nid + (uint64_t)rank;
     }
 }
 
-__global__ void k_sw_slot_step2(SwArgs a, int nslots)
+template <typename A>
+__global__ void k_sw_slot_step2(A a, int nslots)
 {
     const int slot = gidx();
     if (slot >= nslots || !a.S.used[slot]) return;
@@ -373,8 +577,8 @@ __global__ void k_sw_slot_step2(SwArgs a, int nslots)
 }
 
 // Stable compaction of every slab: actives keep their order.
-template <int BS>
-__global__ __launch_bounds__(BS) void k_sw_compact_count(SwArgs a)
+template <int BS, typename A>
+__global__ __launch_bounds__(BS) void k_sw_compact_count(A a)
 {
     const int64_t tile = blockIdx.x;
     const int slot = a.tile_slot[tile];
@@ -387,8 +591,8 @@ __global__ __launch_bounds__(BS) void k_sw_compact_count(SwArgs a)
     if (threadIdx.x == 0) a.tile_act[tile] = tot;
 }
 
-template <int BS>
-__global__ __launch_bounds__(BS) void k_sw_compact_write(SwArgs a)
+template <int BS, typename A>
+__global__ __launch_bounds__(BS) void k_sw_compact_write(A a)
 {
     const int64_t tile = blockIdx.x;
     const int slot = a.tile_slot[tile];
@@ -397,10 +601,11 @@ __global__ __launch_bounds__(BS) void k_sw_compact_write(SwArgs a)
     if (!(i < lim && a.b.active[i])) return;
     const int64_t f = a.S.first_tile[slot];
     int64_t d = a.S.base[slot] + (a.tile_apre[tile] - a.tile_apre[f]) + a.loff[i];
-    copy_particle(a.b, i, a.alt, d);
+    copy_rec(a.b, i, a.alt, d);
 }
 
-__global__ void k_sw_compact_slot(SwArgs a, int nslots)
+template <typename A>
+__global__ void k_sw_compact_slot(A a, int nslots)
 {
     const int slot = gidx();
     if (slot >= nslots || !a.S.used[slot]) return;
@@ -408,13 +613,13 @@ __global__ void k_sw_compact_slot(SwArgs a, int nslots)
     a.S.n[slot] = a.tile_apre[f + nt] - a.tile_apre[f];
 }
 
-template <int BS>
-__global__ __launch_bounds__(BS) void k_sw_copy_back(SwArgs a)
+template <int BS, typename A>
+__global__ __launch_bounds__(BS) void k_sw_copy_back(A a)
 {
     const int64_t tile = blockIdx.x;
     const int slot = a.tile_slot[tile];
     const int64_t i = a.tile_start[tile] + threadIdx.x;
-    if (i < a.S.base[slot] + a.S.n[slot]) copy_particle(a.alt, i, a.b, i);
+    if (i < a.S.base[slot] + a.S.n[slot]) copy_rec(a.alt, i, a.b, i);
 }
 
 // n electrons at the origin with Maxwellian velocities (mcs_maxwell_init).
@@ -437,9 +642,131 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_sw_init(PopBuf b, int64_t base, i
     b.id[d] = (uint64_t)i;
 }
 
+// The same electrons in the FP32 store (draws in FP64, stored rounded).
+__global__ __launch_bounds__(MCS_BLOCK) void k_sw_init32(SplitBuf b, int64_t base, int64_t n, uint64_t seed,
+                                                         uint32_t key, double vth)
+{
+    int64_t i = gidx();
+    if (i >= n) return;
+    double v[3], s;
+    mcs_maxwell_init(seed, key, (uint64_t)i, vth, v, &s);
+    const int64_t d = base + i;
+    b.cell[d] = pack_cell(SW_OX, SW_OY, SW_OZ);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        b.off[q][d] = 0.0f;
+        b.v[q][d] = (float)v[q];
+    }
+    b.w[d] = 1.0f;
+    b.s[d] = (float)s;
+    b.active[d] = 1;
+    b.id[d] = (uint64_t)i;
+}
+
+// Host particles (x, v as n x 3 doubles) into the FP32 store.
+__global__ __launch_bounds__(MCS_BLOCK) void k_sw_from64(SplitBuf b, int64_t base, int64_t n, const double *x,
+                                                         const double *v, const double *w, const double *s,
+                                                         const uint8_t *act)
+{
+    int64_t i = gidx();
+    if (i >= n) return;
+    const int64_t d = base + i;
+    const int32_t o[3] = {SW_OX, SW_OY, SW_OZ}, nmax[3] = {MCS_CX_MAX, MCS_CY_MAX, MCS_CZ_MAX};
+    int32_t c[3];
+    float off[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const double r = x[3 * i + q] * (1.0 / SW_H32);
+        const double fl = floor(r);
+        c[q] = o[q] + (int32_t)fmax(fmin(fl, 1e9), -1e9);
+        off[q] = (float)(r - fl);
+        b.v[q][d] = (float)v[3 * i + q];
+    }
+    rebase32(c, off, nmax);
+    b.cell[d] = pack_cell(c[0], c[1], c[2]);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) b.off[q][d] = off[q];
+    b.w[d] = (float)w[i];
+    b.s[d] = (float)s[i];
+    b.active[d] = act[i];
+    b.id[d] = (uint64_t)i;
+}
+
+// The FP32 store back to FP64 positions (m) and velocities, into n x 3 arrays.
+__global__ __launch_bounds__(MCS_BLOCK) void k_sw_to64(SplitBuf b, int64_t base, int64_t n, double *x, double *v,
+                                                       double *w, double *s)
+{
+    int64_t i = gidx();
+    if (i >= n) return;
+    const int64_t d = base + i;
+    int32_t c[3];
+    unpack_cell(b.cell[d], c[0], c[1], c[2]);
+    const int32_t o[3] = {SW_OX, SW_OY, SW_OZ};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        x[3 * i + q] = ((double)(c[q] - o[q]) + (double)b.off[q][d]) * SW_H32;
+        v[3 * i + q] = (double)b.v[q][d];
+    }
+    w[i] = (double)b.w[d];
+    s[i] = (double)b.s[d];
+}
+
 __global__ void k_sw_zero(int64_t *p, int n)
 {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+// One slot's parameters and zeroed tallies, written in one launch.
+struct SlotInit {
+    int64_t base, cap, n, n0;
+    int32_t relax, block;
+    double e[3], dt, tol;
+    uint32_t key;
+    uint64_t next_id;
+    TableView tv;
+    TableView32 tv32;
+};
+
+__global__ void k_sw_slot_init(SlotDev S, int slot, SlotInit p)
+{
+    if (threadIdx.x != 0) return;
+    S.base[slot] = p.base;
+    S.cap[slot] = p.cap;
+    S.n[slot] = p.n;
+    S.n0[slot] = p.n0;
+    S.used[slot] = 1;
+    S.flags[slot] = 0;
+    S.mode[slot] = SW_MODE_NONE;
+    S.relax[slot] = p.relax;
+    S.block[slot] = p.block;
+    for (int d = 0; d < 3; ++d) S.e[3 * slot + d] = p.e[d];
+    S.dt[slot] = p.dt;
+    S.tol[slot] = p.tol;
+    S.param[slot] = 0.0;
+    S.key[slot] = p.key;
+    S.step[slot] = 0;
+    S.next_id[slot] = p.next_id;
+    S.tv[slot] = p.tv;
+    S.tv32[slot] = p.tv32;
+    for (int q = 0; q < SW_NQ; ++q) {
+        const int k = SW_NQ * slot + q;
+        S.sum[k] = 0.0;
+        S.sumc[k] = 0.0;
+        S.cur[k] = 0.0;
+        S.bm1[k] = 0.0;
+        S.bm2[k] = 0.0;
+        S.last[k] = 0.0;
+    }
+    S.steps[slot] = 0;
+    S.nrec[slot] = 0;
+    S.nblk[slot] = 0;
+    S.curcnt[slot] = 0;
+    S.nact[slot] = 0;
+}
+
+__global__ void k_sw_slot_free(SlotDev S, int slot)
+{
+    if (threadIdx.x == 0) S.used[slot] = 0;
 }
 
 // ---------------------------------------------------------------- host side
@@ -453,12 +780,15 @@ struct HostSlot {
 
 struct mcs_swarm_batch {
     mcs_ctx *ctx;
+    int precision;
     int64_t total_cap;
     int max_slots;
     uint64_t seed;
     PopBuf buf, alt;
+    SplitBuf buf32, alt32;
     SlotDev S;
     SwLog L;
+    SwLog32 L32;
     int64_t *log_count = nullptr;
     int32_t *loff = nullptr, *cnt = nullptr;
     int64_t max_tiles = 0;
@@ -468,7 +798,10 @@ struct mcs_swarm_batch {
     int64_t *tile_act = nullptr, *tile_apre = nullptr;
     int64_t *scan_scratch = nullptr;
     double *tile_part = nullptr;
+    double *stage = nullptr;       // host-exchange staging (FP32 store), 8 doubles per particle
+    int64_t stage_cap = 0;
     std::vector<HostSlot> slots;
+    std::vector<void *> owned;     // every device allocation, freed at destroy
     int64_t ntiles = 0;
     int block = MCS_BLOCK;
     int compact_every = 8;
@@ -478,81 +811,93 @@ struct mcs_swarm_batch {
 };
 
 template <typename T>
-static int32_t dalloc(mcs_ctx *ctx, T **p, int64_t n)
+static int32_t dalloc(mcs_swarm_batch *b, T **p, int64_t n)
 {
-    MCS_HIP(ctx, hipMalloc((void **)p, sizeof(T) * (size_t)(n > 0 ? n : 1)));
-    MCS_HIP(ctx, hipMemset(*p, 0, sizeof(T) * (size_t)(n > 0 ? n : 1)));
+    mcs_ctx *ctx = b->ctx;
+    const size_t bytes = sizeof(T) * (size_t)(n > 0 ? n : 1);
+    MCS_HIP(ctx, hipMalloc((void **)p, bytes));
+    b->owned.push_back((void *)*p);
+    MCS_HIP(ctx, hipMemset(*p, 0, bytes));
     return MCS_OK;
 }
 
-static int32_t alloc_popbuf(mcs_ctx *ctx, PopBuf &b, int64_t cap)
+static int32_t alloc_store(mcs_swarm_batch *b, PopBuf &p, int64_t cap)
 {
     int32_t rc = 0;
-    for (int d = 0; d < 3 && !rc; ++d) {
-        rc |= dalloc(ctx, &b.x[d], cap);
-        rc |= dalloc(ctx, &b.v[d], cap);
-    }
-    rc |= dalloc(ctx, &b.w, cap);
-    rc |= dalloc(ctx, &b.s, cap);
-    rc |= dalloc(ctx, &b.active, cap);
-    rc |= dalloc(ctx, &b.id, cap);
+    for (int d = 0; d < 3; ++d) rc |= dalloc(b, &p.x[d], cap) | dalloc(b, &p.v[d], cap);
+    rc |= dalloc(b, &p.w, cap) | dalloc(b, &p.s, cap) | dalloc(b, &p.active, cap) | dalloc(b, &p.id, cap);
     return rc ? MCS_E_HIP : MCS_OK;
 }
 
-static void free_popbuf(PopBuf &b)
+static int32_t alloc_store(mcs_swarm_batch *b, SplitBuf &p, int64_t cap)
 {
-    for (int d = 0; d < 3; ++d) {
-        hipFree(b.x[d]);
-        hipFree(b.v[d]);
-    }
-    hipFree(b.w);
-    hipFree(b.s);
-    hipFree(b.active);
-    hipFree(b.id);
+    int32_t rc = dalloc(b, &p.cell, cap);
+    for (int d = 0; d < 3; ++d) rc |= dalloc(b, &p.off[d], cap) | dalloc(b, &p.v[d], cap);
+    rc |= dalloc(b, &p.w, cap) | dalloc(b, &p.s, cap) | dalloc(b, &p.active, cap) | dalloc(b, &p.id, cap);
+    return rc ? MCS_E_HIP : MCS_OK;
+}
+
+static void free_all(mcs_swarm_batch *b)
+{
+    for (void *p : b->owned) hipFree(p);
+    b->owned.clear();
 }
 
 extern "C" int32_t mcs_swarm_batch_create(mcs_ctx *ctx, int64_t total_capacity, int32_t max_slots,
-                                          uint64_t seed, mcs_swarm_batch **out)
+                                          uint64_t seed, int32_t precision, mcs_swarm_batch **out)
 {
     if (!ctx || !out) return MCS_E_ARG;
     if (total_capacity < 1 || total_capacity >= ((int64_t)1 << 31) || max_slots < 1 || max_slots > 65536)
         return mcs_fail(ctx, MCS_E_ARG, "swarm batch: capacity must be in [1, 2^31), slots in [1, 65536]");
+    if (precision != MCS_PREC_F64 && precision != MCS_PREC_F32)
+        return mcs_fail(ctx, MCS_E_ARG, "swarm batch: unknown precision %d", precision);
     hipSetDevice(ctx->device);
     mcs_swarm_batch *b = new mcs_swarm_batch();
     b->ctx = ctx;
+    b->precision = precision;
     b->total_cap = total_capacity;
     b->max_slots = max_slots;
     b->seed = seed;
     b->slots.resize(max_slots);
-    int32_t rc = alloc_popbuf(ctx, b->buf, total_capacity);
-    rc |= alloc_popbuf(ctx, b->alt, total_capacity);
+    int32_t rc = 0;
+    if (precision == MCS_PREC_F64) {
+        rc |= alloc_store(b, b->buf, total_capacity) | alloc_store(b, b->alt, total_capacity);
+        SwLog &L = b->L;
+        L.cap = total_capacity;
+        rc |= dalloc(b, &L.parent, L.cap) | dalloc(b, &L.seq, L.cap) | dalloc(b, &L.s, L.cap) |
+              dalloc(b, &L.act, L.cap);
+        for (int d = 0; d < 3; ++d) rc |= dalloc(b, &L.x[d], L.cap) | dalloc(b, &L.v[d], L.cap);
+    } else {
+        rc |= alloc_store(b, b->buf32, total_capacity) | alloc_store(b, b->alt32, total_capacity);
+        SwLog32 &L = b->L32;
+        L.cap = total_capacity;
+        rc |= dalloc(b, &L.parent, L.cap) | dalloc(b, &L.seq, L.cap) | dalloc(b, &L.cell, L.cap) |
+              dalloc(b, &L.s, L.cap) | dalloc(b, &L.act, L.cap);
+        for (int d = 0; d < 3; ++d) rc |= dalloc(b, &L.off[d], L.cap) | dalloc(b, &L.v[d], L.cap);
+    }
     SlotDev &S = b->S;
     int64_t ms = max_slots;
-    rc |= dalloc(ctx, &S.base, ms) | dalloc(ctx, &S.cap, ms) | dalloc(ctx, &S.n, ms) | dalloc(ctx, &S.n0, ms);
-    rc |= dalloc(ctx, &S.used, ms) | dalloc(ctx, &S.flags, ms) | dalloc(ctx, &S.mode, ms) |
-          dalloc(ctx, &S.relax, ms) | dalloc(ctx, &S.block, ms);
-    rc |= dalloc(ctx, &S.e, 3 * ms) | dalloc(ctx, &S.dt, ms) | dalloc(ctx, &S.tol, ms) | dalloc(ctx, &S.param, ms);
-    rc |= dalloc(ctx, &S.key, ms) | dalloc(ctx, &S.step, ms) | dalloc(ctx, &S.next_id, ms);
-    rc |= dalloc(ctx, &S.tv, ms) | dalloc(ctx, &S.first_tile, ms) | dalloc(ctx, &S.ntiles, ms);
-    rc |= dalloc(ctx, &S.sum, SW_NQ * ms) | dalloc(ctx, &S.sumc, SW_NQ * ms) | dalloc(ctx, &S.cur, SW_NQ * ms) |
-          dalloc(ctx, &S.bm1, SW_NQ * ms) | dalloc(ctx, &S.bm2, SW_NQ * ms) | dalloc(ctx, &S.last, SW_NQ * ms);
-    rc |= dalloc(ctx, &S.steps, ms) | dalloc(ctx, &S.nrec, ms) | dalloc(ctx, &S.nblk, ms) |
-          dalloc(ctx, &S.curcnt, ms) | dalloc(ctx, &S.nact, ms);
-    SwLog &L = b->L;
-    L.cap = total_capacity;
-    rc |= dalloc(ctx, &L.parent, L.cap) | dalloc(ctx, &L.seq, L.cap) | dalloc(ctx, &L.s, L.cap) |
-          dalloc(ctx, &L.act, L.cap);
-    for (int d = 0; d < 3; ++d) rc |= dalloc(ctx, &L.x[d], L.cap) | dalloc(ctx, &L.v[d], L.cap);
-    rc |= dalloc(ctx, &b->log_count, 8);
-    rc |= dalloc(ctx, &b->loff, total_capacity + 512) | dalloc(ctx, &b->cnt, total_capacity + 512);
+    rc |= dalloc(b, &S.base, ms) | dalloc(b, &S.cap, ms) | dalloc(b, &S.n, ms) | dalloc(b, &S.n0, ms);
+    rc |= dalloc(b, &S.used, ms) | dalloc(b, &S.flags, ms) | dalloc(b, &S.mode, ms) | dalloc(b, &S.relax, ms) |
+          dalloc(b, &S.block, ms);
+    rc |= dalloc(b, &S.e, 3 * ms) | dalloc(b, &S.dt, ms) | dalloc(b, &S.tol, ms) | dalloc(b, &S.param, ms);
+    rc |= dalloc(b, &S.key, ms) | dalloc(b, &S.step, ms) | dalloc(b, &S.next_id, ms);
+    rc |= dalloc(b, &S.tv, ms) | dalloc(b, &S.tv32, ms) | dalloc(b, &S.first_tile, ms) | dalloc(b, &S.ntiles, ms);
+    rc |= dalloc(b, &S.sum, SW_NQ * ms) | dalloc(b, &S.sumc, SW_NQ * ms) | dalloc(b, &S.cur, SW_NQ * ms) |
+          dalloc(b, &S.bm1, SW_NQ * ms) | dalloc(b, &S.bm2, SW_NQ * ms) | dalloc(b, &S.last, SW_NQ * ms);
+    rc |= dalloc(b, &S.steps, ms) | dalloc(b, &S.nrec, ms) | dalloc(b, &S.nblk, ms) | dalloc(b, &S.curcnt, ms) |
+          dalloc(b, &S.nact, ms);
+    rc |= dalloc(b, &b->log_count, 8);
+    rc |= dalloc(b, &b->loff, total_capacity + 512) | dalloc(b, &b->cnt, total_capacity + 512);
     b->max_tiles = total_capacity / 128 + 2 * (int64_t)max_slots + 2;
-    rc |= dalloc(ctx, &b->tile_slot, b->max_tiles) | dalloc(ctx, &b->tile_start, b->max_tiles);
-    rc |= dalloc(ctx, &b->tile_births, b->max_tiles) | dalloc(ctx, &b->tile_bpre, b->max_tiles + 1);
-    rc |= dalloc(ctx, &b->tile_clones, b->max_tiles) | dalloc(ctx, &b->tile_cpre, b->max_tiles + 1);
-    rc |= dalloc(ctx, &b->tile_act, b->max_tiles) | dalloc(ctx, &b->tile_apre, b->max_tiles + 1);
-    rc |= dalloc(ctx, &b->scan_scratch, b->max_tiles / MCS_SCAN_TILE + 8);
-    rc |= dalloc(ctx, &b->tile_part, b->max_tiles * 2 * SW_NT);
+    rc |= dalloc(b, &b->tile_slot, b->max_tiles) | dalloc(b, &b->tile_start, b->max_tiles);
+    rc |= dalloc(b, &b->tile_births, b->max_tiles) | dalloc(b, &b->tile_bpre, b->max_tiles + 1);
+    rc |= dalloc(b, &b->tile_clones, b->max_tiles) | dalloc(b, &b->tile_cpre, b->max_tiles + 1);
+    rc |= dalloc(b, &b->tile_act, b->max_tiles) | dalloc(b, &b->tile_apre, b->max_tiles + 1);
+    rc |= dalloc(b, &b->scan_scratch, b->max_tiles / MCS_SCAN_TILE + 8);
+    rc |= dalloc(b, &b->tile_part, b->max_tiles * 2 * SW_NT);
     if (rc) {
+        free_all(b);
         delete b;
         return mcs_fail(ctx, MCS_E_HIP, "swarm batch: device allocation failed");
     }
@@ -574,17 +919,7 @@ extern "C" int32_t mcs_swarm_batch_destroy(mcs_swarm_batch *b)
     hipSetDevice(b->ctx->device);
     hipStreamSynchronize(b->ctx->stream);
     drop_graphs(b);
-    free_popbuf(b->buf);
-    free_popbuf(b->alt);
-    SlotDev &S = b->S;
-    void *ptrs[] = {S.base, S.cap, S.n, S.n0, S.used, S.flags, S.mode, S.relax, S.block, S.e, S.dt,
-                    S.tol, S.param, S.key, S.step, S.next_id, S.tv, S.first_tile, S.ntiles, S.sum,
-                    S.sumc, S.cur, S.bm1, S.bm2, S.last, S.steps, S.nrec, S.nblk, S.curcnt, S.nact,
-                    b->L.parent, b->L.seq, b->L.s, b->L.act, b->L.x[0], b->L.x[1], b->L.x[2],
-                    b->L.v[0], b->L.v[1], b->L.v[2], b->log_count, b->loff, b->cnt, b->tile_slot,
-                    b->tile_start, b->tile_births, b->tile_bpre, b->tile_clones, b->tile_cpre,
-                    b->tile_act, b->tile_apre, b->scan_scratch, b->tile_part};
-    for (void *p : ptrs) hipFree(p);
+    free_all(b);
     delete b;
     return MCS_OK;
 }
@@ -623,10 +958,19 @@ static int32_t rebuild_tiles(mcs_swarm_batch *b)
     return MCS_OK;
 }
 
-template <typename T>
-static int32_t put1(mcs_ctx *ctx, T *dev, int slot, const T &v)
+static int32_t ensure_stage(mcs_swarm_batch *b, int64_t n)
 {
-    MCS_HIP(ctx, hipMemcpy(dev + slot, &v, sizeof(T), hipMemcpyHostToDevice));
+    if (n <= b->stage_cap) return MCS_OK;
+    mcs_ctx *ctx = b->ctx;
+    if (b->stage) {
+        hipFree(b->stage);
+        b->owned.erase(std::find(b->owned.begin(), b->owned.end(), (void *)b->stage));
+        b->stage = nullptr;
+    }
+    int64_t cap = std::max<int64_t>(n, 1 << 16);
+    MCS_HIP(ctx, hipMalloc((void **)&b->stage, sizeof(double) * 9 * (size_t)cap));
+    b->owned.push_back(b->stage);
+    b->stage_cap = cap;
     return MCS_OK;
 }
 
@@ -668,51 +1012,67 @@ extern "C" int32_t mcs_swarm_batch_add(mcs_swarm_batch *b, const mcs_table *t,
     }
     if (base < 0 && b->total_cap - at >= cap) base = at;
     if (base < 0) return mcs_fail(ctx, MCS_E_CAPACITY, "swarm batch: no free slab of %lld particles", (long long)cap);
-    MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const unsigned grid = (unsigned)std::max<int64_t>(cdiv(n, MCS_BLOCK), 1);
     // particles
-    if (dev_init && n > 0) {
-        hipLaunchKernelGGL(k_sw_init, dim3((unsigned)cdiv(n, MCS_BLOCK)), dim3(MCS_BLOCK), 0, ctx->stream,
-                           b->buf, base, n, b->seed, prm->instance, prm->init_vth);
-        MCS_LAUNCH_CHECK(ctx);
+    if (dev_init) {
+        if (n > 0) {
+            if (b->precision == MCS_PREC_F64)
+                hipLaunchKernelGGL(k_sw_init, dim3(grid), dim3(MCS_BLOCK), 0, ctx->stream, b->buf, base, n,
+                                   b->seed, prm->instance, prm->init_vth);
+            else
+                hipLaunchKernelGGL(k_sw_init32, dim3(grid), dim3(MCS_BLOCK), 0, ctx->stream, b->buf32, base, n,
+                                   b->seed, prm->instance, prm->init_vth);
+            MCS_LAUNCH_CHECK(ctx);
+        }
+    } else if (b->precision == MCS_PREC_F64) {
         MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    }
-    std::vector<double> tmp((size_t)(dev_init ? 0 : n));
-    for (int d = 0; d < 3 && !dev_init; ++d) {
-        for (int64_t i = 0; i < n; ++i) tmp[i] = x[3 * i + d];
-        MCS_HIP(ctx, hipMemcpy(b->buf.x[d] + base, tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-        for (int64_t i = 0; i < n; ++i) tmp[i] = v[3 * i + d];
-        MCS_HIP(ctx, hipMemcpy(b->buf.v[d] + base, tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    }
-    if (!dev_init) {
+        std::vector<double> tmp((size_t)n);
+        for (int d = 0; d < 3; ++d) {
+            for (int64_t i = 0; i < n; ++i) tmp[i] = x[3 * i + d];
+            MCS_HIP(ctx, hipMemcpy(b->buf.x[d] + base, tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+            for (int64_t i = 0; i < n; ++i) tmp[i] = v[3 * i + d];
+            MCS_HIP(ctx, hipMemcpy(b->buf.v[d] + base, tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        }
         MCS_HIP(ctx, hipMemcpy(b->buf.w + base, w, sizeof(double) * n, hipMemcpyHostToDevice));
         MCS_HIP(ctx, hipMemcpy(b->buf.s + base, s, sizeof(double) * n, hipMemcpyHostToDevice));
         MCS_HIP(ctx, hipMemcpy(b->buf.active + base, active, n, hipMemcpyHostToDevice));
         std::vector<uint64_t> ids((size_t)n);
         for (int64_t i = 0; i < n; ++i) ids[i] = (uint64_t)i;
         MCS_HIP(ctx, hipMemcpy(b->buf.id + base, ids.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice));
+    } else if (n > 0) {
+        MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        int32_t rc = ensure_stage(b, n);
+        if (rc) return rc;
+        double *dx = b->stage, *dv = dx + 3 * n, *dw = dv + 3 * n, *ds = dw + n;
+        uint8_t *da = (uint8_t *)(ds + n);
+        MCS_HIP(ctx, hipMemcpy(dx, x, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+        MCS_HIP(ctx, hipMemcpy(dv, v, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+        MCS_HIP(ctx, hipMemcpy(dw, w, sizeof(double) * n, hipMemcpyHostToDevice));
+        MCS_HIP(ctx, hipMemcpy(ds, s, sizeof(double) * n, hipMemcpyHostToDevice));
+        MCS_HIP(ctx, hipMemcpy(da, active, n, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_sw_from64, dim3(grid), dim3(MCS_BLOCK), 0, ctx->stream, b->buf32, base, n,
+                           (const double *)dx, (const double *)dv, (const double *)dw, (const double *)ds,
+                           (const uint8_t *)da);
+        MCS_LAUNCH_CHECK(ctx);
+        MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
     // slot state
-    SlotDev &S = b->S;
-    int32_t rc = 0;
-    rc |= put1(ctx, S.base, slot, base) | put1(ctx, S.cap, slot, cap) | put1(ctx, S.n, slot, n) |
-          put1(ctx, S.n0, slot, prm->n0);
-    rc |= put1(ctx, S.used, slot, (int32_t)1) | put1(ctx, S.flags, slot, (int32_t)0) |
-          put1(ctx, S.mode, slot, (int32_t)0) | put1(ctx, S.relax, slot, (int32_t)prm->relax_steps) |
-          put1(ctx, S.block, slot, (int32_t)prm->block_steps);
-    for (int d = 0; d < 3; ++d) rc |= put1(ctx, S.e, 3 * slot + d, prm->e[d]);
-    rc |= put1(ctx, S.dt, slot, prm->dt) | put1(ctx, S.tol, slot, prm->tol) | put1(ctx, S.param, slot, 0.0);
-    rc |= put1(ctx, S.key, slot, prm->instance) | put1(ctx, S.step, slot, (uint32_t)0) |
-          put1(ctx, S.next_id, slot, (uint64_t)n);
-    rc |= put1(ctx, S.tv, slot, table_view(t));
-    for (int q = 0; q < SW_NQ; ++q) {
-        int k = SW_NQ * slot + q;
-        rc |= put1(ctx, S.sum, k, 0.0) | put1(ctx, S.sumc, k, 0.0) | put1(ctx, S.cur, k, 0.0) |
-              put1(ctx, S.bm1, k, 0.0) | put1(ctx, S.bm2, k, 0.0) | put1(ctx, S.last, k, 0.0);
-    }
-    rc |= put1(ctx, S.steps, slot, (int64_t)0) | put1(ctx, S.nrec, slot, (int64_t)0) |
-          put1(ctx, S.nblk, slot, (int64_t)0) | put1(ctx, S.curcnt, slot, (int64_t)0) |
-          put1(ctx, S.nact, slot, (int64_t)0);
-    if (rc) return rc;
+    SlotInit p;
+    p.base = base;
+    p.cap = cap;
+    p.n = n;
+    p.n0 = prm->n0;
+    p.relax = prm->relax_steps;
+    p.block = prm->block_steps;
+    for (int d = 0; d < 3; ++d) p.e[d] = prm->e[d];
+    p.dt = prm->dt;
+    p.tol = prm->tol;
+    p.key = prm->instance;
+    p.next_id = (uint64_t)n;
+    p.tv = table_view(t);
+    p.tv32 = table_view32(t);
+    hipLaunchKernelGGL(k_sw_slot_init, dim3(1), dim3(64), 0, ctx->stream, b->S, slot, p);
+    MCS_LAUNCH_CHECK(ctx);
     HostSlot &h = b->slots[slot];
     h.used = true;
     h.base = base;
@@ -729,9 +1089,8 @@ extern "C" int32_t mcs_swarm_batch_remove(mcs_swarm_batch *b, int32_t slot)
     if (!b || slot < 0 || slot >= b->max_slots || !b->slots[slot].used) return MCS_E_ARG;
     mcs_ctx *ctx = b->ctx;
     hipSetDevice(ctx->device);
-    MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    int32_t rc = put1(ctx, b->S.used, slot, (int32_t)0);
-    if (rc) return rc;
+    hipLaunchKernelGGL(k_sw_slot_free, dim3(1), dim3(64), 0, ctx->stream, b->S, slot);
+    MCS_LAUNCH_CHECK(ctx);
     b->slots[slot].used = false;
     b->dirty = true;
     return MCS_OK;
@@ -746,16 +1105,13 @@ extern "C" int32_t mcs_swarm_batch_config(mcs_swarm_batch *b, int32_t block, int
     return MCS_OK;
 }
 
-static SwArgs args_of(mcs_swarm_batch *b)
+template <typename A>
+static void fill_common(mcs_swarm_batch *b, A &a)
 {
-    SwArgs a;
-    a.b = b->buf;
-    a.alt = b->alt;
     a.S = b->S;
     a.tile_slot = b->tile_slot;
     a.tile_start = b->tile_start;
     a.seed = b->seed;
-    a.L = b->L;
     a.log_count = b->log_count;
     a.loff = b->loff;
     a.cnt = b->cnt;
@@ -766,102 +1122,91 @@ static SwArgs args_of(mcs_swarm_batch *b)
     a.tile_act = b->tile_act;
     a.tile_apre = b->tile_apre;
     a.tile_part = b->tile_part;
+}
+
+static SwArgs64 args64(mcs_swarm_batch *b)
+{
+    SwArgs64 a;
+    a.b = b->buf;
+    a.alt = b->alt;
+    a.L = b->L;
+    fill_common(b, a);
     return a;
 }
 
-// The slot of a particle index: binary search over tile starts.
-__device__ __forceinline__ int64_t tile_of(const int64_t *tile_start, int64_t ntiles, int64_t i)
+static SwArgs32 args32(mcs_swarm_batch *b)
 {
-    int64_t lo = 0, hi = ntiles - 1;
-    while (lo < hi) {
-        int64_t mid = (lo + hi + 1) >> 1;
-        if (tile_start[mid] <= i) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
+    SwArgs32 a;
+    a.b = b->buf32;
+    a.alt = b->alt32;
+    a.L = b->L32;
+    fill_common(b, a);
+    return a;
 }
 
-// Children from the birth log to their slabs: rank within the slot in parent
-// order (tile offset + offset within the tile + sequence within the parent),
-// ids next_id + rank.  Grid-stride over the log with a fixed grid.
-__global__ __launch_bounds__(MCS_BLOCK) void k_sw_births(SwArgs a, int64_t logcap, int64_t ntiles)
+template <int BS>
+static void launch_advance(mcs_swarm_batch *b, const SwArgs64 &a, dim3 g)
 {
-    const int64_t cnt = *a.log_count < logcap ? *a.log_count : logcap;
-    for (int64_t r = gidx(); r < cnt; r += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = a.L.parent[r];
-        const int64_t tile = tile_of(a.tile_start, ntiles, i);
-        const int slot = a.tile_slot[tile];
-        const int64_t base = a.S.base[slot], cap = a.S.cap[slot], n = a.S.n[slot];
-        const int64_t f = a.S.first_tile[slot];
-        const int64_t rank = (a.tile_bpre[tile] - a.tile_bpre[f]) + a.loff[i] + a.L.seq[r];
-        if (n + rank >= cap) continue;
-        const int64_t d = base + n + rank;
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            a.b.x[q][d] = a.L.x[q][r];
-            a.b.v[q][d] = a.L.v[q][r];
-        }
-        a.b.w[d] = a.b.w[i];
-        a.b.s[d] = a.L.s[r];
-        a.b.active[d] = a.L.act[r];
-        a.b.id[d] = a.S.next_id[slot] + (uint64_t)rank;
-    }
+    hipLaunchKernelGGL(k_sw_advance<BS>, g, dim3(BS), 0, b->ctx->stream, a);
+}
+
+template <int BS>
+static void launch_advance(mcs_swarm_batch *b, const SwArgs32 &a, dim3 g)
+{
+    hipLaunchKernelGGL(k_sw_advance32<BS>, g, dim3(BS), 0, b->ctx->stream, a);
+}
+
+template <int BS, typename A>
+static int32_t launch_step_t(mcs_swarm_batch *b, const A &a)
+{
+    mcs_ctx *ctx = b->ctx;
+    const dim3 g((unsigned)b->ntiles);
+    hipLaunchKernelGGL(k_sw_zero, dim3(1), dim3(64), 0, ctx->stream, b->log_count, 8);
+    launch_advance<BS>(b, a, g);
+    int32_t rc = mcs_scan_i64_buf(ctx, b->tile_births, b->tile_bpre, b->ntiles, b->scan_scratch);
+    if (rc) return rc;
+    const int64_t logcap = b->precision == MCS_PREC_F64 ? b->L.cap : b->L32.cap;
+    hipLaunchKernelGGL(k_sw_births<A>, dim3((unsigned)std::min<int64_t>(cdiv(logcap, MCS_BLOCK), 2048)),
+                       dim3(MCS_BLOCK), 0, ctx->stream, a, logcap, b->ntiles);
+    hipLaunchKernelGGL(k_sw_slot_step<A>, dim3((unsigned)b->max_slots), dim3(MCS_BLOCK), 0, ctx->stream, a);
+    hipLaunchKernelGGL((k_sw_resample<BS, A>), g, dim3(BS), 0, ctx->stream, a);
+    rc = mcs_scan_i64_buf(ctx, b->tile_clones, b->tile_cpre, b->ntiles, b->scan_scratch);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_sw_clone_place<BS, A>), g, dim3(BS), 0, ctx->stream, a);
+    hipLaunchKernelGGL(k_sw_slot_step2<A>, dim3((unsigned)cdiv(b->max_slots, 64)), dim3(64), 0, ctx->stream, a,
+                       b->max_slots);
+    MCS_LAUNCH_CHECK(ctx);
+    return MCS_OK;
+}
+
+template <int BS, typename A>
+static int32_t launch_compact_t(mcs_swarm_batch *b, const A &a)
+{
+    mcs_ctx *ctx = b->ctx;
+    const dim3 g((unsigned)b->ntiles);
+    hipLaunchKernelGGL((k_sw_compact_count<BS, A>), g, dim3(BS), 0, ctx->stream, a);
+    int32_t rc = mcs_scan_i64_buf(ctx, b->tile_act, b->tile_apre, b->ntiles, b->scan_scratch);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_sw_compact_write<BS, A>), g, dim3(BS), 0, ctx->stream, a);
+    hipLaunchKernelGGL(k_sw_compact_slot<A>, dim3((unsigned)cdiv(b->max_slots, 64)), dim3(64), 0, ctx->stream,
+                       a, b->max_slots);
+    hipLaunchKernelGGL((k_sw_copy_back<BS, A>), g, dim3(BS), 0, ctx->stream, a);
+    MCS_LAUNCH_CHECK(ctx);
+    return MCS_OK;
 }
 
 static int32_t launch_step(mcs_swarm_batch *b)
 {
-    mcs_ctx *ctx = b->ctx;
-    SwArgs a = args_of(b);
-    const dim3 g((unsigned)b->ntiles);
-    hipLaunchKernelGGL(k_sw_zero, dim3(1), dim3(64), 0, ctx->stream, b->log_count, 8);
-    if (b->block == 128)
-        hipLaunchKernelGGL(k_sw_advance<128>, g, dim3(128), 0, ctx->stream, a);
-    else
-        hipLaunchKernelGGL(k_sw_advance<256>, g, dim3(256), 0, ctx->stream, a);
-    int32_t rc = mcs_scan_i64_buf(ctx, b->tile_births, b->tile_bpre, b->ntiles, b->scan_scratch);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_sw_births, dim3((unsigned)std::min<int64_t>(cdiv(b->L.cap, MCS_BLOCK), 2048)), dim3(MCS_BLOCK), 0,
-                       ctx->stream, a, b->L.cap, b->ntiles);
-    hipLaunchKernelGGL(k_sw_slot_step, dim3((unsigned)b->max_slots), dim3(MCS_BLOCK), 0, ctx->stream, a);
-    if (b->block == 128)
-        hipLaunchKernelGGL(k_sw_resample<128>, g, dim3(128), 0, ctx->stream, a);
-    else
-        hipLaunchKernelGGL(k_sw_resample<256>, g, dim3(256), 0, ctx->stream, a);
-    rc = mcs_scan_i64_buf(ctx, b->tile_clones, b->tile_cpre, b->ntiles, b->scan_scratch);
-    if (rc) return rc;
-    if (b->block == 128)
-        hipLaunchKernelGGL(k_sw_clone_place<128>, g, dim3(128), 0, ctx->stream, a);
-    else
-        hipLaunchKernelGGL(k_sw_clone_place<256>, g, dim3(256), 0, ctx->stream, a);
-    hipLaunchKernelGGL(k_sw_slot_step2, dim3((unsigned)cdiv(b->max_slots, 64)), dim3(64), 0, ctx->stream,
-                       a, b->max_slots);
-    MCS_LAUNCH_CHECK(ctx);
-    return MCS_OK;
+    if (b->precision == MCS_PREC_F64)
+        return b->block == 128 ? launch_step_t<128>(b, args64(b)) : launch_step_t<256>(b, args64(b));
+    return b->block == 128 ? launch_step_t<128>(b, args32(b)) : launch_step_t<256>(b, args32(b));
 }
 
 static int32_t launch_compact(mcs_swarm_batch *b)
 {
-    mcs_ctx *ctx = b->ctx;
-    SwArgs a = args_of(b);
-    const dim3 g((unsigned)b->ntiles);
-    if (b->block == 128)
-        hipLaunchKernelGGL(k_sw_compact_count<128>, g, dim3(128), 0, ctx->stream, a);
-    else
-        hipLaunchKernelGGL(k_sw_compact_count<256>, g, dim3(256), 0, ctx->stream, a);
-    int32_t rc = mcs_scan_i64_buf(ctx, b->tile_act, b->tile_apre, b->ntiles, b->scan_scratch);
-    if (rc) return rc;
-    if (b->block == 128)
-        hipLaunchKernelGGL(k_sw_compact_write<128>, g, dim3(128), 0, ctx->stream, a);
-    else
-        hipLaunchKernelGGL(k_sw_compact_write<256>, g, dim3(256), 0, ctx->stream, a);
-    hipLaunchKernelGGL(k_sw_compact_slot, dim3((unsigned)cdiv(b->max_slots, 64)), dim3(64), 0, ctx->stream,
-                       a, b->max_slots);
-    if (b->block == 128)
-        hipLaunchKernelGGL(k_sw_copy_back<128>, g, dim3(128), 0, ctx->stream, a);
-    else
-        hipLaunchKernelGGL(k_sw_copy_back<256>, g, dim3(256), 0, ctx->stream, a);
-    MCS_LAUNCH_CHECK(ctx);
-    return MCS_OK;
+    if (b->precision == MCS_PREC_F64)
+        return b->block == 128 ? launch_compact_t<128>(b, args64(b)) : launch_compact_t<256>(b, args64(b));
+    return b->block == 128 ? launch_compact_t<128>(b, args32(b)) : launch_compact_t<256>(b, args32(b));
 }
 
 static int32_t capture(mcs_swarm_batch *b, int32_t (*fn)(mcs_swarm_batch *), hipGraphExec_t *out)
@@ -907,33 +1252,14 @@ extern "C" int32_t mcs_swarm_batch_step(mcs_swarm_batch *b, int32_t nsteps)
     return MCS_OK;
 }
 
-static int32_t read_tally(mcs_swarm_batch *b, int32_t slot, mcs_swarm_tally *o)
-{
-    mcs_ctx *ctx = b->ctx;
-    const SlotDev &S = b->S;
-    memset(o, 0, sizeof(*o));
-    MCS_HIP(ctx, hipMemcpy(&o->steps, S.steps + slot, sizeof(int64_t), hipMemcpyDeviceToHost));
-    MCS_HIP(ctx, hipMemcpy(&o->n, S.n + slot, sizeof(int64_t), hipMemcpyDeviceToHost));
-    MCS_HIP(ctx, hipMemcpy(&o->nactive, S.nact + slot, sizeof(int64_t), hipMemcpyDeviceToHost));
-    MCS_HIP(ctx, hipMemcpy(&o->nrecorded, S.nrec + slot, sizeof(int64_t), hipMemcpyDeviceToHost));
-    MCS_HIP(ctx, hipMemcpy(&o->nblocks, S.nblk + slot, sizeof(int64_t), hipMemcpyDeviceToHost));
-    MCS_HIP(ctx, hipMemcpy(&o->flags, S.flags + slot, sizeof(int32_t), hipMemcpyDeviceToHost));
-    double sum[SW_NQ], sumc[SW_NQ];
-    MCS_HIP(ctx, hipMemcpy(sum, S.sum + SW_NQ * slot, sizeof(sum), hipMemcpyDeviceToHost));
-    MCS_HIP(ctx, hipMemcpy(sumc, S.sumc + SW_NQ * slot, sizeof(sumc), hipMemcpyDeviceToHost));
-    for (int q = 0; q < SW_NQ; ++q) o->sum[q] = sum[q] + sumc[q];
-    MCS_HIP(ctx, hipMemcpy(o->bm_s1, S.bm1 + SW_NQ * slot, sizeof(o->bm_s1), hipMemcpyDeviceToHost));
-    MCS_HIP(ctx, hipMemcpy(o->bm_s2, S.bm2 + SW_NQ * slot, sizeof(o->bm_s2), hipMemcpyDeviceToHost));
-    MCS_HIP(ctx, hipMemcpy(o->last, S.last + SW_NQ * slot, sizeof(o->last), hipMemcpyDeviceToHost));
-    return MCS_OK;
-}
-
 extern "C" int32_t mcs_swarm_batch_tally(mcs_swarm_batch *b, int32_t slot, mcs_swarm_tally *out)
 {
     if (!b || !out || slot < 0 || slot >= b->max_slots) return MCS_E_ARG;
-    hipSetDevice(b->ctx->device);
-    MCS_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
-    return read_tally(b, slot, out);
+    std::vector<mcs_swarm_tally> all((size_t)slot + 1);
+    int32_t rc = mcs_swarm_batch_tally_all(b, slot + 1, all.data());
+    if (rc) return rc;
+    *out = all[slot];
+    return MCS_OK;
 }
 
 extern "C" int32_t mcs_swarm_batch_tally_all(mcs_swarm_batch *b, int32_t nslots, mcs_swarm_tally *out)
@@ -942,13 +1268,12 @@ extern "C" int32_t mcs_swarm_batch_tally_all(mcs_swarm_batch *b, int32_t nslots,
     mcs_ctx *ctx = b->ctx;
     hipSetDevice(ctx->device);
     MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    // one bulk copy per array
+    if (nslots == 0) return MCS_OK;
     const SlotDev &S = b->S;
     std::vector<int64_t> steps(nslots), n(nslots), nact(nslots), nrec(nslots), nblk(nslots);
     std::vector<int32_t> flags(nslots);
     std::vector<double> sum(SW_NQ * nslots), sumc(SW_NQ * nslots), bm1(SW_NQ * nslots),
         bm2(SW_NQ * nslots), last(SW_NQ * nslots);
-    if (nslots == 0) return MCS_OK;
     MCS_HIP(ctx, hipMemcpy(steps.data(), S.steps, sizeof(int64_t) * nslots, hipMemcpyDeviceToHost));
     MCS_HIP(ctx, hipMemcpy(n.data(), S.n, sizeof(int64_t) * nslots, hipMemcpyDeviceToHost));
     MCS_HIP(ctx, hipMemcpy(nact.data(), S.nact, sizeof(int64_t) * nslots, hipMemcpyDeviceToHost));
@@ -993,6 +1318,22 @@ extern "C" int32_t mcs_swarm_batch_download(mcs_swarm_batch *b, int32_t slot, in
     int64_t m = n < nmax ? n : nmax;
     if (m <= 0) return MCS_OK;
     const int64_t base = b->slots[slot].base;
+    if (b->precision == MCS_PREC_F32) {
+        int32_t rc = ensure_stage(b, m);
+        if (rc) return rc;
+        double *dx = b->stage, *dv = dx + 3 * m, *dw = dv + 3 * m, *ds = dw + m;
+        hipLaunchKernelGGL(k_sw_to64, dim3((unsigned)cdiv(m, MCS_BLOCK)), dim3(MCS_BLOCK), 0, ctx->stream,
+                           b->buf32, base, m, dx, dv, dw, ds);
+        MCS_LAUNCH_CHECK(ctx);
+        MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (x) MCS_HIP(ctx, hipMemcpy(x, dx, sizeof(double) * 3 * m, hipMemcpyDeviceToHost));
+        if (v) MCS_HIP(ctx, hipMemcpy(v, dv, sizeof(double) * 3 * m, hipMemcpyDeviceToHost));
+        if (w) MCS_HIP(ctx, hipMemcpy(w, dw, sizeof(double) * m, hipMemcpyDeviceToHost));
+        if (s) MCS_HIP(ctx, hipMemcpy(s, ds, sizeof(double) * m, hipMemcpyDeviceToHost));
+        if (active) MCS_HIP(ctx, hipMemcpy(active, b->buf32.active + base, m, hipMemcpyDeviceToHost));
+        if (id) MCS_HIP(ctx, hipMemcpy(id, b->buf32.id + base, sizeof(uint64_t) * m, hipMemcpyDeviceToHost));
+        return MCS_OK;
+    }
     std::vector<double> tmp((size_t)m);
     for (int d = 0; d < 3; ++d) {
         if (x) {

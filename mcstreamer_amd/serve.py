@@ -140,11 +140,11 @@ class SwarmServer:
     """Continuous batching of swarm requests in one SwarmBatch."""
 
     def __init__(self, total_capacity, max_slots=4096, seed=0, policy=None, tables=None,
-                 block=256, ctx=None):
+                 block=256, precision="f64", ctx=None):
         self.ctx = ctx or context()
         self.policy = policy or Policy()
         self.tables = tables or GasTables(ctx=self.ctx)
-        self.batch = SwarmBatch(total_capacity, max_slots, seed=seed, ctx=self.ctx)
+        self.batch = SwarmBatch(total_capacity, max_slots, seed=seed, precision=precision, ctx=self.ctx)
         self.batch.configure(block=block, compact_every=self.policy.compact_every)
         self.total_capacity = int(total_capacity)
         self.max_slots = int(max_slots)

@@ -101,12 +101,14 @@ class SwarmBatch:
     `compact_every` steps each slab is compacted (stable, actives only).
     """
 
-    def __init__(self, total_capacity, max_slots, seed=0, ctx=None):
+    def __init__(self, total_capacity, max_slots, seed=0, precision="f64", ctx=None):
         self.ctx = ctx or context()
         self.seed = int(seed)
+        self.precision = precision
+        prec = {"f64": _lib.PREC_F64, "f32": _lib.PREC_F32}[precision]
         h = ctypes.c_void_p()
         check(self.ctx.handle, _lib.lib().mcs_swarm_batch_create(
-            self.ctx.handle, int(total_capacity), int(max_slots), self.seed, ctypes.byref(h)))
+            self.ctx.handle, int(total_capacity), int(max_slots), self.seed, prec, ctypes.byref(h)))
         self.handle = h
         self.max_slots = int(max_slots)
         self._tables = {}

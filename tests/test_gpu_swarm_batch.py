@@ -184,3 +184,54 @@ def test_device_thermal_init_matches_oracle():
     np.testing.assert_array_equal(p.v, ref["v"])
     np.testing.assert_array_equal(p.s, ref["s"])
     assert p.next_id == 3000
+
+
+def _constant_nu_table(nu, ne=64, emax_ev=1e5):
+    from mcstreamer_amd.lxcat import Process
+    e = np.linspace(0.0, emax_ev, ne) * co.eV
+    return jumc.CollisionTable([Process("ELASTIC", 0.0)], e, np.full((1, ne), nu), nu, egrid="uniform")
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_constant_frequency_drift_closed_form(precision):
+    # elastic scattering at constant nu, zero mass ratio: <v_z> -> qE/(m nu),
+    # here time-averaged over steps after relaxation, in both store precisions
+    nu, E, dt = 2e12, 2e5, 1e-12
+    t = _constant_nu_table(nu)
+    batch = SwarmBatch(total_capacity=200_000, max_slots=4, seed=SEED, precision=precision)
+    n = 20000
+    slots = [batch.add(t, n, (0.0, 0.0, E), dt, n, capacity=40_000, instance=k, relax_steps=20,
+                       block_steps=20, init_kT_ev=0.1) for k in range(2)]
+    batch.step(220)
+    W = -co.elementary_charge * E / (co.electron_mass * nu)
+    for slot in slots:
+        est = tally_estimates(batch.tally(slot))
+        mean, err = est["drift_velocity"]
+        print(f"{precision} slot {slot}: W {mean:.6e} analytic {W:.6e} rel err {err:.2e}", flush=True)
+        assert batch.tally(slot).flags == 0
+        assert abs(mean - W) < 5 * err * abs(W) + 2e-3 * abs(W)
+
+
+def test_f32_transport_matches_f64_within_error_bars():
+    # the split-precision store gives the same transport coefficients as the
+    # FP64 one within the statistical errors (independent random streams)
+    tabs = _tables()
+    res = {}
+    for prec in ("f64", "f32"):
+        batch = SwarmBatch(total_capacity=400_000, max_slots=4, seed=SEED, precision=prec)
+        gt = {o2: jumc.CollisionTable(*tabs[o2], egrid="log") for o2 in (0.0, 0.3)}
+        slots = []
+        for k, (en, o2) in enumerate(((100.0, 0.0), (500.0, 0.3))):
+            E = en * co.Td * gases.N_GAS
+            slots.append(batch.add(gt[o2], 8000, (0.0, 0.0, -E), 1e-12 * 100 / en, 8000, capacity=60_000,
+                                   instance=50 + k, relax_steps=300, block_steps=50))
+        batch.step(1300)
+        res[prec] = [tally_estimates(batch.tally(s)) for s in slots]
+        assert all(batch.tally(s).flags == 0 for s in slots)
+    for k in range(2):
+        for name in ("drift_velocity", "mean_energy", "ionization_rate"):
+            (a, ea), (b, eb) = res["f64"][k][name], res["f32"][k][name]
+            print(f"instance {k} {name}: f64 {a:.5e} (+-{ea:.1e}) f32 {b:.5e} (+-{eb:.1e})", flush=True)
+            if not np.isfinite(ea) or a == 0:
+                continue
+            assert abs(a - b) <= 5 * np.hypot(ea * abs(a), eb * abs(b)) + 2e-3 * abs(a), name
