@@ -82,54 +82,59 @@ __device__ __forceinline__ void advance_one(double x[3], double v[3], const doub
             double xi = mcs_u01(&rs) * maxrate;
             const double *r0 = T.rate + k * T.nproc;
             const double *r1 = r0 + T.nproc;
+            // the selected row first (do_one_collision!), then its outcome
+            int sel = -1;
             for (int j = 0; j < T.nproc; ++j) {
-                double nu = wg * r0[j] + (1.0 - wg) * r1[j];
+                double nu = wg * gld(r0 + j) + (1.0 - wg) * gld(r1 + j);
                 if (nu > xi) {
-                    event(j);
-                    int kind = T.kind[j];
-                    if (kind == MCS_PROC_ELASTIC) {
-                        double mr = T.param[j];
-                        double c = mr / (1.0 + mr);
-                        double vcm0 = c * v[0], vcm1 = c * v[1], vcm2 = c * v[2];
-                        double d0 = v[0] - vcm0, d1 = v[1] - vcm1, d2 = v[2] - vcm2;
-                        double nrm = sqrt((d0 * d0 + d1 * d1) + d2 * d2);
-                        double om[3];
-                        mcs_randsphere(&rs, om);
-                        v[0] = om[0] * nrm + vcm0;
-                        v[1] = om[1] * nrm + vcm1;
-                        v[2] = om[2] * nrm + vcm2;
-                    } else if (kind == MCS_PROC_EXCITATION) {
-                        double e1 = eng - T.param[j];
-                        if (e1 < 0.0) e1 = 0.0;
-                        double vabs = sqrt((2.0 * e1) / MCS_ELECTRON_MASS);
-                        double om[3];
-                        mcs_randsphere(&rs, om);
-                        v[0] = om[0] * vabs;
-                        v[1] = om[1] * vabs;
-                        v[2] = om[2] * vabs;
-                    } else if (kind == MCS_PROC_IONIZATION) {
-                        double e1 = eng - T.param[j];
-                        if (e1 < 0.0) e1 = 0.0;
-                        e1 = e1 / 2.0;
-                        double vabs = sqrt((2.0 * e1) / MCS_ELECTRON_MASS);
-                        double om1[3], om2[3];
-                        mcs_randsphere(&rs, om1);
-                        mcs_randsphere(&rs, om2);
-                        v[0] = om1[0] * vabs;
-                        v[1] = om1[1] * vabs;
-                        v[2] = om1[2] * vabs;
-                        double sc = mcs_nextcoll(&rs);
-                        if (TRACK) track_dev(M, x, w);
-                        double vc[3] = {om2[0] * vabs, om2[1] * vabs, om2[2] * vabs};
-                        birth(x, vc, sc, act);
-                    } else if (kind == MCS_PROC_ATTACHMENT) {
-                        if (TRACK) track_dev(M, x, -w);
-                        if (act) deaths += 1;
-                        act = 0;
-                    }
+                    sel = j;
                     break;
                 }
                 xi = xi - nu;
+            }
+            if (sel >= 0) {
+                event(sel);
+                int kind = gld(T.kind + sel);
+                if (kind == MCS_PROC_ELASTIC) {
+                    double mr = gld(T.param + sel);
+                    double c = mr / (1.0 + mr);
+                    double vcm0 = c * v[0], vcm1 = c * v[1], vcm2 = c * v[2];
+                    double d0 = v[0] - vcm0, d1 = v[1] - vcm1, d2 = v[2] - vcm2;
+                    double nrm = sqrt((d0 * d0 + d1 * d1) + d2 * d2);
+                    double om[3];
+                    mcs_randsphere(&rs, om);
+                    v[0] = om[0] * nrm + vcm0;
+                    v[1] = om[1] * nrm + vcm1;
+                    v[2] = om[2] * nrm + vcm2;
+                } else if (kind == MCS_PROC_EXCITATION) {
+                    double e1 = eng - gld(T.param + sel);
+                    if (e1 < 0.0) e1 = 0.0;
+                    double vabs = sqrt((2.0 * e1) / MCS_ELECTRON_MASS);
+                    double om[3];
+                    mcs_randsphere(&rs, om);
+                    v[0] = om[0] * vabs;
+                    v[1] = om[1] * vabs;
+                    v[2] = om[2] * vabs;
+                } else if (kind == MCS_PROC_IONIZATION) {
+                    double e1 = eng - gld(T.param + sel);
+                    if (e1 < 0.0) e1 = 0.0;
+                    e1 = e1 / 2.0;
+                    double vabs = sqrt((2.0 * e1) / MCS_ELECTRON_MASS);
+                    double om1[3], om2[3];
+                    mcs_randsphere(&rs, om1);
+                    mcs_randsphere(&rs, om2);
+                    v[0] = om1[0] * vabs;
+                    v[1] = om1[1] * vabs;
+                    v[2] = om1[2] * vabs;
+                    double sc = mcs_nextcoll(&rs);
+                    if (TRACK) track_dev(M, x, w);
+                    double vc[3] = {om2[0] * vabs, om2[1] * vabs, om2[2] * vabs};
+                    birth(x, vc, sc, act);
+                } else if (kind == MCS_PROC_ATTACHMENT) {
+                    if (TRACK) track_dev(M, x, -w);
+                    if (act) deaths += 1;
+                    act = 0;
+                }
             }
             s = mcs_nextcoll(&rs);
         }

@@ -289,6 +289,7 @@ extern "C" int32_t mcs_table_create(mcs_ctx *ctx, int32_t nproc, int32_t ne, int
     t->egrid = egrid_kind;
     t->maxrate = maxrate;
     t->e0 = energy[0];
+    t->emax = energy[ne - 1];
     t->step = (energy[ne - 1] - energy[0]) / (double)(ne - 1);
     t->inv_dlog = egrid_kind == MCS_EGRID_LOG
                       ? (double)(ne - 1) / mcs_log(energy[ne - 1] / energy[0])

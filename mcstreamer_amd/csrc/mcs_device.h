@@ -3,6 +3,14 @@
 
 #include "mcs_internal.h"
 
+// Loads through the global address space (table pointers that arrive inside
+// structs read from memory would otherwise be loaded with flat instructions).
+template <typename T>
+__device__ __forceinline__ T gld(const T *p)
+{
+    return *(const __attribute__((address_space(1))) T *)p;
+}
+
 struct TableView {
     const double *energy;
     const double *rate;
@@ -60,33 +68,34 @@ __device__ __forceinline__ void indweight_dev(const double *energy, int32_t ne, 
                                               double step, double inv_dlog, double eng,
                                               int64_t &k, double &w, int32_t &flags)
 {
-    if (eng >= energy[ne - 1]) {
+    if (eng >= gld(energy + (ne - 1))) {
         k = ne - 2;
         w = 0.0;
         flags |= MCS_FLAG_TABLE_RANGE;
         return;
     }
     if (egrid == MCS_EGRID_UNIFORM) {
-        double kf = floor((eng - energy[0]) / step);
+        double kf = floor((eng - gld(energy)) / step);
         if (kf < 0.0) kf = 0.0;
         int64_t kk = (int64_t)kf;
         if (kk > ne - 2) kk = ne - 2;
-        double ww = (energy[kk + 1] - eng) / step;
+        double ww = (gld(energy + kk + 1) - eng) / step;
         k = kk;
         w = fmin(fmax(ww, 0.0), 1.0);
     } else {
-        if (eng <= energy[0]) {
+        if (eng <= gld(energy)) {
             k = 0;
             w = 1.0;
             return;
         }
-        double kf = floor(mcs_log(eng / energy[0]) * inv_dlog);
+        double kf = floor(mcs_log(eng / gld(energy)) * inv_dlog);
         if (kf < 0.0) kf = 0.0;
         if (kf > (double)(ne - 2)) kf = (double)(ne - 2);
         int64_t kk = (int64_t)kf;
-        while (kk > 0 && energy[kk] > eng) kk -= 1;
-        while (kk < ne - 2 && energy[kk + 1] <= eng) kk += 1;
-        double ww = (energy[kk + 1] - eng) / (energy[kk + 1] - energy[kk]);
+        while (kk > 0 && gld(energy + kk) > eng) kk -= 1;
+        while (kk < ne - 2 && gld(energy + kk + 1) <= eng) kk += 1;
+        const double e1 = gld(energy + kk + 1), e0 = gld(energy + kk);
+        double ww = (e1 - eng) / (e1 - e0);
         k = kk;
         w = fmin(fmax(ww, 0.0), 1.0);
     }

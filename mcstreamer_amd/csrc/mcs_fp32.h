@@ -43,8 +43,14 @@ struct TableView32 {
     const int32_t *kind;
     const float *param;
     int32_t nproc, ne, egrid;
-    float e0, inv_step, inv_dlog, maxrate, inv_maxrate;
+    float e0, emax, inv_e0, inv_step, inv_dlog2, maxrate, inv_maxrate;
+    uint64_t kindbits;   // 4 bits per process and bit 63 set (nproc <= 15), else 0: kind[] is read
 };
+
+__device__ __forceinline__ int proc_kind(const TableView32 &T, int j)
+{
+    return T.kindbits ? (int)((T.kindbits >> (4 * j)) & 15u) : gld(T.kind + j);
+}
 
 static inline TableView32 table_view32(const mcs_table *t)
 {
@@ -57,10 +63,17 @@ static inline TableView32 table_view32(const mcs_table *t)
     v.ne = t->ne;
     v.egrid = t->egrid;
     v.e0 = (float)t->e0;
+    v.emax = (float)t->emax;
+    v.inv_e0 = (float)(1.0 / t->e0);
     v.inv_step = (float)(1.0 / t->step);
-    v.inv_dlog = (float)t->inv_dlog;
+    v.inv_dlog2 = (float)(t->inv_dlog * 0.6931471805599453);
     v.maxrate = (float)t->maxrate;
     v.inv_maxrate = (float)(1.0 / t->maxrate);
+    v.kindbits = 0;
+    if (t->nproc <= 15) {
+        for (int j = 0; j < t->nproc; ++j) v.kindbits |= (uint64_t)(t->h_kind[j] & 15) << (4 * j);
+        v.kindbits |= (uint64_t)1 << 63;   // marks the bits valid (kind 0 rows give no bits)
+    }
     return v;
 }
 
@@ -83,33 +96,41 @@ __device__ __forceinline__ float energy32(const float v[3])
 }
 
 // indweight (JuMC/src/util.jl:23-30) in FP32: node k with energy[k] <= eng <
-// energy[k+1] (the float nodes), weight of node k.
+// energy[k+1] (the float nodes), weight of node k.  The node index comes from
+// the grid formula (hardware log2 on the log grid), then is corrected by one
+// node against the stored nodes; the general search runs only if that fails.
 __device__ __forceinline__ void indweight32(const TableView32 &T, float eng, int32_t &k, float &w,
                                             int32_t &flags)
 {
     const float *E = T.energy;
-    if (eng >= E[T.ne - 1]) {
+    if (eng >= T.emax) {
         k = T.ne - 2;
         w = 0.0f;
         flags |= MCS_FLAG_TABLE_RANGE;
         return;
     }
-    int32_t kk;
+    float kf;
     if (T.egrid == MCS_EGRID_UNIFORM) {
-        kk = (int32_t)floorf((eng - T.e0) * T.inv_step);
+        kf = floorf((eng - T.e0) * T.inv_step);
     } else {
         if (eng <= T.e0) {
             k = 0;
             w = 1.0f;
             return;
         }
-        kk = (int32_t)floorf(__logf(eng / T.e0) * T.inv_dlog);
+        kf = floorf(__builtin_amdgcn_logf(eng * T.inv_e0) * T.inv_dlog2);
     }
-    kk = kk < 0 ? 0 : (kk > T.ne - 2 ? T.ne - 2 : kk);
-    while (kk > 0 && E[kk] > eng) kk -= 1;
-    while (kk < T.ne - 2 && E[kk + 1] <= eng) kk += 1;
-    const float e1 = E[kk + 1], e0 = E[kk];
-    float ww = (e1 - eng) / (e1 - e0);
+    int32_t kk = (int32_t)fminf(fmaxf(kf, 0.0f), (float)(T.ne - 2));
+    float e0 = gld(E + kk), e1 = gld(E + kk + 1);
+    if ((e0 > eng) | (e1 <= eng)) {
+        kk = e0 > eng ? kk - 1 : kk + 1;
+        kk = kk < 0 ? 0 : (kk > T.ne - 2 ? T.ne - 2 : kk);
+        while (kk > 0 && gld(E + kk) > eng) kk -= 1;
+        while (kk < T.ne - 2 && gld(E + kk + 1) <= eng) kk += 1;
+        e0 = gld(E + kk);
+        e1 = gld(E + kk + 1);
+    }
+    const float ww = (e1 - eng) * __builtin_amdgcn_rcpf(e1 - e0);
     k = kk;
     w = fminf(fmaxf(ww, 0.0f), 1.0f);
 }
@@ -117,7 +138,7 @@ __device__ __forceinline__ void indweight32(const TableView32 &T, float eng, int
 // nextcoll() = -log(rand()) from a 24-bit uniform; 1 - u >= 2^-24.
 __device__ __forceinline__ float nextcoll32(mcs_stream *st)
 {
-    return -__logf(1.0f - mcs_u01f(st));
+    return -0.6931471805599453f * __builtin_amdgcn_logf(1.0f - mcs_u01f(st));
 }
 
 // randsphere() (util.jl:4-12) with phi drawn first; the hardware sine and cosine
@@ -293,51 +314,56 @@ __device__ __forceinline__ void advance_one32(int32_t c[3], float off[3], float 
             float xi = mcs_u01f(&rs) * maxrate;
             const float *r0 = T.rate + k * T.nproc;
             const float *r1 = r0 + T.nproc;
+            // the selected row first (do_one_collision!), then its outcome
+            int sel = -1;
             for (int j = 0; j < T.nproc; ++j) {
-                const float nu = wg * r0[j] + (1.0f - wg) * r1[j];
+                const float nu = wg * gld(r0 + j) + (1.0f - wg) * gld(r1 + j);
                 if (nu > xi) {
-                    event(j);
-                    const int kind = T.kind[j];
-                    if (kind == MCS_PROC_ELASTIC) {
-                        const float mr = T.param[j];
-                        const float cm = mr / (1.0f + mr);
-                        const float vc0 = cm * v[0], vc1 = cm * v[1], vc2 = cm * v[2];
-                        const float e0 = v[0] - vc0, e1 = v[1] - vc1, e2 = v[2] - vc2;
-                        const float nrm = sqrtf((e0 * e0 + e1 * e1) + e2 * e2);
-                        float om[3];
-                        randsphere32(&rs, om);
-                        v[0] = om[0] * nrm + vc0;
-                        v[1] = om[1] * nrm + vc1;
-                        v[2] = om[2] * nrm + vc2;
-                    } else if (kind == MCS_PROC_EXCITATION) {
-                        const float e1 = fmaxf(eng - T.param[j], 0.0f);
-                        const float vabs = sqrtf((2.0f * e1) / MCS_ELECTRON_MASS_F);
-                        float om[3];
-                        randsphere32(&rs, om);
-                        v[0] = om[0] * vabs;
-                        v[1] = om[1] * vabs;
-                        v[2] = om[2] * vabs;
-                    } else if (kind == MCS_PROC_IONIZATION) {
-                        const float e1 = fmaxf(eng - T.param[j], 0.0f) * 0.5f;
-                        const float vabs = sqrtf((2.0f * e1) / MCS_ELECTRON_MASS_F);
-                        float om1[3], om2[3];
-                        randsphere32(&rs, om1);
-                        randsphere32(&rs, om2);
-                        v[0] = om1[0] * vabs;
-                        v[1] = om1[1] * vabs;
-                        v[2] = om1[2] * vabs;
-                        const float sc = nextcoll32(&rs);
-                        if (TRACK) track32(M, c, off, (double)w);
-                        const float vc[3] = {om2[0] * vabs, om2[1] * vabs, om2[2] * vabs};
-                        birth(c, off, vc, sc, act);
-                    } else if (kind == MCS_PROC_ATTACHMENT) {
-                        if (TRACK) track32(M, c, off, -(double)w);
-                        if (act) deaths += 1;
-                        act = 0;
-                    }
+                    sel = j;
                     break;
                 }
                 xi = xi - nu;
+            }
+            if (sel >= 0) {
+                event(sel);
+                const int kind = proc_kind(T, sel);
+                if (kind == MCS_PROC_ELASTIC) {
+                    const float mr = gld(T.param + sel);
+                    const float cm = mr / (1.0f + mr);
+                    const float vc0 = cm * v[0], vc1 = cm * v[1], vc2 = cm * v[2];
+                    const float e0 = v[0] - vc0, e1 = v[1] - vc1, e2 = v[2] - vc2;
+                    const float nrm = sqrtf((e0 * e0 + e1 * e1) + e2 * e2);
+                    float om[3];
+                    randsphere32(&rs, om);
+                    v[0] = om[0] * nrm + vc0;
+                    v[1] = om[1] * nrm + vc1;
+                    v[2] = om[2] * nrm + vc2;
+                } else if (kind == MCS_PROC_EXCITATION) {
+                    const float e1 = fmaxf(eng - gld(T.param + sel), 0.0f);
+                    const float vabs = sqrtf((2.0f * e1) / MCS_ELECTRON_MASS_F);
+                    float om[3];
+                    randsphere32(&rs, om);
+                    v[0] = om[0] * vabs;
+                    v[1] = om[1] * vabs;
+                    v[2] = om[2] * vabs;
+                } else if (kind == MCS_PROC_IONIZATION) {
+                    const float e1 = fmaxf(eng - gld(T.param + sel), 0.0f) * 0.5f;
+                    const float vabs = sqrtf((2.0f * e1) / MCS_ELECTRON_MASS_F);
+                    float om1[3], om2[3];
+                    randsphere32(&rs, om1);
+                    randsphere32(&rs, om2);
+                    v[0] = om1[0] * vabs;
+                    v[1] = om1[1] * vabs;
+                    v[2] = om1[2] * vabs;
+                    const float sc = nextcoll32(&rs);
+                    if (TRACK) track32(M, c, off, (double)w);
+                    const float vc[3] = {om2[0] * vabs, om2[1] * vabs, om2[2] * vabs};
+                    birth(c, off, vc, sc, act);
+                } else if (kind == MCS_PROC_ATTACHMENT) {
+                    if (TRACK) track32(M, c, off, -(double)w);
+                    if (act) deaths += 1;
+                    act = 0;
+                }
             }
             s = nextcoll32(&rs);
         }

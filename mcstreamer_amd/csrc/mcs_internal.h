@@ -93,6 +93,7 @@ struct mcs_table {
     int32_t nproc, ne, egrid;
     double maxrate;
     double e0;        // first energy node
+    double emax;      // last energy node
     double inv_dlog;
     double step;
     double *energy;   // ne

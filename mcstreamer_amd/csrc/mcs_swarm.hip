@@ -161,9 +161,9 @@ __device__ __forceinline__ void tally_add(Neu2 acc[SW_NT], const double v[3], co
     const double *r1 = r0 + T.nproc;
     double ion = 0.0, att = 0.0;
     for (int j = 0; j < T.nproc; ++j) {
-        int kd = T.kind[j];
-        if (kd == MCS_PROC_IONIZATION) ion += w * r0[j] + (1.0 - w) * r1[j];
-        else if (kd == MCS_PROC_ATTACHMENT) att += w * r0[j] + (1.0 - w) * r1[j];
+        int kd = gld(T.kind + j);
+        if (kd == MCS_PROC_IONIZATION) ion += w * gld(r0 + j) + (1.0 - w) * gld(r1 + j);
+        else if (kd == MCS_PROC_ATTACHMENT) att += w * gld(r0 + j) + (1.0 - w) * gld(r1 + j);
     }
     nadd(acc[0], 1.0);
     nadd(acc[1], v[2]);
@@ -185,9 +185,9 @@ __device__ __forceinline__ void tally_add32(Neu2 acc[SW_NT], const float v[3], c
     const float *r1 = r0 + T.nproc;
     float ion = 0.0f, att = 0.0f;
     for (int j = 0; j < T.nproc; ++j) {
-        int kd = T.kind[j];
-        if (kd == MCS_PROC_IONIZATION) ion += w * r0[j] + (1.0f - w) * r1[j];
-        else if (kd == MCS_PROC_ATTACHMENT) att += w * r0[j] + (1.0f - w) * r1[j];
+        const int kd = proc_kind(T, j);
+        if (kd == MCS_PROC_IONIZATION) ion += w * gld(r0 + j) + (1.0f - w) * gld(r1 + j);
+        else if (kd == MCS_PROC_ATTACHMENT) att += w * gld(r0 + j) + (1.0f - w) * gld(r1 + j);
     }
     const double vd[3] = {(double)v[0], (double)v[1], (double)v[2]};
     nadd(acc[0], 1.0);
