@@ -164,6 +164,7 @@ struct mcs_mg {
     mcs_mg_config cfg;
     std::vector<MgLevel> lv;
     double *ainv = nullptr;   // coarse inverse, ncoarse^2
+    float *ainv32 = nullptr;  // its FP32 copy (precision mode)
     int64_t ncoarse = 0;
     double *u_tmp = nullptr, *b_tmp = nullptr;  // host-solve staging (finest size)
     double *partial = nullptr;                   // norm partial sums

@@ -14,6 +14,11 @@
 // memory only where a later operation reads them: before the last smoothing
 // sweep of a level (the prolongation into the finer level reads them), before
 // the coarse-to-fine prolongation of fmg!, and at the start and end of fmg!.
+//
+// Precision mode (MCS_PREC_F32): the finest residual and the update x += z stay
+// FP64; the correction equation L z + r = 0 of each fmg! pass (restrictions,
+// coarse solve, V-cycles) runs on an FP32 copy of the hierarchy.  Every stencil
+// body is a template on the element type, so the FP64 path is unchanged.
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -31,6 +36,11 @@ struct Lvl {
 struct BcCoef {
     double lo[3], hi[3];
 };
+
+// x / 6 and (4 sum) / 8: rounded divisions in FP64 (the restatement's order),
+// multiplications by the exact reciprocal in FP32.
+__device__ __forceinline__ double div6(double x) { return x / 6.0; }
+__device__ __forceinline__ float div6(float x) { return x * (1.0f / 6.0f); }
 
 __device__ __forceinline__ int64_t g_index(const Lvl &L, int64_t i, int64_t j, int64_t k)
 {
@@ -62,18 +72,18 @@ __device__ __forceinline__ void split3(int64_t t64, uint32_t nx, uint32_t ny, in
 
 // L u at interior cell (i, j, k) with folded ghosts, in lplstencil order
 // (+x, +y, +z, -x, -y, -z; Multigrid.jl:124-130,152-158).
-__device__ __forceinline__ double lap_fold(const double *__restrict__ u, const Lvl &L,
-                                          const BcCoef &bc, int64_t i, int64_t j, int64_t k,
-                                          int64_t c, double uc)
+template <typename T>
+__device__ __forceinline__ T lap_fold(const T *__restrict__ u, const Lvl &L, const BcCoef &bc, int64_t i,
+                                     int64_t j, int64_t k, int64_t c, T uc)
 {
     const int64_t sy = L.n[0] + 2, sz = (int64_t)(L.n[0] + 2) * (L.n[1] + 2);
-    double l = -6.0 * uc;
-    l += (i + 1 < L.n[0]) ? u[c + 1] : bc.hi[0] * uc;
-    l += (j + 1 < L.n[1]) ? u[c + sy] : bc.hi[1] * uc;
-    l += (k + 1 < L.n[2]) ? u[c + sz] : bc.hi[2] * uc;
-    l += (i > 0) ? u[c - 1] : bc.lo[0] * uc;
-    l += (j > 0) ? u[c - sy] : bc.lo[1] * uc;
-    l += (k > 0) ? u[c - sz] : bc.lo[2] * uc;
+    T l = T(-6) * uc;
+    l += (i + 1 < L.n[0]) ? u[c + 1] : (T)bc.hi[0] * uc;
+    l += (j + 1 < L.n[1]) ? u[c + sy] : (T)bc.hi[1] * uc;
+    l += (k + 1 < L.n[2]) ? u[c + sz] : (T)bc.hi[2] * uc;
+    l += (i > 0) ? u[c - 1] : (T)bc.lo[0] * uc;
+    l += (j > 0) ? u[c - sy] : (T)bc.lo[1] * uc;
+    l += (k > 0) ? u[c - sz] : (T)bc.lo[2] * uc;
     return l;
 }
 
@@ -81,38 +91,40 @@ __device__ __forceinline__ double lap_fold(const double *__restrict__ u, const L
 
 // One cell of one colour of gauss_seidel! (Multigrid.jl:168-178, redblack.jl:49-61):
 // colour par holds cells with (i+j+k) % 2 == par; each row has nx/2 of them.
-__device__ __forceinline__ void gs_item(double *u, const double *b, const Lvl &L, const BcCoef &bc,
-                                        int par, int64_t t)
+template <typename T>
+__device__ __forceinline__ void gs_item(T *u, const T *b, const Lvl &L, const BcCoef &bc, int par, int64_t t)
 {
     int64_t ti, j, k;
     split3(t, (uint32_t)(L.n[0] / 2), (uint32_t)L.n[1], ti, j, k);
     int64_t i = 2 * ti + ((j + k + par) & 1);
     int64_t c = g_index(L, i + 1, j + 1, k + 1);
-    double uc = u[c];
-    double l = lap_fold(u, L, bc, i, j, k, c, uc);
-    u[c] = uc + (1.0 * (l + b[c])) / 6.0;
+    T uc = u[c];
+    T l = lap_fold(u, L, bc, i, j, k, c, uc);
+    u[c] = uc + div6(T(1) * (l + b[c]));
 }
 
 // Two cells of one colour at once: both cells' operands are loaded before either
 // is stored (within a colour every neighbour read belongs to the other colour).
-__device__ __forceinline__ void gs_item2(double *u, const double *b, const Lvl &L, const BcCoef &bc,
-                                         int par, int64_t t0, int64_t t1, bool has1)
+template <typename T>
+__device__ __forceinline__ void gs_item2(T *u, const T *b, const Lvl &L, const BcCoef &bc, int par, int64_t t0,
+                                         int64_t t1, bool has1)
 {
     int64_t ti0, j0, k0, ti1, j1, k1;
     split3(t0, (uint32_t)(L.n[0] / 2), (uint32_t)L.n[1], ti0, j0, k0);
     split3(has1 ? t1 : t0, (uint32_t)(L.n[0] / 2), (uint32_t)L.n[1], ti1, j1, k1);
     int64_t i0 = 2 * ti0 + ((j0 + k0 + par) & 1), i1 = 2 * ti1 + ((j1 + k1 + par) & 1);
     int64_t c0 = g_index(L, i0 + 1, j0 + 1, k0 + 1), c1 = g_index(L, i1 + 1, j1 + 1, k1 + 1);
-    double u0 = u[c0], u1 = u[c1], b0 = b[c0], b1 = b[c1];
-    double l0 = lap_fold(u, L, bc, i0, j0, k0, c0, u0);
-    double l1 = lap_fold(u, L, bc, i1, j1, k1, c1, u1);
-    u[c0] = u0 + (1.0 * (l0 + b0)) / 6.0;
-    if (has1) u[c1] = u1 + (1.0 * (l1 + b1)) / 6.0;
+    T u0 = u[c0], u1 = u[c1], b0 = b[c0], b1 = b[c1];
+    T l0 = lap_fold(u, L, bc, i0, j0, k0, c0, u0);
+    T l1 = lap_fold(u, L, bc, i1, j1, k1, c1, u1);
+    u[c0] = u0 + div6(T(1) * (l0 + b0));
+    if (has1) u[c1] = u1 + div6(T(1) * (l1 + b1));
 }
 
 // Ghost shell item: x faces (full y-z extent), then y faces (interior x, full z),
 // then z faces (interior x, y).
-__device__ __forceinline__ void bc_item(double *a, const Lvl &L, const BcCoef &bc, int64_t t)
+template <typename T>
+__device__ __forceinline__ void bc_item(T *a, const Lvl &L, const BcCoef &bc, int64_t t)
 {
     const int64_t nx = L.n[0], ny = L.n[1], nz = L.n[2];
     int64_t i, j, k;
@@ -139,7 +151,7 @@ __device__ __forceinline__ void bc_item(double *a, const Lvl &L, const BcCoef &b
     if (i == 0) { f = f * bc.lo[0]; ci = 1; } else if (i == nx + 1) { f = f * bc.hi[0]; ci = nx; }
     if (j == 0) { f = f * bc.lo[1]; cj = 1; } else if (j == ny + 1) { f = f * bc.hi[1]; cj = ny; }
     if (k == 0) { f = f * bc.lo[2]; ck = 1; } else if (k == nz + 1) { f = f * bc.hi[2]; ck = nz; }
-    a[g_index(L, i, j, k)] = f * a[g_index(L, ci, cj, ck)];
+    a[g_index(L, i, j, k)] = (T)f * a[g_index(L, ci, cj, ck)];
 }
 
 __device__ __forceinline__ int64_t bc_count(const Lvl &L)
@@ -151,13 +163,13 @@ __device__ __forceinline__ int64_t bc_count(const Lvl &L)
 // residual! (Multigrid.jl:186-193) at the 8 children of coarse cell t, summed in
 // cubestencil order (x offset slowest) and restricted as 4*sum/8 (restrict!,
 // Multigrid.jl:213-226).  The fine residual itself is not stored.
-__device__ __forceinline__ void resid_restrict_item(double *rh, const double *u, const double *b,
-                                                    double s, const Lvl &F, const BcCoef &bc,
-                                                    int64_t t)
+template <typename T>
+__device__ __forceinline__ void resid_restrict_item(T *rh, const T *u, const T *b, T s, const Lvl &F,
+                                                    const BcCoef &bc, int64_t t)
 {
     int64_t ic, jc, kc;
     split3(t, (uint32_t)(F.n[0] / 2), (uint32_t)(F.n[1] / 2), ic, jc, kc);
-    double sum = 0.0;
+    T sum = T(0);
 #pragma unroll
     for (int dx = 0; dx < 2; ++dx)
 #pragma unroll
@@ -166,19 +178,20 @@ __device__ __forceinline__ void resid_restrict_item(double *rh, const double *u,
             for (int dz = 0; dz < 2; ++dz) {
                 int64_t i = 2 * ic + dx, j = 2 * jc + dy, k = 2 * kc + dz;
                 int64_t c = g_index(F, i + 1, j + 1, k + 1);
-                double uc = u[c];
-                double l = lap_fold(u, F, bc, i, j, k, c, uc);
+                T uc = u[c];
+                T l = lap_fold(u, F, bc, i, j, k, c, uc);
                 sum += s * b[c] + l;
             }
     Lvl C;
     C.n[0] = F.n[0] / 2;
     C.n[1] = F.n[1] / 2;
     C.n[2] = F.n[2] / 2;
-    rh[g_index(C, ic + 1, jc + 1, kc + 1)] = (4.0 * sum) / 8.0;
+    rh[g_index(C, ic + 1, jc + 1, kc + 1)] = (T(4) * sum) / T(8);
 }
 
 // restrict! of a stored fine array.
-__device__ __forceinline__ void restrict_item(double *rh, const double *r, const Lvl &F, int64_t t)
+template <typename T>
+__device__ __forceinline__ void restrict_item(T *rh, const T *r, const Lvl &F, int64_t t)
 {
     const int64_t nc0 = F.n[0] / 2, nc1 = F.n[1] / 2;
     int64_t i, j, k;
@@ -186,7 +199,7 @@ __device__ __forceinline__ void restrict_item(double *rh, const double *r, const
     const int64_t fy = F.n[0] + 2, fz = (int64_t)(F.n[0] + 2) * (F.n[1] + 2);
     const int64_t cy = nc0 + 2, cz = (int64_t)(nc0 + 2) * (nc1 + 2);
     int64_t f = (2 * i + 1) + fy * (2 * j + 1) + fz * (2 * k + 1);
-    double s = 0.0;
+    T s = T(0);
     s += r[f];
     s += r[f + fz];
     s += r[f + fy];
@@ -195,13 +208,13 @@ __device__ __forceinline__ void restrict_item(double *rh, const double *r, const
     s += r[f + 1 + fz];
     s += r[f + 1 + fy];
     s += r[f + 1 + fy + fz];
-    rh[(i + 1) + cy * (j + 1) + cz * (k + 1)] = (4.0 * s) / 8.0;
+    rh[(i + 1) + cy * (j + 1) + cz * (k + 1)] = (T(4) * s) / T(8);
 }
 
 // interpolate! (Multigrid.jl:232-258): weights prod(3 - 2 sh_i)/64 over the coarse
 // neighbours towards each fine child, sh in cubestencil order; add or overwrite.
-__device__ __forceinline__ void interp_item(double *r, const double *rh, const Lvl &F, int add,
-                                            int64_t t)
+template <typename T>
+__device__ __forceinline__ void interp_item(T *r, const T *rh, const Lvl &F, int add, int64_t t)
 {
     const int64_t nc0 = F.n[0] / 2, nc1 = F.n[1] / 2;
     int64_t i, j, k;
@@ -209,7 +222,7 @@ __device__ __forceinline__ void interp_item(double *r, const double *rh, const L
     const int64_t fy = F.n[0] + 2, fz = (int64_t)(F.n[0] + 2) * (F.n[1] + 2);
     const int64_t cy = nc0 + 2, cz = (int64_t)(nc0 + 2) * (nc1 + 2);
     int64_t cc = (i + 1) + cy * (j + 1) + cz * (k + 1);
-    const double w0 = 27.0 / 64.0, w1 = 9.0 / 64.0, w2 = 3.0 / 64.0, w3 = 1.0 / 64.0;
+    const T w0 = T(27) / T(64), w1 = T(9) / T(64), w2 = T(3) / T(64), w3 = T(1) / T(64);
 #pragma unroll
     for (int sx = 0; sx < 2; ++sx)
 #pragma unroll
@@ -218,7 +231,7 @@ __device__ __forceinline__ void interp_item(double *r, const double *rh, const L
             for (int sz = 0; sz < 2; ++sz) {
                 int64_t fidx = (2 * i + 1 + sx) + fy * (2 * j + 1 + sy) + fz * (2 * k + 1 + sz);
                 int64_t ox = 2 * sx - 1, oy = (2 * sy - 1) * cy, oz = (2 * sz - 1) * cz;
-                double v = 0.0;
+                T v = T(0);
                 v += w0 * rh[cc];
                 v += w1 * rh[cc + oz];
                 v += w1 * rh[cc + oy];
@@ -236,25 +249,26 @@ __device__ __forceinline__ void interp_item(double *r, const double *rh, const L
 
 // ---- grid kernels ------------------------------------------------------------
 
-__global__ __launch_bounds__(MCS_BLOCK) void k_mg_bc(double *a, Lvl L, BcCoef bc)
+template <typename T>
+__global__ __launch_bounds__(MCS_BLOCK) void k_mg_bc(T *a, Lvl L, BcCoef bc)
 {
     int64_t t = gidx();
     if (t < bc_count(L)) bc_item(a, L, bc, t);
 }
 
-__global__ __launch_bounds__(MCS_BLOCK) void k_mg_rbgs(double *u, const double *b, Lvl L, BcCoef bc,
-                                                       int par)
+template <typename T>
+__global__ __launch_bounds__(MCS_BLOCK) void k_mg_rbgs(T *u, const T *b, Lvl L, BcCoef bc, int par)
 {
     int64_t t = gidx();
     if (t < (int64_t)(L.n[0] / 2) * L.n[1] * L.n[2]) gs_item(u, b, L, bc, par, t);
 }
 
-__global__ __launch_bounds__(MCS_BLOCK) void k_mg_resid_restrict(double *rh, const double *u,
-                                                                 const double *b, Lvl F, BcCoef bc)
+template <typename T>
+__global__ __launch_bounds__(MCS_BLOCK) void k_mg_resid_restrict(T *rh, const T *u, const T *b, Lvl F, BcCoef bc)
 {
     int64_t t = gidx();
     if (t < (int64_t)(F.n[0] / 2) * (F.n[1] / 2) * (F.n[2] / 2))
-        resid_restrict_item(rh, u, b, 1.0, F, bc, t);
+        resid_restrict_item(rh, u, b, T(1), F, bc, t);
 }
 
 // residual! stored on the interior, with per-block sums of r^2 (fmg!'s norm).
@@ -318,13 +332,15 @@ __global__ void k_sum_partials(const double *partial, int64_t nb, double *out)
     if (threadIdx.x == 0) *out = sh[0];
 }
 
-__global__ __launch_bounds__(MCS_BLOCK) void k_mg_restrict(double *rh, const double *r, Lvl F)
+template <typename T>
+__global__ __launch_bounds__(MCS_BLOCK) void k_mg_restrict(T *rh, const T *r, Lvl F)
 {
     int64_t t = gidx();
     if (t < (int64_t)(F.n[0] / 2) * (F.n[1] / 2) * (F.n[2] / 2)) restrict_item(rh, r, F, t);
 }
 
-__global__ __launch_bounds__(MCS_BLOCK) void k_mg_interp(double *r, const double *rh, Lvl F, int add)
+template <typename T>
+__global__ __launch_bounds__(MCS_BLOCK) void k_mg_interp(T *r, const T *rh, Lvl F, int add)
 {
     int64_t t = gidx();
     if (t < (int64_t)(F.n[0] / 2) * (F.n[1] / 2) * (F.n[2] / 2)) interp_item(r, rh, F, add, t);
@@ -332,10 +348,10 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_mg_interp(double *r, const double
 
 // Coarse direct solve of fmg! (Multigrid.jl:404-410): z = A^-1 btop, one row per
 // thread, columns summed in order.  btop is staged in LDS (at most 4096 cells).
-__global__ __launch_bounds__(MCS_BLOCK) void k_mg_coarse(const double *ainv, const double *res,
-                                                         double *z, Lvl C)
+template <typename T>
+__global__ __launch_bounds__(MCS_BLOCK) void k_mg_coarse(const T *ainv, const T *res, T *z, Lvl C)
 {
-    __shared__ double bt[4096];
+    __shared__ T bt[4096];
     const int64_t N = (int64_t)C.n[0] * C.n[1] * C.n[2];
     for (int64_t col = threadIdx.x; col < N; col += blockDim.x) {
         int64_t i, j, k;
@@ -344,7 +360,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_mg_coarse(const double *ainv, con
     }
     __syncthreads();
     for (int64_t row = threadIdx.x; row < N; row += blockDim.x) {
-        double acc = 0.0;
+        T acc = T(0);
         for (int64_t col = 0; col < N; ++col) acc += ainv[row * N + col] * bt[col];
         int64_t i, j, k;
         split3(row, (uint32_t)C.n[0], (uint32_t)C.n[1], i, j, k);
@@ -352,10 +368,29 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_mg_coarse(const double *ainv, con
     }
 }
 
-__global__ void k_mg_fill0(double *a, int64_t n)
+template <typename T>
+__global__ void k_mg_fill0(T *a, int64_t n)
 {
     int64_t i = gidx();
-    if (i < n) a[i] = 0.0;
+    if (i < n) a[i] = T(0);
+}
+
+// FP64 residual -> FP32 right-hand side of the correction equation (whole array).
+__global__ void k_mg_to32(float *dst, const double *src, int64_t n)
+{
+    int64_t i = gidx();
+    if (i < n) dst[i] = (float)src[i];
+}
+
+// x += z on the interior with an FP32 correction.
+__global__ __launch_bounds__(MCS_BLOCK) void k_mg_axpy32(double *x, const float *z, Lvl L)
+{
+    int64_t t = gidx();
+    if (t >= (int64_t)L.n[0] * L.n[1] * L.n[2]) return;
+    int64_t i, j, k;
+    split3(t, (uint32_t)L.n[0], (uint32_t)L.n[1], i, j, k);
+    int64_t c = g_index(L, i + 1, j + 1, k + 1);
+    x[c] += (double)z[c];
 }
 
 // x += z on the interior (the ghosts are rewritten by the apply! that follows).
@@ -391,7 +426,7 @@ __global__ void k_mg_inhom(double *b, Lvl L, int face, double c, const double *v
 struct Tail {
     Lvl n[MG_MAXL];
     int64_t len[MG_MAXL];
-    int64_t off_x[MG_MAXL];   // LDS offsets (doubles) of the level solutions
+    int64_t off_x[MG_MAXL];   // LDS offsets (elements) of the level solutions
     int64_t off_b[MG_MAXL];   // and of their right-hand sides
     int L;
     int smooth1, smooth2;
@@ -406,7 +441,8 @@ struct Tail {
 // the start of the last round, as the last apply! leaves them.  Called by wave 0
 // only; LDS accesses of one wave complete in order, the fences keep the compiler
 // from reordering them.
-__device__ void coarsest_in_registers(double *x, const double *b, const Lvl &Lv, const BcCoef &bc)
+template <typename T>
+__device__ void coarsest_in_registers(T *x, const T *b, const Lvl &Lv, const BcCoef &bc)
 {
     const int lane = threadIdx.x;
     const int nx = Lv.n[0], ny = Lv.n[1], nz = Lv.n[2];
@@ -419,8 +455,8 @@ __device__ void coarsest_in_registers(double *x, const double *b, const Lvl &Lv,
         k = lane / (nx * ny);
     }
     const int64_t c = g_index(Lv, i + 1, j + 1, k + 1);
-    double u = own ? x[c] : 0.0;
-    const double bb = own ? b[c] : 0.0;
+    T u = own ? x[c] : T(0);
+    const T bb = own ? b[c] : T(0);
     const int par_me = (i + j + k) & 1;
     const int sxp = (i + 1 < nx) ? lane + 1 : lane, sxm = (i > 0) ? lane - 1 : lane;
     const int syp = (j + 1 < ny) ? lane + nx : lane, sym = (j > 0) ? lane - nx : lane;
@@ -435,18 +471,18 @@ __device__ void coarsest_in_registers(double *x, const double *b, const Lvl &Lv,
             __builtin_amdgcn_wave_barrier();
         }
         for (int par = 0; par < 2; ++par) {
-            double vxp = __shfl(u, sxp, MCS_WAVE), vyp = __shfl(u, syp, MCS_WAVE);
-            double vzp = __shfl(u, szp, MCS_WAVE), vxm = __shfl(u, sxm, MCS_WAVE);
-            double vym = __shfl(u, sym, MCS_WAVE), vzm = __shfl(u, szm, MCS_WAVE);
+            T vxp = __shfl(u, sxp, MCS_WAVE), vyp = __shfl(u, syp, MCS_WAVE);
+            T vzp = __shfl(u, szp, MCS_WAVE), vxm = __shfl(u, sxm, MCS_WAVE);
+            T vym = __shfl(u, sym, MCS_WAVE), vzm = __shfl(u, szm, MCS_WAVE);
             if (own && par_me == par) {
-                double l = -6.0 * u;
-                l += (i + 1 < nx) ? vxp : bc.hi[0] * u;
-                l += (j + 1 < ny) ? vyp : bc.hi[1] * u;
-                l += (k + 1 < nz) ? vzp : bc.hi[2] * u;
-                l += (i > 0) ? vxm : bc.lo[0] * u;
-                l += (j > 0) ? vym : bc.lo[1] * u;
-                l += (k > 0) ? vzm : bc.lo[2] * u;
-                u = u + (1.0 * (l + bb)) / 6.0;
+                T l = T(-6) * u;
+                l += (i + 1 < nx) ? vxp : (T)bc.hi[0] * u;
+                l += (j + 1 < ny) ? vyp : (T)bc.hi[1] * u;
+                l += (k + 1 < nz) ? vzp : (T)bc.hi[2] * u;
+                l += (i > 0) ? vxm : (T)bc.lo[0] * u;
+                l += (j > 0) ? vym : (T)bc.lo[1] * u;
+                l += (k > 0) ? vzm : (T)bc.lo[2] * u;
+                u = u + div6(T(1) * (l + bb));
             }
         }
     }
@@ -457,16 +493,17 @@ __device__ void coarsest_in_registers(double *x, const double *b, const Lvl &Lv,
 // solutions and the deeper right-hand sides live in LDS, the start level's
 // right-hand side is read from global memory; x0 is loaded and written back.
 // Phases are separated by barriers and follow the grid path's order exactly.
-__global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, double *x0,
-                                                             const double *b0)
+template <typename E>
+__global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, E *x0, const E *b0)
 {
-    extern __shared__ double lds[];
+    extern __shared__ double lds_raw[];
+    E *lds = (E *)lds_raw;
     const BcCoef bc = T.bc;
     BLOOP(t, T.len[start]) lds[T.off_x[start] + t] = x0[t];
     __syncthreads();
     for (int lev = start; lev < T.L; ++lev) {
-        double *x = lds + T.off_x[lev];
-        const double *b = lev == start ? b0 : lds + T.off_b[lev];
+        E *x = lds + T.off_x[lev];
+        const E *b = lev == start ? b0 : lds + T.off_b[lev];
         const Lvl Lv = T.n[lev];
         const int64_t ncol = (int64_t)(Lv.n[0] / 2) * Lv.n[1] * Lv.n[2];
         for (int it = 0; it < T.smooth1; ++it)
@@ -480,14 +517,14 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
             __syncthreads();
         }
         const int64_t nc = (int64_t)(Lv.n[0] / 2) * (Lv.n[1] / 2) * (Lv.n[2] / 2);
-        BLOOP(t, nc) resid_restrict_item(lds + T.off_b[lev + 1], x, b, 1.0, Lv, bc, t);
-        BLOOP(t, T.len[lev + 1]) lds[T.off_x[lev + 1] + t] = 0.0;
+        BLOOP(t, nc) resid_restrict_item(lds + T.off_b[lev + 1], x, b, E(1), Lv, bc, t);
+        BLOOP(t, T.len[lev + 1]) lds[T.off_x[lev + 1] + t] = E(0);
         __syncthreads();
     }
     {
         const int lev = T.L;
-        double *x = lds + T.off_x[lev];
-        const double *b = lev == start ? b0 : lds + T.off_b[lev];
+        E *x = lds + T.off_x[lev];
+        const E *b = lev == start ? b0 : lds + T.off_b[lev];
         const Lvl Lv = T.n[lev];
         const int64_t N = (int64_t)Lv.n[0] * Lv.n[1] * Lv.n[2];
         if (N <= MCS_WAVE) {
@@ -508,8 +545,8 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
         }
     }
     for (int lev = T.L - 1; lev >= start; --lev) {
-        double *x = lds + T.off_x[lev];
-        const double *b = lev == start ? b0 : lds + T.off_b[lev];
+        E *x = lds + T.off_x[lev];
+        const E *b = lev == start ? b0 : lds + T.off_b[lev];
         const Lvl Lv = T.n[lev];
         const int64_t nc = (int64_t)(Lv.n[0] / 2) * (Lv.n[1] / 2) * (Lv.n[2] / 2);
         BLOOP(t, nc) interp_item(x, lds + T.off_x[lev + 1], Lv, 1, t);
@@ -606,8 +643,10 @@ extern "C" int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg 
         lds_max = 64 * 1024;
     size_t tail_lds = (size_t)lds_max < (size_t)MG_TAIL_LDS ? (size_t)lds_max : (size_t)MG_TAIL_LDS;
     if (tail_lds > 64 * 1024 &&
-        hipFuncSetAttribute((const void *)k_mg_tail, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)tail_lds) != hipSuccess)
+        (hipFuncSetAttribute((const void *)k_mg_tail<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)tail_lds) != hipSuccess ||
+         hipFuncSetAttribute((const void *)k_mg_tail<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)tail_lds) != hipSuccess))
         tail_lds = 64 * 1024;
     if (!ctx || !cfg || !out) return MCS_E_ARG;
     const mcs_mg_config &c = *cfg;
@@ -622,8 +661,9 @@ extern "C" int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg 
     for (int f = 0; f < 6; ++f)
         if (c.bc[f] != MCS_BC_DIRICHLET && c.bc[f] != MCS_BC_NEUMANN)
             return mcs_fail(ctx, MCS_E_ARG, "mg: unknown boundary condition on face %d", f);
-    if (c.precision != MCS_PREC_F64)
-        return mcs_fail(ctx, MCS_E_ARG, "mg: only MCS_PREC_F64 is implemented");
+    if (c.precision != MCS_PREC_F64 && c.precision != MCS_PREC_F32)
+        return mcs_fail(ctx, MCS_E_ARG, "mg: unknown precision %d", c.precision);
+    const bool f32 = c.precision == MCS_PREC_F32;
     hipSetDevice(ctx->device);
     mcs_mg *mg = new mcs_mg();
     mg->ctx = ctx;
@@ -643,6 +683,13 @@ extern "C" int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg 
             return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation failed");
         hipMemset(L.res, 0, sizeof(double) * L.len);
         hipMemset(L.sol, 0, sizeof(double) * L.len);
+        if (f32) {
+            if (hipMalloc(&L.fres, sizeof(float) * L.len) != hipSuccess ||
+                hipMalloc(&L.fsol, sizeof(float) * L.len) != hipSuccess)
+                return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation failed");
+            hipMemset(L.fres, 0, sizeof(float) * L.len);
+            hipMemset(L.fsol, 0, sizeof(float) * L.len);
+        }
         mg->lv.push_back(L);
     }
     const int32_t *nc = mg->lv.back().n;
@@ -656,6 +703,12 @@ extern "C" int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg 
     if (hipMalloc(&mg->ainv, sizeof(double) * ainv.size()) != hipSuccess)
         return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation failed");
     hipMemcpy(mg->ainv, ainv.data(), sizeof(double) * ainv.size(), hipMemcpyHostToDevice);
+    if (f32) {
+        std::vector<float> a32(ainv.begin(), ainv.end());
+        if (hipMalloc(&mg->ainv32, sizeof(float) * a32.size()) != hipSuccess)
+            return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation failed");
+        hipMemcpy(mg->ainv32, a32.data(), sizeof(float) * a32.size(), hipMemcpyHostToDevice);
+    }
     int64_t N0 = (int64_t)c.n[0] * c.n[1] * c.n[2];
     mg->npartial = cdiv(N0, MCS_BLOCK);
     if (hipMalloc(&mg->partial, sizeof(double) * (mg->npartial + 8)) != hipSuccess)
@@ -672,8 +725,11 @@ extern "C" int32_t mcs_mg_destroy(mcs_mg *mg)
     for (auto &L : mg->lv) {
         hipFree(L.res);
         hipFree(L.sol);
+        if (L.fres) hipFree(L.fres);
+        if (L.fsol) hipFree(L.fsol);
     }
     hipFree(mg->ainv);
+    if (mg->ainv32) hipFree(mg->ainv32);
     hipFree(mg->partial);
     if (mg->u_tmp) hipFree(mg->u_tmp);
     if (mg->b_tmp) hipFree(mg->b_tmp);
@@ -712,34 +768,49 @@ static inline int64_t ncoarse_cells(const mcs_mg *mg, int lf)
 
 static inline dim3 grid_for(int64_t n) { return dim3((unsigned)cdiv(n > 0 ? n : 1, MCS_BLOCK)); }
 
-static void mg_bc(mcs_mg *mg, int l, double *a)
+// The level arrays of one precision.
+template <typename T> static T *lres(mcs_mg *mg, int l);
+template <typename T> static T *lsol(mcs_mg *mg, int l);
+template <typename T> static const T *coarse_inv(mcs_mg *mg);
+template <> double *lres<double>(mcs_mg *mg, int l) { return mg->lv[l].res; }
+template <> double *lsol<double>(mcs_mg *mg, int l) { return mg->lv[l].sol; }
+template <> const double *coarse_inv<double>(mcs_mg *mg) { return mg->ainv; }
+template <> float *lres<float>(mcs_mg *mg, int l) { return mg->lv[l].fres; }
+template <> float *lsol<float>(mcs_mg *mg, int l) { return mg->lv[l].fsol; }
+template <> const float *coarse_inv<float>(mcs_mg *mg) { return mg->ainv32; }
+
+template <typename T>
+static void mg_bc(mcs_mg *mg, int l, T *a)
 {
     Lvl L = lvl(mg, l);
     int64_t nx = L.n[0], ny = L.n[1], nz = L.n[2];
     int64_t cnt = 2 * (ny + 2) * (nz + 2) + 2 * nx * (nz + 2) + 2 * nx * ny;
-    hipLaunchKernelGGL(k_mg_bc, grid_for(cnt), dim3(MCS_BLOCK), 0, mg->ctx->stream, a, L, coefs(mg));
+    hipLaunchKernelGGL(k_mg_bc<T>, grid_for(cnt), dim3(MCS_BLOCK), 0, mg->ctx->stream, a, L, coefs(mg));
 }
 
-static void mg_gs(mcs_mg *mg, int l, double *u, const double *b)
+template <typename T>
+static void mg_gs(mcs_mg *mg, int l, T *u, const T *b)
 {
     Lvl L = lvl(mg, l);
     int64_t cnt = (int64_t)(L.n[0] / 2) * L.n[1] * L.n[2];
     ProfScope ps(mg->ctx, "mg_rbgs");
     for (int par = 0; par < 2; ++par)
-        hipLaunchKernelGGL(k_mg_rbgs, grid_for(cnt), dim3(MCS_BLOCK), 0, mg->ctx->stream, u, b, L,
-                           coefs(mg), par);
+        hipLaunchKernelGGL(k_mg_rbgs<T>, grid_for(cnt), dim3(MCS_BLOCK), 0, mg->ctx->stream, u, b, L, coefs(mg),
+                           par);
 }
 
-static void mg_fill0(mcs_mg *mg, double *a, int64_t n)
+template <typename T>
+static void mg_fill0(mcs_mg *mg, T *a, int64_t n)
 {
-    hipLaunchKernelGGL(k_mg_fill0, grid_for(n), dim3(MCS_BLOCK), 0, mg->ctx->stream, a, n);
+    hipLaunchKernelGGL(k_mg_fill0<T>, grid_for(n), dim3(MCS_BLOCK), 0, mg->ctx->stream, a, n);
 }
 
-static void mg_interp(mcs_mg *mg, int lf, double *r, const double *rh, int add)
+template <typename T>
+static void mg_interp(mcs_mg *mg, int lf, T *r, const T *rh, int add)
 {
     ProfScope ps(mg->ctx, "mg_prolong");
-    hipLaunchKernelGGL(k_mg_interp, grid_for(ncoarse_cells(mg, lf)), dim3(MCS_BLOCK), 0,
-                       mg->ctx->stream, r, rh, lvl(mg, lf), add);
+    hipLaunchKernelGGL(k_mg_interp<T>, grid_for(ncoarse_cells(mg, lf)), dim3(MCS_BLOCK), 0, mg->ctx->stream, r,
+                       rh, lvl(mg, lf), add);
 }
 
 static int32_t ensure_bc_dev(mcs_mg *mg)
@@ -752,76 +823,128 @@ static int32_t ensure_bc_dev(mcs_mg *mg)
 
 // LDS bytes of a tail from `level`: the solutions of levels level..L and the
 // right-hand sides of levels level+1..L.
+template <typename T>
 static size_t tail_bytes(const mcs_mg *mg, int level)
 {
-    size_t tot = (size_t)mg->lv[level].len * sizeof(double);
-    for (int l = level + 1; l <= mg->cfg.levels; ++l) tot += 2 * (size_t)mg->lv[l].len * sizeof(double);
+    size_t tot = (size_t)mg->lv[level].len * sizeof(T);
+    for (int l = level + 1; l <= mg->cfg.levels; ++l) tot += 2 * (size_t)mg->lv[l].len * sizeof(T);
     return tot;
 }
 
-static bool in_tail(const mcs_mg *mg, int level) { return tail_bytes(mg, level) <= mg->tail_lds; }
-
-static void mg_tail(mcs_mg *mg, int level, double *x, const double *b)
+template <typename T>
+static bool in_tail(const mcs_mg *mg, int level)
 {
-    Tail T;
-    memset(&T, 0, sizeof(T));
-    T.L = mg->cfg.levels;
-    T.smooth1 = mg->cfg.smooth1;
-    T.smooth2 = mg->cfg.smooth2;
-    T.bc = coefs(mg);
+    return tail_bytes<T>(mg, level) <= mg->tail_lds;
+}
+
+template <typename T>
+static void mg_tail(mcs_mg *mg, int level, T *x, const T *b)
+{
+    Tail Tl;
+    memset(&Tl, 0, sizeof(Tl));
+    Tl.L = mg->cfg.levels;
+    Tl.smooth1 = mg->cfg.smooth1;
+    Tl.smooth2 = mg->cfg.smooth2;
+    Tl.bc = coefs(mg);
     int64_t off = 0;
-    for (int l = 0; l <= T.L; ++l) {
-        T.n[l] = lvl(mg, l);
-        T.len[l] = mg->lv[l].len;
+    for (int l = 0; l <= Tl.L; ++l) {
+        Tl.n[l] = lvl(mg, l);
+        Tl.len[l] = mg->lv[l].len;
         if (l >= level) {
-            T.off_x[l] = off;
-            off += T.len[l];
-            T.off_b[l] = off;
-            if (l > level) off += T.len[l];
+            Tl.off_x[l] = off;
+            off += Tl.len[l];
+            Tl.off_b[l] = off;
+            if (l > level) off += Tl.len[l];
         }
     }
     ProfScope ps(mg->ctx, "mg_tail");
-    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(MG_TAIL_THREADS), (size_t)off * sizeof(double),
-                       mg->ctx->stream, T, level, x, b);
+    hipLaunchKernelGGL(k_mg_tail<T>, dim3(1), dim3(MG_TAIL_THREADS), (size_t)off * sizeof(T), mg->ctx->stream, Tl,
+                       level, x, b);
 }
 
 // mgv! (Multigrid.jl:334-377).  apply! before each smoothing sweep and before the
 // residual is folded into the kernels; the ghosts are materialised before the
 // last post-smoothing sweep (they are what the caller's prolongation reads).
-static void mgv(mcs_mg *mg, double *x, const double *b, int level)
+template <typename T>
+static void mgv(mcs_mg *mg, T *x, const T *b, int level)
 {
-    if (in_tail(mg, level)) {
-        mg_tail(mg, level, x, b);
+    if (in_tail<T>(mg, level)) {
+        mg_tail<T>(mg, level, x, b);
         return;
     }
-    for (int it = 0; it < mg->cfg.smooth1; ++it) mg_gs(mg, level, x, b);
-    if (mg->cfg.smooth2 == 0) mg_bc(mg, level, x);
+    for (int it = 0; it < mg->cfg.smooth1; ++it) mg_gs<T>(mg, level, x, b);
+    if (mg->cfg.smooth2 == 0) mg_bc<T>(mg, level, x);
     {
         ProfScope ps(mg->ctx, "mg_residual");
-        hipLaunchKernelGGL(k_mg_resid_restrict, grid_for(ncoarse_cells(mg, level)), dim3(MCS_BLOCK), 0,
-                           mg->ctx->stream, mg->lv[level + 1].res, (const double *)x, b, lvl(mg, level),
-                           coefs(mg));
+        hipLaunchKernelGGL(k_mg_resid_restrict<T>, grid_for(ncoarse_cells(mg, level)), dim3(MCS_BLOCK), 0,
+                           mg->ctx->stream, lres<T>(mg, level + 1), (const T *)x, b, lvl(mg, level), coefs(mg));
     }
-    double *xh = mg->lv[level + 1].sol;
-    mg_fill0(mg, xh, mg->lv[level + 1].len);
-    mgv(mg, xh, mg->lv[level + 1].res, level + 1);
-    mg_interp(mg, level, x, xh, 1);
+    T *xh = lsol<T>(mg, level + 1);
+    mg_fill0<T>(mg, xh, mg->lv[level + 1].len);
+    mgv<T>(mg, xh, lres<T>(mg, level + 1), level + 1);
+    mg_interp<T>(mg, level, x, xh, 1);
     for (int it = 0; it < mg->cfg.smooth2; ++it) {
-        if (it == mg->cfg.smooth2 - 1) mg_bc(mg, level, x);
-        mg_gs(mg, level, x, b);
+        if (it == mg->cfg.smooth2 - 1) mg_bc<T>(mg, level, x);
+        mg_gs<T>(mg, level, x, b);
     }
 }
 
-static void fmg_body(mcs_mg *mg, double *x, const double *b);
+// The correction part of fmg! (Multigrid.jl:396-426) on the hierarchy of one
+// precision, from the finest residual in lres<T>(0): restrictions, coarse solve
+// and the coarse-to-fine sweep of V-cycles.  Returns the finest correction.
+template <typename T>
+static T *fmg_correction(mcs_mg *mg)
+{
+    mcs_ctx *ctx = mg->ctx;
+    const int L = mg->cfg.levels;
+    for (int k = 0; k < L; ++k) {
+        ProfScope ps(ctx, "mg_restrict");
+        hipLaunchKernelGGL(k_mg_restrict<T>, grid_for(ncoarse_cells(mg, k)), dim3(MCS_BLOCK), 0, ctx->stream,
+                           lres<T>(mg, k + 1), (const T *)lres<T>(mg, k), lvl(mg, k));
+    }
+    hipLaunchKernelGGL(k_mg_coarse<T>, dim3(1), dim3(MCS_BLOCK), 0, ctx->stream, coarse_inv<T>(mg),
+                       (const T *)lres<T>(mg, L), lsol<T>(mg, L), lvl(mg, L));
+    T *z = lsol<T>(mg, L);
+    for (int k = 1; k <= L; ++k) {
+        int lev = L - k;
+        T *z1 = lsol<T>(mg, lev);
+        mg_bc<T>(mg, lev + 1, z);
+        mg_interp<T>(mg, lev, z1, z, 0);
+        mgv<T>(mg, z1, lres<T>(mg, lev), lev);
+        z = z1;
+    }
+    return z;
+}
+
+// The part of fmg! after the convergence test, ending with x += z and apply!.
+// No host interaction, so it is captured once into a HIP graph per (x, b) pair
+// and replayed.  In precision mode the FP64 residual is rounded to FP32 first
+// and the FP32 correction is added to the FP64 solution.
+static void fmg_body(mcs_mg *mg, double *x, const double *b)
+{
+    mcs_ctx *ctx = mg->ctx;
+    (void)b;
+    if (mg->cfg.precision == MCS_PREC_F32) {
+        hipLaunchKernelGGL(k_mg_to32, grid_for(mg->lv[0].len), dim3(MCS_BLOCK), 0, ctx->stream,
+                           mg->lv[0].fres, (const double *)mg->lv[0].res, mg->lv[0].len);
+        float *z = fmg_correction<float>(mg);
+        hipLaunchKernelGGL(k_mg_axpy32, grid_for(ncells(mg, 0)), dim3(MCS_BLOCK), 0, ctx->stream, x,
+                           (const float *)z, lvl(mg, 0));
+    } else {
+        double *z = fmg_correction<double>(mg);
+        hipLaunchKernelGGL(k_mg_axpy, grid_for(ncells(mg, 0)), dim3(MCS_BLOCK), 0, ctx->stream, x,
+                           (const double *)z, lvl(mg, 0));
+    }
+    mg_bc<double>(mg, 0, x);
+}
 
 // fmg! (Multigrid.jl:380-434).  Returns 1 when a correction was applied, 0 when
 // the residual already meets the tolerance, negative on error.
 static int fmg(mcs_mg *mg, double *x, const double *b, double *eps_out)
 {
     mcs_ctx *ctx = mg->ctx;
-    const int L = mg->cfg.levels;
     const MgLevel &F = mg->lv[0];
-    mg_bc(mg, 0, x);
+    mg_bc<double>(mg, 0, x);
     {
         ProfScope ps(ctx, "mg_residual");
         hipLaunchKernelGGL(k_mg_residual, grid_for(ncells(mg, 0)), dim3(MCS_BLOCK), 0, ctx->stream,
@@ -864,35 +987,6 @@ static int fmg(mcs_mg *mg, double *x, const double *b, double *eps_out)
     ProfScope ps(ctx, "mg_cycle");
     fmg_body(mg, x, b);
     return 1;
-}
-
-// The part of fmg! after the convergence test: restrictions, coarse solve, the
-// coarse-to-fine sweep of V-cycles and the update x += z.  No host interaction,
-// so it is captured once into a HIP graph per (x, b) pair and replayed.
-static void fmg_body(mcs_mg *mg, double *x, const double *b)
-{
-    mcs_ctx *ctx = mg->ctx;
-    const int L = mg->cfg.levels;
-    (void)b;
-    for (int k = 0; k < L; ++k) {
-        ProfScope ps(ctx, "mg_restrict");
-        hipLaunchKernelGGL(k_mg_restrict, grid_for(ncoarse_cells(mg, k)), dim3(MCS_BLOCK), 0, ctx->stream,
-                           mg->lv[k + 1].res, (const double *)mg->lv[k].res, lvl(mg, k));
-    }
-    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(MCS_BLOCK), 0, ctx->stream, (const double *)mg->ainv,
-                       (const double *)mg->lv[L].res, mg->lv[L].sol, lvl(mg, L));
-    double *z = mg->lv[L].sol;
-    for (int k = 1; k <= L; ++k) {
-        int lev = L - k;
-        double *z1 = mg->lv[lev].sol;
-        mg_bc(mg, lev + 1, z);
-        mg_interp(mg, lev, z1, z, 0);
-        mgv(mg, z1, mg->lv[lev].res, lev);
-        z = z1;
-    }
-    hipLaunchKernelGGL(k_mg_axpy, grid_for(ncells(mg, 0)), dim3(MCS_BLOCK), 0, ctx->stream, x,
-                       (const double *)z, lvl(mg, 0));
-    mg_bc(mg, 0, x);
 }
 
 // solve (Multigrid.jl:449-467)

@@ -85,3 +85,49 @@ def test_graph_replay_matches_eager(monkeypatch):
     for (ue, ie), (ug, ig) in zip(outs["0"], outs["1"]):
         assert ie == ig
         np.testing.assert_array_equal(ue, ug)
+
+
+@pytest.mark.parametrize("n,bc,levels", [
+    ((32, 32, 64), (N, N, N, N, D, D), 4),
+    ((64, 64, 128), (N, N, N, N, D, D), 5),
+    ((16, 16, 16), (N, D, D, N, D, N), 3),
+])
+def test_fp32_smoother_in_fp64_refinement(n, bc, levels):
+    """Precision mode: FP32 correction cycles inside FP64 refinement reach the same
+    1e-10 relative residual as the FP64 solver, and the same solution to the
+    accuracy that residual allows (the error of L u = -s b is bounded by the
+    residual over the smallest eigenvalue; compared here at 1e-6 relative)."""
+    from mcstreamer_amd import _lib
+    import ctypes
+    rng = np.random.default_rng(7 + sum(n))
+    g = (n[0] + 2) * (n[1] + 2) * (n[2] + 2)
+    b = rng.normal(size=g)
+    s = 0.9
+    out = {}
+    for prec in ("f64", "f32"):
+        faces = [mg.BottomBnd, mg.TopBnd, mg.LeftBnd, mg.RightBnd, mg.FrontBnd, mg.BackBnd]
+        conf = mg.MGConfig(bc=list(zip(faces, bc)), s=s, levels=levels, tolerance=-1e-10, maxiter=50,
+                           precision=prec)
+        ws = mg.Workspace(conf, n)
+        u = np.zeros(g)
+        it = ctypes.c_int32()
+        res = ctypes.c_double()
+        rc = _lib.lib().mcs_mg_solve_host(ws.handle, _lib.dptr(u), _lib.dptr(b), ctypes.byref(it),
+                                          ctypes.byref(res))
+        assert rc == 0
+        out[prec] = (u.copy(), it.value, res.value)
+        print(f"{prec} n={n} iters {it.value} resid {res.value:.3e}", flush=True)
+    brms = np.sqrt((b.reshape(n[2] + 2, n[1] + 2, n[0] + 2)[1:-1, 1:-1, 1:-1] ** 2).mean())
+    for prec in ("f64", "f32"):
+        assert out[prec][2] < 1e-10 * brms * s
+    # the residual of the FP32-mode solution, recomputed independently with numpy
+    u32 = out["f32"][0].reshape(n[2] + 2, n[1] + 2, n[0] + 2)
+    bb = b.reshape(n[2] + 2, n[1] + 2, n[0] + 2)
+    lap = (u32[1:-1, 1:-1, 2:] + u32[1:-1, 1:-1, :-2] + u32[1:-1, 2:, 1:-1] + u32[1:-1, :-2, 1:-1]
+           + u32[2:, 1:-1, 1:-1] + u32[:-2, 1:-1, 1:-1] - 6 * u32[1:-1, 1:-1, 1:-1])
+    r = s * bb[1:-1, 1:-1, 1:-1] + lap
+    assert np.sqrt((r ** 2).mean()) < 1.01e-10 * brms * s
+    u64 = out["f64"][0]
+    assert not np.array_equal(out["f32"][0], u64)   # the FP32 correction path ran
+    scale = np.abs(u64).max()
+    assert np.abs(out["f32"][0] - u64).max() < 1e-6 * scale
