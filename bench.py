@@ -321,6 +321,87 @@ def run_serve(args):
     return out
 
 
+BYTES_PER_PSTEP_F32 = 72.0
+
+
+def run_precision(args):
+    """The precision form of the streamer step (256x256x512 cells of 2.5 um, 4e7
+    super-particles): split store, FP32 step, FP32 correction cycles inside FP64
+    refinement to 1e-10, against an FP64 run from the same seed state."""
+    from mcstreamer_amd import streamer
+    from mcstreamer_amd.jumc import context
+
+    ctx = context()
+    runs = {}
+    for prec in ("f32", "f64"):
+        t0 = time.perf_counter()
+        cfg = streamer.make_config2(precision=prec, npairs=args.c2_pairs, seed=args.seed)
+        sim = streamer.Streamer(cfg["popl"], cfg["fields"], cfg["ws"], cfg["eb"], cfg["dt"], cfg["ntarget"],
+                                cfg["nmax"], scheme="cic")
+        ctx.synchronize()
+        print(f"{prec}: built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+        for i in range(args.warmup):
+            out = sim.step()
+            print(f"{prec} warmup {i}: n={out['advance'].n_in} mg_iters={out['mg_iters']}", file=sys.stderr,
+                  flush=True)
+        ctx.synchronize()
+        ctx.profile_reset()
+        ctx.profile(True)
+        psteps, vcyc, events, iters = 0, 0, None, []
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            out = sim.step()
+            psteps += out["advance"].n_in
+            vcyc += max(out["mg_iters"] - 1, 0)
+            iters.append(out["mg_iters"])
+            ev = np.array(list(out["advance"].events)[:len(cfg["table"].proc)], dtype=np.float64)
+            events = ev if events is None else events + ev
+        ctx.synchronize()
+        el = time.perf_counter() - t0
+        ctx.profile(False)
+        adv_ms, adv_n = ctx.profile_read("advance")
+        f = cfg["fields"]
+        runs[prec] = dict(el=el, psteps=psteps, vcycles=vcyc, events=events, iters=iters,
+                          resid=out["mg_resid"], nactive=out["moments"].nactive,
+                          ex=f.get("ex"), ey=f.get("ey"), ez=f.get("ez"), u=f.get("u"), q=f.get("q"),
+                          adv_avg_s=adv_ms / 1e3 / max(adv_n, 1))
+        print(f"{prec}: {args.steps} steps in {el:.2f} s, n={out['advance'].n_out}", file=sys.stderr, flush=True)
+        del sim, cfg, f, out
+        import gc
+        gc.collect()
+        ctx.synchronize()
+    a, b = runs["f32"], runs["f64"]
+    num = sum(float(np.sum((a[k] - b[k]) ** 2)) for k in ("ex", "ey", "ez"))
+    den = sum(float(np.sum(b[k] ** 2)) for k in ("ex", "ey", "ez"))
+    bpl = BYTES_PER_PSTEP_F32 * a["psteps"] / max(args.steps, 1)
+    ach = bpl / a["adv_avg_s"] / 1e9
+    return {"metric": METRIC, "value": a["psteps"] / a["el"], "unit": "particle-steps/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": a["el"] / args.steps * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32 split (f64 refinement residual/update)", "data": "synthetic",
+            "config": {"workload": "precision streamer step, 256x256x512 mesh h=2.5e-6 m, "
+                                   f"{args.c2_pairs:g}-pair Gaussian seed (sigma 100 um), CIC, "
+                                   "FP32 V(2,2) inside FP64 refinement to 1e-10 rel, merge/split 8-64, "
+                                   "dt=2e-13 s", "mesh": [256, 256, 512],
+                       "particles_per_step": a["psteps"] / max(args.steps, 1), "parallelism": "single GPU"},
+            "vcycles_per_s": a["vcycles"] / a["el"],
+            "fp64_same_workload": {"particle_steps_per_s": b["psteps"] / b["el"],
+                                   "ms_per_step": b["el"] / args.steps * 1e3},
+            "errors_vs_fp64": {
+                "field_l2_rel": float(np.sqrt(num / den)),
+                "potential_l2_rel": float(np.linalg.norm(a["u"] - b["u"]) / np.linalg.norm(b["u"])),
+                "total_charge_rel": float(abs(a["q"].sum() - b["q"].sum()) / np.abs(b["q"]).sum()),
+                "electron_count_rel": float((a["nactive"] - b["nactive"]) / b["nactive"]),
+                "events_rel": [float((x - y) / y) if y else None for x, y in zip(a["events"], b["events"])],
+                "mg_iters_f32": a["iters"], "mg_iters_f64": b["iters"],
+                "final_resid_f32": a["resid"], "final_resid_f64": b["resid"]},
+            "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                         "kernel": "k_advance32<1,1> (split store, FP32 push + collisions)",
+                         "avg_launch_ms": a["adv_avg_s"] * 1e3, "bytes_per_launch": bpl},
+            "cpu_baseline": None}
+
+
 def copy_calibration(nbytes=1 << 30, reps=3):
     """A plain device copy of known size: its FETCH_SIZE/WRITE_SIZE counters give
     the factor that turns the counters of the step's kernels into bytes."""
@@ -339,7 +420,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["streamer", "swarm", "batch", "serve"], default="streamer")
+    ap.add_argument("--workload", choices=["streamer", "swarm", "batch", "serve", "precision"],
+                    default="streamer")
+    ap.add_argument("--c2-pairs", type=int, default=40_000_000)
     ap.add_argument("--instances", type=int, default=1024)
     ap.add_argument("--solve-steps", type=int, default=5000)
     ap.add_argument("--no-autotune", action="store_true")
@@ -367,7 +450,7 @@ def main():
               "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
         sys.exit(2)
     res = {"streamer": run_streamer, "swarm": run_swarm, "batch": run_batch,
-           "serve": run_serve}[args.workload](args)
+           "serve": run_serve, "precision": run_precision}[args.workload](args)
     print(json.dumps(res))
 
 

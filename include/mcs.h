@@ -157,6 +157,14 @@ int32_t mcs_table_destroy(mcs_table *t);
 /* ---- particle population: JuMC.Population (JuMC/src/population.jl:4-36) -----
  * Host layout of x and v is n x 3 row-major (Julia Vector{SVector{3,Float64}}). */
 int32_t mcs_pop_create(mcs_ctx *ctx, int64_t capacity, mcs_pop **out);
+/* Split-precision population (precision mode) on the grid of mesh m: positions
+ * stored as packed cell coordinates (i, j, k < 1024, 1024, 2048) plus an FP32
+ * offset in the cell, FP32 velocity, weight and collision clock.  Upload and
+ * download take and give the same FP64 host layout as an FP64 population;
+ * mcs_advance runs the FP32 step, mcs_sort_by_cell / mcs_deposit /
+ * mcs_resample_cells / mcs_pop_moments accept it; repack and swarm_resample do
+ * not.  No counterpart in the reference (its populations are FP64). */
+int32_t mcs_pop_create_split(mcs_ctx *ctx, int64_t capacity, const mcs_mesh *m, mcs_pop **out);
 int32_t mcs_pop_destroy(mcs_pop *p);
 int32_t mcs_pop_upload(mcs_pop *p, int64_t n, const double *x, const double *v,
                        const double *w, const double *s, const uint8_t *active,

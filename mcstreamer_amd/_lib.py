@@ -162,6 +162,7 @@ SIGNATURES = {
     "mcs_table_create": [_P, _I32, _I32, _I32, _PD, _PD, _PI32, _PD, _D, _PP],
     "mcs_table_destroy": [_P],
     "mcs_pop_create": [_P, _I64, _PP],
+    "mcs_pop_create_split": [_P, _I64, _P, _PP],
     "mcs_pop_destroy": [_P],
     "mcs_pop_upload": [_P, _I64, _PD, _PD, _PD, _PD, _PU8, _PU64, _U64],
     "mcs_pop_download": [_P, _I64, _PI64, _PD, _PD, _PD, _PD, _PU8, _PU64, _PU64],

@@ -4,6 +4,7 @@
 #include <cstring>
 
 #include "mcs_device.h"
+#include "mcs_fp32.h"
 
 // ---------------------------------------------------------------- errors
 
@@ -366,6 +367,129 @@ static void free_buf(PopBuf &b)
     b = PopBuf();
 }
 
+static int32_t alloc_split(mcs_ctx *ctx, SplitBuf &b, int64_t cap)
+{
+    MCS_HIP(ctx, hipMalloc(&b.cell, sizeof(int32_t) * cap));
+    for (int d = 0; d < 3; ++d) {
+        MCS_HIP(ctx, hipMalloc(&b.off[d], sizeof(float) * cap));
+        MCS_HIP(ctx, hipMalloc(&b.v[d], sizeof(float) * cap));
+    }
+    MCS_HIP(ctx, hipMalloc(&b.w, sizeof(float) * cap));
+    MCS_HIP(ctx, hipMalloc(&b.s, sizeof(float) * cap));
+    MCS_HIP(ctx, hipMalloc(&b.active, cap));
+    MCS_HIP(ctx, hipMalloc(&b.id, sizeof(uint64_t) * cap));
+    return MCS_OK;
+}
+
+static void free_split(SplitBuf &b)
+{
+    hipFree(b.cell);
+    for (int d = 0; d < 3; ++d) {
+        hipFree(b.off[d]);
+        hipFree(b.v[d]);
+    }
+    hipFree(b.w);
+    hipFree(b.s);
+    hipFree(b.active);
+    hipFree(b.id);
+    b = SplitBuf();
+}
+
+// Host particles (x, v as n x 3, FP64) into the split store of a population.
+__global__ __launch_bounds__(MCS_BLOCK) void k_split_in(SplitBuf b, int64_t n, const double *x, const double *v,
+                                                        const double *w, const double *s, const uint8_t *act,
+                                                        double inv_h, int32_t n0, int32_t n1, int32_t n2)
+{
+    int64_t i = gidx();
+    if (i >= n) return;
+    const int32_t nmax[3] = {n0, n1, n2};
+    int32_t c[3];
+    float off[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const double r = x[3 * i + q] * inv_h;
+        const double fl = floor(r);
+        c[q] = (int32_t)fmax(fmin(fl, 1e9), -1e9);
+        off[q] = (float)(r - fl);
+        b.v[q][i] = (float)v[3 * i + q];
+    }
+    rebase32(c, off, nmax);
+    b.cell[i] = pack_cell(c[0], c[1], c[2]);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) b.off[q][i] = off[q];
+    b.w[i] = (float)w[i];
+    b.s[i] = (float)s[i];
+    b.active[i] = act[i];
+}
+
+// The split store as FP64 host layout (x, v as n x 3).
+__global__ __launch_bounds__(MCS_BLOCK) void k_split_out(SplitBuf b, int64_t n, double h, double *x, double *v,
+                                                         double *w, double *s)
+{
+    int64_t i = gidx();
+    if (i >= n) return;
+    double xs[3], vs[3], ws;
+    load_state(b, i, h, xs, vs, ws);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        x[3 * i + q] = xs[q];
+        v[3 * i + q] = vs[q];
+    }
+    w[i] = ws;
+    s[i] = (double)b.s[i];
+}
+
+static int32_t alloc_pop_common(mcs_ctx *ctx, mcs_pop *p, int64_t capacity)
+{
+    int64_t nblk = cdiv(capacity, MCS_BLOCK) + 2;
+    if (hipMalloc(&p->log_parent, sizeof(int32_t) * p->logcap) != hipSuccess ||
+        hipMalloc(&p->log_seq, sizeof(int32_t) * p->logcap) != hipSuccess ||
+        hipMalloc(&p->log_act, p->logcap) != hipSuccess ||
+        hipMalloc(&p->i32a, sizeof(int32_t) * (capacity + 1)) != hipSuccess ||
+        hipMalloc(&p->i32b, sizeof(int32_t) * (capacity + 1)) != hipSuccess ||
+        hipMalloc(&p->i32c, sizeof(int32_t) * (capacity + 1)) != hipSuccess ||
+        hipMalloc(&p->u64a, sizeof(uint64_t) * (capacity + 1)) != hipSuccess ||
+        hipMalloc(&p->blk, sizeof(int64_t) * (2 * nblk + 4)) != hipSuccess)
+        return mcs_fail(ctx, MCS_E_HIP, "population: device allocation failed");
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_pop_create_split(mcs_ctx *ctx, int64_t capacity, const mcs_mesh *m, mcs_pop **out)
+{
+    if (!ctx || !out || !m) return MCS_E_ARG;
+    if (capacity < 1 || capacity >= (int64_t)1 << 31)
+        return mcs_fail(ctx, MCS_E_ARG, "population capacity must be in [1, 2^31)");
+    if (m->n[0] > MCS_CX_MAX || m->n[1] > MCS_CY_MAX || m->n[2] > MCS_CZ_MAX)
+        return mcs_fail(ctx, MCS_E_ARG, "split population: mesh exceeds %d x %d x %d cells", MCS_CX_MAX,
+                        MCS_CY_MAX, MCS_CZ_MAX);
+    hipSetDevice(ctx->device);
+    mcs_pop *p = new mcs_pop();
+    p->ctx = ctx;
+    p->cap = capacity;
+    p->prec = MCS_PREC_F32;
+    for (int d = 0; d < 3; ++d) p->gn[d] = m->n[d];
+    p->gh = m->h;
+    p->logcap = capacity;
+    int32_t rc = alloc_split(ctx, p->sbuf[0], capacity);
+    if (!rc) rc = alloc_pop_common(ctx, p, capacity);
+    if (!rc) {
+        if (hipMalloc(&p->log_cell, sizeof(int32_t) * p->logcap) != hipSuccess ||
+            hipMalloc(&p->log_w32, sizeof(float) * p->logcap) != hipSuccess ||
+            hipMalloc(&p->log_s32, sizeof(float) * p->logcap) != hipSuccess)
+            rc = mcs_fail(ctx, MCS_E_HIP, "population: device allocation failed");
+        for (int d = 0; d < 3 && !rc; ++d)
+            if (hipMalloc(&p->log_off[d], sizeof(float) * p->logcap) != hipSuccess ||
+                hipMalloc(&p->log_v32[d], sizeof(float) * p->logcap) != hipSuccess)
+                rc = mcs_fail(ctx, MCS_E_HIP, "population: device allocation failed");
+    }
+    if (rc) {
+        mcs_pop_destroy(p);
+        return rc;
+    }
+    *out = p;
+    return MCS_OK;
+}
+
 extern "C" int32_t mcs_pop_create(mcs_ctx *ctx, int64_t capacity, mcs_pop **out)
 {
     if (!ctx || !out) return MCS_E_ARG;
@@ -407,7 +531,8 @@ extern "C" int32_t mcs_pop_create(mcs_ctx *ctx, int64_t capacity, mcs_pop **out)
 int32_t mcs_pop_ensure_alt(mcs_pop *p)
 {
     if (p->has_alt) return MCS_OK;
-    int32_t rc = alloc_buf(p->ctx, p->buf[1], p->cap);
+    int32_t rc = p->prec == MCS_PREC_F32 ? alloc_split(p->ctx, p->sbuf[1], p->cap)
+                                         : alloc_buf(p->ctx, p->buf[1], p->cap);
     if (rc) return rc;
     p->has_alt = true;
     return MCS_OK;
@@ -418,6 +543,9 @@ void mcs_pop_swap(mcs_pop *p)
     PopBuf t = p->buf[0];
     p->buf[0] = p->buf[1];
     p->buf[1] = t;
+    SplitBuf u = p->sbuf[0];
+    p->sbuf[0] = p->sbuf[1];
+    p->sbuf[1] = u;
 }
 
 extern "C" int32_t mcs_pop_destroy(mcs_pop *p)
@@ -425,8 +553,20 @@ extern "C" int32_t mcs_pop_destroy(mcs_pop *p)
     if (!p) return MCS_OK;
     hipSetDevice(p->ctx->device);
     hipStreamSynchronize(p->ctx->stream);
-    free_buf(p->buf[0]);
-    if (p->has_alt) free_buf(p->buf[1]);
+    if (p->prec == MCS_PREC_F32) {
+        free_split(p->sbuf[0]);
+        if (p->has_alt) free_split(p->sbuf[1]);
+        hipFree(p->log_cell);
+        hipFree(p->log_w32);
+        hipFree(p->log_s32);
+        for (int d = 0; d < 3; ++d) {
+            hipFree(p->log_off[d]);
+            hipFree(p->log_v32[d]);
+        }
+    } else {
+        free_buf(p->buf[0]);
+        if (p->has_alt) free_buf(p->buf[1]);
+    }
     hipFree(p->log_parent);
     hipFree(p->log_seq);
     for (int d = 0; d < 3; ++d) {
@@ -460,26 +600,48 @@ extern "C" int32_t mcs_pop_upload(mcs_pop *p, int64_t n, const double *x, const 
         return mcs_fail(ctx, MCS_E_ARG, "pop_upload: null array");
     hipSetDevice(ctx->device);
     MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    std::vector<double> tmp((size_t)n);
+    if (p->prec == MCS_PREC_F32) {
+        void *scr;
+        int32_t rc = mcs_scratch(ctx, (sizeof(double) * 8 + 1) * (size_t)(n > 0 ? n : 1), &scr);
+        if (rc) return rc;
+        double *dx = (double *)scr, *dv = dx + 3 * n, *dw = dv + 3 * n, *ds = dw + n;
+        uint8_t *da = (uint8_t *)(ds + n);
+        if (n > 0) {
+            MCS_HIP(ctx, hipMemcpy(dx, x, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+            MCS_HIP(ctx, hipMemcpy(dv, v, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+            MCS_HIP(ctx, hipMemcpy(dw, w, sizeof(double) * n, hipMemcpyHostToDevice));
+            MCS_HIP(ctx, hipMemcpy(ds, s, sizeof(double) * n, hipMemcpyHostToDevice));
+            MCS_HIP(ctx, hipMemcpy(da, active, n, hipMemcpyHostToDevice));
+            hipLaunchKernelGGL(k_split_in, dim3((unsigned)cdiv(n, MCS_BLOCK)), dim3(MCS_BLOCK), 0, ctx->stream,
+                               p->sbuf[0], n, (const double *)dx, (const double *)dv, (const double *)dw,
+                               (const double *)ds, (const uint8_t *)da, 1.0 / p->gh, p->gn[0], p->gn[1], p->gn[2]);
+            MCS_LAUNCH_CHECK(ctx);
+            MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        }
+    }
+    std::vector<double> tmp((size_t)(p->prec == MCS_PREC_F32 ? 0 : n));
     PopBuf &b = p->buf[0];
-    for (int d = 0; d < 3; ++d) {
+    for (int d = 0; d < 3 && p->prec != MCS_PREC_F32; ++d) {
         for (int64_t i = 0; i < n; ++i) tmp[i] = x[3 * i + d];
         MCS_HIP(ctx, hipMemcpy(b.x[d], tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
         for (int64_t i = 0; i < n; ++i) tmp[i] = v[3 * i + d];
         MCS_HIP(ctx, hipMemcpy(b.v[d], tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     }
-    MCS_HIP(ctx, hipMemcpy(b.w, w, sizeof(double) * n, hipMemcpyHostToDevice));
-    MCS_HIP(ctx, hipMemcpy(b.s, s, sizeof(double) * n, hipMemcpyHostToDevice));
-    MCS_HIP(ctx, hipMemcpy(b.active, active, n, hipMemcpyHostToDevice));
+    if (p->prec != MCS_PREC_F32) {
+        MCS_HIP(ctx, hipMemcpy(b.w, w, sizeof(double) * n, hipMemcpyHostToDevice));
+        MCS_HIP(ctx, hipMemcpy(b.s, s, sizeof(double) * n, hipMemcpyHostToDevice));
+        MCS_HIP(ctx, hipMemcpy(b.active, active, n, hipMemcpyHostToDevice));
+    }
+    uint64_t *dst_id = p->prec == MCS_PREC_F32 ? p->sbuf[0].id : b.id;
     uint64_t nid = next_id;
     if (id) {
-        MCS_HIP(ctx, hipMemcpy(b.id, id, sizeof(uint64_t) * n, hipMemcpyHostToDevice));
+        MCS_HIP(ctx, hipMemcpy(dst_id, id, sizeof(uint64_t) * n, hipMemcpyHostToDevice));
         for (int64_t i = 0; i < n; ++i)
             if (id[i] + 1 > nid) nid = id[i] + 1;
     } else {
         std::vector<uint64_t> ids((size_t)n);
         for (int64_t i = 0; i < n; ++i) ids[i] = next_id + (uint64_t)i;
-        MCS_HIP(ctx, hipMemcpy(b.id, ids.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice));
+        MCS_HIP(ctx, hipMemcpy(dst_id, ids.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice));
         nid = next_id + (uint64_t)n;
     }
     p->n = n;
@@ -499,6 +661,23 @@ extern "C" int32_t mcs_pop_download(mcs_pop *p, int64_t nmax, int64_t *nout, dou
     if (next_id) *next_id = p->next_id;
     int64_t n = p->n < nmax ? p->n : nmax;
     if (n <= 0) return MCS_OK;
+    if (p->prec == MCS_PREC_F32) {
+        void *scr;
+        int32_t rc = mcs_scratch(ctx, sizeof(double) * 8 * (size_t)n, &scr);
+        if (rc) return rc;
+        double *dx = (double *)scr, *dv = dx + 3 * n, *dw = dv + 3 * n, *ds = dw + n;
+        hipLaunchKernelGGL(k_split_out, dim3((unsigned)cdiv(n, MCS_BLOCK)), dim3(MCS_BLOCK), 0, ctx->stream,
+                           p->sbuf[0], n, p->gh, dx, dv, dw, ds);
+        MCS_LAUNCH_CHECK(ctx);
+        MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (x) MCS_HIP(ctx, hipMemcpy(x, dx, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+        if (v) MCS_HIP(ctx, hipMemcpy(v, dv, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+        if (w) MCS_HIP(ctx, hipMemcpy(w, dw, sizeof(double) * n, hipMemcpyDeviceToHost));
+        if (s) MCS_HIP(ctx, hipMemcpy(s, ds, sizeof(double) * n, hipMemcpyDeviceToHost));
+        if (active) MCS_HIP(ctx, hipMemcpy(active, p->sbuf[0].active, n, hipMemcpyDeviceToHost));
+        if (id) MCS_HIP(ctx, hipMemcpy(id, p->sbuf[0].id, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+        return MCS_OK;
+    }
     PopBuf &b = p->buf[0];
     std::vector<double> tmp((size_t)n);
     for (int d = 0; d < 3; ++d) {
@@ -557,7 +736,8 @@ __device__ __forceinline__ void neu_merge(Neu &a, const Neu &b)
 #define MOM_REC (2 * MOM_NV + 2)  // + maxenergy + nactive
 #define MOM_BLOCKS 512
 
-__global__ __launch_bounds__(MCS_BLOCK) void k_moments(PopBuf b, int64_t n, double *partial)
+template <typename Buf>
+__global__ __launch_bounds__(MCS_BLOCK) void k_moments(Buf b, int64_t n, double gh, double *partial)
 {
     Neu acc[MOM_NV];
 #pragma unroll
@@ -565,12 +745,13 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_moments(PopBuf b, int64_t n, doub
     double emax = -HUGE_VAL;
     double nact = 0.0;
     for (int64_t i = gidx(); i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        double v0 = b.v[0][i], v1 = b.v[1][i], v2 = b.v[2][i];
+        double xs[3], vs[3], w;
+        load_state(b, i, gh, xs, vs, w);
+        double v0 = vs[0], v1 = vs[1], v2 = vs[2];
         double eng = energy_dev(v0, v1, v2);
         emax = fmax(emax, eng);
         if (!b.active[i]) continue;
-        double x0 = b.x[0][i], x1 = b.x[1][i], x2 = b.x[2][i];
-        double w = b.w[i];
+        double x0 = xs[0], x1 = xs[1], x2 = xs[2];
         nact += 1.0;
         neu_add(acc[0], w);
         neu_add(acc[1], w * eng);
@@ -670,7 +851,8 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_moments_final(const double *parti
 
 #define CR_CHUNK 8
 
-__global__ __launch_bounds__(MCS_BLOCK) void k_collrate(PopBuf b, int64_t n, const double *energy,
+template <typename Buf>
+__global__ __launch_bounds__(MCS_BLOCK) void k_collrate(Buf b, int64_t n, const double *energy,
                                                         const double *rate, int32_t nproc,
                                                         int32_t ne, int32_t egrid, double step,
                                                         double inv_dlog, int j0, double *partial)
@@ -681,7 +863,9 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_collrate(PopBuf b, int64_t n, con
     int32_t flags = 0;
     for (int64_t i = gidx(); i < n; i += (int64_t)gridDim.x * blockDim.x) {
         if (!b.active[i]) continue;
-        double eng = energy_dev(b.v[0][i], b.v[1][i], b.v[2][i]);
+        double xs[3], vs[3], wq;
+        load_state(b, i, 0.0, xs, vs, wq);
+        double eng = energy_dev(vs[0], vs[1], vs[2]);
         int64_t k;
         double w;
         indweight_dev(energy, ne, egrid, step, inv_dlog, eng, k, w, flags);
@@ -743,8 +927,12 @@ extern "C" int32_t mcs_pop_moments(mcs_pop *p, const mcs_table *t, mcs_moments *
     if (rc) return rc;
     double *partial = (double *)scr;
     double *res = partial + (size_t)nb * (MOM_REC + 2 * CR_CHUNK);
-    hipLaunchKernelGGL(k_moments, dim3(nb), dim3(MCS_BLOCK), 0, ctx->stream, p->buf[0], p->n,
-                       partial);
+    if (p->prec == MCS_PREC_F32)
+        hipLaunchKernelGGL(k_moments<SplitBuf>, dim3(nb), dim3(MCS_BLOCK), 0, ctx->stream, p->sbuf[0], p->n,
+                           p->gh, partial);
+    else
+        hipLaunchKernelGGL(k_moments<PopBuf>, dim3(nb), dim3(MCS_BLOCK), 0, ctx->stream, p->buf[0], p->n,
+                           0.0, partial);
     hipLaunchKernelGGL(k_moments_final, dim3(MOM_NV + 2), dim3(MCS_BLOCK), 0, ctx->stream, partial, nb, res);
     MCS_LAUNCH_CHECK(ctx);
     double h[MOM_NV + 2];
@@ -762,9 +950,14 @@ extern "C" int32_t mcs_pop_moments(mcs_pop *p, const mcs_table *t, mcs_moments *
     out->nactive = (int64_t)h[MOM_NV + 1];
     if (t) {
         for (int j0 = 0; j0 < t->nproc; j0 += CR_CHUNK) {
-            hipLaunchKernelGGL(k_collrate, dim3(nb), dim3(MCS_BLOCK), 0, ctx->stream, p->buf[0],
-                               p->n, t->energy, t->rate, t->nproc, t->ne, t->egrid, t->step,
-                               t->inv_dlog, j0, partial);
+            if (p->prec == MCS_PREC_F32)
+                hipLaunchKernelGGL(k_collrate<SplitBuf>, dim3(nb), dim3(MCS_BLOCK), 0, ctx->stream, p->sbuf[0],
+                                   p->n, t->energy, t->rate, t->nproc, t->ne, t->egrid, t->step,
+                                   t->inv_dlog, j0, partial);
+            else
+                hipLaunchKernelGGL(k_collrate<PopBuf>, dim3(nb), dim3(MCS_BLOCK), 0, ctx->stream, p->buf[0],
+                                   p->n, t->energy, t->rate, t->nproc, t->ne, t->egrid, t->step,
+                                   t->inv_dlog, j0, partial);
             hipLaunchKernelGGL(k_collrate_final, dim3(1), dim3(64), 0, ctx->stream, partial, nb,
                                res);
             MCS_LAUNCH_CHECK(ctx);

@@ -16,15 +16,6 @@
 #define MCS_CY_MAX (1 << MCS_CY_BITS)
 #define MCS_CZ_MAX (1 << MCS_CZ_BITS)
 
-struct SplitBuf {
-    int32_t *cell = nullptr;
-    float *off[3] = {nullptr, nullptr, nullptr};
-    float *v[3] = {nullptr, nullptr, nullptr};
-    float *w = nullptr, *s = nullptr;
-    uint8_t *active = nullptr;
-    uint64_t *id = nullptr;
-};
-
 __host__ __device__ __forceinline__ int32_t pack_cell(int32_t i, int32_t j, int32_t k)
 {
     return i | (j << MCS_CX_BITS) | (k << (MCS_CX_BITS + MCS_CY_BITS));
@@ -370,3 +361,100 @@ __device__ __forceinline__ void advance_one32(int32_t c[3], float off[3], float 
         trem = trem - tt;
     }
 }
+
+__device__ __forceinline__ void copy_rec(const PopBuf &src, int64_t si, const PopBuf &dst, int64_t di)
+{
+    copy_particle(src, si, dst, di);
+}
+
+__device__ __forceinline__ void copy_rec(const SplitBuf &src, int64_t si, const SplitBuf &dst, int64_t di)
+{
+    dst.cell[di] = src.cell[si];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        dst.off[d][di] = src.off[d][si];
+        dst.v[d][di] = src.v[d][si];
+    }
+    dst.w[di] = src.w[si];
+    dst.s[di] = src.s[si];
+    dst.active[di] = src.active[si];
+    dst.id[di] = src.id[si];
+}
+
+// Position (m), velocity and weight of one particle as FP64, for either store;
+// h is the cell size of the split coordinates.
+__device__ __forceinline__ void load_state(const PopBuf &b, int64_t i, double h, double x[3], double v[3],
+                                           double &w)
+{
+    (void)h;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        x[d] = b.x[d][i];
+        v[d] = b.v[d][i];
+    }
+    w = b.w[i];
+}
+
+__device__ __forceinline__ void load_state(const SplitBuf &b, int64_t i, double h, double x[3], double v[3],
+                                           double &w)
+{
+    int32_t c[3];
+    unpack_cell(b.cell[i], c[0], c[1], c[2]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        x[d] = ((double)c[d] + (double)b.off[d][i]) * h;
+        v[d] = (double)b.v[d][i];
+    }
+    w = (double)b.w[i];
+}
+
+__device__ __forceinline__ double get_w(const PopBuf &b, int64_t i) { return b.w[i]; }
+__device__ __forceinline__ double get_w(const SplitBuf &b, int64_t i) { return (double)b.w[i]; }
+__device__ __forceinline__ void set_w(const PopBuf &b, int64_t i, double w) { b.w[i] = w; }
+__device__ __forceinline__ void set_w(const SplitBuf &b, int64_t i, double w) { b.w[i] = (float)w; }
+
+// Cell of a particle (floor of the position over h per axis), -1 outside [0, n h).
+__device__ __forceinline__ int64_t cell_key(const PopBuf &b, int64_t i, const int32_t n[3], double h)
+{
+    return cell_of(n, h, b.x[0][i], b.x[1][i], b.x[2][i]);
+}
+
+__device__ __forceinline__ int64_t cell_key(const SplitBuf &b, int64_t i, const int32_t n[3], double h)
+{
+    (void)h;
+    int32_t c[3];
+    unpack_cell(b.cell[i], c[0], c[1], c[2]);
+    int64_t q[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        q[d] = (int64_t)c[d] + (int64_t)floorf(b.off[d][i]);
+        if (q[d] < 0 || q[d] >= n[d]) return -1;
+    }
+    return q[0] + (int64_t)n[0] * (q[1] + (int64_t)n[1] * q[2]);
+}
+
+// CIC geometry of a particle: g = x/h - 1/2 per axis, its floor and fraction.
+__device__ __forceinline__ void cic_geom(const PopBuf &b, int64_t i, double h, double fl[3], double fr[3])
+{
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        double gq = b.x[d][i] / h - 0.5;
+        fl[d] = floor(gq);
+        fr[d] = gq - fl[d];
+    }
+}
+
+__device__ __forceinline__ void cic_geom(const SplitBuf &b, int64_t i, double h, double fl[3], double fr[3])
+{
+    (void)h;
+    int32_t c[3];
+    unpack_cell(b.cell[i], c[0], c[1], c[2]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const float g = b.off[d][i] - 0.5f;
+        const float f = floorf(g);
+        fl[d] = (double)c[d] + (double)f;
+        fr[d] = (double)(g - f);
+    }
+}
+

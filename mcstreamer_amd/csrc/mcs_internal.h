@@ -114,6 +114,17 @@ struct PopBuf {
     uint64_t *id = nullptr;
 };
 
+// Split-precision SoA (precision mode): packed cell coordinates (i in bits 0-9,
+// j in 10-19, k in 20-30) plus the position inside the cell in cell units, FP32.
+struct SplitBuf {
+    int32_t *cell = nullptr;
+    float *off[3] = {nullptr, nullptr, nullptr};
+    float *v[3] = {nullptr, nullptr, nullptr};
+    float *w = nullptr, *s = nullptr;
+    uint8_t *active = nullptr;
+    uint64_t *id = nullptr;
+};
+
 struct mcs_pop {
     mcs_ctx *ctx;
     int64_t cap;
@@ -121,6 +132,15 @@ struct mcs_pop {
     uint64_t next_id;
     PopBuf buf[2];
     bool has_alt = false;
+    // precision mode: the split store on a mesh grid (gn cells of gh metres)
+    int prec = 0;
+    SplitBuf sbuf[2];
+    int32_t gn[3] = {0, 0, 0};
+    double gh = 0.0;
+    int32_t *log_cell = nullptr;
+    float *log_off[3] = {nullptr, nullptr, nullptr};
+    float *log_v32[3] = {nullptr, nullptr, nullptr};
+    float *log_w32 = nullptr, *log_s32 = nullptr;
     // birth log (structure of arrays)
     int64_t logcap = 0;
     int32_t *log_parent = nullptr;
@@ -150,7 +170,12 @@ struct mcs_mesh {
     int64_t *cnt64 = nullptr;       // ncells + 2 (per-cell resample output counts)
     int64_t *pre64 = nullptr;       // ncells + 2 (their prefix)
     bool sorted_valid = false;
+    float *face32[3] = {nullptr, nullptr, nullptr};  // FP32 face fields (precision mode)
 };
+
+// FP32 copies of the face fields, allocated and filled on first use; from then on
+// mcs_efield_faces writes both precisions.
+int32_t mcs_mesh_faces32(mcs_mesh *m);
 
 struct MgLevel {
     int32_t n[3];

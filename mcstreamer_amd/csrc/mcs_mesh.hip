@@ -3,6 +3,7 @@
 #include <cstring>
 
 #include "mcs_device.h"
+#include "mcs_fp32.h"
 
 #define M_SHORT 0
 #define M_BIG 1
@@ -56,6 +57,8 @@ extern "C" int32_t mcs_mesh_destroy(mcs_mesh *m)
     hipSetDevice(m->ctx->device);
     hipStreamSynchronize(m->ctx->stream);
     for (int a = 0; a < 7; ++a) hipFree(m->arr[a]);
+    for (int d = 0; d < 3; ++d)
+        if (m->face32[d]) hipFree(m->face32[d]);
     hipFree(m->cell_start);
     hipFree(m->cell_cnt);
     hipFree(m->cnt64);
@@ -129,15 +132,15 @@ static inline MeshGeom geom(const mcs_mesh *m)
 // Counting pass.  Lanes of a wave that share a cell take consecutive ranks from a
 // single atomic (ballot over equal keys), so a mostly sorted population costs one
 // atomic per cell run rather than one per particle.
-__global__ __launch_bounds__(MCS_BLOCK) void k_sort_count(PopBuf b, int64_t n, MeshGeom g,
-                                                          int32_t *cnt, int32_t *key_out,
-                                                          int32_t *rank_out)
+template <typename Buf>
+__global__ __launch_bounds__(MCS_BLOCK) void k_sort_count(Buf b, int64_t n, MeshGeom g, int32_t *cnt,
+                                                          int32_t *key_out, int32_t *rank_out)
 {
     int64_t i = gidx();
     int lane = threadIdx.x & (MCS_WAVE - 1);
     int key = -1;
     if (i < n && b.active[i]) {
-        int64_t c = cell_of(g.n, g.h, b.x[0][i], b.x[1][i], b.x[2][i]);
+        int64_t c = cell_key(b, i, g.n, g.h);
         key = c < 0 ? (int)g.ncells : (int)c;
     }
     unsigned long long todo = __ballot(key >= 0);
@@ -158,7 +161,8 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_sort_count(PopBuf b, int64_t n, M
     }
 }
 
-__global__ __launch_bounds__(MCS_BLOCK) void k_sort_scatter(PopBuf b, int64_t n,
+template <typename Buf>
+__global__ __launch_bounds__(MCS_BLOCK) void k_sort_scatter(Buf b, int64_t n,
                                                             const int32_t *key,
                                                             const int32_t *rank,
                                                             const int32_t *cs, int32_t *perm,
@@ -249,21 +253,19 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_segsort_big(const int32_t *cs, in
     }
 }
 
-__global__ __launch_bounds__(MCS_BLOCK) void k_sort_gather(PopBuf src, PopBuf dst,
-                                                           const int32_t *perm, int64_t na)
+template <typename Buf>
+__global__ __launch_bounds__(MCS_BLOCK) void k_sort_gather(Buf src, Buf dst, const int32_t *perm, int64_t na)
 {
     int64_t pos = gidx();
     if (pos >= na) return;
-    copy_particle(src, perm[pos], dst, pos);
+    copy_rec(src, perm[pos], dst, pos);
 }
 
-extern "C" int32_t mcs_sort_by_cell(mcs_pop *p, mcs_mesh *m)
+template <typename Buf>
+static int32_t sort_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, const Buf &dst)
 {
-    if (!p || !m) return MCS_E_ARG;
     mcs_ctx *ctx = p->ctx;
-    hipSetDevice(ctx->device);
-    int32_t rc = mcs_pop_ensure_alt(p);
-    if (rc) return rc;
+    int32_t rc;
     int64_t n = p->n;
     int64_t nseg = m->ncells + 1;
     hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)cdiv(nseg + 1, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
@@ -272,16 +274,16 @@ extern "C" int32_t mcs_sort_by_cell(mcs_pop *p, mcs_mesh *m)
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, ctx->stream, (int32_t *)ctr, (int64_t)8);
     if (n > 0) {
         ProfScope ps(ctx, "sort_count");
-        hipLaunchKernelGGL(k_sort_count, dim3((unsigned)cdiv(n, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
-                           ctx->stream, p->buf[0], n, geom(m), m->cell_cnt, p->i32a, p->i32b);
+        hipLaunchKernelGGL(k_sort_count<Buf>, dim3((unsigned)cdiv(n, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                           ctx->stream, src, n, geom(m), m->cell_cnt, p->i32a, p->i32b);
     }
     MCS_LAUNCH_CHECK(ctx);
     rc = mcs_scan_i32(ctx, m->cell_cnt, m->cell_start, nseg);
     if (rc) return rc;
     if (n > 0) {
         ProfScope ps(ctx, "sort_scatter");
-        hipLaunchKernelGGL(k_sort_scatter, dim3((unsigned)cdiv(n, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
-                           ctx->stream, p->buf[0], n, (const int32_t *)p->i32a,
+        hipLaunchKernelGGL(k_sort_scatter<Buf>, dim3((unsigned)cdiv(n, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                           ctx->stream, src, n, (const int32_t *)p->i32a,
                            (const int32_t *)p->i32b, (const int32_t *)m->cell_start, p->i32c,
                            p->u64a);
         // the count buffer is free again: it holds the lists of segments to sort
@@ -294,7 +296,7 @@ extern "C" int32_t mcs_sort_by_cell(mcs_pop *p, mcs_mesh *m)
         hipLaunchKernelGGL(k_segsort_big, dim3(512), dim3(MCS_BLOCK), 0, ctx->stream,
                            (const int32_t *)m->cell_start, p->i32c, p->u64a,
                            (const int32_t *)m->cell_cnt, nseg, (const int64_t *)(ctr + M_BIG), p->i32b,
-                           (uint64_t *)p->buf[1].id);
+                           (uint64_t *)dst.id);
     }
     MCS_LAUNCH_CHECK(ctx);
     int64_t na;
@@ -307,8 +309,8 @@ extern "C" int32_t mcs_sort_by_cell(mcs_pop *p, mcs_mesh *m)
     }
     if (na > 0) {
         ProfScope ps(ctx, "sort_gather");
-        hipLaunchKernelGGL(k_sort_gather, dim3((unsigned)cdiv(na, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
-                           ctx->stream, p->buf[0], p->buf[1], (const int32_t *)p->i32c, na);
+        hipLaunchKernelGGL(k_sort_gather<Buf>, dim3((unsigned)cdiv(na, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                           ctx->stream, src, dst, (const int32_t *)p->i32c, na);
     }
     MCS_LAUNCH_CHECK(ctx);
     mcs_pop_swap(p);
@@ -317,10 +319,26 @@ extern "C" int32_t mcs_sort_by_cell(mcs_pop *p, mcs_mesh *m)
     return MCS_OK;
 }
 
+extern "C" int32_t mcs_sort_by_cell(mcs_pop *p, mcs_mesh *m)
+{
+    if (!p || !m) return MCS_E_ARG;
+    mcs_ctx *ctx = p->ctx;
+    hipSetDevice(ctx->device);
+    int32_t rc = mcs_pop_ensure_alt(p);
+    if (rc) return rc;
+    if (p->prec == MCS_PREC_F32) {
+        for (int d = 0; d < 3; ++d)
+            if (p->gn[d] != m->n[d]) return mcs_fail(ctx, MCS_E_ARG, "sort: mesh differs from the population's grid");
+        return sort_impl(p, m, p->sbuf[0], p->sbuf[1]);
+    }
+    return sort_impl(p, m, p->buf[0], p->buf[1]);
+}
+
 // ---------------------------------------------------------------- deposit
 
+template <typename Buf>
 struct DepArgs {
-    PopBuf b;
+    Buf b;
     const int32_t *cs;
     MeshGeom g;
     double *arr;
@@ -336,25 +354,24 @@ struct DepArgs {
 
 // Per particle: g = x/h - 1/2, i0 = floor(g), fr = g - i0, base = (w val)/dV,
 // exactly the quantities a sequential CIC scatter computes per particle.
-__global__ __launch_bounds__(MCS_BLOCK) void k_deposit_prep(DepArgs a)
+template <typename Buf>
+__global__ __launch_bounds__(MCS_BLOCK) void k_deposit_prep(DepArgs<Buf> a)
 {
     int64_t q = gidx();
     if (q >= a.n) return;
     const double h = a.g.h;
     const double dV = (h * h) * h;
-    double fr[3];
+    double fl[3], fr[3];
+    cic_geom(a.b, q, h, fl, fr);
     uint64_t key = 0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        double gq = a.b.x[d][q] / h - 0.5;
-        double fl = floor(gq);
-        fr[d] = gq - fl;
         // particles outside [-1, n] along an axis are never gathered: clamp the field
-        double fc = fmin(fmax(fl + 1.0, 0.0), (double)DEP_MASK);
+        double fc = fmin(fmax(fl[d] + 1.0, 0.0), (double)DEP_MASK);
         key |= ((uint64_t)fc & DEP_MASK) << (21 * d);
     }
     a.pk[q] = key;
-    a.pw[q] = make_double4(fr[0], fr[1], fr[2], (a.b.w[q] * a.val) / dV);
+    a.pw[q] = make_double4(fr[0], fr[1], fr[2], (get_w(a.b, q) * a.val) / dV);
 }
 
 __device__ __forceinline__ void dep_visit(double &acc, uint64_t key, const double4 &wv, int64_t i,
@@ -374,8 +391,8 @@ __device__ __forceinline__ void dep_visit(double &acc, uint64_t key, const doubl
 // the particles of its 27 neighbours in (cell, id) order, which is the order in
 // which a sequential scatter over the sorted population adds them.  No atomics.
 // Loads run four particles ahead of the (sequential) accumulation.
-template <int CIC>
-__global__ __launch_bounds__(MCS_BLOCK) void k_deposit(DepArgs a)
+template <int CIC, typename Buf>
+__global__ __launch_bounds__(MCS_BLOCK) void k_deposit(DepArgs<Buf> a)
 {
     int64_t c = gidx();
     if (c >= a.g.ncells) return;
@@ -388,7 +405,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_deposit(DepArgs a)
     double acc = a.arr[idx];
     if (!CIC) {
         int64_t s0 = a.cs[c], s1 = a.cs[c + 1];
-        for (int64_t q = s0; q < s1; ++q) acc += (a.b.w[q] * a.val) / dV;
+        for (int64_t q = s0; q < s1; ++q) acc += (get_w(a.b, q) * a.val) / dV;
         a.arr[idx] = acc;
         return;
     }
@@ -416,6 +433,10 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_deposit(DepArgs a)
     a.arr[idx] = acc;
 }
 
+template <typename Buf>
+static int32_t deposit_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, int32_t which, int32_t scheme,
+                            double val);
+
 extern "C" int32_t mcs_deposit(mcs_pop *p, mcs_mesh *m, int32_t which, int32_t scheme, double val)
 {
     if (!p || !m) return MCS_E_ARG;
@@ -427,8 +448,17 @@ extern "C" int32_t mcs_deposit(mcs_pop *p, mcs_mesh *m, int32_t which, int32_t s
     if (!m->sorted_valid)
         return mcs_fail(ctx, MCS_E_ARG, "deposit: population is not in cell order (call mcs_sort_by_cell)");
     hipSetDevice(ctx->device);
-    DepArgs a;
-    a.b = p->buf[0];
+    if (p->prec == MCS_PREC_F32) return deposit_impl(p, m, p->sbuf[0], which, scheme, val);
+    return deposit_impl(p, m, p->buf[0], which, scheme, val);
+}
+
+template <typename Buf>
+static int32_t deposit_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, int32_t which, int32_t scheme,
+                            double val)
+{
+    mcs_ctx *ctx = p->ctx;
+    DepArgs<Buf> a;
+    a.b = src;
     a.cs = m->cell_start;
     a.g = geom(m);
     a.arr = m->arr[which];
@@ -445,11 +475,11 @@ extern "C" int32_t mcs_deposit(mcs_pop *p, mcs_mesh *m, int32_t which, int32_t s
         dim3 g((unsigned)cdiv(m->ncells, MCS_BLOCK)), bl(MCS_BLOCK);
         if (scheme == MCS_DEPOSIT_CIC) {
             if (p->n > 0)
-                hipLaunchKernelGGL(k_deposit_prep, dim3((unsigned)cdiv(p->n, MCS_BLOCK)), bl, 0,
+                hipLaunchKernelGGL(k_deposit_prep<Buf>, dim3((unsigned)cdiv(p->n, MCS_BLOCK)), bl, 0,
                                    ctx->stream, a);
-            hipLaunchKernelGGL(k_deposit<1>, g, bl, 0, ctx->stream, a);
+            hipLaunchKernelGGL((k_deposit<1, Buf>), g, bl, 0, ctx->stream, a);
         } else {
-            hipLaunchKernelGGL(k_deposit<0>, g, bl, 0, ctx->stream, a);
+            hipLaunchKernelGGL((k_deposit<0, Buf>), g, bl, 0, ctx->stream, a);
         }
     }
     MCS_LAUNCH_CHECK(ctx);
@@ -490,7 +520,8 @@ __device__ __forceinline__ int64_t clampi(int64_t v, int64_t lo, int64_t hi)
 // Face fields (MCStreamer/src/poisson.jl:47-57).  Ghost faces take the value of
 // the nearest interior face, which is what the reference's ghost copies produce.
 __global__ __launch_bounds__(MCS_BLOCK) void k_faces(MeshGeom g, const double *u, double *ex,
-                                                     double *ey, double *ez, double eb)
+                                                     double *ey, double *ez, double eb, float *fx,
+                                                     float *fy, float *fz)
 {
     int64_t t = gidx();
     const int64_t nx = g.n[0], ny = g.n[1], nz = g.n[2];
@@ -504,6 +535,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_faces(MeshGeom g, const double *u
         int64_t jj = clampi(j, 1, ny), kk = clampi(k, 1, nz);
         int64_t c = i + ux * jj + uxy * kk;
         ex[t] = (u[c] - u[c + 1]) / h;
+        if (fx) fx[t] = (float)ex[t];
         return;
     }
     t -= lx;
@@ -512,6 +544,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_faces(MeshGeom g, const double *u
         int64_t ii = clampi(i, 1, nx), kk = clampi(k, 1, nz);
         int64_t c = ii + ux * j + uxy * kk;
         ey[t] = (u[c] - u[c + ux]) / h;
+        if (fy) fy[t] = (float)ey[t];
         return;
     }
     t -= ly;
@@ -520,6 +553,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_faces(MeshGeom g, const double *u
         int64_t ii = clampi(i, 1, nx), jj = clampi(j, 1, ny);
         int64_t c = ii + ux * jj + uxy * k;
         ez[t] = eb + (u[c] - u[c + uxy]) / h;
+        if (fz) fz[t] = (float)ez[t];
     }
 }
 
@@ -533,7 +567,28 @@ extern "C" int32_t mcs_efield_faces(mcs_mesh *m, double eb)
         ProfScope ps(ctx, "faces");
         hipLaunchKernelGGL(k_faces, dim3((unsigned)cdiv(tot, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
                            ctx->stream, geom(m), (const double *)m->arr[MCS_ARR_U],
-                           m->arr[MCS_ARR_EX], m->arr[MCS_ARR_EY], m->arr[MCS_ARR_EZ], eb);
+                           m->arr[MCS_ARR_EX], m->arr[MCS_ARR_EY], m->arr[MCS_ARR_EZ], eb, m->face32[0],
+                           m->face32[1], m->face32[2]);
+    }
+    MCS_LAUNCH_CHECK(ctx);
+    return MCS_OK;
+}
+
+__global__ void k_to_f32(float *dst, const double *src, int64_t n)
+{
+    int64_t i = gidx();
+    if (i < n) dst[i] = (float)src[i];
+}
+
+int32_t mcs_mesh_faces32(mcs_mesh *m)
+{
+    if (m->face32[0]) return MCS_OK;
+    mcs_ctx *ctx = m->ctx;
+    for (int d = 0; d < 3; ++d) {
+        const int a = MCS_ARR_EX + d;
+        MCS_HIP(ctx, hipMalloc(&m->face32[d], sizeof(float) * m->arr_len[a]));
+        hipLaunchKernelGGL(k_to_f32, dim3((unsigned)cdiv(m->arr_len[a], MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                           ctx->stream, m->face32[d], (const double *)m->arr[a], m->arr_len[a]);
     }
     MCS_LAUNCH_CHECK(ctx);
     return MCS_OK;
@@ -550,10 +605,10 @@ extern "C" int32_t mcs_efield_faces(mcs_mesh *m, double eb)
 // oracle restatement documents the rules).  WRITE = false counts the outputs;
 // WRITE = true replays the same draws and writes them: surviving originals first,
 // then the created particles, ids next_id + (created before this one).
-template <bool WRITE>
+template <bool WRITE, typename Buf>
 __device__ void resample_cell(int64_t c, int64_t a, int64_t e, int32_t ntarget, int32_t nmax,
-                              uint64_t seed, uint32_t instance, uint32_t step, const PopBuf &src,
-                              const PopBuf &dst, int64_t base, uint64_t idbase, int64_t &nout,
+                              uint64_t seed, uint32_t instance, uint32_t step, const Buf &src,
+                              const Buf &dst, int64_t base, uint64_t idbase, int64_t &nout,
                               int64_t &ncreated, int64_t *ctr)
 {
     int64_t pc = e - a;
@@ -566,21 +621,21 @@ __device__ void resample_cell(int64_t c, int64_t a, int64_t e, int32_t ntarget, 
     int64_t cnt = 0, pk = 0;
     for (int64_t i = a; i < e; ++i) {
         cnt += 1;
-        wtotal += src.w[i];
+        wtotal += get_w(src, i);
         if (cnt == nmax) xx = mcs_samplemin(&rs, (double)ntarget);
     }
     const bool merged = pc > nmax;
     const int64_t created_at = merged ? 0 : pc;
     int64_t splits = 0, removed = 0;
     for (int64_t i = a; i < e; ++i) {
-        double wi = src.w[i];
+        double wi = get_w(src, i);
         if (pc > nmax) {
             while ((wtotal * xx < wcum + wi) && pk < ntarget) {
                 pk += 1;
                 if (WRITE) {
                     int64_t d = base + created_at + ncreated;
-                    copy_particle(src, i, dst, d);
-                    dst.w[d] = wtotal / (double)ntarget;
+                    copy_rec(src, i, dst, d);
+                    set_w(dst, d, wtotal / (double)ntarget);
                     dst.id[d] = idbase + (uint64_t)ncreated;
                 }
                 ncreated += 1;
@@ -594,8 +649,8 @@ __device__ void resample_cell(int64_t c, int64_t a, int64_t e, int32_t ntarget, 
                 double wnew = rint(wi / 2.0);
                 if (WRITE) {
                     int64_t d = base + created_at + ncreated;
-                    copy_particle(src, i, dst, d);
-                    dst.w[d] = wnew;
+                    copy_rec(src, i, dst, d);
+                    set_w(dst, d, wnew);
                     dst.id[d] = idbase + (uint64_t)ncreated;
                 }
                 ncreated += 1;
@@ -603,7 +658,7 @@ __device__ void resample_cell(int64_t c, int64_t a, int64_t e, int32_t ntarget, 
                 wkeep = wi - wnew;
                 pc += 1;
             }
-            if (WRITE && wkeep != wi) dst.w[base + (i - a)] = wkeep;  // survivors copied by k_resample_copy
+            if (WRITE && wkeep != wi) set_w(dst, base + (i - a), wkeep);  // survivors copied by k_resample_copy
         }
     }
     nout = (merged ? 0 : (e - a)) + ncreated;
@@ -618,8 +673,9 @@ __device__ void resample_cell(int64_t c, int64_t a, int64_t e, int32_t ntarget, 
     }
 }
 
+template <typename Buf>
 struct ResArgs {
-    PopBuf src, dst;
+    Buf src, dst;
     const int32_t *cs;
     int64_t ncells;
     int32_t ntarget, nmax;
@@ -632,7 +688,8 @@ struct ResArgs {
     int64_t *ctr;
 };
 
-__global__ __launch_bounds__(MCS_BLOCK) void k_resample_count(ResArgs r)
+template <typename Buf>
+__global__ __launch_bounds__(MCS_BLOCK) void k_resample_count(ResArgs<Buf> r)
 {
     int64_t c = gidx();
     if (c > r.ncells) return;
@@ -642,12 +699,13 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_resample_count(ResArgs r)
         return;
     }
     int64_t nout, ncr;
-    resample_cell<false>(c, a, e, r.ntarget, r.nmax, r.seed, r.instance, r.step, r.src, r.dst, 0,
+    resample_cell<false, Buf>(c, a, e, r.ntarget, r.nmax, r.seed, r.instance, r.step, r.src, r.dst, 0,
                          0, nout, ncr, r.ctr);
     r.packed[c] = nout | (ncr << 32);
 }
 
-__global__ __launch_bounds__(MCS_BLOCK) void k_resample_write(ResArgs r)
+template <typename Buf>
+__global__ __launch_bounds__(MCS_BLOCK) void k_resample_write(ResArgs<Buf> r)
 {
     int64_t c = gidx();
     if (c > r.ncells) return;
@@ -662,24 +720,30 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_resample_write(ResArgs r)
         return;
     }
     int64_t nout, ncr;
-    resample_cell<true>(c, a, e, r.ntarget, r.nmax, r.seed, r.instance, r.step, r.src, r.dst, base,
+    resample_cell<true, Buf>(c, a, e, r.ntarget, r.nmax, r.seed, r.instance, r.step, r.src, r.dst, base,
                         r.next_id + (uint64_t)crbase, nout, ncr, nullptr);
 }
 
 // Survivors (particles of cells that are not merged, and those outside the mesh)
 // move as whole records, one thread per particle; their slot is the cell's output
 // start plus their rank in the cell.
-__global__ __launch_bounds__(MCS_BLOCK) void k_resample_copy(ResArgs r, MeshGeom g, int64_t n)
+template <typename Buf>
+__global__ __launch_bounds__(MCS_BLOCK) void k_resample_copy(ResArgs<Buf> r, MeshGeom g, int64_t n)
 {
     int64_t i = gidx();
     if (i >= n) return;
-    int64_t c = cell_of(g.n, g.h, r.src.x[0][i], r.src.x[1][i], r.src.x[2][i]);
+    int64_t c = cell_key(r.src, i, g.n, g.h);
     if (c < 0) c = r.ncells;
     int64_t a = r.cs[c];
     if (c < r.ncells && r.cs[c + 1] - a > r.nmax) return;
     int64_t base = r.prefix[c] & 0xffffffffLL;
-    copy_particle(r.src, i, r.dst, base + (i - a));
+    copy_rec(r.src, i, r.dst, base + (i - a));
 }
+
+template <typename Buf>
+static int32_t resample_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, const Buf &dst, int32_t ntarget,
+                             int32_t nmax, uint64_t seed, uint32_t instance, uint32_t step,
+                             mcs_resample_stats *stats);
 
 extern "C" int32_t mcs_resample_cells(mcs_pop *p, mcs_mesh *m, int32_t ntarget, int32_t nmax,
                                       uint64_t seed, uint32_t instance, uint32_t step,
@@ -694,13 +758,25 @@ extern "C" int32_t mcs_resample_cells(mcs_pop *p, mcs_mesh *m, int32_t ntarget, 
     hipSetDevice(ctx->device);
     int32_t rc = mcs_pop_ensure_alt(p);
     if (rc) return rc;
+    if (p->prec == MCS_PREC_F32)
+        return resample_impl(p, m, p->sbuf[0], p->sbuf[1], ntarget, nmax, seed, instance, step, stats);
+    return resample_impl(p, m, p->buf[0], p->buf[1], ntarget, nmax, seed, instance, step, stats);
+}
+
+template <typename Buf>
+static int32_t resample_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, const Buf &dst, int32_t ntarget,
+                             int32_t nmax, uint64_t seed, uint32_t instance, uint32_t step,
+                             mcs_resample_stats *stats)
+{
+    mcs_ctx *ctx = p->ctx;
+    int32_t rc;
     if (stats) memset(stats, 0, sizeof(*stats));
     int64_t nseg = m->ncells + 1;
     int64_t *ctr = ctx->d_small;
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, ctx->stream, (int32_t *)ctr, (int64_t)16);
-    ResArgs r;
-    r.src = p->buf[0];
-    r.dst = p->buf[1];
+    ResArgs<Buf> r;
+    r.src = src;
+    r.dst = dst;
     r.cs = m->cell_start;
     r.ncells = m->ncells;
     r.ntarget = ntarget;
@@ -715,7 +791,7 @@ extern "C" int32_t mcs_resample_cells(mcs_pop *p, mcs_mesh *m, int32_t ntarget, 
     r.ctr = ctr;
     {
         ProfScope ps(ctx, "resample");
-        hipLaunchKernelGGL(k_resample_count, dim3((unsigned)cdiv(nseg, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+        hipLaunchKernelGGL(k_resample_count<Buf>, dim3((unsigned)cdiv(nseg, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
                            ctx->stream, r);
     }
     MCS_LAUNCH_CHECK(ctx);
@@ -737,9 +813,9 @@ extern "C" int32_t mcs_resample_cells(mcs_pop *p, mcs_mesh *m, int32_t ntarget, 
     {
         ProfScope ps(ctx, "resample");
         if (p->n > 0)
-            hipLaunchKernelGGL(k_resample_copy, dim3((unsigned)cdiv(p->n, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+            hipLaunchKernelGGL(k_resample_copy<Buf>, dim3((unsigned)cdiv(p->n, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
                                ctx->stream, r, geom(m), p->n);
-        hipLaunchKernelGGL(k_resample_write, dim3((unsigned)cdiv(nseg, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+        hipLaunchKernelGGL(k_resample_write<Buf>, dim3((unsigned)cdiv(nseg, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
                            ctx->stream, r);
     }
     MCS_LAUNCH_CHECK(ctx);

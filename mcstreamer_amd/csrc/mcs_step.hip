@@ -3,6 +3,7 @@
 #include <cstring>
 
 #include "mcs_advance.h"
+#include "mcs_fp32.h"
 
 // counters block layout in ctx->d_small
 #define C_LOG 0
@@ -158,6 +159,206 @@ __global__ void k_fill_i64(int64_t *a, int64_t n, int64_t v)
     if (i < n) a[i] = v;
 }
 
+// ---- the same step on the split store (precision mode) ----------------------
+
+struct BirthLog32 {
+    int32_t *parent, *seq, *cell;
+    float *off[3], *v[3];
+    float *w, *s;
+    uint8_t *act;
+    int64_t cap;
+};
+
+struct AdvArgs32 {
+    SplitBuf b;
+    int64_t n0;
+    TableView32 T;
+    MeshView32 M;
+    float acc[3];      // q/m E of a uniform field
+    float dt, inv_h;
+    int32_t nmax[3];   // grid extent for rebasing
+    uint64_t seed;
+    uint32_t instance, step;
+    BirthLog32 L;
+    int32_t *loff;
+    int64_t *btot;
+    int64_t *ctr;
+};
+
+template <int FIELD_MESH, int TRACK>
+__global__ __launch_bounds__(MCS_BLOCK) void k_advance32(AdvArgs32 a)
+{
+    __shared__ int s_ev[MCS_MAX_PROC];
+    __shared__ int s_flights, s_deaths, s_flags;
+    if (threadIdx.x < MCS_MAX_PROC) s_ev[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        s_flights = 0;
+        s_deaths = 0;
+        s_flags = 0;
+    }
+    __syncthreads();
+    const int64_t i = gidx();
+    int nb = 0;
+    int32_t flags = 0;
+    int flights = 0, deaths = 0;
+    if (i < a.n0 && a.b.active[i]) {
+        int32_t c[3];
+        unpack_cell(a.b.cell[i], c[0], c[1], c[2]);
+        float off[3] = {a.b.off[0][i], a.b.off[1][i], a.b.off[2][i]};
+        float v[3] = {a.b.v[0][i], a.b.v[1][i], a.b.v[2][i]};
+        const float w = a.b.w[i];
+        float s = a.b.s[i];
+        uint8_t act = 1;
+        mcs_stream rs;
+        mcs_stream_init(&rs, a.seed, a.instance, MCS_PHASE_ADVANCE, a.b.id[i], a.step);
+        advance_one32<FIELD_MESH, TRACK>(
+            c, off, v, w, s, act, rs, a.dt, a.T, a.M, a.acc, a.inv_h, flags, flights, deaths,
+            [&](const int32_t *bc, const float *boff, const float *bv, float bs, uint8_t bact) {
+                unsigned long long slot = atomicAdd((unsigned long long *)&a.ctr[C_LOG], 1ULL);
+                if ((int64_t)slot < a.L.cap) {
+                    a.L.parent[slot] = (int32_t)i;
+                    a.L.seq[slot] = nb;
+                    a.L.cell[slot] = pack_cell(bc[0], bc[1], bc[2]);
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        a.L.off[d][slot] = boff[d];
+                        a.L.v[d][slot] = bv[d];
+                    }
+                    a.L.w[slot] = w;
+                    a.L.s[slot] = bs;
+                    a.L.act[slot] = bact;
+                } else {
+                    flags |= MCS_FLAG_CAPACITY;
+                }
+                nb += 1;
+            },
+            [&](int j) { atomicAdd(&s_ev[j], 1); });
+        rebase32(c, off, a.nmax);
+        a.b.cell[i] = pack_cell(c[0], c[1], c[2]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            a.b.off[d][i] = off[d];
+            a.b.v[d][i] = v[d];
+        }
+        a.b.s[i] = s;
+        if (!act) a.b.active[i] = 0;
+    }
+    int tot;
+    int ex = block_excl_scan<int>(nb, &tot);
+    if (nb > 0) a.loff[i] = ex;
+    if (threadIdx.x == 0) a.btot[blockIdx.x] = tot;
+    if (flights) atomicAdd(&s_flights, flights);
+    if (deaths) atomicAdd(&s_deaths, deaths);
+    if (flags) atomicOr(&s_flags, flags);
+    __syncthreads();
+    if (threadIdx.x < a.T.nproc && s_ev[threadIdx.x])
+        atomicAdd((unsigned long long *)&a.ctr[C_EVENTS + threadIdx.x], (unsigned long long)s_ev[threadIdx.x]);
+    if (threadIdx.x == 0) {
+        if (s_flights) atomicAdd((unsigned long long *)&a.ctr[C_FLIGHTS], (unsigned long long)s_flights);
+        if (s_deaths) atomicAdd((unsigned long long *)&a.ctr[C_DEATHS], (unsigned long long)s_deaths);
+        if (s_flags) atomicOr((unsigned long long *)&a.ctr[C_FLAGS], (unsigned long long)s_flags);
+    }
+}
+
+__global__ __launch_bounds__(MCS_BLOCK) void k_place_births32(BirthLog32 L, int64_t count, const int32_t *loff,
+                                                              const int64_t *bpre, int64_t n0, uint64_t next_id,
+                                                              SplitBuf b, int32_t n0x, int32_t n1x, int32_t n2x)
+{
+    int64_t r = gidx();
+    if (r >= count) return;
+    int32_t par = L.parent[r];
+    int64_t dst = n0 + bpre[par / MCS_BLOCK] + loff[par] + L.seq[r];
+    int32_t c[3];
+    unpack_cell(L.cell[r], c[0], c[1], c[2]);
+    float off[3] = {L.off[0][r], L.off[1][r], L.off[2][r]};
+    const int32_t nmax[3] = {n0x, n1x, n2x};
+    rebase32(c, off, nmax);
+    b.cell[dst] = pack_cell(c[0], c[1], c[2]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        b.off[d][dst] = off[d];
+        b.v[d][dst] = L.v[d][r];
+    }
+    b.w[dst] = L.w[r];
+    b.s[dst] = L.s[r];
+    b.active[dst] = L.act[r];
+    b.id[dst] = next_id + (uint64_t)(dst - n0);
+}
+
+static BirthLog32 birth_log32(mcs_pop *p)
+{
+    BirthLog32 L;
+    L.parent = p->log_parent;
+    L.seq = p->log_seq;
+    L.cell = p->log_cell;
+    for (int d = 0; d < 3; ++d) {
+        L.off[d] = p->log_off[d];
+        L.v[d] = p->log_v32[d];
+    }
+    L.w = p->log_w32;
+    L.s = p->log_s32;
+    L.act = p->log_act;
+    L.cap = p->logcap;
+    return L;
+}
+
+static int32_t launch_advance32(mcs_pop *p, const mcs_table *t, const mcs_field *field, double dt,
+                                mcs_mesh *tracker, uint64_t seed, uint32_t instance, uint32_t step,
+                                int64_t n0, int64_t nblk, int64_t *btot, int64_t *ctr)
+{
+    mcs_ctx *ctx = p->ctx;
+    mcs_mesh *m = field->kind == MCS_FIELD_MESH ? field->mesh : tracker;
+    if (m)
+        for (int d = 0; d < 3; ++d)
+            if (m->n[d] != p->gn[d] || m->h != p->gh)
+                return mcs_fail(ctx, MCS_E_ARG, "advance: the mesh differs from the population's grid");
+    AdvArgs32 a;
+    a.b = p->sbuf[0];
+    a.n0 = n0;
+    a.T = table_view32(t);
+    memset(&a.M, 0, sizeof(a.M));
+    if (m) {
+        if (field->kind == MCS_FIELD_MESH) {
+            int32_t rc = mcs_mesh_faces32(m);
+            if (rc) return rc;
+        }
+        for (int d = 0; d < 3; ++d) a.M.n[d] = m->n[d];
+        a.M.h = (float)m->h;
+        a.M.inv_h = (float)(1.0 / m->h);
+        a.M.ex = m->face32[0];
+        a.M.ey = m->face32[1];
+        a.M.ez = m->face32[2];
+        a.M.qfixed = m->arr[MCS_ARR_QFIXED];
+        a.M.inv_dv = 1.0 / ((m->h * m->h) * m->h);
+    }
+    for (int d = 0; d < 3; ++d) {
+        a.acc[d] = (float)(MCS_QM * field->e[d]);
+        a.nmax[d] = p->gn[d];
+    }
+    a.dt = (float)dt;
+    a.inv_h = (float)(1.0 / p->gh);
+    a.seed = seed;
+    a.instance = instance;
+    a.step = step;
+    a.L = birth_log32(p);
+    a.loff = p->i32a;
+    a.btot = btot;
+    a.ctr = ctr;
+    ProfScope ps(ctx, "advance");
+    const int fm = field->kind == MCS_FIELD_MESH, tr = tracker != nullptr;
+    dim3 g((unsigned)nblk), bl(MCS_BLOCK);
+    if (fm && tr)
+        hipLaunchKernelGGL((k_advance32<1, 1>), g, bl, 0, ctx->stream, a);
+    else if (fm)
+        hipLaunchKernelGGL((k_advance32<1, 0>), g, bl, 0, ctx->stream, a);
+    else if (tr)
+        hipLaunchKernelGGL((k_advance32<0, 1>), g, bl, 0, ctx->stream, a);
+    else
+        hipLaunchKernelGGL((k_advance32<0, 0>), g, bl, 0, ctx->stream, a);
+    MCS_LAUNCH_CHECK(ctx);
+    return MCS_OK;
+}
+
 extern "C" int32_t mcs_advance(mcs_pop *p, const mcs_table *t, const mcs_field *field, double dt,
                                mcs_mesh *tracker, uint64_t seed, uint32_t instance, uint32_t step,
                                mcs_step_stats *stats)
@@ -181,6 +382,12 @@ extern "C" int32_t mcs_advance(mcs_pop *p, const mcs_table *t, const mcs_field *
     int64_t *ctr = ctx->d_small;
     hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, ctx->stream, ctr, (int64_t)64, (int64_t)0);
 
+    if (field->kind == MCS_FIELD_MESH && tracker && tracker != field->mesh)
+        return mcs_fail(ctx, MCS_E_ARG, "advance: tracker mesh must be the field mesh");
+    if (p->prec == MCS_PREC_F32) {
+        int32_t rc = launch_advance32(p, t, field, dt, tracker, seed, instance, step, n0, nblk, btot, ctr);
+        if (rc) return rc;
+    } else {
     AdvArgs a;
     a.b = p->buf[0];
     a.n0 = n0;
@@ -212,6 +419,7 @@ extern "C" int32_t mcs_advance(mcs_pop *p, const mcs_table *t, const mcs_field *
             hipLaunchKernelGGL((k_advance<0, 0>), g, bl, 0, ctx->stream, a);
     }
     MCS_LAUNCH_CHECK(ctx);
+    }
     int32_t rc = mcs_scan_i64(ctx, btot, bpre, nblk);
     if (rc) return rc;
     MCS_HIP(ctx, hipMemcpyAsync(ctr + C_AUX, bpre + nblk, sizeof(int64_t), hipMemcpyDeviceToDevice,
@@ -235,6 +443,11 @@ extern "C" int32_t mcs_advance(mcs_pop *p, const mcs_table *t, const mcs_field *
                         (long long)n0, (long long)births, (long long)p->cap);
     }
     if (births > 0) {
+        if (p->prec == MCS_PREC_F32)
+            hipLaunchKernelGGL(k_place_births32, dim3((unsigned)cdiv(births, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                               ctx->stream, birth_log32(p), births, (const int32_t *)p->i32a,
+                               (const int64_t *)bpre, n0, p->next_id, p->sbuf[0], p->gn[0], p->gn[1], p->gn[2]);
+        else
         hipLaunchKernelGGL(k_place_births, dim3((unsigned)cdiv(births, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
                            ctx->stream, birth_log(p), births, (const int32_t *)p->i32a,
                            (const int64_t *)bpre, n0, p->next_id, p->buf[0]);
@@ -306,6 +519,8 @@ extern "C" int32_t mcs_repack(mcs_pop *p)
 {
     if (!p) return MCS_E_ARG;
     mcs_ctx *ctx = p->ctx;
+    if (p->prec == MCS_PREC_F32)
+        return mcs_fail(ctx, MCS_E_ARG, "repack: not available on a split-precision population");
     hipSetDevice(ctx->device);
     int64_t n = p->n;
     if (n == 0) return MCS_OK;
@@ -394,6 +609,8 @@ extern "C" int32_t mcs_swarm_resample(mcs_pop *p, int64_t n0, double tol, uint64
 {
     if (!p || n0 <= 0) return MCS_E_ARG;
     mcs_ctx *ctx = p->ctx;
+    if (p->prec == MCS_PREC_F32)
+        return mcs_fail(ctx, MCS_E_ARG, "swarm_resample: not available on a split-precision population");
     hipSetDevice(ctx->device);
     if (did) *did = 0;
     int64_t n = p->n;

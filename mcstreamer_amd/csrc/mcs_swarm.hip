@@ -129,25 +129,6 @@ __device__ __forceinline__ T bscan(T v, T *total)
     return pre + x - v;
 }
 
-__device__ __forceinline__ void copy_rec(const PopBuf &src, int64_t si, const PopBuf &dst, int64_t di)
-{
-    copy_particle(src, si, dst, di);
-}
-
-__device__ __forceinline__ void copy_rec(const SplitBuf &src, int64_t si, const SplitBuf &dst, int64_t di)
-{
-    dst.cell[di] = src.cell[si];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        dst.off[d][di] = src.off[d][si];
-        dst.v[d][di] = src.v[d][si];
-    }
-    dst.w[di] = src.w[si];
-    dst.s[di] = src.s[si];
-    dst.active[di] = src.active[si];
-    dst.id[di] = src.id[si];
-}
-
 // Tallies of one particle state (Swarm.jl:126-130,152,156 with collrate :242-248):
 // count, v_z, energy, summed rate of the ionization rows, of the attachment rows.
 __device__ __forceinline__ void tally_add(Neu2 acc[SW_NT], const double v[3], const TableView &T)

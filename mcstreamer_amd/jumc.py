@@ -144,14 +144,26 @@ class Population:
     (JuMC.Population, population.jl:26-36).  `seed` and `instance` key the
     per-particle Philox streams; `step` counts the advance! calls."""
 
-    def __init__(self, max_particles, particles=None, collisions=None, seed=0, instance=0, ctx=None):
+    def __init__(self, max_particles, particles=None, collisions=None, seed=0, instance=0, ctx=None,
+                 precision="f64", mesh=None):
         self.ctx = ctx or context()
         self.collisions = collisions
         self.seed = int(seed)
         self.instance = int(instance)
         self.step = 0
+        self.precision = precision
         h = ctypes.c_void_p()
-        check(self.ctx.handle, _lib.lib().mcs_pop_create(self.ctx.handle, int(max_particles), ctypes.byref(h)))
+        if precision == "f32":
+            # split store on the grid of `mesh` (a GridFields)
+            if mesh is None:
+                raise ValueError("a split-precision population needs the mesh of its grid")
+            check(self.ctx.handle, _lib.lib().mcs_pop_create_split(self.ctx.handle, int(max_particles),
+                                                                   mesh.handle, ctypes.byref(h)))
+        elif precision == "f64":
+            check(self.ctx.handle, _lib.lib().mcs_pop_create(self.ctx.handle, int(max_particles),
+                                                             ctypes.byref(h)))
+        else:
+            raise ValueError(f"unknown precision {precision!r}")
         self.handle = h
         self.capacity = int(max_particles)
         if particles is not None:

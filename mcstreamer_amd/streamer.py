@@ -188,10 +188,14 @@ def seed_pairs(grid, n, center, sigma, rng):
     return x, v, w, s, act
 
 
-def make_config1(n=(128, 128, 256), h=5e-6, npairs=1_000_000, seed=7, levels=None, cap=None,
-                 ctx=None):
-    """The FP64 streamer workload: C0 gas + 20% attacher, E = 5 MV/m along z,
-    Gaussian seed (sigma 50 um) at (Lx/2, Ly/2, 0.8 Lz), CIC, V(2,2) cycles."""
+def make_streamer(n=(128, 128, 256), h=5e-6, npairs=1_000_000, seed=7, levels=None, cap=None,
+                  precision="f64", tolerance=-1e-8, sigma=50e-6, ctx=None):
+    """A streamer workload on an n-cell box of cell size h: C0 gas + 20% attacher,
+    E = 5 MV/m along z, neutral Gaussian seed of npairs (width sigma) at
+    (Lx/2, Ly/2, 0.8 Lz), CIC, V(2,2) cycles to `tolerance` (negative: relative),
+    merge/split 8..64 per cell, dt = 2e-13 s.  precision "f32" gives the split
+    particle store, the FP32 step and FP32 correction cycles inside FP64
+    refinement; "f64" the FP64 path."""
     from . import gases, lxcat
     from .jumc import CollisionTable, Population, ParticleStates
     import os
@@ -200,16 +204,18 @@ def make_config1(n=(128, 128, 256), h=5e-6, npairs=1_000_000, seed=7, levels=Non
     ctx = ctx or context()
     grid = Grid3(n, h)
     e_ev = np.geomspace(1e-3, 1e3, 4000)
-    path = os.path.join(tempfile.gettempdir(), "mcs_gas_c1.json")
+    path = os.path.join(tempfile.gettempdir(), f"mcs_gas_c1_{os.getpid()}.json")
     gases.write_gas(path, e_ev, with_attacher=True)
     proc, rate, maxrate, _ = lxcat.load_lxcat([path], gases.densities(0.2), e_ev * co.eV)
     table = CollisionTable(proc, e_ev * co.eV, rate, maxrate, egrid="log", ctx=ctx)
     rng = np.random.default_rng(seed)
     L = np.array(n, dtype=np.float64) * h
-    x, v, w, s, act = seed_pairs(grid, npairs, (L[0] / 2, L[1] / 2, 0.8 * L[2]), 50e-6 * (h / 5e-6), rng)
+    x, v, w, s, act = seed_pairs(grid, npairs, (L[0] / 2, L[1] / 2, 0.8 * L[2]), sigma, rng)
     cap = cap or max(4 * npairs, 1 << 20)
-    popl = Population(cap, ParticleStates(x, v, w, s, act), table, seed=seed, ctx=ctx)
     fields = GridFields(grid, ctx=ctx)
+    popl = Population(cap, None, table, seed=seed, ctx=ctx, precision=precision, mesh=fields)
+    popl.upload(ParticleStates(x, v, w, s, act))
+    del x, v, w, s, act
     # ions of the neutral seed: NGP deposit of +1 per electron into qfixed (main.jl:130-132)
     sort_by_cell(popl, fields)
     density(fields, "qfixed", popl, 1.0, "ngp")
@@ -224,8 +230,24 @@ def make_config1(n=(128, 128, 256), h=5e-6, npairs=1_000_000, seed=7, levels=Non
         bc=[(mgmod.BottomBnd, mgmod.Neumann), (mgmod.TopBnd, mgmod.Neumann),
             (mgmod.LeftBnd, mgmod.Neumann), (mgmod.RightBnd, mgmod.Neumann),
             (mgmod.FrontBnd, mgmod.Dirichlet), (mgmod.BackBnd, mgmod.Dirichlet)],
-        s=s_, levels=levels, tolerance=-1e-8, smooth1=2, smooth2=2, maxiter=50)
+        s=s_, levels=levels, tolerance=tolerance, smooth1=2, smooth2=2, maxiter=50, precision=precision)
     ws = mgmod.Workspace(conf, n, ctx=ctx)
     dt = 2e-13
     return dict(grid=grid, table=table, popl=popl, fields=fields, ws=ws, conf=conf, eb=eb, dt=dt,
                 ntarget=8, nmax=64)
+
+
+def make_config1(n=(128, 128, 256), h=5e-6, npairs=1_000_000, seed=7, levels=None, cap=None, ctx=None):
+    """The FP64 streamer workload: 128x128x256 cells of 5 um, 1e6-pair seed of
+    width 50 um, V(2,2) to 1e-8 relative residual."""
+    return make_streamer(n, h, npairs, seed, levels, cap, "f64", -1e-8, 50e-6 * (h / 5e-6), ctx)
+
+
+def make_config2(precision="f32", n=(256, 256, 512), h=2.5e-6, npairs=40_000_000, seed=7, levels=None,
+                 cap=None, sigma=100e-6, ctx=None):
+    """The precision form on the same 0.64 x 0.64 x 1.28 mm box at 2.5 um cells:
+    4e7 seed pairs (width 100 um, so that no cell starts above the 64-particle
+    merge threshold and the super-particle count stays near 4e7), Poisson to
+    1e-10 relative residual.  precision "f64" builds the FP64 reference run."""
+    return make_streamer(n, h, npairs, seed, levels, cap or int(npairs * 1.5), precision, -1e-10, sigma,
+                         ctx)
