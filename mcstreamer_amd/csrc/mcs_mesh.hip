@@ -2,6 +2,8 @@
 // deposition, charge assembly, face fields and per-cell resampling.
 #include <cstring>
 
+#include <vector>
+
 #include "mcs_device.h"
 #include "mcs_fp32.h"
 
@@ -228,6 +230,10 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_segsort_wave(const int32_t *cs, i
     }
 }
 
+#define SEG_HUGE 4096
+#define HUGE_TILE 2048
+
+// One block per segment of 65..4096 particles (longer ones take the grid-wide path).
 __global__ __launch_bounds__(MCS_BLOCK) void k_segsort_big(const int32_t *cs, int32_t *perm,
                                                            uint64_t *sid, const int32_t *lists,
                                                            int64_t nseg, const int64_t *nbig,
@@ -237,6 +243,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_segsort_big(const int32_t *cs, in
     for (int64_t q = blockIdx.x; q < nb; q += gridDim.x) {
         int32_t c = lists[nseg - 1 - q];
         int64_t a = cs[c], e = cs[c + 1];
+        if (e - a > SEG_HUGE) continue;
         for (int64_t t = a + threadIdx.x; t < e; t += blockDim.x) {
             uint64_t key = sid[t];
             int64_t rank = 0;
@@ -251,6 +258,44 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_segsort_big(const int32_t *cs, in
         }
         __syncthreads();
     }
+}
+
+// Segments above SEG_HUGE (in practice the bucket of particles outside the mesh):
+// rank of each element = number of smaller ids in the segment, counted by a 2D
+// grid of (element block, id tile) with the tile in LDS, accumulated atomically.
+__global__ __launch_bounds__(MCS_BLOCK) void k_rank_huge(const uint64_t *sid, int64_t a, int64_t len,
+                                                         int32_t *rank)
+{
+    __shared__ uint64_t tile[HUGE_TILE];
+    const int64_t t0 = (int64_t)blockIdx.y * HUGE_TILE;
+    const int64_t tn = len - t0 < HUGE_TILE ? len - t0 : HUGE_TILE;
+    for (int64_t u = threadIdx.x; u < tn; u += blockDim.x) tile[u] = sid[a + t0 + u];
+    __syncthreads();
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= len) return;
+    const uint64_t key = sid[a + t];
+    int cnt = 0;
+    for (int64_t u = 0; u < tn; ++u) cnt += tile[u] < key ? 1 : 0;
+    if (cnt) atomicAdd(&rank[t], cnt);
+}
+
+__global__ __launch_bounds__(MCS_BLOCK) void k_rank_scatter(const int32_t *perm, const uint64_t *sid, int64_t a,
+                                                            int64_t len, const int32_t *rank, int32_t *tmp_perm,
+                                                            uint64_t *tmp_sid)
+{
+    const int64_t t = gidx();
+    if (t >= len) return;
+    tmp_perm[a + rank[t]] = perm[a + t];
+    tmp_sid[a + rank[t]] = sid[a + t];
+}
+
+__global__ __launch_bounds__(MCS_BLOCK) void k_copy_back_seg(int32_t *perm, uint64_t *sid, int64_t a, int64_t len,
+                                                             const int32_t *tmp_perm, const uint64_t *tmp_sid)
+{
+    const int64_t t = gidx();
+    if (t >= len) return;
+    perm[a + t] = tmp_perm[a + t];
+    sid[a + t] = tmp_sid[a + t];
 }
 
 template <typename Buf>
@@ -297,6 +342,31 @@ static int32_t sort_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, const Buf &dst
                            (const int32_t *)m->cell_start, p->i32c, p->u64a,
                            (const int32_t *)m->cell_cnt, nseg, (const int64_t *)(ctr + M_BIG), p->i32b,
                            (uint64_t *)dst.id);
+        // segments longer than SEG_HUGE: read their list back and rank each on the grid
+        int64_t nbig = 0;
+        MCS_HIP(ctx, hipMemcpyAsync(&nbig, ctr + M_BIG, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+        MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (nbig > 0) {
+            std::vector<int32_t> big((size_t)nbig);
+            MCS_HIP(ctx, hipMemcpy(big.data(), m->cell_cnt + (nseg - nbig), sizeof(int32_t) * nbig,
+                                   hipMemcpyDeviceToHost));
+            for (int32_t c : big) {
+                int32_t se[2];
+                MCS_HIP(ctx, hipMemcpy(se, m->cell_start + c, sizeof(int32_t) * 2, hipMemcpyDeviceToHost));
+                const int64_t a = se[0], len = (int64_t)se[1] - se[0];
+                if (len <= SEG_HUGE) continue;
+                hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)cdiv(len, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                                   ctx->stream, p->i32a, len);
+                hipLaunchKernelGGL(k_rank_huge, dim3((unsigned)cdiv(len, MCS_BLOCK), (unsigned)cdiv(len, HUGE_TILE)),
+                                   dim3(MCS_BLOCK), 0, ctx->stream, (const uint64_t *)p->u64a, a, len, p->i32a);
+                hipLaunchKernelGGL(k_rank_scatter, dim3((unsigned)cdiv(len, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                                   ctx->stream, (const int32_t *)p->i32c, (const uint64_t *)p->u64a, a, len,
+                                   (const int32_t *)p->i32a, p->i32b, (uint64_t *)dst.id);
+                hipLaunchKernelGGL(k_copy_back_seg, dim3((unsigned)cdiv(len, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                                   ctx->stream, p->i32c, p->u64a, a, len, (const int32_t *)p->i32b,
+                                   (const uint64_t *)dst.id);
+            }
+        }
     }
     MCS_LAUNCH_CHECK(ctx);
     int64_t na;

@@ -202,3 +202,24 @@ def test_repack_moments_swarm_resample_parity():
             rc, did_r = R.swarm_resample(P, n0, 0.1, seed=2, step=step)
             assert did == bool(did_r)
             _same_pop(pop, P)
+
+
+def test_sort_handles_huge_segments():
+    # one cell with 12000 particles and 9000 outside the mesh: segments above the
+    # block-sort size take the grid-wide rank path; order must equal the oracle's
+    n, h = (4, 4, 4), 1e-5
+    N = 30000
+    pop, fields, P, M, _ = _pair(n, h, N, 4)
+    rng = np.random.default_rng(1)
+    x = rng.uniform(0, 4e-5, size=(N, 3))
+    x[:12000] = 1.1e-5 + rng.uniform(0, 0.8e-5, size=(12000, 3))
+    x[12000:21000, 2] = 5e-5 + rng.uniform(0, 1e-4, size=9000)
+    ps = pop.download()
+    ps.x[:] = x
+    perm = rng.permutation(N).astype(np.uint64)
+    ps.id = perm
+    pop.upload(ps)
+    P = R.Pop(x, ps.v, ps.w, ps.s, ps.active, perm, cap=P.cap)
+    streamer.sort_by_cell(pop, fields)
+    R.sort_by_cell(P, M)
+    _same_pop(pop, P)
