@@ -207,6 +207,22 @@ struct mcs_mg {
 
 __device__ __forceinline__ int64_t gidx() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
 
+// XCD-aware block order.  gfx950 dispatches the workgroups of a 1D grid
+// round-robin over its 8 XCDs (block b on XCD b % 8), each with its own L2.
+// Renumbering so that XCD x runs one contiguous range of logical blocks keeps
+// neighbouring rows and planes of a stencil, and the cells of neighbouring
+// particles, in the same L2.  A bijection on [0, gridDim.x).
+#define MCS_NXCD 8u
+__device__ __forceinline__ uint32_t xcd_block()
+{
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t per = nb / MCS_NXCD, rem = nb % MCS_NXCD;
+    const uint32_t x = b % MCS_NXCD, k = b / MCS_NXCD;
+    return x * per + (x < rem ? x : rem) + k;
+}
+
+__device__ __forceinline__ int64_t xcd_gidx() { return (int64_t)xcd_block() * blockDim.x + threadIdx.x; }
+
 static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // exclusive scan of int32 counts (len entries) into out (len+1 entries, out[len] = total).

@@ -9,6 +9,7 @@
 
 #define M_SHORT 0
 #define M_BIG 1
+#define M_HUGE 8   // count, then (start, length) pairs
 
 // ---------------------------------------------------------------- mesh object
 
@@ -185,21 +186,36 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_sort_scatter(Buf b, int64_t n,
 // particles go to a list sorted one wave per segment (k_segsort_wave), longer
 // ones to a list sorted one block per segment (k_segsort_big).  The lists share
 // one buffer: short ones fill it from the front, long ones from the back.
+#define SEG_HUGE 4096
+#define HUGE_MAX 200   // (start, length) pairs recorded after the counters in ctx->d_small
+
 __global__ __launch_bounds__(MCS_BLOCK) void k_segsort_list(const int32_t *cs, int64_t nseg,
                                                             int32_t *lists, int64_t *nshort,
-                                                            int64_t *nbig)
+                                                            int64_t *nbig, int64_t *huge)
 {
-    int64_t c = gidx();
-    if (c >= nseg) return;
-    int64_t len = (int64_t)cs[c + 1] - cs[c];
-    if (len <= 1) return;
-    if (len > SEG_SMALL) {
-        unsigned long long k = atomicAdd((unsigned long long *)nbig, 1ULL);
-        lists[nseg - 1 - (int64_t)k] = (int32_t)c;
-    } else {
-        unsigned long long k = atomicAdd((unsigned long long *)nshort, 1ULL);
-        lists[k] = (int32_t)c;
+    const int64_t c = gidx();
+    const int64_t len = c < nseg ? (int64_t)cs[c + 1] - cs[c] : 0;
+    if (len > SEG_HUGE) {
+        unsigned long long k = atomicAdd((unsigned long long *)huge, 1ULL);
+        if (k < HUGE_MAX) {
+            huge[1 + 2 * k] = cs[c];
+            huge[2 + 2 * k] = len;
+        }
     }
+    // one atomic per block and list (the counters are single addresses shared by
+    // the whole grid): threads take consecutive entries in block order
+    const int fs = (len > 1 && len <= SEG_SMALL) ? 1 : 0, fb = len > SEG_SMALL ? 1 : 0;
+    int ts, tb;
+    const int es = block_excl_scan<int>(fs, &ts);
+    const int eb = block_excl_scan<int>(fb, &tb);
+    __shared__ unsigned long long base_s, base_b;
+    if (threadIdx.x == 0) {
+        base_s = ts ? atomicAdd((unsigned long long *)nshort, (unsigned long long)ts) : 0ULL;
+        base_b = tb ? atomicAdd((unsigned long long *)nbig, (unsigned long long)tb) : 0ULL;
+    }
+    __syncthreads();
+    if (fs) lists[base_s + es] = (int32_t)c;
+    if (fb) lists[nseg - 1 - (int64_t)(base_b + eb)] = (int32_t)c;
 }
 
 // One wave per short segment: lane l holds element l; its rank is the number of
@@ -230,7 +246,6 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_segsort_wave(const int32_t *cs, i
     }
 }
 
-#define SEG_HUGE 4096
 #define HUGE_TILE 2048
 
 // One block per segment of 65..4096 particles (longer ones take the grid-wide path).
@@ -316,7 +331,7 @@ static int32_t sort_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, const Buf &dst
     hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)cdiv(nseg + 1, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
                        ctx->stream, m->cell_cnt, nseg + 1);
     int64_t *ctr = ctx->d_small;
-    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, ctx->stream, (int32_t *)ctr, (int64_t)8);
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, ctx->stream, (int32_t *)ctr, (int64_t)(2 * (M_HUGE + 1)));
     if (n > 0) {
         ProfScope ps(ctx, "sort_count");
         hipLaunchKernelGGL(k_sort_count<Buf>, dim3((unsigned)cdiv(n, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
@@ -334,7 +349,7 @@ static int32_t sort_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, const Buf &dst
         // the count buffer is free again: it holds the lists of segments to sort
         hipLaunchKernelGGL(k_segsort_list, dim3((unsigned)cdiv(nseg, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
                            ctx->stream, (const int32_t *)m->cell_start, nseg, m->cell_cnt,
-                           ctr + M_SHORT, ctr + M_BIG);
+                           ctr + M_SHORT, ctr + M_BIG, ctr + M_HUGE);
         hipLaunchKernelGGL(k_segsort_wave, dim3(2048), dim3(MCS_BLOCK), 0, ctx->stream,
                            (const int32_t *)m->cell_start, p->i32c, p->u64a,
                            (const int32_t *)m->cell_cnt, (const int64_t *)(ctr + M_SHORT));
@@ -342,30 +357,26 @@ static int32_t sort_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, const Buf &dst
                            (const int32_t *)m->cell_start, p->i32c, p->u64a,
                            (const int32_t *)m->cell_cnt, nseg, (const int64_t *)(ctr + M_BIG), p->i32b,
                            (uint64_t *)dst.id);
-        // segments longer than SEG_HUGE: read their list back and rank each on the grid
-        int64_t nbig = 0;
-        MCS_HIP(ctx, hipMemcpyAsync(&nbig, ctr + M_BIG, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+        // segments longer than SEG_HUGE: their (start, length) pairs come back in one
+        // read; each is ranked on the grid
+        int64_t hv[1 + 2 * HUGE_MAX];
+        MCS_HIP(ctx, hipMemcpyAsync(hv, ctr + M_HUGE, sizeof(hv), hipMemcpyDeviceToHost, ctx->stream));
         MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        if (nbig > 0) {
-            std::vector<int32_t> big((size_t)nbig);
-            MCS_HIP(ctx, hipMemcpy(big.data(), m->cell_cnt + (nseg - nbig), sizeof(int32_t) * nbig,
-                                   hipMemcpyDeviceToHost));
-            for (int32_t c : big) {
-                int32_t se[2];
-                MCS_HIP(ctx, hipMemcpy(se, m->cell_start + c, sizeof(int32_t) * 2, hipMemcpyDeviceToHost));
-                const int64_t a = se[0], len = (int64_t)se[1] - se[0];
-                if (len <= SEG_HUGE) continue;
-                hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)cdiv(len, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
-                                   ctx->stream, p->i32a, len);
-                hipLaunchKernelGGL(k_rank_huge, dim3((unsigned)cdiv(len, MCS_BLOCK), (unsigned)cdiv(len, HUGE_TILE)),
-                                   dim3(MCS_BLOCK), 0, ctx->stream, (const uint64_t *)p->u64a, a, len, p->i32a);
-                hipLaunchKernelGGL(k_rank_scatter, dim3((unsigned)cdiv(len, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
-                                   ctx->stream, (const int32_t *)p->i32c, (const uint64_t *)p->u64a, a, len,
-                                   (const int32_t *)p->i32a, p->i32b, (uint64_t *)dst.id);
-                hipLaunchKernelGGL(k_copy_back_seg, dim3((unsigned)cdiv(len, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
-                                   ctx->stream, p->i32c, p->u64a, a, len, (const int32_t *)p->i32b,
-                                   (const uint64_t *)dst.id);
-            }
+        if (hv[0] > HUGE_MAX)
+            return mcs_fail(ctx, MCS_E_CAPACITY, "sort: more than %d cells hold over %d particles", HUGE_MAX,
+                            SEG_HUGE);
+        for (int64_t q = 0; q < hv[0]; ++q) {
+            const int64_t a = hv[1 + 2 * q], len = hv[2 + 2 * q];
+            hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)cdiv(len, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                               ctx->stream, p->i32a, len);
+            hipLaunchKernelGGL(k_rank_huge, dim3((unsigned)cdiv(len, MCS_BLOCK), (unsigned)cdiv(len, HUGE_TILE)),
+                               dim3(MCS_BLOCK), 0, ctx->stream, (const uint64_t *)p->u64a, a, len, p->i32a);
+            hipLaunchKernelGGL(k_rank_scatter, dim3((unsigned)cdiv(len, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                               ctx->stream, (const int32_t *)p->i32c, (const uint64_t *)p->u64a, a, len,
+                               (const int32_t *)p->i32a, p->i32b, (uint64_t *)dst.id);
+            hipLaunchKernelGGL(k_copy_back_seg, dim3((unsigned)cdiv(len, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                               ctx->stream, p->i32c, p->u64a, a, len, (const int32_t *)p->i32b,
+                               (const uint64_t *)dst.id);
         }
     }
     MCS_LAUNCH_CHECK(ctx);
@@ -593,7 +604,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_faces(MeshGeom g, const double *u
                                                      double *ey, double *ez, double eb, float *fx,
                                                      float *fy, float *fz)
 {
-    int64_t t = gidx();
+    int64_t t = xcd_gidx();
     const int64_t nx = g.n[0], ny = g.n[1], nz = g.n[2];
     const int64_t ux = nx + 2, uxy = (nx + 2) * (ny + 2);
     const double h = g.h;

@@ -33,9 +33,19 @@ struct Lvl {
     int32_t n[3];
 };
 
+// Boundary factors of the ghost rule, each exactly +1 (Neumann) or -1
+// (Dirichlet), as a sign mask: bit d for the low face of axis d, bit 3 + d for
+// the high face.  Applying a factor is a conditional negation, which equals the
+// product by +-1 exactly.
 struct BcCoef {
-    double lo[3], hi[3];
+    uint32_t neg;
 };
+
+template <typename T>
+__device__ __forceinline__ T bcf(const BcCoef &bc, int bit, T v)
+{
+    return ((bc.neg >> bit) & 1u) ? -v : v;
+}
 
 // x / 6 and (4 sum) / 8: rounded divisions in FP64 (the restatement's order),
 // multiplications by the exact reciprocal in FP32.
@@ -78,12 +88,12 @@ __device__ __forceinline__ T lap_fold(const T *__restrict__ u, const Lvl &L, con
 {
     const int64_t sy = L.n[0] + 2, sz = (int64_t)(L.n[0] + 2) * (L.n[1] + 2);
     T l = T(-6) * uc;
-    l += (i + 1 < L.n[0]) ? u[c + 1] : (T)bc.hi[0] * uc;
-    l += (j + 1 < L.n[1]) ? u[c + sy] : (T)bc.hi[1] * uc;
-    l += (k + 1 < L.n[2]) ? u[c + sz] : (T)bc.hi[2] * uc;
-    l += (i > 0) ? u[c - 1] : (T)bc.lo[0] * uc;
-    l += (j > 0) ? u[c - sy] : (T)bc.lo[1] * uc;
-    l += (k > 0) ? u[c - sz] : (T)bc.lo[2] * uc;
+    l += (i + 1 < L.n[0]) ? u[c + 1] : bcf(bc, 3, uc);
+    l += (j + 1 < L.n[1]) ? u[c + sy] : bcf(bc, 4, uc);
+    l += (k + 1 < L.n[2]) ? u[c + sz] : bcf(bc, 5, uc);
+    l += (i > 0) ? u[c - 1] : bcf(bc, 0, uc);
+    l += (j > 0) ? u[c - sy] : bcf(bc, 1, uc);
+    l += (k > 0) ? u[c - sz] : bcf(bc, 2, uc);
     return l;
 }
 
@@ -146,12 +156,13 @@ __device__ __forceinline__ void bc_item(T *a, const Lvl &L, const BcCoef &bc, in
         i = 1 + ua;
         j = 1 + ub;
     }
-    double f = 1.0;
+    uint32_t flip = 0;
     int64_t ci = i, cj = j, ck = k;
-    if (i == 0) { f = f * bc.lo[0]; ci = 1; } else if (i == nx + 1) { f = f * bc.hi[0]; ci = nx; }
-    if (j == 0) { f = f * bc.lo[1]; cj = 1; } else if (j == ny + 1) { f = f * bc.hi[1]; cj = ny; }
-    if (k == 0) { f = f * bc.lo[2]; ck = 1; } else if (k == nz + 1) { f = f * bc.hi[2]; ck = nz; }
-    a[g_index(L, i, j, k)] = (T)f * a[g_index(L, ci, cj, ck)];
+    if (i == 0) { flip ^= bc.neg & 1u; ci = 1; } else if (i == nx + 1) { flip ^= (bc.neg >> 3) & 1u; ci = nx; }
+    if (j == 0) { flip ^= (bc.neg >> 1) & 1u; cj = 1; } else if (j == ny + 1) { flip ^= (bc.neg >> 4) & 1u; cj = ny; }
+    if (k == 0) { flip ^= (bc.neg >> 2) & 1u; ck = 1; } else if (k == nz + 1) { flip ^= (bc.neg >> 5) & 1u; ck = nz; }
+    const T v = a[g_index(L, ci, cj, ck)];
+    a[g_index(L, i, j, k)] = flip ? -v : v;
 }
 
 __device__ __forceinline__ int64_t bc_count(const Lvl &L)
@@ -259,23 +270,34 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_mg_bc(T *a, Lvl L, BcCoef bc)
 template <typename T>
 __global__ __launch_bounds__(MCS_BLOCK) void k_mg_rbgs(T *u, const T *b, Lvl L, BcCoef bc, int par)
 {
-    int64_t t = gidx();
+    int64_t t = xcd_gidx();
     if (t < (int64_t)(L.n[0] / 2) * L.n[1] * L.n[2]) gs_item(u, b, L, bc, par, t);
 }
 
+// Also zeroes the interior of the coarse solution xh that the next mgv! starts
+// from (its ghosts are always materialised before they are read).
 template <typename T>
-__global__ __launch_bounds__(MCS_BLOCK) void k_mg_resid_restrict(T *rh, const T *u, const T *b, Lvl F, BcCoef bc)
+__global__ __launch_bounds__(MCS_BLOCK) void k_mg_resid_restrict(T *rh, const T *u, const T *b, Lvl F, BcCoef bc,
+                                                                 T *xh)
 {
-    int64_t t = gidx();
-    if (t < (int64_t)(F.n[0] / 2) * (F.n[1] / 2) * (F.n[2] / 2))
+    int64_t t = xcd_gidx();
+    if (t < (int64_t)(F.n[0] / 2) * (F.n[1] / 2) * (F.n[2] / 2)) {
         resid_restrict_item(rh, u, b, T(1), F, bc, t);
+        int64_t ic, jc, kc;
+        split3(t, (uint32_t)(F.n[0] / 2), (uint32_t)(F.n[1] / 2), ic, jc, kc);
+        Lvl C;
+        C.n[0] = F.n[0] / 2;
+        C.n[1] = F.n[1] / 2;
+        C.n[2] = F.n[2] / 2;
+        xh[g_index(C, ic + 1, jc + 1, kc + 1)] = T(0);
+    }
 }
 
 // residual! stored on the interior, with per-block sums of r^2 (fmg!'s norm).
 __global__ __launch_bounds__(MCS_BLOCK) void k_mg_residual(double *r, const double *u, const double *b,
                                                            double s, Lvl L, BcCoef bc, double *partial)
 {
-    int64_t t = gidx();
+    int64_t t = xcd_gidx();
     int64_t N = (int64_t)L.n[0] * L.n[1] * L.n[2];
     double sq = 0.0;
     if (t < N) {
@@ -294,7 +316,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_mg_residual(double *r, const doub
         if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+    if (threadIdx.x == 0) partial[xcd_block()] = sh[0];
 }
 
 __global__ __launch_bounds__(MCS_BLOCK) void k_mg_sumsq(const double *b, Lvl L, double *partial)
@@ -342,7 +364,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_mg_restrict(T *rh, const T *r, Lv
 template <typename T>
 __global__ __launch_bounds__(MCS_BLOCK) void k_mg_interp(T *r, const T *rh, Lvl F, int add)
 {
-    int64_t t = gidx();
+    int64_t t = xcd_gidx();
     if (t < (int64_t)(F.n[0] / 2) * (F.n[1] / 2) * (F.n[2] / 2)) interp_item(r, rh, F, add, t);
 }
 
@@ -476,12 +498,12 @@ __device__ void coarsest_in_registers(T *x, const T *b, const Lvl &Lv, const BcC
             T vym = __shfl(u, sym, MCS_WAVE), vzm = __shfl(u, szm, MCS_WAVE);
             if (own && par_me == par) {
                 T l = T(-6) * u;
-                l += (i + 1 < nx) ? vxp : (T)bc.hi[0] * u;
-                l += (j + 1 < ny) ? vyp : (T)bc.hi[1] * u;
-                l += (k + 1 < nz) ? vzp : (T)bc.hi[2] * u;
-                l += (i > 0) ? vxm : (T)bc.lo[0] * u;
-                l += (j > 0) ? vym : (T)bc.lo[1] * u;
-                l += (k > 0) ? vzm : (T)bc.lo[2] * u;
+                l += (i + 1 < nx) ? vxp : bcf(bc, 3, u);
+                l += (j + 1 < ny) ? vyp : bcf(bc, 4, u);
+                l += (k + 1 < nz) ? vzp : bcf(bc, 5, u);
+                l += (i > 0) ? vxm : bcf(bc, 0, u);
+                l += (j > 0) ? vym : bcf(bc, 1, u);
+                l += (k > 0) ? vzm : bcf(bc, 2, u);
                 u = u + div6(T(1) * (l + bb));
             }
         }
@@ -506,12 +528,13 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
         const E *b = lev == start ? b0 : lds + T.off_b[lev];
         const Lvl Lv = T.n[lev];
         const int64_t ncol = (int64_t)(Lv.n[0] / 2) * Lv.n[1] * Lv.n[2];
-        for (int it = 0; it < T.smooth1; ++it)
+        for (int it = 0; it < T.smooth1; ++it) {
             for (int par = 0; par < 2; ++par) {
                 for (int64_t t = threadIdx.x; t < ncol; t += 2 * (int64_t)blockDim.x)
                     gs_item2(x, b, Lv, bc, par, t, t + blockDim.x, t + blockDim.x < ncol);
                 __syncthreads();
             }
+        }
         if (T.smooth2 == 0) {
             BLOOP(t, bc_count(Lv)) bc_item(x, Lv, bc, t);
             __syncthreads();
@@ -749,9 +772,10 @@ static inline Lvl lvl(const mcs_mg *mg, int l)
 static inline BcCoef coefs(const mcs_mg *mg)
 {
     BcCoef c;
+    c.neg = 0;
     for (int d = 0; d < 3; ++d) {
-        c.lo[d] = mg->cfg.bc[2 * d] == MCS_BC_DIRICHLET ? -1.0 : 1.0;
-        c.hi[d] = mg->cfg.bc[2 * d + 1] == MCS_BC_DIRICHLET ? -1.0 : 1.0;
+        if (mg->cfg.bc[2 * d] == MCS_BC_DIRICHLET) c.neg |= 1u << d;
+        if (mg->cfg.bc[2 * d + 1] == MCS_BC_DIRICHLET) c.neg |= 1u << (3 + d);
     }
     return c;
 }
@@ -877,10 +901,10 @@ static void mgv(mcs_mg *mg, T *x, const T *b, int level)
     {
         ProfScope ps(mg->ctx, "mg_residual");
         hipLaunchKernelGGL(k_mg_resid_restrict<T>, grid_for(ncoarse_cells(mg, level)), dim3(MCS_BLOCK), 0,
-                           mg->ctx->stream, lres<T>(mg, level + 1), (const T *)x, b, lvl(mg, level), coefs(mg));
+                           mg->ctx->stream, lres<T>(mg, level + 1), (const T *)x, b, lvl(mg, level), coefs(mg),
+                           lsol<T>(mg, level + 1));
     }
     T *xh = lsol<T>(mg, level + 1);
-    mg_fill0<T>(mg, xh, mg->lv[level + 1].len);
     mgv<T>(mg, xh, lres<T>(mg, level + 1), level + 1);
     mg_interp<T>(mg, level, x, xh, 1);
     for (int it = 0; it < mg->cfg.smooth2; ++it) {

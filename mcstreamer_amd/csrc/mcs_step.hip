@@ -72,7 +72,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_advance(AdvArgs a)
     }
     __syncthreads();
 
-    const int64_t i = gidx();
+    const int64_t i = xcd_gidx();
     int nb = 0;
     int32_t flags = 0;
     int flights = 0, deaths = 0;
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_advance(AdvArgs a)
     int tot;
     int ex = block_excl_scan<int>(nb, &tot);
     if (nb > 0) a.loff[i] = ex;
-    if (threadIdx.x == 0) a.btot[blockIdx.x] = tot;
+    if (threadIdx.x == 0) a.btot[xcd_block()] = tot;
     if (flights) atomicAdd(&s_flights, flights);
     if (deaths) atomicAdd(&s_deaths, deaths);
     if (flags) atomicOr(&s_flags, flags);
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_advance32(AdvArgs32 a)
         s_flags = 0;
     }
     __syncthreads();
-    const int64_t i = gidx();
+    const int64_t i = xcd_gidx();
     int nb = 0;
     int32_t flags = 0;
     int flights = 0, deaths = 0;
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_advance32(AdvArgs32 a)
     int tot;
     int ex = block_excl_scan<int>(nb, &tot);
     if (nb > 0) a.loff[i] = ex;
-    if (threadIdx.x == 0) a.btot[blockIdx.x] = tot;
+    if (threadIdx.x == 0) a.btot[xcd_block()] = tot;
     if (flights) atomicAdd(&s_flights, flights);
     if (deaths) atomicAdd(&s_deaths, deaths);
     if (flags) atomicOr(&s_flags, flags);
