@@ -29,6 +29,19 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_PSTEP_F64 = 128.0
 
 
+def measured_traffic(kernel, units):
+    """HBM bytes per launch of `kernel` from the FETCH_SIZE/WRITE_SIZE counter runs
+    summarised under profiles/ (bytes per particle-step there, times the
+    particle-steps of one launch here); None when no measurement is recorded."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as fd:
+            rec = json.load(fd)[kernel]
+        return rec["hbm_bytes_per_particle_step"] * units
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline_streamer(sim, ctx_arrays, budget_s=20.0):
     """One full oracle step on the device state (bounded: a single step)."""
     from oracle import refcpu as R
@@ -118,7 +131,8 @@ def run_streamer(args):
                    "parallelism": "single GPU"},
         "vcycles_per_s": vcycles / el,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("k_advance<1,1>", psteps / max(args.steps, 1)),
                      "kernel": "k_advance<1,1> (fused push + collisions, mesh gather, ion tracker)",
                      "avg_launch_ms": adv_avg_s * 1e3,
                      "bytes_per_launch": bytes_per_launch},
