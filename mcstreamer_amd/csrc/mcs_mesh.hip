@@ -432,6 +432,7 @@ struct DepArgs {
 };
 
 #define DEP_MASK 0x1FFFFFULL
+#define DEP_PF 8    // records loaded ahead of the sequential accumulation
 
 // Per particle: g = x/h - 1/2, i0 = floor(g), fr = g - i0, base = (w val)/dV,
 // exactly the quantities a sequential CIC scatter computes per particle.
@@ -471,7 +472,7 @@ __device__ __forceinline__ void dep_visit(double &acc, uint64_t key, const doubl
 // density! as a per-cell gather over the canonical order: cell c sums the shares of
 // the particles of its 27 neighbours in (cell, id) order, which is the order in
 // which a sequential scatter over the sorted population adds them.  No atomics.
-// Loads run four particles ahead of the (sequential) accumulation.
+// Loads run DEP_PF particles ahead of the (sequential) accumulation.
 template <int CIC, typename Buf>
 __global__ __launch_bounds__(MCS_BLOCK) void k_deposit(DepArgs<Buf> a)
 {
@@ -500,6 +501,17 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_deposit(DepArgs<Buf> a)
             if (jj < 0 || jj >= ny) continue;
             int64_t row = nx * (jj + ny * kk);
             int64_t q = a.cs[row + ilo], s1 = a.cs[row + ihi + 1];
+            for (; q + DEP_PF <= s1; q += DEP_PF) {
+                uint64_t kk8[DEP_PF];
+                double4 ww8[DEP_PF];
+#pragma unroll
+                for (int u = 0; u < DEP_PF; ++u) {
+                    kk8[u] = a.pk[q + u];
+                    ww8[u] = a.pw[q + u];
+                }
+#pragma unroll
+                for (int u = 0; u < DEP_PF; ++u) dep_visit(acc, kk8[u], ww8[u], i, j, k);
+            }
             for (; q + 4 <= s1; q += 4) {
                 uint64_t k0 = a.pk[q], k1 = a.pk[q + 1], k2 = a.pk[q + 2], k3 = a.pk[q + 3];
                 double4 w0 = a.pw[q], w1 = a.pw[q + 1], w2 = a.pw[q + 2], w3 = a.pw[q + 3];

@@ -501,15 +501,22 @@ __global__ __launch_bounds__(BS) void k_sw_resample(A a)
     const int64_t i = a.tile_start[tile] + threadIdx.x;
     const int64_t lim = a.S.base[slot] + a.S.n[slot];
     const int mode = a.S.mode[slot];
+    // slots that need no clones (none, or a drop) write only their zero tile total;
+    // k_sw_clone_place skips them
+    if (mode != SW_MODE_CLONE) {
+        if (mode == SW_MODE_DROP && i < lim && a.b.active[i]) {
+            mcs_stream rs;
+            mcs_stream_init(&rs, a.seed, a.S.key[slot], MCS_PHASE_SWARM_RESAMPLE, a.b.id[i], a.S.step[slot]);
+            if (mcs_u01(&rs) < a.S.param[slot]) a.b.active[i] = 0;
+        }
+        if (threadIdx.x == 0) a.tile_clones[tile] = 0;
+        return;
+    }
     int c = 0;
-    if (mode != SW_MODE_NONE && i < lim && a.b.active[i]) {
+    if (i < lim && a.b.active[i]) {
         mcs_stream rs;
         mcs_stream_init(&rs, a.seed, a.S.key[slot], MCS_PHASE_SWARM_RESAMPLE, a.b.id[i], a.S.step[slot]);
-        if (mode == SW_MODE_DROP) {
-            if (mcs_u01(&rs) < a.S.param[slot]) a.b.active[i] = 0;
-        } else {
-            c = (int)mcs_poisson(&rs, a.S.param[slot]);
-        }
+        c = (int)mcs_poisson(&rs, a.S.param[slot]);
     }
     a.cnt[i] = c;
     int tot;
@@ -523,6 +530,7 @@ __global__ __launch_bounds__(BS) void k_sw_clone_place(A a)
 {
     const int64_t tile = blockIdx.x;
     const int slot = a.tile_slot[tile];
+    if (a.S.mode[slot] != SW_MODE_CLONE) return;
     const int64_t i = a.tile_start[tile] + threadIdx.x;
     const int c = a.cnt[i];
     if (c == 0) return;
