@@ -511,6 +511,107 @@ __device__ void coarsest_in_registers(T *x, const T *b, const Lvl &Lv, const BcC
     if (own) x[c] = u;
 }
 
+// Block-wide copy with four independent loads in flight per thread.
+template <typename E>
+__device__ __forceinline__ void tail_copy(E *dst, const E *src, int64_t n)
+{
+    const int64_t bd = blockDim.x;
+    int64_t t = threadIdx.x;
+    for (; t + 3 * bd < n; t += 4 * bd) {
+        const E a0 = src[t], a1 = src[t + bd], a2 = src[t + 2 * bd], a3 = src[t + 3 * bd];
+        dst[t] = a0;
+        dst[t + bd] = a1;
+        dst[t + 2 * bd] = a2;
+        dst[t + 3 * bd] = a3;
+    }
+    for (; t < n; t += bd) dst[t] = src[t];
+}
+
+// The coarsest rounds when the level has at most 16 cells and power-of-two
+// strides: neighbours come from DPP row shifts within lanes 0-15 (row_shl:k gives
+// lane l the value of lane l + k, row_shr:k that of lane l - k) instead of LDS
+// permutes; the arithmetic is that of coarsest_in_registers.  SZ = 0: nz = 1.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v)
+{
+    const uint64_t w = (uint64_t)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)w, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(w >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+
+template <typename T, int SY, int SZ>
+__device__ void coarsest_dpp(T *x, const T *b, const Lvl &Lv, const BcCoef &bc)
+{
+    constexpr int ZP = SZ ? 0x100 + SZ : 0x101, ZM = SZ ? 0x110 + SZ : 0x111;
+    const int lane = threadIdx.x;
+    const int nx = Lv.n[0], ny = Lv.n[1], nz = Lv.n[2];
+    const int N = nx * ny * nz;
+    const bool own = lane < N;
+    int i = 0, j = 0, k = 0;
+    if (own) {
+        i = lane % nx;
+        j = (lane / nx) % ny;
+        k = lane / (nx * ny);
+    }
+    const int64_t c = g_index(Lv, i + 1, j + 1, k + 1);
+    T u = own ? x[c] : T(0);
+    const T bb = own ? b[c] : T(0);
+    const int par_me = (i + j + k) & 1;
+    for (int it = 0; it < 50; ++it) {
+        if (it == 49) {
+            if (own) x[c] = u;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            for (int64_t t = lane; t < bc_count(Lv); t += MCS_WAVE) bc_item(x, Lv, bc, t);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+        for (int par = 0; par < 2; ++par) {
+            const T vxp = dpp_move<0x101>(u), vxm = dpp_move<0x111>(u);
+            const T vyp = dpp_move<0x100 + SY>(u), vym = dpp_move<0x110 + SY>(u);
+            const T vzp = dpp_move<ZP>(u), vzm = dpp_move<ZM>(u);
+            if (own && par_me == par) {
+                T l = T(-6) * u;
+                l += (i + 1 < nx) ? vxp : bcf(bc, 3, u);
+                l += (j + 1 < ny) ? vyp : bcf(bc, 4, u);
+                l += (k + 1 < nz) ? vzp : bcf(bc, 5, u);
+                l += (i > 0) ? vxm : bcf(bc, 0, u);
+                l += (j > 0) ? vym : bcf(bc, 1, u);
+                l += (k > 0) ? vzm : bcf(bc, 2, u);
+                u = u + div6(T(1) * (l + bb));
+            }
+        }
+    }
+    if (own) x[c] = u;
+}
+
+// Picks the DPP form for levels of at most 16 cells with power-of-two x and y
+// strides; returns false when it does not apply.
+template <typename T>
+__device__ bool coarsest_try_dpp(T *x, const T *b, const Lvl &Lv, const BcCoef &bc)
+{
+    const int nx = Lv.n[0], ny = Lv.n[1], nz = Lv.n[2];
+    if (nx * ny * nz > 16) return false;
+    const int sz = nz > 1 ? nx * ny : 0;
+    if (nx == 2 && sz == 0) { coarsest_dpp<T, 2, 0>(x, b, Lv, bc); return true; }
+    if (nx == 2 && sz == 2) { coarsest_dpp<T, 2, 2>(x, b, Lv, bc); return true; }
+    if (nx == 2 && sz == 4) { coarsest_dpp<T, 2, 4>(x, b, Lv, bc); return true; }
+    if (nx == 2 && sz == 8) { coarsest_dpp<T, 2, 8>(x, b, Lv, bc); return true; }
+    if (nx == 4 && sz == 0) { coarsest_dpp<T, 4, 0>(x, b, Lv, bc); return true; }
+    if (nx == 4 && sz == 4) { coarsest_dpp<T, 4, 4>(x, b, Lv, bc); return true; }
+    if (nx == 4 && sz == 8) { coarsest_dpp<T, 4, 8>(x, b, Lv, bc); return true; }
+    if (nx == 8 && sz == 0) { coarsest_dpp<T, 8, 0>(x, b, Lv, bc); return true; }
+    if (nx == 8 && sz == 8) { coarsest_dpp<T, 8, 8>(x, b, Lv, bc); return true; }
+    return false;
+}
+
 // mgv! (Multigrid.jl:334-377) for levels start..L inside one workgroup: the level
 // solutions and the deeper right-hand sides live in LDS, the start level's
 // right-hand side is read from global memory; x0 is loaded and written back.
@@ -521,7 +622,7 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
     extern __shared__ double lds_raw[];
     E *lds = (E *)lds_raw;
     const BcCoef bc = T.bc;
-    BLOOP(t, T.len[start]) lds[T.off_x[start] + t] = x0[t];
+    tail_copy(lds + T.off_x[start], x0, T.len[start]);
     __syncthreads();
     for (int lev = start; lev < T.L; ++lev) {
         E *x = lds + T.off_x[lev];
@@ -551,7 +652,7 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
         const Lvl Lv = T.n[lev];
         const int64_t N = (int64_t)Lv.n[0] * Lv.n[1] * Lv.n[2];
         if (N <= MCS_WAVE) {
-            if (threadIdx.x < MCS_WAVE) coarsest_in_registers(x, b, Lv, bc);
+            if (threadIdx.x < MCS_WAVE && !coarsest_try_dpp(x, b, Lv, bc)) coarsest_in_registers(x, b, Lv, bc);
             __syncthreads();
         } else {
             const int64_t ncol = (int64_t)(Lv.n[0] / 2) * Lv.n[1] * Lv.n[2];
@@ -587,7 +688,7 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
             }
         }
     }
-    BLOOP(t, T.len[start]) x0[t] = lds[T.off_x[start] + t];
+    tail_copy(x0, lds + T.off_x[start], T.len[start]);
 }
 
 // ---------------------------------------------------------------- host side
