@@ -182,6 +182,8 @@ struct MgLevel {
     int64_t len;
     double *res, *res1, *sol;
     float *fres, *fsol;   // FP32 smoother copies (precision mode)
+    double *alt;          // second solution array of the tiled smoother (grid levels)
+    float *falt;
 };
 
 struct mcs_mg {
@@ -197,6 +199,12 @@ struct mcs_mg {
     int32_t *bc_dev = nullptr;                   // boundary kinds on the device
     double tol_abs = 0.0;                        // tolerance in force for the current solve
     size_t tail_lds = 0;                         // LDS budget of the single-workgroup tail
+    int tail_levels = 0;                         // most levels the tail may hold
+    int64_t tail_max = 0;                        // largest level (cells) a tail starts from
+    size_t sm_lds = 0;                           // LDS budget of the tiled smoother; 0: one launch per pass
+    int sm_tile = 16;                            // its largest tile edge
+    int64_t sm_big = 0;                          // levels of this many cells: one sweep per launch
+    int64_t sm_max = 0;                          // levels of more cells: one launch per pass
     bool use_graph = true;                       // replay the cycle body as a HIP graph
     hipGraphExec_t graph_exec = nullptr;
     double *graph_x = nullptr;

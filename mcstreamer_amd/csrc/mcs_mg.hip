@@ -19,6 +19,7 @@
 // FP64; the correction equation L z + r = 0 of each fmg! pass (restrictions,
 // coarse solve, V-cycles) runs on an FP32 copy of the hierarchy.  Every stencil
 // body is a template on the element type, so the FP64 path is unchanged.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -28,6 +29,11 @@
 #define MG_MAXL 13
 #define MG_TAIL_LDS (160 * 1024)
 #define MG_TAIL_THREADS 512
+// Measured on the 128x128x256 streamer solve (7 levels): the single-workgroup
+// tail pays off only on the last two levels, the tiled smoother on levels of up
+// to ~2^18 cells; the finer levels run one launch per pass.
+#define MG_TAIL_MAXCELLS 256
+#define MG_SM_MAXCELLS ((int64_t)1 << 18)
 
 struct Lvl {
     int32_t n[3];
@@ -49,7 +55,18 @@ __device__ __forceinline__ T bcf(const BcCoef &bc, int bit, T v)
 
 // x / 6 and (4 sum) / 8: rounded divisions in FP64 (the restatement's order),
 // multiplications by the exact reciprocal in FP32.
-__device__ __forceinline__ double div6(double x) { return x / 6.0; }
+// FP64 x / 6 is formed as q = x*c, r = fma(-q, 6, x), q + r*c with c = RN(1/6):
+// by Markstein's theorem (q within 1 ulp of x/6, c within 1/2 ulp of 1/6) the
+// result is the correctly rounded quotient, i.e. equal to x / 6.0, while r does
+// not underflow; zeros, tiny and huge |x| and non-finite x take the division.
+__device__ __forceinline__ double div6(double x)
+{
+    const double a = fabs(x);
+    if (__builtin_expect(!(a > 0x1p-960 && a < 0x1p960), 0)) return x / 6.0;
+    const double c = 1.0 / 6.0;
+    const double q = x * c;
+    return __builtin_fma(__builtin_fma(-q, 6.0, x), c, q);
+}
 __device__ __forceinline__ float div6(float x) { return x * (1.0f / 6.0f); }
 
 __device__ __forceinline__ int64_t g_index(const Lvl &L, int64_t i, int64_t j, int64_t k)
@@ -344,7 +361,17 @@ __global__ void k_sum_partials(const double *partial, int64_t nb, double *out)
 {
     __shared__ double sh[MCS_BLOCK];
     double acc = 0.0;
-    for (int64_t q = threadIdx.x; q < nb; q += blockDim.x) acc += partial[q];
+    const int64_t bd = blockDim.x;
+    int64_t q = threadIdx.x;
+    // eight loads in flight, added in the order of the plain loop
+    for (; q + 7 * bd < nb; q += 8 * bd) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = partial[q + k * bd];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc += v[k];
+    }
+    for (; q < nb; q += bd) acc += partial[q];
     sh[threadIdx.x] = acc;
     __syncthreads();
     for (int o = MCS_BLOCK / 2; o > 0; o >>= 1) {
@@ -441,6 +468,286 @@ __global__ void k_mg_inhom(double *b, Lvl L, int face, double c, const double *v
     idx[d2] = a2 + 1;
     int64_t cidx = g_index(L, idx[0], idx[1], idx[2]);
     b[cidx] = b[cidx] + c * val[t];
+}
+
+// ---- tiled smoother: one visit of mgv! to a grid level in two launches --------
+//
+// A workgroup owns a tile of the level and holds the tile grown by H cells (cut
+// at the level's edges) in LDS.  Pass p of the P red-black passes updates its
+// colour on the tile grown by H-1-p cells: by induction every update reads
+// neighbours that are as the launch-per-pass path leaves them, and the outer layer
+// is only read, so after the passes the tile grown by H-P cells holds exactly the
+// global result.  The per-cell arithmetic is that of gs_item, resid_restrict_item,
+// interp_item and bc_item, so results are bitwise those of one launch per pass.
+// Input and output are distinct arrays: neighbouring tiles read each other's
+// halos from the input while the outputs are written.
+
+#define MG_SM_THREADS 512
+#define SM_LD 4
+#define SM_GS 4
+
+struct SmoothArgs {
+    Lvl L;
+    BcCoef bc;
+    int32_t tile[3];     // tile size: even, divides L.n
+    int32_t ntile[3];    // tiles per axis
+    int32_t passes;      // red-black passes, colour 0 first (2 per sweep)
+    int32_t halo;        // passes + resid
+    int32_t ghost_pass;  // ghost shell of u_out from the state after this many passes; -1: none
+    int32_t resid;       // after the passes: residual restricted into rh
+    int32_t interp;      // start: 0 u_in, 1 u_in + interpolate!(xc), 2 interpolate!(xc), 3 zero
+};
+
+// Cells t, t + step, ... of a box (nx, ny, nz) in x-fastest order as a carried
+// mixed-radix counter (no divisions inside the loops).
+struct BoxWalk {
+    int x, y, z, sx, sy, sz, nx, ny;
+    __device__ __forceinline__ BoxWalk(int nx_, int ny_, int t, int step) : nx(nx_), ny(ny_)
+    {
+        x = t % nx;
+        int q = t / nx;
+        y = q % ny;
+        z = q / ny;
+        sx = step % nx;
+        q = step / nx;
+        sy = q % ny;
+        sz = q / ny;
+    }
+    __device__ __forceinline__ void next()
+    {
+        x += sx;
+        y += sy;
+        z += sz;
+        if (x >= nx) { x -= nx; ++y; }
+        if (y >= ny) { y -= ny; ++z; }
+    }
+};
+
+__device__ __forceinline__ int64_t g_at(const Lvl &L, int x, int y, int z)
+{
+    return g_index(L, (int64_t)x + 1, (int64_t)y + 1, (int64_t)z + 1);
+}
+
+// lap_fold on the LDS box: a neighbour outside the level reads the cell itself,
+// so every LDS read stays inside the box, and the ghost rule is applied to it.
+template <typename T>
+__device__ __forceinline__ T lap_box(const T *s, int li, int sy, int sz, int x, int y, int z, const Lvl &L,
+                                    const BcCoef &bc, T uc)
+{
+    const bool xp = x + 1 < L.n[0], yp = y + 1 < L.n[1], zp = z + 1 < L.n[2];
+    const bool xm = x > 0, ym = y > 0, zm = z > 0;
+    const T vxp = s[xp ? li + 1 : li], vyp = s[yp ? li + sy : li], vzp = s[zp ? li + sz : li];
+    const T vxm = s[xm ? li - 1 : li], vym = s[ym ? li - sy : li], vzm = s[zm ? li - sz : li];
+    T l = T(-6) * uc;
+    l += xp ? vxp : bcf(bc, 3, vxp);
+    l += yp ? vyp : bcf(bc, 4, vyp);
+    l += zp ? vzp : bcf(bc, 5, vzp);
+    l += xm ? vxm : bcf(bc, 0, vxm);
+    l += ym ? vym : bcf(bc, 1, vym);
+    l += zm ? vzm : bcf(bc, 2, vzm);
+    return l;
+}
+
+// interp_item's value for fine cell (x, y, z) of level F from the coarse array rh.
+template <typename T>
+__device__ __forceinline__ T prolong_at(const T *rh, const Lvl &F, int x, int y, int z)
+{
+    const int64_t cy = F.n[0] / 2 + 2, cz = (int64_t)(F.n[0] / 2 + 2) * (F.n[1] / 2 + 2);
+    const int64_t cc = (int64_t)((x >> 1) + 1) + cy * ((y >> 1) + 1) + cz * ((z >> 1) + 1);
+    const int64_t ox = 2 * (x & 1) - 1, oy = (2 * (y & 1) - 1) * cy, oz = (2 * (z & 1) - 1) * cz;
+    const T w0 = T(27) / T(64), w1 = T(9) / T(64), w2 = T(3) / T(64), w3 = T(1) / T(64);
+    T v = T(0);
+    v += w0 * rh[cc];
+    v += w1 * rh[cc + oz];
+    v += w1 * rh[cc + oy];
+    v += w2 * rh[cc + oy + oz];
+    v += w1 * rh[cc + ox];
+    v += w2 * rh[cc + ox + oz];
+    v += w2 * rh[cc + ox + oy];
+    v += w3 * rh[cc + ox + oy + oz];
+    return v;
+}
+
+struct SmBox {
+    int t0[3], t1[3], a0[3], R[3];   // tile, and the box of u: the tile grown by H
+    int b0[3], Rb[3];                // box of b: the tile grown by H - 1
+};
+
+// bc_item for the ghost cells whose nearest interior cell lies in this tile, from
+// the LDS state.
+template <typename T>
+__device__ void sm_ghosts(const SmoothArgs &A, const SmBox &B, const T *s, T *u_out)
+{
+    const Lvl &L = A.L;
+    int e0[3], E[3];
+    bool touch = false;
+    for (int d = 0; d < 3; ++d) {
+        const bool lo = B.t0[d] == 0, hi = B.t1[d] == L.n[d];
+        touch |= lo || hi;
+        e0[d] = B.t0[d] - (lo ? 1 : 0);
+        E[d] = B.t1[d] + (hi ? 1 : 0) - e0[d];
+    }
+    if (!touch) return;
+    const int sy = B.R[0], sz = B.R[0] * B.R[1];
+    const uint32_t neg = A.bc.neg;
+    for (BoxWalk w(E[0], E[1], threadIdx.x, blockDim.x); w.z < E[2]; w.next()) {
+        const int x = e0[0] + w.x, y = e0[1] + w.y, z = e0[2] + w.z;
+        uint32_t flip = 0;
+        int cx = x, cy = y, cz = z;
+        bool shell = false;
+        if (x < 0) { flip ^= neg & 1u; cx = 0; shell = true; }
+        else if (x >= L.n[0]) { flip ^= (neg >> 3) & 1u; cx = L.n[0] - 1; shell = true; }
+        if (y < 0) { flip ^= (neg >> 1) & 1u; cy = 0; shell = true; }
+        else if (y >= L.n[1]) { flip ^= (neg >> 4) & 1u; cy = L.n[1] - 1; shell = true; }
+        if (z < 0) { flip ^= (neg >> 2) & 1u; cz = 0; shell = true; }
+        else if (z >= L.n[2]) { flip ^= (neg >> 5) & 1u; cz = L.n[2] - 1; shell = true; }
+        if (!shell) continue;
+        const T v = s[(cx - B.a0[0]) + sy * (cy - B.a0[1]) + sz * (cz - B.a0[2])];
+        u_out[g_at(L, x, y, z)] = flip ? -v : v;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(MG_SM_THREADS) void k_mg_smooth(SmoothArgs A, const T *u_in, const T *b, T *u_out,
+                                                             const T *xc, T *rh)
+{
+    extern __shared__ double lds_raw[];
+    T *s = (T *)lds_raw;
+    const Lvl L = A.L;
+    const BcCoef bc = A.bc;
+    SmBox B;
+    {
+        uint32_t q = xcd_block();
+        for (int d = 0; d < 3; ++d) {
+            int td;
+            if (d < 2) {
+                td = (int)(q % (uint32_t)A.ntile[d]);
+                q /= (uint32_t)A.ntile[d];
+            } else {
+                td = (int)q;
+            }
+            B.t0[d] = td * A.tile[d];
+            B.t1[d] = B.t0[d] + A.tile[d];
+            B.a0[d] = max(B.t0[d] - A.halo, 0);
+            B.R[d] = min(B.t1[d] + A.halo, L.n[d]) - B.a0[d];
+            B.b0[d] = max(B.t0[d] - (A.halo - 1), 0);
+            B.Rb[d] = min(B.t1[d] + A.halo - 1, L.n[d]) - B.b0[d];
+        }
+    }
+    const int sy = B.R[0], sz = B.R[0] * B.R[1];
+    const int by = B.Rb[0], bz = B.Rb[0] * B.Rb[1];
+    T *sb = s + sz * B.R[2];
+    // both boxes, SM_LD cells per thread at a time so that the global loads overlap
+    for (BoxWalk w(B.R[0], B.R[1], threadIdx.x, blockDim.x); w.z < B.R[2];) {
+        T v[SM_LD], vb[SM_LD];
+        int li[SM_LD], lb[SM_LD];
+#pragma unroll
+        for (int q = 0; q < SM_LD; ++q) {
+            li[q] = -1;
+            lb[q] = -1;
+            if (w.z < B.R[2]) {
+                const int x = B.a0[0] + w.x, y = B.a0[1] + w.y, z = B.a0[2] + w.z;
+                li[q] = w.x + sy * w.y + sz * w.z;
+                if (A.interp == 3) {
+                    v[q] = T(0);
+                } else if (A.interp == 2) {
+                    v[q] = prolong_at(xc, L, x, y, z);
+                } else {
+                    v[q] = u_in[g_at(L, x, y, z)];
+                    if (A.interp) v[q] += prolong_at(xc, L, x, y, z);
+                }
+                const int xb = x - B.b0[0], yb = y - B.b0[1], zb = z - B.b0[2];
+                if ((unsigned)xb < (unsigned)B.Rb[0] && (unsigned)yb < (unsigned)B.Rb[1] &&
+                    (unsigned)zb < (unsigned)B.Rb[2]) {
+                    lb[q] = xb + by * yb + bz * zb;
+                    vb[q] = b[g_at(L, x, y, z)];
+                }
+            }
+            w.next();
+        }
+#pragma unroll
+        for (int q = 0; q < SM_LD; ++q) {
+            if (li[q] >= 0) s[li[q]] = v[q];
+            if (lb[q] >= 0) sb[lb[q]] = vb[q];
+        }
+    }
+    __syncthreads();
+    if (A.ghost_pass == 0) {
+        sm_ghosts(A, B, s, u_out);
+        __syncthreads();
+    }
+    for (int p = 0; p < A.passes; ++p) {
+        const int e = A.halo - 1 - p, par = p & 1;
+        int lo[3], W[3];
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = max(B.t0[d] - e, 0);
+            W[d] = min(B.t1[d] + e, L.n[d]) - lo[d];
+        }
+        const int half = (W[0] + 1) / 2;
+        // SM_GS cells of one colour per thread at a time: their right-hand sides are
+        // loaded together, and none of them reads another
+        for (BoxWalk w(half, W[1], threadIdx.x, blockDim.x); w.z < W[2];) {
+            int li[SM_GS], cx[SM_GS], cy[SM_GS], cz[SM_GS];
+            T bv[SM_GS];
+#pragma unroll
+            for (int q = 0; q < SM_GS; ++q) {
+                li[q] = -1;
+                if (w.z < W[2]) {
+                    const int y = lo[1] + w.y, z = lo[2] + w.z;
+                    const int x = lo[0] + 2 * w.x + ((lo[0] + y + z + par) & 1);
+                    if (x < lo[0] + W[0]) {
+                        li[q] = (x - B.a0[0]) + sy * (y - B.a0[1]) + sz * (z - B.a0[2]);
+                        cx[q] = x;
+                        cy[q] = y;
+                        cz[q] = z;
+                        bv[q] = sb[(x - B.b0[0]) + by * (y - B.b0[1]) + bz * (z - B.b0[2])];
+                    }
+                }
+                w.next();
+            }
+#pragma unroll
+            for (int q = 0; q < SM_GS; ++q)
+                if (li[q] >= 0) {
+                    const T uc = s[li[q]];
+                    const T l = lap_box(s, li[q], sy, sz, cx[q], cy[q], cz[q], L, bc, uc);
+                    s[li[q]] = uc + div6(T(1) * (l + bv[q]));
+                }
+        }
+        __syncthreads();
+        if (A.ghost_pass == p + 1) {
+            sm_ghosts(A, B, s, u_out);
+            __syncthreads();
+        }
+    }
+    {
+        const int ox = B.t0[0] - B.a0[0], oy = B.t0[1] - B.a0[1], oz = B.t0[2] - B.a0[2];
+        for (BoxWalk w(A.tile[0], A.tile[1], threadIdx.x, blockDim.x); w.z < A.tile[2]; w.next())
+            u_out[g_at(L, B.t0[0] + w.x, B.t0[1] + w.y, B.t0[2] + w.z)] =
+                s[(ox + w.x) + sy * (oy + w.y) + sz * (oz + w.z)];
+    }
+    if (A.resid) {
+        Lvl C;
+        C.n[0] = L.n[0] / 2;
+        C.n[1] = L.n[1] / 2;
+        C.n[2] = L.n[2] / 2;
+        for (BoxWalk w(A.tile[0] / 2, A.tile[1] / 2, threadIdx.x, blockDim.x); w.z < A.tile[2] / 2; w.next()) {
+            const int ic = B.t0[0] / 2 + w.x, jc = B.t0[1] / 2 + w.y, kc = B.t0[2] / 2 + w.z;
+            T sum = T(0);
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx)
+#pragma unroll
+                for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+                    for (int dz = 0; dz < 2; ++dz) {
+                        const int x = 2 * ic + dx, y = 2 * jc + dy, z = 2 * kc + dz;
+                        const int li = (x - B.a0[0]) + sy * (y - B.a0[1]) + sz * (z - B.a0[2]);
+                        const T uc = s[li];
+                        const T l = lap_box(s, li, sy, sz, x, y, z, L, bc, uc);
+                        sum += T(1) * sb[(x - B.b0[0]) + by * (y - B.b0[1]) + bz * (z - B.b0[2])] + l;
+                    }
+            rh[g_at(C, ic, jc, kc)] = (T(4) * sum) / T(8);
+        }
+    }
 }
 
 // ---- single-workgroup tail: mgv! from a small level down to the coarsest -----
@@ -617,12 +924,23 @@ __device__ bool coarsest_try_dpp(T *x, const T *b, const Lvl &Lv, const BcCoef &
 // right-hand side is read from global memory; x0 is loaded and written back.
 // Phases are separated by barriers and follow the grid path's order exactly.
 template <typename E>
-__global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, E *x0, const E *b0)
+__global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, E *x0, const E *b0, const E *zc,
+                                                            int zero, int fresh)
 {
     extern __shared__ double lds_raw[];
     E *lds = (E *)lds_raw;
     const BcCoef bc = T.bc;
-    tail_copy(lds + T.off_x[start], x0, T.len[start]);
+    if (zc) {
+        // the start level's interior is interpolate!(x, zc) (overwrite); its
+        // ghosts are materialised before they are read
+        const Lvl Lv = T.n[start];
+        BLOOP(t, (int64_t)(Lv.n[0] / 2) * (Lv.n[1] / 2) * (Lv.n[2] / 2))
+            interp_item(lds + T.off_x[start], zc, Lv, 0, t);
+    } else if (zero) {
+        BLOOP(t, T.len[start]) lds[T.off_x[start] + t] = E(0);
+    } else {
+        tail_copy(lds + T.off_x[start], x0, T.len[start]);
+    }
     __syncthreads();
     for (int lev = start; lev < T.L; ++lev) {
         E *x = lds + T.off_x[lev];
@@ -688,6 +1006,12 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
             }
         }
     }
+    if (fresh) {
+        // ghosts of the final state (the apply! the caller would issue next)
+        __syncthreads();
+        BLOOP(t, bc_count(T.n[start])) bc_item(lds + T.off_x[start], T.n[start], bc, t);
+    }
+    __syncthreads();
     tail_copy(x0, lds + T.off_x[start], T.len[start]);
 }
 
@@ -759,6 +1083,8 @@ static int coarse_inverse(const int32_t n[3], const int32_t bc[6], std::vector<d
     return 0;
 }
 
+template <typename T> static bool in_tail(const mcs_mg *mg, int level);
+
 extern "C" int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg **out)
 {
     int lds_max = 0;
@@ -789,10 +1115,43 @@ extern "C" int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg 
         return mcs_fail(ctx, MCS_E_ARG, "mg: unknown precision %d", c.precision);
     const bool f32 = c.precision == MCS_PREC_F32;
     hipSetDevice(ctx->device);
+    size_t sm_lds = (size_t)lds_max < (size_t)MG_TAIL_LDS ? (size_t)lds_max : (size_t)MG_TAIL_LDS;
+    if (hipFuncSetAttribute((const void *)k_mg_smooth<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)sm_lds) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_mg_smooth<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)sm_lds) != hipSuccess)
+        sm_lds = 64 * 1024;
+    // MCS_MG_FUSED=0 selects one launch per pass on the grid levels (same results);
+    // MCS_MG_TAIL_LEVELS caps the levels of the single-workgroup tail (at least 1).
+    const char *fe = getenv("MCS_MG_FUSED");
+    if ((fe && fe[0] == '0') || c.smooth2 < 1 || c.smooth1 > 2 || c.smooth2 > 2) sm_lds = 0;
+    // MCS_MG_SM_TILE: largest tile edge of the tiled smoother; MCS_MG_SM_MAXCELLS:
+    // levels of more cells take one launch per pass (the halos would cost more
+    // traffic than the launches they save); MCS_MG_SM_BIG: levels of at least this
+    // many cells take one sweep per launch instead of two
+    const char *se = getenv("MCS_MG_SM_TILE");
+    int sm_tile = se ? atoi(se) : 8;
+    if (sm_tile < 2) sm_tile = 2;
+    const char *me = getenv("MCS_MG_SM_MAXCELLS");
+    int64_t sm_max = me ? atoll(me) : MG_SM_MAXCELLS;
+    const char *be = getenv("MCS_MG_SM_BIG");
+    int64_t sm_big = be ? atoll(be) : ((int64_t)1 << 62);
+    // MCS_MG_TAIL_MAXCELLS: largest level the tail starts from
+    const char *tm = getenv("MCS_MG_TAIL_MAXCELLS");
+    const int64_t tail_max = tm ? atoll(tm) : MG_TAIL_MAXCELLS;
+    const char *te = getenv("MCS_MG_TAIL_LEVELS");
+    int tail_levels = te ? atoi(te) : MG_MAXL;
+    if (tail_levels < 1) tail_levels = 1;
     mcs_mg *mg = new mcs_mg();
     mg->ctx = ctx;
     mg->cfg = c;
     mg->tail_lds = tail_lds;
+    mg->tail_levels = tail_levels;
+    mg->tail_max = tail_max;
+    mg->sm_lds = sm_lds;
+    mg->sm_tile = sm_tile;
+    mg->sm_big = sm_big;
+    mg->sm_max = sm_max;
     const char *ge = getenv("MCS_MG_GRAPH");
     mg->use_graph = !(ge && ge[0] == '0');
     for (int l = 0; l <= c.levels; ++l) {
@@ -802,6 +1161,8 @@ extern "C" int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg 
         L.fres = nullptr;
         L.fsol = nullptr;
         L.res1 = nullptr;
+        L.alt = nullptr;
+        L.falt = nullptr;
         if (hipMalloc(&L.res, sizeof(double) * L.len) != hipSuccess ||
             hipMalloc(&L.sol, sizeof(double) * L.len) != hipSuccess)
             return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation failed");
@@ -815,6 +1176,24 @@ extern "C" int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg 
             hipMemset(L.fsol, 0, sizeof(float) * L.len);
         }
         mg->lv.push_back(L);
+    }
+    // second solution arrays of the tiled smoother, on the levels outside the tail
+    if (sm_lds) {
+        for (int l = 0; l < c.levels; ++l) {
+            MgLevel &L = mg->lv[l];
+            if ((in_tail<double>(mg, l) && in_tail<float>(mg, l)) ||
+                (int64_t)L.n[0] * L.n[1] * L.n[2] > sm_max)
+                continue;
+            if (f32) {
+                if (hipMalloc(&L.falt, sizeof(float) * L.len) != hipSuccess)
+                    return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation failed");
+                hipMemset(L.falt, 0, sizeof(float) * L.len);
+            } else {
+                if (hipMalloc(&L.alt, sizeof(double) * L.len) != hipSuccess)
+                    return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation failed");
+                hipMemset(L.alt, 0, sizeof(double) * L.len);
+            }
+        }
     }
     const int32_t *nc = mg->lv.back().n;
     mg->ncoarse = (int64_t)nc[0] * nc[1] * nc[2];
@@ -851,6 +1230,8 @@ extern "C" int32_t mcs_mg_destroy(mcs_mg *mg)
         hipFree(L.sol);
         if (L.fres) hipFree(L.fres);
         if (L.fsol) hipFree(L.fsol);
+        if (L.alt) hipFree(L.alt);
+        if (L.falt) hipFree(L.falt);
     }
     hipFree(mg->ainv);
     if (mg->ainv32) hipFree(mg->ainv32);
@@ -903,6 +1284,9 @@ template <> const double *coarse_inv<double>(mcs_mg *mg) { return mg->ainv; }
 template <> float *lres<float>(mcs_mg *mg, int l) { return mg->lv[l].fres; }
 template <> float *lsol<float>(mcs_mg *mg, int l) { return mg->lv[l].fsol; }
 template <> const float *coarse_inv<float>(mcs_mg *mg) { return mg->ainv32; }
+template <typename T> static T *lalt(mcs_mg *mg, int l);
+template <> double *lalt<double>(mcs_mg *mg, int l) { return mg->lv[l].alt; }
+template <> float *lalt<float>(mcs_mg *mg, int l) { return mg->lv[l].falt; }
 
 template <typename T>
 static void mg_bc(mcs_mg *mg, int l, T *a)
@@ -959,11 +1343,13 @@ static size_t tail_bytes(const mcs_mg *mg, int level)
 template <typename T>
 static bool in_tail(const mcs_mg *mg, int level)
 {
-    return tail_bytes<T>(mg, level) <= mg->tail_lds;
+    const int32_t *n = mg->lv[level].n;
+    return tail_bytes<T>(mg, level) <= mg->tail_lds && mg->cfg.levels - level < mg->tail_levels &&
+           (level == mg->cfg.levels || (int64_t)n[0] * n[1] * n[2] <= mg->tail_max);
 }
 
 template <typename T>
-static void mg_tail(mcs_mg *mg, int level, T *x, const T *b)
+static void mg_tail(mcs_mg *mg, int level, T *x, const T *b, const T *zc, bool zero, bool fresh)
 {
     Tail Tl;
     memset(&Tl, 0, sizeof(Tl));
@@ -984,19 +1370,127 @@ static void mg_tail(mcs_mg *mg, int level, T *x, const T *b)
     }
     ProfScope ps(mg->ctx, "mg_tail");
     hipLaunchKernelGGL(k_mg_tail<T>, dim3(1), dim3(MG_TAIL_THREADS), (size_t)off * sizeof(T), mg->ctx->stream, Tl,
-                       level, x, b);
+                       level, x, b, zc, zero ? 1 : 0, fresh ? 1 : 0);
+}
+
+// Tile of the tiled smoother for `level` with halo H: per axis the largest even
+// divisor of the level size up to sm_tile, shrunk until the boxes of u (grown by
+// H) and b (grown by H - 1) fit in LDS.
+template <typename T>
+static bool smooth_plan(const mcs_mg *mg, int level, int halo, SmoothArgs &A, size_t &lds, unsigned &nblocks)
+{
+    const int32_t *n = mg->lv[level].n;
+    for (int target = mg->sm_tile; target >= 2; target /= 2) {
+        size_t box = sizeof(T), boxb = sizeof(T);
+        nblocks = 1;
+        for (int d = 0; d < 3; ++d) {
+            int t = 2;
+            for (int c = 2; c <= target && c <= n[d]; c += 2)
+                if (n[d] % c == 0) t = c;
+            if (n[d] % t) return false;
+            A.tile[d] = t;
+            A.ntile[d] = n[d] / t;
+            nblocks *= (unsigned)A.ntile[d];
+            box *= (size_t)std::min(t + 2 * halo, n[d]);
+            boxb *= (size_t)std::min(t + 2 * halo - 2, n[d]);
+        }
+        if (box + boxb <= mg->sm_lds) {
+            lds = box + boxb;
+            return true;
+        }
+    }
+    return false;
+}
+
+template <typename T>
+static bool smooth_launch(mcs_mg *mg, int level, const T *u_in, const T *b, T *u_out, int passes, int ghost_pass,
+                          int resid, const T *xc, T *rh, int interp)
+{
+    SmoothArgs A;
+    memset(&A, 0, sizeof(A));
+    A.L = lvl(mg, level);
+    A.bc = coefs(mg);
+    A.passes = passes;
+    A.halo = passes + resid;
+    A.ghost_pass = ghost_pass;
+    A.resid = resid;
+    A.interp = interp;
+    size_t lds = 0;
+    unsigned nb = 0;
+    if (!smooth_plan<T>(mg, level, A.halo, A, lds, nb)) return false;
+    ProfScope ps(mg->ctx, "mg_rbgs");
+    hipLaunchKernelGGL(k_mg_smooth<T>, dim3(nb), dim3(MG_SM_THREADS), lds, mg->ctx->stream, A, u_in, b, u_out,
+                       xc, rh);
+    return true;
+}
+
+template <typename T>
+static bool smooth_fits(mcs_mg *mg, int level)
+{
+    if (!mg->sm_lds || !lalt<T>(mg, level)) return false;
+    if ((int64_t)mg->lv[level].n[0] * mg->lv[level].n[1] * mg->lv[level].n[2] > mg->sm_max) return false;
+    SmoothArgs A;
+    size_t lds;
+    unsigned nb;
+    return smooth_plan<T>(mg, level, 2 * mg->cfg.smooth1 + 1, A, lds, nb) &&
+           smooth_plan<T>(mg, level, 2 * mg->cfg.smooth2, A, lds, nb);
 }
 
 // mgv! (Multigrid.jl:334-377).  apply! before each smoothing sweep and before the
 // residual is folded into the kernels; the ghosts are materialised before the
 // last post-smoothing sweep (they are what the caller's prolongation reads).
+// The call also takes the steps around it: zero starts from x = 0 (the xh .= 0
+// of the caller's mgv!); for the coarse-to-fine sweep of fmg!, zc != nullptr
+// starts from x = interpolate!(zc) (zc's ghosts materialised) and fresh leaves
+// the ghosts of the final x (fmg!'s apply! before its next interpolate!).
+// x itself is only written, so it may not alias zc.
 template <typename T>
-static void mgv(mcs_mg *mg, T *x, const T *b, int level)
+static void mgv(mcs_mg *mg, T *x, const T *b, int level, bool zero, const T *zc = nullptr, bool fresh = false)
 {
     if (in_tail<T>(mg, level)) {
-        mg_tail<T>(mg, level, x, b);
+        mg_tail<T>(mg, level, x, b, zc, zero, fresh);
         return;
     }
+    if (smooth_fits<T>(mg, level)) {
+        // pre-smoothing ending with the restricted residual; the coarse cycle;
+        // post-smoothing starting with the added prolongation, the ghosts as the
+        // last apply! leaves them (before the last sweep).  Each launch reads one
+        // of x and y and writes the other; the launch count is kept even so that
+        // the result ends in x.
+        T *xh = lsol<T>(mg, level + 1);
+        T *rh = lres<T>(mg, level + 1);
+        const int s1 = mg->cfg.smooth1, s2 = mg->cfg.smooth2;
+        int spl = (int64_t)mg->lv[level].n[0] * mg->lv[level].n[1] * mg->lv[level].n[2] >= mg->sm_big ? 1 : 2;
+        int k1 = s1 > 0 ? (s1 + spl - 1) / spl : 1, k2 = (s2 + spl - 1) / spl;
+        if ((k1 + k2) % 2) {
+            spl = 2;
+            k1 = 1;
+            k2 = 1;
+        }
+        T *cur = x, *oth = lalt<T>(mg, level);
+        int left = s1;
+        for (int i = 0; i < k1; ++i) {
+            const int sw = std::min(spl, left);
+            left -= sw;
+            const bool last = i == k1 - 1;
+            smooth_launch<T>(mg, level, cur, b, oth, 2 * sw, -1, last ? 1 : 0, i == 0 ? zc : (const T *)nullptr,
+                             last ? rh : (T *)nullptr, i > 0 ? 0 : zc ? 2 : zero ? 3 : 0);
+            std::swap(cur, oth);
+        }
+        mgv<T>(mg, xh, rh, level + 1, true);
+        left = s2;
+        for (int i = 0; i < k2; ++i) {
+            const int sw = std::min(spl, left);
+            left -= sw;
+            const bool last = i == k2 - 1;
+            smooth_launch<T>(mg, level, cur, b, oth, 2 * sw, last ? (fresh ? 2 * sw : 2 * sw - 2) : -1, 0,
+                             i == 0 ? (const T *)xh : (const T *)nullptr, (T *)nullptr, i == 0 ? 1 : 0);
+            std::swap(cur, oth);
+        }
+        return;
+    }
+    if (zc) mg_interp<T>(mg, level, x, zc, 0);
+    else if (zero) mg_fill0<T>(mg, x, mg->lv[level].len);
     for (int it = 0; it < mg->cfg.smooth1; ++it) mg_gs<T>(mg, level, x, b);
     if (mg->cfg.smooth2 == 0) mg_bc<T>(mg, level, x);
     {
@@ -1006,12 +1500,13 @@ static void mgv(mcs_mg *mg, T *x, const T *b, int level)
                            lsol<T>(mg, level + 1));
     }
     T *xh = lsol<T>(mg, level + 1);
-    mgv<T>(mg, xh, lres<T>(mg, level + 1), level + 1);
+    mgv<T>(mg, xh, lres<T>(mg, level + 1), level + 1, false);
     mg_interp<T>(mg, level, x, xh, 1);
     for (int it = 0; it < mg->cfg.smooth2; ++it) {
         if (it == mg->cfg.smooth2 - 1) mg_bc<T>(mg, level, x);
         mg_gs<T>(mg, level, x, b);
     }
+    if (fresh) mg_bc<T>(mg, level, x);
 }
 
 // The correction part of fmg! (Multigrid.jl:396-426) on the hierarchy of one
@@ -1030,12 +1525,13 @@ static T *fmg_correction(mcs_mg *mg)
     hipLaunchKernelGGL(k_mg_coarse<T>, dim3(1), dim3(MCS_BLOCK), 0, ctx->stream, coarse_inv<T>(mg),
                        (const T *)lres<T>(mg, L), lsol<T>(mg, L), lvl(mg, L));
     T *z = lsol<T>(mg, L);
+    // apply!(z) of the coarse solution; the later ones are left by each mgv!
+    // (fresh), which also starts from interpolate!(z1, z)
+    mg_bc<T>(mg, L, z);
     for (int k = 1; k <= L; ++k) {
         int lev = L - k;
         T *z1 = lsol<T>(mg, lev);
-        mg_bc<T>(mg, lev + 1, z);
-        mg_interp<T>(mg, lev, z1, z, 0);
-        mgv<T>(mg, z1, lres<T>(mg, lev), lev);
+        mgv<T>(mg, z1, lres<T>(mg, lev), lev, false, (const T *)z, true);
         z = z1;
     }
     return z;

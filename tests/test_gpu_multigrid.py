@@ -131,3 +131,54 @@ def test_fp32_smoother_in_fp64_refinement(n, bc, levels):
     assert not np.array_equal(out["f32"][0], u64)   # the FP32 correction path ran
     scale = np.abs(u64).max()
     assert np.abs(out["f32"][0] - u64).max() < 1e-6 * scale
+
+
+@pytest.mark.parametrize("n,bc,levels,smooth,prec,knobs", [
+    ((32, 32, 64), (N, N, N, N, D, D), 4, (2, 2), "f64", {}),
+    ((48, 40, 56), (D, N, D, N, N, D), 3, (1, 2), "f64", {}),
+    ((64, 64, 128), (N, N, N, N, D, D), 5, (2, 1), "f64", {}),
+    ((32, 32, 64), (N, N, N, N, D, D), 4, (2, 2), "f32", {}),
+    ((32, 32, 64), (N, D, N, N, D, D), 4, (2, 2), "f64", {"MCS_MG_SM_BIG": "0", "MCS_MG_SM_TILE": "8"}),
+    ((48, 40, 56), (D, N, D, N, N, D), 3, (2, 2), "f64", {"MCS_MG_SM_BIG": "0"}),
+    # more tiles than run at once: later tiles must not see what earlier ones wrote
+    ((64, 64, 128), (N, N, N, N, D, D), 5, (2, 2), "f64", {"MCS_MG_SM_TILE": "8"}),
+    ((128, 128, 256), (N, N, N, N, D, D), 6, (2, 2), "f64", {"MCS_MG_TAIL_LEVELS": "13"}),
+    ((128, 128, 256), (N, N, N, N, D, D), 6, (2, 2), "f64",
+     {"MCS_MG_TAIL_LEVELS": "3", "MCS_MG_SM_MAXCELLS": "100000000", "MCS_MG_SM_TILE": "16"}),
+])
+def test_tiled_smoother_matches_per_pass(monkeypatch, n, bc, levels, smooth, prec, knobs):
+    """The tiled smoother (a level's pre-smoothing with the restricted residual,
+    and the prolongation with post-smoothing, one or two sweeps per launch, halos
+    recomputed in LDS) equals one launch per red-black pass bitwise, on every
+    level but the coarsest (tail capped to one level unless stated), for tile sizes
+    that are not powers of two, for one- and two-sweep smoothing, and on grids of
+    more tiles than the device runs at once."""
+    from mcstreamer_amd import _lib
+    import ctypes
+    rng = np.random.default_rng(3 + sum(n))
+    g = (n[0] + 2) * (n[1] + 2) * (n[2] + 2)
+    b = rng.normal(size=g)
+    u0 = rng.normal(size=g) * 0.1
+    faces = [mg.BottomBnd, mg.TopBnd, mg.LeftBnd, mg.RightBnd, mg.FrontBnd, mg.BackBnd]
+    monkeypatch.setenv("MCS_MG_TAIL_LEVELS", "1")
+    monkeypatch.delenv("MCS_MG_SM_TILE", raising=False)
+    monkeypatch.delenv("MCS_MG_SM_BIG", raising=False)
+    monkeypatch.delenv("MCS_MG_SM_MAXCELLS", raising=False)
+    for k, v in knobs.items():
+        monkeypatch.setenv(k, v)
+    out = {}
+    for fused in ("0", "1"):
+        monkeypatch.setenv("MCS_MG_FUSED", fused)
+        conf = mg.MGConfig(bc=list(zip(faces, bc)), s=0.9, levels=levels, tolerance=-1e-9, maxiter=50,
+                           smooth1=smooth[0], smooth2=smooth[1], precision=prec)
+        ws = mg.Workspace(conf, n)
+        u = u0.copy()
+        it = ctypes.c_int32()
+        res = ctypes.c_double()
+        rc = _lib.lib().mcs_mg_solve_host(ws.handle, _lib.dptr(u), _lib.dptr(b), ctypes.byref(it),
+                                          ctypes.byref(res))
+        assert rc == 0
+        out[fused] = (u, it.value)
+        print(f"fused={fused} n={n} iters {it.value} resid {res.value:.3e}", flush=True)
+    assert out["0"][1] == out["1"][1]
+    np.testing.assert_array_equal(out["0"][0], out["1"][0])
