@@ -171,6 +171,7 @@ struct mcs_mesh {
     int64_t *pre64 = nullptr;       // ncells + 2 (their prefix)
     bool sorted_valid = false;
     float *face32[3] = {nullptr, nullptr, nullptr};  // FP32 face fields (precision mode)
+    int32_t *dep_list = nullptr;    // ncells + 1: count, then the cells the CIC gather gives a wavefront
 };
 
 // FP32 copies of the face fields, allocated and filled on first use; from then on

@@ -66,6 +66,7 @@ extern "C" int32_t mcs_mesh_destroy(mcs_mesh *m)
     hipFree(m->cell_cnt);
     hipFree(m->cnt64);
     hipFree(m->pre64);
+    if (m->dep_list) hipFree(m->dep_list);
     delete m;
     return MCS_OK;
 }
@@ -429,10 +430,12 @@ struct DepArgs {
     uint64_t *pk;
     double4 *pw;
     int64_t n;
+    int32_t *dense;   // [0] count, then cells with more than DEP_DENSE records to visit
 };
 
 #define DEP_MASK 0x1FFFFFULL
-#define DEP_PF 8    // records loaded ahead of the sequential accumulation
+#define DEP_PF 8       // records loaded ahead of the sequential accumulation
+#define DEP_DENSE 64   // cells visiting more particle records take a wavefront each
 
 // Per particle: g = x/h - 1/2, i0 = floor(g), fr = g - i0, base = (w val)/dV,
 // exactly the quantities a sequential CIC scatter computes per particle.
@@ -440,6 +443,7 @@ template <typename Buf>
 __global__ __launch_bounds__(MCS_BLOCK) void k_deposit_prep(DepArgs<Buf> a)
 {
     int64_t q = gidx();
+    if (q == 0) a.dense[0] = 0;
     if (q >= a.n) return;
     const double h = a.g.h;
     const double dV = (h * h) * h;
@@ -454,6 +458,21 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_deposit_prep(DepArgs<Buf> a)
     }
     a.pk[q] = key;
     a.pw[q] = make_double4(fr[0], fr[1], fr[2], (get_w(a.b, q) * a.val) / dV);
+}
+
+// The share of one particle record in cell (i, j, k): false when it has none.
+__device__ __forceinline__ bool dep_term(uint64_t key, const double4 &wv, int64_t i, int64_t j, int64_t k,
+                                         double &term)
+{
+    int64_t ix = (int64_t)(key & DEP_MASK) - 1;
+    int64_t iy = (int64_t)((key >> 21) & DEP_MASK) - 1;
+    int64_t iz = (int64_t)((key >> 42) & DEP_MASK) - 1;
+    if ((i != ix && i != ix + 1) || (j != iy && j != iy + 1) || (k != iz && k != iz + 1)) return false;
+    double wx = i == ix ? 1.0 - wv.x : wv.x;
+    double wy = j == iy ? 1.0 - wv.y : wv.y;
+    double wz = k == iz ? 1.0 - wv.z : wv.z;
+    term = wv.w * ((wx * wy) * wz);
+    return true;
 }
 
 __device__ __forceinline__ void dep_visit(double &acc, uint64_t key, const double4 &wv, int64_t i,
@@ -493,6 +512,22 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_deposit(DepArgs<Buf> a)
     }
     // the three neighbours along x of one (y, z) row hold one contiguous range
     const int64_t ilo = i > 0 ? i - 1 : 0, ihi = i + 1 < nx ? i + 1 : nx - 1;
+    {
+        // cells with many records go to k_deposit_dense
+        int64_t tot = 0;
+        for (int dk = -1; dk <= 1; ++dk)
+            for (int dj = -1; dj <= 1; ++dj) {
+                const int64_t kk = k + dk, jj = j + dj;
+                if (kk < 0 || kk >= nz || jj < 0 || jj >= ny) continue;
+                const int64_t row = nx * (jj + ny * kk);
+                tot += a.cs[row + ihi + 1] - a.cs[row + ilo];
+            }
+        if (tot > DEP_DENSE) {
+            const int slot = atomicAdd(a.dense, 1);
+            a.dense[1 + slot] = (int32_t)c;
+            return;
+        }
+    }
     for (int dk = -1; dk <= 1; ++dk) {
         int64_t kk = k + dk;
         if (kk < 0 || kk >= nz) continue;
@@ -524,6 +559,78 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_deposit(DepArgs<Buf> a)
         }
     }
     a.arr[idx] = acc;
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+    const uint64_t w = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)w, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(w >> 32), l);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// The CIC gather of the listed cells, one wavefront per cell: the lanes load 64
+// consecutive records of a range at once (the next 64 while the current ones are
+// added) and compute their shares; the shares are then added to the cell's sum
+// lane by lane in record order (a ballot of the lanes with a share), which is
+// the order of k_deposit's sequential loop, so the sum is bitwise the same.
+template <typename Buf>
+__global__ __launch_bounds__(MCS_BLOCK) void k_deposit_dense(DepArgs<Buf> a)
+{
+    const int lane = threadIdx.x & (MCS_WAVE - 1);
+    const int wpb = blockDim.x / MCS_WAVE;
+    const int nw = gridDim.x * wpb;
+    const int ndense = a.dense[0];
+    const int64_t nx = a.g.n[0], ny = a.g.n[1], nz = a.g.n[2];
+    for (int e = blockIdx.x * wpb + threadIdx.x / MCS_WAVE; e < ndense; e += nw) {
+        const int64_t c = a.dense[1 + e];
+        const uint32_t cu = (uint32_t)c, qy = cu / (uint32_t)nx;
+        const int64_t i = cu - qy * (uint32_t)nx, j = qy % (uint32_t)ny, k = qy / (uint32_t)ny;
+        const int64_t idx = (i + 1) + (nx + 2) * ((j + 1) + (ny + 2) * (k + 1));
+        const int64_t ilo = i > 0 ? i - 1 : 0, ihi = i + 1 < nx ? i + 1 : nx - 1;
+        double acc = a.arr[idx];
+        for (int dk = -1; dk <= 1; ++dk) {
+            const int64_t kk = k + dk;
+            if (kk < 0 || kk >= nz) continue;
+            for (int dj = -1; dj <= 1; ++dj) {
+                const int64_t jj = j + dj;
+                if (jj < 0 || jj >= ny) continue;
+                const int64_t row = nx * (jj + ny * kk);
+                int64_t q = a.cs[row + ilo];
+                const int64_t s1 = a.cs[row + ihi + 1];
+                uint64_t kc = 0;
+                double4 wc = make_double4(0.0, 0.0, 0.0, 0.0);
+                bool hc = q + lane < s1;
+                if (hc) {
+                    kc = a.pk[q + lane];
+                    wc = a.pw[q + lane];
+                }
+                while (q < s1) {
+                    const int64_t qn = q + MCS_WAVE;
+                    uint64_t kn = 0;
+                    double4 wn = make_double4(0.0, 0.0, 0.0, 0.0);
+                    const bool hn = qn + lane < s1;
+                    if (hn) {
+                        kn = a.pk[qn + lane];
+                        wn = a.pw[qn + lane];
+                    }
+                    double term = 0.0;
+                    const bool on = hc && dep_term(kc, wc, i, j, k, term);
+                    uint64_t m = __ballot(on);
+                    while (m) {
+                        const int l = __builtin_ctzll(m);
+                        acc += readlane_f64(term, l);
+                        m &= m - 1;
+                    }
+                    q = qn;
+                    kc = kn;
+                    wc = wn;
+                    hc = hn;
+                }
+            }
+        }
+        if (lane == 0) a.arr[idx] = acc;
+    }
 }
 
 template <typename Buf>
@@ -563,14 +670,18 @@ static int32_t deposit_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, int32_t whi
     a.pk = p->dep_k;
     a.pw = (double4 *)p->dep_w;
     a.n = p->n;
+    if (scheme == MCS_DEPOSIT_CIC && !m->dep_list)
+        MCS_HIP(ctx, hipMalloc(&m->dep_list, sizeof(int32_t) * (m->ncells + 1)));
+    a.dense = m->dep_list;
     {
         ProfScope ps(ctx, "deposit");
         dim3 g((unsigned)cdiv(m->ncells, MCS_BLOCK)), bl(MCS_BLOCK);
         if (scheme == MCS_DEPOSIT_CIC) {
-            if (p->n > 0)
-                hipLaunchKernelGGL(k_deposit_prep<Buf>, dim3((unsigned)cdiv(p->n, MCS_BLOCK)), bl, 0,
-                                   ctx->stream, a);
+            // the prep kernel also clears the list count (one block at least)
+            hipLaunchKernelGGL(k_deposit_prep<Buf>, dim3((unsigned)cdiv(p->n > 0 ? p->n : 1, MCS_BLOCK)), bl, 0,
+                               ctx->stream, a);
             hipLaunchKernelGGL((k_deposit<1, Buf>), g, bl, 0, ctx->stream, a);
+            hipLaunchKernelGGL(k_deposit_dense<Buf>, dim3(2048), bl, 0, ctx->stream, a);
         } else {
             hipLaunchKernelGGL((k_deposit<0, Buf>), g, bl, 0, ctx->stream, a);
         }

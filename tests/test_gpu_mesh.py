@@ -73,10 +73,18 @@ def test_sort_handles_long_segments():
     _same_pop(pop, P)
 
 
-@pytest.mark.parametrize("scheme", ["ngp", "cic"])
-def test_deposit_parity(scheme):
+@pytest.mark.parametrize("scheme,blob", [("ngp", False), ("cic", False), ("cic", True)])
+def test_deposit_parity(scheme, blob):
     n, h = (12, 10, 14), 2e-6
     pop, fields, P, M, _ = _pair(n, h, 20000, 3, p_outside=0.02)
+    if blob:
+        # a dense core and sparse edges: cells on both sides of the wavefront
+        # threshold of the CIC gather
+        ps = pop.download()
+        rng = np.random.default_rng(11)
+        ps.x[:] = np.array(n) * h / 2 + rng.normal(size=ps.x.shape) * 1.5 * h
+        pop.upload(ps)
+        P = R.Pop(ps.x, ps.v, ps.w, ps.s, ps.active, ps.id, cap=P.cap)
     streamer.sort_by_cell(pop, fields)
     R.sort_by_cell(P, M)
     streamer.density(fields, "qpart", pop, -1.0, scheme)
