@@ -30,9 +30,10 @@
 #define MG_TAIL_LDS (160 * 1024)
 #define MG_TAIL_THREADS 512
 // Measured on the 128x128x256 streamer solve (7 levels): the single-workgroup
-// tail pays off only on the last two levels, the tiled smoother on levels of up
-// to ~2^18 cells; the finer levels run one launch per pass.
-#define MG_TAIL_MAXCELLS 256
+// tail pays off from levels of up to ~2^13 cells (it also needs the levels in
+// LDS), the tiled smoother on levels of up to ~2^18 cells; the finer levels run
+// one launch per pass.
+#define MG_TAIL_MAXCELLS 8192
 #define MG_SM_MAXCELLS ((int64_t)1 << 18)
 
 struct Lvl {
@@ -920,8 +921,8 @@ __device__ bool coarsest_try_dpp(T *x, const T *b, const Lvl &Lv, const BcCoef &
 }
 
 // mgv! (Multigrid.jl:334-377) for levels start..L inside one workgroup: the level
-// solutions and the deeper right-hand sides live in LDS, the start level's
-// right-hand side is read from global memory; x0 is loaded and written back.
+// solutions and right-hand sides live in LDS; x0 and b0 are loaded (x0 unless
+// the start is zero or a prolongation), x0 is written back.
 // Phases are separated by barriers and follow the grid path's order exactly.
 template <typename E>
 __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, E *x0, const E *b0, const E *zc,
@@ -941,10 +942,11 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
     } else {
         tail_copy(lds + T.off_x[start], x0, T.len[start]);
     }
+    tail_copy(lds + T.off_b[start], b0, T.len[start]);
     __syncthreads();
     for (int lev = start; lev < T.L; ++lev) {
         E *x = lds + T.off_x[lev];
-        const E *b = lev == start ? b0 : lds + T.off_b[lev];
+        const E *b = lds + T.off_b[lev];
         const Lvl Lv = T.n[lev];
         const int64_t ncol = (int64_t)(Lv.n[0] / 2) * Lv.n[1] * Lv.n[2];
         for (int it = 0; it < T.smooth1; ++it) {
@@ -966,7 +968,7 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
     {
         const int lev = T.L;
         E *x = lds + T.off_x[lev];
-        const E *b = lev == start ? b0 : lds + T.off_b[lev];
+        const E *b = lds + T.off_b[lev];
         const Lvl Lv = T.n[lev];
         const int64_t N = (int64_t)Lv.n[0] * Lv.n[1] * Lv.n[2];
         if (N <= MCS_WAVE) {
@@ -988,7 +990,7 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
     }
     for (int lev = T.L - 1; lev >= start; --lev) {
         E *x = lds + T.off_x[lev];
-        const E *b = lev == start ? b0 : lds + T.off_b[lev];
+        const E *b = lds + T.off_b[lev];
         const Lvl Lv = T.n[lev];
         const int64_t nc = (int64_t)(Lv.n[0] / 2) * (Lv.n[1] / 2) * (Lv.n[2] / 2);
         BLOOP(t, nc) interp_item(x, lds + T.off_x[lev + 1], Lv, 1, t);
@@ -1330,13 +1332,13 @@ static int32_t ensure_bc_dev(mcs_mg *mg)
     return MCS_OK;
 }
 
-// LDS bytes of a tail from `level`: the solutions of levels level..L and the
-// right-hand sides of levels level+1..L.
+// LDS bytes of a tail from `level`: the solutions and right-hand sides of
+// levels level..L.
 template <typename T>
 static size_t tail_bytes(const mcs_mg *mg, int level)
 {
-    size_t tot = (size_t)mg->lv[level].len * sizeof(T);
-    for (int l = level + 1; l <= mg->cfg.levels; ++l) tot += 2 * (size_t)mg->lv[l].len * sizeof(T);
+    size_t tot = 0;
+    for (int l = level; l <= mg->cfg.levels; ++l) tot += 2 * (size_t)mg->lv[l].len * sizeof(T);
     return tot;
 }
 
@@ -1365,7 +1367,7 @@ static void mg_tail(mcs_mg *mg, int level, T *x, const T *b, const T *zc, bool z
             Tl.off_x[l] = off;
             off += Tl.len[l];
             Tl.off_b[l] = off;
-            if (l > level) off += Tl.len[l];
+            off += Tl.len[l];
         }
     }
     ProfScope ps(mg->ctx, "mg_tail");
