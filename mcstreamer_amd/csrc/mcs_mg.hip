@@ -854,10 +854,69 @@ __device__ __forceinline__ float dpp_move(float v)
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 
+// x / 6 without a branch: the correction of div6 with the sign of x kept (so
+// zeros come out as x / 6.0 does); `bad` records an x for which the correction
+// is not proven (tiny, huge, non-finite), whose caller then recomputes with
+// the division.  FP32 multiplies by the reciprocal, as div6 does.
+__device__ __forceinline__ double div6_flag(double x, bool &bad)
+{
+    const double a = fabs(x);
+    bad |= !(a < 0x1p960) || (a != 0.0 && !(a > 0x1p-960));
+    const double c = 1.0 / 6.0;
+    const double q = x * c;
+    return copysign(__builtin_fma(__builtin_fma(-q, 6.0, x), c, q), x);
+}
+__device__ __forceinline__ float div6_flag(float x, bool &) { return x * (1.0f / 6.0f); }
+
+// The 50 rounds on the lanes' registers; EXACT divides, otherwise div6_flag.
+// Every lane computes both colours' updates and keeps its own: no divergence.
+// Returns whether any update of this lane needed the division.
+template <typename T, int SY, int SZ, bool EXACT>
+__device__ __forceinline__ bool coarsest_dpp_rounds(T *x, const Lvl &Lv, const BcCoef &bc, T u, const T bb,
+                                                    int64_t c, bool own, int i, int j, int k)
+{
+    constexpr int ZP = SZ ? 0x100 + SZ : 0x101, ZM = SZ ? 0x110 + SZ : 0x111;
+    const int lane = threadIdx.x;
+    const int nx = Lv.n[0], ny = Lv.n[1], nz = Lv.n[2];
+    const bool up[2] = {own && ((i + j + k) & 1) == 0, own && ((i + j + k) & 1) == 1};
+    const bool ixp = i + 1 < nx, iyp = j + 1 < ny, izp = k + 1 < nz, ixm = i > 0, iym = j > 0, izm = k > 0;
+    bool bad = false;
+    for (int it = 0; it < 50; ++it) {
+        if (it == 49) {
+            if (own) x[c] = u;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            for (int64_t t = lane; t < bc_count(Lv); t += MCS_WAVE) bc_item(x, Lv, bc, t);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+#pragma unroll
+        for (int par = 0; par < 2; ++par) {
+            const T vxp = dpp_move<0x101>(u), vxm = dpp_move<0x111>(u);
+            const T vyp = dpp_move<0x100 + SY>(u), vym = dpp_move<0x110 + SY>(u);
+            const T vzp = dpp_move<ZP>(u), vzm = dpp_move<ZM>(u);
+            T l = T(-6) * u;
+            l += ixp ? vxp : bcf(bc, 3, u);
+            l += iyp ? vyp : bcf(bc, 4, u);
+            l += izp ? vzp : bcf(bc, 5, u);
+            l += ixm ? vxm : bcf(bc, 0, u);
+            l += iym ? vym : bcf(bc, 1, u);
+            l += izm ? vzm : bcf(bc, 2, u);
+            const T y = T(1) * (l + bb);
+            bool bd = false;
+            const T q = EXACT ? y / T(6) : div6_flag(y, bd);
+            bad |= bd && up[par];
+            const T un = u + q;
+            u = up[par] ? un : u;
+        }
+    }
+    if (own) x[c] = u;
+    return bad;
+}
+
 template <typename T, int SY, int SZ>
 __device__ void coarsest_dpp(T *x, const T *b, const Lvl &Lv, const BcCoef &bc)
 {
-    constexpr int ZP = SZ ? 0x100 + SZ : 0x101, ZM = SZ ? 0x110 + SZ : 0x111;
     const int lane = threadIdx.x;
     const int nx = Lv.n[0], ny = Lv.n[1], nz = Lv.n[2];
     const int N = nx * ny * nz;
@@ -869,35 +928,15 @@ __device__ void coarsest_dpp(T *x, const T *b, const Lvl &Lv, const BcCoef &bc)
         k = lane / (nx * ny);
     }
     const int64_t c = g_index(Lv, i + 1, j + 1, k + 1);
-    T u = own ? x[c] : T(0);
+    const T u0 = own ? x[c] : T(0);
     const T bb = own ? b[c] : T(0);
-    const int par_me = (i + j + k) & 1;
-    for (int it = 0; it < 50; ++it) {
-        if (it == 49) {
-            if (own) x[c] = u;
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            for (int64_t t = lane; t < bc_count(Lv); t += MCS_WAVE) bc_item(x, Lv, bc, t);
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-        }
-        for (int par = 0; par < 2; ++par) {
-            const T vxp = dpp_move<0x101>(u), vxm = dpp_move<0x111>(u);
-            const T vyp = dpp_move<0x100 + SY>(u), vym = dpp_move<0x110 + SY>(u);
-            const T vzp = dpp_move<ZP>(u), vzm = dpp_move<ZM>(u);
-            if (own && par_me == par) {
-                T l = T(-6) * u;
-                l += (i + 1 < nx) ? vxp : bcf(bc, 3, u);
-                l += (j + 1 < ny) ? vyp : bcf(bc, 4, u);
-                l += (k + 1 < nz) ? vzp : bcf(bc, 5, u);
-                l += (i > 0) ? vxm : bcf(bc, 0, u);
-                l += (j > 0) ? vym : bcf(bc, 1, u);
-                l += (k > 0) ? vzm : bcf(bc, 2, u);
-                u = u + div6(T(1) * (l + bb));
-            }
-        }
+    const bool bad = coarsest_dpp_rounds<T, SY, SZ, false>(x, Lv, bc, u0, bb, c, own, i, j, k);
+    if (__builtin_amdgcn_ballot_w64(bad)) {
+        // an update the correction does not cover: redo all rounds dividing
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        coarsest_dpp_rounds<T, SY, SZ, true>(x, Lv, bc, u0, bb, c, own, i, j, k);
     }
-    if (own) x[c] = u;
 }
 
 // Picks the DPP form for levels of at most 16 cells with power-of-two x and y
