@@ -433,25 +433,37 @@ __global__ void k_mg_to32(float *dst, const double *src, int64_t n)
 }
 
 // x += z on the interior with an FP32 correction.
-__global__ __launch_bounds__(MCS_BLOCK) void k_mg_axpy32(double *x, const float *z, Lvl L)
+// x .+= z; apply!(x) at the end of fmg!: a boundary cell also writes the ghost
+// cells whose nearest interior cell it is, as bc_item would from the new value.
+template <typename Z>
+__global__ __launch_bounds__(MCS_BLOCK) void k_mg_axpy_bc(double *x, const Z *z, Lvl L, BcCoef bc)
 {
-    int64_t t = gidx();
+    int64_t t = xcd_gidx();
     if (t >= (int64_t)L.n[0] * L.n[1] * L.n[2]) return;
     int64_t i, j, k;
     split3(t, (uint32_t)L.n[0], (uint32_t)L.n[1], i, j, k);
-    int64_t c = g_index(L, i + 1, j + 1, k + 1);
-    x[c] += (double)z[c];
-}
-
-// x += z on the interior (the ghosts are rewritten by the apply! that follows).
-__global__ __launch_bounds__(MCS_BLOCK) void k_mg_axpy(double *x, const double *z, Lvl L)
-{
-    int64_t t = gidx();
-    if (t >= (int64_t)L.n[0] * L.n[1] * L.n[2]) return;
-    int64_t i, j, k;
-        split3(t, (uint32_t)L.n[0], (uint32_t)L.n[1], i, j, k);
-    int64_t c = g_index(L, i + 1, j + 1, k + 1);
-    x[c] += z[c];
+    const int64_t c = g_index(L, i + 1, j + 1, k + 1);
+    const double v = x[c] + (double)z[c];
+    x[c] = v;
+    const int n0 = L.n[0], n1 = L.n[1], n2 = L.n[2];
+    if (i > 0 && i + 1 < n0 && j > 0 && j + 1 < n1 && k > 0 && k + 1 < n2) return;
+    // per axis: the cell itself, its low ghost, its high ghost (with their flips)
+    const int xs[3] = {(int)i, -1, n0}, ys[3] = {(int)j, -1, n1}, zs[3] = {(int)k, -1, n2};
+    const bool xo[3] = {true, i == 0, i + 1 == n0}, yo[3] = {true, j == 0, j + 1 == n1},
+               zo[3] = {true, k == 0, k + 1 == n2};
+    const uint32_t neg = bc.neg;
+    const uint32_t fx[3] = {0u, neg & 1u, (neg >> 3) & 1u}, fy[3] = {0u, (neg >> 1) & 1u, (neg >> 4) & 1u},
+                   fz[3] = {0u, (neg >> 2) & 1u, (neg >> 5) & 1u};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int bq = 0; bq < 3; ++bq)
+#pragma unroll
+            for (int cq = 0; cq < 3; ++cq) {
+                if ((a | bq | cq) == 0 || !xo[a] || !yo[bq] || !zo[cq]) continue;
+                const uint32_t flip = fx[a] ^ fy[bq] ^ fz[cq];
+                x[g_index(L, (int64_t)xs[a] + 1, (int64_t)ys[bq] + 1, (int64_t)zs[cq] + 1)] = flip ? -v : v;
+            }
 }
 
 // setinhom! (boundaries.jl:202-214): b += (2/s) val on the first valid layer.
@@ -1533,7 +1545,10 @@ static void mgv(mcs_mg *mg, T *x, const T *b, int level, bool zero, const T *zc 
     if (zc) mg_interp<T>(mg, level, x, zc, 0);
     else if (zero) mg_fill0<T>(mg, x, mg->lv[level].len);
     for (int it = 0; it < mg->cfg.smooth1; ++it) mg_gs<T>(mg, level, x, b);
-    if (mg->cfg.smooth2 == 0) mg_bc<T>(mg, level, x);
+    // the correction of level 0 is only added to the solution (interior): its
+    // ghost shell is never read, so its apply! launches are left out
+    const bool ghosts = level > 0;
+    if (mg->cfg.smooth2 == 0 && ghosts) mg_bc<T>(mg, level, x);
     {
         ProfScope ps(mg->ctx, "mg_residual");
         hipLaunchKernelGGL(k_mg_resid_restrict<T>, grid_for(ncoarse_cells(mg, level)), dim3(MCS_BLOCK), 0,
@@ -1544,10 +1559,10 @@ static void mgv(mcs_mg *mg, T *x, const T *b, int level, bool zero, const T *zc 
     mgv<T>(mg, xh, lres<T>(mg, level + 1), level + 1, false);
     mg_interp<T>(mg, level, x, xh, 1);
     for (int it = 0; it < mg->cfg.smooth2; ++it) {
-        if (it == mg->cfg.smooth2 - 1) mg_bc<T>(mg, level, x);
+        if (it == mg->cfg.smooth2 - 1 && ghosts) mg_bc<T>(mg, level, x);
         mg_gs<T>(mg, level, x, b);
     }
-    if (fresh) mg_bc<T>(mg, level, x);
+    if (fresh && ghosts) mg_bc<T>(mg, level, x);
 }
 
 // The correction part of fmg! (Multigrid.jl:396-426) on the hierarchy of one
@@ -1590,23 +1605,24 @@ static void fmg_body(mcs_mg *mg, double *x, const double *b)
         hipLaunchKernelGGL(k_mg_to32, grid_for(mg->lv[0].len), dim3(MCS_BLOCK), 0, ctx->stream,
                            mg->lv[0].fres, (const double *)mg->lv[0].res, mg->lv[0].len);
         float *z = fmg_correction<float>(mg);
-        hipLaunchKernelGGL(k_mg_axpy32, grid_for(ncells(mg, 0)), dim3(MCS_BLOCK), 0, ctx->stream, x,
-                           (const float *)z, lvl(mg, 0));
+        hipLaunchKernelGGL(k_mg_axpy_bc<float>, grid_for(ncells(mg, 0)), dim3(MCS_BLOCK), 0, ctx->stream, x,
+                           (const float *)z, lvl(mg, 0), coefs(mg));
     } else {
         double *z = fmg_correction<double>(mg);
-        hipLaunchKernelGGL(k_mg_axpy, grid_for(ncells(mg, 0)), dim3(MCS_BLOCK), 0, ctx->stream, x,
-                           (const double *)z, lvl(mg, 0));
+        hipLaunchKernelGGL(k_mg_axpy_bc<double>, grid_for(ncells(mg, 0)), dim3(MCS_BLOCK), 0, ctx->stream, x,
+                           (const double *)z, lvl(mg, 0), coefs(mg));
     }
-    mg_bc<double>(mg, 0, x);
 }
 
 // fmg! (Multigrid.jl:380-434).  Returns 1 when a correction was applied, 0 when
-// the residual already meets the tolerance, negative on error.
-static int fmg(mcs_mg *mg, double *x, const double *b, double *eps_out)
+// the residual already meets the tolerance, negative on error.  ghosts_fresh:
+// x's ghost shell is already what apply!(x) writes (the previous correction of
+// this solve ended with it), so the first apply! is skipped.
+static int fmg(mcs_mg *mg, double *x, const double *b, double *eps_out, bool ghosts_fresh)
 {
     mcs_ctx *ctx = mg->ctx;
     const MgLevel &F = mg->lv[0];
-    mg_bc<double>(mg, 0, x);
+    if (!ghosts_fresh) mg_bc<double>(mg, 0, x);
     {
         ProfScope ps(ctx, "mg_residual");
         hipLaunchKernelGGL(k_mg_residual, grid_for(ncells(mg, 0)), dim3(MCS_BLOCK), 0, ctx->stream,
@@ -1679,7 +1695,7 @@ static int32_t mg_solve_dev(mcs_mg *mg, double *u, const double *b, int32_t *ite
     int cont = 1;
     double eps = 0.0;
     while (cont && iter <= mg->cfg.maxiter) {
-        cont = fmg(mg, u, b, &eps);
+        cont = fmg(mg, u, b, &eps, iter > 0);
         if (cont < 0) return mcs_fail(ctx, MCS_E_HIP, "mg: device error during the cycle");
         iter += 1;
     }
