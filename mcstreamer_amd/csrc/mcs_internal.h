@@ -206,6 +206,7 @@ struct mcs_mg {
     int sm_tile = 16;                            // its largest tile edge
     int64_t sm_big = 0;                          // levels of this many cells: one sweep per launch
     int64_t sm_max = 0;                          // levels of more cells: one launch per pass
+    bool split0 = false;                         // level-0 correction in the colour-split layout
     bool use_graph = true;                       // replay the cycle body as a HIP graph
     hipGraphExec_t graph_exec = nullptr;
     double *graph_x = nullptr;

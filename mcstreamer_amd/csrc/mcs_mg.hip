@@ -75,6 +75,15 @@ __device__ __forceinline__ int64_t g_index(const Lvl &L, int64_t i, int64_t j, i
     return i + (int64_t)(L.n[0] + 2) * (j + (int64_t)(L.n[1] + 2) * k);
 }
 
+// Colour-split index (the level-0 correction, see k_mg_rbgs_s): cell (i, j, k) of
+// colour p = (i+j+k) % 2 at p*N/2 + (i>>1) + nx/2*(j + ny*k).
+__device__ __forceinline__ int64_t s_index(const Lvl &L, int64_t i, int64_t j, int64_t k)
+{
+    const int64_t nxh = L.n[0] / 2;
+    const int64_t half = nxh * L.n[1] * L.n[2];
+    return ((i + j + k) & 1) * half + (i >> 1) + nxh * (j + (int64_t)L.n[1] * k);
+}
+
 // t -> (i, j, k) with i fastest, in 32-bit arithmetic (levels hold < 2^31 cells).
 __device__ __forceinline__ void split3(int64_t t64, uint32_t nx, uint32_t ny, int64_t &i, int64_t &j,
                                        int64_t &k)
@@ -312,8 +321,9 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_mg_resid_restrict(T *rh, const T 
 }
 
 // residual! stored on the interior, with per-block sums of r^2 (fmg!'s norm).
+// split: r is stored in the colour-split layout of the level-0 correction.
 __global__ __launch_bounds__(MCS_BLOCK) void k_mg_residual(double *r, const double *u, const double *b,
-                                                           double s, Lvl L, BcCoef bc, double *partial)
+                                                           double s, Lvl L, BcCoef bc, double *partial, int split)
 {
     int64_t t = xcd_gidx();
     int64_t N = (int64_t)L.n[0] * L.n[1] * L.n[2];
@@ -324,7 +334,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_mg_residual(double *r, const doub
         int64_t c = g_index(L, i + 1, j + 1, k + 1);
         double uc = u[c];
         double rv = s * b[c] + lap_fold(u, L, bc, i, j, k, c, uc);
-        r[c] = rv;
+        r[split ? s_index(L, i, j, k) : c] = rv;
         sq = rv * rv;
     }
     __shared__ double sh[MCS_BLOCK];
@@ -436,14 +446,14 @@ __global__ void k_mg_to32(float *dst, const double *src, int64_t n)
 // x .+= z; apply!(x) at the end of fmg!: a boundary cell also writes the ghost
 // cells whose nearest interior cell it is, as bc_item would from the new value.
 template <typename Z>
-__global__ __launch_bounds__(MCS_BLOCK) void k_mg_axpy_bc(double *x, const Z *z, Lvl L, BcCoef bc)
+__global__ __launch_bounds__(MCS_BLOCK) void k_mg_axpy_bc(double *x, const Z *z, Lvl L, BcCoef bc, int split)
 {
     int64_t t = xcd_gidx();
     if (t >= (int64_t)L.n[0] * L.n[1] * L.n[2]) return;
     int64_t i, j, k;
     split3(t, (uint32_t)L.n[0], (uint32_t)L.n[1], i, j, k);
     const int64_t c = g_index(L, i + 1, j + 1, k + 1);
-    const double v = x[c] + (double)z[c];
+    const double v = x[c] + (double)z[split ? s_index(L, i, j, k) : c];
     x[c] = v;
     const int n0 = L.n[0], n1 = L.n[1], n2 = L.n[2];
     if (i > 0 && i + 1 < n0 && j > 0 && j + 1 < n1 && k > 0 && k + 1 < n2) return;
@@ -481,6 +491,142 @@ __global__ void k_mg_inhom(double *b, Lvl L, int face, double c, const double *v
     idx[d2] = a2 + 1;
     int64_t cidx = g_index(L, idx[0], idx[1], idx[2]);
     b[cidx] = b[cidx] + c * val[t];
+}
+
+// ---- colour-split layout of the level-0 correction ----------------------------
+//
+// The finest level's correction and right-hand side (private to the solver, their
+// ghosts never read) are stored by colour: cell (i, j, k) of colour
+// p = (i+j+k) % 2 at p*N/2 + (i>>1) + nx/2*(j + ny*k).  A red-black pass then reads
+// and writes only its colour's half and the other colour's neighbours, instead
+// of every line of both arrays; the arithmetic is unchanged.
+
+// lap_fold on the split layout: o is the other colour's half, off the cell's
+// offset within a half.  Out-of-level neighbours read the cell's own offset (a
+// valid index) and are replaced by the ghost rule.
+template <typename T>
+__device__ __forceinline__ T lap_split(const T *__restrict__ o, const Lvl &L, const BcCoef &bc, int64_t i,
+                                      int64_t j, int64_t k, int64_t off, T uc)
+{
+    const int64_t sy = L.n[0] / 2, sz = (int64_t)(L.n[0] / 2) * L.n[1];
+    const bool xp = i + 1 < L.n[0], yp = j + 1 < L.n[1], zp = k + 1 < L.n[2], xm = i > 0, ym = j > 0, zm = k > 0;
+    const int64_t odd = i & 1;
+    const T vxp = o[xp ? off + odd : off], vyp = o[yp ? off + sy : off], vzp = o[zp ? off + sz : off];
+    const T vxm = o[xm ? off - 1 + odd : off], vym = o[ym ? off - sy : off], vzm = o[zm ? off - sz : off];
+    T l = T(-6) * uc;
+    l += xp ? vxp : bcf(bc, 3, uc);
+    l += yp ? vyp : bcf(bc, 4, uc);
+    l += zp ? vzp : bcf(bc, 5, uc);
+    l += xm ? vxm : bcf(bc, 0, uc);
+    l += ym ? vym : bcf(bc, 1, uc);
+    l += zm ? vzm : bcf(bc, 2, uc);
+    return l;
+}
+
+// gs_item on the split layout: thread t is the t-th cell of colour par.
+template <typename T>
+__global__ __launch_bounds__(MCS_BLOCK) void k_mg_rbgs_s(T *u, const T *b, Lvl L, BcCoef bc, int par)
+{
+    const int64_t t = xcd_gidx();
+    const int64_t half = (int64_t)(L.n[0] / 2) * L.n[1] * L.n[2];
+    if (t >= half) return;
+    int64_t ti, j, k;
+    split3(t, (uint32_t)(L.n[0] / 2), (uint32_t)L.n[1], ti, j, k);
+    const int64_t i = 2 * ti + ((j + k + par) & 1);
+    T *up = u + par * half;
+    const T uc = up[t];
+    const T l = lap_split(u + (1 - par) * half, L, bc, i, j, k, t, uc);
+    up[t] = uc + div6(T(1) * (l + b[par * half + t]));
+}
+
+// resid_restrict_item with the fine arrays split (the coarse ones keep ghosts).
+template <typename T>
+__global__ __launch_bounds__(MCS_BLOCK) void k_mg_resid_restrict_s(T *rh, const T *u, const T *b, Lvl F, BcCoef bc,
+                                                                   T *xh)
+{
+    const int64_t t = xcd_gidx();
+    const int64_t half = (int64_t)(F.n[0] / 2) * F.n[1] * F.n[2];
+    if (t >= (int64_t)(F.n[0] / 2) * (F.n[1] / 2) * (F.n[2] / 2)) return;
+    int64_t ic, jc, kc;
+    split3(t, (uint32_t)(F.n[0] / 2), (uint32_t)(F.n[1] / 2), ic, jc, kc);
+    T sum = T(0);
+#pragma unroll
+    for (int dx = 0; dx < 2; ++dx)
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+            for (int dz = 0; dz < 2; ++dz) {
+                const int64_t i = 2 * ic + dx, j = 2 * jc + dy, k = 2 * kc + dz;
+                const int64_t p = (i + j + k) & 1, off = (i >> 1) + (int64_t)(F.n[0] / 2) * (j + (int64_t)F.n[1] * k);
+                const T uc = u[p * half + off];
+                const T l = lap_split(u + (1 - p) * half, F, bc, i, j, k, off, uc);
+                sum += T(1) * b[p * half + off] + l;
+            }
+    Lvl C;
+    C.n[0] = F.n[0] / 2;
+    C.n[1] = F.n[1] / 2;
+    C.n[2] = F.n[2] / 2;
+    const int64_t cidx = g_index(C, ic + 1, jc + 1, kc + 1);
+    rh[cidx] = (T(4) * sum) / T(8);
+    xh[cidx] = T(0);
+}
+
+// restrict_item of a split fine array.
+template <typename T>
+__global__ __launch_bounds__(MCS_BLOCK) void k_mg_restrict_s(T *rh, const T *r, Lvl F)
+{
+    const int64_t t = gidx();
+    if (t >= (int64_t)(F.n[0] / 2) * (F.n[1] / 2) * (F.n[2] / 2)) return;
+    int64_t i, j, k;
+    split3(t, (uint32_t)(F.n[0] / 2), (uint32_t)(F.n[1] / 2), i, j, k);
+    const int64_t cy = F.n[0] / 2 + 2, cz = (int64_t)(F.n[0] / 2 + 2) * (F.n[1] / 2 + 2);
+    const int64_t x0 = 2 * i, y0 = 2 * j, z0 = 2 * k;
+    T s = T(0);
+    s += r[s_index(F, x0, y0, z0)];
+    s += r[s_index(F, x0, y0, z0 + 1)];
+    s += r[s_index(F, x0, y0 + 1, z0)];
+    s += r[s_index(F, x0, y0 + 1, z0 + 1)];
+    s += r[s_index(F, x0 + 1, y0, z0)];
+    s += r[s_index(F, x0 + 1, y0, z0 + 1)];
+    s += r[s_index(F, x0 + 1, y0 + 1, z0)];
+    s += r[s_index(F, x0 + 1, y0 + 1, z0 + 1)];
+    rh[(i + 1) + cy * (j + 1) + cz * (k + 1)] = (T(4) * s) / T(8);
+}
+
+// interp_item into a split fine array (add or overwrite).
+template <typename T>
+__global__ __launch_bounds__(MCS_BLOCK) void k_mg_interp_s(T *r, const T *rh, Lvl F, int add)
+{
+    const int64_t t = xcd_gidx();
+    if (t >= (int64_t)(F.n[0] / 2) * (F.n[1] / 2) * (F.n[2] / 2)) return;
+    const int64_t nc0 = F.n[0] / 2, nc1 = F.n[1] / 2;
+    int64_t i, j, k;
+    split3(t, (uint32_t)nc0, (uint32_t)nc1, i, j, k);
+    const int64_t cy = nc0 + 2, cz = (int64_t)(nc0 + 2) * (nc1 + 2);
+    const int64_t cc = (i + 1) + cy * (j + 1) + cz * (k + 1);
+    const T w0 = T(27) / T(64), w1 = T(9) / T(64), w2 = T(3) / T(64), w3 = T(1) / T(64);
+#pragma unroll
+    for (int sx = 0; sx < 2; ++sx)
+#pragma unroll
+        for (int sy = 0; sy < 2; ++sy)
+#pragma unroll
+            for (int sz = 0; sz < 2; ++sz) {
+                const int64_t f = s_index(F, 2 * i + sx, 2 * j + sy, 2 * k + sz);
+                const int64_t ox = 2 * sx - 1, oy = (2 * sy - 1) * cy, oz = (2 * sz - 1) * cz;
+                T v = T(0);
+                v += w0 * rh[cc];
+                v += w1 * rh[cc + oz];
+                v += w1 * rh[cc + oy];
+                v += w2 * rh[cc + oy + oz];
+                v += w1 * rh[cc + ox];
+                v += w2 * rh[cc + ox + oz];
+                v += w2 * rh[cc + ox + oy];
+                v += w3 * rh[cc + ox + oy + oz];
+                if (add)
+                    r[f] += v;
+                else
+                    r[f] = v;
+            }
 }
 
 // ---- tiled smoother: one visit of mgv! to a grid level in two launches --------
@@ -1137,6 +1283,7 @@ static int coarse_inverse(const int32_t n[3], const int32_t bc[6], std::vector<d
 }
 
 template <typename T> static bool in_tail(const mcs_mg *mg, int level);
+template <typename T> static bool smooth_fits(mcs_mg *mg, int level);
 
 extern "C" int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg **out)
 {
@@ -1248,6 +1395,14 @@ extern "C" int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg 
             }
         }
     }
+    // the level-0 correction in the colour-split layout when level 0 runs one
+    // launch per pass (MCS_MG_SPLIT0=0: ghost layout, same results)
+    {
+        const char *ze = getenv("MCS_MG_SPLIT0");
+        const bool per_pass = f32 ? (!in_tail<float>(mg, 0) && !smooth_fits<float>(mg, 0))
+                                  : (!in_tail<double>(mg, 0) && !smooth_fits<double>(mg, 0));
+        mg->split0 = c.levels > 0 && per_pass && !(ze && ze[0] == '0');
+    }
     const int32_t *nc = mg->lv.back().n;
     mg->ncoarse = (int64_t)nc[0] * nc[1] * nc[2];
     if (mg->ncoarse > 4096)
@@ -1351,14 +1506,19 @@ static void mg_bc(mcs_mg *mg, int l, T *a)
 }
 
 template <typename T>
-static void mg_gs(mcs_mg *mg, int l, T *u, const T *b)
+static void mg_gs(mcs_mg *mg, int l, T *u, const T *b, bool split = false)
 {
     Lvl L = lvl(mg, l);
     int64_t cnt = (int64_t)(L.n[0] / 2) * L.n[1] * L.n[2];
     ProfScope ps(mg->ctx, "mg_rbgs");
-    for (int par = 0; par < 2; ++par)
-        hipLaunchKernelGGL(k_mg_rbgs<T>, grid_for(cnt), dim3(MCS_BLOCK), 0, mg->ctx->stream, u, b, L, coefs(mg),
-                           par);
+    for (int par = 0; par < 2; ++par) {
+        if (split)
+            hipLaunchKernelGGL(k_mg_rbgs_s<T>, grid_for(cnt), dim3(MCS_BLOCK), 0, mg->ctx->stream, u, b, L,
+                               coefs(mg), par);
+        else
+            hipLaunchKernelGGL(k_mg_rbgs<T>, grid_for(cnt), dim3(MCS_BLOCK), 0, mg->ctx->stream, u, b, L,
+                               coefs(mg), par);
+    }
 }
 
 template <typename T>
@@ -1368,11 +1528,15 @@ static void mg_fill0(mcs_mg *mg, T *a, int64_t n)
 }
 
 template <typename T>
-static void mg_interp(mcs_mg *mg, int lf, T *r, const T *rh, int add)
+static void mg_interp(mcs_mg *mg, int lf, T *r, const T *rh, int add, bool split = false)
 {
     ProfScope ps(mg->ctx, "mg_prolong");
-    hipLaunchKernelGGL(k_mg_interp<T>, grid_for(ncoarse_cells(mg, lf)), dim3(MCS_BLOCK), 0, mg->ctx->stream, r,
-                       rh, lvl(mg, lf), add);
+    if (split)
+        hipLaunchKernelGGL(k_mg_interp_s<T>, grid_for(ncoarse_cells(mg, lf)), dim3(MCS_BLOCK), 0, mg->ctx->stream,
+                           r, rh, lvl(mg, lf), add);
+    else
+        hipLaunchKernelGGL(k_mg_interp<T>, grid_for(ncoarse_cells(mg, lf)), dim3(MCS_BLOCK), 0, mg->ctx->stream,
+                           r, rh, lvl(mg, lf), add);
 }
 
 static int32_t ensure_bc_dev(mcs_mg *mg)
@@ -1542,25 +1706,31 @@ static void mgv(mcs_mg *mg, T *x, const T *b, int level, bool zero, const T *zc 
         }
         return;
     }
-    if (zc) mg_interp<T>(mg, level, x, zc, 0);
+    const bool sp = level == 0 && mg->split0;
+    if (zc) mg_interp<T>(mg, level, x, zc, 0, sp);
     else if (zero) mg_fill0<T>(mg, x, mg->lv[level].len);
-    for (int it = 0; it < mg->cfg.smooth1; ++it) mg_gs<T>(mg, level, x, b);
+    for (int it = 0; it < mg->cfg.smooth1; ++it) mg_gs<T>(mg, level, x, b, sp);
     // the correction of level 0 is only added to the solution (interior): its
     // ghost shell is never read, so its apply! launches are left out
     const bool ghosts = level > 0;
     if (mg->cfg.smooth2 == 0 && ghosts) mg_bc<T>(mg, level, x);
     {
         ProfScope ps(mg->ctx, "mg_residual");
-        hipLaunchKernelGGL(k_mg_resid_restrict<T>, grid_for(ncoarse_cells(mg, level)), dim3(MCS_BLOCK), 0,
-                           mg->ctx->stream, lres<T>(mg, level + 1), (const T *)x, b, lvl(mg, level), coefs(mg),
-                           lsol<T>(mg, level + 1));
+        if (sp)
+            hipLaunchKernelGGL(k_mg_resid_restrict_s<T>, grid_for(ncoarse_cells(mg, level)), dim3(MCS_BLOCK), 0,
+                               mg->ctx->stream, lres<T>(mg, level + 1), (const T *)x, b, lvl(mg, level),
+                               coefs(mg), lsol<T>(mg, level + 1));
+        else
+            hipLaunchKernelGGL(k_mg_resid_restrict<T>, grid_for(ncoarse_cells(mg, level)), dim3(MCS_BLOCK), 0,
+                               mg->ctx->stream, lres<T>(mg, level + 1), (const T *)x, b, lvl(mg, level),
+                               coefs(mg), lsol<T>(mg, level + 1));
     }
     T *xh = lsol<T>(mg, level + 1);
     mgv<T>(mg, xh, lres<T>(mg, level + 1), level + 1, false);
-    mg_interp<T>(mg, level, x, xh, 1);
+    mg_interp<T>(mg, level, x, xh, 1, sp);
     for (int it = 0; it < mg->cfg.smooth2; ++it) {
         if (it == mg->cfg.smooth2 - 1 && ghosts) mg_bc<T>(mg, level, x);
-        mg_gs<T>(mg, level, x, b);
+        mg_gs<T>(mg, level, x, b, sp);
     }
     if (fresh && ghosts) mg_bc<T>(mg, level, x);
 }
@@ -1575,8 +1745,12 @@ static T *fmg_correction(mcs_mg *mg)
     const int L = mg->cfg.levels;
     for (int k = 0; k < L; ++k) {
         ProfScope ps(ctx, "mg_restrict");
-        hipLaunchKernelGGL(k_mg_restrict<T>, grid_for(ncoarse_cells(mg, k)), dim3(MCS_BLOCK), 0, ctx->stream,
-                           lres<T>(mg, k + 1), (const T *)lres<T>(mg, k), lvl(mg, k));
+        if (k == 0 && mg->split0)
+            hipLaunchKernelGGL(k_mg_restrict_s<T>, grid_for(ncoarse_cells(mg, k)), dim3(MCS_BLOCK), 0, ctx->stream,
+                               lres<T>(mg, k + 1), (const T *)lres<T>(mg, k), lvl(mg, k));
+        else
+            hipLaunchKernelGGL(k_mg_restrict<T>, grid_for(ncoarse_cells(mg, k)), dim3(MCS_BLOCK), 0, ctx->stream,
+                               lres<T>(mg, k + 1), (const T *)lres<T>(mg, k), lvl(mg, k));
     }
     hipLaunchKernelGGL(k_mg_coarse<T>, dim3(1), dim3(MCS_BLOCK), 0, ctx->stream, coarse_inv<T>(mg),
                        (const T *)lres<T>(mg, L), lsol<T>(mg, L), lvl(mg, L));
@@ -1606,11 +1780,11 @@ static void fmg_body(mcs_mg *mg, double *x, const double *b)
                            mg->lv[0].fres, (const double *)mg->lv[0].res, mg->lv[0].len);
         float *z = fmg_correction<float>(mg);
         hipLaunchKernelGGL(k_mg_axpy_bc<float>, grid_for(ncells(mg, 0)), dim3(MCS_BLOCK), 0, ctx->stream, x,
-                           (const float *)z, lvl(mg, 0), coefs(mg));
+                           (const float *)z, lvl(mg, 0), coefs(mg), mg->split0 ? 1 : 0);
     } else {
         double *z = fmg_correction<double>(mg);
         hipLaunchKernelGGL(k_mg_axpy_bc<double>, grid_for(ncells(mg, 0)), dim3(MCS_BLOCK), 0, ctx->stream, x,
-                           (const double *)z, lvl(mg, 0), coefs(mg));
+                           (const double *)z, lvl(mg, 0), coefs(mg), mg->split0 ? 1 : 0);
     }
 }
 
@@ -1626,7 +1800,8 @@ static int fmg(mcs_mg *mg, double *x, const double *b, double *eps_out, bool gho
     {
         ProfScope ps(ctx, "mg_residual");
         hipLaunchKernelGGL(k_mg_residual, grid_for(ncells(mg, 0)), dim3(MCS_BLOCK), 0, ctx->stream,
-                           F.res, (const double *)x, b, mg->cfg.s, lvl(mg, 0), coefs(mg), mg->partial);
+                           F.res, (const double *)x, b, mg->cfg.s, lvl(mg, 0), coefs(mg), mg->partial,
+                           mg->split0 ? 1 : 0);
     }
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(MCS_BLOCK), 0, ctx->stream,
                        (const double *)mg->partial, mg->npartial, mg->partial + mg->npartial);

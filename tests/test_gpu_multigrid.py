@@ -182,3 +182,33 @@ def test_tiled_smoother_matches_per_pass(monkeypatch, n, bc, levels, smooth, pre
         print(f"fused={fused} n={n} iters {it.value} resid {res.value:.3e}", flush=True)
     assert out["0"][1] == out["1"][1]
     np.testing.assert_array_equal(out["0"][0], out["1"][0])
+
+
+@pytest.mark.parametrize("n,levels,prec", [((64, 64, 128), 5, "f64"), ((128, 128, 256), 6, "f64"),
+                                          ((64, 64, 128), 5, "f32")])
+def test_split_level0_matches_ghost_layout(monkeypatch, n, levels, prec):
+    """The level-0 correction in the colour-split layout (residual, restriction,
+    prolongation, passes and x += z indexing it) gives bitwise the solution of the
+    ghost layout."""
+    from mcstreamer_amd import _lib
+    import ctypes
+    rng = np.random.default_rng(5 + sum(n))
+    g = (n[0] + 2) * (n[1] + 2) * (n[2] + 2)
+    b = rng.normal(size=g)
+    u0 = rng.normal(size=g) * 0.1
+    faces = [mg.BottomBnd, mg.TopBnd, mg.LeftBnd, mg.RightBnd, mg.FrontBnd, mg.BackBnd]
+    out = {}
+    for split in ("0", "1"):
+        monkeypatch.setenv("MCS_MG_SPLIT0", split)
+        conf = mg.MGConfig(bc=list(zip(faces, (N, D, N, N, D, D))), s=0.9, levels=levels, tolerance=-1e-9,
+                           maxiter=50, precision=prec)
+        ws = mg.Workspace(conf, n)
+        u = u0.copy()
+        it = ctypes.c_int32()
+        res = ctypes.c_double()
+        assert _lib.lib().mcs_mg_solve_host(ws.handle, _lib.dptr(u), _lib.dptr(b), ctypes.byref(it),
+                                            ctypes.byref(res)) == 0
+        out[split] = (u, it.value, res.value)
+        print(f"split={split} n={n} iters {it.value} resid {res.value:.3e}", flush=True)
+    assert out["0"][1] == out["1"][1] and out["0"][2] == out["1"][2]
+    np.testing.assert_array_equal(out["0"][0], out["1"][0])
