@@ -153,7 +153,7 @@ struct mcs_pop {
     int32_t *i32a = nullptr, *i32b = nullptr, *i32c = nullptr;
     uint64_t *u64a = nullptr;
     int64_t *blk = nullptr;   // block totals, then their prefix (2 * (cap/256 + 2) + 4)
-    void *dep_w = nullptr;     // CIC deposit records (double4 x cap), allocated on first use
+    void *dep_w = nullptr;     // CIC deposit shares (8 doubles x cap), allocated on first use
     uint64_t *dep_k = nullptr;
 };
 

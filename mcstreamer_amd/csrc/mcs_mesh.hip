@@ -426,9 +426,9 @@ struct DepArgs {
     double *arr;
     double val;
     // per-particle CIC record, computed once: the lower cell index per axis packed
-    // in 21-bit fields (offset by one), and {fr_x, fr_y, fr_z, w*val/dV}
+    // in 21-bit fields (offset by one), and the shares of the 8 cells it feeds
     uint64_t *pk;
-    double4 *pw;
+    double *pc;       // 8 per particle, corner (dx, dy, dz) at 4 dx + 2 dy + dz
     int64_t n;
     int32_t *dense;   // [0] count, then cells with more than DEP_DENSE records to visit
 };
@@ -437,8 +437,10 @@ struct DepArgs {
 #define DEP_PF 8       // records loaded ahead of the sequential accumulation
 #define DEP_DENSE 64   // cells visiting more particle records take a wavefront each
 
-// Per particle: g = x/h - 1/2, i0 = floor(g), fr = g - i0, base = (w val)/dV,
-// exactly the quantities a sequential CIC scatter computes per particle.
+// Per particle: g = x/h - 1/2, i0 = floor(g), fr = g - i0, base = (w val)/dV and
+// the share base * ((wx wy) wz) of each of the 8 cells i0 + (dx, dy, dz), with
+// w = 1 - fr towards i0 and fr towards i0 + 1: exactly the products a
+// sequential CIC scatter adds.
 template <typename Buf>
 __global__ __launch_bounds__(MCS_BLOCK) void k_deposit_prep(DepArgs<Buf> a)
 {
@@ -457,41 +459,89 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_deposit_prep(DepArgs<Buf> a)
         key |= ((uint64_t)fc & DEP_MASK) << (21 * d);
     }
     a.pk[q] = key;
-    a.pw[q] = make_double4(fr[0], fr[1], fr[2], (get_w(a.b, q) * a.val) / dV);
+    const double base = (get_w(a.b, q) * a.val) / dV;
+    double sh[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const double wx = (c & 4) ? fr[0] : 1.0 - fr[0];
+        const double wy = (c & 2) ? fr[1] : 1.0 - fr[1];
+        const double wz = (c & 1) ? fr[2] : 1.0 - fr[2];
+        sh[c] = base * ((wx * wy) * wz);
+    }
+    double4 *dst = (double4 *)(a.pc + 8 * q);
+    dst[0] = make_double4(sh[0], sh[1], sh[2], sh[3]);
+    dst[1] = make_double4(sh[4], sh[5], sh[6], sh[7]);
 }
 
-// The share of one particle record in cell (i, j, k): false when it has none.
-__device__ __forceinline__ bool dep_term(uint64_t key, const double4 &wv, int64_t i, int64_t j, int64_t k,
-                                         double &term)
+// Loads that stay inside the arrays whatever the predicate (a record past the
+// range reads record 0 and is discarded).
+template <typename Buf>
+__device__ __forceinline__ uint64_t dep_key(const DepArgs<Buf> &a, int64_t q, int64_t s1)
 {
-    int64_t ix = (int64_t)(key & DEP_MASK) - 1;
-    int64_t iy = (int64_t)((key >> 21) & DEP_MASK) - 1;
-    int64_t iz = (int64_t)((key >> 42) & DEP_MASK) - 1;
-    if ((i != ix && i != ix + 1) || (j != iy && j != iy + 1) || (k != iz && k != iz + 1)) return false;
-    double wx = i == ix ? 1.0 - wv.x : wv.x;
-    double wy = j == iy ? 1.0 - wv.y : wv.y;
-    double wz = k == iz ? 1.0 - wv.z : wv.z;
-    term = wv.w * ((wx * wy) * wz);
-    return true;
+    const uint64_t v = a.pk[q < s1 ? q : 0];
+    return q < s1 ? v : ~0ULL;
 }
 
-__device__ __forceinline__ void dep_visit(double &acc, uint64_t key, const double4 &wv, int64_t i,
-                                          int64_t j, int64_t k)
+template <typename Buf>
+__device__ __forceinline__ double dep_share(const DepArgs<Buf> &a, bool on, int64_t q, int corner)
 {
-    int64_t ix = (int64_t)(key & DEP_MASK) - 1;
-    int64_t iy = (int64_t)((key >> 21) & DEP_MASK) - 1;
-    int64_t iz = (int64_t)((key >> 42) & DEP_MASK) - 1;
-    if ((i != ix && i != ix + 1) || (j != iy && j != iy + 1) || (k != iz && k != iz + 1)) return;
-    double wx = i == ix ? 1.0 - wv.x : wv.x;
-    double wy = j == iy ? 1.0 - wv.y : wv.y;
-    double wz = k == iz ? 1.0 - wv.z : wv.z;
-    acc += wv.w * ((wx * wy) * wz);
+    return a.pc[on ? 8 * q + corner : 0];
+}
+
+// Whether a record (key) feeds cell (i, j, k), and at which corner.
+__device__ __forceinline__ bool dep_corner(uint64_t key, int i, int j, int k, int &corner)
+{
+    const int dx = i + 1 - (int)(key & DEP_MASK);
+    const int dy = j + 1 - (int)((key >> 21) & DEP_MASK);
+    const int dz = k + 1 - (int)((key >> 42) & DEP_MASK);
+    corner = 4 * dx + 2 * dy + dz;
+    return (uint32_t)dx <= 1u && (uint32_t)dy <= 1u && (uint32_t)dz <= 1u;
+}
+
+// The records a cell visits, as one sequence: the ranges of its nine neighbour
+// rows (three consecutive cells along x each) in (dk, dj) order, which is the
+// order of the sequential scatter over the sorted population.  Rows outside the
+// mesh are empty.
+struct DepRows {
+    int32_t start[9];
+    int32_t pre[10];   // prefix of the range lengths (record counts are int32, as cell starts)
+};
+
+__device__ __forceinline__ void dep_rows(const int32_t *cs, int64_t i, int64_t j, int64_t k, int64_t nx, int64_t ny,
+                                         int64_t nz, DepRows &R)
+{
+    const int64_t ilo = i > 0 ? i - 1 : 0, ihi = i + 1 < nx ? i + 1 : nx - 1;
+    R.pre[0] = 0;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const int64_t kk = k + r / 3 - 1, jj = j + r % 3 - 1;
+        int32_t s0 = 0, len = 0;
+        if (kk >= 0 && kk < nz && jj >= 0 && jj < ny) {
+            const int64_t row = nx * (jj + ny * kk);
+            s0 = cs[row + ilo];
+            len = cs[row + ihi + 1] - s0;
+        }
+        R.start[r] = s0;
+        R.pre[r + 1] = R.pre[r] + len;
+    }
+}
+
+// Record index of position v (< pre[9]) of the sequence.
+__device__ __forceinline__ int32_t dep_at(const DepRows &R, int32_t v)
+{
+    int32_t q = R.start[0] + v;
+#pragma unroll
+    for (int r = 1; r < 9; ++r)
+        if (v >= R.pre[r]) q = R.start[r] + (v - R.pre[r]);
+    return q;
 }
 
 // density! as a per-cell gather over the canonical order: cell c sums the shares of
 // the particles of its 27 neighbours in (cell, id) order, which is the order in
 // which a sequential scatter over the sorted population adds them.  No atomics.
-// Loads run DEP_PF particles ahead of the (sequential) accumulation.
+// Keys run DEP_PF records ahead along the cell's sequence; a share is loaded only
+// for a record that feeds the cell.  Cells with many records go to
+// k_deposit_dense.
 template <int CIC, typename Buf>
 __global__ __launch_bounds__(MCS_BLOCK) void k_deposit(DepArgs<Buf> a)
 {
@@ -503,60 +553,43 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_deposit(DepArgs<Buf> a)
     int64_t idx = (i + 1) + (nx + 2) * ((j + 1) + (ny + 2) * (k + 1));
     const double h = a.g.h;
     const double dV = (h * h) * h;
-    double acc = a.arr[idx];
     if (!CIC) {
+        double acc = a.arr[idx];
         int64_t s0 = a.cs[c], s1 = a.cs[c + 1];
         for (int64_t q = s0; q < s1; ++q) acc += (get_w(a.b, q) * a.val) / dV;
         a.arr[idx] = acc;
         return;
     }
-    // the three neighbours along x of one (y, z) row hold one contiguous range
-    const int64_t ilo = i > 0 ? i - 1 : 0, ihi = i + 1 < nx ? i + 1 : nx - 1;
-    {
-        // cells with many records go to k_deposit_dense
-        int64_t tot = 0;
-        for (int dk = -1; dk <= 1; ++dk)
-            for (int dj = -1; dj <= 1; ++dj) {
-                const int64_t kk = k + dk, jj = j + dj;
-                if (kk < 0 || kk >= nz || jj < 0 || jj >= ny) continue;
-                const int64_t row = nx * (jj + ny * kk);
-                tot += a.cs[row + ihi + 1] - a.cs[row + ilo];
-            }
-        if (tot > DEP_DENSE) {
-            const int slot = atomicAdd(a.dense, 1);
-            a.dense[1 + slot] = (int32_t)c;
-            return;
-        }
+    DepRows R;
+    dep_rows(a.cs, i, j, k, nx, ny, nz, R);
+    const int32_t tot = R.pre[9];
+    if (tot > DEP_DENSE) {
+        const int slot = atomicAdd(a.dense, 1);
+        a.dense[1 + slot] = (int32_t)c;
+        return;
     }
-    for (int dk = -1; dk <= 1; ++dk) {
-        int64_t kk = k + dk;
-        if (kk < 0 || kk >= nz) continue;
-        for (int dj = -1; dj <= 1; ++dj) {
-            int64_t jj = j + dj;
-            if (jj < 0 || jj >= ny) continue;
-            int64_t row = nx * (jj + ny * kk);
-            int64_t q = a.cs[row + ilo], s1 = a.cs[row + ihi + 1];
-            for (; q + DEP_PF <= s1; q += DEP_PF) {
-                uint64_t kk8[DEP_PF];
-                double4 ww8[DEP_PF];
+    if (tot == 0) return;
+    double acc = a.arr[idx];
+    const int ii = (int)i, jj = (int)j, kk = (int)k;
+    for (int32_t v = 0; v < tot; v += DEP_PF) {
+        int32_t q8[DEP_PF];
+        uint64_t kk8[DEP_PF];
 #pragma unroll
-                for (int u = 0; u < DEP_PF; ++u) {
-                    kk8[u] = a.pk[q + u];
-                    ww8[u] = a.pw[q + u];
-                }
-#pragma unroll
-                for (int u = 0; u < DEP_PF; ++u) dep_visit(acc, kk8[u], ww8[u], i, j, k);
-            }
-            for (; q + 4 <= s1; q += 4) {
-                uint64_t k0 = a.pk[q], k1 = a.pk[q + 1], k2 = a.pk[q + 2], k3 = a.pk[q + 3];
-                double4 w0 = a.pw[q], w1 = a.pw[q + 1], w2 = a.pw[q + 2], w3 = a.pw[q + 3];
-                dep_visit(acc, k0, w0, i, j, k);
-                dep_visit(acc, k1, w1, i, j, k);
-                dep_visit(acc, k2, w2, i, j, k);
-                dep_visit(acc, k3, w3, i, j, k);
-            }
-            for (; q < s1; ++q) dep_visit(acc, a.pk[q], a.pw[q], i, j, k);
+        for (int u = 0; u < DEP_PF; ++u) {
+            q8[u] = v + u < tot ? dep_at(R, v + u) : 0;
+            kk8[u] = dep_key(a, q8[u], v + u < tot ? q8[u] + 1 : 0);
         }
+        double t8[DEP_PF];
+        bool on8[DEP_PF];
+#pragma unroll
+        for (int u = 0; u < DEP_PF; ++u) {
+            int corner;
+            on8[u] = dep_corner(kk8[u], ii, jj, kk, corner);
+            t8[u] = dep_share(a, on8[u], q8[u], corner);
+        }
+#pragma unroll
+        for (int u = 0; u < DEP_PF; ++u)
+            if (on8[u]) acc += t8[u];
     }
     a.arr[idx] = acc;
 }
@@ -569,11 +602,12 @@ __device__ __forceinline__ double readlane_f64(double v, int l)
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
-// The CIC gather of the listed cells, one wavefront per cell: the lanes load 64
-// consecutive records of a range at once (the next 64 while the current ones are
-// added) and compute their shares; the shares are then added to the cell's sum
-// lane by lane in record order (a ballot of the lanes with a share), which is
-// the order of k_deposit's sequential loop, so the sum is bitwise the same.
+// The CIC gather of the listed cells, one wavefront per cell: the lanes take 64
+// consecutive positions of the cell's record sequence at once (the keys of the
+// next 64 and the shares of the current ones in flight together), and the shares
+// are added to the cell's sum lane by lane in sequence order (a ballot of the
+// lanes with a share), which is the order of k_deposit's sequential loop, so the
+// sum is bitwise the same.
 template <typename Buf>
 __global__ __launch_bounds__(MCS_BLOCK) void k_deposit_dense(DepArgs<Buf> a)
 {
@@ -587,47 +621,28 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_deposit_dense(DepArgs<Buf> a)
         const uint32_t cu = (uint32_t)c, qy = cu / (uint32_t)nx;
         const int64_t i = cu - qy * (uint32_t)nx, j = qy % (uint32_t)ny, k = qy / (uint32_t)ny;
         const int64_t idx = (i + 1) + (nx + 2) * ((j + 1) + (ny + 2) * (k + 1));
-        const int64_t ilo = i > 0 ? i - 1 : 0, ihi = i + 1 < nx ? i + 1 : nx - 1;
+        DepRows R;
+        dep_rows(a.cs, i, j, k, nx, ny, nz, R);
+        const int32_t tot = R.pre[9];
+        const int ii = (int)i, jj = (int)j, kk = (int)k;
         double acc = a.arr[idx];
-        for (int dk = -1; dk <= 1; ++dk) {
-            const int64_t kk = k + dk;
-            if (kk < 0 || kk >= nz) continue;
-            for (int dj = -1; dj <= 1; ++dj) {
-                const int64_t jj = j + dj;
-                if (jj < 0 || jj >= ny) continue;
-                const int64_t row = nx * (jj + ny * kk);
-                int64_t q = a.cs[row + ilo];
-                const int64_t s1 = a.cs[row + ihi + 1];
-                uint64_t kc = 0;
-                double4 wc = make_double4(0.0, 0.0, 0.0, 0.0);
-                bool hc = q + lane < s1;
-                if (hc) {
-                    kc = a.pk[q + lane];
-                    wc = a.pw[q + lane];
-                }
-                while (q < s1) {
-                    const int64_t qn = q + MCS_WAVE;
-                    uint64_t kn = 0;
-                    double4 wn = make_double4(0.0, 0.0, 0.0, 0.0);
-                    const bool hn = qn + lane < s1;
-                    if (hn) {
-                        kn = a.pk[qn + lane];
-                        wn = a.pw[qn + lane];
-                    }
-                    double term = 0.0;
-                    const bool on = hc && dep_term(kc, wc, i, j, k, term);
-                    uint64_t m = __ballot(on);
-                    while (m) {
-                        const int l = __builtin_ctzll(m);
-                        acc += readlane_f64(term, l);
-                        m &= m - 1;
-                    }
-                    q = qn;
-                    kc = kn;
-                    wc = wn;
-                    hc = hn;
-                }
+        int32_t qc = lane < tot ? dep_at(R, lane) : 0;
+        uint64_t kc = dep_key(a, qc, lane < tot ? qc + 1 : 0);
+        for (int32_t v = 0; v < tot; v += MCS_WAVE) {
+            const int32_t vn = v + MCS_WAVE + lane;
+            const int32_t qn = vn < tot ? dep_at(R, vn) : 0;
+            const uint64_t kn = dep_key(a, qn, vn < tot ? qn + 1 : 0);
+            int corner;
+            const bool on = dep_corner(kc, ii, jj, kk, corner);
+            const double t = dep_share(a, on, qc, corner);
+            uint64_t m = __ballot(on);
+            while (m) {
+                const int l = __builtin_ctzll(m);
+                acc += readlane_f64(t, l);
+                m &= m - 1;
             }
+            qc = qn;
+            kc = kn;
         }
         if (lane == 0) a.arr[idx] = acc;
     }
@@ -664,11 +679,11 @@ static int32_t deposit_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, int32_t whi
     a.arr = m->arr[which];
     a.val = val;
     if (scheme == MCS_DEPOSIT_CIC && !p->dep_w) {
-        MCS_HIP(ctx, hipMalloc(&p->dep_w, sizeof(double4) * p->cap));
+        MCS_HIP(ctx, hipMalloc(&p->dep_w, sizeof(double) * 8 * p->cap));
         MCS_HIP(ctx, hipMalloc(&p->dep_k, sizeof(uint64_t) * p->cap));
     }
     a.pk = p->dep_k;
-    a.pw = (double4 *)p->dep_w;
+    a.pc = (double *)p->dep_w;
     a.n = p->n;
     if (scheme == MCS_DEPOSIT_CIC && !m->dep_list)
         MCS_HIP(ctx, hipMalloc(&m->dep_list, sizeof(int32_t) * (m->ncells + 1)));
