@@ -252,7 +252,8 @@ __device__ __forceinline__ void restrict_item(T *rh, const T *r, const Lvl &F, i
 // interpolate! (Multigrid.jl:232-258): weights prod(3 - 2 sh_i)/64 over the coarse
 // neighbours towards each fine child, sh in cubestencil order; add or overwrite.
 template <typename T>
-__device__ __forceinline__ void interp_item(T *r, const T *rh, const Lvl &F, int add, int64_t t)
+__device__ __forceinline__ void interp_item(T *__restrict__ r, const T *__restrict__ rh, const Lvl &F, int add,
+                                            int64_t t)
 {
     const int64_t nc0 = F.n[0] / 2, nc1 = F.n[1] / 2;
     int64_t i, j, k;
@@ -400,7 +401,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_mg_restrict(T *rh, const T *r, Lv
 }
 
 template <typename T>
-__global__ __launch_bounds__(MCS_BLOCK) void k_mg_interp(T *r, const T *rh, Lvl F, int add)
+__global__ __launch_bounds__(MCS_BLOCK) void k_mg_interp(T *__restrict__ r, const T *__restrict__ rh, Lvl F, int add)
 {
     int64_t t = xcd_gidx();
     if (t < (int64_t)(F.n[0] / 2) * (F.n[1] / 2) * (F.n[2] / 2)) interp_item(r, rh, F, add, t);
@@ -595,7 +596,8 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_mg_restrict_s(T *rh, const T *r, 
 
 // interp_item into a split fine array (add or overwrite).
 template <typename T>
-__global__ __launch_bounds__(MCS_BLOCK) void k_mg_interp_s(T *r, const T *rh, Lvl F, int add)
+__global__ __launch_bounds__(MCS_BLOCK) void k_mg_interp_s(T *__restrict__ r, const T *__restrict__ rh, Lvl F,
+                                                           int add)
 {
     const int64_t t = xcd_gidx();
     if (t >= (int64_t)(F.n[0] / 2) * (F.n[1] / 2) * (F.n[2] / 2)) return;
