@@ -918,6 +918,7 @@ struct Tail {
     int64_t len[MG_MAXL];
     int64_t off_x[MG_MAXL];   // LDS offsets (elements) of the level solutions
     int64_t off_b[MG_MAXL];   // and of their right-hand sides
+    int64_t off_t;            // LDS offset of the fine-residual scratch (start level's cells)
     int L;
     int smooth1, smooth2;
     BcCoef bc;
@@ -1119,6 +1120,103 @@ __device__ bool coarsest_try_dpp(T *x, const T *b, const Lvl &Lv, const BcCoef &
     return false;
 }
 
+// ---- per-item bodies of the tail, in 32-bit arithmetic (levels < 2^31 cells) --
+
+// lap_fold at (i, j, k), index c of the ghost layout.
+template <typename T>
+__device__ __forceinline__ T tail_lap(const T *u, const Lvl &L, const BcCoef &bc, int i, int j, int k, int c, T uc)
+{
+    const int sy = L.n[0] + 2, sz = (L.n[0] + 2) * (L.n[1] + 2);
+    T l = T(-6) * uc;
+    l += (i + 1 < L.n[0]) ? u[c + 1] : bcf(bc, 3, uc);
+    l += (j + 1 < L.n[1]) ? u[c + sy] : bcf(bc, 4, uc);
+    l += (k + 1 < L.n[2]) ? u[c + sz] : bcf(bc, 5, uc);
+    l += (i > 0) ? u[c - 1] : bcf(bc, 0, uc);
+    l += (j > 0) ? u[c - sy] : bcf(bc, 1, uc);
+    l += (k > 0) ? u[c - sz] : bcf(bc, 2, uc);
+    return l;
+}
+
+__device__ __forceinline__ int tail_cell(const Lvl &L, int t, int &i, int &j, int &k)
+{
+    const int q = t / L.n[0];
+    i = t - q * L.n[0];
+    k = q / L.n[1];
+    j = q - k * L.n[1];
+    return (i + 1) + (L.n[0] + 2) * ((j + 1) + (L.n[1] + 2) * (k + 1));
+}
+
+// One colour pass of gauss_seidel! (gs_item's arithmetic).
+template <typename T>
+__device__ __forceinline__ void tail_gs(T *u, const T *b, const Lvl &L, const BcCoef &bc, int par)
+{
+    const int nxh = L.n[0] / 2, ny = L.n[1];
+    const int ncol = nxh * ny * L.n[2];
+    for (int t = threadIdx.x; t < ncol; t += blockDim.x) {
+        const int q = t / nxh, ti = t - q * nxh;
+        const int k = q / ny, j = q - k * ny;
+        const int i = 2 * ti + ((j + k + par) & 1);
+        const int c = (i + 1) + (L.n[0] + 2) * ((j + 1) + (ny + 2) * (k + 1));
+        const T uc = u[c];
+        const T l = tail_lap(u, L, bc, i, j, k, c, uc);
+        u[c] = uc + div6(T(1) * (l + b[c]));
+    }
+}
+
+// resid_restrict_item in two phases: every fine cell's s b + L u into rt, then
+// the sums of the 8 children in cubestencil order (the same additions).
+template <typename T>
+__device__ __forceinline__ void tail_resid(T *rt, const T *u, const T *b, const Lvl &L, const BcCoef &bc)
+{
+    const int N = L.n[0] * L.n[1] * L.n[2];
+    for (int t = threadIdx.x; t < N; t += blockDim.x) {
+        int i, j, k;
+        const int c = tail_cell(L, t, i, j, k);
+        const T uc = u[c];
+        rt[t] = T(1) * b[c] + tail_lap(u, L, bc, i, j, k, c, uc);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void tail_restrict(T *rh, T *xh, const T *rt, const Lvl &F)
+{
+    Lvl C;
+    C.n[0] = F.n[0] / 2;
+    C.n[1] = F.n[1] / 2;
+    C.n[2] = F.n[2] / 2;
+    const int NC = C.n[0] * C.n[1] * C.n[2];
+    const int sy = F.n[0], sz = F.n[0] * F.n[1];
+    for (int t = threadIdx.x; t < NC; t += blockDim.x) {
+        int ic, jc, kc;
+        const int cc = tail_cell(C, t, ic, jc, kc);
+        const int f = 2 * ic + sy * (2 * jc) + sz * (2 * kc);
+        T sum = T(0);
+        sum += rt[f];
+        sum += rt[f + sz];
+        sum += rt[f + sy];
+        sum += rt[f + sy + sz];
+        sum += rt[f + 1];
+        sum += rt[f + 1 + sz];
+        sum += rt[f + 1 + sy];
+        sum += rt[f + 1 + sy + sz];
+        rh[cc] = (T(4) * sum) / T(8);
+        xh[cc] = T(0);
+    }
+}
+
+// interpolate! into level L one fine cell per thread (interp_item's value).
+template <typename T>
+__device__ __forceinline__ void tail_interp(T *x, const T *rh, const Lvl &L, bool add)
+{
+    const int N = L.n[0] * L.n[1] * L.n[2];
+    for (int t = threadIdx.x; t < N; t += blockDim.x) {
+        int i, j, k;
+        const int c = tail_cell(L, t, i, j, k);
+        const T v = prolong_at(rh, L, i, j, k);
+        x[c] = add ? x[c] + v : v;
+    }
+}
+
 // mgv! (Multigrid.jl:334-377) for levels start..L inside one workgroup: the level
 // solutions and right-hand sides live in LDS; x0 and b0 are loaded (x0 unless
 // the start is zero or a prolongation), x0 is written back.
@@ -1130,12 +1228,11 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
     extern __shared__ double lds_raw[];
     E *lds = (E *)lds_raw;
     const BcCoef bc = T.bc;
+    E *rt = lds + T.off_t;   // fine residuals of the level being restricted
     if (zc) {
         // the start level's interior is interpolate!(x, zc) (overwrite); its
         // ghosts are materialised before they are read
-        const Lvl Lv = T.n[start];
-        BLOOP(t, (int64_t)(Lv.n[0] / 2) * (Lv.n[1] / 2) * (Lv.n[2] / 2))
-            interp_item(lds + T.off_x[start], zc, Lv, 0, t);
+        tail_interp(lds + T.off_x[start], zc, T.n[start], false);
     } else if (zero) {
         BLOOP(t, T.len[start]) lds[T.off_x[start] + t] = E(0);
     } else {
@@ -1147,11 +1244,9 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
         E *x = lds + T.off_x[lev];
         const E *b = lds + T.off_b[lev];
         const Lvl Lv = T.n[lev];
-        const int64_t ncol = (int64_t)(Lv.n[0] / 2) * Lv.n[1] * Lv.n[2];
         for (int it = 0; it < T.smooth1; ++it) {
             for (int par = 0; par < 2; ++par) {
-                for (int64_t t = threadIdx.x; t < ncol; t += 2 * (int64_t)blockDim.x)
-                    gs_item2(x, b, Lv, bc, par, t, t + blockDim.x, t + blockDim.x < ncol);
+                tail_gs(x, b, Lv, bc, par);
                 __syncthreads();
             }
         }
@@ -1159,9 +1254,11 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
             BLOOP(t, bc_count(Lv)) bc_item(x, Lv, bc, t);
             __syncthreads();
         }
-        const int64_t nc = (int64_t)(Lv.n[0] / 2) * (Lv.n[1] / 2) * (Lv.n[2] / 2);
-        BLOOP(t, nc) resid_restrict_item(lds + T.off_b[lev + 1], x, b, E(1), Lv, bc, t);
-        BLOOP(t, T.len[lev + 1]) lds[T.off_x[lev + 1] + t] = E(0);
+        tail_resid(rt, x, b, Lv, bc);
+        __syncthreads();
+        // restricted residual and the zeroed interior of the next solution (its
+        // ghosts are materialised before they are read)
+        tail_restrict(lds + T.off_b[lev + 1], lds + T.off_x[lev + 1], rt, Lv);
         __syncthreads();
     }
     {
@@ -1191,18 +1288,15 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
         E *x = lds + T.off_x[lev];
         const E *b = lds + T.off_b[lev];
         const Lvl Lv = T.n[lev];
-        const int64_t nc = (int64_t)(Lv.n[0] / 2) * (Lv.n[1] / 2) * (Lv.n[2] / 2);
-        BLOOP(t, nc) interp_item(x, lds + T.off_x[lev + 1], Lv, 1, t);
+        tail_interp(x, lds + T.off_x[lev + 1], Lv, true);
         __syncthreads();
-        const int64_t ncol = (int64_t)(Lv.n[0] / 2) * Lv.n[1] * Lv.n[2];
         for (int it = 0; it < T.smooth2; ++it) {
             if (it == T.smooth2 - 1) {
                 BLOOP(t, bc_count(Lv)) bc_item(x, Lv, bc, t);
                 __syncthreads();
             }
             for (int par = 0; par < 2; ++par) {
-                for (int64_t t = threadIdx.x; t < ncol; t += 2 * (int64_t)blockDim.x)
-                    gs_item2(x, b, Lv, bc, par, t, t + blockDim.x, t + blockDim.x < ncol);
+                tail_gs(x, b, Lv, bc, par);
                 __syncthreads();
             }
         }
@@ -1550,13 +1644,14 @@ static int32_t ensure_bc_dev(mcs_mg *mg)
 }
 
 // LDS bytes of a tail from `level`: the solutions and right-hand sides of
-// levels level..L.
+// levels level..L, and the fine residuals of the start level.
 template <typename T>
 static size_t tail_bytes(const mcs_mg *mg, int level)
 {
     size_t tot = 0;
     for (int l = level; l <= mg->cfg.levels; ++l) tot += 2 * (size_t)mg->lv[l].len * sizeof(T);
-    return tot;
+    const int32_t *n = mg->lv[level].n;
+    return tot + (size_t)n[0] * n[1] * n[2] * sizeof(T);
 }
 
 template <typename T>
@@ -1587,6 +1682,8 @@ static void mg_tail(mcs_mg *mg, int level, T *x, const T *b, const T *zc, bool z
             off += Tl.len[l];
         }
     }
+    Tl.off_t = off;
+    off += (int64_t)mg->lv[level].n[0] * mg->lv[level].n[1] * mg->lv[level].n[2];
     ProfScope ps(mg->ctx, "mg_tail");
     hipLaunchKernelGGL(k_mg_tail<T>, dim3(1), dim3(MG_TAIL_THREADS), (size_t)off * sizeof(T), mg->ctx->stream, Tl,
                        level, x, b, zc, zero ? 1 : 0, fresh ? 1 : 0);
