@@ -107,6 +107,24 @@ __device__ __forceinline__ void split3(int64_t t64, uint32_t nx, uint32_t ny, in
     k = r;
 }
 
+// q = t / d and r = t - q d for 0 <= t < 2^24 and 1 <= d < 2^24, with rd = 1/d
+// rounded to float: the float quotient is within one of the true one (t is
+// exact in float and the two roundings are below 2^-23 relative), and one
+// correction step makes it exact.
+__device__ __forceinline__ int fdivmod(int t, int d, float rd, int &r)
+{
+    int q = (int)((float)t * rd);
+    r = t - q * d;
+    if (r < 0) {
+        q -= 1;
+        r += d;
+    } else if (r >= d) {
+        q += 1;
+        r -= d;
+    }
+    return q;
+}
+
 // L u at interior cell (i, j, k) with folded ghosts, in lplstencil order
 // (+x, +y, +z, -x, -y, -z; Multigrid.jl:124-130,152-158).
 template <typename T>
@@ -665,14 +683,11 @@ struct BoxWalk {
     int x, y, z, sx, sy, sz, nx, ny;
     __device__ __forceinline__ BoxWalk(int nx_, int ny_, int t, int step) : nx(nx_), ny(ny_)
     {
-        x = t % nx;
-        int q = t / nx;
-        y = q % ny;
-        z = q / ny;
-        sx = step % nx;
-        q = step / nx;
-        sy = q % ny;
-        sz = q / ny;
+        const float rx = 1.0f / (float)nx, ry = 1.0f / (float)ny;
+        int q = fdivmod(t, nx, rx, x);
+        z = fdivmod(q, ny, ry, y);
+        q = fdivmod(step, nx, rx, sx);
+        sz = fdivmod(q, ny, ry, sy);
     }
     __device__ __forceinline__ void next()
     {
@@ -1139,10 +1154,8 @@ __device__ __forceinline__ T tail_lap(const T *u, const Lvl &L, const BcCoef &bc
 
 __device__ __forceinline__ int tail_cell(const Lvl &L, int t, int &i, int &j, int &k)
 {
-    const int q = t / L.n[0];
-    i = t - q * L.n[0];
-    k = q / L.n[1];
-    j = q - k * L.n[1];
+    const int q = fdivmod(t, L.n[0], 1.0f / (float)L.n[0], i);
+    k = fdivmod(q, L.n[1], 1.0f / (float)L.n[1], j);
     return (i + 1) + (L.n[0] + 2) * ((j + 1) + (L.n[1] + 2) * (k + 1));
 }
 
@@ -1152,9 +1165,11 @@ __device__ __forceinline__ void tail_gs(T *u, const T *b, const Lvl &L, const Bc
 {
     const int nxh = L.n[0] / 2, ny = L.n[1];
     const int ncol = nxh * ny * L.n[2];
+    const float rx = 1.0f / (float)nxh, ry = 1.0f / (float)ny;
     for (int t = threadIdx.x; t < ncol; t += blockDim.x) {
-        const int q = t / nxh, ti = t - q * nxh;
-        const int k = q / ny, j = q - k * ny;
+        int ti, j;
+        const int q = fdivmod(t, nxh, rx, ti);
+        const int k = fdivmod(q, ny, ry, j);
         const int i = 2 * ti + ((j + k + par) & 1);
         const int c = (i + 1) + (L.n[0] + 2) * ((j + 1) + (ny + 2) * (k + 1));
         const T uc = u[c];
