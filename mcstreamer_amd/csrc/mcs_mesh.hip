@@ -4,6 +4,8 @@
 
 #include <vector>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "mcs_device.h"
 #include "mcs_fp32.h"
 
@@ -247,7 +249,6 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_segsort_wave(const int32_t *cs, i
     }
 }
 
-#define HUGE_TILE 2048
 
 // One block per segment of 65..4096 particles (longer ones take the grid-wide path).
 __global__ __launch_bounds__(MCS_BLOCK) void k_segsort_big(const int32_t *cs, int32_t *perm,
@@ -274,35 +275,6 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_segsort_big(const int32_t *cs, in
         }
         __syncthreads();
     }
-}
-
-// Segments above SEG_HUGE (in practice the bucket of particles outside the mesh):
-// rank of each element = number of smaller ids in the segment, counted by a 2D
-// grid of (element block, id tile) with the tile in LDS, accumulated atomically.
-__global__ __launch_bounds__(MCS_BLOCK) void k_rank_huge(const uint64_t *sid, int64_t a, int64_t len,
-                                                         int32_t *rank)
-{
-    __shared__ uint64_t tile[HUGE_TILE];
-    const int64_t t0 = (int64_t)blockIdx.y * HUGE_TILE;
-    const int64_t tn = len - t0 < HUGE_TILE ? len - t0 : HUGE_TILE;
-    for (int64_t u = threadIdx.x; u < tn; u += blockDim.x) tile[u] = sid[a + t0 + u];
-    __syncthreads();
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= len) return;
-    const uint64_t key = sid[a + t];
-    int cnt = 0;
-    for (int64_t u = 0; u < tn; ++u) cnt += tile[u] < key ? 1 : 0;
-    if (cnt) atomicAdd(&rank[t], cnt);
-}
-
-__global__ __launch_bounds__(MCS_BLOCK) void k_rank_scatter(const int32_t *perm, const uint64_t *sid, int64_t a,
-                                                            int64_t len, const int32_t *rank, int32_t *tmp_perm,
-                                                            uint64_t *tmp_sid)
-{
-    const int64_t t = gidx();
-    if (t >= len) return;
-    tmp_perm[a + rank[t]] = perm[a + t];
-    tmp_sid[a + rank[t]] = sid[a + t];
 }
 
 __global__ __launch_bounds__(MCS_BLOCK) void k_copy_back_seg(int32_t *perm, uint64_t *sid, int64_t a, int64_t len,
@@ -358,8 +330,9 @@ static int32_t sort_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, const Buf &dst
                            (const int32_t *)m->cell_start, p->i32c, p->u64a,
                            (const int32_t *)m->cell_cnt, nseg, (const int64_t *)(ctr + M_BIG), p->i32b,
                            (uint64_t *)dst.id);
-        // segments longer than SEG_HUGE: their (start, length) pairs come back in one
-        // read; each is ranked on the grid
+        // segments longer than SEG_HUGE (in practice the bucket of particles outside
+        // the mesh): their (start, length) pairs come back in one read; each is
+        // ordered by id with a radix sort of (id, position) pairs (ids are unique)
         int64_t hv[1 + 2 * HUGE_MAX];
         MCS_HIP(ctx, hipMemcpyAsync(hv, ctr + M_HUGE, sizeof(hv), hipMemcpyDeviceToHost, ctx->stream));
         MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -368,13 +341,18 @@ static int32_t sort_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, const Buf &dst
                             SEG_HUGE);
         for (int64_t q = 0; q < hv[0]; ++q) {
             const int64_t a = hv[1 + 2 * q], len = hv[2 + 2 * q];
-            hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)cdiv(len, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
-                               ctx->stream, p->i32a, len);
-            hipLaunchKernelGGL(k_rank_huge, dim3((unsigned)cdiv(len, MCS_BLOCK), (unsigned)cdiv(len, HUGE_TILE)),
-                               dim3(MCS_BLOCK), 0, ctx->stream, (const uint64_t *)p->u64a, a, len, p->i32a);
-            hipLaunchKernelGGL(k_rank_scatter, dim3((unsigned)cdiv(len, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
-                               ctx->stream, (const int32_t *)p->i32c, (const uint64_t *)p->u64a, a, len,
-                               (const int32_t *)p->i32a, p->i32b, (uint64_t *)dst.id);
+            const uint64_t *kin = (const uint64_t *)p->u64a + a;
+            uint64_t *kout = (uint64_t *)dst.id + a;
+            const int32_t *vin = (const int32_t *)p->i32c + a;
+            int32_t *vout = p->i32b + a;
+            size_t tb = 0;
+            MCS_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tb, kin, kout, vin, vout, (size_t)len, 0u, 64u,
+                                                   ctx->stream));
+            void *tmp = nullptr;
+            rc = mcs_scratch(ctx, tb, &tmp);
+            if (rc) return rc;
+            MCS_HIP(ctx, rocprim::radix_sort_pairs(tmp, tb, kin, kout, vin, vout, (size_t)len, 0u, 64u,
+                                                   ctx->stream));
             hipLaunchKernelGGL(k_copy_back_seg, dim3((unsigned)cdiv(len, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
                                ctx->stream, p->i32c, p->u64a, a, len, (const int32_t *)p->i32b,
                                (const uint64_t *)dst.id);
