@@ -29,11 +29,11 @@
 #define MG_MAXL 13
 #define MG_TAIL_LDS (160 * 1024)
 #define MG_TAIL_THREADS 512
-// Measured on the 128x128x256 streamer solve (7 levels): the single-workgroup
-// tail pays off from levels of up to ~2^13 cells (it also needs the levels in
-// LDS), the tiled smoother on levels of up to ~2^18 cells; the finer levels run
-// one launch per pass.
-#define MG_TAIL_MAXCELLS 8192
+// Measured on the 128x128x256 and 256x256x512 streamer solves: the
+// single-workgroup tail pays off from levels of up to ~2^10 cells (it also needs
+// the levels in LDS), the tiled smoother on levels of up to ~2^18 cells; the
+// finer levels run one launch per pass.
+#define MG_TAIL_MAXCELLS 1024
 #define MG_SM_MAXCELLS ((int64_t)1 << 18)
 
 struct Lvl {
