@@ -304,6 +304,33 @@ __device__ __forceinline__ void interp_item(T *__restrict__ r, const T *__restri
             }
 }
 
+// bc_item for every ghost cell whose nearest interior cell is (i, j, k), which
+// holds v: up to 7 ghost cells of a corner cell, none of an inner cell.
+template <typename T>
+__device__ __forceinline__ void ghost_write(T *a, const Lvl &L, const BcCoef &bc, int64_t i, int64_t j, int64_t k,
+                                            T v)
+{
+    const int n0 = L.n[0], n1 = L.n[1], n2 = L.n[2];
+    if (i > 0 && i + 1 < n0 && j > 0 && j + 1 < n1 && k > 0 && k + 1 < n2) return;
+    // per axis: the cell itself, its low ghost, its high ghost (with their flips)
+    const int xs[3] = {(int)i, -1, n0}, ys[3] = {(int)j, -1, n1}, zs[3] = {(int)k, -1, n2};
+    const bool xo[3] = {true, i == 0, i + 1 == n0}, yo[3] = {true, j == 0, j + 1 == n1},
+               zo[3] = {true, k == 0, k + 1 == n2};
+    const uint32_t neg = bc.neg;
+    const uint32_t fx[3] = {0u, neg & 1u, (neg >> 3) & 1u}, fy[3] = {0u, (neg >> 1) & 1u, (neg >> 4) & 1u},
+                   fz[3] = {0u, (neg >> 2) & 1u, (neg >> 5) & 1u};
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+        for (int ay = 0; ay < 3; ++ay)
+#pragma unroll
+            for (int az = 0; az < 3; ++az) {
+                if ((ax | ay | az) == 0 || !xo[ax] || !yo[ay] || !zo[az]) continue;
+                const uint32_t flip = fx[ax] ^ fy[ay] ^ fz[az];
+                a[g_index(L, (int64_t)xs[ax] + 1, (int64_t)ys[ay] + 1, (int64_t)zs[az] + 1)] = flip ? -v : v;
+            }
+}
+
 // ---- grid kernels ------------------------------------------------------------
 
 template <typename T>
@@ -313,11 +340,29 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_mg_bc(T *a, Lvl L, BcCoef bc)
     if (t < bc_count(L)) bc_item(a, L, bc, t);
 }
 
+// ghosts: after the update, apply! of the level as it stands then (the pass's
+// cell with its new value and its pair cell along x, of the other colour, which
+// this pass does not change), folded into the pass instead of a launch.
 template <typename T>
-__global__ __launch_bounds__(MCS_BLOCK) void k_mg_rbgs(T *u, const T *b, Lvl L, BcCoef bc, int par)
+__global__ __launch_bounds__(MCS_BLOCK) void k_mg_rbgs(T *u, const T *b, Lvl L, BcCoef bc, int par, int ghosts)
 {
     int64_t t = xcd_gidx();
-    if (t < (int64_t)(L.n[0] / 2) * L.n[1] * L.n[2]) gs_item(u, b, L, bc, par, t);
+    if (t >= (int64_t)(L.n[0] / 2) * L.n[1] * L.n[2]) return;
+    if (!ghosts) {
+        gs_item(u, b, L, bc, par, t);
+        return;
+    }
+    int64_t ti, j, k;
+    split3(t, (uint32_t)(L.n[0] / 2), (uint32_t)L.n[1], ti, j, k);
+    const int64_t i = 2 * ti + ((j + k + par) & 1);
+    const int64_t c = g_index(L, i + 1, j + 1, k + 1);
+    const T uc = u[c];
+    const T l = lap_fold(u, L, bc, i, j, k, c, uc);
+    const T un = uc + div6(T(1) * (l + b[c]));
+    u[c] = un;
+    ghost_write(u, L, bc, i, j, k, un);
+    const int64_t ip = i ^ 1;
+    ghost_write(u, L, bc, ip, j, k, u[c + (ip - i)]);
 }
 
 // Also zeroes the interior of the coarse solution xh that the next mgv! starts
@@ -474,25 +519,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_mg_axpy_bc(double *x, const Z *z,
     const int64_t c = g_index(L, i + 1, j + 1, k + 1);
     const double v = x[c] + (double)z[split ? s_index(L, i, j, k) : c];
     x[c] = v;
-    const int n0 = L.n[0], n1 = L.n[1], n2 = L.n[2];
-    if (i > 0 && i + 1 < n0 && j > 0 && j + 1 < n1 && k > 0 && k + 1 < n2) return;
-    // per axis: the cell itself, its low ghost, its high ghost (with their flips)
-    const int xs[3] = {(int)i, -1, n0}, ys[3] = {(int)j, -1, n1}, zs[3] = {(int)k, -1, n2};
-    const bool xo[3] = {true, i == 0, i + 1 == n0}, yo[3] = {true, j == 0, j + 1 == n1},
-               zo[3] = {true, k == 0, k + 1 == n2};
-    const uint32_t neg = bc.neg;
-    const uint32_t fx[3] = {0u, neg & 1u, (neg >> 3) & 1u}, fy[3] = {0u, (neg >> 1) & 1u, (neg >> 4) & 1u},
-                   fz[3] = {0u, (neg >> 2) & 1u, (neg >> 5) & 1u};
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int bq = 0; bq < 3; ++bq)
-#pragma unroll
-            for (int cq = 0; cq < 3; ++cq) {
-                if ((a | bq | cq) == 0 || !xo[a] || !yo[bq] || !zo[cq]) continue;
-                const uint32_t flip = fx[a] ^ fy[bq] ^ fz[cq];
-                x[g_index(L, (int64_t)xs[a] + 1, (int64_t)ys[bq] + 1, (int64_t)zs[cq] + 1)] = flip ? -v : v;
-            }
+    ghost_write(x, L, bc, i, j, k, v);
 }
 
 // setinhom! (boundaries.jl:202-214): b += (2/s) val on the first valid layer.
@@ -1616,8 +1643,10 @@ static void mg_bc(mcs_mg *mg, int l, T *a)
     hipLaunchKernelGGL(k_mg_bc<T>, grid_for(cnt), dim3(MCS_BLOCK), 0, mg->ctx->stream, a, L, coefs(mg));
 }
 
+// One sweep (colour 0, then 1); ghosts: the colour-1 pass also writes the ghost
+// shell of the state the sweep ends with (ghost layout only).
 template <typename T>
-static void mg_gs(mcs_mg *mg, int l, T *u, const T *b, bool split = false)
+static void mg_gs(mcs_mg *mg, int l, T *u, const T *b, bool split = false, bool ghosts = false)
 {
     Lvl L = lvl(mg, l);
     int64_t cnt = (int64_t)(L.n[0] / 2) * L.n[1] * L.n[2];
@@ -1628,7 +1657,7 @@ static void mg_gs(mcs_mg *mg, int l, T *u, const T *b, bool split = false)
                                coefs(mg), par);
         else
             hipLaunchKernelGGL(k_mg_rbgs<T>, grid_for(cnt), dim3(MCS_BLOCK), 0, mg->ctx->stream, u, b, L,
-                               coefs(mg), par);
+                               coefs(mg), par, ghosts && par == 1 ? 1 : 0);
     }
 }
 
@@ -1842,11 +1871,16 @@ static void mgv(mcs_mg *mg, T *x, const T *b, int level, bool zero, const T *zc 
     T *xh = lsol<T>(mg, level + 1);
     mgv<T>(mg, xh, lres<T>(mg, level + 1), level + 1, false);
     mg_interp<T>(mg, level, x, xh, 1, sp);
-    for (int it = 0; it < mg->cfg.smooth2; ++it) {
-        if (it == mg->cfg.smooth2 - 1 && ghosts) mg_bc<T>(mg, level, x);
-        mg_gs<T>(mg, level, x, b, sp);
+    // the ghost shell the caller reads: before the last sweep (the second-to-last
+    // sweep's colour-1 pass writes it), or after it (fresh: the last pass writes
+    // it); a single sweep takes the launch before it
+    const int s2 = mg->cfg.smooth2;
+    if (ghosts && !fresh && s2 == 1) mg_bc<T>(mg, level, x);
+    for (int it = 0; it < s2; ++it) {
+        const bool gh = ghosts && (fresh ? it == s2 - 1 : it == s2 - 2);
+        mg_gs<T>(mg, level, x, b, sp, gh);
     }
-    if (fresh && ghosts) mg_bc<T>(mg, level, x);
+    if (fresh && ghosts && s2 == 0) mg_bc<T>(mg, level, x);
 }
 
 // The correction part of fmg! (Multigrid.jl:396-426) on the hierarchy of one
