@@ -171,7 +171,11 @@ struct mcs_mesh {
     int64_t *pre64 = nullptr;       // ncells + 2 (their prefix)
     bool sorted_valid = false;
     float *face32[3] = {nullptr, nullptr, nullptr};  // FP32 face fields (precision mode)
-    int32_t *dep_list = nullptr;    // ncells + 1: count, then the cells the CIC gather gives a wavefront
+    int32_t *dep_list = nullptr;    // ncells + 2: count, the cells the CIC gather gives a wavefront, a counter
+    int32_t *dep_host = nullptr;    // host memory the device writes the count of such cells to
+    int32_t *dep_host_dev = nullptr;
+    int32_t dep_dense = 0;          // records a cell visits above which it takes a wavefront
+    int64_t dep_lane_listed = 0;    // above this many such cells (last call), one lane per cell instead
 };
 
 // FP32 copies of the face fields, allocated and filled on first use; from then on

@@ -69,6 +69,7 @@ extern "C" int32_t mcs_mesh_destroy(mcs_mesh *m)
     hipFree(m->cnt64);
     hipFree(m->pre64);
     if (m->dep_list) hipFree(m->dep_list);
+    if (m->dep_host) hipHostFree(m->dep_host);
     delete m;
     return MCS_OK;
 }
@@ -408,12 +409,20 @@ struct DepArgs {
     uint64_t *pk;
     double *pc;       // 8 per particle, corner (dx, dy, dz) at 4 dx + 2 dy + dz
     int64_t n;
-    int32_t *dense;   // [0] count, then cells with more than DEP_DENSE records to visit
+    int32_t *dense;   // [0] count, then cells with more than dense_min records to visit
+    int32_t dense_min;
+    // cells with more than cand_min records (the listing threshold), counted for
+    // the next call's choice of gather: dense[cand_at] counts them when they are
+    // not listed, and k_deposit_dense writes the count to *count_out (host memory)
+    int32_t cand_min;
+    int64_t cand_at;
+    int32_t *count_out;
 };
 
 #define DEP_MASK 0x1FFFFFULL
 #define DEP_PF 8       // records loaded ahead of the sequential accumulation
-#define DEP_DENSE 64   // cells visiting more particle records take a wavefront each
+#define DEP_DENSE 64   // default: cells visiting more particle records take a wavefront each
+#define DEP_LANE_LISTED (1 << 20)   // default: above this many such cells, one lane per cell
 
 // Per particle: g = x/h - 1/2, i0 = floor(g), fr = g - i0, base = (w val)/dV and
 // the share base * ((wx wy) wz) of each of the 8 cells i0 + (dx, dy, dz), with
@@ -423,7 +432,10 @@ template <typename Buf>
 __global__ __launch_bounds__(MCS_BLOCK) void k_deposit_prep(DepArgs<Buf> a)
 {
     int64_t q = gidx();
-    if (q == 0) a.dense[0] = 0;
+    if (q == 0) {
+        a.dense[0] = 0;
+        a.dense[a.cand_at] = 0;
+    }
     if (q >= a.n) return;
     const double h = a.g.h;
     const double dV = (h * h) * h;
@@ -504,7 +516,9 @@ __device__ __forceinline__ void dep_rows(const int32_t *cs, int64_t i, int64_t j
     }
 }
 
-// Record index of position v (< pre[9]) of the sequence.
+// Record index of position v (< pre[9]) of the sequence.  (Each candidate is
+// computed before the select: a select among the row starts themselves becomes
+// an indexed load from private memory.)
 __device__ __forceinline__ int32_t dep_at(const DepRows &R, int32_t v)
 {
     int32_t q = R.start[0] + v;
@@ -541,22 +555,34 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_deposit(DepArgs<Buf> a)
     DepRows R;
     dep_rows(a.cs, i, j, k, nx, ny, nz, R);
     const int32_t tot = R.pre[9];
-    if (tot > DEP_DENSE) {
-        const int slot = atomicAdd(a.dense, 1);
-        a.dense[1 + slot] = (int32_t)c;
+    if (a.dense_min > a.cand_min) {
+        const uint64_t cm = __ballot(tot > a.cand_min);
+        if (cm && (threadIdx.x & (MCS_WAVE - 1)) == (uint32_t)(__ffsll((unsigned long long)cm) - 1))
+            atomicAdd(a.dense + a.cand_at, __popcll(cm));
+    }
+    // listed cells: one atomic per wavefront, slots in lane order
+    const bool listed = tot > a.dense_min;
+    const uint64_t lm = __ballot(listed);
+    if (listed) {
+        const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(lm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)lm, 0));
+        int base = 0;
+        if (below == 0) base = atomicAdd(a.dense, __popcll(lm));
+        base = __shfl(base, __ffsll((unsigned long long)lm) - 1);
+        a.dense[1 + base + below] = (int32_t)c;
         return;
     }
     if (tot == 0) return;
     double acc = a.arr[idx];
     const int ii = (int)i, jj = (int)j, kk = (int)k;
-    for (int32_t v = 0; v < tot; v += DEP_PF) {
-        int32_t q8[DEP_PF];
-        uint64_t kk8[DEP_PF];
+    int32_t q8[DEP_PF];
+    uint64_t kk8[DEP_PF];
 #pragma unroll
-        for (int u = 0; u < DEP_PF; ++u) {
-            q8[u] = v + u < tot ? dep_at(R, v + u) : 0;
-            kk8[u] = dep_key(a, q8[u], v + u < tot ? q8[u] + 1 : 0);
-        }
+    for (int u = 0; u < DEP_PF; ++u) {
+        q8[u] = u < tot ? dep_at(R, u) : 0;
+        kk8[u] = dep_key(a, q8[u], u < tot ? q8[u] + 1 : 0);
+    }
+    for (int32_t v = 0; v < tot; v += DEP_PF) {
+        // shares of these records while the keys of the next ones load
         double t8[DEP_PF];
         bool on8[DEP_PF];
 #pragma unroll
@@ -566,35 +592,44 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_deposit(DepArgs<Buf> a)
             t8[u] = dep_share(a, on8[u], q8[u], corner);
         }
 #pragma unroll
+        for (int u = 0; u < DEP_PF; ++u) {
+            const int32_t vn = v + DEP_PF + u;
+            q8[u] = vn < tot ? dep_at(R, vn) : 0;
+            kk8[u] = dep_key(a, q8[u], vn < tot ? q8[u] + 1 : 0);
+        }
+#pragma unroll
         for (int u = 0; u < DEP_PF; ++u)
             if (on8[u]) acc += t8[u];
     }
     a.arr[idx] = acc;
 }
 
-__device__ __forceinline__ double readlane_f64(double v, int l)
-{
-    const uint64_t w = (uint64_t)__double_as_longlong(v);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)w, l);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(w >> 32), l);
-    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
+#define DEP_SLOTS (MCS_WAVE + 8)   // per wavefront: 64 compacted shares and 8 pad slots
+#define DEP_UNROLL 1               // sequence positions per round: DEP_UNROLL x 64
 
-// The CIC gather of the listed cells, one wavefront per cell: the lanes take 64
-// consecutive positions of the cell's record sequence at once (the keys of the
-// next 64 and the shares of the current ones in flight together), and the shares
-// are added to the cell's sum lane by lane in sequence order (a ballot of the
-// lanes with a share), which is the order of k_deposit's sequential loop, so the
-// sum is bitwise the same.
+// The CIC gather of the listed cells, one wavefront per cell.  The lanes take
+// 4 x 64 consecutive positions of the cell's record sequence at once (the keys of
+// the next 256 are in flight while the shares of the current ones load); per 64
+// positions, the lanes whose record feeds the cell write their shares to LDS in
+// sequence order (compacted by a ballot prefix), and lane 0 adds them to the
+// cell's sum one by one, in that order: the additions of k_deposit's sequential
+// loop, so the sum is bitwise the same.  Slots past the last share hold -0.0,
+// which leaves any sum unchanged under round-to-nearest, so lane 0 adds whole
+// groups of eight.
 template <typename Buf>
 __global__ __launch_bounds__(MCS_BLOCK) void k_deposit_dense(DepArgs<Buf> a)
 {
+    __shared__ __align__(16) double stage[(MCS_BLOCK / MCS_WAVE) * DEP_SLOTS];
     const int lane = threadIdx.x & (MCS_WAVE - 1);
     const int wpb = blockDim.x / MCS_WAVE;
     const int nw = gridDim.x * wpb;
+    double *st = stage + (threadIdx.x / MCS_WAVE) * DEP_SLOTS;
     const int ndense = a.dense[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        *(volatile int32_t *)a.count_out = a.dense_min > a.cand_min ? a.dense[a.cand_at] : ndense;
     const int64_t nx = a.g.n[0], ny = a.g.n[1], nz = a.g.n[2];
-    for (int e = blockIdx.x * wpb + threadIdx.x / MCS_WAVE; e < ndense; e += nw) {
+    const int e0 = __builtin_amdgcn_readfirstlane(blockIdx.x * wpb + threadIdx.x / MCS_WAVE);
+    for (int e = e0; e < ndense; e += nw) {
         const int64_t c = a.dense[1 + e];
         const uint32_t cu = (uint32_t)c, qy = cu / (uint32_t)nx;
         const int64_t i = cu - qy * (uint32_t)nx, j = qy % (uint32_t)ny, k = qy / (uint32_t)ny;
@@ -604,23 +639,59 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_deposit_dense(DepArgs<Buf> a)
         const int32_t tot = R.pre[9];
         const int ii = (int)i, jj = (int)j, kk = (int)k;
         double acc = a.arr[idx];
-        int32_t qc = lane < tot ? dep_at(R, lane) : 0;
-        uint64_t kc = dep_key(a, qc, lane < tot ? qc + 1 : 0);
-        for (int32_t v = 0; v < tot; v += MCS_WAVE) {
-            const int32_t vn = v + MCS_WAVE + lane;
-            const int32_t qn = vn < tot ? dep_at(R, vn) : 0;
-            const uint64_t kn = dep_key(a, qn, vn < tot ? qn + 1 : 0);
-            int corner;
-            const bool on = dep_corner(kc, ii, jj, kk, corner);
-            const double t = dep_share(a, on, qc, corner);
-            uint64_t m = __ballot(on);
-            while (m) {
-                const int l = __builtin_ctzll(m);
-                acc += readlane_f64(t, l);
-                m &= m - 1;
+        int32_t qc[DEP_UNROLL];
+        uint64_t kc[DEP_UNROLL];
+#pragma unroll
+        for (int u = 0; u < DEP_UNROLL; ++u) {
+            const int32_t vu = u * MCS_WAVE + lane;
+            qc[u] = vu < tot ? dep_at(R, vu) : 0;
+            kc[u] = dep_key(a, qc[u], vu < tot ? qc[u] + 1 : 0);
+        }
+        for (int32_t v = 0; v < tot; v += DEP_UNROLL * MCS_WAVE) {
+            // shares of these positions, keys of the next ones
+            double t[DEP_UNROLL];
+            bool on[DEP_UNROLL];
+#pragma unroll
+            for (int u = 0; u < DEP_UNROLL; ++u) {
+                int corner;
+                on[u] = dep_corner(kc[u], ii, jj, kk, corner);
+                t[u] = dep_share(a, on[u], qc[u], corner);
             }
-            qc = qn;
-            kc = kn;
+#pragma unroll
+            for (int u = 0; u < DEP_UNROLL; ++u) {
+                const int32_t vn = v + (DEP_UNROLL + u) * MCS_WAVE + lane;
+                qc[u] = vn < tot ? dep_at(R, vn) : 0;
+                kc[u] = dep_key(a, qc[u], vn < tot ? qc[u] + 1 : 0);
+            }
+#pragma unroll
+            for (int u = 0; u < DEP_UNROLL; ++u) {
+                const uint64_t m = __ballot(on[u]);
+                if (m == 0) continue;
+                const int cnt = __popcll(m);
+                const int below =
+                    __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                // on lanes: their share at their rank (below cnt); lanes cnt..cnt+7: pad
+                if (on[u]) st[below] = t[u];
+                if (lane >= cnt && lane < cnt + 8) st[lane] = -0.0;
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                if (lane == 0) {
+                    for (int g = 0; g < cnt; g += 8) {
+                        const double2 p0 = *(const double2 *)(st + g), p1 = *(const double2 *)(st + g + 2);
+                        const double2 p2 = *(const double2 *)(st + g + 4), p3 = *(const double2 *)(st + g + 6);
+                        acc += p0.x;
+                        acc += p0.y;
+                        acc += p1.x;
+                        acc += p1.y;
+                        acc += p2.x;
+                        acc += p2.y;
+                        acc += p3.x;
+                        acc += p3.y;
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            }
         }
         if (lane == 0) a.arr[idx] = acc;
     }
@@ -663,9 +734,28 @@ static int32_t deposit_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, int32_t whi
     a.pk = p->dep_k;
     a.pc = (double *)p->dep_w;
     a.n = p->n;
-    if (scheme == MCS_DEPOSIT_CIC && !m->dep_list)
-        MCS_HIP(ctx, hipMalloc(&m->dep_list, sizeof(int32_t) * (m->ncells + 1)));
+    if (scheme == MCS_DEPOSIT_CIC && !m->dep_list) {
+        MCS_HIP(ctx, hipMalloc(&m->dep_list, sizeof(int32_t) * (m->ncells + 2)));
+        MCS_HIP(ctx, hipHostMalloc((void **)&m->dep_host, 64, hipHostMallocMapped | hipHostMallocCoherent));
+        *(volatile int32_t *)m->dep_host = 0;
+        MCS_HIP(ctx, hipHostGetDevicePointer((void **)&m->dep_host_dev, m->dep_host, 0));
+        const char *e = getenv("MCS_DEP_DENSE");
+        m->dep_dense = e ? atoi(e) : DEP_DENSE;
+        if (m->dep_dense < 1) m->dep_dense = DEP_DENSE;
+        const char *f = getenv("MCS_DEP_LANE_LISTED");
+        m->dep_lane_listed = f ? atoll(f) : DEP_LANE_LISTED;
+    }
     a.dense = m->dep_list;
+    a.cand_min = m->dep_dense;
+    a.cand_at = m->ncells + 1;
+    a.count_out = m->dep_host_dev;
+    // The previous call's count of cells above the listing threshold (written by
+    // the device into host memory; any value gives the same sums): many such
+    // cells fill the device with one lane per cell, few do not and take a
+    // wavefront each.
+    a.dense_min = scheme == MCS_DEPOSIT_CIC && *(volatile int32_t *)m->dep_host > m->dep_lane_listed
+                      ? INT32_MAX
+                      : m->dep_dense;
     {
         ProfScope ps(ctx, "deposit");
         dim3 g((unsigned)cdiv(m->ncells, MCS_BLOCK)), bl(MCS_BLOCK);

@@ -968,6 +968,17 @@ struct Tail {
 
 #define BLOOP(t, N) for (int64_t t = threadIdx.x; t < (N); t += blockDim.x)
 
+// Level lev's extents, read element by element (a copy of the indexed struct
+// goes through private memory).
+__device__ __forceinline__ Lvl tail_lvl(const Tail &T, int lev)
+{
+    Lvl L;
+    L.n[0] = T.n[lev].n[0];
+    L.n[1] = T.n[lev].n[1];
+    L.n[2] = T.n[lev].n[2];
+    return L;
+}
+
 // The 50 (apply!, gauss_seidel!) rounds of the coarsest level (Multigrid.jl:340-346)
 // with one cell per lane of the first wave, neighbours exchanged by shuffles; the
 // arithmetic is that of gs_item.  The ghosts are materialised from the values at
@@ -1274,7 +1285,7 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
     if (zc) {
         // the start level's interior is interpolate!(x, zc) (overwrite); its
         // ghosts are materialised before they are read
-        tail_interp(lds + T.off_x[start], zc, T.n[start], false);
+        tail_interp(lds + T.off_x[start], zc, tail_lvl(T, start), false);
     } else if (zero) {
         BLOOP(t, T.len[start]) lds[T.off_x[start] + t] = E(0);
     } else {
@@ -1285,7 +1296,7 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
     for (int lev = start; lev < T.L; ++lev) {
         E *x = lds + T.off_x[lev];
         const E *b = lds + T.off_b[lev];
-        const Lvl Lv = T.n[lev];
+        const Lvl Lv = tail_lvl(T, lev);
         for (int it = 0; it < T.smooth1; ++it) {
             for (int par = 0; par < 2; ++par) {
                 tail_gs(x, b, Lv, bc, par);
@@ -1307,7 +1318,7 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
         const int lev = T.L;
         E *x = lds + T.off_x[lev];
         const E *b = lds + T.off_b[lev];
-        const Lvl Lv = T.n[lev];
+        const Lvl Lv = tail_lvl(T, lev);
         const int64_t N = (int64_t)Lv.n[0] * Lv.n[1] * Lv.n[2];
         if (N <= MCS_WAVE) {
             if (threadIdx.x < MCS_WAVE && !coarsest_try_dpp(x, b, Lv, bc)) coarsest_in_registers(x, b, Lv, bc);
@@ -1329,7 +1340,7 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
     for (int lev = T.L - 1; lev >= start; --lev) {
         E *x = lds + T.off_x[lev];
         const E *b = lds + T.off_b[lev];
-        const Lvl Lv = T.n[lev];
+        const Lvl Lv = tail_lvl(T, lev);
         tail_interp(x, lds + T.off_x[lev + 1], Lv, true);
         __syncthreads();
         for (int it = 0; it < T.smooth2; ++it) {
@@ -1346,7 +1357,8 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
     if (fresh) {
         // ghosts of the final state (the apply! the caller would issue next)
         __syncthreads();
-        BLOOP(t, bc_count(T.n[start])) bc_item(lds + T.off_x[start], T.n[start], bc, t);
+        const Lvl Ls = tail_lvl(T, start);
+        BLOOP(t, bc_count(Ls)) bc_item(lds + T.off_x[start], Ls, bc, t);
     }
     __syncthreads();
     tail_copy(x0, lds + T.off_x[start], T.len[start]);

@@ -73,8 +73,17 @@ def test_sort_handles_long_segments():
     _same_pop(pop, P)
 
 
-@pytest.mark.parametrize("scheme,blob", [("ngp", False), ("cic", False), ("cic", True)])
-def test_deposit_parity(scheme, blob):
+@pytest.mark.parametrize("scheme,blob,dense,lane_listed", [
+    ("ngp", False, None, None), ("cic", False, None, None), ("cic", True, None, None),
+    # a wavefront per listed cell over most cells; one lane per cell everywhere
+    ("cic", True, 8, None), ("cic", False, 4, None), ("cic", True, 8, -1), ("cic", True, 64, -1),
+    # the first call lists the dense cells, the second (their count above 5) does not
+    ("cic", True, 8, 5)])
+def test_deposit_parity(scheme, blob, dense, lane_listed, monkeypatch):
+    if dense is not None:
+        monkeypatch.setenv("MCS_DEP_DENSE", str(dense))
+    if lane_listed is not None:
+        monkeypatch.setenv("MCS_DEP_LANE_LISTED", str(lane_listed))
     n, h = (12, 10, 14), 2e-6
     pop, fields, P, M, _ = _pair(n, h, 20000, 3, p_outside=0.02)
     if blob:
@@ -90,6 +99,11 @@ def test_deposit_parity(scheme, blob):
     streamer.density(fields, "qpart", pop, -1.0, scheme)
     R.deposit(P, M, 1, 0 if scheme == "ngp" else 1, -1.0)
     np.testing.assert_array_equal(fields.get("qpart"), M.qpart)
+    if lane_listed is not None and lane_listed > 0:
+        # again, with the other gather (the deposit adds to the array)
+        streamer.density(fields, "qpart", pop, -1.0, scheme)
+        R.deposit(P, M, 1, 1, -1.0)
+        np.testing.assert_array_equal(fields.get("qpart"), M.qpart)
     # charge conservation (exact up to rounding) for the particles inside
     inside = np.all((P.view()["x"] >= 0) & (P.view()["x"] < np.array(n) * h), axis=1)
     if scheme == "ngp":
