@@ -44,10 +44,14 @@ extern "C" int32_t mcs_ctx_create(int32_t device, mcs_ctx **out)
     }
     if (hipHostMalloc((void **)&c->h_small, 512 * sizeof(int64_t), hipHostMallocDefault) !=
             hipSuccess ||
-        hipMalloc((void **)&c->d_small, 512 * sizeof(int64_t)) != hipSuccess) {
+        hipMalloc((void **)&c->d_small, 512 * sizeof(int64_t)) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_pub, (MCS_PUB_WORDS + 1) * sizeof(uint64_t),
+                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&c->d_pub, c->h_pub, 0) != hipSuccess) {
         delete c;
         return MCS_E_HIP;
     }
+    __atomic_store_n(c->h_pub, (uint64_t)0, __ATOMIC_RELEASE);
     *out = c;
     return MCS_OK;
 }
@@ -65,6 +69,7 @@ extern "C" int32_t mcs_ctx_destroy(mcs_ctx *ctx)
     if (ctx->scratch) hipFree(ctx->scratch);
     if (ctx->d_small) hipFree(ctx->d_small);
     if (ctx->h_small) hipHostFree(ctx->h_small);
+    if (ctx->h_pub) hipHostFree(ctx->h_pub);
     hipStreamDestroy(ctx->stream);
     delete ctx;
     return MCS_OK;
@@ -150,13 +155,53 @@ int32_t mcs_scratch(mcs_ctx *ctx, size_t bytes, void **out)
     return MCS_OK;
 }
 
+__global__ void k_publish(const uint64_t *src, int n, const uint64_t *src2, int n2, uint64_t *pub, uint64_t seq)
+{
+    for (int t = threadIdx.x; t < n + n2; t += blockDim.x) pub[1 + t] = t < n ? src[t] : src2[t - n];
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) mcs_pub_raise(pub, seq);
+}
+
+uint64_t mcs_pub_next(mcs_ctx *ctx)
+{
+    return ++ctx->pub_seq;
+}
+
+int32_t mcs_pub_wait(mcs_ctx *ctx, uint64_t seq, void *host, int n)
+{
+    // spin on the sequence number; now and then ask the stream, so that a failed
+    // or lost publication ends in an error instead of a hang
+    for (uint64_t it = 1;; ++it) {
+        if (__atomic_load_n(ctx->h_pub, __ATOMIC_ACQUIRE) == seq) break;
+        if ((it & 1023) == 0) {
+            const hipError_t q = hipStreamQuery(ctx->stream);
+            if (q == hipErrorNotReady) continue;
+            if (__atomic_load_n(ctx->h_pub, __ATOMIC_ACQUIRE) == seq) break;
+            if (q != hipSuccess)
+                return mcs_fail(ctx, MCS_E_HIP, "read-back: stream failed: %s", hipGetErrorString(q));
+            return mcs_fail(ctx, MCS_E_HIP, "read-back: sequence %llu not published by an idle stream",
+                            (unsigned long long)seq);
+        }
+    }
+    if (n > 0) memcpy(host, (const void *)(ctx->h_pub + 1), sizeof(uint64_t) * n);
+    return MCS_OK;
+}
+
+int32_t mcs_readback2(mcs_ctx *ctx, const void *dev, int n, const void *dev2, int n2, void *host)
+{
+    if (n < 0 || n2 < 0 || n + n2 > MCS_PUB_WORDS)
+        return mcs_fail(ctx, MCS_E_ARG, "read-back of %d words", n + n2);
+    const uint64_t seq = mcs_pub_next(ctx);
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(MCS_WAVE), 0, ctx->stream, (const uint64_t *)dev, n,
+                       (const uint64_t *)dev2, n2, ctx->d_pub, seq);
+    MCS_LAUNCH_CHECK(ctx);
+    return mcs_pub_wait(ctx, seq, host, n + n2);
+}
+
 int32_t mcs_readback(mcs_ctx *ctx, const int64_t *dev, int64_t *host, int n)
 {
-    MCS_HIP(ctx, hipMemcpyAsync(ctx->h_small, dev, sizeof(int64_t) * n, hipMemcpyDeviceToHost,
-                                ctx->stream));
-    MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    memcpy(host, ctx->h_small, sizeof(int64_t) * n);
-    return MCS_OK;
+    return mcs_readback2(ctx, dev, n, nullptr, 0, host);
 }
 
 // ---------------------------------------------------------------- scans

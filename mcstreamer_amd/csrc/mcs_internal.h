@@ -40,7 +40,14 @@ struct mcs_ctx {
     // growable device scratch used by scans and reductions
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    // host memory (mapped, coherent) the device publishes small results to:
+    // [0] the sequence number of the last publication, then its words
+    uint64_t *h_pub = nullptr;
+    uint64_t *d_pub = nullptr;   // its device address
+    uint64_t pub_seq = 0;
 };
+
+#define MCS_PUB_WORDS 511
 
 // set error text and return the code
 int32_t mcs_fail(mcs_ctx *ctx, int32_t code, const char *fmt, ...);
@@ -85,8 +92,24 @@ struct ProfScope {
 
 // scratch of at least `bytes`, reused across calls on the context's stream
 int32_t mcs_scratch(mcs_ctx *ctx, size_t bytes, void **out);
-// pinned read-back of n int64 values from device
+// Read-back of n (<= MCS_PUB_WORDS) int64 values from device memory: a kernel
+// copies them to ctx->h_pub and raises the sequence number, which the host
+// waits for (no stream synchronisation, no copy engine).
 int32_t mcs_readback(mcs_ctx *ctx, const int64_t *dev, int64_t *host, int n);
+// the same for n 8-byte words at dev followed by n2 at dev2
+int32_t mcs_readback2(mcs_ctx *ctx, const void *dev, int n, const void *dev2, int n2, void *host);
+// For kernels that publish by themselves: the sequence number the next
+// publication is to carry, and the wait for it (then n words are copied out).
+uint64_t mcs_pub_next(mcs_ctx *ctx);
+int32_t mcs_pub_wait(mcs_ctx *ctx, uint64_t seq, void *host, int n);
+
+// Device side of a publication by one thread, after the words are stored to
+// pub[1..]: the words are made visible to the host before the sequence number.
+__device__ __forceinline__ void mcs_pub_raise(uint64_t *pub, uint64_t seq)
+{
+    __threadfence_system();
+    __hip_atomic_store(pub, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 struct mcs_table {
     mcs_ctx *ctx;

@@ -11,6 +11,7 @@
 
 #define M_SHORT 0
 #define M_BIG 1
+#define M_TOT 2    // records in the mesh's segments (cell_start[nseg])
 #define M_HUGE 8   // count, then (start, length) pairs
 
 // ---------------------------------------------------------------- mesh object
@@ -198,6 +199,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_segsort_list(const int32_t *cs, i
                                                             int64_t *nbig, int64_t *huge)
 {
     const int64_t c = gidx();
+    if (c == 0) huge[M_TOT - M_HUGE] = cs[nseg];
     const int64_t len = c < nseg ? (int64_t)cs[c + 1] - cs[c] : 0;
     if (len > SEG_HUGE) {
         unsigned long long k = atomicAdd((unsigned long long *)huge, 1ULL);
@@ -314,6 +316,7 @@ static int32_t sort_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, const Buf &dst
     MCS_LAUNCH_CHECK(ctx);
     rc = mcs_scan_i32(ctx, m->cell_cnt, m->cell_start, nseg);
     if (rc) return rc;
+    int64_t tot = 0;   // no particles: every segment is empty
     if (n > 0) {
         ProfScope ps(ctx, "sort_scatter");
         hipLaunchKernelGGL(k_sort_scatter<Buf>, dim3((unsigned)cdiv(n, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
@@ -334,9 +337,12 @@ static int32_t sort_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, const Buf &dst
         // segments longer than SEG_HUGE (in practice the bucket of particles outside
         // the mesh): their (start, length) pairs come back in one read; each is
         // ordered by id with a radix sort of (id, position) pairs (ids are unique)
-        int64_t hv[1 + 2 * HUGE_MAX];
-        MCS_HIP(ctx, hipMemcpyAsync(hv, ctr + M_HUGE, sizeof(hv), hipMemcpyDeviceToHost, ctx->stream));
-        MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        // (with the record count in front)
+        int64_t hb[2 + 2 * HUGE_MAX];
+        rc = mcs_readback2(ctx, ctr + M_TOT, 1, ctr + M_HUGE, 1 + 2 * HUGE_MAX, hb);
+        if (rc) return rc;
+        tot = hb[0];
+        const int64_t *hv = hb + 1;
         if (hv[0] > HUGE_MAX)
             return mcs_fail(ctx, MCS_E_CAPACITY, "sort: more than %d cells hold over %d particles", HUGE_MAX,
                             SEG_HUGE);
@@ -360,14 +366,7 @@ static int32_t sort_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, const Buf &dst
         }
     }
     MCS_LAUNCH_CHECK(ctx);
-    int64_t na;
-    {
-        int32_t tot;
-        MCS_HIP(ctx, hipMemcpyAsync(&tot, m->cell_start + nseg, sizeof(int32_t),
-                                    hipMemcpyDeviceToHost, ctx->stream));
-        MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        na = tot;
-    }
+    const int64_t na = tot;
     if (na > 0) {
         ProfScope ps(ctx, "sort_gather");
         hipLaunchKernelGGL(k_sort_gather<Buf>, dim3((unsigned)cdiv(na, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
@@ -1084,14 +1083,12 @@ static int32_t resample_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, const Buf 
     MCS_LAUNCH_CHECK(ctx);
     rc = mcs_scan_i64(ctx, m->cnt64, m->pre64, nseg);
     if (rc) return rc;
-    MCS_HIP(ctx, hipMemcpyAsync(ctx->h_small, m->pre64 + nseg, sizeof(int64_t), hipMemcpyDeviceToHost,
-                                ctx->stream));
-    MCS_HIP(ctx, hipMemcpyAsync(ctx->h_small + 1, ctr, sizeof(int64_t) * 8, hipMemcpyDeviceToHost,
-                                ctx->stream));
-    MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    int64_t tot = ctx->h_small[0];
+    int64_t hb[9];
+    rc = mcs_readback2(ctx, m->pre64 + nseg, 1, ctr, 8, hb);
+    if (rc) return rc;
+    int64_t tot = hb[0];
     int64_t h[8];
-    memcpy(h, ctx->h_small + 1, sizeof(h));
+    memcpy(h, hb + 1, sizeof(h));
     int64_t nout = tot & 0xffffffffLL;
     int64_t ncreated = tot >> 32;
     if (nout > p->cap)

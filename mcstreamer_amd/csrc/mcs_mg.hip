@@ -432,7 +432,9 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_mg_sumsq(const double *b, Lvl L, 
     if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
-__global__ void k_sum_partials(const double *partial, int64_t nb, double *out)
+// The sum of the partials to *out and, when pub is given, published to the host
+// with sequence number seq.
+__global__ void k_sum_partials(const double *partial, int64_t nb, double *out, uint64_t *pub, uint64_t seq)
 {
     __shared__ double sh[MCS_BLOCK];
     double acc = 0.0;
@@ -453,7 +455,13 @@ __global__ void k_sum_partials(const double *partial, int64_t nb, double *out)
         if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *out = sh[0];
+    if (threadIdx.x == 0) {
+        *out = sh[0];
+        if (pub) {
+            pub[1] = (uint64_t)__double_as_longlong(sh[0]);
+            mcs_pub_raise(pub, seq);
+        }
+    }
 }
 
 template <typename T>
@@ -1963,14 +1971,11 @@ static int fmg(mcs_mg *mg, double *x, const double *b, double *eps_out, bool gho
                            F.res, (const double *)x, b, mg->cfg.s, lvl(mg, 0), coefs(mg), mg->partial,
                            mg->split0 ? 1 : 0);
     }
+    const uint64_t seq = mcs_pub_next(ctx);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(MCS_BLOCK), 0, ctx->stream,
-                       (const double *)mg->partial, mg->npartial, mg->partial + mg->npartial);
+                       (const double *)mg->partial, mg->npartial, mg->partial + mg->npartial, ctx->d_pub, seq);
     double ss = 0.0;
-    if (hipMemcpyAsync(ctx->h_small, mg->partial + mg->npartial, sizeof(double),
-                       hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-        hipStreamSynchronize(ctx->stream) != hipSuccess)
-        return -1;
-    memcpy(&ss, ctx->h_small, sizeof(double));
+    if (hipGetLastError() != hipSuccess || mcs_pub_wait(ctx, seq, &ss, 1) != MCS_OK) return -1;
     double eps = sqrt(ss) / sqrt((double)F.n[0] * (double)F.n[1] * (double)F.n[2]);
     *eps_out = eps;
     if (mg->cfg.s * mg->tol_abs > eps) return 0;
@@ -2015,13 +2020,13 @@ static int32_t mg_solve_dev(mcs_mg *mg, double *u, const double *b, int32_t *ite
         int64_t N = (int64_t)F.n[0] * F.n[1] * F.n[2];
         hipLaunchKernelGGL(k_mg_sumsq, dim3((unsigned)cdiv(N, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
                            ctx->stream, b, F, mg->partial);
+        const uint64_t seq = mcs_pub_next(ctx);
         hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(MCS_BLOCK), 0, ctx->stream,
-                           (const double *)mg->partial, mg->npartial, mg->partial + mg->npartial);
+                           (const double *)mg->partial, mg->npartial, mg->partial + mg->npartial, ctx->d_pub, seq);
+        MCS_LAUNCH_CHECK(ctx);
         double ss = 0.0;
-        MCS_HIP(ctx, hipMemcpyAsync(ctx->h_small, mg->partial + mg->npartial, sizeof(double),
-                                    hipMemcpyDeviceToHost, ctx->stream));
-        MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        memcpy(&ss, ctx->h_small, sizeof(double));
+        int32_t rc = mcs_pub_wait(ctx, seq, &ss, 1);
+        if (rc) return rc;
         double brms = sqrt(ss) / sqrt((double)F.n[0] * (double)F.n[1] * (double)F.n[2]);
         mg->tol_abs = -mg->cfg.tolerance * brms;
         if (!(mg->tol_abs > 0.0)) mg->tol_abs = 2.2250738585072014e-308;

@@ -10,7 +10,6 @@
 #define C_FLIGHTS 1
 #define C_DEATHS 2
 #define C_FLAGS 3
-#define C_AUX 4
 #define C_EVENTS 8
 
 struct BirthLog {
@@ -422,13 +421,12 @@ extern "C" int32_t mcs_advance(mcs_pop *p, const mcs_table *t, const mcs_field *
     }
     int32_t rc = mcs_scan_i64(ctx, btot, bpre, nblk);
     if (rc) return rc;
-    MCS_HIP(ctx, hipMemcpyAsync(ctr + C_AUX, bpre + nblk, sizeof(int64_t), hipMemcpyDeviceToDevice,
-                                ctx->stream));
-    int64_t h[C_EVENTS + MCS_MAX_PROC];
-    rc = mcs_readback(ctx, ctr, h, C_EVENTS + t->nproc);
+    // the counters, then the births (the scan's total)
+    int64_t h[C_EVENTS + MCS_MAX_PROC + 1];
+    rc = mcs_readback2(ctx, ctr, C_EVENTS + t->nproc, bpre + nblk, 1, h);
     if (rc) return rc;
     int64_t logged = h[C_LOG];
-    int64_t births = h[C_AUX];
+    int64_t births = h[C_EVENTS + t->nproc];
     int32_t flags = (int32_t)h[C_FLAGS];
     if (stats) {
         stats->births = births;
