@@ -102,33 +102,37 @@ __device__ __forceinline__ void indweight_dev(const double *energy, int32_t ne, 
 }
 
 // normdivrem + staggered (MCStreamer/src/gridfields.jl:194-209) for one axis;
-// normdivrem is @fastmath there, x * (1/h) stands for x / h.
-__device__ __forceinline__ void axis_index_dev(double xc, double inv_h, int32_t n, int64_t &m,
-                                               double &d, int64_t &m1, double &d1)
+// normdivrem is @fastmath there, x * (1/h) stands for x / h.  Indices are 32-bit:
+// every mesh array has fewer than 2^31 elements (checked at creation), and xc is
+// inside (0, n h) here.
+__device__ __forceinline__ void axis_index_dev(double xc, double inv_h, int32_t n, uint32_t &m,
+                                               double &d, uint32_t &m1, double &d1)
 {
     double r = xc * inv_h;
-    int64_t mi = (int64_t)trunc(r);
+    int32_t mi = (int32_t)trunc(r);
     if (mi > n - 1) mi = n - 1;
     double di = r - (double)mi;
-    m = mi;
+    m = (uint32_t)mi;
     d = di;
     if (di < 0.5) {
-        m1 = mi;
+        m1 = (uint32_t)mi;
         d1 = di + 0.5;
     } else {
-        m1 = mi + 1;
+        m1 = (uint32_t)mi + 1u;
         d1 = di - 0.5;
     }
 }
 
+// (offsets from one base pointer, so that the +1 neighbours are load offsets)
 __device__ __forceinline__ double trilerp_dev(const double *__restrict__ a, int64_t sx,
                                               int64_t sxy, int64_t base, double dx, double dy,
                                               double dz)
 {
-    double c00 = a[base] * (1.0 - dx) + a[base + 1] * dx;
-    double c10 = a[base + sx] * (1.0 - dx) + a[base + sx + 1] * dx;
-    double c01 = a[base + sxy] * (1.0 - dx) + a[base + sxy + 1] * dx;
-    double c11 = a[base + sxy + sx] * (1.0 - dx) + a[base + sxy + sx + 1] * dx;
+    const double *__restrict__ p = a + base;
+    double c00 = p[0] * (1.0 - dx) + p[1] * dx;
+    double c10 = p[sx] * (1.0 - dx) + p[sx + 1] * dx;
+    double c01 = p[sxy] * (1.0 - dx) + p[sxy + 1] * dx;
+    double c11 = p[sxy + sx] * (1.0 - dx) + p[sxy + sx + 1] * dx;
     double c0 = c00 * (1.0 - dy) + c10 * dy;
     double c1 = c01 * (1.0 - dy) + c11 * dy;
     return c0 * (1.0 - dz) + c1 * dz;
@@ -147,23 +151,23 @@ __device__ __forceinline__ void efield_mesh_dev(const MeshView &M, const double 
         e[2] = 0.0;
         return;
     }
-    int64_t mx, my, mz, m1x, m1y, m1z;
+    uint32_t mx, my, mz, m1x, m1y, m1z;
     double dx, dy, dz, d1x, d1y, d1z;
     axis_index_dev(x[0], M.inv_h, M.n[0], mx, dx, m1x, d1x);
     axis_index_dev(x[1], M.inv_h, M.n[1], my, dy, m1y, d1y);
     axis_index_dev(x[2], M.inv_h, M.n[2], mz, dz, m1z, d1z);
-    int64_t nx = M.n[0], ny = M.n[1];
+    const uint32_t nx = (uint32_t)M.n[0], ny = (uint32_t)M.n[1];
     {
-        int64_t sx = nx + 1, sxy = (nx + 1) * (ny + 2);
-        e[0] = trilerp_dev(M.ex, sx, sxy, mx + sx * m1y + sxy * m1z, dx, d1y, d1z);
+        const uint32_t sx = nx + 1, sxy = (nx + 1) * (ny + 2);
+        e[0] = trilerp_dev(M.ex, sx, sxy, (int64_t)(mx + sx * m1y + sxy * m1z), dx, d1y, d1z);
     }
     {
-        int64_t sx = nx + 2, sxy = (nx + 2) * (ny + 1);
-        e[1] = trilerp_dev(M.ey, sx, sxy, m1x + sx * my + sxy * m1z, d1x, dy, d1z);
+        const uint32_t sx = nx + 2, sxy = (nx + 2) * (ny + 1);
+        e[1] = trilerp_dev(M.ey, sx, sxy, (int64_t)(m1x + sx * my + sxy * m1z), d1x, dy, d1z);
     }
     {
-        int64_t sx = nx + 2, sxy = (nx + 2) * (ny + 2);
-        e[2] = trilerp_dev(M.ez, sx, sxy, m1x + sx * m1y + sxy * mz, d1x, d1y, dz);
+        const uint32_t sx = nx + 2, sxy = (nx + 2) * (ny + 2);
+        e[2] = trilerp_dev(M.ez, sx, sxy, (int64_t)(m1x + sx * m1y + sxy * mz), d1x, d1y, dz);
     }
     double e2 = (e[0] * e[0] + e[1] * e[1]) + e[2] * e[2];
     if (e2 > MCS_EFIELD_LIMIT2) flags |= MCS_FLAG_FIELD;

@@ -33,7 +33,9 @@ extern "C" int32_t mcs_mesh_create(mcs_ctx *ctx, const int32_t n[3], double h, m
     if (n[0] < 1 || n[1] < 1 || n[2] < 1 || !(h > 0.0))
         return mcs_fail(ctx, MCS_E_ARG, "mesh: sizes must be >= 1 and h > 0");
     int64_t ncells = (int64_t)n[0] * n[1] * n[2];
-    if (ncells >= ((int64_t)1 << 31) - 2) return mcs_fail(ctx, MCS_E_ARG, "mesh too large");
+    // every array (ghosts included) is indexed with 32-bit integers on the device
+    if ((int64_t)(n[0] + 2) * (n[1] + 2) * (n[2] + 2) >= ((int64_t)1 << 31) - 2)
+        return mcs_fail(ctx, MCS_E_ARG, "mesh too large");
     hipSetDevice(ctx->device);
     mcs_mesh *m = new mcs_mesh();
     m->ctx = ctx;

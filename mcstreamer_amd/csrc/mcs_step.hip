@@ -58,8 +58,10 @@ struct AdvArgs {
 // collisions.jl:236-269 and the outcomes of electron.jl:108-139.  Children go to a
 // birth log tagged (parent, ordinal); k_place writes them after the parents in
 // parent order, which is the append order of the single-threaded reference.
+// Five waves per SIMD (96 registers; a few cold values spill): the field gathers'
+// latency is hidden better than at the four that 106 registers allow.
 template <int FIELD_MESH, int TRACK>
-__global__ __launch_bounds__(MCS_BLOCK) void k_advance(AdvArgs a)
+__global__ __launch_bounds__(MCS_BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_advance(AdvArgs a)
 {
     __shared__ int s_ev[MCS_MAX_PROC];
     __shared__ int s_flights, s_deaths, s_flags;

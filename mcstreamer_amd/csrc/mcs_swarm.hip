@@ -270,9 +270,10 @@ __global__ __launch_bounds__(BS) void k_sw_advance(SwArgs64 a)
     if (flags) atomicOr(&a.S.flags[slot], flags);
 }
 
-// The same step on the FP32 split store.
+// The same step on the FP32 split store, at eight waves per SIMD (64 registers;
+// a few cold values spill): more waves hide the table and field loads better.
 template <int BS>
-__global__ __launch_bounds__(BS) void k_sw_advance32(SwArgs32 a)
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_sw_advance32(SwArgs32 a)
 {
     const int64_t tile = blockIdx.x;
     const int slot = a.tile_slot[tile];
