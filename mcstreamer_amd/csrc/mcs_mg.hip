@@ -1278,29 +1278,14 @@ __device__ __forceinline__ void tail_interp(T *x, const T *rh, const Lvl &L, boo
     }
 }
 
-// mgv! (Multigrid.jl:334-377) for levels start..L inside one workgroup: the level
-// solutions and right-hand sides live in LDS; x0 and b0 are loaded (x0 unless
-// the start is zero or a prolongation), x0 is written back.
+// mgv! (Multigrid.jl:334-377) for levels start..L inside one workgroup, on the
+// level solutions and right-hand sides in LDS (x and b of `start` loaded).
 // Phases are separated by barriers and follow the grid path's order exactly.
 template <typename E>
-__global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, E *x0, const E *b0, const E *zc,
-                                                            int zero, int fresh)
+__device__ void tail_vcycle(const Tail &T, E *lds, int start)
 {
-    extern __shared__ double lds_raw[];
-    E *lds = (E *)lds_raw;
     const BcCoef bc = T.bc;
     E *rt = lds + T.off_t;   // fine residuals of the level being restricted
-    if (zc) {
-        // the start level's interior is interpolate!(x, zc) (overwrite); its
-        // ghosts are materialised before they are read
-        tail_interp(lds + T.off_x[start], zc, tail_lvl(T, start), false);
-    } else if (zero) {
-        BLOOP(t, T.len[start]) lds[T.off_x[start] + t] = E(0);
-    } else {
-        tail_copy(lds + T.off_x[start], x0, T.len[start]);
-    }
-    tail_copy(lds + T.off_b[start], b0, T.len[start]);
-    __syncthreads();
     for (int lev = start; lev < T.L; ++lev) {
         E *x = lds + T.off_x[lev];
         const E *b = lds + T.off_b[lev];
@@ -1362,6 +1347,29 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
             }
         }
     }
+}
+
+// mgv! for levels start..L in one workgroup: x0 and b0 are loaded (x0 unless the
+// start is zero or a prolongation), x0 is written back.
+template <typename E>
+__global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, E *x0, const E *b0, const E *zc,
+                                                            int zero, int fresh)
+{
+    extern __shared__ double lds_raw[];
+    E *lds = (E *)lds_raw;
+    const BcCoef bc = T.bc;
+    if (zc) {
+        // the start level's interior is interpolate!(x, zc) (overwrite); its
+        // ghosts are materialised before they are read
+        tail_interp(lds + T.off_x[start], zc, tail_lvl(T, start), false);
+    } else if (zero) {
+        BLOOP(t, T.len[start]) lds[T.off_x[start] + t] = E(0);
+    } else {
+        tail_copy(lds + T.off_x[start], x0, T.len[start]);
+    }
+    tail_copy(lds + T.off_b[start], b0, T.len[start]);
+    __syncthreads();
+    tail_vcycle(T, lds, start);
     if (fresh) {
         // ghosts of the final state (the apply! the caller would issue next)
         __syncthreads();
@@ -1369,6 +1377,71 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
         BLOOP(t, bc_count(Ls)) bc_item(lds + T.off_x[start], Ls, bc, t);
     }
     __syncthreads();
+    tail_copy(x0, lds + T.off_x[start], T.len[start]);
+}
+
+// The coarse part of fmg!'s correction (Multigrid.jl:396-426) in one workgroup:
+// the right-hand sides of levels start..L restricted in LDS from rfine (the
+// residual of level start - 1, ghost layout; restrict_item's arithmetic), the
+// direct solve of the coarsest level (k_mg_coarse's), its apply!, then for each
+// level L-1..start the interpolation of the coarser solution, mgv! and apply!
+// (the fresh shell the next interpolation reads).  The start level's solution,
+// shell included, is written to x0.
+template <typename E>
+__global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail_fmg(Tail T, int start, E *x0, const E *rfine,
+                                                                const E *ainv)
+{
+    extern __shared__ double lds_raw[];
+    E *lds = (E *)lds_raw;
+    const BcCoef bc = T.bc;
+    {
+        const Lvl F = tail_lvl(T, start - 1), C = tail_lvl(T, start);
+        E *rh = lds + T.off_b[start];
+        BLOOP(t, T.len[start]) rh[t] = E(0);
+        __syncthreads();
+        BLOOP(t, (int64_t)C.n[0] * C.n[1] * C.n[2]) restrict_item(rh, rfine, F, t);
+        __syncthreads();
+    }
+    for (int lev = start; lev < T.L; ++lev) {
+        const Lvl F = tail_lvl(T, lev), C = tail_lvl(T, lev + 1);
+        E *rh = lds + T.off_b[lev + 1];
+        BLOOP(t, T.len[lev + 1]) rh[t] = E(0);
+        __syncthreads();
+        BLOOP(t, (int64_t)C.n[0] * C.n[1] * C.n[2]) restrict_item(rh, (const E *)(lds + T.off_b[lev]), F, t);
+        __syncthreads();
+    }
+    {
+        const Lvl C = tail_lvl(T, T.L);
+        const int64_t N = (int64_t)C.n[0] * C.n[1] * C.n[2];
+        E *z = lds + T.off_x[T.L];
+        const E *bt = lds + T.off_b[T.L];
+        BLOOP(t, T.len[T.L]) z[t] = E(0);
+        __syncthreads();
+        BLOOP(row, N) {
+            E acc = E(0);
+            for (int64_t col = 0; col < N; ++col) {
+                int64_t i, j, k;
+                split3(col, (uint32_t)C.n[0], (uint32_t)C.n[1], i, j, k);
+                acc += ainv[row * N + col] * bt[g_index(C, i + 1, j + 1, k + 1)];
+            }
+            int64_t i, j, k;
+            split3(row, (uint32_t)C.n[0], (uint32_t)C.n[1], i, j, k);
+            z[g_index(C, i + 1, j + 1, k + 1)] = acc;
+        }
+        __syncthreads();
+        BLOOP(t, bc_count(C)) bc_item(z, C, bc, t);
+        __syncthreads();
+    }
+    for (int lev = T.L - 1; lev >= start; --lev) {
+        const Lvl Lv = tail_lvl(T, lev);
+        E *x = lds + T.off_x[lev];
+        tail_interp(x, lds + T.off_x[lev + 1], Lv, false);
+        __syncthreads();
+        tail_vcycle(T, lds, lev);
+        __syncthreads();
+        BLOOP(t, bc_count(Lv)) bc_item(x, Lv, bc, t);
+        __syncthreads();
+    }
     tail_copy(x0, lds + T.off_x[start], T.len[start]);
 }
 
@@ -1454,6 +1527,10 @@ extern "C" int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg 
         (hipFuncSetAttribute((const void *)k_mg_tail<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)tail_lds) != hipSuccess ||
          hipFuncSetAttribute((const void *)k_mg_tail<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)tail_lds) != hipSuccess ||
+         hipFuncSetAttribute((const void *)k_mg_tail_fmg<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)tail_lds) != hipSuccess ||
+         hipFuncSetAttribute((const void *)k_mg_tail_fmg<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)tail_lds) != hipSuccess))
         tail_lds = 64 * 1024;
     if (!ctx || !cfg || !out) return MCS_E_ARG;
@@ -1497,6 +1574,8 @@ extern "C" int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg 
     // MCS_MG_TAIL_MAXCELLS: largest level the tail starts from
     const char *tm = getenv("MCS_MG_TAIL_MAXCELLS");
     const int64_t tail_max = tm ? atoll(tm) : MG_TAIL_MAXCELLS;
+    const char *fte = getenv("MCS_MG_FMG_TAIL");
+    const int fmg_tail = fte ? atoi(fte) : 1;
     const char *te = getenv("MCS_MG_TAIL_LEVELS");
     int tail_levels = te ? atoi(te) : MG_MAXL;
     if (tail_levels < 1) tail_levels = 1;
@@ -1504,6 +1583,7 @@ extern "C" int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg 
     mg->ctx = ctx;
     mg->cfg = c;
     mg->tail_lds = tail_lds;
+    mg->fmg_tail = fmg_tail;
     mg->tail_levels = tail_levels;
     mg->tail_max = tail_max;
     mg->sm_lds = sm_lds;
@@ -1727,9 +1807,8 @@ static bool in_tail(const mcs_mg *mg, int level)
 }
 
 template <typename T>
-static void mg_tail(mcs_mg *mg, int level, T *x, const T *b, const T *zc, bool zero, bool fresh)
+static int64_t tail_setup(mcs_mg *mg, int level, Tail &Tl)
 {
-    Tail Tl;
     memset(&Tl, 0, sizeof(Tl));
     Tl.L = mg->cfg.levels;
     Tl.smooth1 = mg->cfg.smooth1;
@@ -1748,6 +1827,31 @@ static void mg_tail(mcs_mg *mg, int level, T *x, const T *b, const T *zc, bool z
     }
     Tl.off_t = off;
     off += (int64_t)mg->lv[level].n[0] * mg->lv[level].n[1] * mg->lv[level].n[2];
+    return off;
+}
+
+// The first level of fmg!'s coarse-to-fine sweep that k_mg_tail_fmg takes on
+// (with all coarser ones), or -1: a level whose mgv! is a tail, below the finest
+// (its right-hand side is restricted from the level above, in the ghost layout).
+template <typename T>
+static int fmg_tail_start(const mcs_mg *mg)
+{
+    if (!mg->fmg_tail) return -1;
+    const int L = mg->cfg.levels;
+    int S = -1;
+    for (int l = L - 1; l >= 1; --l) {
+        if (!in_tail<T>(mg, l)) break;
+        S = l;
+    }
+    if (S == 1 && mg->split0) return -1;
+    return S;
+}
+
+template <typename T>
+static void mg_tail(mcs_mg *mg, int level, T *x, const T *b, const T *zc, bool zero, bool fresh)
+{
+    Tail Tl;
+    const int64_t off = tail_setup<T>(mg, level, Tl);
     ProfScope ps(mg->ctx, "mg_tail");
     hipLaunchKernelGGL(k_mg_tail<T>, dim3(1), dim3(MG_TAIL_THREADS), (size_t)off * sizeof(T), mg->ctx->stream, Tl,
                        level, x, b, zc, zero ? 1 : 0, fresh ? 1 : 0);
@@ -1911,7 +2015,9 @@ static T *fmg_correction(mcs_mg *mg)
 {
     mcs_ctx *ctx = mg->ctx;
     const int L = mg->cfg.levels;
-    for (int k = 0; k < L; ++k) {
+    const int S = fmg_tail_start<T>(mg);
+    const int kend = S > 0 ? S - 1 : L;   // restrictions on the grid path
+    for (int k = 0; k < kend; ++k) {
         ProfScope ps(ctx, "mg_restrict");
         if (k == 0 && mg->split0)
             hipLaunchKernelGGL(k_mg_restrict_s<T>, grid_for(ncoarse_cells(mg, k)), dim3(MCS_BLOCK), 0, ctx->stream,
@@ -1920,13 +2026,24 @@ static T *fmg_correction(mcs_mg *mg)
             hipLaunchKernelGGL(k_mg_restrict<T>, grid_for(ncoarse_cells(mg, k)), dim3(MCS_BLOCK), 0, ctx->stream,
                                lres<T>(mg, k + 1), (const T *)lres<T>(mg, k), lvl(mg, k));
     }
-    hipLaunchKernelGGL(k_mg_coarse<T>, dim3(1), dim3(MCS_BLOCK), 0, ctx->stream, coarse_inv<T>(mg),
-                       (const T *)lres<T>(mg, L), lsol<T>(mg, L), lvl(mg, L));
+    int top = L;   // the finest level solved so far
     T *z = lsol<T>(mg, L);
-    // apply!(z) of the coarse solution; the later ones are left by each mgv!
-    // (fresh), which also starts from interpolate!(z1, z)
-    mg_bc<T>(mg, L, z);
-    for (int k = 1; k <= L; ++k) {
+    if (S > 0) {
+        Tail Tl;
+        const int64_t words = tail_setup<T>(mg, S, Tl);
+        ProfScope ps(ctx, "mg_tail");
+        hipLaunchKernelGGL(k_mg_tail_fmg<T>, dim3(1), dim3(MG_TAIL_THREADS), (size_t)words * sizeof(T), ctx->stream,
+                           Tl, S, lsol<T>(mg, S), (const T *)lres<T>(mg, S - 1), coarse_inv<T>(mg));
+        top = S;
+        z = lsol<T>(mg, S);
+    } else {
+        hipLaunchKernelGGL(k_mg_coarse<T>, dim3(1), dim3(MCS_BLOCK), 0, ctx->stream, coarse_inv<T>(mg),
+                           (const T *)lres<T>(mg, L), lsol<T>(mg, L), lvl(mg, L));
+        // apply!(z) of the coarse solution; the later ones are left by each mgv!
+        // (fresh), which also starts from interpolate!(z1, z)
+        mg_bc<T>(mg, L, z);
+    }
+    for (int k = L - top + 1; k <= L; ++k) {
         int lev = L - k;
         T *z1 = lsol<T>(mg, lev);
         mgv<T>(mg, z1, lres<T>(mg, lev), lev, false, (const T *)z, true);

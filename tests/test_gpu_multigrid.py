@@ -212,3 +212,39 @@ def test_split_level0_matches_ghost_layout(monkeypatch, n, levels, prec):
         print(f"split={split} n={n} iters {it.value} resid {res.value:.3e}", flush=True)
     assert out["0"][1] == out["1"][1] and out["0"][2] == out["1"][2]
     np.testing.assert_array_equal(out["0"][0], out["1"][0])
+
+
+@pytest.mark.parametrize("n,bc,levels,smooth,prec,knobs", [
+    ((128, 128, 256), (N, N, N, N, D, D), 6, (2, 2), "f64", {}),
+    ((64, 64, 128), (N, D, N, N, D, D), 5, (2, 2), "f32", {}),
+    ((48, 40, 56), (D, N, D, N, N, D), 3, (1, 2), "f64", {}),
+    ((64, 64, 128), (N, N, N, N, D, D), 5, (2, 1), "f64", {"MCS_MG_TAIL_MAXCELLS": "8192"}),
+    ((32, 32, 64), (N, N, N, N, D, D), 4, (2, 2), "f64", {"MCS_MG_TAIL_MAXCELLS": "100000000"}),
+])
+def test_fmg_tail_matches_grid_path(monkeypatch, n, bc, levels, smooth, prec, knobs):
+    """fmg!'s restrictions, coarse solve and coarse-to-fine V-cycles of the tail
+    levels in one launch give bitwise the solution of one launch per step."""
+    from mcstreamer_amd import _lib
+    import ctypes
+    rng = np.random.default_rng(11 + sum(n))
+    g = (n[0] + 2) * (n[1] + 2) * (n[2] + 2)
+    b = rng.normal(size=g)
+    u0 = rng.normal(size=g) * 0.1
+    faces = [mg.BottomBnd, mg.TopBnd, mg.LeftBnd, mg.RightBnd, mg.FrontBnd, mg.BackBnd]
+    for k, v in knobs.items():
+        monkeypatch.setenv(k, v)
+    out = {}
+    for ft in ("0", "1"):
+        monkeypatch.setenv("MCS_MG_FMG_TAIL", ft)
+        conf = mg.MGConfig(bc=list(zip(faces, bc)), s=0.9, levels=levels, tolerance=-1e-9, maxiter=50,
+                           smooth1=smooth[0], smooth2=smooth[1], precision=prec)
+        ws = mg.Workspace(conf, n)
+        u = u0.copy()
+        it = ctypes.c_int32()
+        res = ctypes.c_double()
+        assert _lib.lib().mcs_mg_solve_host(ws.handle, _lib.dptr(u), _lib.dptr(b), ctypes.byref(it),
+                                            ctypes.byref(res)) == 0
+        out[ft] = (u, it.value, res.value)
+        print(f"fmg_tail={ft} n={n} iters {it.value} resid {res.value:.3e}", flush=True)
+    assert out["0"][1] == out["1"][1] and out["0"][2] == out["1"][2]
+    np.testing.assert_array_equal(out["0"][0], out["1"][0])
