@@ -207,9 +207,10 @@ __device__ __forceinline__ void tile_reduce(Neu2 acc[SW_NT], double *out)
 }
 
 // advance! of every slot's particles (JuMC/src/mixed_population.jl:24-54), births
-// to the log, and the post-step tallies of parents and children.  FP64 store.
+// to the log, and the post-step tallies of parents and children.  FP64 store; six
+// waves per SIMD (a few cold values spill), which hides the loads better than four.
 template <int BS>
-__global__ __launch_bounds__(BS) void k_sw_advance(SwArgs64 a)
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_sw_advance(SwArgs64 a)
 {
     const int64_t tile = blockIdx.x;
     const int slot = a.tile_slot[tile];
