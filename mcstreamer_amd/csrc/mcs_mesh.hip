@@ -9,6 +9,8 @@
 #include "mcs_device.h"
 #include "mcs_fp32.h"
 
+#define MESH_BOX_COPIES 64   // copies of the occupied-cell box (blocks spread their atomics)
+#define MESH_BOX_WORDS (6 * MESH_BOX_COPIES)   // the box copies after the cell counts
 #define M_SHORT 0
 #define M_BIG 1
 #define M_TOT 2    // records in the mesh's segments (cell_start[nseg])
@@ -50,7 +52,7 @@ extern "C" int32_t mcs_mesh_create(mcs_ctx *ctx, const int32_t n[3], double h, m
         hipMemset(m->arr[a], 0, sizeof(double) * m->arr_len[a]);
     }
     if (hipMalloc(&m->cell_start, sizeof(int32_t) * (ncells + 2)) != hipSuccess ||
-        hipMalloc(&m->cell_cnt, sizeof(int32_t) * (ncells + 2)) != hipSuccess ||
+        hipMalloc(&m->cell_cnt, sizeof(int32_t) * (ncells + 2 + MESH_BOX_WORDS)) != hipSuccess ||
         hipMalloc(&m->cnt64, sizeof(int64_t) * (ncells + 2)) != hipSuccess ||
         hipMalloc(&m->pre64, sizeof(int64_t) * (ncells + 2)) != hipSuccess)
         return mcs_fail(ctx, MCS_E_HIP, "mesh: device allocation failed");
@@ -130,6 +132,11 @@ struct MeshGeom {
     int64_t ncells;
 };
 
+static inline int32_t *mesh_box(const mcs_mesh *m)
+{
+    return m->cell_cnt + m->ncells + 2;
+}
+
 static inline MeshGeom geom(const mcs_mesh *m)
 {
     MeshGeom g;
@@ -142,16 +149,95 @@ static inline MeshGeom geom(const mcs_mesh *m)
 // Counting pass.  Lanes of a wave that share a cell take consecutive ranks from a
 // single atomic (ballot over equal keys), so a mostly sorted population costs one
 // atomic per cell run rather than one per particle.
+// The box of the cells that hold particles, kept after the cell counts (at
+// cell_cnt + ncells + 2, zeroed with them) in MESH_BOX_COPIES copies of six
+// words: d = max over particles of n[d] - i_d and 3 + d = max of i_d + 1, so
+// that zero means no particle.
+
+struct CellBox {
+    int lo[3], ext[3];   // first cell and extent per axis; ext[0] == 0: empty
+};
+
+// The six box words, the maxima over the copies (every thread of the block calls).
+__device__ __forceinline__ void box_words(const int32_t *box, int32_t bw[6])
+{
+    __shared__ int32_t sw[6];
+    if (threadIdx.x < 6) {
+        int v = 0;
+        for (int c = 0; c < MESH_BOX_COPIES; ++c) v = max(v, box[6 * c + threadIdx.x]);
+        sw[threadIdx.x] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < 6; ++w) bw[w] = sw[w];
+}
+
+// The box grown by g cells per side, clamped to the mesh.
+__device__ __forceinline__ CellBox cell_box(const int32_t *bw, const int32_t n[3], int g)
+{
+    CellBox B;
+    if (bw[3] == 0) {
+        B.lo[0] = B.lo[1] = B.lo[2] = 0;
+        B.ext[0] = B.ext[1] = B.ext[2] = 0;
+        return B;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int lo = max(n[d] - bw[d] - g, 0), hi = min(bw[3 + d] - 1 + g, n[d] - 1);
+        B.lo[d] = lo;
+        B.ext[d] = hi - lo + 1;
+    }
+    return B;
+}
+
+// Cell index (linear, x fastest) of position t of the box, in increasing order.
+__device__ __forceinline__ int64_t box_cell(const CellBox &B, const int32_t n[3], int64_t t, int64_t &i,
+                                            int64_t &j, int64_t &k)
+{
+    const int64_t q = t / B.ext[0];
+    i = B.lo[0] + (t - q * B.ext[0]);
+    j = B.lo[1] + q % B.ext[1];
+    k = B.lo[2] + q / B.ext[1];
+    return i + (int64_t)n[0] * (j + (int64_t)n[1] * k);
+}
+
 template <typename Buf>
 __global__ __launch_bounds__(MCS_BLOCK) void k_sort_count(Buf b, int64_t n, MeshGeom g, int32_t *cnt,
-                                                          int32_t *key_out, int32_t *rank_out)
+                                                          int32_t *key_out, int32_t *rank_out, int32_t *box)
 {
     int64_t i = gidx();
     int lane = threadIdx.x & (MCS_WAVE - 1);
     int key = -1;
+    // box words of this particle (0: none)
+    int bw[6] = {0, 0, 0, 0, 0, 0};
     if (i < n && b.active[i]) {
         int64_t c = cell_key(b, i, g.n, g.h);
         key = c < 0 ? (int)g.ncells : (int)c;
+        if (c >= 0) {
+            const uint32_t cu = (uint32_t)c, nx = (uint32_t)g.n[0], ny = (uint32_t)g.n[1];
+            const uint32_t q = cu / nx, ci = cu - q * nx, cj = q % ny, ck = q / ny;
+            bw[0] = g.n[0] - (int)ci;
+            bw[1] = g.n[1] - (int)cj;
+            bw[2] = g.n[2] - (int)ck;
+            bw[3] = (int)ci + 1;
+            bw[4] = (int)cj + 1;
+            bw[5] = (int)ck + 1;
+        }
+    }
+    {
+        // block maxima, then one atomic per word into the block's copy of the box
+        __shared__ int sbw[MCS_BLOCK / MCS_WAVE][6];
+#pragma unroll
+        for (int w = 0; w < 6; ++w) {
+            const int v = __reduce_max_sync(~0ULL, bw[w]);
+            if (lane == 0) sbw[threadIdx.x / MCS_WAVE][w] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < 6) {
+            int v = 0;
+            for (int w = 0; w < MCS_BLOCK / MCS_WAVE; ++w) v = max(v, sbw[w][threadIdx.x]);
+            if (v > 0) atomicMax(box + 6 * (blockIdx.x % MESH_BOX_COPIES) + threadIdx.x, v);
+        }
     }
     unsigned long long todo = __ballot(key >= 0);
     int rank = 0;
@@ -196,34 +282,50 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_sort_scatter(Buf b, int64_t n,
 #define SEG_HUGE 4096
 #define HUGE_MAX 200   // (start, length) pairs recorded after the counters in ctx->d_small
 
-__global__ __launch_bounds__(MCS_BLOCK) void k_segsort_list(const int32_t *cs, int64_t nseg,
-                                                            int32_t *lists, int64_t *nshort,
+__global__ __launch_bounds__(MCS_BLOCK) void k_segsort_list(const int32_t *cs, int64_t nseg, MeshGeom g,
+                                                            const int32_t *box, int32_t *lists, int64_t *nshort,
                                                             int64_t *nbig, int64_t *huge)
 {
-    const int64_t c = gidx();
-    if (c == 0) huge[M_TOT - M_HUGE] = cs[nseg];
-    const int64_t len = c < nseg ? (int64_t)cs[c + 1] - cs[c] : 0;
-    if (len > SEG_HUGE) {
-        unsigned long long k = atomicAdd((unsigned long long *)huge, 1ULL);
-        if (k < HUGE_MAX) {
-            huge[1 + 2 * k] = cs[c];
-            huge[2 + 2 * k] = len;
+    if (gidx() == 0) huge[M_TOT - M_HUGE] = cs[nseg];
+    // the segments that can hold two records or more: the occupied box's cells,
+    // then the bucket of particles outside the mesh (segment nseg - 1)
+    int32_t bw[6];
+    box_words(box, bw);
+    const CellBox B = cell_box(bw, g.n, 0);
+    const int64_t nbox = (int64_t)B.ext[0] * B.ext[1] * B.ext[2];
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base <= nbox; base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t t = base + threadIdx.x;
+        int64_t c = -1;
+        if (t < nbox) {
+            int64_t i, j, k;
+            c = box_cell(B, g.n, t, i, j, k);
+        } else if (t == nbox) {
+            c = nseg - 1;
         }
+        const int64_t len = c >= 0 ? (int64_t)cs[c + 1] - cs[c] : 0;
+        if (len > SEG_HUGE) {
+            unsigned long long k = atomicAdd((unsigned long long *)huge, 1ULL);
+            if (k < HUGE_MAX) {
+                huge[1 + 2 * k] = cs[c];
+                huge[2 + 2 * k] = len;
+            }
+        }
+        // one atomic per block and list (the counters are single addresses shared
+        // by the whole grid): threads take consecutive entries in block order
+        const int fs = (len > 1 && len <= SEG_SMALL) ? 1 : 0, fb = len > SEG_SMALL ? 1 : 0;
+        int ts, tb;
+        const int es = block_excl_scan<int>(fs, &ts);
+        const int eb = block_excl_scan<int>(fb, &tb);
+        __shared__ unsigned long long base_s, base_b;
+        if (threadIdx.x == 0) {
+            base_s = ts ? atomicAdd((unsigned long long *)nshort, (unsigned long long)ts) : 0ULL;
+            base_b = tb ? atomicAdd((unsigned long long *)nbig, (unsigned long long)tb) : 0ULL;
+        }
+        __syncthreads();
+        if (fs) lists[base_s + es] = (int32_t)c;
+        if (fb) lists[nseg - 1 - (int64_t)(base_b + eb)] = (int32_t)c;
+        __syncthreads();
     }
-    // one atomic per block and list (the counters are single addresses shared by
-    // the whole grid): threads take consecutive entries in block order
-    const int fs = (len > 1 && len <= SEG_SMALL) ? 1 : 0, fb = len > SEG_SMALL ? 1 : 0;
-    int ts, tb;
-    const int es = block_excl_scan<int>(fs, &ts);
-    const int eb = block_excl_scan<int>(fb, &tb);
-    __shared__ unsigned long long base_s, base_b;
-    if (threadIdx.x == 0) {
-        base_s = ts ? atomicAdd((unsigned long long *)nshort, (unsigned long long)ts) : 0ULL;
-        base_b = tb ? atomicAdd((unsigned long long *)nbig, (unsigned long long)tb) : 0ULL;
-    }
-    __syncthreads();
-    if (fs) lists[base_s + es] = (int32_t)c;
-    if (fb) lists[nseg - 1 - (int64_t)(base_b + eb)] = (int32_t)c;
 }
 
 // One wave per short segment: lane l holds element l; its rank is the number of
@@ -306,14 +408,15 @@ static int32_t sort_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, const Buf &dst
     int32_t rc;
     int64_t n = p->n;
     int64_t nseg = m->ncells + 1;
-    hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)cdiv(nseg + 1, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
-                       ctx->stream, m->cell_cnt, nseg + 1);
+    // the counts and, after them, the box of the occupied cells
+    hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)cdiv(nseg + 1 + MESH_BOX_WORDS, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                       ctx->stream, m->cell_cnt, nseg + 1 + MESH_BOX_WORDS);
     int64_t *ctr = ctx->d_small;
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, ctx->stream, (int32_t *)ctr, (int64_t)(2 * (M_HUGE + 1)));
     if (n > 0) {
         ProfScope ps(ctx, "sort_count");
         hipLaunchKernelGGL(k_sort_count<Buf>, dim3((unsigned)cdiv(n, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
-                           ctx->stream, src, n, geom(m), m->cell_cnt, p->i32a, p->i32b);
+                           ctx->stream, src, n, geom(m), m->cell_cnt, p->i32a, p->i32b, mesh_box(m));
     }
     MCS_LAUNCH_CHECK(ctx);
     rc = mcs_scan_i32(ctx, m->cell_cnt, m->cell_start, nseg);
@@ -326,9 +429,9 @@ static int32_t sort_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, const Buf &dst
                            (const int32_t *)p->i32b, (const int32_t *)m->cell_start, p->i32c,
                            p->u64a);
         // the count buffer is free again: it holds the lists of segments to sort
-        hipLaunchKernelGGL(k_segsort_list, dim3((unsigned)cdiv(nseg, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
-                           ctx->stream, (const int32_t *)m->cell_start, nseg, m->cell_cnt,
-                           ctr + M_SHORT, ctr + M_BIG, ctr + M_HUGE);
+        hipLaunchKernelGGL(k_segsort_list, dim3((unsigned)std::min<int64_t>(cdiv(nseg, MCS_BLOCK), 2048)),
+                           dim3(MCS_BLOCK), 0, ctx->stream, (const int32_t *)m->cell_start, nseg, geom(m),
+                           (const int32_t *)mesh_box(m), m->cell_cnt, ctr + M_SHORT, ctr + M_BIG, ctr + M_HUGE);
         hipLaunchKernelGGL(k_segsort_wave, dim3(2048), dim3(MCS_BLOCK), 0, ctx->stream,
                            (const int32_t *)m->cell_start, p->i32c, p->u64a,
                            (const int32_t *)m->cell_cnt, (const int64_t *)(ctr + M_SHORT));
@@ -418,6 +521,7 @@ struct DepArgs {
     int32_t cand_min;
     int64_t cand_at;
     int32_t *count_out;
+    const int32_t *box;   // the occupied-cell box of the last sort
 };
 
 #define DEP_MASK 0x1FFFFFULL
@@ -536,13 +640,11 @@ __device__ __forceinline__ int32_t dep_at(const DepRows &R, int32_t v)
 // for a record that feeds the cell.  Cells with many records go to
 // k_deposit_dense.
 template <int CIC, typename Buf>
-__global__ __launch_bounds__(MCS_BLOCK) void k_deposit(DepArgs<Buf> a)
+__device__ __forceinline__ void deposit_cell(const DepArgs<Buf> &a, const CellBox &B, int64_t t)
 {
-    int64_t c = gidx();
-    if (c >= a.g.ncells) return;
+    int64_t i, j, k;
+    const int64_t c = box_cell(B, a.g.n, t, i, j, k);
     const int64_t nx = a.g.n[0], ny = a.g.n[1], nz = a.g.n[2];
-    uint32_t cu = (uint32_t)c, qy = cu / (uint32_t)nx;
-    int64_t i = cu - qy * (uint32_t)nx, j = qy % (uint32_t)ny, k = qy / (uint32_t)ny;
     int64_t idx = (i + 1) + (nx + 2) * ((j + 1) + (ny + 2) * (k + 1));
     const double h = a.g.h;
     const double dV = (h * h) * h;
@@ -605,12 +707,24 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_deposit(DepArgs<Buf> a)
     a.arr[idx] = acc;
 }
 
+template <int CIC, typename Buf>
+__global__ __launch_bounds__(MCS_BLOCK) void k_deposit(DepArgs<Buf> a)
+{
+    // the cells that particles can reach: the occupied box, grown by one for CIC
+    int32_t bw[6];
+    box_words(a.box, bw);
+    const CellBox B = cell_box(bw, a.g.n, CIC ? 1 : 0);
+    const int64_t nbox = (int64_t)B.ext[0] * B.ext[1] * B.ext[2];
+    for (int64_t t = gidx(); t < nbox; t += (int64_t)gridDim.x * blockDim.x)
+        deposit_cell<CIC>(a, B, t);
+}
+
 #define DEP_SLOTS (MCS_WAVE + 8)   // per wavefront: 64 compacted shares and 8 pad slots
 #define DEP_UNROLL 1               // sequence positions per round: DEP_UNROLL x 64
 
 // The CIC gather of the listed cells, one wavefront per cell.  The lanes take
-// 4 x 64 consecutive positions of the cell's record sequence at once (the keys of
-// the next 256 are in flight while the shares of the current ones load); per 64
+// DEP_UNROLL x 64 consecutive positions of the cell's record sequence at once
+// (the keys of the next ones are in flight while their shares load); per 64
 // positions, the lanes whose record feeds the cell write their shares to LDS in
 // sequence order (compacted by a ballot prefix), and lane 0 adds them to the
 // cell's sum one by one, in that order: the additions of k_deposit's sequential
@@ -747,6 +861,7 @@ static int32_t deposit_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, int32_t whi
         m->dep_lane_listed = f ? atoll(f) : DEP_LANE_LISTED;
     }
     a.dense = m->dep_list;
+    a.box = mesh_box(m);
     a.cand_min = m->dep_dense;
     a.cand_at = m->ncells + 1;
     a.count_out = m->dep_host_dev;
@@ -759,7 +874,8 @@ static int32_t deposit_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, int32_t whi
                       : m->dep_dense;
     {
         ProfScope ps(ctx, "deposit");
-        dim3 g((unsigned)cdiv(m->ncells, MCS_BLOCK)), bl(MCS_BLOCK);
+        // a grid-stride walk over the occupied box (at most 4096 blocks)
+        dim3 g((unsigned)std::min<int64_t>(cdiv(m->ncells, MCS_BLOCK), 4096)), bl(MCS_BLOCK);
         if (scheme == MCS_DEPOSIT_CIC) {
             // the prep kernel also clears the list count (one block at least)
             hipLaunchKernelGGL(k_deposit_prep<Buf>, dim3((unsigned)cdiv(p->n > 0 ? p->n : 1, MCS_BLOCK)), bl, 0,
@@ -893,6 +1009,8 @@ int32_t mcs_mesh_faces32(mcs_mesh *m)
 // oracle restatement documents the rules).  WRITE = false counts the outputs;
 // WRITE = true replays the same draws and writes them: surviving originals first,
 // then the created particles, ids next_id + (created before this one).
+#define RS_PF 8   // weights loaded ahead in resample_cell
+
 template <bool WRITE, typename Buf>
 __device__ void resample_cell(int64_t c, int64_t a, int64_t e, int32_t ntarget, int32_t nmax,
                               uint64_t seed, uint32_t instance, uint32_t step, const Buf &src,
@@ -907,46 +1025,62 @@ __device__ void resample_cell(int64_t c, int64_t a, int64_t e, int32_t ntarget, 
     mcs_stream_init(&rs, seed, instance, MCS_PHASE_CELL_RESAMPLE, (uint64_t)c, step);
     double wtotal = 0.0, xx = 0.0, wcum = 0.0;
     int64_t cnt = 0, pk = 0;
-    for (int64_t i = a; i < e; ++i) {
-        cnt += 1;
-        wtotal += get_w(src, i);
-        if (cnt == nmax) xx = mcs_samplemin(&rs, (double)ntarget);
+    // weights read RS_PF at a time (the loads in flight together), used in order
+    for (int64_t i0 = a; i0 < e; i0 += RS_PF) {
+        double wv[RS_PF];
+#pragma unroll
+        for (int u = 0; u < RS_PF; ++u) wv[u] = i0 + u < e ? get_w(src, i0 + u) : 0.0;
+#pragma unroll
+        for (int u = 0; u < RS_PF; ++u) {
+            if (i0 + u >= e) break;
+            cnt += 1;
+            wtotal += wv[u];
+            if (cnt == nmax) xx = mcs_samplemin(&rs, (double)ntarget);
+        }
     }
     const bool merged = pc > nmax;
     const int64_t created_at = merged ? 0 : pc;
     int64_t splits = 0, removed = 0;
-    for (int64_t i = a; i < e; ++i) {
-        double wi = get_w(src, i);
-        if (pc > nmax) {
-            while ((wtotal * xx < wcum + wi) && pk < ntarget) {
-                pk += 1;
-                if (WRITE) {
-                    int64_t d = base + created_at + ncreated;
-                    copy_rec(src, i, dst, d);
-                    set_w(dst, d, wtotal / (double)ntarget);
-                    dst.id[d] = idbase + (uint64_t)ncreated;
+    for (int64_t i0 = a; i0 < e; i0 += RS_PF) {
+        double wv[RS_PF];
+#pragma unroll
+        for (int u = 0; u < RS_PF; ++u) wv[u] = i0 + u < e ? get_w(src, i0 + u) : 0.0;
+#pragma unroll
+        for (int u = 0; u < RS_PF; ++u) {
+            const int64_t i = i0 + u;
+            if (i >= e) break;
+            const double wi = wv[u];
+            if (pc > nmax) {
+                while ((wtotal * xx < wcum + wi) && pk < ntarget) {
+                    pk += 1;
+                    if (WRITE) {
+                        int64_t d = base + created_at + ncreated;
+                        copy_rec(src, i, dst, d);
+                        set_w(dst, d, wtotal / (double)ntarget);
+                        dst.id[d] = idbase + (uint64_t)ncreated;
+                    }
+                    ncreated += 1;
+                    if (pk < ntarget) xx = xx + mcs_samplemin(&rs, (double)(ntarget - pk)) * (1.0 - xx);
                 }
-                ncreated += 1;
-                if (pk < ntarget) xx = xx + mcs_samplemin(&rs, (double)(ntarget - pk)) * (1.0 - xx);
-            }
-            wcum = wcum + wi;
-            removed += 1;
-        } else {
-            double wkeep = wi;
-            if (pc < ntarget && wi > 2.0) {
-                double wnew = rint(wi / 2.0);
-                if (WRITE) {
-                    int64_t d = base + created_at + ncreated;
-                    copy_rec(src, i, dst, d);
-                    set_w(dst, d, wnew);
-                    dst.id[d] = idbase + (uint64_t)ncreated;
+                wcum = wcum + wi;
+                removed += 1;
+            } else {
+                double wkeep = wi;
+                if (pc < ntarget && wi > 2.0) {
+                    double wnew = rint(wi / 2.0);
+                    if (WRITE) {
+                        int64_t d = base + created_at + ncreated;
+                        copy_rec(src, i, dst, d);
+                        set_w(dst, d, wnew);
+                        dst.id[d] = idbase + (uint64_t)ncreated;
+                    }
+                    ncreated += 1;
+                    splits += 1;
+                    wkeep = wi - wnew;
+                    pc += 1;
                 }
-                ncreated += 1;
-                splits += 1;
-                wkeep = wi - wnew;
-                pc += 1;
+                if (WRITE && wkeep != wi) set_w(dst, base + (i - a), wkeep);  // survivors copied by k_resample_copy
             }
-            if (WRITE && wkeep != wi) set_w(dst, base + (i - a), wkeep);  // survivors copied by k_resample_copy
         }
     }
     nout = (merged ? 0 : (e - a)) + ncreated;
