@@ -10,7 +10,8 @@
 #include "mcs_fp32.h"
 
 #define MESH_BOX_COPIES 64   // copies of the occupied-cell box (blocks spread their atomics)
-#define MESH_BOX_WORDS (6 * MESH_BOX_COPIES)   // the box copies after the cell counts
+#define MESH_BOX_FINAL (6 * MESH_BOX_COPIES)   // the reduced box after the copies
+#define MESH_BOX_WORDS (MESH_BOX_FINAL + 6)   // the box copies and the reduced box after the cell counts
 #define M_SHORT 0
 #define M_BIG 1
 #define M_TOT 2    // records in the mesh's segments (cell_start[nseg])
@@ -152,7 +153,8 @@ static inline MeshGeom geom(const mcs_mesh *m)
 // The box of the cells that hold particles, kept after the cell counts (at
 // cell_cnt + ncells + 2, zeroed with them) in MESH_BOX_COPIES copies of six
 // words: d = max over particles of n[d] - i_d and 3 + d = max of i_d + 1, so
-// that zero means no particle.
+// that zero means no particle.  The segment listing reduces the copies and
+// stores the six words at MESH_BOX_FINAL for the later walks.
 
 struct CellBox {
     int lo[3], ext[3];   // first cell and extent per axis; ext[0] == 0: empty
@@ -283,7 +285,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_sort_scatter(Buf b, int64_t n,
 #define HUGE_MAX 200   // (start, length) pairs recorded after the counters in ctx->d_small
 
 __global__ __launch_bounds__(MCS_BLOCK) void k_segsort_list(const int32_t *cs, int64_t nseg, MeshGeom g,
-                                                            const int32_t *box, int32_t *lists, int64_t *nshort,
+                                                            int32_t *box, int32_t *lists, int64_t *nshort,
                                                             int64_t *nbig, int64_t *huge)
 {
     if (gidx() == 0) huge[M_TOT - M_HUGE] = cs[nseg];
@@ -291,6 +293,8 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_segsort_list(const int32_t *cs, i
     // then the bucket of particles outside the mesh (segment nseg - 1)
     int32_t bw[6];
     box_words(box, bw);
+    // the reduced box for the later walks (every block writes the same words)
+    if (threadIdx.x < 6) box[MESH_BOX_FINAL + threadIdx.x] = bw[threadIdx.x];
     const CellBox B = cell_box(bw, g.n, 0);
     const int64_t nbox = (int64_t)B.ext[0] * B.ext[1] * B.ext[2];
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base <= nbox; base += (int64_t)gridDim.x * blockDim.x) {
@@ -431,7 +435,7 @@ static int32_t sort_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, const Buf &dst
         // the count buffer is free again: it holds the lists of segments to sort
         hipLaunchKernelGGL(k_segsort_list, dim3((unsigned)std::min<int64_t>(cdiv(nseg, MCS_BLOCK), 2048)),
                            dim3(MCS_BLOCK), 0, ctx->stream, (const int32_t *)m->cell_start, nseg, geom(m),
-                           (const int32_t *)mesh_box(m), m->cell_cnt, ctr + M_SHORT, ctr + M_BIG, ctr + M_HUGE);
+                           mesh_box(m), m->cell_cnt, ctr + M_SHORT, ctr + M_BIG, ctr + M_HUGE);
         hipLaunchKernelGGL(k_segsort_wave, dim3(2048), dim3(MCS_BLOCK), 0, ctx->stream,
                            (const int32_t *)m->cell_start, p->i32c, p->u64a,
                            (const int32_t *)m->cell_cnt, (const int64_t *)(ctr + M_SHORT));
@@ -710,10 +714,9 @@ __device__ __forceinline__ void deposit_cell(const DepArgs<Buf> &a, const CellBo
 template <int CIC, typename Buf>
 __global__ __launch_bounds__(MCS_BLOCK) void k_deposit(DepArgs<Buf> a)
 {
-    // the cells that particles can reach: the occupied box, grown by one for CIC
-    int32_t bw[6];
-    box_words(a.box, bw);
-    const CellBox B = cell_box(bw, a.g.n, CIC ? 1 : 0);
+    // the cells that particles can reach: the occupied box (reduced by the sort's
+    // segment listing), grown by one for CIC
+    const CellBox B = cell_box(a.box + MESH_BOX_FINAL, a.g.n, CIC ? 1 : 0);
     const int64_t nbox = (int64_t)B.ext[0] * B.ext[1] * B.ext[2];
     for (int64_t t = gidx(); t < nbox; t += (int64_t)gridDim.x * blockDim.x)
         deposit_cell<CIC>(a, B, t);
@@ -874,8 +877,9 @@ static int32_t deposit_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, int32_t whi
                       : m->dep_dense;
     {
         ProfScope ps(ctx, "deposit");
-        // a grid-stride walk over the occupied box (at most 4096 blocks)
-        dim3 g((unsigned)std::min<int64_t>(cdiv(m->ncells, MCS_BLOCK), 4096)), bl(MCS_BLOCK);
+        // one thread per cell of the occupied box (the grid covers the mesh; the
+        // threads past the box end at once)
+        dim3 g((unsigned)cdiv(m->ncells, MCS_BLOCK)), bl(MCS_BLOCK);
         if (scheme == MCS_DEPOSIT_CIC) {
             // the prep kernel also clears the list count (one block at least)
             hipLaunchKernelGGL(k_deposit_prep<Buf>, dim3((unsigned)cdiv(p->n > 0 ? p->n : 1, MCS_BLOCK)), bl, 0,
