@@ -128,6 +128,10 @@ typedef struct mcs_mg_config {
                            a negative value -t means t times the rms of b (relative mode) */
     int32_t maxiter;
     int32_t precision;  /* MCS_PREC_F64, or MCS_PREC_F32 smoother inside FP64 refinement */
+    int32_t ndim;       /* 3 (also 0), or 2: an (n[0], n[1]) grid, n[2] == 1, bc[0..3], FP64,
+                           arrays (n[0]+2)(n[1]+2) -- Multigrid.jl with N = 2 */
+    int32_t cyl;        /* 0: CartesianConnector; 1 + d: CylindricalConnector{d+1} on dimension d
+                           (Multigrid.jl:31-39; 2D only) */
 } mcs_mg_config;
 
 /* ---- context ------------------------------------------------------------- */
@@ -233,8 +237,9 @@ int32_t mcs_resample_cells(mcs_pop *p, mcs_mesh *m, int32_t ntarget, int32_t nma
  * restriction, trilinear prolongation and the coarse direct solve. */
 int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg **out);
 int32_t mcs_mg_destroy(mcs_mg *mg);
-/* Solve L u + s b = 0 on host arrays with ghosts ((nx+2)(ny+2)(nz+2), x fastest);
- * u is the initial guess on entry and the solution on return. */
+/* Solve L u + s b = 0 on host arrays with ghosts ((nx+2)(ny+2)(nz+2), x fastest;
+ * (nx+2)(ny+2) for ndim 2); u is the initial guess on entry and the solution on
+ * return. */
 int32_t mcs_mg_solve_host(mcs_mg *mg, double *u, const double *b, int32_t *iters,
                           double *resid);
 /* Same, on a mesh: u = mesh U, b = mesh Q (device resident). */

@@ -100,6 +100,8 @@ class MGConfigC(ctypes.Structure):
         ("tolerance", ctypes.c_double),
         ("maxiter", ctypes.c_int32),
         ("precision", ctypes.c_int32),
+        ("ndim", ctypes.c_int32),
+        ("cyl", ctypes.c_int32),
     ]
 
 

@@ -201,6 +201,11 @@ struct mcs_mesh {
     int64_t dep_lane_listed = 0;    // above this many such cells (last call), one lane per cell instead
 };
 
+// The two-dimensional solver (mcs_mg2.hip): creation for cfg->ndim == 2, and the
+// solve on device arrays with ghosts.
+int32_t mcs_mg2_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg **out);
+int32_t mcs_mg2_solve_dev(mcs_mg *mg, double *u, const double *b, int32_t *iters, double *resid);
+
 // FP32 copies of the face fields, allocated and filled on first use; from then on
 // mcs_efield_faces writes both precisions.
 int32_t mcs_mesh_faces32(mcs_mesh *m);

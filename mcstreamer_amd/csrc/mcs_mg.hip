@@ -1534,6 +1534,9 @@ extern "C" int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg 
                              (int)tail_lds) != hipSuccess))
         tail_lds = 64 * 1024;
     if (!ctx || !cfg || !out) return MCS_E_ARG;
+    if (cfg->ndim == 2) return mcs_mg2_create(ctx, cfg, out);
+    if (cfg->ndim != 0 && cfg->ndim != 3) return mcs_fail(ctx, MCS_E_ARG, "mg: ndim is 2 or 3");
+    if (cfg->cyl != 0) return mcs_fail(ctx, MCS_E_ARG, "mg: the cylindrical connector is for 2D grids");
     const mcs_mg_config &c = *cfg;
     if (c.levels < 0 || c.levels >= MG_MAXL || c.smooth1 < 0 || c.smooth2 < 0 || c.maxiter < 0)
         return mcs_fail(ctx, MCS_E_ARG, "mg: bad levels/smoothing/maxiter");
@@ -1674,6 +1677,7 @@ extern "C" int32_t mcs_mg_destroy(mcs_mg *mg)
     for (auto &L : mg->lv) {
         hipFree(L.res);
         hipFree(L.sol);
+        if (L.res1) hipFree(L.res1);
         if (L.fres) hipFree(L.fres);
         if (L.fsol) hipFree(L.fsol);
         if (L.alt) hipFree(L.alt);
@@ -2179,7 +2183,8 @@ extern "C" int32_t mcs_mg_solve_host(mcs_mg *mg, double *u, const double *b, int
     MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     MCS_HIP(ctx, hipMemcpy(mg->u_tmp, u, sizeof(double) * len, hipMemcpyHostToDevice));
     MCS_HIP(ctx, hipMemcpy(mg->b_tmp, b, sizeof(double) * len, hipMemcpyHostToDevice));
-    int32_t rc = mg_solve_dev(mg, mg->u_tmp, mg->b_tmp, iters, resid);
+    int32_t rc = mg->cfg.ndim == 2 ? mcs_mg2_solve_dev(mg, mg->u_tmp, mg->b_tmp, iters, resid)
+                                   : mg_solve_dev(mg, mg->u_tmp, mg->b_tmp, iters, resid);
     MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     MCS_HIP(ctx, hipMemcpy(u, mg->u_tmp, sizeof(double) * len, hipMemcpyDeviceToHost));
     return rc;
@@ -2189,6 +2194,7 @@ extern "C" int32_t mcs_mg_solve_mesh(mcs_mg *mg, mcs_mesh *m, int32_t *iters, do
 {
     if (!mg || !m) return MCS_E_ARG;
     mcs_ctx *ctx = mg->ctx;
+    if (mg->cfg.ndim == 2) return mcs_fail(ctx, MCS_E_ARG, "mg: a 2D solver has no 3D mesh; use the host solve");
     for (int d = 0; d < 3; ++d)
         if (m->n[d] != mg->cfg.n[d]) return mcs_fail(ctx, MCS_E_ARG, "mg: mesh size differs from the solver's");
     hipSetDevice(ctx->device);
@@ -2199,6 +2205,7 @@ extern "C" int32_t mcs_mg_setinhom_mesh(mcs_mg *mg, mcs_mesh *m, int32_t face, c
 {
     if (!mg || !m || !val || face < 0 || face > 5) return MCS_E_ARG;
     mcs_ctx *ctx = mg->ctx;
+    if (mg->cfg.ndim == 2) return mcs_fail(ctx, MCS_E_ARG, "mg: a 2D solver has no 3D mesh");
     hipSetDevice(ctx->device);
     Lvl L = lvl(mg, 0);
     int d = face / 2;

@@ -1,0 +1,485 @@
+// Multigrid in two dimensions (Multigrid/src/Multigrid.jl with N = 2) with the
+// Cartesian or the cylindrical connector (Multigrid.jl:31-39), the reference's
+// axisymmetric (r, z) solver (MCStreamer/src/main.jl:139 passes
+// CylindricalConnector{1}).  One launch per phase; the phases and their
+// arithmetic are those of the three-dimensional solver in mcs_mg.hip with the
+// 5-point stencil: lplstencil +x, +y, -x, -y; redblack2 colours; 4 children per
+// coarse cell (x offset slowest); prolongation weights prod(3 - 2 sh) / 16.  The
+// coarse direct solve uses the Cartesian operator, as buildmat does.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mcs_internal.h"
+
+namespace {
+
+struct Lv2 {
+    int32_t n[2];
+};
+
+__device__ __forceinline__ int64_t g2(const Lv2 &L, int64_t i, int64_t j)   // interior (i, j), 0-based
+{
+    return (i + 1) + (int64_t)(L.n[0] + 2) * (j + 1);
+}
+
+// Connector factor of the neighbour at offset sgn along d of the cell with
+// interior index i along d: 1 + sgn / (2 i + 1) on the cylindrical dimension
+// (cyl = 1 + d), 1 elsewhere (x * 1.0 is x).
+__device__ __forceinline__ double conn2(int32_t cyl, int d, int sgn, int64_t i)
+{
+    if (cyl != d + 1) return 1.0;
+    return 1.0 + (double)sgn / (double)(2 * i + 1);
+}
+
+__device__ __forceinline__ double lap2(const double *u, const Lv2 &L, int32_t cyl, int64_t i, int64_t j, int64_t c)
+{
+    const int64_t sy = L.n[0] + 2;
+    double l = -4.0 * u[c];
+    l += u[c + 1] * conn2(cyl, 0, 1, i);
+    l += u[c + sy] * conn2(cyl, 1, 1, j);
+    l += u[c - 1] * conn2(cyl, 0, -1, i);
+    l += u[c - sy] * conn2(cyl, 1, -1, j);
+    return l;
+}
+
+// apply! in 2D: a ghost cell takes the nearest interior value times the factor
+// (+1 Neumann, -1 Dirichlet) of every face it lies beyond, which is what the
+// face-by-face copies x-, x+, y-, y+ leave (corners included).
+__global__ void k2_bc(double *a, Lv2 L, uint32_t neg)
+{
+    const int64_t t = gidx();
+    const int64_t sx = L.n[0] + 2, tot = sx * (L.n[1] + 2);
+    if (t >= tot) return;
+    const int64_t i = t % sx, j = t / sx;
+    const bool xl = i == 0, xh = i == L.n[0] + 1, yl = j == 0, yh = j == L.n[1] + 1;
+    if (!(xl || xh || yl || yh)) return;
+    const int64_t vi = xl ? 1 : (xh ? L.n[0] : i), vj = yl ? 1 : (yh ? L.n[1] : j);
+    uint32_t flip = 0;
+    if (xl) flip ^= neg & 1u;
+    if (xh) flip ^= (neg >> 1) & 1u;
+    if (yl) flip ^= (neg >> 2) & 1u;
+    if (yh) flip ^= (neg >> 3) & 1u;
+    const double v = a[vi + sx * vj];
+    a[t] = flip ? -v : v;
+}
+
+// One colour of gauss_seidel! (redblack2: colour par = cells with i + j % 2 == par).
+__global__ void k2_gs(double *u, const double *b, Lv2 L, int32_t cyl, int par)
+{
+    const int64_t t = gidx();
+    const int64_t nxh = L.n[0] / 2;
+    if (t >= nxh * L.n[1]) return;
+    const int64_t j = t / nxh, i = 2 * (t - j * nxh) + ((j + par) & 1);
+    const int64_t c = g2(L, i, j);
+    const double l = lap2(u, L, cyl, i, j, c);
+    u[c] += (1.0 * (l + b[c])) / 4.0;
+}
+
+// residual! (r = s b + L u on the interior) with per-block sums of r^2 when
+// partial is given (thread order, then a fixed tree).
+__global__ __launch_bounds__(MCS_BLOCK) void k2_resid(double *r, const double *u, const double *b, double s, Lv2 L,
+                                                      int32_t cyl, double *partial)
+{
+    __shared__ double sh[MCS_BLOCK];
+    const int64_t t = gidx();
+    double q = 0.0;
+    if (t < (int64_t)L.n[0] * L.n[1]) {
+        const int64_t j = t / L.n[0], i = t - j * L.n[0];
+        const int64_t c = g2(L, i, j);
+        const double v = s * b[c] + lap2(u, L, cyl, i, j, c);
+        r[c] = v;
+        q = v * v;
+    }
+    if (!partial) return;
+    sh[threadIdx.x] = q;
+    __syncthreads();
+    for (int o = MCS_BLOCK / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// Per-block sums of b^2 over the interior (the relative tolerance's rms of b).
+__global__ __launch_bounds__(MCS_BLOCK) void k2_sumsq(const double *b, Lv2 L, double *partial)
+{
+    __shared__ double sh[MCS_BLOCK];
+    const int64_t t = gidx();
+    double q = 0.0;
+    if (t < (int64_t)L.n[0] * L.n[1]) {
+        const int64_t j = t / L.n[0], i = t - j * L.n[0];
+        const double v = b[g2(L, i, j)];
+        q = v * v;
+    }
+    sh[threadIdx.x] = q;
+    __syncthreads();
+    for (int o = MCS_BLOCK / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// The partials summed in order by one block, the total published to the host.
+__global__ __launch_bounds__(MCS_BLOCK) void k2_sum(const double *partial, int64_t nb, uint64_t *pub, uint64_t seq)
+{
+    __shared__ double sh[MCS_BLOCK];
+    double acc = 0.0;
+    for (int64_t q = threadIdx.x; q < nb; q += blockDim.x) acc += partial[q];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = MCS_BLOCK / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        pub[1] = (uint64_t)__double_as_longlong(sh[0]);
+        mcs_pub_raise(pub, seq);
+    }
+}
+
+// restrict!: rh = 4 (sum of the 4 children) / 4, children (0,0), (0,1), (1,0), (1,1).
+__global__ void k2_restrict(double *rh, const double *r, Lv2 F)
+{
+    const int64_t t = gidx();
+    const int64_t nc0 = F.n[0] / 2, nc1 = F.n[1] / 2;
+    if (t >= nc0 * nc1) return;
+    const int64_t jc = t / nc0, ic = t - jc * nc0;
+    const int64_t fy = F.n[0] + 2;
+    const int64_t f = (2 * ic + 1) + fy * (2 * jc + 1);
+    double s = 0.0;
+    s += r[f];
+    s += r[f + fy];
+    s += r[f + 1];
+    s += r[f + 1 + fy];
+    rh[(ic + 1) + (nc0 + 2) * (jc + 1)] = (4.0 * s) / 4.0;
+}
+
+// interpolate! into fine cell (i, j): the four coarse values towards the cell,
+// weights prod(3 - 2 sh) / 16 in cubestencil order; added or written.
+__global__ void k2_interp(double *r, const double *rh, Lv2 F, int add)
+{
+    const int64_t t = gidx();
+    if (t >= (int64_t)F.n[0] * F.n[1]) return;
+    const int64_t j = t / F.n[0], i = t - j * F.n[0];
+    const int64_t nc0 = F.n[0] / 2, cy = nc0 + 2;
+    const int64_t ic = i / 2, jc = j / 2, sx = i & 1, sy = j & 1;
+    const int64_t cc = (ic + 1) + cy * (jc + 1);
+    const int64_t ddx = 2 * sx - 1, ddy = 2 * sy - 1;
+    double v = 0.0;
+#pragma unroll
+    for (int hx = 0; hx < 2; ++hx)
+#pragma unroll
+        for (int hy = 0; hy < 2; ++hy) {
+            const double w = (double)((3 - 2 * hx) * (3 - 2 * hy)) / 16.0;
+            v += w * rh[cc + ddx * hx + cy * ddy * hy];
+        }
+    const int64_t c = g2(F, i, j);
+    r[c] = add ? r[c] + v : v;
+}
+
+// z = A^-1 btop on the coarsest interior, one row per thread, columns in order.
+__global__ void k2_coarse(const double *ainv, const double *res, double *z, Lv2 C)
+{
+    const int64_t N = (int64_t)C.n[0] * C.n[1];
+    for (int64_t row = gidx(); row < N; row += (int64_t)gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (int64_t col = 0; col < N; ++col) {
+            const int64_t jc = col / C.n[0], ic = col - jc * C.n[0];
+            acc += ainv[row * N + col] * res[g2(C, ic, jc)];
+        }
+        const int64_t j = row / C.n[0], i = row - j * C.n[0];
+        z[g2(C, i, j)] = acc;
+    }
+}
+
+__global__ void k2_fill0(double *a, int64_t n)
+{
+    const int64_t t = gidx();
+    if (t < n) a[t] = 0.0;
+}
+
+__global__ void k2_axpy(double *x, const double *z, int64_t n)
+{
+    const int64_t t = gidx();
+    if (t < n) x[t] += z[t];
+}
+
+inline unsigned nblk(int64_t n)
+{
+    return (unsigned)((n + MCS_BLOCK - 1) / MCS_BLOCK);
+}
+
+Lv2 lv2(const mcs_mg *mg, int l)
+{
+    Lv2 L;
+    L.n[0] = mg->lv[l].n[0];
+    L.n[1] = mg->lv[l].n[1];
+    return L;
+}
+
+uint32_t neg2(const mcs_mg *mg)
+{
+    uint32_t m = 0;
+    for (int f = 0; f < 4; ++f)
+        if (mg->cfg.bc[f] == MCS_BC_DIRICHLET) m |= 1u << f;
+    return m;
+}
+
+// buildmat in 2D (Cartesian 5-point operator, ghost rule folded), inverted by the
+// Gauss-Jordan elimination of the CPU restatement.
+int coarse_inverse2(const int32_t n[2], const int32_t bc[4], std::vector<double> &ainv)
+{
+    const int64_t N = (int64_t)n[0] * n[1];
+    std::vector<double> a((size_t)(N * N), 0.0);
+    ainv.assign((size_t)(N * N), 0.0);
+    for (int64_t j = 0; j < n[1]; ++j)
+        for (int64_t i = 0; i < n[0]; ++i) {
+            const int64_t row = i + n[0] * j;
+            const int64_t c[2] = {i, j};
+            a[row * N + row] -= 4.0;
+            for (int q = 0; q < 4; ++q) {
+                const int d = q % 2, sgn = q < 2 ? 1 : -1;
+                int64_t cc[2] = {c[0], c[1]};
+                cc[d] += sgn;
+                if (cc[d] < 0 || cc[d] >= n[d]) {
+                    const int face = 2 * d + (sgn > 0 ? 1 : 0);
+                    a[row * N + row] += bc[face] == MCS_BC_DIRICHLET ? -1.0 : 1.0;
+                } else {
+                    a[row * N + cc[0] + n[0] * cc[1]] += 1.0;
+                }
+            }
+        }
+    for (int64_t r = 0; r < N; ++r) ainv[r * N + r] = 1.0;
+    for (int64_t col = 0; col < N; ++col) {
+        int64_t piv = col;
+        double best = fabs(a[col * N + col]);
+        for (int64_t r = col + 1; r < N; ++r)
+            if (fabs(a[r * N + col]) > best) {
+                best = fabs(a[r * N + col]);
+                piv = r;
+            }
+        if (best == 0.0) return 1;
+        if (piv != col)
+            for (int64_t c = 0; c < N; ++c) {
+                std::swap(a[col * N + c], a[piv * N + c]);
+                std::swap(ainv[col * N + c], ainv[piv * N + c]);
+            }
+        const double d = a[col * N + col];
+        for (int64_t c = 0; c < N; ++c) {
+            a[col * N + c] = a[col * N + c] / d;
+            ainv[col * N + c] = ainv[col * N + c] / d;
+        }
+        for (int64_t r = 0; r < N; ++r) {
+            if (r == col) continue;
+            const double fct = a[r * N + col];
+            if (fct == 0.0) continue;
+            for (int64_t c = 0; c < N; ++c) {
+                a[r * N + c] = a[r * N + c] - fct * a[col * N + c];
+                ainv[r * N + c] = ainv[r * N + c] - fct * ainv[col * N + c];
+            }
+        }
+    }
+    return 0;
+}
+
+void bc2(mcs_mg *mg, int l, double *a)
+{
+    const Lv2 L = lv2(mg, l);
+    hipLaunchKernelGGL(k2_bc, dim3(nblk(mg->lv[l].len)), dim3(MCS_BLOCK), 0, mg->ctx->stream, a, L, neg2(mg));
+}
+
+void gs2(mcs_mg *mg, int l, double *u, const double *b)
+{
+    const Lv2 L = lv2(mg, l);
+    const int64_t cnt = (int64_t)(L.n[0] / 2) * L.n[1];
+    for (int par = 0; par < 2; ++par)
+        hipLaunchKernelGGL(k2_gs, dim3(nblk(cnt)), dim3(MCS_BLOCK), 0, mg->ctx->stream, u, b, L, mg->cfg.cyl, par);
+}
+
+void fill0(mcs_mg *mg, double *a, int64_t n)
+{
+    hipLaunchKernelGGL(k2_fill0, dim3(nblk(n)), dim3(MCS_BLOCK), 0, mg->ctx->stream, a, n);
+}
+
+// mgv! (Multigrid.jl:334-377).  apply!(r) before restrict! is left out: the
+// restriction reads interior cells only.
+void mgv2(mcs_mg *mg, double *x, const double *b, int level)
+{
+    hipStream_t st = mg->ctx->stream;
+    if (level == mg->cfg.levels) {
+        for (int it = 0; it < 50; ++it) {
+            bc2(mg, level, x);
+            gs2(mg, level, x, b);
+        }
+        return;
+    }
+    for (int it = 0; it < mg->cfg.smooth1; ++it) {
+        bc2(mg, level, x);
+        gs2(mg, level, x, b);
+    }
+    const Lv2 L = lv2(mg, level);
+    double *r = mg->lv[level].res1;
+    bc2(mg, level, x);
+    hipLaunchKernelGGL(k2_resid, dim3(nblk((int64_t)L.n[0] * L.n[1])), dim3(MCS_BLOCK), 0, st, r, (const double *)x,
+                       b, 1.0, L, mg->cfg.cyl, (double *)nullptr);
+    double *rh = mg->lv[level + 1].res;
+    hipLaunchKernelGGL(k2_restrict, dim3(nblk((int64_t)(L.n[0] / 2) * (L.n[1] / 2))), dim3(MCS_BLOCK), 0, st, rh,
+                       (const double *)r, L);
+    double *xh = mg->lv[level + 1].sol;
+    fill0(mg, xh, mg->lv[level + 1].len);
+    mgv2(mg, xh, rh, level + 1);
+    hipLaunchKernelGGL(k2_interp, dim3(nblk((int64_t)L.n[0] * L.n[1])), dim3(MCS_BLOCK), 0, st, x,
+                       (const double *)xh, L, 1);
+    for (int it = 0; it < mg->cfg.smooth2; ++it) {
+        bc2(mg, level, x);
+        gs2(mg, level, x, b);
+    }
+}
+
+// fmg! (Multigrid.jl:380-434).  Returns 1 after a correction, 0 when converged,
+// negative on a device error.
+int fmg2(mcs_mg *mg, double *x, const double *b, double *eps_out)
+{
+    mcs_ctx *ctx = mg->ctx;
+    hipStream_t st = ctx->stream;
+    const int L = mg->cfg.levels;
+    const Lv2 F = lv2(mg, 0);
+    const int64_t N0 = (int64_t)F.n[0] * F.n[1];
+    bc2(mg, 0, x);
+    fill0(mg, mg->lv[0].res, mg->lv[0].len);
+    hipLaunchKernelGGL(k2_resid, dim3(nblk(N0)), dim3(MCS_BLOCK), 0, st, mg->lv[0].res, (const double *)x, b,
+                       mg->cfg.s, F, mg->cfg.cyl, mg->partial);
+    const uint64_t seq = mcs_pub_next(ctx);
+    hipLaunchKernelGGL(k2_sum, dim3(1), dim3(MCS_BLOCK), 0, st, (const double *)mg->partial, (int64_t)nblk(N0),
+                       ctx->d_pub, seq);
+    double ss = 0.0;
+    if (hipGetLastError() != hipSuccess || mcs_pub_wait(ctx, seq, &ss, 1) != MCS_OK) return -1;
+    const double eps = sqrt(ss) / sqrt((double)F.n[0] * (double)F.n[1]);
+    *eps_out = eps;
+    if (mg->cfg.s * mg->tol_abs > eps) return 0;
+    for (int k = 0; k < L; ++k) {
+        const Lv2 Lk = lv2(mg, k);
+        hipLaunchKernelGGL(k2_restrict, dim3(nblk((int64_t)(Lk.n[0] / 2) * (Lk.n[1] / 2))), dim3(MCS_BLOCK), 0, st,
+                           mg->lv[k + 1].res, (const double *)mg->lv[k].res, Lk);
+    }
+    hipLaunchKernelGGL(k2_coarse, dim3(nblk(mg->ncoarse)), dim3(MCS_BLOCK), 0, st, (const double *)mg->ainv,
+                       (const double *)mg->lv[L].res, mg->lv[L].sol, lv2(mg, L));
+    double *z = mg->lv[L].sol;
+    for (int k = 1; k <= L; ++k) {
+        const int lev = L - k;
+        double *z1 = mg->lv[lev].sol;
+        fill0(mg, z1, mg->lv[lev].len);
+        bc2(mg, lev + 1, z);
+        const Lv2 Ll = lv2(mg, lev);
+        hipLaunchKernelGGL(k2_interp, dim3(nblk((int64_t)Ll.n[0] * Ll.n[1])), dim3(MCS_BLOCK), 0, st, z1,
+                           (const double *)z, Ll, 0);
+        mgv2(mg, z1, mg->lv[lev].res, lev);
+        z = z1;
+    }
+    hipLaunchKernelGGL(k2_axpy, dim3(nblk(mg->lv[0].len)), dim3(MCS_BLOCK), 0, st, x, (const double *)z,
+                       mg->lv[0].len);
+    bc2(mg, 0, x);
+    return hipGetLastError() == hipSuccess ? 1 : -1;
+}
+
+}  // namespace
+
+int32_t mcs_mg2_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg **out)
+{
+    const mcs_mg_config &c = *cfg;
+    if (c.n[2] != 1) return mcs_fail(ctx, MCS_E_ARG, "mg: a 2D solve has n[2] == 1");
+    if (c.levels < 0 || c.levels >= 16 || c.smooth1 < 0 || c.smooth2 < 0 || c.maxiter < 0)
+        return mcs_fail(ctx, MCS_E_ARG, "mg: bad levels/smoothing/maxiter");
+    if (c.cyl < 0 || c.cyl > 2) return mcs_fail(ctx, MCS_E_ARG, "mg: cyl is 0 (Cartesian), 1 or 2");
+    if (c.precision != MCS_PREC_F64) return mcs_fail(ctx, MCS_E_ARG, "mg: the 2D solve is FP64");
+    for (int d = 0; d < 2; ++d)
+        if (c.n[d] < 1 || c.n[d] % (1 << c.levels))
+            return mcs_fail(ctx, MCS_E_ARG, "mg: n[%d]=%d is not divisible by 2^levels", d, c.n[d]);
+    if ((c.n[0] >> c.levels) % 2)
+        return mcs_fail(ctx, MCS_E_ARG, "mg: red-black needs an even x size on every level (redblack.jl:13)");
+    const int64_t nc = (int64_t)(c.n[0] >> c.levels) * (c.n[1] >> c.levels);
+    if (nc > 4096) return mcs_fail(ctx, MCS_E_ARG, "mg: coarsest level above 4096 cells; add levels");
+    hipSetDevice(ctx->device);
+    mcs_mg *mg = new mcs_mg();
+    mg->ctx = ctx;
+    mg->cfg = c;
+    mg->use_graph = false;
+    for (int l = 0; l <= c.levels; ++l) {
+        MgLevel L;
+        memset(&L, 0, sizeof(L));
+        L.n[0] = c.n[0] >> l;
+        L.n[1] = c.n[1] >> l;
+        L.n[2] = 1;
+        L.len = (int64_t)(L.n[0] + 2) * (L.n[1] + 2);
+        if (hipMalloc(&L.res, sizeof(double) * L.len) != hipSuccess ||
+            hipMalloc(&L.res1, sizeof(double) * L.len) != hipSuccess ||
+            hipMalloc(&L.sol, sizeof(double) * L.len) != hipSuccess) {
+            mg->lv.push_back(L);
+            mcs_mg_destroy(mg);
+            return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation failed");
+        }
+        hipMemsetAsync(L.res, 0, sizeof(double) * L.len, ctx->stream);
+        hipMemsetAsync(L.res1, 0, sizeof(double) * L.len, ctx->stream);
+        hipMemsetAsync(L.sol, 0, sizeof(double) * L.len, ctx->stream);
+        mg->lv.push_back(L);
+    }
+    const int32_t ncn[2] = {c.n[0] >> c.levels, c.n[1] >> c.levels};
+    std::vector<double> ainv;
+    if (coarse_inverse2(ncn, c.bc, ainv)) {
+        mcs_mg_destroy(mg);
+        return mcs_fail(ctx, MCS_E_ARG, "mg: singular coarse operator (all-Neumann boundaries?)");
+    }
+    mg->ncoarse = nc;
+    if (hipMalloc(&mg->ainv, sizeof(double) * ainv.size()) != hipSuccess) {
+        mcs_mg_destroy(mg);
+        return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation failed");
+    }
+    hipMemcpy(mg->ainv, ainv.data(), sizeof(double) * ainv.size(), hipMemcpyHostToDevice);
+    mg->npartial = nblk((int64_t)c.n[0] * c.n[1]);
+    if (hipMalloc(&mg->partial, sizeof(double) * (mg->npartial + 8)) != hipSuccess) {
+        mcs_mg_destroy(mg);
+        return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation failed");
+    }
+    MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *out = mg;
+    return MCS_OK;
+}
+
+// solve (Multigrid.jl:449-467) on device arrays with ghosts ((n0+2)(n1+2)).
+int32_t mcs_mg2_solve_dev(mcs_mg *mg, double *u, const double *b, int32_t *iters, double *resid)
+{
+    mcs_ctx *ctx = mg->ctx;
+    const Lv2 F = lv2(mg, 0);
+    mg->tol_abs = mg->cfg.tolerance;
+    if (mg->cfg.tolerance < 0.0) {
+        const int64_t N0 = (int64_t)F.n[0] * F.n[1];
+        hipLaunchKernelGGL(k2_sumsq, dim3(nblk(N0)), dim3(MCS_BLOCK), 0, ctx->stream, b, F, mg->partial);
+        const uint64_t seq = mcs_pub_next(ctx);
+        hipLaunchKernelGGL(k2_sum, dim3(1), dim3(MCS_BLOCK), 0, ctx->stream, (const double *)mg->partial,
+                           (int64_t)nblk(N0), ctx->d_pub, seq);
+        MCS_LAUNCH_CHECK(ctx);
+        double ss = 0.0;
+        int32_t rc = mcs_pub_wait(ctx, seq, &ss, 1);
+        if (rc) return rc;
+        const double brms = sqrt(ss) / sqrt((double)F.n[0] * (double)F.n[1]);
+        mg->tol_abs = -mg->cfg.tolerance * brms;
+        if (!(mg->tol_abs > 0.0)) mg->tol_abs = 2.2250738585072014e-308;
+    }
+    int iter = 0, cont = 1;
+    double eps = 0.0;
+    while (cont && iter <= mg->cfg.maxiter) {
+        cont = fmg2(mg, u, b, &eps);
+        if (cont < 0) return mcs_fail(ctx, MCS_E_HIP, "mg: device error during the cycle");
+        iter += 1;
+    }
+    if (iters) *iters = iter;
+    if (resid) *resid = eps;
+    if (iter > mg->cfg.maxiter)
+        return mcs_fail(ctx, MCS_E_NONCONVERGED, "Convergence failed with [maxiter=] %d iterations",
+                        mg->cfg.maxiter);
+    return MCS_OK;
+}

@@ -1071,10 +1071,326 @@ static int ref_fmg(const ref_mg *cfg, ref_ws *ws, double *x, const double *b, do
     return 1;
 }
 
+/* ----------------------------------------------- multigrid in two dimensions
+ * Multigrid.jl with N = 2: lplstencil (+x, +y, -x, -y), redblack2, cubestencil
+ * children (x offset slowest), interpolation weights prod(3 - 2 sh) / 16, and the
+ * connector (Multigrid.jl:31-39) on every neighbour term of the laplacian and the
+ * residual.  The coarse operator of buildmat stays Cartesian (Multigrid.jl:261-295
+ * takes no connector). */
+
+static inline int64_t gsz2(const int32_t n[2])
+{
+    return (int64_t)(n[0] + 2) * (n[1] + 2);
+}
+
+/* apply! in 2D: faces x-, x+, y-, y+; each ghost line (full extent, ghosts
+ * included) is coef * (adjacent valid line). */
+void ref_mg2_apply_bc(const int32_t n[2], const int32_t bc[4], double *a)
+{
+    int64_t ext[2] = {n[0] + 2, n[1] + 2};
+    int64_t str[2] = {1, n[0] + 2};
+    for (int face = 0; face < 4; ++face) {
+        int d = face / 2, d1 = 1 - d;
+        double coef = bc[face] == 0 ? -1.0 : 1.0;
+        int64_t g = (face & 1) ? n[d] + 1 : 0;
+        int64_t vld = (face & 1) ? n[d] : 1;
+        for (int64_t b1 = 0; b1 < ext[d1]; ++b1)
+            a[b1 * str[d1] + g * str[d]] = coef * a[b1 * str[d1] + vld * str[d]];
+    }
+}
+
+/* CylindricalConnector{D} (Multigrid.jl:37-39): the neighbour at offset sgn along
+ * dimension d of the cell with interior index i (0-based) along d enters as
+ * x * (1.0 + sgn / (2 i + 1)); with the Cartesian connector (and along the other
+ * dimension) the factor is 1.0, which leaves x unchanged. */
+static inline double conn2(int32_t cyl, int d, int sgn, int64_t i)
+{
+    if (cyl != d + 1) return 1.0;
+    return 1.0 + (double)sgn / (double)(2 * i + 1);
+}
+
+static inline double lap2(const int32_t n[2], int32_t cyl, const double *u, int64_t i, int64_t j)
+{
+    int64_t sy = n[0] + 2, c = (i + 1) + sy * (j + 1);
+    double l = -4.0 * u[c];
+    l += u[c + 1] * conn2(cyl, 0, 1, i);
+    l += u[c + sy] * conn2(cyl, 1, 1, j);
+    l += u[c - 1] * conn2(cyl, 0, -1, i);
+    l += u[c - sy] * conn2(cyl, 1, -1, j);
+    return l;
+}
+
+/* gauss_seidel! with redblack2 (redblack.jl:38-45): colour 0 = cells with even
+ * i + j in interior coordinates, then colour 1; u += 1.0 (L u + b) / 4. */
+void ref_mg2_gauss_seidel(const int32_t n[2], int32_t cyl, double *u, const double *b)
+{
+    int64_t sy = n[0] + 2;
+    for (int par = 0; par < 2; ++par)
+        for (int64_t j = 0; j < n[1]; ++j)
+            for (int64_t i = 0; i < n[0]; ++i) {
+                if (((i + j) & 1) != par) continue;
+                int64_t c = (i + 1) + sy * (j + 1);
+                double l = lap2(n, cyl, u, i, j);
+                u[c] += (1.0 * (l + b[c])) / 4.0;
+            }
+}
+
+void ref_mg2_residual(const int32_t n[2], int32_t cyl, double *r, const double *u, const double *b, double s)
+{
+    int64_t sy = n[0] + 2;
+    for (int64_t j = 0; j < n[1]; ++j)
+        for (int64_t i = 0; i < n[0]; ++i) {
+            int64_t c = (i + 1) + sy * (j + 1);
+            r[c] = s * b[c] + lap2(n, cyl, u, i, j);
+        }
+}
+
+/* restrict! in 2D: rh = 4 * (sum of the 4 children) / 4. */
+void ref_mg2_restrict(const int32_t nf[2], double *rh, const double *r)
+{
+    int32_t nc[2] = {nf[0] / 2, nf[1] / 2};
+    int64_t fy = nf[0] + 2, cy = nc[0] + 2;
+    for (int64_t j = 1; j <= nc[1]; ++j)
+        for (int64_t i = 1; i <= nc[0]; ++i) {
+            int64_t f = (2 * i - 1) + fy * (2 * j - 1);
+            double s = 0.0;
+            for (int dx = 0; dx < 2; ++dx)
+                for (int dy = 0; dy < 2; ++dy) s += r[f + dx + fy * dy];
+            rh[i + cy * j] = (4.0 * s) / 4.0;
+        }
+}
+
+/* interpolate! in 2D: weights prod(3 - 2 sh) / 16. */
+void ref_mg2_interpolate(const int32_t nf[2], double *r, const double *rh, int32_t add)
+{
+    int32_t nc[2] = {nf[0] / 2, nf[1] / 2};
+    int64_t fy = nf[0] + 2, cy = nc[0] + 2;
+    for (int64_t j = 1; j <= nc[1]; ++j)
+        for (int64_t i = 1; i <= nc[0]; ++i) {
+            int64_t cc = i + cy * j;
+            for (int sx = 0; sx < 2; ++sx)
+                for (int sy = 0; sy < 2; ++sy) {
+                    int64_t fidx = (2 * i - 1 + sx) + fy * (2 * j - 1 + sy);
+                    int64_t ddx = 2 * sx - 1, ddy = 2 * sy - 1;
+                    double v = 0.0;
+                    for (int hx = 0; hx < 2; ++hx)
+                        for (int hy = 0; hy < 2; ++hy) {
+                            double wgt = (double)((3 - 2 * hx) * (3 - 2 * hy)) / 16.0;
+                            v += wgt * rh[cc + ddx * hx + cy * ddy * hy];
+                        }
+                    if (add)
+                        r[fidx] += v;
+                    else
+                        r[fidx] = v;
+                }
+        }
+}
+
+/* buildmat in 2D (Cartesian 5-point operator, ghost rule folded), Gauss-Jordan. */
+int32_t ref_mg2_coarse_inverse(const int32_t n[2], const int32_t bc[4], double *ainv)
+{
+    int64_t N = (int64_t)n[0] * n[1];
+    double *a = (double *)calloc((size_t)(N * N), sizeof(double));
+    for (int64_t j = 0; j < n[1]; ++j)
+        for (int64_t i = 0; i < n[0]; ++i) {
+            int64_t row = i + n[0] * j;
+            int64_t c[2] = {i, j};
+            a[row * N + row] -= 4.0;
+            for (int q = 0; q < 4; ++q) {   /* +x, +y, -x, -y */
+                int d = q % 2;
+                int sgn = q < 2 ? 1 : -1;
+                int64_t cc[2] = {c[0], c[1]};
+                cc[d] += sgn;
+                if (cc[d] < 0 || cc[d] >= n[d]) {
+                    int face = 2 * d + (sgn > 0 ? 1 : 0);
+                    a[row * N + row] += bc[face] == 0 ? -1.0 : 1.0;
+                } else {
+                    a[row * N + cc[0] + n[0] * cc[1]] += 1.0;
+                }
+            }
+        }
+    for (int64_t r = 0; r < N; ++r)
+        for (int64_t c = 0; c < N; ++c) ainv[r * N + c] = r == c ? 1.0 : 0.0;
+    for (int64_t col = 0; col < N; ++col) {
+        int64_t piv = col;
+        double best = fabs(a[col * N + col]);
+        for (int64_t r = col + 1; r < N; ++r)
+            if (fabs(a[r * N + col]) > best) {
+                best = fabs(a[r * N + col]);
+                piv = r;
+            }
+        if (best == 0.0) {
+            free(a);
+            return 1;
+        }
+        if (piv != col) {
+            for (int64_t c = 0; c < N; ++c) {
+                double t = a[col * N + c];
+                a[col * N + c] = a[piv * N + c];
+                a[piv * N + c] = t;
+                t = ainv[col * N + c];
+                ainv[col * N + c] = ainv[piv * N + c];
+                ainv[piv * N + c] = t;
+            }
+        }
+        double d = a[col * N + col];
+        for (int64_t c = 0; c < N; ++c) {
+            a[col * N + c] = a[col * N + c] / d;
+            ainv[col * N + c] = ainv[col * N + c] / d;
+        }
+        for (int64_t r = 0; r < N; ++r) {
+            if (r == col) continue;
+            double fct = a[r * N + col];
+            if (fct == 0.0) continue;
+            for (int64_t c = 0; c < N; ++c) {
+                a[r * N + c] = a[r * N + c] - fct * a[col * N + c];
+                ainv[r * N + c] = ainv[r * N + c] - fct * ainv[col * N + c];
+            }
+        }
+    }
+    free(a);
+    return 0;
+}
+
+typedef struct {
+    int32_t n[16][2];
+    double *res[16], *res1[16], *sol[16];
+    double *ainv;
+    int64_t ncoarse;
+    double tol_abs;
+} ref_ws2;
+
+static void ref_mgv2(const ref_mg *cfg, ref_ws2 *ws, double *x, const double *b, int level)
+{
+    const int32_t *n = ws->n[level];
+    if (level == cfg->levels) {
+        for (int it = 0; it < 50; ++it) {
+            ref_mg2_apply_bc(n, cfg->bc, x);
+            ref_mg2_gauss_seidel(n, cfg->cyl, x, b);
+        }
+        return;
+    }
+    for (int it = 0; it < cfg->smooth1; ++it) {
+        ref_mg2_apply_bc(n, cfg->bc, x);
+        ref_mg2_gauss_seidel(n, cfg->cyl, x, b);
+    }
+    double *r = ws->res1[level];
+    ref_mg2_apply_bc(n, cfg->bc, x);
+    ref_mg2_residual(n, cfg->cyl, r, x, b, 1.0);
+    double *rh = ws->res[level + 1];
+    ref_mg2_apply_bc(n, cfg->bc, r);
+    ref_mg2_restrict(n, rh, r);
+    double *xh = ws->sol[level + 1];
+    memset(xh, 0, sizeof(double) * (size_t)gsz2(ws->n[level + 1]));
+    ref_mgv2(cfg, ws, xh, rh, level + 1);
+    ref_mg2_interpolate(n, x, xh, 1);
+    for (int it = 0; it < cfg->smooth2; ++it) {
+        ref_mg2_apply_bc(n, cfg->bc, x);
+        ref_mg2_gauss_seidel(n, cfg->cyl, x, b);
+    }
+}
+
+static int ref_fmg2(const ref_mg *cfg, ref_ws2 *ws, double *x, const double *b, double *eps_out)
+{
+    int L = cfg->levels;
+    const int32_t *n = ws->n[0];
+    ref_mg2_apply_bc(n, cfg->bc, x);
+    memset(ws->res[0], 0, sizeof(double) * (size_t)gsz2(n));
+    ref_mg2_residual(n, cfg->cyl, ws->res[0], x, b, cfg->s);
+    double ss = 0.0;
+    int64_t tot = gsz2(n);
+    for (int64_t c = 0; c < tot; ++c) ss += ws->res[0][c] * ws->res[0][c];
+    double eps = sqrt(ss) / sqrt((double)n[0] * (double)n[1]);
+    *eps_out = eps;
+    if (cfg->s * ws->tol_abs > eps) return 0;
+    for (int k = 0; k < L; ++k) ref_mg2_restrict(ws->n[k], ws->res[k + 1], ws->res[k]);
+    {
+        const int32_t *nc = ws->n[L];
+        double *z = ws->sol[L];
+        int64_t N = ws->ncoarse, cy = nc[0] + 2;
+        double *bt = (double *)malloc(sizeof(double) * (size_t)N);
+        for (int64_t j = 0; j < nc[1]; ++j)
+            for (int64_t i = 0; i < nc[0]; ++i) bt[i + nc[0] * j] = ws->res[L][(i + 1) + cy * (j + 1)];
+        for (int64_t j = 0; j < nc[1]; ++j)
+            for (int64_t i = 0; i < nc[0]; ++i) {
+                int64_t row = i + nc[0] * j;
+                double acc = 0.0;
+                for (int64_t c = 0; c < N; ++c) acc += ws->ainv[row * N + c] * bt[c];
+                z[(i + 1) + cy * (j + 1)] = acc;
+            }
+        free(bt);
+    }
+    double *z = ws->sol[L];
+    for (int k = 1; k <= L; ++k) {
+        int lev = L - k;
+        double *z1 = ws->sol[lev];
+        memset(z1, 0, sizeof(double) * (size_t)gsz2(ws->n[lev]));
+        ref_mg2_apply_bc(ws->n[lev + 1], cfg->bc, z);
+        ref_mg2_interpolate(ws->n[lev], z1, z, 0);
+        ref_mgv2(cfg, ws, z1, ws->res[lev], lev);
+        z = z1;
+    }
+    for (int64_t c = 0; c < tot; ++c) x[c] += z[c];
+    ref_mg2_apply_bc(n, cfg->bc, x);
+    return 1;
+}
+
+static int32_t ref_mg2_solve(const ref_mg *cfg, double *u, const double *b, int32_t *iters, double *resid)
+{
+    ref_ws2 ws;
+    memset(&ws, 0, sizeof(ws));
+    if (cfg->levels < 0 || cfg->levels > 15 || cfg->n[2] != 1 || cfg->cyl < 0 || cfg->cyl > 2) return 1;
+    for (int l = 0; l <= cfg->levels; ++l) {
+        for (int d = 0; d < 2; ++d) {
+            if (cfg->n[d] % (1 << l)) return 1;
+            ws.n[l][d] = cfg->n[d] >> l;
+        }
+        int64_t sz = gsz2(ws.n[l]);
+        ws.res[l] = (double *)calloc((size_t)sz, sizeof(double));
+        ws.res1[l] = (double *)calloc((size_t)sz, sizeof(double));
+        ws.sol[l] = (double *)calloc((size_t)sz, sizeof(double));
+    }
+    const int32_t *nc = ws.n[cfg->levels];
+    ws.ncoarse = (int64_t)nc[0] * nc[1];
+    ws.ainv = (double *)malloc(sizeof(double) * (size_t)(ws.ncoarse * ws.ncoarse));
+    int32_t rc = ref_mg2_coarse_inverse(nc, cfg->bc, ws.ainv);
+    ws.tol_abs = cfg->tolerance;
+    if (cfg->tolerance < 0.0) {
+        const int32_t *n0 = ws.n[0];
+        double ss = 0.0;
+        for (int64_t j = 1; j <= n0[1]; ++j)
+            for (int64_t i = 1; i <= n0[0]; ++i) {
+                double v = b[i + (int64_t)(n0[0] + 2) * j];
+                ss += v * v;
+            }
+        double brms = sqrt(ss) / sqrt((double)n0[0] * (double)n0[1]);
+        ws.tol_abs = -cfg->tolerance * brms;
+        if (!(ws.tol_abs > 0.0)) ws.tol_abs = 2.2250738585072014e-308;
+    }
+    int iter = 0;
+    int cont = 1;
+    double eps = 0.0;
+    while (rc == 0 && cont && iter <= cfg->maxiter) {
+        cont = ref_fmg2(cfg, &ws, u, b, &eps);
+        iter += 1;
+    }
+    *iters = iter;
+    *resid = eps;
+    for (int l = 0; l <= cfg->levels; ++l) {
+        free(ws.res[l]);
+        free(ws.res1[l]);
+        free(ws.sol[l]);
+    }
+    free(ws.ainv);
+    if (rc) return 1;
+    return iter > cfg->maxiter ? 6 : 0;
+}
+
 /* solve (Multigrid.jl:449-467): fmg! until converged or iter > maxiter. */
 int32_t ref_mg_solve(const ref_mg *cfg, double *u, const double *b, int32_t *iters,
                      double *resid)
 {
+    if (cfg->ndim == 2) return ref_mg2_solve(cfg, u, b, iters, resid);
     ref_ws ws;
     memset(&ws, 0, sizeof(ws));
     if (cfg->levels < 0 || cfg->levels > 7) return 1;

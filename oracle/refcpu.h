@@ -111,7 +111,17 @@ typedef struct ref_mg {
     int32_t smooth2;
     int32_t maxiter;
     double tolerance;
+    int32_t ndim;        /* 3 (also 0) or 2 (n[2] == 1; bc[0..3]) */
+    int32_t cyl;         /* 0 CartesianConnector; 1 + d: CylindricalConnector{d + 1} */
 } ref_mg;
+
+/* 2D pieces (Multigrid.jl with N = 2), exported for the tests; cyl as in ref_mg */
+void ref_mg2_apply_bc(const int32_t n[2], const int32_t bc[4], double *a);
+void ref_mg2_gauss_seidel(const int32_t n[2], int32_t cyl, double *u, const double *b);
+void ref_mg2_residual(const int32_t n[2], int32_t cyl, double *r, const double *u, const double *b, double s);
+void ref_mg2_restrict(const int32_t nf[2], double *rh, const double *r);
+void ref_mg2_interpolate(const int32_t nf[2], double *r, const double *rh, int32_t add);
+int32_t ref_mg2_coarse_inverse(const int32_t n[2], const int32_t bc[4], double *ainv);
 
 /* elementary pieces, exported for the tests */
 void ref_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

@@ -82,7 +82,7 @@ class RefMG(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int32 * 3), ("bc", ctypes.c_int32 * 6), ("s", ctypes.c_double),
                 ("levels", ctypes.c_int32), ("smooth1", ctypes.c_int32),
                 ("smooth2", ctypes.c_int32), ("maxiter", ctypes.c_int32),
-                ("tolerance", ctypes.c_double)]
+                ("tolerance", ctypes.c_double), ("ndim", ctypes.c_int32), ("cyl", ctypes.c_int32)]
 
 
 _lib = None
@@ -348,11 +348,16 @@ def resample_cells(pop, mesh, ntarget, nmax, seed=0, instance=0, step=0):
     return rc, st
 
 
-def mg_solve(n, bc, s, levels, smooth1, smooth2, tolerance, maxiter, u, b):
+def mg_solve(n, bc, s, levels, smooth1, smooth2, tolerance, maxiter, u, b, cyl=0):
+    """n of length 3 (Cartesian 3D) or 2 (2D; cyl = 1 + d puts the cylindrical
+    connector on dimension d); bc has 2 entries per dimension."""
     cfg = RefMG()
+    nd = len(n)
+    cfg.ndim = nd
+    cfg.cyl = cyl
     for d in range(3):
-        cfg.n[d] = n[d]
-    for f in range(6):
+        cfg.n[d] = n[d] if d < nd else 1
+    for f in range(2 * nd):
         cfg.bc[f] = bc[f]
     cfg.s = s
     cfg.levels = levels
@@ -380,6 +385,23 @@ def mg_op(name, *args):
             conv.append(_d(a))
         elif isinstance(a, np.ndarray) and a.dtype == np.int32:
             conv.append(a.ctypes.data_as(_PI32))
+        else:
+            conv.append(a)
+    return fn(*conv)
+
+
+def mg_op2(name, *args):
+    """Direct access to the 2D multigrid building blocks for tests."""
+    L = lib()
+    fn = getattr(L, "ref_mg2_" + name)
+    conv = []
+    for a in args:
+        if isinstance(a, np.ndarray) and a.dtype == np.float64:
+            conv.append(_d(a))
+        elif isinstance(a, np.ndarray) and a.dtype == np.int32:
+            conv.append(a.ctypes.data_as(_PI32))
+        elif isinstance(a, float):
+            conv.append(ctypes.c_double(a))
         else:
             conv.append(a)
     return fn(*conv)
