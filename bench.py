@@ -441,6 +441,110 @@ def run_precision(args):
             "cpu_baseline": None}
 
 
+BYTES_PER_UPDATE_GS = 32   # per updated cell of a colour pass: read u, b and the neighbours' share, write u
+
+
+def run_mg2d(args):
+    """The reference's axisymmetric Poisson solve (MCStreamer/src/main.jl:130-145:
+    CylindricalConnector{1}, Neumann on the axis, Dirichlet elsewhere, V(2,2),
+    s = dz^2/eps0) on its production grid (R = 1.5 cm, L = 10 cm, 1536 x 10240
+    cells) with a synthetic streamer-head charge; inputs resident on the device.
+    `value` is cell-solves/s (cells x solves per second)."""
+    import ctypes
+    import torch
+    from mcstreamer_amd import _lib
+    from mcstreamer_amd import constants as co
+    from mcstreamer_amd import multigrid as mg
+    from mcstreamer_amd.jumc import context
+
+    ctx = context()
+    M, N = args.mg2d_grid
+    R, L = 0.015, 0.1
+    dr, dz = R / M, L / N
+    levels = args.mg2d_levels
+    rc = (np.arange(M) + 0.5) * dr
+    zc = (np.arange(N) + 0.5) * dz
+    # a space-charge layer at the head of an axial channel (C/m^3)
+    q = 1e3 * np.exp(-(rc[:, None] / 2e-4) ** 2) * np.exp(-((zc[None, :] - 0.06) / 5e-4) ** 2)
+    b = np.zeros((M + 2, N + 2))
+    b[1:-1, 1:-1] = q
+    conf = mg.MGConfig(bc=[(mg.BottomBnd, mg.Neumann), (mg.TopBnd, mg.Dirichlet), (mg.LeftBnd, mg.Dirichlet),
+                           (mg.RightBnd, mg.Dirichlet)], s=dz * dz / co.epsilon_0, levels=levels,
+                       tolerance=-1e-8, smooth1=2, smooth2=2, ndim=2, conn=mg.CylindricalConnector(1))
+    ws = mg.Workspace(conf, (M, N), ctx=ctx)
+    dev = torch.device("cuda", 0)
+    bf = torch.from_numpy(np.ascontiguousarray(b.T).reshape(-1)).to(dev)
+    u0 = torch.zeros_like(bf)
+    u = torch.zeros_like(bf)
+    torch.cuda.synchronize()
+    it = ctypes.c_int32()
+    res = ctypes.c_double()
+
+    def solve():
+        u.copy_(u0)
+        torch.cuda.synchronize()
+        rc_ = _lib.lib().mcs_mg_solve_device(ws.handle, ctypes.c_void_p(u.data_ptr()),
+                                              ctypes.c_void_p(bf.data_ptr()), ctypes.byref(it), ctypes.byref(res))
+        if rc_ != 0:
+            _lib.check(ctx.handle, rc_)
+
+    for _ in range(args.warmup):
+        solve()
+    ctx.synchronize()
+    ctx.profile_reset()
+    ctx.profile(True)
+    t_solve = 0.0
+    iters = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        solve()
+        ctx.synchronize()
+        t_solve += time.perf_counter() - t0
+        iters.append(it.value)
+    ctx.profile(False)
+    sw_ms, sw_n = ctx.profile_read("mg2_sweep0")
+    sweep_s = sw_ms / 1e3 / max(sw_n, 1)
+    cells = float(M) * N
+    bytes_sweep = BYTES_PER_UPDATE_GS * cells   # two colour passes update every cell once
+    ach = bytes_sweep / sweep_s / 1e9 if sweep_s > 0 else 0.0
+    result = {
+        "metric": "cell-solves/s (axisymmetric Poisson, production grid)",
+        "value": cells * args.steps / t_solve,
+        "unit": "cell-solves/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_solve / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"axisymmetric multigrid solve, {M}x{N} cells (R=1.5 cm, L=10 cm), "
+                               f"CylindricalConnector{{1}}, V(2,2), {levels} levels, 1e-8 relative residual, "
+                               "Neumann axis, Dirichlet elsewhere", "grid": [M, N], "levels": levels,
+                   "fmg_iterations": iters, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k2_gs on the finest level (one sweep = two colour passes)",
+                     "avg_launch_ms": sweep_s * 1e3, "bytes_per_launch": bytes_sweep},
+    }
+    if not args.no_cpu_baseline:
+        from oracle import refcpu as Rf
+        # bounded sample: the same problem on a grid 4 x coarser per axis (two levels fewer)
+        m4, n4 = M // 4, N // 4
+        bq = np.zeros((n4 + 2, m4 + 2))
+        bq[1:-1, 1:-1] = q[::4, ::4].T
+        uq = np.zeros_like(bq)
+        t0 = time.perf_counter()
+        rcq, itq, _ = Rf.mg_solve((m4, n4), (mg.Neumann, mg.Dirichlet, mg.Dirichlet, mg.Dirichlet),
+                                  (4 * dz) ** 2 / co.epsilon_0, levels - 2, 2, 2, -1e-8, 50,
+                                  uq.reshape(-1), bq.reshape(-1), cyl=1)
+        el = time.perf_counter() - t0
+        result["cpu_baseline"] = {"value": float(m4) * n4 / el, "unit": "cell-solves/s", "cores": 1,
+                                  "kind": "port",
+                                  "sample": f"one solve of the same problem on {m4}x{n4} cells ({itq} fmg "
+                                            f"iterations), single-threaded restatement, {el:.1f} s"}
+    else:
+        result["cpu_baseline"] = None
+    return result
+
+
 def copy_calibration(nbytes=1 << 30, reps=3):
     """A plain device copy of known size: its FETCH_SIZE/WRITE_SIZE counters give
     the factor that turns the counters of the step's kernels into bytes."""
@@ -459,7 +563,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["streamer", "swarm", "batch", "serve", "precision"],
+    ap.add_argument("--workload", choices=["streamer", "swarm", "batch", "serve", "precision", "mg2d"],
                     default="streamer")
     ap.add_argument("--c2-pairs", type=int, default=40_000_000)
     ap.add_argument("--instances", type=int, default=1024)
@@ -474,6 +578,8 @@ def main():
     ap.add_argument("--precision", choices=["f64", "f32"], default="f32",
                     help="store and step precision of the batched/served swarm workloads")
     ap.add_argument("--mesh", type=int, nargs=3, default=[128, 128, 256])
+    ap.add_argument("--mg2d-grid", type=int, nargs=2, default=[1536, 10240])
+    ap.add_argument("--mg2d-levels", type=int, default=8)
     ap.add_argument("--h", type=float, default=5e-6)
     ap.add_argument("--npairs", type=int, default=1_000_000)
     ap.add_argument("--swarm-n", type=int, default=10_000)
@@ -489,7 +595,7 @@ def main():
               "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
         sys.exit(2)
     res = {"streamer": run_streamer, "swarm": run_swarm, "batch": run_batch,
-           "serve": run_serve, "precision": run_precision}[args.workload](args)
+           "serve": run_serve, "precision": run_precision, "mg2d": run_mg2d}[args.workload](args)
     print(json.dumps(res))
 
 

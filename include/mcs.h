@@ -242,6 +242,8 @@ int32_t mcs_mg_destroy(mcs_mg *mg);
  * return. */
 int32_t mcs_mg_solve_host(mcs_mg *mg, double *u, const double *b, int32_t *iters,
                           double *resid);
+/* Same on device arrays of the context's device (ghost layout as above). */
+int32_t mcs_mg_solve_device(mcs_mg *mg, double *u, const double *b, int32_t *iters, double *resid);
 /* Same, on a mesh: u = mesh U, b = mesh Q (device resident). */
 int32_t mcs_mg_solve_mesh(mcs_mg *mg, mcs_mesh *m, int32_t *iters, double *resid);
 /* setinhom! (boundaries.jl:202-214): b += (2/s) val on the first valid layer of face

@@ -187,6 +187,7 @@ SIGNATURES = {
     "mcs_mg_destroy": [_P],
     "mcs_mg_solve_host": [_P, _PD, _PD, _PI32, _PD],
     "mcs_mg_solve_mesh": [_P, _P, _PI32, _PD],
+    "mcs_mg_solve_device": [_P, _P, _P, _PI32, _PD],
     "mcs_mg_setinhom_mesh": [_P, _P, _I32, _PD],
     "mcs_swarm_batch_create": [_P, _I64, _I32, _U64, _I32, _PP],
     "mcs_swarm_batch_destroy": [_P],

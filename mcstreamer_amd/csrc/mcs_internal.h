@@ -235,6 +235,7 @@ struct mcs_mg {
     int tail_levels = 0;                         // most levels the tail may hold
     int64_t tail_max = 0;                        // largest level (cells) a tail starts from
     int fmg_tail = 1;                            // fmg!'s tail levels in one launch (k_mg_tail_fmg)
+    int tail2 = 0;                               // 2D solver: first level of the one-workgroup V-cycle
     size_t sm_lds = 0;                           // LDS budget of the tiled smoother; 0: one launch per pass
     int sm_tile = 16;                            // its largest tile edge
     int64_t sm_big = 0;                          // levels of this many cells: one sweep per launch

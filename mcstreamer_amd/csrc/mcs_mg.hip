@@ -2190,6 +2190,13 @@ extern "C" int32_t mcs_mg_solve_host(mcs_mg *mg, double *u, const double *b, int
     return rc;
 }
 
+extern "C" int32_t mcs_mg_solve_device(mcs_mg *mg, double *u, const double *b, int32_t *iters, double *resid)
+{
+    if (!mg || !u || !b) return MCS_E_ARG;
+    hipSetDevice(mg->ctx->device);
+    return mg->cfg.ndim == 2 ? mcs_mg2_solve_dev(mg, u, b, iters, resid) : mg_solve_dev(mg, u, b, iters, resid);
+}
+
 extern "C" int32_t mcs_mg_solve_mesh(mcs_mg *mg, mcs_mesh *m, int32_t *iters, double *resid)
 {
     if (!mg || !m) return MCS_E_ARG;

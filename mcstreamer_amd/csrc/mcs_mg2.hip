@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -34,14 +35,24 @@ __device__ __forceinline__ double conn2(int32_t cyl, int d, int sgn, int64_t i)
     return 1.0 + (double)sgn / (double)(2 * i + 1);
 }
 
-__device__ __forceinline__ double lap2(const double *u, const Lv2 &L, int32_t cyl, int64_t i, int64_t j, int64_t c)
+// The laplacian with apply!'s ghost values folded in: a neighbour beyond face f
+// is the cell itself times the face's factor, which is what the ghost holds
+// when apply! ran since the cell last changed (the cell is not updated between
+// apply! and its own update, nor before the residual).
+__device__ __forceinline__ double lap2(const double *u, const Lv2 &L, int32_t cyl, uint32_t neg, int64_t i,
+                                       int64_t j, int64_t c)
 {
     const int64_t sy = L.n[0] + 2;
-    double l = -4.0 * u[c];
-    l += u[c + 1] * conn2(cyl, 0, 1, i);
-    l += u[c + sy] * conn2(cyl, 1, 1, j);
-    l += u[c - 1] * conn2(cyl, 0, -1, i);
-    l += u[c - sy] * conn2(cyl, 1, -1, j);
+    const double uc = u[c];
+    const double vxp = i + 1 < L.n[0] ? u[c + 1] : ((neg >> 1) & 1u ? -uc : uc);
+    const double vyp = j + 1 < L.n[1] ? u[c + sy] : ((neg >> 3) & 1u ? -uc : uc);
+    const double vxm = i > 0 ? u[c - 1] : (neg & 1u ? -uc : uc);
+    const double vym = j > 0 ? u[c - sy] : ((neg >> 2) & 1u ? -uc : uc);
+    double l = -4.0 * uc;
+    l += vxp * conn2(cyl, 0, 1, i);
+    l += vyp * conn2(cyl, 1, 1, j);
+    l += vxm * conn2(cyl, 0, -1, i);
+    l += vym * conn2(cyl, 1, -1, j);
     return l;
 }
 
@@ -67,21 +78,21 @@ __global__ void k2_bc(double *a, Lv2 L, uint32_t neg)
 }
 
 // One colour of gauss_seidel! (redblack2: colour par = cells with i + j % 2 == par).
-__global__ void k2_gs(double *u, const double *b, Lv2 L, int32_t cyl, int par)
+__global__ void k2_gs(double *u, const double *b, Lv2 L, int32_t cyl, uint32_t neg, int par)
 {
     const int64_t t = gidx();
     const int64_t nxh = L.n[0] / 2;
     if (t >= nxh * L.n[1]) return;
     const int64_t j = t / nxh, i = 2 * (t - j * nxh) + ((j + par) & 1);
     const int64_t c = g2(L, i, j);
-    const double l = lap2(u, L, cyl, i, j, c);
+    const double l = lap2(u, L, cyl, neg, i, j, c);
     u[c] += (1.0 * (l + b[c])) / 4.0;
 }
 
 // residual! (r = s b + L u on the interior) with per-block sums of r^2 when
 // partial is given (thread order, then a fixed tree).
 __global__ __launch_bounds__(MCS_BLOCK) void k2_resid(double *r, const double *u, const double *b, double s, Lv2 L,
-                                                      int32_t cyl, double *partial)
+                                                      int32_t cyl, uint32_t neg, double *partial)
 {
     __shared__ double sh[MCS_BLOCK];
     const int64_t t = gidx();
@@ -89,7 +100,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k2_resid(double *r, const double *u
     if (t < (int64_t)L.n[0] * L.n[1]) {
         const int64_t j = t / L.n[0], i = t - j * L.n[0];
         const int64_t c = g2(L, i, j);
-        const double v = s * b[c] + lap2(u, L, cyl, i, j, c);
+        const double v = s * b[c] + lap2(u, L, cyl, neg, i, j, c);
         r[c] = v;
         q = v * v;
     }
@@ -208,6 +219,160 @@ __global__ void k2_axpy(double *x, const double *z, int64_t n)
     if (t < n) x[t] += z[t];
 }
 
+// ---- the coarse levels in one workgroup ---------------------------------------
+
+#define MG2_TAIL_THREADS 512
+#define MG2_MAXL 16
+
+struct Tail2 {
+    Lv2 n[MG2_MAXL];
+    int64_t len[MG2_MAXL];
+    int64_t off_x[MG2_MAXL], off_b[MG2_MAXL];   // LDS offsets (doubles) of the levels' x and b
+    int64_t off_r;                              // fine residual scratch (start level, ghost layout)
+    int L, smooth1, smooth2, cyl;
+    uint32_t neg;
+};
+
+#define TLOOP(t, N) for (int64_t t = threadIdx.x; t < (N); t += blockDim.x)
+
+__device__ __forceinline__ Lv2 t2lvl(const Tail2 &T, int l)
+{
+    Lv2 L;
+    L.n[0] = T.n[l].n[0];
+    L.n[1] = T.n[l].n[1];
+    return L;
+}
+
+__device__ __forceinline__ void t2_gs(double *u, const double *b, const Lv2 &L, const Tail2 &T, int par)
+{
+    const int64_t nxh = L.n[0] / 2;
+    TLOOP(t, nxh * L.n[1])
+    {
+        const int64_t j = t / nxh, i = 2 * (t - j * nxh) + ((j + par) & 1);
+        const int64_t c = g2(L, i, j);
+        const double l = lap2(u, L, T.cyl, T.neg, i, j, c);
+        u[c] += (1.0 * (l + b[c])) / 4.0;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void t2_bc(double *a, const Lv2 &L, uint32_t neg)
+{
+    const int64_t sx = L.n[0] + 2, tot = sx * (L.n[1] + 2);
+    TLOOP(t, tot)
+    {
+        const int64_t i = t % sx, j = t / sx;
+        const bool xl = i == 0, xh = i == L.n[0] + 1, yl = j == 0, yh = j == L.n[1] + 1;
+        if (xl || xh || yl || yh) {
+            const int64_t vi = xl ? 1 : (xh ? L.n[0] : i), vj = yl ? 1 : (yh ? L.n[1] : j);
+            uint32_t flip = 0;
+            if (xl) flip ^= neg & 1u;
+            if (xh) flip ^= (neg >> 1) & 1u;
+            if (yl) flip ^= (neg >> 2) & 1u;
+            if (yh) flip ^= (neg >> 3) & 1u;
+            const double v = a[vi + sx * vj];
+            a[t] = flip ? -v : v;
+        }
+    }
+    __syncthreads();
+}
+
+// mgv! from level `start` (x and b of that level in LDS), as mgv2 does it.
+__device__ void t2_vcycle(const Tail2 &T, double *lds, int start)
+{
+    double *r = lds + T.off_r;
+    for (int lev = start; lev < T.L; ++lev) {
+        double *x = lds + T.off_x[lev];
+        const double *b = lds + T.off_b[lev];
+        const Lv2 L = t2lvl(T, lev);
+        for (int it = 0; it < T.smooth1; ++it) {
+            t2_gs(x, b, L, T, 0);
+            t2_gs(x, b, L, T, 1);
+        }
+        if (T.smooth2 == 0) t2_bc(x, L, T.neg);
+        TLOOP(t, (int64_t)L.n[0] * L.n[1])
+        {
+            const int64_t j = t / L.n[0], i = t - j * L.n[0];
+            const int64_t c = g2(L, i, j);
+            r[c] = 1.0 * b[c] + lap2(x, L, T.cyl, T.neg, i, j, c);
+        }
+        __syncthreads();
+        const Lv2 C = t2lvl(T, lev + 1);
+        double *bh = lds + T.off_b[lev + 1], *xh = lds + T.off_x[lev + 1];
+        TLOOP(t, T.len[lev + 1]) xh[t] = 0.0;
+        TLOOP(t, (int64_t)C.n[0] * C.n[1])
+        {
+            const int64_t jc = t / C.n[0], ic = t - jc * C.n[0];
+            const int64_t fy = L.n[0] + 2;
+            const int64_t f = (2 * ic + 1) + fy * (2 * jc + 1);
+            double sum = 0.0;
+            sum += r[f];
+            sum += r[f + fy];
+            sum += r[f + 1];
+            sum += r[f + 1 + fy];
+            bh[g2(C, ic, jc)] = (4.0 * sum) / 4.0;
+        }
+        __syncthreads();
+    }
+    {
+        double *x = lds + T.off_x[T.L];
+        const double *b = lds + T.off_b[T.L];
+        const Lv2 L = t2lvl(T, T.L);
+        for (int it = 0; it < 50; ++it) {
+            if (it == 49) t2_bc(x, L, T.neg);
+            t2_gs(x, b, L, T, 0);
+            t2_gs(x, b, L, T, 1);
+        }
+    }
+    for (int lev = T.L - 1; lev >= start; --lev) {
+        double *x = lds + T.off_x[lev];
+        const double *b = lds + T.off_b[lev];
+        const double *xh = lds + T.off_x[lev + 1];
+        const Lv2 L = t2lvl(T, lev);
+        TLOOP(t, (int64_t)L.n[0] * L.n[1])
+        {
+            const int64_t j = t / L.n[0], i = t - j * L.n[0];
+            const int64_t cy = L.n[0] / 2 + 2;
+            const int64_t ic = i / 2, jc = j / 2, sx = i & 1, sy = j & 1;
+            const int64_t cc = (ic + 1) + cy * (jc + 1);
+            const int64_t ddx = 2 * sx - 1, ddy = 2 * sy - 1;
+            double v = 0.0;
+#pragma unroll
+            for (int hx = 0; hx < 2; ++hx)
+#pragma unroll
+                for (int hy = 0; hy < 2; ++hy) {
+                    const double w = (double)((3 - 2 * hx) * (3 - 2 * hy)) / 16.0;
+                    v += w * xh[cc + ddx * hx + cy * ddy * hy];
+                }
+            const int64_t c = g2(L, i, j);
+            x[c] = x[c] + v;
+        }
+        __syncthreads();
+        for (int it = 0; it < T.smooth2; ++it) {
+            if (it == T.smooth2 - 1) t2_bc(x, L, T.neg);
+            t2_gs(x, b, L, T, 0);
+            t2_gs(x, b, L, T, 1);
+        }
+    }
+}
+
+// mgv! of levels start..L in one workgroup: b0 and x0 (or zero) loaded, x0 (ghost
+// shell included, as the cycle leaves it) written back.
+__global__ __launch_bounds__(MG2_TAIL_THREADS) void k2_tail(Tail2 T, int start, double *x0, const double *b0,
+                                                            int zero)
+{
+    extern __shared__ double lds[];
+    double *x = lds + T.off_x[start], *b = lds + T.off_b[start];
+    TLOOP(t, T.len[start])
+    {
+        b[t] = b0[t];
+        x[t] = zero ? 0.0 : x0[t];
+    }
+    __syncthreads();
+    t2_vcycle(T, lds, start);
+    TLOOP(t, T.len[start]) x0[t] = x[t];
+}
+
 inline unsigned nblk(int64_t n)
 {
     return (unsigned)((n + MCS_BLOCK - 1) / MCS_BLOCK);
@@ -296,8 +461,11 @@ void gs2(mcs_mg *mg, int l, double *u, const double *b)
 {
     const Lv2 L = lv2(mg, l);
     const int64_t cnt = (int64_t)(L.n[0] / 2) * L.n[1];
+    // the finest level's sweeps are timed on their own (one sweep per record)
+    ProfScope ps(mg->ctx, l == 0 ? "mg2_sweep0" : "mg2_sweep");
     for (int par = 0; par < 2; ++par)
-        hipLaunchKernelGGL(k2_gs, dim3(nblk(cnt)), dim3(MCS_BLOCK), 0, mg->ctx->stream, u, b, L, mg->cfg.cyl, par);
+        hipLaunchKernelGGL(k2_gs, dim3(nblk(cnt)), dim3(MCS_BLOCK), 0, mg->ctx->stream, u, b, L, mg->cfg.cyl,
+                           neg2(mg), par);
 }
 
 void fill0(mcs_mg *mg, double *a, int64_t n)
@@ -305,37 +473,88 @@ void fill0(mcs_mg *mg, double *a, int64_t n)
     hipLaunchKernelGGL(k2_fill0, dim3(nblk(n)), dim3(MCS_BLOCK), 0, mg->ctx->stream, a, n);
 }
 
-// mgv! (Multigrid.jl:334-377).  apply!(r) before restrict! is left out: the
-// restriction reads interior cells only.
-void mgv2(mcs_mg *mg, double *x, const double *b, int level)
+int64_t tail2_setup(const mcs_mg *mg, int start, Tail2 &T)
+{
+    memset(&T, 0, sizeof(T));
+    T.L = mg->cfg.levels;
+    T.smooth1 = mg->cfg.smooth1;
+    T.smooth2 = mg->cfg.smooth2;
+    T.cyl = mg->cfg.cyl;
+    T.neg = neg2(mg);
+    int64_t off = 0;
+    for (int l = 0; l <= T.L; ++l) {
+        T.n[l] = lv2(mg, l);
+        T.len[l] = mg->lv[l].len;
+        if (l >= start) {
+            T.off_x[l] = off;
+            off += T.len[l];
+            T.off_b[l] = off;
+            off += T.len[l];
+        }
+    }
+    T.off_r = off;
+    off += mg->lv[start].len;
+    return off;   // doubles
+}
+
+// The first level from which a V-cycle runs in one workgroup (its levels fit the
+// LDS budget), or levels + 1 when none does.
+int tail2_start(const mcs_mg *mg)
+{
+    if (mg->tail_lds == 0) return mg->cfg.levels + 1;
+    int S = mg->cfg.levels + 1;
+    for (int l = mg->cfg.levels; l >= 0; --l) {
+        Tail2 T;
+        if ((size_t)tail2_setup(mg, l, T) * sizeof(double) > mg->tail_lds) break;
+        S = l;
+    }
+    return S;
+}
+
+void tail2(mcs_mg *mg, int level, double *x, const double *b, bool zero)
+{
+    Tail2 T;
+    const int64_t words = tail2_setup(mg, level, T);
+    hipLaunchKernelGGL(k2_tail, dim3(1), dim3(MG2_TAIL_THREADS), (size_t)words * sizeof(double), mg->ctx->stream, T,
+                       level, x, b, zero ? 1 : 0);
+}
+
+// mgv! (Multigrid.jl:334-377).  The sweeps and the residual fold apply! in
+// (lap2); the ghost shell is written where a later step reads it: before the
+// last sweep (the coarsest level's 50th, the last post-smoothing one), whose
+// values the caller's interpolation reads, as the reference leaves them; without
+// post-smoothing, before the residual.  apply!(r) before restrict! is left out:
+// the restriction reads interior cells only.
+void mgv2(mcs_mg *mg, double *x, const double *b, int level, bool zero = false)
 {
     hipStream_t st = mg->ctx->stream;
+    if (level >= mg->tail2) {
+        tail2(mg, level, x, b, zero);
+        return;
+    }
+    if (zero) fill0(mg, x, mg->lv[level].len);
     if (level == mg->cfg.levels) {
         for (int it = 0; it < 50; ++it) {
-            bc2(mg, level, x);
+            if (it == 49) bc2(mg, level, x);
             gs2(mg, level, x, b);
         }
         return;
     }
-    for (int it = 0; it < mg->cfg.smooth1; ++it) {
-        bc2(mg, level, x);
-        gs2(mg, level, x, b);
-    }
+    for (int it = 0; it < mg->cfg.smooth1; ++it) gs2(mg, level, x, b);
     const Lv2 L = lv2(mg, level);
     double *r = mg->lv[level].res1;
-    bc2(mg, level, x);
+    if (mg->cfg.smooth2 == 0) bc2(mg, level, x);
     hipLaunchKernelGGL(k2_resid, dim3(nblk((int64_t)L.n[0] * L.n[1])), dim3(MCS_BLOCK), 0, st, r, (const double *)x,
-                       b, 1.0, L, mg->cfg.cyl, (double *)nullptr);
+                       b, 1.0, L, mg->cfg.cyl, neg2(mg), (double *)nullptr);
     double *rh = mg->lv[level + 1].res;
     hipLaunchKernelGGL(k2_restrict, dim3(nblk((int64_t)(L.n[0] / 2) * (L.n[1] / 2))), dim3(MCS_BLOCK), 0, st, rh,
                        (const double *)r, L);
     double *xh = mg->lv[level + 1].sol;
-    fill0(mg, xh, mg->lv[level + 1].len);
-    mgv2(mg, xh, rh, level + 1);
+    mgv2(mg, xh, rh, level + 1, true);
     hipLaunchKernelGGL(k2_interp, dim3(nblk((int64_t)L.n[0] * L.n[1])), dim3(MCS_BLOCK), 0, st, x,
                        (const double *)xh, L, 1);
     for (int it = 0; it < mg->cfg.smooth2; ++it) {
-        bc2(mg, level, x);
+        if (it == mg->cfg.smooth2 - 1) bc2(mg, level, x);
         gs2(mg, level, x, b);
     }
 }
@@ -349,10 +568,11 @@ int fmg2(mcs_mg *mg, double *x, const double *b, double *eps_out)
     const int L = mg->cfg.levels;
     const Lv2 F = lv2(mg, 0);
     const int64_t N0 = (int64_t)F.n[0] * F.n[1];
-    bc2(mg, 0, x);
+    // apply!(x) folded into the residual; x's ghost shell is already what apply!
+    // writes (the previous correction, or the solve's start, left it)
     fill0(mg, mg->lv[0].res, mg->lv[0].len);
     hipLaunchKernelGGL(k2_resid, dim3(nblk(N0)), dim3(MCS_BLOCK), 0, st, mg->lv[0].res, (const double *)x, b,
-                       mg->cfg.s, F, mg->cfg.cyl, mg->partial);
+                       mg->cfg.s, F, mg->cfg.cyl, neg2(mg), mg->partial);
     const uint64_t seq = mcs_pub_next(ctx);
     hipLaunchKernelGGL(k2_sum, dim3(1), dim3(MCS_BLOCK), 0, st, (const double *)mg->partial, (int64_t)nblk(N0),
                        ctx->d_pub, seq);
@@ -444,6 +664,16 @@ int32_t mcs_mg2_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg **out)
         mcs_mg_destroy(mg);
         return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation failed");
     }
+    int lds_max = 0;
+    if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) != hipSuccess)
+        lds_max = 64 * 1024;
+    mg->tail_lds = (size_t)lds_max;
+    if (mg->tail_lds > 64 * 1024 &&
+        hipFuncSetAttribute((const void *)k2_tail, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess)
+        mg->tail_lds = 64 * 1024;
+    const char *te = getenv("MCS_MG2_TAIL");
+    if (te && atoi(te) == 0) mg->tail_lds = 0;
+    mg->tail2 = tail2_start(mg);
     MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *out = mg;
     return MCS_OK;
@@ -471,6 +701,7 @@ int32_t mcs_mg2_solve_dev(mcs_mg *mg, double *u, const double *b, int32_t *iters
     }
     int iter = 0, cont = 1;
     double eps = 0.0;
+    bc2(mg, 0, u);   // fmg!'s first apply!(x) (the later ones follow each x += z)
     while (cont && iter <= mg->cfg.maxiter) {
         cont = fmg2(mg, u, b, &eps);
         if (cont < 0) return mcs_fail(ctx, MCS_E_HIP, "mg: device error during the cycle");

@@ -15,16 +15,21 @@ pytestmark = pytest.mark.gpu
 D, N = 0, 1
 
 
+@pytest.mark.parametrize("tail", ["1", "0"])
 @pytest.mark.parametrize("n,bc,levels,smooth,cyl,tol", [
     ((32, 64), (N, D, D, D), 3, (2, 2), 1, -1e-10),
     ((64, 32), (N, N, D, N), 4, (2, 2), 0, -1e-10),
     ((48, 80), (D, N, N, D), 3, (1, 2), 2, -1e-9),
     ((256, 512), (N, D, D, D), 6, (2, 2), 1, -1e-10),
     ((128, 128), (N, D, D, D), 5, (2, 1), 1, 1e-6),
+    ((96, 64), (N, D, D, D), 3, (2, 0), 1, -1e-8),
 ])
-def test_solve_2d_matches_restatement(n, bc, levels, smooth, cyl, tol):
+def test_solve_2d_matches_restatement(monkeypatch, n, bc, levels, smooth, cyl, tol, tail):
+    """(tail "1": the coarse levels of every V-cycle in one workgroup; "0": one
+    launch per phase on every level.)"""
     from mcstreamer_amd import _lib
     from mcstreamer_amd import multigrid as mg
+    monkeypatch.setenv("MCS_MG2_TAIL", tail)
     rng = np.random.default_rng(17 + sum(n))
     g = (n[0] + 2) * (n[1] + 2)
     b = rng.normal(size=g)
