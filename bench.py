@@ -526,20 +526,23 @@ def run_mg2d(args):
     }
     if not args.no_cpu_baseline:
         from oracle import refcpu as Rf
-        # bounded sample: the same problem on a grid 4 x coarser per axis (two levels fewer)
-        m4, n4 = M // 4, N // 4
-        bq = np.zeros((n4 + 2, m4 + 2))
-        bq[1:-1, 1:-1] = q[::4, ::4].T
-        uq = np.zeros_like(bq)
+        # bounded sample: whole solves of the same problem, one after another, until
+        # at least 10 s of single-threaded work
+        bq = np.ascontiguousarray(b.T).reshape(-1)
+        nsol, itq = 0, []
         t0 = time.perf_counter()
-        rcq, itq, _ = Rf.mg_solve((m4, n4), (mg.Neumann, mg.Dirichlet, mg.Dirichlet, mg.Dirichlet),
-                                  (4 * dz) ** 2 / co.epsilon_0, levels - 2, 2, 2, -1e-8, 50,
-                                  uq.reshape(-1), bq.reshape(-1), cyl=1)
-        el = time.perf_counter() - t0
-        result["cpu_baseline"] = {"value": float(m4) * n4 / el, "unit": "cell-solves/s", "cores": 1,
+        el = 0.0
+        while nsol == 0 or (el < 10.0 and nsol < 20):
+            uq = np.zeros_like(bq)
+            rcq, iq, _ = Rf.mg_solve((M, N), (mg.Neumann, mg.Dirichlet, mg.Dirichlet, mg.Dirichlet),
+                                     dz * dz / co.epsilon_0, levels, 2, 2, -1e-8, 50, uq, bq, cyl=1)
+            itq.append(iq)
+            nsol += 1
+            el = time.perf_counter() - t0
+        result["cpu_baseline"] = {"value": cells * nsol / el, "unit": "cell-solves/s", "cores": 1,
                                   "kind": "port",
-                                  "sample": f"one solve of the same problem on {m4}x{n4} cells ({itq} fmg "
-                                            f"iterations), single-threaded restatement, {el:.1f} s"}
+                                  "sample": f"{nsol} solves of the same problem ({itq[0]} fmg iterations each), "
+                                            f"single-threaded restatement, {el:.1f} s"}
     else:
         result["cpu_baseline"] = None
     return result

@@ -139,7 +139,17 @@ __global__ __launch_bounds__(MCS_BLOCK) void k2_sum(const double *partial, int64
 {
     __shared__ double sh[MCS_BLOCK];
     double acc = 0.0;
-    for (int64_t q = threadIdx.x; q < nb; q += blockDim.x) acc += partial[q];
+    const int64_t bd = blockDim.x;
+    int64_t q = threadIdx.x;
+    // eight loads in flight, added in the order of the plain loop
+    for (; q + 7 * bd < nb; q += 8 * bd) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = partial[q + k * bd];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc += v[k];
+    }
+    for (; q < nb; q += bd) acc += partial[q];
     sh[threadIdx.x] = acc;
     __syncthreads();
     for (int o = MCS_BLOCK / 2; o > 0; o >>= 1) {
@@ -192,17 +202,30 @@ __global__ void k2_interp(double *r, const double *rh, Lv2 F, int add)
     r[c] = add ? r[c] + v : v;
 }
 
-// z = A^-1 btop on the coarsest interior, one row per thread, columns in order.
-__global__ void k2_coarse(const double *ainv, const double *res, double *z, Lv2 C)
+// z = A^-1 btop on the coarsest interior (at most 4096 cells), one row per
+// thread of one block, columns in order; btop staged in LDS.
+__global__ __launch_bounds__(MCS_BLOCK) void k2_coarse(const double *ainv, const double *res, double *z, Lv2 C)
 {
-    const int64_t N = (int64_t)C.n[0] * C.n[1];
-    for (int64_t row = gidx(); row < N; row += (int64_t)gridDim.x * blockDim.x) {
+    __shared__ double bt[4096];
+    const int N = C.n[0] * C.n[1];
+    for (int col = threadIdx.x; col < N; col += blockDim.x) {
+        const int jc = col / C.n[0], ic = col - jc * C.n[0];
+        bt[col] = res[g2(C, ic, jc)];
+    }
+    __syncthreads();
+    for (int row = threadIdx.x; row < N; row += blockDim.x) {
+        const double *a = ainv + (int64_t)row * N;
         double acc = 0.0;
-        for (int64_t col = 0; col < N; ++col) {
-            const int64_t jc = col / C.n[0], ic = col - jc * C.n[0];
-            acc += ainv[row * N + col] * res[g2(C, ic, jc)];
+        int col = 0;
+        for (; col + 8 <= N; col += 8) {
+            double av[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) av[k] = a[col + k];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) acc += av[k] * bt[col + k];
         }
-        const int64_t j = row / C.n[0], i = row - j * C.n[0];
+        for (; col < N; ++col) acc += a[col] * bt[col];
+        const int j = row / C.n[0], i = row - j * C.n[0];
         z[g2(C, i, j)] = acc;
     }
 }
@@ -586,7 +609,7 @@ int fmg2(mcs_mg *mg, double *x, const double *b, double *eps_out)
         hipLaunchKernelGGL(k2_restrict, dim3(nblk((int64_t)(Lk.n[0] / 2) * (Lk.n[1] / 2))), dim3(MCS_BLOCK), 0, st,
                            mg->lv[k + 1].res, (const double *)mg->lv[k].res, Lk);
     }
-    hipLaunchKernelGGL(k2_coarse, dim3(nblk(mg->ncoarse)), dim3(MCS_BLOCK), 0, st, (const double *)mg->ainv,
+    hipLaunchKernelGGL(k2_coarse, dim3(1), dim3(MCS_BLOCK), 0, st, (const double *)mg->ainv,
                        (const double *)mg->lv[L].res, mg->lv[L].sol, lv2(mg, L));
     double *z = mg->lv[L].sol;
     for (int k = 1; k <= L; ++k) {
