@@ -86,6 +86,7 @@ class Workspace:
 
     def __init__(self, conf, n, ctx=None):
         self.ctx = ctx or context()
+        self.conf = conf
         self.n = tuple(int(v) for v in n)
         cfg = conf.cstruct(self.n)
         h = ctypes.c_void_p()

@@ -53,3 +53,17 @@ def test_fifo_latency_model():
     lat = fifo_latencies(reqs, [1.0, 1.0, 1.0, 1.0])
     np.testing.assert_allclose(lat, [1.0, 1.5, 2.4, 1.0])
     assert percentiles(lat)["p50"] == pytest.approx(1.25)
+
+
+def test_checkpoint_rejects_foreign_files(tmp_path):
+    """A checkpoint file of another format is refused before anything is built."""
+    import json
+
+    import numpy as np
+    import pytest
+
+    from mcstreamer_amd import checkpoint
+    path = str(tmp_path / "other.npz")
+    np.savez(path, meta=np.array(json.dumps({"format": "something else"})))
+    with pytest.raises(ValueError):
+        checkpoint.load(path)
