@@ -48,7 +48,7 @@ extern "C" {
 #define MCS_EGRID_LOG 1     /* log-spaced nodes, rate linear in energy between nodes */
 
 /* Field descriptor kinds: Swarm.FixedField (Swarm/src/Swarm.jl:28-32) and
- * MCStreamer.FieldInterp (MCStreamer/src/gridfields.jl:189-290). */
+ * MCStreamer.FieldInterp (MCStreamer/src/gridfields.jl:86-190). */
 #define MCS_FIELD_UNIFORM 0
 #define MCS_FIELD_MESH 1
 
@@ -66,7 +66,7 @@ extern "C" {
 #define MCS_DEPOSIT_NGP 0 /* MCStreamer/src/poisson.jl:65-84 */
 #define MCS_DEPOSIT_CIC 1 /* trilinear cloud-in-cell */
 
-/* Multigrid boundary conditions (Multigrid/src/boundaries.jl:161-169). */
+/* Multigrid boundary conditions (Multigrid/src/boundaries.jl:100-108). */
 #define MCS_BC_DIRICHLET 0
 #define MCS_BC_NEUMANN 1
 
@@ -183,10 +183,10 @@ int32_t mcs_pop_size(mcs_pop *p, int64_t *n);
 int32_t mcs_pop_moments(mcs_pop *p, const mcs_table *t, mcs_moments *out);
 
 /* ---- the fused step: JuMC.advance! (JuMC/src/mixed_population.jl:24-54) ---
- * Yoshida push (electron.jl:51-62), null-collision sampling (collisions.jl:236-269),
+ * Yoshida push (electron.jl:51-62), null-collision sampling (collisions.jl:159-192),
  * collision kinematics (electron.jl:108-139) and births/deaths (collisions.jl:173-192,
  * population.jl:88-106) in one kernel.  tracker = NULL is VoidCollisionTracker; a mesh
- * adds +-w/dV ion charge to its QFIXED array (gridfields.jl:298-324). */
+ * adds +-w/dV ion charge to its QFIXED array (gridfields.jl:193-224). */
 int32_t mcs_advance(mcs_pop *p, const mcs_table *t, const mcs_field *field, double dt,
                     mcs_mesh *tracker, uint64_t seed, uint32_t instance, uint32_t step,
                     mcs_step_stats *stats);
@@ -200,7 +200,7 @@ int32_t mcs_swarm_resample(mcs_pop *p, int64_t n0, double tol, uint64_t seed,
                            uint32_t instance, uint32_t step, int32_t *did);
 
 /* ---- mesh: 3D Cartesian generalisation of MCStreamer.Grid/GridFields
- * (MCStreamer/src/grid.jl:4-100, gridfields.jl:107-184); origin at 0, spacing h. */
+ * (MCStreamer/src/grid.jl:4-100, gridfields.jl:1-84); origin at 0, spacing h. */
 int32_t mcs_mesh_create(mcs_ctx *ctx, const int32_t n[3], double h, mcs_mesh **out);
 int32_t mcs_mesh_destroy(mcs_mesh *m);
 int32_t mcs_mesh_set(mcs_mesh *m, int32_t which, const double *host);
@@ -246,7 +246,7 @@ int32_t mcs_mg_solve_host(mcs_mg *mg, double *u, const double *b, int32_t *iters
 int32_t mcs_mg_solve_device(mcs_mg *mg, double *u, const double *b, int32_t *iters, double *resid);
 /* Same, on a mesh: u = mesh U, b = mesh Q (device resident). */
 int32_t mcs_mg_solve_mesh(mcs_mg *mg, mcs_mesh *m, int32_t *iters, double *resid);
-/* setinhom! (boundaries.jl:202-214): b += (2/s) val on the first valid layer of face
+/* setinhom! (boundaries.jl:141-153): b += (2/s) val on the first valid layer of face
  * `face` (0..5); val has the face's interior size, first index fastest. */
 int32_t mcs_mg_setinhom_mesh(mcs_mg *mg, mcs_mesh *m, int32_t face, const double *val);
 

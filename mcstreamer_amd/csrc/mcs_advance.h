@@ -5,7 +5,7 @@
 
 #include "mcs_device.h"
 
-// track1 (MCStreamer/src/gridfields.jl:303-313): val/dV into the NGP cell.
+// track1 (MCStreamer/src/gridfields.jl:203-213): val/dV into the NGP cell.
 __device__ __forceinline__ void track_dev(const MeshView &M, const double x[3], double val)
 {
     int64_t c[3];
@@ -23,7 +23,7 @@ __device__ __forceinline__ void track_dev(const MeshView &M, const double x[3], 
 // One particle through one time step dt (JuMC/src/mixed_population.jl:31-51):
 // free flights to the next (null) collision, the Yoshida push of
 // electron.jl:51-62 with the field evaluated three times per flight, the
-// sampler of collisions.jl:236-269 and the outcomes of electron.jl:108-139.
+// sampler of collisions.jl:159-192 and the outcomes of electron.jl:108-139.
 // `birth(x, v, s, active)` receives each ionization child (weight w), `event(j)`
 // each selected table row.  A particle removed by attachment keeps flying
 // until dt is used up (the loop does not test `active`).

@@ -101,7 +101,7 @@ __device__ __forceinline__ void indweight_dev(const double *energy, int32_t ne, 
     }
 }
 
-// normdivrem + staggered (MCStreamer/src/gridfields.jl:194-209) for one axis;
+// normdivrem + staggered (MCStreamer/src/gridfields.jl:94-109) for one axis;
 // normdivrem is @fastmath there, x * (1/h) stands for x / h.  Indices are 32-bit:
 // every mesh array has fewer than 2^31 elements (checked at creation), and xc is
 // inside (0, n h) here.
@@ -138,7 +138,7 @@ __device__ __forceinline__ double trilerp_dev(const double *__restrict__ a, int6
     return c0 * (1.0 - dz) + c1 * dz;
 }
 
-// FieldInterp (MCStreamer/src/gridfields.jl:215-290) in 3D Cartesian form: zero
+// FieldInterp (MCStreamer/src/gridfields.jl:115-190) in 3D Cartesian form: zero
 // outside the open box, staggered trilinear per component, |E| > 1e8 flagged.
 __device__ __forceinline__ void efield_mesh_dev(const MeshView &M, const double x[3],
                                                 double e[3], int32_t &flags)

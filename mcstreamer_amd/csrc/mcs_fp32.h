@@ -209,7 +209,7 @@ __device__ __forceinline__ void efield32(const MeshView32 &M, const int32_t c[3]
     if (e2 > (float)MCS_EFIELD_LIMIT2) flags |= MCS_FLAG_FIELD;
 }
 
-// NGP ion-charge tracker (gridfields.jl:303-313) from split coordinates.
+// NGP ion-charge tracker (gridfields.jl:203-213) from split coordinates.
 __device__ __forceinline__ void track32(const MeshView32 &M, const int32_t c[3], const float off[3],
                                         double val)
 {
@@ -239,7 +239,7 @@ __device__ __forceinline__ void rebase32(int32_t c[3], float off[3], const int32
 
 // One particle through one time step in FP32 (JuMC/src/mixed_population.jl:31-51;
 // flights, Yoshida push of electron.jl:51-62 in a mesh field or its exact
-// parabola in a uniform one, collisions of collisions.jl:236-269 and
+// parabola in a uniform one, collisions of collisions.jl:159-192 and
 // electron.jl:108-139).  Draws: 24-bit uniforms, one word each, in the order of
 // the FP64 body.  `birth(c, off, v, s, active)` receives each ionization child
 // at the parent's position, `event(j)` each selected table row.

@@ -4,7 +4,7 @@
 // operation is a coalesced stencil kernel over one level, and the levels below
 // a few thousand cells run as one single-workgroup kernel.
 //
-// Ghost handling.  apply! (boundaries.jl:180-194) sets every ghost plane to
+// Ghost handling.  apply! (boundaries.jl:110-133) sets every ghost plane to
 // +-1 times the adjacent valid plane, x first, then y, then z; a ghost cell
 // therefore equals the product of the +-1 factors of its ghost directions
 // times the nearest interior cell.  For the 7-point stencil the ghost neighbour
@@ -530,7 +530,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_mg_axpy_bc(double *x, const Z *z,
     ghost_write(x, L, bc, i, j, k, v);
 }
 
-// setinhom! (boundaries.jl:202-214): b += (2/s) val on the first valid layer.
+// setinhom! (boundaries.jl:141-153): b += (2/s) val on the first valid layer.
 __global__ void k_mg_inhom(double *b, Lvl L, int face, double c, const double *val)
 {
     int d = face / 2, hi = face & 1;

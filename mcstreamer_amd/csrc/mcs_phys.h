@@ -33,7 +33,7 @@
 #define MCS_YOSHIDA_D2 MCS_YOSHIDA_W0
 #define MCS_YOSHIDA_D3 MCS_YOSHIDA_W1
 
-/* |E| above this aborts the reference (gridfields.jl:279); compared squared. */
+/* |E| above this aborts the reference (gridfields.jl:179); compared squared. */
 #define MCS_EFIELD_LIMIT2 1e16
 
 #endif /* MCS_PHYS_H */

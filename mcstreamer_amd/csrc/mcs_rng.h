@@ -3,7 +3,7 @@
  * particle step needs, written once for both the HIP kernels and host C code.
  *
  * Why this header exists: the reference draws every random number from Julia's
- * global task-local RNG (`rand()` in JuMC/src/util.jl:4-17, collisions.jl:243,
+ * global task-local RNG (`rand()` in JuMC/src/util.jl:4-17, collisions.jl:166,
  * electron.jl:111-125).  Here every draw comes from a Philox4x32-10 stream keyed
  * per particle, so the GPU and any CPU restatement that includes this header see
  * the same numbers in the same order, whatever the thread schedule.

@@ -55,7 +55,7 @@ struct AdvArgs {
 // (JuMC/src/mixed_population.jl:27 fixes the range, so children are not advanced).
 // Per particle, free flights to the next (null) collision until dt is used up
 // (:31-51), the Yoshida push of electron.jl:51-62 for each flight, the sampler of
-// collisions.jl:236-269 and the outcomes of electron.jl:108-139.  Children go to a
+// collisions.jl:159-192 and the outcomes of electron.jl:108-139.  Children go to a
 // birth log tagged (parent, ordinal); k_place writes them after the parents in
 // parent order, which is the append order of the single-threaded reference.
 // Five waves per SIMD (96 registers; a few cold values spill): the field gathers'

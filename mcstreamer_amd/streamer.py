@@ -1,9 +1,9 @@
 """Host mirror of the MCStreamer step (MCStreamer/src/main.jl:217-315) on a 3D
 Cartesian mesh, on top of libmcs.
 
-  Grid / GridFields (grid.jl:4-100, gridfields.jl:107-184) -> Grid3, GridFields
-  FieldInterp (gridfields.jl:189-290)      -> FieldInterp
-  CollisionTracker (gridfields.jl:298-324) -> CollisionTracker
+  Grid / GridFields (grid.jl:4-100, gridfields.jl:1-84) -> Grid3, GridFields
+  FieldInterp (gridfields.jl:86-190)      -> FieldInterp
+  CollisionTracker (gridfields.jl:193-224) -> CollisionTracker
   density! (poisson.jl:65-84)              -> density
   poisson! (poisson.jl:6-58)               -> poisson
   resample! (resample.jl:7-84)             -> resample
@@ -84,7 +84,7 @@ class GridFields:
 
 
 class FieldInterp:
-    """Staggered trilinear interpolation of the face fields (gridfields.jl:215-290)."""
+    """Staggered trilinear interpolation of the face fields (gridfields.jl:115-190)."""
 
     def __init__(self, fields):
         self.fields = fields

@@ -130,7 +130,7 @@ static inline int64_t cidx(const int32_t n[3], int64_t i, int64_t j, int64_t k)
     return i + (int64_t)(n[0] + 2) * (j + (int64_t)(n[1] + 2) * k);
 }
 
-/* normdivrem + staggered (MCStreamer/src/gridfields.jl:194-209) for one axis:
+/* normdivrem + staggered (MCStreamer/src/gridfields.jl:94-109) for one axis:
  * face index m with offset d, and ghost-inclusive centre index m1 with offset d1.
  * normdivrem is @fastmath in the reference, which licenses x * (1/h) for x / h;
  * this restatement fixes that choice. */
@@ -164,7 +164,7 @@ static inline double trilerp(const double *a, int64_t sx, int64_t sxy, int64_t b
     return c0 * (1.0 - dz) + c1 * dz;
 }
 
-/* FieldInterp (MCStreamer/src/gridfields.jl:215-290), 3D Cartesian: zero outside
+/* FieldInterp (MCStreamer/src/gridfields.jl:115-190), 3D Cartesian: zero outside
  * the open box (inside(), grid.jl:45-49); each component interpolated trilinearly
  * from its staggered face array; |E| > 1e8 V/m raises the field flag. */
 void ref_efield(const ref_field *f, const double x[3], double e[3], int32_t *flags)
@@ -233,7 +233,7 @@ void ref_yoshida(const ref_field *f, double x[3], double v[3], double dt, int32_
     for (int d = 0; d < 3; ++d) x[d] = x[d] + c4 * v[d];
 }
 
-/* track1 (MCStreamer/src/gridfields.jl:303-313): val/dV into the NGP cell of x. */
+/* track1 (MCStreamer/src/gridfields.jl:203-213): val/dV into the NGP cell of x. */
 static void track1(ref_mesh *m, const double x[3], double val)
 {
     if (!m) return;
@@ -267,8 +267,8 @@ static int append(ref_pop *p, const double x[3], const double v[3], double w, do
 }
 
 /* advance! (JuMC/src/mixed_population.jl:24-54) with do_one_collision!
- * (collisions.jl:236-269), collide (electron.jl:108-139), apply!
- * (collisions.jl:173-192) and the tracker (gridfields.jl:315-318).
+ * (collisions.jl:159-192), collide (electron.jl:108-139), apply!
+ * (collisions.jl:173-192) and the tracker (gridfields.jl:215-218).
  * The particle range is fixed when the loop starts, so children are not
  * advanced in the step they are born; a particle removed by attachment keeps
  * flying and colliding until its time step is used up. */
@@ -793,7 +793,7 @@ static inline int64_t gsz(const int32_t n[3])
     return (int64_t)(n[0] + 2) * (n[1] + 2) * (n[2] + 2);
 }
 
-/* apply! with setbc! (Multigrid/src/boundaries.jl:171-194): faces in the order
+/* apply! with setbc! (Multigrid/src/boundaries.jl:110-133): faces in the order
  * x-, x+, y-, y+, z-, z+; each ghost plane (full extent, ghosts included) is
  * coef * (adjacent valid plane), coef = -1 Dirichlet, +1 Neumann. */
 void ref_mg_apply_bc(const int32_t n[3], const int32_t bc[6], double *a)

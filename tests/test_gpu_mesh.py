@@ -1,7 +1,7 @@
 """GPU parity of the mesh side of the step against the oracle: canonical cell
 sort, NGP/CIC deposition (density!, poisson.jl:65-84), charge assembly and
 face fields (poisson.jl:19-57), the fused step with the mesh gather and the ion
-tracker (gridfields.jl:215-324), per-cell resampling (resample.jl:7-84),
+tracker (gridfields.jl:115-224), per-cell resampling (resample.jl:7-84),
 repack!, the population reductions and Swarm.resample!.
 
 Bar: integer/index results (order, ids, counts) bit-exact; floating results

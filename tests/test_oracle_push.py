@@ -1,5 +1,5 @@
 """Pins of the Yoshida push (JuMC/src/electron.jl:38-62) and of the staggered
-field gather (MCStreamer/src/gridfields.jl:194-290, here in 3D Cartesian form)."""
+field gather (MCStreamer/src/gridfields.jl:94-190, here in 3D Cartesian form)."""
 import ctypes
 
 import numpy as np
