@@ -16,6 +16,7 @@ of the same state.
 import argparse
 import json
 import os
+import tempfile
 import sys
 import time
 
@@ -176,8 +177,7 @@ def run_swarm(args):
 
     ctx = jumc.context()
     e_ev = np.geomspace(1e-3, 1e3, 4000)
-    path = os.path.join(ROOT, "_work", "bench_gas_c0.json")
-    os.makedirs(os.path.dirname(path), exist_ok=True)
+    path = os.path.join(tempfile.gettempdir(), f"mcs_bench_gas_c0_{os.getpid()}.json")
     gases.write_gas(path, e_ev, with_attacher=False)
     proc, rate, maxrate, _ = lxcat.load_lxcat([path], gases.densities(0.0), e_ev * co.eV)
     table = jumc.CollisionTable(proc, e_ev * co.eV, rate, maxrate, egrid="log")

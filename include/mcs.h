@@ -78,6 +78,7 @@ typedef struct mcs_table mcs_table;
 typedef struct mcs_pop mcs_pop;
 typedef struct mcs_mesh mcs_mesh;
 typedef struct mcs_mg mcs_mg;
+typedef struct mcs_cgrid mcs_cgrid;
 
 typedef struct mcs_field {
     int32_t kind;      /* MCS_FIELD_UNIFORM or MCS_FIELD_MESH */
@@ -318,6 +319,54 @@ int32_t mcs_swarm_batch_download(mcs_swarm_batch *b, int32_t slot, int64_t nmax,
 int32_t mcs_pop_init_thermal(mcs_pop *p, int64_t n, uint64_t seed, uint32_t instance, double vth);
 /* Launch shape and compaction cadence (workgroup size 128 or 256). */
 int32_t mcs_swarm_batch_config(mcs_swarm_batch *b, int32_t block, int32_t compact_every);
+
+/* ---- axisymmetric (r, z) grid: MCStreamer.Grid / GridFields as the reference
+ * builds them (MCStreamer/src/grid.jl:4-90, gridfields.jl:1-84): M cells in r over
+ * [0, R], N in z over [0, L].  Cell-centred arrays QFIXED, QPART, Q, U are
+ * (M+2)(N+2) with one ghost layer (index (i, j), i in 0..M+1, j in 0..N+1, r
+ * fastest, the layout of the two-dimensional multigrid with ndim 2, cyl 1);
+ * face arrays ER, EZ are (M+3)(N+3) (calloc_faces, i in 0..M+2, j in 0..N+2).
+ * Spacings follow the reference's LinRanges: step(rc), step(zc) for FieldInterp,
+ * faces and dV; step(rf), step(zf) for cellindex. */
+#define MCS_CARR_QFIXED 0
+#define MCS_CARR_QPART 1
+#define MCS_CARR_Q 2
+#define MCS_CARR_U 3
+#define MCS_CARR_ER 4
+#define MCS_CARR_EZ 5
+/* Grid(R, L, M, N) (grid.jl:33-42) with its arrays zeroed (M, N >= 2). */
+int32_t mcs_cgrid_create(mcs_ctx *ctx, double R, double L, int32_t M, int32_t N, mcs_cgrid **out);
+int32_t mcs_cgrid_destroy(mcs_cgrid *g);
+/* steps[0..3] = dr(grid), dz(grid) (grid.jl:74-75: steps of rc, zc), step(rf), step(zf). */
+int32_t mcs_cgrid_steps(const mcs_cgrid *g, double steps[4]);
+int32_t mcs_cgrid_set(mcs_cgrid *g, int32_t which, const double *host);
+int32_t mcs_cgrid_get(mcs_cgrid *g, int32_t which, double *host);
+int32_t mcs_cgrid_zero(mcs_cgrid *g, int32_t which);
+/* density! (poisson.jl:65-84) on the axisymmetric grid: every active particle in
+ * the NGP cell I = cellindex(grid, x) (grid.jl:84-90) that the ghosted array and
+ * rc both hold adds (w val)/dV(grid, I[1]), dV = 2 pi rc[i] dr dz (grid.jl:76).
+ * Per cell the terms are added in particle order; no particle order is needed. */
+int32_t mcs_cyl_density(mcs_pop *p, mcs_cgrid *g, int32_t which, double val);
+/* poisson! (poisson.jl:6-58) up to the field solve: q = e (qfixed + qpart) rounded
+ * to FP32 as the Float32 denoiser input (poisson.jl:31-33, NullDenoiser, no fluid
+ * electrons), then Multigrid.solve of u (the previous solution is the initial
+ * guess) with `mg` (ndim 2, n = (M, N), cyl 1), then the face fields with eb.
+ * The free radial boundary (freebc.jl) is not applied: the solve is the first of
+ * solve_freebc!'s two.  MCS_E_NONCONVERGED still leaves the faces computed. */
+int32_t mcs_cyl_poisson(mcs_cgrid *g, mcs_mg *mg, double eb, int32_t *iters, double *resid);
+/* Face fields from U (poisson.jl:50-58): er = (u[i-1,j] - u[i,j])/dr and
+ * ez = eb + (u[i,j-1] - u[i,j])/dz for i in 1..M+1, j in 1..N+1, then the ghost
+ * copies ez[0,:] = ez[1,:], ez[M+1,:] = ez[M,:], er[:,0] = er[:,1], er[:,N+1] = er[:,N].
+ * U's ghosts must hold the boundary values (the solve leaves them so). */
+int32_t mcs_cyl_efield_faces(mcs_cgrid *g, double eb);
+/* advance! (mixed_population.jl:24-54) in the axisymmetric FieldInterp
+ * (gridfields.jl:115-190: bilinear er, ez on the staggered faces, er rotated by
+ * x/r, y/r; zero outside 0 < r < R, 0 < z < L) with, when track != 0, the
+ * CollisionTracker of the same grid (gridfields.jl:203-218: +-w/dV into QFIXED at
+ * the NGP cell; atomics, so the order of same-cell additions is not fixed).
+ * FP64 populations only; otherwise as mcs_advance. */
+int32_t mcs_advance_cyl(mcs_pop *p, const mcs_table *t, mcs_cgrid *g, double dt, int32_t track,
+                        uint64_t seed, uint32_t instance, uint32_t step, mcs_step_stats *stats);
 
 #ifdef __cplusplus
 }

@@ -199,6 +199,16 @@ SIGNATURES = {
     "mcs_swarm_batch_download": [_P, _I32, _I64, _PI64, _PD, _PD, _PD, _PD, _PU8, _PU64],
     "mcs_swarm_batch_config": [_P, _I32, _I32],
     "mcs_pop_init_thermal": [_P, _I64, _U64, _U32, _D],
+    "mcs_cgrid_create": [_P, _D, _D, _I32, _I32, _PP],
+    "mcs_cgrid_destroy": [_P],
+    "mcs_cgrid_steps": [_P, _PD],
+    "mcs_cgrid_set": [_P, _I32, _PD],
+    "mcs_cgrid_get": [_P, _I32, _PD],
+    "mcs_cgrid_zero": [_P, _I32],
+    "mcs_cyl_density": [_P, _P, _I32, _D],
+    "mcs_cyl_poisson": [_P, _P, _D, _PI32, _PD],
+    "mcs_cyl_efield_faces": [_P, _D],
+    "mcs_advance_cyl": [_P, _P, _P, _D, _I32, _U64, _U32, _U32, ctypes.POINTER(StepStats)],
 }
 
 _lib = None

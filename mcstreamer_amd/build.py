@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "libmcs.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MCS_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["mcs_api.hip", "mcs_step.hip", "mcs_mesh.hip", "mcs_mg.hip", "mcs_mg2.hip", "mcs_swarm.hip"]
+SOURCES = ["mcs_api.hip", "mcs_step.hip", "mcs_mesh.hip", "mcs_mg.hip", "mcs_mg2.hip", "mcs_swarm.hip", "mcs_cyl.hip"]
 
 # Contraction stays off so that a*b+c rounds twice, as in the reference and the
 # CPU restatement; packed FP32 instructions are disabled for all device code.
@@ -43,7 +43,7 @@ def _compile(src):
 
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=min(5, len(SOURCES))) as ex:
+    with ThreadPoolExecutor(max_workers=min(7, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]

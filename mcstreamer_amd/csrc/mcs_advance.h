@@ -20,6 +20,30 @@ __device__ __forceinline__ void track_dev(const MeshView &M, const double x[3], 
     atomicAdd(&M.qfixed[idx], val / dV);
 }
 
+// The field at x: uniform (FIELD 0), the 3D mesh's faces (1) or the
+// axisymmetric grid's (2).
+template <int FIELD>
+__device__ __forceinline__ void field_at(const MeshView &M, const double x[3], const double *euni, double e[3],
+                                         int32_t &flags)
+{
+    if (FIELD == 2) {
+        efield_cyl_dev(M.C, x, e, flags);
+    } else if (FIELD == 1) {
+        efield_mesh_dev(M, x, e, flags);
+    } else {
+        e[0] = euni[0];
+        e[1] = euni[1];
+        e[2] = euni[2];
+    }
+}
+
+template <int FIELD>
+__device__ __forceinline__ void track_at(const MeshView &M, const double x[3], double val)
+{
+    if (FIELD == 2) track_cyl_dev(M.C, x, val);
+    else track_dev(M, x, val);
+}
+
 // One particle through one time step dt (JuMC/src/mixed_population.jl:31-51):
 // free flights to the next (null) collision, the Yoshida push of
 // electron.jl:51-62 with the field evaluated three times per flight, the
@@ -57,17 +81,17 @@ __device__ __forceinline__ void advance_one(double x[3], double v[3], const doub
             double f2 = (MCS_YOSHIDA_D2 * tt) * MCS_QM;
 #pragma unroll
             for (int d = 0; d < 3; ++d) x[d] = x[d] + k1 * v[d];
-            if (FIELD_MESH) efield_mesh_dev(M, x, e, flags); else { e[0] = euni[0]; e[1] = euni[1]; e[2] = euni[2]; }
+            field_at<FIELD_MESH>(M, x, euni, e, flags);
 #pragma unroll
             for (int d = 0; d < 3; ++d) v[d] = v[d] + f1 * e[d];
 #pragma unroll
             for (int d = 0; d < 3; ++d) x[d] = x[d] + k2 * v[d];
-            if (FIELD_MESH) efield_mesh_dev(M, x, e, flags); else { e[0] = euni[0]; e[1] = euni[1]; e[2] = euni[2]; }
+            field_at<FIELD_MESH>(M, x, euni, e, flags);
 #pragma unroll
             for (int d = 0; d < 3; ++d) v[d] = v[d] + f2 * e[d];
 #pragma unroll
             for (int d = 0; d < 3; ++d) x[d] = x[d] + k2 * v[d];
-            if (FIELD_MESH) efield_mesh_dev(M, x, e, flags); else { e[0] = euni[0]; e[1] = euni[1]; e[2] = euni[2]; }
+            field_at<FIELD_MESH>(M, x, euni, e, flags);
 #pragma unroll
             for (int d = 0; d < 3; ++d) v[d] = v[d] + f1 * e[d];
 #pragma unroll
@@ -127,11 +151,11 @@ __device__ __forceinline__ void advance_one(double x[3], double v[3], const doub
                     v[1] = om1[1] * vabs;
                     v[2] = om1[2] * vabs;
                     double sc = mcs_nextcoll(&rs);
-                    if (TRACK) track_dev(M, x, w);
+                    if (TRACK) track_at<FIELD_MESH>(M, x, w);
                     double vc[3] = {om2[0] * vabs, om2[1] * vabs, om2[2] * vabs};
                     birth(x, vc, sc, act);
                 } else if (kind == MCS_PROC_ATTACHMENT) {
-                    if (TRACK) track_dev(M, x, -w);
+                    if (TRACK) track_at<FIELD_MESH>(M, x, -w);
                     if (act) deaths += 1;
                     act = 0;
                 }

@@ -1,0 +1,280 @@
+// mcs_cyl.hip -- the axisymmetric (r, z) grid of MCStreamer: Grid and GridFields
+// (MCStreamer/src/grid.jl:4-90, gridfields.jl:1-84), the cylindrical density!
+// (poisson.jl:65-84) and poisson!'s charge assembly and face fields
+// (poisson.jl:6-58).  FieldInterp and track1 on this grid are device functions
+// of the particle step (mcs_device.h); mcs_advance_cyl is in mcs_step.hip.
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <cmath>
+
+#include "mcs_device.h"
+
+namespace {
+
+// Element j (0-based) of LinRange(a, b, d + 1): Base.lerpi, (1 - t) a + t b with
+// t = j / d.
+double lerpi(int64_t j, int64_t d, double a, double b)
+{
+    const double t = (double)j / (double)d;
+    return (1.0 - t) * a + t * b;
+}
+
+// Centres of LinRange(0, X, n + 1) as grid.jl:37-40 builds them: the LinRange
+// 0.5 * (f[2:end] + f[1:end-1]) of n points; its first and last element.
+void centres(double X, int64_t n, double &c0, double &c1)
+{
+    const double f1 = lerpi(0, n, 0.0, X), f2 = lerpi(1, n, 0.0, X);
+    const double fn = lerpi(n - 1, n, 0.0, X), fn1 = lerpi(n, n, 0.0, X);
+    c0 = 0.5 * (f2 + f1);
+    c1 = 0.5 * (fn1 + fn);
+}
+
+int64_t centre_len(const mcs_cgrid *g) { return (int64_t)(g->M + 2) * (g->N + 2); }
+int64_t face_len(const mcs_cgrid *g) { return (int64_t)(g->M + 3) * (g->N + 3); }
+
+__global__ void kc_zero(double *a, int64_t n)
+{
+    const int64_t i = gidx();
+    if (i < n) a[i] = 0.0;
+}
+
+// Key of each particle slot: its cell, or `none` when the particle is inactive or
+// outside what density! adds to; the value is the slot.
+__global__ __launch_bounds__(MCS_BLOCK) void kc_keys(PopBuf b, int64_t n, CylView C, uint32_t none,
+                                                     uint32_t *key, uint32_t *val)
+{
+    const int64_t p = gidx();
+    if (p >= n) return;
+    uint32_t k = none;
+    if (b.active[p]) {
+        const double x[3] = {b.x[0][p], b.x[1][p], b.x[2][p]};
+        int64_t i;
+        const int64_t c = cyl_cell_dev(C, x, i);
+        if (c >= 0) k = (uint32_t)c;
+    }
+    key[p] = k;
+    val[p] = (uint32_t)p;
+}
+
+// One thread per run of equal keys in the stably sorted (key, slot) pairs: the
+// cell's terms (w val)/dV added to its value in slot order.
+__global__ __launch_bounds__(MCS_BLOCK) void kc_runs(const uint32_t *key, const uint32_t *val, int64_t n,
+                                                     uint32_t none, const double *w, const double *dv, int32_t M,
+                                                     double v, double *arr)
+{
+    const int64_t t = gidx();
+    if (t >= n) return;
+    const uint32_t k = key[t];
+    if (k == none || (t > 0 && key[t - 1] == k)) return;
+    const double d = dv[k % (uint32_t)(M + 2) - 1];
+    double acc = arr[k];
+    for (int64_t q = t; q < n && key[q] == k; ++q) acc = acc + (w[val[q]] * v) / d;
+    arr[k] = acc;
+}
+
+// q = Float64(Float32(e (qfixed + qpart))) on the valid cells (poisson.jl:19-33).
+__global__ __launch_bounds__(MCS_BLOCK) void kc_charge(const double *qf, const double *qp, double *q, int32_t M,
+                                                       int32_t N)
+{
+    const int64_t t = gidx();
+    if (t >= (int64_t)M * N) return;
+    const int64_t i = t % M + 1, j = t / M + 1, c = i + (int64_t)(M + 2) * j;
+    double q0 = qf[c];
+    q0 = q0 + qp[c];
+    q0 = q0 * MCS_ELEMENTARY_CHARGE;
+    q[c] = (double)(float)q0;
+}
+
+// er, ez on i in 1..M+1, j in 1..N+1 (poisson.jl:50-55).
+__global__ __launch_bounds__(MCS_BLOCK) void kc_faces(const double *u, double *er, double *ez, int32_t M, int32_t N,
+                                                      double dr, double dz, double eb)
+{
+    const int64_t t = gidx();
+    if (t >= (int64_t)(M + 1) * (N + 1)) return;
+    const int64_t i = t % (M + 1) + 1, j = t / (M + 1) + 1;
+    const int64_t su = M + 2, sf = M + 3;
+    er[i + sf * j] = (u[(i - 1) + su * j] - u[i + su * j]) / dr;
+    ez[i + sf * j] = eb + (u[i + su * (j - 1)] - u[i + su * j]) / dz;
+}
+
+// The ghost copies of poisson.jl:55-58; the rows and columns written are not
+// among those read.
+__global__ void kc_face_ghosts(double *er, double *ez, int32_t M, int32_t N)
+{
+    const int64_t t = gidx();
+    const int64_t sf = M + 3;
+    if (t < N + 3) {
+        ez[0 + sf * t] = ez[1 + sf * t];
+        ez[(M + 1) + sf * t] = ez[M + sf * t];
+    }
+    if (t < M + 3) {
+        er[t] = er[t + sf];
+        er[t + sf * (N + 1)] = er[t + sf * N];
+    }
+}
+
+}  // namespace
+
+extern "C" int32_t mcs_cgrid_create(mcs_ctx *ctx, double R, double L, int32_t M, int32_t N, mcs_cgrid **out)
+{
+    if (!ctx || !out) return MCS_E_ARG;
+    if (M < 2 || N < 2 || !(R > 0.0) || !(L > 0.0))
+        return mcs_fail(ctx, MCS_E_ARG, "cgrid: M, N must be >= 2 and R, L > 0");
+    if ((int64_t)(M + 3) * (N + 3) >= ((int64_t)1 << 31))
+        return mcs_fail(ctx, MCS_E_ARG, "cgrid too large");
+    hipSetDevice(ctx->device);
+    mcs_cgrid *g = new mcs_cgrid();
+    g->ctx = ctx;
+    g->M = M;
+    g->N = N;
+    g->R = R;
+    g->L = L;
+    double r0, r1, z0, z1;
+    centres(R, M, r0, r1);
+    centres(L, N, z0, z1);
+    g->dr = (r1 - r0) / (double)(M - 1);   // step(rc)
+    g->dz = (z1 - z0) / (double)(N - 1);   // step(zc)
+    g->drf = (R - 0.0) / (double)M;        // step(rf)
+    g->dzf = (L - 0.0) / (double)N;
+    std::vector<double> dv(M);
+    const double two_pi = 2.0 * 3.141592653589793;
+    for (int32_t i = 0; i < M; ++i) dv[i] = two_pi * lerpi(i, M - 1, r0, r1) * g->dr * g->dz;
+    for (int a = 0; a < 6; ++a) {
+        g->arr_len[a] = a < 4 ? centre_len(g) : face_len(g);
+        if (hipMalloc(&g->arr[a], sizeof(double) * g->arr_len[a]) != hipSuccess) {
+            mcs_cgrid_destroy(g);
+            return mcs_fail(ctx, MCS_E_HIP, "cgrid: device allocation failed");
+        }
+        hipMemsetAsync(g->arr[a], 0, sizeof(double) * g->arr_len[a], ctx->stream);
+    }
+    if (hipMalloc(&g->dv, sizeof(double) * M) != hipSuccess) {
+        mcs_cgrid_destroy(g);
+        return mcs_fail(ctx, MCS_E_HIP, "cgrid: device allocation failed");
+    }
+    MCS_HIP(ctx, hipMemcpyAsync(g->dv, dv.data(), sizeof(double) * M, hipMemcpyHostToDevice, ctx->stream));
+    MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *out = g;
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_cgrid_destroy(mcs_cgrid *g)
+{
+    if (!g) return MCS_OK;
+    hipSetDevice(g->ctx->device);
+    hipStreamSynchronize(g->ctx->stream);
+    for (int a = 0; a < 6; ++a)
+        if (g->arr[a]) hipFree(g->arr[a]);
+    if (g->dv) hipFree(g->dv);
+    delete g;
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_cgrid_steps(const mcs_cgrid *g, double steps[4])
+{
+    if (!g || !steps) return MCS_E_ARG;
+    steps[0] = g->dr;
+    steps[1] = g->dz;
+    steps[2] = g->drf;
+    steps[3] = g->dzf;
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_cgrid_set(mcs_cgrid *g, int32_t which, const double *host)
+{
+    if (!g || !host || which < 0 || which > 5) return MCS_E_ARG;
+    mcs_ctx *ctx = g->ctx;
+    hipSetDevice(ctx->device);
+    MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    MCS_HIP(ctx, hipMemcpy(g->arr[which], host, sizeof(double) * g->arr_len[which], hipMemcpyHostToDevice));
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_cgrid_get(mcs_cgrid *g, int32_t which, double *host)
+{
+    if (!g || !host || which < 0 || which > 5) return MCS_E_ARG;
+    mcs_ctx *ctx = g->ctx;
+    hipSetDevice(ctx->device);
+    MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    MCS_HIP(ctx, hipMemcpy(host, g->arr[which], sizeof(double) * g->arr_len[which], hipMemcpyDeviceToHost));
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_cgrid_zero(mcs_cgrid *g, int32_t which)
+{
+    if (!g || which < 0 || which > 5) return MCS_E_ARG;
+    mcs_ctx *ctx = g->ctx;
+    hipSetDevice(ctx->device);
+    hipLaunchKernelGGL(kc_zero, dim3((unsigned)cdiv(g->arr_len[which], MCS_BLOCK)), dim3(MCS_BLOCK), 0, ctx->stream,
+                       g->arr[which], g->arr_len[which]);
+    MCS_LAUNCH_CHECK(ctx);
+    return MCS_OK;
+}
+
+// density!: stable radix sort of (cell, slot) pairs, then one thread per cell run.
+extern "C" int32_t mcs_cyl_density(mcs_pop *p, mcs_cgrid *g, int32_t which, double val)
+{
+    if (!p || !g || (which != MCS_CARR_QFIXED && which != MCS_CARR_QPART)) return MCS_E_ARG;
+    mcs_ctx *ctx = p->ctx;
+    if (g->ctx != ctx) return mcs_fail(ctx, MCS_E_ARG, "cyl_density: grid of another context");
+    if (p->prec != MCS_PREC_F64) return mcs_fail(ctx, MCS_E_ARG, "cyl_density: FP64 populations only");
+    hipSetDevice(ctx->device);
+    const int64_t n = p->n;
+    if (n == 0) return MCS_OK;
+    const uint32_t none = (uint32_t)centre_len(g);
+    unsigned bits = 1;
+    while (bits < 32 && (none >> bits) != 0) ++bits;
+    uint32_t *kin = (uint32_t *)p->i32a, *kout = (uint32_t *)p->i32b;
+    uint32_t *vin = (uint32_t *)p->i32c, *vout = (uint32_t *)p->u64a;
+    ProfScope ps(ctx, "cyl_density");
+    const MeshView mv = cyl_view(g);
+    const dim3 grid((unsigned)cdiv(n, MCS_BLOCK)), blk(MCS_BLOCK);
+    hipLaunchKernelGGL(kc_keys, grid, blk, 0, ctx->stream, p->buf[0], n, mv.C, none, kin, vin);
+    MCS_LAUNCH_CHECK(ctx);
+    size_t tb = 0;
+    MCS_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tb, kin, kout, vin, vout, (size_t)n, 0u, bits, ctx->stream));
+    void *tmp = nullptr;
+    int32_t rc = mcs_scratch(ctx, tb, &tmp);
+    if (rc) return rc;
+    MCS_HIP(ctx, rocprim::radix_sort_pairs(tmp, tb, kin, kout, vin, vout, (size_t)n, 0u, bits, ctx->stream));
+    hipLaunchKernelGGL(kc_runs, grid, blk, 0, ctx->stream, (const uint32_t *)kout, (const uint32_t *)vout, n, none,
+                       (const double *)p->buf[0].w, (const double *)g->dv, g->M, val, g->arr[which]);
+    MCS_LAUNCH_CHECK(ctx);
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_cyl_efield_faces(mcs_cgrid *g, double eb)
+{
+    if (!g) return MCS_E_ARG;
+    mcs_ctx *ctx = g->ctx;
+    hipSetDevice(ctx->device);
+    ProfScope ps(ctx, "cyl_faces");
+    const int64_t nf = (int64_t)(g->M + 1) * (g->N + 1);
+    hipLaunchKernelGGL(kc_faces, dim3((unsigned)cdiv(nf, MCS_BLOCK)), dim3(MCS_BLOCK), 0, ctx->stream,
+                       (const double *)g->arr[MCS_CARR_U], g->arr[MCS_CARR_ER], g->arr[MCS_CARR_EZ], g->M, g->N,
+                       g->dr, g->dz, eb);
+    const int64_t ng = (g->M > g->N ? g->M : g->N) + 3;
+    hipLaunchKernelGGL(kc_face_ghosts, dim3((unsigned)cdiv(ng, MCS_BLOCK)), dim3(MCS_BLOCK), 0, ctx->stream,
+                       g->arr[MCS_CARR_ER], g->arr[MCS_CARR_EZ], g->M, g->N);
+    MCS_LAUNCH_CHECK(ctx);
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_cyl_poisson(mcs_cgrid *g, mcs_mg *mg, double eb, int32_t *iters, double *resid)
+{
+    if (!g || !mg) return MCS_E_ARG;
+    mcs_ctx *ctx = g->ctx;
+    if (mg->ctx != ctx) return mcs_fail(ctx, MCS_E_ARG, "cyl_poisson: solver of another context");
+    if (mg->cfg.ndim != 2 || mg->cfg.n[0] != g->M || mg->cfg.n[1] != g->N || mg->cfg.cyl != 1 ||
+        mg->cfg.precision != MCS_PREC_F64)
+        return mcs_fail(ctx, MCS_E_ARG, "cyl_poisson: the solver must be FP64, ndim 2, n = (M, N), cyl 1");
+    hipSetDevice(ctx->device);
+    const int64_t nc = (int64_t)g->M * g->N;
+    hipLaunchKernelGGL(kc_charge, dim3((unsigned)cdiv(nc, MCS_BLOCK)), dim3(MCS_BLOCK), 0, ctx->stream,
+                       (const double *)g->arr[MCS_CARR_QFIXED], (const double *)g->arr[MCS_CARR_QPART],
+                       g->arr[MCS_CARR_Q], g->M, g->N);
+    MCS_LAUNCH_CHECK(ctx);
+    int32_t rc = mcs_mg2_solve_dev(mg, g->arr[MCS_CARR_U], g->arr[MCS_CARR_Q], iters, resid);
+    if (rc != MCS_OK && rc != MCS_E_NONCONVERGED) return rc;
+    int32_t rf = mcs_cyl_efield_faces(g, eb);
+    return rf ? rf : rc;
+}

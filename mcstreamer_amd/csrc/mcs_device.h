@@ -20,11 +20,22 @@ struct TableView {
     double step, inv_dlog, maxrate;
 };
 
+// The axisymmetric grid as FieldInterp and track1 read it.
+struct CylView {
+    int32_t M, N;
+    double R, L;
+    double dr, dz, drf, dzf;
+    const double *er, *ez;   // (M+3)(N+3)
+    double *qfixed;          // (M+2)(N+2)
+    const double *dv;        // M
+};
+
 struct MeshView {
     int32_t n[3];
     double h, inv_h;
     const double *ex, *ey, *ez;
     double *qfixed;
+    CylView C;
 };
 
 static inline TableView table_view(const mcs_table *t)
@@ -53,7 +64,109 @@ static inline MeshView mesh_view(const mcs_mesh *m)
     v.ey = m ? m->arr[MCS_ARR_EY] : nullptr;
     v.ez = m ? m->arr[MCS_ARR_EZ] : nullptr;
     v.qfixed = m ? m->arr[MCS_ARR_QFIXED] : nullptr;
+    memset(&v.C, 0, sizeof(v.C));
     return v;
+}
+
+static inline MeshView cyl_view(const mcs_cgrid *g)
+{
+    MeshView v;
+    memset(&v, 0, sizeof(v));
+    v.C.M = g->M;
+    v.C.N = g->N;
+    v.C.R = g->R;
+    v.C.L = g->L;
+    v.C.dr = g->dr;
+    v.C.dz = g->dz;
+    v.C.drf = g->drf;
+    v.C.dzf = g->dzf;
+    v.C.er = g->arr[MCS_CARR_ER];
+    v.C.ez = g->arr[MCS_CARR_EZ];
+    v.C.qfixed = g->arr[MCS_CARR_QFIXED];
+    v.C.dv = g->dv;
+    return v;
+}
+
+// fld(x, y) for y > 0: the largest integer q with q y <= x, exactly (the quotient
+// is rounded, so floor(x / y) can be one too high; x - q y has the sign of its
+// fused evaluation).
+__device__ __forceinline__ double fld_dev(double x, double y)
+{
+    double q = floor(x / y);
+    if (fma(-q, y, x) < 0.0) q -= 1.0;
+    else if (fma(-(q + 1.0), y, x) >= 0.0) q += 1.0;
+    return q;
+}
+
+// cellindex (MCStreamer/src/grid.jl:84-90) with the bounds tests of density! and
+// track1 (the ghosted array: i in 0..M+1, j in 0..N+1; rc: i in 1..M): the flat
+// index i + (M+2) j of the cell, or -1.
+__device__ __forceinline__ int64_t cyl_cell_dev(const CylView &C, const double x[3], int64_t &i)
+{
+    const double rho = sqrt(x[0] * x[0] + x[1] * x[1]);
+    const double fi = fld_dev(rho - 0.0, C.drf), fj = fld_dev(x[2] - 0.0, C.dzf);
+    if (!(fi >= 0.0 && fi < (double)C.M && fj >= -1.0 && fj <= (double)C.N)) return -1;
+    i = (int64_t)fi + 1;
+    return i + (int64_t)(C.M + 2) * ((int64_t)fj + 1);
+}
+
+// FieldInterp (MCStreamer/src/gridfields.jl:115-190) on the axisymmetric grid:
+// normdivrem and staggered (:94-109) per axis, bilinear er (faces in r, staggered
+// in z) and ez (staggered in r, faces in z), er rotated by x/r and y/r.
+__device__ __forceinline__ void efield_cyl_dev(const CylView &C, const double x[3], double e[3],
+                                               int32_t &flags)
+{
+    const double r = sqrt(x[0] * x[0] + x[1] * x[1]);
+    if (!(0.0 < r && r < C.R && 0.0 < x[2] && x[2] < C.L)) {
+        e[0] = 0.0;
+        e[1] = 0.0;
+        e[2] = 0.0;
+        return;
+    }
+    const double qr = r / C.dr, qz = x[2] / C.dz;
+    // trunc; qr < R / dr and qz < L / dz keep these within M and N (the min only
+    // guards the arrays)
+    const int32_t i = min((int32_t)qr, C.M), j = min((int32_t)qz, C.N);
+    const double dr = qr - (double)i, dz = qz - (double)j;
+    int32_t i1, j1;
+    double dr1, dz1;
+    if (dr < 0.5) {
+        i1 = i;
+        dr1 = dr + 0.5;
+    } else {
+        i1 = i + 1;
+        dr1 = dr - 0.5;
+    }
+    if (dz < 0.5) {
+        j1 = j;
+        dz1 = dz + 0.5;
+    } else {
+        j1 = j + 1;
+        dz1 = dz - 0.5;
+    }
+    const int64_t S = C.M + 3;
+    const double *er = C.er + (i + 1) + S * j1;   // er[i + 1, j1] (1-based i)
+    const double *ez = C.ez + i1 + S * (j + 1);   // ez[i1, j + 1]
+    const double er00 = gld(er), er10 = gld(er + 1), er01 = gld(er + S), er11 = gld(er + S + 1);
+    const double ez00 = gld(ez), ez10 = gld(ez + 1), ez01 = gld(ez + S), ez11 = gld(ez + S + 1);
+    const double er1 = ((er00 * (1.0 - dr) * (1.0 - dz1) + er10 * dr * (1.0 - dz1)) + er01 * (1.0 - dr) * dz1) +
+                       er11 * dr * dz1;
+    const double ez1 = ((ez00 * (1.0 - dr1) * (1.0 - dz) + ez10 * dr1 * (1.0 - dz)) + ez01 * (1.0 - dr1) * dz) +
+                       ez11 * dr1 * dz;
+    e[0] = er1 * x[0] / r;
+    e[1] = er1 * x[1] / r;
+    e[2] = ez1;
+    const double e2 = (e[0] * e[0] + e[1] * e[1]) + e[2] * e[2];
+    if (e2 > MCS_EFIELD_LIMIT2) flags |= MCS_FLAG_FIELD;
+}
+
+// track1 (MCStreamer/src/gridfields.jl:203-213) on the axisymmetric grid.
+__device__ __forceinline__ void track_cyl_dev(const CylView &C, const double x[3], double val)
+{
+    int64_t i;
+    const int64_t c = cyl_cell_dev(C, x, i);
+    if (c < 0) return;
+    atomicAdd(&C.qfixed[c], val / gld(C.dv + (i - 1)));
 }
 
 // energy(p) = 0.5*m*(v1^2 + v2^2 + v3^2)  (JuMC/src/electron.jl:27)

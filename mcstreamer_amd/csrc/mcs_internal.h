@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <map>
 #include <string>
 #include <vector>
@@ -199,6 +200,18 @@ struct mcs_mesh {
     int32_t *dep_host_dev = nullptr;
     int32_t dep_dense = 0;          // records a cell visits above which it takes a wavefront
     int64_t dep_lane_listed = 0;    // above this many such cells (last call), one lane per cell instead
+};
+
+// Axisymmetric grid (mcs_cyl.hip).
+struct mcs_cgrid {
+    mcs_ctx *ctx;
+    int32_t M, N;
+    double R, L;
+    double dr, dz;     // step(rc), step(zc)
+    double drf, dzf;   // step(rf), step(zf)
+    double *arr[6];    // QFIXED, QPART, Q, U: (M+2)(N+2); ER, EZ: (M+3)(N+3)
+    int64_t arr_len[6];
+    double *dv;        // dV(grid, i) for i = 1..M
 };
 
 // The two-dimensional solver (mcs_mg2.hip): creation for cfg->ndim == 2, and the

@@ -360,6 +360,54 @@ static int32_t launch_advance32(mcs_pop *p, const mcs_table *t, const mcs_field 
     return MCS_OK;
 }
 
+// After the step kernel: the births' prefix over blocks, the counters and
+// flags in one read-back, then the children placed after the parents.
+static int32_t advance_finish(mcs_pop *p, const mcs_table *t, int64_t n0, int64_t nblk, int64_t *btot,
+                              int64_t *bpre, int64_t *ctr, mcs_step_stats *stats)
+{
+    mcs_ctx *ctx = p->ctx;
+    int32_t rc = mcs_scan_i64(ctx, btot, bpre, nblk);
+    if (rc) return rc;
+    // the counters, then the births (the scan's total)
+    int64_t h[C_EVENTS + MCS_MAX_PROC + 1];
+    rc = mcs_readback2(ctx, ctr, C_EVENTS + t->nproc, bpre + nblk, 1, h);
+    if (rc) return rc;
+    int64_t logged = h[C_LOG];
+    int64_t births = h[C_EVENTS + t->nproc];
+    int32_t flags = (int32_t)h[C_FLAGS];
+    if (stats) {
+        stats->births = births;
+        stats->deaths = h[C_DEATHS];
+        stats->flights = h[C_FLIGHTS];
+        for (int j = 0; j < t->nproc; ++j) stats->events[j] = h[C_EVENTS + j];
+        stats->flags = flags;
+    }
+    if (logged > p->logcap || n0 + births > p->cap) {
+        if (stats) stats->flags |= MCS_FLAG_CAPACITY;
+        return mcs_fail(ctx, MCS_E_CAPACITY, "Maximum number of particles reached (%lld + %lld > %lld)",
+                        (long long)n0, (long long)births, (long long)p->cap);
+    }
+    if (births > 0) {
+        if (p->prec == MCS_PREC_F32)
+            hipLaunchKernelGGL(k_place_births32, dim3((unsigned)cdiv(births, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                               ctx->stream, birth_log32(p), births, (const int32_t *)p->i32a,
+                               (const int64_t *)bpre, n0, p->next_id, p->sbuf[0], p->gn[0], p->gn[1], p->gn[2]);
+        else
+        hipLaunchKernelGGL(k_place_births, dim3((unsigned)cdiv(births, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                           ctx->stream, birth_log(p), births, (const int32_t *)p->i32a,
+                           (const int64_t *)bpre, n0, p->next_id, p->buf[0]);
+        MCS_LAUNCH_CHECK(ctx);
+        p->n = n0 + births;
+        p->next_id += (uint64_t)births;
+    }
+    if (stats) stats->n_out = p->n;
+    if (flags & MCS_FLAG_FIELD) return mcs_fail(ctx, MCS_E_FIELD, "Too high electric field");
+    if (flags & MCS_FLAG_TABLE_RANGE)
+        return mcs_fail(ctx, MCS_E_TABLE_RANGE, "particle energy beyond the collision table");
+    return MCS_OK;
+}
+
+
 extern "C" int32_t mcs_advance(mcs_pop *p, const mcs_table *t, const mcs_field *field, double dt,
                                mcs_mesh *tracker, uint64_t seed, uint32_t instance, uint32_t step,
                                mcs_step_stats *stats)
@@ -421,45 +469,57 @@ extern "C" int32_t mcs_advance(mcs_pop *p, const mcs_table *t, const mcs_field *
     }
     MCS_LAUNCH_CHECK(ctx);
     }
-    int32_t rc = mcs_scan_i64(ctx, btot, bpre, nblk);
-    if (rc) return rc;
-    // the counters, then the births (the scan's total)
-    int64_t h[C_EVENTS + MCS_MAX_PROC + 1];
-    rc = mcs_readback2(ctx, ctr, C_EVENTS + t->nproc, bpre + nblk, 1, h);
-    if (rc) return rc;
-    int64_t logged = h[C_LOG];
-    int64_t births = h[C_EVENTS + t->nproc];
-    int32_t flags = (int32_t)h[C_FLAGS];
+    return advance_finish(p, t, n0, nblk, btot, bpre, ctr, stats);
+}
+
+
+// advance! in the axisymmetric field of grid g, tracking into g's QFIXED when
+// track != 0.
+extern "C" int32_t mcs_advance_cyl(mcs_pop *p, const mcs_table *t, mcs_cgrid *g, double dt, int32_t track,
+                                   uint64_t seed, uint32_t instance, uint32_t step, mcs_step_stats *stats)
+{
+    if (!p || !t || !g) return MCS_E_ARG;
+    mcs_ctx *ctx = p->ctx;
+    hipSetDevice(ctx->device);
+    if (!(dt >= 0.0)) return mcs_fail(ctx, MCS_E_ARG, "advance: dt must be >= 0");
+    if (p->prec != MCS_PREC_F64) return mcs_fail(ctx, MCS_E_ARG, "advance_cyl: FP64 populations only");
+    if (g->ctx != ctx) return mcs_fail(ctx, MCS_E_ARG, "advance_cyl: grid of another context");
+    if (stats) memset(stats, 0, sizeof(*stats));
+    const int64_t n0 = p->n;
     if (stats) {
-        stats->births = births;
-        stats->deaths = h[C_DEATHS];
-        stats->flights = h[C_FLIGHTS];
-        for (int j = 0; j < t->nproc; ++j) stats->events[j] = h[C_EVENTS + j];
-        stats->flags = flags;
+        stats->n_in = n0;
+        stats->n_out = n0;
     }
-    if (logged > p->logcap || n0 + births > p->cap) {
-        if (stats) stats->flags |= MCS_FLAG_CAPACITY;
-        return mcs_fail(ctx, MCS_E_CAPACITY, "Maximum number of particles reached (%lld + %lld > %lld)",
-                        (long long)n0, (long long)births, (long long)p->cap);
-    }
-    if (births > 0) {
-        if (p->prec == MCS_PREC_F32)
-            hipLaunchKernelGGL(k_place_births32, dim3((unsigned)cdiv(births, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
-                               ctx->stream, birth_log32(p), births, (const int32_t *)p->i32a,
-                               (const int64_t *)bpre, n0, p->next_id, p->sbuf[0], p->gn[0], p->gn[1], p->gn[2]);
+    if (n0 == 0) return MCS_OK;
+    int64_t nblk = cdiv(n0, MCS_BLOCK);
+    int64_t *btot = p->blk;
+    int64_t *bpre = p->blk + (nblk + 1);
+    int64_t *ctr = ctx->d_small;
+    hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, ctx->stream, ctr, (int64_t)64, (int64_t)0);
+    AdvArgs a;
+    a.b = p->buf[0];
+    a.n0 = n0;
+    a.T = table_view(t);
+    a.M = cyl_view(g);
+    for (int d = 0; d < 3; ++d) a.e[d] = 0.0;
+    a.dt = dt;
+    a.seed = seed;
+    a.instance = instance;
+    a.step = step;
+    a.L = birth_log(p);
+    a.loff = p->i32a;
+    a.btot = btot;
+    a.ctr = ctr;
+    {
+        ProfScope ps(ctx, "advance");
+        dim3 gr((unsigned)nblk), bl(MCS_BLOCK);
+        if (track)
+            hipLaunchKernelGGL((k_advance<2, 1>), gr, bl, 0, ctx->stream, a);
         else
-        hipLaunchKernelGGL(k_place_births, dim3((unsigned)cdiv(births, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
-                           ctx->stream, birth_log(p), births, (const int32_t *)p->i32a,
-                           (const int64_t *)bpre, n0, p->next_id, p->buf[0]);
-        MCS_LAUNCH_CHECK(ctx);
-        p->n = n0 + births;
-        p->next_id += (uint64_t)births;
+            hipLaunchKernelGGL((k_advance<2, 0>), gr, bl, 0, ctx->stream, a);
     }
-    if (stats) stats->n_out = p->n;
-    if (flags & MCS_FLAG_FIELD) return mcs_fail(ctx, MCS_E_FIELD, "Too high electric field");
-    if (flags & MCS_FLAG_TABLE_RANGE)
-        return mcs_fail(ctx, MCS_E_TABLE_RANGE, "particle energy beyond the collision table");
-    return MCS_OK;
+    MCS_LAUNCH_CHECK(ctx);
+    return advance_finish(p, t, n0, nblk, btot, bpre, ctr, stats);
 }
 
 // ---------------------------------------------------------------- repack!

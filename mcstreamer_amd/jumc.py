@@ -221,12 +221,22 @@ class Population:
 def advance(popl, efield, dt, tracker=None, *, step=None, stats=True):
     """advance! (mixed_population.jl:24-54) on the GPU.  Returns StepStats.
 
-    `efield` is a FixedField or a streamer FieldInterp; `tracker` is None
-    (VoidCollisionTracker) or a CollisionTracker."""
+    `efield` is a FixedField, a streamer FieldInterp or an axisymmetric
+    FieldInterp (axisym); `tracker` is None (VoidCollisionTracker) or a
+    CollisionTracker of the field's grid."""
     st = _lib.StepStats()
+    s = popl.step if step is None else int(step)
+    if getattr(efield, "axisymmetric", False):
+        if tracker is not None and tracker.fields is not efield.fields:
+            raise ValueError("the tracker must belong to the field's grid")
+        check(popl.ctx.handle, _lib.lib().mcs_advance_cyl(
+            popl.handle, popl.collisions.handle, efield.fields.handle, float(dt), int(tracker is not None),
+            popl.seed, popl.instance, s, ctypes.byref(st) if stats else None))
+        if step is None:
+            popl.step += 1
+        return st
     fld = efield.cstruct()
     trk = tracker.mesh_handle() if tracker is not None else None
-    s = popl.step if step is None else int(step)
     check(popl.ctx.handle, _lib.lib().mcs_advance(
         popl.handle, popl.collisions.handle, ctypes.byref(fld), float(dt), trk,
         popl.seed, popl.instance, s, ctypes.byref(st) if stats else None))

@@ -167,8 +167,140 @@ static inline double trilerp(const double *a, int64_t sx, int64_t sxy, int64_t b
 /* FieldInterp (MCStreamer/src/gridfields.jl:115-190), 3D Cartesian: zero outside
  * the open box (inside(), grid.jl:45-49); each component interpolated trilinearly
  * from its staggered face array; |E| > 1e8 V/m raises the field flag. */
+/* ---------------------------------------------------- axisymmetric grid */
+
+/* Base.lerpi: element j of LinRange(a, b, d + 1) is (1 - t) a + t b, t = j / d. */
+static double lerpi(int64_t j, int64_t d, double a, double b)
+{
+    double t = (double)j / (double)d;
+    return (1.0 - t) * a + t * b;
+}
+
+/* Grid(R, L, M, N) (grid.jl:33-42, 74-76): rf = LinRange(0, R, M + 1) and
+ * rc = 0.5 * (rf[2:end] + rf[1:end-1]), a LinRange from 0.5 (rf[2] + rf[1]) to
+ * 0.5 (rf[M+1] + rf[M]) of M points; dr = step(rc) = (last - first) / (M - 1);
+ * likewise in z; dV(i) = 2 pi rc[i] dr dz.  steps = (dr, dz, step(rf), step(zf)). */
+void ref_cgrid_geometry(double R, double L, int32_t M, int32_t N, double steps[4], double *dv)
+{
+    double r0 = 0.5 * (lerpi(1, M, 0.0, R) + lerpi(0, M, 0.0, R));
+    double r1 = 0.5 * (lerpi(M, M, 0.0, R) + lerpi(M - 1, M, 0.0, R));
+    double z0 = 0.5 * (lerpi(1, N, 0.0, L) + lerpi(0, N, 0.0, L));
+    double z1 = 0.5 * (lerpi(N, N, 0.0, L) + lerpi(N - 1, N, 0.0, L));
+    double dr = (r1 - r0) / (double)(M - 1), dz = (z1 - z0) / (double)(N - 1);
+    steps[0] = dr;
+    steps[1] = dz;
+    steps[2] = (R - 0.0) / (double)M;
+    steps[3] = (L - 0.0) / (double)N;
+    double two_pi = 2.0 * 3.141592653589793;
+    for (int32_t i = 0; i < M; ++i) dv[i] = two_pi * lerpi(i, M - 1, r0, r1) * dr * dz;
+}
+
+/* fld(x, y), y > 0 (Julia Base: the largest integer q with q <= x / y, the exact
+ * quotient): floor of the rounded quotient, lowered by one when q y exceeds x. */
+double ref_fld(double x, double y)
+{
+    double q = floor(x / y);
+    if (fma(-q, y, x) < 0.0) q -= 1.0;
+    else if (fma(-(q + 1.0), y, x) >= 0.0) q += 1.0;
+    return q;
+}
+
+/* cellindex (grid.jl:84-90) with the bounds tests of density! and track1 (the
+ * ghosted array holds i in 0..M+1, j in 0..N+1; rc holds i in 1..M): flat index
+ * i + (M+2) j, or -1; *ci = i. */
+static int64_t cyl_cell(const ref_cgrid *g, const double x[3], int64_t *ci)
+{
+    double rho = sqrt(x[0] * x[0] + x[1] * x[1]);
+    double fi = ref_fld(rho - 0.0, g->drf), fj = ref_fld(x[2] - 0.0, g->dzf);
+    if (!(fi >= 0.0 && fi < (double)g->M && fj >= -1.0 && fj <= (double)g->N)) return -1;
+    *ci = (int64_t)fi + 1;
+    return *ci + (int64_t)(g->M + 2) * ((int64_t)fj + 1);
+}
+
+/* FieldInterp (gridfields.jl:115-190) on the axisymmetric grid: zero unless
+ * 0 < r < R and 0 < z < L (inside, grid.jl:45-49); normdivrem (x / y, trunc) and
+ * staggered per axis (:94-109); er bilinear on (i + 1, j1), ez on (i1, j + 1)
+ * with the 0-based origin of the offset arrays; er rotated by x / r, y / r. */
+static void cyl_efield(const ref_cgrid *g, const double x[3], double e[3], int32_t *flags)
+{
+    double r = sqrt(x[0] * x[0] + x[1] * x[1]);
+    if (!(0.0 < r && r < g->R && 0.0 < x[2] && x[2] < g->L)) {
+        e[0] = 0.0;
+        e[1] = 0.0;
+        e[2] = 0.0;
+        return;
+    }
+    double qr = r / g->dr, qz = x[2] / g->dz;
+    int64_t i = (int64_t)trunc(qr), j = (int64_t)trunc(qz);
+    if (i > g->M) i = g->M;   /* never taken: r < R bounds qr by M */
+    if (j > g->N) j = g->N;
+    double dr = qr - (double)i, dz = qz - (double)j;
+    int64_t i1, j1;
+    double dr1, dz1;
+    if (dr < 0.5) { i1 = i; dr1 = dr + 0.5; } else { i1 = i + 1; dr1 = dr - 0.5; }
+    if (dz < 0.5) { j1 = j; dz1 = dz + 0.5; } else { j1 = j + 1; dz1 = dz - 0.5; }
+    int64_t S = g->M + 3;
+    const double *er = g->er, *ez = g->ez;
+    int64_t I = i + 1, J = j + 1;
+    double er1 = er[I + S * j1] * (1.0 - dr) * (1.0 - dz1) + er[I + 1 + S * j1] * dr * (1.0 - dz1) +
+                 er[I + S * (j1 + 1)] * (1.0 - dr) * dz1 + er[I + 1 + S * (j1 + 1)] * dr * dz1;
+    double ez1 = ez[i1 + S * J] * (1.0 - dr1) * (1.0 - dz) + ez[i1 + 1 + S * J] * dr1 * (1.0 - dz) +
+                 ez[i1 + S * (J + 1)] * (1.0 - dr1) * dz + ez[i1 + 1 + S * (J + 1)] * dr1 * dz;
+    e[0] = er1 * x[0] / r;
+    e[1] = er1 * x[1] / r;
+    e[2] = ez1;
+    double e2 = (e[0] * e[0] + e[1] * e[1]) + e[2] * e[2];
+    if (e2 > MCS_EFIELD_LIMIT2) *flags |= FLAG_FIELD;
+}
+
+/* density! (poisson.jl:65-84) on the axisymmetric grid, particles in order. */
+void ref_cyl_density(const ref_pop *p, ref_cgrid *g, double *arr, double val)
+{
+    for (int64_t k = 0; k < p->n; ++k) {
+        if (!p->active[k]) continue;
+        int64_t i;
+        int64_t c = cyl_cell(g, p->x + 3 * k, &i);
+        if (c < 0) continue;
+        arr[c] += p->w[k] * val / g->dv[i - 1];
+    }
+}
+
+/* poisson.jl:19-33: q = Float64(Float32(e (qfixed + qpart))) on the valid cells
+ * (one thread slice, no fluid electrons, NullDenoiser). */
+void ref_cyl_charge(ref_cgrid *g)
+{
+    int64_t su = g->M + 2;
+    for (int64_t j = 1; j <= g->N; ++j)
+        for (int64_t i = 1; i <= g->M; ++i) {
+            int64_t c = i + su * j;
+            double q0 = g->qfixed[c];
+            q0 += g->qpart[c];
+            q0 *= MCS_ELEMENTARY_CHARGE;
+            g->q[c] = (double)(float)q0;
+        }
+}
+
+/* poisson.jl:50-58: face fields from u, then the ghost copies. */
+void ref_cyl_faces(ref_cgrid *g, double eb)
+{
+    int64_t M = g->M, N = g->N, su = M + 2, sf = M + 3;
+    for (int64_t i = 1; i <= M + 1; ++i)
+        for (int64_t j = 1; j <= N + 1; ++j) {
+            g->er[i + sf * j] = (g->u[(i - 1) + su * j] - g->u[i + su * j]) / g->dr;
+            g->ez[i + sf * j] = eb + (g->u[i + su * (j - 1)] - g->u[i + su * j]) / g->dz;
+        }
+    for (int64_t j = 0; j <= N + 2; ++j) g->ez[0 + sf * j] = g->ez[1 + sf * j];
+    for (int64_t j = 0; j <= N + 2; ++j) g->ez[(M + 1) + sf * j] = g->ez[M + sf * j];
+    for (int64_t i = 0; i <= M + 2; ++i) g->er[i] = g->er[i + sf];
+    for (int64_t i = 0; i <= M + 2; ++i) g->er[i + sf * (N + 1)] = g->er[i + sf * N];
+}
+
 void ref_efield(const ref_field *f, const double x[3], double e[3], int32_t *flags)
 {
+    if (f->kind == 2) {
+        cyl_efield(f->cgrid, x, e, flags);
+        return;
+    }
     if (f->kind == 0) {
         e[0] = f->e[0];
         e[1] = f->e[1];
@@ -247,6 +379,16 @@ static void track1(ref_mesh *m, const double x[3], double val)
     m->qfixed[cidx(m->n, c[0] + 1, c[1] + 1, c[2] + 1)] += val / dV;
 }
 
+/* track1 on the axisymmetric grid (gridfields.jl:203-213). */
+static void track1_cyl(ref_cgrid *g, const double x[3], double val)
+{
+    if (!g) return;
+    int64_t i;
+    int64_t c = cyl_cell(g, x, &i);
+    if (c < 0) return;
+    g->qfixed[c] += val / g->dv[i - 1];
+}
+
 /* add_particle! (JuMC/src/population.jl:88-97) */
 static int append(ref_pop *p, const double x[3], const double v[3], double w, double s,
                   uint8_t active)
@@ -272,9 +414,31 @@ static int append(ref_pop *p, const double x[3], const double v[3], double w, do
  * The particle range is fixed when the loop starts, so children are not
  * advanced in the step they are born; a particle removed by attachment keeps
  * flying and colliding until its time step is used up. */
+static int32_t advance_impl(ref_pop *p, const ref_table *t, const ref_field *f, double dt,
+                            ref_mesh *tracker, ref_cgrid *ctracker, uint64_t seed, uint32_t instance,
+                            uint32_t step, ref_stats *st);
+
 int32_t ref_advance(ref_pop *p, const ref_table *t, const ref_field *f, double dt,
                     ref_mesh *tracker, uint64_t seed, uint32_t instance, uint32_t step,
                     ref_stats *st)
+{
+    return advance_impl(p, t, f, dt, tracker, NULL, seed, instance, step, st);
+}
+
+/* advance! in the axisymmetric FieldInterp of g, tracking into g->qfixed. */
+int32_t ref_advance_cyl(ref_pop *p, const ref_table *t, ref_cgrid *g, double dt, int32_t track,
+                        uint64_t seed, uint32_t instance, uint32_t step, ref_stats *st)
+{
+    ref_field f;
+    memset(&f, 0, sizeof(f));
+    f.kind = 2;
+    f.cgrid = g;
+    return advance_impl(p, t, &f, dt, NULL, track ? g : NULL, seed, instance, step, st);
+}
+
+static int32_t advance_impl(ref_pop *p, const ref_table *t, const ref_field *f, double dt,
+                            ref_mesh *tracker, ref_cgrid *ctracker, uint64_t seed, uint32_t instance,
+                            uint32_t step, ref_stats *st)
 {
     memset(st, 0, sizeof(*st));
     int64_t n0 = p->n;
@@ -345,6 +509,7 @@ int32_t ref_advance(ref_pop *p, const ref_table *t, const ref_field *f, double d
                             }
                             double sc = mcs_nextcoll(&rs);
                             track1(tracker, x, w);
+                            track1_cyl(ctracker, x, w);
                             /* apply!: parent state written, child appended */
                             if (append(p, x, vc, w, sc, act))
                                 st->births += 1;
@@ -352,6 +517,7 @@ int32_t ref_advance(ref_pop *p, const ref_table *t, const ref_field *f, double d
                                 st->flags |= FLAG_CAPACITY;
                         } else if (kind == 4) { /* Attachment, electron.jl:129-131 */
                             track1(tracker, x, -w);
+                            track1_cyl(ctracker, x, -w);
                             if (act) st->deaths += 1;
                             act = 0;
                         }

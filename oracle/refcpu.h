@@ -62,11 +62,27 @@ typedef struct ref_mesh {
     int64_t *cell_start; /* ncells + 2 */
 } ref_mesh;
 
+/* Axisymmetric grid (MCStreamer/src/grid.jl:4-90): M x N cells over [0,R] x [0,L]. */
+typedef struct ref_cgrid {
+    int32_t M, N;
+    double R, L;
+    double dr, dz;       /* step(rc), step(zc) */
+    double drf, dzf;     /* step(rf), step(zf) */
+    double *dv;          /* dV(grid, i), i = 1..M */
+    double *qfixed;      /* (M+2)(N+2), r fastest */
+    double *qpart;
+    double *q;
+    double *u;
+    double *er;          /* (M+3)(N+3) */
+    double *ez;
+} ref_cgrid;
+
 typedef struct ref_field {
-    int32_t kind;        /* 0 uniform, 1 mesh */
+    int32_t kind;        /* 0 uniform, 1 mesh, 2 axisymmetric grid */
     int32_t pad;
     double e[3];
     const ref_mesh *mesh;
+    const ref_cgrid *cgrid;
 } ref_field;
 
 typedef struct ref_stats {
@@ -148,6 +164,13 @@ int32_t ref_swarm_resample(ref_pop *p, int64_t n0, double tol, uint64_t seed,
                            uint32_t instance, uint32_t step, int32_t *did);
 
 /* mesh pieces */
+void ref_cgrid_geometry(double R, double L, int32_t M, int32_t N, double steps[4], double *dv);
+double ref_fld(double x, double y);
+int32_t ref_advance_cyl(ref_pop *p, const ref_table *t, ref_cgrid *g, double dt, int32_t track,
+                        uint64_t seed, uint32_t instance, uint32_t step, ref_stats *st);
+void ref_cyl_density(const ref_pop *p, ref_cgrid *g, double *arr, double val);
+void ref_cyl_charge(ref_cgrid *g);
+void ref_cyl_faces(ref_cgrid *g, double eb);
 int64_t ref_cell_key(const ref_mesh *m, const double x[3]);
 int32_t ref_sort_by_cell(ref_pop *p, ref_mesh *m, ref_pop *scratch);
 void ref_deposit(const ref_pop *p, ref_mesh *m, int32_t which, int32_t scheme, double val);

@@ -52,9 +52,16 @@ class RefMesh(ctypes.Structure):
                 ("ey", _PD), ("ez", _PD), ("cell_start", _PI64)]
 
 
+class RefCGrid(ctypes.Structure):
+    _fields_ = [("M", ctypes.c_int32), ("N", ctypes.c_int32), ("R", ctypes.c_double), ("L", ctypes.c_double),
+                ("dr", ctypes.c_double), ("dz", ctypes.c_double), ("drf", ctypes.c_double),
+                ("dzf", ctypes.c_double), ("dv", _PD), ("qfixed", _PD), ("qpart", _PD), ("q", _PD),
+                ("u", _PD), ("er", _PD), ("ez", _PD)]
+
+
 class RefField(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("pad", ctypes.c_int32), ("e", ctypes.c_double * 3),
-                ("mesh", ctypes.POINTER(RefMesh))]
+                ("mesh", ctypes.POINTER(RefMesh)), ("cgrid", ctypes.POINTER(RefCGrid))]
 
 
 class RefStats(ctypes.Structure):
@@ -117,6 +124,20 @@ def lib():
         L.ref_swarm_resample.argtypes = [P(RefPop), ctypes.c_int64, ctypes.c_double,
                                          ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _PI32]
         L.ref_swarm_resample.restype = ctypes.c_int32
+        L.ref_cgrid_geometry.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
+                                         _PD, _PD]
+        L.ref_cgrid_geometry.restype = None
+        L.ref_fld.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.ref_fld.restype = ctypes.c_double
+        L.ref_advance_cyl.argtypes = [P(RefPop), P(RefTable), P(RefCGrid), ctypes.c_double, ctypes.c_int32,
+                                      ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, P(RefStats)]
+        L.ref_advance_cyl.restype = ctypes.c_int32
+        L.ref_cyl_density.argtypes = [P(RefPop), P(RefCGrid), _PD, ctypes.c_double]
+        L.ref_cyl_density.restype = None
+        L.ref_cyl_charge.argtypes = [P(RefCGrid)]
+        L.ref_cyl_charge.restype = None
+        L.ref_cyl_faces.argtypes = [P(RefCGrid), ctypes.c_double]
+        L.ref_cyl_faces.restype = None
         L.ref_cell_key.argtypes = [P(RefMesh), _PD]
         L.ref_cell_key.restype = ctypes.c_int64
         L.ref_sort_by_cell.argtypes = [P(RefPop), P(RefMesh), P(RefPop)]
@@ -251,6 +272,73 @@ class Mesh:
         m.cell_start = self.cell_start.ctypes.data_as(_PI64)
         self._c = m
         return m
+
+
+class CGrid:
+    """Axisymmetric grid Grid(R, L, M, N) with GridFields' arrays (r fastest):
+    qfixed, qpart, q, u (M+2)(N+2); er, ez (M+3)(N+3)."""
+
+    def __init__(self, R, L, M, N):
+        self.R, self.L, self.M, self.N = float(R), float(L), int(M), int(N)
+        self.steps = np.zeros(4)
+        self.dv = np.zeros(self.M)
+        lib().ref_cgrid_geometry(self.R, self.L, self.M, self.N, _d(self.steps), _d(self.dv))
+        c = (self.M + 2) * (self.N + 2)
+        f = (self.M + 3) * (self.N + 3)
+        self.qfixed, self.qpart, self.q, self.u = (np.zeros(c) for _ in range(4))
+        self.er, self.ez = np.zeros(f), np.zeros(f)
+        self._c = None
+
+    def cstruct(self):
+        g = RefCGrid()
+        g.M, g.N, g.R, g.L = self.M, self.N, self.R, self.L
+        g.dr, g.dz, g.drf, g.dzf = (float(v) for v in self.steps)
+        g.dv = _d(self.dv)
+        for k in ("qfixed", "qpart", "q", "u", "er", "ez"):
+            setattr(g, k, _d(getattr(self, k)))
+        self._c = g
+        return g
+
+
+def fld(x, y):
+    return lib().ref_fld(float(x), float(y))
+
+
+def advance_cyl(pop, table, grid, dt, track=True, seed=0, instance=0, step=0):
+    p = pop.cstruct()
+    t = table.cstruct()
+    st = RefStats()
+    g = grid.cstruct()
+    lib().ref_advance_cyl(ctypes.byref(p), ctypes.byref(t), ctypes.byref(g), float(dt), int(bool(track)),
+                          int(seed), int(instance), int(step), ctypes.byref(st))
+    pop.sync_from(p)
+    return st
+
+
+def cyl_efield(grid, x):
+    """The axisymmetric FieldInterp at x; returns (e, flags)."""
+    f = RefField()
+    f.kind = 2
+    g = grid.cstruct()
+    f.cgrid = ctypes.pointer(g)
+    e = np.zeros(3)
+    fl = ctypes.c_int32(0)
+    xx = np.ascontiguousarray(x, dtype=np.float64)
+    lib().ref_efield(ctypes.byref(f), _d(xx), _d(e), ctypes.byref(fl))
+    return e, fl.value
+
+
+def cyl_density(pop, grid, which, val):
+    p = pop.cstruct()
+    lib().ref_cyl_density(ctypes.byref(p), ctypes.byref(grid.cstruct()), _d(getattr(grid, which)), float(val))
+
+
+def cyl_charge(grid):
+    lib().ref_cyl_charge(ctypes.byref(grid.cstruct()))
+
+
+def cyl_faces(grid, eb):
+    lib().ref_cyl_faces(ctypes.byref(grid.cstruct()), float(eb))
 
 
 def uniform_field(e):

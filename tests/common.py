@@ -1,5 +1,6 @@
 """Shared builders for the tests: synthetic tables, populations, meshes."""
 import os
+import tempfile
 
 import numpy as np
 
@@ -17,9 +18,8 @@ def gas_table(ne=4000, egrid="log", o2=0.0, tmpdir=None):
         e_ev = np.geomspace(1e-3, 1e3, ne)
     else:
         e_ev = np.linspace(0.0, 1e3, ne)
-    d = tmpdir or os.path.join(HERE, "..", "_work")
-    os.makedirs(d, exist_ok=True)
-    path = os.path.join(d, f"gas_{egrid}_{ne}_{int(o2 > 0)}.json")
+    d = tmpdir or tempfile.gettempdir()
+    path = os.path.join(d, f"mcs_gas_{egrid}_{ne}_{int(o2 > 0)}_{os.getpid()}.json")
     gases.write_gas(path, e_ev, with_attacher=o2 > 0)
     proc, rate, maxrate, _ = lxcat.load_lxcat([path], gases.densities(o2), e_ev * co.eV)
     return proc, e_ev * co.eV, rate, maxrate

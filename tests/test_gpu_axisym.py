@@ -1,0 +1,181 @@
+"""The axisymmetric (r, z) grid on the GPU against the oracle: cylindrical
+density! (bitwise: per cell, terms in particle order), poisson!'s charge, solve
+and face fields (bitwise), the particle step in the axisymmetric FieldInterp
+with the ion tracker (particles bitwise; the tracker's same-cell additions are
+atomics, so qfixed within 1e-12), and a few Poisson + advance + repack steps."""
+import numpy as np
+import pytest
+
+from mcstreamer_amd import axisym, jumc
+from mcstreamer_amd import constants as co
+from oracle import refcpu as R
+from tests.common import gas_table, ref_table
+
+pytestmark = pytest.mark.gpu
+
+RR, LL, MM, NN = 0.64e-3, 1.28e-3, 64, 128
+
+
+def _states(n, seed, grow=1.05):
+    """Electrons over a cylinder a little larger than the grid (so some fall in
+    the z ghost rows or outside), some inactive, 1 eV Maxwellian."""
+    rng = np.random.default_rng(seed)
+    r = RR * grow * np.sqrt(rng.uniform(size=n))
+    th = rng.uniform(0, 2 * np.pi, n)
+    z = rng.uniform(-0.03 * LL, LL * 1.03, n)
+    x = np.stack([r * np.cos(th), r * np.sin(th), z], axis=1)
+    v = rng.normal(size=(n, 3)) * np.sqrt(co.eV / co.electron_mass)
+    w = rng.integers(1, 4, n).astype(np.float64)
+    s = -np.log1p(-rng.uniform(size=n))
+    act = (rng.uniform(size=n) > 0.05).astype(np.uint8)
+    return x, v, w, s, act
+
+
+def _setup(n=20000, seed=1, o2=0.2):
+    proc, energy, rate, maxrate = gas_table(o2=o2)
+    table = jumc.CollisionTable(proc, energy, rate, maxrate, egrid="log")
+    T = ref_table(proc, energy, rate, maxrate)
+    x, v, w, s, act = _states(n, seed)
+    popl = jumc.Population(4 * n, jumc.ParticleStates(x, v, w, s, act), table, seed=11)
+    P = R.Pop(x, v, w, s, act, cap=4 * n)
+    grid = axisym.Grid(RR, LL, MM, NN)
+    fields = axisym.GridFields(grid)
+    G = R.CGrid(RR, LL, MM, NN)
+    return table, T, popl, P, grid, fields, G
+
+
+def _shape(G, name):
+    return (G.N + 3, G.M + 3) if name in ("er", "ez") else (G.N + 2, G.M + 2)
+
+
+def _smooth_potential(G, amp=40.0, seed=3):
+    """A potential with fields of a few MV/m: a Gaussian bump plus random noise."""
+    rng = np.random.default_rng(seed)
+    dr, dz = G.steps[0], G.steps[1]
+    I = (np.arange(G.M + 2)[None, :] - 0.5) * dr
+    J = (np.arange(G.N + 2)[:, None] - 0.5) * dz
+    u = amp * np.exp(-(I / (0.3 * RR)) ** 2 - ((J - 0.6 * LL) / (0.2 * LL)) ** 2)
+    return u + 0.01 * amp * rng.normal(size=u.shape)
+
+
+def test_geometry_matches_oracle():
+    f = axisym.GridFields(axisym.Grid(RR, LL, MM, NN))
+    np.testing.assert_array_equal(f.steps(), R.CGrid(RR, LL, MM, NN).steps)
+    g = axisym.Grid(3e-3, 1e-2, 300, 1000)
+    np.testing.assert_array_equal(axisym.GridFields(g).steps(), R.CGrid(3e-3, 1e-2, 300, 1000).steps)
+
+
+def test_density_matches_oracle_bitwise():
+    table, T, popl, P, grid, fields, G = _setup(n=50000)
+    rng = np.random.default_rng(5)
+    q0 = rng.normal(size=_shape(G, "qfixed")) * 1e15
+    fields.set("qfixed", q0)
+    G.qfixed[:] = q0.ravel()
+    axisym.density(fields, "qfixed", popl, 1.0)
+    R.cyl_density(P, G, "qfixed", 1.0)
+    axisym.density(fields, "qpart", popl, -1.0)
+    R.cyl_density(P, G, "qpart", -1.0)
+    axisym.density(fields, "qpart", popl, 0.5)      # accumulates into what is there
+    R.cyl_density(P, G, "qpart", 0.5)
+    for k in ("qfixed", "qpart"):
+        np.testing.assert_array_equal(fields.get(k).ravel(), getattr(G, k), err_msg=k)
+    # the z ghost rows received the particles just outside in z, as the reference's arrays do
+    qp = fields.get("qpart")
+    assert np.abs(qp[0]).sum() > 0 and np.abs(qp[-1]).sum() > 0
+
+
+def test_faces_match_oracle_bitwise():
+    G = R.CGrid(RR, LL, MM, NN)
+    fields = axisym.GridFields(axisym.Grid(RR, LL, MM, NN))
+    u = _smooth_potential(G)
+    fields.set("u", u)
+    G.u[:] = u.ravel()
+    eb = -5e6
+    from mcstreamer_amd import _lib
+    _lib.check(fields.ctx.handle, _lib.lib().mcs_cyl_efield_faces(fields.handle, eb))
+    R.cyl_faces(G, eb)
+    for k in ("er", "ez"):
+        np.testing.assert_array_equal(fields.get(k).ravel(), getattr(G, k), err_msg=k)
+
+
+def _ref_poisson(G, P, conf, eb):
+    G.qpart[:] = 0.0
+    R.cyl_density(P, G, "qpart", -1.0)
+    R.cyl_charge(G)
+    rc, it, res = R.mg_solve((G.M, G.N), conf.bc, conf.s, conf.levels, conf.smooth1, conf.smooth2,
+                             conf.tolerance, conf.maxiter, G.u, G.q, cyl=1)
+    R.cyl_faces(G, eb)
+    return it
+
+
+def test_poisson_matches_oracle_bitwise():
+    table, T, popl, P, grid, fields, G = _setup(n=40000, seed=2)
+    # ions where the electrons are, less a few: a net charge of the electrons' order
+    axisym.density(fields, "qfixed", popl, 0.9)
+    R.cyl_density(P, G, "qfixed", 0.9)
+    conf = axisym.poisson_config(grid, tolerance=-1e-10)
+    ws = axisym.mgmod.Workspace(conf, (MM, NN))
+    eb = -5e6
+    it, res = axisym.poisson(fields, popl, eb, ws)
+    it_ref = _ref_poisson(G, P, conf, eb)
+    print(f"mg iterations {it} / oracle {it_ref}", flush=True)
+    assert it == it_ref
+    for k in ("q", "u", "er", "ez"):
+        np.testing.assert_array_equal(fields.get(k).ravel(), getattr(G, k), err_msg=k)
+    assert np.abs(G.er).max() > 1e4
+
+
+def test_advance_in_axisymmetric_field_matches_oracle():
+    table, T, popl, P, grid, fields, G = _setup(n=20000, seed=4)
+    u = _smooth_potential(G, amp=60.0)
+    fields.set("u", u)
+    G.u[:] = u.ravel()
+    from mcstreamer_amd import _lib
+    _lib.check(fields.ctx.handle, _lib.lib().mcs_cyl_efield_faces(fields.handle, -5e6))
+    R.cyl_faces(G, -5e6)
+    field, tracker = axisym.FieldInterp(fields), axisym.CollisionTracker(fields)
+    dt = 1e-12
+    for step in range(3):
+        st = jumc.advance(popl, field, dt, tracker, step=step)
+        rs = R.advance_cyl(P, T, G, dt, track=True, seed=popl.seed, step=step)
+        print(f"step {step}: n {rs.n_out} births {rs.births} deaths {rs.deaths}", flush=True)
+        assert (st.n_out, st.births, st.deaths, st.flights) == (rs.n_out, rs.births, rs.deaths, rs.flights)
+        assert list(st.events)[:len(T.kind)] == list(rs.events)[:len(T.kind)]
+    assert rs.births > 0 and rs.deaths > 0
+    g = popl.download()
+    r = P.view()
+    np.testing.assert_array_equal(g.id, r["id"])
+    np.testing.assert_array_equal(g.active, r["active"])
+    for k in ("x", "v", "w", "s"):
+        np.testing.assert_array_equal(getattr(g, k), r[k], err_msg=k)
+    qf = fields.get("qfixed").ravel()
+    np.testing.assert_allclose(qf, G.qfixed, rtol=1e-12, atol=1e-12 * np.abs(G.qfixed).max())
+    assert np.abs(G.qfixed).max() > 0
+
+
+def test_axisymmetric_streamer_steps():
+    """poisson! + advance! (field and tracker of the same grid) + repack!, four
+    steps, against the same chain in the oracle."""
+    table, T, popl, P, grid, fields, G = _setup(n=20000, seed=6)
+    axisym.density(fields, "qfixed", popl, 1.0)
+    R.cyl_density(P, G, "qfixed", 1.0)
+    conf = axisym.poisson_config(grid)
+    ws = axisym.mgmod.Workspace(conf, (MM, NN))
+    field, tracker = axisym.FieldInterp(fields), axisym.CollisionTracker(fields)
+    eb, dt = -5e6, 1e-12
+    for step in range(4):
+        it, _ = axisym.poisson(fields, popl, eb, ws)
+        it_ref = _ref_poisson(G, P, conf, eb)
+        st = jumc.advance(popl, field, dt, tracker, step=step)
+        rs = R.advance_cyl(P, T, G, dt, track=True, seed=popl.seed, step=step)
+        jumc.repack(popl)
+        R.repack(P)
+        print(f"step {step}: mg {it}/{it_ref} n {P.n} births {rs.births}", flush=True)
+        assert it == it_ref
+        assert (st.births, st.deaths) == (rs.births, rs.deaths)
+    g = popl.download()
+    r = P.view()
+    np.testing.assert_array_equal(g.id, r["id"])
+    for k in ("x", "v", "w", "s"):
+        np.testing.assert_allclose(getattr(g, k), r[k], rtol=1e-12, atol=0, err_msg=k)
+    np.testing.assert_allclose(fields.get("u").ravel(), G.u, rtol=1e-10, atol=1e-12 * np.abs(G.u).max())
