@@ -548,6 +548,129 @@ def run_mg2d(args):
     return result
 
 
+def run_axisym(args):
+    """The reference's production geometry (MCStreamer/samples/input.toml: R = 1.5 cm,
+    L = 10 cm, 1536 x 10240 cells, eb = 15 kV/cm, dt = 1e-12 s, Poisson every step):
+    one step is poisson! (cylindrical density!, charge, axisymmetric multigrid with
+    8 levels and the reference's absolute tolerance 1e8 e, faces), advance! in the
+    axisymmetric FieldInterp with the ion tracker, and repack!.  The seed is a
+    Gaussian of width 1e-4 m at z0 = 1 cm on the axis ([seed] of the sample input)
+    of `--axisym-n` electrons over as many ions; C1 gas (N2-like + 20 % O2-like)."""
+    from mcstreamer_amd import axisym, gases, jumc, lxcat
+    from mcstreamer_amd import constants as co
+
+    ctx = jumc.context()
+    e_ev = np.geomspace(1e-3, 1e3, 4000)
+    path = os.path.join(tempfile.gettempdir(), f"mcs_bench_gas_c1_{os.getpid()}.json")
+    gases.write_gas(path, e_ev, with_attacher=True)
+    proc, rate, maxrate, _ = lxcat.load_lxcat([path], gases.densities(0.2), e_ev * co.eV)
+    table = jumc.CollisionTable(proc, e_ev * co.eV, rate, maxrate, egrid="log")
+    M, N = args.mg2d_grid
+    grid = axisym.Grid(0.015, 0.1, M, N)
+    fields = axisym.GridFields(grid)
+    n = int(args.axisym_n)
+    rng = np.random.default_rng(args.seed)
+    wseed = 1e-4
+    x = rng.normal(size=(n, 3)) * wseed
+    x[:, 2] += 1e-2
+    v = np.zeros((n, 3))
+    s = -np.log1p(-rng.uniform(size=n))
+    popl = jumc.Population(3 * n, jumc.ParticleStates(x, v, np.ones(n), s, np.ones(n, dtype=np.uint8)), table,
+                           seed=args.seed)
+    del x, v, s
+    axisym.density(fields, "qfixed", popl, 1.0)
+    # 8 levels (6 x 40 coarsest): the reference's default 9 leaves 3 cells in r, an
+    # odd size for which redblack2 (redblack.jl:38-46) visits the ghost column
+    conf = axisym.poisson_config(grid, levels=args.mg2d_levels)
+    ws = axisym.mgmod.Workspace(conf, (M, N))
+    field, tracker = axisym.FieldInterp(fields), axisym.CollisionTracker(fields)
+    eb, dt = 1.5e6, 1e-12
+
+    def step(i):
+        it, _ = axisym.poisson(fields, popl, eb, ws)
+        st = jumc.advance(popl, field, dt, tracker, step=i)
+        jumc.repack(popl)
+        return it, st
+
+    for i in range(args.warmup):
+        it, st = step(i)
+        print(f"warmup {i}: n={st.n_in} mg_iters={it} births={st.births}", file=sys.stderr, flush=True)
+    ctx.synchronize()
+    ctx.profile_reset()
+    ctx.profile(True)
+    psteps, iters = 0, []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        it, st = step(args.warmup + i)
+        psteps += st.n_in
+        iters.append(it)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    ctx.profile(False)
+    kt = {}
+    for k in ("advance", "cyl_density", "cyl_faces", "mg2_sweep0", "repack"):
+        ms, cnt = ctx.profile_read(k)
+        kt[k] = {"ms_total": ms, "launches": cnt}
+    adv_s = kt["advance"]["ms_total"] / 1e3 / max(kt["advance"]["launches"], 1)
+    bytes_per_launch = BYTES_PER_PSTEP_F64 * psteps / max(args.steps, 1)
+    ach = bytes_per_launch / adv_s / 1e9 if adv_s > 0 else 0.0
+    result = {
+        "metric": "particle-steps/s (axisymmetric streamer step, production grid)",
+        "value": psteps / el, "unit": "particle-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"FP64 axisymmetric streamer step, {M}x{N} cells (R=1.5 cm, L=10 cm), "
+                               f"{n:g}-electron Gaussian seed (w=1e-4 m, z0=1 cm), eb=15 kV/cm, dt=1e-12 s, "
+                               "poisson! ({conf.levels} levels, tol 1e8 e) + advance! (cylindrical field, tracker) + repack!",
+                   "grid": [M, N], "particles_per_step": psteps / max(args.steps, 1), "fmg_iterations": iters,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_advance<2,1> (fused push + collisions, axisymmetric gather, ion tracker)",
+                     "avg_launch_ms": adv_s * 1e3, "bytes_per_launch": bytes_per_launch},
+        "kernels_ms": kt,
+    }
+    if not args.no_cpu_baseline:
+        from oracle import refcpu as Rf
+        # bounded sample: one oracle poisson! on the full grid from the device state,
+        # and oracle advance! steps of the first particles until 8 s; the rate is
+        # particles / (poisson time + per-particle advance time x particles)
+        ps = popl.download()
+        nn = len(ps)
+        G = Rf.CGrid(0.015, 0.1, M, N)
+        G.qfixed[:] = fields.get("qfixed").ravel()
+        G.u[:] = fields.get("u").ravel()
+        P = Rf.Pop(ps.x, ps.v, ps.w, ps.s, ps.active, ps.id, cap=3 * nn + 1000, next_id=ps.next_id)
+        t0 = time.perf_counter()
+        G.qpart[:] = 0.0
+        Rf.cyl_density(P, G, "qpart", -1.0)
+        Rf.cyl_charge(G)
+        rcq, itq, _ = Rf.mg_solve((M, N), conf.bc, conf.s, conf.levels, conf.smooth1, conf.smooth2,
+                                  conf.tolerance, conf.maxiter, G.u, G.q, cyl=1)
+        Rf.cyl_faces(G, eb)
+        t_poisson = time.perf_counter() - t0
+        k = min(nn, 200_000)
+        S = Rf.Pop(ps.x[:k], ps.v[:k], ps.w[:k], ps.s[:k], ps.active[:k], ps.id[:k], cap=3 * k + 1000,
+                   next_id=ps.next_id)
+        done, t0 = 0, time.perf_counter()
+        el_a = 0.0
+        while done == 0 or el_a < 8.0:
+            done += S.n
+            Rf.advance_cyl(S, Rf.Table(table.energy, table.rate, table.kind, table.param, table.maxrate,
+                                       egrid=table.egrid), G, dt, track=True, seed=args.seed, step=10_000 + done)
+            Rf.repack(S)
+            el_a = time.perf_counter() - t0
+        per_p = el_a / done
+        result["cpu_baseline"] = {
+            "value": nn / (t_poisson + per_p * nn), "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"one oracle poisson! on the full grid ({itq} fmg iterations, {t_poisson:.1f} s) plus "
+                      f"{done} oracle particle-steps of the first {k} particles ({el_a:.1f} s), scaled to the "
+                      f"{nn} particles of one step; single-threaded"}
+    else:
+        result["cpu_baseline"] = None
+    return result
+
+
 def copy_calibration(nbytes=1 << 30, reps=3):
     """A plain device copy of known size: its FETCH_SIZE/WRITE_SIZE counters give
     the factor that turns the counters of the step's kernels into bytes."""
@@ -566,7 +689,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["streamer", "swarm", "batch", "serve", "precision", "mg2d"],
+    ap.add_argument("--workload", choices=["streamer", "swarm", "batch", "serve", "precision", "mg2d", "axisym"],
                     default="streamer")
     ap.add_argument("--c2-pairs", type=int, default=40_000_000)
     ap.add_argument("--instances", type=int, default=1024)
@@ -583,6 +706,7 @@ def main():
     ap.add_argument("--mesh", type=int, nargs=3, default=[128, 128, 256])
     ap.add_argument("--mg2d-grid", type=int, nargs=2, default=[1536, 10240])
     ap.add_argument("--mg2d-levels", type=int, default=8)
+    ap.add_argument("--axisym-n", type=float, default=2e7)
     ap.add_argument("--h", type=float, default=5e-6)
     ap.add_argument("--npairs", type=int, default=1_000_000)
     ap.add_argument("--swarm-n", type=int, default=10_000)
@@ -598,7 +722,8 @@ def main():
               "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
         sys.exit(2)
     res = {"streamer": run_streamer, "swarm": run_swarm, "batch": run_batch,
-           "serve": run_serve, "precision": run_precision, "mg2d": run_mg2d}[args.workload](args)
+           "serve": run_serve, "precision": run_precision, "mg2d": run_mg2d,
+           "axisym": run_axisym}[args.workload](args)
     print(json.dumps(res))
 
 

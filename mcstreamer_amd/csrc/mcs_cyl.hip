@@ -41,9 +41,10 @@ __global__ void kc_zero(double *a, int64_t n)
 // Key of each particle slot: its cell, or `none` when the particle is inactive or
 // outside what density! adds to; the value is the slot.
 __global__ __launch_bounds__(MCS_BLOCK) void kc_keys(PopBuf b, int64_t n, CylView C, uint32_t none,
-                                                     uint32_t *key, uint32_t *val)
+                                                     uint32_t *key, uint32_t *val, int32_t *longs)
 {
     const int64_t p = gidx();
+    if (p == 0) longs[0] = 0;   // the long-run list, filled by kc_runs
     if (p >= n) return;
     uint32_t k = none;
     if (b.active[p]) {
@@ -56,20 +57,100 @@ __global__ __launch_bounds__(MCS_BLOCK) void kc_keys(PopBuf b, int64_t n, CylVie
     val[p] = (uint32_t)p;
 }
 
-// One thread per run of equal keys in the stably sorted (key, slot) pairs: the
-// cell's terms (w val)/dV added to its value in slot order.
-__global__ __launch_bounds__(MCS_BLOCK) void kc_runs(const uint32_t *key, const uint32_t *val, int64_t n,
-                                                     uint32_t none, const double *w, const double *dv, int32_t M,
-                                                     double v, double *arr)
+// Run bounds of the stably sorted (key, slot) pairs: the first and one past the
+// last position of each cell's run, stored at the cell (only cells that have a
+// run are written, and only those are read).
+__global__ __launch_bounds__(MCS_BLOCK) void kc_bounds(const uint32_t *key, int64_t n, uint32_t none,
+                                                       int32_t *first, int32_t *last)
+{
+    const int64_t t = gidx();
+    if (t >= n) return;
+    const uint32_t k = key[t];
+    if (k == none) return;
+    if (t == 0 || key[t - 1] != k) first[k] = (int32_t)t;
+    if (t == n - 1 || key[t + 1] != k) last[k] = (int32_t)(t + 1);
+}
+
+#define CYL_LONG 256    // runs longer than this are summed by a wavefront each
+
+// The term (w val)/dV of every sorted pair, at its sorted position (parallel; the
+// runs below only add them up in order).
+__global__ __launch_bounds__(MCS_BLOCK) void kc_terms(const uint32_t *key, const uint32_t *val, int64_t n,
+                                                      uint32_t none, const double *w, const double *dv, int32_t M,
+                                                      double v, double *term)
+{
+    const int64_t t = gidx();
+    if (t >= n) return;
+    const uint32_t k = key[t];
+    if (k == none) return;
+    term[t] = (w[val[t]] * v) / dv[k % (uint32_t)(M + 2) - 1];
+}
+
+// One thread per run of at most CYL_LONG: the cell's terms added to its value in
+// slot order.  Longer runs are listed for kc_runs_long.
+__global__ __launch_bounds__(MCS_BLOCK) void kc_runs(const uint32_t *key, int64_t n, uint32_t none,
+                                                     const int32_t *first, const int32_t *last, const double *term,
+                                                     double *arr, int32_t *longs)
 {
     const int64_t t = gidx();
     if (t >= n) return;
     const uint32_t k = key[t];
     if (k == none || (t > 0 && key[t - 1] == k)) return;
-    const double d = dv[k % (uint32_t)(M + 2) - 1];
+    const int32_t a = first[k], b = last[k];
+    if (b - a > CYL_LONG) {
+        longs[1 + atomicAdd(&longs[0], 1)] = (int32_t)k;
+        return;
+    }
     double acc = arr[k];
-    for (int64_t q = t; q < n && key[q] == k; ++q) acc = acc + (w[val[q]] * v) / d;
+    int32_t q = a;
+    for (; q + 8 <= b; q += 8) {
+        double c[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) c[u] = term[q + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = acc + c[u];
+    }
+    for (; q < b; ++q) acc = acc + term[q];
     arr[k] = acc;
+}
+
+// Lane u's value of v, as a wave-uniform value (u uniform).
+__device__ __forceinline__ double lane_value(double v, int u)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), u);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), u);
+    return __hiloint2double(hi, lo);
+}
+
+// One wavefront per listed run: the run's terms come in chunks of 64, one per
+// lane (three chunks in flight), and every lane adds a chunk's terms in slot
+// order, reading them lane by lane, so the sum is the same sequence of additions
+// as one thread's.
+__global__ __launch_bounds__(MCS_BLOCK) void kc_runs_long(const int32_t *first, const int32_t *last,
+                                                          const double *term, double *arr, const int32_t *longs)
+{
+    const int lane = threadIdx.x & (MCS_WAVE - 1);
+    const int64_t wave = (int64_t)blockIdx.x * (MCS_BLOCK / MCS_WAVE) + threadIdx.x / MCS_WAVE;
+    const int64_t nwaves = (int64_t)gridDim.x * (MCS_BLOCK / MCS_WAVE);
+    const int32_t count = longs[0];
+    for (int64_t r = wave; r < count; r += nwaves) {
+        const uint32_t k = (uint32_t)longs[1 + r];
+        const int32_t a = first[k], b = last[k];
+        double acc = arr[k];
+        double c0 = a + lane < b ? term[a + lane] : 0.0;
+        double c1 = a + MCS_WAVE + lane < b ? term[a + MCS_WAVE + lane] : 0.0;
+        double c2 = a + 2 * MCS_WAVE + lane < b ? term[a + 2 * MCS_WAVE + lane] : 0.0;
+        for (int32_t q = a; q < b; q += MCS_WAVE) {
+            const int32_t qn = q + 3 * MCS_WAVE + lane;
+            const double c3 = qn < b ? term[qn] : 0.0;
+            const int m = b - q < MCS_WAVE ? b - q : MCS_WAVE;
+            for (int u = 0; u < m; ++u) acc = acc + lane_value(c0, u);
+            c0 = c1;
+            c1 = c2;
+            c2 = c3;
+        }
+        if (lane == 0) arr[k] = acc;
+    }
 }
 
 // q = Float64(Float32(e (qfixed + qpart))) on the valid cells (poisson.jl:19-33).
@@ -165,6 +246,9 @@ extern "C" int32_t mcs_cgrid_destroy(mcs_cgrid *g)
     for (int a = 0; a < 6; ++a)
         if (g->arr[a]) hipFree(g->arr[a]);
     if (g->dv) hipFree(g->dv);
+    if (g->run_first) hipFree(g->run_first);
+    if (g->run_last) hipFree(g->run_last);
+    if (g->run_long) hipFree(g->run_long);
     delete g;
     return MCS_OK;
 }
@@ -210,7 +294,8 @@ extern "C" int32_t mcs_cgrid_zero(mcs_cgrid *g, int32_t which)
     return MCS_OK;
 }
 
-// density!: stable radix sort of (cell, slot) pairs, then one thread per cell run.
+// density!: stable radix sort of (cell, slot) pairs, then one thread per cell run
+// (a wavefront per run longer than CYL_LONG).
 extern "C" int32_t mcs_cyl_density(mcs_pop *p, mcs_cgrid *g, int32_t which, double val)
 {
     if (!p || !g || (which != MCS_CARR_QFIXED && which != MCS_CARR_QPART)) return MCS_E_ARG;
@@ -228,16 +313,34 @@ extern "C" int32_t mcs_cyl_density(mcs_pop *p, mcs_cgrid *g, int32_t which, doub
     ProfScope ps(ctx, "cyl_density");
     const MeshView mv = cyl_view(g);
     const dim3 grid((unsigned)cdiv(n, MCS_BLOCK)), blk(MCS_BLOCK);
-    hipLaunchKernelGGL(kc_keys, grid, blk, 0, ctx->stream, p->buf[0], n, mv.C, none, kin, vin);
+    if (!g->run_first) {
+        const int64_t nc = centre_len(g);
+        if (hipMalloc(&g->run_first, sizeof(int32_t) * nc) != hipSuccess ||
+            hipMalloc(&g->run_last, sizeof(int32_t) * nc) != hipSuccess ||
+            hipMalloc(&g->run_long, sizeof(int32_t) * (nc + 1)) != hipSuccess)
+            return mcs_fail(ctx, MCS_E_HIP, "cyl_density: device allocation failed");
+    }
+    hipLaunchKernelGGL(kc_keys, grid, blk, 0, ctx->stream, p->buf[0], n, mv.C, none, kin, vin, g->run_long);
     MCS_LAUNCH_CHECK(ctx);
     size_t tb = 0;
     MCS_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tb, kin, kout, vin, vout, (size_t)n, 0u, bits, ctx->stream));
+    // one scratch buffer: the sort's temporary storage, then the terms
+    const size_t need = tb > sizeof(double) * (size_t)n ? tb : sizeof(double) * (size_t)n;
     void *tmp = nullptr;
-    int32_t rc = mcs_scratch(ctx, tb, &tmp);
+    int32_t rc = mcs_scratch(ctx, need, &tmp);
     if (rc) return rc;
     MCS_HIP(ctx, rocprim::radix_sort_pairs(tmp, tb, kin, kout, vin, vout, (size_t)n, 0u, bits, ctx->stream));
-    hipLaunchKernelGGL(kc_runs, grid, blk, 0, ctx->stream, (const uint32_t *)kout, (const uint32_t *)vout, n, none,
-                       (const double *)p->buf[0].w, (const double *)g->dv, g->M, val, g->arr[which]);
+    double *term = (double *)tmp;
+    hipLaunchKernelGGL(kc_bounds, grid, blk, 0, ctx->stream, (const uint32_t *)kout, n, none, g->run_first,
+                       g->run_last);
+    hipLaunchKernelGGL(kc_terms, grid, blk, 0, ctx->stream, (const uint32_t *)kout, (const uint32_t *)vout, n, none,
+                       (const double *)p->buf[0].w, (const double *)g->dv, g->M, val, term);
+    hipLaunchKernelGGL(kc_runs, grid, blk, 0, ctx->stream, (const uint32_t *)kout, n, none,
+                       (const int32_t *)g->run_first, (const int32_t *)g->run_last, (const double *)term,
+                       g->arr[which], g->run_long);
+    hipLaunchKernelGGL(kc_runs_long, dim3(512), blk, 0, ctx->stream, (const int32_t *)g->run_first,
+                       (const int32_t *)g->run_last, (const double *)term, g->arr[which],
+                       (const int32_t *)g->run_long);
     MCS_LAUNCH_CHECK(ctx);
     return MCS_OK;
 }

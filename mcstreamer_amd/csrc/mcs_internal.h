@@ -212,6 +212,8 @@ struct mcs_cgrid {
     double *arr[6];    // QFIXED, QPART, Q, U: (M+2)(N+2); ER, EZ: (M+3)(N+3)
     int64_t arr_len[6];
     double *dv;        // dV(grid, i) for i = 1..M
+    int32_t *run_first = nullptr, *run_last = nullptr;   // per cell: its run in the density's sorted pairs
+    int32_t *run_long = nullptr;                         // count, then the cells whose run takes a wavefront
 };
 
 // The two-dimensional solver (mcs_mg2.hip): creation for cfg->ndim == 2, and the

@@ -84,6 +84,37 @@ def test_density_matches_oracle_bitwise():
     assert np.abs(qp[0]).sum() > 0 and np.abs(qp[-1]).sum() > 0
 
 
+def test_density_long_runs_bitwise():
+    """Cells holding thousands of particles (summed by a wavefront each, in chunks
+    of 64 with a partial last chunk) next to short runs."""
+    proc, energy, rate, maxrate = gas_table()
+    table = jumc.CollisionTable(proc, energy, rate, maxrate, egrid="log")
+    rng = np.random.default_rng(9)
+    n = 60000
+    x = rng.normal(size=(n, 3)) * 2e-5
+    x[:, 2] += 0.5 * LL
+    x[:7000] = rng.uniform(0.0, RR * 0.7, size=(7000, 3))       # spread: short runs
+    v = np.zeros((n, 3))
+    w = rng.uniform(0.5, 2.0, n)
+    s = np.ones(n)
+    act = np.ones(n, dtype=np.uint8)
+    popl = jumc.Population(n, jumc.ParticleStates(x, v, w, s, act), table)
+    P = R.Pop(x, v, w, s, act, cap=n)
+    fields = axisym.GridFields(axisym.Grid(RR, LL, MM, NN))
+    G = R.CGrid(RR, LL, MM, NN)
+    axisym.density(fields, "qpart", popl, -1.0)
+    R.cyl_density(P, G, "qpart", -1.0)
+    got = fields.get("qpart").ravel()
+    np.testing.assert_array_equal(got, G.qpart)
+    assert (G.qpart != 0).sum() > 100
+    # the densest cell holds far more than one wavefront's chunk
+    drf, dzf = G.steps[2], G.steps[3]
+    i = np.floor(np.hypot(x[:, 0], x[:, 1]) / drf).astype(int)
+    j = np.floor(x[:, 2] / dzf).astype(int)
+    _, c = np.unique(i * 100000 + j, return_counts=True)
+    assert c.max() > 2000
+
+
 def test_faces_match_oracle_bitwise():
     G = R.CGrid(RR, LL, MM, NN)
     fields = axisym.GridFields(axisym.Grid(RR, LL, MM, NN))
