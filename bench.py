@@ -551,8 +551,9 @@ def run_mg2d(args):
 def run_axisym(args):
     """The reference's production geometry (MCStreamer/samples/input.toml: R = 1.5 cm,
     L = 10 cm, 1536 x 10240 cells, eb = 15 kV/cm, dt = 1e-12 s, Poisson every step):
-    one step is poisson! (cylindrical density!, charge, axisymmetric multigrid with
-    8 levels and the reference's absolute tolerance 1e8 e, faces), advance! in the
+    one step is poisson! (cylindrical density!, charge, solve_freebc!'s two solves
+    of the axisymmetric multigrid with 8 levels and the reference's absolute
+    tolerance 1e8 e around the free radial boundary, faces), advance! in the
     axisymmetric FieldInterp with the ion tracker, and repack!.  The seed is a
     Gaussian of width 1e-4 m at z0 = 1 cm on the axis ([seed] of the sample input)
     of `--axisym-n` electrons over as many ions; C1 gas (N2-like + 20 % O2-like)."""
@@ -621,7 +622,8 @@ def run_axisym(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"FP64 axisymmetric streamer step, {M}x{N} cells (R=1.5 cm, L=10 cm), "
                                f"{n:g}-electron Gaussian seed (w=1e-4 m, z0=1 cm), eb=15 kV/cm, dt=1e-12 s, "
-                               "poisson! ({conf.levels} levels, tol 1e8 e) + advance! (cylindrical field, tracker) + repack!",
+                               f"poisson! with solve_freebc! ({conf.levels} levels, tol 1e8 e) + advance! "
+                               "(cylindrical field, tracker) + repack!",
                    "grid": [M, N], "particles_per_step": psteps / max(args.steps, 1), "fmg_iterations": iters,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -642,12 +644,7 @@ def run_axisym(args):
         G.u[:] = fields.get("u").ravel()
         P = Rf.Pop(ps.x, ps.v, ps.w, ps.s, ps.active, ps.id, cap=3 * nn + 1000, next_id=ps.next_id)
         t0 = time.perf_counter()
-        G.qpart[:] = 0.0
-        Rf.cyl_density(P, G, "qpart", -1.0)
-        Rf.cyl_charge(G)
-        rcq, itq, _ = Rf.mg_solve((M, N), conf.bc, conf.s, conf.levels, conf.smooth1, conf.smooth2,
-                                  conf.tolerance, conf.maxiter, G.u, G.q, cyl=1)
-        Rf.cyl_faces(G, eb)
+        itq = Rf.cyl_poisson(G, P, conf, eb, freebc=True)
         t_poisson = time.perf_counter() - t0
         k = min(nn, 200_000)
         S = Rf.Pop(ps.x[:k], ps.v[:k], ps.w[:k], ps.s[:k], ps.active[:k], ps.id[:k], cap=3 * k + 1000,

@@ -347,13 +347,20 @@ int32_t mcs_cgrid_zero(mcs_cgrid *g, int32_t which);
  * rc both hold adds (w val)/dV(grid, I[1]), dV = 2 pi rc[i] dr dz (grid.jl:76).
  * Per cell the terms are added in particle order; no particle order is needed. */
 int32_t mcs_cyl_density(mcs_pop *p, mcs_cgrid *g, int32_t which, double val);
-/* poisson! (poisson.jl:6-58) up to the field solve: q = e (qfixed + qpart) rounded
- * to FP32 as the Float32 denoiser input (poisson.jl:31-33, NullDenoiser, no fluid
- * electrons), then Multigrid.solve of u (the previous solution is the initial
- * guess) with `mg` (ndim 2, n = (M, N), cyl 1), then the face fields with eb.
- * The free radial boundary (freebc.jl) is not applied: the solve is the first of
- * solve_freebc!'s two.  MCS_E_NONCONVERGED still leaves the faces computed. */
-int32_t mcs_cyl_poisson(mcs_cgrid *g, mcs_mg *mg, double eb, int32_t *iters, double *resid);
+/* poisson! (poisson.jl:6-58): q = e (qfixed + qpart) rounded to FP32 as the Float32
+ * denoiser input (poisson.jl:31-33, NullDenoiser, no fluid electrons), then
+ * Multigrid.solve of u from the previous u with `mg` (ndim 2, n = (M, N), cyl 1),
+ * and with freebc != 0 the rest of solve_freebc! (freebc.jl:4-36): the radial
+ * gradient at r = R, its DST-II over f_n = L k_n (I1/I0 + K1/K0)(k_n R) / 2
+ * (k_n = n pi / L), the DST-III back into the boundary value, setinhom! at r = R,
+ * a second solve and the value into the ghost row; then the face fields with eb.
+ * The transforms are direct sums (sequential in the index) over a table of
+ * sin(pi m / 2N).  iters = the solves' iterations together.  MCS_E_NONCONVERGED
+ * still leaves the faces computed. */
+int32_t mcs_cyl_poisson(mcs_cgrid *g, mcs_mg *mg, double eb, int32_t freebc, int32_t *iters, double *resid);
+/* The free boundary's f_n, n = 1..N (host computation: libstdc++ Bessel functions
+ * below x = 50, large-argument expansions above). */
+int32_t mcs_cgrid_freebc_coeffs(const mcs_cgrid *g, double *f);
 /* Face fields from U (poisson.jl:50-58): er = (u[i-1,j] - u[i,j])/dr and
  * ez = eb + (u[i,j-1] - u[i,j])/dz for i in 1..M+1, j in 1..N+1, then the ghost
  * copies ez[0,:] = ez[1,:], ez[M+1,:] = ez[M,:], er[:,0] = er[:,1], er[:,N+1] = er[:,N].

@@ -206,7 +206,8 @@ SIGNATURES = {
     "mcs_cgrid_get": [_P, _I32, _PD],
     "mcs_cgrid_zero": [_P, _I32],
     "mcs_cyl_density": [_P, _P, _I32, _D],
-    "mcs_cyl_poisson": [_P, _P, _D, _PI32, _PD],
+    "mcs_cyl_poisson": [_P, _P, _D, _I32, _PI32, _PD],
+    "mcs_cgrid_freebc_coeffs": [_P, _PD],
     "mcs_cyl_efield_faces": [_P, _D],
     "mcs_advance_cyl": [_P, _P, _P, _D, _I32, _U64, _U32, _U32, ctypes.POINTER(StepStats)],
 }

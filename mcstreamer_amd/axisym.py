@@ -6,12 +6,11 @@
   CollisionTracker (gridfields.jl:193-224)  -> CollisionTracker
   density! (poisson.jl:65-84)               -> density
   poisson! (poisson.jl:6-58)                -> poisson
+  solve_freebc! (freebc.jl:4-36)            -> poisson(freebc=True)
   the solver of main.jl:134-145             -> poisson_config
 Particles stay in 3D Cartesian coordinates; the grid is indexed by
 r = sqrt(x^2 + y^2) and z.  Arrays come back as (rows j, columns i) = (z, r),
-the reference's column-major (i, j) layout read row-major.  poisson performs the
-first of solve_freebc!'s two solves (homogeneous Dirichlet at r = R); the free
-radial boundary itself (freebc.jl) is not part of this module.
+the reference's column-major (i, j) layout read row-major.
 """
 import ctypes
 
@@ -147,15 +146,24 @@ def poisson_config(grid, levels=None, tolerance=1.0e8 * co.elementary_charge, ma
         maxiter=maxiter, ndim=2, conn=mgmod.CylindricalConnector(1))
 
 
-def poisson(fields, popl, eb, ws):
+def freebc_coeffs(fields):
+    """f_n of solve_freebc! (freebc.jl:16-18) for n = 1..N."""
+    f = np.zeros(fields.grid.N)
+    check(fields.ctx.handle, _lib.lib().mcs_cgrid_freebc_coeffs(fields.handle, dptr(f)))
+    return f
+
+
+def poisson(fields, popl, eb, ws, freebc=True):
     """poisson! (poisson.jl:6-58): qpart from the electrons (val -1), q = e (qfixed
-    + qpart) rounded through FP32, solve for u from the previous u, face fields
-    with eb on z.  Returns (iterations, residual)."""
+    + qpart) rounded through FP32, solve_freebc! (freebc.jl:4-36) for u from the
+    previous u (freebc=False: its first, homogeneous solve only), face fields with
+    eb on z.  Returns (iterations of the solves together, residual)."""
     fields.zero("qpart")
     density(fields, "qpart", popl, -1.0)
     it = ctypes.c_int32()
     res = ctypes.c_double()
-    rc = _lib.lib().mcs_cyl_poisson(fields.handle, ws.handle, float(eb), ctypes.byref(it), ctypes.byref(res))
+    rc = _lib.lib().mcs_cyl_poisson(fields.handle, ws.handle, float(eb), int(bool(freebc)), ctypes.byref(it),
+                                    ctypes.byref(res))
     if rc not in (_lib.MCS_OK, _lib.MCS_E_NONCONVERGED):
         check(fields.ctx.handle, rc)
     ws.last_iters, ws.last_resid = it.value, res.value

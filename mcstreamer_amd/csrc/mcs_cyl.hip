@@ -194,7 +194,130 @@ __global__ void kc_face_ghosts(double *er, double *ez, int32_t M, int32_t N)
     }
 }
 
+// ---- the free radial boundary (MCStreamer/src/freebc.jl:4-36)
+
+// grad = (u[ghost row, j] - u[last row, j]) / dr over the valid columns.
+__global__ void kf_grad(const double *u, double *grad, int32_t M, int32_t N, double dr)
+{
+    const int64_t j = gidx();
+    if (j >= N) return;
+    const int64_t su = M + 2;
+    grad[j] = (u[(M + 1) + su * (j + 1)] - u[M + su * (j + 1)]) / dr;
+}
+
+// am = (-(dr/2) RODFT10(grad)) ./ f with RODFT10 (DST-II) Y_k = 2 sum_j X_j sin(pi (2j+1)(k+1) / 2N);
+// S[m] = sin(pi m / 2N) for m in [0, 4N), the phase index reduced mod 4N.
+__global__ __launch_bounds__(MCS_BLOCK) void kf_dst2(const double *grad, const double *S, const double *f,
+                                                     int32_t N, double dr, double *am)
+{
+    const int64_t k = gidx();
+    if (k >= N) return;
+    const int64_t four = 4 * (int64_t)N, step = 2 * (k + 1);
+    int64_t m = k + 1;
+    double acc = 0.0;
+    for (int32_t j = 0; j < N; ++j) {
+        acc = acc + grad[j] * S[m];
+        m += step;
+        if (m >= four) m -= four;
+    }
+    const double y = 2.0 * acc;
+    const double bm = -(dr / 2.0) * y;
+    am[k] = bm / f[k];
+}
+
+// bcond = RODFT01(am) / 2 + RODFT01(am1), am1 = am's last element alone; RODFT01
+// (DST-III) Y_k = (-1)^k X_{N-1} + 2 sum_{j < N-1} X_j sin(pi (j+1)(2k+1) / 2N).
+__global__ __launch_bounds__(MCS_BLOCK) void kf_dst3(const double *am, const double *S, int32_t N, double *bcond)
+{
+    const int64_t k = gidx();
+    if (k >= N) return;
+    const int64_t four = 4 * (int64_t)N, step = 2 * k + 1;
+    int64_t m = step;
+    double acc = 0.0;
+    for (int32_t j = 0; j + 1 < N; ++j) {
+        acc = acc + am[j] * S[m];
+        m += step;
+        if (m >= four) m -= four;
+    }
+    const double last = am[N - 1];
+    const double sg = (k & 1) ? -last : last;
+    const double t = sg + 2.0 * acc;
+    const double t1 = sg + 2.0 * 0.0;
+    bcond[k] = t / 2.0 + t1;
+}
+
+// setinhom!(mg, b, TopBnd(), bcond) (boundaries.jl:141-153): b += c bcond on the
+// last valid row, c = (2/s) (1 + 1/(2M - 1)) (the cylindrical connector's factor).
+__global__ void kf_inhom(double *b, const double *bcond, int32_t M, int32_t N, double c)
+{
+    const int64_t j = gidx();
+    if (j >= N) return;
+    double *p = b + M + (int64_t)(M + 2) * (j + 1);
+    *p = *p + c * bcond[j];
+}
+
+// u[end, :] .+= 2 .* bcond on the ghost row (freebc.jl:35).
+__global__ void kf_ghost(double *u, const double *bcond, int32_t M, int32_t N)
+{
+    const int64_t j = gidx();
+    if (j >= N) return;
+    double *p = u + (M + 1) + (int64_t)(M + 2) * (j + 1);
+    *p = *p + 2.0 * bcond[j];
+}
+
+// I1/I0 + K1/K0 at x > 0 (the scaled forms of freebc.jl:17-18 have the same
+// ratios): libstdc++'s cyl_bessel_i/k below x = 50, above it the large-argument
+// expansions (DLMF 10.40.1-2: terms a_k(nu)/x^k, a_k = prod_{m<=k} (4 nu^2 -
+// (2m-1)^2) / (8 m)), whose e^{-2x} remainders are below double precision there.
+double bessel_ratio_sum(double x)
+{
+    if (x < 50.0)
+        return std::cyl_bessel_i(1.0, x) / std::cyl_bessel_i(0.0, x) +
+               std::cyl_bessel_k(1.0, x) / std::cyl_bessel_k(0.0, x);
+    double t0 = 1.0, t1 = 1.0, i0 = 1.0, i1 = 1.0, k0 = 1.0, k1 = 1.0;
+    for (int m = 1; m < 200; ++m) {
+        const double q = (2.0 * m - 1.0) * (2.0 * m - 1.0);
+        t0 *= (0.0 - q) / (8.0 * x * m);
+        t1 *= (4.0 - q) / (8.0 * x * m);
+        const double sg = (m & 1) ? -1.0 : 1.0;
+        i0 += sg * t0;
+        i1 += sg * t1;
+        k0 += t0;
+        k1 += t1;
+        if (std::fabs(t0) < 1e-20 && std::fabs(t1) < 1e-20) break;
+    }
+    return i1 / i0 + k1 / k0;
+}
+
 }  // namespace
+
+extern "C" int32_t mcs_cgrid_freebc_coeffs(const mcs_cgrid *g, double *f)
+{
+    if (!g || !f) return MCS_E_ARG;
+    const double c = 3.141592653589793 / g->L;
+    for (int32_t n = 1; n <= g->N; ++n) {
+        const double k = c * (double)n;
+        const double x = k * g->R;
+        f[n - 1] = g->L * k * bessel_ratio_sum(x) / 2.0;
+    }
+    return MCS_OK;
+}
+
+static int32_t freebc_setup(mcs_cgrid *g)
+{
+    mcs_ctx *ctx = g->ctx;
+    if (g->fb_S) return MCS_OK;
+    const int64_t N = g->N;
+    std::vector<double> S(4 * N), f(N);
+    for (int64_t m = 0; m < 4 * N; ++m) S[m] = std::sin(3.141592653589793 * (double)m / (2.0 * (double)N));
+    mcs_cgrid_freebc_coeffs(g, f.data());
+    if (hipMalloc(&g->fb_S, sizeof(double) * 4 * N) != hipSuccess ||
+        hipMalloc(&g->fb_f, sizeof(double) * 4 * N) != hipSuccess)
+        return mcs_fail(ctx, MCS_E_HIP, "freebc: device allocation failed");
+    MCS_HIP(ctx, hipMemcpy(g->fb_S, S.data(), sizeof(double) * 4 * N, hipMemcpyHostToDevice));
+    MCS_HIP(ctx, hipMemcpy(g->fb_f, f.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    return MCS_OK;
+}
 
 extern "C" int32_t mcs_cgrid_create(mcs_ctx *ctx, double R, double L, int32_t M, int32_t N, mcs_cgrid **out)
 {
@@ -249,6 +372,8 @@ extern "C" int32_t mcs_cgrid_destroy(mcs_cgrid *g)
     if (g->run_first) hipFree(g->run_first);
     if (g->run_last) hipFree(g->run_last);
     if (g->run_long) hipFree(g->run_long);
+    if (g->fb_S) hipFree(g->fb_S);
+    if (g->fb_f) hipFree(g->fb_f);
     delete g;
     return MCS_OK;
 }
@@ -362,7 +487,8 @@ extern "C" int32_t mcs_cyl_efield_faces(mcs_cgrid *g, double eb)
     return MCS_OK;
 }
 
-extern "C" int32_t mcs_cyl_poisson(mcs_cgrid *g, mcs_mg *mg, double eb, int32_t *iters, double *resid)
+extern "C" int32_t mcs_cyl_poisson(mcs_cgrid *g, mcs_mg *mg, double eb, int32_t freebc, int32_t *iters,
+                                   double *resid)
 {
     if (!g || !mg) return MCS_E_ARG;
     mcs_ctx *ctx = g->ctx;
@@ -370,14 +496,44 @@ extern "C" int32_t mcs_cyl_poisson(mcs_cgrid *g, mcs_mg *mg, double eb, int32_t 
     if (mg->cfg.ndim != 2 || mg->cfg.n[0] != g->M || mg->cfg.n[1] != g->N || mg->cfg.cyl != 1 ||
         mg->cfg.precision != MCS_PREC_F64)
         return mcs_fail(ctx, MCS_E_ARG, "cyl_poisson: the solver must be FP64, ndim 2, n = (M, N), cyl 1");
+    if (freebc && mg->cfg.bc[1] != MCS_BC_DIRICHLET)
+        return mcs_fail(ctx, MCS_E_ARG, "cyl_poisson: the free boundary needs Dirichlet at r = R");
     hipSetDevice(ctx->device);
+    if (freebc) {
+        int32_t rs = freebc_setup(g);
+        if (rs) return rs;
+    }
     const int64_t nc = (int64_t)g->M * g->N;
+    double *u = g->arr[MCS_CARR_U], *q = g->arr[MCS_CARR_Q];
     hipLaunchKernelGGL(kc_charge, dim3((unsigned)cdiv(nc, MCS_BLOCK)), dim3(MCS_BLOCK), 0, ctx->stream,
-                       (const double *)g->arr[MCS_CARR_QFIXED], (const double *)g->arr[MCS_CARR_QPART],
-                       g->arr[MCS_CARR_Q], g->M, g->N);
+                       (const double *)g->arr[MCS_CARR_QFIXED], (const double *)g->arr[MCS_CARR_QPART], q, g->M,
+                       g->N);
     MCS_LAUNCH_CHECK(ctx);
-    int32_t rc = mcs_mg2_solve_dev(mg, g->arr[MCS_CARR_U], g->arr[MCS_CARR_Q], iters, resid);
+    int32_t it1 = 0, it2 = 0;
+    int32_t rc = mcs_mg2_solve_dev(mg, u, q, &it1, resid);
     if (rc != MCS_OK && rc != MCS_E_NONCONVERGED) return rc;
+    if (freebc) {
+        // solve_freebc! (freebc.jl:4-36): the homogeneous solve's radial gradient at
+        // r = R, its sine transform over the Bessel factors back into a boundary
+        // value, that value into b, a second solve, and the value into the ghost row
+        ProfScope ps(ctx, "cyl_freebc");
+        const int32_t M = g->M, N = g->N;
+        const dim3 gb((unsigned)cdiv(N, MCS_BLOCK)), bl(MCS_BLOCK);
+        double *grad = g->fb_f + N, *am = g->fb_f + 2 * N, *bcond = g->fb_f + 3 * N;
+        hipLaunchKernelGGL(kf_grad, gb, bl, 0, ctx->stream, (const double *)u, grad, M, N, g->dr);
+        hipLaunchKernelGGL(kf_dst2, gb, bl, 0, ctx->stream, (const double *)grad, (const double *)g->fb_S,
+                           (const double *)g->fb_f, N, g->dr, am);
+        hipLaunchKernelGGL(kf_dst3, gb, bl, 0, ctx->stream, (const double *)am, (const double *)g->fb_S, N, bcond);
+        const double c = (2.0 / mg->cfg.s) * (1.0 + 1.0 / (double)(2 * M - 1));
+        hipLaunchKernelGGL(kf_inhom, gb, bl, 0, ctx->stream, q, (const double *)bcond, M, N, c);
+        MCS_LAUNCH_CHECK(ctx);
+        int32_t rc2 = mcs_mg2_solve_dev(mg, u, q, &it2, resid);
+        if (rc2 != MCS_OK && rc2 != MCS_E_NONCONVERGED) return rc2;
+        if (rc2) rc = rc2;
+        hipLaunchKernelGGL(kf_ghost, gb, bl, 0, ctx->stream, u, (const double *)bcond, M, N);
+        MCS_LAUNCH_CHECK(ctx);
+    }
+    if (iters) *iters = it1 + it2;
     int32_t rf = mcs_cyl_efield_faces(g, eb);
     return rf ? rf : rc;
 }

@@ -214,6 +214,8 @@ struct mcs_cgrid {
     double *dv;        // dV(grid, i) for i = 1..M
     int32_t *run_first = nullptr, *run_last = nullptr;   // per cell: its run in the density's sorted pairs
     int32_t *run_long = nullptr;                         // count, then the cells whose run takes a wavefront
+    double *fb_S = nullptr;   // free boundary: sin(pi m / 2N), m < 4N
+    double *fb_f = nullptr;   // free boundary: f (N), then grad, am, bcond (N each)
 };
 
 // The two-dimensional solver (mcs_mg2.hip): creation for cfg->ndim == 2, and the

@@ -295,6 +295,56 @@ void ref_cyl_faces(ref_cgrid *g, double eb)
     for (int64_t i = 0; i <= M + 2; ++i) g->er[i + sf * (N + 1)] = g->er[i + sf * N];
 }
 
+/* solve_freebc! (MCStreamer/src/freebc.jl:4-36) after its first solve:
+ * grad = (u[ghost row] - u[last row]) / dr; am = (-(dr/2) RODFT10(grad)) ./ f;
+ * bcond = RODFT01(am) / 2 + RODFT01(am1), am1 = am's last element alone.  FFTW's
+ * RODFT10 is Y_k = 2 sum_j X_j sin(pi (j + 1/2)(k + 1) / N) and RODFT01 is
+ * Y_k = (-1)^k X_{N-1} + 2 sum_{j < N-1} X_j sin(pi (j + 1)(k + 1/2) / N); here as
+ * direct sums in index order over S[m] = sin(pi m / 2N).  f is the caller's
+ * f_n = L k_n (I1/I0 + K1/K0)(k_n R) / 2. */
+void ref_cyl_freebc_bcond(const ref_cgrid *g, const double *f, double *bcond)
+{
+    int64_t M = g->M, N = g->N, su = M + 2, four = 4 * N;
+    double *S = malloc(sizeof(double) * four);
+    double *grad = malloc(sizeof(double) * N);
+    double *am = malloc(sizeof(double) * N);
+    for (int64_t m = 0; m < four; ++m) S[m] = sin(3.141592653589793 * (double)m / (2.0 * (double)N));
+    for (int64_t j = 0; j < N; ++j) grad[j] = (g->u[(M + 1) + su * (j + 1)] - g->u[M + su * (j + 1)]) / g->dr;
+    for (int64_t k = 0; k < N; ++k) {
+        double acc = 0.0;
+        for (int64_t j = 0; j < N; ++j) acc += grad[j] * S[((2 * j + 1) * (k + 1)) % four];
+        double bm = -(g->dr / 2.0) * (2.0 * acc);
+        am[k] = bm / f[k];
+    }
+    for (int64_t k = 0; k < N; ++k) {
+        double acc = 0.0;
+        for (int64_t j = 0; j + 1 < N; ++j) acc += am[j] * S[((j + 1) * (2 * k + 1)) % four];
+        double sg = (k & 1) ? -am[N - 1] : am[N - 1];
+        double t = sg + 2.0 * acc;
+        double t1 = sg + 2.0 * 0.0;
+        bcond[k] = t / 2.0 + t1;
+    }
+    free(S);
+    free(grad);
+    free(am);
+}
+
+/* setinhom!(mg, q, TopBnd(), bcond) (Multigrid/src/boundaries.jl:141-153) with the
+ * cylindrical connector: q[last row] += (2/s) (1 + 1/(2M - 1)) bcond. */
+void ref_cyl_freebc_inhom(ref_cgrid *g, double s, const double *bcond)
+{
+    int64_t M = g->M, su = M + 2;
+    double c = (2.0 / s) * (1.0 + 1.0 / (double)(2 * M - 1));
+    for (int64_t j = 0; j < g->N; ++j) g->q[M + su * (j + 1)] += c * bcond[j];
+}
+
+/* u[end, :] .+= 2 .* bcond (freebc.jl:35). */
+void ref_cyl_freebc_ghost(ref_cgrid *g, const double *bcond)
+{
+    int64_t M = g->M, su = M + 2;
+    for (int64_t j = 0; j < g->N; ++j) g->u[(M + 1) + su * (j + 1)] += 2.0 * bcond[j];
+}
+
 void ref_efield(const ref_field *f, const double x[3], double e[3], int32_t *flags)
 {
     if (f->kind == 2) {

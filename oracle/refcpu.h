@@ -171,6 +171,9 @@ int32_t ref_advance_cyl(ref_pop *p, const ref_table *t, ref_cgrid *g, double dt,
 void ref_cyl_density(const ref_pop *p, ref_cgrid *g, double *arr, double val);
 void ref_cyl_charge(ref_cgrid *g);
 void ref_cyl_faces(ref_cgrid *g, double eb);
+void ref_cyl_freebc_bcond(const ref_cgrid *g, const double *f, double *bcond);
+void ref_cyl_freebc_inhom(ref_cgrid *g, double s, const double *bcond);
+void ref_cyl_freebc_ghost(ref_cgrid *g, const double *bcond);
 int64_t ref_cell_key(const ref_mesh *m, const double x[3]);
 int32_t ref_sort_by_cell(ref_pop *p, ref_mesh *m, ref_pop *scratch);
 void ref_deposit(const ref_pop *p, ref_mesh *m, int32_t which, int32_t scheme, double val);

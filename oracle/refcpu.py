@@ -138,6 +138,12 @@ def lib():
         L.ref_cyl_charge.restype = None
         L.ref_cyl_faces.argtypes = [P(RefCGrid), ctypes.c_double]
         L.ref_cyl_faces.restype = None
+        L.ref_cyl_freebc_bcond.argtypes = [P(RefCGrid), _PD, _PD]
+        L.ref_cyl_freebc_bcond.restype = None
+        L.ref_cyl_freebc_inhom.argtypes = [P(RefCGrid), ctypes.c_double, _PD]
+        L.ref_cyl_freebc_inhom.restype = None
+        L.ref_cyl_freebc_ghost.argtypes = [P(RefCGrid), _PD]
+        L.ref_cyl_freebc_ghost.restype = None
         L.ref_cell_key.argtypes = [P(RefMesh), _PD]
         L.ref_cell_key.restype = ctypes.c_int64
         L.ref_sort_by_cell.argtypes = [P(RefPop), P(RefMesh), P(RefPop)]
@@ -339,6 +345,46 @@ def cyl_charge(grid):
 
 def cyl_faces(grid, eb):
     lib().ref_cyl_faces(ctypes.byref(grid.cstruct()), float(eb))
+
+
+def freebc_coeffs(grid):
+    """f_n = L k_n (I1/I0 + K1/K0)(k_n R) / 2, k_n = (pi / L) n (freebc.jl:16-18),
+    from scipy's exponentially scaled Bessel functions (the scaled ratios equal the
+    unscaled ones)."""
+    from scipy.special import i0e, i1e, k0e, k1e
+    k = (np.pi / grid.L) * np.arange(1, grid.N + 1)
+    x = k * grid.R
+    return grid.L * k * (i1e(x) / i0e(x) + k1e(x) / k0e(x)) / 2
+
+
+def cyl_poisson(grid, pop, conf, eb, f=None, freebc=True):
+    """poisson! on the axisymmetric grid: qpart from the electrons, the charge,
+    solve_freebc! (or its first solve alone), the faces.  Returns the solves'
+    iterations together."""
+    grid.qpart[:] = 0.0
+    cyl_density(pop, grid, "qpart", -1.0)
+    cyl_charge(grid)
+    args = ((grid.M, grid.N), conf.bc, conf.s, conf.levels, conf.smooth1, conf.smooth2, conf.tolerance,
+            conf.maxiter)
+    _, it, _ = mg_solve(*args, grid.u, grid.q, cyl=1)
+    if freebc:
+        f = freebc_coeffs(grid) if f is None else np.ascontiguousarray(f, dtype=np.float64)
+        bcond = np.zeros(grid.N)
+        g = grid.cstruct()
+        lib().ref_cyl_freebc_bcond(ctypes.byref(g), _d(f), _d(bcond))
+        lib().ref_cyl_freebc_inhom(ctypes.byref(g), float(conf.s), _d(bcond))
+        _, it2, _ = mg_solve(*args, grid.u, grid.q, cyl=1)
+        lib().ref_cyl_freebc_ghost(ctypes.byref(grid.cstruct()), _d(bcond))
+        it += it2
+    cyl_faces(grid, eb)
+    return it
+
+
+def freebc_bcond(grid, f):
+    bcond = np.zeros(grid.N)
+    lib().ref_cyl_freebc_bcond(ctypes.byref(grid.cstruct()), _d(np.ascontiguousarray(f, dtype=np.float64)),
+                               _d(bcond))
+    return bcond
 
 
 def uniform_field(e):

@@ -129,17 +129,22 @@ def test_faces_match_oracle_bitwise():
         np.testing.assert_array_equal(fields.get(k).ravel(), getattr(G, k), err_msg=k)
 
 
-def _ref_poisson(G, P, conf, eb):
-    G.qpart[:] = 0.0
-    R.cyl_density(P, G, "qpart", -1.0)
-    R.cyl_charge(G)
-    rc, it, res = R.mg_solve((G.M, G.N), conf.bc, conf.s, conf.levels, conf.smooth1, conf.smooth2,
-                             conf.tolerance, conf.maxiter, G.u, G.q, cyl=1)
-    R.cyl_faces(G, eb)
-    return it
+def _ref_poisson(G, P, conf, eb, fields=None, freebc=False):
+    """The oracle's poisson!; with the free boundary it takes the library's f_n
+    (pinned against scipy separately), so that the chains compare bitwise."""
+    f = axisym.freebc_coeffs(fields) if freebc else None
+    return R.cyl_poisson(G, P, conf, eb, f=f, freebc=freebc)
 
 
-def test_poisson_matches_oracle_bitwise():
+def test_freebc_coefficients_match_scipy():
+    for (Rr, L, M, N) in [(RR, LL, MM, NN), (0.015, 0.1, 1536, 10240)]:
+        fields = axisym.GridFields(axisym.Grid(Rr, L, M, N))
+        np.testing.assert_allclose(axisym.freebc_coeffs(fields), R.freebc_coeffs(R.CGrid(Rr, L, M, N)),
+                                   rtol=2e-14)
+
+
+@pytest.mark.parametrize("freebc", [False, True])
+def test_poisson_matches_oracle_bitwise(freebc):
     table, T, popl, P, grid, fields, G = _setup(n=40000, seed=2)
     # ions where the electrons are, less a few: a net charge of the electrons' order
     axisym.density(fields, "qfixed", popl, 0.9)
@@ -147,9 +152,9 @@ def test_poisson_matches_oracle_bitwise():
     conf = axisym.poisson_config(grid, tolerance=-1e-10)
     ws = axisym.mgmod.Workspace(conf, (MM, NN))
     eb = -5e6
-    it, res = axisym.poisson(fields, popl, eb, ws)
-    it_ref = _ref_poisson(G, P, conf, eb)
-    print(f"mg iterations {it} / oracle {it_ref}", flush=True)
+    it, res = axisym.poisson(fields, popl, eb, ws, freebc=freebc)
+    it_ref = _ref_poisson(G, P, conf, eb, fields, freebc)
+    print(f"freebc {freebc}: mg iterations {it} / oracle {it_ref}", flush=True)
     assert it == it_ref
     for k in ("q", "u", "er", "ez"):
         np.testing.assert_array_equal(fields.get(k).ravel(), getattr(G, k), err_msg=k)
@@ -185,8 +190,8 @@ def test_advance_in_axisymmetric_field_matches_oracle():
 
 
 def test_axisymmetric_streamer_steps():
-    """poisson! + advance! (field and tracker of the same grid) + repack!, four
-    steps, against the same chain in the oracle."""
+    """poisson! (with the free boundary) + advance! (field and tracker of the same
+    grid) + repack!, four steps, against the same chain in the oracle."""
     table, T, popl, P, grid, fields, G = _setup(n=20000, seed=6)
     axisym.density(fields, "qfixed", popl, 1.0)
     R.cyl_density(P, G, "qfixed", 1.0)
@@ -196,7 +201,7 @@ def test_axisymmetric_streamer_steps():
     eb, dt = -5e6, 1e-12
     for step in range(4):
         it, _ = axisym.poisson(fields, popl, eb, ws)
-        it_ref = _ref_poisson(G, P, conf, eb)
+        it_ref = _ref_poisson(G, P, conf, eb, fields, freebc=True)
         st = jumc.advance(popl, field, dt, tracker, step=step)
         rs = R.advance_cyl(P, T, G, dt, track=True, seed=popl.seed, step=step)
         jumc.repack(popl)

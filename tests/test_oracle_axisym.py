@@ -3,8 +3,9 @@ poisson.jl:6-84) pinned without the reference: exact floor against rational
 arithmetic, the LinRange geometry against its closed forms, the staggered
 bilinear field against linear fields it must reproduce, density! against a
 brute-force placement with exact cell indices, the face fields against a
-linear potential, and the push in a uniform radial or axial field against the
-exact parabola."""
+linear potential, the push in a uniform radial or axial field against the
+exact parabola, and solve_freebc! (freebc.jl) against scipy's DST-II/III and
+against a point charge between grounded planes with no outer wall."""
 from fractions import Fraction
 import math
 
@@ -206,3 +207,77 @@ def test_push_in_uniform_axial_and_radial_fields_is_the_parabola():
     assert P.x[0, 1] == 0.0 and P.v[0, 1] == 0.0 and P.x[0, 2] == 1e-3
     np.testing.assert_allclose(P.x[0, 0], 5e-4 + 0.5 * qm * a * dt * dt, rtol=1e-12)
     np.testing.assert_allclose(P.v[0, 0], qm * a * dt, rtol=1e-12)
+
+
+def _cyl_conf(G, tol=-1e-12, levels=4):
+    from mcstreamer_amd import multigrid as mgmod
+    return mgmod.MGConfig(
+        bc=[(mgmod.LeftBnd, mgmod.Dirichlet), (mgmod.RightBnd, mgmod.Dirichlet),
+            (mgmod.TopBnd, mgmod.Dirichlet), (mgmod.BottomBnd, mgmod.Neumann)],
+        s=G.steps[1] ** 2 / co.epsilon_0, levels=levels, tolerance=tol, smooth1=2, smooth2=2, maxiter=50,
+        ndim=2, conn=mgmod.CylindricalConnector(1))
+
+
+def test_freebc_transforms_are_fftw_r2r():
+    """The boundary value of solve_freebc! against scipy's DST-II/III, which are
+    FFTW's RODFT10/RODFT01 (same definitions, same normalisation)."""
+    from scipy.fft import dst
+    G = R.CGrid(1e-3, 2e-3, 16, 64)
+    rng = np.random.default_rng(4)
+    G.u[:] = rng.normal(size=G.u.size)
+    f = R.freebc_coeffs(G)
+    got = R.freebc_bcond(G, f)
+    M, N, dr = G.M, G.N, G.steps[0]
+    u = G.u.reshape(N + 2, M + 2)
+    grad = (u[1:N + 1, M + 1] - u[1:N + 1, M]) / dr
+    am = (-(dr / 2) * dst(grad, type=2)) / f
+    am1 = np.zeros(N)
+    am1[-1] = am[-1]
+    exp = dst(am, type=3) / 2 + dst(am1, type=3)
+    np.testing.assert_allclose(got, exp, rtol=1e-11, atol=1e-12 * np.abs(exp).max())
+
+
+def test_freebc_coefficients_large_argument_limit():
+    """f_n / (L k_n / 2) = I1/I0 + K1/K0 tends to 2 - 1/(4 x^2) + ... for large x."""
+    G = R.CGrid(1.5e-2, 0.1, 8, 10240)
+    f = R.freebc_coeffs(G)
+    k = np.pi / G.L * np.arange(1, G.N + 1)
+    x = k * G.R
+    ratio = f / (G.L * k / 2)
+    big = x > 200
+    np.testing.assert_allclose(ratio[big], 2 - 1 / (4 * x[big] ** 2), rtol=1e-9)
+    assert (np.diff(ratio[big]) > 0).all()
+
+
+def test_freebc_reproduces_point_charge_between_grounded_planes():
+    """A compact charge Q on the axis between the grounded planes z = 0 and z = L,
+    no outer wall: phi = Q / (pi eps0 L) sum_n sin(k z0) sin(k z) K0(k r).  With the
+    free boundary the solution matches it across the grid (to discretisation); the
+    homogeneous solve alone (a grounded wall at r = R) does not."""
+    from scipy.special import k0
+    Rr, L, M, N = 1e-3, 2e-3, 64, 128
+    Q, z0, sig = 1e-13, 1e-3, 3e-5
+    out = {}
+    for fb in (True, False):
+        G = R.CGrid(Rr, L, M, N)
+        dr, dz = G.steps[0], G.steps[1]
+        rc = (np.arange(M) + 0.5) * dr
+        zc = (np.arange(N) + 0.5) * dz
+        rho = Q / ((2 * np.pi) ** 1.5 * sig ** 3) * np.exp(-(rc[None, :] ** 2 + (zc[:, None] - z0) ** 2) / (2 * sig ** 2))
+        qf = np.zeros((N + 2, M + 2))
+        qf[1:N + 1, 1:M + 1] = rho / co.elementary_charge
+        G.qfixed[:] = qf.ravel()
+        P = R.Pop(np.zeros((0, 3)), np.zeros((0, 3)), [], [], [], cap=1)
+        R.cyl_poisson(G, P, _cyl_conf(G), 0.0, freebc=fb)
+        out[fb] = G.u.reshape(N + 2, M + 2)[1:N + 1, 1:M + 1]
+    n = np.arange(1, 400)[:, None]
+    kk = n * np.pi / L
+    cols = [M // 2, M - 1]
+    for c in cols:
+        r = rc[c]
+        phi = Q / (np.pi * co.epsilon_0 * L) * (np.sin(kk * z0) * np.sin(kk * zc[None, :]) * k0(kk * r)).sum(axis=0)
+        sel = np.abs(phi) > 0.2 * np.abs(phi).max()
+        err_free = np.abs(out[True][sel, c] - phi[sel]) / np.abs(phi[sel])
+        err_wall = np.abs(out[False][sel, c] - phi[sel]) / np.abs(phi[sel])
+        assert err_free.max() < 0.02, (c, err_free.max())
+        assert err_wall.max() > 0.2, (c, err_wall.max())
