@@ -205,45 +205,64 @@ __global__ void kf_grad(const double *u, double *grad, int32_t M, int32_t N, dou
     grad[j] = (u[(M + 1) + su * (j + 1)] - u[M + su * (j + 1)]) / dr;
 }
 
+// The transforms' sums are formed in a fixed order: lane l of a wavefront adds the
+// terms j = l, l + 64, ... in sequence, then the 64 partial sums meet in the xor
+// butterfly (offsets 32, 16, ..., 1), which leaves the same value in every lane.
+__device__ __forceinline__ double wave_tree(double acc)
+{
+#pragma unroll
+    for (int off = MCS_WAVE / 2; off >= 1; off >>= 1) acc = acc + __shfl_xor(acc, off, MCS_WAVE);
+    return acc;
+}
+
 // am = (-(dr/2) RODFT10(grad)) ./ f with RODFT10 (DST-II) Y_k = 2 sum_j X_j sin(pi (2j+1)(k+1) / 2N);
-// S[m] = sin(pi m / 2N) for m in [0, 4N), the phase index reduced mod 4N.
+// S[m] = sin(pi m / 2N) for m in [0, 4N), the phase index reduced mod 4N.  One
+// wavefront per k.
 __global__ __launch_bounds__(MCS_BLOCK) void kf_dst2(const double *grad, const double *S, const double *f,
                                                      int32_t N, double dr, double *am)
 {
-    const int64_t k = gidx();
+    const int lane = threadIdx.x & (MCS_WAVE - 1);
+    const int64_t k = (int64_t)blockIdx.x * (MCS_BLOCK / MCS_WAVE) + threadIdx.x / MCS_WAVE;
     if (k >= N) return;
-    const int64_t four = 4 * (int64_t)N, step = 2 * (k + 1);
-    int64_t m = k + 1;
+    const int64_t four = 4 * (int64_t)N, inc = (2 * MCS_WAVE * (k + 1)) % four;
+    int64_t m = ((2 * lane + 1) * (k + 1)) % four;
     double acc = 0.0;
-    for (int32_t j = 0; j < N; ++j) {
+    for (int32_t j = lane; j < N; j += MCS_WAVE) {
         acc = acc + grad[j] * S[m];
-        m += step;
+        m += inc;
         if (m >= four) m -= four;
     }
-    const double y = 2.0 * acc;
-    const double bm = -(dr / 2.0) * y;
-    am[k] = bm / f[k];
+    acc = wave_tree(acc);
+    if (lane == 0) {
+        const double y = 2.0 * acc;
+        const double bm = -(dr / 2.0) * y;
+        am[k] = bm / f[k];
+    }
 }
 
 // bcond = RODFT01(am) / 2 + RODFT01(am1), am1 = am's last element alone; RODFT01
 // (DST-III) Y_k = (-1)^k X_{N-1} + 2 sum_{j < N-1} X_j sin(pi (j+1)(2k+1) / 2N).
 __global__ __launch_bounds__(MCS_BLOCK) void kf_dst3(const double *am, const double *S, int32_t N, double *bcond)
 {
-    const int64_t k = gidx();
+    const int lane = threadIdx.x & (MCS_WAVE - 1);
+    const int64_t k = (int64_t)blockIdx.x * (MCS_BLOCK / MCS_WAVE) + threadIdx.x / MCS_WAVE;
     if (k >= N) return;
-    const int64_t four = 4 * (int64_t)N, step = 2 * k + 1;
-    int64_t m = step;
+    const int64_t four = 4 * (int64_t)N, inc = (MCS_WAVE * (2 * k + 1)) % four;
+    int64_t m = ((lane + 1) * (2 * k + 1)) % four;
     double acc = 0.0;
-    for (int32_t j = 0; j + 1 < N; ++j) {
+    for (int32_t j = lane; j + 1 < N; j += MCS_WAVE) {
         acc = acc + am[j] * S[m];
-        m += step;
+        m += inc;
         if (m >= four) m -= four;
     }
-    const double last = am[N - 1];
-    const double sg = (k & 1) ? -last : last;
-    const double t = sg + 2.0 * acc;
-    const double t1 = sg + 2.0 * 0.0;
-    bcond[k] = t / 2.0 + t1;
+    acc = wave_tree(acc);
+    if (lane == 0) {
+        const double last = am[N - 1];
+        const double sg = (k & 1) ? -last : last;
+        const double t = sg + 2.0 * acc;
+        const double t1 = sg + 2.0 * 0.0;
+        bcond[k] = t / 2.0 + t1;
+    }
 }
 
 // setinhom!(mg, b, TopBnd(), bcond) (boundaries.jl:141-153): b += c bcond on the
@@ -521,9 +540,10 @@ extern "C" int32_t mcs_cyl_poisson(mcs_cgrid *g, mcs_mg *mg, double eb, int32_t 
         const dim3 gb((unsigned)cdiv(N, MCS_BLOCK)), bl(MCS_BLOCK);
         double *grad = g->fb_f + N, *am = g->fb_f + 2 * N, *bcond = g->fb_f + 3 * N;
         hipLaunchKernelGGL(kf_grad, gb, bl, 0, ctx->stream, (const double *)u, grad, M, N, g->dr);
-        hipLaunchKernelGGL(kf_dst2, gb, bl, 0, ctx->stream, (const double *)grad, (const double *)g->fb_S,
+        const dim3 gw((unsigned)cdiv(N, MCS_BLOCK / MCS_WAVE));   // a wavefront per output
+        hipLaunchKernelGGL(kf_dst2, gw, bl, 0, ctx->stream, (const double *)grad, (const double *)g->fb_S,
                            (const double *)g->fb_f, N, g->dr, am);
-        hipLaunchKernelGGL(kf_dst3, gb, bl, 0, ctx->stream, (const double *)am, (const double *)g->fb_S, N, bcond);
+        hipLaunchKernelGGL(kf_dst3, gw, bl, 0, ctx->stream, (const double *)am, (const double *)g->fb_S, N, bcond);
         const double c = (2.0 / mg->cfg.s) * (1.0 + 1.0 / (double)(2 * M - 1));
         hipLaunchKernelGGL(kf_inhom, gb, bl, 0, ctx->stream, q, (const double *)bcond, M, N, c);
         MCS_LAUNCH_CHECK(ctx);

@@ -300,8 +300,20 @@ void ref_cyl_faces(ref_cgrid *g, double eb)
  * bcond = RODFT01(am) / 2 + RODFT01(am1), am1 = am's last element alone.  FFTW's
  * RODFT10 is Y_k = 2 sum_j X_j sin(pi (j + 1/2)(k + 1) / N) and RODFT01 is
  * Y_k = (-1)^k X_{N-1} + 2 sum_{j < N-1} X_j sin(pi (j + 1)(k + 1/2) / N); here as
- * direct sums in index order over S[m] = sin(pi m / 2N).  f is the caller's
+ * direct sums over S[m] = sin(pi m / 2N) in a fixed order: 64 partial sums (the
+ * terms j = l, l + 64, ... in sequence for l < 64), then the xor butterfly over
+ * the partials (offsets 32, ..., 1), read at partial 0.  f is the caller's
  * f_n = L k_n (I1/I0 + K1/K0)(k_n R) / 2. */
+static double tree64(double *p)
+{
+    double q[64];
+    for (int off = 32; off >= 1; off >>= 1) {
+        for (int l = 0; l < 64; ++l) q[l] = p[l] + p[l ^ off];
+        for (int l = 0; l < 64; ++l) p[l] = q[l];
+    }
+    return p[0];
+}
+
 void ref_cyl_freebc_bcond(const ref_cgrid *g, const double *f, double *bcond)
 {
     int64_t M = g->M, N = g->N, su = M + 2, four = 4 * N;
@@ -310,15 +322,22 @@ void ref_cyl_freebc_bcond(const ref_cgrid *g, const double *f, double *bcond)
     double *am = malloc(sizeof(double) * N);
     for (int64_t m = 0; m < four; ++m) S[m] = sin(3.141592653589793 * (double)m / (2.0 * (double)N));
     for (int64_t j = 0; j < N; ++j) grad[j] = (g->u[(M + 1) + su * (j + 1)] - g->u[M + su * (j + 1)]) / g->dr;
+    double part[64];
     for (int64_t k = 0; k < N; ++k) {
-        double acc = 0.0;
-        for (int64_t j = 0; j < N; ++j) acc += grad[j] * S[((2 * j + 1) * (k + 1)) % four];
+        for (int l = 0; l < 64; ++l) {
+            part[l] = 0.0;
+            for (int64_t j = l; j < N; j += 64) part[l] += grad[j] * S[((2 * j + 1) * (k + 1)) % four];
+        }
+        double acc = tree64(part);
         double bm = -(g->dr / 2.0) * (2.0 * acc);
         am[k] = bm / f[k];
     }
     for (int64_t k = 0; k < N; ++k) {
-        double acc = 0.0;
-        for (int64_t j = 0; j + 1 < N; ++j) acc += am[j] * S[((j + 1) * (2 * k + 1)) % four];
+        for (int l = 0; l < 64; ++l) {
+            part[l] = 0.0;
+            for (int64_t j = l; j + 1 < N; j += 64) part[l] += am[j] * S[((j + 1) * (2 * k + 1)) % four];
+        }
+        double acc = tree64(part);
         double sg = (k & 1) ? -am[N - 1] : am[N - 1];
         double t = sg + 2.0 * acc;
         double t1 = sg + 2.0 * 0.0;
