@@ -366,6 +366,16 @@ int32_t mcs_cgrid_freebc_coeffs(const mcs_cgrid *g, double *f);
  * copies ez[0,:] = ez[1,:], ez[M+1,:] = ez[M,:], er[:,0] = er[:,1], er[:,N+1] = er[:,N].
  * U's ghosts must hold the boundary values (the solve leaves them so). */
 int32_t mcs_cyl_efield_faces(mcs_cgrid *g, double eb);
+/* Canonical cell order on the axisymmetric grid: inactive particles dropped, the
+ * rest sorted by (cell, id) over resample!'s cells (cellindex within the ghosted
+ * per-cell arrays, i in 1..M+1, j in 0..N+1), particles outside last.  Needs
+ * (M+2)(N+2) < 2^24 and ids below 2^40. */
+int32_t mcs_cyl_sort_by_cell(mcs_pop *p, mcs_cgrid *g);
+/* MCStreamer.resample! (resample.jl:7-84) per cell of the axisymmetric grid, on the
+ * canonical order of mcs_cyl_sort_by_cell (called right before; the rules and
+ * streams are those of mcs_resample_cells, keyed by the flat cell index). */
+int32_t mcs_cyl_resample(mcs_pop *p, mcs_cgrid *g, int32_t ntarget, int32_t nmax, uint64_t seed,
+                         uint32_t instance, uint32_t step, mcs_resample_stats *stats);
 /* advance! (mixed_population.jl:24-54) in the axisymmetric FieldInterp
  * (gridfields.jl:115-190: bilinear er, ez on the staggered faces, er rotated by
  * x/r, y/r; zero outside 0 < r < R, 0 < z < L) with, when track != 0, the

@@ -208,6 +208,8 @@ SIGNATURES = {
     "mcs_cyl_density": [_P, _P, _I32, _D],
     "mcs_cyl_poisson": [_P, _P, _D, _I32, _PI32, _PD],
     "mcs_cgrid_freebc_coeffs": [_P, _PD],
+    "mcs_cyl_sort_by_cell": [_P, _P],
+    "mcs_cyl_resample": [_P, _P, _I32, _I32, _U64, _U32, _U32, ctypes.POINTER(ResampleStats)],
     "mcs_cyl_efield_faces": [_P, _D],
     "mcs_advance_cyl": [_P, _P, _P, _D, _I32, _U64, _U32, _U32, ctypes.POINTER(StepStats)],
 }

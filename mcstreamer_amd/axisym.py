@@ -7,6 +7,7 @@
   density! (poisson.jl:65-84)               -> density
   poisson! (poisson.jl:6-58)                -> poisson
   solve_freebc! (freebc.jl:4-36)            -> poisson(freebc=True)
+  resample! (resample.jl:7-84)              -> sort_by_cell + resample
   the solver of main.jl:134-145             -> poisson_config
 Particles stay in 3D Cartesian coordinates; the grid is indexed by
 r = sqrt(x^2 + y^2) and z.  Arrays come back as (rows j, columns i) = (z, r),
@@ -168,3 +169,18 @@ def poisson(fields, popl, eb, ws, freebc=True):
         check(fields.ctx.handle, rc)
     ws.last_iters, ws.last_resid = it.value, res.value
     return it.value, res.value
+
+
+def sort_by_cell(popl, fields):
+    """Canonical (cell, id) order over resample!'s cells; inactive particles dropped."""
+    check(popl.ctx.handle, _lib.lib().mcs_cyl_sort_by_cell(popl.handle, fields.handle))
+
+
+def resample(popl, fields, ntarget, nmax, step=0):
+    """resample! (resample.jl:7-84) per (r, z) cell; sorts first."""
+    sort_by_cell(popl, fields)
+    st = _lib.ResampleStats()
+    check(popl.ctx.handle, _lib.lib().mcs_cyl_resample(popl.handle, fields.handle, int(ntarget), int(nmax),
+                                                       popl.seed, popl.instance, int(step), ctypes.byref(st)))
+    return st
+

@@ -393,6 +393,10 @@ extern "C" int32_t mcs_cgrid_destroy(mcs_cgrid *g)
     if (g->run_long) hipFree(g->run_long);
     if (g->fb_S) hipFree(g->fb_S);
     if (g->fb_f) hipFree(g->fb_f);
+    if (g->cs) hipFree(g->cs);
+    if (g->cs_new) hipFree(g->cs_new);
+    if (g->cnt64) hipFree(g->cnt64);
+    if (g->pre64) hipFree(g->pre64);
     delete g;
     return MCS_OK;
 }

@@ -216,6 +216,10 @@ struct mcs_cgrid {
     int32_t *run_long = nullptr;                         // count, then the cells whose run takes a wavefront
     double *fb_S = nullptr;   // free boundary: sin(pi m / 2N), m < 4N
     double *fb_f = nullptr;   // free boundary: f (N), then grad, am, bcond (N each)
+    // canonical (cell, id) order over resample!'s cells, (M+2)(N+2) + 2 entries each
+    int32_t *cs = nullptr, *cs_new = nullptr;
+    int64_t *cnt64 = nullptr, *pre64 = nullptr;
+    bool sorted_valid = false;
 };
 
 // The two-dimensional solver (mcs_mg2.hip): creation for cfg->ndim == 2, and the

@@ -1189,31 +1189,59 @@ extern "C" int32_t mcs_resample_cells(mcs_pop *p, mcs_mesh *m, int32_t ntarget, 
     return resample_impl(p, m, p->buf[0], p->buf[1], ntarget, nmax, seed, instance, step, stats);
 }
 
+// The cell lists a resample works on: starts (ncells + 2), per-cell packed counts
+// and their prefix (ncells + 2 each), and the new starts.
+struct CellLists {
+    int32_t *cs;
+    int64_t ncells;
+    int64_t *cnt64, *pre64;
+    int32_t *new_cs;
+};
+
+template <typename Buf, typename Copy>
+static int32_t resample_run(mcs_pop *p, const CellLists &L, const Buf &src, const Buf &dst, int32_t ntarget,
+                            int32_t nmax, uint64_t seed, uint32_t instance, uint32_t step,
+                            mcs_resample_stats *stats, Copy &&copy_survivors);
+
 template <typename Buf>
 static int32_t resample_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, const Buf &dst, int32_t ntarget,
                              int32_t nmax, uint64_t seed, uint32_t instance, uint32_t step,
                              mcs_resample_stats *stats)
 {
+    const CellLists L = {m->cell_start, m->ncells, m->cnt64, m->pre64, m->cell_cnt};
+    const MeshGeom g = geom(m);
+    return resample_run(p, L, src, dst, ntarget, nmax, seed, instance, step, stats,
+                        [&](const ResArgs<Buf> &r, int64_t n) {
+                            hipLaunchKernelGGL(k_resample_copy<Buf>, dim3((unsigned)cdiv(n, MCS_BLOCK)),
+                                               dim3(MCS_BLOCK), 0, p->ctx->stream, r, g, n);
+                        });
+}
+
+template <typename Buf, typename Copy>
+static int32_t resample_run(mcs_pop *p, const CellLists &L, const Buf &src, const Buf &dst, int32_t ntarget,
+                            int32_t nmax, uint64_t seed, uint32_t instance, uint32_t step,
+                            mcs_resample_stats *stats, Copy &&copy_survivors)
+{
     mcs_ctx *ctx = p->ctx;
     int32_t rc;
     if (stats) memset(stats, 0, sizeof(*stats));
-    int64_t nseg = m->ncells + 1;
+    int64_t nseg = L.ncells + 1;
     int64_t *ctr = ctx->d_small;
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, ctx->stream, (int32_t *)ctr, (int64_t)16);
     ResArgs<Buf> r;
     r.src = src;
     r.dst = dst;
-    r.cs = m->cell_start;
-    r.ncells = m->ncells;
+    r.cs = L.cs;
+    r.ncells = L.ncells;
     r.ntarget = ntarget;
     r.nmax = nmax;
     r.seed = seed;
     r.instance = instance;
     r.step = step;
-    r.packed = m->cnt64;
-    r.prefix = m->pre64;
+    r.packed = L.cnt64;
+    r.prefix = L.pre64;
     r.next_id = p->next_id;
-    r.new_cs = m->cell_cnt;
+    r.new_cs = L.new_cs;
     r.ctr = ctr;
     {
         ProfScope ps(ctx, "resample");
@@ -1221,10 +1249,10 @@ static int32_t resample_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, const Buf 
                            ctx->stream, r);
     }
     MCS_LAUNCH_CHECK(ctx);
-    rc = mcs_scan_i64(ctx, m->cnt64, m->pre64, nseg);
+    rc = mcs_scan_i64(ctx, L.cnt64, L.pre64, nseg);
     if (rc) return rc;
     int64_t hb[9];
-    rc = mcs_readback2(ctx, m->pre64 + nseg, 1, ctr, 8, hb);
+    rc = mcs_readback2(ctx, L.pre64 + nseg, 1, ctr, 8, hb);
     if (rc) return rc;
     int64_t tot = hb[0];
     int64_t h[8];
@@ -1236,15 +1264,13 @@ static int32_t resample_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, const Buf 
                         (long long)nout, (long long)p->cap);
     {
         ProfScope ps(ctx, "resample");
-        if (p->n > 0)
-            hipLaunchKernelGGL(k_resample_copy<Buf>, dim3((unsigned)cdiv(p->n, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
-                               ctx->stream, r, geom(m), p->n);
+        if (p->n > 0) copy_survivors(r, p->n);
         hipLaunchKernelGGL(k_resample_write<Buf>, dim3((unsigned)cdiv(nseg, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
                            ctx->stream, r);
     }
     MCS_LAUNCH_CHECK(ctx);
-    MCS_HIP(ctx, hipMemcpyAsync(m->cell_start, m->cell_cnt, sizeof(int32_t) * (nseg + 1),
-                                hipMemcpyDeviceToDevice, ctx->stream));
+    MCS_HIP(ctx, hipMemcpyAsync(L.cs, L.new_cs, sizeof(int32_t) * (nseg + 1), hipMemcpyDeviceToDevice,
+                                ctx->stream));
     mcs_pop_swap(p);
     if (stats) {
         stats->n_in = p->n;
@@ -1257,4 +1283,151 @@ static int32_t resample_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, const Buf 
     p->n = nout;
     p->next_id += (uint64_t)ncreated;
     return MCS_OK;
+}
+
+// ------------------------------------------- the axisymmetric grid: cell order, resample!
+
+// resample!'s cell of x on the axisymmetric grid (resample.jl:21-23): cellindex
+// with the bounds of its per-cell arrays (calloc_centers: i in 0..M+1, j in
+// 0..N+1, no test against rc), flat i + (M+2) j; -1 outside.
+__device__ __forceinline__ int64_t cyl_rs_cell(const CylView &C, const double x[3])
+{
+    const double rho = sqrt(x[0] * x[0] + x[1] * x[1]);
+    const double fi = fld_dev(rho - 0.0, C.drf), fj = fld_dev(x[2] - 0.0, C.dzf);
+    if (!(fi >= -1.0 && fi <= (double)C.M && fj >= -1.0 && fj <= (double)C.N)) return -1;
+    return ((int64_t)fi + 1) + (int64_t)(C.M + 2) * ((int64_t)fj + 1);
+}
+
+// (cell, id) keys of the canonical order: cell << 40 | id, the cells outside as
+// cell ncg, inactive particles as ~0 (dropped: they sort after every kept one).
+__global__ __launch_bounds__(MCS_BLOCK) void kr_keys(PopBuf b, int64_t n, CylView C, uint64_t ncg, uint64_t *key,
+                                                     int32_t *val)
+{
+    const int64_t p = gidx();
+    if (p >= n) return;
+    uint64_t k = ~0ULL;
+    if (b.active[p]) {
+        const double x[3] = {b.x[0][p], b.x[1][p], b.x[2][p]};
+        const int64_t c = cyl_rs_cell(C, x);
+        k = ((c < 0 ? ncg : (uint64_t)c) << 40) | b.id[p];
+    }
+    key[p] = k;
+    val[p] = (int32_t)p;
+}
+
+// cs[c] = the first sorted key of cell c or later (c = 0..ncg + 1); the last is
+// the number of kept particles, also stored as an 8-byte word at *na.
+__global__ __launch_bounds__(MCS_BLOCK) void kr_starts(const uint64_t *key, int64_t n, int64_t ncg, int32_t *cs,
+                                                       int64_t *na)
+{
+    const int64_t c = gidx();
+    if (c > ncg + 1) return;
+    const uint64_t t = (uint64_t)c << 40;
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (key[mid] < t) lo = mid + 1;
+        else hi = mid;
+    }
+    cs[c] = (int32_t)lo;
+    if (c == ncg + 1) *na = lo;
+}
+
+static int32_t cgrid_lists(mcs_cgrid *g)
+{
+    if (g->cs) return MCS_OK;
+    const int64_t ncg = (int64_t)(g->M + 2) * (g->N + 2);
+    if (hipMalloc(&g->cs, sizeof(int32_t) * (ncg + 2)) != hipSuccess ||
+        hipMalloc(&g->cs_new, sizeof(int32_t) * (ncg + 2)) != hipSuccess ||
+        hipMalloc(&g->cnt64, sizeof(int64_t) * (ncg + 2)) != hipSuccess ||
+        hipMalloc(&g->pre64, sizeof(int64_t) * (ncg + 2)) != hipSuccess)
+        return mcs_fail(g->ctx, MCS_E_HIP, "cgrid: device allocation failed");
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_cyl_sort_by_cell(mcs_pop *p, mcs_cgrid *g)
+{
+    if (!p || !g) return MCS_E_ARG;
+    mcs_ctx *ctx = p->ctx;
+    if (g->ctx != ctx) return mcs_fail(ctx, MCS_E_ARG, "cyl_sort: grid of another context");
+    if (p->prec != MCS_PREC_F64) return mcs_fail(ctx, MCS_E_ARG, "cyl_sort: FP64 populations only");
+    const int64_t ncg = (int64_t)(g->M + 2) * (g->N + 2);
+    if (ncg + 1 >= ((int64_t)1 << 24)) return mcs_fail(ctx, MCS_E_ARG, "cyl_sort: more than 2^24 cells");
+    if (p->next_id >= ((uint64_t)1 << 40)) return mcs_fail(ctx, MCS_E_ARG, "cyl_sort: ids beyond 2^40");
+    hipSetDevice(ctx->device);
+    int32_t rc = mcs_pop_ensure_alt(p);
+    if (rc) return rc;
+    rc = cgrid_lists(g);
+    if (rc) return rc;
+    const int64_t n = p->n;
+    ProfScope ps(ctx, "cyl_sort");
+    int64_t na = 0;
+    if (n > 0) {
+        uint64_t *kin = (uint64_t *)p->u64a;
+        int32_t *vin = p->i32a, *vout = p->i32b;
+        size_t tb = 0;
+        MCS_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tb, kin, kin, vin, vout, (size_t)n, 0u, 64u, ctx->stream));
+        const size_t kbytes = ((sizeof(uint64_t) * (size_t)n + 255) / 256) * 256;
+        void *tmp = nullptr;
+        rc = mcs_scratch(ctx, kbytes + tb, &tmp);
+        if (rc) return rc;
+        uint64_t *kout = (uint64_t *)tmp;
+        const MeshView mv = cyl_view(g);
+        hipLaunchKernelGGL(kr_keys, dim3((unsigned)cdiv(n, MCS_BLOCK)), dim3(MCS_BLOCK), 0, ctx->stream,
+                           p->buf[0], n, mv.C, (uint64_t)ncg, kin, vin);
+        MCS_HIP(ctx, rocprim::radix_sort_pairs((char *)tmp + kbytes, tb, kin, kout, vin, vout, (size_t)n, 0u, 64u,
+                                               ctx->stream));
+        hipLaunchKernelGGL(kr_starts, dim3((unsigned)cdiv(ncg + 2, MCS_BLOCK)), dim3(MCS_BLOCK), 0, ctx->stream,
+                           (const uint64_t *)kout, n, ncg, g->cs, g->cnt64 + ncg + 1);
+        MCS_LAUNCH_CHECK(ctx);
+        rc = mcs_readback(ctx, g->cnt64 + ncg + 1, &na, 1);
+        if (rc) return rc;
+        if (na > 0)
+            hipLaunchKernelGGL(k_sort_gather<PopBuf>, dim3((unsigned)cdiv(na, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                               ctx->stream, p->buf[0], p->buf[1], (const int32_t *)vout, na);
+        MCS_LAUNCH_CHECK(ctx);
+        mcs_pop_swap(p);
+    } else {
+        hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)cdiv(ncg + 2, MCS_BLOCK)), dim3(MCS_BLOCK), 0, ctx->stream,
+                           g->cs, ncg + 2);
+        MCS_LAUNCH_CHECK(ctx);
+    }
+    p->n = na;
+    g->sorted_valid = true;
+    return MCS_OK;
+}
+
+// Survivors of the (r, z) resample, as k_resample_copy with the axisymmetric cell.
+__global__ __launch_bounds__(MCS_BLOCK) void k_resample_copy_cyl(ResArgs<PopBuf> r, CylView C, int64_t n)
+{
+    int64_t i = gidx();
+    if (i >= n) return;
+    const double x[3] = {r.src.x[0][i], r.src.x[1][i], r.src.x[2][i]};
+    int64_t c = cyl_rs_cell(C, x);
+    if (c < 0) c = r.ncells;
+    int64_t a = r.cs[c];
+    if (c < r.ncells && r.cs[c + 1] - a > r.nmax) return;
+    int64_t base = r.prefix[c] & 0xffffffffLL;
+    copy_rec(r.src, i, r.dst, base + (i - a));
+}
+
+extern "C" int32_t mcs_cyl_resample(mcs_pop *p, mcs_cgrid *g, int32_t ntarget, int32_t nmax, uint64_t seed,
+                                    uint32_t instance, uint32_t step, mcs_resample_stats *stats)
+{
+    if (!p || !g) return MCS_E_ARG;
+    mcs_ctx *ctx = p->ctx;
+    if (ntarget < 1 || nmax < ntarget) return mcs_fail(ctx, MCS_E_ARG, "resample: need 1 <= ntarget <= nmax");
+    if (!g->sorted_valid)
+        return mcs_fail(ctx, MCS_E_ARG, "resample: population is not in cell order (call mcs_cyl_sort_by_cell)");
+    hipSetDevice(ctx->device);
+    int32_t rc = mcs_pop_ensure_alt(p);
+    if (rc) return rc;
+    const int64_t ncg = (int64_t)(g->M + 2) * (g->N + 2);
+    const CellLists L = {g->cs, ncg, g->cnt64, g->pre64, g->cs_new};
+    const CylView C = cyl_view(g).C;
+    return resample_run(p, L, p->buf[0], p->buf[1], ntarget, nmax, seed, instance, step, stats,
+                        [&](const ResArgs<PopBuf> &r, int64_t n) {
+                            hipLaunchKernelGGL(k_resample_copy_cyl, dim3((unsigned)cdiv(n, MCS_BLOCK)),
+                                               dim3(MCS_BLOCK), 0, ctx->stream, r, C, n);
+                        });
 }

@@ -784,6 +784,41 @@ int32_t ref_sort_by_cell(ref_pop *p, ref_mesh *m, ref_pop *scr)
     return 0;
 }
 
+/* Canonical order on the axisymmetric grid: actives by (cell, id) over resample!'s
+ * cells (cellindex within calloc_centers' bounds: i in 0..M+1, j in 0..N+1, flat
+ * i + (M+2) j; resample.jl:21-23), outside particles last; cell_start has
+ * (M+2)(N+2) + 2 entries. */
+int32_t ref_cyl_sort_by_cell(ref_pop *p, const ref_cgrid *g, int64_t *cell_start, ref_pop *scr)
+{
+    int64_t ncg = (int64_t)(g->M + 2) * (g->N + 2);
+    sort_item *items = (sort_item *)malloc(sizeof(sort_item) * (size_t)(p->n + 1));
+    int64_t na = 0;
+    for (int64_t i = 0; i < p->n; ++i) {
+        if (!p->active[i]) continue;
+        const double *x = &p->x[3 * i];
+        double rho = sqrt(x[0] * x[0] + x[1] * x[1]);
+        double fi = ref_fld(rho - 0.0, g->drf), fj = ref_fld(x[2] - 0.0, g->dzf);
+        int64_t key = ncg;
+        if (fi >= -1.0 && fi <= (double)g->M && fj >= -1.0 && fj <= (double)g->N)
+            key = ((int64_t)fi + 1) + (int64_t)(g->M + 2) * ((int64_t)fj + 1);
+        items[na].key = key;
+        items[na].id = p->id[i];
+        items[na].src = i;
+        na++;
+    }
+    qsort(items, (size_t)na, sizeof(sort_item), cmp_item);
+    for (int64_t c = 0; c <= ncg + 1; ++c) cell_start[c] = 0;
+    for (int64_t a = 0; a < na; ++a) {
+        copy_slot(scr, a, p, items[a].src);
+        cell_start[items[a].key + 1] += 1;
+    }
+    for (int64_t c = 1; c <= ncg + 1; ++c) cell_start[c] += cell_start[c - 1];
+    for (int64_t a = 0; a < na; ++a) copy_slot(p, a, scr, a);
+    p->n = na;
+    free(items);
+    return 0;
+}
+
 /* density! (MCStreamer/src/poisson.jl:65-84): arr[cell] += w*val/dV per active
  * particle in population order (NGP), or the same amount spread with trilinear
  * weights over the 8 nearest cell centres (CIC); shares outside the mesh drop. */

@@ -174,6 +174,7 @@ void ref_cyl_faces(ref_cgrid *g, double eb);
 void ref_cyl_freebc_bcond(const ref_cgrid *g, const double *f, double *bcond);
 void ref_cyl_freebc_inhom(ref_cgrid *g, double s, const double *bcond);
 void ref_cyl_freebc_ghost(ref_cgrid *g, const double *bcond);
+int32_t ref_cyl_sort_by_cell(ref_pop *p, const ref_cgrid *g, int64_t *cell_start, ref_pop *scr);
 int64_t ref_cell_key(const ref_mesh *m, const double x[3]);
 int32_t ref_sort_by_cell(ref_pop *p, ref_mesh *m, ref_pop *scratch);
 void ref_deposit(const ref_pop *p, ref_mesh *m, int32_t which, int32_t scheme, double val);

@@ -144,6 +144,8 @@ def lib():
         L.ref_cyl_freebc_inhom.restype = None
         L.ref_cyl_freebc_ghost.argtypes = [P(RefCGrid), _PD]
         L.ref_cyl_freebc_ghost.restype = None
+        L.ref_cyl_sort_by_cell.argtypes = [P(RefPop), P(RefCGrid), _PI64, P(RefPop)]
+        L.ref_cyl_sort_by_cell.restype = ctypes.c_int32
         L.ref_cell_key.argtypes = [P(RefMesh), _PD]
         L.ref_cell_key.restype = ctypes.c_int64
         L.ref_sort_by_cell.argtypes = [P(RefPop), P(RefMesh), P(RefPop)]
@@ -378,6 +380,35 @@ def cyl_poisson(grid, pop, conf, eb, f=None, freebc=True):
         it += it2
     cyl_faces(grid, eb)
     return it
+
+
+def cyl_sort_by_cell(pop, grid):
+    ncg = (grid.M + 2) * (grid.N + 2)
+    grid.cell_start = np.zeros(ncg + 2, dtype=np.int64)
+    p = pop.cstruct()
+    scr = pop.scratch()
+    sp = scr.cstruct()
+    lib().ref_cyl_sort_by_cell(ctypes.byref(p), ctypes.byref(grid.cstruct()),
+                               grid.cell_start.ctypes.data_as(_PI64), ctypes.byref(sp))
+    pop.sync_from(p)
+
+
+def cyl_resample(pop, grid, ntarget, nmax, seed=0, instance=0, step=0):
+    """resample! per (r, z) cell: the oracle's per-cell resample over the cell
+    lists of cyl_sort_by_cell (sorts first)."""
+    cyl_sort_by_cell(pop, grid)
+    ncg = (grid.M + 2) * (grid.N + 2)
+    m = RefMesh()
+    m.n[0], m.n[1], m.n[2] = ncg, 1, 1
+    m.cell_start = grid.cell_start.ctypes.data_as(_PI64)
+    p = pop.cstruct()
+    scr = pop.scratch()
+    sp = scr.cstruct()
+    st = RefResampleStats()
+    rc = lib().ref_resample_cells(ctypes.byref(p), ctypes.byref(m), int(ntarget), int(nmax), int(seed),
+                                  int(instance), int(step), ctypes.byref(sp), ctypes.byref(st))
+    pop.sync_from(p)
+    return rc, st
 
 
 def freebc_bcond(grid, f):

@@ -115,6 +115,32 @@ def test_density_long_runs_bitwise():
     assert c.max() > 2000
 
 
+def test_cyl_resample_matches_oracle_bitwise():
+    """Canonical (cell, id) order and resample! per (r, z) cell, twice (merged cells,
+    splits, the outside bucket, inactive particles dropped)."""
+    table, T, popl, P, grid, fields, G = _setup(n=60000, seed=8)
+    x, v, w, s, act = _states(60000, 8)
+    rng = np.random.default_rng(12)
+    # a dense knot of light particles (merged) among heavy ones (split where thin)
+    x[:20000] = rng.normal(size=(20000, 3)) * 1.5e-5 + np.array([1e-4, 0.0, 0.5 * LL])
+    w[20000:] = rng.integers(1, 9, 40000)
+    popl.upload(jumc.ParticleStates(x, v, w, s, act))
+    P = R.Pop(x, v, w, s, act, cap=4 * 60000)
+    for step in range(2):
+        st = axisym.resample(popl, fields, 10, 15, step=step)
+        rc, rs = R.cyl_resample(P, G, 10, 15, seed=popl.seed, step=step)
+        print(f"resample {step}: n {st.n_in} -> {st.n_out}, merged {st.cells_merged}, splits {st.splits}", flush=True)
+        assert rc == 0
+        assert (st.n_out, st.cells_merged, st.clones, st.splits, st.removed) == \
+            (rs.n_out, rs.cells_merged, rs.clones, rs.splits, rs.removed)
+    assert rs.cells_merged > 0 and rs.splits > 0
+    g = popl.download()
+    r = P.view()
+    np.testing.assert_array_equal(g.id, r["id"])
+    for k in ("x", "v", "w", "s", "active"):
+        np.testing.assert_array_equal(getattr(g, k), r[k], err_msg=k)
+
+
 def test_faces_match_oracle_bitwise():
     G = R.CGrid(RR, LL, MM, NN)
     fields = axisym.GridFields(axisym.Grid(RR, LL, MM, NN))
@@ -191,7 +217,7 @@ def test_advance_in_axisymmetric_field_matches_oracle():
 
 def test_axisymmetric_streamer_steps():
     """poisson! (with the free boundary) + advance! (field and tracker of the same
-    grid) + repack!, four steps, against the same chain in the oracle."""
+    grid) + repack! + resample!, four steps, against the same chain in the oracle."""
     table, T, popl, P, grid, fields, G = _setup(n=20000, seed=6)
     axisym.density(fields, "qfixed", popl, 1.0)
     R.cyl_density(P, G, "qfixed", 1.0)
@@ -206,7 +232,10 @@ def test_axisymmetric_streamer_steps():
         rs = R.advance_cyl(P, T, G, dt, track=True, seed=popl.seed, step=step)
         jumc.repack(popl)
         R.repack(P)
-        print(f"step {step}: mg {it}/{it_ref} n {P.n} births {rs.births}", flush=True)
+        ss = axisym.resample(popl, fields, 10, 15, step=step)
+        rcr, rr = R.cyl_resample(P, G, 10, 15, seed=popl.seed, step=step)
+        assert ss.n_out == rr.n_out
+        print(f"step {step}: mg {it}/{it_ref} n {P.n} births {rs.births} merged {rr.cells_merged}", flush=True)
         assert it == it_ref
         assert (st.births, st.deaths) == (rs.births, rs.deaths)
     g = popl.download()

@@ -281,3 +281,45 @@ def test_freebc_reproduces_point_charge_between_grounded_planes():
         err_wall = np.abs(out[False][sel, c] - phi[sel]) / np.abs(phi[sel])
         assert err_free.max() < 0.02, (c, err_free.max())
         assert err_wall.max() > 0.2, (c, err_wall.max())
+
+
+def test_cyl_resample_per_cell_rules():
+    """resample! on the (r, z) grid: cells over nmax end with ntarget equal weights
+    and their total weight; thin cells split integer weights above 2; particles
+    outside the per-cell arrays are kept as they are."""
+    Rr, L, M, N = 1e-3, 2e-3, 8, 16
+    G = R.CGrid(Rr, L, M, N)
+    drf, dzf = G.steps[2], G.steps[3]
+    rng = np.random.default_rng(7)
+    n = 4000
+    r = np.concatenate([rng.uniform(0, 0.3 * drf, 300), rng.uniform(0, 1.3 * Rr, n - 300)])
+    th = rng.uniform(0, 2 * np.pi, n)
+    z = np.concatenate([np.full(300, 0.5 * dzf + 5 * dzf), rng.uniform(-2 * dzf, L + 2 * dzf, n - 300)])
+    x = np.stack([r * np.cos(th), r * np.sin(th), z], axis=1)
+    w = rng.integers(1, 6, n).astype(np.float64)
+    P = R.Pop(x, np.zeros((n, 3)), w, np.ones(n), np.ones(n, dtype=np.uint8), cap=4 * n)
+
+    def cells(xx):
+        i = np.floor(np.hypot(xx[:, 0], xx[:, 1]) / drf).astype(int) + 1
+        j = np.floor(xx[:, 2] / dzf).astype(int) + 1
+        ok = (i <= M + 1) & (j >= 0) & (j <= N + 1)
+        return np.where(ok, i + (M + 2) * j, -1)
+
+    c0 = cells(x)
+    rc, st = R.cyl_resample(P, G, 10, 15, seed=3, step=1)
+    assert rc == 0
+    v = P.view()
+    c1 = cells(v["x"])
+    assert st.cells_merged >= 1
+    for c in np.unique(c0):
+        w_in = w[c0 == c].sum()
+        w_out = v["w"][c1 == c].sum()
+        np.testing.assert_allclose(w_out, w_in, rtol=1e-12)
+        k = (c1 == c).sum()
+        if c >= 0 and (c0 == c).sum() > 15:
+            assert k == 10 and np.allclose(v["w"][c1 == c], w_in / 10, rtol=1e-15)
+        if c < 0:
+            assert k == (c0 == c).sum()
+    # canonical order: cells ascending, outside last
+    key = np.where(c1 < 0, 10 ** 9, c1)
+    assert (np.diff(key) >= 0).all()
