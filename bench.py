@@ -554,9 +554,10 @@ def run_axisym(args):
     one step is poisson! (cylindrical density!, charge, solve_freebc!'s two solves
     of the axisymmetric multigrid with 8 levels and the reference's absolute
     tolerance 1e8 e around the free radial boundary, faces), advance! in the
-    axisymmetric FieldInterp with the ion tracker, and repack!.  The seed is a
-    Gaussian of width 1e-4 m at z0 = 1 cm on the axis ([seed] of the sample input)
-    of `--axisym-n` electrons over as many ions; C1 gas (N2-like + 20 % O2-like)."""
+    axisymmetric FieldInterp with the ion tracker, repack!, and resample! per
+    (r, z) cell with the sample input's ntarget = 10, nmax = 15.  `--axisym-n`
+    electrons over as many ions fill a column (see below); C1 gas (N2-like +
+    20 % O2-like)."""
     from mcstreamer_amd import axisym, gases, jumc, lxcat
     from mcstreamer_amd import constants as co
 
@@ -571,9 +572,14 @@ def run_axisym(args):
     fields = axisym.GridFields(grid)
     n = int(args.axisym_n)
     rng = np.random.default_rng(args.seed)
-    wseed = 1e-4
-    x = rng.normal(size=(n, 3)) * wseed
-    x[:, 2] += 1e-2
+    # a plasma column of radius 4 mm over z = 1..5 cm, r uniform so that every cell
+    # of the column holds about n / (410 x 4096) = 12 electrons: inside the
+    # resampling window (ntarget 10, nmax 15 of the sample input), so the
+    # population keeps its size from step to step
+    r = rng.uniform(0.0, 4e-3, n)
+    th = rng.uniform(0.0, 2 * np.pi, n)
+    x = np.stack([r * np.cos(th), r * np.sin(th), rng.uniform(1e-2, 5e-2, n)], axis=1)
+    del r, th
     v = np.zeros((n, 3))
     s = -np.log1p(-rng.uniform(size=n))
     popl = jumc.Population(3 * n, jumc.ParticleStates(x, v, np.ones(n), s, np.ones(n, dtype=np.uint8)), table,
@@ -591,6 +597,7 @@ def run_axisym(args):
         it, _ = axisym.poisson(fields, popl, eb, ws)
         st = jumc.advance(popl, field, dt, tracker, step=i)
         jumc.repack(popl)
+        axisym.resample(popl, fields, 10, 15, step=i)
         return it, st
 
     for i in range(args.warmup):
@@ -609,7 +616,7 @@ def run_axisym(args):
     el = time.perf_counter() - t0
     ctx.profile(False)
     kt = {}
-    for k in ("advance", "cyl_density", "cyl_faces", "mg2_sweep0", "repack"):
+    for k in ("advance", "cyl_density", "cyl_faces", "cyl_freebc", "mg2_sweep0", "cyl_sort", "resample"):
         ms, cnt = ctx.profile_read(k)
         kt[k] = {"ms_total": ms, "launches": cnt}
     adv_s = kt["advance"]["ms_total"] / 1e3 / max(kt["advance"]["launches"], 1)
@@ -621,9 +628,9 @@ def run_axisym(args):
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"FP64 axisymmetric streamer step, {M}x{N} cells (R=1.5 cm, L=10 cm), "
-                               f"{n:g}-electron Gaussian seed (w=1e-4 m, z0=1 cm), eb=15 kV/cm, dt=1e-12 s, "
+                               f"{n:g} electrons in a 4 mm x 4 cm column (~12 per cell), eb=15 kV/cm, dt=1e-12 s, "
                                f"poisson! with solve_freebc! ({conf.levels} levels, tol 1e8 e) + advance! "
-                               "(cylindrical field, tracker) + repack!",
+                               "(cylindrical field, tracker) + repack! + resample! (10..15 per cell)",
                    "grid": [M, N], "particles_per_step": psteps / max(args.steps, 1), "fmg_iterations": iters,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
