@@ -1,0 +1,168 @@
+"""The reference's input-file driver for axisymmetric runs (MCStreamer/src/main.jl:15-197
+`main`, :217-315 `nsteps`, timesteps.jl:5-28 `TimeStepper`/`atstep`), on libmcs.
+
+    run = driver.setup("input.toml")    # main(finput; run=false)
+    run.nsteps(k)                        # nsteps(...) for k steps
+    driver.main("input.toml")            # the whole run to tmax
+
+The TOML schema is the reference's (samples/input.toml): eb and field_units,
+nair, n, maxp, ntarget, nmax, dt, tmax, cross_sections (LXCat JSON paths,
+relative to the input file), rng_seed, initial_weight, poisson_levels,
+[domain] R L M N, [seed] z0 z1 w, [interval] poisson output resample.  Sections
+this build does not carry (photoionization, init_files, denoise, needle) raise
+NotImplementedError rather than being ignored.  The seed positions come from
+numpy's generator seeded with rng_seed (the reference draws them from Julia's
+global RNG; the distribution, sampseg, is the same); everything after the
+seed draws from the Philox streams keyed by rng_seed.  Output (every
+interval.output) is one .npz per call with the grid arrays (u, er, ez, qfixed,
+qpart) in place of the reference's JLD2 files (io.jl).
+"""
+import json
+import os
+
+import numpy as np
+
+from . import axisym, lxcat
+from . import constants as co
+from .jumc import CollisionTable, ParticleStates, Population, advance, context, repack
+
+UNSUPPORTED = ("photoionization", "init_files", "denoise", "needle")
+
+
+class TimeStepper:
+    """TimeStepper (timesteps.jl:5-20): fires at t >= start + ... as atstep does:
+    when t >= step * i, call f(i) and advance i (the reference ignores `start`
+    in atstep; so does this)."""
+
+    def __init__(self, step, start=0.0):
+        self.i = 0
+        self.start = start
+        self.step = step
+
+    def atstep(self, f, t):
+        if t >= self.step * self.i:
+            f(self.i)
+            self.i += 1
+
+
+def geteb(inp):
+    """geteb (main.jl:346-357): -eb times the unit's scale."""
+    units = inp.get("field_units", "Td")
+    nair = float(inp.get("nair", co.nair))
+    scale = {"Td": co.Td * nair, "kV/cm": 1e5, "MV/m": 1e6, "V/m": 1.0, "": 1.0}[units]
+    return -float(inp["eb"]) * scale
+
+
+def sampseg(z0, z1, w, n, rng):
+    """sampseg (main.jl:321-340): n points around the segment z0..z1 on the axis
+    with a Gaussian blur of width w."""
+    assert z1 >= z0
+    l = z1 - z0
+    p = l / (l + np.sqrt(2 * np.pi) * w)
+    seg = rng.uniform(size=n) < p
+    x = rng.normal(size=(n, 3)) * w
+    u = rng.uniform(size=n)
+    x[seg, 2] = z0 + u[seg] * l
+    cap = ~seg
+    x[cap, 2] += np.where(x[cap, 2] > 0, z1, z0)
+    return x
+
+
+def load_input(path):
+    import tomli
+
+    with open(path, "rb") as fd:
+        inp = tomli.load(fd)
+    bad = [k for k in UNSUPPORTED if k in inp]
+    if bad:
+        raise NotImplementedError(f"{path}: sections {bad} are not supported by this build")
+    return inp
+
+
+class Run:
+    """The state main() builds for an axisymmetric run."""
+
+    def __init__(self, finput, ctx=None, energy_points=100_000):
+        inp = load_input(finput)
+        self.input = inp
+        self.ctx = ctx or context()
+        self.outpath = os.path.splitext(os.path.abspath(finput))[0]
+        seed = int(inp.get("rng_seed", 0))
+        nair = float(inp.get("nair", co.nair))
+        dom = inp["domain"]
+        self.grid = axisym.Grid(float(dom["R"]), float(dom["L"]), int(dom["M"]), int(dom["N"]))
+        self.fields = axisym.GridFields(self.grid, ctx=self.ctx)
+        self.ntarget, self.nmax = int(inp["ntarget"]), int(inp["nmax"])
+        densities = {"N2": nair * 0.79, "O2": nair * 0.21}
+        energy = np.linspace(0.0, 1000 * co.eV, energy_points)   # LinRange(0, 1000 eV, 100_000)
+        files = [os.path.join(os.path.dirname(os.path.abspath(finput)), f) for f in inp["cross_sections"]]
+        proc, rate, maxrate, _ = lxcat.load_lxcat(files, densities, energy)
+        self.table = CollisionTable(proc, energy, rate, maxrate, egrid="uniform", ctx=self.ctx)
+        self.dt = float(inp["dt"])
+        self.tmax = float(inp["tmax"])
+        self.eb = geteb(inp)
+        s = inp["seed"]
+        w, z0 = float(s["w"]), float(s["z0"])
+        z1 = float(s.get("z1", z0))
+        n = int(inp["n"])
+        rng = np.random.default_rng(seed)
+        x = sampseg(z0, z1, w, n, rng)
+        wt = float(inp.get("initial_weight", 1.0))
+        sclock = -np.log1p(-rng.uniform(size=n))   # ElectronState's nextcoll()
+        self.popl = Population(int(inp["maxp"]), ParticleStates(x, np.zeros((n, 3)), np.full(n, wt), sclock,
+                                                                  np.ones(n, dtype=np.uint8)),
+                               self.table, seed=seed, ctx=self.ctx)
+        axisym.density(self.fields, "qfixed", self.popl, 1.0)     # main.jl:130-132
+        levels = inp.get("poisson_levels")
+        self.conf = axisym.poisson_config(self.grid, levels=None if levels is None else int(levels))
+        self.ws = axisym.mgmod.Workspace(self.conf, (self.grid.M, self.grid.N), ctx=self.ctx)
+        self.efield = axisym.FieldInterp(self.fields)
+        self.tracker = axisym.CollisionTracker(self.fields)
+        iv = inp["interval"]
+        self.poisson_ts = TimeStepper(float(iv["poisson"]))
+        self.output_ts = TimeStepper(float(iv["output"]))
+        self.resample_ts = TimeStepper(float(iv["resample"]), self.dt)
+        self.i = 0
+        self.log = []
+
+    def save_fields(self, j):
+        os.makedirs(self.outpath, exist_ok=True)
+        arrs = {k: self.fields.get(k) for k in ("u", "er", "ez", "qfixed", "qpart")}
+        meta = json.dumps({"t": self.i * self.dt, "step": self.i, "grid": [self.grid.R, self.grid.L,
+                                                                           self.grid.M, self.grid.N]})
+        np.savez(os.path.join(self.outpath, f"{j:04d}.npz"), meta=np.array(meta), **arrs)
+
+    def nsteps(self, k, output=True):
+        """k steps of nsteps (main.jl:238-315): Poisson, output, advance!,
+        diagnostics, resample (repack! first), repack!."""
+        for _ in range(k):
+            self.i += 1
+            t = (self.i - 1) * self.dt
+            self.poisson_ts.atstep(lambda j: axisym.poisson(self.fields, self.popl, self.eb, self.ws), t)
+            if output:
+                self.output_ts.atstep(self.save_fields, t)
+            st = advance(self.popl, self.efield, self.dt, self.tracker, step=self.i - 1)
+            m = self.popl.moments()
+            rec = {"t": t, "n": st.n_out, "births": st.births, "deaths": st.deaths, "nactive": m.nactive,
+                   "weight": m.weight, "mean_energy_ev": (m.wenergy / m.weight / co.eV) if m.weight else 0.0}
+
+            def do_resample(j):
+                repack(self.popl)
+                rs = axisym.resample(self.popl, self.fields, self.ntarget, self.nmax, step=self.i - 1)
+                rec["resample"] = (rs.n_in, rs.n_out)
+
+            self.resample_ts.atstep(do_resample, t)
+            repack(self.popl)
+            self.log.append(rec)
+        return self.log[-1] if self.log else None
+
+
+def setup(finput, ctx=None, **kw):
+    return Run(finput, ctx=ctx, **kw)
+
+
+def main(finput, ctx=None):
+    """main(finput) (main.jl:15-197) to tmax."""
+    run = Run(finput, ctx=ctx)
+    run.nsteps(int(run.tmax // run.dt))
+    return run

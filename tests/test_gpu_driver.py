@@ -1,0 +1,69 @@
+"""The input-file driver on the GPU: a small axisymmetric run from a TOML file in
+the reference's schema (MCStreamer/src/main.jl), steps, output files, and the
+weight bookkeeping of resample!."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from mcstreamer_amd import driver, gases
+
+pytestmark = pytest.mark.gpu
+
+TOML = """
+eb = 50.0
+field_units = "kV/cm"
+n = 4000
+maxp = 200000
+ntarget = 10
+nmax = 15
+dt = 1e-12
+tmax = 4e-12
+cross_sections = ["gas.json"]
+rng_seed = 3
+[domain]
+R = 1e-3
+L = 2e-3
+M = 32
+N = 64
+[seed]
+z0 = 1e-3
+w = 1e-4
+[interval]
+poisson = 1e-12
+output = 2e-12
+resample = 1e-12
+"""
+
+
+def _gas(path):
+    items = gases.n2_like_items(np.geomspace(1e-3, 1e3, 400)) + gases.o2_like_items(np.geomspace(1e-3, 1e3, 400))
+    for it in items:
+        it["target"] = {"N2like": "N2", "O2like": "O2"}[it["target"]]
+    with open(path, "w") as f:
+        json.dump(items, f)
+
+
+def test_driver_runs_an_axisymmetric_input(tmp_path):
+    _gas(tmp_path / "gas.json")
+    p = tmp_path / "input.toml"
+    p.write_text(TOML)
+    run = driver.setup(str(p), energy_points=4000)
+    assert run.eb == -5e6
+    w0 = run.popl.moments().weight
+    log = run.nsteps(4)
+    print(run.log, flush=True)
+    assert len(run.log) == 4
+    # output at t = 0 and 2 ps
+    out = sorted(os.listdir(run.outpath))
+    assert out == ["0000.npz", "0001.npz"]
+    with np.load(os.path.join(run.outpath, "0001.npz"), allow_pickle=False) as z:
+        assert z["u"].shape == (66, 34) and z["er"].shape == (67, 35)
+        assert np.abs(z["er"]).max() > 0
+    # weight: births add, attachments remove, resample keeps it
+    n_born = sum(r["births"] for r in run.log)
+    assert log["nactive"] > 0
+    assert run.popl.moments().weight <= w0 + n_born + 1e-6
+    for r in run.log:
+        assert "resample" in r
