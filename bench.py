@@ -591,7 +591,7 @@ def run_axisym(args):
     conf = axisym.poisson_config(grid, levels=args.mg2d_levels)
     ws = axisym.mgmod.Workspace(conf, (M, N))
     field, tracker = axisym.FieldInterp(fields), axisym.CollisionTracker(fields)
-    eb, dt = 1.5e6, 1e-12
+    eb, dt = -1.5e6, 1e-12   # geteb (main.jl:346-357) of eb = 15 kV/cm
 
     def step(i):
         it, _ = axisym.poisson(fields, popl, eb, ws)

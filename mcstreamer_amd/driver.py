@@ -30,9 +30,8 @@ UNSUPPORTED = ("photoionization", "init_files", "denoise", "needle")
 
 
 class TimeStepper:
-    """TimeStepper (timesteps.jl:5-20): fires at t >= start + ... as atstep does:
-    when t >= step * i, call f(i) and advance i (the reference ignores `start`
-    in atstep; so does this)."""
+    """TimeStepper and atstep (timesteps.jl:5-28): when t >= step * i, call f(i)
+    and advance i.  `start` is stored but, as in the reference's atstep, unused."""
 
     def __init__(self, step, start=0.0):
         self.i = 0
@@ -150,6 +149,7 @@ class Run:
                 repack(self.popl)
                 rs = axisym.resample(self.popl, self.fields, self.ntarget, self.nmax, step=self.i - 1)
                 rec["resample"] = (rs.n_in, rs.n_out)
+                rec["weight_resampled"] = self.popl.moments().weight
 
             self.resample_ts.atstep(do_resample, t)
             repack(self.popl)

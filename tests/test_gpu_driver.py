@@ -61,9 +61,8 @@ def test_driver_runs_an_axisymmetric_input(tmp_path):
     with np.load(os.path.join(run.outpath, "0001.npz"), allow_pickle=False) as z:
         assert z["u"].shape == (66, 34) and z["er"].shape == (67, 35)
         assert np.abs(z["er"]).max() > 0
-    # weight: births add, attachments remove, resample keeps it
-    n_born = sum(r["births"] for r in run.log)
-    assert log["nactive"] > 0
-    assert run.popl.moments().weight <= w0 + n_born + 1e-6
+    # resample! keeps the weight (main.jl:300-306 asserts it to 1e-5)
+    assert log["nactive"] > 0 and run.log[0]["weight"] == w0
     for r in run.log:
-        assert "resample" in r
+        np.testing.assert_allclose(r["weight_resampled"], r["weight"], rtol=1e-12)
+    assert run.log[0]["resample"][1] < run.log[0]["resample"][0]   # the seed's dense cells merged
