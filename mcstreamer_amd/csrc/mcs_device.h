@@ -25,6 +25,7 @@ struct CylView {
     int32_t M, N;
     double R, L;
     double dr, dz, drf, dzf;
+    double inv_dr, inv_dz;   // 1/dr, 1/dz: normdivrem's quotients
     const double *er, *ez;   // (M+3)(N+3)
     double *qfixed;          // (M+2)(N+2)
     const double *dv;        // M
@@ -80,6 +81,8 @@ static inline MeshView cyl_view(const mcs_cgrid *g)
     v.C.dz = g->dz;
     v.C.drf = g->drf;
     v.C.dzf = g->dzf;
+    v.C.inv_dr = 1.0 / g->dr;
+    v.C.inv_dz = 1.0 / g->dz;
     v.C.er = g->arr[MCS_CARR_ER];
     v.C.ez = g->arr[MCS_CARR_EZ];
     v.C.qfixed = g->arr[MCS_CARR_QFIXED];
@@ -113,6 +116,8 @@ __device__ __forceinline__ int64_t cyl_cell_dev(const CylView &C, const double x
 // FieldInterp (MCStreamer/src/gridfields.jl:115-190) on the axisymmetric grid:
 // normdivrem and staggered (:94-109) per axis, bilinear er (faces in r, staggered
 // in z) and ez (staggered in r, faces in z), er rotated by x/r and y/r.
+// normdivrem is @fastmath, which licenses x * (1/y) for its x / y (the 3D gather
+// reads it the same way); the rotation's divisions by r are exact divisions.
 __device__ __forceinline__ void efield_cyl_dev(const CylView &C, const double x[3], double e[3],
                                                int32_t &flags)
 {
@@ -123,7 +128,7 @@ __device__ __forceinline__ void efield_cyl_dev(const CylView &C, const double x[
         e[2] = 0.0;
         return;
     }
-    const double qr = r / C.dr, qz = x[2] / C.dz;
+    const double qr = r * C.inv_dr, qz = x[2] * C.inv_dz;
     // trunc; qr < R / dr and qz < L / dz keep these within M and N (the min only
     // guards the arrays)
     const int32_t i = min((int32_t)qr, C.M), j = min((int32_t)qz, C.N);

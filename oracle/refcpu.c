@@ -218,8 +218,8 @@ static int64_t cyl_cell(const ref_cgrid *g, const double x[3], int64_t *ci)
 }
 
 /* FieldInterp (gridfields.jl:115-190) on the axisymmetric grid: zero unless
- * 0 < r < R and 0 < z < L (inside, grid.jl:45-49); normdivrem (x / y, trunc) and
- * staggered per axis (:94-109); er bilinear on (i + 1, j1), ez on (i1, j + 1)
+ * 0 < r < R and 0 < z < L (inside, grid.jl:45-49); normdivrem (@fastmath x / y,
+ * restated as x * (1/y) like the 3D gather; trunc) and staggered per axis (:94-109); er bilinear on (i + 1, j1), ez on (i1, j + 1)
  * with the 0-based origin of the offset arrays; er rotated by x / r, y / r. */
 static void cyl_efield(const ref_cgrid *g, const double x[3], double e[3], int32_t *flags)
 {
@@ -230,7 +230,7 @@ static void cyl_efield(const ref_cgrid *g, const double x[3], double e[3], int32
         e[2] = 0.0;
         return;
     }
-    double qr = r / g->dr, qz = x[2] / g->dz;
+    double qr = r * (1.0 / g->dr), qz = x[2] * (1.0 / g->dz);
     int64_t i = (int64_t)trunc(qr), j = (int64_t)trunc(qz);
     if (i > g->M) i = g->M;   /* never taken: r < R bounds qr by M */
     if (j > g->N) j = g->N;
