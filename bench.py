@@ -502,6 +502,24 @@ def run_mg2d(args):
         t_solve += time.perf_counter() - t0
         iters.append(it.value)
     ctx.profile(False)
+    # the finest-level sweep timed on its own: the solves above replay the cycle
+    # from a HIP graph (no per-launch records), so two more solves launch phase
+    # by phase on a second workspace
+    os.environ["MCS_MG_GRAPH"] = "0"
+    try:
+        ws_eager = mg.Workspace(conf, (M, N), ctx=ctx)
+    finally:
+        del os.environ["MCS_MG_GRAPH"]
+    ctx.profile_reset()
+    ctx.profile(True)
+    for _ in range(2):
+        u.copy_(u0)
+        torch.cuda.synchronize()
+        _lib.check(ctx.handle, _lib.lib().mcs_mg_solve_device(ws_eager.handle, ctypes.c_void_p(u.data_ptr()),
+                                                              ctypes.c_void_p(bf.data_ptr()), ctypes.byref(it),
+                                                              ctypes.byref(res)))
+    ctx.synchronize()
+    ctx.profile(False)
     sw_ms, sw_n = ctx.profile_read("mg2_sweep0")
     sweep_s = sw_ms / 1e3 / max(sw_n, 1)
     cells = float(M) * N

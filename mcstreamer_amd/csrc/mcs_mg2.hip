@@ -582,6 +582,8 @@ void mgv2(mcs_mg *mg, double *x, const double *b, int level, bool zero = false)
     }
 }
 
+void fmg2_body(mcs_mg *mg, double *x, const double *b);
+
 // fmg! (Multigrid.jl:380-434).  Returns 1 after a correction, 0 when converged,
 // negative on a device error.
 int fmg2(mcs_mg *mg, double *x, const double *b, double *eps_out)
@@ -604,6 +606,49 @@ int fmg2(mcs_mg *mg, double *x, const double *b, double *eps_out)
     const double eps = sqrt(ss) / sqrt((double)F.n[0] * (double)F.n[1]);
     *eps_out = eps;
     if (mg->cfg.s * mg->tol_abs > eps) return 0;
+    if (mg->use_graph) {
+        // the correction is the same launch sequence for a given (x, b): captured
+        // once (profiling records left out) and replayed
+        if (!mg->graph_exec || mg->graph_x != x || mg->graph_b != b) {
+            if (mg->graph_exec) {
+                hipGraphExecDestroy(mg->graph_exec);
+                mg->graph_exec = nullptr;
+            }
+            const bool prof = ctx->prof;
+            ctx->prof = false;
+            hipGraph_t g = nullptr;
+            if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) != hipSuccess) {
+                ctx->prof = prof;
+                return -1;
+            }
+            fmg2_body(mg, x, b);
+            const hipError_t e = hipStreamEndCapture(st, &g);
+            ctx->prof = prof;
+            if (e != hipSuccess || hipGraphInstantiate(&mg->graph_exec, g, nullptr, nullptr, 0) != hipSuccess) {
+                if (g) hipGraphDestroy(g);
+                return -1;
+            }
+            hipGraphDestroy(g);
+            mg->graph_x = x;
+            mg->graph_b = b;
+        }
+        ProfScope ps(ctx, "mg2_cycle");
+        if (hipGraphLaunch(mg->graph_exec, st) != hipSuccess) return -1;
+        return hipGetLastError() == hipSuccess ? 1 : -1;
+    }
+    {
+        ProfScope ps(ctx, "mg2_cycle");
+        fmg2_body(mg, x, b);
+    }
+    return hipGetLastError() == hipSuccess ? 1 : -1;
+}
+
+// The correction of fmg! after the convergence test: restrictions, the coarse
+// solve, the interpolate! + mgv! sweep up the levels, x += z and apply!(x).
+void fmg2_body(mcs_mg *mg, double *x, const double *b)
+{
+    hipStream_t st = mg->ctx->stream;
+    const int L = mg->cfg.levels;
     for (int k = 0; k < L; ++k) {
         const Lv2 Lk = lv2(mg, k);
         hipLaunchKernelGGL(k2_restrict, dim3(nblk((int64_t)(Lk.n[0] / 2) * (Lk.n[1] / 2))), dim3(MCS_BLOCK), 0, st,
@@ -626,7 +671,6 @@ int fmg2(mcs_mg *mg, double *x, const double *b, double *eps_out)
     hipLaunchKernelGGL(k2_axpy, dim3(nblk(mg->lv[0].len)), dim3(MCS_BLOCK), 0, st, x, (const double *)z,
                        mg->lv[0].len);
     bc2(mg, 0, x);
-    return hipGetLastError() == hipSuccess ? 1 : -1;
 }
 
 }  // namespace
@@ -650,7 +694,8 @@ int32_t mcs_mg2_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg **out)
     mcs_mg *mg = new mcs_mg();
     mg->ctx = ctx;
     mg->cfg = c;
-    mg->use_graph = false;
+    const char *ge = getenv("MCS_MG_GRAPH");
+    mg->use_graph = !(ge && ge[0] == '0');
     for (int l = 0; l <= c.levels; ++l) {
         MgLevel L;
         memset(&L, 0, sizeof(L));

@@ -73,3 +73,35 @@ def test_wrapper_2d_axisymmetric_solution():
     it, res = mg.solve(conf, x, b, ws)
     assert it >= 2
     assert np.abs(x[1:-1, 1:-1] - ue).max() < 2e-3
+
+
+def test_graph_replays_match_one_launch_per_phase(monkeypatch):
+    """The fmg correction replayed from a HIP graph (captured once per (u, b)),
+    three solves in a row with host work and synchronisation between them, each
+    bitwise equal to the same solve launched phase by phase."""
+    from mcstreamer_amd import _lib
+    from mcstreamer_amd import multigrid as mg
+    n, levels = (96, 128), 4
+    g = (n[0] + 2) * (n[1] + 2)
+    rng = np.random.default_rng(5)
+    bs = [rng.normal(size=g) * (1 + k) for k in range(3)]
+    out = {}
+    for graph in ("0", "1"):
+        monkeypatch.setenv("MCS_MG_GRAPH", graph)
+        conf = mg.MGConfig(bc=[(mg.BottomBnd, mg.Neumann), (mg.TopBnd, mg.Dirichlet), (mg.LeftBnd, mg.Dirichlet),
+                               (mg.RightBnd, mg.Dirichlet)], s=0.7, levels=levels, tolerance=-1e-10, maxiter=50,
+                           ndim=2, conn=mg.CylindricalConnector(1))
+        ws = mg.Workspace(conf, n)
+        res = []
+        for b in bs:
+            u = np.zeros(g)
+            it = ctypes.c_int32()
+            rr = ctypes.c_double()
+            assert _lib.lib().mcs_mg_solve_host(ws.handle, _lib.dptr(u), _lib.dptr(b), ctypes.byref(it),
+                                                ctypes.byref(rr)) == 0
+            res.append((it.value, u.copy()))
+            ws.ctx.synchronize()
+        out[graph] = res
+    for (i0, u0), (i1, u1) in zip(out["0"], out["1"]):
+        assert i0 == i1 and i0 >= 2
+        np.testing.assert_array_equal(u0, u1)
