@@ -244,7 +244,9 @@ __global__ void k2_axpy(double *x, const double *z, int64_t n)
 
 // ---- the coarse levels in one workgroup ---------------------------------------
 
-#define MG2_TAIL_THREADS 512
+#ifndef MG2_TAIL_THREADS
+#define MG2_TAIL_THREADS 1024
+#endif
 #define MG2_MAXL 16
 
 struct Tail2 {
