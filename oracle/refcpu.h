@@ -8,14 +8,24 @@
  * each function in refcpu.c), with two stated substitutions that the GPU uses
  * too: random numbers come from the per-particle Philox streams of
  * mcstreamer_amd/csrc/mcs_rng.h (the reference calls the global rand()), and
- * the 3D Cartesian mesh generalises the reference's 2D cylindrical grid.
+ * the 3D Cartesian mesh generalises the reference's 2D cylindrical grid (which
+ * the ref_cgrid / ref_cyl_* functions restate as it is).
  *
  * Pins (tests/): Philox known-answer vectors; elementary functions against
  * libm; table builder against the closed-form cross sections; Yoshida push
  * against the exact parabola; kinematics invariants; null-collision statistics;
  * swarm drift against the analytic constant-frequency model; repack against
  * the rank-matching formulation; multigrid against a dense solve; resampling
- * weight conservation and systematic-sampling bounds.  See each test's header.
+ * weight conservation and systematic-sampling bounds.  Axisymmetric grid
+ * (test_oracle_axisym.py): ref_fld against exact rational floors; the
+ * LinRange geometry and dV against closed forms; the cylindrical field against
+ * linear fields it must reproduce; density against brute-force exact cells;
+ * faces against a linear potential; the charge's FP32 rounding; the push in
+ * uniform axial and radial fields against the parabola; the free boundary's
+ * transforms against scipy's DST-II/III and its solution against a point
+ * charge between grounded planes; the (r, z) resample against its per-cell
+ * rules.  The free boundary's Bessel factors are an input (the product's f_n,
+ * itself checked against scipy).  See each test's header.
  */
 #ifndef REFCPU_H
 #define REFCPU_H
