@@ -84,6 +84,8 @@ class Run:
     def __init__(self, finput, ctx=None, energy_points=100_000):
         inp = load_input(finput)
         self.input = inp
+        self.finput = os.path.abspath(finput)
+        self.energy_points = int(energy_points)
         self.ctx = ctx or context()
         self.outpath = os.path.splitext(os.path.abspath(finput))[0]
         seed = int(inp.get("rng_seed", 0))
@@ -166,3 +168,40 @@ def main(finput, ctx=None):
     run = Run(finput, ctx=ctx)
     run.nsteps(int(run.tmax // run.dt))
     return run
+
+
+CHECKPOINT_FORMAT = "mcstreamer_amd axisymmetric checkpoint 1"
+
+
+def save(run, path):
+    """The state of `run` after its last step to one .npz (no pickles): the
+    particles in their order with ids and the next id, qfixed, the potential the
+    next solve starts from, the face fields of the last solve, the step and the
+    TimeSteppers' counters.  The input
+    file is re-read on load (its path is stored)."""
+    ps = run.popl.download()
+    meta = {"format": CHECKPOINT_FORMAT, "finput": run.finput, "energy_points": run.energy_points, "i": run.i,
+            "ts": [run.poisson_ts.i, run.output_ts.i, run.resample_ts.i], "next_id": int(ps.next_id)}
+    np.savez(path, meta=np.array(json.dumps(meta)), x=ps.x, v=ps.v, w=ps.w, s=ps.s, active=ps.active, id=ps.id,
+             qfixed=run.fields.get("qfixed"), u=run.fields.get("u"), er=run.fields.get("er"),
+             ez=run.fields.get("ez"))
+
+
+def load(path, ctx=None):
+    """A Run continuing the one saved in `path` (the Philox streams are keyed by
+    particle id and step, so the continuation is the uninterrupted run's)."""
+    with np.load(path, allow_pickle=False) as z:
+        meta = json.loads(str(z["meta"]))
+        if meta.get("format") != CHECKPOINT_FORMAT:
+            raise ValueError(f"{path}: not a {CHECKPOINT_FORMAT} file")
+        arrs = {k: z[k] for k in ("x", "v", "w", "s", "active", "id", "qfixed", "u", "er", "ez")}
+    run = Run(meta["finput"], ctx=ctx, energy_points=int(meta["energy_points"]))
+    run.popl.upload(ParticleStates(arrs["x"], arrs["v"], arrs["w"], arrs["s"], arrs["active"], arrs["id"],
+                                   int(meta["next_id"])))
+    run.fields.set("qfixed", arrs["qfixed"])
+    for k in ("u", "er", "ez"):
+        run.fields.set(k, arrs[k])
+    run.i = int(meta["i"])
+    run.poisson_ts.i, run.output_ts.i, run.resample_ts.i = (int(v) for v in meta["ts"])
+    return run
+

@@ -66,3 +66,25 @@ def test_driver_runs_an_axisymmetric_input(tmp_path):
     for r in run.log:
         np.testing.assert_allclose(r["weight_resampled"], r["weight"], rtol=1e-12)
     assert run.log[0]["resample"][1] < run.log[0]["resample"][0]   # the seed's dense cells merged
+
+
+def test_driver_checkpoint_continues_bitwise(tmp_path):
+    _gas(tmp_path / "gas.json")
+    p = tmp_path / "input.toml"
+    p.write_text(TOML)
+    a = driver.setup(str(p), energy_points=4000)
+    a.nsteps(2)
+    ck = str(tmp_path / "ck.npz")
+    driver.save(a, ck)
+    a.nsteps(3)
+    b = driver.load(ck)
+    assert b.i == 2
+    b.nsteps(3)
+    ga, gb = a.popl.download(), b.popl.download()
+    assert len(ga) == len(gb) and ga.next_id == gb.next_id
+    np.testing.assert_array_equal(ga.id, gb.id)
+    for k in ("x", "v", "w", "s", "active"):
+        np.testing.assert_array_equal(getattr(ga, k), getattr(gb, k), err_msg=k)
+    for k in ("u", "qfixed", "er", "ez"):
+        np.testing.assert_array_equal(a.fields.get(k), b.fields.get(k), err_msg=k)
+
