@@ -3,6 +3,8 @@ density! (bitwise: per cell, terms in particle order), poisson!'s charge, solve
 and face fields (bitwise), the particle step in the axisymmetric FieldInterp
 with the ion tracker (particles bitwise; the tracker's same-cell additions are
 atomics, so qfixed within 1e-12), and a few Poisson + advance + repack steps."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -247,3 +249,29 @@ def test_axisymmetric_streamer_steps():
     for k in ("x", "v", "w", "s"):
         np.testing.assert_allclose(getattr(g, k), r[k], rtol=1e-12, atol=0, err_msg=k)
     np.testing.assert_allclose(fields.get("u").ravel(), G.u, rtol=1e-10, atol=1e-12 * np.abs(G.u).max())
+
+
+def test_argument_errors():
+    """Grid sizes, a solver that does not match the grid, the free boundary
+    without Dirichlet at r = R, and resample! before the canonical sort."""
+    from mcstreamer_amd import _lib
+    from mcstreamer_amd import multigrid as mg
+    with pytest.raises(ValueError):
+        axisym.Grid(RR, LL, 1, NN)
+    fields = axisym.GridFields(axisym.Grid(RR, LL, MM, NN))
+    it = ctypes.c_int32()
+    res = ctypes.c_double()
+    wrong_n = axisym.poisson_config(axisym.Grid(RR, LL, 2 * MM, NN))
+    ws = mg.Workspace(wrong_n, (2 * MM, NN))
+    assert _lib.lib().mcs_cyl_poisson(fields.handle, ws.handle, 0.0, 1, ctypes.byref(it),
+                                      ctypes.byref(res)) == _lib.MCS_E_ARG
+    neumann_wall = mg.MGConfig(bc=[(mg.LeftBnd, mg.Dirichlet), (mg.RightBnd, mg.Dirichlet),
+                                   (mg.TopBnd, mg.Neumann), (mg.BottomBnd, mg.Neumann)],
+                               s=1.0, levels=3, tolerance=1e-3, ndim=2, conn=mg.CylindricalConnector(1))
+    ws2 = mg.Workspace(neumann_wall, (MM, NN))
+    assert _lib.lib().mcs_cyl_poisson(fields.handle, ws2.handle, 0.0, 1, ctypes.byref(it),
+                                      ctypes.byref(res)) == _lib.MCS_E_ARG
+    assert b"Dirichlet" in _lib.lib().mcs_last_error(fields.ctx.handle)
+    table, T, popl, P, grid, f2, G = _setup(n=1000, seed=2)
+    st = _lib.ResampleStats()
+    assert _lib.lib().mcs_cyl_resample(popl.handle, f2.handle, 10, 15, 0, 0, 0, ctypes.byref(st)) == _lib.MCS_E_ARG
