@@ -368,8 +368,8 @@ int32_t mcs_cgrid_freebc_coeffs(const mcs_cgrid *g, double *f);
 int32_t mcs_cyl_efield_faces(mcs_cgrid *g, double eb);
 /* Canonical cell order on the axisymmetric grid: inactive particles dropped, the
  * rest sorted by (cell, id) over resample!'s cells (cellindex within the ghosted
- * per-cell arrays, i in 1..M+1, j in 0..N+1), particles outside last.  Needs
- * (M+2)(N+2) < 2^24 and ids below 2^40. */
+ * per-cell arrays, i in 1..M+1, j in 0..N+1), particles outside last.  The bits
+ * of (M+2)(N+2) + 1 and of the next id must fit 64 together. */
 int32_t mcs_cyl_sort_by_cell(mcs_pop *p, mcs_cgrid *g);
 /* MCStreamer.resample! (resample.jl:7-84) per cell of the axisymmetric grid, on the
  * canonical order of mcs_cyl_sort_by_cell (called right before; the rules and

@@ -1298,18 +1298,19 @@ __device__ __forceinline__ int64_t cyl_rs_cell(const CylView &C, const double x[
     return ((int64_t)fi + 1) + (int64_t)(C.M + 2) * ((int64_t)fj + 1);
 }
 
-// (cell, id) keys of the canonical order: cell << 40 | id, the cells outside as
-// cell ncg, inactive particles as ~0 (dropped: they sort after every kept one).
-__global__ __launch_bounds__(MCS_BLOCK) void kr_keys(PopBuf b, int64_t n, CylView C, uint64_t ncg, uint64_t *key,
-                                                     int32_t *val)
+// (cell, id) keys of the canonical order: cell << idbits | id (ids are below
+// 2^idbits), the cells outside as cell ncg, inactive particles as cell ncg + 1
+// (dropped: they sort after every kept one).
+__global__ __launch_bounds__(MCS_BLOCK) void kr_keys(PopBuf b, int64_t n, CylView C, uint64_t ncg, int idbits,
+                                                     uint64_t *key, int32_t *val)
 {
     const int64_t p = gidx();
     if (p >= n) return;
-    uint64_t k = ~0ULL;
+    uint64_t k = (ncg + 1) << idbits;
     if (b.active[p]) {
         const double x[3] = {b.x[0][p], b.x[1][p], b.x[2][p]};
         const int64_t c = cyl_rs_cell(C, x);
-        k = ((c < 0 ? ncg : (uint64_t)c) << 40) | b.id[p];
+        k = ((c < 0 ? ncg : (uint64_t)c) << idbits) | b.id[p];
     }
     key[p] = k;
     val[p] = (int32_t)p;
@@ -1317,12 +1318,12 @@ __global__ __launch_bounds__(MCS_BLOCK) void kr_keys(PopBuf b, int64_t n, CylVie
 
 // cs[c] = the first sorted key of cell c or later (c = 0..ncg + 1); the last is
 // the number of kept particles, also stored as an 8-byte word at *na.
-__global__ __launch_bounds__(MCS_BLOCK) void kr_starts(const uint64_t *key, int64_t n, int64_t ncg, int32_t *cs,
-                                                       int64_t *na)
+__global__ __launch_bounds__(MCS_BLOCK) void kr_starts(const uint64_t *key, int64_t n, int64_t ncg, int idbits,
+                                                       int32_t *cs, int64_t *na)
 {
     const int64_t c = gidx();
     if (c > ncg + 1) return;
-    const uint64_t t = (uint64_t)c << 40;
+    const uint64_t t = (uint64_t)c << idbits;
     int64_t lo = 0, hi = n;
     while (lo < hi) {
         const int64_t mid = (lo + hi) >> 1;
@@ -1352,8 +1353,13 @@ extern "C" int32_t mcs_cyl_sort_by_cell(mcs_pop *p, mcs_cgrid *g)
     if (g->ctx != ctx) return mcs_fail(ctx, MCS_E_ARG, "cyl_sort: grid of another context");
     if (p->prec != MCS_PREC_F64) return mcs_fail(ctx, MCS_E_ARG, "cyl_sort: FP64 populations only");
     const int64_t ncg = (int64_t)(g->M + 2) * (g->N + 2);
-    if (ncg + 1 >= ((int64_t)1 << 24)) return mcs_fail(ctx, MCS_E_ARG, "cyl_sort: more than 2^24 cells");
-    if (p->next_id >= ((uint64_t)1 << 40)) return mcs_fail(ctx, MCS_E_ARG, "cyl_sort: ids beyond 2^40");
+    // the key holds the id in its low idbits bits and the cell above; the sort
+    // looks at those bits only
+    int idbits = 1, cbits = 1;
+    while (idbits < 63 && (p->next_id >> idbits) != 0) ++idbits;
+    while (cbits < 63 && ((uint64_t)(ncg + 1) >> cbits) != 0) ++cbits;
+    if (idbits + cbits > 64) return mcs_fail(ctx, MCS_E_ARG, "cyl_sort: cell and id do not fit 64 bits");
+    const unsigned endbit = (unsigned)(idbits + cbits);
     hipSetDevice(ctx->device);
     int32_t rc = mcs_pop_ensure_alt(p);
     if (rc) return rc;
@@ -1366,7 +1372,7 @@ extern "C" int32_t mcs_cyl_sort_by_cell(mcs_pop *p, mcs_cgrid *g)
         uint64_t *kin = (uint64_t *)p->u64a;
         int32_t *vin = p->i32a, *vout = p->i32b;
         size_t tb = 0;
-        MCS_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tb, kin, kin, vin, vout, (size_t)n, 0u, 64u, ctx->stream));
+        MCS_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tb, kin, kin, vin, vout, (size_t)n, 0u, endbit, ctx->stream));
         const size_t kbytes = ((sizeof(uint64_t) * (size_t)n + 255) / 256) * 256;
         void *tmp = nullptr;
         rc = mcs_scratch(ctx, kbytes + tb, &tmp);
@@ -1374,11 +1380,11 @@ extern "C" int32_t mcs_cyl_sort_by_cell(mcs_pop *p, mcs_cgrid *g)
         uint64_t *kout = (uint64_t *)tmp;
         const MeshView mv = cyl_view(g);
         hipLaunchKernelGGL(kr_keys, dim3((unsigned)cdiv(n, MCS_BLOCK)), dim3(MCS_BLOCK), 0, ctx->stream,
-                           p->buf[0], n, mv.C, (uint64_t)ncg, kin, vin);
-        MCS_HIP(ctx, rocprim::radix_sort_pairs((char *)tmp + kbytes, tb, kin, kout, vin, vout, (size_t)n, 0u, 64u,
+                           p->buf[0], n, mv.C, (uint64_t)ncg, idbits, kin, vin);
+        MCS_HIP(ctx, rocprim::radix_sort_pairs((char *)tmp + kbytes, tb, kin, kout, vin, vout, (size_t)n, 0u, endbit,
                                                ctx->stream));
         hipLaunchKernelGGL(kr_starts, dim3((unsigned)cdiv(ncg + 2, MCS_BLOCK)), dim3(MCS_BLOCK), 0, ctx->stream,
-                           (const uint64_t *)kout, n, ncg, g->cs, g->cnt64 + ncg + 1);
+                           (const uint64_t *)kout, n, ncg, idbits, g->cs, g->cnt64 + ncg + 1);
         MCS_LAUNCH_CHECK(ctx);
         rc = mcs_readback(ctx, g->cnt64 + ncg + 1, &na, 1);
         if (rc) return rc;
