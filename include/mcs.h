@@ -25,6 +25,8 @@ extern "C" {
  * MCS_E_FIELD        : MCStreamer/src/gridfields.jl:178-187 "Too high electric field" (|E| > 1e8 V/m)
  * MCS_E_TABLE_RANGE  : JuMC/src/util.jl:27 (energy beyond the last table node; BoundsError)
  * MCS_E_NONCONVERGED : Multigrid/src/Multigrid.jl:458-464 (logged there, returned here)
+ * MCS_E_NAN          : JuMC/src/util.jl:27 with a NaN energy (floor(Int, NaN) throws an
+ *                      InexactError): a particle whose velocity holds a NaN collides
  */
 #define MCS_OK 0
 #define MCS_E_ARG 1
@@ -93,7 +95,8 @@ typedef struct mcs_step_stats {
     int64_t deaths;
     int64_t flights;                /* free flights (collision or end of step) */
     int64_t events[MCS_MAX_PROC];   /* collisions selected per table row (null included) */
-    int32_t flags;                  /* bit 0 table range, bit 1 field limit, bit 2 capacity */
+    int32_t flags;                  /* bit 0 table range, bit 1 field limit, bit 2 capacity,
+                                       bit 3 NaN energy */
 } mcs_step_stats;
 
 typedef struct mcs_moments {
@@ -284,7 +287,7 @@ typedef struct mcs_swarm_tally {
     int64_t nactive;      /* actives after the last step's advance */
     int64_t nrecorded;    /* steps entering the time averages */
     int64_t nblocks;      /* complete blocks */
-    int32_t flags;        /* bit 0 table range, bit 1 field, bit 2 capacity */
+    int32_t flags;        /* bit 0 table range, bit 1 field, bit 2 capacity, bit 3 NaN energy */
     int32_t pad;
     double sum[MCS_TALLY_NQ];    /* sum over recorded steps of the per-step means */
     double bm_s1[MCS_TALLY_NQ];  /* sum of block means */

@@ -1,5 +1,6 @@
 // mcs_api.hip -- context, collision table, particle population and the
 // population reductions of libmcs.
+#include <atomic>
 #include <cstdarg>
 #include <cstring>
 
@@ -583,6 +584,12 @@ int32_t mcs_pop_ensure_alt(mcs_pop *p)
     return MCS_OK;
 }
 
+void mcs_pop_touch(mcs_pop *p)
+{
+    static std::atomic<uint64_t> serial{0};
+    p->gen = serial.fetch_add(1, std::memory_order_relaxed) + 1;
+}
+
 void mcs_pop_swap(mcs_pop *p)
 {
     PopBuf t = p->buf[0];
@@ -643,6 +650,7 @@ extern "C" int32_t mcs_pop_upload(mcs_pop *p, int64_t n, const double *x, const 
                         (long long)n, (long long)p->cap);
     if (n > 0 && (!x || !v || !w || !s || !active))
         return mcs_fail(ctx, MCS_E_ARG, "pop_upload: null array");
+    mcs_pop_touch(p);
     hipSetDevice(ctx->device);
     MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (p->prec == MCS_PREC_F32) {

@@ -186,6 +186,12 @@ __device__ __forceinline__ void indweight_dev(const double *energy, int32_t ne, 
                                               double step, double inv_dlog, double eng,
                                               int64_t &k, double &w, int32_t &flags)
 {
+    if (!(eng == eng)) {   // a NaN velocity: the reference's floor(Int, NaN) throws
+        k = 0;
+        w = 0.0;
+        flags |= MCS_FLAG_NAN;
+        return;
+    }
     if (eng >= gld(energy + (ne - 1))) {
         k = ne - 2;
         w = 0.0;

@@ -94,6 +94,12 @@ __device__ __forceinline__ void indweight32(const TableView32 &T, float eng, int
                                             int32_t &flags)
 {
     const float *E = T.energy;
+    if (!(eng == eng)) {
+        k = 0;
+        w = 0.0f;
+        flags |= MCS_FLAG_NAN;
+        return;
+    }
     if (eng >= T.emax) {
         k = T.ne - 2;
         w = 0.0f;

@@ -22,6 +22,7 @@
 #define MCS_FLAG_TABLE_RANGE 1
 #define MCS_FLAG_FIELD 2
 #define MCS_FLAG_CAPACITY 4
+#define MCS_FLAG_NAN 8
 
 struct ProfEntry {
     double total_ms = 0.0;
@@ -154,6 +155,11 @@ struct mcs_pop {
     int64_t cap;
     int64_t n;
     uint64_t next_id;
+    // Serial of the population's current order and count: every call that moves,
+    // adds, removes or reorders particles takes a fresh one (mcs_pop_touch), so a
+    // mesh or grid that recorded the serial at its sort knows whether its cell
+    // starts still describe this population.
+    uint64_t gen = 0;
     PopBuf buf[2];
     bool has_alt = false;
     // precision mode: the split store on a mesh grid (gn cells of gh metres)
@@ -193,7 +199,7 @@ struct mcs_mesh {
     int32_t *cell_cnt = nullptr;    // ncells + 2 (counts, then the long-segment list)
     int64_t *cnt64 = nullptr;       // ncells + 2 (per-cell resample output counts)
     int64_t *pre64 = nullptr;       // ncells + 2 (their prefix)
-    bool sorted_valid = false;
+    uint64_t sorted_gen = 0;        // mcs_pop::gen of the population the cell starts describe
     float *face32[3] = {nullptr, nullptr, nullptr};  // FP32 face fields (precision mode)
     int32_t *dep_list = nullptr;    // ncells + 2: count, the cells the CIC gather gives a wavefront, a counter
     int32_t *dep_host = nullptr;    // host memory the device writes the count of such cells to
@@ -219,7 +225,7 @@ struct mcs_cgrid {
     // canonical (cell, id) order over resample!'s cells, (M+2)(N+2) + 2 entries each
     int32_t *cs = nullptr, *cs_new = nullptr;
     int64_t *cnt64 = nullptr, *pre64 = nullptr;
-    bool sorted_valid = false;
+    uint64_t sorted_gen = 0;   // mcs_pop::gen of the population the cell starts describe
 };
 
 // The two-dimensional solver (mcs_mg2.hip): creation for cfg->ndim == 2, and the
@@ -298,4 +304,8 @@ int32_t mcs_scan_i64_buf(mcs_ctx *ctx, const int64_t *in, int64_t *out, int64_t 
 #define MCS_SCAN_TILE 2048
 
 int32_t mcs_pop_ensure_alt(mcs_pop *p);
+// Gives p a fresh order serial (unique over all populations of the process).
+void mcs_pop_touch(mcs_pop *p);
+// Whether cell starts recorded at serial `gen` describe p's current order.
+static inline bool mcs_pop_sorted(const mcs_pop *p, uint64_t gen) { return gen != 0 && gen == p->gen; }
 void mcs_pop_swap(mcs_pop *p);

@@ -484,7 +484,8 @@ static int32_t sort_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, const Buf &dst
     MCS_LAUNCH_CHECK(ctx);
     mcs_pop_swap(p);
     p->n = na;
-    m->sorted_valid = true;
+    mcs_pop_touch(p);
+    m->sorted_gen = p->gen;
     return MCS_OK;
 }
 
@@ -827,8 +828,9 @@ extern "C" int32_t mcs_deposit(mcs_pop *p, mcs_mesh *m, int32_t which, int32_t s
         return mcs_fail(ctx, MCS_E_ARG, "deposit: target must be QFIXED or QPART");
     if (scheme != MCS_DEPOSIT_NGP && scheme != MCS_DEPOSIT_CIC)
         return mcs_fail(ctx, MCS_E_ARG, "deposit: unknown scheme %d", scheme);
-    if (!m->sorted_valid)
-        return mcs_fail(ctx, MCS_E_ARG, "deposit: population is not in cell order (call mcs_sort_by_cell)");
+    if (!mcs_pop_sorted(p, m->sorted_gen))
+        return mcs_fail(ctx, MCS_E_ARG,
+                        "deposit: the population is not in this mesh's cell order (call mcs_sort_by_cell)");
     hipSetDevice(ctx->device);
     if (p->prec == MCS_PREC_F32) return deposit_impl(p, m, p->sbuf[0], which, scheme, val);
     return deposit_impl(p, m, p->buf[0], which, scheme, val);
@@ -1179,14 +1181,19 @@ extern "C" int32_t mcs_resample_cells(mcs_pop *p, mcs_mesh *m, int32_t ntarget, 
     mcs_ctx *ctx = p->ctx;
     if (ntarget < 1 || nmax < ntarget)
         return mcs_fail(ctx, MCS_E_ARG, "resample: need 1 <= ntarget <= nmax");
-    if (!m->sorted_valid)
-        return mcs_fail(ctx, MCS_E_ARG, "resample: population is not in cell order (call mcs_sort_by_cell)");
+    if (!mcs_pop_sorted(p, m->sorted_gen))
+        return mcs_fail(ctx, MCS_E_ARG,
+                        "resample: the population is not in this mesh's cell order (call mcs_sort_by_cell)");
     hipSetDevice(ctx->device);
     int32_t rc = mcs_pop_ensure_alt(p);
     if (rc) return rc;
-    if (p->prec == MCS_PREC_F32)
-        return resample_impl(p, m, p->sbuf[0], p->sbuf[1], ntarget, nmax, seed, instance, step, stats);
-    return resample_impl(p, m, p->buf[0], p->buf[1], ntarget, nmax, seed, instance, step, stats);
+    mcs_pop_touch(p);
+    rc = p->prec == MCS_PREC_F32
+             ? resample_impl(p, m, p->sbuf[0], p->sbuf[1], ntarget, nmax, seed, instance, step, stats)
+             : resample_impl(p, m, p->buf[0], p->buf[1], ntarget, nmax, seed, instance, step, stats);
+    // the output is in canonical order and the cell starts were rewritten for it
+    if (rc == MCS_OK) m->sorted_gen = p->gen;
+    return rc;
 }
 
 // The cell lists a resample works on: starts (ncells + 2), per-cell packed counts
@@ -1399,7 +1406,8 @@ extern "C" int32_t mcs_cyl_sort_by_cell(mcs_pop *p, mcs_cgrid *g)
         MCS_LAUNCH_CHECK(ctx);
     }
     p->n = na;
-    g->sorted_valid = true;
+    mcs_pop_touch(p);
+    g->sorted_gen = p->gen;
     return MCS_OK;
 }
 
@@ -1423,17 +1431,21 @@ extern "C" int32_t mcs_cyl_resample(mcs_pop *p, mcs_cgrid *g, int32_t ntarget, i
     if (!p || !g) return MCS_E_ARG;
     mcs_ctx *ctx = p->ctx;
     if (ntarget < 1 || nmax < ntarget) return mcs_fail(ctx, MCS_E_ARG, "resample: need 1 <= ntarget <= nmax");
-    if (!g->sorted_valid)
-        return mcs_fail(ctx, MCS_E_ARG, "resample: population is not in cell order (call mcs_cyl_sort_by_cell)");
+    if (!mcs_pop_sorted(p, g->sorted_gen))
+        return mcs_fail(ctx, MCS_E_ARG,
+                        "resample: the population is not in this grid's cell order (call mcs_cyl_sort_by_cell)");
     hipSetDevice(ctx->device);
     int32_t rc = mcs_pop_ensure_alt(p);
     if (rc) return rc;
     const int64_t ncg = (int64_t)(g->M + 2) * (g->N + 2);
     const CellLists L = {g->cs, ncg, g->cnt64, g->pre64, g->cs_new};
     const CylView C = cyl_view(g).C;
-    return resample_run(p, L, p->buf[0], p->buf[1], ntarget, nmax, seed, instance, step, stats,
-                        [&](const ResArgs<PopBuf> &r, int64_t n) {
-                            hipLaunchKernelGGL(k_resample_copy_cyl, dim3((unsigned)cdiv(n, MCS_BLOCK)),
-                                               dim3(MCS_BLOCK), 0, ctx->stream, r, C, n);
-                        });
+    mcs_pop_touch(p);
+    rc = resample_run(p, L, p->buf[0], p->buf[1], ntarget, nmax, seed, instance, step, stats,
+                      [&](const ResArgs<PopBuf> &r, int64_t n) {
+                          hipLaunchKernelGGL(k_resample_copy_cyl, dim3((unsigned)cdiv(n, MCS_BLOCK)),
+                                             dim3(MCS_BLOCK), 0, ctx->stream, r, C, n);
+                      });
+    if (rc == MCS_OK) g->sorted_gen = p->gen;
+    return rc;
 }

@@ -402,6 +402,7 @@ static int32_t advance_finish(mcs_pop *p, const mcs_table *t, int64_t n0, int64_
     }
     if (stats) stats->n_out = p->n;
     if (flags & MCS_FLAG_FIELD) return mcs_fail(ctx, MCS_E_FIELD, "Too high electric field");
+    if (flags & MCS_FLAG_NAN) return mcs_fail(ctx, MCS_E_NAN, "NaN particle energy (a NaN velocity)");
     if (flags & MCS_FLAG_TABLE_RANGE)
         return mcs_fail(ctx, MCS_E_TABLE_RANGE, "particle energy beyond the collision table");
     return MCS_OK;
@@ -418,6 +419,7 @@ extern "C" int32_t mcs_advance(mcs_pop *p, const mcs_table *t, const mcs_field *
     if (!(dt >= 0.0)) return mcs_fail(ctx, MCS_E_ARG, "advance: dt must be >= 0");
     if (field->kind == MCS_FIELD_MESH && !field->mesh)
         return mcs_fail(ctx, MCS_E_ARG, "advance: mesh field without a mesh");
+    mcs_pop_touch(p);
     if (stats) memset(stats, 0, sizeof(*stats));
     const int64_t n0 = p->n;
     if (stats) {
@@ -484,6 +486,7 @@ extern "C" int32_t mcs_advance_cyl(mcs_pop *p, const mcs_table *t, mcs_cgrid *g,
     if (!(dt >= 0.0)) return mcs_fail(ctx, MCS_E_ARG, "advance: dt must be >= 0");
     if (p->prec != MCS_PREC_F64) return mcs_fail(ctx, MCS_E_ARG, "advance_cyl: FP64 populations only");
     if (g->ctx != ctx) return mcs_fail(ctx, MCS_E_ARG, "advance_cyl: grid of another context");
+    mcs_pop_touch(p);
     if (stats) memset(stats, 0, sizeof(*stats));
     const int64_t n0 = p->n;
     if (stats) {
@@ -581,6 +584,7 @@ extern "C" int32_t mcs_repack(mcs_pop *p)
     mcs_ctx *ctx = p->ctx;
     if (p->prec == MCS_PREC_F32)
         return mcs_fail(ctx, MCS_E_ARG, "repack: not available on a split-precision population");
+    mcs_pop_touch(p);
     hipSetDevice(ctx->device);
     int64_t n = p->n;
     if (n == 0) return MCS_OK;
@@ -671,6 +675,7 @@ extern "C" int32_t mcs_swarm_resample(mcs_pop *p, int64_t n0, double tol, uint64
     mcs_ctx *ctx = p->ctx;
     if (p->prec == MCS_PREC_F32)
         return mcs_fail(ctx, MCS_E_ARG, "swarm_resample: not available on a split-precision population");
+    mcs_pop_touch(p);
     hipSetDevice(ctx->device);
     if (did) *did = 0;
     int64_t n = p->n;

@@ -1350,6 +1350,7 @@ extern "C" int32_t mcs_pop_init_thermal(mcs_pop *p, int64_t n, uint64_t seed, ui
     hipSetDevice(ctx->device);
     if (n < 0 || n > p->cap || !(vth >= 0.0))
         return mcs_fail(ctx, MCS_E_ARG, "pop_init_thermal: n must be in [0, capacity], vth >= 0");
+    mcs_pop_touch(p);
     if (n > 0) {
         hipLaunchKernelGGL(k_sw_init, dim3((unsigned)cdiv(n, MCS_BLOCK)), dim3(MCS_BLOCK), 0, ctx->stream,
                            p->buf[0], (int64_t)0, n, seed, instance, vth);

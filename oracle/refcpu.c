@@ -18,6 +18,7 @@
 #define FLAG_TABLE_RANGE 1
 #define FLAG_FIELD 2
 #define FLAG_CAPACITY 4
+#define FLAG_NAN 8
 
 /* ---------------------------------------------------------------- helpers */
 
@@ -75,7 +76,8 @@ static inline double energy3(const double *v)
  * weight of that node.  Uniform grid: k = floor((eng - e0)/step),
  * w = (e[k+1] - eng)/step.  Log grid: k located exactly by node comparisons,
  * w = (e[k+1] - eng)/(e[k+1] - e[k]).  Energies past the last node flag a range
- * error (the reference throws a BoundsError) and use the last interval. */
+ * error (the reference throws a BoundsError) and use the last interval; a NaN
+ * energy flags NaN (floor(Int, NaN) throws an InexactError there). */
 int32_t ref_indweight(const ref_table *t, double eng, int64_t *kout, double *wout)
 {
     const double *e = t->energy;
@@ -83,6 +85,11 @@ int32_t ref_indweight(const ref_table *t, double eng, int64_t *kout, double *wou
     int32_t flag = 0;
     int64_t k;
     double w;
+    if (!(eng == eng)) {
+        *kout = 0;
+        *wout = 0.0;
+        return FLAG_NAN;
+    }
     if (eng >= e[ne - 1]) {
         *kout = ne - 2;
         *wout = 0.0;

@@ -25,7 +25,15 @@
  * transforms against scipy's DST-II/III and its solution against a point
  * charge between grounded planes; the (r, z) resample against its per-cell
  * rules.  The free boundary's Bessel factors are an input (the product's f_n,
- * itself checked against scipy).  See each test's header.
+ * itself checked against scipy).  Mesh operators (test_oracle_mesh_pins.py):
+ * the 3D and (r, z) canonical sorts against numpy's lexsort of (cell, id) over
+ * independently computed cells, and their cell starts against a cumulative
+ * bincount; the CIC deposit against the tent assignment function, charge and
+ * first-moment conservation, its special points and dropped outside shares;
+ * the NGP deposit against cell binning; the reciprocal in normdivrem against
+ * the true quotient (it moves cell indices only where the interpolant is
+ * continuous).  Each of these checks fails on a transposed cell key or a
+ * dropped half-cell stagger.  See each test's header.
  */
 #ifndef REFCPU_H
 #define REFCPU_H
