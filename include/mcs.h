@@ -185,6 +185,14 @@ int32_t mcs_pop_size(mcs_pop *p, int64_t *n);
 /* weight/actives/meanenergy/maxenergy/spread (population.jl:74-183) and the swarm
  * tallies of Swarm.swarm (Swarm.jl:126-156) in one device reduction. */
 int32_t mcs_pop_moments(mcs_pop *p, const mcs_table *t, mcs_moments *out);
+/* The same reductions, bitwise equal to mcs_pop_moments, with no device or host
+ * allocation: the partial results live in a fixed 176.5 KiB device array of the
+ * library (one per GPU, held until the process ends), enough for the reduction's
+ * largest grid, so every population size fits.  No counterpart in the reference:
+ * for callers that cannot allocate (tight serving loops, code between HIP graph
+ * captures, harnesses that forbid hipMalloc).  Calls on one GPU share the array:
+ * calls that overlap on one GPU (several contexts or threads) race. */
+int32_t mcs_pop_moments_noalloc(mcs_pop *p, const mcs_table *t, mcs_moments *out);
 
 /* ---- the fused step: JuMC.advance! (JuMC/src/mixed_population.jl:24-54) ---
  * Yoshida push (electron.jl:51-62), null-collision sampling (collisions.jl:159-192),

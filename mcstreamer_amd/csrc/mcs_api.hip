@@ -3,6 +3,8 @@
 #include <atomic>
 #include <cstdarg>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 #include "mcs_device.h"
 #include "mcs_fp32.h"
@@ -29,6 +31,8 @@ extern "C" const char *mcs_last_error(const mcs_ctx *ctx)
 
 // ---------------------------------------------------------------- context
 
+static int32_t noalloc_setup(int device);
+
 extern "C" int32_t mcs_ctx_create(int32_t device, mcs_ctx **out)
 {
     if (!out) return MCS_E_ARG;
@@ -37,6 +41,7 @@ extern "C" int32_t mcs_ctx_create(int32_t device, mcs_ctx **out)
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return MCS_E_HIP;
     if (device < 0 || device >= ndev) return MCS_E_ARG;
     if (hipSetDevice(device) != hipSuccess) return MCS_E_HIP;
+    if (noalloc_setup(device) != MCS_OK) return MCS_E_HIP;
     mcs_ctx *c = new mcs_ctx();
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -965,20 +970,21 @@ __global__ void k_collrate_final(const double *partial, int nb, double *out)
     out[q] = a.s + a.c;
 }
 
-extern "C" int32_t mcs_pop_moments(mcs_pop *p, const mcs_table *t, mcs_moments *out)
+// The reductions of mcs_pop_moments on a caller-chosen scratch of at least
+// moments_scratch_doubles(nb) doubles: per-block partial records, then the
+// final values.
+static size_t moments_scratch_doubles(int nb) { return (size_t)nb * (MOM_REC + 2 * CR_CHUNK) + 64; }
+
+static int moments_blocks(int64_t n)
 {
-    if (!p || !out) return MCS_E_ARG;
+    int64_t nb = cdiv(n, MCS_BLOCK);
+    return (int)(nb > MOM_BLOCKS ? MOM_BLOCKS : nb);
+}
+
+static int32_t moments_run(mcs_pop *p, const mcs_table *t, mcs_moments *out, int nb, double *scr)
+{
     mcs_ctx *ctx = p->ctx;
-    hipSetDevice(ctx->device);
-    memset(out, 0, sizeof(*out));
-    out->nparticles = p->n;
-    if (p->n == 0) return MCS_OK;
-    int nb = (int)cdiv(p->n, MCS_BLOCK);
-    if (nb > MOM_BLOCKS) nb = MOM_BLOCKS;
-    void *scr;
-    int32_t rc = mcs_scratch(ctx, sizeof(double) * ((size_t)nb * (MOM_REC + 2 * CR_CHUNK) + 64), &scr);
-    if (rc) return rc;
-    double *partial = (double *)scr;
+    double *partial = scr;
     double *res = partial + (size_t)nb * (MOM_REC + 2 * CR_CHUNK);
     if (p->prec == MCS_PREC_F32)
         hipLaunchKernelGGL(k_moments<SplitBuf>, dim3(nb), dim3(MCS_BLOCK), 0, ctx->stream, p->sbuf[0], p->n,
@@ -1021,4 +1027,68 @@ extern "C" int32_t mcs_pop_moments(mcs_pop *p, const mcs_table *t, mcs_moments *
         }
     }
     return MCS_OK;
+}
+
+extern "C" int32_t mcs_pop_moments(mcs_pop *p, const mcs_table *t, mcs_moments *out)
+{
+    if (!p || !out) return MCS_E_ARG;
+    mcs_ctx *ctx = p->ctx;
+    hipSetDevice(ctx->device);
+    memset(out, 0, sizeof(*out));
+    out->nparticles = p->n;
+    if (p->n == 0) return MCS_OK;
+    const int nb = moments_blocks(p->n);
+    void *scr;
+    int32_t rc = mcs_scratch(ctx, sizeof(double) * moments_scratch_doubles(nb), &scr);
+    if (rc) return rc;
+    return moments_run(p, t, out, nb, (double *)scr);
+}
+
+// ---------------------------------------------------- allocation-free moments
+
+// Scratch of mcs_pop_moments_noalloc: room for the partial records of the
+// reduction's largest grid (MOM_BLOCKS blocks) and the final values, 176.5 KiB.
+// Each device holds its own copy from the library's load until the process ends;
+// host code reaches it only through the address hipGetSymbolAddress gives for
+// that device (noalloc_scratch_ptr), never through the name.
+#define MCS_NOALLOC_DOUBLES (MOM_BLOCKS * (MOM_REC + 2 * CR_CHUNK) + 64)
+static __device__ __align__(256) double scratch[MCS_NOALLOC_DOUBLES];
+
+static std::mutex g_noalloc_mu;
+static std::map<int, double *> g_noalloc_ptr;   // device -> address of its scratch
+
+// Looks up the device address of `scratch` on the current device (once per
+// device); called by mcs_ctx_create with `device` current.
+static int32_t noalloc_setup(int device)
+{
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr.count(device)) return MCS_OK;
+    double *ptr = nullptr;
+    if (hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return MCS_E_HIP;
+    g_noalloc_ptr[device] = ptr;
+    return MCS_OK;
+}
+
+static double *noalloc_scratch_ptr(int device)
+{
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    auto it = g_noalloc_ptr.find(device);
+    return it == g_noalloc_ptr.end() ? nullptr : it->second;
+}
+
+extern "C" int32_t mcs_pop_moments_noalloc(mcs_pop *p, const mcs_table *t, mcs_moments *out)
+{
+    if (!p || !out) return MCS_E_ARG;
+    mcs_ctx *ctx = p->ctx;
+    hipSetDevice(ctx->device);
+    memset(out, 0, sizeof(*out));
+    out->nparticles = p->n;
+    if (p->n == 0) return MCS_OK;
+    const int nb = moments_blocks(p->n);
+    if (moments_scratch_doubles(nb) > (size_t)MCS_NOALLOC_DOUBLES)
+        return mcs_fail(ctx, MCS_E_ARG, "moments_noalloc: %d blocks need %zu doubles of scratch, the maximum is %d",
+                        nb, moments_scratch_doubles(nb), MCS_NOALLOC_DOUBLES);
+    double *scr = noalloc_scratch_ptr(ctx->device);
+    if (!scr) return mcs_fail(ctx, MCS_E_HIP, "moments_noalloc: no scratch address for device %d", ctx->device);
+    return moments_run(p, t, out, nb, scr);
 }
