@@ -198,7 +198,10 @@ int32_t mcs_pop_moments_noalloc(mcs_pop *p, const mcs_table *t, mcs_moments *out
  * Yoshida push (electron.jl:51-62), null-collision sampling (collisions.jl:159-192),
  * collision kinematics (electron.jl:108-139) and births/deaths (collisions.jl:173-192,
  * population.jl:88-106) in one kernel.  tracker = NULL is VoidCollisionTracker; a mesh
- * adds +-w/dV ion charge to its QFIXED array (gridfields.jl:193-224). */
+ * adds +-w/dV ion charge to its QFIXED array (gridfields.jl:193-224): the step logs
+ * its deposits and adds each cell's in particle order (then event order within a
+ * particle) to the cell's value -- the reference's sequential order, deterministic,
+ * no floating-point atomics. */
 int32_t mcs_advance(mcs_pop *p, const mcs_table *t, const mcs_field *field, double dt,
                     mcs_mesh *tracker, uint64_t seed, uint32_t instance, uint32_t step,
                     mcs_step_stats *stats);
@@ -391,7 +394,7 @@ int32_t mcs_cyl_resample(mcs_pop *p, mcs_cgrid *g, int32_t ntarget, int32_t nmax
  * (gridfields.jl:115-190: bilinear er, ez on the staggered faces, er rotated by
  * x/r, y/r; zero outside 0 < r < R, 0 < z < L) with, when track != 0, the
  * CollisionTracker of the same grid (gridfields.jl:203-218: +-w/dV into QFIXED at
- * the NGP cell; atomics, so the order of same-cell additions is not fixed).
+ * the NGP cell, added per cell in particle order, as in mcs_advance).
  * FP64 populations only; otherwise as mcs_advance. */
 int32_t mcs_advance_cyl(mcs_pop *p, const mcs_table *t, mcs_cgrid *g, double dt, int32_t track,
                         uint64_t seed, uint32_t instance, uint32_t step, mcs_step_stats *stats);

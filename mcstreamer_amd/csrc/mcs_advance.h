@@ -5,19 +5,34 @@
 
 #include "mcs_device.h"
 
-// track1 (MCStreamer/src/gridfields.jl:203-213): val/dV into the NGP cell.
-__device__ __forceinline__ void track_dev(const MeshView &M, const double x[3], double val)
+// track1 (MCStreamer/src/gridfields.jl:203-213): the NGP cell of x (flat index
+// into the ghosted cell-centred array) and the amount val/dV it receives; false
+// when x is outside the mesh.  FIELD 2 is the axisymmetric grid (dV = 2 pi rc dr
+// dz of the cell's radial index), otherwise the 3D mesh (dV = h^3).  Every mesh
+// array has fewer than 2^31 elements (checked at creation).
+template <int FIELD>
+__device__ __forceinline__ bool track_cell(const MeshView &M, const double x[3], double val, int32_t &cell,
+                                           double &q)
 {
+    if (FIELD == 2) {
+        int64_t i;
+        const int64_t c = cyl_cell_dev(M.C, x, i);
+        if (c < 0) return false;
+        cell = (int32_t)c;
+        q = val / gld(M.C.dv + (i - 1));
+        return true;
+    }
     int64_t c[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         double f = floor(x[d] / M.h);
-        if (!(f >= 0.0 && f < (double)M.n[d])) return;
+        if (!(f >= 0.0 && f < (double)M.n[d])) return false;
         c[d] = (int64_t)f;
     }
-    double dV = (M.h * M.h) * M.h;
-    int64_t idx = (c[0] + 1) + (int64_t)(M.n[0] + 2) * ((c[1] + 1) + (int64_t)(M.n[1] + 2) * (c[2] + 1));
-    atomicAdd(&M.qfixed[idx], val / dV);
+    const double dV = (M.h * M.h) * M.h;
+    cell = (int32_t)((c[0] + 1) + (int64_t)(M.n[0] + 2) * ((c[1] + 1) + (int64_t)(M.n[1] + 2) * (c[2] + 1)));
+    q = val / dV;
+    return true;
 }
 
 // The field at x: uniform (FIELD 0), the 3D mesh's faces (1) or the
@@ -37,26 +52,21 @@ __device__ __forceinline__ void field_at(const MeshView &M, const double x[3], c
     }
 }
 
-template <int FIELD>
-__device__ __forceinline__ void track_at(const MeshView &M, const double x[3], double val)
-{
-    if (FIELD == 2) track_cyl_dev(M.C, x, val);
-    else track_dev(M, x, val);
-}
-
 // One particle through one time step dt (JuMC/src/mixed_population.jl:31-51):
 // free flights to the next (null) collision, the Yoshida push of
 // electron.jl:51-62 with the field evaluated three times per flight, the
 // sampler of collisions.jl:159-192 and the outcomes of electron.jl:108-139.
 // `birth(x, v, s, active)` receives each ionization child (weight w), `event(j)`
-// each selected table row.  A particle removed by attachment keeps flying
-// until dt is used up (the loop does not test `active`).
-template <int FIELD_MESH, int TRACK, typename Birth, typename Event>
+// each selected table row, and with TRACK `track(x, +-w)` each ion-charge deposit
+// of the CollisionTracker (gridfields.jl:203-218: +w at an ionization, -w at an
+// attachment, at the particle's position).  A particle removed by attachment
+// keeps flying until dt is used up (the loop does not test `active`).
+template <int FIELD_MESH, int TRACK, typename Birth, typename Event, typename Track>
 __device__ __forceinline__ void advance_one(double x[3], double v[3], const double w, double &s,
                                             uint8_t &act, mcs_stream &rs, const double dt,
                                             const TableView &T, const MeshView &M,
                                             const double *euni, int32_t &flags, int &flights,
-                                            int &deaths, Birth &&birth, Event &&event)
+                                            int &deaths, Birth &&birth, Event &&event, Track &&track)
 {
     const double maxrate = T.maxrate;
     double trem = dt;
@@ -151,11 +161,11 @@ __device__ __forceinline__ void advance_one(double x[3], double v[3], const doub
                     v[1] = om1[1] * vabs;
                     v[2] = om1[2] * vabs;
                     double sc = mcs_nextcoll(&rs);
-                    if (TRACK) track_at<FIELD_MESH>(M, x, w);
+                    if (TRACK) track(x, w);
                     double vc[3] = {om2[0] * vabs, om2[1] * vabs, om2[2] * vabs};
                     birth(x, vc, sc, act);
                 } else if (kind == MCS_PROC_ATTACHMENT) {
-                    if (TRACK) track_at<FIELD_MESH>(M, x, -w);
+                    if (TRACK) track(x, -w);
                     if (act) deaths += 1;
                     act = 0;
                 }

@@ -268,7 +268,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_scan_down(const TI *in, int64_t l
 }
 
 template <typename TI, typename TO>
-static int32_t scan_impl(mcs_ctx *ctx, const TI *in, TO *out, int64_t len)
+static int32_t scan_impl(mcs_ctx *ctx, const TI *in, TO *out, int64_t len, TO *total = nullptr)
 {
     int64_t nb = cdiv(len, SCAN_TILE);
     if (nb == 0) nb = 1;
@@ -279,11 +279,16 @@ static int32_t scan_impl(mcs_ctx *ctx, const TI *in, TO *out, int64_t len)
     hipLaunchKernelGGL((k_scan_reduce<TI, TO>), dim3((unsigned)nb), dim3(MCS_BLOCK), 0,
                        ctx->stream, in, len, bsum);
     hipLaunchKernelGGL((k_scan_top<TO>), dim3(1), dim3(MCS_BLOCK), 0, ctx->stream, bsum, nb,
-                       out + len);
+                       total ? total : out + len);
     hipLaunchKernelGGL((k_scan_down<TI, TO>), dim3((unsigned)nb), dim3(MCS_BLOCK), 0,
                        ctx->stream, in, len, bsum, out);
     MCS_LAUNCH_CHECK(ctx);
     return MCS_OK;
+}
+
+int32_t mcs_scan_i64_to(mcs_ctx *ctx, const int64_t *in, int64_t *out, int64_t len, int64_t *total)
+{
+    return scan_impl<int64_t, int64_t>(ctx, in, out, len, total);
 }
 
 // Same scan with a caller-owned block-sum buffer (cdiv(len, 2048) + 1 entries),
@@ -579,6 +584,28 @@ extern "C" int32_t mcs_pop_create(mcs_ctx *ctx, int64_t capacity, mcs_pop **out)
     return MCS_OK;
 }
 
+int32_t mcs_pop_ensure_track(mcs_pop *p)
+{
+    if (p->trk_parent) return MCS_OK;
+    const int64_t nblk = cdiv(p->cap, MCS_BLOCK) + 2;
+    if (hipMalloc(&p->trk_parent, sizeof(int32_t) * p->logcap) != hipSuccess ||
+        hipMalloc(&p->trk_seq, sizeof(int32_t) * p->logcap) != hipSuccess ||
+        hipMalloc(&p->trk_cell, sizeof(int32_t) * p->logcap) != hipSuccess ||
+        hipMalloc(&p->trk_val, sizeof(double) * p->logcap) != hipSuccess ||
+        hipMalloc(&p->tblk, sizeof(int64_t) * (2 * nblk + 4)) != hipSuccess) {
+        hipFree(p->trk_parent);
+        hipFree(p->trk_seq);
+        hipFree(p->trk_cell);
+        hipFree(p->trk_val);
+        hipFree(p->tblk);
+        p->trk_parent = p->trk_seq = p->trk_cell = nullptr;
+        p->trk_val = nullptr;
+        p->tblk = nullptr;
+        return mcs_fail(p->ctx, MCS_E_HIP, "population: tracker log allocation failed");
+    }
+    return MCS_OK;
+}
+
 int32_t mcs_pop_ensure_alt(mcs_pop *p)
 {
     if (p->has_alt) return MCS_OK;
@@ -640,6 +667,11 @@ extern "C" int32_t mcs_pop_destroy(mcs_pop *p)
     hipFree(p->blk);
     if (p->dep_w) hipFree(p->dep_w);
     if (p->dep_k) hipFree(p->dep_k);
+    hipFree(p->trk_parent);
+    hipFree(p->trk_seq);
+    hipFree(p->trk_cell);
+    hipFree(p->trk_val);
+    hipFree(p->tblk);
     delete p;
     return MCS_OK;
 }

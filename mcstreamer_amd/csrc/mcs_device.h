@@ -165,15 +165,6 @@ __device__ __forceinline__ void efield_cyl_dev(const CylView &C, const double x[
     if (e2 > MCS_EFIELD_LIMIT2) flags |= MCS_FLAG_FIELD;
 }
 
-// track1 (MCStreamer/src/gridfields.jl:203-213) on the axisymmetric grid.
-__device__ __forceinline__ void track_cyl_dev(const CylView &C, const double x[3], double val)
-{
-    int64_t i;
-    const int64_t c = cyl_cell_dev(C, x, i);
-    if (c < 0) return;
-    atomicAdd(&C.qfixed[c], val / gld(C.dv + (i - 1)));
-}
-
 // energy(p) = 0.5*m*(v1^2 + v2^2 + v3^2)  (JuMC/src/electron.jl:27)
 __device__ __forceinline__ double energy_dev(double v0, double v1, double v2)
 {

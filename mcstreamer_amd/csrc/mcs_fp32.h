@@ -215,19 +215,21 @@ __device__ __forceinline__ void efield32(const MeshView32 &M, const int32_t c[3]
     if (e2 > (float)MCS_EFIELD_LIMIT2) flags |= MCS_FLAG_FIELD;
 }
 
-// NGP ion-charge tracker (gridfields.jl:203-213) from split coordinates.
-__device__ __forceinline__ void track32(const MeshView32 &M, const int32_t c[3], const float off[3],
-                                        double val)
+// NGP ion-charge tracker (gridfields.jl:203-213) from split coordinates: the
+// cell (flat index into the ghosted array) and the amount val/dV it receives;
+// false outside the mesh.
+__device__ __forceinline__ bool track_cell32(const MeshView32 &M, const int32_t c[3], const float off[3],
+                                             double val, int32_t &cell, double &amount)
 {
     int32_t q[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         q[d] = c[d] + (int32_t)floorf(off[d]);
-        if (q[d] < 0 || q[d] >= M.n[d]) return;
+        if (q[d] < 0 || q[d] >= M.n[d]) return false;
     }
-    const int64_t idx = (int64_t)(q[0] + 1) +
-                        (int64_t)(M.n[0] + 2) * ((int64_t)(q[1] + 1) + (int64_t)(M.n[1] + 2) * (q[2] + 1));
-    atomicAdd(&M.qfixed[idx], val * M.inv_dv);
+    cell = (q[0] + 1) + (M.n[0] + 2) * ((q[1] + 1) + (M.n[1] + 2) * (q[2] + 1));
+    amount = val * M.inv_dv;
+    return true;
 }
 
 // Moves whole cells from the offset into the packed coordinates, clamped to
@@ -248,13 +250,15 @@ __device__ __forceinline__ void rebase32(int32_t c[3], float off[3], const int32
 // parabola in a uniform one, collisions of collisions.jl:159-192 and
 // electron.jl:108-139).  Draws: 24-bit uniforms, one word each, in the order of
 // the FP64 body.  `birth(c, off, v, s, active)` receives each ionization child
-// at the parent's position, `event(j)` each selected table row.
-template <int FIELD_MESH, int TRACK, typename Birth, typename Event>
+// at the parent's position, `event(j)` each selected table row, and with TRACK
+// `track(c, off, +-w)` each ion-charge deposit of the CollisionTracker.
+template <int FIELD_MESH, int TRACK, typename Birth, typename Event, typename Track>
 __device__ __forceinline__ void advance_one32(int32_t c[3], float off[3], float v[3], const float w,
                                               float &s, uint8_t &act, mcs_stream &rs, const float dt,
                                               const TableView32 &T, const MeshView32 &M,
                                               const float *acc_uni, const float inv_h, int32_t &flags,
-                                              int &flights, int &deaths, Birth &&birth, Event &&event)
+                                              int &flights, int &deaths, Birth &&birth, Event &&event,
+                                              Track &&track)
 {
     const float maxrate = T.maxrate;
     float trem = dt;
@@ -353,11 +357,11 @@ __device__ __forceinline__ void advance_one32(int32_t c[3], float off[3], float 
                     v[1] = om1[1] * vabs;
                     v[2] = om1[2] * vabs;
                     const float sc = nextcoll32(&rs);
-                    if (TRACK) track32(M, c, off, (double)w);
+                    if (TRACK) track(c, off, (double)w);
                     const float vc[3] = {om2[0] * vabs, om2[1] * vabs, om2[2] * vabs};
                     birth(c, off, vc, sc, act);
                 } else if (kind == MCS_PROC_ATTACHMENT) {
-                    if (TRACK) track32(M, c, off, -(double)w);
+                    if (TRACK) track(c, off, -(double)w);
                     if (act) deaths += 1;
                     act = 0;
                 }

@@ -185,6 +185,12 @@ struct mcs_pop {
     int64_t *blk = nullptr;   // block totals, then their prefix (2 * (cap/256 + 2) + 4)
     void *dep_w = nullptr;     // CIC deposit shares (8 doubles x cap), allocated on first use
     uint64_t *dep_k = nullptr;
+    // ion-tracker deposits of a step (logcap each; then their cell-sorted copy),
+    // and per-block deposit counts and their prefix; allocated on the first
+    // tracking step
+    int32_t *trk_parent = nullptr, *trk_seq = nullptr, *trk_cell = nullptr;
+    double *trk_val = nullptr;
+    int64_t *tblk = nullptr;
 };
 
 struct mcs_mesh {
@@ -301,9 +307,13 @@ int32_t mcs_scan_i32(mcs_ctx *ctx, const int32_t *in, int32_t *out, int64_t len)
 // exclusive scan of int64 values (len entries) into out (len+1 entries).
 int32_t mcs_scan_i64(mcs_ctx *ctx, const int64_t *in, int64_t *out, int64_t len);
 int32_t mcs_scan_i64_buf(mcs_ctx *ctx, const int64_t *in, int64_t *out, int64_t len, int64_t *bsum);
+// exclusive scan of len int64 values into out (len entries) with the total at *total.
+int32_t mcs_scan_i64_to(mcs_ctx *ctx, const int64_t *in, int64_t *out, int64_t len, int64_t *total);
 #define MCS_SCAN_TILE 2048
 
 int32_t mcs_pop_ensure_alt(mcs_pop *p);
+// the tracker's deposit log (allocated once)
+int32_t mcs_pop_ensure_track(mcs_pop *p);
 // Gives p a fresh order serial (unique over all populations of the process).
 void mcs_pop_touch(mcs_pop *p);
 // Whether cell starts recorded at serial `gen` describe p's current order.

@@ -2,6 +2,8 @@
 // repack! and the swarm's global resample.
 #include <cstring>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "mcs_advance.h"
 #include "mcs_fp32.h"
 
@@ -10,6 +12,9 @@
 #define C_FLIGHTS 1
 #define C_DEATHS 2
 #define C_FLAGS 3
+#define C_TLOG 4     // tracker deposits logged
+#define C_BIRTHS 5   // births (the births' scan total)
+#define C_TTOT 6     // tracker deposits (their scan total)
 #define C_EVENTS 8
 
 struct BirthLog {
@@ -36,6 +41,59 @@ static inline BirthLog birth_log(mcs_pop *p)
     return L;
 }
 
+// The ion tracker's deposits of one step (CollisionTracker, gridfields.jl:203-218):
+// each (cell, amount) is logged under (parent slot, ordinal within the parent) at
+// a slot taken from a counter; per-thread counts and their block prefix
+// (toff, tbtot) give every deposit its place in parent order afterwards, where
+// the deposits are sorted stably by cell and each cell's run is added to its
+// value in that order -- the order of the reference's sequential loop, with no
+// floating-point atomics.
+struct TrackLog {
+    int32_t *parent, *seq, *cell;
+    double *val;
+    int64_t cap;
+    int32_t *toff;
+    int64_t *tbtot;
+};
+
+static inline TrackLog track_log(mcs_pop *p)
+{
+    TrackLog T;
+    T.parent = p->trk_parent;
+    T.seq = p->trk_seq;
+    T.cell = p->trk_cell;
+    T.val = p->trk_val;
+    T.cap = p->logcap;
+    T.toff = p->i32b;
+    T.tbtot = p->tblk;
+    return T;
+}
+
+__device__ __forceinline__ void track_push(const TrackLog &TL, int64_t *ctr, int64_t parent, int &nt,
+                                           int32_t cell, double amount, int32_t &flags)
+{
+    const unsigned long long slot = atomicAdd((unsigned long long *)&ctr[C_TLOG], 1ULL);
+    if ((int64_t)slot < TL.cap) {
+        TL.parent[slot] = (int32_t)parent;
+        TL.seq[slot] = nt;
+        TL.cell[slot] = cell;
+        TL.val[slot] = amount;
+    } else {
+        flags |= MCS_FLAG_CAPACITY;
+    }
+    nt += 1;
+}
+
+// The per-thread deposit counts of a block: offsets of each thread's deposits
+// within the block, the block's total at its logical index.
+__device__ __forceinline__ void track_block_counts(const TrackLog &TL, int64_t i, int nt)
+{
+    int ttot;
+    const int tex = block_excl_scan<int>(nt, &ttot);
+    if (nt > 0) TL.toff[i] = tex;
+    if (threadIdx.x == 0) TL.tbtot[xcd_block()] = ttot;
+}
+
 struct AdvArgs {
     PopBuf b;
     int64_t n0;
@@ -49,6 +107,7 @@ struct AdvArgs {
     int32_t *loff;
     int64_t *btot;
     int64_t *ctr;
+    TrackLog TL;
 };
 
 // The fused step: one thread per particle slot that exists when the step starts
@@ -74,7 +133,7 @@ __global__ __launch_bounds__(MCS_BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8)
     __syncthreads();
 
     const int64_t i = xcd_gidx();
-    int nb = 0;
+    int nb = 0, nt = 0;
     int32_t flags = 0;
     int flights = 0, deaths = 0;
     if (i < a.n0 && a.b.active[i]) {
@@ -105,7 +164,13 @@ __global__ __launch_bounds__(MCS_BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8)
                 }
                 nb += 1;
             },
-            [&](int j) { atomicAdd(&s_ev[j], 1); });
+            [&](int j) { atomicAdd(&s_ev[j], 1); },
+            [&](const double *tx, double val) {
+                int32_t cell;
+                double amount;
+                if (track_cell<FIELD_MESH>(a.M, tx, val, cell, amount))
+                    track_push(a.TL, a.ctr, i, nt, cell, amount, flags);
+            });
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             a.b.x[d][i] = x[d];
@@ -118,6 +183,7 @@ __global__ __launch_bounds__(MCS_BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8)
     int ex = block_excl_scan<int>(nb, &tot);
     if (nb > 0) a.loff[i] = ex;
     if (threadIdx.x == 0) a.btot[xcd_block()] = tot;
+    if (TRACK) track_block_counts(a.TL, i, nt);
     if (flights) atomicAdd(&s_flights, flights);
     if (deaths) atomicAdd(&s_deaths, deaths);
     if (flags) atomicOr(&s_flags, flags);
@@ -184,6 +250,7 @@ struct AdvArgs32 {
     int32_t *loff;
     int64_t *btot;
     int64_t *ctr;
+    TrackLog TL;
 };
 
 template <int FIELD_MESH, int TRACK>
@@ -199,7 +266,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_advance32(AdvArgs32 a)
     }
     __syncthreads();
     const int64_t i = xcd_gidx();
-    int nb = 0;
+    int nb = 0, nt = 0;
     int32_t flags = 0;
     int flights = 0, deaths = 0;
     if (i < a.n0 && a.b.active[i]) {
@@ -233,7 +300,13 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_advance32(AdvArgs32 a)
                 }
                 nb += 1;
             },
-            [&](int j) { atomicAdd(&s_ev[j], 1); });
+            [&](int j) { atomicAdd(&s_ev[j], 1); },
+            [&](const int32_t *tc, const float *toff, double val) {
+                int32_t cell;
+                double amount;
+                if (track_cell32(a.M, tc, toff, val, cell, amount))
+                    track_push(a.TL, a.ctr, i, nt, cell, amount, flags);
+            });
         rebase32(c, off, a.nmax);
         a.b.cell[i] = pack_cell(c[0], c[1], c[2]);
 #pragma unroll
@@ -248,6 +321,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_advance32(AdvArgs32 a)
     int ex = block_excl_scan<int>(nb, &tot);
     if (nb > 0) a.loff[i] = ex;
     if (threadIdx.x == 0) a.btot[xcd_block()] = tot;
+    if (TRACK) track_block_counts(a.TL, i, nt);
     if (flights) atomicAdd(&s_flights, flights);
     if (deaths) atomicAdd(&s_deaths, deaths);
     if (flags) atomicOr(&s_flags, flags);
@@ -345,6 +419,7 @@ static int32_t launch_advance32(mcs_pop *p, const mcs_table *t, const mcs_field 
     a.loff = p->i32a;
     a.btot = btot;
     a.ctr = ctr;
+    a.TL = track_log(p);
     ProfScope ps(ctx, "advance");
     const int fm = field->kind == MCS_FIELD_MESH, tr = tracker != nullptr;
     dim3 g((unsigned)nblk), bl(MCS_BLOCK);
@@ -360,20 +435,88 @@ static int32_t launch_advance32(mcs_pop *p, const mcs_table *t, const mcs_field 
     return MCS_OK;
 }
 
-// After the step kernel: the births' prefix over blocks, the counters and
-// flags in one read-back, then the children placed after the parents.
-static int32_t advance_finish(mcs_pop *p, const mcs_table *t, int64_t n0, int64_t nblk, int64_t *btot,
-                              int64_t *bpre, int64_t *ctr, mcs_step_stats *stats)
+// Deposits placed in parent order: the deposit logged at slot r, the seq-th of
+// its parent, goes to (deposits of earlier blocks) + (of earlier threads in the
+// block) + seq.
+__global__ __launch_bounds__(MCS_BLOCK) void k_track_place(TrackLog TL, int64_t count, const int64_t *tpre,
+                                                           uint32_t *keys, double *vals)
+{
+    const int64_t r = gidx();
+    if (r >= count) return;
+    const int32_t par = TL.parent[r];
+    const int64_t dst = tpre[par / MCS_BLOCK] + TL.toff[par] + TL.seq[r];
+    keys[dst] = (uint32_t)TL.cell[r];
+    vals[dst] = TL.val[r];
+}
+
+// The deposits sorted stably by cell: the first thread of each cell's run adds
+// the run to the cell's value in order, ((q + d1) + d2) + ..., as the
+// reference's sequential loop does.
+__global__ __launch_bounds__(MCS_BLOCK) void k_track_sum(const uint32_t *keys, const double *vals, int64_t count,
+                                                         double *qfixed)
+{
+    const int64_t r = gidx();
+    if (r >= count) return;
+    const uint32_t c = keys[r];
+    if (r > 0 && keys[r - 1] == c) return;
+    double q = qfixed[c];
+    for (int64_t j = r; j < count && keys[j] == c; ++j) q = q + vals[j];
+    qfixed[c] = q;
+}
+
+// The tracker's array and its cell count (the ghosted array's length), or none.
+struct TrackTarget {
+    double *qfixed = nullptr;
+    int64_t ncells = 0;
+};
+
+// The step's tracker deposits (count of them, logged in p's track log) added to
+// the target array in parent order per cell.
+static int32_t track_apply(mcs_pop *p, const TrackTarget &tt, int64_t count, const int64_t *tpre)
 {
     mcs_ctx *ctx = p->ctx;
-    int32_t rc = mcs_scan_i64(ctx, btot, bpre, nblk);
+    if (count == 0) return MCS_OK;
+    uint32_t *keys = (uint32_t *)p->i32c, *keys_out = (uint32_t *)p->trk_cell;
+    double *vals = (double *)p->u64a, *vals_out = p->trk_val;
+    ProfScope ps(ctx, "track");
+    hipLaunchKernelGGL(k_track_place, dim3((unsigned)cdiv(count, MCS_BLOCK)), dim3(MCS_BLOCK), 0, ctx->stream,
+                       track_log(p), count, tpre, keys, vals);
+    MCS_LAUNCH_CHECK(ctx);
+    unsigned bits = 1;
+    while (bits < 32 && ((uint64_t)(tt.ncells - 1) >> bits) != 0) ++bits;
+    size_t tb = 0;
+    MCS_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tb, keys, keys_out, vals, vals_out, (size_t)count, 0u, bits,
+                                           ctx->stream));
+    void *tmp = nullptr;
+    int32_t rc = mcs_scratch(ctx, tb, &tmp);
     if (rc) return rc;
-    // the counters, then the births (the scan's total)
-    int64_t h[C_EVENTS + MCS_MAX_PROC + 1];
-    rc = mcs_readback2(ctx, ctr, C_EVENTS + t->nproc, bpre + nblk, 1, h);
+    MCS_HIP(ctx, rocprim::radix_sort_pairs(tmp, tb, keys, keys_out, vals, vals_out, (size_t)count, 0u, bits,
+                                           ctx->stream));
+    hipLaunchKernelGGL(k_track_sum, dim3((unsigned)cdiv(count, MCS_BLOCK)), dim3(MCS_BLOCK), 0, ctx->stream,
+                       (const uint32_t *)keys_out, (const double *)vals_out, count, tt.qfixed);
+    MCS_LAUNCH_CHECK(ctx);
+    return MCS_OK;
+}
+
+// After the step kernel: the births' prefix over blocks (and the tracker
+// deposits'), the counters and flags in one read-back, then the children placed
+// after the parents and the deposits added to the tracker's array.
+static int32_t advance_finish(mcs_pop *p, const mcs_table *t, int64_t n0, int64_t nblk, int64_t *btot,
+                              int64_t *bpre, int64_t *ctr, const TrackTarget &tt, mcs_step_stats *stats)
+{
+    mcs_ctx *ctx = p->ctx;
+    int32_t rc = mcs_scan_i64_to(ctx, btot, bpre, nblk, ctr + C_BIRTHS);
     if (rc) return rc;
-    int64_t logged = h[C_LOG];
-    int64_t births = h[C_EVENTS + t->nproc];
+    int64_t *tpre = p->tblk + (nblk + 1);
+    if (tt.qfixed) {
+        rc = mcs_scan_i64_to(ctx, p->tblk, tpre, nblk, ctr + C_TTOT);
+        if (rc) return rc;
+    }
+    int64_t h[C_EVENTS + MCS_MAX_PROC];
+    rc = mcs_readback(ctx, ctr, h, C_EVENTS + t->nproc);
+    if (rc) return rc;
+    const int64_t logged = h[C_LOG], births = h[C_BIRTHS];
+    const int64_t tlogged = tt.qfixed ? h[C_TLOG] : 0, tcount = tt.qfixed ? h[C_TTOT] : 0;
     int32_t flags = (int32_t)h[C_FLAGS];
     if (stats) {
         stats->births = births;
@@ -382,23 +525,30 @@ static int32_t advance_finish(mcs_pop *p, const mcs_table *t, int64_t n0, int64_
         for (int j = 0; j < t->nproc; ++j) stats->events[j] = h[C_EVENTS + j];
         stats->flags = flags;
     }
-    if (logged > p->logcap || n0 + births > p->cap) {
+    if (logged > p->logcap || n0 + births > p->cap || tlogged > p->logcap) {
         if (stats) stats->flags |= MCS_FLAG_CAPACITY;
         return mcs_fail(ctx, MCS_E_CAPACITY, "Maximum number of particles reached (%lld + %lld > %lld)",
                         (long long)n0, (long long)births, (long long)p->cap);
     }
+    if (tcount != tlogged)
+        return mcs_fail(ctx, MCS_E_HIP, "advance: %lld tracker deposits logged, %lld counted", (long long)tlogged,
+                        (long long)tcount);
     if (births > 0) {
         if (p->prec == MCS_PREC_F32)
             hipLaunchKernelGGL(k_place_births32, dim3((unsigned)cdiv(births, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
                                ctx->stream, birth_log32(p), births, (const int32_t *)p->i32a,
                                (const int64_t *)bpre, n0, p->next_id, p->sbuf[0], p->gn[0], p->gn[1], p->gn[2]);
         else
-        hipLaunchKernelGGL(k_place_births, dim3((unsigned)cdiv(births, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
-                           ctx->stream, birth_log(p), births, (const int32_t *)p->i32a,
-                           (const int64_t *)bpre, n0, p->next_id, p->buf[0]);
+            hipLaunchKernelGGL(k_place_births, dim3((unsigned)cdiv(births, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
+                               ctx->stream, birth_log(p), births, (const int32_t *)p->i32a,
+                               (const int64_t *)bpre, n0, p->next_id, p->buf[0]);
         MCS_LAUNCH_CHECK(ctx);
         p->n = n0 + births;
         p->next_id += (uint64_t)births;
+    }
+    if (tt.qfixed) {
+        rc = track_apply(p, tt, tcount, tpre);
+        if (rc) return rc;
     }
     if (stats) stats->n_out = p->n;
     if (flags & MCS_FLAG_FIELD) return mcs_fail(ctx, MCS_E_FIELD, "Too high electric field");
@@ -435,6 +585,13 @@ extern "C" int32_t mcs_advance(mcs_pop *p, const mcs_table *t, const mcs_field *
 
     if (field->kind == MCS_FIELD_MESH && tracker && tracker != field->mesh)
         return mcs_fail(ctx, MCS_E_ARG, "advance: tracker mesh must be the field mesh");
+    TrackTarget tt;
+    if (tracker) {
+        int32_t rc = mcs_pop_ensure_track(p);
+        if (rc) return rc;
+        tt.qfixed = tracker->arr[MCS_ARR_QFIXED];
+        tt.ncells = tracker->ncg;
+    }
     if (p->prec == MCS_PREC_F32) {
         int32_t rc = launch_advance32(p, t, field, dt, tracker, seed, instance, step, n0, nblk, btot, ctr);
         if (rc) return rc;
@@ -455,6 +612,7 @@ extern "C" int32_t mcs_advance(mcs_pop *p, const mcs_table *t, const mcs_field *
     a.loff = p->i32a;
     a.btot = btot;
     a.ctr = ctr;
+    a.TL = track_log(p);
     {
         ProfScope ps(ctx, "advance");
         int fm = field->kind == MCS_FIELD_MESH;
@@ -471,7 +629,7 @@ extern "C" int32_t mcs_advance(mcs_pop *p, const mcs_table *t, const mcs_field *
     }
     MCS_LAUNCH_CHECK(ctx);
     }
-    return advance_finish(p, t, n0, nblk, btot, bpre, ctr, stats);
+    return advance_finish(p, t, n0, nblk, btot, bpre, ctr, tt, stats);
 }
 
 
@@ -498,6 +656,13 @@ extern "C" int32_t mcs_advance_cyl(mcs_pop *p, const mcs_table *t, mcs_cgrid *g,
     int64_t *btot = p->blk;
     int64_t *bpre = p->blk + (nblk + 1);
     int64_t *ctr = ctx->d_small;
+    TrackTarget tt;
+    if (track) {
+        int32_t rc = mcs_pop_ensure_track(p);
+        if (rc) return rc;
+        tt.qfixed = g->arr[MCS_CARR_QFIXED];
+        tt.ncells = (int64_t)(g->M + 2) * (g->N + 2);
+    }
     hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, ctx->stream, ctr, (int64_t)64, (int64_t)0);
     AdvArgs a;
     a.b = p->buf[0];
@@ -513,6 +678,7 @@ extern "C" int32_t mcs_advance_cyl(mcs_pop *p, const mcs_table *t, mcs_cgrid *g,
     a.loff = p->i32a;
     a.btot = btot;
     a.ctr = ctr;
+    a.TL = track_log(p);
     {
         ProfScope ps(ctx, "advance");
         dim3 gr((unsigned)nblk), bl(MCS_BLOCK);
@@ -522,7 +688,7 @@ extern "C" int32_t mcs_advance_cyl(mcs_pop *p, const mcs_table *t, mcs_cgrid *g,
             hipLaunchKernelGGL((k_advance<2, 0>), gr, bl, 0, ctx->stream, a);
     }
     MCS_LAUNCH_CHECK(ctx);
-    return advance_finish(p, t, n0, nblk, btot, bpre, ctr, stats);
+    return advance_finish(p, t, n0, nblk, btot, bpre, ctr, tt, stats);
 }
 
 // ---------------------------------------------------------------- repack!

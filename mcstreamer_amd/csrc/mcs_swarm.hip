@@ -253,7 +253,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6, 8))) void
                 if (bact) tally_add(acc, bv, T);
                 nb += 1;
             },
-            [&](int) {});
+            [&](int) {}, [&](const double *, double) {});
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             a.b.x[d][i] = x[d];
@@ -322,7 +322,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void
                 if (bact) tally_add32(acc, bv, T);
                 nb += 1;
             },
-            [&](int) {});
+            [&](int) {}, [&](const int32_t *, const float *, double) {});
         const int32_t nmax[3] = {MCS_CX_MAX, MCS_CY_MAX, MCS_CZ_MAX};
         rebase32(c, off, nmax);
         a.b.cell[i] = pack_cell(c[0], c[1], c[2]);
