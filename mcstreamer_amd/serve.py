@@ -163,10 +163,11 @@ class SwarmServer:
                               relax_steps=P.relax_steps, block_steps=P.block_steps,
                               init_kT_ev=P.init_kT_ev)
 
-    def run(self, requests, wall_limit=None, log=None):
+    def run(self, requests, wall_limit=None, log=None, on_retire=None):
         """Serve `requests` (sorted by arrival) in real time; returns (results,
         unfinished requests).  With wall_limit, stops admitting and stepping after
-        that many seconds."""
+        that many seconds.  on_retire(slot, result) is called for each retired
+        request while its slab still holds its particles."""
         P = self.policy
         pending = list(requests)
         waiting, active, results = [], {}, []
@@ -212,6 +213,8 @@ class SwarmServer:
                 if t.flags or P.done(t.steps, est["drift_velocity"][1], r.target):
                     results.append(Result(r.rid, r.arrival, adm, now, int(t.steps),
                                           est["drift_velocity"][1] <= r.target, int(t.flags), est))
+                    if on_retire:
+                        on_retire(slot, results[-1])
                     self.batch.remove(slot)
                     del active[slot]
                     load -= self.work(r)

@@ -7,7 +7,9 @@ wait for a free slab instead of failing."""
 import numpy as np
 import pytest
 
-from mcstreamer_amd.serve import GasTables, PlainServer, Policy, Request, SwarmServer
+from mcstreamer_amd import constants as co
+from mcstreamer_amd.serve import GasTables, PlainServer, Policy, Request, SwarmServer, _field, request_dt
+from oracle import refcpu as R
 
 pytestmark = pytest.mark.gpu
 
@@ -39,6 +41,46 @@ def tables():
 
 def _key(res):
     return {r.rid: r for r in res}
+
+
+def test_retired_requests_equal_lone_oracle_trajectories(tables):
+    """Served with the FP64 store: at its retirement step, each request's
+    particles (active ones, in slab order) equal, bitwise, a lone oracle
+    population keyed (seed, request id) stepped as many times -- thermal start
+    (phase INIT), then advance! and Swarm.resample! per step (Swarm.jl:124-142)
+    -- and its drift-velocity estimate equals the oracle's per-step means within
+    1e-12 (the device sums in another order, compensated)."""
+    reqs = _requests()[:4]
+    srv = SwarmServer(400_000, max_slots=8, seed=SEED, policy=_policy(), tables=tables, precision="f64")
+    snaps = {}
+
+    def grab(slot, res):
+        snaps[res.rid] = (srv.batch.download(slot), srv.batch.tally(slot))
+
+    res, left = srv.run(reqs, on_retire=grab)
+    assert not left and len(res) == len(reqs)
+    pol = _policy()
+    vth = float(np.sqrt(pol.init_kT_ev * co.eV / co.electron_mass))
+    for r in reqs:
+        got, t = snaps[r.rid]
+        table = tables.get(r.o2)
+        T = R.Table(table.energy, table.rate, table.kind, table.param, table.maxrate, egrid=table.egrid)
+        P = R.swarm_init(r.n0, SEED, r.rid, vth, cap=pol.capacity(r.n0) * 4)
+        f = R.uniform_field(_field(r.en_td))
+        vz = []
+        for st in range(int(t.steps)):
+            R.advance(P, T, f, request_dt(r.en_td), None, seed=SEED, instance=r.rid, step=st)
+            m = R.moments(P, T)
+            vz.append(m.sum_v[2] / m.nactive)
+            R.swarm_resample(P, r.n0, pol.tol, seed=SEED, instance=r.rid, step=st)
+        ref = P.view()
+        a, g = ref["active"] == 1, got.active == 1
+        print(f"request {r.rid}: {t.steps} steps, actives {a.sum()} / {g.sum()}", flush=True)
+        np.testing.assert_array_equal(got.id[g], ref["id"][a])
+        for key in ("x", "v", "w", "s"):
+            np.testing.assert_array_equal(getattr(got, key)[g], ref[key][a], err_msg=key)
+        rec = np.array(vz[pol.relax_steps:])
+        np.testing.assert_allclose(t.sum[0], rec.sum(), rtol=1e-12)
 
 
 def test_results_same_alone_and_in_mix(tables):

@@ -97,3 +97,142 @@ def test_split_streamer_steps_track_fp64():
     e64, e32 = out["f64"]["events"], out["f32"]["events"]
     nz = e64 > 100
     assert np.all(np.abs(e32[nz] - e64[nz]) <= 0.03 * e64[nz] + 5 * np.sqrt(e64[nz]))
+
+
+def _oracle_chain_from(sim, n, h, conf, table, cap):
+    """The oracle's FP64 copy of a streamer's current state (particles as the
+    device holds them, its fixed charge) and the step chain of nsteps
+    (main.jl:238-313) as the FP64 parity tests compose it."""
+    from oracle import refcpu as R
+    ps = sim.popl.download()
+    P = R.Pop(ps.x, ps.v, ps.w, ps.s, ps.active, ps.id, cap=cap, next_id=ps.next_id)
+    M = R.Mesh(n, h)
+    M.qfixed[:] = sim.fields.get("qfixed")
+    T = R.Table(table.energy, table.rate, table.kind, table.param, table.maxrate, egrid=table.egrid)
+    R.sort_by_cell(P, M)
+
+    def step(i):
+        M.qpart[:] = 0.0
+        R.deposit(P, M, 1, 1, -1.0)
+        R.assemble_charge(M)
+        R.mg_solve(n, conf.bc, conf.s, conf.levels, conf.smooth1, conf.smooth2, conf.tolerance, conf.maxiter,
+                   M.u, M.q)
+        R.efield_faces(M, sim.eb)
+        st = R.advance(P, T, R.mesh_field(M), sim.dt, M, seed=sim.popl.seed, step=i)
+        R.sort_by_cell(P, M)
+        R.resample_cells(P, M, sim.ntarget, sim.nmax, seed=sim.popl.seed, step=i)
+        return st
+
+    return P, M, T, step
+
+
+def test_split_steps_against_the_oracle_fp64():
+    """The precision form (split store, FP32 step, FP32 smoother in FP64
+    refinement to 1e-10) at 1/64 of its cells -- the same 0.64 x 0.64 x 1.28 mm
+    box at 10 um, 6e5 seed pairs of width 100 um -- against the oracle's FP64
+    chain from the same state, over 8 steps of 2e-13 s.  Bounds (relative to the
+    oracle): the face field (all three components together) L2 1e-5, total
+    charge 1e-4 of sum |q|, active electrons 1e-4, events per process
+    1e-3 + 5 sqrt(count); the space-charge potential (near zero: the seed is
+    neutral, so its differences are those of the particles' sampled motion)
+    L2 1e-2.  The transverse field alone is reported, not bounded, for the same
+    reason."""
+    n, h = (64, 64, 128), 1e-5
+    cfg = streamer.make_streamer(n=n, h=h, npairs=600_000, seed=17, precision="f32", tolerance=-1e-10,
+                                 sigma=1e-4, cap=2_000_000)
+    sim = streamer.Streamer(cfg["popl"], cfg["fields"], cfg["ws"], cfg["eb"], cfg["dt"], cfg["ntarget"],
+                            cfg["nmax"])
+    P, M, T, ostep = _oracle_chain_from(sim, n, h, cfg["conf"], cfg["table"], 2_000_000)
+    ev_g = np.zeros(len(T.kind))
+    ev_o = np.zeros(len(T.kind))
+    for i in range(8):
+        r = sim.step()
+        st = ostep(i)
+        ev_g += np.array(list(r["advance"].events)[: len(T.kind)])
+        ev_o += np.array(list(st.events)[: len(T.kind)])
+        print(f"step {i}: n {r['advance'].n_out} / {st.n_out} births {r['advance'].births} / {st.births} "
+              f"mg {r['mg_iters']}", flush=True)
+    f = cfg["fields"]
+    err = {}
+    diff2 = ref2 = 0.0
+    for k, ref in (("ex", M.ex), ("ey", M.ey), ("ez", M.ez)):
+        g = f.get(k).ravel()
+        err[k] = np.linalg.norm(g - ref) / np.linalg.norm(ref)
+        diff2 += float(np.sum((g - ref) ** 2))
+        ref2 += float(np.sum(ref ** 2))
+    err["field"] = np.sqrt(diff2 / ref2)
+    err["u"] = np.linalg.norm(f.get("u").ravel() - M.u) / np.linalg.norm(M.u)
+    q_g, q_o = f.get("q").sum(), M.q.sum()
+    err["charge"] = abs(q_g - q_o) / np.abs(M.q).sum()
+    na_g = int(sim.popl.moments().nactive)
+    na_o = int(P.view()["active"].sum())
+    err["electrons"] = abs(na_g - na_o) / na_o
+    print("errors against the FP64 oracle: " + ", ".join(f"{k} {v:.3e}" for k, v in err.items()), flush=True)
+    print(f"events device {ev_g.astype(int).tolist()} oracle {ev_o.astype(int).tolist()}", flush=True)
+    assert err["field"] < 1e-5 and err["u"] < 1e-2
+    assert err["charge"] < 1e-4
+    assert err["electrons"] < 1e-4
+    assert np.all(np.abs(ev_g - ev_o) <= 1e-3 * ev_o + 5 * np.sqrt(ev_o) + 1)
+
+
+def _cic_inside_fraction(x, n, h):
+    """Fraction of each particle's CIC cloud that lands on interior cells (the
+    cell-centred tent weights per axis, dropped outside 0..n-1); 0 for particles
+    outside the mesh, which deposit nothing."""
+    frac = np.ones(len(x))
+    inside = np.ones(len(x), dtype=bool)
+    for d in range(3):
+        g = x[:, d] / h - 0.5
+        lo = np.floor(g)
+        f = g - lo
+        w = np.where((lo >= 0) & (lo < n[d]), 1.0 - f, 0.0) + np.where((lo + 1 >= 0) & (lo + 1 < n[d]), f, 0.0)
+        frac *= w
+        c = np.floor(x[:, d] / h)
+        inside &= (c >= 0) & (c < n[d])
+    return np.where(inside, frac, 0.0)
+
+
+def test_split_full_config2_solve_and_charge_properties():
+    """The precision form at its full size (256 x 256 x 512 cells of 2.5 um, 4e7
+    seed pairs): after poisson!, the residual of the solved potential,
+    rms(L u + s b) with the boundary rules, is at most 1e-10 rms(s b); the ion
+    charge is the count of seed particles inside (NGP, unit weights) and the
+    electron charge the CIC-weighted count (shares outside the mesh dropped)."""
+    cfg = streamer.make_config2()
+    n, h = (256, 256, 512), 2.5e-6
+    conf, fields = cfg["conf"], cfg["fields"]
+    sim = streamer.Streamer(cfg["popl"], fields, cfg["ws"], cfg["eb"], cfg["dt"], cfg["ntarget"], cfg["nmax"])
+    it, res = streamer.poisson(fields, sim.popl, sim.eb, sim.ws, "cic")
+    print(f"full config 2: poisson {it} iterations, resid {res:.3e}", flush=True)
+    dV = h ** 3
+    ps = sim.popl.download()
+    act = ps.active == 1
+    cells = np.floor(ps.x[act] / h)
+    inside = np.all((cells >= 0) & (cells < np.array(n)), axis=1)
+    qfixed = fields.get("qfixed")
+    assert qfixed.sum() * dV == pytest.approx(float(inside.sum()), rel=1e-12)
+    want = -(ps.w[act] * _cic_inside_fraction(ps.x[act], n, h)).sum()
+    qpart = fields.get("qpart")
+    print(f"electron charge {qpart.sum() * dV:.9e} expected {want:.9e}", flush=True)
+    assert qpart.sum() * dV == pytest.approx(want, rel=1e-6)
+    del ps, cells, qfixed, qpart
+    # residual on the interior with the ghost rules (Dirichlet: -u, Neumann: +u)
+    u = fields.get("u").reshape(n[2] + 2, n[1] + 2, n[0] + 2)
+    b = fields.get("q").reshape(u.shape)[1:-1, 1:-1, 1:-1]
+    ui = u[1:-1, 1:-1, 1:-1]
+    g = np.pad(ui, 1)
+    sgn = {0: -1.0, 1: 1.0}
+    bc = conf.bc   # x-, x+, y-, y+, z-, z+
+    g[1:-1, 1:-1, 0] = sgn[bc[0]] * ui[:, :, 0]
+    g[1:-1, 1:-1, -1] = sgn[bc[1]] * ui[:, :, -1]
+    g[1:-1, 0, 1:-1] = sgn[bc[2]] * ui[:, 0, :]
+    g[1:-1, -1, 1:-1] = sgn[bc[3]] * ui[:, -1, :]
+    g[0, 1:-1, 1:-1] = sgn[bc[4]] * ui[0]
+    g[-1, 1:-1, 1:-1] = sgn[bc[5]] * ui[-1]
+    lap = (g[1:-1, 1:-1, 2:] + g[1:-1, 1:-1, :-2] + g[1:-1, 2:, 1:-1] + g[1:-1, :-2, 1:-1]
+           + g[2:, 1:-1, 1:-1] + g[:-2, 1:-1, 1:-1] - 6.0 * ui)
+    sb = conf.s * b
+    rms_r = np.sqrt(np.mean((lap + sb) ** 2))
+    rms_sb = np.sqrt(np.mean(sb ** 2))
+    print(f"rms residual {rms_r:.3e}, rms(s b) {rms_sb:.3e}, ratio {rms_r / rms_sb:.3e}", flush=True)
+    assert rms_r <= 1.0001e-10 * rms_sb
