@@ -194,6 +194,12 @@ int32_t mcs_pop_moments(mcs_pop *p, const mcs_table *t, mcs_moments *out);
  * calls that overlap on one GPU (several contexts or threads) race. */
 int32_t mcs_pop_moments_noalloc(mcs_pop *p, const mcs_table *t, mcs_moments *out);
 
+/* Energies (J, energy(p) of JuMC/src/electron.jl:27) of the active particles in slot
+ * order into energy[0..n-1] (n = the active count, returned even when it exceeds nmax,
+ * which is then MCS_E_ARG): the sample of the swarm's Kolmogorov-Smirnov stop
+ * (Swarm/src/Swarm.jl:288-306). */
+int32_t mcs_pop_energies(mcs_pop *p, int64_t nmax, int64_t *n, double *energy);
+
 /* ---- the fused step: JuMC.advance! (JuMC/src/mixed_population.jl:24-54) ---
  * Yoshida push (electron.jl:51-62), null-collision sampling (collisions.jl:159-192),
  * collision kinematics (electron.jl:108-139) and births/deaths (collisions.jl:173-192,

@@ -171,6 +171,7 @@ SIGNATURES = {
     "mcs_pop_size": [_P, _PI64],
     "mcs_pop_moments": [_P, _P, ctypes.POINTER(Moments)],
     "mcs_pop_moments_noalloc": [_P, _P, ctypes.POINTER(Moments)],
+    "mcs_pop_energies": [_P, _I64, _PI64, _PD],
     "mcs_advance": [_P, _P, ctypes.POINTER(Field), _D, _P, _U64, _U32, _U32, ctypes.POINTER(StepStats)],
     "mcs_repack": [_P],
     "mcs_swarm_resample": [_P, _I64, _D, _U64, _U32, _U32, _PI32],

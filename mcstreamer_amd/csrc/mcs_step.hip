@@ -734,14 +734,62 @@ static int32_t count_actives(mcs_pop *p, int64_t *A, int64_t **bpre_out)
     int64_t nblk = cdiv(n, MCS_BLOCK);
     int64_t *btot = p->blk;
     int64_t *bpre = p->blk + (nblk + 1);
+    const uint8_t *act = p->prec == MCS_PREC_F32 ? p->sbuf[0].active : p->buf[0].active;
     hipLaunchKernelGGL(k_count_active, dim3((unsigned)nblk), dim3(MCS_BLOCK), 0, ctx->stream,
-                       p->buf[0].active, n, btot);
+                       act, n, btot);
     MCS_LAUNCH_CHECK(ctx);
     int32_t rc = mcs_scan_i64(ctx, btot, bpre, nblk);
     if (rc) return rc;
     rc = mcs_readback(ctx, bpre + nblk, A, 1);
     *bpre_out = bpre;
     return rc;
+}
+
+// Energies (J) of the active particles in slot order (energy, JuMC/src/electron.jl:27),
+// the sample of the swarm's Kolmogorov-Smirnov test (Swarm/src/Swarm.jl:288-306).
+template <typename Buf>
+__global__ __launch_bounds__(MCS_BLOCK) void k_active_energy(Buf b, int64_t n, const int64_t *bpre, double *out)
+{
+    const int64_t i = gidx();
+    const int a = (i < n && b.active[i]) ? 1 : 0;
+    int tot;
+    const int ex = block_excl_scan<int>(a, &tot);
+    if (!a) return;
+    const double v0 = b.v[0][i], v1 = b.v[1][i], v2 = b.v[2][i];
+    out[bpre[blockIdx.x] + ex] = energy_dev(v0, v1, v2);
+}
+
+extern "C" int32_t mcs_pop_energies(mcs_pop *p, int64_t nmax, int64_t *nout, double *energy)
+{
+    if (!p || !nout) return MCS_E_ARG;
+    mcs_ctx *ctx = p->ctx;
+    hipSetDevice(ctx->device);
+    *nout = 0;
+    const int64_t n = p->n;
+    if (n == 0) return MCS_OK;
+    int64_t A;
+    int64_t *bpre;
+    int32_t rc = count_actives(p, &A, &bpre);
+    if (rc) return rc;
+    *nout = A;
+    if (A > nmax) return mcs_fail(ctx, MCS_E_ARG, "pop_energies: %lld actives, room for %lld", (long long)A,
+                                  (long long)nmax);
+    if (A == 0) return MCS_OK;
+    if (!energy) return mcs_fail(ctx, MCS_E_ARG, "pop_energies: null output");
+    void *scr;
+    rc = mcs_scratch(ctx, sizeof(double) * (size_t)A, &scr);
+    if (rc) return rc;
+    const dim3 g((unsigned)cdiv(n, MCS_BLOCK)), bl(MCS_BLOCK);
+    if (p->prec == MCS_PREC_F32)
+        hipLaunchKernelGGL(k_active_energy<SplitBuf>, g, bl, 0, ctx->stream, p->sbuf[0], n, (const int64_t *)bpre,
+                           (double *)scr);
+    else
+        hipLaunchKernelGGL(k_active_energy<PopBuf>, g, bl, 0, ctx->stream, p->buf[0], n, (const int64_t *)bpre,
+                           (double *)scr);
+    MCS_LAUNCH_CHECK(ctx);
+    MCS_HIP(ctx, hipMemcpyAsync(energy, scr, sizeof(double) * (size_t)A, hipMemcpyDeviceToHost, ctx->stream));
+    MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return MCS_OK;
 }
 
 extern "C" int32_t mcs_repack(mcs_pop *p)

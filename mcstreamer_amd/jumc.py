@@ -203,6 +203,13 @@ class Population:
         check(self.ctx.handle, _lib.lib().mcs_pop_size(self.handle, ctypes.byref(n)))
         return n.value
 
+    def energies(self):
+        """Energies (J) of the active particles in slot order."""
+        n = ctypes.c_int64()
+        out = np.zeros(max(self.nparticles(), 1))
+        check(self.ctx.handle, _lib.lib().mcs_pop_energies(self.handle, out.size, ctypes.byref(n), dptr(out)))
+        return out[: n.value].copy()
+
     def moments(self, table=None):
         m = _lib.Moments()
         check(self.ctx.handle, _lib.lib().mcs_pop_moments(

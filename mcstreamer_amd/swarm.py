@@ -1,11 +1,14 @@
 """Host mirror of Swarm.jl (Swarm/src/Swarm.jl) on top of libmcs.
 
-  swarm(en; ...) (Swarm.jl:48-186) -> swarm
-  resample!(popl, n, tol) (:193-223) -> resample
-  gains(proc) (:229-239), collrate (:242-248) -> gains, collrates
+  main(; nsamples, en) (Swarm.jl:34-45)   -> main
+  swarm(en; ...) (:48-186)               -> swarm_scan (over E/N), swarm (one E/N)
+  resample!(popl, n, tol) (:193-223)     -> resample
+  gains(proc) (:229-239), collrate (:242-248) -> gains, the moments' collrate sums
+  stats(dfs) (:253-269)                  -> stats
+  KSTester, kstest (:274-306)            -> KSTester, ks_statistic, ks_pvalue
 Per step: advance! with a FixedField, the position tallies (:126-130) from one
-device reduction, and the global resample towards n0.  The KS-test early stop
-(:132-134) is not reproduced: runs are fixed-length.
+device reduction, the KS stop every ntest steps on the device-computed energies,
+and the global resample towards n0.
 """
 import ctypes
 
@@ -39,29 +42,105 @@ def swarm_step(popl, field, dt, n0, tol=0.1):
     return st
 
 
-def swarm(popl, en_td, dt, nt, n0, n_gas, l=100, tol=0.1):
-    """Run nt steps and return the transport coefficients of Swarm.jl:144-169:
-    mobility, diffusion (OLS of sigma^2 over the last l steps), nu, alpha, eta
-    and the per-process rates k (table order)."""
+def ks_statistic(x, y):
+    """Two-sample Kolmogorov-Smirnov statistics (delta, delta+, delta-) as the
+    reference's test computes them (HypothesisTests' ksstats, called from
+    Swarm.jl:297): the pooled sample in stable sorted order, the running sum of
+    +1/n_x for points of x and -1/n_y for points of y, its maximum and minus
+    its minimum.  (With tied values this evaluates the difference inside a tie
+    as well, like the reference.)"""
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    nx, ny = len(x), len(y)
+    order = np.argsort(np.concatenate([x, y]), kind="stable")
+    steps = np.concatenate([np.full(nx, 1.0 / nx), np.full(ny, -1.0 / ny)])[order]
+    cdf = np.cumsum(steps)
+    dp = float(cdf.max())
+    dn = float(-cdf.min())
+    return max(dp, dn), dp, dn
+
+
+def kolmogorov_sf(t):
+    """P(K > t) of the Kolmogorov distribution (the limit law of sqrt(n) D_n):
+    for t > 1 the alternating series 2 sum_{k>=1} (-1)^(k-1) exp(-2 k^2 t^2), for
+    t <= 1 one minus the theta-transformed series
+    sqrt(2 pi)/t sum_{k>=1} exp(-(2k-1)^2 pi^2 / (8 t^2)) (Kolmogorov 1933)."""
+    t = float(t)
+    if t <= 0.0:
+        return 1.0
+    if t <= 1.0:
+        acc = 0.0
+        for k in range(1, 60):
+            term = np.exp(-((2 * k - 1) ** 2) * np.pi ** 2 / (8.0 * t * t))
+            acc += term
+            if term < 1e-300:
+                break
+        return 1.0 - np.sqrt(2.0 * np.pi) / t * acc
+    acc = 0.0
+    for k in range(1, 200):
+        term = np.exp(-2.0 * k * k * t * t)
+        acc += term if k % 2 else -term
+        if term < 1e-300:
+            break
+    return 2.0 * acc
+
+
+def ks_pvalue(x, y):
+    """Two-sided p-value of the approximate two-sample KS test
+    (ApproximateTwoSampleKSTest): P(K > sqrt(n_x n_y / (n_x + n_y)) delta)."""
+    d, _, _ = ks_statistic(x, y)
+    n = len(x) * len(y) / (len(x) + len(y))
+    return kolmogorov_sf(np.sqrt(n) * d)
+
+
+class KSTester:
+    """The early stop of Swarm.jl:274-306: each test compares the energies of the
+    active particles with those of the previous test; the run stops when the two
+    samples cannot be told apart (p-value above pval)."""
+
+    def __init__(self):
+        self.prev = None
+
+    def test(self, popl, pval):
+        eng = popl.energies()
+        stop = False
+        if self.prev is not None and len(self.prev) > 0 and len(eng) > 0:
+            if ks_pvalue(eng, self.prev) > pval:
+                stop = True
+        self.prev = eng
+        return stop
+
+
+def swarm(popl, en_td, dt, nt, n0, n_gas, l=100, tol=0.1, ntest=1000, pval=0.75, resampling=True):
+    """One E/N of Swarm.swarm (Swarm.jl:124-169): up to nt steps of advance!, the
+    position tallies, the KS stop every ntest steps, resample! towards n0 and the
+    capacity stop; then the transport coefficients: mobility, diffusion (OLS of
+    sigma^2 over the last l steps), nu, alpha, eta and the per-process rates k
+    (table order).  The field points along -z (electrons drift along +z)."""
     eb = en_td * n_gas * co.Td
     field = FixedField(-eb)
     table = popl.collisions
     t = np.arange(nt) * dt
     sigma2 = np.zeros((nt, 3))
-    last = nt
-    for i in range(nt):
+    kst = KSTester()
+    stopped = None
+    i = 0
+    for i in range(1, nt + 1):
         advance(popl, field, dt, step=popl.step)
         m = popl.moments()
         n = m.nactive
         X = np.array(m.sum_x[:]) / n
         X2 = np.array(m.sum_x2[:]) / n
-        sigma2[i] = X2 - X ** 2
-        resample(popl, n0, tol, step=popl.step)
+        sigma2[i - 1] = X2 - X ** 2
+        if ntest and i % ntest == 0 and kst.test(popl, pval):
+            stopped = "ks"
+            break
+        if resampling:
+            resample(popl, n0, tol, step=popl.step)
         popl.step += 1
         if popl.nparticles() > 0.9 * popl.capacity:
-            last = i + 1
+            stopped = "capacity"
             break
-    i = last
     A = np.stack([np.ones(l), t[i - l:i]], axis=1)
     coef, *_ = np.linalg.lstsq(A, sigma2[i - l:i], rcond=None)
     D = coef[1] / 2
@@ -82,7 +161,60 @@ def swarm(popl, en_td, dt, nt, n0, n_gas, l=100, tol=0.1):
         "k": k1,
         "mean_energy": m.sum_energy / m.nactive,
         "steps": i,
+        "stopped": stopped,
+        "sigma2": sigma2[:i].copy(),
     }
+
+
+COEFFICIENTS = ("mobility", "diffusion_l", "diffusion_t", "diffusion", "nu", "alpha", "eta")
+
+
+def swarm_scan(en, table, n_gas, n0=10_000, trelax=2e-9, maxp=10 ** 7, ntest=1000, pval=0.75,
+               resampling=True, seed=0, instance0=0, l=100, ctx=None):
+    """Swarm.swarm over a list of E/N values (Swarm.jl:48-186): dt = 1/maxrate/4,
+    nt = ceil(trelax/dt); for each E/N a fresh population of n0 electrons at rest
+    at the origin (its own Philox instance: instance0 + index), run by `swarm`.
+    Returns the table's columns: en, the coefficients and the rates k (one array
+    per column, k as an (len(en), nproc) array)."""
+    from .jumc import Population
+
+    dt = 1.0 / table.maxrate / 4.0
+    nt = int(np.ceil(trelax / dt))
+    cols = {name: [] for name in ("en",) + COEFFICIENTS}
+    cols["k"] = []
+    for j, e in enumerate(en):
+        popl = Population(maxp, None, table, seed=seed, instance=instance0 + j, ctx=ctx)
+        popl.init_thermal(n0, 0.0)
+        r = swarm(popl, float(e), dt, nt, n0, n_gas, l=l, ntest=ntest, pval=pval, resampling=resampling)
+        for name in ("en",) + COEFFICIENTS:
+            cols[name].append(r[name])
+        cols["k"].append(r["k"])
+    return {k: np.array(v) for k, v in cols.items()}
+
+
+def stats(dfs):
+    """Swarm.stats (Swarm.jl:253-269): per column the mean over the replicas and,
+    except for en, the standard error std/sqrt(nsamples) (std with the n - 1
+    divisor) as name + "_err"."""
+    names = list(dfs[0])
+    if any(list(d) != names for d in dfs):
+        raise ValueError("replicas with different columns")
+    out = {}
+    n = len(dfs)
+    for name in names:
+        stack = np.array([np.asarray(d[name], dtype=np.float64) for d in dfs])
+        out[name] = stack.mean(axis=0)
+        if name != "en":
+            out[name + "_err"] = stack.std(axis=0, ddof=1) / np.sqrt(n)
+    return out
+
+
+def main(table, n_gas, nsamples=10, en=None, **kw):
+    """Swarm.main (Swarm.jl:34-45): nsamples independent replicas of swarm_scan
+    (replica r uses Philox instances r * len(en) + j) and their statistics."""
+    en = 10 ** np.linspace(-1, 3, 100) if en is None else np.asarray(en, dtype=np.float64)
+    dfs = [swarm_scan(en, table, n_gas, instance0=r * len(en), **kw) for r in range(nsamples)]
+    return stats(dfs)
 
 
 TALLY_NAMES = ("drift_velocity", "mean_energy", "ionization_rate", "attachment_rate")
