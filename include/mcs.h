@@ -280,6 +280,7 @@ int32_t mcs_mg_setinhom_mesh(mcs_mg *mg, mcs_mesh *m, int32_t face, const double
  * `compact_every` steps a stable compaction of the slab (the reference never
  * repacks a swarm; compaction only reorders, it changes no particle). */
 #define MCS_TALLY_NQ 4   /* v_z, energy, ionization rate, attachment rate */
+#define MCS_DIFFUSION_WINDOW 100   /* steps of the diffusion fit (the l of Swarm.jl:140) */
 
 typedef struct mcs_swarm_batch mcs_swarm_batch;
 
@@ -310,6 +311,10 @@ typedef struct mcs_swarm_tally {
     double bm_s1[MCS_TALLY_NQ];  /* sum of block means */
     double bm_s2[MCS_TALLY_NQ];  /* sum of squared block means */
     double last[MCS_TALLY_NQ];   /* per-step means of the last step */
+    double diffusion[3];  /* D_x, D_y, D_z: half the least-squares slope of sigma^2(t) =
+                             <x_d^2> - <x_d>^2 over the actives after each step, fitted over
+                             the last `window` steps (Swarm.jl:126-130,144-150) */
+    int64_t window;       /* steps in that fit: min(steps, MCS_DIFFUSION_WINDOW) */
 } mcs_swarm_tally;
 
 /* precision MCS_PREC_F64: the FP64 store and step (bit-exact with a lone FP64

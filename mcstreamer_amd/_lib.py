@@ -136,6 +136,8 @@ class SwarmTally(ctypes.Structure):
         ("bm_s1", ctypes.c_double * TALLY_NQ),
         ("bm_s2", ctypes.c_double * TALLY_NQ),
         ("last", ctypes.c_double * TALLY_NQ),
+        ("diffusion", ctypes.c_double * 3),
+        ("window", ctypes.c_int64),
     ]
 
 

@@ -21,6 +21,8 @@
 
 #define SW_NQ MCS_TALLY_NQ
 #define SW_NT (SW_NQ + 1)     // count + the tallied quantities
+#define SW_NX 6               // position moments: sum x_d, sum x_d^2
+#define SW_WIN MCS_DIFFUSION_WINDOW
 #define SW_MODE_NONE 0
 #define SW_MODE_DROP 1
 #define SW_MODE_CLONE 2
@@ -44,6 +46,9 @@ struct SlotDev {
     // tallies: per slot SW_NQ each
     double *sum, *sumc, *cur, *bm1, *bm2, *last;
     int64_t *steps, *nrec, *nblk, *curcnt, *nact;
+    // diffusion: sigma^2 per axis of the last SW_WIN steps (ring by step number)
+    // and the least-squares D = slope / 2 over them, 3 per slot
+    double *ring, *diff;
 };
 
 // Birth logs: one record per ionization child, written in any order and placed
@@ -79,6 +84,7 @@ struct SwArgsT {
     int32_t *loff, *cnt;
     int64_t *tile_births, *tile_bpre, *tile_clones, *tile_cpre, *tile_act, *tile_apre;
     double *tile_part;   // ntiles x SW_NT x 2 (Neumaier sum, compensation)
+    double *tile_xpart;  // ntiles x SW_NX x 2: the position moments after the step
 };
 using SwArgs64 = SwArgsT<PopBuf, SwLog>;
 using SwArgs32 = SwArgsT<SplitBuf, SwLog32>;
@@ -178,13 +184,13 @@ __device__ __forceinline__ void tally_add32(Neu2 acc[SW_NT], const float v[3], c
     nadd(acc[4], (double)att);
 }
 
-// Block-reduce SW_NT Neumaier pairs into tile_part[tile].
-template <int BS>
-__device__ __forceinline__ void tile_reduce(Neu2 acc[SW_NT], double *out)
+// Block-reduce NQ Neumaier pairs into out (a tile's partials).
+template <int BS, int NQ = SW_NT>
+__device__ __forceinline__ void tile_reduce(Neu2 acc[NQ], double *out)
 {
     __shared__ double sh[BS], shc[BS];
 #pragma unroll
-    for (int q = 0; q < SW_NT; ++q) {
+    for (int q = 0; q < NQ; ++q) {
         sh[threadIdx.x] = acc[q].s;
         shc[threadIdx.x] = acc[q].c;
         __syncthreads();
@@ -401,6 +407,123 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_sw_births(A a, int64_t logcap, in
     }
 }
 
+// Position of particle i (FP64 store: x itself; split store: the virtual grid's
+// cell and offset, as the download converts them).
+__device__ __forceinline__ void sw_pos(const PopBuf &b, int64_t i, double x[3])
+{
+    x[0] = b.x[0][i];
+    x[1] = b.x[1][i];
+    x[2] = b.x[2][i];
+}
+
+__device__ __forceinline__ void sw_pos(const SplitBuf &b, int64_t i, double x[3])
+{
+    int32_t c[3];
+    unpack_cell(b.cell[i], c[0], c[1], c[2]);
+    const int32_t o[3] = {SW_OX, SW_OY, SW_OZ};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) x[q] = ((double)(c[q] - o[q]) + (double)b.off[q][i]) * SW_H32;
+}
+
+// The position moments of Swarm.jl:126-127 after the step (the slot's parents and
+// the children placed after them; actives only), per tile, as Neumaier pairs:
+// sum x_d, sum x_d^2.  The count is the tallies' (the same actives).
+template <int BS, typename A>
+__global__ __launch_bounds__(BS) void k_sw_xmom(A a)
+{
+    const int64_t tile = blockIdx.x;
+    const int slot = a.tile_slot[tile];
+    const int64_t i = a.tile_start[tile] + threadIdx.x;
+    const int64_t f = a.S.first_tile[slot], nt = a.S.ntiles[slot];
+    const int64_t births = a.tile_bpre[f + nt] - a.tile_bpre[f];
+    int64_t lim = a.S.n[slot] + births;
+    if (lim > a.S.cap[slot]) lim = a.S.cap[slot];
+    lim += a.S.base[slot];
+    Neu2 acc[SW_NX];
+#pragma unroll
+    for (int q = 0; q < SW_NX; ++q) acc[q] = Neu2{0.0, 0.0};
+    if (a.S.used[slot] && i < lim && a.b.active[i]) {
+        double x[3];
+        sw_pos(a.b, i, x);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            nadd(acc[d], x[d]);
+            nadd(acc[3 + d], x[d] * x[d]);
+        }
+    }
+    tile_reduce<BS, SW_NX>(acc, a.tile_xpart + tile * (2 * SW_NX));
+}
+
+// Sum of NQ Neumaier pairs over the slot's tiles [f, f + nt) by one workgroup;
+// the totals (sum + compensation) to tot.
+template <int NQ>
+__device__ __forceinline__ void slot_merge(const double *part, int64_t f, int64_t nt, double *tot)
+{
+    Neu2 acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = Neu2{0.0, 0.0};
+    for (int64_t t = threadIdx.x; t < nt; t += blockDim.x) {
+        const double *p = part + (f + t) * (2 * NQ);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) nmerge(acc[q], Neu2{p[2 * q], p[2 * q + 1]});
+    }
+    __shared__ double sh[MCS_BLOCK], shc[MCS_BLOCK];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        sh[threadIdx.x] = acc[q].s;
+        shc[threadIdx.x] = acc[q].c;
+        __syncthreads();
+        for (int o = MCS_BLOCK / 2; o > 0; o >>= 1) {
+            if (threadIdx.x < o) {
+                Neu2 x{sh[threadIdx.x], shc[threadIdx.x]};
+                nmerge(x, Neu2{sh[threadIdx.x + o], shc[threadIdx.x + o]});
+                sh[threadIdx.x] = x.s;
+                shc[threadIdx.x] = x.c;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) tot[q] = sh[0] + shc[0];
+        __syncthreads();
+    }
+}
+
+// sigma^2 of this step into the slot's ring, and D = slope / 2 of the least-squares
+// line through (t, sigma^2) over the last min(steps, SW_WIN) steps (Swarm.jl:144-150:
+// [ones(l) t] \ sigma2 over the last l steps), per axis.
+__device__ void slot_diffusion(const SlotDev &S, int slot, int64_t st, int64_t nact, const double *xm)
+{
+    double *ring = S.ring + (int64_t)slot * SW_WIN * 3;
+    const int64_t idx = st % SW_WIN;
+    for (int d = 0; d < 3; ++d) {
+        double s2 = 0.0;
+        if (nact > 0) {
+            const double X = xm[d] / (double)nact, X2 = xm[3 + d] / (double)nact;
+            s2 = X2 - X * X;
+        }
+        ring[3 * idx + d] = s2;
+    }
+    const int64_t m = st + 1 < SW_WIN ? st + 1 : SW_WIN;
+    const double dt = S.dt[slot];
+    for (int d = 0; d < 3; ++d) {
+        double D = 0.0;
+        if (m >= 2) {
+            // steps st - j, j = 0..m-1; centred at their mean
+            const double jbar = 0.5 * (double)(m - 1);
+            double ybar = 0.0;
+            for (int64_t j = 0; j < m; ++j) ybar += ring[3 * ((st - j) % SW_WIN) + d];
+            ybar /= (double)m;
+            double sxy = 0.0, sxx = 0.0;
+            for (int64_t j = 0; j < m; ++j) {
+                const double u = jbar - (double)j;   // (step - mean step)
+                sxy += u * (ring[3 * ((st - j) % SW_WIN) + d] - ybar);
+                sxx += u * u;
+            }
+            D = sxy / (sxx * dt) / 2.0;
+        }
+        S.diff[3 * slot + d] = D;
+    }
+}
+
 // Per slot: births -> n, next_id; tallies of this step; resample decision of
 // Swarm.resample! (Swarm.jl:194-199).  One workgroup per slot.
 template <typename A>
@@ -436,6 +559,8 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_sw_slot_step(A a)
         if (threadIdx.x == 0) tot[q] = sh[0] + shc[0];
         __syncthreads();
     }
+    __shared__ double xm[SW_NX];
+    slot_merge<SW_NX>(a.tile_xpart, f, nt, xm);
     if (threadIdx.x != 0) return;
     // births
     int64_t births = a.tile_bpre[f + nt] - a.tile_bpre[f];
@@ -450,6 +575,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_sw_slot_step(A a)
     const int64_t nact = (int64_t)tot[0];
     a.S.nact[slot] = nact;
     const int64_t st = a.S.steps[slot];
+    slot_diffusion(a.S, slot, st, nact, xm);
     double m[SW_NQ];
     for (int q = 0; q < SW_NQ; ++q) {
         m[q] = nact > 0 ? tot[q + 1] / (double)nact : 0.0;
@@ -748,6 +874,7 @@ __global__ void k_sw_slot_init(SlotDev S, int slot, SlotInit p)
         S.bm2[k] = 0.0;
         S.last[k] = 0.0;
     }
+    for (int d = 0; d < 3; ++d) S.diff[3 * slot + d] = 0.0;
     S.steps[slot] = 0;
     S.nrec[slot] = 0;
     S.nblk[slot] = 0;
@@ -789,6 +916,7 @@ struct mcs_swarm_batch {
     int64_t *tile_act = nullptr, *tile_apre = nullptr;
     int64_t *scan_scratch = nullptr;
     double *tile_part = nullptr;
+    double *tile_xpart = nullptr;
     double *stage = nullptr;       // host-exchange staging (FP32 store), 8 doubles per particle
     int64_t stage_cap = 0;
     std::vector<HostSlot> slots;
@@ -878,6 +1006,7 @@ extern "C" int32_t mcs_swarm_batch_create(mcs_ctx *ctx, int64_t total_capacity, 
           dalloc(b, &S.bm1, SW_NQ * ms) | dalloc(b, &S.bm2, SW_NQ * ms) | dalloc(b, &S.last, SW_NQ * ms);
     rc |= dalloc(b, &S.steps, ms) | dalloc(b, &S.nrec, ms) | dalloc(b, &S.nblk, ms) | dalloc(b, &S.curcnt, ms) |
           dalloc(b, &S.nact, ms);
+    rc |= dalloc(b, &S.ring, (int64_t)SW_WIN * 3 * ms) | dalloc(b, &S.diff, 3 * ms);
     rc |= dalloc(b, &b->log_count, 8);
     rc |= dalloc(b, &b->loff, total_capacity + 512) | dalloc(b, &b->cnt, total_capacity + 512);
     b->max_tiles = total_capacity / 128 + 2 * (int64_t)max_slots + 2;
@@ -887,6 +1016,7 @@ extern "C" int32_t mcs_swarm_batch_create(mcs_ctx *ctx, int64_t total_capacity, 
     rc |= dalloc(b, &b->tile_act, b->max_tiles) | dalloc(b, &b->tile_apre, b->max_tiles + 1);
     rc |= dalloc(b, &b->scan_scratch, b->max_tiles / MCS_SCAN_TILE + 8);
     rc |= dalloc(b, &b->tile_part, b->max_tiles * 2 * SW_NT);
+    rc |= dalloc(b, &b->tile_xpart, b->max_tiles * 2 * SW_NX);
     if (rc) {
         free_all(b);
         delete b;
@@ -1113,6 +1243,7 @@ static void fill_common(mcs_swarm_batch *b, A &a)
     a.tile_act = b->tile_act;
     a.tile_apre = b->tile_apre;
     a.tile_part = b->tile_part;
+    a.tile_xpart = b->tile_xpart;
 }
 
 static SwArgs64 args64(mcs_swarm_batch *b)
@@ -1159,6 +1290,7 @@ static int32_t launch_step_t(mcs_swarm_batch *b, const A &a)
     const int64_t logcap = b->precision == MCS_PREC_F64 ? b->L.cap : b->L32.cap;
     hipLaunchKernelGGL(k_sw_births<A>, dim3((unsigned)std::min<int64_t>(cdiv(logcap, MCS_BLOCK), 2048)),
                        dim3(MCS_BLOCK), 0, ctx->stream, a, logcap, b->ntiles);
+    hipLaunchKernelGGL((k_sw_xmom<BS, A>), g, dim3(BS), 0, ctx->stream, a);
     hipLaunchKernelGGL(k_sw_slot_step<A>, dim3((unsigned)b->max_slots), dim3(MCS_BLOCK), 0, ctx->stream, a);
     hipLaunchKernelGGL((k_sw_resample<BS, A>), g, dim3(BS), 0, ctx->stream, a);
     rc = mcs_scan_i64_buf(ctx, b->tile_clones, b->tile_cpre, b->ntiles, b->scan_scratch);
@@ -1276,6 +1408,8 @@ extern "C" int32_t mcs_swarm_batch_tally_all(mcs_swarm_batch *b, int32_t nslots,
     MCS_HIP(ctx, hipMemcpy(bm1.data(), S.bm1, sizeof(double) * SW_NQ * nslots, hipMemcpyDeviceToHost));
     MCS_HIP(ctx, hipMemcpy(bm2.data(), S.bm2, sizeof(double) * SW_NQ * nslots, hipMemcpyDeviceToHost));
     MCS_HIP(ctx, hipMemcpy(last.data(), S.last, sizeof(double) * SW_NQ * nslots, hipMemcpyDeviceToHost));
+    std::vector<double> diff(3 * (size_t)nslots);
+    MCS_HIP(ctx, hipMemcpy(diff.data(), S.diff, sizeof(double) * 3 * nslots, hipMemcpyDeviceToHost));
     for (int k = 0; k < nslots; ++k) {
         mcs_swarm_tally &o = out[k];
         memset(&o, 0, sizeof(o));
@@ -1291,6 +1425,8 @@ extern "C" int32_t mcs_swarm_batch_tally_all(mcs_swarm_batch *b, int32_t nslots,
             o.bm_s2[q] = bm2[SW_NQ * k + q];
             o.last[q] = last[SW_NQ * k + q];
         }
+        for (int d = 0; d < 3; ++d) o.diffusion[d] = diff[3 * k + d];
+        o.window = std::min<int64_t>(steps[k], SW_WIN);
     }
     return MCS_OK;
 }

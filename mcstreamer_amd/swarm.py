@@ -345,7 +345,10 @@ class SwarmBatch:
 
 def tally_estimates(t):
     """Time-averaged tallies of one slot and their relative statistical errors
-    (standard error of the block means over |mean|; inf with fewer than 2 blocks)."""
+    (standard error of the block means over |mean|; inf with fewer than 2 blocks),
+    and the diffusion coefficients of the last steps' least-squares fit
+    (Swarm.jl:144-150; no error estimate, as in the reference, which takes errors
+    from replicas)."""
     out = {}
     nrec = max(int(t.nrecorded), 1)
     nb = int(t.nblocks)
@@ -358,4 +361,9 @@ def tally_estimates(t):
         else:
             err = np.inf
         out[name] = (mean, err)
+    if hasattr(t, "diffusion"):
+        D = np.array(t.diffusion[:])
+        out["diffusion_l"] = (float(D[2]), np.inf)
+        out["diffusion_t"] = (float((D[0] + D[1]) / 2), np.inf)
+        out["diffusion"] = (float(D.sum() / 3), np.inf)
     return out

@@ -68,12 +68,15 @@ def _run(block, compact_every, steps=24, relax=4, bsteps=5, only=None):
     batch.step(steps)
     for k, d in enumerate(insts):
         P, T, f, proc = oracles[k]
-        means = []
+        means, sig = [], []
         for st in range(steps):
             R.advance(P, T, f, d["dt"], None, seed=SEED, instance=d["inst"], step=st)
             m = R.moments(P, T)
             means.append(_oracle_means(m, proc, m.nactive))
+            X = np.array(m.sum_x[:]) / m.nactive
+            sig.append(np.array(m.sum_x2[:]) / m.nactive - X ** 2)
             R.swarm_resample(P, d["n0"], 0.1, seed=SEED, instance=d["inst"], step=st)
+        d = dict(d, sigma2=np.array(sig))
         yield batch, slots[k], P, np.array(means), d
 
 
@@ -100,11 +103,32 @@ def _check(batch, slot, P, means, relax, bsteps, steps):
     np.testing.assert_allclose(np.array(t.bm_s2), (bm ** 2).sum(axis=0), rtol=1e-12)
 
 
+def _check_diffusion(t, d, steps):
+    """D = slope / 2 of numpy's least-squares line through the oracle's sigma^2 of
+    the last min(steps, 100) steps (Swarm.jl:144-150)."""
+    m = min(steps, 100)
+    tt = np.arange(steps - m, steps) * d["dt"]
+    want = np.array([np.polyfit(tt, d["sigma2"][steps - m:, q], 1)[0] / 2 for q in range(3)])
+    assert t.window == m
+    np.testing.assert_allclose(np.array(t.diffusion), want, rtol=1e-7, atol=1e-12 * np.abs(want).max())
+
+
 @pytest.mark.parametrize("block,compact_every", [(256, 4), (128, 1), (256, 0)])
 def test_batch_matches_lone_oracle(block, compact_every):
     steps, relax, bsteps = 24, 4, 5
     for batch, slot, P, means, d in _run(block, compact_every, steps, relax, bsteps):
         _check(batch, slot, P, means, relax, bsteps, steps)
+        _check_diffusion(batch.tally(slot), d, steps)
+
+
+def test_batch_diffusion_window_wraps():
+    # 150 steps: the fit covers the last 100 of them, the ring has wrapped
+    steps = 150
+    for batch, slot, P, means, d in _run(256, 4, steps, 4, 5, only=[0, 2]):
+        t = batch.tally(slot)
+        _check_diffusion(t, d, steps)
+        print(f"slot {slot}: D {np.array(t.diffusion)}", flush=True)
+        assert np.all(np.array(t.diffusion) > 0)
 
 
 def test_batch_slot_independent_of_mix():
@@ -162,7 +186,8 @@ def test_batch_add_remove_between_steps():
     with pytest.raises(Exception, match="no free slab"):
         batch.add(gt, pd, (0, 0, -E), dt, 100, capacity=40_000, instance=14)
     est = tally_estimates(batch.tally(sb))
-    assert set(est) == {"drift_velocity", "mean_energy", "ionization_rate", "attachment_rate"}
+    assert set(est) == {"drift_velocity", "mean_energy", "ionization_rate", "attachment_rate", "diffusion_l",
+                        "diffusion_t", "diffusion"}
 
 
 def test_device_thermal_init_matches_oracle():
