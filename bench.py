@@ -7,11 +7,17 @@ deposit + Poisson solve + face fields + fused push/collisions (mesh gather, ion
 tracker) + diagnostics + canonical cell sort + per-cell resample.
 
 Prints one JSON line.  `value` is particle-steps/s of whole steps with the
-state resident in HBM.  `roofline` is for the dominant kernel (the fused
-advance): algorithmic bytes per launch (128 B per particle-step in FP64) over its
-average duration measured with HIP events on the library's stream.
-`cpu_baseline` times the oracle (single-threaded C restatement) on one full step
-of the same state.
+state resident in HBM.  `phase_share` splits the step by HIP events.
+`roofline` is for the step's largest share, the multigrid solve: algorithmic
+bytes per solve over its duration (HIP events around each solve);
+`advance_roofline` is the fused step's (128 B per particle-step), with its VALU
+issue rate beside it.  `late_run` times steps of the same run near its end.
+`cpu_baseline` times whole oracle steps of the same state on every thread the
+OpenMP build of the oracle gets, with the single-threaded figure beside it.
+`swarm` holds the swarm half of the metric: batched swarm solves/s against one
+plain FP64 launch sequence per instance, and served requests (p50/p99 latency,
+solves/s) at the configuration's arrival rate and at a rate the device
+sustains, against a plain FP64 server on the same stream.
 """
 import argparse
 import json
@@ -59,45 +65,72 @@ def measured_traffic(kernel, units):
         return None
 
 
-def cpu_baseline_streamer(sim, ctx_arrays, min_s=10.0, max_steps=30):
-    """Whole oracle steps from the device state after warmup, one after another,
-    until at least min_s seconds of single-threaded CPU work (bounded sample)."""
+def cpu_baseline_streamer(sim, table, omp, min_s=10.0, max_steps=30):
+    """Whole oracle steps from the device state, one after another, until at least
+    min_s seconds of CPU work (bounded sample): the OpenMP build on every thread
+    OpenMP offers (omp), or the single-threaded checker."""
     from oracle import refcpu as R
 
-    popl, fields = sim.popl, sim.fields
-    ps = popl.download()
-    n0 = len(ps)
-    P = R.Pop(ps.x, ps.v, ps.w, ps.s, ps.active, ps.id, cap=int(n0 * 3) + 1000, next_id=ps.next_id)
-    grid = fields.grid
-    M = R.Mesh(grid.n, grid.h)
-    M.qfixed[:] = fields.get("qfixed")
-    M.u[:] = fields.get("u")
-    tab = ctx_arrays["table"]
-    T = R.Table(tab.energy, tab.rate, tab.kind, tab.param, tab.maxrate, egrid=tab.egrid)
-    conf = sim.ws_conf
-    psteps, steps, iters = 0, 0, []
-    t0 = time.perf_counter()
-    el = 0.0
-    while steps < max_steps and (steps == 0 or el < min_s):
-        R.sort_by_cell(P, M)
-        M.qpart[:] = 0.0
-        R.deposit(P, M, 1, 1, -1.0)
-        R.assemble_charge(M)
-        rc, it, _ = R.mg_solve(grid.n, conf.bc, conf.s, conf.levels, conf.smooth1, conf.smooth2,
-                               conf.tolerance, conf.maxiter, M.u, M.q)
-        R.efield_faces(M, sim.eb)
-        f = R.mesh_field(M)
-        psteps += P.n
-        R.advance(P, T, f, sim.dt, M, seed=popl.seed, step=10_000 + steps)
-        R.sort_by_cell(P, M)
-        R.resample_cells(P, M, sim.ntarget, sim.nmax, seed=popl.seed, step=10_000 + steps)
-        iters.append(it)
-        steps += 1
-        el = time.perf_counter() - t0
-    return {"value": psteps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} full streamer steps of the oracle from the device state after warmup "
+    prev = R.use_openmp(omp)
+    try:
+        popl, fields = sim.popl, sim.fields
+        ps = popl.download()
+        n0 = len(ps)
+        P = R.Pop(ps.x, ps.v, ps.w, ps.s, ps.active, ps.id, cap=int(n0 * 3) + 1000, next_id=ps.next_id)
+        grid = fields.grid
+        M = R.Mesh(grid.n, grid.h)
+        M.qfixed[:] = fields.get("qfixed")
+        M.u[:] = fields.get("u")
+        T = R.Table(table.energy, table.rate, table.kind, table.param, table.maxrate, egrid=table.egrid)
+        conf = sim.ws_conf
+        psteps, steps, iters = 0, 0, []
+        t0 = time.perf_counter()
+        el = 0.0
+        while steps < max_steps and (steps == 0 or el < min_s):
+            R.sort_by_cell(P, M)
+            M.qpart[:] = 0.0
+            R.deposit(P, M, 1, 1, -1.0)
+            R.assemble_charge(M)
+            rc, it, _ = R.mg_solve(grid.n, conf.bc, conf.s, conf.levels, conf.smooth1, conf.smooth2,
+                                   conf.tolerance, conf.maxiter, M.u, M.q)
+            R.efield_faces(M, sim.eb)
+            f = R.mesh_field(M)
+            psteps += P.n
+            R.advance(P, T, f, sim.dt, M, seed=popl.seed, step=10_000 + steps)
+            R.sort_by_cell(P, M)
+            R.resample_cells(P, M, sim.ntarget, sim.nmax, seed=popl.seed, step=10_000 + steps)
+            iters.append(it)
+            steps += 1
+            el = time.perf_counter() - t0
+        cores = R.threads()
+    finally:
+        R.use_openmp(prev)
+    return {"value": psteps / el, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{steps} full streamer steps of the oracle from the device state after the timed steps "
                       f"({n0} particles at the start, {grid.n[0]}x{grid.n[1]}x{grid.n[2]} mesh, "
-                      f"{sum(iters)} fmg iterations), single-threaded, {el:.1f} s"}
+                      f"{sum(iters)} fmg iterations), {el:.1f} s on {cores} thread(s)"
+                      + (" (OpenMP build: particle loop in ordered chunks, threaded grid loops, the deposit "
+                         "as a per-cell gather; the canonical sort and the resample stay sequential)"
+                         if omp else "")}
+
+
+def read_profile_json(name, key):
+    """A record of profiles/<name> (measured by rocprofv3 counter runs), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", name)) as fd:
+            return json.load(fd)[key]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+# MI355X engine clock and SIMD count; a wave64 FP64 VALU instruction issues over 4
+# cycles (16 FP64 lanes per SIMD and cycle: 78.6 TFLOP/s FMA peak)
+CLOCK_HZ = 2.4e9
+SIMDS = 1024
+VALU_PEAK_WAVE_INSTR_PER_S = SIMDS * CLOCK_HZ / 4.0
+
+PHASES = ("mg_solve", "advance", "track", "deposit", "sort_count", "sort_scatter", "sort_gather", "resample",
+          "faces")
 
 
 def run_streamer(args):
@@ -118,23 +151,44 @@ def run_streamer(args):
     ctx.profile(True)
     psteps = 0
     vcycles = 0
+    iters = []
     t0 = time.perf_counter()
     for i in range(args.steps):
         out = sim.step()
         psteps += out["advance"].n_in
         vcycles += max(out["mg_iters"] - 1, 0)
+        iters.append(out["mg_iters"])
     ctx.synchronize()
     el = time.perf_counter() - t0
     ctx.profile(False)
     kt = {}
-    for k in ("advance", "sort_count", "sort_scatter", "sort_gather", "deposit", "faces", "resample",
-              "mg_rbgs", "mg_residual", "mg_restrict", "mg_prolong"):
+    for k in PHASES + ("mg_rbgs", "mg_residual", "mg_restrict", "mg_prolong"):
         ms, cnt = ctx.profile_read(k)
         kt[k] = {"ms_total": ms, "launches": cnt}
+    cells = float(n[0]) * n[1] * n[2]
+    # the dominant phase: the multigrid solve (HIP events around each mcs_mg_solve_mesh)
+    solve_s = kt["mg_solve"]["ms_total"] / 1e3
+    nsolve = max(kt["mg_solve"]["launches"], 1)
+    solve_bytes = (vcycles * BYTES_PER_CELL_FMG_PASS + len(iters) * 16.0) * cells / nsolve
+    solve_ach = solve_bytes / (solve_s / nsolve) / 1e9 if solve_s > 0 else 0.0
+    mg_traffic = read_profile_json("hbm_traffic.json", "mg_solve")
+    # the fused step
     adv_ms, adv_n = kt["advance"]["ms_total"], kt["advance"]["launches"]
     adv_avg_s = adv_ms / 1e3 / max(adv_n, 1)
-    bytes_per_launch = BYTES_PER_PSTEP_F64 * psteps / max(args.steps, 1)
+    pps = psteps / max(args.steps, 1)
+    bytes_per_launch = BYTES_PER_PSTEP_F64 * pps
     achieved = bytes_per_launch / adv_avg_s / 1e9
+    adv_traffic = read_profile_json("hbm_traffic.json", "k_advance<1,1>")
+    valu = read_profile_json("valu_counters.json", "k_advance<1,1>")
+    valu_obj = None
+    if valu:
+        ips = valu["valu_instr_per_particle_step"] * pps / adv_avg_s
+        valu_obj = {"bound": "valu", "achieved": ips, "peak": VALU_PEAK_WAVE_INSTR_PER_S,
+                    "unit": "wave64 VALU instructions/s", "frac": ips / VALU_PEAK_WAVE_INSTR_PER_S,
+                    "note": "SQ_INSTS_VALU per particle-step (rocprofv3 counters, profiles/valu_counters.json) "
+                            "x particle-steps per launch / HIP-event launch time; peak = 1024 SIMDs x 2.4 GHz / "
+                            "4 cycles, the FP64 issue rate (FP32 and integer instructions issue faster)"}
+    step_ms = el / args.steps * 1e3
     result = {
         "metric": METRIC,
         "value": psteps / el,
@@ -142,7 +196,7 @@ def run_streamer(args):
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": el / args.steps * 1e3,
+        "ms_per_step": step_ms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -151,22 +205,54 @@ def run_streamer(args):
         "config": {"workload": f"FP64 streamer step, {n[0]}x{n[1]}x{n[2]} mesh h={args.h:g} m, "
                                f"{args.npairs:g}-pair Gaussian seed, CIC, V(2,2) to 1e-8 rel, "
                                "merge/split 8-64, dt=2e-13 s",
-                   "mesh": list(n), "particles_per_step": psteps / max(args.steps, 1),
+                   "mesh": list(n), "particles_per_step": pps, "fmg_iterations": iters,
                    "parallelism": "single GPU"},
-        "vcycles_per_s": vcycles / el,
-        "vcycle_roofline": vcycle_roofline(vcycles / el, n),
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic("k_advance<1,1>", psteps / max(args.steps, 1)),
-                     "kernel": "k_advance<1,1> (fused push + collisions, mesh gather, ion tracker)",
-                     "avg_launch_ms": adv_avg_s * 1e3,
-                     "bytes_per_launch": bytes_per_launch},
+        "phase_share": {k: kt[k]["ms_total"] / (el * 1e3) for k in PHASES},
+        "vcycles_per_s": vcycles / solve_s if solve_s > 0 else None,
+        "vcycle_note": "fmg! corrections (V-cycle passes) per second of solve time (HIP events around the solves)",
+        "roofline": {"bound": "hbm", "achieved": solve_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": solve_ach / HBM_PEAK_GBS,
+                     "traffic": mg_traffic["hbm_bytes_per_correction"] * vcycles / nsolve if mg_traffic else None,
+                     "kernel": "multigrid solve (mcs_mg_solve_mesh: the k_mg_* kernels, the step's largest share)",
+                     "avg_launch_ms": solve_s / nsolve * 1e3, "bytes_per_launch": solve_bytes,
+                     "bytes_note": "per fmg! correction 24+24 B/cell (finest residual, update) + 165 B/cell x 8/7 "
+                                   "(V(2,2) cycles of the FMG sweep), + 16 B/cell for the final residual check"},
+        "advance_roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": achieved / HBM_PEAK_GBS,
+                             "traffic": adv_traffic["hbm_bytes_per_particle_step"] * pps if adv_traffic else None,
+                             "kernel": "k_advance<1,1> (fused push + collisions, mesh gather, ion tracker log)",
+                             "avg_launch_ms": adv_avg_s * 1e3, "bytes_per_launch": bytes_per_launch,
+                             "valu": valu_obj},
         "kernels_ms": kt,
     }
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline_streamer(sim, {"table": cfg["table"]})
+        result["cpu_baseline"] = cpu_baseline_streamer(sim, cfg["table"], omp=True)
+        result["cpu_baseline"]["single_core"] = cpu_baseline_streamer(sim, cfg["table"], omp=False)
+        print(f"cpu baseline: {result['cpu_baseline']['value']:.3e} on {result['cpu_baseline']['cores']} threads, "
+              f"{result['cpu_baseline']['single_core']['value']:.3e} on 1", file=sys.stderr, flush=True)
     else:
         result["cpu_baseline"] = None
+    if args.late_steps > 0:
+        # the same run further on: untimed steps up to late_at, then late_steps timed
+        done = args.warmup + args.steps
+        while done < args.late_at:
+            sim.step(diagnostics=False)
+            done += 1
+        ctx.synchronize()
+        lp = 0
+        t0 = time.perf_counter()
+        for _ in range(args.late_steps):
+            out = sim.step()
+            lp += out["advance"].n_in
+        ctx.synchronize()
+        lel = time.perf_counter() - t0
+        result["late_run"] = {"from_step": done, "steps": args.late_steps, "ms_per_step": lel / args.late_steps * 1e3,
+                              "particle_steps_per_s": lp / lel, "particles_per_step": lp / args.late_steps,
+                              "fmg_iterations_last": out["mg_iters"]}
+        print(f"late run: steps {done}-{done + args.late_steps}, {lel / args.late_steps * 1e3:.2f} ms/step",
+              file=sys.stderr, flush=True)
+    if args.swarm:
+        result["swarm"] = swarm_block(args)
     return result
 
 
@@ -308,6 +394,113 @@ def run_batch(args):
                          "frac": ach / HBM_PEAK_GBS, "traffic": None,
                          "kernel": "whole batched step (128 B per particle-step over the step time)"},
             "cpu_baseline": None}
+
+
+def swarm_block(args):
+    """The swarm half of the metric, bounded so that the default run stays a few
+    minutes: config 3 (batched FP32 swarms against one plain FP64 launch sequence
+    per instance) and config 4 (served requests against a plain FP64 server, both
+    in real time on the same Poisson streams)."""
+    from mcstreamer_amd import jumc, swarm
+    from mcstreamer_amd.serve import (GasTables, PlainServer, Policy, SwarmServer, _field, draw_requests,
+                                      percentiles, request_dt)
+
+    ctx = jumc.context()
+    pol = Policy()
+    t00 = time.perf_counter()
+
+    def note(msg):
+        print(f"[swarm {time.perf_counter() - t00:6.1f} s] {msg}", file=sys.stderr, flush=True)
+
+    tables = GasTables(ctx=ctx)
+    tables.warm()
+    note("gas tables ready")
+    out = {}
+    # ---- config 3
+    insts = c3_instances(args.instances, args.seed)
+    total = sum(cdiv512(pol.capacity(r.n0)) for r in insts)
+    b = swarm.SwarmBatch(total, len(insts), seed=args.seed, precision="f32")
+    for r in insts:
+        b.add(tables.get(r.o2), r.n0, _field(r.en_td), request_dt(r.en_td), r.n0, pol.capacity(r.n0),
+              instance=r.rid, relax_steps=pol.relax_steps, block_steps=pol.block_steps)
+    ctx.synchronize()
+    note(f"c3: {len(insts)} instances admitted, capacity {total}")
+    b.autotune()
+    note(f"c3: autotuned {getattr(b, 'tuned', None)}")
+    b.step(3)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    b.step(args.c3_steps)
+    ctx.synchronize()
+    t_step = (time.perf_counter() - t0) / args.c3_steps
+    flagged = sum(1 for t in b.tallies(len(insts)) if t.flags)
+    del b
+    print(f"c3 batched: {t_step * 1e3:.1f} ms per step", file=sys.stderr, flush=True)
+    # plain FP64 per instance: a stratified sample (every k-th instance by n0), each
+    # timed over c3_plain_steps steps after c3_plain_relax steps
+    order = sorted(insts, key=lambda r: r.n0)
+    k = max(len(order) // args.c3_plain_sample, 1)
+    sample = order[k // 2::k][: args.c3_plain_sample]
+    per_step = []
+    for r in sample:
+        tab = tables.get(r.o2)
+        pop = jumc.Population(2 * pol.capacity(r.n0), None, tab, seed=args.seed, instance=r.rid, ctx=ctx)
+        pop.init_thermal(r.n0, pol.init_kT_ev)
+        f = jumc.FixedField(_field(r.en_td)[2])
+        for _ in range(args.c3_plain_relax):
+            swarm.swarm_step(pop, f, request_dt(r.en_td), r.n0)
+            pop.moments(tab)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.c3_plain_steps):
+            swarm.swarm_step(pop, f, request_dt(r.en_td), r.n0)
+            pop.moments(tab)
+        ctx.synchronize()
+        per_step.append((time.perf_counter() - t1) / args.c3_plain_steps)
+        note(f"c3 plain: instance {r.rid} n0 {r.n0} {per_step[-1] * 1e3:.2f} ms/step")
+        del pop
+    plain_solve_s = float(np.mean(per_step)) * args.solve_steps
+    batched = len(insts) / (args.solve_steps * t_step)
+    out["c3"] = {
+        "workload": f"{len(insts)} instances, E/N log-uniform 10-1000 Td, O2 0-0.3, n0 log-uniform 1e3-1e6, "
+                    f"{args.solve_steps} steps each, dt=1e-12 s*100/(E/N)",
+        "batched_f32_solves_per_s": batched, "batched_ms_per_step": t_step * 1e3, "slots_flagged": flagged,
+        "plain_f64_solves_per_s": 1.0 / plain_solve_s,
+        "plain_method": f"stratified sample of {len(sample)} instances by n0, each timed over "
+                        f"{args.c3_plain_steps} steps after {args.c3_plain_relax} (advance!, tallies, resample!); "
+                        f"mean step time x {args.solve_steps} steps per solve",
+        "speedup_vs_plain": batched * plain_solve_s}
+    print(f"c3: batched {batched:.3f} solves/s, plain {1.0 / plain_solve_s:.3f} solves/s", file=sys.stderr,
+          flush=True)
+    # ---- config 4: the configuration's rate, and a rate the device sustains
+    c4 = {}
+    for label, rate, dur in (("config_rate", args.rate, args.c4_seconds), ("sustained", args.c4_low_rate,
+                                                                           args.c4_low_seconds)):
+        reqs = draw_requests(rate, dur, seed=args.seed + 1)
+        srv = SwarmServer(args.serve_capacity, max_slots=4096, seed=args.seed, policy=pol, tables=tables,
+                          precision="f32")
+        note(f"c4 {label}: {len(reqs)} requests at {rate:g}/s")
+        t0 = time.perf_counter()
+        done, left = srv.run(reqs, wall_limit=dur * 1.5,
+                             log=lambda r: note(f"c4: request {r.rid} retired after {r.steps} steps"))
+        el = time.perf_counter() - t0
+        rec = {"rate_req_per_s": rate, "seconds": dur, "arrived": len(reqs), "completed": len(done),
+               "unfinished": len(left), "converged": sum(1 for r in done if r.converged),
+               "solves_per_s": len(done) / el, "latency_s": percentiles([r.latency for r in done])}
+        if label == "sustained":
+            ps = PlainServer(seed=args.seed, policy=pol, tables=tables, ctx=ctx)
+            note("c4 plain server")
+            t0 = time.perf_counter()
+            pdone, pleft = ps.run(reqs, wall_limit=dur * 1.5)
+            pel = time.perf_counter() - t0
+            rec["plain_f64"] = {"completed": len(pdone), "unfinished": len(pleft), "solves_per_s": len(pdone) / pel,
+                                "latency_s": percentiles([r.latency for r in pdone]),
+                                "method": "one plain FP64 launch sequence per request, served one at a time in "
+                                          "arrival order in real time on the same stream"}
+        c4[label] = rec
+        print(f"c4 {label}: {len(done)}/{len(reqs)} done, p50/p99 {rec['latency_s']}", file=sys.stderr, flush=True)
+    out["c4"] = c4
+    return out
 
 
 def cdiv512(n):
@@ -711,8 +904,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["streamer", "swarm", "batch", "serve", "precision", "mg2d", "axisym"],
-                    default="streamer")
+    ap.add_argument("--workload", choices=["streamer", "swarm", "batch", "serve", "precision", "mg2d", "axisym",
+                                           "swarmblock"], default="streamer")
     ap.add_argument("--c2-pairs", type=int, default=40_000_000)
     ap.add_argument("--instances", type=int, default=1024)
     ap.add_argument("--solve-steps", type=int, default=5000)
@@ -734,6 +927,16 @@ def main():
     ap.add_argument("--swarm-n", type=int, default=10_000)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--swarm", action="store_true", help="add the swarm block (configs 3 and 4) to the streamer line")
+    ap.add_argument("--late-at", type=int, default=480, help="step of the run at which the late sample starts")
+    ap.add_argument("--late-steps", type=int, default=20, help="timed steps of the late sample (0: none)")
+    ap.add_argument("--c3-steps", type=int, default=10)
+    ap.add_argument("--c3-plain-sample", type=int, default=32)
+    ap.add_argument("--c3-plain-relax", type=int, default=100)
+    ap.add_argument("--c3-plain-steps", type=int, default=20)
+    ap.add_argument("--c4-seconds", type=float, default=15.0)
+    ap.add_argument("--c4-low-rate", type=float, default=1.0)
+    ap.add_argument("--c4-low-seconds", type=float, default=30.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device to device (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -745,7 +948,7 @@ def main():
         sys.exit(2)
     res = {"streamer": run_streamer, "swarm": run_swarm, "batch": run_batch,
            "serve": run_serve, "precision": run_precision, "mg2d": run_mg2d,
-           "axisym": run_axisym}[args.workload](args)
+           "axisym": run_axisym, "swarmblock": swarm_block}[args.workload](args)
     print(json.dumps(res))
 
 

@@ -145,8 +145,9 @@ const char *mcs_last_error(const mcs_ctx *ctx);
 int32_t mcs_ctx_synchronize(mcs_ctx *ctx);
 /* The hipStream_t the context launches on (for callers that time or order work). */
 int32_t mcs_ctx_stream(mcs_ctx *ctx, void **stream);
-/* Per-kernel HIP-event timing of the hot kernels; names: "advance", "sort_count",
- * "sort_scatter", "sort_gather", "deposit", "faces", "resample", "mg_rbgs", "mg_residual",
+/* Per-kernel HIP-event timing of the hot kernels; names: "advance", "track" (the ion
+ * tracker's ordered sums), "sort_count", "sort_scatter", "sort_gather", "deposit", "faces",
+ * "resample", "mg_solve" (a whole mcs_mg_solve_mesh), "mg_rbgs", "mg_residual",
  * "mg_restrict", "mg_prolong". */
 int32_t mcs_ctx_profile(mcs_ctx *ctx, int32_t enable);
 int32_t mcs_ctx_profile_read(mcs_ctx *ctx, const char *kernel, double *total_ms, int64_t *launches);

@@ -2205,6 +2205,7 @@ extern "C" int32_t mcs_mg_solve_mesh(mcs_mg *mg, mcs_mesh *m, int32_t *iters, do
     for (int d = 0; d < 3; ++d)
         if (m->n[d] != mg->cfg.n[d]) return mcs_fail(ctx, MCS_E_ARG, "mg: mesh size differs from the solver's");
     hipSetDevice(ctx->device);
+    ProfScope ps(ctx, "mg_solve");
     return mg_solve_dev(mg, m->arr[MCS_ARR_U], m->arr[MCS_ARR_Q], iters, resid);
 }
 

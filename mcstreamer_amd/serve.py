@@ -281,6 +281,27 @@ class PlainServer:
                 if P.done(steps, est["drift_velocity"][1], r.target):
                     return steps, est
 
+    def run(self, requests, wall_limit=None):
+        """Serve `requests` (sorted by arrival) in real time, one at a time in
+        arrival order; returns (results, unfinished).  With wall_limit, stops
+        starting requests after that many seconds (the one running completes)."""
+        results, t0 = [], time.perf_counter()
+        for k, r in enumerate(requests):
+            now = time.perf_counter() - t0
+            if wall_limit is not None and now > wall_limit:
+                return results, list(requests[k:])
+            if r.arrival > now:
+                if wall_limit is not None and r.arrival > wall_limit:
+                    return results, list(requests[k:])
+                time.sleep(r.arrival - now)
+            start = time.perf_counter() - t0
+            steps, est = self.solve(r)
+            self.ctx.synchronize()
+            fin = time.perf_counter() - t0
+            results.append(Result(r.rid, r.arrival, start, fin, steps,
+                                  est["drift_velocity"][1] <= r.target, 0, est))
+        return results, []
+
     def service_times(self, requests):
         """Measured service time of each request run alone on the device."""
         out = []

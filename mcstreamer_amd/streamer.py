@@ -239,8 +239,9 @@ def make_streamer(n=(128, 128, 256), h=5e-6, npairs=1_000_000, seed=7, levels=No
 
 def make_config1(n=(128, 128, 256), h=5e-6, npairs=1_000_000, seed=7, levels=None, cap=None, ctx=None):
     """The FP64 streamer workload: 128x128x256 cells of 5 um, 1e6-pair seed of
-    width 50 um, V(2,2) to 1e-8 relative residual."""
-    return make_streamer(n, h, npairs, seed, levels, cap, "f64", -1e-8, 50e-6 * (h / 5e-6), ctx)
+    width 50 um, V(2,2) to 1e-8 relative residual.  The default capacity (16
+    particles per seed pair) holds the avalanche of the full 500-step run."""
+    return make_streamer(n, h, npairs, seed, levels, cap or 16 * npairs, "f64", -1e-8, 50e-6 * (h / 5e-6), ctx)
 
 
 def make_config2(precision="f32", n=(256, 256, 512), h=2.5e-6, npairs=40_000_000, seed=7, levels=None,

@@ -441,28 +441,31 @@ void ref_yoshida(const ref_field *f, double x[3], double v[3], double dt, int32_
     for (int d = 0; d < 3; ++d) x[d] = x[d] + c4 * v[d];
 }
 
-/* track1 (MCStreamer/src/gridfields.jl:203-213): val/dV into the NGP cell of x. */
-static void track1(ref_mesh *m, const double x[3], double val)
+/* track1 (MCStreamer/src/gridfields.jl:203-213): the NGP cell of x in the fixed
+ * charge array and the amount val/dV it receives (NULL outside the mesh). */
+static double *track1(ref_mesh *m, const double x[3], double val, double *amount)
 {
-    if (!m) return;
+    if (!m) return NULL;
     int64_t c[3];
     for (int d = 0; d < 3; ++d) {
         double f = floor(x[d] / m->h);
-        if (!(f >= 0.0 && f < (double)m->n[d])) return;
+        if (!(f >= 0.0 && f < (double)m->n[d])) return NULL;
         c[d] = (int64_t)f;
     }
     double dV = (m->h * m->h) * m->h;
-    m->qfixed[cidx(m->n, c[0] + 1, c[1] + 1, c[2] + 1)] += val / dV;
+    *amount = val / dV;
+    return &m->qfixed[cidx(m->n, c[0] + 1, c[1] + 1, c[2] + 1)];
 }
 
 /* track1 on the axisymmetric grid (gridfields.jl:203-213). */
-static void track1_cyl(ref_cgrid *g, const double x[3], double val)
+static double *track1_cyl(ref_cgrid *g, const double x[3], double val, double *amount)
 {
-    if (!g) return;
+    if (!g) return NULL;
     int64_t i;
     int64_t c = cyl_cell(g, x, &i);
-    if (c < 0) return;
-    g->qfixed[c] += val / g->dv[i - 1];
+    if (c < 0) return NULL;
+    *amount = val / g->dv[i - 1];
+    return &g->qfixed[c];
 }
 
 /* add_particle! (JuMC/src/population.jl:88-97) */
@@ -512,15 +515,64 @@ int32_t ref_advance_cyl(ref_pop *p, const ref_table *t, ref_cgrid *g, double dt,
     return advance_impl(p, t, &f, dt, NULL, track ? g : NULL, seed, instance, step, st);
 }
 
-static int32_t advance_impl(ref_pop *p, const ref_table *t, const ref_field *f, double dt,
-                            ref_mesh *tracker, ref_cgrid *ctracker, uint64_t seed, uint32_t instance,
-                            uint32_t step, ref_stats *st)
+/* What one chunk of the particle loop produces, in particle order: the children
+ * apply! appends and the tracker's deposits.  The loop's chunks are consecutive
+ * particle ranges; their outputs are applied chunk after chunk, which is the
+ * order of the reference's sequential loop (children are not advanced in the
+ * step they are born, and the fixed-charge array is not read by the step). */
+typedef struct {
+    double x[3], v[3], w, s;
+    uint8_t act;
+} birth_rec;
+
+typedef struct {
+    double *q;
+    double amount;
+} dep_rec;
+
+typedef struct {
+    birth_rec *b;
+    int64_t nb, cb;
+    dep_rec *d;
+    int64_t nd, cd;
+    ref_stats st;
+} chunk_out;
+
+static void push_birth(chunk_out *o, const double x[3], const double v[3], double w, double s, uint8_t act)
 {
-    memset(st, 0, sizeof(*st));
-    int64_t n0 = p->n;
-    st->n_in = n0;
+    if (o->nb == o->cb) {
+        o->cb = o->cb ? 2 * o->cb : 256;
+        o->b = (birth_rec *)realloc(o->b, sizeof(birth_rec) * (size_t)o->cb);
+    }
+    birth_rec *r = &o->b[o->nb++];
+    for (int d = 0; d < 3; ++d) {
+        r->x[d] = x[d];
+        r->v[d] = v[d];
+    }
+    r->w = w;
+    r->s = s;
+    r->act = act;
+}
+
+static void push_deposit(chunk_out *o, double *q, double amount)
+{
+    if (!q) return;
+    if (o->nd == o->cd) {
+        o->cd = o->cd ? 2 * o->cd : 256;
+        o->d = (dep_rec *)realloc(o->d, sizeof(dep_rec) * (size_t)o->cd);
+    }
+    o->d[o->nd].q = q;
+    o->d[o->nd].amount = amount;
+    o->nd++;
+}
+
+static void advance_range(ref_pop *p, const ref_table *t, const ref_field *f, double dt, ref_mesh *tracker,
+                          ref_cgrid *ctracker, uint64_t seed, uint32_t instance, uint32_t step, int64_t i0,
+                          int64_t i1, chunk_out *o)
+{
+    ref_stats *st = &o->st;
     double maxrate = t->maxrate;
-    for (int64_t i = 0; i < n0; ++i) {
+    for (int64_t i = i0; i < i1; ++i) {
         if (!p->active[i]) continue;
         mcs_stream rs;
         mcs_stream_init(&rs, seed, instance, MCS_PHASE_ADVANCE, p->id[i], step);
@@ -584,16 +636,19 @@ static int32_t advance_impl(ref_pop *p, const ref_table *t, const ref_field *f, 
                                 vc[d] = om2[d] * vabs;
                             }
                             double sc = mcs_nextcoll(&rs);
-                            track1(tracker, x, w);
-                            track1_cyl(ctracker, x, w);
+                            double amount = 0.0;
+                            double *q = track1(tracker, x, w, &amount);
+                            push_deposit(o, q, amount);
+                            q = track1_cyl(ctracker, x, w, &amount);
+                            push_deposit(o, q, amount);
                             /* apply!: parent state written, child appended */
-                            if (append(p, x, vc, w, sc, act))
-                                st->births += 1;
-                            else
-                                st->flags |= FLAG_CAPACITY;
+                            push_birth(o, x, vc, w, sc, act);
                         } else if (kind == 4) { /* Attachment, electron.jl:129-131 */
-                            track1(tracker, x, -w);
-                            track1_cyl(ctracker, x, -w);
+                            double amount = 0.0;
+                            double *q = track1(tracker, x, -w, &amount);
+                            push_deposit(o, q, amount);
+                            q = track1_cyl(ctracker, x, -w, &amount);
+                            push_deposit(o, q, amount);
                             if (act) st->deaths += 1;
                             act = 0;
                         }
@@ -613,6 +668,50 @@ static int32_t advance_impl(ref_pop *p, const ref_table *t, const ref_field *f, 
         p->s[i] = s;
         p->active[i] = act;
     }
+}
+
+#ifdef _OPENMP
+#include <omp.h>
+#define REF_THREADS omp_get_max_threads()
+#else
+#define REF_THREADS 1
+#endif
+
+/* Threads of the particle loop and the grid loops (1 in the single-threaded build). */
+int32_t ref_threads(void) { return REF_THREADS; }
+
+static int32_t advance_impl(ref_pop *p, const ref_table *t, const ref_field *f, double dt,
+                            ref_mesh *tracker, ref_cgrid *ctracker, uint64_t seed, uint32_t instance,
+                            uint32_t step, ref_stats *st)
+{
+    memset(st, 0, sizeof(*st));
+    const int64_t n0 = p->n;
+    st->n_in = n0;
+    const int nch = REF_THREADS;
+    chunk_out *out = (chunk_out *)calloc((size_t)nch, sizeof(chunk_out));
+#pragma omp parallel for schedule(static, 1)
+    for (int c = 0; c < nch; ++c) {
+        const int64_t i0 = n0 * c / nch, i1 = n0 * (c + 1) / nch;
+        advance_range(p, t, f, dt, tracker, ctracker, seed, instance, step, i0, i1, &out[c]);
+    }
+    for (int c = 0; c < nch; ++c) {
+        chunk_out *o = &out[c];
+        st->flags |= o->st.flags;
+        st->flights += o->st.flights;
+        st->deaths += o->st.deaths;
+        for (int j = 0; j < REF_MAX_PROC; ++j) st->events[j] += o->st.events[j];
+        for (int64_t k = 0; k < o->nd; ++k) *o->d[k].q += o->d[k].amount;
+        for (int64_t k = 0; k < o->nb; ++k) {
+            const birth_rec *r = &o->b[k];
+            if (append(p, r->x, r->v, r->w, r->s, r->act))
+                st->births += 1;
+            else
+                st->flags |= FLAG_CAPACITY;
+        }
+        free(o->b);
+        free(o->d);
+    }
+    free(out);
     st->n_out = p->n;
     return st->flags;
 }
@@ -829,9 +928,66 @@ int32_t ref_cyl_sort_by_cell(ref_pop *p, const ref_cgrid *g, int64_t *cell_start
 /* density! (MCStreamer/src/poisson.jl:65-84): arr[cell] += w*val/dV per active
  * particle in population order (NGP), or the same amount spread with trilinear
  * weights over the 8 nearest cell centres (CIC); shares outside the mesh drop. */
+#ifdef _OPENMP
+/* The same sums as a gather per cell, for the OpenMP build (the CPU baseline):
+ * cell C takes the shares of the particles of its 27 neighbour cells, walked in
+ * increasing cell order -- increasing particle order in the canonical order of
+ * the last ref_sort_by_cell, which must describe p -- so every cell adds its
+ * terms in the order of the sequential loop. */
+static void deposit_gather(const ref_pop *p, ref_mesh *m, double *arr, int32_t scheme, double val)
+{
+    const int64_t nx = m->n[0], ny = m->n[1], nz = m->n[2];
+    const double h = m->h, dV = (h * h) * h;
+    const int64_t *cs = m->cell_start;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t cz = 0; cz < nz; ++cz)
+        for (int64_t cy = 0; cy < ny; ++cy)
+            for (int64_t cx = 0; cx < nx; ++cx) {
+                double acc = arr[cidx(m->n, cx + 1, cy + 1, cz + 1)];
+                const int r = scheme == 0 ? 0 : 1;
+                for (int64_t dz = -r; dz <= r; ++dz)
+                    for (int64_t dy = -r; dy <= r; ++dy)
+                        for (int64_t dx = -r; dx <= r; ++dx) {
+                            const int64_t kx = cx + dx, ky = cy + dy, kz = cz + dz;
+                            if (kx < 0 || kx >= nx || ky < 0 || ky >= ny || kz < 0 || kz >= nz) continue;
+                            const int64_t key = kx + nx * (ky + ny * kz);
+                            for (int64_t i = cs[key]; i < cs[key + 1]; ++i) {
+                                if (!p->active[i]) continue;
+                                const double base = (p->w[i] * val) / dV;
+                                if (scheme == 0) {
+                                    acc += base;
+                                    continue;
+                                }
+                                const double *x = &p->x[3 * i];
+                                double wgt[3];
+                                int hit = 1;
+                                const int64_t tc[3] = {cx, cy, cz};
+                                for (int d = 0; d < 3 && hit; ++d) {
+                                    double g = x[d] / h - 0.5;
+                                    double fl = floor(g);
+                                    int64_t i0 = (int64_t)fl;
+                                    double fw = g - fl;
+                                    if (tc[d] == i0) wgt[d] = 1.0 - fw;
+                                    else if (tc[d] == i0 + 1) wgt[d] = fw;
+                                    else hit = 0;
+                                }
+                                if (hit) acc += base * ((wgt[0] * wgt[1]) * wgt[2]);
+                            }
+                        }
+                arr[cidx(m->n, cx + 1, cy + 1, cz + 1)] = acc;
+            }
+}
+#endif
+
 void ref_deposit(const ref_pop *p, ref_mesh *m, int32_t which, int32_t scheme, double val)
 {
     double *arr = which == 0 ? m->qfixed : m->qpart;
+#ifdef _OPENMP
+    if (omp_get_max_threads() > 1) {
+        deposit_gather(p, m, arr, scheme, val);
+        return;
+    }
+#endif
     double h = m->h;
     double dV = (h * h) * h;
     int64_t ncells = (int64_t)m->n[0] * m->n[1] * m->n[2];
@@ -880,6 +1036,7 @@ void ref_assemble_charge(ref_mesh *m)
 {
     int64_t tot = (int64_t)(m->n[0] + 2) * (m->n[1] + 2) * (m->n[2] + 2);
     for (int64_t c = 0; c < tot; ++c) m->q[c] = 0.0;
+#pragma omp parallel for schedule(static)
     for (int64_t k = 1; k <= m->n[2]; ++k)
         for (int64_t j = 1; j <= m->n[1]; ++j)
             for (int64_t i = 1; i <= m->n[0]; ++i) {
@@ -898,6 +1055,7 @@ void ref_efield_faces(ref_mesh *m, double eb)
 #define U(i, j, k) u[cidx(m->n, (i), (j), (k))]
     /* EX (nx+1)(ny+2)(nz+2) */
 #define EX(i, j, k) m->ex[(i) + (nx + 1) * ((j) + (ny + 2) * (k))]
+#pragma omp parallel for schedule(static)
     for (int64_t k = 1; k <= nz; ++k)
         for (int64_t j = 1; j <= ny; ++j)
             for (int64_t i = 0; i <= nx; ++i) EX(i, j, k) = (U(i, j, k) - U(i + 1, j, k)) / h;
@@ -913,6 +1071,7 @@ void ref_efield_faces(ref_mesh *m, double eb)
         }
 #undef EX
 #define EY(i, j, k) m->ey[(i) + (nx + 2) * ((j) + (ny + 1) * (k))]
+#pragma omp parallel for schedule(static)
     for (int64_t k = 1; k <= nz; ++k)
         for (int64_t j = 0; j <= ny; ++j)
             for (int64_t i = 1; i <= nx; ++i) EY(i, j, k) = (U(i, j, k) - U(i, j + 1, k)) / h;
@@ -928,6 +1087,7 @@ void ref_efield_faces(ref_mesh *m, double eb)
         }
 #undef EY
 #define EZ(i, j, k) m->ez[(i) + (nx + 2) * ((j) + (ny + 2) * (k))]
+#pragma omp parallel for schedule(static)
     for (int64_t k = 0; k <= nz; ++k)
         for (int64_t j = 1; j <= ny; ++j)
             for (int64_t i = 1; i <= nx; ++i)
@@ -1100,6 +1260,7 @@ void ref_mg_gauss_seidel(const int32_t n[3], double *u, const double *b)
 {
     int64_t sy = n[0] + 2, sz = (int64_t)(n[0] + 2) * (n[1] + 2);
     for (int par = 0; par < 2; ++par)
+#pragma omp parallel for schedule(static) if (n[2] >= 16)
         for (int64_t k = 1; k <= n[2]; ++k)
             for (int64_t j = 1; j <= n[1]; ++j)
                 for (int64_t i = 1; i <= n[0]; ++i) {
@@ -1120,6 +1281,7 @@ void ref_mg_gauss_seidel(const int32_t n[3], double *u, const double *b)
 void ref_mg_residual(const int32_t n[3], double *r, const double *u, const double *b, double s)
 {
     int64_t sy = n[0] + 2, sz = (int64_t)(n[0] + 2) * (n[1] + 2);
+#pragma omp parallel for schedule(static) if (n[2] >= 16)
     for (int64_t k = 1; k <= n[2]; ++k)
         for (int64_t j = 1; j <= n[1]; ++j)
             for (int64_t i = 1; i <= n[0]; ++i) {
@@ -1142,6 +1304,7 @@ void ref_mg_restrict(const int32_t nf[3], double *rh, const double *r)
     int32_t nc[3] = {nf[0] / 2, nf[1] / 2, nf[2] / 2};
     int64_t fy = nf[0] + 2, fz = (int64_t)(nf[0] + 2) * (nf[1] + 2);
     int64_t cy = nc[0] + 2, cz = (int64_t)(nc[0] + 2) * (nc[1] + 2);
+#pragma omp parallel for schedule(static) if (nc[2] >= 16)
     for (int64_t k = 1; k <= nc[2]; ++k)
         for (int64_t j = 1; j <= nc[1]; ++j)
             for (int64_t i = 1; i <= nc[0]; ++i) {
@@ -1162,6 +1325,7 @@ void ref_mg_interpolate(const int32_t nf[3], double *r, const double *rh, int32_
     int32_t nc[3] = {nf[0] / 2, nf[1] / 2, nf[2] / 2};
     int64_t fy = nf[0] + 2, fz = (int64_t)(nf[0] + 2) * (nf[1] + 2);
     int64_t cy = nc[0] + 2, cz = (int64_t)(nc[0] + 2) * (nc[1] + 2);
+#pragma omp parallel for schedule(static) if (nc[2] >= 16)
     for (int64_t k = 1; k <= nc[2]; ++k)
         for (int64_t j = 1; j <= nc[1]; ++j)
             for (int64_t i = 1; i <= nc[0]; ++i) {
@@ -1343,6 +1507,7 @@ static int ref_fmg(const ref_mg *cfg, ref_ws *ws, double *x, const double *b, do
         ref_mgv(cfg, ws, z1, ws->res[lev], lev);
         z = z1;
     }
+#pragma omp parallel for schedule(static) if (tot >= 65536)
     for (int64_t c = 0; c < tot; ++c) x[c] += z[c];
     ref_mg_apply_bc(n, cfg->bc, x);
     return 1;

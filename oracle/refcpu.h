@@ -42,6 +42,8 @@
 
 #define REF_MAX_PROC 32
 
+int32_t ref_threads(void);
+
 typedef struct ref_pop {
     int64_t n;
     int64_t cap;

@@ -13,18 +13,40 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "refcpu.c")
 LIB = os.path.join(HERE, "librefcpu.so")
+# the same source with OpenMP: the all-core CPU baseline (chunked particle loop,
+# threaded grid loops, the deposit as a per-cell gather); results equal the
+# single-threaded build's except the residual norms, summed in thread order
+LIB_OMP = os.path.join(HERE, "librefcpu_omp.so")
 MAXP = 32
 
 CFLAGS = ["-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared", "-std=c11", "-lm"]
 
 
-def build(omp=False):
-    """Compile the oracle (single-threaded; it is the checker, not a product path)."""
-    tmp = LIB + ".tmp"
-    cmd = ["gcc", *CFLAGS, SRC, "-o", tmp]
-    subprocess.run(cmd, check=True, capture_output=True, text=True)
-    os.replace(tmp, LIB)
+def build(omp=True):
+    """Compile the oracle (single-threaded; it is the checker, not a product path)
+    and, with omp, its OpenMP build for the all-core CPU baseline."""
+    for out, extra in ((LIB, []), (LIB_OMP, ["-fopenmp"])) if omp else ((LIB, []),):
+        tmp = out + ".tmp"
+        subprocess.run(["gcc", *CFLAGS, *extra, SRC, "-o", tmp], check=True, capture_output=True, text=True)
+        os.replace(tmp, out)
     return LIB
+
+
+_use_omp = False
+
+
+def threads():
+    """Threads the selected build runs its loops on."""
+    return int(lib().ref_threads())
+
+
+def use_openmp(flag):
+    """Route the wrapper functions to the OpenMP build (True) or the
+    single-threaded checker (False); returns the previous setting."""
+    global _use_omp
+    prev = _use_omp
+    _use_omp = bool(flag)
+    return prev
 
 
 _PD = ctypes.POINTER(ctypes.c_double)
@@ -92,15 +114,15 @@ class RefMG(ctypes.Structure):
                 ("tolerance", ctypes.c_double), ("ndim", ctypes.c_int32), ("cyl", ctypes.c_int32)]
 
 
-_lib = None
+_libs = {}
 
 
 def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(SRC):
+    path = LIB_OMP if _use_omp else LIB
+    if path not in _libs:
+        if not os.path.exists(path) or os.path.getmtime(path) < os.path.getmtime(SRC):
             build()
-        L = ctypes.CDLL(LIB)
+        L = ctypes.CDLL(path)
         P = ctypes.POINTER
         L.ref_philox.argtypes = [_PU32, _PU32, _PU32]
         L.ref_stream_u01.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
@@ -164,8 +186,9 @@ def lib():
         L.ref_mg_coarse_inverse.argtypes = [_PI32, _PI32, _PD]
         L.ref_mg_solve.argtypes = [P(RefMG), _PD, _PD, _PI32, _PD]
         L.ref_mg_solve.restype = ctypes.c_int32
-        _lib = L
-    return _lib
+        L.ref_threads.restype = ctypes.c_int32
+        _libs[path] = L
+    return _libs[path]
 
 
 def _d(a):
