@@ -43,6 +43,7 @@ extern "C" {
 #define MCS_PROC_EXCITATION 2
 #define MCS_PROC_IONIZATION 3
 #define MCS_PROC_ATTACHMENT 4
+#define MCS_PROC_PHOTOEMISSION 5 /* electron.jl:88-105; parameters by mcs_table_set_photoemission */
 #define MCS_MAX_PROC 32
 
 /* Energy grid of a collision table. */
@@ -162,6 +163,16 @@ int32_t mcs_table_create(mcs_ctx *ctx, int32_t nproc, int32_t ne, int32_t egrid_
                          const double *energy, const double *rate, const int32_t *kind,
                          const double *param, double maxrate, mcs_table **out);
 int32_t mcs_table_destroy(mcs_table *t);
+/* PhotoEmission(numin, numax, photon_multiplier, photon_weight) (JuMC/src/electron.jl:
+ * 88-105) for row `row`, created with kind MCS_PROC_PHOTOEMISSION (its param is
+ * unused).  The row's collision (electron.jl:141-144, collisions.jl:125-141) leaves
+ * the electron as it is and emits Poisson((w / photon_multiplier) / photon_weight)
+ * photons at its position, each of weight photon_weight, velocity c randsphere()
+ * and absorption rate nu = exp(log numin + (log numax - log numin) U); the logs are
+ * taken here, as in the reference's constructor.  Every photoemission row must be
+ * set before a step uses the table; such a table needs mcs_advance_mixed. */
+int32_t mcs_table_set_photoemission(mcs_table *t, int32_t row, double numin, double numax,
+                                    double photon_multiplier, double photon_weight);
 
 /* ---- particle population: JuMC.Population (JuMC/src/population.jl:4-36) -----
  * Host layout of x and v is n x 3 row-major (Julia Vector{SVector{3,Float64}}). */
@@ -212,6 +223,46 @@ int32_t mcs_pop_energies(mcs_pop *p, int64_t nmax, int64_t *n, double *energy);
 int32_t mcs_advance(mcs_pop *p, const mcs_table *t, const mcs_field *field, double dt,
                     mcs_mesh *tracker, uint64_t seed, uint32_t instance, uint32_t step,
                     mcs_step_stats *stats);
+
+/* ---- photons: a population of PhotonState (JuMC/src/photon.jl:11-18) ----------
+ * x, v (|v| = c), nu (absorption rate, 1/s), w, s, active, id; host layout as
+ * mcs_pop_upload, nu one double per photon.  mcs_pop_size, mcs_repack,
+ * mcs_pop_destroy and mcs_pop_moments take a photon population (the moments'
+ * nparticles, nactive, weight and position/velocity sums; its energy fields and
+ * collrate are 0: a photon has no energy in the reference, photon.jl:22, and t
+ * may be NULL); mcs_pop_upload/download, the sorts, deposits and resampling do not. */
+int32_t mcs_pop_create_photons(mcs_ctx *ctx, int64_t capacity, mcs_pop **out);
+int32_t mcs_photons_upload(mcs_pop *p, int64_t n, const double *x, const double *v, const double *nu,
+                           const double *w, const double *s, const uint8_t *active,
+                           const uint64_t *id, uint64_t next_id);
+int32_t mcs_photons_download(mcs_pop *p, int64_t nmax, int64_t *n, double *x, double *v, double *nu,
+                             double *w, double *s, uint8_t *active, uint64_t *id, uint64_t *next_id);
+
+/* advance!(mpopl, efield, dt, tracker) over MultiPopulation(:electron => electrons,
+ * :photon => photons) (JuMC/src/mixed_population.jl:24-54; the populations in the
+ * order main.jl:105-122 builds them):
+ *  1. the electron loop of mcs_advance over the electrons present at the call; a
+ *     PhotoEmission collision appends its photons to `photons`, in the order of
+ *     the emitting electrons' slots, then of their collisions;
+ *  2. the photon loop of ZhelezniakCollisions(numax) (photon.jl:27-56) over every
+ *     photon present after the emission, the new ones included: free flights
+ *     x += t v, a collision every s/numax, absorbed when nu > numax U
+ *     (PhotoIonization): an electron at rest with the photon's weight and active
+ *     flag is appended to `electrons` (in photon order), the photon is removed,
+ *     and the tracker adds +w/dV at the photon's position (gridfields.jl:219-222).
+ * A removed particle keeps flying until dt is used up, as in the reference.  The
+ * tracker's additions of both loops reach each cell in that order.  Draws: the
+ * electron streams as in mcs_advance (the photon count from the emitting
+ * electron's stream), each new photon's direction, nu and s from a stream of its
+ * own id, each photon's flights from another.  estats describes loop 1 (births =
+ * ionization children); pstats loop 2: n_in photons before the call, n_out after
+ * the emission, births = electrons appended, deaths = photons removed, flights,
+ * events[0] = absorptions.  photons may be NULL when t has no photoemission row.
+ * FP64 electron populations only. */
+int32_t mcs_advance_mixed(mcs_pop *electrons, mcs_pop *photons, const mcs_table *t, double numax,
+                          const mcs_field *field, double dt, mcs_mesh *tracker, uint64_t seed,
+                          uint32_t instance, uint32_t step, mcs_step_stats *estats,
+                          mcs_step_stats *pstats);
 
 /* repack! (population.jl:191-215): last active particle into the first hole. */
 int32_t mcs_repack(mcs_pop *p);
@@ -410,6 +461,11 @@ int32_t mcs_cyl_resample(mcs_pop *p, mcs_cgrid *g, int32_t ntarget, int32_t nmax
  * FP64 populations only; otherwise as mcs_advance. */
 int32_t mcs_advance_cyl(mcs_pop *p, const mcs_table *t, mcs_cgrid *g, double dt, int32_t track,
                         uint64_t seed, uint32_t instance, uint32_t step, mcs_step_stats *stats);
+/* mcs_advance_mixed in the axisymmetric FieldInterp of g, tracking into g's QFIXED
+ * when track != 0. */
+int32_t mcs_advance_cyl_mixed(mcs_pop *electrons, mcs_pop *photons, const mcs_table *t, double numax,
+                              mcs_cgrid *g, double dt, int32_t track, uint64_t seed, uint32_t instance,
+                              uint32_t step, mcs_step_stats *estats, mcs_step_stats *pstats);
 
 #ifdef __cplusplus
 }

@@ -19,7 +19,7 @@ MCS_E_TABLE_RANGE = 5
 MCS_E_NONCONVERGED = 6
 MCS_E_NAN = 7
 
-PROC_NULL, PROC_ELASTIC, PROC_EXCITATION, PROC_IONIZATION, PROC_ATTACHMENT = range(5)
+PROC_NULL, PROC_ELASTIC, PROC_EXCITATION, PROC_IONIZATION, PROC_ATTACHMENT, PROC_PHOTOEMISSION = range(6)
 MAX_PROC = 32
 EGRID_UNIFORM, EGRID_LOG = 0, 1
 FIELD_UNIFORM, FIELD_MESH = 0, 1
@@ -216,6 +216,14 @@ SIGNATURES = {
     "mcs_cyl_resample": [_P, _P, _I32, _I32, _U64, _U32, _U32, ctypes.POINTER(ResampleStats)],
     "mcs_cyl_efield_faces": [_P, _D],
     "mcs_advance_cyl": [_P, _P, _P, _D, _I32, _U64, _U32, _U32, ctypes.POINTER(StepStats)],
+    "mcs_table_set_photoemission": [_P, _I32, _D, _D, _D, _D],
+    "mcs_pop_create_photons": [_P, _I64, _PP],
+    "mcs_photons_upload": [_P, _I64, _PD, _PD, _PD, _PD, _PD, _PU8, _PU64, _U64],
+    "mcs_photons_download": [_P, _I64, _PI64, _PD, _PD, _PD, _PD, _PD, _PU8, _PU64, _PU64],
+    "mcs_advance_mixed": [_P, _P, _P, _D, ctypes.POINTER(Field), _D, _P, _U64, _U32, _U32,
+                          ctypes.POINTER(StepStats), ctypes.POINTER(StepStats)],
+    "mcs_advance_cyl_mixed": [_P, _P, _P, _D, _P, _D, _I32, _U64, _U32, _U32, ctypes.POINTER(StepStats),
+                              ctypes.POINTER(StepStats)],
 }
 
 _lib = None

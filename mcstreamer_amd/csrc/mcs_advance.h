@@ -3,7 +3,15 @@
 // (mcs_swarm.hip).
 #pragma once
 
+#include <type_traits>
+
 #include "mcs_device.h"
+
+// The photoemission sink of a step without photons: never called (tables with a
+// photoemission row are refused before such a step starts).
+struct NoPhoto {
+    __device__ void operator()(int, const double *, uint8_t, int64_t) const {}
+};
 
 // track1 (MCStreamer/src/gridfields.jl:203-213): the NGP cell of x (flat index
 // into the ghosted cell-centred array) and the amount val/dV it receives; false
@@ -59,15 +67,20 @@ __device__ __forceinline__ void field_at(const MeshView &M, const double x[3], c
 // `birth(x, v, s, active)` receives each ionization child (weight w), `event(j)`
 // each selected table row, and with TRACK `track(x, +-w)` each ion-charge deposit
 // of the CollisionTracker (gridfields.jl:203-218: +w at an ionization, -w at an
-// attachment, at the particle's position).  A particle removed by attachment
-// keeps flying until dt is used up (the loop does not test `active`).
-template <int FIELD_MESH, int TRACK, typename Birth, typename Event, typename Track>
+// attachment, at the particle's position).  With a `photo` sink, a PhotoEmission
+// row (electron.jl:141-144) draws its photon count Poisson((w / multiplier) /
+// photon_weight) from the particle's stream and passes (row, x, active, count) to
+// `photo`; the electron is unchanged.  A particle removed by attachment keeps
+// flying until dt is used up (the loop does not test `active`).
+template <int FIELD_MESH, int TRACK, typename Birth, typename Event, typename Track, typename Photo = NoPhoto>
 __device__ __forceinline__ void advance_one(double x[3], double v[3], const double w, double &s,
                                             uint8_t &act, mcs_stream &rs, const double dt,
                                             const TableView &T, const MeshView &M,
                                             const double *euni, int32_t &flags, int &flights,
-                                            int &deaths, Birth &&birth, Event &&event, Track &&track)
+                                            int &deaths, Birth &&birth, Event &&event, Track &&track,
+                                            Photo &&photo = Photo())
 {
+    constexpr bool PHOTO = !std::is_same<typename std::decay<Photo>::type, NoPhoto>::value;
     const double maxrate = T.maxrate;
     double trem = dt;
     const double c1 = MCS_YOSHIDA_C1, c2 = MCS_YOSHIDA_C2;
@@ -168,6 +181,10 @@ __device__ __forceinline__ void advance_one(double x[3], double v[3], const doub
                     if (TRACK) track(x, -w);
                     if (act) deaths += 1;
                     act = 0;
+                } else if (PHOTO && kind == MCS_PROC_PHOTOEMISSION) {
+                    const double *ph = T.photo + 4 * sel;
+                    const int64_t np = mcs_poisson(&rs, (w / gld(ph + 2)) / gld(ph + 3));
+                    photo(sel, x, act, np);
                 }
             }
             s = mcs_nextcoll(&rs);

@@ -1,6 +1,7 @@
 // mcs_api.hip -- context, collision table, particle population and the
 // population reductions of libmcs.
 #include <atomic>
+#include <cmath>
 #include <cstdarg>
 #include <cstring>
 #include <map>
@@ -336,7 +337,7 @@ extern "C" int32_t mcs_table_create(mcs_ctx *ctx, int32_t nproc, int32_t ne, int
     if (egrid_kind == MCS_EGRID_LOG && !(energy[0] > 0.0))
         return mcs_fail(ctx, MCS_E_ARG, "table: log grid needs a positive first node");
     for (int j = 0; j < nproc; ++j)
-        if (kind[j] < MCS_PROC_NULL || kind[j] > MCS_PROC_ATTACHMENT)
+        if (kind[j] < MCS_PROC_NULL || kind[j] > MCS_PROC_PHOTOEMISSION)
             return mcs_fail(ctx, MCS_E_ARG, "table: process %d has unknown kind %d", j, kind[j]);
     hipSetDevice(ctx->device);
     mcs_table *t = new mcs_table();
@@ -352,7 +353,11 @@ extern "C" int32_t mcs_table_create(mcs_ctx *ctx, int32_t nproc, int32_t ne, int
                       ? (double)(ne - 1) / mcs_log(energy[ne - 1] / energy[0])
                       : 0.0;
     t->h_kind.assign(kind, kind + nproc);
+    t->h_photo.assign((size_t)nproc * 4, 0.0);
+    t->photo_set.assign((size_t)nproc, 0);
+    for (int j = 0; j < nproc; ++j) t->nphoto += kind[j] == MCS_PROC_PHOTOEMISSION;
     if (hipMalloc(&t->energy, sizeof(double) * ne) != hipSuccess ||
+        hipMalloc(&t->photo, sizeof(double) * 4 * (size_t)nproc) != hipSuccess ||
         hipMalloc(&t->rate, sizeof(double) * (size_t)ne * nproc) != hipSuccess ||
         hipMalloc(&t->kind, sizeof(int32_t) * nproc) != hipSuccess ||
         hipMalloc(&t->param, sizeof(double) * nproc) != hipSuccess) {
@@ -363,6 +368,7 @@ extern "C" int32_t mcs_table_create(mcs_ctx *ctx, int32_t nproc, int32_t ne, int
     hipMemcpy(t->rate, rate, sizeof(double) * (size_t)ne * nproc, hipMemcpyHostToDevice);
     hipMemcpy(t->kind, kind, sizeof(int32_t) * nproc, hipMemcpyHostToDevice);
     hipMemcpy(t->param, param, sizeof(double) * nproc, hipMemcpyHostToDevice);
+    hipMemcpy(t->photo, t->h_photo.data(), sizeof(double) * 4 * (size_t)nproc, hipMemcpyHostToDevice);
     {
         std::vector<float> e32(energy, energy + ne), r32(rate, rate + (size_t)ne * nproc),
             p32(param, param + nproc);
@@ -380,10 +386,33 @@ extern "C" int32_t mcs_table_create(mcs_ctx *ctx, int32_t nproc, int32_t ne, int
     return MCS_OK;
 }
 
+// PhotoEmission's constructor (electron.jl:101-104) takes the logs of the rates.
+extern "C" int32_t mcs_table_set_photoemission(mcs_table *t, int32_t row, double numin, double numax,
+                                               double photon_multiplier, double photon_weight)
+{
+    if (!t) return MCS_E_ARG;
+    mcs_ctx *ctx = t->ctx;
+    if (row < 0 || row >= t->nproc || t->h_kind[row] != MCS_PROC_PHOTOEMISSION)
+        return mcs_fail(ctx, MCS_E_ARG, "table: row %d is not a photoemission row", row);
+    if (!(numin > 0.0) || !(numax >= numin) || !(photon_multiplier > 0.0) || !(photon_weight > 0.0))
+        return mcs_fail(ctx, MCS_E_ARG,
+                        "table: photoemission needs 0 < numin <= numax and positive multiplier and weight");
+    double *r = &t->h_photo[(size_t)row * 4];
+    r[0] = std::log(numin);
+    r[1] = std::log(numax);
+    r[2] = photon_multiplier;
+    r[3] = photon_weight;
+    hipSetDevice(ctx->device);
+    MCS_HIP(ctx, hipMemcpy(t->photo + (size_t)row * 4, r, sizeof(double) * 4, hipMemcpyHostToDevice));
+    t->photo_set[row] = 1;
+    return MCS_OK;
+}
+
 extern "C" int32_t mcs_table_destroy(mcs_table *t)
 {
     if (!t) return MCS_OK;
     hipSetDevice(t->ctx->device);
+    hipFree(t->photo);
     hipFree(t->energy);
     hipFree(t->rate);
     hipFree(t->kind);
@@ -397,8 +426,9 @@ extern "C" int32_t mcs_table_destroy(mcs_table *t)
 
 // ---------------------------------------------------------------- population
 
-static int32_t alloc_buf(mcs_ctx *ctx, PopBuf &b, int64_t cap)
+static int32_t alloc_buf(mcs_ctx *ctx, PopBuf &b, int64_t cap, bool photon = false)
 {
+    if (photon) MCS_HIP(ctx, hipMalloc(&b.nu, sizeof(double) * cap));
     for (int d = 0; d < 3; ++d) {
         MCS_HIP(ctx, hipMalloc(&b.x[d], sizeof(double) * cap));
         MCS_HIP(ctx, hipMalloc(&b.v[d], sizeof(double) * cap));
@@ -412,6 +442,7 @@ static int32_t alloc_buf(mcs_ctx *ctx, PopBuf &b, int64_t cap)
 
 static void free_buf(PopBuf &b)
 {
+    hipFree(b.nu);
     for (int d = 0; d < 3; ++d) {
         hipFree(b.x[d]);
         hipFree(b.v[d]);
@@ -546,7 +577,7 @@ extern "C" int32_t mcs_pop_create_split(mcs_ctx *ctx, int64_t capacity, const mc
     return MCS_OK;
 }
 
-extern "C" int32_t mcs_pop_create(mcs_ctx *ctx, int64_t capacity, mcs_pop **out)
+static int32_t pop_create_fp64(mcs_ctx *ctx, int64_t capacity, int species, mcs_pop **out)
 {
     if (!ctx || !out) return MCS_E_ARG;
     if (capacity < 1 || capacity >= (int64_t)1 << 31)
@@ -557,8 +588,10 @@ extern "C" int32_t mcs_pop_create(mcs_ctx *ctx, int64_t capacity, mcs_pop **out)
     p->cap = capacity;
     p->n = 0;
     p->next_id = 0;
-    int32_t rc = alloc_buf(ctx, p->buf[0], capacity);
+    p->species = species;
+    int32_t rc = alloc_buf(ctx, p->buf[0], capacity, species == MCS_SPECIES_PHOTON);
     if (rc) {
+        free_buf(p->buf[0]);
         delete p;
         return rc;
     }
@@ -581,6 +614,46 @@ extern "C" int32_t mcs_pop_create(mcs_ctx *ctx, int64_t capacity, mcs_pop **out)
             return mcs_fail(ctx, MCS_E_HIP, "population: device allocation failed");
     }
     *out = p;
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_pop_create(mcs_ctx *ctx, int64_t capacity, mcs_pop **out)
+{
+    return pop_create_fp64(ctx, capacity, MCS_SPECIES_ELECTRON, out);
+}
+
+extern "C" int32_t mcs_pop_create_photons(mcs_ctx *ctx, int64_t capacity, mcs_pop **out)
+{
+    return pop_create_fp64(ctx, capacity, MCS_SPECIES_PHOTON, out);
+}
+
+int32_t mcs_pop_ensure_bursts(mcs_pop *p)
+{
+    if (p->bur_parent) return MCS_OK;
+    const int64_t nblk = cdiv(p->cap, MCS_BLOCK) + 2;
+    bool ok = hipMalloc(&p->bur_parent, sizeof(int32_t) * p->logcap) == hipSuccess &&
+              hipMalloc(&p->bur_row, sizeof(int32_t) * p->logcap) == hipSuccess &&
+              hipMalloc(&p->bur_base, sizeof(int64_t) * p->logcap) == hipSuccess &&
+              hipMalloc(&p->bur_n, sizeof(int64_t) * p->logcap) == hipSuccess &&
+              hipMalloc(&p->bur_act, p->logcap) == hipSuccess &&
+              hipMalloc(&p->poff, sizeof(int64_t) * (p->cap + 1)) == hipSuccess &&
+              hipMalloc(&p->pblk, sizeof(int64_t) * (2 * nblk + 4)) == hipSuccess;
+    for (int d = 0; d < 3 && ok; ++d) ok = hipMalloc(&p->bur_x[d], sizeof(double) * p->logcap) == hipSuccess;
+    if (!ok) {
+        hipFree(p->bur_parent);
+        hipFree(p->bur_row);
+        hipFree(p->bur_base);
+        hipFree(p->bur_n);
+        hipFree(p->bur_act);
+        hipFree(p->poff);
+        hipFree(p->pblk);
+        for (int d = 0; d < 3; ++d) hipFree(p->bur_x[d]);
+        p->bur_parent = p->bur_row = nullptr;
+        p->bur_base = p->bur_n = p->poff = p->pblk = nullptr;
+        p->bur_act = nullptr;
+        for (int d = 0; d < 3; ++d) p->bur_x[d] = nullptr;
+        return mcs_fail(p->ctx, MCS_E_HIP, "population: photon burst log allocation failed");
+    }
     return MCS_OK;
 }
 
@@ -610,7 +683,7 @@ int32_t mcs_pop_ensure_alt(mcs_pop *p)
 {
     if (p->has_alt) return MCS_OK;
     int32_t rc = p->prec == MCS_PREC_F32 ? alloc_split(p->ctx, p->sbuf[1], p->cap)
-                                         : alloc_buf(p->ctx, p->buf[1], p->cap);
+                                         : alloc_buf(p->ctx, p->buf[1], p->cap, p->species == MCS_SPECIES_PHOTON);
     if (rc) return rc;
     p->has_alt = true;
     return MCS_OK;
@@ -672,15 +745,49 @@ extern "C" int32_t mcs_pop_destroy(mcs_pop *p)
     hipFree(p->trk_cell);
     hipFree(p->trk_val);
     hipFree(p->tblk);
+    hipFree(p->bur_parent);
+    hipFree(p->bur_row);
+    hipFree(p->bur_base);
+    hipFree(p->bur_n);
+    hipFree(p->bur_act);
+    for (int d = 0; d < 3; ++d) hipFree(p->bur_x[d]);
+    hipFree(p->poff);
+    hipFree(p->pblk);
     delete p;
     return MCS_OK;
 }
+
+static int32_t upload_impl(mcs_pop *p, int64_t n, const double *x, const double *v, const double *w,
+                           const double *s, const uint8_t *active, const uint64_t *id, uint64_t next_id);
 
 extern "C" int32_t mcs_pop_upload(mcs_pop *p, int64_t n, const double *x, const double *v,
                                   const double *w, const double *s, const uint8_t *active,
                                   const uint64_t *id, uint64_t next_id)
 {
     if (!p) return MCS_E_ARG;
+    if (p->species == MCS_SPECIES_PHOTON)
+        return mcs_fail(p->ctx, MCS_E_ARG, "pop_upload: a photon population takes mcs_photons_upload");
+    return upload_impl(p, n, x, v, w, s, active, id, next_id);
+}
+
+extern "C" int32_t mcs_photons_upload(mcs_pop *p, int64_t n, const double *x, const double *v, const double *nu,
+                                      const double *w, const double *s, const uint8_t *active,
+                                      const uint64_t *id, uint64_t next_id)
+{
+    if (!p) return MCS_E_ARG;
+    mcs_ctx *ctx = p->ctx;
+    if (p->species != MCS_SPECIES_PHOTON)
+        return mcs_fail(ctx, MCS_E_ARG, "photons_upload: not a photon population");
+    if (n > 0 && !nu) return mcs_fail(ctx, MCS_E_ARG, "photons_upload: null array");
+    int32_t rc = upload_impl(p, n, x, v, w, s, active, id, next_id);
+    if (rc) return rc;
+    if (n > 0) MCS_HIP(ctx, hipMemcpy(p->buf[0].nu, nu, sizeof(double) * n, hipMemcpyHostToDevice));
+    return MCS_OK;
+}
+
+static int32_t upload_impl(mcs_pop *p, int64_t n, const double *x, const double *v, const double *w,
+                           const double *s, const uint8_t *active, const uint64_t *id, uint64_t next_id)
+{
     mcs_ctx *ctx = p->ctx;
     if (n < 0 || n > p->cap)
         return mcs_fail(ctx, MCS_E_CAPACITY, "Maximum number of particles reached (%lld > %lld)",
@@ -739,11 +846,37 @@ extern "C" int32_t mcs_pop_upload(mcs_pop *p, int64_t n, const double *x, const 
     return MCS_OK;
 }
 
+static int32_t download_impl(mcs_pop *p, int64_t nmax, int64_t *nout, double *x, double *v, double *w,
+                             double *s, uint8_t *active, uint64_t *id, uint64_t *next_id);
+
 extern "C" int32_t mcs_pop_download(mcs_pop *p, int64_t nmax, int64_t *nout, double *x,
                                     double *v, double *w, double *s, uint8_t *active,
                                     uint64_t *id, uint64_t *next_id)
 {
     if (!p) return MCS_E_ARG;
+    if (p->species == MCS_SPECIES_PHOTON)
+        return mcs_fail(p->ctx, MCS_E_ARG, "pop_download: a photon population takes mcs_photons_download");
+    return download_impl(p, nmax, nout, x, v, w, s, active, id, next_id);
+}
+
+extern "C" int32_t mcs_photons_download(mcs_pop *p, int64_t nmax, int64_t *nout, double *x, double *v,
+                                        double *nu, double *w, double *s, uint8_t *active, uint64_t *id,
+                                        uint64_t *next_id)
+{
+    if (!p) return MCS_E_ARG;
+    mcs_ctx *ctx = p->ctx;
+    if (p->species != MCS_SPECIES_PHOTON)
+        return mcs_fail(ctx, MCS_E_ARG, "photons_download: not a photon population");
+    int32_t rc = download_impl(p, nmax, nout, x, v, w, s, active, id, next_id);
+    if (rc) return rc;
+    const int64_t n = p->n < nmax ? p->n : nmax;
+    if (n > 0 && nu) MCS_HIP(ctx, hipMemcpy(nu, p->buf[0].nu, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return MCS_OK;
+}
+
+static int32_t download_impl(mcs_pop *p, int64_t nmax, int64_t *nout, double *x, double *v, double *w,
+                             double *s, uint8_t *active, uint64_t *id, uint64_t *next_id)
+{
     mcs_ctx *ctx = p->ctx;
     hipSetDevice(ctx->device);
     MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1039,6 +1172,11 @@ static int32_t moments_run(mcs_pop *p, const mcs_table *t, mcs_moments *out, int
     }
     out->maxenergy = h[MOM_NV];
     out->nactive = (int64_t)h[MOM_NV + 1];
+    if (p->species == MCS_SPECIES_PHOTON) {
+        // photons carry no energy (photon.jl:22) and no collision table rates
+        out->wenergy = out->sum_energy = out->maxenergy = 0.0;
+        return MCS_OK;
+    }
     if (t) {
         for (int j0 = 0; j0 < t->nproc; j0 += CR_CHUNK) {
             if (p->prec == MCS_PREC_F32)

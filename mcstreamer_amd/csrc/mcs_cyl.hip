@@ -447,6 +447,7 @@ extern "C" int32_t mcs_cgrid_zero(mcs_cgrid *g, int32_t which)
 extern "C" int32_t mcs_cyl_density(mcs_pop *p, mcs_cgrid *g, int32_t which, double val)
 {
     if (!p || !g || (which != MCS_CARR_QFIXED && which != MCS_CARR_QPART)) return MCS_E_ARG;
+    MCS_ELECTRONS_ONLY(p, "cyl_density");
     mcs_ctx *ctx = p->ctx;
     if (g->ctx != ctx) return mcs_fail(ctx, MCS_E_ARG, "cyl_density: grid of another context");
     if (p->prec != MCS_PREC_F64) return mcs_fail(ctx, MCS_E_ARG, "cyl_density: FP64 populations only");

@@ -16,6 +16,7 @@ struct TableView {
     const double *rate;
     const int32_t *kind;
     const double *param;
+    const double *photo;   // 4 per row: log numin, log numax, photon_multiplier, photon_weight
     int32_t nproc, ne, egrid;
     double step, inv_dlog, maxrate;
 };
@@ -46,6 +47,7 @@ static inline TableView table_view(const mcs_table *t)
     v.rate = t->rate;
     v.kind = t->kind;
     v.param = t->param;
+    v.photo = t->photo;
     v.nproc = t->nproc;
     v.ne = t->ne;
     v.egrid = t->egrid;
@@ -324,6 +326,7 @@ __device__ __forceinline__ void copy_particle(const PopBuf &src, int64_t si, con
     dst.s[di] = src.s[si];
     dst.active[di] = src.active[si];
     dst.id[di] = src.id[si];
+    if (src.nu) dst.nu[di] = src.nu[si];
 }
 
 // cell index of a position (grid.jl:84-90 generalised): -1 outside [0, n h)^3

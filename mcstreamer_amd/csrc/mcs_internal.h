@@ -128,6 +128,13 @@ struct mcs_table {
     // FP32 copies for the split-precision kernels
     float *energy32 = nullptr, *rate32 = nullptr, *param32 = nullptr;
     std::vector<int32_t> h_kind;
+    // PhotoEmission rows: (log numin, log numax, photon_multiplier, photon_weight)
+    // per row (nproc x 4, device and host), which rows are set, how many rows
+    // are photoemission rows
+    double *photo = nullptr;
+    std::vector<double> h_photo;
+    std::vector<char> photo_set;
+    int32_t nphoto = 0;
 };
 
 // Particle SoA in HBM.  Buffer set 0 is live; set 1 receives sort/resample output.
@@ -137,6 +144,7 @@ struct PopBuf {
     double *w = nullptr, *s = nullptr;
     uint8_t *active = nullptr;
     uint64_t *id = nullptr;
+    double *nu = nullptr;   // photon populations: absorption rate
 };
 
 // Split-precision SoA (precision mode): packed cell coordinates (i in bits 0-9,
@@ -191,7 +199,29 @@ struct mcs_pop {
     int32_t *trk_parent = nullptr, *trk_seq = nullptr, *trk_cell = nullptr;
     double *trk_val = nullptr;
     int64_t *tblk = nullptr;
+    // MCS_SPECIES_PHOTON: a photon population (buf[].nu allocated)
+    int species = 0;
+    // photon bursts of a step's PhotoEmission collisions (logcap each): parent
+    // slot, the parent's photons before the burst, the burst's photon count, table
+    // row, active flag and position; per-slot photon offsets within the block and
+    // per-block photon totals and their prefix.  Allocated on the first step
+    // that emits.
+    int32_t *bur_parent = nullptr, *bur_row = nullptr;
+    int64_t *bur_base = nullptr, *bur_n = nullptr;
+    uint8_t *bur_act = nullptr;
+    double *bur_x[3] = {nullptr, nullptr, nullptr};
+    int64_t *poff = nullptr, *pblk = nullptr;
 };
+
+#define MCS_SPECIES_ELECTRON 0
+#define MCS_SPECIES_PHOTON 1
+
+// Entry points of electron populations refuse a photon population.
+#define MCS_ELECTRONS_ONLY(p, fn)                                                            \
+    do {                                                                                    \
+        if ((p)->species == MCS_SPECIES_PHOTON)                                             \
+            return mcs_fail((p)->ctx, MCS_E_ARG, fn ": not for a photon population");       \
+    } while (0)
 
 struct mcs_mesh {
     mcs_ctx *ctx;
@@ -314,6 +344,8 @@ int32_t mcs_scan_i64_to(mcs_ctx *ctx, const int64_t *in, int64_t *out, int64_t l
 int32_t mcs_pop_ensure_alt(mcs_pop *p);
 // the tracker's deposit log (allocated once)
 int32_t mcs_pop_ensure_track(mcs_pop *p);
+// the photon-burst log of an emitting electron population (allocated once)
+int32_t mcs_pop_ensure_bursts(mcs_pop *p);
 // Gives p a fresh order serial (unique over all populations of the process).
 void mcs_pop_touch(mcs_pop *p);
 // Whether cell starts recorded at serial `gen` describe p's current order.

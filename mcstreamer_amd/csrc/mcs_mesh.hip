@@ -492,6 +492,7 @@ static int32_t sort_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, const Buf &dst
 extern "C" int32_t mcs_sort_by_cell(mcs_pop *p, mcs_mesh *m)
 {
     if (!p || !m) return MCS_E_ARG;
+    MCS_ELECTRONS_ONLY(p, "sort_by_cell");
     mcs_ctx *ctx = p->ctx;
     hipSetDevice(ctx->device);
     int32_t rc = mcs_pop_ensure_alt(p);
@@ -823,6 +824,7 @@ static int32_t deposit_impl(mcs_pop *p, mcs_mesh *m, const Buf &src, int32_t whi
 extern "C" int32_t mcs_deposit(mcs_pop *p, mcs_mesh *m, int32_t which, int32_t scheme, double val)
 {
     if (!p || !m) return MCS_E_ARG;
+    MCS_ELECTRONS_ONLY(p, "deposit");
     mcs_ctx *ctx = p->ctx;
     if (which != MCS_ARR_QFIXED && which != MCS_ARR_QPART)
         return mcs_fail(ctx, MCS_E_ARG, "deposit: target must be QFIXED or QPART");
@@ -1178,6 +1180,7 @@ extern "C" int32_t mcs_resample_cells(mcs_pop *p, mcs_mesh *m, int32_t ntarget, 
                                       mcs_resample_stats *stats)
 {
     if (!p || !m) return MCS_E_ARG;
+    MCS_ELECTRONS_ONLY(p, "resample_cells");
     mcs_ctx *ctx = p->ctx;
     if (ntarget < 1 || nmax < ntarget)
         return mcs_fail(ctx, MCS_E_ARG, "resample: need 1 <= ntarget <= nmax");
@@ -1356,6 +1359,7 @@ static int32_t cgrid_lists(mcs_cgrid *g)
 extern "C" int32_t mcs_cyl_sort_by_cell(mcs_pop *p, mcs_cgrid *g)
 {
     if (!p || !g) return MCS_E_ARG;
+    MCS_ELECTRONS_ONLY(p, "cyl_sort_by_cell");
     mcs_ctx *ctx = p->ctx;
     if (g->ctx != ctx) return mcs_fail(ctx, MCS_E_ARG, "cyl_sort: grid of another context");
     if (p->prec != MCS_PREC_F64) return mcs_fail(ctx, MCS_E_ARG, "cyl_sort: FP64 populations only");
@@ -1429,6 +1433,7 @@ extern "C" int32_t mcs_cyl_resample(mcs_pop *p, mcs_cgrid *g, int32_t ntarget, i
                                     uint32_t instance, uint32_t step, mcs_resample_stats *stats)
 {
     if (!p || !g) return MCS_E_ARG;
+    MCS_ELECTRONS_ONLY(p, "cyl_resample");
     mcs_ctx *ctx = p->ctx;
     if (ntarget < 1 || nmax < ntarget) return mcs_fail(ctx, MCS_E_ARG, "resample: need 1 <= ntarget <= nmax");
     if (!mcs_pop_sorted(p, g->sorted_gen))

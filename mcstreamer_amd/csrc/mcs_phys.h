@@ -17,6 +17,7 @@
 #define MCS_EPSILON_0 8.854187817620389e-12
 #define MCS_TD 1e-21
 #define MCS_NAIR 2.6867811e+25
+#define MCS_C 299792458.0   /* speed of light, Constants.jl:39; the photon speed (collisions.jl:136) */
 
 /* charge(p)/mass(p) of an electron, (-e)/m_e rounded once. */
 #define MCS_QM (-175882002360.2124)

@@ -49,6 +49,8 @@
 #define MCS_PHASE_SWARM_RESAMPLE 1u
 #define MCS_PHASE_CELL_RESAMPLE 2u
 #define MCS_PHASE_INIT 3u
+#define MCS_PHASE_PHOTON 4u   /* a photon's flights (ZhelezniakCollisions) */
+#define MCS_PHASE_EMIT 5u     /* a new photon's direction, nu and s */
 
 MCS_FN void mcs_philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                               uint32_t k0, uint32_t k1, uint32_t out[4])
@@ -314,6 +316,25 @@ MCS_FN void mcs_maxwell_init(uint64_t seed, uint32_t instance, uint64_t id, doub
     r = sqrt(-2.0 * mcs_log(1.0 - u3));
     mcs_sincos2pi(u4, &sn, &cs);
     v[2] = vth * (r * cs);
+    *s = mcs_nextcoll(&st);
+}
+
+/* One photon of a PhotoEmission collision (JuMC/src/collisions.jl:134-139):
+ * v = c randsphere(), nu = exp(lmin + (lmax - lmin) rand()), then s = nextcoll(),
+ * drawn in that order from phase EMIT of the photon's own id.  c is the speed of
+ * light (299792458 m/s). */
+MCS_FN void mcs_photon_emit(uint64_t seed, uint32_t instance, uint64_t id, uint32_t step, double lmin,
+                            double lmax, double v[3], double *nu, double *s)
+{
+    mcs_stream st;
+    mcs_stream_init(&st, seed, instance, MCS_PHASE_EMIT, id, step);
+    double om[3];
+    mcs_randsphere(&st, om);
+    v[0] = om[0] * 299792458.0;
+    v[1] = om[1] * 299792458.0;
+    v[2] = om[2] * 299792458.0;
+    double u = mcs_u01(&st);
+    *nu = mcs_exp(lmin + (lmax - lmin) * u);
     *s = mcs_nextcoll(&st);
 }
 

@@ -3,6 +3,7 @@
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "mcs_advance.h"
 #include "mcs_fp32.h"
@@ -16,6 +17,8 @@
 #define C_BIRTHS 5   // births (the births' scan total)
 #define C_TTOT 6     // tracker deposits (their scan total)
 #define C_EVENTS 8
+#define C_BLOG (C_EVENTS + MCS_MAX_PROC)   // photon bursts logged
+#define C_PTOT (C_BLOG + 1)                // photons emitted (their scan total)
 
 struct BirthLog {
     int32_t *parent, *seq;
@@ -94,6 +97,35 @@ __device__ __forceinline__ void track_block_counts(const TrackLog &TL, int64_t i
     if (threadIdx.x == 0) TL.tbtot[xcd_block()] = ttot;
 }
 
+// The photon bursts of a step's PhotoEmission collisions: each burst with at
+// least one photon is logged under its parent slot with the parent's photons
+// before it (base); per-thread photon totals give, after a block scan (poff, per
+// slot) and a scan of the block totals (pbtot), every burst its first photon's
+// place in parent order.
+struct BurstLog {
+    int32_t *parent, *row;
+    int64_t *base, *n;
+    uint8_t *act;
+    double *x[3];
+    int64_t cap;
+    int64_t *poff, *pbtot;
+};
+
+static inline BurstLog burst_log(mcs_pop *p)
+{
+    BurstLog B;
+    B.parent = p->bur_parent;
+    B.row = p->bur_row;
+    B.base = p->bur_base;
+    B.n = p->bur_n;
+    B.act = p->bur_act;
+    for (int d = 0; d < 3; ++d) B.x[d] = p->bur_x[d];
+    B.cap = p->logcap;
+    B.poff = p->poff;
+    B.pbtot = p->pblk;
+    return B;
+}
+
 struct AdvArgs {
     PopBuf b;
     int64_t n0;
@@ -117,6 +149,117 @@ struct AdvArgs {
 // collisions.jl:159-192 and the outcomes of electron.jl:108-139.  Children go to a
 // birth log tagged (parent, ordinal); k_place writes them after the parents in
 // parent order, which is the append order of the single-threaded reference.
+// With PHOTO, PhotoEmission collisions log photon bursts (BurstLog).  k_advance
+// below keeps its own copy of this body without the photon sink: instantiated
+// through this function it needs 120 bytes of scratch per lane instead of 56
+// (the compiler's resource-usage report), which its performance cannot spare.
+template <int FIELD_MESH, int TRACK, int PHOTO>
+__device__ __forceinline__ void advance_body(const AdvArgs &a, const BurstLog &BL)
+{
+    __shared__ int s_ev[MCS_MAX_PROC];
+    __shared__ int s_flights, s_deaths, s_flags;
+    if (threadIdx.x < MCS_MAX_PROC) s_ev[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        s_flights = 0;
+        s_deaths = 0;
+        s_flags = 0;
+    }
+    __syncthreads();
+
+    const int64_t i = xcd_gidx();
+    int nb = 0, nt = 0;
+    int64_t nph = 0;
+    int32_t flags = 0;
+    int flights = 0, deaths = 0;
+    if (i < a.n0 && a.b.active[i]) {
+        double x[3] = {a.b.x[0][i], a.b.x[1][i], a.b.x[2][i]};
+        double v[3] = {a.b.v[0][i], a.b.v[1][i], a.b.v[2][i]};
+        const double w = a.b.w[i];
+        double s = a.b.s[i];
+        uint8_t act = 1;
+        mcs_stream rs;
+        mcs_stream_init(&rs, a.seed, a.instance, MCS_PHASE_ADVANCE, a.b.id[i], a.step);
+        auto birth = [&](const double *bx, const double *bv, double bs, uint8_t bact) {
+            unsigned long long slot = atomicAdd((unsigned long long *)&a.ctr[C_LOG], 1ULL);
+            if ((int64_t)slot < a.L.cap) {
+                a.L.parent[slot] = (int32_t)i;
+                a.L.seq[slot] = nb;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    a.L.x[d][slot] = bx[d];
+                    a.L.v[d][slot] = bv[d];
+                }
+                a.L.w[slot] = w;
+                a.L.s[slot] = bs;
+                a.L.act[slot] = bact;
+            } else {
+                flags |= MCS_FLAG_CAPACITY;
+            }
+            nb += 1;
+        };
+        auto event = [&](int j) { atomicAdd(&s_ev[j], 1); };
+        auto track = [&](const double *tx, double val) {
+            int32_t cell;
+            double amount;
+            if (track_cell<FIELD_MESH>(a.M, tx, val, cell, amount))
+                track_push(a.TL, a.ctr, i, nt, cell, amount, flags);
+        };
+        if constexpr (PHOTO) {
+            auto photo = [&](int row, const double *px, uint8_t pact, int64_t np) {
+                if (np <= 0) return;
+                const unsigned long long slot = atomicAdd((unsigned long long *)&a.ctr[C_BLOG], 1ULL);
+                if ((int64_t)slot < BL.cap) {
+                    BL.parent[slot] = (int32_t)i;
+                    BL.row[slot] = row;
+                    BL.base[slot] = nph;
+                    BL.n[slot] = np;
+                    BL.act[slot] = pact;
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) BL.x[d][slot] = px[d];
+                } else {
+                    flags |= MCS_FLAG_CAPACITY;
+                }
+                nph += np;
+            };
+            advance_one<FIELD_MESH, TRACK>(x, v, w, s, act, rs, a.dt, a.T, a.M, a.e, flags, flights, deaths,
+                                           birth, event, track, photo);
+        } else {
+            advance_one<FIELD_MESH, TRACK>(x, v, w, s, act, rs, a.dt, a.T, a.M, a.e, flags, flights, deaths,
+                                           birth, event, track);
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            a.b.x[d][i] = x[d];
+            a.b.v[d][i] = v[d];
+        }
+        a.b.s[i] = s;
+        if (!act) a.b.active[i] = 0;
+    }
+    int tot;
+    int ex = block_excl_scan<int>(nb, &tot);
+    if (nb > 0) a.loff[i] = ex;
+    if (threadIdx.x == 0) a.btot[xcd_block()] = tot;
+    if (TRACK) track_block_counts(a.TL, i, nt);
+    if constexpr (PHOTO) {
+        int64_t ptot;
+        const int64_t pex = block_excl_scan<int64_t>(nph, &ptot);
+        if (nph > 0) BL.poff[i] = pex;
+        if (threadIdx.x == 0) BL.pbtot[xcd_block()] = ptot;
+    }
+    if (flights) atomicAdd(&s_flights, flights);
+    if (deaths) atomicAdd(&s_deaths, deaths);
+    if (flags) atomicOr(&s_flags, flags);
+    __syncthreads();
+    if (threadIdx.x < a.T.nproc && s_ev[threadIdx.x])
+        atomicAdd((unsigned long long *)&a.ctr[C_EVENTS + threadIdx.x],
+                  (unsigned long long)s_ev[threadIdx.x]);
+    if (threadIdx.x == 0) {
+        if (s_flights) atomicAdd((unsigned long long *)&a.ctr[C_FLIGHTS], (unsigned long long)s_flights);
+        if (s_deaths) atomicAdd((unsigned long long *)&a.ctr[C_DEATHS], (unsigned long long)s_deaths);
+        if (s_flags) atomicOr((unsigned long long *)&a.ctr[C_FLAGS], (unsigned long long)s_flags);
+    }
+}
+
 // Five waves per SIMD (96 registers; a few cold values spill): the field gathers'
 // latency is hidden better than at the four that 106 registers allow.
 template <int FIELD_MESH, int TRACK>
@@ -196,6 +339,13 @@ __global__ __launch_bounds__(MCS_BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8)
         if (s_deaths) atomicAdd((unsigned long long *)&a.ctr[C_DEATHS], (unsigned long long)s_deaths);
         if (s_flags) atomicOr((unsigned long long *)&a.ctr[C_FLAGS], (unsigned long long)s_flags);
     }
+}
+
+// The same step for a table with PhotoEmission rows.
+template <int FIELD_MESH, int TRACK>
+__global__ __launch_bounds__(MCS_BLOCK) void k_advance_photo(AdvArgs a, BurstLog BL)
+{
+    advance_body<FIELD_MESH, TRACK, 1>(a, BL);
 }
 
 // Children land at n0 + (births of earlier blocks) + (births of earlier threads in
@@ -498,11 +648,105 @@ static int32_t track_apply(mcs_pop *p, const TrackTarget &tt, int64_t count, con
     return MCS_OK;
 }
 
-// After the step kernel: the births' prefix over blocks (and the tracker
-// deposits'), the counters and flags in one read-back, then the children placed
-// after the parents and the deposits added to the tracker's array.
-static int32_t advance_finish(mcs_pop *p, const mcs_table *t, int64_t n0, int64_t nblk, int64_t *btot,
-                              int64_t *bpre, int64_t *ctr, const TrackTarget &tt, mcs_step_stats *stats)
+// Photon emission of an electron step: the photons of the logged bursts go to
+// `photons`, drawn from streams of their own ids.
+struct EmitArgs {
+    mcs_pop *photons = nullptr;
+    const double *photo = nullptr;   // the table's photoemission parameters, 4 per row
+    uint64_t seed = 0;
+    uint32_t instance = 0, step = 0;
+};
+
+__global__ void k_zero_u64(uint64_t *a, int64_t n)
+{
+    const int64_t i = gidx();
+    if (i < n) a[i] = 0;
+}
+
+// Each logged burst (all hold at least one photon, so their places differ) marks
+// the place of its first photon among the step's photons with (place << 32) | slot.
+__global__ __launch_bounds__(MCS_BLOCK) void k_burst_mark(BurstLog BL, int64_t count, const int64_t *ppre,
+                                                          uint64_t *marks)
+{
+    const int64_t r = gidx();
+    if (r >= count) return;
+    const int32_t par = BL.parent[r];
+    const int64_t start = ppre[par / MCS_BLOCK] + BL.poff[par] + BL.base[r];
+    marks[start] = ((uint64_t)start << 32) | (uint64_t)r;
+}
+
+// Photon q of the step, whose burst the running maximum of the marks names, at
+// slot n0 + q with id next_id + q: c randsphere(), nu and s from the stream of its
+// id (collisions.jl:134-139), weight photon_weight, the burst's position and
+// active flag.
+__global__ __launch_bounds__(MCS_BLOCK) void k_place_photons(const uint64_t *owner, int64_t total, BurstLog BL,
+                                                             const double *photo, int64_t n0, uint64_t next_id,
+                                                             PopBuf b, uint64_t seed, uint32_t instance,
+                                                             uint32_t step)
+{
+    const int64_t q = gidx();
+    if (q >= total) return;
+    const uint32_t r = (uint32_t)(owner[q] & 0xffffffffull);
+    const int32_t row = BL.row[r];
+    const double lmin = photo[4 * row], lmax = photo[4 * row + 1], pw = photo[4 * row + 3];
+    const int64_t dst = n0 + q;
+    const uint64_t id = next_id + (uint64_t)q;
+    double v[3], nu, s;
+    mcs_photon_emit(seed, instance, id, step, lmin, lmax, v, &nu, &s);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        b.x[d][dst] = BL.x[d][r];
+        b.v[d][dst] = v[d];
+    }
+    b.nu[dst] = nu;
+    b.w[dst] = pw;
+    b.s[dst] = s;
+    b.active[dst] = BL.act[r];
+    b.id[dst] = id;
+}
+
+// The step's `total` photons from its `nbur` logged bursts, appended to the
+// photon population in parent order (apply! of MultiplePhotonOutcome,
+// collisions.jl:125-141).
+static int32_t emit_photons(mcs_pop *p, const EmitArgs &em, int64_t nbur, int64_t total, const int64_t *ppre)
+{
+    mcs_ctx *ctx = p->ctx;
+    mcs_pop *ph = em.photons;
+    ProfScope ps(ctx, "photon_emit");
+    size_t tb = 0;
+    uint64_t *none = nullptr;
+    MCS_HIP(ctx, rocprim::inclusive_scan(nullptr, tb, none, none, (size_t)total, rocprim::maximum<uint64_t>(),
+                                         ctx->stream));
+    void *scr = nullptr;
+    int32_t rc = mcs_scratch(ctx, 2 * sizeof(uint64_t) * (size_t)total + tb + 256, &scr);
+    if (rc) return rc;
+    uint64_t *marks = (uint64_t *)scr, *owner = marks + total;
+    void *tmp = (void *)(((uintptr_t)(owner + total) + 255) & ~(uintptr_t)255);
+    hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)cdiv(total, MCS_BLOCK)), dim3(MCS_BLOCK), 0, ctx->stream, marks,
+                       total);
+    hipLaunchKernelGGL(k_burst_mark, dim3((unsigned)cdiv(nbur, MCS_BLOCK)), dim3(MCS_BLOCK), 0, ctx->stream,
+                       burst_log(p), nbur, ppre, marks);
+    MCS_LAUNCH_CHECK(ctx);
+    MCS_HIP(ctx, rocprim::inclusive_scan(tmp, tb, marks, owner, (size_t)total, rocprim::maximum<uint64_t>(),
+                                         ctx->stream));
+    hipLaunchKernelGGL(k_place_photons, dim3((unsigned)cdiv(total, MCS_BLOCK)), dim3(MCS_BLOCK), 0, ctx->stream,
+                       (const uint64_t *)owner, total, burst_log(p), em.photo, ph->n, ph->next_id, ph->buf[0],
+                       em.seed, em.instance, em.step);
+    MCS_LAUNCH_CHECK(ctx);
+    ph->n += total;
+    ph->next_id += (uint64_t)total;
+    mcs_pop_touch(ph);
+    return MCS_OK;
+}
+
+// After the step kernel of population p: the births' prefix over blocks (and the
+// tracker deposits', and the photons'), the counters and flags in one read-back,
+// then the children placed after the particles of `dst` (p itself, or the
+// electrons for the photon loop), the photons emitted and the deposits added to
+// the tracker's array.  nev table rows are counted.
+static int32_t advance_finish(mcs_pop *p, mcs_pop *dst, int nev, int64_t nblk, int64_t *btot, int64_t *bpre,
+                              int64_t *ctr, const TrackTarget &tt, mcs_step_stats *stats,
+                              const EmitArgs *em = nullptr)
 {
     mcs_ctx *ctx = p->ctx;
     int32_t rc = mcs_scan_i64_to(ctx, btot, bpre, nblk, ctr + C_BIRTHS);
@@ -512,39 +756,57 @@ static int32_t advance_finish(mcs_pop *p, const mcs_table *t, int64_t n0, int64_
         rc = mcs_scan_i64_to(ctx, p->tblk, tpre, nblk, ctr + C_TTOT);
         if (rc) return rc;
     }
-    int64_t h[C_EVENTS + MCS_MAX_PROC];
-    rc = mcs_readback(ctx, ctr, h, C_EVENTS + t->nproc);
+    int64_t *ppre = em ? p->pblk + (nblk + 1) : nullptr;
+    if (em) {
+        rc = mcs_scan_i64_to(ctx, p->pblk, ppre, nblk, ctr + C_PTOT);
+        if (rc) return rc;
+    }
+    int64_t h[C_PTOT + 1];
+    rc = mcs_readback(ctx, ctr, h, em ? C_PTOT + 1 : C_EVENTS + nev);
     if (rc) return rc;
     const int64_t logged = h[C_LOG], births = h[C_BIRTHS];
     const int64_t tlogged = tt.qfixed ? h[C_TLOG] : 0, tcount = tt.qfixed ? h[C_TTOT] : 0;
+    const int64_t blogged = em ? h[C_BLOG] : 0, ptot = em ? h[C_PTOT] : 0;
     int32_t flags = (int32_t)h[C_FLAGS];
     if (stats) {
         stats->births = births;
         stats->deaths = h[C_DEATHS];
         stats->flights = h[C_FLIGHTS];
-        for (int j = 0; j < t->nproc; ++j) stats->events[j] = h[C_EVENTS + j];
+        for (int j = 0; j < nev; ++j) stats->events[j] = h[C_EVENTS + j];
         stats->flags = flags;
     }
-    if (logged > p->logcap || n0 + births > p->cap || tlogged > p->logcap) {
+    const int64_t nd = dst->n;
+    if (logged > p->logcap || nd + births > dst->cap || tlogged > p->logcap) {
         if (stats) stats->flags |= MCS_FLAG_CAPACITY;
         return mcs_fail(ctx, MCS_E_CAPACITY, "Maximum number of particles reached (%lld + %lld > %lld)",
-                        (long long)n0, (long long)births, (long long)p->cap);
+                        (long long)nd, (long long)births, (long long)dst->cap);
+    }
+    if (em && (blogged > p->logcap || em->photons->n + ptot > em->photons->cap)) {
+        if (stats) stats->flags |= MCS_FLAG_CAPACITY;
+        return mcs_fail(ctx, MCS_E_CAPACITY, "Maximum number of photons reached (%lld + %lld > %lld)",
+                        (long long)em->photons->n, (long long)ptot, (long long)em->photons->cap);
     }
     if (tcount != tlogged)
         return mcs_fail(ctx, MCS_E_HIP, "advance: %lld tracker deposits logged, %lld counted", (long long)tlogged,
                         (long long)tcount);
     if (births > 0) {
-        if (p->prec == MCS_PREC_F32)
+        if (dst != p) mcs_pop_touch(dst);
+        if (dst->prec == MCS_PREC_F32)
             hipLaunchKernelGGL(k_place_births32, dim3((unsigned)cdiv(births, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
                                ctx->stream, birth_log32(p), births, (const int32_t *)p->i32a,
-                               (const int64_t *)bpre, n0, p->next_id, p->sbuf[0], p->gn[0], p->gn[1], p->gn[2]);
+                               (const int64_t *)bpre, nd, dst->next_id, dst->sbuf[0], dst->gn[0], dst->gn[1],
+                               dst->gn[2]);
         else
             hipLaunchKernelGGL(k_place_births, dim3((unsigned)cdiv(births, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
                                ctx->stream, birth_log(p), births, (const int32_t *)p->i32a,
-                               (const int64_t *)bpre, n0, p->next_id, p->buf[0]);
+                               (const int64_t *)bpre, nd, dst->next_id, dst->buf[0]);
         MCS_LAUNCH_CHECK(ctx);
-        p->n = n0 + births;
-        p->next_id += (uint64_t)births;
+        dst->n = nd + births;
+        dst->next_id += (uint64_t)births;
+    }
+    if (em && ptot > 0) {
+        rc = emit_photons(p, *em, blogged, ptot, ppre);
+        if (rc) return rc;
     }
     if (tt.qfixed) {
         rc = track_apply(p, tt, tcount, tpre);
@@ -558,17 +820,18 @@ static int32_t advance_finish(mcs_pop *p, const mcs_table *t, int64_t n0, int64_
     return MCS_OK;
 }
 
-
-extern "C" int32_t mcs_advance(mcs_pop *p, const mcs_table *t, const mcs_field *field, double dt,
-                               mcs_mesh *tracker, uint64_t seed, uint32_t instance, uint32_t step,
-                               mcs_step_stats *stats)
+// The electron loop of advance! (mcs_advance), emitting photons when em is set.
+static int32_t electron_step(mcs_pop *p, const mcs_table *t, const mcs_field *field, double dt, mcs_mesh *tracker,
+                             uint64_t seed, uint32_t instance, uint32_t step, mcs_step_stats *stats,
+                             const EmitArgs *em)
 {
-    if (!p || !t || !field) return MCS_E_ARG;
     mcs_ctx *ctx = p->ctx;
     hipSetDevice(ctx->device);
     if (!(dt >= 0.0)) return mcs_fail(ctx, MCS_E_ARG, "advance: dt must be >= 0");
     if (field->kind == MCS_FIELD_MESH && !field->mesh)
         return mcs_fail(ctx, MCS_E_ARG, "advance: mesh field without a mesh");
+    if (field->kind == MCS_FIELD_MESH && tracker && tracker != field->mesh)
+        return mcs_fail(ctx, MCS_E_ARG, "advance: tracker mesh must be the field mesh");
     mcs_pop_touch(p);
     if (stats) memset(stats, 0, sizeof(*stats));
     const int64_t n0 = p->n;
@@ -581,10 +844,6 @@ extern "C" int32_t mcs_advance(mcs_pop *p, const mcs_table *t, const mcs_field *
     int64_t *btot = p->blk;
     int64_t *bpre = p->blk + (nblk + 1);
     int64_t *ctr = ctx->d_small;
-    hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, ctx->stream, ctr, (int64_t)64, (int64_t)0);
-
-    if (field->kind == MCS_FIELD_MESH && tracker && tracker != field->mesh)
-        return mcs_fail(ctx, MCS_E_ARG, "advance: tracker mesh must be the field mesh");
     TrackTarget tt;
     if (tracker) {
         int32_t rc = mcs_pop_ensure_track(p);
@@ -592,53 +851,63 @@ extern "C" int32_t mcs_advance(mcs_pop *p, const mcs_table *t, const mcs_field *
         tt.qfixed = tracker->arr[MCS_ARR_QFIXED];
         tt.ncells = tracker->ncg;
     }
+    if (em) {
+        int32_t rc = mcs_pop_ensure_bursts(p);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, ctx->stream, ctr, (int64_t)64, (int64_t)0);
     if (p->prec == MCS_PREC_F32) {
         int32_t rc = launch_advance32(p, t, field, dt, tracker, seed, instance, step, n0, nblk, btot, ctr);
         if (rc) return rc;
     } else {
-    AdvArgs a;
-    a.b = p->buf[0];
-    a.n0 = n0;
-    a.T = table_view(t);
-    a.M = mesh_view(field->kind == MCS_FIELD_MESH ? field->mesh : tracker);
-    if (field->kind == MCS_FIELD_MESH && tracker && tracker != field->mesh)
-        return mcs_fail(ctx, MCS_E_ARG, "advance: tracker mesh must be the field mesh");
-    for (int d = 0; d < 3; ++d) a.e[d] = field->e[d];
-    a.dt = dt;
-    a.seed = seed;
-    a.instance = instance;
-    a.step = step;
-    a.L = birth_log(p);
-    a.loff = p->i32a;
-    a.btot = btot;
-    a.ctr = ctr;
-    a.TL = track_log(p);
-    {
+        AdvArgs a;
+        a.b = p->buf[0];
+        a.n0 = n0;
+        a.T = table_view(t);
+        a.M = mesh_view(field->kind == MCS_FIELD_MESH ? field->mesh : tracker);
+        for (int d = 0; d < 3; ++d) a.e[d] = field->e[d];
+        a.dt = dt;
+        a.seed = seed;
+        a.instance = instance;
+        a.step = step;
+        a.L = birth_log(p);
+        a.loff = p->i32a;
+        a.btot = btot;
+        a.ctr = ctr;
+        a.TL = track_log(p);
         ProfScope ps(ctx, "advance");
-        int fm = field->kind == MCS_FIELD_MESH;
-        int tr = tracker != nullptr;
+        const int fm = field->kind == MCS_FIELD_MESH;
+        const int tr = tracker != nullptr;
         dim3 g((unsigned)nblk), bl(MCS_BLOCK);
-        if (fm && tr)
+        if (em) {
+            if (fm && tr)
+                hipLaunchKernelGGL((k_advance_photo<1, 1>), g, bl, 0, ctx->stream, a, burst_log(p));
+            else if (fm)
+                hipLaunchKernelGGL((k_advance_photo<1, 0>), g, bl, 0, ctx->stream, a, burst_log(p));
+            else if (tr)
+                hipLaunchKernelGGL((k_advance_photo<0, 1>), g, bl, 0, ctx->stream, a, burst_log(p));
+            else
+                hipLaunchKernelGGL((k_advance_photo<0, 0>), g, bl, 0, ctx->stream, a, burst_log(p));
+        } else if (fm && tr) {
             hipLaunchKernelGGL((k_advance<1, 1>), g, bl, 0, ctx->stream, a);
-        else if (fm)
+        } else if (fm) {
             hipLaunchKernelGGL((k_advance<1, 0>), g, bl, 0, ctx->stream, a);
-        else if (tr)
+        } else if (tr) {
             hipLaunchKernelGGL((k_advance<0, 1>), g, bl, 0, ctx->stream, a);
-        else
+        } else {
             hipLaunchKernelGGL((k_advance<0, 0>), g, bl, 0, ctx->stream, a);
+        }
     }
     MCS_LAUNCH_CHECK(ctx);
-    }
-    return advance_finish(p, t, n0, nblk, btot, bpre, ctr, tt, stats);
+    return advance_finish(p, p, t->nproc, nblk, btot, bpre, ctr, tt, stats, em);
 }
 
-
-// advance! in the axisymmetric field of grid g, tracking into g's QFIXED when
-// track != 0.
-extern "C" int32_t mcs_advance_cyl(mcs_pop *p, const mcs_table *t, mcs_cgrid *g, double dt, int32_t track,
-                                   uint64_t seed, uint32_t instance, uint32_t step, mcs_step_stats *stats)
+// The electron loop of advance! in the axisymmetric field of grid g, tracking into
+// g's QFIXED when track != 0, emitting photons when em is set.
+static int32_t electron_step_cyl(mcs_pop *p, const mcs_table *t, mcs_cgrid *g, double dt, int32_t track,
+                                 uint64_t seed, uint32_t instance, uint32_t step, mcs_step_stats *stats,
+                                 const EmitArgs *em)
 {
-    if (!p || !t || !g) return MCS_E_ARG;
     mcs_ctx *ctx = p->ctx;
     hipSetDevice(ctx->device);
     if (!(dt >= 0.0)) return mcs_fail(ctx, MCS_E_ARG, "advance: dt must be >= 0");
@@ -663,6 +932,10 @@ extern "C" int32_t mcs_advance_cyl(mcs_pop *p, const mcs_table *t, mcs_cgrid *g,
         tt.qfixed = g->arr[MCS_CARR_QFIXED];
         tt.ncells = (int64_t)(g->M + 2) * (g->N + 2);
     }
+    if (em) {
+        int32_t rc = mcs_pop_ensure_bursts(p);
+        if (rc) return rc;
+    }
     hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, ctx->stream, ctr, (int64_t)64, (int64_t)0);
     AdvArgs a;
     a.b = p->buf[0];
@@ -682,13 +955,288 @@ extern "C" int32_t mcs_advance_cyl(mcs_pop *p, const mcs_table *t, mcs_cgrid *g,
     {
         ProfScope ps(ctx, "advance");
         dim3 gr((unsigned)nblk), bl(MCS_BLOCK);
-        if (track)
+        if (em && track)
+            hipLaunchKernelGGL((k_advance_photo<2, 1>), gr, bl, 0, ctx->stream, a, burst_log(p));
+        else if (em)
+            hipLaunchKernelGGL((k_advance_photo<2, 0>), gr, bl, 0, ctx->stream, a, burst_log(p));
+        else if (track)
             hipLaunchKernelGGL((k_advance<2, 1>), gr, bl, 0, ctx->stream, a);
         else
             hipLaunchKernelGGL((k_advance<2, 0>), gr, bl, 0, ctx->stream, a);
     }
     MCS_LAUNCH_CHECK(ctx);
-    return advance_finish(p, t, n0, nblk, btot, bpre, ctr, tt, stats);
+    return advance_finish(p, p, t->nproc, nblk, btot, bpre, ctr, tt, stats, em);
+}
+
+static int32_t single_population_checks(mcs_pop *p, const mcs_table *t)
+{
+    if (p->species == MCS_SPECIES_PHOTON)
+        return mcs_fail(p->ctx, MCS_E_ARG, "advance: photons move in mcs_advance_mixed");
+    if (t->nphoto > 0)
+        return mcs_fail(p->ctx, MCS_E_ARG, "advance: a table with photoemission rows needs mcs_advance_mixed");
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_advance(mcs_pop *p, const mcs_table *t, const mcs_field *field, double dt,
+                               mcs_mesh *tracker, uint64_t seed, uint32_t instance, uint32_t step,
+                               mcs_step_stats *stats)
+{
+    if (!p || !t || !field) return MCS_E_ARG;
+    int32_t rc = single_population_checks(p, t);
+    if (rc) return rc;
+    return electron_step(p, t, field, dt, tracker, seed, instance, step, stats, nullptr);
+}
+
+extern "C" int32_t mcs_advance_cyl(mcs_pop *p, const mcs_table *t, mcs_cgrid *g, double dt, int32_t track,
+                                   uint64_t seed, uint32_t instance, uint32_t step, mcs_step_stats *stats)
+{
+    if (!p || !t || !g) return MCS_E_ARG;
+    int32_t rc = single_population_checks(p, t);
+    if (rc) return rc;
+    return electron_step_cyl(p, t, g, dt, track, seed, instance, step, stats, nullptr);
+}
+
+// ---------------------------------------------------------------- photons
+
+struct PhotonArgs {
+    PopBuf b;
+    int64_t n0;
+    double numax, dt;
+    uint64_t seed;
+    uint32_t instance, step;
+    BirthLog L;
+    int32_t *loff;
+    int64_t *btot;
+    int64_t *ctr;
+    TrackLog TL;
+    MeshView M;
+};
+
+// The photon loop of advance! (mixed_population.jl:27-51) with
+// ZhelezniakCollisions(numax) (photon.jl:27-56): one thread per photon slot that
+// exists when the loop starts.  Free flights x += t v to the next (null) collision
+// at s/numax; a collision draws xi = numax U and absorbs when nu > xi
+// (PhotoIonization): the new electron (at x, at rest, weight w, s = nextcoll(),
+// the photon's active flag) goes to the birth log, the tracker (TK 1: 3D mesh,
+// 2: axisymmetric grid, 0: none) gets +w at x, the photon is removed; then the
+// photon's s = nextcoll().  A removed photon keeps flying, as in the reference.
+template <int TK>
+__global__ __launch_bounds__(MCS_BLOCK) void k_photon_advance(PhotonArgs a)
+{
+    __shared__ int s_flights, s_deaths, s_abs, s_flags;
+    if (threadIdx.x == 0) {
+        s_flights = 0;
+        s_deaths = 0;
+        s_abs = 0;
+        s_flags = 0;
+    }
+    __syncthreads();
+    const int64_t i = xcd_gidx();
+    int nb = 0, nt = 0, flights = 0, deaths = 0, absorbed = 0;
+    int32_t flags = 0;
+    if (i < a.n0 && a.b.active[i]) {
+        double x[3] = {a.b.x[0][i], a.b.x[1][i], a.b.x[2][i]};
+        const double v[3] = {a.b.v[0][i], a.b.v[1][i], a.b.v[2][i]};
+        const double nu = a.b.nu[i], w = a.b.w[i];
+        double s = a.b.s[i];
+        uint8_t act = 1;
+        mcs_stream rs;
+        mcs_stream_init(&rs, a.seed, a.instance, MCS_PHASE_PHOTON, a.b.id[i], a.step);
+        double trem = a.dt;
+        while (trem > 0.0) {
+            const double tnext = s / a.numax;
+            double tt;
+            bool collides;
+            if (trem > tnext) {
+                tt = tnext;
+                collides = true;
+            } else {
+                tt = trem;
+                collides = false;
+                s = s - tt * a.numax;
+            }
+#pragma unroll
+            for (int d = 0; d < 3; ++d) x[d] = x[d] + tt * v[d];
+            flights += 1;
+            if (collides) {
+                const double xi = mcs_u01(&rs) * a.numax;
+                if (nu > xi) {
+                    absorbed += 1;
+                    const double sc = mcs_nextcoll(&rs);
+                    if (TK) {
+                        int32_t cell;
+                        double amount;
+                        if (track_cell<TK == 2 ? 2 : 1>(a.M, x, w, cell, amount))
+                            track_push(a.TL, a.ctr, i, nt, cell, amount, flags);
+                    }
+                    const unsigned long long slot = atomicAdd((unsigned long long *)&a.ctr[C_LOG], 1ULL);
+                    if ((int64_t)slot < a.L.cap) {
+                        a.L.parent[slot] = (int32_t)i;
+                        a.L.seq[slot] = nb;
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) {
+                            a.L.x[d][slot] = x[d];
+                            a.L.v[d][slot] = 0.0;
+                        }
+                        a.L.w[slot] = w;
+                        a.L.s[slot] = sc;
+                        a.L.act[slot] = act;
+                    } else {
+                        flags |= MCS_FLAG_CAPACITY;
+                    }
+                    nb += 1;
+                    if (act) deaths += 1;
+                    act = 0;
+                }
+                s = mcs_nextcoll(&rs);
+            }
+            trem = trem - tt;
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) a.b.x[d][i] = x[d];
+        a.b.s[i] = s;
+        if (!act) a.b.active[i] = 0;
+    }
+    int tot;
+    const int ex = block_excl_scan<int>(nb, &tot);
+    if (nb > 0) a.loff[i] = ex;
+    if (threadIdx.x == 0) a.btot[xcd_block()] = tot;
+    if (TK) track_block_counts(a.TL, i, nt);
+    if (flights) atomicAdd(&s_flights, flights);
+    if (deaths) atomicAdd(&s_deaths, deaths);
+    if (absorbed) atomicAdd(&s_abs, absorbed);
+    if (flags) atomicOr(&s_flags, flags);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_flights) atomicAdd((unsigned long long *)&a.ctr[C_FLIGHTS], (unsigned long long)s_flights);
+        if (s_deaths) atomicAdd((unsigned long long *)&a.ctr[C_DEATHS], (unsigned long long)s_deaths);
+        if (s_abs) atomicAdd((unsigned long long *)&a.ctr[C_EVENTS], (unsigned long long)s_abs);
+        if (s_flags) atomicOr((unsigned long long *)&a.ctr[C_FLAGS], (unsigned long long)s_flags);
+    }
+}
+
+// The photon loop over the photons of ph, appending photoelectrons to el; tk and
+// M select the tracker (1: the 3D mesh of M, 2: the axisymmetric grid of M).
+static int32_t photon_step(mcs_pop *ph, mcs_pop *el, double numax, double dt, const MeshView &M, int tk,
+                           const TrackTarget &tt, uint64_t seed, uint32_t instance, uint32_t step,
+                           mcs_step_stats *stats)
+{
+    mcs_ctx *ctx = ph->ctx;
+    mcs_pop_touch(ph);
+    const int64_t n0 = ph->n;
+    if (stats) stats->n_out = n0;
+    if (n0 == 0) return MCS_OK;
+    const int64_t nblk = cdiv(n0, MCS_BLOCK);
+    int64_t *btot = ph->blk;
+    int64_t *bpre = ph->blk + (nblk + 1);
+    int64_t *ctr = ctx->d_small;
+    if (tk) {
+        int32_t rc = mcs_pop_ensure_track(ph);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, ctx->stream, ctr, (int64_t)64, (int64_t)0);
+    PhotonArgs a;
+    a.b = ph->buf[0];
+    a.n0 = n0;
+    a.numax = numax;
+    a.dt = dt;
+    a.seed = seed;
+    a.instance = instance;
+    a.step = step;
+    a.L = birth_log(ph);
+    a.loff = ph->i32a;
+    a.btot = btot;
+    a.ctr = ctr;
+    a.TL = track_log(ph);
+    a.M = M;
+    {
+        ProfScope ps(ctx, "photon_advance");
+        dim3 g((unsigned)nblk), bl(MCS_BLOCK);
+        if (tk == 2)
+            hipLaunchKernelGGL(k_photon_advance<2>, g, bl, 0, ctx->stream, a);
+        else if (tk == 1)
+            hipLaunchKernelGGL(k_photon_advance<1>, g, bl, 0, ctx->stream, a);
+        else
+            hipLaunchKernelGGL(k_photon_advance<0>, g, bl, 0, ctx->stream, a);
+    }
+    MCS_LAUNCH_CHECK(ctx);
+    return advance_finish(ph, el, 1, nblk, btot, bpre, ctr, tt, stats, nullptr);
+}
+
+static int32_t mixed_checks(mcs_pop *e, mcs_pop *ph, const mcs_table *t, double numax)
+{
+    mcs_ctx *ctx = e->ctx;
+    if (e->species != MCS_SPECIES_ELECTRON)
+        return mcs_fail(ctx, MCS_E_ARG, "advance_mixed: the first population must hold electrons");
+    if (t->ctx != ctx) return mcs_fail(ctx, MCS_E_ARG, "advance_mixed: table of another context");
+    if (!ph) {
+        if (t->nphoto > 0)
+            return mcs_fail(ctx, MCS_E_ARG, "advance_mixed: a table with photoemission rows needs photons");
+        return MCS_OK;
+    }
+    if (ph->species != MCS_SPECIES_PHOTON)
+        return mcs_fail(ctx, MCS_E_ARG, "advance_mixed: the second population must hold photons");
+    if (ph->ctx != ctx) return mcs_fail(ctx, MCS_E_ARG, "advance_mixed: photons of another context");
+    if (e->prec != MCS_PREC_F64)
+        return mcs_fail(ctx, MCS_E_ARG, "advance_mixed: FP64 electron populations only");
+    if (!(numax > 0.0)) return mcs_fail(ctx, MCS_E_ARG, "advance_mixed: numax must be positive");
+    for (int j = 0; j < t->nproc; ++j)
+        if (t->h_kind[j] == MCS_PROC_PHOTOEMISSION && !t->photo_set[j])
+            return mcs_fail(ctx, MCS_E_ARG, "advance_mixed: photoemission row %d has no parameters", j);
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_advance_mixed(mcs_pop *e, mcs_pop *ph, const mcs_table *t, double numax,
+                                     const mcs_field *field, double dt, mcs_mesh *tracker, uint64_t seed,
+                                     uint32_t instance, uint32_t step, mcs_step_stats *estats,
+                                     mcs_step_stats *pstats)
+{
+    if (!e || !t || !field) return MCS_E_ARG;
+    int32_t rc = mixed_checks(e, ph, t, numax);
+    if (rc) return rc;
+    if (pstats) memset(pstats, 0, sizeof(*pstats));
+    if (!ph) return electron_step(e, t, field, dt, tracker, seed, instance, step, estats, nullptr);
+    if (pstats) pstats->n_in = ph->n;
+    EmitArgs em;
+    em.photons = ph;
+    em.photo = t->photo;
+    em.seed = seed;
+    em.instance = instance;
+    em.step = step;
+    rc = electron_step(e, t, field, dt, tracker, seed, instance, step, estats, t->nphoto > 0 ? &em : nullptr);
+    if (rc) return rc;
+    TrackTarget tt;
+    if (tracker) {
+        tt.qfixed = tracker->arr[MCS_ARR_QFIXED];
+        tt.ncells = tracker->ncg;
+    }
+    return photon_step(ph, e, numax, dt, mesh_view(tracker), tracker ? 1 : 0, tt, seed, instance, step, pstats);
+}
+
+extern "C" int32_t mcs_advance_cyl_mixed(mcs_pop *e, mcs_pop *ph, const mcs_table *t, double numax, mcs_cgrid *g,
+                                         double dt, int32_t track, uint64_t seed, uint32_t instance,
+                                         uint32_t step, mcs_step_stats *estats, mcs_step_stats *pstats)
+{
+    if (!e || !t || !g) return MCS_E_ARG;
+    int32_t rc = mixed_checks(e, ph, t, numax);
+    if (rc) return rc;
+    if (pstats) memset(pstats, 0, sizeof(*pstats));
+    if (!ph) return electron_step_cyl(e, t, g, dt, track, seed, instance, step, estats, nullptr);
+    if (pstats) pstats->n_in = ph->n;
+    EmitArgs em;
+    em.photons = ph;
+    em.photo = t->photo;
+    em.seed = seed;
+    em.instance = instance;
+    em.step = step;
+    rc = electron_step_cyl(e, t, g, dt, track, seed, instance, step, estats, t->nphoto > 0 ? &em : nullptr);
+    if (rc) return rc;
+    TrackTarget tt;
+    if (track) {
+        tt.qfixed = g->arr[MCS_CARR_QFIXED];
+        tt.ncells = (int64_t)(g->M + 2) * (g->N + 2);
+    }
+    return photon_step(ph, e, numax, dt, cyl_view(g), track ? 2 : 0, tt, seed, instance, step, pstats);
 }
 
 // ---------------------------------------------------------------- repack!
@@ -762,6 +1310,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_active_energy(Buf b, int64_t n, c
 extern "C" int32_t mcs_pop_energies(mcs_pop *p, int64_t nmax, int64_t *nout, double *energy)
 {
     if (!p || !nout) return MCS_E_ARG;
+    MCS_ELECTRONS_ONLY(p, "pop_energies");
     mcs_ctx *ctx = p->ctx;
     hipSetDevice(ctx->device);
     *nout = 0;
@@ -886,6 +1435,7 @@ extern "C" int32_t mcs_swarm_resample(mcs_pop *p, int64_t n0, double tol, uint64
                                       uint32_t instance, uint32_t step, int32_t *did)
 {
     if (!p || n0 <= 0) return MCS_E_ARG;
+    MCS_ELECTRONS_ONLY(p, "swarm_resample");
     mcs_ctx *ctx = p->ctx;
     if (p->prec == MCS_PREC_F32)
         return mcs_fail(ctx, MCS_E_ARG, "swarm_resample: not available on a split-precision population");

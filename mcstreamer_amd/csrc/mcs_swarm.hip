@@ -1106,6 +1106,7 @@ extern "C" int32_t mcs_swarm_batch_add(mcs_swarm_batch *b, const mcs_table *t,
     if (prm->capacity < 1 || n < 0 || n > prm->capacity || prm->n0 < 1 || !(prm->dt >= 0.0) ||
         prm->block_steps < 1 || prm->relax_steps < 0)
         return mcs_fail(ctx, MCS_E_ARG, "swarm batch: bad instance parameters");
+    if (t->nphoto > 0) return mcs_fail(ctx, MCS_E_ARG, "swarm batch: tables with photoemission rows need photons");
     const bool dev_init = (x == nullptr);
     if (n > 0 && !dev_init && (!v || !w || !s || !active))
         return mcs_fail(ctx, MCS_E_ARG, "swarm batch: null array");
@@ -1482,6 +1483,7 @@ extern "C" int32_t mcs_swarm_batch_download(mcs_swarm_batch *b, int32_t slot, in
 extern "C" int32_t mcs_pop_init_thermal(mcs_pop *p, int64_t n, uint64_t seed, uint32_t instance, double vth)
 {
     if (!p) return MCS_E_ARG;
+    MCS_ELECTRONS_ONLY(p, "pop_init_thermal");
     mcs_ctx *ctx = p->ctx;
     hipSetDevice(ctx->device);
     if (n < 0 || n > p->cap || !(vth >= 0.0))

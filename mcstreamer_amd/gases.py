@@ -4,7 +4,10 @@ that load_lxcat reads (JuMC/data/LXCat-June2013.json has the same fields).
 N2-like: elastic 6e-20 m^2 (mass ratio 3.9e-5); excitations at 6.2, 8.4 and
 11.0 eV with sigma = 2e-21 (1 - eps_th/eps); ionization I = 15.6 eV with
 sigma = 2.5e-20 ln(eps/I)/(eps/I).  O2-like attacher: sigma = 1.5e-22
-exp(-(eps - 6.5)^2 / 2) m^2 (centre 6.5 eV, width 1 eV).
+exp(-(eps - 6.5)^2 / 2) m^2 (centre 6.5 eV, width 1 eV).  Photoemission (the
+layout of the reference's PHOTOEMISSION items, lxcat.jl:85-92): the N2-like
+ionization cross section times `rescale`, absorbed by the O2-like gas with
+chi between 1.09e-23 and 6.21e-21 m^2 (Zhelezniak's range at 300 K).
 """
 import json
 import os
@@ -79,10 +82,34 @@ def o2_like_items(nodes_ev):
     ]
 
 
-def write_gas(path, nodes_ev, with_attacher=True):
+CHI_MIN, CHI_MAX = 1.09e-23, 6.21e-21
+
+
+def photo_items(nodes_ev, rescale=0.05):
+    e = np.asarray(nodes_ev, dtype=np.float64)
+    ion = 15.6
+    r = np.where(e > ion, e / ion, 1.0)
+    sig = np.where(e > ion, 2.5e-20 * np.log(r) / r, 0.0)
+    return [
+        {
+            "target": "N2like",
+            "kind": "PHOTOEMISSION",
+            "absorber": "O2like",
+            "chi_min": CHI_MIN,
+            "chi_max": CHI_MAX,
+            "rescale": rescale,
+            "comment": "synthetic photoemission: the ionization cross section, rescaled",
+            "data": [[float(x), float(s)] for x, s in zip(e, sig)],
+        }
+    ]
+
+
+def write_gas(path, nodes_ev, with_attacher=True, with_photo=False, photo_rescale=0.05):
     items = n2_like_items(nodes_ev)
     if with_attacher:
         items += o2_like_items(nodes_ev)
+    if with_photo:
+        items += photo_items(nodes_ev, photo_rescale)
     with open(path, "w") as f:
         json.dump(items, f)
     return path

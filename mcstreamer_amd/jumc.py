@@ -7,9 +7,12 @@ Reference names map as follows (Julia's `!` dropped):
   repack! (population.jl:191-215)                 -> repack
   nparticles/actives/weight/meanenergy/maxenergy (population.jl:63-158)
   FixedField (Swarm/src/Swarm.jl:28-32)           -> FixedField
+  PhotonState / ZhelezniakCollisions (photon.jl)  -> PhotonPopulation / ZhelezniakCollisions
+  MultiPopulation (mixed_population.jl:4-18)      -> MultiPopulation
 All compute runs in libmcs on the GPU; this module only marshals arguments.
 """
 import ctypes
+import math
 
 import numpy as np
 
@@ -94,6 +97,18 @@ class CollisionTable:
             self.ctx.handle, nproc, ne, self.egrid, dptr(self.energy), dptr(emajor),
             i32ptr(self.kind), dptr(self.param), self.maxrate, ctypes.byref(h)))
         self.handle = h
+        for j, p in enumerate(self.proc):
+            if p.code == _lib.PROC_PHOTOEMISSION:
+                numin, numax, mult, pw = p.photo
+                check(self.ctx.handle, _lib.lib().mcs_table_set_photoemission(
+                    h, j, float(numin), float(numax), float(mult), float(pw)))
+
+    def photoemission(self):
+        """The first PhotoEmission process, or None (main.jl:115)."""
+        for p in self.proc:
+            if p.code == _lib.PROC_PHOTOEMISSION:
+                return p
+        return None
 
     def __len__(self):
         return len(self.proc)
@@ -225,12 +240,121 @@ class Population:
             pass
 
 
+class ZhelezniakCollisions:
+    """The photons' collision table (photon.jl:27-43): one PhotoIonization
+    process at the photon's own rate nu, maxrate numax."""
+
+    def __init__(self, numax):
+        self.numax = float(numax)
+
+    @classmethod
+    def for_table(cls, table):
+        """ZhelezniakCollisions(exp(log_numax)) of the table's first PhotoEmission
+        row (main.jl:115-118), or None without such a row."""
+        p = table.photoemission()
+        return None if p is None else cls(math.exp(math.log(p.photo[1])))
+
+
+class PhotonStates:
+    """Host copy of a photon population's columns (PhotonState, photon.jl:11-18)."""
+
+    def __init__(self, x, v, nu, w, s, active, id=None, next_id=None):
+        self.x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        n = self.x.shape[0]
+        self.v = np.ascontiguousarray(v, dtype=np.float64).reshape(n, 3)
+        self.nu = np.ascontiguousarray(nu, dtype=np.float64).reshape(n)
+        self.w = np.ascontiguousarray(w, dtype=np.float64).reshape(n)
+        self.s = np.ascontiguousarray(s, dtype=np.float64).reshape(n)
+        self.active = np.ascontiguousarray(active, dtype=np.uint8).reshape(n)
+        self.id = None if id is None else np.ascontiguousarray(id, dtype=np.uint64).reshape(n)
+        self.next_id = next_id
+
+    def __len__(self):
+        return self.x.shape[0]
+
+
+class PhotonPopulation:
+    """A device-resident photon population (Population of PhotonState,
+    main.jl:116-121), empty unless `photons` are given."""
+
+    def __init__(self, max_particles, photons=None, collisions=None, ctx=None):
+        self.ctx = ctx or context()
+        self.collisions = collisions
+        h = ctypes.c_void_p()
+        check(self.ctx.handle, _lib.lib().mcs_pop_create_photons(self.ctx.handle, int(max_particles),
+                                                                 ctypes.byref(h)))
+        self.handle = h
+        self.capacity = int(max_particles)
+        if photons is not None:
+            self.upload(photons)
+
+    def upload(self, ps, next_id=0):
+        nid = ps.next_id if ps.next_id is not None else next_id
+        check(self.ctx.handle, _lib.lib().mcs_photons_upload(
+            self.handle, len(ps), dptr(ps.x), dptr(ps.v), dptr(ps.nu), dptr(ps.w), dptr(ps.s), u8ptr(ps.active),
+            u64ptr(ps.id) if ps.id is not None else None, int(nid)))
+
+    def download(self):
+        n = self.nparticles()
+        x, v = np.zeros((n, 3)), np.zeros((n, 3))
+        nu, w, s = np.zeros(n), np.zeros(n), np.zeros(n)
+        act = np.zeros(n, dtype=np.uint8)
+        ids = np.zeros(n, dtype=np.uint64)
+        nn, nid = ctypes.c_int64(), ctypes.c_uint64()
+        check(self.ctx.handle, _lib.lib().mcs_photons_download(
+            self.handle, n, ctypes.byref(nn), dptr(x), dptr(v), dptr(nu), dptr(w), dptr(s), u8ptr(act),
+            u64ptr(ids), ctypes.byref(nid)))
+        return PhotonStates(x, v, nu, w, s, act, ids, nid.value)
+
+    def nparticles(self):
+        n = ctypes.c_int64()
+        check(self.ctx.handle, _lib.lib().mcs_pop_size(self.handle, ctypes.byref(n)))
+        return n.value
+
+    def moments(self, table=None):
+        m = _lib.Moments()
+        check(self.ctx.handle, _lib.lib().mcs_pop_moments(self.handle, None, ctypes.byref(m)))
+        return m
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.lib().mcs_pop_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class MultiPopulation:
+    """MultiPopulation(:electron => electron, :photon => photon)
+    (mixed_population.jl:4-18); the electrons' (seed, instance, step) key every
+    stream of the step."""
+
+    def __init__(self, electron, photon=None):
+        self.electron = electron
+        self.photon = photon
+
+    def __getitem__(self, name):
+        return {"electron": self.electron, "photon": self.photon}[name]
+
+    @property
+    def step(self):
+        return self.electron.step
+
+    @step.setter
+    def step(self, value):
+        self.electron.step = value
+
+
 def advance(popl, efield, dt, tracker=None, *, step=None, stats=True):
-    """advance! (mixed_population.jl:24-54) on the GPU.  Returns StepStats.
+    """advance! (mixed_population.jl:24-54) on the GPU.  Returns StepStats, or
+    (electron StepStats, photon StepStats) for a MultiPopulation.
 
     `efield` is a FixedField, a streamer FieldInterp or an axisymmetric
     FieldInterp (axisym); `tracker` is None (VoidCollisionTracker) or a
     CollisionTracker of the field's grid."""
+    if isinstance(popl, MultiPopulation):
+        return _advance_mixed(popl, efield, dt, tracker, step=step)
     st = _lib.StepStats()
     s = popl.step if step is None else int(step)
     if getattr(efield, "axisymmetric", False):
@@ -250,6 +374,29 @@ def advance(popl, efield, dt, tracker=None, *, step=None, stats=True):
     if step is None:
         popl.step += 1
     return st
+
+
+def _advance_mixed(mpopl, efield, dt, tracker, *, step=None):
+    el, ph = mpopl.electron, mpopl.photon
+    numax = ph.collisions.numax if ph is not None and ph.collisions is not None else 0.0
+    est, pst = _lib.StepStats(), _lib.StepStats()
+    s = el.step if step is None else int(step)
+    phh = ph.handle if ph is not None else None
+    if getattr(efield, "axisymmetric", False):
+        if tracker is not None and tracker.fields is not efield.fields:
+            raise ValueError("the tracker must belong to the field's grid")
+        check(el.ctx.handle, _lib.lib().mcs_advance_cyl_mixed(
+            el.handle, phh, el.collisions.handle, numax, efield.fields.handle, float(dt), int(tracker is not None),
+            el.seed, el.instance, s, ctypes.byref(est), ctypes.byref(pst)))
+    else:
+        fld = efield.cstruct()
+        trk = tracker.mesh_handle() if tracker is not None else None
+        check(el.ctx.handle, _lib.lib().mcs_advance_mixed(
+            el.handle, phh, el.collisions.handle, numax, ctypes.byref(fld), float(dt), trk, el.seed, el.instance,
+            s, ctypes.byref(est), ctypes.byref(pst)))
+    if step is None:
+        el.step += 1
+    return est, pst
 
 
 def repack(popl):

@@ -27,16 +27,19 @@ KINDS = {
     "EXCITATION": _lib.PROC_EXCITATION,
     "IONIZATION": _lib.PROC_IONIZATION,
     "ATTACHMENT": _lib.PROC_ATTACHMENT,
+    "PHOTOEMISSION": _lib.PROC_PHOTOEMISSION,
 }
 
 
 @dataclass(frozen=True)
 class Process:
-    """One collision process (JuMC/src/electron.jl:70-84, collisions.jl:80)."""
+    """One collision process (JuMC/src/electron.jl:70-105, collisions.jl:80)."""
 
-    kind: str           # ELASTIC, EXCITATION, IONIZATION, ATTACHMENT or NULL
+    kind: str           # ELASTIC, EXCITATION, IONIZATION, ATTACHMENT, PHOTOEMISSION or NULL
     param: float        # threshold in J, or the mass ratio for ELASTIC
     signature: str = ""
+    # PhotoEmission(numin, numax, photon_multiplier, photon_weight) (electron.jl:88-105)
+    photo: tuple = None
 
     @property
     def code(self):
@@ -48,10 +51,13 @@ def _signature(item):
     return f"e + {item['target']} -> {product}... ({item['kind']})"
 
 
-def load_lxcat(fnames, densities, energy):
+def load_lxcat(fnames, densities, energy, photon_weight=1.0, photon_multiplier=1.0):
     """Return (proc, rate, maxrate, origperm) like JuMC.load_lxcat.
 
-    `energy` is the node grid in J; `rate` has shape (nproc, len(energy)).
+    `energy` is the node grid in J; `rate` has shape (nproc, len(energy)).  A
+    PHOTOEMISSION item's cross section is multiplied by photon_multiplier
+    (lxcat.jl:46-49) and its process is PhotoEmission(c N_absorber chi_min,
+    c N_absorber chi_max, photon_multiplier, photon_weight) (lxcat.jl:85-92).
     """
     energy = np.asarray(energy, dtype=np.float64)
     db = []
@@ -64,8 +70,6 @@ def load_lxcat(fnames, densities, energy):
         dens = float(densities.get(item["target"], 0.0))
         if dens == 0.0:
             continue
-        if item["kind"] == "PHOTOEMISSION":
-            raise NotImplementedError("photo-emission processes are not part of this build")
         item = dict(item)
         e0 = np.array([d[0] * co.eV for d in item["data"]], dtype=np.float64)
         cs0 = np.array([d[1] for d in item["data"]], dtype=np.float64)
@@ -75,6 +79,8 @@ def load_lxcat(fnames, densities, energy):
             cs0 = cs0 * item["rescale"]
         if "weight_scale" in item:
             cs0 = cs0 * item["weight_scale"]
+        if item["kind"] == "PHOTOEMISSION":
+            cs0 = cs0 * photon_multiplier
         sigma = np.interp(energy, e0, cs0)
         item["nu"] = dens * v * sigma
         targets.setdefault(item["target"], []).append(item)
@@ -92,6 +98,12 @@ def load_lxcat(fnames, densities, energy):
         for p in ps:
             if p["kind"] == "ELASTIC":
                 proc.append(Process("ELASTIC", float(p["mass_ratio"]), _signature(p)))
+            elif p["kind"] == "PHOTOEMISSION":
+                nabs = float(densities[p["absorber"]])
+                numin = co.c * nabs * float(p["chi_min"])
+                numax = co.c * nabs * float(p["chi_max"])
+                proc.append(Process("PHOTOEMISSION", 0.0, _signature(p),
+                                    (numin, numax, float(photon_multiplier), float(photon_weight))))
             else:
                 proc.append(Process(p["kind"], float(p["threshold"]) * co.eV, _signature(p)))
             rows.append(p["nu"])

@@ -494,14 +494,14 @@ static int append(ref_pop *p, const double x[3], const double v[3], double w, do
  * advanced in the step they are born; a particle removed by attachment keeps
  * flying and colliding until its time step is used up. */
 static int32_t advance_impl(ref_pop *p, const ref_table *t, const ref_field *f, double dt,
-                            ref_mesh *tracker, ref_cgrid *ctracker, uint64_t seed, uint32_t instance,
-                            uint32_t step, ref_stats *st);
+                            ref_mesh *tracker, ref_cgrid *ctracker, ref_pop *ph, uint64_t seed,
+                            uint32_t instance, uint32_t step, ref_stats *st);
 
 int32_t ref_advance(ref_pop *p, const ref_table *t, const ref_field *f, double dt,
                     ref_mesh *tracker, uint64_t seed, uint32_t instance, uint32_t step,
                     ref_stats *st)
 {
-    return advance_impl(p, t, f, dt, tracker, NULL, seed, instance, step, st);
+    return advance_impl(p, t, f, dt, tracker, NULL, NULL, seed, instance, step, st);
 }
 
 /* advance! in the axisymmetric FieldInterp of g, tracking into g->qfixed. */
@@ -512,7 +512,7 @@ int32_t ref_advance_cyl(ref_pop *p, const ref_table *t, ref_cgrid *g, double dt,
     memset(&f, 0, sizeof(f));
     f.kind = 2;
     f.cgrid = g;
-    return advance_impl(p, t, &f, dt, NULL, track ? g : NULL, seed, instance, step, st);
+    return advance_impl(p, t, &f, dt, NULL, track ? g : NULL, NULL, seed, instance, step, st);
 }
 
 /* What one chunk of the particle loop produces, in particle order: the children
@@ -530,13 +530,36 @@ typedef struct {
     double amount;
 } dep_rec;
 
+/* a PhotoEmission collision's photons (count n > 0) */
+typedef struct {
+    double x[3];
+    int64_t n;
+    int32_t row;
+    uint8_t act;
+} burst_rec;
+
 typedef struct {
     birth_rec *b;
     int64_t nb, cb;
     dep_rec *d;
     int64_t nd, cd;
+    burst_rec *u;
+    int64_t nu, cu;
     ref_stats st;
 } chunk_out;
+
+static void push_burst(chunk_out *o, int32_t row, const double x[3], uint8_t act, int64_t n)
+{
+    if (o->nu == o->cu) {
+        o->cu = o->cu ? 2 * o->cu : 64;
+        o->u = (burst_rec *)realloc(o->u, sizeof(burst_rec) * (size_t)o->cu);
+    }
+    burst_rec *r = &o->u[o->nu++];
+    for (int d = 0; d < 3; ++d) r->x[d] = x[d];
+    r->n = n;
+    r->row = row;
+    r->act = act;
+}
 
 static void push_birth(chunk_out *o, const double x[3], const double v[3], double w, double s, uint8_t act)
 {
@@ -651,6 +674,14 @@ static void advance_range(ref_pop *p, const ref_table *t, const ref_field *f, do
                             push_deposit(o, q, amount);
                             if (act) st->deaths += 1;
                             act = 0;
+                        } else if (kind == 5 && t->photo) {
+                            /* PhotoEmission, electron.jl:141-144: nphot =
+                             * rand(Poisson(w / multiplier / photon_weight)); the
+                             * electron is unchanged, its photons are appended by
+                             * apply! (collisions.jl:125-141) */
+                            const double *pr = &t->photo[4 * j];
+                            int64_t np = mcs_poisson(&rs, (w / pr[2]) / pr[3]);
+                            if (np > 0) push_burst(o, j, x, act, np);
                         }
                         break;
                     } else {
@@ -680,9 +711,32 @@ static void advance_range(ref_pop *p, const ref_table *t, const ref_field *f, do
 /* Threads of the particle loop and the grid loops (1 in the single-threaded build). */
 int32_t ref_threads(void) { return REF_THREADS; }
 
+/* add_particle!(photons, PhotonState(x, c randsphere(), nu, photon_weight,
+ * nextcoll(), act)) (collisions.jl:134-139), the draws from the new photon's id. */
+static int append_photon(ref_pop *ph, const double x[3], double lmin, double lmax, double pw, uint8_t act,
+                         uint64_t seed, uint32_t instance, uint32_t step)
+{
+    if (ph->n >= ph->cap) return 0;
+    const int64_t i = ph->n;
+    double v[3], nu, s;
+    mcs_photon_emit(seed, instance, ph->next_id, step, lmin, lmax, v, &nu, &s);
+    for (int d = 0; d < 3; ++d) {
+        ph->x[3 * i + d] = x[d];
+        ph->v[3 * i + d] = v[d];
+    }
+    ph->nu[i] = nu;
+    ph->w[i] = pw;
+    ph->s[i] = s;
+    ph->active[i] = act;
+    ph->id[i] = ph->next_id;
+    ph->next_id += 1;
+    ph->n += 1;
+    return 1;
+}
+
 static int32_t advance_impl(ref_pop *p, const ref_table *t, const ref_field *f, double dt,
-                            ref_mesh *tracker, ref_cgrid *ctracker, uint64_t seed, uint32_t instance,
-                            uint32_t step, ref_stats *st)
+                            ref_mesh *tracker, ref_cgrid *ctracker, ref_pop *ph, uint64_t seed,
+                            uint32_t instance, uint32_t step, ref_stats *st)
 {
     memset(st, 0, sizeof(*st));
     const int64_t n0 = p->n;
@@ -708,12 +762,140 @@ static int32_t advance_impl(ref_pop *p, const ref_table *t, const ref_field *f, 
             else
                 st->flags |= FLAG_CAPACITY;
         }
+        for (int64_t k = 0; k < o->nu; ++k) {
+            const burst_rec *r = &o->u[k];
+            const double *pr = &t->photo[4 * r->row];
+            for (int64_t m = 0; m < r->n; ++m)
+                if (!ph || !append_photon(ph, r->x, pr[0], pr[1], pr[3], r->act, seed, instance, step))
+                    st->flags |= FLAG_CAPACITY;
+        }
         free(o->b);
         free(o->d);
+        free(o->u);
     }
     free(out);
     st->n_out = p->n;
     return st->flags;
+}
+
+/* The photon loop of advance! (mixed_population.jl:27-51) with
+ * ZhelezniakCollisions(numax) (photon.jl:27-56) over photons [i0, i1): straight
+ * flights (advance_free, photon.jl:54-56), a collision draws xi = numax rand()
+ * (collisions.jl:165) and absorbs when nu > xi: collide(PhotoIonization)
+ * (photon.jl:45-51) makes an electron at rest with s = nextcoll(), the tracker
+ * adds +w at the photon (gridfields.jl:219-222), apply! removes the photon and
+ * appends the electron (collisions.jl:117-123); then s = nextcoll(). */
+static void photon_range(ref_pop *ph, double numax, double dt, ref_mesh *tracker, ref_cgrid *ctracker,
+                         uint64_t seed, uint32_t instance, uint32_t step, int64_t i0, int64_t i1, chunk_out *o)
+{
+    ref_stats *st = &o->st;
+    const double zero[3] = {0.0, 0.0, 0.0};
+    for (int64_t i = i0; i < i1; ++i) {
+        if (!ph->active[i]) continue;
+        mcs_stream rs;
+        mcs_stream_init(&rs, seed, instance, MCS_PHASE_PHOTON, ph->id[i], step);
+        double x[3] = {ph->x[3 * i], ph->x[3 * i + 1], ph->x[3 * i + 2]};
+        const double v[3] = {ph->v[3 * i], ph->v[3 * i + 1], ph->v[3 * i + 2]};
+        const double nu = ph->nu[i], w = ph->w[i];
+        double s = ph->s[i];
+        uint8_t act = 1;
+        double trem = dt;
+        while (trem > 0.0) {
+            const double tnext = s / numax;
+            double tt;
+            int collides;
+            if (trem > tnext) {
+                tt = tnext;
+                collides = 1;
+            } else {
+                tt = trem;
+                collides = 0;
+                s = s - tt * numax;
+            }
+            for (int d = 0; d < 3; ++d) x[d] = x[d] + tt * v[d];
+            st->flights += 1;
+            if (collides) {
+                const double xi = mcs_u01(&rs) * numax;
+                if (nu > xi) {
+                    st->events[0] += 1;
+                    const double sc = mcs_nextcoll(&rs);
+                    double amount = 0.0;
+                    double *q = track1(tracker, x, w, &amount);
+                    push_deposit(o, q, amount);
+                    q = track1_cyl(ctracker, x, w, &amount);
+                    push_deposit(o, q, amount);
+                    push_birth(o, x, zero, w, sc, act);
+                    if (act) st->deaths += 1;
+                    act = 0;
+                }
+                s = mcs_nextcoll(&rs);
+            }
+            trem = trem - tt;
+        }
+        for (int d = 0; d < 3; ++d) ph->x[3 * i + d] = x[d];
+        ph->s[i] = s;
+        ph->active[i] = act;
+    }
+}
+
+static int32_t photon_impl(ref_pop *ph, ref_pop *e, double numax, double dt, ref_mesh *tracker,
+                           ref_cgrid *ctracker, uint64_t seed, uint32_t instance, uint32_t step, ref_stats *st)
+{
+    const int64_t n0 = ph->n;
+    const int nch = REF_THREADS;
+    chunk_out *out = (chunk_out *)calloc((size_t)nch, sizeof(chunk_out));
+#pragma omp parallel for schedule(static, 1)
+    for (int c = 0; c < nch; ++c) {
+        const int64_t i0 = n0 * c / nch, i1 = n0 * (c + 1) / nch;
+        photon_range(ph, numax, dt, tracker, ctracker, seed, instance, step, i0, i1, &out[c]);
+    }
+    for (int c = 0; c < nch; ++c) {
+        chunk_out *o = &out[c];
+        st->flags |= o->st.flags;
+        st->flights += o->st.flights;
+        st->deaths += o->st.deaths;
+        st->events[0] += o->st.events[0];
+        for (int64_t k = 0; k < o->nd; ++k) *o->d[k].q += o->d[k].amount;
+        for (int64_t k = 0; k < o->nb; ++k) {
+            const birth_rec *r = &o->b[k];
+            if (append(e, r->x, r->v, r->w, r->s, r->act))
+                st->births += 1;
+            else
+                st->flags |= FLAG_CAPACITY;
+        }
+        free(o->b);
+        free(o->d);
+        free(o->u);
+    }
+    free(out);
+    st->n_out = ph->n;
+    return st->flags;
+}
+
+int32_t ref_advance_mixed(ref_pop *e, ref_pop *ph, const ref_table *t, double numax, const ref_field *f,
+                          double dt, ref_mesh *tracker, uint64_t seed, uint32_t instance, uint32_t step,
+                          ref_stats *est, ref_stats *pst)
+{
+    memset(pst, 0, sizeof(*pst));
+    pst->n_in = ph ? ph->n : 0;
+    int32_t fl = advance_impl(e, t, f, dt, tracker, NULL, ph, seed, instance, step, est);
+    if (!ph) return fl;
+    return fl | photon_impl(ph, e, numax, dt, tracker, NULL, seed, instance, step, pst);
+}
+
+int32_t ref_advance_cyl_mixed(ref_pop *e, ref_pop *ph, const ref_table *t, double numax, ref_cgrid *g,
+                              double dt, int32_t track, uint64_t seed, uint32_t instance, uint32_t step,
+                              ref_stats *est, ref_stats *pst)
+{
+    ref_field f;
+    memset(&f, 0, sizeof(f));
+    f.kind = 2;
+    f.cgrid = g;
+    memset(pst, 0, sizeof(*pst));
+    pst->n_in = ph ? ph->n : 0;
+    int32_t fl = advance_impl(e, t, &f, dt, NULL, track ? g : NULL, ph, seed, instance, step, est);
+    if (!ph) return fl;
+    return fl | photon_impl(ph, e, numax, dt, NULL, track ? g : NULL, seed, instance, step, pst);
 }
 
 static void copy_slot(ref_pop *dst, int64_t di, const ref_pop *src, int64_t si)
@@ -726,6 +908,7 @@ static void copy_slot(ref_pop *dst, int64_t di, const ref_pop *src, int64_t si)
     dst->s[di] = src->s[si];
     dst->active[di] = src->active[si];
     dst->id[di] = src->id[si];
+    if (src->nu) dst->nu[di] = src->nu[si];
 }
 
 /* repack! (JuMC/src/population.jl:191-215): two pointers, the last active

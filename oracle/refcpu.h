@@ -54,6 +54,7 @@ typedef struct ref_pop {
     double *s;
     uint8_t *active;
     uint64_t *id;
+    double *nu;      /* photons: absorption rate (NULL for electrons) */
 } ref_pop;
 
 typedef struct ref_table {
@@ -66,6 +67,8 @@ typedef struct ref_table {
     const int32_t *kind;
     const double *param;
     double maxrate;
+    const double *photo; /* per row: log numin, log numax, photon_multiplier, photon_weight
+                            (PhotoEmission rows, kind 5; NULL without such rows) */
 } ref_table;
 
 typedef struct ref_mesh {
@@ -174,6 +177,16 @@ void ref_yoshida(const ref_field *f, double x[3], double v[3], double dt, int32_
 int32_t ref_advance(ref_pop *p, const ref_table *t, const ref_field *f, double dt,
                     ref_mesh *tracker, uint64_t seed, uint32_t instance, uint32_t step,
                     ref_stats *st);
+/* advance! over MultiPopulation(:electron => e, :photon => ph): the electron loop
+ * (photons of PhotoEmission collisions appended to ph), then the photon loop of
+ * ZhelezniakCollisions(numax) (photoelectrons appended to e).  est/pst as the
+ * product's mcs_advance_mixed. */
+int32_t ref_advance_mixed(ref_pop *e, ref_pop *ph, const ref_table *t, double numax, const ref_field *f,
+                          double dt, ref_mesh *tracker, uint64_t seed, uint32_t instance, uint32_t step,
+                          ref_stats *est, ref_stats *pst);
+int32_t ref_advance_cyl_mixed(ref_pop *e, ref_pop *ph, const ref_table *t, double numax, ref_cgrid *g,
+                              double dt, int32_t track, uint64_t seed, uint32_t instance, uint32_t step,
+                              ref_stats *est, ref_stats *pst);
 /* JuMC.repack! */
 void ref_repack(ref_pop *p);
 /* population reductions and Swarm tallies */

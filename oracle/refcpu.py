@@ -59,13 +59,14 @@ _PU32 = ctypes.POINTER(ctypes.c_uint32)
 
 class RefPop(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int64), ("cap", ctypes.c_int64), ("next_id", ctypes.c_uint64),
-                ("x", _PD), ("v", _PD), ("w", _PD), ("s", _PD), ("active", _PU8), ("id", _PU64)]
+                ("x", _PD), ("v", _PD), ("w", _PD), ("s", _PD), ("active", _PU8), ("id", _PU64),
+                ("nu", _PD)]
 
 
 class RefTable(ctypes.Structure):
     _fields_ = [("nproc", ctypes.c_int32), ("ne", ctypes.c_int32), ("egrid", ctypes.c_int32),
                 ("pad", ctypes.c_int32), ("energy", _PD), ("rate", _PD), ("kind", _PI32),
-                ("param", _PD), ("maxrate", ctypes.c_double)]
+                ("param", _PD), ("maxrate", ctypes.c_double), ("photo", _PD)]
 
 
 class RefMesh(ctypes.Structure):
@@ -138,6 +139,14 @@ def lib():
                                   P(RefMesh), ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                   P(RefStats)]
         L.ref_advance.restype = ctypes.c_int32
+        L.ref_advance_mixed.argtypes = [P(RefPop), P(RefPop), P(RefTable), ctypes.c_double, P(RefField),
+                                        ctypes.c_double, P(RefMesh), ctypes.c_uint64, ctypes.c_uint32,
+                                        ctypes.c_uint32, P(RefStats), P(RefStats)]
+        L.ref_advance_mixed.restype = ctypes.c_int32
+        L.ref_advance_cyl_mixed.argtypes = [P(RefPop), P(RefPop), P(RefTable), ctypes.c_double, P(RefCGrid),
+                                            ctypes.c_double, ctypes.c_int32, ctypes.c_uint64, ctypes.c_uint32,
+                                            ctypes.c_uint32, P(RefStats), P(RefStats)]
+        L.ref_advance_cyl_mixed.restype = ctypes.c_int32
         L.ref_repack.argtypes = [P(RefPop)]
         L.ref_moments_compute.argtypes = [P(RefPop), P(RefTable), P(RefMoments)]
         L.ref_swarm_init.argtypes = [P(RefPop), ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint32,
@@ -201,11 +210,16 @@ def _d(a):
 class Pop:
     """Numpy-backed population for the oracle (x, v as n x 3)."""
 
-    def __init__(self, x, v, w, s, active, ids=None, cap=None, next_id=None):
+    def __init__(self, x, v, w, s, active, ids=None, cap=None, next_id=None, nu=None):
         x = np.asarray(x, dtype=np.float64).reshape(-1, 3)
         n = x.shape[0]
         cap = max(int(cap if cap is not None else n), n, 1)
         self.cap = cap
+        # photons (PhotonState, photon.jl:11-18) carry nu; electrons do not
+        self.nu = None
+        if nu is not None:
+            self.nu = np.zeros(cap)
+            self.nu[:n] = nu
         self.x = np.zeros((cap, 3))
         self.v = np.zeros((cap, 3))
         self.w = np.zeros(cap)
@@ -234,6 +248,8 @@ class Pop:
         p.s = _d(self.s)
         p.active = self.active.ctypes.data_as(_PU8)
         p.id = self.id.ctypes.data_as(_PU64)
+        if self.nu is not None:
+            p.nu = _d(self.nu)
         return p
 
     def sync_from(self, p):
@@ -242,15 +258,33 @@ class Pop:
 
     def view(self):
         n = self.n
-        return dict(x=self.x[:n].copy(), v=self.v[:n].copy(), w=self.w[:n].copy(),
-                    s=self.s[:n].copy(), active=self.active[:n].copy(), id=self.id[:n].copy())
+        d = dict(x=self.x[:n].copy(), v=self.v[:n].copy(), w=self.w[:n].copy(),
+                 s=self.s[:n].copy(), active=self.active[:n].copy(), id=self.id[:n].copy())
+        if self.nu is not None:
+            d["nu"] = self.nu[:n].copy()
+        return d
+
+    @classmethod
+    def photons(cls, cap, x=None, v=None, nu=None, w=None, s=None, active=None, ids=None, next_id=None):
+        """A photon population (empty unless the columns are given)."""
+        if x is None:
+            z = np.zeros((0, 3))
+            return cls(z, z, [], [], [], cap=cap, next_id=next_id or 0, nu=[])
+        return cls(x, v, w, s, active, ids=ids, cap=cap, next_id=next_id, nu=nu)
 
     def scratch(self):
         return Pop(np.zeros((0, 3)), np.zeros((0, 3)), [], [], [], cap=self.cap)
 
 
 class Table:
-    def __init__(self, energy, rate, kind, param, maxrate, egrid=0):
+    def __init__(self, energy, rate, kind, param, maxrate, egrid=0, photo=None):
+        # photo: {row: (numin, numax, photon_multiplier, photon_weight)} for the
+        # PhotoEmission rows (kind 5); the logs as PhotoEmission's constructor takes them
+        self.photo = None
+        if photo:
+            self.photo = np.zeros((len(kind), 4))
+            for j, (numin, numax, mult, pw) in photo.items():
+                self.photo[j] = (np.log(numin), np.log(numax), mult, pw)
         self.energy = np.ascontiguousarray(energy, dtype=np.float64)
         self.rate_pm = np.ascontiguousarray(rate, dtype=np.float64)  # nproc x ne
         self.rate = np.ascontiguousarray(self.rate_pm.T)            # energy-major
@@ -269,6 +303,8 @@ class Table:
         t.kind = self.kind.ctypes.data_as(_PI32)
         t.param = _d(self.param)
         t.maxrate = self.maxrate
+        if self.photo is not None:
+            t.photo = _d(self.photo)
         return t
 
 
@@ -468,6 +504,38 @@ def advance(pop, table, field, dt, tracker_mesh=None, seed=0, instance=0, step=0
                   int(seed), int(instance), int(step), ctypes.byref(st))
     pop.sync_from(p)
     return st
+
+
+def advance_mixed(epop, ppop, table, numax, field, dt, tracker_mesh=None, seed=0, instance=0, step=0):
+    """advance! over electrons and photons; returns (electron stats, photon stats)."""
+    L = lib()
+    e = epop.cstruct()
+    p = ppop.cstruct() if ppop is not None else None
+    t = table.cstruct()
+    est, pst = RefStats(), RefStats()
+    trk = ctypes.pointer(tracker_mesh.cstruct()) if tracker_mesh is not None else None
+    L.ref_advance_mixed(ctypes.byref(e), ctypes.byref(p) if p is not None else None, ctypes.byref(t),
+                        float(numax), ctypes.byref(field), float(dt), trk, int(seed), int(instance), int(step),
+                        ctypes.byref(est), ctypes.byref(pst))
+    epop.sync_from(e)
+    if ppop is not None:
+        ppop.sync_from(p)
+    return est, pst
+
+
+def advance_cyl_mixed(epop, ppop, table, numax, grid, dt, track=True, seed=0, instance=0, step=0):
+    L = lib()
+    e = epop.cstruct()
+    p = ppop.cstruct() if ppop is not None else None
+    t = table.cstruct()
+    est, pst = RefStats(), RefStats()
+    L.ref_advance_cyl_mixed(ctypes.byref(e), ctypes.byref(p) if p is not None else None, ctypes.byref(t),
+                            float(numax), ctypes.byref(grid.cstruct()), float(dt), int(bool(track)), int(seed),
+                            int(instance), int(step), ctypes.byref(est), ctypes.byref(pst))
+    epop.sync_from(e)
+    if ppop is not None:
+        ppop.sync_from(p)
+    return est, pst
 
 
 def repack(pop):

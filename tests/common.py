@@ -12,22 +12,26 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 GOLDEN = os.path.join(HERE, "golden")
 
 
-def gas_table(ne=4000, egrid="log", o2=0.0, tmpdir=None):
-    """(proc, energy_J, rate, maxrate) of the synthetic N2-like (+O2-like) gas."""
+def gas_table(ne=4000, egrid="log", o2=0.0, tmpdir=None, photo=False, photon_weight=1.0, photon_multiplier=1.0,
+              photo_rescale=0.05):
+    """(proc, energy_J, rate, maxrate) of the synthetic N2-like (+O2-like) gas,
+    with photoemission (absorbed by the O2-like gas) when `photo`."""
     if egrid == "log":
         e_ev = np.geomspace(1e-3, 1e3, ne)
     else:
         e_ev = np.linspace(0.0, 1e3, ne)
     d = tmpdir or tempfile.gettempdir()
-    path = os.path.join(d, f"mcs_gas_{egrid}_{ne}_{int(o2 > 0)}_{os.getpid()}.json")
-    gases.write_gas(path, e_ev, with_attacher=o2 > 0)
-    proc, rate, maxrate, _ = lxcat.load_lxcat([path], gases.densities(o2), e_ev * co.eV)
+    path = os.path.join(d, f"mcs_gas_{egrid}_{ne}_{int(o2 > 0)}_{int(photo)}_{photo_rescale}_{os.getpid()}.json")
+    gases.write_gas(path, e_ev, with_attacher=o2 > 0, with_photo=photo, photo_rescale=photo_rescale)
+    proc, rate, maxrate, _ = lxcat.load_lxcat([path], gases.densities(o2), e_ev * co.eV,
+                                              photon_weight=photon_weight, photon_multiplier=photon_multiplier)
     return proc, e_ev * co.eV, rate, maxrate
 
 
 def ref_table(proc, energy, rate, maxrate, egrid="log"):
+    photo = {j: p.photo for j, p in enumerate(proc) if p.kind == "PHOTOEMISSION"}
     return R.Table(energy, rate, [p.code for p in proc], [p.param for p in proc], maxrate,
-                   egrid=1 if egrid == "log" else 0)
+                   egrid=1 if egrid == "log" else 0, photo=photo or None)
 
 
 def maxwell_states(n, kT_ev=1.0, seed=0, x0=(0.0, 0.0, 0.0), spread=0.0):
