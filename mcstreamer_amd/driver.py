@@ -8,8 +8,11 @@
 The TOML schema is the reference's (samples/input.toml): eb and field_units,
 nair, n, maxp, ntarget, nmax, dt, tmax, cross_sections (LXCat JSON paths,
 relative to the input file), rng_seed, initial_weight, poisson_levels,
-[domain] R L M N, [seed] z0 z1 w, [interval] poisson output resample.  Sections
-this build does not carry (photoionization, init_files, denoise, needle) raise
+[domain] R L M N, [seed] z0 z1 w, [interval] poisson output resample,
+[photoionization] photon_weight photon_multiplier (main.jl:59-68): with a
+PHOTOEMISSION process in the cross sections the run carries a photon population
+of maxp slots (main.jl:113-122) and steps MultiPopulation(electrons, photons).
+Sections this build does not carry (init_files, denoise, needle) raise
 NotImplementedError rather than being ignored.  The seed positions come from
 numpy's generator seeded with rng_seed (the reference draws them from Julia's
 global RNG; the distribution, sampseg, is the same); everything after the
@@ -24,9 +27,10 @@ import numpy as np
 
 from . import axisym, lxcat
 from . import constants as co
-from .jumc import CollisionTable, ParticleStates, Population, advance, context, repack
+from .jumc import (CollisionTable, MultiPopulation, ParticleStates, PhotonPopulation, PhotonStates, Population,
+                   ZhelezniakCollisions, advance, context, repack)
 
-UNSUPPORTED = ("photoionization", "init_files", "denoise", "needle")
+UNSUPPORTED = ("init_files", "denoise", "needle")
 
 
 class TimeStepper:
@@ -97,7 +101,10 @@ class Run:
         densities = {"N2": nair * 0.79, "O2": nair * 0.21}
         energy = np.linspace(0.0, 1000 * co.eV, energy_points)   # LinRange(0, 1000 eV, 100_000)
         files = [os.path.join(os.path.dirname(os.path.abspath(finput)), f) for f in inp["cross_sections"]]
-        proc, rate, maxrate, _ = lxcat.load_lxcat(files, densities, energy)
+        photo = inp.get("photoionization", {})
+        proc, rate, maxrate, _ = lxcat.load_lxcat(files, densities, energy,
+                                                  photon_weight=float(photo.get("photon_weight", 1.0)),
+                                                  photon_multiplier=float(photo.get("photon_multiplier", 1.0)))
         self.table = CollisionTable(proc, energy, rate, maxrate, egrid="uniform", ctx=self.ctx)
         self.dt = float(inp["dt"])
         self.tmax = float(inp["tmax"])
@@ -113,6 +120,12 @@ class Run:
         self.popl = Population(int(inp["maxp"]), ParticleStates(x, np.zeros((n, 3)), np.full(n, wt), sclock,
                                                                   np.ones(n, dtype=np.uint8)),
                                self.table, seed=seed, ctx=self.ctx)
+        # photo-ionization only with photo-emission (main.jl:113-122)
+        pcolls = ZhelezniakCollisions.for_table(self.table)
+        self.photons = None
+        if pcolls is not None:
+            self.photons = PhotonPopulation(int(inp["maxp"]), collisions=pcolls, ctx=self.ctx)
+        self.mpopl = MultiPopulation(self.popl, self.photons)
         axisym.density(self.fields, "qfixed", self.popl, 1.0)     # main.jl:130-132
         levels = inp.get("poisson_levels")
         self.conf = axisym.poisson_config(self.grid, levels=None if levels is None else int(levels))
@@ -142,12 +155,22 @@ class Run:
             self.poisson_ts.atstep(lambda j: axisym.poisson(self.fields, self.popl, self.eb, self.ws), t)
             if output:
                 self.output_ts.atstep(self.save_fields, t)
-            st = advance(self.popl, self.efield, self.dt, self.tracker, step=self.i - 1)
+            if self.photons is None:
+                st = advance(self.popl, self.efield, self.dt, self.tracker, step=self.i - 1)
+                pst = None
+            else:
+                st, pst = advance(self.mpopl, self.efield, self.dt, self.tracker, step=self.i - 1)
             m = self.popl.moments()
             rec = {"t": t, "n": st.n_out, "births": st.births, "deaths": st.deaths, "nactive": m.nactive,
                    "weight": m.weight, "mean_energy_ev": (m.wenergy / m.weight / co.eV) if m.weight else 0.0}
+            if pst is not None:
+                pm = self.photons.moments()
+                rec.update(photons=pm.nactive, photon_weight=pm.weight if pm.nactive else 0.0,
+                           photons_emitted=pst.n_out - pst.n_in, photoelectrons=pst.births, n=st.n_out + pst.births)
 
             def do_resample(j):
+                if self.photons is not None:
+                    repack(self.photons)       # main.jl:285
                 repack(self.popl)
                 rs = axisym.resample(self.popl, self.fields, self.ntarget, self.nmax, step=self.i - 1)
                 rec["resample"] = (rs.n_in, rs.n_out)
@@ -182,9 +205,14 @@ def save(run, path):
     ps = run.popl.download()
     meta = {"format": CHECKPOINT_FORMAT, "finput": run.finput, "energy_points": run.energy_points, "i": run.i,
             "ts": [run.poisson_ts.i, run.output_ts.i, run.resample_ts.i], "next_id": int(ps.next_id)}
+    extra = {}
+    if run.photons is not None:
+        ph = run.photons.download()
+        meta["photon_next_id"] = int(ph.next_id)
+        extra = {f"photon_{k}": getattr(ph, k) for k in ("x", "v", "nu", "w", "s", "active", "id")}
     np.savez(path, meta=np.array(json.dumps(meta)), x=ps.x, v=ps.v, w=ps.w, s=ps.s, active=ps.active, id=ps.id,
              qfixed=run.fields.get("qfixed"), u=run.fields.get("u"), er=run.fields.get("er"),
-             ez=run.fields.get("ez"))
+             ez=run.fields.get("ez"), **extra)
 
 
 def load(path, ctx=None):
@@ -195,7 +223,14 @@ def load(path, ctx=None):
         if meta.get("format") != CHECKPOINT_FORMAT:
             raise ValueError(f"{path}: not a {CHECKPOINT_FORMAT} file")
         arrs = {k: z[k] for k in ("x", "v", "w", "s", "active", "id", "qfixed", "u", "er", "ez")}
+        phot = {k: z[f"photon_{k}"] for k in ("x", "v", "nu", "w", "s", "active", "id")} \
+            if "photon_next_id" in meta else None
     run = Run(meta["finput"], ctx=ctx, energy_points=int(meta["energy_points"]))
+    if (phot is None) != (run.photons is None):
+        raise ValueError(f"{path}: photon population does not match the input's cross sections")
+    if phot is not None:
+        run.photons.upload(PhotonStates(phot["x"], phot["v"], phot["nu"], phot["w"], phot["s"], phot["active"],
+                                        phot["id"], int(meta["photon_next_id"])))
     run.popl.upload(ParticleStates(arrs["x"], arrs["v"], arrs["w"], arrs["s"], arrs["active"], arrs["id"],
                                    int(meta["next_id"])))
     run.fields.set("qfixed", arrs["qfixed"])

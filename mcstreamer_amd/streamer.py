@@ -18,7 +18,7 @@ import numpy as np
 from . import _lib
 from . import constants as co
 from ._lib import check, dptr
-from .jumc import advance, context
+from .jumc import MultiPopulation, advance, context, repack
 from . import multigrid as mgmod
 
 
@@ -146,10 +146,15 @@ def resample(popl, fields, ntarget, nmax, step=0):
 class Streamer:
     """The per-step loop of nsteps (main.jl:217-315) with Poisson and resampling
     every step: deposit + solve + faces, advance (mesh field, ion tracker),
-    diagnostics, canonical sort (the repack), per-cell resample."""
+    diagnostics, canonical sort (the repack), per-cell resample.  With a photon
+    population (a table with PhotoEmission rows) the step advances
+    MultiPopulation(electrons, photons) and repacks the photons before the
+    resample (main.jl:285)."""
 
-    def __init__(self, popl, fields, ws, eb, dt, ntarget, nmax, scheme="cic"):
+    def __init__(self, popl, fields, ws, eb, dt, ntarget, nmax, scheme="cic", photons=None):
         self.popl = popl
+        self.photons = photons
+        self.mpopl = MultiPopulation(popl, photons) if photons is not None else None
         self.fields = fields
         self.ws = ws
         self.eb = float(eb)
@@ -164,12 +169,21 @@ class Streamer:
 
     def step(self, diagnostics=True):
         it, res = poisson(self.fields, self.popl, self.eb, self.ws, self.scheme)
-        st = advance(self.popl, self.field, self.dt, self.tracker, step=self.i)
+        pst = None
+        if self.mpopl is None:
+            st = advance(self.popl, self.field, self.dt, self.tracker, step=self.i)
+        else:
+            st, pst = advance(self.mpopl, self.field, self.dt, self.tracker, step=self.i)
         mom = self.popl.moments() if diagnostics else None
+        if self.photons is not None:
+            repack(self.photons)
         sort_by_cell(self.popl, self.fields)
         rs = resample(self.popl, self.fields, self.ntarget, self.nmax, step=self.i)
         self.i += 1
-        return dict(mg_iters=it, mg_resid=res, advance=st, moments=mom, resample=rs)
+        out = dict(mg_iters=it, mg_resid=res, advance=st, moments=mom, resample=rs)
+        if pst is not None:
+            out["photons"] = pst
+        return out
 
     def nsteps(self, n):
         out = None
@@ -189,24 +203,30 @@ def seed_pairs(grid, n, center, sigma, rng):
 
 
 def make_streamer(n=(128, 128, 256), h=5e-6, npairs=1_000_000, seed=7, levels=None, cap=None,
-                  precision="f64", tolerance=-1e-8, sigma=50e-6, ctx=None):
+                  precision="f64", tolerance=-1e-8, sigma=50e-6, ctx=None, photo=None):
     """A streamer workload on an n-cell box of cell size h: C0 gas + 20% attacher,
     E = 5 MV/m along z, neutral Gaussian seed of npairs (width sigma) at
     (Lx/2, Ly/2, 0.8 Lz), CIC, V(2,2) cycles to `tolerance` (negative: relative),
     merge/split 8..64 per cell, dt = 2e-13 s.  precision "f32" gives the split
     particle store, the FP32 step and FP32 correction cycles inside FP64
-    refinement; "f64" the FP64 path."""
+    refinement; "f64" the FP64 path.  photo = dict(rescale=, photon_multiplier=,
+    photon_weight=) adds the gas's photoemission and a photon population of the
+    same capacity (returned as "photons")."""
     from . import gases, lxcat
-    from .jumc import CollisionTable, Population, ParticleStates
+    from .jumc import CollisionTable, PhotonPopulation, Population, ParticleStates, ZhelezniakCollisions
     import os
     import tempfile
 
     ctx = ctx or context()
     grid = Grid3(n, h)
     e_ev = np.geomspace(1e-3, 1e3, 4000)
-    path = os.path.join(tempfile.gettempdir(), f"mcs_gas_c1_{os.getpid()}.json")
-    gases.write_gas(path, e_ev, with_attacher=True)
-    proc, rate, maxrate, _ = lxcat.load_lxcat([path], gases.densities(0.2), e_ev * co.eV)
+    ph = dict(photo or {})
+    path = os.path.join(tempfile.gettempdir(), f"mcs_gas_c1_{int(photo is not None)}_{os.getpid()}.json")
+    gases.write_gas(path, e_ev, with_attacher=True, with_photo=photo is not None,
+                    photo_rescale=float(ph.get("rescale", 0.05)))
+    proc, rate, maxrate, _ = lxcat.load_lxcat([path], gases.densities(0.2), e_ev * co.eV,
+                                              photon_weight=float(ph.get("photon_weight", 1.0)),
+                                              photon_multiplier=float(ph.get("photon_multiplier", 1.0)))
     table = CollisionTable(proc, e_ev * co.eV, rate, maxrate, egrid="log", ctx=ctx)
     rng = np.random.default_rng(seed)
     L = np.array(n, dtype=np.float64) * h
@@ -233,15 +253,18 @@ def make_streamer(n=(128, 128, 256), h=5e-6, npairs=1_000_000, seed=7, levels=No
         s=s_, levels=levels, tolerance=tolerance, smooth1=2, smooth2=2, maxiter=50, precision=precision)
     ws = mgmod.Workspace(conf, n, ctx=ctx)
     dt = 2e-13
+    zc = ZhelezniakCollisions.for_table(table)
+    photons = PhotonPopulation(cap, collisions=zc, ctx=ctx) if zc is not None else None
     return dict(grid=grid, table=table, popl=popl, fields=fields, ws=ws, conf=conf, eb=eb, dt=dt,
-                ntarget=8, nmax=64)
+                ntarget=8, nmax=64, photons=photons)
 
 
-def make_config1(n=(128, 128, 256), h=5e-6, npairs=1_000_000, seed=7, levels=None, cap=None, ctx=None):
+def make_config1(n=(128, 128, 256), h=5e-6, npairs=1_000_000, seed=7, levels=None, cap=None, ctx=None, photo=None):
     """The FP64 streamer workload: 128x128x256 cells of 5 um, 1e6-pair seed of
     width 50 um, V(2,2) to 1e-8 relative residual.  The default capacity (16
     particles per seed pair) holds the avalanche of the full 500-step run."""
-    return make_streamer(n, h, npairs, seed, levels, cap or 16 * npairs, "f64", -1e-8, 50e-6 * (h / 5e-6), ctx)
+    return make_streamer(n, h, npairs, seed, levels, cap or 16 * npairs, "f64", -1e-8, 50e-6 * (h / 5e-6), ctx,
+                         photo)
 
 
 def make_config2(precision="f32", n=(256, 256, 512), h=2.5e-6, npairs=40_000_000, seed=7, levels=None,

@@ -37,10 +37,15 @@ resample = 1e-12
 """
 
 
-def _gas(path):
-    items = gases.n2_like_items(np.geomspace(1e-3, 1e3, 400)) + gases.o2_like_items(np.geomspace(1e-3, 1e3, 400))
+def _gas(path, photo=False):
+    e = np.geomspace(1e-3, 1e3, 400)
+    items = gases.n2_like_items(e) + gases.o2_like_items(e)
+    if photo:
+        items += gases.photo_items(e, rescale=0.5)
     for it in items:
         it["target"] = {"N2like": "N2", "O2like": "O2"}[it["target"]]
+        if "absorber" in it:
+            it["absorber"] = "O2"
     with open(path, "w") as f:
         json.dump(items, f)
 
@@ -88,3 +93,30 @@ def test_driver_checkpoint_continues_bitwise(tmp_path):
     for k in ("u", "qfixed", "er", "ez"):
         np.testing.assert_array_equal(a.fields.get(k), b.fields.get(k), err_msg=k)
 
+
+
+def test_driver_photoionization_run_and_checkpoint(tmp_path):
+    """[photoionization] with a PHOTOEMISSION process: photons are emitted, fly,
+    are absorbed into photoelectrons, and a checkpoint (electrons and photons)
+    continues bitwise."""
+    _gas(tmp_path / "gas.json", photo=True)
+    p = tmp_path / "input.toml"
+    p.write_text(TOML + "\n[photoionization]\nphoton_weight = 1.0\nphoton_multiplier = 4.0\n")
+    a = driver.setup(str(p), energy_points=4000)
+    assert a.photons is not None and a.table.photoemission().photo[2] == 4.0
+    a.nsteps(3, output=False)
+    ck = str(tmp_path / "ck.npz")
+    driver.save(a, ck)
+    a.nsteps(4, output=False)
+    print([(r["photons"], r["photons_emitted"], r["photoelectrons"]) for r in a.log], flush=True)
+    assert sum(r["photons_emitted"] for r in a.log) > 0
+    assert sum(r["photoelectrons"] for r in a.log) > 0
+    b = driver.load(ck)
+    b.nsteps(4, output=False)
+    for pa, pb, keys in ((a.popl, b.popl, ("x", "v", "w", "s", "active", "id")),
+                         (a.photons, b.photons, ("x", "v", "nu", "w", "s", "active", "id"))):
+        ga, gb = pa.download(), pb.download()
+        assert len(ga) == len(gb) and ga.next_id == gb.next_id
+        for k in keys:
+            np.testing.assert_array_equal(getattr(ga, k), getattr(gb, k), err_msg=k)
+    np.testing.assert_array_equal(a.fields.get("qfixed"), b.fields.get("qfixed"))

@@ -186,3 +186,51 @@ def test_photon_misuse_is_refused():
     with pytest.raises(_lib.MCSError, match="Maximum number of photons"):
         for _ in range(5):
             jumc.advance(jumc.MultiPopulation(el, ph), jumc.FixedField(-5e6), 1e-12)
+
+
+def test_streamer_steps_with_photons_match_oracle():
+    """Whole 3D streamer steps with photoionization: poisson! (CIC deposit,
+    multigrid, faces), advance! over electrons and photons with the tracker,
+    photon repack!, the canonical sort and resample!, against the same chain in
+    the oracle; counts bit-exact every step, particles and photons bitwise."""
+    n, npairs, cap = (32, 32, 64), 20_000, 400_000
+    cfg = S.make_config1(n=n, npairs=npairs, cap=cap, seed=5,
+                         photo=dict(rescale=1.0, photon_multiplier=8.0, photon_weight=1.0))
+    # the seed at 20 eV instead of at rest, so that the first steps photoemit
+    ps = cfg["popl"].download()
+    rng = np.random.default_rng(6)
+    v = rng.normal(size=ps.v.shape) * np.sqrt(20.0 * co.eV / co.electron_mass)
+    cfg["popl"].upload(jumc.ParticleStates(ps.x, v, ps.w, ps.s, ps.active, ps.id, ps.next_id))
+    sim = S.Streamer(cfg["popl"], cfg["fields"], cfg["ws"], cfg["eb"], cfg["dt"], cfg["ntarget"], cfg["nmax"],
+                     photons=cfg["photons"])
+    ps = sim.popl.download()
+    E = R.Pop(ps.x, ps.v, ps.w, ps.s, ps.active, ps.id, cap=cap, next_id=ps.next_id)
+    P = R.Pop.photons(cap=cap)
+    M = R.Mesh(n, 5e-6)
+    M.qfixed[:] = sim.fields.get("qfixed")
+    proc = cfg["table"].proc
+    T = ref_table(proc, cfg["table"].energy, cfg["table"].rate, cfg["table"].maxrate)
+    numax = cfg["photons"].collisions.numax
+    conf = cfg["conf"]
+    R.sort_by_cell(E, M)
+    for i in range(3):
+        out = sim.step()
+        M.qpart[:] = 0.0
+        R.deposit(E, M, 1, 1, -1.0)
+        R.assemble_charge(M)
+        rc, it, res = R.mg_solve(n, conf.bc, conf.s, conf.levels, conf.smooth1, conf.smooth2, conf.tolerance,
+                                 conf.maxiter, M.u, M.q)
+        R.efield_faces(M, sim.eb)
+        est, pst = R.advance_mixed(E, P, T, numax, R.mesh_field(M), sim.dt, M, seed=sim.popl.seed, step=i)
+        R.repack(P)
+        R.sort_by_cell(E, M)
+        rc2, rst = R.resample_cells(E, M, sim.ntarget, sim.nmax, seed=sim.popl.seed, step=i)
+        print(f"step {i}: electrons {E.n} photons {P.n} emitted {pst.n_out - pst.n_in} absorbed {pst.events[0]}",
+              flush=True)
+        assert out["mg_iters"] == it
+        _check_stats(out["advance"], est, len(proc), f"electron loop, step {i}")
+        _check_stats(out["photons"], pst, 1, f"photon loop, step {i}")
+        assert out["resample"].n_out == rst.n_out
+    assert pst.n_out > 0
+    _compare(jumc.MultiPopulation(sim.popl, sim.photons), E, P)
+    np.testing.assert_array_equal(sim.fields.get("qfixed"), M.qfixed)
