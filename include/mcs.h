@@ -163,15 +163,16 @@ int32_t mcs_table_create(mcs_ctx *ctx, int32_t nproc, int32_t ne, int32_t egrid_
                          const double *energy, const double *rate, const int32_t *kind,
                          const double *param, double maxrate, mcs_table **out);
 int32_t mcs_table_destroy(mcs_table *t);
-/* PhotoEmission(numin, numax, photon_multiplier, photon_weight) (JuMC/src/electron.jl:
- * 88-105) for row `row`, created with kind MCS_PROC_PHOTOEMISSION (its param is
- * unused).  The row's collision (electron.jl:141-144, collisions.jl:125-141) leaves
- * the electron as it is and emits Poisson((w / photon_multiplier) / photon_weight)
- * photons at its position, each of weight photon_weight, velocity c randsphere()
- * and absorption rate nu = exp(log numin + (log numax - log numin) U); the logs are
- * taken here, as in the reference's constructor.  Every photoemission row must be
- * set before a step uses the table; such a table needs mcs_advance_mixed. */
-int32_t mcs_table_set_photoemission(mcs_table *t, int32_t row, double numin, double numax,
+/* The fields of PhotoEmission (JuMC/src/electron.jl:88-105) for row `row`, created
+ * with kind MCS_PROC_PHOTOEMISSION (its param is unused): log_numin and log_numax
+ * as the struct stores them (the logs its constructor takes), photon_multiplier,
+ * photon_weight.  The row's collision (electron.jl:141-144, collisions.jl:125-141)
+ * leaves the electron as it is and emits Poisson((w / photon_multiplier) /
+ * photon_weight) photons at its position, each of weight photon_weight, velocity
+ * c randsphere() and absorption rate nu = exp(log_numin + (log_numax - log_numin) U).
+ * Every photoemission row must be set before a step uses the table; such a table
+ * needs mcs_advance_mixed. */
+int32_t mcs_table_set_photoemission(mcs_table *t, int32_t row, double log_numin, double log_numax,
                                     double photon_multiplier, double photon_weight);
 
 /* ---- particle population: JuMC.Population (JuMC/src/population.jl:4-36) -----

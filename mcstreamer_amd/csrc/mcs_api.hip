@@ -386,20 +386,20 @@ extern "C" int32_t mcs_table_create(mcs_ctx *ctx, int32_t nproc, int32_t ne, int
     return MCS_OK;
 }
 
-// PhotoEmission's constructor (electron.jl:101-104) takes the logs of the rates.
-extern "C" int32_t mcs_table_set_photoemission(mcs_table *t, int32_t row, double numin, double numax,
+extern "C" int32_t mcs_table_set_photoemission(mcs_table *t, int32_t row, double log_numin, double log_numax,
                                                double photon_multiplier, double photon_weight)
 {
     if (!t) return MCS_E_ARG;
     mcs_ctx *ctx = t->ctx;
     if (row < 0 || row >= t->nproc || t->h_kind[row] != MCS_PROC_PHOTOEMISSION)
         return mcs_fail(ctx, MCS_E_ARG, "table: row %d is not a photoemission row", row);
-    if (!(numin > 0.0) || !(numax >= numin) || !(photon_multiplier > 0.0) || !(photon_weight > 0.0))
+    if (!(log_numax >= log_numin) || !std::isfinite(log_numin) || !std::isfinite(log_numax) ||
+        !(photon_multiplier > 0.0) || !(photon_weight > 0.0))
         return mcs_fail(ctx, MCS_E_ARG,
-                        "table: photoemission needs 0 < numin <= numax and positive multiplier and weight");
+                        "table: photoemission needs finite log_numin <= log_numax and positive multiplier and weight");
     double *r = &t->h_photo[(size_t)row * 4];
-    r[0] = std::log(numin);
-    r[1] = std::log(numax);
+    r[0] = log_numin;
+    r[1] = log_numax;
     r[2] = photon_multiplier;
     r[3] = photon_weight;
     hipSetDevice(ctx->device);

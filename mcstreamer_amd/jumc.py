@@ -101,7 +101,7 @@ class CollisionTable:
             if p.code == _lib.PROC_PHOTOEMISSION:
                 numin, numax, mult, pw = p.photo
                 check(self.ctx.handle, _lib.lib().mcs_table_set_photoemission(
-                    h, j, float(numin), float(numax), float(mult), float(pw)))
+                    h, j, math.log(numin), math.log(numax), float(mult), float(pw)))
 
     def photoemission(self):
         """The first PhotoEmission process, or None (main.jl:115)."""

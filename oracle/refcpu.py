@@ -7,6 +7,7 @@ refcpu.c with gcc (contraction off) into oracle/librefcpu.so.
 import ctypes
 import os
 import subprocess
+from math import log as _log
 
 import numpy as np
 
@@ -284,7 +285,7 @@ class Table:
         if photo:
             self.photo = np.zeros((len(kind), 4))
             for j, (numin, numax, mult, pw) in photo.items():
-                self.photo[j] = (np.log(numin), np.log(numax), mult, pw)
+                self.photo[j] = (_log(numin), _log(numax), mult, pw)
         self.energy = np.ascontiguousarray(energy, dtype=np.float64)
         self.rate_pm = np.ascontiguousarray(rate, dtype=np.float64)  # nproc x ne
         self.rate = np.ascontiguousarray(self.rate_pm.T)            # energy-major
