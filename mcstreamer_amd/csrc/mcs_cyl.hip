@@ -368,7 +368,10 @@ extern "C" int32_t mcs_cgrid_create(mcs_ctx *ctx, double R, double L, int32_t M,
             mcs_cgrid_destroy(g);
             return mcs_fail(ctx, MCS_E_HIP, "cgrid: device allocation failed");
         }
-        hipMemsetAsync(g->arr[a], 0, sizeof(double) * g->arr_len[a], ctx->stream);
+        if (hipMemsetAsync(g->arr[a], 0, sizeof(double) * g->arr_len[a], ctx->stream) != hipSuccess) {
+            mcs_cgrid_destroy(g);
+            return mcs_fail(ctx, MCS_E_HIP, "cgrid: clearing the arrays failed");
+        }
     }
     if (hipMalloc(&g->dv, sizeof(double) * M) != hipSuccess) {
         mcs_cgrid_destroy(g);

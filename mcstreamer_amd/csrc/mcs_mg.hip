@@ -1607,14 +1607,14 @@ extern "C" int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg 
         if (hipMalloc(&L.res, sizeof(double) * L.len) != hipSuccess ||
             hipMalloc(&L.sol, sizeof(double) * L.len) != hipSuccess)
             return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation failed");
-        hipMemset(L.res, 0, sizeof(double) * L.len);
-        hipMemset(L.sol, 0, sizeof(double) * L.len);
+        MCS_HIP(ctx, hipMemset(L.res, 0, sizeof(double) * L.len));
+        MCS_HIP(ctx, hipMemset(L.sol, 0, sizeof(double) * L.len));
         if (f32) {
             if (hipMalloc(&L.fres, sizeof(float) * L.len) != hipSuccess ||
                 hipMalloc(&L.fsol, sizeof(float) * L.len) != hipSuccess)
                 return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation failed");
-            hipMemset(L.fres, 0, sizeof(float) * L.len);
-            hipMemset(L.fsol, 0, sizeof(float) * L.len);
+            MCS_HIP(ctx, hipMemset(L.fres, 0, sizeof(float) * L.len));
+            MCS_HIP(ctx, hipMemset(L.fsol, 0, sizeof(float) * L.len));
         }
         mg->lv.push_back(L);
     }
@@ -1628,11 +1628,11 @@ extern "C" int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg 
             if (f32) {
                 if (hipMalloc(&L.falt, sizeof(float) * L.len) != hipSuccess)
                     return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation failed");
-                hipMemset(L.falt, 0, sizeof(float) * L.len);
+                MCS_HIP(ctx, hipMemset(L.falt, 0, sizeof(float) * L.len));
             } else {
                 if (hipMalloc(&L.alt, sizeof(double) * L.len) != hipSuccess)
                     return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation failed");
-                hipMemset(L.alt, 0, sizeof(double) * L.len);
+                MCS_HIP(ctx, hipMemset(L.alt, 0, sizeof(double) * L.len));
             }
         }
     }

@@ -712,10 +712,13 @@ int32_t mcs_mg2_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg **out)
             mcs_mg_destroy(mg);
             return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation failed");
         }
-        hipMemsetAsync(L.res, 0, sizeof(double) * L.len, ctx->stream);
-        hipMemsetAsync(L.res1, 0, sizeof(double) * L.len, ctx->stream);
-        hipMemsetAsync(L.sol, 0, sizeof(double) * L.len, ctx->stream);
         mg->lv.push_back(L);
+        if (hipMemsetAsync(L.res, 0, sizeof(double) * L.len, ctx->stream) != hipSuccess ||
+            hipMemsetAsync(L.res1, 0, sizeof(double) * L.len, ctx->stream) != hipSuccess ||
+            hipMemsetAsync(L.sol, 0, sizeof(double) * L.len, ctx->stream) != hipSuccess) {
+            mcs_mg_destroy(mg);
+            return mcs_fail(ctx, MCS_E_HIP, "mg: clearing the level arrays failed");
+        }
     }
     const int32_t ncn[2] = {c.n[0] >> c.levels, c.n[1] >> c.levels};
     std::vector<double> ainv;
