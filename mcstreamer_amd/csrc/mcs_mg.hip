@@ -438,7 +438,7 @@ __global__ void k_sum_partials(const double *partial, int64_t nb, double *out, u
 {
     __shared__ double sh[MCS_BLOCK];
     double acc = 0.0;
-    const int64_t bd = blockDim.x;
+    const int64_t bd = MCS_BLOCK;
     int64_t q = threadIdx.x;
     // eight loads in flight, added in the order of the plain loop
     for (; q + 7 * bd < nb; q += 8 * bd) {
@@ -485,13 +485,13 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_mg_coarse(const T *ainv, const T 
 {
     __shared__ T bt[4096];
     const int64_t N = (int64_t)C.n[0] * C.n[1] * C.n[2];
-    for (int64_t col = threadIdx.x; col < N; col += blockDim.x) {
+    for (int64_t col = threadIdx.x; col < N; col += MCS_BLOCK) {
         int64_t i, j, k;
         split3(col, (uint32_t)C.n[0], (uint32_t)C.n[1], i, j, k);
         bt[col] = res[g_index(C, i + 1, j + 1, k + 1)];
     }
     __syncthreads();
-    for (int64_t row = threadIdx.x; row < N; row += blockDim.x) {
+    for (int64_t row = threadIdx.x; row < N; row += MCS_BLOCK) {
         T acc = T(0);
         for (int64_t col = 0; col < N; ++col) acc += ainv[row * N + col] * bt[col];
         int64_t i, j, k;
@@ -801,7 +801,7 @@ __device__ void sm_ghosts(const SmoothArgs &A, const SmBox &B, const T *s, T *u_
     if (!touch) return;
     const int sy = B.R[0], sz = B.R[0] * B.R[1];
     const uint32_t neg = A.bc.neg;
-    for (BoxWalk w(E[0], E[1], threadIdx.x, blockDim.x); w.z < E[2]; w.next()) {
+    for (BoxWalk w(E[0], E[1], threadIdx.x, MG_SM_THREADS); w.z < E[2]; w.next()) {
         const int x = e0[0] + w.x, y = e0[1] + w.y, z = e0[2] + w.z;
         uint32_t flip = 0;
         int cx = x, cy = y, cz = z;
@@ -849,7 +849,7 @@ __global__ __launch_bounds__(MG_SM_THREADS) void k_mg_smooth(SmoothArgs A, const
     const int by = B.Rb[0], bz = B.Rb[0] * B.Rb[1];
     T *sb = s + sz * B.R[2];
     // both boxes, SM_LD cells per thread at a time so that the global loads overlap
-    for (BoxWalk w(B.R[0], B.R[1], threadIdx.x, blockDim.x); w.z < B.R[2];) {
+    for (BoxWalk w(B.R[0], B.R[1], threadIdx.x, MG_SM_THREADS); w.z < B.R[2];) {
         T v[SM_LD], vb[SM_LD];
         int li[SM_LD], lb[SM_LD];
 #pragma unroll
@@ -897,7 +897,7 @@ __global__ __launch_bounds__(MG_SM_THREADS) void k_mg_smooth(SmoothArgs A, const
         const int half = (W[0] + 1) / 2;
         // SM_GS cells of one colour per thread at a time: their right-hand sides are
         // loaded together, and none of them reads another
-        for (BoxWalk w(half, W[1], threadIdx.x, blockDim.x); w.z < W[2];) {
+        for (BoxWalk w(half, W[1], threadIdx.x, MG_SM_THREADS); w.z < W[2];) {
             int li[SM_GS], cx[SM_GS], cy[SM_GS], cz[SM_GS];
             T bv[SM_GS];
 #pragma unroll
@@ -932,7 +932,7 @@ __global__ __launch_bounds__(MG_SM_THREADS) void k_mg_smooth(SmoothArgs A, const
     }
     {
         const int ox = B.t0[0] - B.a0[0], oy = B.t0[1] - B.a0[1], oz = B.t0[2] - B.a0[2];
-        for (BoxWalk w(A.tile[0], A.tile[1], threadIdx.x, blockDim.x); w.z < A.tile[2]; w.next())
+        for (BoxWalk w(A.tile[0], A.tile[1], threadIdx.x, MG_SM_THREADS); w.z < A.tile[2]; w.next())
             u_out[g_at(L, B.t0[0] + w.x, B.t0[1] + w.y, B.t0[2] + w.z)] =
                 s[(ox + w.x) + sy * (oy + w.y) + sz * (oz + w.z)];
     }
@@ -941,7 +941,7 @@ __global__ __launch_bounds__(MG_SM_THREADS) void k_mg_smooth(SmoothArgs A, const
         C.n[0] = L.n[0] / 2;
         C.n[1] = L.n[1] / 2;
         C.n[2] = L.n[2] / 2;
-        for (BoxWalk w(A.tile[0] / 2, A.tile[1] / 2, threadIdx.x, blockDim.x); w.z < A.tile[2] / 2; w.next()) {
+        for (BoxWalk w(A.tile[0] / 2, A.tile[1] / 2, threadIdx.x, MG_SM_THREADS); w.z < A.tile[2] / 2; w.next()) {
             const int ic = B.t0[0] / 2 + w.x, jc = B.t0[1] / 2 + w.y, kc = B.t0[2] / 2 + w.z;
             T sum = T(0);
 #pragma unroll
@@ -974,7 +974,7 @@ struct Tail {
     BcCoef bc;
 };
 
-#define BLOOP(t, N) for (int64_t t = threadIdx.x; t < (N); t += blockDim.x)
+#define BLOOP(t, N) for (int64_t t = threadIdx.x; t < (N); t += MG_TAIL_THREADS)
 
 // Level lev's extents, read element by element (a copy of the indexed struct
 // goes through private memory).
@@ -1045,7 +1045,7 @@ __device__ void coarsest_in_registers(T *x, const T *b, const Lvl &Lv, const BcC
 template <typename E>
 __device__ __forceinline__ void tail_copy(E *dst, const E *src, int64_t n)
 {
-    const int64_t bd = blockDim.x;
+    const int64_t bd = MG_TAIL_THREADS;
     int64_t t = threadIdx.x;
     for (; t + 3 * bd < n; t += 4 * bd) {
         const E a0 = src[t], a1 = src[t + bd], a2 = src[t + 2 * bd], a3 = src[t + 3 * bd];
@@ -1212,7 +1212,7 @@ __device__ __forceinline__ void tail_gs(T *u, const T *b, const Lvl &L, const Bc
     const int nxh = L.n[0] / 2, ny = L.n[1];
     const int ncol = nxh * ny * L.n[2];
     const float rx = 1.0f / (float)nxh, ry = 1.0f / (float)ny;
-    for (int t = threadIdx.x; t < ncol; t += blockDim.x) {
+    for (int t = threadIdx.x; t < ncol; t += MG_TAIL_THREADS) {
         int ti, j;
         const int q = fdivmod(t, nxh, rx, ti);
         const int k = fdivmod(q, ny, ry, j);
@@ -1230,7 +1230,7 @@ template <typename T>
 __device__ __forceinline__ void tail_resid(T *rt, const T *u, const T *b, const Lvl &L, const BcCoef &bc)
 {
     const int N = L.n[0] * L.n[1] * L.n[2];
-    for (int t = threadIdx.x; t < N; t += blockDim.x) {
+    for (int t = threadIdx.x; t < N; t += MG_TAIL_THREADS) {
         int i, j, k;
         const int c = tail_cell(L, t, i, j, k);
         const T uc = u[c];
@@ -1247,7 +1247,7 @@ __device__ __forceinline__ void tail_restrict(T *rh, T *xh, const T *rt, const L
     C.n[2] = F.n[2] / 2;
     const int NC = C.n[0] * C.n[1] * C.n[2];
     const int sy = F.n[0], sz = F.n[0] * F.n[1];
-    for (int t = threadIdx.x; t < NC; t += blockDim.x) {
+    for (int t = threadIdx.x; t < NC; t += MG_TAIL_THREADS) {
         int ic, jc, kc;
         const int cc = tail_cell(C, t, ic, jc, kc);
         const int f = 2 * ic + sy * (2 * jc) + sz * (2 * kc);
@@ -1270,7 +1270,7 @@ template <typename T>
 __device__ __forceinline__ void tail_interp(T *x, const T *rh, const Lvl &L, bool add)
 {
     const int N = L.n[0] * L.n[1] * L.n[2];
-    for (int t = threadIdx.x; t < N; t += blockDim.x) {
+    for (int t = threadIdx.x; t < N; t += MG_TAIL_THREADS) {
         int i, j, k;
         const int c = tail_cell(L, t, i, j, k);
         const T v = prolong_at(rh, L, i, j, k);

@@ -139,7 +139,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k2_sum(const double *partial, int64
 {
     __shared__ double sh[MCS_BLOCK];
     double acc = 0.0;
-    const int64_t bd = blockDim.x;
+    const int64_t bd = MCS_BLOCK;
     int64_t q = threadIdx.x;
     // eight loads in flight, added in the order of the plain loop
     for (; q + 7 * bd < nb; q += 8 * bd) {
@@ -208,12 +208,12 @@ __global__ __launch_bounds__(MCS_BLOCK) void k2_coarse(const double *ainv, const
 {
     __shared__ double bt[4096];
     const int N = C.n[0] * C.n[1];
-    for (int col = threadIdx.x; col < N; col += blockDim.x) {
+    for (int col = threadIdx.x; col < N; col += MCS_BLOCK) {
         const int jc = col / C.n[0], ic = col - jc * C.n[0];
         bt[col] = res[g2(C, ic, jc)];
     }
     __syncthreads();
-    for (int row = threadIdx.x; row < N; row += blockDim.x) {
+    for (int row = threadIdx.x; row < N; row += MCS_BLOCK) {
         const double *a = ainv + (int64_t)row * N;
         double acc = 0.0;
         int col = 0;
@@ -258,7 +258,7 @@ struct Tail2 {
     uint32_t neg;
 };
 
-#define TLOOP(t, N) for (int64_t t = threadIdx.x; t < (N); t += blockDim.x)
+#define TLOOP(t, N) for (int64_t t = threadIdx.x; t < (N); t += MG2_TAIL_THREADS)
 
 __device__ __forceinline__ Lv2 t2lvl(const Tail2 &T, int l)
 {
