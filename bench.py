@@ -335,7 +335,7 @@ def run_batch(args):
     ctx.synchronize()
     t_admit = time.perf_counter() - t0
     print(f"admitted {len(insts)} instances, capacity {total}, {t_admit:.1f} s", file=sys.stderr, flush=True)
-    tune = b.autotune() if not args.no_autotune else {}
+    tune = b.autotune(log=lambda m: print(m, file=sys.stderr, flush=True)) if not args.no_autotune else {}
     print(f"autotune: {tune}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         b.step(1)
@@ -425,7 +425,7 @@ def swarm_block(args):
               instance=r.rid, relax_steps=pol.relax_steps, block_steps=pol.block_steps)
     ctx.synchronize()
     note(f"c3: {len(insts)} instances admitted, capacity {total}")
-    b.autotune()
+    b.autotune(log=note)
     note(f"c3: autotuned {getattr(b, 'tuned', None)}")
     b.step(3)
     ctx.synchronize()
@@ -444,17 +444,23 @@ def swarm_block(args):
     per_step = []
     for r in sample:
         tab = tables.get(r.o2)
-        pop = jumc.Population(2 * pol.capacity(r.n0), None, tab, seed=args.seed, instance=r.rid, ctx=ctx)
+        # the plain server's launch sequence: capacity and repack! cadence as there
+        pop = jumc.Population(4 * pol.capacity(r.n0), None, tab, seed=args.seed, instance=r.rid, ctx=ctx)
         pop.init_thermal(r.n0, pol.init_kT_ev)
         f = jumc.FixedField(_field(r.en_td)[2])
-        for _ in range(args.c3_plain_relax):
+
+        def plain_step(k):
             swarm.swarm_step(pop, f, request_dt(r.en_td), r.n0)
             pop.moments(tab)
+            if pol.compact_every and (k + 1) % pol.compact_every == 0:
+                jumc.repack(pop)
+
+        for k in range(args.c3_plain_relax):
+            plain_step(k)
         ctx.synchronize()
         t1 = time.perf_counter()
-        for _ in range(args.c3_plain_steps):
-            swarm.swarm_step(pop, f, request_dt(r.en_td), r.n0)
-            pop.moments(tab)
+        for k in range(args.c3_plain_steps):
+            plain_step(args.c3_plain_relax + k)
         ctx.synchronize()
         per_step.append((time.perf_counter() - t1) / args.c3_plain_steps)
         note(f"c3 plain: instance {r.rid} n0 {r.n0} {per_step[-1] * 1e3:.2f} ms/step")
@@ -467,7 +473,8 @@ def swarm_block(args):
         "batched_f32_solves_per_s": batched, "batched_ms_per_step": t_step * 1e3, "slots_flagged": flagged,
         "plain_f64_solves_per_s": 1.0 / plain_solve_s,
         "plain_method": f"stratified sample of {len(sample)} instances by n0, each timed over "
-                        f"{args.c3_plain_steps} steps after {args.c3_plain_relax} (advance!, tallies, resample!); "
+                        f"{args.c3_plain_steps} steps after {args.c3_plain_relax} (advance!, tallies, resample!, repack! every "
+                        f"{pol.compact_every} steps); "
                         f"mean step time x {args.solve_steps} steps per solve",
         "speedup_vs_plain": batched * plain_solve_s}
     print(f"c3: batched {batched:.3f} solves/s, plain {1.0 / plain_solve_s:.3f} solves/s", file=sys.stderr,

@@ -364,11 +364,14 @@ extern "C" int32_t mcs_table_create(mcs_ctx *ctx, int32_t nproc, int32_t ne, int
         delete t;
         return mcs_fail(ctx, MCS_E_HIP, "table: device allocation failed");
     }
-    hipMemcpy(t->energy, energy, sizeof(double) * ne, hipMemcpyHostToDevice);
-    hipMemcpy(t->rate, rate, sizeof(double) * (size_t)ne * nproc, hipMemcpyHostToDevice);
-    hipMemcpy(t->kind, kind, sizeof(int32_t) * nproc, hipMemcpyHostToDevice);
-    hipMemcpy(t->param, param, sizeof(double) * nproc, hipMemcpyHostToDevice);
-    hipMemcpy(t->photo, t->h_photo.data(), sizeof(double) * 4 * (size_t)nproc, hipMemcpyHostToDevice);
+    if (mcs_h2d(ctx, t->energy, energy, sizeof(double) * ne) != hipSuccess ||
+        mcs_h2d(ctx, t->rate, rate, sizeof(double) * (size_t)ne * nproc) != hipSuccess ||
+        mcs_h2d(ctx, t->kind, kind, sizeof(int32_t) * nproc) != hipSuccess ||
+        mcs_h2d(ctx, t->param, param, sizeof(double) * nproc) != hipSuccess ||
+        mcs_h2d(ctx, t->photo, t->h_photo.data(), sizeof(double) * 4 * (size_t)nproc) != hipSuccess) {
+        mcs_table_destroy(t);
+        return mcs_fail(ctx, MCS_E_HIP, "table: copy to the device failed");
+    }
     {
         std::vector<float> e32(energy, energy + ne), r32(rate, rate + (size_t)ne * nproc),
             p32(param, param + nproc);
@@ -378,9 +381,12 @@ extern "C" int32_t mcs_table_create(mcs_ctx *ctx, int32_t nproc, int32_t ne, int
             mcs_table_destroy(t);
             return mcs_fail(ctx, MCS_E_HIP, "table: device allocation failed");
         }
-        hipMemcpy(t->energy32, e32.data(), sizeof(float) * ne, hipMemcpyHostToDevice);
-        hipMemcpy(t->rate32, r32.data(), sizeof(float) * r32.size(), hipMemcpyHostToDevice);
-        hipMemcpy(t->param32, p32.data(), sizeof(float) * nproc, hipMemcpyHostToDevice);
+        if (mcs_h2d(ctx, t->energy32, e32.data(), sizeof(float) * ne) != hipSuccess ||
+            mcs_h2d(ctx, t->rate32, r32.data(), sizeof(float) * r32.size()) != hipSuccess ||
+            mcs_h2d(ctx, t->param32, p32.data(), sizeof(float) * nproc) != hipSuccess) {
+            mcs_table_destroy(t);
+            return mcs_fail(ctx, MCS_E_HIP, "table: copy to the device failed");
+        }
     }
     *out = t;
     return MCS_OK;
@@ -403,7 +409,7 @@ extern "C" int32_t mcs_table_set_photoemission(mcs_table *t, int32_t row, double
     r[2] = photon_multiplier;
     r[3] = photon_weight;
     hipSetDevice(ctx->device);
-    MCS_HIP(ctx, hipMemcpy(t->photo + (size_t)row * 4, r, sizeof(double) * 4, hipMemcpyHostToDevice));
+    MCS_HIP(ctx, mcs_h2d(ctx, t->photo + (size_t)row * 4, r, sizeof(double) * 4));
     t->photo_set[row] = 1;
     return MCS_OK;
 }
@@ -781,7 +787,7 @@ extern "C" int32_t mcs_photons_upload(mcs_pop *p, int64_t n, const double *x, co
     if (n > 0 && !nu) return mcs_fail(ctx, MCS_E_ARG, "photons_upload: null array");
     int32_t rc = upload_impl(p, n, x, v, w, s, active, id, next_id);
     if (rc) return rc;
-    if (n > 0) MCS_HIP(ctx, hipMemcpy(p->buf[0].nu, nu, sizeof(double) * n, hipMemcpyHostToDevice));
+    if (n > 0) MCS_HIP(ctx, mcs_h2d(ctx, p->buf[0].nu, nu, sizeof(double) * n));
     return MCS_OK;
 }
 
@@ -804,11 +810,11 @@ static int32_t upload_impl(mcs_pop *p, int64_t n, const double *x, const double 
         double *dx = (double *)scr, *dv = dx + 3 * n, *dw = dv + 3 * n, *ds = dw + n;
         uint8_t *da = (uint8_t *)(ds + n);
         if (n > 0) {
-            MCS_HIP(ctx, hipMemcpy(dx, x, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
-            MCS_HIP(ctx, hipMemcpy(dv, v, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
-            MCS_HIP(ctx, hipMemcpy(dw, w, sizeof(double) * n, hipMemcpyHostToDevice));
-            MCS_HIP(ctx, hipMemcpy(ds, s, sizeof(double) * n, hipMemcpyHostToDevice));
-            MCS_HIP(ctx, hipMemcpy(da, active, n, hipMemcpyHostToDevice));
+            MCS_HIP(ctx, mcs_h2d(ctx, dx, x, sizeof(double) * 3 * n));
+            MCS_HIP(ctx, mcs_h2d(ctx, dv, v, sizeof(double) * 3 * n));
+            MCS_HIP(ctx, mcs_h2d(ctx, dw, w, sizeof(double) * n));
+            MCS_HIP(ctx, mcs_h2d(ctx, ds, s, sizeof(double) * n));
+            MCS_HIP(ctx, mcs_h2d(ctx, da, active, n));
             hipLaunchKernelGGL(k_split_in, dim3((unsigned)cdiv(n, MCS_BLOCK)), dim3(MCS_BLOCK), 0, ctx->stream,
                                p->sbuf[0], n, (const double *)dx, (const double *)dv, (const double *)dw,
                                (const double *)ds, (const uint8_t *)da, 1.0 / p->gh, p->gn[0], p->gn[1], p->gn[2]);
@@ -820,25 +826,25 @@ static int32_t upload_impl(mcs_pop *p, int64_t n, const double *x, const double 
     PopBuf &b = p->buf[0];
     for (int d = 0; d < 3 && p->prec != MCS_PREC_F32; ++d) {
         for (int64_t i = 0; i < n; ++i) tmp[i] = x[3 * i + d];
-        MCS_HIP(ctx, hipMemcpy(b.x[d], tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        MCS_HIP(ctx, mcs_h2d(ctx, b.x[d], tmp.data(), sizeof(double) * n));
         for (int64_t i = 0; i < n; ++i) tmp[i] = v[3 * i + d];
-        MCS_HIP(ctx, hipMemcpy(b.v[d], tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        MCS_HIP(ctx, mcs_h2d(ctx, b.v[d], tmp.data(), sizeof(double) * n));
     }
     if (p->prec != MCS_PREC_F32) {
-        MCS_HIP(ctx, hipMemcpy(b.w, w, sizeof(double) * n, hipMemcpyHostToDevice));
-        MCS_HIP(ctx, hipMemcpy(b.s, s, sizeof(double) * n, hipMemcpyHostToDevice));
-        MCS_HIP(ctx, hipMemcpy(b.active, active, n, hipMemcpyHostToDevice));
+        MCS_HIP(ctx, mcs_h2d(ctx, b.w, w, sizeof(double) * n));
+        MCS_HIP(ctx, mcs_h2d(ctx, b.s, s, sizeof(double) * n));
+        MCS_HIP(ctx, mcs_h2d(ctx, b.active, active, n));
     }
     uint64_t *dst_id = p->prec == MCS_PREC_F32 ? p->sbuf[0].id : b.id;
     uint64_t nid = next_id;
     if (id) {
-        MCS_HIP(ctx, hipMemcpy(dst_id, id, sizeof(uint64_t) * n, hipMemcpyHostToDevice));
+        MCS_HIP(ctx, mcs_h2d(ctx, dst_id, id, sizeof(uint64_t) * n));
         for (int64_t i = 0; i < n; ++i)
             if (id[i] + 1 > nid) nid = id[i] + 1;
     } else {
         std::vector<uint64_t> ids((size_t)n);
         for (int64_t i = 0; i < n; ++i) ids[i] = next_id + (uint64_t)i;
-        MCS_HIP(ctx, hipMemcpy(dst_id, ids.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice));
+        MCS_HIP(ctx, mcs_h2d(ctx, dst_id, ids.data(), sizeof(uint64_t) * n));
         nid = next_id + (uint64_t)n;
     }
     p->n = n;

@@ -333,8 +333,8 @@ static int32_t freebc_setup(mcs_cgrid *g)
     if (hipMalloc(&g->fb_S, sizeof(double) * 4 * N) != hipSuccess ||
         hipMalloc(&g->fb_f, sizeof(double) * 4 * N) != hipSuccess)
         return mcs_fail(ctx, MCS_E_HIP, "freebc: device allocation failed");
-    MCS_HIP(ctx, hipMemcpy(g->fb_S, S.data(), sizeof(double) * 4 * N, hipMemcpyHostToDevice));
-    MCS_HIP(ctx, hipMemcpy(g->fb_f, f.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    MCS_HIP(ctx, mcs_h2d(ctx, g->fb_S, S.data(), sizeof(double) * 4 * N));
+    MCS_HIP(ctx, mcs_h2d(ctx, g->fb_f, f.data(), sizeof(double) * N));
     return MCS_OK;
 }
 
@@ -420,7 +420,7 @@ extern "C" int32_t mcs_cgrid_set(mcs_cgrid *g, int32_t which, const double *host
     mcs_ctx *ctx = g->ctx;
     hipSetDevice(ctx->device);
     MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    MCS_HIP(ctx, hipMemcpy(g->arr[which], host, sizeof(double) * g->arr_len[which], hipMemcpyHostToDevice));
+    MCS_HIP(ctx, mcs_h2d(ctx, g->arr[which], host, sizeof(double) * g->arr_len[which]));
     return MCS_OK;
 }
 

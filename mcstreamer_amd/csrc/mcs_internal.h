@@ -70,6 +70,28 @@ int32_t mcs_fail(mcs_ctx *ctx, int32_t code, const char *fmt, ...);
                             hipGetErrorString(e_), __FILE__, __LINE__);                     \
     } while (0)
 
+// Every transfer and fill of the library runs on the context's stream.  That
+// stream is non-blocking, so it does not wait for work on the null stream: a
+// plain hipMemcpy or hipMemset there is not ordered with the kernels queued on
+// ctx->stream, and a fill still running when a later kernel has written its
+// buffer wipes what the kernel wrote.
+//
+// Host-to-device copy queued on ctx->stream and waited for, so that kernels
+// queued afterwards read the new data and the host buffer is free on return.
+static inline hipError_t mcs_h2d(mcs_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(ctx->stream);
+}
+
+// Zero fill queued on ctx->stream (ordered before every kernel queued after it).
+// For set-up code only: a fill recorded into a captured graph is not used here.
+static inline hipError_t mcs_zero(mcs_ctx *ctx, void *dst, size_t bytes)
+{
+    return hipMemsetAsync(dst, 0, bytes, ctx->stream);
+}
+
 // Records a HIP event pair around the enclosed launches when profiling is on.
 struct ProfScope {
     mcs_ctx *ctx;

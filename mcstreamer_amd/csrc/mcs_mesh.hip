@@ -50,14 +50,14 @@ extern "C" int32_t mcs_mesh_create(mcs_ctx *ctx, const int32_t n[3], double h, m
         m->arr_len[a] = arr_len(n, a);
         if (hipMalloc(&m->arr[a], sizeof(double) * m->arr_len[a]) != hipSuccess)
             return mcs_fail(ctx, MCS_E_HIP, "mesh: device allocation failed");
-        MCS_HIP(ctx, hipMemset(m->arr[a], 0, sizeof(double) * m->arr_len[a]));
+        MCS_HIP(ctx, mcs_zero(ctx, m->arr[a], sizeof(double) * m->arr_len[a]));
     }
     if (hipMalloc(&m->cell_start, sizeof(int32_t) * (ncells + 2)) != hipSuccess ||
         hipMalloc(&m->cell_cnt, sizeof(int32_t) * (ncells + 2 + MESH_BOX_WORDS)) != hipSuccess ||
         hipMalloc(&m->cnt64, sizeof(int64_t) * (ncells + 2)) != hipSuccess ||
         hipMalloc(&m->pre64, sizeof(int64_t) * (ncells + 2)) != hipSuccess)
         return mcs_fail(ctx, MCS_E_HIP, "mesh: device allocation failed");
-    MCS_HIP(ctx, hipMemset(m->cell_start, 0, sizeof(int32_t) * (ncells + 2)));
+    MCS_HIP(ctx, mcs_zero(ctx, m->cell_start, sizeof(int32_t) * (ncells + 2)));
     *out = m;
     return MCS_OK;
 }
@@ -86,8 +86,7 @@ extern "C" int32_t mcs_mesh_set(mcs_mesh *m, int32_t which, const double *host)
     mcs_ctx *ctx = m->ctx;
     hipSetDevice(ctx->device);
     MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    MCS_HIP(ctx, hipMemcpy(m->arr[which], host, sizeof(double) * m->arr_len[which],
-                           hipMemcpyHostToDevice));
+    MCS_HIP(ctx, mcs_h2d(ctx, m->arr[which], host, sizeof(double) * m->arr_len[which]));
     return MCS_OK;
 }
 

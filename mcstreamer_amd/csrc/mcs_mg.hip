@@ -1607,14 +1607,14 @@ extern "C" int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg 
         if (hipMalloc(&L.res, sizeof(double) * L.len) != hipSuccess ||
             hipMalloc(&L.sol, sizeof(double) * L.len) != hipSuccess)
             return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation failed");
-        MCS_HIP(ctx, hipMemset(L.res, 0, sizeof(double) * L.len));
-        MCS_HIP(ctx, hipMemset(L.sol, 0, sizeof(double) * L.len));
+        MCS_HIP(ctx, mcs_zero(ctx, L.res, sizeof(double) * L.len));
+        MCS_HIP(ctx, mcs_zero(ctx, L.sol, sizeof(double) * L.len));
         if (f32) {
             if (hipMalloc(&L.fres, sizeof(float) * L.len) != hipSuccess ||
                 hipMalloc(&L.fsol, sizeof(float) * L.len) != hipSuccess)
                 return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation failed");
-            MCS_HIP(ctx, hipMemset(L.fres, 0, sizeof(float) * L.len));
-            MCS_HIP(ctx, hipMemset(L.fsol, 0, sizeof(float) * L.len));
+            MCS_HIP(ctx, mcs_zero(ctx, L.fres, sizeof(float) * L.len));
+            MCS_HIP(ctx, mcs_zero(ctx, L.fsol, sizeof(float) * L.len));
         }
         mg->lv.push_back(L);
     }
@@ -1628,11 +1628,11 @@ extern "C" int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg 
             if (f32) {
                 if (hipMalloc(&L.falt, sizeof(float) * L.len) != hipSuccess)
                     return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation failed");
-                MCS_HIP(ctx, hipMemset(L.falt, 0, sizeof(float) * L.len));
+                MCS_HIP(ctx, mcs_zero(ctx, L.falt, sizeof(float) * L.len));
             } else {
                 if (hipMalloc(&L.alt, sizeof(double) * L.len) != hipSuccess)
                     return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation failed");
-                MCS_HIP(ctx, hipMemset(L.alt, 0, sizeof(double) * L.len));
+                MCS_HIP(ctx, mcs_zero(ctx, L.alt, sizeof(double) * L.len));
             }
         }
     }
@@ -1654,12 +1654,12 @@ extern "C" int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg 
         return mcs_fail(ctx, MCS_E_ARG, "mg: coarsest operator is singular (all-Neumann?)");
     if (hipMalloc(&mg->ainv, sizeof(double) * ainv.size()) != hipSuccess)
         return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation failed");
-    hipMemcpy(mg->ainv, ainv.data(), sizeof(double) * ainv.size(), hipMemcpyHostToDevice);
+    MCS_HIP(ctx, mcs_h2d(ctx, mg->ainv, ainv.data(), sizeof(double) * ainv.size()));
     if (f32) {
         std::vector<float> a32(ainv.begin(), ainv.end());
         if (hipMalloc(&mg->ainv32, sizeof(float) * a32.size()) != hipSuccess)
             return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation failed");
-        hipMemcpy(mg->ainv32, a32.data(), sizeof(float) * a32.size(), hipMemcpyHostToDevice);
+        MCS_HIP(ctx, mcs_h2d(ctx, mg->ainv32, a32.data(), sizeof(float) * a32.size()));
     }
     int64_t N0 = (int64_t)c.n[0] * c.n[1] * c.n[2];
     mg->npartial = cdiv(N0, MCS_BLOCK);
@@ -1787,7 +1787,7 @@ static int32_t ensure_bc_dev(mcs_mg *mg)
 {
     if (mg->bc_dev) return MCS_OK;
     MCS_HIP(mg->ctx, hipMalloc(&mg->bc_dev, sizeof(int32_t) * 6));
-    MCS_HIP(mg->ctx, hipMemcpy(mg->bc_dev, mg->cfg.bc, sizeof(int32_t) * 6, hipMemcpyHostToDevice));
+    MCS_HIP(mg->ctx, mcs_h2d(mg->ctx, mg->bc_dev, mg->cfg.bc, sizeof(int32_t) * 6));
     return MCS_OK;
 }
 
@@ -2181,8 +2181,8 @@ extern "C" int32_t mcs_mg_solve_host(mcs_mg *mg, double *u, const double *b, int
         MCS_HIP(ctx, hipMalloc(&mg->b_tmp, sizeof(double) * len));
     }
     MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    MCS_HIP(ctx, hipMemcpy(mg->u_tmp, u, sizeof(double) * len, hipMemcpyHostToDevice));
-    MCS_HIP(ctx, hipMemcpy(mg->b_tmp, b, sizeof(double) * len, hipMemcpyHostToDevice));
+    MCS_HIP(ctx, mcs_h2d(ctx, mg->u_tmp, u, sizeof(double) * len));
+    MCS_HIP(ctx, mcs_h2d(ctx, mg->b_tmp, b, sizeof(double) * len));
     int32_t rc = mg->cfg.ndim == 2 ? mcs_mg2_solve_dev(mg, mg->u_tmp, mg->b_tmp, iters, resid)
                                    : mg_solve_dev(mg, mg->u_tmp, mg->b_tmp, iters, resid);
     MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));

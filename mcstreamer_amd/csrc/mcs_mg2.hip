@@ -731,7 +731,10 @@ int32_t mcs_mg2_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg **out)
         mcs_mg_destroy(mg);
         return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation failed");
     }
-    hipMemcpy(mg->ainv, ainv.data(), sizeof(double) * ainv.size(), hipMemcpyHostToDevice);
+    if (mcs_h2d(ctx, mg->ainv, ainv.data(), sizeof(double) * ainv.size()) != hipSuccess) {
+        mcs_mg_destroy(mg);
+        return mcs_fail(ctx, MCS_E_HIP, "mg: copy to the device failed");
+    }
     mg->npartial = nblk((int64_t)c.n[0] * c.n[1]);
     if (hipMalloc(&mg->partial, sizeof(double) * (mg->npartial + 8)) != hipSuccess) {
         mcs_mg_destroy(mg);

@@ -936,7 +936,7 @@ static int32_t dalloc(mcs_swarm_batch *b, T **p, int64_t n)
     const size_t bytes = sizeof(T) * (size_t)(n > 0 ? n : 1);
     MCS_HIP(ctx, hipMalloc((void **)p, bytes));
     b->owned.push_back((void *)*p);
-    MCS_HIP(ctx, hipMemset(*p, 0, bytes));
+    MCS_HIP(ctx, mcs_zero(ctx, *p, bytes));
     return MCS_OK;
 }
 
@@ -1071,11 +1071,11 @@ static int32_t rebuild_tiles(mcs_swarm_batch *b)
     b->ntiles = (int64_t)ts.size();
     if (b->ntiles > b->max_tiles) return mcs_fail(ctx, MCS_E_ARG, "swarm batch: too many tiles");
     if (b->ntiles) {
-        MCS_HIP(ctx, hipMemcpy(b->tile_slot, ts.data(), sizeof(int32_t) * ts.size(), hipMemcpyHostToDevice));
-        MCS_HIP(ctx, hipMemcpy(b->tile_start, tst.data(), sizeof(int64_t) * tst.size(), hipMemcpyHostToDevice));
+        MCS_HIP(ctx, mcs_h2d(ctx, b->tile_slot, ts.data(), sizeof(int32_t) * ts.size()));
+        MCS_HIP(ctx, mcs_h2d(ctx, b->tile_start, tst.data(), sizeof(int64_t) * tst.size()));
     }
-    MCS_HIP(ctx, hipMemcpy(b->S.first_tile, first.data(), sizeof(int32_t) * first.size(), hipMemcpyHostToDevice));
-    MCS_HIP(ctx, hipMemcpy(b->S.ntiles, nt.data(), sizeof(int32_t) * nt.size(), hipMemcpyHostToDevice));
+    MCS_HIP(ctx, mcs_h2d(ctx, b->S.first_tile, first.data(), sizeof(int32_t) * first.size()));
+    MCS_HIP(ctx, mcs_h2d(ctx, b->S.ntiles, nt.data(), sizeof(int32_t) * nt.size()));
     return MCS_OK;
 }
 
@@ -1151,27 +1151,27 @@ extern "C" int32_t mcs_swarm_batch_add(mcs_swarm_batch *b, const mcs_table *t,
         std::vector<double> tmp((size_t)n);
         for (int d = 0; d < 3; ++d) {
             for (int64_t i = 0; i < n; ++i) tmp[i] = x[3 * i + d];
-            MCS_HIP(ctx, hipMemcpy(b->buf.x[d] + base, tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+            MCS_HIP(ctx, mcs_h2d(ctx, b->buf.x[d] + base, tmp.data(), sizeof(double) * n));
             for (int64_t i = 0; i < n; ++i) tmp[i] = v[3 * i + d];
-            MCS_HIP(ctx, hipMemcpy(b->buf.v[d] + base, tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+            MCS_HIP(ctx, mcs_h2d(ctx, b->buf.v[d] + base, tmp.data(), sizeof(double) * n));
         }
-        MCS_HIP(ctx, hipMemcpy(b->buf.w + base, w, sizeof(double) * n, hipMemcpyHostToDevice));
-        MCS_HIP(ctx, hipMemcpy(b->buf.s + base, s, sizeof(double) * n, hipMemcpyHostToDevice));
-        MCS_HIP(ctx, hipMemcpy(b->buf.active + base, active, n, hipMemcpyHostToDevice));
+        MCS_HIP(ctx, mcs_h2d(ctx, b->buf.w + base, w, sizeof(double) * n));
+        MCS_HIP(ctx, mcs_h2d(ctx, b->buf.s + base, s, sizeof(double) * n));
+        MCS_HIP(ctx, mcs_h2d(ctx, b->buf.active + base, active, n));
         std::vector<uint64_t> ids((size_t)n);
         for (int64_t i = 0; i < n; ++i) ids[i] = (uint64_t)i;
-        MCS_HIP(ctx, hipMemcpy(b->buf.id + base, ids.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice));
+        MCS_HIP(ctx, mcs_h2d(ctx, b->buf.id + base, ids.data(), sizeof(uint64_t) * n));
     } else if (n > 0) {
         MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
         int32_t rc = ensure_stage(b, n);
         if (rc) return rc;
         double *dx = b->stage, *dv = dx + 3 * n, *dw = dv + 3 * n, *ds = dw + n;
         uint8_t *da = (uint8_t *)(ds + n);
-        MCS_HIP(ctx, hipMemcpy(dx, x, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
-        MCS_HIP(ctx, hipMemcpy(dv, v, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
-        MCS_HIP(ctx, hipMemcpy(dw, w, sizeof(double) * n, hipMemcpyHostToDevice));
-        MCS_HIP(ctx, hipMemcpy(ds, s, sizeof(double) * n, hipMemcpyHostToDevice));
-        MCS_HIP(ctx, hipMemcpy(da, active, n, hipMemcpyHostToDevice));
+        MCS_HIP(ctx, mcs_h2d(ctx, dx, x, sizeof(double) * 3 * n));
+        MCS_HIP(ctx, mcs_h2d(ctx, dv, v, sizeof(double) * 3 * n));
+        MCS_HIP(ctx, mcs_h2d(ctx, dw, w, sizeof(double) * n));
+        MCS_HIP(ctx, mcs_h2d(ctx, ds, s, sizeof(double) * n));
+        MCS_HIP(ctx, mcs_h2d(ctx, da, active, n));
         hipLaunchKernelGGL(k_sw_from64, dim3(grid), dim3(MCS_BLOCK), 0, ctx->stream, b->buf32, base, n,
                            (const double *)dx, (const double *)dv, (const double *)dw, (const double *)ds,
                            (const uint8_t *)da);

@@ -249,12 +249,13 @@ class SwarmBatch:
         check(self.ctx.handle, _lib.lib().mcs_swarm_batch_config(self.handle, int(block), int(compact_every)))
 
     def autotune(self, candidates=((256, 4), (128, 4), (256, 8), (128, 8), (256, 16), (256, 2)),
-                 min_steps=2):
+                 min_steps=2, log=None):
         """Project-written autotuner of the launch shape: runs real steps under each
         (workgroup size, compaction cadence) candidate, one compaction per trial,
         and keeps the fastest per step.  The choice never changes results (a slot's
         trajectory is the same under every shape and cadence); the trial steps count
-        as ordinary steps.  Returns {candidate: seconds per step}."""
+        as ordinary steps.  log(message), when given, hears each trial as it ends.
+        Returns {candidate: seconds per step}."""
         import time as _t
         timings = {}
         for blk, ce in candidates:
@@ -266,6 +267,8 @@ class SwarmBatch:
             self.step(k)
             self.ctx.synchronize()
             timings[(blk, ce)] = (_t.perf_counter() - t0) / k
+            if log:
+                log(f"autotune {blk} threads, compaction every {ce}: {timings[(blk, ce)] * 1e3:.2f} ms per step")
         best = min(timings, key=timings.get)
         self.configure(*best)
         self.tuned = best
