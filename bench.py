@@ -251,7 +251,7 @@ def run_streamer(args):
                               "fmg_iterations_last": out["mg_iters"]}
         print(f"late run: steps {done}-{done + args.late_steps}, {lel / args.late_steps * 1e3:.2f} ms/step",
               file=sys.stderr, flush=True)
-    if args.swarm:
+    if not args.no_swarm:
         result["swarm"] = swarm_block(args)
     return result
 
@@ -934,7 +934,8 @@ def main():
     ap.add_argument("--swarm-n", type=int, default=10_000)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--swarm", action="store_true", help="add the swarm block (configs 3 and 4) to the streamer line")
+    ap.add_argument("--no-swarm", action="store_true",
+                    help="leave the swarm block (batched and served swarms) out of the streamer line")
     ap.add_argument("--late-at", type=int, default=480, help="step of the run at which the late sample starts")
     ap.add_argument("--late-steps", type=int, default=20, help="timed steps of the late sample (0: none)")
     ap.add_argument("--c3-steps", type=int, default=10)
