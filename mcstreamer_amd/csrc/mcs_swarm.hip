@@ -23,6 +23,7 @@
 #define SW_NT (SW_NQ + 1)     // count + the tallied quantities
 #define SW_NX 6               // position moments: sum x_d, sum x_d^2
 #define SW_WIN MCS_DIFFUSION_WINDOW
+#define SW_XCHUNK 64          // tiles per workgroup of the position moments
 #define SW_MODE_NONE 0
 #define SW_MODE_DROP 1
 #define SW_MODE_CLONE 2
@@ -84,7 +85,7 @@ struct SwArgsT {
     int32_t *loff, *cnt;
     int64_t *tile_births, *tile_bpre, *tile_clones, *tile_cpre, *tile_act, *tile_apre;
     double *tile_part;   // ntiles x SW_NT x 2 (Neumaier sum, compensation)
-    double *tile_xpart;  // ntiles x SW_NX x 2: the position moments after the step
+    double *tile_xpart;  // ntiles x SW_NX x 2: the position moments after the step, at each chunk's first tile
 };
 using SwArgs64 = SwArgsT<PopBuf, SwLog>;
 using SwArgs32 = SwArgsT<SplitBuf, SwLog32>;
@@ -184,31 +185,40 @@ __device__ __forceinline__ void tally_add32(Neu2 acc[SW_NT], const float v[3], c
     nadd(acc[4], (double)att);
 }
 
-// Block-reduce NQ Neumaier pairs into out (a tile's partials).
+// Block-reduce NQ Neumaier pairs into out (a tile's partials): a butterfly over
+// the lanes of each wavefront (nmerge is symmetric, so every lane ends with the
+// wave's total), then one barrier and the waves' totals merged in wave order.
+// The order is fixed by the tile alone, so a slot's tallies do not depend on
+// the other slots.
 template <int BS, int NQ = SW_NT>
 __device__ __forceinline__ void tile_reduce(Neu2 acc[NQ], double *out)
 {
-    __shared__ double sh[BS], shc[BS];
+    constexpr int NW = BS / MCS_WAVE;
+    __shared__ double sh[NW][NQ][2];
+    const int lane = threadIdx.x & (MCS_WAVE - 1), wid = threadIdx.x / MCS_WAVE;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        sh[threadIdx.x] = acc[q].s;
-        shc[threadIdx.x] = acc[q].c;
-        __syncthreads();
-        for (int o = BS / 2; o > 0; o >>= 1) {
-            if (threadIdx.x < o) {
-                Neu2 a{sh[threadIdx.x], shc[threadIdx.x]};
-                Neu2 b{sh[threadIdx.x + o], shc[threadIdx.x + o]};
-                nmerge(a, b);
-                sh[threadIdx.x] = a.s;
-                shc[threadIdx.x] = a.c;
-            }
-            __syncthreads();
+#pragma unroll
+        for (int o = MCS_WAVE / 2; o > 0; o >>= 1) {
+            const Neu2 b{__shfl_xor(acc[q].s, o, MCS_WAVE), __shfl_xor(acc[q].c, o, MCS_WAVE)};
+            nmerge(acc[q], b);
         }
-        if (threadIdx.x == 0) {
-            out[2 * q] = sh[0];
-            out[2 * q + 1] = shc[0];
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            sh[wid][q][0] = acc[q].s;
+            sh[wid][q][1] = acc[q].c;
         }
-        __syncthreads();
+    }
+    __syncthreads();
+    if (threadIdx.x < NQ) {
+        const int q = threadIdx.x;
+        Neu2 a{sh[0][q][0], sh[0][q][1]};
+#pragma unroll
+        for (int w = 1; w < NW; ++w) nmerge(a, Neu2{sh[w][q][0], sh[w][q][1]});
+        out[2 * q] = a.s;
+        out[2 * q + 1] = a.c;
     }
 }
 
@@ -426,44 +436,54 @@ __device__ __forceinline__ void sw_pos(const SplitBuf &b, int64_t i, double x[3]
 }
 
 // The position moments of Swarm.jl:126-127 after the step (the slot's parents and
-// the children placed after them; actives only), per tile, as Neumaier pairs:
-// sum x_d, sum x_d^2.  The count is the tallies' (the same actives).
+// the children placed after them; actives only) as Neumaier pairs: sum x_d,
+// sum x_d^2; the count is the tallies' (the same actives).  One workgroup per
+// (slot, chunk of SW_XCHUNK tiles): each thread walks the chunk at a stride of the
+// workgroup, and the chunk's partials go to the slot of its first tile.  The
+// chunks start at fixed offsets from the slab base, so a slot's sums do not
+// depend on the other slots.
 template <int BS, typename A>
 __global__ __launch_bounds__(BS) void k_sw_xmom(A a)
 {
-    const int64_t tile = blockIdx.x;
-    const int slot = a.tile_slot[tile];
-    const int64_t i = a.tile_start[tile] + threadIdx.x;
+    const int slot = blockIdx.x;
+    const int64_t chunk = blockIdx.y;
+    if (!a.S.used[slot]) return;
     const int64_t f = a.S.first_tile[slot], nt = a.S.ntiles[slot];
+    if (chunk * SW_XCHUNK >= nt) return;
     const int64_t births = a.tile_bpre[f + nt] - a.tile_bpre[f];
     int64_t lim = a.S.n[slot] + births;
     if (lim > a.S.cap[slot]) lim = a.S.cap[slot];
-    lim += a.S.base[slot];
+    const int64_t base = a.S.base[slot];
+    const int64_t i0 = base + chunk * (SW_XCHUNK * BS);
+    const int64_t i1 = min(base + lim, i0 + (int64_t)SW_XCHUNK * BS);
     Neu2 acc[SW_NX];
 #pragma unroll
     for (int q = 0; q < SW_NX; ++q) acc[q] = Neu2{0.0, 0.0};
-    if (a.S.used[slot] && i < lim && a.b.active[i]) {
-        double x[3];
-        sw_pos(a.b, i, x);
+#pragma unroll 4
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += BS) {
+        if (a.b.active[i]) {
+            double x[3];
+            sw_pos(a.b, i, x);
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            nadd(acc[d], x[d]);
-            nadd(acc[3 + d], x[d] * x[d]);
+            for (int d = 0; d < 3; ++d) {
+                nadd(acc[d], x[d]);
+                nadd(acc[3 + d], x[d] * x[d]);
+            }
         }
     }
-    tile_reduce<BS, SW_NX>(acc, a.tile_xpart + tile * (2 * SW_NX));
+    tile_reduce<BS, SW_NX>(acc, a.tile_xpart + (f + chunk * SW_XCHUNK) * (2 * SW_NX));
 }
 
-// Sum of NQ Neumaier pairs over the slot's tiles [f, f + nt) by one workgroup;
-// the totals (sum + compensation) to tot.
+// Sum of NQ Neumaier pairs at the tiles f, f + stride, ... (n of them) by one
+// workgroup; the totals (sum + compensation) to tot.
 template <int NQ>
-__device__ __forceinline__ void slot_merge(const double *part, int64_t f, int64_t nt, double *tot)
+__device__ __forceinline__ void slot_merge(const double *part, int64_t f, int64_t n, int64_t stride, double *tot)
 {
     Neu2 acc[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) acc[q] = Neu2{0.0, 0.0};
-    for (int64_t t = threadIdx.x; t < nt; t += blockDim.x) {
-        const double *p = part + (f + t) * (2 * NQ);
+    for (int64_t t = threadIdx.x; t < n; t += MCS_BLOCK) {
+        const double *p = part + (f + t * stride) * (2 * NQ);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) nmerge(acc[q], Neu2{p[2 * q], p[2 * q + 1]});
     }
@@ -560,7 +580,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_sw_slot_step(A a)
         __syncthreads();
     }
     __shared__ double xm[SW_NX];
-    slot_merge<SW_NX>(a.tile_xpart, f, nt, xm);
+    slot_merge<SW_NX>(a.tile_xpart, f, (nt + SW_XCHUNK - 1) / SW_XCHUNK, SW_XCHUNK, xm);
     if (threadIdx.x != 0) return;
     // births
     int64_t births = a.tile_bpre[f + nt] - a.tile_bpre[f];
@@ -1291,7 +1311,14 @@ static int32_t launch_step_t(mcs_swarm_batch *b, const A &a)
     const int64_t logcap = b->precision == MCS_PREC_F64 ? b->L.cap : b->L32.cap;
     hipLaunchKernelGGL(k_sw_births<A>, dim3((unsigned)std::min<int64_t>(cdiv(logcap, MCS_BLOCK), 2048)),
                        dim3(MCS_BLOCK), 0, ctx->stream, a, logcap, b->ntiles);
-    hipLaunchKernelGGL((k_sw_xmom<BS, A>), g, dim3(BS), 0, ctx->stream, a);
+    int64_t maxnt = 1;
+    for (const HostSlot &h : b->slots)
+        if (h.used) maxnt = std::max<int64_t>(maxnt, cdiv(h.cap, (int64_t)BS));
+    const int64_t nchunk = cdiv(maxnt, (int64_t)SW_XCHUNK);
+    if (nchunk > 65535) return mcs_fail(ctx, MCS_E_ARG, "swarm batch: a slab of %lld particles is too large",
+                                        (long long)(maxnt * BS));
+    hipLaunchKernelGGL((k_sw_xmom<BS, A>), dim3((unsigned)b->max_slots, (unsigned)nchunk), dim3(BS), 0,
+                       ctx->stream, a);
     hipLaunchKernelGGL(k_sw_slot_step<A>, dim3((unsigned)b->max_slots), dim3(MCS_BLOCK), 0, ctx->stream, a);
     hipLaunchKernelGGL((k_sw_resample<BS, A>), g, dim3(BS), 0, ctx->stream, a);
     rc = mcs_scan_i64_buf(ctx, b->tile_clones, b->tile_cpre, b->ntiles, b->scan_scratch);
