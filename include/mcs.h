@@ -397,6 +397,31 @@ int32_t mcs_swarm_batch_download(mcs_swarm_batch *b, int32_t slot, int64_t nmax,
 int32_t mcs_pop_init_thermal(mcs_pop *p, int64_t n, uint64_t seed, uint32_t instance, double vth);
 /* Launch shape and compaction cadence (workgroup size 128 or 256). */
 int32_t mcs_swarm_batch_config(mcs_swarm_batch *b, int32_t block, int32_t compact_every);
+/* Majorant of the null-collision sampler (collisions.jl:159-192 samples the next
+ * collision at the table's one maxrate, mixed_population.jl:35-44).
+ * MCS_MAJORANT_GLOBAL: that maxrate (the default; the only mode of an FP64 batch,
+ * which stays bit-exact with a lone FP64 population).  MCS_MAJORANT_BANDED (FP32
+ * batches): the flight clock runs at the majorant of an energy band [0, cap] that
+ * holds the particle's energy with a quarter of headroom (mcs_table_majorants); a
+ * flight whose parabola would leave the band stops where it reaches the cap and
+ * the remaining exponential clock continues at the next band's majorant.  The
+ * process of real collisions is the same Poisson process (the null-collision
+ * method is exact for any majorant above the real rate), with far fewer null
+ * events at low energy; the particle's draws are no longer those of the global
+ * form, so results agree with it statistically, not draw for draw.  Rows of kind
+ * MCS_PROC_NULL are not sampled in this mode (a null event is the rest of the band
+ * majorant).  Capture of the step graph is redone at the next step. */
+#define MCS_MAJORANT_GLOBAL 0
+#define MCS_MAJORANT_BANDED 1
+int32_t mcs_swarm_batch_majorant(mcs_swarm_batch *b, int32_t mode);
+/* The energy bands of MCS_MAJORANT_BANDED: *nbands bands (at most nmax are
+ * written), band k holding energies [0, ecap[k]] (J) with majorant rate[k] (1/s):
+ * at least the summed rate of the table's non-null rows at every node up to the
+ * first node above ecap[k] (so at every energy of the band, the rates being
+ * linear between nodes), raised by 1e-5 for the FP32 rounding of the rows.  The
+ * caps double from band to band up to the last node's energy; the band above
+ * (ecap = +inf) has the table's maxrate. */
+int32_t mcs_table_majorants(const mcs_table *t, int32_t nmax, int32_t *nbands, double *ecap, double *rate);
 
 /* ---- axisymmetric (r, z) grid: MCStreamer.Grid / GridFields as the reference
  * builds them (MCStreamer/src/grid.jl:4-90, gridfields.jl:1-84): M cells in r over

@@ -27,6 +27,7 @@ ARR_QFIXED, ARR_QPART, ARR_Q, ARR_U, ARR_EX, ARR_EY, ARR_EZ = range(7)
 DEPOSIT_NGP, DEPOSIT_CIC = 0, 1
 BC_DIRICHLET, BC_NEUMANN = 0, 1
 PREC_F64, PREC_F32 = 0, 1
+MAJORANT_GLOBAL, MAJORANT_BANDED = 0, 1
 
 _ERRNAMES = {
     MCS_E_ARG: "invalid argument",
@@ -202,6 +203,8 @@ SIGNATURES = {
     "mcs_swarm_batch_tally_all": [_P, _I32, ctypes.POINTER(SwarmTally)],
     "mcs_swarm_batch_download": [_P, _I32, _I64, _PI64, _PD, _PD, _PD, _PD, _PU8, _PU64],
     "mcs_swarm_batch_config": [_P, _I32, _I32],
+    "mcs_swarm_batch_majorant": [_P, _I32],
+    "mcs_table_majorants": [_P, _I32, _PI32, _PD, _PD],
     "mcs_pop_init_thermal": [_P, _I64, _U64, _U32, _D],
     "mcs_cgrid_create": [_P, _D, _D, _I32, _I32, _PP],
     "mcs_cgrid_destroy": [_P],

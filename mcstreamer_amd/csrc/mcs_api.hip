@@ -320,6 +320,45 @@ int32_t mcs_scan_i64(mcs_ctx *ctx, const int64_t *in, int64_t *out, int64_t len)
 
 // ---------------------------------------------------------------- table
 
+// Energy bands of the banded majorant: caps emax 2^(k - (MCS_NBAND - 1)), k = 0 ..
+// MCS_NBAND - 1, each with the largest summed rate of the non-null rows over the
+// nodes up to the first node above the cap (the rates are linear between nodes,
+// so that bounds the band), raised by 1e-5 for the FP32 rows; then the unbounded
+// band at maxrate.  FP32 triples (cap, majorant, 1 / majorant) for the device.
+static void table_bands(mcs_table *t, const double *energy, const double *rate, const int32_t *kind,
+                        std::vector<float> &b32)
+{
+    const int32_t ne = t->ne, nproc = t->nproc;
+    std::vector<double> tot((size_t)ne, 0.0);
+    for (int32_t k = 0; k < ne; ++k)
+        for (int32_t j = 0; j < nproc; ++j)
+            if (kind[j] != MCS_PROC_NULL) tot[k] += rate[(size_t)k * nproc + j];
+    t->h_band_cap.clear();
+    t->h_band_rate.clear();
+    b32.clear();
+    for (int b = 0; b < MCS_NBAND; ++b) {
+        const double cap = t->emax * std::ldexp(1.0, b - (MCS_NBAND - 1));
+        double m = 0.0;
+        for (int32_t k = 0; k < ne; ++k) {
+            m = std::max(m, tot[k]);
+            if (energy[k] > cap) break;
+        }
+        m = std::min(m * (1.0 + 1e-5), t->maxrate);
+        if (!(m > 0.0)) m = t->maxrate;
+        t->h_band_cap.push_back(cap);
+        t->h_band_rate.push_back(m);
+        b32.push_back((float)cap);
+        b32.push_back((float)m);
+        b32.push_back((float)(1.0 / m));
+    }
+    t->h_band_cap.push_back(HUGE_VAL);
+    t->h_band_rate.push_back(t->maxrate);
+    b32.push_back(HUGE_VALF);
+    b32.push_back((float)t->maxrate);
+    b32.push_back((float)(1.0 / t->maxrate));
+    t->nband = MCS_NBAND;
+}
+
 extern "C" int32_t mcs_table_create(mcs_ctx *ctx, int32_t nproc, int32_t ne, int32_t egrid_kind,
                                     const double *energy, const double *rate,
                                     const int32_t *kind, const double *param, double maxrate,
@@ -388,7 +427,31 @@ extern "C" int32_t mcs_table_create(mcs_ctx *ctx, int32_t nproc, int32_t ne, int
             return mcs_fail(ctx, MCS_E_HIP, "table: copy to the device failed");
         }
     }
+    {
+        std::vector<float> b32;
+        table_bands(t, energy, rate, kind, b32);
+        if (hipMalloc(&t->band32, sizeof(float) * b32.size()) != hipSuccess) {
+            mcs_table_destroy(t);
+            return mcs_fail(ctx, MCS_E_HIP, "table: device allocation failed");
+        }
+        if (mcs_h2d(ctx, t->band32, b32.data(), sizeof(float) * b32.size()) != hipSuccess) {
+            mcs_table_destroy(t);
+            return mcs_fail(ctx, MCS_E_HIP, "table: copy to the device failed");
+        }
+    }
     *out = t;
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_table_majorants(const mcs_table *t, int32_t nmax, int32_t *nbands, double *ecap, double *rate)
+{
+    if (!t || !nbands || nmax < 0 || (nmax > 0 && (!ecap || !rate))) return MCS_E_ARG;
+    const int32_t nb = (int32_t)t->h_band_cap.size();
+    *nbands = nb;
+    for (int32_t k = 0; k < nb && k < nmax; ++k) {
+        ecap[k] = t->h_band_cap[k];
+        rate[k] = t->h_band_rate[k];
+    }
     return MCS_OK;
 }
 
@@ -426,6 +489,7 @@ extern "C" int32_t mcs_table_destroy(mcs_table *t)
     hipFree(t->energy32);
     hipFree(t->rate32);
     hipFree(t->param32);
+    hipFree(t->band32);
     delete t;
     return MCS_OK;
 }

@@ -36,6 +36,8 @@ struct TableView32 {
     int32_t nproc, ne, egrid;
     float e0, emax, inv_e0, inv_step, inv_dlog2, maxrate, inv_maxrate;
     uint64_t kindbits;   // 4 bits per process and bit 63 set (nproc <= 15), else 0: kind[] is read
+    const float *band;   // MCS_NBAND + 1 triples (cap, majorant, 1 / majorant), the last unbounded
+    float inv_cap0;      // 1 / the first band's cap
 };
 
 __device__ __forceinline__ int proc_kind(const TableView32 &T, int j)
@@ -60,6 +62,8 @@ static inline TableView32 table_view32(const mcs_table *t)
     v.inv_dlog2 = (float)(t->inv_dlog * 0.6931471805599453);
     v.maxrate = (float)t->maxrate;
     v.inv_maxrate = (float)(1.0 / t->maxrate);
+    v.band = t->band32;
+    v.inv_cap0 = (float)(1.0 / t->h_band_cap[0]);
     v.kindbits = 0;
     if (t->nproc <= 15) {
         for (int j = 0; j < t->nproc; ++j) v.kindbits |= (uint64_t)(t->h_kind[j] & 15) << (4 * j);
@@ -362,6 +366,138 @@ __device__ __forceinline__ void advance_one32(int32_t c[3], float off[3], float 
                     birth(c, off, vc, sc, act);
                 } else if (kind == MCS_PROC_ATTACHMENT) {
                     if (TRACK) track(c, off, -(double)w);
+                    if (act) deaths += 1;
+                    act = 0;
+                }
+            }
+            s = nextcoll32(&rs);
+        }
+        trem = trem - tt;
+    }
+}
+
+// The band of a particle of energy eng under the banded majorant: the first band
+// whose cap is at least 1.25 eng (a quarter of headroom, so that few flights run
+// into the cap), the unbounded band MCS_NBAND above the last cap or for a NaN.
+__device__ __forceinline__ int band_of(const TableView32 &T, float eng)
+{
+    const float e = 1.25f * eng;
+    const float r = e * T.inv_cap0;
+    if (!(r > 1.0f)) return r == r ? 0 : MCS_NBAND;
+    int b = (int)ceilf(__builtin_amdgcn_logf(r));
+    b = b < 0 ? 0 : (b > MCS_NBAND ? MCS_NBAND : b);
+    while (b < MCS_NBAND && gld(T.band + 3 * b) < e) b += 1;
+    return b;
+}
+
+// advance_one32 in a uniform field with the banded majorant (mcs_swarm_batch_majorant):
+// each flight's exponential clock runs at the majorant M of the particle's band
+// [0, cap]; |v + a t|^2 is convex in t and below the cap at the flight's start, so
+// it stays below up to the flight's end when it is below there, and otherwise the
+// flight stops where the parabola reaches the cap (the remaining clock s - t M
+// carries over to the next band, the exponential being memoryless).  At a
+// collision xi = U M selects among the non-null rows in table order (a null event
+// when it passes them all), then the outcomes of advance_one32.
+template <typename Birth>
+__device__ __forceinline__ void advance_one32_banded(int32_t c[3], float off[3], float v[3], const float w,
+                                                     float &s, uint8_t &act, mcs_stream &rs, const float dt,
+                                                     const TableView32 &T, const float *acc, const float inv_h,
+                                                     int32_t &flags, int &flights, int &deaths, Birth &&birth)
+{
+    const bool field = acc[0] != 0.0f || acc[1] != 0.0f || acc[2] != 0.0f;
+    float trem = dt;
+    while (trem > 0.0f) {
+        const int b = band_of(T, energy32(v));
+        const float cap = gld(T.band + 3 * b), M = gld(T.band + 3 * b + 1), invM = gld(T.band + 3 * b + 2);
+        const float tnext = s * invM;
+        float tt;
+        bool collides;
+        if (trem > tnext) {
+            tt = tnext;
+            collides = true;
+        } else {
+            tt = trem;
+            collides = false;
+        }
+        if (b < MCS_NBAND && field) {
+            const float ve[3] = {v[0] + acc[0] * tt, v[1] + acc[1] * tt, v[2] + acc[2] * tt};
+            if (energy32(ve) > cap) {
+                // positive root of A t^2 + 2 B t + C = 0, |v + a t|^2 = 2 cap / m (C < 0:
+                // the energy starts below the cap), in FP64 (A = |a|^2 reaches 1e37);
+                // for B > 0 in the form without cancellation, -C / (B + sqrt(B^2 - A C))
+                const double a0 = acc[0], a1 = acc[1], a2 = acc[2];
+                const double v0 = v[0], v1 = v[1], v2 = v[2];
+                const double A = (a0 * a0 + a1 * a1) + a2 * a2;
+                const double B = (v0 * a0 + v1 * a1) + v2 * a2;
+                const double C = ((v0 * v0 + v1 * v1) + v2 * v2) - (double)cap / (double)MCS_HALF_ME_F;
+                const double sq = sqrt(fmax(B * B - A * C, 0.0));
+                const double tc = B > 0.0 ? (-C) / (B + sq) : (sq - B) / A;
+                tt = fminf((float)tc, tt);
+                collides = false;
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const float dv = acc[d] * tt;
+            off[d] = off[d] + (v[d] + 0.5f * dv) * (tt * inv_h);
+            v[d] = v[d] + dv;
+        }
+        flights += 1;
+        if (!collides) {
+            s = fmaxf(s - tt * M, 0.0f);
+        } else {
+            const float eng = energy32(v);
+            int32_t k;
+            float wg;
+            indweight32(T, eng, k, wg, flags);
+            float xi = mcs_u01f(&rs) * M;
+            const float *r0 = T.rate + k * T.nproc;
+            const float *r1 = r0 + T.nproc;
+            int sel = -1, kind = MCS_PROC_NULL;
+            for (int j = 0; j < T.nproc; ++j) {
+                const int kj = proc_kind(T, j);
+                if (kj == MCS_PROC_NULL) continue;
+                const float nu = wg * gld(r0 + j) + (1.0f - wg) * gld(r1 + j);
+                if (nu > xi) {
+                    sel = j;
+                    kind = kj;
+                    break;
+                }
+                xi = xi - nu;
+            }
+            if (sel >= 0) {
+                if (kind == MCS_PROC_ELASTIC) {
+                    const float mr = gld(T.param + sel);
+                    const float cm = mr / (1.0f + mr);
+                    const float vc0 = cm * v[0], vc1 = cm * v[1], vc2 = cm * v[2];
+                    const float e0 = v[0] - vc0, e1 = v[1] - vc1, e2 = v[2] - vc2;
+                    const float nrm = sqrtf((e0 * e0 + e1 * e1) + e2 * e2);
+                    float om[3];
+                    randsphere32(&rs, om);
+                    v[0] = om[0] * nrm + vc0;
+                    v[1] = om[1] * nrm + vc1;
+                    v[2] = om[2] * nrm + vc2;
+                } else if (kind == MCS_PROC_EXCITATION) {
+                    const float e1 = fmaxf(eng - gld(T.param + sel), 0.0f);
+                    const float vabs = sqrtf((2.0f * e1) / MCS_ELECTRON_MASS_F);
+                    float om[3];
+                    randsphere32(&rs, om);
+                    v[0] = om[0] * vabs;
+                    v[1] = om[1] * vabs;
+                    v[2] = om[2] * vabs;
+                } else if (kind == MCS_PROC_IONIZATION) {
+                    const float e1 = fmaxf(eng - gld(T.param + sel), 0.0f) * 0.5f;
+                    const float vabs = sqrtf((2.0f * e1) / MCS_ELECTRON_MASS_F);
+                    float om1[3], om2[3];
+                    randsphere32(&rs, om1);
+                    randsphere32(&rs, om2);
+                    v[0] = om1[0] * vabs;
+                    v[1] = om1[1] * vabs;
+                    v[2] = om1[2] * vabs;
+                    const float sc = nextcoll32(&rs);
+                    const float vc[3] = {om2[0] * vabs, om2[1] * vabs, om2[2] * vabs};
+                    birth(c, off, vc, sc, act);
+                } else if (kind == MCS_PROC_ATTACHMENT) {
                     if (act) deaths += 1;
                     act = 0;
                 }

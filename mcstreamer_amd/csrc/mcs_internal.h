@@ -24,6 +24,10 @@
 #define MCS_FLAG_CAPACITY 4
 #define MCS_FLAG_NAN 8
 
+// energy bands of the banded majorant below the unbounded one (caps doubling up
+// to the table's last node)
+#define MCS_NBAND 20
+
 struct ProfEntry {
     double total_ms = 0.0;
     int64_t launches = 0;
@@ -149,6 +153,11 @@ struct mcs_table {
     double *param;    // nproc
     // FP32 copies for the split-precision kernels
     float *energy32 = nullptr, *rate32 = nullptr, *param32 = nullptr;
+    // energy bands of the banded majorant (mcs_table_majorants): per band
+    // (cap, majorant, 1 / majorant) in FP32, nband bands + the unbounded one
+    float *band32 = nullptr;
+    int32_t nband = 0;
+    std::vector<double> h_band_cap, h_band_rate;
     std::vector<int32_t> h_kind;
     // PhotoEmission rows: (log numin, log numax, photon_multiplier, photon_weight)
     // per row (nproc x 4, device and host), which rows are set, how many rows

@@ -289,7 +289,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6, 8))) void
 
 // The same step on the FP32 split store, at eight waves per SIMD (64 registers;
 // a few cold values spill): more waves hide the table and field loads better.
-template <int BS>
+template <int BS, int BANDED>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_sw_advance32(SwArgs32 a)
 {
     const int64_t tile = blockIdx.x;
@@ -316,29 +316,32 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void
                                 (float)(MCS_QM * a.S.e[3 * slot + 2])};
         MeshView32 M;
         int flights = 0, deaths = 0;
-        advance_one32<0, 0>(
-            c, off, v, w, s, act, rs, (float)a.S.dt[slot], T, M, acc_u, (float)(1.0 / SW_H32), flags,
-            flights, deaths,
-            [&](const int32_t *bc, const float *boff, const float *bv, float bs, uint8_t bact) {
-                unsigned long long r = atomicAdd((unsigned long long *)a.log_count, 1ULL);
-                if ((int64_t)r < a.L.cap) {
-                    a.L.parent[r] = i;
-                    a.L.seq[r] = nb;
-                    a.L.cell[r] = pack_cell(bc[0], bc[1], bc[2]);
+        auto birth = [&](const int32_t *bc, const float *boff, const float *bv, float bs, uint8_t bact) {
+            unsigned long long r = atomicAdd((unsigned long long *)a.log_count, 1ULL);
+            if ((int64_t)r < a.L.cap) {
+                a.L.parent[r] = i;
+                a.L.seq[r] = nb;
+                a.L.cell[r] = pack_cell(bc[0], bc[1], bc[2]);
 #pragma unroll
-                    for (int d = 0; d < 3; ++d) {
-                        a.L.off[d][r] = boff[d];
-                        a.L.v[d][r] = bv[d];
-                    }
-                    a.L.s[r] = bs;
-                    a.L.act[r] = bact;
-                } else {
-                    flags |= MCS_FLAG_CAPACITY;
+                for (int d = 0; d < 3; ++d) {
+                    a.L.off[d][r] = boff[d];
+                    a.L.v[d][r] = bv[d];
                 }
-                if (bact) tally_add32(acc, bv, T);
-                nb += 1;
-            },
-            [&](int) {}, [&](const int32_t *, const float *, double) {});
+                a.L.s[r] = bs;
+                a.L.act[r] = bact;
+            } else {
+                flags |= MCS_FLAG_CAPACITY;
+            }
+            if (bact) tally_add32(acc, bv, T);
+            nb += 1;
+        };
+        if (BANDED)
+            advance_one32_banded(c, off, v, w, s, act, rs, (float)a.S.dt[slot], T, acc_u, (float)(1.0 / SW_H32),
+                                 flags, flights, deaths, birth);
+        else
+            advance_one32<0, 0>(c, off, v, w, s, act, rs, (float)a.S.dt[slot], T, M, acc_u, (float)(1.0 / SW_H32),
+                                flags, flights, deaths, birth, [&](int) {},
+                                [&](const int32_t *, const float *, double) {});
         const int32_t nmax[3] = {MCS_CX_MAX, MCS_CY_MAX, MCS_CZ_MAX};
         rebase32(c, off, nmax);
         a.b.cell[i] = pack_cell(c[0], c[1], c[2]);
@@ -944,6 +947,7 @@ struct mcs_swarm_batch {
     int64_t ntiles = 0;
     int block = MCS_BLOCK;
     int compact_every = 8;
+    int majorant = MCS_MAJORANT_GLOBAL;
     int64_t step_count = 0;
     bool dirty = true;
     hipGraphExec_t g_step = nullptr, g_compact = nullptr;
@@ -1247,6 +1251,17 @@ extern "C" int32_t mcs_swarm_batch_config(mcs_swarm_batch *b, int32_t block, int
     return MCS_OK;
 }
 
+extern "C" int32_t mcs_swarm_batch_majorant(mcs_swarm_batch *b, int32_t mode)
+{
+    if (!b || (mode != MCS_MAJORANT_GLOBAL && mode != MCS_MAJORANT_BANDED)) return MCS_E_ARG;
+    if (mode == MCS_MAJORANT_BANDED && b->precision != MCS_PREC_F32)
+        return mcs_fail(b->ctx, MCS_E_ARG, "swarm batch: the banded majorant is for FP32 batches (an FP64 batch "
+                                           "stays draw for draw with a lone FP64 population)");
+    b->majorant = mode;
+    b->dirty = true;
+    return MCS_OK;
+}
+
 template <typename A>
 static void fill_common(mcs_swarm_batch *b, A &a)
 {
@@ -1296,7 +1311,10 @@ static void launch_advance(mcs_swarm_batch *b, const SwArgs64 &a, dim3 g)
 template <int BS>
 static void launch_advance(mcs_swarm_batch *b, const SwArgs32 &a, dim3 g)
 {
-    hipLaunchKernelGGL(k_sw_advance32<BS>, g, dim3(BS), 0, b->ctx->stream, a);
+    if (b->majorant == MCS_MAJORANT_BANDED)
+        hipLaunchKernelGGL((k_sw_advance32<BS, 1>), g, dim3(BS), 0, b->ctx->stream, a);
+    else
+        hipLaunchKernelGGL((k_sw_advance32<BS, 0>), g, dim3(BS), 0, b->ctx->stream, a);
 }
 
 template <int BS, typename A>

@@ -110,6 +110,17 @@ class CollisionTable:
                 return p
         return None
 
+    def majorants(self):
+        """The energy bands of the banded majorant (mcs_table_majorants):
+        (caps in J, majorant rates in 1/s), the last band unbounded."""
+        nb = ctypes.c_int32()
+        check(self.ctx.handle, _lib.lib().mcs_table_majorants(self.handle, 0, ctypes.byref(nb), None, None))
+        cap = np.zeros(nb.value)
+        rate = np.zeros(nb.value)
+        check(self.ctx.handle, _lib.lib().mcs_table_majorants(self.handle, nb.value, ctypes.byref(nb), dptr(cap),
+                                                              dptr(rate)))
+        return cap, rate
+
     def __len__(self):
         return len(self.proc)
 

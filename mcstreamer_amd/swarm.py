@@ -231,9 +231,14 @@ class SwarmBatch:
     step of every slot is advance! + the per-step tallies (mean v_z, mean energy,
     mean ionization and attachment rate over the actives) + resample!; every
     `compact_every` steps each slab is compacted (stable, actives only).
+
+    `majorant` is the null-collision majorant (mcs_swarm_batch_majorant):
+    "global", the table's maxrate as in the reference, or "banded" (FP32 only),
+    the maxrate of the particle's energy band; an FP32 batch defaults to "banded",
+    an FP64 batch is always "global".
     """
 
-    def __init__(self, total_capacity, max_slots, seed=0, precision="f64", ctx=None):
+    def __init__(self, total_capacity, max_slots, seed=0, precision="f64", ctx=None, majorant=None):
         self.ctx = ctx or context()
         self.seed = int(seed)
         self.precision = precision
@@ -244,6 +249,14 @@ class SwarmBatch:
         self.handle = h
         self.max_slots = int(max_slots)
         self._tables = {}
+        self.set_majorant(majorant or ("banded" if precision == "f32" else "global"))
+
+    def set_majorant(self, mode):
+        """"global" or "banded" (see the class docstring); the step graph is
+        captured again at the next step."""
+        m = {"global": _lib.MAJORANT_GLOBAL, "banded": _lib.MAJORANT_BANDED}[mode]
+        check(self.ctx.handle, _lib.lib().mcs_swarm_batch_majorant(self.handle, m))
+        self.majorant = mode
 
     def configure(self, block=256, compact_every=8):
         check(self.ctx.handle, _lib.lib().mcs_swarm_batch_config(self.handle, int(block), int(compact_every)))
