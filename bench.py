@@ -14,10 +14,11 @@ bytes per solve over its duration (HIP events around each solve);
 issue rate beside it.  `late_run` times steps of the same run near its end.
 `cpu_baseline` times whole oracle steps of the same state on every thread the
 OpenMP build of the oracle gets, with the single-threaded figure beside it.
-`swarm` holds the swarm half of the metric: batched swarm solves/s against one
-plain FP64 launch sequence per instance, and served requests (p50/p99 latency,
-solves/s) at the configuration's arrival rate and at a rate the device
-sustains, against a plain FP64 server on the same stream.
+`swarm` holds the swarm half of the metric: batched swarm solves/s (FP32, banded
+null-collision majorant, and the same batch with the table's one maxrate)
+against one plain FP64 launch sequence per instance, and served requests
+(p50/p99 latency, solves/s) at the configuration's arrival rate and at a rate
+the device sustains, against a plain FP64 server on the same stream.
 """
 import argparse
 import json
@@ -428,7 +429,7 @@ def swarm_block(args):
     tune = b.autotune(log=note)
     note(f"c3: autotuned {getattr(b, 'tuned', None)}")
     # timed at the stage the plain sample is timed at (after c3_plain_relax steps)
-    done = sum(max(2, ce) + 1 for _, ce in tune)
+    done = b.tune_steps
     b.step(max(3, args.c3_plain_relax - done))
     ctx.synchronize()
     note("c3: relaxed")

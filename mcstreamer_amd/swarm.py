@@ -265,26 +265,32 @@ class SwarmBatch:
                  min_steps=2, log=None):
         """Project-written autotuner of the launch shape: runs real steps under each
         (workgroup size, compaction cadence) candidate, one compaction per trial,
-        and keeps the fastest per step.  The choice never changes results (a slot's
-        trajectory is the same under every shape and cadence); the trial steps count
-        as ordinary steps.  log(message), when given, hears each trial as it ends.
-        Returns {candidate: seconds per step}."""
+        and keeps the fastest per step.  The populations change as they relax, so
+        the candidates run twice, forward and then in reverse order, and each is
+        scored by the mean of its two trials.  The choice never changes results (a
+        slot's trajectory is the same under every shape and cadence); the trial
+        steps count as ordinary steps.  log(message), when given, hears each trial
+        as it ends.  Returns {candidate: seconds per step}."""
         import time as _t
-        timings = {}
-        for blk, ce in candidates:
-            self.configure(blk, ce)
-            self.step(1)                      # capture the graphs for this shape
-            self.ctx.synchronize()
-            k = max(min_steps, ce)
-            t0 = _t.perf_counter()
-            self.step(k)
-            self.ctx.synchronize()
-            timings[(blk, ce)] = (_t.perf_counter() - t0) / k
-            if log:
-                log(f"autotune {blk} threads, compaction every {ce}: {timings[(blk, ce)] * 1e3:.2f} ms per step")
+        trials = {c: [] for c in candidates}
+        for order in (list(candidates), list(reversed(candidates))):
+            for blk, ce in order:
+                self.configure(blk, ce)
+                self.step(1)                      # capture the graphs for this shape
+                self.ctx.synchronize()
+                k = max(min_steps, ce)
+                t0 = _t.perf_counter()
+                self.step(k)
+                self.ctx.synchronize()
+                trials[(blk, ce)].append((_t.perf_counter() - t0) / k)
+                if log:
+                    log(f"autotune {blk} threads, compaction every {ce}: {trials[(blk, ce)][-1] * 1e3:.2f} ms "
+                        "per step")
+        timings = {c: sum(v) / len(v) for c, v in trials.items()}
         best = min(timings, key=timings.get)
         self.configure(*best)
         self.tuned = best
+        self.tune_steps = sum(2 * (max(min_steps, ce) + 1) for _, ce in candidates)
         return timings
 
     def add(self, table, particles, e, dt, n0, capacity, instance, tol=0.1, relax_steps=0,
