@@ -336,6 +336,13 @@ struct mcs_mg {
     int64_t sm_max = 0;                          // levels of more cells: one launch per pass
     bool split0 = false;                         // level-0 correction in the colour-split layout
     bool use_graph = true;                       // replay the cycle body as a HIP graph
+    // Tabulated tails: below the tail's start level the cycle is a fixed linear
+    // map of that level's right-hand side (x starts at zero, the coarse boundary
+    // rules are homogeneous), kept as a (ghosted length x cells) row-major matrix
+    // of the working precision: tm_z for mgv! from x = 0, tm_f for fmg!'s coarse
+    // part from the restricted right-hand side.  tm_level -1: none.
+    int tm_level = -1;
+    void *tm_z = nullptr, *tm_f = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     double *graph_x = nullptr;
     const double *graph_b = nullptr;

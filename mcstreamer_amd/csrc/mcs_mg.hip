@@ -1352,11 +1352,9 @@ __device__ void tail_vcycle(const Tail &T, E *lds, int start)
 // mgv! for levels start..L in one workgroup: x0 and b0 are loaded (x0 unless the
 // start is zero or a prolongation), x0 is written back.
 template <typename E>
-__global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, E *x0, const E *b0, const E *zc,
-                                                            int zero, int fresh)
+__device__ __forceinline__ void tail_body(const Tail &T, int start, E *x0, const E *b0, const E *zc, int zero,
+                                          int fresh, E *lds)
 {
-    extern __shared__ double lds_raw[];
-    E *lds = (E *)lds_raw;
     const BcCoef bc = T.bc;
     if (zc) {
         // the start level's interior is interpolate!(x, zc) (overwrite); its
@@ -1380,6 +1378,24 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
     tail_copy(x0, lds + T.off_x[start], T.len[start]);
 }
 
+template <typename E>
+__global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, E *x0, const E *b0, const E *zc,
+                                                            int zero, int fresh)
+{
+    extern __shared__ double lds_raw[];
+    tail_body(T, start, x0, b0, zc, zero, fresh, (E *)lds_raw);
+}
+
+// Tabulation of the tail (tabulate_tails): block q runs mgv! from x = 0 on its
+// own pair of start-level arrays, q lengths into x0 and b0.
+template <typename E>
+__global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail_batch(Tail T, int start, E *x0, const E *b0)
+{
+    extern __shared__ double lds_raw[];
+    const int64_t off = (int64_t)blockIdx.x * T.len[start];
+    tail_body(T, start, x0 + off, b0 + off, (const E *)nullptr, 1, 0, (E *)lds_raw);
+}
+
 // The coarse part of fmg!'s correction (Multigrid.jl:396-426) in one workgroup:
 // the right-hand sides of levels start..L restricted in LDS from rfine (the
 // residual of level start - 1, ghost layout; restrict_item's arithmetic), the
@@ -1388,11 +1404,9 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail(Tail T, int start, 
 // (the fresh shell the next interpolation reads).  The start level's solution,
 // shell included, is written to x0.
 template <typename E>
-__global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail_fmg(Tail T, int start, E *x0, const E *rfine,
-                                                                const E *ainv)
+__device__ __forceinline__ void tail_fmg_body(const Tail &T, int start, E *x0, const E *rfine, const E *ainv,
+                                              E *lds)
 {
-    extern __shared__ double lds_raw[];
-    E *lds = (E *)lds_raw;
     const BcCoef bc = T.bc;
     {
         const Lvl F = tail_lvl(T, start - 1), C = tail_lvl(T, start);
@@ -1443,6 +1457,93 @@ __global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail_fmg(Tail T, int sta
         __syncthreads();
     }
     tail_copy(x0, lds + T.off_x[start], T.len[start]);
+}
+
+template <typename E>
+__global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail_fmg(Tail T, int start, E *x0, const E *rfine,
+                                                                const E *ainv)
+{
+    extern __shared__ double lds_raw[];
+    tail_fmg_body(T, start, x0, rfine, ainv, (E *)lds_raw);
+}
+
+// Tabulation of fmg!'s coarse part: block q on its own output and fine residual
+// (rlen apart).
+template <typename E>
+__global__ __launch_bounds__(MG_TAIL_THREADS) void k_mg_tail_fmg_batch(Tail T, int start, E *x0, const E *rfine,
+                                                                      const E *ainv, int64_t rlen)
+{
+    extern __shared__ double lds_raw[];
+    tail_fmg_body(T, start, x0 + (int64_t)blockIdx.x * T.len[start], rfine + (int64_t)blockIdx.x * rlen, ainv,
+                  (E *)lds_raw);
+}
+
+// ---- tabulated tails ---------------------------------------------------------
+//
+// Below the tail's start level S the cycle is linear in S's right-hand side:
+// mgv! starts from x = 0 (the caller's xh .= 0), the coarse levels' boundary
+// rules are the homogeneous ghost = +-valid, and every operation (sweeps,
+// residual, restriction, the direct coarse solve, prolongation) is linear.  The
+// whole ghosted output of S is therefore x = M b over b's valid cells.  M is
+// tabulated once per solver by running the tail itself on unit right-hand sides
+// and applied as one wide matrix-vector product in place of the single-workgroup
+// chain.  Same operator; the rounding differs from the sweeps' at the level of
+// the last bits of each entry.
+
+// Unit inputs: array q (len apart) is zero except at one valid cell, which holds
+// v: cell q of level L (i fastest), or with child = 1 the first child
+// (2i, 2j, 2k) of coarse cell q on the finer level L (whose restriction
+// 4 v / 8 then puts v / 2 in cell q).
+template <typename E>
+__global__ void k_tm_units(E *a, int64_t len, Lvl L, int64_t ncols, int child, E v)
+{
+    const int64_t t = gidx();
+    if (t >= len * ncols) return;
+    const int64_t q = t / len, p = t - q * len;
+    const uint32_t cx = child ? L.n[0] / 2 : L.n[0], cy = child ? L.n[1] / 2 : L.n[1];
+    int64_t i, j, k;
+    split3(q, cx, cy, i, j, k);
+    const int64_t pos = child ? g_index(L, 2 * i + 1, 2 * j + 1, 2 * k + 1) : g_index(L, i + 1, j + 1, k + 1);
+    a[t] = p == pos ? v : E(0);
+}
+
+// M[r][q] = X[q][r] (row-major rows x ncols from ncols arrays of rows).
+template <typename E>
+__global__ void k_tm_transpose(E *M, const E *X, int64_t rows, int64_t ncols)
+{
+    const int64_t t = gidx();
+    if (t >= rows * ncols) return;
+    const int64_t r = t / ncols, q = t - r * ncols;
+    M[t] = X[q * rows + r];
+}
+
+#define MG_TM_WAVES 4
+// x = M b on level L: b's valid cells staged in LDS in cell order, a wavefront per
+// output position (ghosts included), 64 columns per step in FP64 with fused
+// multiply-adds, the lane sums combined by a fixed butterfly: the same sum on
+// every run.
+template <typename E>
+__global__ __launch_bounds__(MG_TM_WAVES * MCS_WAVE) void k_mg_tailmat(const E *__restrict__ M,
+                                                                      const E *__restrict__ b, E *__restrict__ x,
+                                                                      Lvl L, int64_t rows)
+{
+    __shared__ E bs[MG_TAIL_MAXCELLS];
+    const int C = L.n[0] * L.n[1] * L.n[2];
+    for (int t = threadIdx.x; t < C; t += MG_TM_WAVES * MCS_WAVE) {
+        int64_t i, j, k;
+        split3(t, (uint32_t)L.n[0], (uint32_t)L.n[1], i, j, k);
+        bs[t] = b[g_index(L, i + 1, j + 1, k + 1)];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & (MCS_WAVE - 1);
+    const int64_t r = (int64_t)blockIdx.x * MG_TM_WAVES + (threadIdx.x / MCS_WAVE);
+    if (r >= rows) return;
+    const E *row = M + r * C;
+    double acc = 0.0;
+    for (int q = lane; q < C; q += MCS_WAVE) acc = __builtin_fma((double)row[q], (double)bs[q], acc);
+#pragma unroll
+    for (int o = MCS_WAVE / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, MCS_WAVE);
+    if (lane == 0) x[r] = (E)acc;
 }
 
 // ---------------------------------------------------------------- host side
@@ -1515,6 +1616,7 @@ static int coarse_inverse(const int32_t n[3], const int32_t bc[6], std::vector<d
 
 template <typename T> static bool in_tail(const mcs_mg *mg, int level);
 template <typename T> static bool smooth_fits(mcs_mg *mg, int level);
+template <typename T> static int32_t tabulate_tails(mcs_mg *mg);
 
 extern "C" int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg **out)
 {
@@ -1665,6 +1767,8 @@ extern "C" int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg 
     mg->npartial = cdiv(N0, MCS_BLOCK);
     if (hipMalloc(&mg->partial, sizeof(double) * (mg->npartial + 8)) != hipSuccess)
         return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation failed");
+    const int32_t trc = f32 ? tabulate_tails<float>(mg) : tabulate_tails<double>(mg);
+    if (trc) return trc;
     *out = mg;
     return MCS_OK;
 }
@@ -1689,6 +1793,8 @@ extern "C" int32_t mcs_mg_destroy(mcs_mg *mg)
     if (mg->u_tmp) hipFree(mg->u_tmp);
     if (mg->b_tmp) hipFree(mg->b_tmp);
     if (mg->bc_dev) hipFree(mg->bc_dev);
+    if (mg->tm_z) hipFree(mg->tm_z);
+    if (mg->tm_f) hipFree(mg->tm_f);
     if (mg->graph_exec) hipGraphExecDestroy(mg->graph_exec);
     delete mg;
     return MCS_OK;
@@ -1851,9 +1957,98 @@ static int fmg_tail_start(const mcs_mg *mg)
     return S;
 }
 
+// x = M b on `level` from a tabulated tail.
+template <typename T>
+static void tail_matvec(mcs_mg *mg, const void *M, const T *b, T *x, int level)
+{
+    ProfScope ps(mg->ctx, "mg_tail");
+    const int64_t rows = mg->lv[level].len;
+    hipLaunchKernelGGL(k_mg_tailmat<T>, dim3((unsigned)cdiv(rows, MG_TM_WAVES)), dim3(MG_TM_WAVES * MCS_WAVE), 0,
+                       mg->ctx->stream, (const T *)M, b, x, lvl(mg, level), rows);
+}
+
+// Tabulates the tails of the working precision T (the mcs_mg fields tm_*): M_z
+// from mgv! on unit right-hand sides of the tail's start level S, and, when
+// fmg!'s coarse part starts at S too, M_f from unit residuals of level S - 1
+// whose restrictions are the unit right-hand sides.  Off with
+// MCS_MG_TAIL_MATRIX=0, or when S has more cells than the matrix-vector kernel
+// stages.
+template <typename T>
+static int32_t tabulate_tails(mcs_mg *mg)
+{
+    mcs_ctx *ctx = mg->ctx;
+    mg->tm_level = -1;
+    const char *e = getenv("MCS_MG_TAIL_MATRIX");
+    if (e && e[0] == '0') return MCS_OK;
+    int S = -1;
+    for (int l = 1; l <= mg->cfg.levels; ++l)
+        if (in_tail<T>(mg, l)) {
+            S = l;
+            break;
+        }
+    if (S < 1) return MCS_OK;
+    const int64_t C = ncells(mg, S), R = mg->lv[S].len;
+    if (C > MG_TAIL_MAXCELLS) return MCS_OK;
+    Tail Tl;
+    const int64_t words = tail_setup<T>(mg, S, Tl);
+    const size_t lds = (size_t)words * sizeof(T);
+    if (lds > 64 * 1024 &&
+        (hipFuncSetAttribute((const void *)k_mg_tail_batch<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)mg->tail_lds) != hipSuccess ||
+         hipFuncSetAttribute((const void *)k_mg_tail_fmg_batch<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)mg->tail_lds) != hipSuccess))
+        return MCS_OK;
+    const int Sf = fmg_tail_start<T>(mg);
+    const int64_t Rf = Sf == S ? mg->lv[S - 1].len : 0;
+    T *X = nullptr, *B = nullptr, *Mz = nullptr, *Mf = nullptr;
+    auto release = [&]() {
+        if (X) hipFree(X);
+        if (B) hipFree(B);
+        if (Mz) hipFree(Mz);
+        if (Mf) hipFree(Mf);
+    };
+    if (hipMalloc(&X, sizeof(T) * C * R) != hipSuccess ||
+        hipMalloc(&B, sizeof(T) * C * std::max(R, Rf)) != hipSuccess ||
+        hipMalloc(&Mz, sizeof(T) * R * C) != hipSuccess || (Rf && hipMalloc(&Mf, sizeof(T) * R * C) != hipSuccess)) {
+        release();
+        return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation of the tail tables failed");
+    }
+    hipLaunchKernelGGL(k_tm_units<T>, grid_for(C * R), dim3(MCS_BLOCK), 0, ctx->stream, B, R, lvl(mg, S), C, 0,
+                       T(1));
+    hipLaunchKernelGGL(k_mg_tail_batch<T>, dim3((unsigned)C), dim3(MG_TAIL_THREADS), lds, ctx->stream, Tl, S, X,
+                       (const T *)B);
+    hipLaunchKernelGGL(k_tm_transpose<T>, grid_for(C * R), dim3(MCS_BLOCK), 0, ctx->stream, Mz, (const T *)X, R, C);
+    if (Rf) {
+        hipLaunchKernelGGL(k_tm_units<T>, grid_for(C * Rf), dim3(MCS_BLOCK), 0, ctx->stream, B, Rf,
+                           lvl(mg, S - 1), C, 1, T(2));
+        hipLaunchKernelGGL(k_mg_tail_fmg_batch<T>, dim3((unsigned)C), dim3(MG_TAIL_THREADS), lds, ctx->stream, Tl,
+                           S, X, (const T *)B, coarse_inv<T>(mg), Rf);
+        hipLaunchKernelGGL(k_tm_transpose<T>, grid_for(C * R), dim3(MCS_BLOCK), 0, ctx->stream, Mf, (const T *)X,
+                           R, C);
+    }
+    const hipError_t le = hipGetLastError();
+    const hipError_t se = hipStreamSynchronize(ctx->stream);
+    hipFree(X);
+    hipFree(B);
+    X = B = nullptr;
+    if (le != hipSuccess || se != hipSuccess) {
+        release();
+        return mcs_fail(ctx, MCS_E_HIP, "mg: tabulating the tail failed (%s)",
+                        hipGetErrorString(le != hipSuccess ? le : se));
+    }
+    mg->tm_level = S;
+    mg->tm_z = Mz;
+    mg->tm_f = Mf;
+    return MCS_OK;
+}
+
 template <typename T>
 static void mg_tail(mcs_mg *mg, int level, T *x, const T *b, const T *zc, bool zero, bool fresh)
 {
+    if (level == mg->tm_level && zero && !zc && !fresh) {
+        tail_matvec<T>(mg, mg->tm_z, b, x, level);
+        return;
+    }
     Tail Tl;
     const int64_t off = tail_setup<T>(mg, level, Tl);
     ProfScope ps(mg->ctx, "mg_tail");
@@ -1996,8 +2191,10 @@ static void mgv(mcs_mg *mg, T *x, const T *b, int level, bool zero, const T *zc 
                                mg->ctx->stream, lres<T>(mg, level + 1), (const T *)x, b, lvl(mg, level),
                                coefs(mg), lsol<T>(mg, level + 1));
     }
+    // xh's interior was zeroed with the residual; a tail reads no ghost before it
+    // writes it, so for a tail this start is x = 0 (which a tabulated tail needs)
     T *xh = lsol<T>(mg, level + 1);
-    mgv<T>(mg, xh, lres<T>(mg, level + 1), level + 1, false);
+    mgv<T>(mg, xh, lres<T>(mg, level + 1), level + 1, in_tail<T>(mg, level + 1));
     mg_interp<T>(mg, level, x, xh, 1, sp);
     // the ghost shell the caller reads: before the last sweep (the second-to-last
     // sweep's colour-1 pass writes it), or after it (fresh: the last pass writes
@@ -2032,7 +2229,18 @@ static T *fmg_correction(mcs_mg *mg)
     }
     int top = L;   // the finest level solved so far
     T *z = lsol<T>(mg, L);
-    if (S > 0) {
+    if (S > 0 && S == mg->tm_level && mg->tm_f) {
+        // the restricted right-hand side of S (restrict_item's arithmetic, as the
+        // tail forms it), then the tabulated coarse part
+        {
+            ProfScope ps(ctx, "mg_restrict");
+            hipLaunchKernelGGL(k_mg_restrict<T>, grid_for(ncoarse_cells(mg, S - 1)), dim3(MCS_BLOCK), 0, ctx->stream,
+                               lres<T>(mg, S), (const T *)lres<T>(mg, S - 1), lvl(mg, S - 1));
+        }
+        tail_matvec<T>(mg, mg->tm_f, (const T *)lres<T>(mg, S), lsol<T>(mg, S), S);
+        top = S;
+        z = lsol<T>(mg, S);
+    } else if (S > 0) {
         Tail Tl;
         const int64_t words = tail_setup<T>(mg, S, Tl);
         ProfScope ps(ctx, "mg_tail");

@@ -46,18 +46,26 @@ def _check_stats(g, r, nproc, tag):
     assert got == want, tag
 
 
-def _compare(mp, E, P):
+def _compare(mp, E, P, rtol=0.0):
+    """Ids and flags bitwise; states bitwise (rtol 0) or to rtol relative to
+    each array's largest magnitude."""
+    def same(a, b, msg):
+        if rtol == 0.0:
+            np.testing.assert_array_equal(a, b, err_msg=msg)
+        else:
+            np.testing.assert_allclose(a, b, rtol=rtol, atol=rtol * np.abs(b).max(initial=0.0), err_msg=msg)
+
     g, r = mp.electron.download(), E.view()
     np.testing.assert_array_equal(g.id, r["id"])
     np.testing.assert_array_equal(g.active, r["active"])
     for k in ("x", "v", "w", "s"):
-        np.testing.assert_array_equal(getattr(g, k), r[k], err_msg="electron " + k)
+        same(getattr(g, k), r[k], "electron " + k)
     g, r = mp.photon.download(), P.view()
     assert g.next_id == P.next_id
     np.testing.assert_array_equal(g.id, r["id"])
     np.testing.assert_array_equal(g.active, r["active"])
     for k in ("x", "v", "nu", "w", "s"):
-        np.testing.assert_array_equal(getattr(g, k), r[k], err_msg="photon " + k)
+        same(getattr(g, k), r[k], "photon " + k)
 
 
 @pytest.mark.parametrize("tracked", [True, False])
@@ -192,7 +200,9 @@ def test_streamer_steps_with_photons_match_oracle():
     """Whole 3D streamer steps with photoionization: poisson! (CIC deposit,
     multigrid, faces), advance! over electrons and photons with the tracker,
     photon repack!, the canonical sort and resample!, against the same chain in
-    the oracle; counts bit-exact every step, particles and photons bitwise."""
+    the oracle; counts bit-exact every step, ids bitwise, particle and photon
+    states to 1e-12 relative (the solver's tabulated tail rounds the potential
+    differently from the oracle's sweeps in the last bits)."""
     n, npairs, cap = (32, 32, 64), 20_000, 400_000
     cfg = S.make_config1(n=n, npairs=npairs, cap=cap, seed=5,
                          photo=dict(rescale=1.0, photon_multiplier=8.0, photon_weight=1.0))
@@ -232,5 +242,6 @@ def test_streamer_steps_with_photons_match_oracle():
         _check_stats(out["photons"], pst, 1, f"photon loop, step {i}")
         assert out["resample"].n_out == rst.n_out
     assert pst.n_out > 0
-    _compare(jumc.MultiPopulation(sim.popl, sim.photons), E, P)
-    np.testing.assert_array_equal(sim.fields.get("qfixed"), M.qfixed)
+    _compare(jumc.MultiPopulation(sim.popl, sim.photons), E, P, rtol=1e-12)
+    np.testing.assert_allclose(sim.fields.get("qfixed").ravel(), M.qfixed, rtol=1e-12,
+                               atol=1e-12 * np.abs(M.qfixed).max())

@@ -3,8 +3,8 @@ Poisson and resampling every step) against the oracle composed in the same
 order: deposit, charge, multigrid solve, faces, advance with the mesh field and
 ion tracker, canonical sort, per-cell resample.  Ids, counts and events are
 bit-exact; floating state is compared at 1e-12 relative (the 3D solver's
-residual norms are summed in another order than the oracle's, which moves
-the potential by rounding)."""
+tabulated tail and its residual norms round differently from the oracle's
+sweeps and sums, which moves the potential in the last bits)."""
 import numpy as np
 import pytest
 
@@ -57,7 +57,14 @@ def _compare_state(sim, P, M, u_rtol):
     r = P.view()
     np.testing.assert_array_equal(g.id, r["id"])
     np.testing.assert_array_equal(g.active, r["active"])
-    for k in ("x", "v", "w", "s"):
+    # vectors to 1e-12 of each particle's |x| or |v| (a component near zero is a
+    # difference of large terms), scalars to 1e-12 relative
+    for k in ("x", "v"):
+        a, b = getattr(g, k), r[k]
+        err = np.abs(a - b).max(axis=1)
+        bound = 1e-12 * np.linalg.norm(b, axis=1)
+        assert np.all(err <= bound), f"{k}: {np.count_nonzero(err > bound)} particles, worst {np.max(err / np.maximum(bound, 1e-300)):.3g} x bound"
+    for k in ("w", "s"):
         np.testing.assert_allclose(getattr(g, k), r[k], rtol=1e-12, atol=0, err_msg=k)
     np.testing.assert_allclose(sim.fields.get("u").ravel(), M.u, rtol=u_rtol, atol=u_rtol * np.abs(M.u).max())
     np.testing.assert_allclose(sim.fields.get("qfixed").ravel(), M.qfixed, rtol=1e-12,
