@@ -479,6 +479,12 @@ int32_t mcs_cyl_sort_by_cell(mcs_pop *p, mcs_cgrid *g);
  * streams are those of mcs_resample_cells, keyed by the flat cell index). */
 int32_t mcs_cyl_resample(mcs_pop *p, mcs_cgrid *g, int32_t ntarget, int32_t nmax, uint64_t seed,
                          uint32_t instance, uint32_t step, mcs_resample_stats *stats);
+/* GridFields.p and .wtotal (gridfields.jl:41-48) as the last mcs_cyl_resample
+ * left them (resample.jl:16-30, 63-72): per cell of the ghosted (M+2)(N+2)
+ * layout the count of its particles plus one per split, and the sum of their
+ * weights; zeros before the first resample.  Read by the HDF5 export
+ * (plot.jl:210-240) and SavedGridFields (io.jl:9-44). */
+int32_t mcs_cgrid_resample_fields(mcs_cgrid *g, int64_t *p, double *wtotal);
 /* advance! (mixed_population.jl:24-54) in the axisymmetric FieldInterp
  * (gridfields.jl:115-190: bilinear er, ez on the staggered faces, er rotated by
  * x/r, y/r; zero outside 0 < r < R, 0 < z < L) with, when track != 0, the

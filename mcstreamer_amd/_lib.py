@@ -217,6 +217,7 @@ SIGNATURES = {
     "mcs_cgrid_freebc_coeffs": [_P, _PD],
     "mcs_cyl_sort_by_cell": [_P, _P],
     "mcs_cyl_resample": [_P, _P, _I32, _I32, _U64, _U32, _U32, ctypes.POINTER(ResampleStats)],
+    "mcs_cgrid_resample_fields": [_P, _PI64, _PD],
     "mcs_cyl_efield_faces": [_P, _D],
     "mcs_advance_cyl": [_P, _P, _P, _D, _I32, _U64, _U32, _U32, ctypes.POINTER(StepStats)],
     "mcs_table_set_photoemission": [_P, _I32, _D, _D, _D, _D],

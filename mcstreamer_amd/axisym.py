@@ -9,6 +9,8 @@
   solve_freebc! (freebc.jl:4-36)            -> poisson(freebc=True)
   resample! (resample.jl:7-84)              -> sort_by_cell + resample
   the solver of main.jl:134-145             -> poisson_config
+  SavedGridFields (io.jl:9-44), h5export    -> saved_fields, h5export
+    (plot.jl:210-240)
 Particles stay in 3D Cartesian coordinates; the grid is indexed by
 r = sqrt(x^2 + y^2) and z.  Arrays come back as (rows j, columns i) = (z, r),
 the reference's column-major (i, j) layout read row-major.
@@ -58,6 +60,19 @@ class Grid:
     def rc(self):
         return np.array([_lerpi(i, self.M - 1, *self._rc) for i in range(self.M)])
 
+    @property
+    def zc(self):
+        z0, z1 = self._centres(self.L, self.N)
+        return np.array([_lerpi(j, self.N - 1, z0, z1) for j in range(self.N)])
+
+    @property
+    def rf(self):
+        return np.array([_lerpi(i, self.M, 0.0, self.R) for i in range(self.M + 1)])
+
+    @property
+    def zf(self):
+        return np.array([_lerpi(j, self.N, 0.0, self.L) for j in range(self.N + 1)])
+
     def dV(self, i):
         """dV(grid, i) = 2 pi rc[i] dr dz (grid.jl:76), i 1-based."""
         return 2.0 * 3.141592653589793 * float(self.rc[i - 1]) * self.dr * self.dz
@@ -96,6 +111,16 @@ class GridFields:
 
     def zero(self, name):
         check(self.ctx.handle, _lib.lib().mcs_cgrid_zero(self.handle, CARR[name]))
+
+    def resample_fields(self):
+        """(p, wtotal) as the last resample left them (gridfields.jl:41-48,
+        resample.jl:16-30, 63-72), shape (N+2, M+2); zeros before any resample."""
+        M, N = self.grid.M, self.grid.N
+        p = np.zeros((N + 2, M + 2), dtype=np.int64)
+        w = np.zeros((N + 2, M + 2))
+        check(self.ctx.handle, _lib.lib().mcs_cgrid_resample_fields(
+            self.handle, p.ctypes.data_as(_lib._PI64), dptr(w)))
+        return p, w
 
     def __del__(self):
         try:
@@ -184,3 +209,39 @@ def resample(popl, fields, ntarget, nmax, step=0):
                                                        popl.seed, popl.instance, int(step), ctypes.byref(st)))
     return st
 
+
+def saved_fields(fields):
+    """SavedGridFields(fields) (io.jl:9-44): the grid's arrays as the reference saves
+    them, each in this module's (rows j, columns i) layout with its ghosts:
+    qfixed, qpart, q0, q, u, er, ez, p, wtotal.  q0 is poisson!'s charge before
+    the denoiser, ((qfixed + qpart) - ne) e with no fluid electrons
+    (poisson.jl:19-27); q is the same rounded through Float32 (the NullDenoiser's
+    input), as the last poisson left it."""
+    qfixed, qpart = fields.get("qfixed"), fields.get("qpart")
+    p, wtotal = fields.resample_fields()
+    return {"qfixed": qfixed, "qpart": qpart, "q0": ((qfixed + qpart) - 0.0) * co.elementary_charge,
+            "q": fields.get("q"), "u": fields.get("u"), "er": fields.get("er"), "ez": fields.get("ez"),
+            "p": p, "wtotal": wtotal}
+
+
+def h5export(fields, filename, saved=None):
+    """h5export (MCStreamer/src/plot.jl:210-240): an HDF5 file with group `grid`
+    (R, L as Float64 scalars, M, N as Int64 scalars, rc, rf, zc, zf as vectors) and
+    the datasets qfixed, qpart, q0, q, u, p, wtotal (cells 1:M x 1:N), er
+    (1:M+1 x 1:N) and ez (1:M x 1:N+1).  HDF5.jl stores a Julia (M, N) matrix with
+    the dimensions (N, M) of its column-major memory, which is the numpy array of
+    shape (N, M) here; p is Int64, the rest Float64.  `saved` (saved_fields'
+    dictionary) is read from the device when not given."""
+    from . import h5
+
+    g = fields.grid
+    M, N = g.M, g.N
+    sv = saved if saved is not None else saved_fields(fields)
+    tree = {"grid": {"R": np.float64(g.R), "L": np.float64(g.L), "M": np.int64(M), "N": np.int64(N),
+                     "rc": g.rc, "rf": g.rf, "zc": g.zc, "zf": g.zf}}
+    for k in ("qfixed", "qpart", "q0", "q", "u", "wtotal"):
+        tree[k] = sv[k][1:N + 1, 1:M + 1]
+    tree["p"] = np.asarray(sv["p"], dtype=np.int64)[1:N + 1, 1:M + 1]
+    tree["er"] = sv["er"][1:N + 1, 1:M + 2]
+    tree["ez"] = sv["ez"][1:N + 2, 1:M + 1]
+    h5.write(filename, tree)

@@ -400,7 +400,26 @@ extern "C" int32_t mcs_cgrid_destroy(mcs_cgrid *g)
     if (g->cs_new) hipFree(g->cs_new);
     if (g->cnt64) hipFree(g->cnt64);
     if (g->pre64) hipFree(g->pre64);
+    if (g->rs_p) hipFree(g->rs_p);
+    if (g->rs_w) hipFree(g->rs_w);
     delete g;
+    return MCS_OK;
+}
+
+extern "C" int32_t mcs_cgrid_resample_fields(mcs_cgrid *g, int64_t *p, double *wtotal)
+{
+    if (!g || !p || !wtotal) return MCS_E_ARG;
+    mcs_ctx *ctx = g->ctx;
+    const int64_t ncg = (int64_t)(g->M + 2) * (g->N + 2);
+    hipSetDevice(ctx->device);
+    if (!g->rs_p) {
+        memset(p, 0, sizeof(int64_t) * ncg);
+        memset(wtotal, 0, sizeof(double) * ncg);
+        return MCS_OK;
+    }
+    MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    MCS_HIP(ctx, hipMemcpy(p, g->rs_p, sizeof(int64_t) * ncg, hipMemcpyDeviceToHost));
+    MCS_HIP(ctx, hipMemcpy(wtotal, g->rs_w, sizeof(double) * ncg, hipMemcpyDeviceToHost));
     return MCS_OK;
 }
 

@@ -293,7 +293,13 @@ struct mcs_cgrid {
     int32_t *cs = nullptr, *cs_new = nullptr;
     int64_t *cnt64 = nullptr, *pre64 = nullptr;
     uint64_t sorted_gen = 0;   // mcs_pop::gen of the population the cell starts describe
+    // resample!'s p and wtotal per cell ((M+2)(N+2) each), as the last resample left them
+    int64_t *rs_p = nullptr;
+    double *rs_w = nullptr;
 };
+
+// Allocates (zeroed) the grid's resample p and wtotal arrays if it has none.
+int32_t mcs_cgrid_resample_arrays(mcs_cgrid *g);
 
 // The two-dimensional solver (mcs_mg2.hip): creation for cfg->ndim == 2, and the
 // solve on device arrays with ghosts.

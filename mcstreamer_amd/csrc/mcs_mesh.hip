@@ -1018,15 +1018,20 @@ int32_t mcs_mesh_faces32(mcs_mesh *m)
 // then the created particles, ids next_id + (created before this one).
 #define RS_PF 8   // weights loaded ahead in resample_cell
 
+// pfinal and wfinal receive the cell's p and wtotal as resample! leaves them
+// (resample.jl:16-30, 63-72): the count of its particles plus one per split, and
+// the sum of their weights in order.
 template <bool WRITE, typename Buf>
 __device__ void resample_cell(int64_t c, int64_t a, int64_t e, int32_t ntarget, int32_t nmax,
                               uint64_t seed, uint32_t instance, uint32_t step, const Buf &src,
                               const Buf &dst, int64_t base, uint64_t idbase, int64_t &nout,
-                              int64_t &ncreated, int64_t *ctr)
+                              int64_t &ncreated, int64_t *ctr, int64_t &pfinal, double &wfinal)
 {
     int64_t pc = e - a;
     nout = 0;
     ncreated = 0;
+    pfinal = 0;
+    wfinal = 0.0;
     if (pc == 0) return;
     mcs_stream rs;
     mcs_stream_init(&rs, seed, instance, MCS_PHASE_CELL_RESAMPLE, (uint64_t)c, step);
@@ -1091,6 +1096,8 @@ __device__ void resample_cell(int64_t c, int64_t a, int64_t e, int32_t ntarget, 
         }
     }
     nout = (merged ? 0 : (e - a)) + ncreated;
+    pfinal = pc;
+    wfinal = wtotal;
     if (!WRITE && ctr) {
         if (merged) {
             atomicAdd((unsigned long long *)&ctr[R_MERGED], 1ULL);
@@ -1115,6 +1122,8 @@ struct ResArgs {
     uint64_t next_id;
     int32_t *new_cs;
     int64_t *ctr;
+    int64_t *pcell;    // per cell p and wtotal as resample! leaves them, or null
+    double *wcell;
 };
 
 template <typename Buf>
@@ -1127,10 +1136,15 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_resample_count(ResArgs<Buf> r)
         r.packed[c] = e - a;
         return;
     }
-    int64_t nout, ncr;
+    int64_t nout, ncr, pf;
+    double wf;
     resample_cell<false, Buf>(c, a, e, r.ntarget, r.nmax, r.seed, r.instance, r.step, r.src, r.dst, 0,
-                         0, nout, ncr, r.ctr);
+                         0, nout, ncr, r.ctr, pf, wf);
     r.packed[c] = nout | (ncr << 32);
+    if (r.pcell) {
+        r.pcell[c] = pf;
+        r.wcell[c] = wf;
+    }
 }
 
 template <typename Buf>
@@ -1148,9 +1162,10 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_resample_write(ResArgs<Buf> r)
         r.new_cs[c + 1] = (int32_t)(tot & 0xffffffffLL);
         return;
     }
-    int64_t nout, ncr;
+    int64_t nout, ncr, pf;
+    double wf;
     resample_cell<true, Buf>(c, a, e, r.ntarget, r.nmax, r.seed, r.instance, r.step, r.src, r.dst, base,
-                        r.next_id + (uint64_t)crbase, nout, ncr, nullptr);
+                        r.next_id + (uint64_t)crbase, nout, ncr, nullptr, pf, wf);
 }
 
 // Survivors (particles of cells that are not merged, and those outside the mesh)
@@ -1205,6 +1220,8 @@ struct CellLists {
     int64_t ncells;
     int64_t *cnt64, *pre64;
     int32_t *new_cs;
+    int64_t *pcell = nullptr;   // resample!'s p and wtotal per cell, when kept
+    double *wcell = nullptr;
 };
 
 template <typename Buf, typename Copy>
@@ -1252,6 +1269,8 @@ static int32_t resample_run(mcs_pop *p, const CellLists &L, const Buf &src, cons
     r.next_id = p->next_id;
     r.new_cs = L.new_cs;
     r.ctr = ctr;
+    r.pcell = L.pcell;
+    r.wcell = L.wcell;
     {
         ProfScope ps(ctx, "resample");
         hipLaunchKernelGGL(k_resample_count<Buf>, dim3((unsigned)cdiv(nseg, MCS_BLOCK)), dim3(MCS_BLOCK), 0,
@@ -1352,6 +1371,19 @@ static int32_t cgrid_lists(mcs_cgrid *g)
         hipMalloc(&g->cnt64, sizeof(int64_t) * (ncg + 2)) != hipSuccess ||
         hipMalloc(&g->pre64, sizeof(int64_t) * (ncg + 2)) != hipSuccess)
         return mcs_fail(g->ctx, MCS_E_HIP, "cgrid: device allocation failed");
+    return mcs_cgrid_resample_arrays(g);
+}
+
+// resample!'s per-cell p and wtotal, zero until the first resample.
+int32_t mcs_cgrid_resample_arrays(mcs_cgrid *g)
+{
+    if (g->rs_p) return MCS_OK;
+    const int64_t ncg = (int64_t)(g->M + 2) * (g->N + 2);
+    if (hipMalloc(&g->rs_p, sizeof(int64_t) * ncg) != hipSuccess ||
+        hipMalloc(&g->rs_w, sizeof(double) * ncg) != hipSuccess)
+        return mcs_fail(g->ctx, MCS_E_HIP, "cgrid: device allocation failed");
+    MCS_HIP(g->ctx, mcs_zero(g->ctx, g->rs_p, sizeof(int64_t) * ncg));
+    MCS_HIP(g->ctx, mcs_zero(g->ctx, g->rs_w, sizeof(double) * ncg));
     return MCS_OK;
 }
 
@@ -1442,7 +1474,7 @@ extern "C" int32_t mcs_cyl_resample(mcs_pop *p, mcs_cgrid *g, int32_t ntarget, i
     int32_t rc = mcs_pop_ensure_alt(p);
     if (rc) return rc;
     const int64_t ncg = (int64_t)(g->M + 2) * (g->N + 2);
-    const CellLists L = {g->cs, ncg, g->cnt64, g->pre64, g->cs_new};
+    const CellLists L = {g->cs, ncg, g->cnt64, g->pre64, g->cs_new, g->rs_p, g->rs_w};
     const CylView C = cyl_view(g).C;
     mcs_pop_touch(p);
     rc = resample_run(p, L, p->buf[0], p->buf[1], ntarget, nmax, seed, instance, step, stats,

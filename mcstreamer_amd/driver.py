@@ -17,8 +17,12 @@ NotImplementedError rather than being ignored.  The seed positions come from
 numpy's generator seeded with rng_seed (the reference draws them from Julia's
 global RNG; the distribution, sampseg, is the same); everything after the
 seed draws from the Philox streams keyed by rng_seed.  Output (every
-interval.output) is one .npz per call with the grid arrays (u, er, ez, qfixed,
-qpart) in place of the reference's JLD2 files (io.jl).
+interval.output, main.jl:245-249) is {j:04d}.h5 with the fields of
+SavedGridFields (io.jl:9-44) in the layout of the reference's HDF5 export
+(plot.jl:210-240, axisym.h5export), which HDF5 readers (HDF5.jl, h5py) open
+directly, in place of the reference's JLD2 file {j:04d}.jld (Julia type
+records, which only JLD2 reads); beside it {j:04d}.npz holds the ghosted
+arrays u, er, ez, qfixed, qpart.
 """
 import json
 import os
@@ -141,10 +145,12 @@ class Run:
 
     def save_fields(self, j):
         os.makedirs(self.outpath, exist_ok=True)
-        arrs = {k: self.fields.get(k) for k in ("u", "er", "ez", "qfixed", "qpart")}
+        sv = axisym.saved_fields(self.fields)
+        arrs = {k: sv[k] for k in ("u", "er", "ez", "qfixed", "qpart")}
         meta = json.dumps({"t": self.i * self.dt, "step": self.i, "grid": [self.grid.R, self.grid.L,
                                                                            self.grid.M, self.grid.N]})
         np.savez(os.path.join(self.outpath, f"{j:04d}.npz"), meta=np.array(meta), **arrs)
+        axisym.h5export(self.fields, os.path.join(self.outpath, f"{j:04d}.h5"), saved=sv)
 
     def nsteps(self, k, output=True):
         """k steps of nsteps (main.jl:238-315): Poisson, output, advance!,

@@ -1298,6 +1298,28 @@ void ref_efield_faces(ref_mesh *m, double eb)
  * p < ntarget splits particles of w > 2 into round(w/2) and w - round(w/2),
  * raising p each time.  Output per cell: surviving originals, then the new
  * particles in creation order (new ids in output order); outside particles last. */
+/* GridFields.p and .wtotal as resample! (MCStreamer/src/resample.jl:7-76) leaves
+ * them: the first pass (:16-30) counts the cell's active particles into p and
+ * adds their weights into wtotal in order; the second pass (:33-76) adds one to
+ * p at each split, which happens to a particle with weight > 2 while p is below
+ * ntarget (:63-72) -- so later particles of the cell see the grown count; a cell
+ * above nmax is merged and its p stays. */
+void ref_resample_counts(const ref_pop *p, const int64_t *cell_start, int64_t ncells, int32_t ntarget,
+                         int32_t nmax, int64_t *pcell, double *wcell)
+{
+    for (int64_t c = 0; c < ncells; ++c) {
+        const int64_t a = cell_start[c], b = cell_start[c + 1];
+        int64_t pc = b - a;
+        double wtotal = 0.0;
+        for (int64_t i = a; i < b; ++i) wtotal += p->w[i];
+        if (pc <= nmax)
+            for (int64_t i = a; i < b; ++i)
+                if (pc < ntarget && p->w[i] > 2.0) pc += 1;
+        pcell[c] = pc;
+        wcell[c] = wtotal;
+    }
+}
+
 int32_t ref_resample_cells(ref_pop *p, ref_mesh *m, int32_t ntarget, int32_t nmax,
                            uint64_t seed, uint32_t instance, uint32_t step, ref_pop *scr,
                            ref_resample_stats *st)

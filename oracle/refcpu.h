@@ -213,6 +213,10 @@ int32_t ref_sort_by_cell(ref_pop *p, ref_mesh *m, ref_pop *scratch);
 void ref_deposit(const ref_pop *p, ref_mesh *m, int32_t which, int32_t scheme, double val);
 void ref_assemble_charge(ref_mesh *m);
 void ref_efield_faces(ref_mesh *m, double eb);
+/* resample!'s p and wtotal per cell of a sorted population (cell_start of
+ * ncells + 1 entries): the count plus one per split, and the weight sum. */
+void ref_resample_counts(const ref_pop *p, const int64_t *cell_start, int64_t ncells, int32_t ntarget,
+                         int32_t nmax, int64_t *pcell, double *wcell);
 int32_t ref_resample_cells(ref_pop *p, ref_mesh *m, int32_t ntarget, int32_t nmax,
                            uint64_t seed, uint32_t instance, uint32_t step,
                            ref_pop *scratch, ref_resample_stats *st);

@@ -165,6 +165,9 @@ def lib():
                                       ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, P(RefStats)]
         L.ref_advance_cyl.restype = ctypes.c_int32
         L.ref_cyl_density.argtypes = [P(RefPop), P(RefCGrid), _PD, ctypes.c_double]
+        L.ref_resample_counts.argtypes = [P(RefPop), _PI64, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                          _PI64, _PD]
+        L.ref_resample_counts.restype = None
         L.ref_cyl_density.restype = None
         L.ref_cyl_charge.argtypes = [P(RefCGrid)]
         L.ref_cyl_charge.restype = None
@@ -451,6 +454,19 @@ def cyl_sort_by_cell(pop, grid):
     lib().ref_cyl_sort_by_cell(ctypes.byref(p), ctypes.byref(grid.cstruct()),
                                grid.cell_start.ctypes.data_as(_PI64), ctypes.byref(sp))
     pop.sync_from(p)
+
+
+def cyl_resample_counts(pop, grid, ntarget, nmax):
+    """resample!'s p and wtotal on the (r, z) grid's (M+2)(N+2) cells, from the
+    cell lists of cyl_sort_by_cell (sorts first; the population is unchanged
+    otherwise), as arrays of shape (N+2, M+2)."""
+    cyl_sort_by_cell(pop, grid)
+    ncg = (grid.M + 2) * (grid.N + 2)
+    pc = np.zeros(ncg, dtype=np.int64)
+    wc = np.zeros(ncg)
+    lib().ref_resample_counts(ctypes.byref(pop.cstruct()), grid.cell_start.ctypes.data_as(_PI64), ncg,
+                              int(ntarget), int(nmax), pc.ctypes.data_as(_PI64), _d(wc))
+    return pc.reshape(grid.N + 2, grid.M + 2), wc.reshape(grid.N + 2, grid.M + 2)
 
 
 def cyl_resample(pop, grid, ntarget, nmax, seed=0, instance=0, step=0):

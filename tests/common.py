@@ -68,3 +68,78 @@ def smooth_potential(n, h, amp=0.5, seed=0):
     u = amp * (np.sin(2 * np.pi * i / nx + ph[0]) * np.cos(2 * np.pi * j / ny + ph[1])
                + np.sin(np.pi * k / nz + ph[2]))
     return u.reshape(-1)
+
+
+# ---- reading HDF5 files back with the HDF5 library ---------------------------
+# The image carries the HDF5 library (h5py 3.3 for a Python 3.9 interpreter, and
+# h5dump / h5diff).  The files this package writes are read back with it as a
+# separate process; tests that need it skip where it is absent.
+H5PY = "/opt/conda/bin/python3.9"
+H5DUMP = "/opt/conda/bin/h5dump"
+H5DIFF = "/opt/conda/bin/h5diff"
+
+_H5READ = r"""
+import sys, h5py, numpy as np
+out = {}
+meta = []
+def visit(name, obj):
+    if isinstance(obj, h5py.Dataset):
+        out[name] = obj[()]
+        meta.append(f"{name}\t{obj.dtype.str}\t{obj.shape}")
+with h5py.File(sys.argv[1], "r") as f:
+    f.visititems(visit)
+np.savez(sys.argv[2], **{k.replace("/", "|"): np.asarray(v) for k, v in out.items()})
+print("\n".join(meta))
+"""
+
+
+def have_h5py():
+    import subprocess
+    if not os.path.exists(H5PY):
+        return False
+    r = subprocess.run([H5PY, "-c", "import h5py"], capture_output=True)
+    return r.returncode == 0
+
+
+def h5read(path):
+    """{dataset path: array} and {dataset path: (dtype str, shape)} of an HDF5
+    file, read by the HDF5 library (h5py)."""
+    import subprocess
+    out = path + ".read.npz"
+    r = subprocess.run([H5PY, "-c", _H5READ, path, out], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"h5py could not read {path}:\n{r.stderr}")
+    meta = {}
+    for line in r.stdout.strip().splitlines():
+        name, dt, shape = line.split("\t")
+        meta[name] = (dt, shape)
+    with np.load(out) as z:
+        data = {k.replace("|", "/"): z[k] for k in z.files}
+    os.remove(out)
+    return data, meta
+
+
+def h5write_lib(path, tree):
+    """The same tree written by the HDF5 library itself (h5py), for h5diff."""
+    import subprocess
+    flat = {}
+
+    def walk(prefix, t):
+        for k, v in t.items():
+            if isinstance(v, dict):
+                walk(prefix + k + "/", v)
+            else:
+                flat[prefix + k] = np.asarray(v)
+
+    walk("", tree)
+    src = path + ".src.npz"
+    np.savez(src, **{k.replace("/", "|"): v for k, v in flat.items()})
+    script = ("import sys, h5py, numpy as np\n"
+              "z = np.load(sys.argv[1])\n"
+              "with h5py.File(sys.argv[2], 'w') as f:\n"
+              "    for k in z.files:\n"
+              "        f.create_dataset(k.replace('|', '/'), data=z[k])\n")
+    r = subprocess.run([H5PY, "-c", script, src, path], capture_output=True, text=True)
+    os.remove(src)
+    if r.returncode != 0:
+        raise RuntimeError(r.stderr)

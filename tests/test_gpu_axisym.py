@@ -119,7 +119,8 @@ def test_density_long_runs_bitwise():
 
 def test_cyl_resample_matches_oracle_bitwise():
     """Canonical (cell, id) order and resample! per (r, z) cell, twice (merged cells,
-    splits, the outside bucket, inactive particles dropped)."""
+    splits, the outside bucket, inactive particles dropped), with the p and
+    wtotal each resample leaves on the grid."""
     table, T, popl, P, grid, fields, G = _setup(n=60000, seed=8)
     x, v, w, s, act = _states(60000, 8)
     rng = np.random.default_rng(12)
@@ -129,9 +130,17 @@ def test_cyl_resample_matches_oracle_bitwise():
     popl.upload(jumc.ParticleStates(x, v, w, s, act))
     P = R.Pop(x, v, w, s, act, cap=4 * 60000)
     merged = splits = 0
+    pz, wz = fields.resample_fields()
+    assert not pz.any() and not wz.any()     # zeros before the first resample
     for step in range(2):
+        pc, wc = R.cyl_resample_counts(P, G, 10, 15)
         st = axisym.resample(popl, fields, 10, 15, step=step)
         rc, rs = R.cyl_resample(P, G, 10, 15, seed=popl.seed, step=step)
+        # GridFields.p and .wtotal as this resample! left them
+        pg, wg = fields.resample_fields()
+        np.testing.assert_array_equal(pg, pc)
+        np.testing.assert_array_equal(wg, wc)
+        assert wg.max() > 0.0 and (step > 0 or pg.max() > 15)   # the first resample merges crowded cells
         print(f"resample {step}: n {st.n_in} -> {st.n_out}, merged {st.cells_merged}, splits {st.splits}", flush=True)
         assert rc == 0
         assert (st.n_out, st.cells_merged, st.clones, st.splits, st.removed) == \

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from mcstreamer_amd import driver, gases
+from tests.common import h5read, have_h5py
 
 pytestmark = pytest.mark.gpu
 
@@ -62,10 +63,23 @@ def test_driver_runs_an_axisymmetric_input(tmp_path):
     assert len(run.log) == 4
     # output at t = 0 and 2 ps
     out = sorted(os.listdir(run.outpath))
-    assert out == ["0000.npz", "0001.npz"]
+    assert out == ["0000.h5", "0000.npz", "0001.h5", "0001.npz"]
     with np.load(os.path.join(run.outpath, "0001.npz"), allow_pickle=False) as z:
         assert z["u"].shape == (66, 34) and z["er"].shape == (67, 35)
         assert np.abs(z["er"]).max() > 0
+        u1, er1, qfixed1 = z["u"], z["er"], z["qfixed"]
+    # the HDF5 export of the same output (plot.jl:210-240), read by the HDF5 library
+    print(f"HDF5 library present to read the export: {have_h5py()}", flush=True)
+    if have_h5py():
+        data, meta = h5read(os.path.join(run.outpath, "0001.h5"))
+        assert meta["u"] == ("<f8", "(64, 32)") and meta["er"] == ("<f8", "(64, 33)")
+        assert meta["ez"] == ("<f8", "(65, 32)") and meta["p"] == ("<i8", "(64, 32)")
+        assert int(data["grid/M"]) == 32 and int(data["grid/N"]) == 64
+        np.testing.assert_array_equal(data["u"], u1[1:65, 1:33])
+        np.testing.assert_array_equal(data["er"], er1[1:65, 1:34])
+        np.testing.assert_array_equal(data["qfixed"], qfixed1[1:65, 1:33])
+        # written after the first resample: p counts the cells' particles (+ splits)
+        assert data["p"].sum() > 0 and data["wtotal"].sum() > 0
     # resample! keeps the weight (main.jl:300-306 asserts it to 1e-5)
     assert log["nactive"] > 0 and run.log[0]["weight"] == w0
     for r in run.log:
