@@ -383,10 +383,9 @@ __device__ void t2_vcycle(const Tail2 &T, double *lds, int start)
 
 // mgv! of levels start..L in one workgroup: b0 and x0 (or zero) loaded, x0 (ghost
 // shell included, as the cycle leaves it) written back.
-__global__ __launch_bounds__(MG2_TAIL_THREADS) void k2_tail(Tail2 T, int start, double *x0, const double *b0,
-                                                            int zero)
+__device__ __forceinline__ void tail2_body(const Tail2 &T, int start, double *x0, const double *b0, int zero,
+                                           double *lds)
 {
-    extern __shared__ double lds[];
     double *x = lds + T.off_x[start], *b = lds + T.off_b[start];
     TLOOP(t, T.len[start])
     {
@@ -396,6 +395,70 @@ __global__ __launch_bounds__(MG2_TAIL_THREADS) void k2_tail(Tail2 T, int start, 
     __syncthreads();
     t2_vcycle(T, lds, start);
     TLOOP(t, T.len[start]) x0[t] = x[t];
+}
+
+__global__ __launch_bounds__(MG2_TAIL_THREADS) void k2_tail(Tail2 T, int start, double *x0, const double *b0,
+                                                            int zero)
+{
+    extern __shared__ double lds[];
+    tail2_body(T, start, x0, b0, zero, lds);
+}
+
+// Tabulation of the tail: block q runs mgv! from x = 0 on its own pair of
+// start-level arrays, q lengths into x0 and b0.
+__global__ __launch_bounds__(MG2_TAIL_THREADS) void k2_tail_batch(Tail2 T, int start, double *x0, const double *b0)
+{
+    extern __shared__ double lds[];
+    const int64_t off = (int64_t)blockIdx.x * T.len[start];
+    tail2_body(T, start, x0 + off, b0 + off, 1, lds);
+}
+
+// ---- tabulated tail ----------------------------------------------------------
+//
+// As in three dimensions (mcs_mg.hip): below the tail's start level S the cycle
+// from x = 0 is a fixed linear map of S's right-hand side, tabulated once by
+// running the tail on unit right-hand sides and then applied as one wide
+// matrix-vector product.  Same operator; the product rounds differently from
+// the sweeps.
+
+// Unit inputs: array q (len apart) zero except at valid cell q of level L.
+__global__ void k2_units(double *a, int64_t len, Lv2 L, int64_t ncols)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= len * ncols) return;
+    const int64_t q = t / len, p = t - q * len;
+    a[t] = p == g2(L, q % L.n[0], q / L.n[0]) ? 1.0 : 0.0;
+}
+
+// M[r][q] = X[q][r].
+__global__ void k2_transpose(double *M, const double *X, int64_t rows, int64_t ncols)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= rows * ncols) return;
+    const int64_t r = t / ncols, q = t - r * ncols;
+    M[t] = X[q * rows + r];
+}
+
+#define MG2_TM_WAVES 4
+// x = M b on level L: b's valid cells staged in LDS in cell order, a wavefront per
+// output position (ghosts included), fused multiply-adds and a fixed butterfly.
+__global__ __launch_bounds__(MG2_TM_WAVES * MCS_WAVE) void k2_tailmat(const double *__restrict__ M,
+                                                                     const double *__restrict__ b,
+                                                                     double *__restrict__ x, Lv2 L, int64_t rows)
+{
+    extern __shared__ double bs[];
+    const int C = L.n[0] * L.n[1];
+    for (int t = threadIdx.x; t < C; t += MG2_TM_WAVES * MCS_WAVE) bs[t] = b[g2(L, t % L.n[0], t / L.n[0])];
+    __syncthreads();
+    const int lane = threadIdx.x & (MCS_WAVE - 1);
+    const int64_t r = (int64_t)blockIdx.x * MG2_TM_WAVES + (threadIdx.x / MCS_WAVE);
+    if (r >= rows) return;
+    const double *row = M + r * C;
+    double acc = 0.0;
+    for (int q = lane; q < C; q += MCS_WAVE) acc = __builtin_fma(row[q], bs[q], acc);
+#pragma unroll
+    for (int o = MCS_WAVE / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, MCS_WAVE);
+    if (lane == 0) x[r] = acc;
 }
 
 inline unsigned nblk(int64_t n)
@@ -538,6 +601,14 @@ int tail2_start(const mcs_mg *mg)
 
 void tail2(mcs_mg *mg, int level, double *x, const double *b, bool zero)
 {
+    if (zero && level == mg->tm_level) {
+        const int64_t rows = mg->lv[level].len;
+        const Lv2 L = lv2(mg, level);
+        hipLaunchKernelGGL(k2_tailmat, dim3((unsigned)((rows + MG2_TM_WAVES - 1) / MG2_TM_WAVES)),
+                           dim3(MG2_TM_WAVES * MCS_WAVE), sizeof(double) * (size_t)L.n[0] * L.n[1], mg->ctx->stream,
+                           (const double *)mg->tm_z, b, x, L, rows);
+        return;
+    }
     Tail2 T;
     const int64_t words = tail2_setup(mg, level, T);
     hipLaunchKernelGGL(k2_tail, dim3(1), dim3(MG2_TAIL_THREADS), (size_t)words * sizeof(double), mg->ctx->stream, T,
@@ -585,6 +656,52 @@ void mgv2(mcs_mg *mg, double *x, const double *b, int level, bool zero = false)
 }
 
 void fmg2_body(mcs_mg *mg, double *x, const double *b);
+
+// Tabulates mgv! from x = 0 at the tail's start level (mcs_mg::tm_level and
+// tm_z); off with MCS_MG_TAIL_MATRIX=0, and when the start is the finest level or
+// holds more than 4096 cells.
+int32_t tabulate_tail2(mcs_mg *mg)
+{
+    mcs_ctx *ctx = mg->ctx;
+    mg->tm_level = -1;
+    const char *e = getenv("MCS_MG_TAIL_MATRIX");
+    if (e && e[0] == '0') return MCS_OK;
+    const int S = mg->tail2;
+    if (S < 1 || S > mg->cfg.levels) return MCS_OK;
+    const Lv2 L = lv2(mg, S);
+    const int64_t C = (int64_t)L.n[0] * L.n[1], R = mg->lv[S].len;
+    if (C > 4096) return MCS_OK;
+    Tail2 T;
+    const size_t lds = (size_t)tail2_setup(mg, S, T) * sizeof(double);
+    if (lds > 64 * 1024 &&
+        hipFuncSetAttribute((const void *)k2_tail_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)mg->tail_lds) != hipSuccess)
+        return MCS_OK;
+    double *X = nullptr, *B = nullptr, *M = nullptr;
+    if (hipMalloc(&X, sizeof(double) * C * R) != hipSuccess || hipMalloc(&B, sizeof(double) * C * R) != hipSuccess ||
+        hipMalloc(&M, sizeof(double) * R * C) != hipSuccess) {
+        if (X) hipFree(X);
+        if (B) hipFree(B);
+        if (M) hipFree(M);
+        return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation of the tail table failed");
+    }
+    hipLaunchKernelGGL(k2_units, dim3(nblk(C * R)), dim3(MCS_BLOCK), 0, ctx->stream, B, R, L, C);
+    hipLaunchKernelGGL(k2_tail_batch, dim3((unsigned)C), dim3(MG2_TAIL_THREADS), lds, ctx->stream, T, S, X,
+                       (const double *)B);
+    hipLaunchKernelGGL(k2_transpose, dim3(nblk(C * R)), dim3(MCS_BLOCK), 0, ctx->stream, M, (const double *)X, R, C);
+    const hipError_t le = hipGetLastError();
+    const hipError_t se = hipStreamSynchronize(ctx->stream);
+    hipFree(X);
+    hipFree(B);
+    if (le != hipSuccess || se != hipSuccess) {
+        hipFree(M);
+        return mcs_fail(ctx, MCS_E_HIP, "mg: tabulating the tail failed (%s)",
+                        hipGetErrorString(le != hipSuccess ? le : se));
+    }
+    mg->tm_level = S;
+    mg->tm_z = M;
+    return MCS_OK;
+}
 
 // fmg! (Multigrid.jl:380-434).  Returns 1 after a correction, 0 when converged,
 // negative on a device error.
@@ -751,6 +868,11 @@ int32_t mcs_mg2_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg **out)
     if (te && atoi(te) == 0) mg->tail_lds = 0;
     mg->tail2 = tail2_start(mg);
     MCS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const int32_t trc = tabulate_tail2(mg);
+    if (trc) {
+        mcs_mg_destroy(mg);
+        return trc;
+    }
     *out = mg;
     return MCS_OK;
 }

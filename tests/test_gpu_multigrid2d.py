@@ -2,7 +2,10 @@
 CPU restatement: Cartesian and cylindrical connectors, Dirichlet/Neumann mixes,
 one- and two-sweep smoothing, absolute and relative tolerances.  The cell
 arithmetic is the restatement's, so with equal iteration counts the solutions
-agree bitwise (only the residual norm's summation order differs)."""
+agree bitwise (only the residual norm's summation order differs) when the
+coarse levels run as sweeps; with the tail tabulated (the default) the cycle
+below its start level is one matrix-vector product, which rounds differently:
+the solutions agree to 1e-12."""
 import ctypes
 
 import numpy as np
@@ -15,7 +18,7 @@ pytestmark = pytest.mark.gpu
 D, N = 0, 1
 
 
-@pytest.mark.parametrize("tail", ["1", "0"])
+@pytest.mark.parametrize("tail", ["matrix", "1", "0"])
 @pytest.mark.parametrize("n,bc,levels,smooth,cyl,tol", [
     ((32, 64), (N, D, D, D), 3, (2, 2), 1, -1e-10),
     ((64, 32), (N, N, D, N), 4, (2, 2), 0, -1e-10),
@@ -26,10 +29,11 @@ D, N = 0, 1
 ])
 def test_solve_2d_matches_restatement(monkeypatch, n, bc, levels, smooth, cyl, tol, tail):
     """(tail "1": the coarse levels of every V-cycle in one workgroup; "0": one
-    launch per phase on every level.)"""
+    launch per phase on every level; "matrix": the tail tabulated.)"""
     from mcstreamer_amd import _lib
     from mcstreamer_amd import multigrid as mg
-    monkeypatch.setenv("MCS_MG2_TAIL", tail)
+    monkeypatch.setenv("MCS_MG2_TAIL", "0" if tail == "0" else "1")
+    monkeypatch.setenv("MCS_MG_TAIL_MATRIX", "1" if tail == "matrix" else "0")
     rng = np.random.default_rng(17 + sum(n))
     g = (n[0] + 2) * (n[1] + 2)
     b = rng.normal(size=g)
@@ -49,8 +53,14 @@ def test_solve_2d_matches_restatement(monkeypatch, n, bc, levels, smooth, cyl, t
     assert rc == 0
     print(f"n={n} cyl={cyl} iters gpu {it.value} ref {it_r} resid {res.value:.3e} {res_r:.3e}", flush=True)
     assert it.value == it_r
-    np.testing.assert_array_equal(u_g, u_r)
-    assert res.value == pytest.approx(res_r, rel=1e-10)
+    if tail == "matrix":
+        np.testing.assert_allclose(u_g, u_r, rtol=0, atol=1e-12 * np.abs(u_r).max())
+        # the residual's rms moves by at most |L| (8 with the connector's factors
+        # below 2) times the largest change of u, plus its own rounding
+        assert abs(res.value - res_r) <= 8.0 * np.abs(u_g - u_r).max() + 1e-10 * res_r
+    else:
+        np.testing.assert_array_equal(u_g, u_r)
+        assert res.value == pytest.approx(res_r, rel=1e-10)
 
 
 def test_wrapper_2d_axisymmetric_solution():
