@@ -430,6 +430,19 @@ __global__ void k2_units(double *a, int64_t len, Lv2 L, int64_t ncols)
     a[t] = p == g2(L, q % L.n[0], q / L.n[0]) ? 1.0 : 0.0;
 }
 
+// a = 0 except 1 at valid cell q of level L.
+__global__ void k2_unit1(double *a, int64_t len, Lv2 L, int64_t q)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < len) a[p] = p == g2(L, q % L.n[0], q / L.n[0]) ? 1.0 : 0.0;
+}
+
+__global__ void k2_copy(double *dst, const double *src, int64_t n)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) dst[p] = src[p];
+}
+
 // M[r][q] = X[q][r].
 __global__ void k2_transpose(double *M, const double *X, int64_t rows, int64_t ncols)
 {
@@ -656,10 +669,12 @@ void mgv2(mcs_mg *mg, double *x, const double *b, int level, bool zero = false)
 }
 
 void fmg2_body(mcs_mg *mg, double *x, const double *b);
+double *fmg2_bottom(mcs_mg *mg, int S);
 
 // Tabulates mgv! from x = 0 at the tail's start level (mcs_mg::tm_level and
-// tm_z); off with MCS_MG_TAIL_MATRIX=0, and when the start is the finest level or
-// holds more than 4096 cells.
+// tm_z) and fmg!'s coarse part from that level's right-hand side (tm_f); off with
+// MCS_MG_TAIL_MATRIX=0, and when the start is the finest level or holds more
+// than 4096 cells.
 int32_t tabulate_tail2(mcs_mg *mg)
 {
     mcs_ctx *ctx = mg->ctx;
@@ -700,6 +715,29 @@ int32_t tabulate_tail2(mcs_mg *mg)
     }
     mg->tm_level = S;
     mg->tm_z = M;
+    // fmg!'s coarse part from S's right-hand side: one unit right-hand side at a
+    // time through the same launches, its solution a column of the table
+    double *Xf = nullptr, *Mf = nullptr;
+    if (hipMalloc(&Xf, sizeof(double) * C * R) != hipSuccess || hipMalloc(&Mf, sizeof(double) * R * C) != hipSuccess) {
+        if (Xf) hipFree(Xf);
+        return mcs_fail(ctx, MCS_E_HIP, "mg: device allocation of the tail table failed");
+    }
+    for (int64_t q = 0; q < C; ++q) {
+        hipLaunchKernelGGL(k2_unit1, dim3(nblk(R)), dim3(MCS_BLOCK), 0, ctx->stream, mg->lv[S].res, R, L, q);
+        const double *zs = fmg2_bottom(mg, S);
+        hipLaunchKernelGGL(k2_copy, dim3(nblk(R)), dim3(MCS_BLOCK), 0, ctx->stream, Xf + q * R, zs, R);
+    }
+    hipLaunchKernelGGL(k2_transpose, dim3(nblk(C * R)), dim3(MCS_BLOCK), 0, ctx->stream, Mf, (const double *)Xf, R,
+                       C);
+    const hipError_t fe = hipGetLastError();
+    const hipError_t fs = hipStreamSynchronize(ctx->stream);
+    hipFree(Xf);
+    if (fe != hipSuccess || fs != hipSuccess) {
+        hipFree(Mf);
+        return mcs_fail(ctx, MCS_E_HIP, "mg: tabulating fmg!'s coarse part failed (%s)",
+                        hipGetErrorString(fe != hipSuccess ? fe : fs));
+    }
+    mg->tm_f = Mf;
     return MCS_OK;
 }
 
@@ -762,22 +800,23 @@ int fmg2(mcs_mg *mg, double *x, const double *b, double *eps_out)
     return hipGetLastError() == hipSuccess ? 1 : -1;
 }
 
-// The correction of fmg! after the convergence test: restrictions, the coarse
-// solve, the interpolate! + mgv! sweep up the levels, x += z and apply!(x).
-void fmg2_body(mcs_mg *mg, double *x, const double *b)
+// fmg!'s restrictions of the right-hand sides from level `from` down to the
+// coarsest (Multigrid.jl:400-402).
+void fmg2_restrict(mcs_mg *mg, int from, int to)
+{
+    for (int k = from; k < to; ++k) {
+        const Lv2 Lk = lv2(mg, k);
+        hipLaunchKernelGGL(k2_restrict, dim3(nblk((int64_t)(Lk.n[0] / 2) * (Lk.n[1] / 2))), dim3(MCS_BLOCK), 0,
+                           mg->ctx->stream, mg->lv[k + 1].res, (const double *)mg->lv[k].res, Lk);
+    }
+}
+
+// fmg!'s interpolate! + mgv! sweep (Multigrid.jl:418-429) from the solution z of
+// level top + 1 up to level `to`; returns the solution of `to`.
+double *fmg2_sweep(mcs_mg *mg, double *z, int top, int to)
 {
     hipStream_t st = mg->ctx->stream;
-    const int L = mg->cfg.levels;
-    for (int k = 0; k < L; ++k) {
-        const Lv2 Lk = lv2(mg, k);
-        hipLaunchKernelGGL(k2_restrict, dim3(nblk((int64_t)(Lk.n[0] / 2) * (Lk.n[1] / 2))), dim3(MCS_BLOCK), 0, st,
-                           mg->lv[k + 1].res, (const double *)mg->lv[k].res, Lk);
-    }
-    hipLaunchKernelGGL(k2_coarse, dim3(1), dim3(MCS_BLOCK), 0, st, (const double *)mg->ainv,
-                       (const double *)mg->lv[L].res, mg->lv[L].sol, lv2(mg, L));
-    double *z = mg->lv[L].sol;
-    for (int k = 1; k <= L; ++k) {
-        const int lev = L - k;
+    for (int lev = top; lev >= to; --lev) {
         double *z1 = mg->lv[lev].sol;
         fill0(mg, z1, mg->lv[lev].len);
         bc2(mg, lev + 1, z);
@@ -786,6 +825,40 @@ void fmg2_body(mcs_mg *mg, double *x, const double *b)
                            (const double *)z, Ll, 0);
         mgv2(mg, z1, mg->lv[lev].res, lev);
         z = z1;
+    }
+    return z;
+}
+
+// fmg!'s coarse part from level S's right-hand side in res[S]: the restrictions
+// below it, the coarse solve and the sweep up to S; the result in sol[S].
+double *fmg2_bottom(mcs_mg *mg, int S)
+{
+    const int L = mg->cfg.levels;
+    fmg2_restrict(mg, S, L);
+    hipLaunchKernelGGL(k2_coarse, dim3(1), dim3(MCS_BLOCK), 0, mg->ctx->stream, (const double *)mg->ainv,
+                       (const double *)mg->lv[L].res, mg->lv[L].sol, lv2(mg, L));
+    return fmg2_sweep(mg, mg->lv[L].sol, L - 1, S);
+}
+
+// The correction of fmg! after the convergence test: restrictions, the coarse
+// solve, the interpolate! + mgv! sweep up the levels, x += z and apply!(x).  With
+// the coarse part tabulated (tm_f), level tm_level's solution is its product
+// with the restricted right-hand side.
+void fmg2_body(mcs_mg *mg, double *x, const double *b)
+{
+    hipStream_t st = mg->ctx->stream;
+    double *z;
+    if (mg->tm_f) {
+        const int S = mg->tm_level;
+        fmg2_restrict(mg, 0, S);
+        const int64_t rows = mg->lv[S].len;
+        const Lv2 LS = lv2(mg, S);
+        hipLaunchKernelGGL(k2_tailmat, dim3((unsigned)((rows + MG2_TM_WAVES - 1) / MG2_TM_WAVES)),
+                           dim3(MG2_TM_WAVES * MCS_WAVE), sizeof(double) * (size_t)LS.n[0] * LS.n[1], st,
+                           (const double *)mg->tm_f, (const double *)mg->lv[S].res, mg->lv[S].sol, LS, rows);
+        z = fmg2_sweep(mg, mg->lv[S].sol, S - 1, 0);
+    } else {
+        z = fmg2_bottom(mg, 0);
     }
     hipLaunchKernelGGL(k2_axpy, dim3(nblk(mg->lv[0].len)), dim3(MCS_BLOCK), 0, st, x, (const double *)z,
                        mg->lv[0].len);
