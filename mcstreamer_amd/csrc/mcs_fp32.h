@@ -94,6 +94,18 @@ __device__ __forceinline__ float energy32(const float v[3])
 // energy[k+1] (the float nodes), weight of node k.  The node index comes from
 // the grid formula (hardware log2 on the log grid), then is corrected by one
 // node against the stored nodes; the general search runs only if that fails.
+// Table loads: from global memory (TL = 0), or (TL = 1) from the copy a kernel
+// staged in LDS, whose pointers the TableView32 then holds.
+template <int TL, typename V>
+__device__ __forceinline__ V tld(const V *p)
+{
+    if constexpr (TL != 0)
+        return *(const __attribute__((address_space(3))) V *)p;
+    else
+        return gld(p);
+}
+
+template <int TL = 0>
 __device__ __forceinline__ void indweight32(const TableView32 &T, float eng, int32_t &k, float &w,
                                             int32_t &flags)
 {
@@ -122,14 +134,14 @@ __device__ __forceinline__ void indweight32(const TableView32 &T, float eng, int
         kf = floorf(__builtin_amdgcn_logf(eng * T.inv_e0) * T.inv_dlog2);
     }
     int32_t kk = (int32_t)fminf(fmaxf(kf, 0.0f), (float)(T.ne - 2));
-    float e0 = gld(E + kk), e1 = gld(E + kk + 1);
+    float e0 = tld<TL>(E + kk), e1 = tld<TL>(E + kk + 1);
     if ((e0 > eng) | (e1 <= eng)) {
         kk = e0 > eng ? kk - 1 : kk + 1;
         kk = kk < 0 ? 0 : (kk > T.ne - 2 ? T.ne - 2 : kk);
-        while (kk > 0 && gld(E + kk) > eng) kk -= 1;
-        while (kk < T.ne - 2 && gld(E + kk + 1) <= eng) kk += 1;
-        e0 = gld(E + kk);
-        e1 = gld(E + kk + 1);
+        while (kk > 0 && tld<TL>(E + kk) > eng) kk -= 1;
+        while (kk < T.ne - 2 && tld<TL>(E + kk + 1) <= eng) kk += 1;
+        e0 = tld<TL>(E + kk);
+        e1 = tld<TL>(E + kk + 1);
     }
     const float ww = (e1 - eng) * __builtin_amdgcn_rcpf(e1 - e0);
     k = kk;
@@ -256,7 +268,7 @@ __device__ __forceinline__ void rebase32(int32_t c[3], float off[3], const int32
 // the FP64 body.  `birth(c, off, v, s, active)` receives each ionization child
 // at the parent's position, `event(j)` each selected table row, and with TRACK
 // `track(c, off, +-w)` each ion-charge deposit of the CollisionTracker.
-template <int FIELD_MESH, int TRACK, typename Birth, typename Event, typename Track>
+template <int FIELD_MESH, int TRACK, int TL = 0, typename Birth, typename Event, typename Track>
 __device__ __forceinline__ void advance_one32(int32_t c[3], float off[3], float v[3], const float w,
                                               float &s, uint8_t &act, mcs_stream &rs, const float dt,
                                               const TableView32 &T, const MeshView32 &M,
@@ -315,14 +327,14 @@ __device__ __forceinline__ void advance_one32(int32_t c[3], float off[3], float 
             const float eng = energy32(v);
             int32_t k;
             float wg;
-            indweight32(T, eng, k, wg, flags);
+            indweight32<TL>(T, eng, k, wg, flags);
             float xi = mcs_u01f(&rs) * maxrate;
             const float *r0 = T.rate + k * T.nproc;
             const float *r1 = r0 + T.nproc;
             // the selected row first (do_one_collision!), then its outcome
             int sel = -1;
             for (int j = 0; j < T.nproc; ++j) {
-                const float nu = wg * gld(r0 + j) + (1.0f - wg) * gld(r1 + j);
+                const float nu = wg * tld<TL>(r0 + j) + (1.0f - wg) * tld<TL>(r1 + j);
                 if (nu > xi) {
                     sel = j;
                     break;
@@ -333,7 +345,7 @@ __device__ __forceinline__ void advance_one32(int32_t c[3], float off[3], float 
                 event(sel);
                 const int kind = proc_kind(T, sel);
                 if (kind == MCS_PROC_ELASTIC) {
-                    const float mr = gld(T.param + sel);
+                    const float mr = tld<TL>(T.param + sel);
                     const float cm = mr / (1.0f + mr);
                     const float vc0 = cm * v[0], vc1 = cm * v[1], vc2 = cm * v[2];
                     const float e0 = v[0] - vc0, e1 = v[1] - vc1, e2 = v[2] - vc2;
@@ -344,7 +356,7 @@ __device__ __forceinline__ void advance_one32(int32_t c[3], float off[3], float 
                     v[1] = om[1] * nrm + vc1;
                     v[2] = om[2] * nrm + vc2;
                 } else if (kind == MCS_PROC_EXCITATION) {
-                    const float e1 = fmaxf(eng - gld(T.param + sel), 0.0f);
+                    const float e1 = fmaxf(eng - tld<TL>(T.param + sel), 0.0f);
                     const float vabs = sqrtf((2.0f * e1) / MCS_ELECTRON_MASS_F);
                     float om[3];
                     randsphere32(&rs, om);
@@ -352,7 +364,7 @@ __device__ __forceinline__ void advance_one32(int32_t c[3], float off[3], float 
                     v[1] = om[1] * vabs;
                     v[2] = om[2] * vabs;
                 } else if (kind == MCS_PROC_IONIZATION) {
-                    const float e1 = fmaxf(eng - gld(T.param + sel), 0.0f) * 0.5f;
+                    const float e1 = fmaxf(eng - tld<TL>(T.param + sel), 0.0f) * 0.5f;
                     const float vabs = sqrtf((2.0f * e1) / MCS_ELECTRON_MASS_F);
                     float om1[3], om2[3];
                     randsphere32(&rs, om1);

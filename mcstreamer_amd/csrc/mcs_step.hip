@@ -485,6 +485,144 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_advance32(AdvArgs32 a)
     }
 }
 
+// The FP32 step with the collision table staged in LDS: one workgroup of 1024
+// threads per CU copies the FP32 rates, energy nodes and parameters (ne (nproc
+// + 1) + nproc floats, 128 KB for the 7-row, 4000-node gas) into LDS and then
+// runs k_advance32's body over chunks of 1024 particle slots, each chunk as
+// four logical blocks of MCS_BLOCK slots with their own birth and deposit
+// scans, so that the placement kernels and every result are those of
+// k_advance32 (the table's values are the same; only where they are read from
+// differs).
+#define ADV_LDS_THREADS 1024
+#define ADV_LDS_GROUPS (ADV_LDS_THREADS / MCS_BLOCK)
+
+// Exclusive scan over the MCS_BLOCK threads of this thread's group.
+__device__ __forceinline__ int group_excl_scan(int v, int *total, int *wsum)
+{
+    const int lane = threadIdx.x & (MCS_WAVE - 1), wid = threadIdx.x / MCS_WAVE;
+    const int g0 = (wid / (MCS_BLOCK / MCS_WAVE)) * (MCS_BLOCK / MCS_WAVE);
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < MCS_WAVE; o <<= 1) {
+        const int y = __shfl_up(x, o, MCS_WAVE);
+        if (lane >= o) x += y;
+    }
+    if (lane == MCS_WAVE - 1) wsum[wid] = x;
+    __syncthreads();
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < MCS_BLOCK / MCS_WAVE; ++k) {
+        const int sk = wsum[g0 + k];
+        if (g0 + k < wid) pre += sk;
+        tot += sk;
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + x - v;
+}
+
+template <int FIELD_MESH, int TRACK>
+__global__ __launch_bounds__(ADV_LDS_THREADS) void k_advance32_lds(AdvArgs32 a, int64_t nchunks, int64_t nblk)
+{
+    extern __shared__ float tab[];
+    __shared__ int s_ev[MCS_MAX_PROC];
+    __shared__ int s_flights, s_deaths, s_flags;
+    __shared__ int s_wsum[ADV_LDS_THREADS / MCS_WAVE];
+    const int nr = a.T.ne * a.T.nproc;
+    float *lrate = tab, *lenergy = tab + nr, *lparam = lenergy + a.T.ne;
+    for (int q = threadIdx.x; q < nr; q += ADV_LDS_THREADS) lrate[q] = gld(a.T.rate + q);
+    for (int q = threadIdx.x; q < a.T.ne; q += ADV_LDS_THREADS) lenergy[q] = gld(a.T.energy + q);
+    if (threadIdx.x < a.T.nproc) lparam[threadIdx.x] = gld(a.T.param + threadIdx.x);
+    if (threadIdx.x < MCS_MAX_PROC) s_ev[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        s_flights = 0;
+        s_deaths = 0;
+        s_flags = 0;
+    }
+    __syncthreads();
+    TableView32 T = a.T;
+    T.rate = lrate;
+    T.energy = lenergy;
+    T.param = lparam;
+    const int grp = threadIdx.x / MCS_BLOCK, gt = threadIdx.x % MCS_BLOCK;
+    int32_t flags = 0;
+    int flights = 0, deaths = 0;
+    for (int64_t ch = xcd_block(); ch < nchunks; ch += gridDim.x) {
+        const int64_t lb = ch * ADV_LDS_GROUPS + grp;
+        const int64_t i = lb * MCS_BLOCK + gt;
+        int nb = 0, nt = 0;
+        if (i < a.n0 && a.b.active[i]) {
+            int32_t c[3];
+            unpack_cell(a.b.cell[i], c[0], c[1], c[2]);
+            float off[3] = {a.b.off[0][i], a.b.off[1][i], a.b.off[2][i]};
+            float v[3] = {a.b.v[0][i], a.b.v[1][i], a.b.v[2][i]};
+            const float w = a.b.w[i];
+            float s = a.b.s[i];
+            uint8_t act = 1;
+            mcs_stream rs;
+            mcs_stream_init(&rs, a.seed, a.instance, MCS_PHASE_ADVANCE, a.b.id[i], a.step);
+            advance_one32<FIELD_MESH, TRACK, 1>(
+                c, off, v, w, s, act, rs, a.dt, T, a.M, a.acc, a.inv_h, flags, flights, deaths,
+                [&](const int32_t *bc, const float *boff, const float *bv, float bs, uint8_t bact) {
+                    unsigned long long slot = atomicAdd((unsigned long long *)&a.ctr[C_LOG], 1ULL);
+                    if ((int64_t)slot < a.L.cap) {
+                        a.L.parent[slot] = (int32_t)i;
+                        a.L.seq[slot] = nb;
+                        a.L.cell[slot] = pack_cell(bc[0], bc[1], bc[2]);
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) {
+                            a.L.off[d][slot] = boff[d];
+                            a.L.v[d][slot] = bv[d];
+                        }
+                        a.L.w[slot] = w;
+                        a.L.s[slot] = bs;
+                        a.L.act[slot] = bact;
+                    } else {
+                        flags |= MCS_FLAG_CAPACITY;
+                    }
+                    nb += 1;
+                },
+                [&](int j) { atomicAdd(&s_ev[j], 1); },
+                [&](const int32_t *tc, const float *toff, double val) {
+                    int32_t cell;
+                    double amount;
+                    if (track_cell32(a.M, tc, toff, val, cell, amount))
+                        track_push(a.TL, a.ctr, i, nt, cell, amount, flags);
+                });
+            rebase32(c, off, a.nmax);
+            a.b.cell[i] = pack_cell(c[0], c[1], c[2]);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                a.b.off[d][i] = off[d];
+                a.b.v[d][i] = v[d];
+            }
+            a.b.s[i] = s;
+            if (!act) a.b.active[i] = 0;
+        }
+        int tot;
+        const int ex = group_excl_scan(nb, &tot, s_wsum);
+        if (nb > 0) a.loff[i] = ex;
+        if (gt == 0 && lb < nblk) a.btot[lb] = tot;
+        if (TRACK) {
+            int ttot;
+            const int tex = group_excl_scan(nt, &ttot, s_wsum);
+            if (nt > 0) a.TL.toff[i] = tex;
+            if (gt == 0 && lb < nblk) a.TL.tbtot[lb] = ttot;
+        }
+    }
+    if (flights) atomicAdd(&s_flights, flights);
+    if (deaths) atomicAdd(&s_deaths, deaths);
+    if (flags) atomicOr(&s_flags, flags);
+    __syncthreads();
+    if (threadIdx.x < a.T.nproc && s_ev[threadIdx.x])
+        atomicAdd((unsigned long long *)&a.ctr[C_EVENTS + threadIdx.x], (unsigned long long)s_ev[threadIdx.x]);
+    if (threadIdx.x == 0) {
+        if (s_flights) atomicAdd((unsigned long long *)&a.ctr[C_FLIGHTS], (unsigned long long)s_flights);
+        if (s_deaths) atomicAdd((unsigned long long *)&a.ctr[C_DEATHS], (unsigned long long)s_deaths);
+        if (s_flags) atomicOr((unsigned long long *)&a.ctr[C_FLAGS], (unsigned long long)s_flags);
+    }
+}
+
 __global__ __launch_bounds__(MCS_BLOCK) void k_place_births32(BirthLog32 L, int64_t count, const int32_t *loff,
                                                               const int64_t *bpre, int64_t n0, uint64_t next_id,
                                                               SplitBuf b, int32_t n0x, int32_t n1x, int32_t n2x)
@@ -525,6 +663,39 @@ static BirthLog32 birth_log32(mcs_pop *p)
     L.act = p->log_act;
     L.cap = p->logcap;
     return L;
+}
+
+// The device's compute units (one workgroup of k_advance32_lds each).
+static int64_t adv_lds_cus(mcs_ctx *ctx)
+{
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || cus < 1)
+        cus = 256;
+    return cus;
+}
+
+// Whether k_advance32_lds may take tab_bytes of dynamic LDS (the attribute set
+// once per process for each instantiation).
+static bool adv_lds_ready(mcs_ctx *ctx, size_t tab_bytes)
+{
+    static int ok = -1;
+    if (ok < 0) {
+        int lds_max = 0;
+        if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) != hipSuccess)
+            lds_max = 64 * 1024;
+        const int want = 144 * 1024;
+        ok = lds_max >= want + 4096 &&
+             hipFuncSetAttribute((const void *)k_advance32_lds<1, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 want) == hipSuccess &&
+             hipFuncSetAttribute((const void *)k_advance32_lds<1, 0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 want) == hipSuccess &&
+             hipFuncSetAttribute((const void *)k_advance32_lds<0, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 want) == hipSuccess &&
+             hipFuncSetAttribute((const void *)k_advance32_lds<0, 0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 want) == hipSuccess;
+        if (!ok) (void)hipGetLastError();
+    }
+    return ok && tab_bytes <= 144 * 1024;
 }
 
 static int32_t launch_advance32(mcs_pop *p, const mcs_table *t, const mcs_field *field, double dt,
@@ -572,6 +743,26 @@ static int32_t launch_advance32(mcs_pop *p, const mcs_table *t, const mcs_field 
     a.TL = track_log(p);
     ProfScope ps(ctx, "advance");
     const int fm = field->kind == MCS_FIELD_MESH, tr = tracker != nullptr;
+    // MCS_ADV_LDS=1: the table in LDS (k_advance32_lds) when it fits beside the
+    // kernel's own and the process kinds travel as bits.  Same results; on the
+    // 256x256x512 workload 12.9 ms per launch against 11.2 ms reading the table
+    // from global memory, where it stays in L2, so the default reads it there.
+    const size_t tab_bytes = sizeof(float) * ((size_t)t->ne * (t->nproc + 1) + (size_t)t->nproc);
+    const char *le = getenv("MCS_ADV_LDS");
+    if (le && le[0] == '1' && a.T.kindbits != 0 && tab_bytes <= 144 * 1024 && adv_lds_ready(ctx, tab_bytes)) {
+        const int64_t nchunks = cdiv(n0, (int64_t)ADV_LDS_THREADS);
+        const dim3 gl((unsigned)std::min<int64_t>(nchunks, adv_lds_cus(ctx))), bll(ADV_LDS_THREADS);
+        if (fm && tr)
+            hipLaunchKernelGGL((k_advance32_lds<1, 1>), gl, bll, tab_bytes, ctx->stream, a, nchunks, nblk);
+        else if (fm)
+            hipLaunchKernelGGL((k_advance32_lds<1, 0>), gl, bll, tab_bytes, ctx->stream, a, nchunks, nblk);
+        else if (tr)
+            hipLaunchKernelGGL((k_advance32_lds<0, 1>), gl, bll, tab_bytes, ctx->stream, a, nchunks, nblk);
+        else
+            hipLaunchKernelGGL((k_advance32_lds<0, 0>), gl, bll, tab_bytes, ctx->stream, a, nchunks, nblk);
+        MCS_LAUNCH_CHECK(ctx);
+        return MCS_OK;
+    }
     dim3 g((unsigned)nblk), bl(MCS_BLOCK);
     if (fm && tr)
         hipLaunchKernelGGL((k_advance32<1, 1>), g, bl, 0, ctx->stream, a);

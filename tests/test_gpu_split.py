@@ -8,6 +8,7 @@ and statistically for whole steps."""
 import numpy as np
 import pytest
 
+from mcstreamer_amd import constants as co
 from mcstreamer_amd import jumc, streamer
 from tests.common import random_particles
 
@@ -236,3 +237,38 @@ def test_split_full_config2_solve_and_charge_properties():
     rms_sb = np.sqrt(np.mean(sb ** 2))
     print(f"rms residual {rms_r:.3e}, rms(s b) {rms_sb:.3e}, ratio {rms_r / rms_sb:.3e}", flush=True)
     assert rms_r <= 1.0001e-10 * rms_sb
+
+
+@pytest.mark.parametrize("npairs", [60_000, 300_000])
+def test_split_step_with_table_in_lds_equals_global_reads(monkeypatch, npairs):
+    """The FP32 step with the collision table staged in LDS (MCS_ADV_LDS=1: one
+    1024-thread workgroup per compute unit walking chunks of 1024 slots) against
+    the default step reading the table from global memory: whole streamer steps (births
+    placed through the per-256-slot scans, the ion tracker's deposits) equal
+    bitwise, for fewer chunks than compute units and for more."""
+    out = {}
+    for lds in ("0", "1"):
+        monkeypatch.setenv("MCS_ADV_LDS", lds)
+        cfg = streamer.make_streamer(n=(32, 32, 64), h=2e-5, npairs=npairs, seed=13, precision="f32",
+                                     tolerance=-1e-10, sigma=1.5e-4)
+        # the seed at 30 eV instead of at rest, so that the first steps ionize
+        ps = cfg["popl"].download()
+        v = np.random.default_rng(6).normal(size=ps.v.shape) * np.sqrt(30.0 * co.eV / co.electron_mass)
+        cfg["popl"].upload(jumc.ParticleStates(ps.x, v, ps.w, ps.s, ps.active, ps.id, ps.next_id))
+        sim = streamer.Streamer(cfg["popl"], cfg["fields"], cfg["ws"], cfg["eb"], cfg["dt"], cfg["ntarget"],
+                                cfg["nmax"])
+        stats = []
+        for k in range(4):
+            r = sim.step()
+            a = r["advance"]
+            stats.append((a.n_out, a.births, a.deaths, a.flights, tuple(list(a.events)[:7]), r["mg_iters"]))
+        print(f"lds={lds}: {stats[-1]}", flush=True)
+        out[lds] = (stats, sim.popl.download(), cfg["fields"].get("qfixed"), cfg["fields"].get("u"))
+    assert out["0"][0] == out["1"][0]
+    assert out["0"][0][-1][1] > 0        # births happened
+    g0, g1 = out["0"][1], out["1"][1]
+    np.testing.assert_array_equal(g0.id, g1.id)
+    for k in ("x", "v", "w", "s", "active"):
+        np.testing.assert_array_equal(getattr(g0, k), getattr(g1, k), err_msg=k)
+    np.testing.assert_array_equal(out["0"][2], out["1"][2])
+    np.testing.assert_array_equal(out["0"][3], out["1"][3])
