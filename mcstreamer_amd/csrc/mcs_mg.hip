@@ -593,6 +593,64 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_mg_rbgs_s(T *u, const T *b, Lvl L
     up[t] = uc + div6(T(1) * (l + b[par * half + t]));
 }
 
+// The same pass with two consecutive cells of a row per thread (a half row has
+// an even count): the pair's own values, right-hand sides and y/z neighbours
+// are aligned pairs loaded and stored together, the three x neighbours they
+// share loaded once; each cell's arithmetic is lap_split's and gs_item's.
+template <typename T>
+struct Pair2;
+template <>
+struct Pair2<double> {
+    using V = double2;
+};
+template <>
+struct Pair2<float> {
+    using V = float2;
+};
+
+template <typename T>
+__global__ __launch_bounds__(MCS_BLOCK) void k_mg_rbgs_s2(T *u, const T *b, Lvl L, BcCoef bc, int par)
+{
+    using V = typename Pair2<T>::V;
+    const int64_t half = (int64_t)(L.n[0] / 2) * L.n[1] * L.n[2];
+    const int64_t t = 2 * xcd_gidx();
+    if (t >= half) return;
+    int64_t ti, j, k;
+    split3(t, (uint32_t)(L.n[0] / 2), (uint32_t)L.n[1], ti, j, k);
+    const int64_t odd = (j + k + par) & 1;
+    const int64_t i0 = 2 * ti + odd, i1 = i0 + 2;
+    const int64_t sy = L.n[0] / 2, sz = (int64_t)(L.n[0] / 2) * L.n[1];
+    T *up = u + par * half;
+    const T *o = u + (1 - par) * half;
+    const V uc = *(const V *)(up + t);
+    const V bb = *(const V *)(b + par * half + t);
+    const bool yp = j + 1 < L.n[1], zp = k + 1 < L.n[2], ym = j > 0, zm = k > 0;
+    const V vyp = *(const V *)(o + (yp ? t + sy : t)), vzp = *(const V *)(o + (zp ? t + sz : t));
+    const V vym = *(const V *)(o + (ym ? t - sy : t)), vzm = *(const V *)(o + (zm ? t - sz : t));
+    // x neighbours: cell 0 at offset t, cell 1 at t + 1 (lap_split's indices)
+    const bool xp0 = i0 + 1 < L.n[0], xm0 = i0 > 0, xp1 = i1 + 1 < L.n[0], xm1 = i1 > 0;
+    const T vxp0 = o[xp0 ? t + odd : t], vxm0 = o[xm0 ? t - 1 + odd : t];
+    const T vxp1 = o[xp1 ? t + 1 + odd : t + 1], vxm1 = o[xm1 ? t + odd : t + 1];
+    T l0 = T(-6) * uc.x;
+    l0 += xp0 ? vxp0 : bcf(bc, 3, uc.x);
+    l0 += yp ? vyp.x : bcf(bc, 4, uc.x);
+    l0 += zp ? vzp.x : bcf(bc, 5, uc.x);
+    l0 += xm0 ? vxm0 : bcf(bc, 0, uc.x);
+    l0 += ym ? vym.x : bcf(bc, 1, uc.x);
+    l0 += zm ? vzm.x : bcf(bc, 2, uc.x);
+    T l1 = T(-6) * uc.y;
+    l1 += xp1 ? vxp1 : bcf(bc, 3, uc.y);
+    l1 += yp ? vyp.y : bcf(bc, 4, uc.y);
+    l1 += zp ? vzp.y : bcf(bc, 5, uc.y);
+    l1 += xm1 ? vxm1 : bcf(bc, 0, uc.y);
+    l1 += ym ? vym.y : bcf(bc, 1, uc.y);
+    l1 += zm ? vzm.y : bcf(bc, 2, uc.y);
+    V r;
+    r.x = uc.x + div6(T(1) * (l0 + bb.x));
+    r.y = uc.y + div6(T(1) * (l1 + bb.y));
+    *(V *)(up + t) = r;
+}
+
 // resid_restrict_item with the fine arrays split (the coarse ones keep ghosts).
 template <typename T>
 __global__ __launch_bounds__(MCS_BLOCK) void k_mg_resid_restrict_s(T *rh, const T *u, const T *b, Lvl F, BcCoef bc,
@@ -1853,6 +1911,14 @@ static void mg_bc(mcs_mg *mg, int l, T *a)
     hipLaunchKernelGGL(k_mg_bc<T>, grid_for(cnt), dim3(MCS_BLOCK), 0, mg->ctx->stream, a, L, coefs(mg));
 }
 
+// MCS_MG_RBGS_SINGLE=1: the colour-split passes take one cell per thread
+// (k_mg_rbgs_s) instead of two (same results).
+static bool mg_rbgs_single()
+{
+    const char *e = getenv("MCS_MG_RBGS_SINGLE");
+    return e && e[0] == '1';
+}
+
 // One sweep (colour 0, then 1); ghosts: the colour-1 pass also writes the ghost
 // shell of the state the sweep ends with (ghost layout only).
 template <typename T>
@@ -1862,12 +1928,17 @@ static void mg_gs(mcs_mg *mg, int l, T *u, const T *b, bool split = false, bool 
     int64_t cnt = (int64_t)(L.n[0] / 2) * L.n[1] * L.n[2];
     ProfScope ps(mg->ctx, "mg_rbgs");
     for (int par = 0; par < 2; ++par) {
-        if (split)
-            hipLaunchKernelGGL(k_mg_rbgs_s<T>, grid_for(cnt), dim3(MCS_BLOCK), 0, mg->ctx->stream, u, b, L,
-                               coefs(mg), par);
-        else
+        if (split) {
+            if ((L.n[0] / 2) % 2 == 0 && !mg_rbgs_single())
+                hipLaunchKernelGGL(k_mg_rbgs_s2<T>, grid_for(cnt / 2), dim3(MCS_BLOCK), 0, mg->ctx->stream, u, b,
+                                   L, coefs(mg), par);
+            else
+                hipLaunchKernelGGL(k_mg_rbgs_s<T>, grid_for(cnt), dim3(MCS_BLOCK), 0, mg->ctx->stream, u, b, L,
+                                   coefs(mg), par);
+        } else {
             hipLaunchKernelGGL(k_mg_rbgs<T>, grid_for(cnt), dim3(MCS_BLOCK), 0, mg->ctx->stream, u, b, L,
                                coefs(mg), par, ghosts && par == 1 ? 1 : 0);
+        }
     }
 }
 

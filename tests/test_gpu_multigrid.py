@@ -304,3 +304,32 @@ def test_fmg_tail_matches_grid_path(monkeypatch, n, bc, levels, smooth, prec, kn
         print(f"fmg_tail={ft} n={n} iters {it.value} resid {res.value:.3e}", flush=True)
     assert out["0"][1] == out["1"][1] and out["0"][2] == out["1"][2]
     np.testing.assert_array_equal(out["0"][0], out["1"][0])
+
+
+@pytest.mark.parametrize("n,levels,prec", [((128, 128, 256), 6, "f64"), ((64, 64, 128), 5, "f32")])
+def test_split_passes_two_cells_per_thread(monkeypatch, n, levels, prec):
+    """The colour-split level-0 passes with two cells of a row per thread (pairs
+    loaded and stored together) give bitwise the solution of one cell per
+    thread."""
+    from mcstreamer_amd import _lib
+    import ctypes
+    rng = np.random.default_rng(23 + sum(n))
+    g = (n[0] + 2) * (n[1] + 2) * (n[2] + 2)
+    b = rng.normal(size=g)
+    u0 = rng.normal(size=g) * 0.1
+    faces = [mg.BottomBnd, mg.TopBnd, mg.LeftBnd, mg.RightBnd, mg.FrontBnd, mg.BackBnd]
+    out = {}
+    for single in ("1", "0"):
+        monkeypatch.setenv("MCS_MG_RBGS_SINGLE", single)
+        conf = mg.MGConfig(bc=list(zip(faces, (D, N, N, D, D, D))), s=0.9, levels=levels, tolerance=-1e-9,
+                           maxiter=50, precision=prec)
+        ws = mg.Workspace(conf, n)
+        u = u0.copy()
+        it = ctypes.c_int32()
+        res = ctypes.c_double()
+        assert _lib.lib().mcs_mg_solve_host(ws.handle, _lib.dptr(u), _lib.dptr(b), ctypes.byref(it),
+                                            ctypes.byref(res)) == 0
+        out[single] = (u, it.value, res.value)
+        print(f"single={single} n={n} iters {it.value} resid {res.value:.3e}", flush=True)
+    assert out["0"][1] == out["1"][1] and out["0"][2] == out["1"][2]
+    np.testing.assert_array_equal(out["0"][0], out["1"][0])
