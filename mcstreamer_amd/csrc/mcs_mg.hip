@@ -754,7 +754,7 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_mg_interp_s(T *__restrict__ r, co
 // Input and output are distinct arrays: neighbouring tiles read each other's
 // halos from the input while the outputs are written.
 
-#define MG_SM_THREADS 512
+#define MG_SM_THREADS 1024
 #define SM_LD 4
 #define SM_GS 4
 
