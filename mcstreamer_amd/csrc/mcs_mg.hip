@@ -651,6 +651,60 @@ __global__ __launch_bounds__(MCS_BLOCK) void k_mg_rbgs_s2(T *u, const T *b, Lvl 
     *(V *)(up + t) = r;
 }
 
+// interp_item into a split fine array for two consecutive coarse cells of a row
+// per thread (a coarse row has an even count): the two cells' children of one
+// colour, row and plane sit side by side, so each of the eight (colour, y, z)
+// child pairs is one paired store (and load when adding); each child's value is
+// interp_item's sum, term by term.
+template <typename T>
+__global__ __launch_bounds__(MCS_BLOCK) void k_mg_interp_s2(T *__restrict__ r, const T *__restrict__ rh, Lvl F,
+                                                            int add)
+{
+    using V = typename Pair2<T>::V;
+    const int64_t nc0 = F.n[0] / 2, nc1 = F.n[1] / 2;
+    const int64_t t = 2 * xcd_gidx();
+    if (t >= nc0 * nc1 * (F.n[2] / 2)) return;
+    int64_t i, j, k;
+    split3(t, (uint32_t)nc0, (uint32_t)nc1, i, j, k);
+    const int64_t cy = nc0 + 2, cz = (int64_t)(nc0 + 2) * (nc1 + 2);
+    const int64_t cc = (i + 1) + cy * (j + 1) + cz * (k + 1);
+    const T w0 = T(27) / T(64), w1 = T(9) / T(64), w2 = T(3) / T(64), w3 = T(1) / T(64);
+#pragma unroll
+    for (int sx = 0; sx < 2; ++sx)
+#pragma unroll
+        for (int sy = 0; sy < 2; ++sy)
+#pragma unroll
+            for (int sz = 0; sz < 2; ++sz) {
+                const int64_t f = s_index(F, 2 * i + sx, 2 * j + sy, 2 * k + sz);
+                const int64_t ox = 2 * sx - 1, oy = (2 * sy - 1) * cy, oz = (2 * sz - 1) * cz;
+                T v[2];
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const int64_t c = cc + e;
+                    T a = T(0);
+                    a += w0 * rh[c];
+                    a += w1 * rh[c + oz];
+                    a += w1 * rh[c + oy];
+                    a += w2 * rh[c + oy + oz];
+                    a += w1 * rh[c + ox];
+                    a += w2 * rh[c + ox + oz];
+                    a += w2 * rh[c + ox + oy];
+                    a += w3 * rh[c + ox + oy + oz];
+                    v[e] = a;
+                }
+                V out;
+                if (add) {
+                    const V old = *(const V *)(r + f);
+                    out.x = old.x + v[0];
+                    out.y = old.y + v[1];
+                } else {
+                    out.x = v[0];
+                    out.y = v[1];
+                }
+                *(V *)(r + f) = out;
+            }
+}
+
 // resid_restrict_item with the fine arrays split (the coarse ones keep ghosts).
 template <typename T>
 __global__ __launch_bounds__(MCS_BLOCK) void k_mg_resid_restrict_s(T *rh, const T *u, const T *b, Lvl F, BcCoef bc,
@@ -1911,8 +1965,8 @@ static void mg_bc(mcs_mg *mg, int l, T *a)
     hipLaunchKernelGGL(k_mg_bc<T>, grid_for(cnt), dim3(MCS_BLOCK), 0, mg->ctx->stream, a, L, coefs(mg));
 }
 
-// MCS_MG_RBGS_SINGLE=1: the colour-split passes take one cell per thread
-// (k_mg_rbgs_s) instead of two (same results).
+// MCS_MG_RBGS_SINGLE=1: the colour-split passes and prolongation take one cell
+// per thread (k_mg_rbgs_s, k_mg_interp_s) instead of two (same results).
 static bool mg_rbgs_single()
 {
     const char *e = getenv("MCS_MG_RBGS_SINGLE");
@@ -1952,7 +2006,10 @@ template <typename T>
 static void mg_interp(mcs_mg *mg, int lf, T *r, const T *rh, int add, bool split = false)
 {
     ProfScope ps(mg->ctx, "mg_prolong");
-    if (split)
+    if (split && (mg->lv[lf].n[0] / 2) % 2 == 0 && !mg_rbgs_single())
+        hipLaunchKernelGGL(k_mg_interp_s2<T>, grid_for(ncoarse_cells(mg, lf) / 2), dim3(MCS_BLOCK), 0,
+                           mg->ctx->stream, r, rh, lvl(mg, lf), add);
+    else if (split)
         hipLaunchKernelGGL(k_mg_interp_s<T>, grid_for(ncoarse_cells(mg, lf)), dim3(MCS_BLOCK), 0, mg->ctx->stream,
                            r, rh, lvl(mg, lf), add);
     else
