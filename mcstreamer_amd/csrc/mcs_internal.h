@@ -338,6 +338,7 @@ struct mcs_mg {
     int tail2 = 0;                               // 2D solver: first level of the one-workgroup V-cycle
     size_t sm_lds = 0;                           // LDS budget of the tiled smoother; 0: one launch per pass
     int sm_tile = 16;                            // its largest tile edge
+    unsigned sm_minblocks = 0;                   // smaller tiles until a level has this many blocks
     int64_t sm_big = 0;                          // levels of this many cells: one sweep per launch
     int64_t sm_max = 0;                          // levels of more cells: one launch per pass
     bool split0 = false;                         // level-0 correction in the colour-split layout

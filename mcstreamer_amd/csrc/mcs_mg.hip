@@ -1784,6 +1784,10 @@ extern "C" int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg 
     const char *se = getenv("MCS_MG_SM_TILE");
     int sm_tile = se ? atoi(se) : 8;
     if (sm_tile < 2) sm_tile = 2;
+    // MCS_MG_SM_MINBLOCKS: smaller tiles on a level until it has this many (64:
+    // 16x16x32 takes 4^3 tiles, 128 workgroups instead of 16; 0: the largest tile)
+    const char *mbe = getenv("MCS_MG_SM_MINBLOCKS");
+    const unsigned sm_minblocks = mbe ? (unsigned)atoi(mbe) : 64u;
     const char *me = getenv("MCS_MG_SM_MAXCELLS");
     int64_t sm_max = me ? atoll(me) : MG_SM_MAXCELLS;
     const char *be = getenv("MCS_MG_SM_BIG");
@@ -1804,6 +1808,7 @@ extern "C" int32_t mcs_mg_create(mcs_ctx *ctx, const mcs_mg_config *cfg, mcs_mg 
     mg->tail_levels = tail_levels;
     mg->tail_max = tail_max;
     mg->sm_lds = sm_lds;
+    mg->sm_minblocks = sm_minblocks;
     mg->sm_tile = sm_tile;
     mg->sm_big = sm_big;
     mg->sm_max = sm_max;
@@ -2186,7 +2191,8 @@ static void mg_tail(mcs_mg *mg, int level, T *x, const T *b, const T *zc, bool z
 
 // Tile of the tiled smoother for `level` with halo H: per axis the largest even
 // divisor of the level size up to sm_tile, shrunk until the boxes of u (grown by
-// H) and b (grown by H - 1) fit in LDS.
+// H) and b (grown by H - 1) fit in LDS and the level has at least sm_minblocks
+// tiles (the tile does not change the results).
 template <typename T>
 static bool smooth_plan(const mcs_mg *mg, int level, int halo, SmoothArgs &A, size_t &lds, unsigned &nblocks)
 {
@@ -2205,7 +2211,7 @@ static bool smooth_plan(const mcs_mg *mg, int level, int halo, SmoothArgs &A, si
             box *= (size_t)std::min(t + 2 * halo, n[d]);
             boxb *= (size_t)std::min(t + 2 * halo - 2, n[d]);
         }
-        if (box + boxb <= mg->sm_lds) {
+        if (box + boxb <= mg->sm_lds && (nblocks >= mg->sm_minblocks || target <= 2)) {
             lds = box + boxb;
             return true;
         }
